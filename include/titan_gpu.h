@@ -1,0 +1,197 @@
+/*
+ * titan_gpu.h -- C ABI of the MI355X (gfx950) implementation of TITAN's explicit ideal-MHD step.
+ *
+ * The reference (SergeyMSU/TITAN_C-) has no library boundary: its step sits behind C++ member
+ * functions of one executable.  Each entry point below names the reference seam it replaces.
+ * Conventions:
+ *   - return value 0 = success, negative = TITAN_E_* status (the reference prints and calls
+ *     exit(-1) at the corresponding sites; this library returns instead);
+ *   - all arrays are host arrays owned by the caller, in REFERENCE numbering (cells in the
+ *     caller's order, faces in New_connect order); no pointer is retained after a call;
+ *   - one context per host thread; every call is synchronous on the context's HIP stream;
+ *   - the state of a cell is 8 FP64 primitives (rho, Vx, Vy, Vz, p, Bx, By, Bz), B in the
+ *     reference's Gaussian code units (magnetic pressure B^2/8pi).
+ */
+#ifndef TITAN_GPU_H
+#define TITAN_GPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TITAN_OK 0
+#define TITAN_E_ARG (-1)            /* invalid argument / size                                 */
+#define TITAN_E_HIP (-2)            /* HIP runtime error                                       */
+#define TITAN_E_RIEMANN (-3)        /* impossible wave ordering (Phys_param.cpp:1194-1206,1834-1839) */
+#define TITAN_E_NONFINITE_STATE (-4)/* NaN density (Setka_physics.cpp:1962-1978)               */
+#define TITAN_E_TOPOLOGY (-5)       /* face/cell topology violates Setka::Test_geometr          */
+#define TITAN_E_NOT_TRAINED (-6)    /* surrogate requested before parameters were set          */
+
+/* face boundary codes (Type_Gran, Setka_physics.cpp:212-236; ZERO_GRADIENT is an addition) */
+#define TITAN_BC_INTERIOR 0
+#define TITAN_BC_INNER_HARD 1       /* fixed state (plasma_condition 2)                    */
+#define TITAN_BC_OUTER_HARD 2       /* fixed state (plasma_condition 2)                    */
+#define TITAN_BC_OUTER_SOFT 3       /* cell copy + suction fix (plasma_condition 1)        */
+#define TITAN_BC_ZERO_GRADIENT 4    /* pure cell copy                                      */
+
+/* geometry modes */
+#define TITAN_GEOM_REF 0            /* Gran::Culc_measure / Cell::Culc_volume formulas     */
+#define TITAN_GEOM_GCL 1            /* exactly conservative (bilinear faces, swept volumes) */
+
+/* arithmetic of the face flux */
+#define TITAN_PREC_FP64 0
+#define TITAN_PREC_MIXED 1          /* FP32 Riemann arithmetic; FP64 states, accumulation, geometry */
+
+/* per-cell charge-exchange source */
+#define TITAN_SRC_NONE 0
+#define TITAN_SRC_EXACT 1           /* FP64 Maxwellian quadrature (Lindsay-Stebbings sigma) */
+#define TITAN_SRC_MLP 2             /* BF16-MFMA surrogate, exact fallback out of range     */
+
+typedef struct titan_ctx titan_ctx;
+
+/* Mesh in reference numbering (TITAN/source/Gran.h:252-282, Cell.h:17-36, Yzel.h:352). */
+typedef struct {
+    int64_t n_nodes, n_cells, n_faces;
+    const int32_t *cell_nodes;   /* [n_cells*8] hexahedron (Cell::yzels)                       */
+    const int32_t *face_nodes;   /* [n_faces*4] (Gran::yzels)                                  */
+    const int32_t *face_cells;   /* [n_faces*2] Gran::cells[0], cells[1] (-1 on the boundary)  */
+    const int32_t *cell_faces;   /* [n_cells*6] Cell::grans, ascending face id                 */
+    const int8_t *face_bc;       /* [n_faces] TITAN_BC_*                                       */
+    const int8_t *face_method;   /* [n_faces] 0 Lax, 1 HLL, 2 HLLC, 3 HLLD (Gran::Get_method)  */
+    const uint8_t *face_flags;   /* [n_faces] bit0 left_ydar (TS_hard), bit1 contact; nullable */
+    const double *face_bstate;   /* [n_faces*8] fixed state of hard boundary faces; nullable   */
+    const int8_t *cell_zone;     /* [n_cells] 1..4 = Type_cell::Zone_1..Zone_4                 */
+    const uint8_t *cell_is_tvd;  /* [n_cells] Cell::is_TVD (0 => B frozen)                     */
+    const double *node_xyz;      /* [n_nodes*3] initial coordinates (Yzel::coord[0])           */
+    const double *node_disp;     /* [n_nodes*3] motion amplitude vectors D_i; nullable         */
+} titan_mesh;
+
+/* Parameters the hot path reads (Phys_param.h:95-122, set_parameters.txt names in brackets). */
+typedef struct {
+    double gamma;                /* [gamma]                                                    */
+    double kfl;                  /* [KFL]                                                      */
+    double v_inf;                /* [Velosity_inf]                                             */
+    int32_t move_mesh;           /* [move_setka]: nodes move by x += dt*omega*cos(omega t+phase)*D_i */
+    int32_t n_disc;              /* wave-speed estimate of chlld (0 on the step path)          */
+    int32_t geom_mode;           /* TITAN_GEOM_*                                               */
+    int32_t precision;           /* TITAN_PREC_*                                               */
+    int32_t source_mode;         /* TITAN_SRC_*                                                */
+    int32_t cx_nodes;            /* quadrature nodes per direction of the exact source (e.g. 100) */
+    double motion_omega, motion_phase;
+} titan_params;
+
+typedef struct {
+    int64_t steps_done;
+    double dt_last;              /* dt used by the last step                                   */
+    double dt_next;              /* min over interior faces computed in the last step          */
+    double t_sim;                /* simulated time                                             */
+    int64_t n_rho_floor, n_p_floor; /* floor activations (Setka_physics.cpp:1784,1981), last step */
+    int64_t n_fallback;          /* cells that used the exact source in MLP mode, last step    */
+    double ms_total;             /* wall time of the titan_go call (HIP events)                */
+} titan_step_stats;
+
+/* ---- context ------------------------------------------------------------------------------- */
+/* Replaces Setka::Setka + Calculating_measure + Init_physics for the hot-path subset. */
+int titan_create(titan_ctx **ctx, const titan_mesh *mesh, const titan_params *params);
+int titan_destroy(titan_ctx *ctx);
+/* Last error of the context; index = reference face/cell id of the first failure or -1. */
+int titan_last_error(titan_ctx *ctx, int64_t *index);
+
+/* ---- state ---------------------------------------------------------------------------------- */
+/* Cell::parameters[layer] for the 8 plasma keys, [n_cells*8] cell-major. */
+int titan_set_state(titan_ctx *ctx, const double *prim, int layer);
+int titan_get_state(titan_ctx *ctx, double *prim, int layer);
+/* Yzel::coord[layer], [n_nodes*3]. */
+int titan_set_nodes(titan_ctx *ctx, const double *xyz, int layer);
+int titan_get_nodes(titan_ctx *ctx, double *xyz, int layer);
+/* Neutral-H moments per cell for the charge-exchange source: [n_cells*5] = n_H, c_H^2, V_H. */
+int titan_set_neutrals(titan_ctx *ctx, const double *nh5);
+
+/* ---- geometry ------------------------------------------------------------------------------- */
+/* Setka::Calculating_measure(layer) (Setka.cpp:2465-2502) on the device, in the context's
+ * geometry mode.  titan_get_geometry copies any non-NULL output in reference numbering. */
+int titan_calculating_measure(titan_ctx *ctx, int layer);
+int titan_get_geometry(titan_ctx *ctx, int layer, double *cell_center, double *cell_volume,
+                       double *face_center, double *face_normal, double *face_area);
+
+/* ---- face loop -------------------------------------------------------------------------------- */
+/* One pass of the face loop of Setka::Go (Setka_physics.cpp:1634-1662) calling
+ * Setka::Culc_Gran_Potok (Setka_physics.cpp:2280-2571) for every face, with states of layer
+ * `now`, geometry of layer `now` and, when the mesh moves, face velocities from layers now and
+ * 1-now with time step dt.  Outputs (nullable): face_flux [n_faces*9] = 8 fluxes * area
+ * (Prho, PVx..PVz, Pp(energy), PBx..PBz) and PdivB; dt_min = min over interior faces of the
+ * CFL time step; branch [n_faces] = n_state<<4 | ibn<<3 | region. */
+int titan_gran_potok(titan_ctx *ctx, int now, double dt, double *face_flux, double *dt_min,
+                     uint8_t *branch);
+
+/* ---- step --------------------------------------------------------------------------------- */
+/* Setka::Go(is_inner_area, steps) (Setka_physics.cpp:1484-2278) over the whole mesh: the first
+ * call copies layer 0 to layer 1 and starts the lagged dt at 1e-5; later calls continue. */
+int titan_go(titan_ctx *ctx, int64_t steps, titan_step_stats *stats);
+/* Which layer holds the newest state after titan_go. */
+int titan_current_layer(titan_ctx *ctx);
+
+/* ---- Riemann solver ---------------------------------------------------------------------- */
+/* Phys_param::chlld (Phys_param.cpp:882-1878; Phys_param.h:240-251) over n independent face
+ * states on the device.  normal [n*3] unit, w [n] (nullable = 0), qL/qR [n*8], K passive
+ * scalars kL/kR [n*K] (K <= 4), outputs q [n*8], k [n*K], ds [n*3] = (dsl, dsc, dsr),
+ * branch [n].  precision = TITAN_PREC_*.  Returns TITAN_E_RIEMANN if any state hit the
+ * reference's fatal branch (its q is then 0). */
+int titan_chlld_batch(int64_t n, const int8_t *n_state, const int8_t *n_disc, const uint8_t *flags,
+                      const double *normal, const double *w, const double *qL, const double *qR,
+                      int K, const double *kL, const double *kR, double *q, double *k, double *ds,
+                      uint8_t *branch, int precision);
+
+/* ---- charge-exchange source ---------------------------------------------------------------- */
+/* Stands in for Setka::Calc_sourse_MF_Bera (Setka_physics.cpp:766-1258) as the plasma source:
+ * out [n_cells*5] = exchange rate, momentum (3), energy (lab frame), per unit volume and time,
+ * for the states of `layer` and the neutrals of titan_set_neutrals.  mode = TITAN_SRC_*. */
+int titan_sources(titan_ctx *ctx, int layer, int mode, double *out);
+/* Standalone exact source for n states: in [n*10] = n_p, c_p^2, V_p(3), n_H, c_H^2, V_H(3). */
+int titan_cx_exact_batch(int64_t n, const double *in10, int nodes, double *out5);
+
+/* ---- surrogate network (8-64-64-64-5 SiLU) ------------------------------------------------- */
+/* Parameter layout (FP32): W1[64x8] b1[64] W2[64x64] b2[64] W3[64x64] b3[64] W4[5x64] b4[5],
+ * rows = outputs; plus input normalisation mean/std [8+8], output scale [5], input box
+ * lo/hi [8+8] outside which the exact source is used. */
+int64_t titan_mlp_n_params(void);
+int titan_mlp_set(titan_ctx *ctx, const float *params, const float *in_norm16, const float *out_scale5,
+                  const float *box16);
+int titan_mlp_get(titan_ctx *ctx, float *params);
+/* Forward pass of n samples of already-normalised inputs x [n*8] -> y [n*5] on the BF16-MFMA
+ * kernel (FP32 accumulation). */
+int titan_mlp_forward(titan_ctx *ctx, int64_t n, const float *x, float *y);
+/* Loss (MSE) and gradient [n_params] of n samples on the device in FP32. */
+int titan_mlp_loss_grad(titan_ctx *ctx, int64_t n, const float *x, const float *y, float *loss,
+                        float *grad);
+/* Adam training on the device: n samples x [n*8], y [n*5] (normalised), `epochs` passes of
+ * mini-batches of `batch`, learning rate lr; loss_hist [epochs] (nullable). */
+int titan_mlp_train(titan_ctx *ctx, int64_t n, const float *x, const float *y, int epochs, int batch,
+                    float lr, uint64_t seed, float *loss_hist);
+
+/* ---- mesh construction helpers (host) -------------------------------------------------------- */
+/* Setka::New_connect (Setka.cpp:2066-2249): faces of n_cells hexahedra; face_nodes [6*n_cells*4],
+ * face_cells [6*n_cells*2], cell_faces [n_cells*6] sized by the caller; returns n_faces (< 0 on
+ * error). */
+int64_t titan_mesh_connect(int64_t n_cells, const int32_t *cell_nodes, int64_t n_nodes,
+                           int32_t *face_nodes, int32_t *face_cells, int32_t *cell_faces);
+/* Synthetic meshes: jittered n^3 cube on [0,1]^3 ((n+1)^3 nodes), and a spherical shell of
+ * n_layers radial layers x 6*n_edge^2 cube-sphere quads (radii [n_layers+1]). */
+int titan_mesh_cube(int32_t n, double jitter, uint64_t seed, double *xyz, int32_t *cell_nodes);
+int titan_mesh_shell(int32_t n_edge, int32_t n_layers, const double *radii, double *xyz,
+                     int32_t *cell_nodes);
+
+/* ---- timing --------------------------------------------------------------------------------- */
+/* Mean device time (ms) per launch of the named kernel family over the last titan_go call,
+ * measured with HIP events on the context stream: which = 0 step kernel (flux+update),
+ * 1 geometry, 2 motion, 3 source.  Returns -1 if not measured. */
+double titan_kernel_ms(titan_ctx *ctx, int which);
+int titan_set_timing(titan_ctx *ctx, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

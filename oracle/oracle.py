@@ -1,0 +1,215 @@
+"""ctypes loader of the CPU oracle (liboracle.so).  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module, and only
+as the checker or the timed CPU baseline; the product path never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+
+
+class OrMesh(ctypes.Structure):
+    _fields_ = [("n_nodes", _I64), ("n_cells", _I64), ("n_faces", _I64), ("cell_nodes", _P), ("cell_faces", _P),
+                ("face_nodes", _P), ("face_cells", _P), ("face_bc", _P), ("face_method", _P), ("face_flags", _P),
+                ("face_bstate", _P), ("cell_zone", _P), ("cell_is_tvd", _P)]
+
+
+class OrParams(ctypes.Structure):
+    _fields_ = [("gamma", _D), ("kfl", _D), ("v_inf", _D), ("move_mesh", ctypes.c_int32), ("n_disc", ctypes.c_int32),
+                ("motion_omega", _D), ("motion_phase", _D)]
+
+
+class OrGeom(ctypes.Structure):
+    _fields_ = [("cell_center", _P), ("cell_volume", _P), ("face_center", _P), ("face_normal", _P),
+                ("face_area", _P)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "or_chlld": (_I, [_I, _I, _I, _D, _D, _D, _D, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+            "or_chlld_batch": (_I, [_I64, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P]),
+            "or_get_bazis": (None, [_P, _P, _P]),
+            "or_get_potok": (None, [_D] * 12 + [_P]),
+            "or_measure": (None, [_P, _P, _P]),
+            "or_measure_gcl": (None, [_P, _P, _P]),
+            "or_det_cos": (_D, [_D]),
+            "or_motion": (None, [_I64, _P, _P, _D, _D, _D, _D, _P]),
+            "or_face_flux": (_I, [_P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
+            "or_update": (_I, [_P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
+            "or_go": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _P, _P, _P]),
+            "or_new_connect": (_I64, [_I64, _P, _I64, _P, _P, _P]),
+            "or_cx_rate_ref": (_D, [_D, _D, _D, _I, _I]),
+            "or_sigma_ls": (_D, [_D]),
+            "or_cx_source": (None, [_D, _D, _P, _D, _D, _P, _I, _I, _P]),
+            "or_mlp_n_params": (_I64, []),
+            "or_mlp_forward": (None, [_P, _I64, _P, _P]),
+            "or_mlp_loss_grad": (_D, [_P, _I64, _P, _P, _P]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class OracleMesh:
+    """Holds contiguous copies of a configs.Mesh and its ctypes view."""
+
+    def __init__(self, mesh):
+        self.m = mesh
+        self.arrs = dict(
+            cell_nodes=np.ascontiguousarray(mesh.cell_nodes, np.int32), cell_faces=np.ascontiguousarray(mesh.cell_faces, np.int32),
+            face_nodes=np.ascontiguousarray(mesh.face_nodes, np.int32), face_cells=np.ascontiguousarray(mesh.face_cells, np.int32),
+            face_bc=np.ascontiguousarray(mesh.face_bc, np.int8), face_method=np.ascontiguousarray(mesh.face_method, np.int8),
+            face_flags=None if mesh.face_flags is None else np.ascontiguousarray(mesh.face_flags, np.uint8),
+            face_bstate=None if mesh.face_bstate is None else np.ascontiguousarray(mesh.face_bstate, np.float64),
+            cell_zone=np.ascontiguousarray(mesh.cell_zone, np.int8), cell_is_tvd=np.ascontiguousarray(mesh.cell_is_tvd, np.uint8))
+        c = OrMesh()
+        c.n_nodes, c.n_cells, c.n_faces = mesh.n_nodes, mesh.n_cells, mesh.n_faces
+        for k, v in self.arrs.items():
+            setattr(c, k, _p(v))
+        self.c = c
+
+
+def params_c(params):
+    p = OrParams()
+    for k, _ in OrParams._fields_:
+        setattr(p, k, params[k])
+    return p
+
+
+def measure(om, xyz, gcl=False):
+    m = om.m
+    out = dict(cell_center=np.empty((m.n_cells, 3)), cell_volume=np.empty(m.n_cells),
+               face_center=np.empty((m.n_faces, 3)), face_normal=np.empty((m.n_faces, 3)), face_area=np.empty(m.n_faces))
+    g = OrGeom(*[_p(out[k]) for k, _ in OrGeom._fields_])
+    x = np.ascontiguousarray(xyz, np.float64)
+    (lib().or_measure_gcl if gcl else lib().or_measure)(ctypes.byref(om.c), _p(x), ctypes.byref(g))
+    return out
+
+
+def face_flux(om, params, prim, geom, x1=None, x2=None, dt=1e-5):
+    m = om.m
+    g = OrGeom(*[_p(geom[k]) for k, _ in OrGeom._fields_])
+    prim = np.ascontiguousarray(prim, np.float64)
+    flux = np.empty((m.n_faces, 9))
+    dtf = np.empty(m.n_faces)
+    br = np.empty(m.n_faces, np.uint8)
+    x1 = None if x1 is None else np.ascontiguousarray(x1, np.float64)
+    x2 = None if x2 is None else np.ascontiguousarray(x2, np.float64)
+    rc = lib().or_face_flux(ctypes.byref(om.c), ctypes.byref(params_c(params)), _p(prim), ctypes.byref(g), _p(x1),
+                            _p(x2), float(dt), _p(flux), _p(dtf), _p(br))
+    return rc, flux, dtf, br
+
+
+def go(om, params, xyz, prim, steps, gcl=False, nthreads=1, disp=None, t_sim=0.0):
+    """Runs or_go; returns (rc, prim_final, xyz_final, dt_hist, t_sim)."""
+    m = om.m
+    coords = np.zeros((2, m.n_nodes, 3))
+    coords[0] = xyz
+    state = np.zeros((2, m.n_cells, 8))
+    state[0] = prim
+    dth = np.zeros(max(1, steps))
+    layer = ctypes.c_int(0)
+    ts = ctypes.c_double(t_sim)
+    d = None if disp is None else np.ascontiguousarray(disp, np.float64)
+    rc = lib().or_go(ctypes.byref(om.c), ctypes.byref(params_c(params)), _p(coords), _p(state), _p(d), int(steps),
+                     1 if gcl else 0, int(nthreads), ctypes.byref(layer), _p(dth), ctypes.byref(ts))
+    return rc, state[layer.value].copy(), coords[layer.value].copy(), dth[:steps], ts.value
+
+
+def chlld_batch(n_state, normal, qL, qR, w=None, n_disc=None, flags=None, kL=None, kR=None):
+    qL = np.ascontiguousarray(qL, np.float64).reshape(-1, 8)
+    qR = np.ascontiguousarray(qR, np.float64).reshape(-1, 8)
+    n = qL.shape[0]
+    ns = np.ascontiguousarray(np.broadcast_to(n_state, (n,)), np.int8)
+    nd = None if n_disc is None else np.ascontiguousarray(np.broadcast_to(n_disc, (n,)), np.int8)
+    fl = None if flags is None else np.ascontiguousarray(np.broadcast_to(flags, (n,)), np.uint8)
+    nrm = np.ascontiguousarray(normal, np.float64).reshape(n, 3)
+    ww = None if w is None else np.ascontiguousarray(np.broadcast_to(w, (n,)), np.float64)
+    K = 0 if kL is None else int(np.asarray(kL).reshape(n, -1).shape[1])
+    kLa = None if K == 0 else np.ascontiguousarray(kL, np.float64).reshape(n, K)
+    kRa = None if K == 0 else np.ascontiguousarray(kR, np.float64).reshape(n, K)
+    q = np.zeros((n, 8))
+    k = np.zeros((n, max(K, 1)))
+    ds = np.zeros((n, 3))
+    br = np.zeros(n, np.uint8)
+    rc = lib().or_chlld_batch(n, _p(ns), _p(nd), _p(fl), _p(nrm), _p(ww), _p(qL), _p(qR), K, _p(kLa), _p(kRa), _p(q),
+                              _p(k) if K else None, _p(ds), _p(br))
+    return q, k[:, :K], ds, br, rc
+
+
+def get_bazis(n):
+    n = np.ascontiguousarray(n, np.float64)
+    t, m = np.zeros(3), np.zeros(3)
+    lib().or_get_bazis(_p(n), _p(t), _p(m))
+    return t, m
+
+
+def get_potok(gamma, prim, n):
+    out = np.zeros(8)
+    lib().or_get_potok(gamma, prim[0], prim[4], prim[1], prim[2], prim[3], prim[5], prim[6], prim[7], n[0], n[1], n[2],
+                       _p(out))
+    return out
+
+
+def new_connect(cell_nodes, n_nodes):
+    cn = np.ascontiguousarray(cell_nodes, np.int32).reshape(-1)
+    nc = cn.size // 8
+    fn = np.empty(24 * nc, np.int32)
+    fc = np.empty(12 * nc, np.int32)
+    cf = np.empty(6 * nc, np.int32)
+    nf = lib().or_new_connect(nc, _p(cn), int(n_nodes), _p(fn), _p(fc), _p(cf))
+    return fn[:4 * nf].reshape(nf, 4), fc[:2 * nf].reshape(nf, 2), cf.reshape(nc, 6)
+
+
+def cx_source(n_p, cp2, vp, n_h, ch2, vh, nodes=100):
+    out = np.zeros(5)
+    vp = np.ascontiguousarray(vp, np.float64)
+    vh = np.ascontiguousarray(vh, np.float64)
+    lib().or_cx_source(n_p, cp2, _p(vp), n_h, ch2, _p(vh), nodes, nodes, _p(out))
+    return out
+
+
+def mlp_forward(params, x):
+    P = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 8)
+    y = np.zeros((x.shape[0], 5))
+    lib().or_mlp_forward(_p(P), x.shape[0], _p(x), _p(y))
+    return y
+
+
+def mlp_loss_grad(params, x, y):
+    P = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 8)
+    y = np.ascontiguousarray(y, np.float64).reshape(-1, 5)
+    g = np.zeros(P.size)
+    loss = lib().or_mlp_loss_grad(_p(P), x.shape[0], _p(x), _p(y), _p(g))
+    return loss, g

@@ -1,0 +1,1304 @@
+/*
+ * titan_oracle.c -- CPU restatement of TITAN's explicit ideal-MHD step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker for the HIP path and the CPU baseline of the
+ * benchmark.  Nothing in the product links it.  Build: gcc -O2 -ffp-contract=off -fopenmp
+ * (no FMA contraction, IEEE division and square root), as in oracle/Makefile.
+ *
+ * The arithmetic follows the reference expression by expression (C evaluates a+b+c as (a+b)+c,
+ * the reference's compiler does the same with contraction off).  Reference citations are given
+ * per function.  Eigen small-vector reductions (Vector3d::dot / norm / normalized) are taken in
+ * the order (x0*y0 + x1*y1) + x2*y2, which is the order of Eigen's unaligned-packet reduction of
+ * a 3-vector; the reference binary cannot be built here (Eigen absent), so this is an assumption
+ * that only affects the last bit.
+ */
+#include "titan_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define KV(x) ((x) * (x))
+#define KVV(x, y, z) ((x) * (x) + (y) * (y) + (z) * (z))
+/* std::max / std::min semantics (first argument returned on ties and NaN) */
+#define SMAX(a, b) (((a) < (b)) ? (b) : (a))
+#define SMIN(a, b) (((b) < (a)) ? (b) : (a))
+
+static const double OR_PI = 3.14159265358979323846;
+static const double OR_EPSB = 1E-4; /* Header.h:96 */
+
+static double or_spi4(void) { return sqrt(4.0 * OR_PI); } /* Header.h:94 */
+
+/* get_bazis, Help_funk.cpp:219-232: t = normalize(a x n), m = n x t, a = x or y. */
+void or_get_bazis(const double *n, double *t, double *m)
+{
+    double a0 = 1.0, a1 = 0.0, a2 = 0.0;
+    double nd = (n[0] * a0 + n[1] * a1) + n[2] * a2;
+    if (fabs(nd) > 0.9) { a0 = 0.0; a1 = 1.0; a2 = 0.0; }
+    double c0 = a1 * n[2] - a2 * n[1];
+    double c1 = a2 * n[0] - a0 * n[2];
+    double c2 = a0 * n[1] - a1 * n[0];
+    double z = (c0 * c0 + c1 * c1) + c2 * c2;
+    if (z > 0.0) {
+        double s = sqrt(z);
+        c0 = c0 / s; c1 = c1 / s; c2 = c2 / s;
+    }
+    t[0] = c0; t[1] = c1; t[2] = c2;
+    m[0] = n[1] * t[2] - n[2] * t[1];
+    m[1] = n[2] * t[0] - n[0] * t[2];
+    m[2] = n[0] * t[1] - n[1] * t[0];
+}
+
+static double dot3(const double *a, const double *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+/* scalarProductFast(a..., b...) = ax*bx + ay*by + az*bz (Help_funk.cpp:246-251) */
+static double spf(double ax, double ay, double az, double bx, double by, double bz)
+{
+    return ax * bx + ay * by + az * bz;
+}
+
+/* Phys_param::Get_Potok, Phys_param.cpp:826-874 (used by tests as a consistency oracle). */
+void or_get_potok(double gamma, double rho, double p, double u, double v, double w, double bx,
+                  double by, double bz, double n1, double n2, double n3, double *potok)
+{
+    const double spi4 = or_spi4();
+    double n[3] = {n1, n2, n3}, t[3], m[3];
+    or_get_bazis(n, t, m);
+    double vn = spf(u, v, w, n1, n2, n3);
+    double vt = spf(u, v, w, t[0], t[1], t[2]);
+    double vm = spf(u, v, w, m[0], m[1], m[2]);
+    double Bn = spf(bx, by, bz, n1, n2, n3) / spi4;
+    double Bt = spf(bx, by, bz, t[0], t[1], t[2]) / spi4;
+    double Bm = spf(bx, by, bz, m[0], m[1], m[2]) / spi4;
+    double bb = KVV(bx, by, bz) / spi4 / spi4;
+    double vv = KVV(u, v, w);
+    double e = p / (gamma - 1) + rho * vv / 2.0 + bb / 2.0;
+    double pT = p + bb / 2.0;
+    potok[0] = rho * vn;
+    potok[7] = (e + pT) * vn - Bn * spf(bx, by, bz, u, v, w) / spi4;
+    double Pvn = rho * KV(vn) + pT - KV(Bn);
+    double Pvt = rho * vn * vt - Bn * Bt;
+    double Pvm = rho * vn * vm - Bn * Bm;
+    double Pbt = Bt * vn - Bn * vt;
+    double Pbm = Bm * vn - Bn * vm;
+    Pbt *= spi4;
+    Pbm *= spi4;
+    potok[1] = Pvn * n[0] + Pvt * t[0] + Pvm * m[0];
+    potok[2] = Pvn * n[1] + Pvt * t[1] + Pvm * m[1];
+    potok[3] = Pvn * n[2] + Pvt * t[2] + Pvm * m[2];
+    potok[4] = Pbt * t[0] + Pbm * m[0];
+    potok[5] = Pbt * t[1] + Pbm * m[1];
+    potok[6] = Pbt * t[2] + Pbm * m[2];
+}
+
+#define OR_KMAX 8
+
+/* Phys_param::chlld, Phys_param.cpp:882-1878.  n_state 0 Lax, 1 HLL, 2 HLLC, 3 HLLD.
+ * flags bit0 = left_ydar, bit1 = contact; null_bn is always false on the step path
+ * (Setka_physics.cpp:2420-2424) and is not exposed. */
+int or_chlld(int n_state, int n_disc, int flags, double al, double be, double ge, double w,
+             const double *qL, const double *qR, int K, const double *kL, const double *kR,
+             double *q, double *k, double *dsl, double *dsc, double *dsr, int *branch)
+{
+    const double spi4 = or_spi4();
+    const double gamma = 5.0 / 3.0;
+    const double g1 = gamma - 1.0;
+    const int left_ydar = flags & 1, contact = (flags >> 1) & 1;
+    double FL[8], FR[8], UL[8], UR[8], UZ[8];
+    double FLk[OR_KMAX], FRk[OR_KMAX], ULk[OR_KMAX], URk[OR_KMAX], UZk[OR_KMAX];
+    if (K > OR_KMAX) return -2;
+    double wv = w;
+
+    double r1 = qL[0], u1 = qL[1], v1 = qL[2], w1 = qL[3], p1 = qL[4];
+    double bx1 = qL[5] / spi4, by1 = qL[6] / spi4, bz1 = qL[7] / spi4;
+    double r2 = qR[0], u2 = qR[1], v2 = qR[2], w2 = qR[3], p2 = qR[4];
+    double bx2 = qR[5] / spi4, by2 = qR[6] / spi4, bz2 = qR[7] / spi4;
+
+    double ro = (r2 + r1) / 2.0;
+    double ap = (p2 + p1) / 2.0;
+
+    double n[3] = {al, be, ge}, t[3], m[3];
+    or_get_bazis(n, t, m);
+
+    double vL[3], vR[3], bL[3], bR[3];
+    vL[0] = spf(n[0], n[1], n[2], u1, v1, w1);
+    vL[1] = spf(t[0], t[1], t[2], u1, v1, w1);
+    vL[2] = spf(m[0], m[1], m[2], u1, v1, w1);
+    vR[0] = spf(n[0], n[1], n[2], u2, v2, w2);
+    vR[1] = spf(t[0], t[1], t[2], u2, v2, w2);
+    vR[2] = spf(m[0], m[1], m[2], u2, v2, w2);
+    bL[0] = spf(n[0], n[1], n[2], bx1, by1, bz1);
+    bL[1] = spf(t[0], t[1], t[2], bx1, by1, bz1);
+    bL[2] = spf(m[0], m[1], m[2], bx1, by1, bz1);
+    bR[0] = spf(n[0], n[1], n[2], bx2, by2, bz2);
+    bR[1] = spf(t[0], t[1], t[2], bx2, by2, bz2);
+    bR[2] = spf(m[0], m[1], m[2], bx2, by2, bz2);
+
+    double aaL = bL[0] / sqrt(r1);
+    double b2L = dot3(bL, bL);
+    double b21 = b2L / r1;
+    double cL = sqrt(gamma * p1 / r1);
+    double qp = sqrt(b21 + cL * (cL + 2.0 * aaL));
+    double qm = sqrt(b21 + cL * (cL - 2.0 * aaL));
+    double cfL = (qp + qm) / 2.0;
+    double ptL = p1 + b2L / 2.0;
+
+    double aaR = bR[0] / sqrt(r2);
+    double b2R = dot3(bR, bR);
+    double b22 = b2R / r2;
+    double cR = sqrt(gamma * p2 / r2);
+    qp = sqrt(b22 + cR * (cR + 2.0 * aaR));
+    qm = sqrt(b22 + cR * (cR - 2.0 * aaR));
+    double cfR = (qp + qm) / 2.0;
+    double ptR = p2 + b2R / 2.0;
+
+    double aC = (aaL + aaR) / 2.0;
+    double b2o = (b22 + b21) / 2.0;
+    double cC = sqrt(gamma * ap / ro);
+    qp = sqrt(b2o + cC * (cC + 2.0 * aC));
+    qm = sqrt(b2o + cC * (cC - 2.0 * aC));
+    double cfC = (qp + qm) / 2.0;
+    double vC1 = (vL[0] + vR[0]) / 2.0;
+
+    double SL = 0.0, SR = 0.0;
+    if (left_ydar) n_disc = 2;
+    if (n_disc == 0) {
+        SL = SMIN((vL[0] - cfL), (vR[0] - cfR));
+        SR = SMAX((vL[0] + cfL), (vR[0] + cfR));
+    } else if (n_disc == 1) {
+        SL = SMIN((vL[0] - cfL), (vC1 - cfC));
+        SR = SMAX((vR[0] + cfR), (vC1 + cfC));
+    } else if (n_disc == 2) {
+        double SL_1 = SMIN((vL[0] - cfL), (vC1 - cfC));
+        double SR_1 = SMAX((vR[0] + cfR), (vC1 + cfC));
+        double SL_2 = SMIN((vL[0] - cfL), (vR[0] - cfR));
+        double SR_2 = SMAX((vL[0] + cfL), (vR[0] + cfR));
+        double oo = 0.6;
+        double oo1 = 1.0 - oo;
+        SL = oo * SL_1 + oo1 * SL_2;
+        SR = oo * SR_1 + oo1 * SR_2;
+    } else if (n_disc == 3) {
+        SL = SMIN(vL[0], vR[0]) - SMAX(cfL, cfR);
+        SR = SMAX(vL[0], vR[0]) + SMAX(cfL, cfR);
+    }
+    *dsr = SR;
+    *dsl = SL;
+
+    double suR = SR - vR[0];
+    double suL = SL - vL[0];
+    double SM = (suR * r2 * vR[0] - ptR + ptL - suL * r1 * vL[0]) / (suR * r2 - suL * r1);
+    *dsc = SM;
+    if (contact) wv = SM;
+
+    if (n_state == 0) {
+        double TR0 = fabs(vL[0] + vR[0]) / 2.0 + cfC;
+        double TL0 = -TR0;
+        SR = TR0;
+        SL = TL0;
+    }
+
+    double upt1 = (KV(u1) + KV(v1) + KV(w1)) / 2.0;
+    double sbv1 = u1 * bx1 + v1 * by1 + w1 * bz1;
+    double upt2 = (KV(u2) + KV(v2) + KV(w2)) / 2.0;
+    double sbv2 = u2 * bx2 + v2 * by2 + w2 * bz2;
+    double e1 = p1 / g1 + r1 * upt1 + b2L / 2.0;
+    double e2 = p2 / g1 + r2 * upt2 + b2R / 2.0;
+
+    FL[0] = r1 * vL[0];
+    FL[1] = r1 * vL[0] * vL[0] + ptL - KV(bL[0]);
+    FL[2] = r1 * vL[0] * vL[1] - bL[0] * bL[1];
+    FL[3] = r1 * vL[0] * vL[2] - bL[0] * bL[2];
+    FL[4] = (e1 + ptL) * vL[0] - bL[0] * sbv1;
+    FL[5] = 0.0;
+    FL[6] = vL[0] * bL[1] - vL[1] * bL[0];
+    FL[7] = vL[0] * bL[2] - vL[2] * bL[0];
+    for (int i = 0; i < K; i++) FLk[i] = kL[i] * vL[0];
+
+    FR[0] = r2 * vR[0];
+    FR[1] = r2 * vR[0] * vR[0] + ptR - KV(bR[0]);
+    FR[2] = r2 * vR[0] * vR[1] - bR[0] * bR[1];
+    FR[3] = r2 * vR[0] * vR[2] - bR[0] * bR[2];
+    FR[4] = (e2 + ptR) * vR[0] - bR[0] * sbv2;
+    FR[5] = 0.0;
+    FR[6] = vR[0] * bR[1] - vR[1] * bR[0];
+    FR[7] = vR[0] * bR[2] - vR[2] * bR[0];
+    for (int i = 0; i < K; i++) FRk[i] = kR[i] * vR[0];
+
+    UL[0] = r1; UL[4] = e1; UR[0] = r2; UR[4] = e2;
+    for (int i = 0; i < K; i++) { ULk[i] = kL[i]; URk[i] = kR[i]; }
+    for (int i = 0; i < 3; i++) {
+        UL[i + 1] = r1 * vL[i];
+        UL[i + 5] = bL[i];
+        UR[i + 1] = r2 * vR[i];
+        UR[i + 5] = bR[i];
+    }
+    for (int ik = 0; ik < 8; ik++) UZ[ik] = (SR * UR[ik] - SL * UL[ik] + FL[ik] - FR[ik]) / (SR - SL);
+    for (int ik = 0; ik < K; ik++) UZk[ik] = (SR * URk[ik] - SL * ULk[ik] + FLk[ik] - FRk[ik]) / (SR - SL);
+
+    double qq[8];      /* rotated-frame flux, components 0..7 */
+    double SN, wbn;
+    int region = 0, ibn = 0;
+
+    if (n_state <= 1) { /* Lax or HLL, Phys_param.cpp:1133-1257 */
+        double dq[8], FW[8], dqk[OR_KMAX], FWk[OR_KMAX];
+        for (int ik = 0; ik < 8; ik++) dq[ik] = UR[ik] - UL[ik];
+        for (int ik = 0; ik < K; ik++) dqk[ik] = URk[ik] - ULk[ik];
+        if (left_ydar) wv = SL;
+        double TL = SL, TR = SR;
+        if (SL > wv || left_ydar) {
+            TL = 0.0;
+            for (int ik = 0; ik < 8; ik++) FW[ik] = wv * UL[ik];
+            for (int ik = 0; ik < K; ik++) FWk[ik] = wv * ULk[ik];
+            region = 1;
+        } else if (SL <= wv && wv <= SR) {
+            for (int ik = 0; ik < 8; ik++) FW[ik] = wv * UZ[ik];
+            for (int ik = 0; ik < K; ik++) FWk[ik] = wv * UZk[ik];
+            region = 2;
+        } else if (SR < wv) {
+            TR = 0.0;
+            for (int ik = 0; ik < 8; ik++) FW[ik] = wv * UR[ik];
+            for (int ik = 0; ik < K; ik++) FWk[ik] = wv * URk[ik];
+            region = 3;
+        } else {
+            if (branch) *branch = (n_state << 4);
+            return -1;
+        }
+        double a = TR * TL;
+        double b = TR - TL;
+        for (int ik = 0; ik < 8; ik++) qq[ik] = (TR * FL[ik] - TL * FR[ik] + a * dq[ik]) / b - FW[ik];
+        for (int ik = 0; ik < K; ik++) k[ik] = (TR * FLk[ik] - TL * FRk[ik] + a * dqk[ik]) / b - FWk[ik];
+        SN = SMAX(fabs(SL), fabs(SR));
+        if (wv >= SR) wbn = wv * bR[0];
+        else if (wv <= SL) wbn = wv * bL[0];
+        else wbn = wv * (bL[0] + bR[0]) / 2.0;
+    } else if (n_state == 2) { /* HLLC, Phys_param.cpp:1258-1475 */
+        double suRm = suR / (SR - SM);
+        double suLm = suL / (SL - SM);
+        double rzR = r2 * suRm;
+        double rzL = r1 * suLm;
+        double vzR[3], vzL[3], bzR[3], bzL[3];
+        vzR[0] = SM;
+        vzL[0] = SM;
+        double ptz = (suR * r2 * ptL - suL * r1 * ptR + r1 * r2 * suR * suL * (vR[0] - vL[0])) /
+                     (suR * r2 - suL * r1);
+        bzR[0] = UZ[5];
+        bzL[0] = UZ[5];
+        vzR[1] = vR[1] + UZ[5] * (bR[1] - UZ[6]) / suR / r2;
+        vzR[2] = vR[2] + UZ[5] * (bR[2] - UZ[7]) / suR / r2;
+        vzL[1] = vL[1] + UZ[5] * (bL[1] - UZ[6]) / suL / r1;
+        vzL[2] = vL[2] + UZ[5] * (bL[2] - UZ[7]) / suL / r1;
+        bzR[1] = UZ[6];
+        bzR[2] = UZ[7];
+        bzL[1] = bzR[1];
+        bzL[2] = bzR[2];
+        double sbvz = (UZ[5] * UZ[1] + UZ[6] * UZ[2] + UZ[7] * UZ[3]) / UZ[0];
+        double ezR = e2 * suRm + (ptz * SM - ptR * vR[0] + UZ[5] * (sbv2 - sbvz)) / (SR - SM);
+        double ezL = e1 * suLm + (ptz * SM - ptL * vL[0] + UZ[5] * (sbv1 - sbvz)) / (SL - SM);
+        if (fabs(UZ[5]) <= OR_EPSB) {
+            vzR[1] = vR[1];
+            vzR[2] = vR[2];
+            vzL[1] = vL[1];
+            vzL[2] = vL[2];
+            bzR[1] = bR[1] * suRm;
+            bzR[2] = bR[2] * suRm;
+            bzL[1] = bL[1] * suLm;
+            bzL[2] = bL[2] * suLm;
+        }
+        double UZL[8], UZR[8], UZLk[OR_KMAX], UZRk[OR_KMAX];
+        for (int ik = 0; ik < K; ik++) { UZLk[ik] = kL[ik] * suLm; UZRk[ik] = kR[ik] * suRm; }
+        UZL[0] = rzL; UZL[4] = ezL; UZR[0] = rzR; UZR[4] = ezR;
+        for (int ik = 0; ik < 3; ik++) {
+            UZL[ik + 1] = vzL[ik] * rzL;
+            UZL[ik + 5] = bzL[ik];
+            UZR[ik + 1] = vzR[ik] * rzR;
+            UZR[ik + 5] = bzR[ik];
+        }
+        if (left_ydar) wv = SL;
+        if (SL >= wv || left_ydar) {
+            for (int ik = 0; ik < 8; ik++) qq[ik] = FL[ik] - wv * UL[ik];
+            for (int ik = 0; ik < K; ik++) k[ik] = FLk[ik] - wv * ULk[ik];
+            region = 1;
+        } else if (SL <= wv && SM >= wv) {
+            for (int ik = 0; ik < 8; ik++) qq[ik] = FL[ik] + SL * (UZL[ik] - UL[ik]) - wv * UZL[ik];
+            for (int ik = 0; ik < K; ik++) k[ik] = FLk[ik] + SL * (UZLk[ik] - kL[ik]) - wv * UZLk[ik];
+            region = 2;
+        } else if (SM <= wv && SR > wv) {
+            for (int ik = 0; ik < 8; ik++) qq[ik] = FR[ik] + SR * (UZR[ik] - UR[ik]) - wv * UZR[ik];
+            for (int ik = 0; ik < K; ik++) k[ik] = FRk[ik] + SR * (UZRk[ik] - kR[ik]) - wv * UZRk[ik];
+            region = 3;
+        } else if (SR <= wv) {
+            for (int ik = 0; ik < 8; ik++) qq[ik] = FR[ik] - wv * UR[ik];
+            for (int ik = 0; ik < K; ik++) k[ik] = FRk[ik] - wv * URk[ik];
+            region = 4;
+        } else {
+            if (branch) *branch = (n_state << 4);
+            return -1;
+        }
+        SN = SMAX(fabs(SL), fabs(SR));
+        if (wv > SR) wbn = wv * bR[0];
+        else if (wv < SL) wbn = wv * bL[0];
+        else wbn = wv * (bL[0] + bR[0]) / 2.0;
+    } else if (n_state == 3) { /* HLLD, Phys_param.cpp:1476-1870 */
+        double vzR[3], vzL[3], vzzL[3], vzzR[3], bzR[3], bzL[3], bzzL[3], bzzR[3];
+        double ptz = (suR * r2 * ptL - suL * r1 * ptR + r1 * r2 * suR * suL * (vR[0] - vL[0])) /
+                     (suR * r2 - suL * r1);
+        vzL[0] = SM; vzR[0] = SM; vzzL[0] = SM; vzzR[0] = SM;
+        double suRm = suR / (SR - SM);
+        double suLm = suL / (SL - SM);
+        double rzR = r2 * suRm;
+        double rzL = r1 * suLm;
+        double qzL[OR_KMAX], qzR[OR_KMAX];
+        for (int ik = 0; ik < K; ik++) { qzL[ik] = kL[ik] * suLm; qzR[ik] = kR[ik] * suRm; }
+        double bn = UZ[5];
+        double bn2 = bn * bn;
+        bzL[0] = bn; bzR[0] = bn; bzzL[0] = bn; bzzR[0] = bn;
+        double tvR, tbR;
+        double ttR = r2 * suR * (SR - SM) - bn2;
+        if (fabs(ttR) < 0.000000001) { tvR = 0.0; tbR = 0.0; }
+        else { tvR = (SM - vR[0]) / ttR; tbR = (r2 * suR * suR - bn2) / ttR; }
+        double tvL, tbL;
+        double ttL = r1 * suL * (SL - SM) - bn2;
+        if (fabs(ttL) < 0.000000001) { tvL = 0.0; tbL = 0.0; }
+        else { tvL = (SM - vL[0]) / ttL; tbL = (r1 * suL * suL - bn2) / ttL; }
+        vzL[1] = vL[1] - bn * bL[1] * tvL;
+        vzL[2] = vL[2] - bn * bL[2] * tvL;
+        vzR[1] = vR[1] - bn * bR[1] * tvR;
+        vzR[2] = vR[2] - bn * bR[2] * tvR;
+        bzL[1] = bL[1] * tbL;
+        bzL[2] = bL[2] * tbL;
+        bzR[1] = bR[1] * tbR;
+        bzR[2] = bR[2] * tbR;
+        double sbvL = bzL[0] * vzL[0] + bzL[1] * vzL[1] + bzL[2] * vzL[2];
+        double sbvR = bzR[0] * vzR[0] + bzR[1] * vzR[1] + bzR[2] * vzR[2];
+        double ezR = e2 * suRm + (ptz * SM - ptR * vR[0] + bn * (sbv2 - sbvR)) / (SR - SM);
+        double ezL = e1 * suLm + (ptz * SM - ptL * vL[0] + bn * (sbv1 - sbvL)) / (SL - SM);
+        double rzzR = rzR;
+        double rzzL = rzL;
+        double rzRs = sqrt(rzR);
+        double rzLs = sqrt(rzL);
+        double rzss = rzRs + rzLs;
+        double rzps = rzRs * rzLs;
+        double SZL = SM - fabs(bn) / rzLs;
+        double SZR = SM + fabs(bn) / rzRs;
+        double sbn;
+        if (fabs(bn) > OR_EPSB) { sbn = fabs(bn) / bn; ibn = 1; }
+        else { sbn = 0.0; ibn = 0; SZL = SM; SZR = SM; }
+        vzzL[1] = (rzLs * vzL[1] + rzRs * vzR[1] + sbn * (bzR[1] - bzL[1])) / rzss;
+        vzzL[2] = (rzLs * vzL[2] + rzRs * vzR[2] + sbn * (bzR[2] - bzL[2])) / rzss;
+        vzzR[1] = vzzL[1];
+        vzzR[2] = vzzL[2];
+        bzzL[1] = (rzLs * bzR[1] + rzRs * bzL[1] + sbn * rzps * (vzR[1] - vzL[1])) / rzss;
+        bzzL[2] = (rzLs * bzR[2] + rzRs * bzL[2] + sbn * rzps * (vzR[2] - vzL[2])) / rzss;
+        bzzR[1] = bzzL[1];
+        bzzR[2] = bzzL[2];
+        double sbzz = bzzL[0] * vzzL[0] + bzzL[1] * vzzL[1] + bzzL[2] * vzzL[2];
+        double ezzR = ezR + rzRs * sbn * (sbvR - sbzz);
+        double ezzL = ezL - rzLs * sbn * (sbvL - sbzz);
+        double UZL[8], UZR[8], UZZL[8], UZZR[8];
+        UZL[0] = rzL; UZL[4] = ezL; UZR[0] = rzR; UZR[4] = ezR;
+        for (int ik = 0; ik < 3; ik++) {
+            UZL[ik + 1] = vzL[ik] * rzL;
+            UZL[ik + 5] = bzL[ik];
+            UZR[ik + 1] = vzR[ik] * rzR;
+            UZR[ik + 5] = bzR[ik];
+        }
+        UZZL[0] = rzzL; UZZL[4] = ezzL; UZZR[0] = rzzR; UZZR[4] = ezzR;
+        for (int ik = 0; ik < 3; ik++) {
+            UZZL[ik + 1] = vzzL[ik] * rzzL;
+            UZZL[ik + 5] = bzzL[ik];
+            UZZR[ik + 1] = vzzR[ik] * rzzR;
+            UZZR[ik + 5] = bzzR[ik];
+        }
+        /* passive scalars: UZZLk = UZLk = qzL (Phys_param.cpp:1560-1566,1656-1662) */
+        int j = -1;
+        if (left_ydar) wv = SL;
+        if (SL >= wv || left_ydar) {
+            for (int ik = 0; ik < 8; ik++) qq[ik] = FL[ik] - wv * UL[ik];
+            for (int ik = 0; ik < K; ik++) k[ik] = FLk[ik] - wv * ULk[ik];
+            j = 1;
+        } else if (SL <= wv && SZL >= wv) {
+            for (int ik = 0; ik < 8; ik++) qq[ik] = FL[ik] + SL * (UZL[ik] - UL[ik]) - wv * UZL[ik];
+            for (int ik = 0; ik < K; ik++) k[ik] = FLk[ik] + SL * (qzL[ik] - ULk[ik]) - wv * qzL[ik];
+            j = 2;
+        }
+        if (ibn == 1) {
+            if (SZL <= wv && SM >= wv) {
+                for (int ik = 0; ik < 8; ik++)
+                    qq[ik] = FL[ik] + SZL * (UZZL[ik] - UZL[ik]) + SL * (UZL[ik] - UL[ik]) - wv * UZZL[ik];
+                for (int ik = 0; ik < K; ik++)
+                    k[ik] = FLk[ik] + SZL * (qzL[ik] - qzL[ik]) + SL * (qzL[ik] - ULk[ik]) - wv * qzL[ik];
+                j = 3;
+            } else if (SM <= wv && SZR >= wv) {
+                for (int ik = 0; ik < 8; ik++)
+                    qq[ik] = FR[ik] + SZR * (UZZR[ik] - UZR[ik]) + SR * (UZR[ik] - UR[ik]) - wv * UZZR[ik];
+                for (int ik = 0; ik < K; ik++)
+                    k[ik] = FRk[ik] + SZR * (qzR[ik] - qzR[ik]) + SR * (qzR[ik] - URk[ik]) - wv * qzR[ik];
+                j = 4;
+            }
+        }
+        if (SZR <= wv && SR >= wv) {
+            for (int ik = 0; ik < 8; ik++) qq[ik] = FR[ik] + SR * (UZR[ik] - UR[ik]) - wv * UZR[ik];
+            for (int ik = 0; ik < K; ik++) k[ik] = FRk[ik] + SR * (qzR[ik] - URk[ik]) - wv * qzR[ik];
+            j = 5;
+        } else if (SR <= wv) {
+            for (int ik = 0; ik < 8; ik++) qq[ik] = FR[ik] - wv * UR[ik];
+            for (int ik = 0; ik < K; ik++) k[ik] = FRk[ik] - wv * URk[ik];
+            j = 6;
+        }
+        if (j == -1) {
+            if (branch) *branch = (n_state << 4) | (ibn << 3);
+            return -1;
+        }
+        region = j;
+        SN = SMAX(fabs(SL), fabs(SR));
+        if (wv > SR) wbn = wv * bR[0];
+        else if (wv < SL) wbn = wv * bL[0];
+        else wbn = wv * (bL[0] + bR[0]) / 2.0;
+    } else {
+        return -3;
+    }
+
+    /* Bn flux and rotation back to the lab frame (identical in all three solvers) */
+    double qv0 = qq[1], qv1 = qq[2], qv2 = qq[3];
+    double qb0 = -SN * (bR[0] - bL[0]) - wbn, qb1 = qq[6], qb2 = qq[7];
+    q[0] = qq[0];
+    q[4] = qq[4];
+    for (int ik = 0; ik < 3; ik++) {
+        q[ik + 1] = n[ik] * qv0 + t[ik] * qv1 + m[ik] * qv2;
+        double b = n[ik] * qb0 + t[ik] * qb1 + m[ik] * qb2;
+        q[ik + 5] = spi4 * b;
+    }
+    if (branch) *branch = (n_state << 4) | (ibn << 3) | region;
+    return 0;
+}
+
+int or_chlld_batch(int64_t n, const int8_t *n_state, const int8_t *n_disc, const uint8_t *flags,
+                   const double *normal, const double *w, const double *qL, const double *qR,
+                   int K, const double *kL, const double *kR, double *q, double *k, double *ds,
+                   uint8_t *branch)
+{
+    int status = 0;
+    for (int64_t i = 0; i < n; i++) {
+        double dsl, dsc, dsr;
+        int br = 0;
+        int rc = or_chlld(n_state[i], n_disc ? n_disc[i] : 0, flags ? flags[i] : 0, normal[3 * i],
+                          normal[3 * i + 1], normal[3 * i + 2], w ? w[i] : 0.0, qL + 8 * i, qR + 8 * i, K,
+                          K ? kL + K * i : NULL, K ? kR + K * i : NULL, q + 8 * i, K ? k + K * i : NULL,
+                          &dsl, &dsc, &dsr, &br);
+        if (ds) { ds[3 * i] = dsl; ds[3 * i + 1] = dsc; ds[3 * i + 2] = dsr; }
+        if (branch) branch[i] = (uint8_t)br;
+        if (rc != 0 && status == 0) status = rc;
+    }
+    return status;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Geometry, reference mode.                                                                  */
+
+static void tri_area_add(const double *a, const double *b, const double *c, double *S)
+{
+    /* triangleArea3D, Help_funk.cpp:193-217 */
+    double v1x = b[0] - a[0], v1y = b[1] - a[1], v1z = b[2] - a[2];
+    double v2x = c[0] - a[0], v2y = c[1] - a[1], v2z = c[2] - a[2];
+    double cx = v1y * v2z - v1z * v2y;
+    double cy = v1z * v2x - v1x * v2z;
+    double cz = v1x * v2y - v1y * v2x;
+    *S += 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+}
+
+static double tet_volume(const double *A, const double *B, const double *C, const double *D)
+{
+    /* tetrahedronVolume, Help_funk.cpp:253-272 */
+    double abx = B[0] - A[0], aby = B[1] - A[1], abz = B[2] - A[2];
+    double acx = C[0] - A[0], acy = C[1] - A[1], acz = C[2] - A[2];
+    double adx = D[0] - A[0], ady = D[1] - A[1], adz = D[2] - A[2];
+    double cx = aby * acz - abz * acy;
+    double cy = abz * acx - abx * acz;
+    double cz = abx * acy - aby * acx;
+    double d = cx * adx + cy * ady + cz * adz;
+    return fabs(d) / 6.0;
+}
+
+void or_measure(const or_mesh *m, const double *X, or_geom *g)
+{
+    /* Cell::Culc_center (Cell.cpp:909-930) */
+    for (int64_t c = 0; c < m->n_cells; c++) {
+        double xc = 0.0, yc = 0.0, zc = 0.0;
+        for (int i = 0; i < 8; i++) {
+            const double *x = X + 3 * (int64_t)m->cell_nodes[8 * c + i];
+            xc += x[0]; yc += x[1]; zc += x[2];
+        }
+        g->cell_center[3 * c] = xc / 8;
+        g->cell_center[3 * c + 1] = yc / 8;
+        g->cell_center[3 * c + 2] = zc / 8;
+    }
+    /* Gran::Culc_measure (Gran.cpp:112-218) */
+    for (int64_t f = 0; f < m->n_faces; f++) {
+        const double *x[4];
+        for (int i = 0; i < 4; i++) x[i] = X + 3 * (int64_t)m->face_nodes[4 * f + i];
+        double xc = 0.0, yc = 0.0, zc = 0.0;
+        for (int i = 0; i < 4; i++) { xc += x[i][0]; yc += x[i][1]; zc += x[i][2]; }
+        xc /= 4; yc /= 4; zc /= 4;
+        g->face_center[3 * f] = xc;
+        g->face_center[3 * f + 1] = yc;
+        g->face_center[3 * f + 2] = zc;
+        double S = 0.0;
+        tri_area_add(x[0], x[1], x[2], &S);
+        tri_area_add(x[0], x[3], x[2], &S);
+        g->face_area[f] = S;
+        double x1 = (x[0][0] + x[1][0]) / 2.0, y1 = (x[0][1] + x[1][1]) / 2.0, z1 = (x[0][2] + x[1][2]) / 2.0;
+        double x2 = (x[1][0] + x[2][0]) / 2.0, y2 = (x[1][1] + x[2][1]) / 2.0, z2 = (x[1][2] + x[2][2]) / 2.0;
+        double x3 = (x[2][0] + x[3][0]) / 2.0, y3 = (x[2][1] + x[3][1]) / 2.0, z3 = (x[2][2] + x[3][2]) / 2.0;
+        double ax = x2 - x1, ay = y2 - y1, az = z2 - z1;
+        double bx = x3 - x1, by = y3 - y1, bz = z3 - z1;
+        double n1 = ay * bz - az * by;
+        double n2 = az * bx - ax * bz;
+        double n3 = ax * by - ay * bx;
+        double nn = sqrt(KVV(n1, n2, n3));
+        n1 /= nn; n2 /= nn; n3 /= nn;
+        const double *cc = g->cell_center + 3 * (int64_t)m->face_cells[2 * f];
+        if (spf(n1, n2, n3, cc[0] - xc, cc[1] - yc, cc[2] - zc) > 0.0) { n1 = -n1; n2 = -n2; n3 = -n3; }
+        g->face_normal[3 * f] = n1;
+        g->face_normal[3 * f + 1] = n2;
+        g->face_normal[3 * f + 2] = n3;
+    }
+    /* Cell::Culc_volume, method 1 (Cell.cpp:958-1000): 2 tets per face, faces ascending */
+    for (int64_t c = 0; c < m->n_cells; c++) {
+        const double *cc = g->cell_center + 3 * c;
+        double V = 0.0;
+        for (int j = 0; j < 6; j++) {
+            int64_t f = m->cell_faces[6 * c + j];
+            const double *x0 = X + 3 * (int64_t)m->face_nodes[4 * f + 0];
+            const double *x1 = X + 3 * (int64_t)m->face_nodes[4 * f + 1];
+            const double *x2 = X + 3 * (int64_t)m->face_nodes[4 * f + 2];
+            const double *x3 = X + 3 * (int64_t)m->face_nodes[4 * f + 3];
+            V += tet_volume(cc, x0, x1, x2);
+            V += tet_volume(cc, x0, x3, x2);
+        }
+        g->cell_volume[c] = V;
+    }
+}
+
+/* Geometry, exactly-conservative mode (not in the reference): face vector area
+ * S = 1/2 (x2-x0) x (x3-x1) (exact for a bilinear quad), unit normal S/|S| oriented
+ * cells[0] -> cells[1], area |S|, and cell volume V = 1/3 sum_f s_f xbar_f . S_f, which is exact
+ * for hexahedra with bilinear faces.  Cell and face centres are the reference's node means. */
+void or_measure_gcl(const or_mesh *m, const double *X, or_geom *g)
+{
+    for (int64_t c = 0; c < m->n_cells; c++) {
+        double xc = 0.0, yc = 0.0, zc = 0.0;
+        for (int i = 0; i < 8; i++) {
+            const double *x = X + 3 * (int64_t)m->cell_nodes[8 * c + i];
+            xc += x[0]; yc += x[1]; zc += x[2];
+        }
+        g->cell_center[3 * c] = xc / 8;
+        g->cell_center[3 * c + 1] = yc / 8;
+        g->cell_center[3 * c + 2] = zc / 8;
+    }
+    for (int64_t f = 0; f < m->n_faces; f++) {
+        const double *x[4];
+        for (int i = 0; i < 4; i++) x[i] = X + 3 * (int64_t)m->face_nodes[4 * f + i];
+        double xc = 0.0, yc = 0.0, zc = 0.0;
+        for (int i = 0; i < 4; i++) { xc += x[i][0]; yc += x[i][1]; zc += x[i][2]; }
+        xc /= 4; yc /= 4; zc /= 4;
+        g->face_center[3 * f] = xc;
+        g->face_center[3 * f + 1] = yc;
+        g->face_center[3 * f + 2] = zc;
+        double ax = x[2][0] - x[0][0], ay = x[2][1] - x[0][1], az = x[2][2] - x[0][2];
+        double bx = x[3][0] - x[1][0], by = x[3][1] - x[1][1], bz = x[3][2] - x[1][2];
+        double s1 = 0.5 * (ay * bz - az * by);
+        double s2 = 0.5 * (az * bx - ax * bz);
+        double s3 = 0.5 * (ax * by - ay * bx);
+        double A = sqrt(KVV(s1, s2, s3));
+        double n1 = s1 / A, n2 = s2 / A, n3 = s3 / A;
+        const double *cc = g->cell_center + 3 * (int64_t)m->face_cells[2 * f];
+        if (spf(n1, n2, n3, cc[0] - xc, cc[1] - yc, cc[2] - zc) > 0.0) { n1 = -n1; n2 = -n2; n3 = -n3; }
+        g->face_normal[3 * f] = n1;
+        g->face_normal[3 * f + 1] = n2;
+        g->face_normal[3 * f + 2] = n3;
+        g->face_area[f] = A;
+    }
+    for (int64_t c = 0; c < m->n_cells; c++) {
+        double V = 0.0;
+        for (int j = 0; j < 6; j++) {
+            int64_t f = m->cell_faces[6 * c + j];
+            double s = (m->face_cells[2 * f] == c) ? 1.0 : -1.0;
+            const double *n = g->face_normal + 3 * f;
+            const double *fc = g->face_center + 3 * f;
+            double A = g->face_area[f];
+            V += s * (spf(fc[0], fc[1], fc[2], n[0], n[1], n[2]) * A);
+        }
+        g->cell_volume[c] = V / 3.0;
+    }
+}
+
+/* Gran::culc_velosity, Gran.cpp:571-589 */
+double or_face_velocity(const or_mesh *m, int64_t f, const double *x1, const double *x2,
+                        const double *normal1, double dt)
+{
+    const double *n = normal1 + 3 * f;
+    double ddot = 0.0;
+    for (int i = 0; i < 4; i++) {
+        int64_t v = m->face_nodes[4 * f + i];
+        double d0 = x2[3 * v] - x1[3 * v];
+        double d1 = x2[3 * v + 1] - x1[3 * v + 1];
+        double d2 = x2[3 * v + 2] - x1[3 * v + 2];
+        ddot = ddot + ((d0 * n[0] + d1 * n[1]) + d2 * n[2]);
+    }
+    ddot = ddot / 4 / dt;
+    return ddot;
+}
+
+/* Exactly-conservative face velocity: w = (volume swept by the face) / (dt * area_old).
+ * The swept region is the hexahedron (old face, new face) with bilinear sides; its volume is
+ * 1/3 sum over its 6 faces of xbar . S. */
+static double hex_volume_exact(const double P[8][3])
+{
+    static const int lf[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4}, {2, 3, 7, 6}, {1, 2, 6, 5}, {0, 4, 7, 3}};
+    double V = 0.0;
+    for (int j = 0; j < 6; j++) {
+        const double *x0 = P[lf[j][0]], *x1 = P[lf[j][1]], *x2 = P[lf[j][2]], *x3 = P[lf[j][3]];
+        double ax = x2[0] - x0[0], ay = x2[1] - x0[1], az = x2[2] - x0[2];
+        double bx = x3[0] - x1[0], by = x3[1] - x1[1], bz = x3[2] - x1[2];
+        double s1 = 0.5 * (ay * bz - az * by);
+        double s2 = 0.5 * (az * bx - ax * bz);
+        double s3 = 0.5 * (ax * by - ay * bx);
+        double xc = (((x0[0] + x1[0]) + x2[0]) + x3[0]) / 4;
+        double yc = (((x0[1] + x1[1]) + x2[1]) + x3[1]) / 4;
+        double zc = (((x0[2] + x1[2]) + x2[2]) + x3[2]) / 4;
+        V += spf(xc, yc, zc, s1, s2, s3);
+    }
+    return V / 3.0;
+}
+
+static double or_face_velocity_gcl(const or_mesh *m, int64_t f, const double *x1, const double *x2,
+                                   const double *normal1, const double *area1, double dt)
+{
+    /* hexahedron: bottom = old face (nodes 0..3), top = new face (4..7); oriented so that the
+     * outward normal of the old-face side is -n (the face sweeps along +n for positive w) */
+    double P[8][3];
+    for (int i = 0; i < 4; i++) {
+        int64_t v = m->face_nodes[4 * f + i];
+        for (int d = 0; d < 3; d++) { P[i][d] = x1[3 * v + d]; P[4 + i][d] = x2[3 * v + d]; }
+    }
+    double V = hex_volume_exact(P);
+    /* orientation: local face 0 = (0,3,2,1) of the old nodes; its vector area is -S_old(0,1,2,3).
+     * The sign of V is + when the nodes move along the node-order normal of (0,1,2,3). */
+    const double *n = normal1 + 3 * f;
+    const double *x0 = P[0], *xa = P[1], *xb = P[2], *xd = P[3];
+    double ax = xb[0] - x0[0], ay = xb[1] - x0[1], az = xb[2] - x0[2];
+    double bx = xd[0] - xa[0], by = xd[1] - xa[1], bz = xd[2] - xa[2];
+    double s1 = ay * bz - az * by, s2 = az * bx - ax * bz, s3 = ax * by - ay * bx;
+    double sg = (spf(s1, s2, s3, n[0], n[1], n[2]) >= 0.0) ? 1.0 : -1.0;
+    return (sg * V) / (dt * area1[f]);
+}
+
+/* Deterministic cosine (Cody-Waite reduction by pi/2 in three parts, Taylor polynomials on
+ * [-pi/4, pi/4]).  Used only by the prescribed node motion so the CPU and the GPU move nodes
+ * identically; accurate to a few ulp for |x| < 1e5. */
+double or_det_cos(double x)
+{
+    const double two_over_pi = 0.63661977236758134308;
+    const double p1 = 1.5707963267341256e+00;
+    const double p2 = 6.0771005065061922e-11;
+    const double p3 = 2.0222662487959506e-21;
+    double kf = nearbyint(x * two_over_pi);
+    double r = ((x - kf * p1) - kf * p2) - kf * p3;
+    long k = (long)kf;
+    double r2 = r * r;
+    double s = r * (1.0 + r2 * (-1.0 / 6.0 + r2 * (1.0 / 120.0 + r2 * (-1.0 / 5040.0 + r2 * (1.0 / 362880.0 +
+               r2 * (-1.0 / 39916800.0 + r2 * (1.0 / 6227020800.0 + r2 * (-1.0 / 1307674368000.0))))))));
+    double c = 1.0 + r2 * (-0.5 + r2 * (1.0 / 24.0 + r2 * (-1.0 / 720.0 + r2 * (1.0 / 40320.0 + r2 * (-1.0 / 3628800.0 +
+               r2 * (1.0 / 479001600.0 + r2 * (-1.0 / 87178291200.0 + r2 * (1.0 / 20922789888000.0))))))));
+    switch (((k % 4) + 4) % 4) {
+    case 0: return c;
+    case 1: return -s;
+    case 2: return -c;
+    default: return s;
+    }
+}
+
+void or_motion(int64_t n_nodes, const double *x_old, const double *disp, double dt, double t,
+               double omega, double phase, double *x_new)
+{
+    double fac = dt * (omega * or_det_cos(omega * t + phase));
+    for (int64_t i = 0; i < 3 * n_nodes; i++) x_new[i] = x_old[i] + fac * disp[i];
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Face states: Setka::Snos_on_Gran, first order (TVD off) and boundary faces.                */
+
+static double polar_angle(double x, double y)
+{
+    /* Help_funk.cpp:78-106 */
+    if (fabs(x) + fabs(y) < 0.000001) return 0.0;
+    if (x < 0) return atan(y / x) + 1.0 * OR_PI;
+    else if (x > 0 && y >= 0) return atan(y / x);
+    else if (x > 0 && y < 0) return atan(y / x) + 2.0 * OR_PI;
+    else if (y > 0 && x >= 0 && x <= 0) return OR_PI / 2.0;
+    else if (y < 0 && x >= 0 && x <= 0) return 3.0 * OR_PI / 2.0;
+    return 0.0;
+}
+
+static void spherical_skorost(double z, double x, double y, double Vz, double Vx, double Vy,
+                              double *Vr, double *Vphi, double *Vtheta)
+{
+    /* Help_funk.cpp:122-135 */
+    double r_1 = sqrt(x * x + y * y + z * z);
+    double the_1 = acos(z / r_1);
+    double phi_1 = polar_angle(x, y);
+    *Vr = Vx * sin(the_1) * cos(phi_1) + Vy * sin(the_1) * sin(phi_1) + Vz * cos(the_1);
+    *Vtheta = Vx * cos(the_1) * cos(phi_1) + Vy * cos(the_1) * sin(phi_1) - Vz * sin(the_1);
+    *Vphi = -Vx * sin(phi_1) + Vy * cos(phi_1);
+}
+
+static void dekard_skorost(double z, double x, double y, double Vr, double Vphi, double Vtheta,
+                           double *Vz, double *Vx, double *Vy)
+{
+    /* Help_funk.cpp:108-120 */
+    double r_2 = sqrt(x * x + y * y + z * z);
+    double the_2 = acos(z / r_2);
+    double phi_2 = polar_angle(x, y);
+    *Vx = Vr * sin(the_2) * cos(phi_2) + Vtheta * cos(the_2) * cos(phi_2) - Vphi * sin(phi_2);
+    *Vy = Vr * sin(the_2) * sin(phi_2) + Vtheta * cos(the_2) * sin(phi_2) + Vphi * cos(phi_2);
+    *Vz = Vr * cos(the_2) - Vtheta * sin(the_2);
+}
+
+/* Zone_1 spherical rescaling of a cell state to the face centre (Setka_gran_snos.cpp:744-873). */
+static void zone1_state(const double *s, const double *Ac, const double *G, double gamma, double *out)
+{
+    double r1 = sqrt((Ac[0] * Ac[0] + Ac[1] * Ac[1]) + Ac[2] * Ac[2]);
+    double rr = sqrt((G[0] * G[0] + G[1] * G[1]) + G[2] * G[2]);
+    double the1 = acos(G[2] / rr);
+    for (int ik = 0; ik < 2; ik++) {
+        const double *V = s + (ik == 0 ? 1 : 5);
+        double *O = out + (ik == 0 ? 1 : 5);
+        double VA0, VA1, VA2;
+        if (the1 > OR_PI / 9 && the1 < 8 * OR_PI / 9) {
+            spherical_skorost(Ac[2], Ac[0], Ac[1], V[2], V[0], V[1], &VA0, &VA1, &VA2);
+            if (ik == 1) {
+                VA0 = VA0 * (KV(r1 / rr));
+                VA1 = VA1 * ((r1) / (rr));
+                VA2 = VA2 * ((r1) / (rr));
+            }
+            dekard_skorost(G[2], G[0], G[1], VA0, VA1, VA2, &O[2], &O[0], &O[1]);
+        } else {
+            spherical_skorost(Ac[0], Ac[1], Ac[2], V[0], V[1], V[2], &VA0, &VA1, &VA2);
+            if (ik == 1) {
+                VA0 = VA0 * (KV(r1 / rr));
+                VA1 = VA1 * ((r1) / (rr));
+                VA2 = VA2 * ((r1) / (rr));
+            }
+            dekard_skorost(G[0], G[1], G[2], VA0, VA1, VA2, &O[0], &O[1], &O[2]);
+        }
+    }
+    out[0] = s[0] * KV(r1) / KV(rr);
+    out[4] = s[4] * pow(r1, 2.0 * gamma) / pow(rr, 2.0 * gamma);
+}
+
+int or_face_states(const or_mesh *m, const or_params *p, int64_t f, const double *prim,
+                   const or_geom *g, double *qL, double *qR)
+{
+    int bc = m->face_bc[f];
+    int64_t A = m->face_cells[2 * f];
+    const double *sA = prim + 8 * A;
+    if (bc != OR_BC_INTERIOR) {
+        const double *n = g->face_normal + 3 * f;
+        if (bc == OR_BC_OUTER_SOFT) { /* plasma_condition 1: Setka_gran_snos.cpp:18-50 */
+            for (int i = 0; i < 8; i++) qL[i] = sA[i];
+            if (n[0] < -0.95) {
+                if (qL[1] > p->v_inf / 7.0) qL[1] = p->v_inf / 5.0;
+            } else {
+                if (qL[1] * n[0] + qL[2] * n[1] + qL[3] * n[2] < 0.0) {
+                    qL[1] = 0.1 * n[0];
+                    qL[2] = 0.1 * n[1];
+                    qL[3] = 0.1 * n[2];
+                }
+            }
+            for (int i = 0; i < 8; i++) qR[i] = qL[i];
+        } else if (bc == OR_BC_INNER_HARD || bc == OR_BC_OUTER_HARD) { /* plasma_condition 2 */
+            for (int i = 0; i < 8; i++) { qL[i] = sA[i]; qR[i] = m->face_bstate[8 * f + i]; }
+        } else if (bc == OR_BC_ZERO_GRADIENT) {
+            for (int i = 0; i < 8; i++) { qL[i] = sA[i]; qR[i] = sA[i]; }
+        } else {
+            return -10;
+        }
+        return 0;
+    }
+    int64_t B = m->face_cells[2 * f + 1];
+    const double *sB = prim + 8 * B;
+    const double *G = g->face_center + 3 * f;
+    if (m->cell_zone[A] == 1) zone1_state(sA, g->cell_center + 3 * A, G, p->gamma, qL);
+    else for (int i = 0; i < 8; i++) qL[i] = sA[i];
+    if (m->cell_zone[B] == 1) zone1_state(sB, g->cell_center + 3 * B, G, p->gamma, qR);
+    else for (int i = 0; i < 8; i++) qR[i] = sB[i];
+    return 0;
+}
+
+/* Setka::Culc_Gran_Potok (Setka_physics.cpp:2280-2571), all faces.  x_now1/x_now2 are used only
+ * when the mesh moves (face velocity).  gcl = 0 reference geometry. */
+static int face_flux_impl(const or_mesh *m, const or_params *p, const double *prim, const or_geom *g,
+                          const double *x1, const double *x2, double dt, int gcl, double *flux,
+                          double *dt_face, uint8_t *branch, int nthreads)
+{
+    int status = 0;
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t f = 0; f < m->n_faces; f++) {
+        double qL[8], qR[8], q[8], dsl, dsc, dsr;
+        int br = 0;
+        double area = g->face_area[f];
+        const double *n = g->face_normal + 3 * f;
+        double w = 0.0;
+        if (p->move_mesh) {
+            w = gcl ? or_face_velocity_gcl(m, f, x1, x2, g->face_normal, g->face_area, dt)
+                    : or_face_velocity(m, f, x1, x2, g->face_normal, dt);
+        }
+        double loc_time = 10000000.0;
+        double dist = 0.0;
+        int interior = (m->face_bc[f] == OR_BC_INTERIOR);
+        if (interior) {
+            const double *ca = g->cell_center + 3 * (int64_t)m->face_cells[2 * f];
+            const double *cb = g->cell_center + 3 * (int64_t)m->face_cells[2 * f + 1];
+            double d0 = ca[0] - cb[0], d1 = ca[1] - cb[1], d2 = ca[2] - cb[2];
+            dist = sqrt(KVV(d0, d1, d2)) / 2.0;
+        }
+        int rc = or_face_states(m, p, f, prim, g, qL, qR);
+        if (rc == 0)
+            rc = or_chlld(m->face_method[f], p->n_disc, m->face_flags ? m->face_flags[f] : 0, n[0], n[1],
+                          n[2], w, qL, qR, 0, NULL, NULL, q, NULL, &dsl, &dsc, &dsr, &br);
+        if (rc != 0) {
+#pragma omp critical(or_status)
+            { if (status == 0) status = rc; }
+            for (int i = 0; i < 9; i++) flux[9 * f + i] = 0.0;
+            dt_face[f] = loc_time;
+            if (branch) branch[f] = (uint8_t)br;
+            continue;
+        }
+        if (interior) {
+            double dnt = p->kfl * dist / (SMAX(fabs(dsl), fabs(dsr)) + fabs(w));
+            if (dnt < loc_time) loc_time = SMIN(loc_time, dnt);
+        }
+        for (int i = 0; i < 8; i++) flux[9 * f + i] = q[i] * area;
+        flux[9 * f + 8] = 0.5 * spf(n[0], n[1], n[2], qL[5] + qR[5], qL[6] + qR[6], qL[7] + qR[7]) * area;
+        dt_face[f] = loc_time;
+        if (branch) branch[f] = (uint8_t)br;
+    }
+    return status;
+}
+
+int or_face_flux(const or_mesh *m, const or_params *p, const double *prim, const or_geom *g,
+                 const double *x_now1, const double *x_now2, double dt, double *flux, double *dt_face,
+                 uint8_t *branch)
+{
+    return face_flux_impl(m, p, prim, g, x_now1, x_now2, dt, 0, flux, dt_face, branch, 1);
+}
+
+/* Cell loop of Setka::Go (Setka_physics.cpp:1695-2027) for the plasma without passive scalars
+ * and PUI: flux sum in the cell's ascending face order, Powell source, FP64 pressure recovery,
+ * Zone_2 magnetic decay, floors, B frozen for cells with is_TVD == false. */
+static int update_impl(const or_mesh *m, const or_params *p, const double *prim_old, const double *vol_old,
+                       const double *vol_new, const double *flux, const double *src, double time,
+                       double *prim_new, int64_t *n_rho_floor, int64_t *n_p_floor, int nthreads)
+{
+    const double cpi4 = 4.0 * OR_PI;
+    const double g1 = 5.0 / 3.0 - 1.0;
+    int64_t nrf = 0, npf = 0;
+    int status = 0;
+    (void)nthreads;
+    (void)p;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : nrf, npf) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t c = 0; c < m->n_cells; c++) {
+        double Volume = vol_old[c];
+        double Volume2 = vol_new[c];
+        double P[9];
+        for (int i = 0; i < 9; i++) P[i] = 0.0;
+        for (int j = 0; j < 6; j++) {
+            int64_t f = m->cell_faces[6 * c + j];
+            int sign_potok = 1;
+            if (m->face_cells[2 * f] != c) sign_potok = -1;
+            for (int i = 0; i < 9; i++) P[i] += sign_potok * flux[9 * f + i];
+        }
+        const double *s = prim_old + 8 * c;
+        double rho = s[0], vx = s[1], vy = s[2], vz = s[3], pp = s[4], bx = s[5], by = s[6], bz = s[7];
+        double Smx = 0.0, Smy = 0.0, Smz = 0.0, SE = 0.0;
+        if (src) { Smx = src[5 * c + 1]; Smy = src[5 * c + 2]; Smz = src[5 * c + 3]; SE = src[5 * c + 4]; }
+        double dsk = spf(vx, vy, vz, bx, by, bz);
+        double rho3 = rho * Volume / Volume2 - time * P[0] / Volume2;
+        int rho_null = 0;
+        if (rho3 < 1e-7) { rho_null = 1; rho3 = 0.1; nrf++; }
+        double u3, v3, w3, bx3, by3, bz3, p3;
+        if (!rho_null) {
+            u3 = (rho * vx * Volume / Volume2 - time * (P[1] + (bx / cpi4) * P[8]) / Volume2 + time * Smx) / rho3;
+            v3 = (rho * vy * Volume / Volume2 - time * (P[2] + (by / cpi4) * P[8]) / Volume2 + time * Smy) / rho3;
+            w3 = (rho * vz * Volume / Volume2 - time * (P[3] + (bz / cpi4) * P[8]) / Volume2 + time * Smz) / rho3;
+            bx3 = bx * Volume / Volume2 - time * (P[5] + vx * P[8]) / Volume2;
+            by3 = by * Volume / Volume2 - time * (P[6] + vy * P[8]) / Volume2;
+            bz3 = bz * Volume / Volume2 - time * (P[7] + vz * P[8]) / Volume2;
+            p3 = (((pp / g1 + 0.5 * rho * KVV(vx, vy, vz) + KVV(bx, by, bz) / 25.13274122871834590768) * Volume / Volume2
+                   - time * (P[4] + (dsk / cpi4) * P[8]) / Volume2 + time * SE)
+                  - 0.5 * rho3 * KVV(u3, v3, w3) - KVV(bx3, by3, bz3) / 25.13274122871834590768) * g1;
+            if (m->cell_zone[c] == 2) {
+                double tau = 1.0;
+                double bx4 = bx3 - time * bx3 / tau;
+                double by4 = by3 - time * by3 / tau;
+                double bz4 = bz3 - time * bz3 / tau;
+                double p4 = p3 + (time / tau - KV(time / tau)) * (g1) * KVV(bx3, by3, bz3) / (4.0 * OR_PI);
+                bx3 = bx4; by3 = by4; bz3 = bz4; p3 = p4;
+            }
+        } else {
+            u3 = vx; v3 = vy; w3 = vz; bx3 = bx; by3 = by; bz3 = bz;
+            p3 = rho3 / 2.0;
+        }
+        if (isnan(rho3)) {
+#pragma omp critical(or_status2)
+            { if (status == 0) status = -20; }
+        }
+        if (p3 < 0.00000001) { p3 = 0.000001; npf++; }
+        double *o = prim_new + 8 * c;
+        o[0] = rho3; o[1] = u3; o[2] = v3; o[3] = w3; o[4] = p3;
+        if (m->cell_is_tvd[c]) { o[5] = bx3; o[6] = by3; o[7] = bz3; }
+        else { o[5] = bx; o[6] = by; o[7] = bz; }
+    }
+    if (n_rho_floor) *n_rho_floor = nrf;
+    if (n_p_floor) *n_p_floor = npf;
+    return status;
+}
+
+int or_update(const or_mesh *m, const or_params *p, const double *prim_old, const double *vol_old,
+              const double *vol_new, const double *flux, const double *src, double dt, double *prim_new,
+              int64_t *n_rho_floor, int64_t *n_p_floor)
+{
+    return update_impl(m, p, prim_old, vol_old, vol_new, flux, src, dt, prim_new, n_rho_floor, n_p_floor, 1);
+}
+
+static void geom_alloc(or_geom *g, const or_mesh *m)
+{
+    g->cell_center = (double *)malloc(sizeof(double) * 3 * m->n_cells);
+    g->cell_volume = (double *)malloc(sizeof(double) * m->n_cells);
+    g->face_center = (double *)malloc(sizeof(double) * 3 * m->n_faces);
+    g->face_normal = (double *)malloc(sizeof(double) * 3 * m->n_faces);
+    g->face_area = (double *)malloc(sizeof(double) * m->n_faces);
+}
+static void geom_free(or_geom *g)
+{
+    free(g->cell_center); free(g->cell_volume); free(g->face_center); free(g->face_normal); free(g->face_area);
+}
+
+/* Setka::Go, Setka_physics.cpp:1484-2278: lagged dt (first step 1e-5), layer swap, optional
+ * motion and re-measure of the new layer, face loop, cell loop. */
+int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, const double *disp,
+          int64_t steps, int gcl, int nthreads, int *final_layer, double *dt_hist, double *t_sim)
+{
+    int64_t nn3 = 3 * m->n_nodes, nc8 = 8 * m->n_cells;
+    double *X[2] = {coords, coords + nn3};
+    double *S[2] = {prim, prim + nc8};
+    or_geom G[2];
+    geom_alloc(&G[0], m);
+    geom_alloc(&G[1], m);
+    double *flux = (double *)malloc(sizeof(double) * 9 * m->n_faces);
+    double *dtf = (double *)malloc(sizeof(double) * m->n_faces);
+    memcpy(X[1], X[0], sizeof(double) * nn3);
+    if (gcl) { or_measure_gcl(m, X[0], &G[0]); or_measure_gcl(m, X[1], &G[1]); }
+    else { or_measure(m, X[0], &G[0]); or_measure(m, X[1], &G[1]); }
+    memcpy(S[1], S[0], sizeof(double) * nc8);
+    int now1 = 1, now2 = 0;
+    double time = 0.00001, loc_time = 0.00001;
+    double tsim = t_sim ? *t_sim : 0.0;
+    int status = 0;
+    for (int64_t step = 1; step <= steps; step++) {
+        now2 = now1;
+        now1 = (now1 + 1) % 2;
+        time = loc_time;
+        loc_time = 100000000.0;
+        if (dt_hist) dt_hist[step - 1] = time;
+        if (p->move_mesh) {
+            or_motion(m->n_nodes, X[now1], disp, time, tsim, p->motion_omega, p->motion_phase, X[now2]);
+            if (gcl) or_measure_gcl(m, X[now2], &G[now2]);
+            else or_measure(m, X[now2], &G[now2]);
+        }
+        int rc = face_flux_impl(m, p, S[now1], &G[now1], X[now1], X[now2], time, gcl, flux, dtf, NULL, nthreads);
+        if (rc && !status) status = rc;
+        for (int64_t f = 0; f < m->n_faces; f++)
+            if (dtf[f] < loc_time) loc_time = SMIN(loc_time, dtf[f]);
+        rc = update_impl(m, p, S[now1], G[now1].cell_volume, G[now2].cell_volume, flux, NULL, time, S[now2],
+                         NULL, NULL, nthreads);
+        if (rc && !status) status = rc;
+        tsim += time;
+        if (status) break;
+    }
+    if (final_layer) *final_layer = now2;
+    if (t_sim) *t_sim = tsim;
+    geom_free(&G[0]);
+    geom_free(&G[1]);
+    free(flux);
+    free(dtf);
+    return status;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Setka::New_connect (Setka.cpp:2066-2249): 6 faces per hexahedron in a fixed local order,
+ * duplicates dropped keeping the first created copy (its node order and cells[0] = creator),
+ * the second creator appended as cells[1]; each cell's faces in ascending global id. */
+static const int OR_LOCAL_FACES[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                                         {2, 3, 7, 6}, {1, 2, 6, 5}, {0, 4, 7, 3}};
+
+typedef struct { int32_t k[4]; int64_t id; } or_fkey;
+
+static int fkey_cmp(const void *a, const void *b)
+{
+    const or_fkey *x = (const or_fkey *)a, *y = (const or_fkey *)b;
+    for (int i = 0; i < 4; i++) {
+        if (x->k[i] != y->k[i]) return x->k[i] < y->k[i] ? -1 : 1;
+    }
+    return x->id < y->id ? -1 : (x->id > y->id);
+}
+
+int64_t or_new_connect(int64_t n_cells, const int32_t *cell_nodes, int64_t n_nodes, int32_t *face_nodes,
+                       int32_t *face_cells, int32_t *cell_faces)
+{
+    (void)n_nodes;
+    int64_t nt = 6 * n_cells;
+    or_fkey *keys = (or_fkey *)malloc(sizeof(or_fkey) * nt);
+    for (int64_t c = 0; c < n_cells; c++) {
+        for (int l = 0; l < 6; l++) {
+            or_fkey *kk = &keys[6 * c + l];
+            for (int i = 0; i < 4; i++) kk->k[i] = cell_nodes[8 * c + OR_LOCAL_FACES[l][i]];
+            /* sort the 4 node ids (insertion sort) to compare node sets */
+            for (int i = 1; i < 4; i++) {
+                int32_t v = kk->k[i];
+                int j = i - 1;
+                while (j >= 0 && kk->k[j] > v) { kk->k[j + 1] = kk->k[j]; j--; }
+                kk->k[j + 1] = v;
+            }
+            kk->id = 6 * c + l;
+        }
+    }
+    qsort(keys, nt, sizeof(or_fkey), fkey_cmp);
+    /* twin[i] = id of the other copy of the same face, or -1 */
+    int64_t *twin = (int64_t *)malloc(sizeof(int64_t) * nt);
+    for (int64_t i = 0; i < nt; i++) twin[i] = -1;
+    for (int64_t i = 0; i + 1 < nt; i++) {
+        if (memcmp(keys[i].k, keys[i + 1].k, sizeof(int32_t) * 4) == 0) {
+            twin[keys[i].id] = keys[i + 1].id;
+            twin[keys[i + 1].id] = keys[i].id;
+        }
+    }
+    free(keys);
+    int64_t *gid = (int64_t *)malloc(sizeof(int64_t) * nt);
+    int64_t nf = 0;
+    for (int64_t i = 0; i < nt; i++) {
+        int64_t c = i / 6;
+        int l = (int)(i % 6);
+        if (twin[i] >= 0 && twin[i] < i) { gid[i] = gid[twin[i]]; continue; }
+        gid[i] = nf;
+        for (int j = 0; j < 4; j++) face_nodes[4 * nf + j] = cell_nodes[8 * c + OR_LOCAL_FACES[l][j]];
+        face_cells[2 * nf] = (int32_t)c;
+        face_cells[2 * nf + 1] = twin[i] >= 0 ? (int32_t)(twin[i] / 6) : -1;
+        nf++;
+    }
+    for (int64_t c = 0; c < n_cells; c++) {
+        int64_t v[6];
+        for (int l = 0; l < 6; l++) v[l] = gid[6 * c + l];
+        for (int i = 1; i < 6; i++) {
+            int64_t x = v[i];
+            int j = i - 1;
+            while (j >= 0 && v[j] > x) { v[j + 1] = v[j]; j--; }
+            v[j + 1] = x;
+        }
+        for (int l = 0; l < 6; l++) cell_faces[6 * c + l] = (int32_t)v[l];
+    }
+    free(twin);
+    free(gid);
+    return nf;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Charge exchange.                                                                           */
+
+/* MK_source.cpp:32-103 restated for one (U, c): midpoint rule in (R0 in [0,5c], alpha in
+ * [0,pi]) of g sigma(g) R0^2 sin(alpha) exp(-(R0/c)^2), times 2 pi dal dR0 / (sqrt(pi) c)^3. */
+double or_cx_rate_ref(double U, double c, double a2, int N1, int N2)
+{
+    double S = 0.0;
+    double dal = OR_PI / N2;
+    double dR0 = 5.0 * c / N1;
+    for (int ii = 0; ii < N1; ++ii) {
+        double R0 = (ii + 0.5) * 5.0 * c / N1;
+        for (int jj = 0; jj < N2; ++jj) {
+            double al = (jj + 0.5) * OR_PI / N2;
+            double R = sqrt(KV(R0 * sin(al)) + KV(U - R0 * cos(al)));
+            double sg = KV(1.0 - a2 * log(R));
+            S += R * sg * KV(R0) * sin(al) * exp(-KV(R0 / c));
+        }
+    }
+    S *= 2.0 * OR_PI * dal * dR0;
+    S /= pow(sqrt(OR_PI) * c, 3);
+    return S;
+}
+
+/* Lindsay & Stebbings (2005), J. Geophys. Res. 110, A12213, eq. (5):
+ *   sigma(E) = (4.15 - 0.531 ln E)^2 (1 - exp(-67.3/E))^4.5 * 1e-16 cm^2, E in keV.
+ * In code units: E = m_p g^2 / 2 with g in 10.3804 km/s; the result is multiplied by
+ * n_char * L_char = 0.06 cm^-3 * 4.2113 AU so that rates come out per unit code time. */
+double or_sigma_ls(double g)
+{
+    const double e_per_g2 = 0.5 * 1.67262192369e-24 * (1.03804e6 * 1.03804e6) / 1.602176634e-9; /* keV */
+    const double nl = 0.06 * 4.2113 * 1.495978707e13;
+    double E = e_per_g2 * g * g;
+    if (E < 1e-12) E = 1e-12;
+    double a = 4.15 - 0.531 * log(E);
+    double b = 1.0 - exp(-67.3 / E);
+    return a * a * pow(b, 4.5) * 1e-16 * nl;
+}
+
+/* Maxwellian charge-exchange source for the plasma.  With v_p = V_p + w_p, v_H = V_H + w_H
+ * (Gaussian, thermal speeds c_p, c_H), the relative velocity g = v_p - v_H is Gaussian about
+ * dV = V_p - V_H with thermal speed c = sqrt(c_p^2 + c_H^2), and E[w_p + w_H | g] =
+ * kappa (g - dV) with kappa = (c_p^2 - c_H^2)/c^2.  Hence, with J_k = <|g| sigma |g|^k (g.e)^..>:
+ *   rate  R   = n_p n_H J0,            J0 = < g sigma >
+ *   S_m       = -n_p n_H J1 e,         J1 = < g sigma (g.e) >
+ *   S_E (lab) = -(n_p n_H / 2) [ (V_p + V_H).e J1 + kappa (J2 - U J1) ],  J2 = < g^3 sigma >
+ * where e = dV/|dV| and U = |dV|.  The three averages are the 2-D integrals of MK_source.cpp
+ * (R0 in [0, 5c], alpha in [0, pi], midpoint rule), with the Lindsay-Stebbings sigma.
+ * out = {R, S_mx, S_my, S_mz, S_E}. */
+void or_cx_source(double n_p, double cp2, const double *vp, double n_h, double ch2, const double *vh,
+                  int N1, int N2, double *out)
+{
+    double dv[3] = {vp[0] - vh[0], vp[1] - vh[1], vp[2] - vh[2]};
+    double U = sqrt((dv[0] * dv[0] + dv[1] * dv[1]) + dv[2] * dv[2]);
+    double c2 = cp2 + ch2;
+    double c = sqrt(c2);
+    double e[3] = {1.0, 0.0, 0.0};
+    if (U > 0.0) { e[0] = dv[0] / U; e[1] = dv[1] / U; e[2] = dv[2] / U; }
+    double J0 = 0.0, J1 = 0.0, J2 = 0.0;
+    double dal = OR_PI / N2;
+    double dR0 = 5.0 * c / N1;
+    for (int ii = 0; ii < N1; ++ii) {
+        double R0 = (ii + 0.5) * dR0;
+        double wr = R0 * R0 * exp(-(R0 * R0) / c2);
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int jj = 0; jj < N2; ++jj) {
+            double al = (jj + 0.5) * dal;
+            double sa = sin(al), ca = cos(al);
+            double gpar = U - R0 * ca;
+            double gper = R0 * sa;
+            double g2 = gpar * gpar + gper * gper;
+            double g = sqrt(g2);
+            double gs = g * or_sigma_ls(g) * sa;
+            s0 += gs;
+            s1 += gs * gpar;
+            s2 += gs * g2;
+        }
+        J0 += wr * s0;
+        J1 += wr * s1;
+        J2 += wr * s2;
+    }
+    double norm = 2.0 * OR_PI * dal * dR0 / (OR_PI * sqrt(OR_PI) * c2 * c);
+    J0 *= norm; J1 *= norm; J2 *= norm;
+    double nn = n_p * n_h;
+    double kappa = (cp2 - ch2) / c2;
+    double sv = (vp[0] + vh[0]) * e[0] + (vp[1] + vh[1]) * e[1] + (vp[2] + vh[2]) * e[2];
+    out[0] = nn * J0;
+    out[1] = -nn * J1 * e[0];
+    out[2] = -nn * J1 * e[1];
+    out[3] = -nn * J1 * e[2];
+    out[4] = -0.5 * nn * (sv * J1 + kappa * (J2 - U * J1));
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* FP64 MLP 8-64-64-64-5 with SiLU (the surrogate's CPU reference for gradient checks).        */
+
+#define MLP_IN 8
+#define MLP_H 64
+#define MLP_OUT 5
+
+int64_t or_mlp_n_params(void)
+{
+    return (int64_t)MLP_H * MLP_IN + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H * MLP_H + MLP_H + MLP_OUT * MLP_H + MLP_OUT;
+}
+
+static double silu(double x) { return x / (1.0 + exp(-x)); }
+static double dsilu(double x)
+{
+    double s = 1.0 / (1.0 + exp(-x));
+    return s * (1.0 + x * (1.0 - s));
+}
+
+typedef struct { const double *W[4], *b[4]; int nin[4], nout[4]; } mlp_view;
+
+static mlp_view mlp_layout(const double *P)
+{
+    mlp_view v;
+    int dims[5] = {MLP_IN, MLP_H, MLP_H, MLP_H, MLP_OUT};
+    const double *p = P;
+    for (int l = 0; l < 4; l++) {
+        v.nin[l] = dims[l];
+        v.nout[l] = dims[l + 1];
+        v.W[l] = p; p += dims[l] * dims[l + 1];
+        v.b[l] = p; p += dims[l + 1];
+    }
+    return v;
+}
+
+void or_mlp_forward(const double *params, int64_t n, const double *x, double *y)
+{
+    mlp_view v = mlp_layout(params);
+    for (int64_t s = 0; s < n; s++) {
+        double a[MLP_H], z[MLP_H];
+        for (int i = 0; i < MLP_IN; i++) a[i] = x[MLP_IN * s + i];
+        for (int l = 0; l < 4; l++) {
+            for (int o = 0; o < v.nout[l]; o++) {
+                double acc = v.b[l][o];
+                for (int i = 0; i < v.nin[l]; i++) acc += v.W[l][o * v.nin[l] + i] * a[i];
+                z[o] = acc;
+            }
+            for (int o = 0; o < v.nout[l]; o++) a[o] = (l < 3) ? silu(z[o]) : z[o];
+        }
+        for (int o = 0; o < MLP_OUT; o++) y[MLP_OUT * s + o] = a[o];
+    }
+}
+
+double or_mlp_loss_grad(const double *params, int64_t n, const double *x, const double *yt, double *grad)
+{
+    mlp_view v = mlp_layout(params);
+    int64_t np = or_mlp_n_params();
+    for (int64_t i = 0; i < np; i++) grad[i] = 0.0;
+    mlp_view gv = mlp_layout(grad);
+    double loss = 0.0;
+    for (int64_t s = 0; s < n; s++) {
+        double A[5][MLP_H], Z[4][MLP_H];
+        for (int i = 0; i < MLP_IN; i++) A[0][i] = x[MLP_IN * s + i];
+        for (int l = 0; l < 4; l++) {
+            for (int o = 0; o < v.nout[l]; o++) {
+                double acc = v.b[l][o];
+                for (int i = 0; i < v.nin[l]; i++) acc += v.W[l][o * v.nin[l] + i] * A[l][i];
+                Z[l][o] = acc;
+                A[l + 1][o] = (l < 3) ? silu(acc) : acc;
+            }
+        }
+        double d[MLP_H];
+        for (int o = 0; o < MLP_OUT; o++) {
+            double e = A[4][o] - yt[MLP_OUT * s + o];
+            loss += e * e;
+            d[o] = 2.0 * e / (double)(n * MLP_OUT);
+        }
+        for (int l = 3; l >= 0; l--) {
+            double *gW = (double *)gv.W[l], *gb = (double *)gv.b[l];
+            double dn[MLP_H];
+            for (int i = 0; i < v.nin[l]; i++) dn[i] = 0.0;
+            for (int o = 0; o < v.nout[l]; o++) {
+                gb[o] += d[o];
+                for (int i = 0; i < v.nin[l]; i++) {
+                    gW[o * v.nin[l] + i] += d[o] * A[l][i];
+                    dn[i] += v.W[l][o * v.nin[l] + i] * d[o];
+                }
+            }
+            if (l > 0)
+                for (int i = 0; i < v.nin[l]; i++) d[i] = dn[i] * dsilu(Z[l - 1][i]);
+        }
+    }
+    return loss / (double)(n * MLP_OUT);
+}

@@ -1,0 +1,142 @@
+/*
+ * titan_oracle.h -- CPU restatement of the TITAN ideal-MHD step (TEST INFRASTRUCTURE ONLY).
+ *
+ * This library is the parity checker for the HIP implementation.  Only the test suite,
+ * the smoke entry and the CPU-baseline leg of the benchmark may load it.  The product
+ * (libtitan_gpu.so) never links or calls it.
+ *
+ * Every function restates the arithmetic of the reference (SergeyMSU/TITAN_C-) in the
+ * same operation order, so that FP64 results can be compared bit-for-bit:
+ *   or_chlld            Phys_param::chlld            TITAN/source/Phys_param.cpp:882-1878
+ *   or_get_bazis        get_bazis                    TITAN/source/Help_funk.cpp:219-232
+ *   or_measure          Setka::Calculating_measure   TITAN/source/Setka.cpp:2465-2502
+ *                         Cell::Culc_center          TITAN/source/Cell.cpp:909-930
+ *                         Gran::Culc_measure         TITAN/source/Gran.cpp:112-218
+ *                         Cell::Culc_volume (m=1)    TITAN/source/Cell.cpp:958-1000
+ *   or_face_velocity    Gran::culc_velosity          TITAN/source/Gran.cpp:571-589
+ *   or_face_states      Setka::Snos_on_Gran (first order + boundary)
+ *                                                    TITAN/source/Setka_gran_snos.cpp:3-180,737-1071
+ *   or_face_flux        Setka::Culc_Gran_Potok       TITAN/source/Setka_physics.cpp:2280-2571
+ *   or_update           cell loop of Setka::Go       TITAN/source/Setka_physics.cpp:1671-2027
+ *   or_go               Setka::Go                    TITAN/source/Setka_physics.cpp:1484-2278
+ *   or_new_connect      Setka::New_connect           TITAN/source/Setka.cpp:2066-2249
+ *   or_cx_rate_table    Culc_nu_maxwel               MK_source/MK_source/MK_source/MK_source.cpp:38-103
+ */
+#ifndef TITAN_ORACLE_H
+#define TITAN_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* boundary codes of a face (reference Type_Gran, Setka_physics.cpp:212-236) */
+enum { OR_BC_INTERIOR = 0, OR_BC_INNER_HARD = 1, OR_BC_OUTER_HARD = 2, OR_BC_OUTER_SOFT = 3,
+       OR_BC_ZERO_GRADIENT = 4 };
+
+/* Mesh in reference numbering (faces in New_connect creation order). */
+typedef struct {
+    int64_t n_nodes, n_cells, n_faces;
+    const int32_t *cell_nodes;   /* [n_cells*8] hexahedron nodes                      */
+    const int32_t *cell_faces;   /* [n_cells*6] faces of a cell, ascending face id    */
+    const int32_t *face_nodes;   /* [n_faces*4]                                       */
+    const int32_t *face_cells;   /* [n_faces*2] cells[0], cells[1] (-1 on boundary)   */
+    const int8_t  *face_bc;      /* [n_faces] OR_BC_*                                 */
+    const int8_t  *face_method;  /* [n_faces] 0 Lax, 1 HLL, 2 HLLC, 3 HLLD            */
+    const uint8_t *face_flags;   /* [n_faces] bit0 left_ydar (TS_hard), bit1 contact  */
+    const double  *face_bstate;  /* [n_faces*8] fixed boundary state (rho,V,p,B)      */
+    const int8_t  *cell_zone;    /* [n_cells] 1..4 (Type_cell::Zone_1..Zone_4)        */
+    const uint8_t *cell_is_tvd;  /* [n_cells] 0 => B frozen                            */
+} or_mesh;
+
+typedef struct {
+    double gamma;        /* 5/3                                   */
+    double kfl;          /* CFL number KFL                        */
+    double v_inf;        /* Velosity_inf (outer-soft suction fix) */
+    int32_t move_mesh;   /* move_setka                            */
+    int32_t n_disc;      /* wave-speed estimate passed to chlld   */
+    /* prescribed node motion: x += dt * (omega*cos(omega*t+phase)) * D_i */
+    double motion_omega, motion_phase;
+} or_params;
+
+typedef struct {
+    double *cell_center;  /* [n_cells*3] */
+    double *cell_volume;  /* [n_cells]   */
+    double *face_center;  /* [n_faces*3] */
+    double *face_normal;  /* [n_faces*3] */
+    double *face_area;    /* [n_faces]   */
+} or_geom;
+
+/* ---- Riemann solver ---- */
+/* Returns 0, or -1 on the reference's fatal "wrong choice" branches. branch = n_state*16 + region
+ * (+8 when HLLD's |Bn|>epsb, i.e. ibn=1); dsl/dsc/dsr as in the reference. */
+int or_chlld(int n_state, int n_disc, int flags, double al, double be, double ge, double w,
+             const double *qL, const double *qR, int K, const double *kL, const double *kR,
+             double *q, double *k, double *dsl, double *dsc, double *dsr, int *branch);
+int or_chlld_batch(int64_t n, const int8_t *n_state, const int8_t *n_disc, const uint8_t *flags,
+                   const double *normal, const double *w, const double *qL, const double *qR,
+                   int K, const double *kL, const double *kR, double *q, double *k,
+                   double *ds, uint8_t *branch);
+void or_get_bazis(const double *n, double *t, double *m);
+/* physical flux F(U).n in the lab frame (Phys_param::Get_Potok, Phys_param.cpp:826-874);
+ * output order of the reference: [mass, mom x3, B x3, energy] */
+void or_get_potok(double gamma, double rho, double p, double u, double v, double w, double bx,
+                  double by, double bz, double n1, double n2, double n3, double *potok);
+
+/* ---- geometry ---- */
+void or_measure(const or_mesh *m, const double *coords, or_geom *g);
+void or_measure_gcl(const or_mesh *m, const double *coords, or_geom *g);
+double or_face_velocity(const or_mesh *m, int64_t f, const double *x_now1, const double *x_now2,
+                        const double *normal_now1, double dt);
+double or_det_cos(double x);
+void or_motion(int64_t n_nodes, const double *x_old, const double *disp, double dt, double t,
+               double omega, double phase, double *x_new);
+
+/* ---- face / cell passes ---- */
+int or_face_states(const or_mesh *m, const or_params *p, int64_t f, const double *prim,
+                   const or_geom *g, double *qL, double *qR);
+/* flux[f*9 + 0..7] = chlld flux * area, flux[f*9+8] = PdivB; dt_face[f] (1e7 on non-interior) */
+int or_face_flux(const or_mesh *m, const or_params *p, const double *prim, const or_geom *g,
+                 const double *x_now1, const double *x_now2, double dt, double *flux,
+                 double *dt_face, uint8_t *branch);
+/* src: optional [n_cells*5] sources (mass-rate, m_x, m_y, m_z, E) per unit volume and time */
+int or_update(const or_mesh *m, const or_params *p, const double *prim_old, const double *vol_old,
+              const double *vol_new, const double *flux, const double *src, double dt,
+              double *prim_new, int64_t *n_rho_floor, int64_t *n_p_floor);
+
+/* Full explicit loop of Setka::Go.  coords[2][n_nodes*3], prim[2][n_cells*8] hold both time
+ * layers and are updated in place; disp [n_nodes*3] is the motion field (may be NULL when not
+ * moving).  gcl selects the exactly-conservative geometry.  On return *final_layer says which
+ * layer holds the newest state; dt_hist (optional, [steps]) receives the dt used at each step. */
+int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, const double *disp,
+          int64_t steps, int gcl, int nthreads, int *final_layer, double *dt_hist,
+          double *t_sim);
+
+/* ---- mesh indexing (New_connect) ---- */
+/* Builds faces of n_cells hexahedra; arrays sized 6*n_cells by the caller; returns n_faces. */
+int64_t or_new_connect(int64_t n_cells, const int32_t *cell_nodes, int64_t n_nodes,
+                       int32_t *face_nodes, int32_t *face_cells, int32_t *cell_faces);
+
+/* ---- charge exchange ---- */
+/* I(U,c) of MK_source.cpp with sigma = (1 - a2 ln g)^2, N1 x N2 midpoint rule. */
+double or_cx_rate_ref(double U, double c, double a2, int N1, int N2);
+/* Lindsay-Stebbings (2005) cross-section in code units (g in units of 10.3804 km/s). */
+double or_sigma_ls(double g);
+/* Exact FP64 Maxwellian charge-exchange source for one cell (per unit volume and time).
+ * in: n_p, c_p^2 (=p/rho), V_p[3], n_H, c_H^2, V_H[3].  out[5]: rate, S_m (3), S_E (lab frame) */
+void or_cx_source(double n_p, double cp2, const double *vp, double n_h, double ch2,
+                  const double *vh, int N1, int N2, double *out);
+
+/* ---- MLP (8 -> 64 -> 64 -> 64 -> 5, SiLU) in FP64 ---- */
+/* params layout: W1[64*8] b1[64] W2[64*64] b2[64] W3[64*64] b3[64] W4[5*64] b4[5] (row = out) */
+int64_t or_mlp_n_params(void);
+void or_mlp_forward(const double *params, int64_t n, const double *x, double *y);
+/* mean-squared-error loss over n samples and its gradient (same layout as params) */
+double or_mlp_loss_grad(const double *params, int64_t n, const double *x, const double *y_target,
+                        double *grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

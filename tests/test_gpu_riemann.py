@@ -1,0 +1,74 @@
+"""Parity of the device Riemann solvers (titan_chlld_batch) with the oracle's Phys_param::chlld."""
+import numpy as np
+import pytest
+
+from helpers import random_normals, random_states
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(rng, n):
+    qL = random_states(rng, n)
+    qR = random_states(rng, n)
+    # a third of the faces get near-equal states (smooth flow), some get |Bn| below epsb
+    k = n // 3
+    qR[:k] = qL[:k] * (1.0 + 1e-3 * rng.normal(size=(k, 8)))
+    nrm = random_normals(rng, n)
+    w = rng.normal(0.0, 2.0, n)
+    w[: n // 4] = 0.0
+    return qL, qR, nrm, w
+
+
+@pytest.mark.parametrize("method", [0, 1, 2, 3])
+def test_chlld_fp64_bit_exact(T, O, method):
+    rng = np.random.default_rng(100 + method)
+    n = 250_000
+    qL, qR, nrm, w = _case(rng, n)
+    n_disc = rng.integers(0, 4, n).astype(np.int8)
+    flags = rng.choice(np.array([0, 0, 0, 1, 2], np.uint8), n)
+    ref = O.chlld_batch(method, nrm, qL, qR, w=w, n_disc=n_disc, flags=flags)
+    got = T.chlld(method, nrm, qL, qR, w=w, n_disc=n_disc, flags=flags)
+    ok = (ref[3] & 7) != 0  # region 0 = fatal branch, excluded from the flux comparison
+    np.testing.assert_array_equal(got[3], ref[3])          # branch codes: n_state, ibn, region
+    np.testing.assert_array_equal(got[2], ref[2])          # dsl, dsc, dsr
+    np.testing.assert_array_equal(got[0][ok], ref[0][ok])  # fluxes, bit for bit
+
+
+def test_chlld_passive_scalars_bit_exact(T, O):
+    rng = np.random.default_rng(7)
+    n = 60_000
+    qL, qR, nrm, w = _case(rng, n)
+    kL = np.abs(rng.normal(1.0, 0.5, (n, 2)))
+    kR = np.abs(rng.normal(1.0, 0.5, (n, 2)))
+    for method in range(4):
+        ref = O.chlld_batch(method, nrm, qL, qR, w=w, kL=kL, kR=kR)
+        got = T.chlld(method, nrm, qL, qR, w=w, kL=kL, kR=kR)
+        ok = (ref[3] & 7) != 0
+        np.testing.assert_array_equal(got[1][ok], ref[1][ok])
+        np.testing.assert_array_equal(got[0][ok], ref[0][ok])
+
+
+def test_chlld_brio_wu_interface(T, O):
+    s4 = np.sqrt(4 * np.pi)
+    qL = np.array([[1.0, 0, 0, 0, 1.0, 0.75 * s4, 1.0 * s4, 0.0]])
+    qR = np.array([[0.125, 0, 0, 0, 0.1, 0.75 * s4, -1.0 * s4, 0.0]])
+    for method in range(4):
+        for w in (0.0, -0.3, 0.7):
+            ref = O.chlld_batch(method, np.array([[1.0, 0, 0]]), qL, qR, w=np.array([w]))
+            got = T.chlld(method, np.array([[1.0, 0, 0]]), qL, qR, w=np.array([w]))
+            np.testing.assert_array_equal(got[0], ref[0])
+            np.testing.assert_array_equal(got[3], ref[3])
+
+
+def test_chlld_fp32_close(T, O):
+    rng = np.random.default_rng(11)
+    n = 100_000
+    qL, qR, nrm, w = _case(rng, n)
+    for method in (2, 3):
+        ref = O.chlld_batch(method, nrm, qL, qR, w=w)
+        got = T.chlld(method, nrm, qL, qR, w=w, precision=T.PREC_MIXED)
+        scale = np.abs(ref[0]).max(axis=1, keepdims=True) + 1e-30
+        err = np.abs(got[0] - ref[0]) / scale
+        # FP32 Riemann arithmetic: errors relative to the face's largest flux component
+        assert np.quantile(err, 0.999) < 1e-4, np.quantile(err, 0.999)
+        assert (got[3] == ref[3]).mean() > 0.999

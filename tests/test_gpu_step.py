@@ -1,0 +1,137 @@
+"""Parity of the device geometry, face loop and full step with the oracle (Setka::Go)."""
+import hashlib
+
+import numpy as np
+import pytest
+
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def small_shell(C, moving=False, n_edge=6, n_layers=10, **kw):
+    return C.shell(n_edge=n_edge, n_layers=n_layers, moving=moving, r_zone1_au=80.0, **kw)
+
+
+@pytest.mark.parametrize("gcl", [False, True])
+def test_geometry_bit_exact(T, C, O, gcl):
+    for mesh, prim, params in (C.cube_brio_wu(n=12), small_shell(C)):
+        params = dict(params, geom_mode=T.GEOM_GCL if gcl else T.GEOM_REF)
+        with T.Setka(mesh, params) as s:
+            g = s.geometry(0)
+        ref = O.measure(O.OracleMesh(mesh), mesh.node_xyz, gcl=gcl)
+        for k in ref:
+            np.testing.assert_array_equal(g[k], ref[k], err_msg=k)
+
+
+def test_face_loop_cube_bit_exact(T, C, O):
+    mesh, prim, params = C.cube_brio_wu()
+    with T.Setka(mesh, params) as s:
+        s.set_state(prim, 0)
+        flux, dtmin, br = s.Culc_Gran_Potok(0, 1e-5)
+        g = s.geometry(0)
+    rc, rflux, rdt, rbr = O.face_flux(O.OracleMesh(mesh), params, prim, g)
+    assert rc == 0
+    np.testing.assert_array_equal(br, rbr)
+    np.testing.assert_array_equal(flux, rflux)
+    assert dtmin == min(1e8, rdt.min())
+
+
+def test_face_loop_shell_zone1(T, C, O):
+    """Faces touching Zone_1 cells use acos/atan/sin/cos/pow: GPU libm vs glibc differ in the
+    last ulp there, so those faces are compared to 1e-12 relative; all others bit for bit."""
+    mesh, prim, params = small_shell(C)
+    with T.Setka(mesh, params) as s:
+        s.set_state(prim, 0)
+        flux, dtmin, br = s.Culc_Gran_Potok(0, 1e-5)
+        g = s.geometry(0)
+    rc, rflux, rdt, rbr = O.face_flux(O.OracleMesh(mesh), params, prim, g)
+    assert rc == 0
+    np.testing.assert_array_equal(br, rbr)
+    z1 = mesh.cell_zone[mesh.face_cells[:, 0]] == 1
+    fb = mesh.face_cells[:, 1]
+    z1 |= (fb >= 0) & (mesh.cell_zone[np.maximum(fb, 0)] == 1)
+    assert z1.any() and (~z1).any()
+    np.testing.assert_array_equal(flux[~z1], rflux[~z1])
+    scale = np.abs(rflux[z1]).max(axis=1, keepdims=True)
+    assert (np.abs(flux[z1] - rflux[z1]) / scale).max() < 1e-12
+    assert abs(dtmin - min(1e8, rdt.min())) <= 1e-12 * dtmin
+
+
+def test_go_cube_brio_wu_100_steps_bit_exact(T, C, O):
+    mesh, prim, params = C.cube_brio_wu()
+    with T.Setka(mesh, params) as s:
+        s.set_state(prim, 0)
+        st = s.Go(100)
+        got = s.get_state()
+    rc, ref, _, dth, _ = O.go(O.OracleMesh(mesh), params, mesh.node_xyz, prim, 100, nthreads=8)
+    assert rc == 0
+    assert st["dt_last"] == dth[-1]
+    np.testing.assert_array_equal(got, ref)
+
+
+def _run_gpu(T, mesh, prim, params, steps):
+    with T.Setka(mesh, params) as s:
+        s.set_state(prim, 0)
+        s.Go(steps)
+        return s.get_state(), s.get_nodes(s.current_layer())
+
+
+def test_go_shell_fixed(T, C, O):
+    mesh, prim, params = small_shell(C)
+    got, _ = _run_gpu(T, mesh, prim, params, 30)
+    rc, ref, _, _, _ = O.go(O.OracleMesh(mesh), params, mesh.node_xyz, prim, 30, nthreads=8)
+    assert rc == 0
+    assert rel_err(got, ref, 1e-30).max() < 1e-11
+
+
+def test_go_shell_moving(T, C, O):
+    mesh, prim, params = small_shell(C, moving=True)
+    got, xyz = _run_gpu(T, mesh, prim, params, 30)
+    rc, ref, rxyz, _, _ = O.go(O.OracleMesh(mesh), params, mesh.node_xyz, prim, 30, nthreads=8,
+                               disp=mesh.node_disp)
+    assert rc == 0
+    assert np.abs(xyz - mesh.node_xyz).max() > 0.0
+    assert rel_err(xyz, rxyz, 1e-30).max() < 1e-12
+    assert rel_err(got, ref, 1e-30).max() < 1e-11
+
+
+def test_go_cube_moving_bit_exact(T, C, O):
+    """No transcendental functions on this path: the moving-mesh step (motion law, REF geometry,
+    face velocities) must agree bit for bit."""
+    mesh, prim, params = C.cube_brio_wu(n=16)
+    rng = np.random.default_rng(3)
+    disp = np.zeros_like(mesh.node_xyz)
+    xyz = mesh.node_xyz
+    inner = np.all((xyz > 1e-9) & (xyz < 1 - 1e-9), axis=1)
+    disp[inner] = 0.01 * rng.normal(size=(inner.sum(), 3))
+    mesh.node_disp = disp
+    params = dict(params, move_mesh=1, motion_omega=3.0, motion_phase=0.4)
+    got, gx = _run_gpu(T, mesh, prim, params, 40)
+    rc, ref, rx, _, _ = O.go(O.OracleMesh(mesh), params, mesh.node_xyz, prim, 40, nthreads=8, disp=disp)
+    assert rc == 0
+    np.testing.assert_array_equal(gx, rx)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_repeatable_bit_for_bit(T, C):
+    mesh, prim, params = small_shell(C, moving=True)
+    h = []
+    for _ in range(2):
+        got, _ = _run_gpu(T, mesh, prim, params, 20)
+        h.append(hashlib.sha256(got.tobytes()).hexdigest())
+    assert h[0] == h[1]
+
+
+def test_gcl_free_stream(T, C):
+    """Exactly conservative geometry: a uniform state on the moving shell stays uniform."""
+    mesh, prim, params = small_shell(C, moving=True)
+    u = np.array([1.0, 0.3, -0.2, 0.1, 2.0, 0.5, 0.4, -0.3])
+    prim = np.tile(u, (mesh.n_cells, 1))
+    mesh.face_bc[mesh.face_bc != T.BC_INTERIOR] = T.BC_ZERO_GRADIENT
+    mesh.face_method[:] = 3
+    mesh.cell_zone[:] = 3
+    params = dict(params, geom_mode=T.GEOM_GCL)
+    got, xyz = _run_gpu(T, mesh, prim, params, 20)
+    assert np.abs(xyz - mesh.node_xyz).max() > 1e-4
+    assert rel_err(got, np.broadcast_to(u, got.shape), 1e-30).max() < 1e-11

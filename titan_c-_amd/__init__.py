@@ -1,0 +1,350 @@
+"""titan_c-_amd: MI355X (gfx950) implementation of TITAN's explicit ideal-MHD step.
+
+Python view of the C ABI in include/titan_gpu.h (libtitan_gpu.so, built by build.py).  The class
+`Setka` mirrors the reference's step interface (TITAN/source/Setka.h:121,205,221,223):
+``Setka.Go``, ``Setka.Culc_Gran_Potok``, ``Setka.Calculating_measure``; ``chlld`` mirrors
+``Phys_param::chlld`` (Phys_param.h:240-251) for a batch of faces.  All compute runs in the HIP
+library; this module only marshals arrays.  There is no CPU fallback: loading fails loudly when
+the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libtitan_gpu.so")
+
+TITAN_OK = 0
+TITAN_E_ARG = -1
+TITAN_E_HIP = -2
+TITAN_E_RIEMANN = -3
+TITAN_E_NONFINITE_STATE = -4
+TITAN_E_TOPOLOGY = -5
+TITAN_E_NOT_TRAINED = -6
+BC_INTERIOR, BC_INNER_HARD, BC_OUTER_HARD, BC_OUTER_SOFT, BC_ZERO_GRADIENT = 0, 1, 2, 3, 4
+GEOM_REF, GEOM_GCL = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+SRC_NONE, SRC_EXACT, SRC_MLP = 0, 1, 2
+
+_ERRNAMES = {
+    TITAN_E_ARG: "TITAN_E_ARG", TITAN_E_HIP: "TITAN_E_HIP", TITAN_E_RIEMANN: "TITAN_E_RIEMANN",
+    TITAN_E_NONFINITE_STATE: "TITAN_E_NONFINITE_STATE", TITAN_E_TOPOLOGY: "TITAN_E_TOPOLOGY",
+    TITAN_E_NOT_TRAINED: "TITAN_E_NOT_TRAINED",
+}
+
+
+class TitanError(RuntimeError):
+    def __init__(self, code, what, index=-1):
+        self.code = code
+        self.index = index
+        super().__init__("%s returned %s (%d), index %d" % (what, _ERRNAMES.get(code, "?"), code, index))
+
+
+class MeshC(ctypes.Structure):
+    _fields_ = [("n_nodes", ctypes.c_int64), ("n_cells", ctypes.c_int64), ("n_faces", ctypes.c_int64),
+                ("cell_nodes", ctypes.c_void_p), ("face_nodes", ctypes.c_void_p), ("face_cells", ctypes.c_void_p),
+                ("cell_faces", ctypes.c_void_p), ("face_bc", ctypes.c_void_p), ("face_method", ctypes.c_void_p),
+                ("face_flags", ctypes.c_void_p), ("face_bstate", ctypes.c_void_p), ("cell_zone", ctypes.c_void_p),
+                ("cell_is_tvd", ctypes.c_void_p), ("node_xyz", ctypes.c_void_p), ("node_disp", ctypes.c_void_p)]
+
+
+class ParamsC(ctypes.Structure):
+    _fields_ = [("gamma", ctypes.c_double), ("kfl", ctypes.c_double), ("v_inf", ctypes.c_double),
+                ("move_mesh", ctypes.c_int32), ("n_disc", ctypes.c_int32), ("geom_mode", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("source_mode", ctypes.c_int32), ("cx_nodes", ctypes.c_int32),
+                ("motion_omega", ctypes.c_double), ("motion_phase", ctypes.c_double)]
+
+
+class StatsC(ctypes.Structure):
+    _fields_ = [("steps_done", ctypes.c_int64), ("dt_last", ctypes.c_double), ("dt_next", ctypes.c_double),
+                ("t_sim", ctypes.c_double), ("n_rho_floor", ctypes.c_int64), ("n_p_floor", ctypes.c_int64),
+                ("n_fallback", ctypes.c_int64), ("ms_total", ctypes.c_double)]
+
+
+_lib = None
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+
+_SIGS = {
+    "titan_create": (_I, [ctypes.POINTER(_P), ctypes.POINTER(MeshC), ctypes.POINTER(ParamsC)]),
+    "titan_destroy": (_I, [_P]),
+    "titan_last_error": (_I, [_P, ctypes.POINTER(_I64)]),
+    "titan_set_state": (_I, [_P, _P, _I]),
+    "titan_get_state": (_I, [_P, _P, _I]),
+    "titan_set_nodes": (_I, [_P, _P, _I]),
+    "titan_get_nodes": (_I, [_P, _P, _I]),
+    "titan_set_neutrals": (_I, [_P, _P]),
+    "titan_calculating_measure": (_I, [_P, _I]),
+    "titan_get_geometry": (_I, [_P, _I, _P, _P, _P, _P, _P]),
+    "titan_gran_potok": (_I, [_P, _I, _D, _P, _P, _P]),
+    "titan_go": (_I, [_P, _I64, ctypes.POINTER(StatsC)]),
+    "titan_current_layer": (_I, [_P]),
+    "titan_chlld_batch": (_I, [_I64, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _I]),
+    "titan_sources": (_I, [_P, _I, _I, _P]),
+    "titan_cx_exact_batch": (_I, [_I64, _P, _I, _P]),
+    "titan_mlp_n_params": (_I64, []),
+    "titan_mlp_set": (_I, [_P, _P, _P, _P, _P]),
+    "titan_mlp_get": (_I, [_P, _P]),
+    "titan_mlp_forward": (_I, [_P, _I64, _P, _P]),
+    "titan_mlp_loss_grad": (_I, [_P, _I64, _P, _P, _P, _P]),
+    "titan_mlp_train": (_I, [_P, _I64, _P, _P, _I, _I, ctypes.c_float, ctypes.c_uint64, _P]),
+    "titan_mesh_connect": (_I64, [_I64, _P, _I64, _P, _P, _P]),
+    "titan_mesh_cube": (_I, [ctypes.c_int32, _D, ctypes.c_uint64, _P, _P]),
+    "titan_mesh_shell": (_I, [ctypes.c_int32, ctypes.c_int32, _P, _P, _P]),
+    "titan_kernel_ms": (_D, [_P, _I]),
+    "titan_set_timing": (_I, [_P, _I]),
+}
+EXPORTS = tuple(_SIGS)
+
+
+def lib():
+    """Loads libtitan_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libtitan_gpu.so is not built (%s); run titan_c-_amd/build.py" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _c(a, dtype):
+    return None if a is None else np.ascontiguousarray(a, dtype=dtype)
+
+
+def _check(rc, what, ctx=None):
+    if rc != TITAN_OK:
+        idx = -1
+        if ctx is not None:
+            i = ctypes.c_int64(-1)
+            lib().titan_last_error(ctx, ctypes.byref(i))
+            idx = i.value
+        raise TitanError(rc, what, idx)
+
+
+# ---- mesh helpers ---------------------------------------------------------------------------
+def mesh_connect(cell_nodes, n_nodes):
+    """Setka::New_connect (Setka.cpp:2066-2249): returns face_nodes, face_cells, cell_faces."""
+    cn = _c(cell_nodes, np.int32).reshape(-1)
+    nc = cn.size // 8
+    fnodes = np.empty(6 * nc * 4, np.int32)
+    fcells = np.empty(6 * nc * 2, np.int32)
+    cfaces = np.empty(nc * 6, np.int32)
+    nf = lib().titan_mesh_connect(nc, _ptr(cn), int(n_nodes), _ptr(fnodes), _ptr(fcells), _ptr(cfaces))
+    if nf < 0:
+        raise TitanError(int(nf), "titan_mesh_connect")
+    return fnodes[:4 * nf].reshape(nf, 4).copy(), fcells[:2 * nf].reshape(nf, 2).copy(), cfaces.reshape(nc, 6)
+
+
+def mesh_cube(n, jitter, seed):
+    xyz = np.empty(((n + 1) ** 3, 3))
+    cn = np.empty((n ** 3, 8), np.int32)
+    _check(lib().titan_mesh_cube(n, jitter, seed, _ptr(xyz), _ptr(cn)), "titan_mesh_cube")
+    return xyz, cn
+
+
+def mesh_shell(n_edge, radii):
+    radii = _c(radii, np.float64)
+    nl = radii.size - 1
+    ns = 6 * n_edge * n_edge + 2
+    xyz = np.empty(((nl + 1) * ns, 3))
+    cn = np.empty((nl * 6 * n_edge * n_edge, 8), np.int32)
+    _check(lib().titan_mesh_shell(n_edge, nl, _ptr(radii), _ptr(xyz), _ptr(cn)), "titan_mesh_shell")
+    return xyz, cn
+
+
+# ---- Riemann solver -----------------------------------------------------------------------------
+def chlld(n_state, normal, qL, qR, w=None, n_disc=None, flags=None, kL=None, kR=None, precision=PREC_FP64):
+    """Phys_param::chlld over a batch.  Returns (q [n,8], k [n,K], ds [n,3], branch [n], rc)."""
+    qL = _c(qL, np.float64).reshape(-1, 8)
+    qR = _c(qR, np.float64).reshape(-1, 8)
+    n = qL.shape[0]
+    ns = _c(np.broadcast_to(n_state, (n,)), np.int8)
+    nd = None if n_disc is None else _c(np.broadcast_to(n_disc, (n,)), np.int8)
+    fl = None if flags is None else _c(np.broadcast_to(flags, (n,)), np.uint8)
+    nrm = _c(normal, np.float64).reshape(n, 3)
+    ww = None if w is None else _c(np.broadcast_to(w, (n,)), np.float64)
+    K = 0 if kL is None else int(np.asarray(kL).reshape(n, -1).shape[1])
+    kLa = None if K == 0 else _c(kL, np.float64).reshape(n, K)
+    kRa = None if K == 0 else _c(kR, np.float64).reshape(n, K)
+    q = np.zeros((n, 8))
+    k = np.zeros((n, max(K, 1)))
+    ds = np.zeros((n, 3))
+    br = np.zeros(n, np.uint8)
+    rc = lib().titan_chlld_batch(n, _ptr(ns), _ptr(nd), _ptr(fl), _ptr(nrm), _ptr(ww), _ptr(qL), _ptr(qR), K,
+                                 _ptr(kLa), _ptr(kRa), _ptr(q), _ptr(k) if K else None, _ptr(ds), _ptr(br),
+                                 int(precision))
+    if rc not in (TITAN_OK, TITAN_E_RIEMANN):
+        _check(rc, "titan_chlld_batch")
+    return q, k[:, :K], ds, br, rc
+
+
+def cx_exact_batch(in10, nodes=100):
+    a = _c(in10, np.float64).reshape(-1, 10)
+    out = np.zeros((a.shape[0], 5))
+    _check(lib().titan_cx_exact_batch(a.shape[0], _ptr(a), int(nodes), _ptr(out)), "titan_cx_exact_batch")
+    return out
+
+
+# ---- context ---------------------------------------------------------------------------------
+class Setka:
+    """One device context: mesh, both time layers of state and geometry, step scalars."""
+
+    def __init__(self, mesh, params):
+        self._keep = []
+        m = MeshC()
+        m.n_nodes, m.n_cells, m.n_faces = mesh.n_nodes, mesh.n_cells, mesh.n_faces
+
+        def keep(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt).reshape(-1)
+            self._keep.append(a)
+            return a.ctypes.data
+
+        m.cell_nodes = keep(mesh.cell_nodes, np.int32)
+        m.face_nodes = keep(mesh.face_nodes, np.int32)
+        m.face_cells = keep(mesh.face_cells, np.int32)
+        m.cell_faces = keep(mesh.cell_faces, np.int32)
+        m.face_bc = keep(mesh.face_bc, np.int8)
+        m.face_method = keep(mesh.face_method, np.int8)
+        m.face_flags = keep(mesh.face_flags, np.uint8)
+        m.face_bstate = keep(mesh.face_bstate, np.float64)
+        m.cell_zone = keep(mesh.cell_zone, np.int8)
+        m.cell_is_tvd = keep(mesh.cell_is_tvd, np.uint8)
+        m.node_xyz = keep(mesh.node_xyz, np.float64)
+        m.node_disp = keep(mesh.node_disp, np.float64)
+        p = ParamsC()
+        for k, _ in ParamsC._fields_:
+            setattr(p, k, params[k])
+        self.n_cells, self.n_faces, self.n_nodes = mesh.n_cells, mesh.n_faces, mesh.n_nodes
+        self.params = dict(params)
+        ctx = ctypes.c_void_p()
+        _check(lib().titan_create(ctypes.byref(ctx), ctypes.byref(m), ctypes.byref(p)), "titan_create")
+        self._keep = []
+        self.ctx = ctx
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().titan_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_state(self, prim, layer=0):
+        a = _c(prim, np.float64).reshape(self.n_cells, 8)
+        _check(lib().titan_set_state(self.ctx, _ptr(a), layer), "titan_set_state", self.ctx)
+
+    def get_state(self, layer=None):
+        if layer is None:
+            layer = self.current_layer()
+        a = np.empty((self.n_cells, 8))
+        _check(lib().titan_get_state(self.ctx, _ptr(a), layer), "titan_get_state", self.ctx)
+        return a
+
+    def set_nodes(self, xyz, layer):
+        a = _c(xyz, np.float64).reshape(self.n_nodes, 3)
+        _check(lib().titan_set_nodes(self.ctx, _ptr(a), layer), "titan_set_nodes", self.ctx)
+
+    def get_nodes(self, layer):
+        a = np.empty((self.n_nodes, 3))
+        _check(lib().titan_get_nodes(self.ctx, _ptr(a), layer), "titan_get_nodes", self.ctx)
+        return a
+
+    def set_neutrals(self, nh5):
+        a = _c(nh5, np.float64).reshape(self.n_cells, 5)
+        _check(lib().titan_set_neutrals(self.ctx, _ptr(a)), "titan_set_neutrals", self.ctx)
+
+    def Calculating_measure(self, layer):
+        _check(lib().titan_calculating_measure(self.ctx, layer), "titan_calculating_measure", self.ctx)
+
+    def geometry(self, layer):
+        nc, nf = self.n_cells, self.n_faces
+        cc, cv = np.empty((nc, 3)), np.empty(nc)
+        fc, fn, fa = np.empty((nf, 3)), np.empty((nf, 3)), np.empty(nf)
+        _check(lib().titan_get_geometry(self.ctx, layer, _ptr(cc), _ptr(cv), _ptr(fc), _ptr(fn), _ptr(fa)),
+               "titan_get_geometry", self.ctx)
+        return dict(cell_center=cc, cell_volume=cv, face_center=fc, face_normal=fn, face_area=fa)
+
+    def Culc_Gran_Potok(self, now, dt, want_branch=True):
+        """The face loop: returns (flux [nf,9], dt_min, branch [nf])."""
+        flux = np.empty((self.n_faces, 9))
+        dtm = ctypes.c_double(0.0)
+        br = np.empty(self.n_faces, np.uint8) if want_branch else None
+        _check(lib().titan_gran_potok(self.ctx, now, float(dt), _ptr(flux), ctypes.byref(dtm), _ptr(br)),
+               "titan_gran_potok", self.ctx)
+        return flux, dtm.value, br
+
+    def Go(self, steps):
+        st = StatsC()
+        _check(lib().titan_go(self.ctx, int(steps), ctypes.byref(st)), "titan_go", self.ctx)
+        return {k: getattr(st, k) for k, _ in StatsC._fields_}
+
+    def current_layer(self):
+        return lib().titan_current_layer(self.ctx)
+
+    def sources(self, layer, mode):
+        out = np.empty((self.n_cells, 5))
+        _check(lib().titan_sources(self.ctx, layer, mode, _ptr(out)), "titan_sources", self.ctx)
+        return out
+
+    def set_timing(self, on=True):
+        _check(lib().titan_set_timing(self.ctx, 1 if on else 0), "titan_set_timing", self.ctx)
+
+    def kernel_ms(self, which):
+        return lib().titan_kernel_ms(self.ctx, which)
+
+    # ---- surrogate ----
+    def mlp_set(self, params, in_norm16, out_scale5, box16):
+        P = _c(params, np.float32)
+        a, b, c = _c(in_norm16, np.float32), _c(out_scale5, np.float32), _c(box16, np.float32)
+        _check(lib().titan_mlp_set(self.ctx, _ptr(P), _ptr(a), _ptr(b), _ptr(c)), "titan_mlp_set", self.ctx)
+
+    def mlp_get(self):
+        P = np.empty(lib().titan_mlp_n_params(), np.float32)
+        _check(lib().titan_mlp_get(self.ctx, _ptr(P)), "titan_mlp_get", self.ctx)
+        return P
+
+    def mlp_forward(self, x):
+        x = _c(x, np.float32).reshape(-1, 8)
+        y = np.empty((x.shape[0], 5), np.float32)
+        _check(lib().titan_mlp_forward(self.ctx, x.shape[0], _ptr(x), _ptr(y)), "titan_mlp_forward", self.ctx)
+        return y
+
+    def mlp_loss_grad(self, x, y):
+        x = _c(x, np.float32).reshape(-1, 8)
+        y = _c(y, np.float32).reshape(-1, 5)
+        loss = ctypes.c_float(0)
+        g = np.empty(lib().titan_mlp_n_params(), np.float32)
+        _check(lib().titan_mlp_loss_grad(self.ctx, x.shape[0], _ptr(x), _ptr(y), ctypes.byref(loss), _ptr(g)),
+               "titan_mlp_loss_grad", self.ctx)
+        return loss.value, g
+
+    def mlp_train(self, x, y, epochs, batch, lr, seed=0):
+        x = _c(x, np.float32).reshape(-1, 8)
+        y = _c(y, np.float32).reshape(-1, 5)
+        hist = np.zeros(max(1, epochs), np.float32)
+        _check(lib().titan_mlp_train(self.ctx, x.shape[0], _ptr(x), _ptr(y), int(epochs), int(batch), float(lr),
+                                     int(seed), _ptr(hist)), "titan_mlp_train", self.ctx)
+        return hist[:epochs]

@@ -1,0 +1,245 @@
+"""Synthetic inputs of the five workloads (meshes, boundary data, initial states, neutrals).
+
+  cube   : 32^3 jittered hexahedra on [0,1]^3, Brio-Wu along x, HLLD everywhere, zero-gradient
+           boundaries, CFL 0.4 (the reference CPU-runnable case, 100 steps).
+  shell  : spherical shell 1..1500 AU, n_edge x n_edge cube-sphere quads per panel x n_layers
+           log-spaced layers; inner solar-wind boundary, interstellar inflow (outer hard on the
+           upwind half, outer soft elsewhere); HLLC on faces touching the r < 80 AU zone (Zone_1),
+           HLLD elsewhere.  1M cells: n_edge 37 x 120 layers; 4M: 53 x 240.
+  moving : the shell with prescribed node motion (radial, smooth in angle, tapered to zero at both
+           boundaries, amplitude up to 5 % of the local radius).
+Nondimensionalisation follows the reference's characteristic scales (Phys_param.h:117-122):
+length 1 AU = 0.237455, density 0.06 cm^-3, velocity 10.3804 km/s, B 0.328842 uG
+(= sqrt(char_p)), pressure 1.08137e-13 dyn/cm^2.
+"""
+import math
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import (BC_INNER_HARD, BC_INTERIOR, BC_OUTER_HARD, BC_OUTER_SOFT, BC_ZERO_GRADIENT, GEOM_REF,
+               PREC_FP64, SRC_NONE, mesh_connect, mesh_cube, mesh_shell)
+
+GAMMA = 5.0 / 3.0
+R0_AU = 0.237455            # 1 AU in code units (Phys_param.h:117)
+CHAR_RHO = 0.06             # cm^-3
+CHAR_V = 10.3804            # km/s
+CHAR_B_UG = 0.328842        # micro-gauss
+CHAR_P = 1.08137e-13        # dyn/cm^2
+K_B = 1.380649e-16
+M_P = 1.67262192369e-24
+V_INF = -2.54327            # Velosity_inf (Phys_param.h:95)
+SPI4 = math.sqrt(4.0 * math.pi)
+
+
+@dataclass
+class Mesh:
+    n_nodes: int
+    n_cells: int
+    n_faces: int
+    cell_nodes: np.ndarray
+    face_nodes: np.ndarray
+    face_cells: np.ndarray
+    cell_faces: np.ndarray
+    face_bc: np.ndarray
+    face_method: np.ndarray
+    face_flags: Optional[np.ndarray]
+    face_bstate: Optional[np.ndarray]
+    cell_zone: np.ndarray
+    cell_is_tvd: np.ndarray
+    node_xyz: np.ndarray
+    node_disp: Optional[np.ndarray] = None
+
+
+def splitmix_uniform(seed, index):
+    """Counter-based U[0,1) (SplitMix64), identical to the C++ mesh generator's."""
+    with np.errstate(over="ignore"):
+        x = (np.uint64(seed) * np.uint64(0x2545F4914F6CDD1D) + np.asarray(index, dtype=np.uint64))
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        x = x ^ (x >> np.uint64(31))
+    return (x >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def default_params(**kw):
+    p = dict(gamma=GAMMA, kfl=0.4, v_inf=V_INF, move_mesh=0, n_disc=0, geom_mode=GEOM_REF, precision=PREC_FP64,
+             source_mode=SRC_NONE, cx_nodes=100, motion_omega=0.0, motion_phase=0.0)
+    p.update(kw)
+    return p
+
+
+def _finish(xyz, cn, bc_fn, method_fn, zone, tvd_fn=None, bstate_fn=None, disp=None):
+    fn, fcells, cf = mesh_connect(cn, xyz.shape[0])
+    nf = fn.shape[0]
+    fcen = xyz[fn].mean(axis=1)
+    ccen = xyz[cn].mean(axis=1)
+    boundary = fcells[:, 1] < 0
+    bc = bc_fn(fcen, ccen, fn, fcells, boundary).astype(np.int8)
+    bc[~boundary] = BC_INTERIOR
+    method = method_fn(fcells, zone).astype(np.int8)
+    bstate = bstate_fn(fcen, bc) if bstate_fn else None
+    tvd = np.ones(cn.shape[0], np.uint8)
+    if tvd_fn is not None:
+        tvd = tvd_fn(fcells, bc).astype(np.uint8)
+    return Mesh(n_nodes=xyz.shape[0], n_cells=cn.shape[0], n_faces=nf, cell_nodes=cn, face_nodes=fn,
+                face_cells=fcells, cell_faces=cf, face_bc=bc, face_method=method, face_flags=None,
+                face_bstate=bstate, cell_zone=zone.astype(np.int8), cell_is_tvd=tvd, node_xyz=xyz,
+                node_disp=disp), ccen
+
+
+# ---------------------------------------------------------------------------------------------
+def cube_brio_wu(n=32, jitter=0.15, seed_mesh=1, seed_state=2):
+    """Brio-Wu shock tube on a jittered n^3 cube; returns (mesh, prim0 [nc,8], params)."""
+    xyz, cn = mesh_cube(n, jitter, seed_mesh)
+    zone = np.full(cn.shape[0], 3, np.int8)
+    mesh, cc = _finish(xyz, cn, lambda fc, cc, fn, fcl, b: np.where(b, BC_ZERO_GRADIENT, BC_INTERIOR),
+                       lambda fcl, z: np.full(fcl.shape[0], 3), zone)
+    nc = cn.shape[0]
+    prim = np.zeros((nc, 8))
+    left = cc[:, 0] < 0.5
+    prim[:, 0] = np.where(left, 1.0, 0.125)
+    prim[:, 4] = np.where(left, 1.0, 0.1)
+    prim[:, 5] = 0.75 * SPI4
+    prim[:, 6] = np.where(left, 1.0, -1.0) * SPI4
+    noise = 2.0 * splitmix_uniform(seed_state, np.arange(nc)) - 1.0
+    prim[:, 0] *= 1.0 + 0.01 * noise
+    return mesh, prim, default_params()
+
+
+# ---------------------------------------------------------------------------------------------
+def solar_wind_state(x):
+    """Supersonic wind: n=7 cm^-3, V=375 km/s radial, T=5e4 K at 1 AU; rho ~ r^-2,
+    p ~ r^-2gamma; Parker B (|B| = 5 nT at 1 AU on the equator), B_r = -B0 (R0/r)^2,
+    B_phi = -B_r sin(theta) r/R0 (the form of Setka_physics.cpp:510-513)."""
+    x = np.asarray(x, dtype=np.float64)
+    r = np.linalg.norm(x, axis=1)
+    rho1 = 7.0 / CHAR_RHO
+    v1 = 375.0 / CHAR_V
+    p1 = 2.0 * 7.0 * K_B * 5.0e4 / CHAR_P
+    b_au = 50.0 / CHAR_B_UG
+    B0 = b_au / math.sqrt(2.0)
+    s = R0_AU / r
+    out = np.zeros((x.shape[0], 8))
+    out[:, 0] = rho1 * s ** 2
+    out[:, 1:4] = v1 * x / r[:, None]
+    out[:, 4] = p1 * s ** (2.0 * GAMMA)
+    theta = np.arccos(np.clip(x[:, 2] / r, -1.0, 1.0))
+    phi = np.arctan2(x[:, 1], x[:, 0])
+    br = -B0 * s ** 2
+    bphi = -br * np.sin(theta) * (r / R0_AU)
+    rhat = x / r[:, None]
+    phat = np.stack([-np.sin(phi), np.cos(phi), np.zeros_like(phi)], axis=1)
+    out[:, 5:8] = br[:, None] * rhat + bphi[:, None] * phat
+    return out
+
+
+def lism_state():
+    """Interstellar inflow: n_p = 0.04 cm^-3, V = 26.4 km/s along -x, T = 7500 K, |B| = 4 uG at
+    60 degrees to the flow direction."""
+    rho = 0.04 / CHAR_RHO
+    v = -26.4 / CHAR_V
+    p = 2.0 * 0.04 * K_B * 7500.0 / CHAR_P
+    b = 4.0 / CHAR_B_UG
+    a = math.radians(60.0)
+    return np.array([rho, v, 0.0, 0.0, p, -b * math.cos(a), b * math.sin(a), 0.0])
+
+
+def shell_radii(n_layers, r_out_au=1500.0):
+    return R0_AU * r_out_au ** (np.arange(n_layers + 1) / n_layers)
+
+
+def shell(n_edge=37, n_layers=120, moving=False, seed_motion=3, r_zone1_au=80.0, r_blend_au=100.0,
+          amp=0.05, omega=100.0):
+    """Heliosphere-like shell; returns (mesh, prim0, params)."""
+    radii = shell_radii(n_layers)
+    xyz, cn = mesh_shell(n_edge, radii)
+    rin, rout = radii[0], radii[-1]
+    rmid = math.sqrt(rin * rout)
+    ccen = xyz[cn].mean(axis=1)
+    rc = np.linalg.norm(ccen, axis=1)
+    zone = np.where(rc < r_zone1_au * R0_AU, 1, 3).astype(np.int8)
+    lism = lism_state()
+
+    def bc_fn(fc, cc, fn, fcl, b):
+        rf = np.linalg.norm(fc, axis=1)
+        inner = b & (rf < rmid)
+        outer = b & ~inner
+        # outward normal at the outer sphere is radial: inflow where V_inf . n < 0, i.e. x > 0
+        bc = np.full(fc.shape[0], BC_INTERIOR)
+        bc[inner] = BC_INNER_HARD
+        bc[outer & (fc[:, 0] > 0.0)] = BC_OUTER_HARD
+        bc[outer & (fc[:, 0] <= 0.0)] = BC_OUTER_SOFT
+        return bc
+
+    def method_fn(fcl, z):
+        za = z[fcl[:, 0]]
+        zb = np.where(fcl[:, 1] >= 0, z[np.maximum(fcl[:, 1], 0)], 3)
+        return np.where((za == 1) | (zb == 1), 2, 3)
+
+    def bstate_fn(fc, bc):
+        bs = np.zeros((fc.shape[0], 8))
+        ih = bc == BC_INNER_HARD
+        bs[ih] = solar_wind_state(fc[ih])
+        bs[bc == BC_OUTER_HARD] = lism
+        return bs
+
+    def tvd_fn(fcl, bc):
+        t = np.ones(cn.shape[0], np.uint8)
+        t[fcl[bc == BC_INNER_HARD, 0]] = 0   # Init_physics: cells on Inner_Hard faces (Setka_physics.cpp:585)
+        return t
+
+    disp = None
+    if moving:
+        rn = np.linalg.norm(xyz, axis=1)
+        th = np.arccos(np.clip(xyz[:, 2] / rn, -1.0, 1.0))
+        ph = np.arctan2(xyz[:, 1], xyz[:, 0])
+        u = splitmix_uniform(seed_motion, np.arange(3))
+        phase = 2.0 * math.pi * u
+        taper = np.clip((1.0 - rin / rn) * (1.0 - rn / rout) * 4.0, 0.0, 1.0)
+        amp_n = amp * rn * np.sin(3.0 * th + phase[0]) * np.sin(2.0 * ph + phase[1]) * taper
+        disp = amp_n[:, None] * xyz / rn[:, None]
+    mesh, _ = _finish(xyz, cn, bc_fn, method_fn, zone, tvd_fn=tvd_fn, bstate_fn=bstate_fn, disp=disp)
+    # interior: r^-2 wind blended to the interstellar state around r_blend (tanh in r)
+    wind = solar_wind_state(ccen)
+    w = 0.5 * (1.0 + np.tanh((rc - r_blend_au * R0_AU) / (0.2 * r_blend_au * R0_AU)))
+    prim = (1.0 - w)[:, None] * wind + w[:, None] * lism[None, :]
+    params = default_params()
+    if moving:
+        params.update(move_mesh=1, motion_omega=omega, motion_phase=float(2.0 * math.pi * splitmix_uniform(seed_motion, 3)))
+    return mesh, prim, params
+
+
+def neutrals(n_cells, seed=5):
+    """Per-cell neutral-H moments [n_H, c_H^2, V_H(3)] in code units: n_H ~ U(0.08, 0.2) cm^-3,
+    V_H = 26 km/s along the inflow axis (-x) with +-30 % per-component perturbation, T_H ~
+    U(6e3, 2e4) K (c_H^2 = 2 k T / m_p)."""
+    idx = np.arange(n_cells, dtype=np.uint64) * np.uint64(6)
+    u = [splitmix_uniform(seed, idx + np.uint64(k)) for k in range(6)]
+    out = np.zeros((n_cells, 5))
+    out[:, 0] = (0.08 + 0.12 * u[0]) / CHAR_RHO
+    T = 6.0e3 + 1.4e4 * u[1]
+    out[:, 1] = 2.0 * K_B * T / M_P / (CHAR_V * 1e5) ** 2
+    v0 = 26.0 / CHAR_V
+    out[:, 2] = -v0 + 0.3 * v0 * (2.0 * u[2] - 1.0)
+    out[:, 3] = 0.3 * v0 * (2.0 * u[3] - 1.0)
+    out[:, 4] = 0.3 * v0 * (2.0 * u[4] - 1.0)
+    return out
+
+
+WORKLOADS = {
+    "cube32": dict(fn="cube", n=32),
+    "shell1m": dict(fn="shell", n_edge=37, n_layers=120),
+    "shell1m_moving": dict(fn="shell", n_edge=37, n_layers=120, moving=True),
+    "shell4m_moving": dict(fn="shell", n_edge=53, n_layers=240, moving=True),
+}
+
+
+def build(name, **over):
+    spec = dict(WORKLOADS[name])
+    spec.update(over)
+    fn = spec.pop("fn")
+    if fn == "cube":
+        return cube_brio_wu(**spec)
+    return shell(**spec)

@@ -1,0 +1,82 @@
+// cx_device.h -- exact FP64 Maxwellian charge-exchange source (device).
+//
+// Rate integral of MK_source/MK_source/MK_source/MK_source.cpp:38-103 (midpoint rule in
+// R0 in [0, 5c], alpha in [0, pi]) extended by the first and third velocity moments, with the
+// Lindsay & Stebbings (2005) cross-section.  For Maxwellian protons (V_p, c_p) and atoms
+// (V_H, c_H) the relative velocity is Maxwellian about dV = V_p - V_H with c^2 = c_p^2 + c_H^2:
+//   J0 = <g s(g)>, J1 = <g s(g) g.e>, J2 = <g^3 s(g)>, e = dV/|dV|, U = |dV|
+//   rate = n_p n_H J0;  S_m = -n_p n_H J1 e;
+//   S_E  = -(n_p n_H / 2) [ (V_p + V_H).e J1 + kappa (J2 - U J1) ],  kappa = (c_p^2 - c_H^2)/c^2.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace titan {
+
+// sigma_LS(E) = (4.15 - 0.531 ln E)^2 (1 - exp(-67.3/E))^4.5 1e-16 cm^2, E = m_p g^2/2 in keV,
+// times n_char L_char (0.06 cm^-3 x 4.2113 AU) so that rates are per unit code time.
+__device__ __forceinline__ double sigma_ls(double g) {
+    const double e_per_g2 = 0.5 * 1.67262192369e-24 * (1.03804e6 * 1.03804e6) / 1.602176634e-9;
+    const double nl = 0.06 * 4.2113 * 1.495978707e13;
+    double E = e_per_g2 * g * g;
+    if (E < 1e-12) E = 1e-12;
+    const double a = 4.15 - 0.531 * log(E);
+    const double b = 1.0 - exp(-67.3 / E);
+    return a * a * pow(b, 4.5) * 1e-16 * nl;
+}
+
+// One state; sc[2*j] = sin, sc[2*j+1] = cos of the j-th alpha node (N2 nodes).
+__device__ __forceinline__ void cx_source_one(double n_p, double cp2, const double vp[3], double n_h, double ch2,
+                                              const double vh[3], int N1, int N2, const double *sc,
+                                              double out[5]) {
+    const double kPiL = 3.14159265358979323846;
+    const double dv0 = vp[0] - vh[0], dv1 = vp[1] - vh[1], dv2 = vp[2] - vh[2];
+    const double U = sqrt((dv0 * dv0 + dv1 * dv1) + dv2 * dv2);
+    const double c2 = cp2 + ch2;
+    const double c = sqrt(c2);
+    double e0 = 1.0, e1 = 0.0, e2 = 0.0;
+    if (U > 0.0) { e0 = dv0 / U; e1 = dv1 / U; e2 = dv2 / U; }
+    double J0 = 0.0, J1 = 0.0, J2 = 0.0;
+    const double dal = kPiL / N2;
+    const double dR0 = 5.0 * c / N1;
+    for (int ii = 0; ii < N1; ++ii) {
+        const double R0 = (ii + 0.5) * dR0;
+        const double wr = R0 * R0 * exp(-(R0 * R0) / c2);
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int jj = 0; jj < N2; ++jj) {
+            const double sa = sc[2 * jj], ca = sc[2 * jj + 1];
+            const double gpar = U - R0 * ca;
+            const double gper = R0 * sa;
+            const double g2 = gpar * gpar + gper * gper;
+            const double g = sqrt(g2);
+            const double gs = g * sigma_ls(g) * sa;
+            s0 += gs;
+            s1 += gs * gpar;
+            s2 += gs * g2;
+        }
+        J0 += wr * s0;
+        J1 += wr * s1;
+        J2 += wr * s2;
+    }
+    const double norm = 2.0 * kPiL * dal * dR0 / (kPiL * sqrt(kPiL) * c2 * c);
+    J0 *= norm; J1 *= norm; J2 *= norm;
+    const double nn = n_p * n_h;
+    const double kappa = (cp2 - ch2) / c2;
+    const double sv = (vp[0] + vh[0]) * e0 + (vp[1] + vh[1]) * e1 + (vp[2] + vh[2]) * e2;
+    out[0] = nn * J0;
+    out[1] = -nn * J1 * e0;
+    out[2] = -nn * J1 * e1;
+    out[3] = -nn * J1 * e2;
+    out[4] = -0.5 * nn * (sv * J1 + kappa * (J2 - U * J1));
+}
+
+// Fills sc[0..2*N2) cooperatively (block-wide); caller syncs.
+__device__ __forceinline__ void cx_fill_angles(double *sc, int N2) {
+    const double kPiL = 3.14159265358979323846;
+    for (int j = threadIdx.x; j < N2; j += blockDim.x) {
+        const double al = (j + 0.5) * (kPiL / N2);
+        sc[2 * j] = sin(al);
+        sc[2 * j + 1] = cos(al);
+    }
+}
+
+}  // namespace titan

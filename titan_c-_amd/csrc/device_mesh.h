@@ -1,0 +1,66 @@
+// device_mesh.h -- device-resident mesh, geometry and step scalars shared by host and kernels.
+//
+// Layout in HBM (all arrays in the device numbering: cells along a Morton curve of their
+// initial centres, faces ordered by their lower device cell, nodes by first use):
+//   cell_nodes [Nc*8] i32, cell_faces [Nc*6] i32 (a cell's faces in ascending REFERENCE face id,
+//   the summation order of Setka_physics.cpp:1702-1733), cell_code [Nc] u32,
+//   face_nodes [Nf*4] i32, face_cells [Nf*2] i32, face_code [Nf] u32, face_bidx [Nf] i32,
+//   bstate [Nb*8] f64, node xyz [2][Nn*3] f64, prim [2][Nc*8] f64 (cell-major, 64 B per cell),
+//   geometry per layer: cell centre [Nc*3], volume [Nc], face centre/normal [Nf*3], area [Nf].
+#pragma once
+#include <stdint.h>
+
+namespace titan {
+
+// cell_code bits: 0..5 = sign of face j (1: the cell is cells[0] of its j-th face),
+// 6 = is_TVD, 8..11 = zone (Type_cell::Zone_1..4)
+__host__ __device__ inline uint32_t cell_code_make(uint32_t fsign, uint32_t tvd, uint32_t zone) {
+    return (fsign & 63u) | ((tvd & 1u) << 6) | ((zone & 15u) << 8);
+}
+// face_code bits: 0..3 = boundary code, 4..7 = Riemann method, 8..15 = chlld flags
+__host__ __device__ inline uint32_t face_code_make(uint32_t bc, uint32_t method, uint32_t flags) {
+    return (bc & 15u) | ((method & 15u) << 4) | ((flags & 255u) << 8);
+}
+
+struct DevMesh {
+    int64_t n_nodes, n_cells, n_faces;
+    const int32_t *cell_nodes;
+    const int32_t *cell_faces;
+    const uint32_t *cell_code;
+    const int32_t *face_nodes;
+    const int32_t *face_cells;
+    const uint32_t *face_code;
+    const int32_t *face_bidx;
+    const double *bstate;
+    const int32_t *face_ref;   // device face -> reference face id (error reports)
+    const int32_t *cell_ref;   // device cell -> reference cell id
+};
+
+struct DevGeom {
+    double *cell_center;
+    double *cell_volume;
+    double *face_center;
+    double *face_normal;
+    double *face_area;
+};
+
+struct DevParams {
+    double gamma, kfl, v_inf;
+    int32_t move_mesh, n_disc, geom_mode, pad;
+    double motion_omega, motion_phase;
+};
+
+// Device-side scalars of the step loop (one instance per context).
+struct StepScalars {
+    double dt;                       // dt of the current step (lagged, Setka_physics.cpp:1606)
+    double t;                        // simulated time at the start of the current step
+    double motion_fac;               // dt * omega * cos(omega t + phase)
+    double pad0;
+    unsigned long long dtmin_bits[2];  // running min of the CFL step, slot = step & 1
+    unsigned long long n_rho_floor, n_p_floor, n_fallback;
+    int err_code;
+    int pad1;
+    long long err_index;             // smallest reference id that failed, or LLONG_MAX
+};
+
+}  // namespace titan

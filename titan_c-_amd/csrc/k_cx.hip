@@ -1,0 +1,77 @@
+// k_cx.hip -- exact FP64 charge-exchange source kernels (FP64, no contraction).
+#include "cx_device.h"
+#include "launch.h"
+
+namespace titan {
+
+__global__ void __launch_bounds__(256) k_cx_exact(int64_t n, const double *__restrict__ in10, int nodes,
+                                                  double *__restrict__ out5) {
+    extern __shared__ double sc[];
+    cx_fill_angles(sc, nodes);
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *a = in10 + 10 * i;
+    const double vp[3] = {a[2], a[3], a[4]}, vh[3] = {a[7], a[8], a[9]};
+    double o[5];
+    cx_source_one(a[0], a[1], vp, a[5], a[6], vh, nodes, nodes, sc, o);
+#pragma unroll
+    for (int k = 0; k < 5; k++) out5[5 * i + k] = o[k];
+}
+
+// Per cell: n_p = rho, c_p^2 = p/rho, V_p from the state; neutrals [5] = n_H, c_H^2, V_H.
+__device__ __forceinline__ void cx_cell(const double *__restrict__ prim, const double *__restrict__ neutrals,
+                                        int64_t c, int nodes, const double *sc, double o[5]) {
+    const double *s = prim + 8 * c;
+    const double *h = neutrals + 5 * c;
+    const double vp[3] = {s[1], s[2], s[3]}, vh[3] = {h[2], h[3], h[4]};
+    cx_source_one(s[0], s[4] / s[0], vp, h[0], h[1], vh, nodes, nodes, sc, o);
+}
+
+__global__ void __launch_bounds__(256) k_cx_cells(int64_t nc, const double *__restrict__ prim,
+                                                  const double *__restrict__ neutrals, int nodes,
+                                                  double *__restrict__ src) {
+    extern __shared__ double sc[];
+    cx_fill_angles(sc, nodes);
+    __syncthreads();
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double o[5];
+    cx_cell(prim, neutrals, c, nodes, sc, o);
+#pragma unroll
+    for (int k = 0; k < 5; k++) src[5 * c + k] = o[k];
+}
+
+// Exact source for the cells of a list (the surrogate's out-of-range fallback).
+__global__ void __launch_bounds__(256) k_cx_list(const int32_t *__restrict__ list, const unsigned long long *count,
+                                                 int64_t cap, const double *__restrict__ prim,
+                                                 const double *__restrict__ neutrals, int nodes,
+                                                 double *__restrict__ src) {
+    extern __shared__ double sc[];
+    cx_fill_angles(sc, nodes);
+    __syncthreads();
+    const int64_t n = (int64_t)min((unsigned long long)cap, *count);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = list[i];
+        double o[5];
+        cx_cell(prim, neutrals, c, nodes, sc, o);
+#pragma unroll
+        for (int k = 0; k < 5; k++) src[5 * c + k] = o[k];
+    }
+}
+
+void launch_cx_exact(hipStream_t s, int64_t n, const double *in10, int nodes, double *out5) {
+    hipLaunchKernelGGL(k_cx_exact, dim3((unsigned)((n + 255) / 256)), dim3(256), 16 * nodes, s, n, in10, nodes, out5);
+}
+
+void launch_cx_cells(hipStream_t s, int64_t nc, const double *prim, const double *neutrals, int nodes, double *src) {
+    hipLaunchKernelGGL(k_cx_cells, dim3((unsigned)((nc + 255) / 256)), dim3(256), 16 * nodes, s, nc, prim, neutrals,
+                       nodes, src);
+}
+
+void launch_cx_list(hipStream_t s, const int32_t *list, const unsigned long long *count, int64_t cap,
+                    const double *prim, const double *neutrals, int nodes, double *src) {
+    hipLaunchKernelGGL(k_cx_list, dim3(1024), dim3(256), 16 * nodes, s, list, count, cap, prim, neutrals, nodes, src);
+}
+
+}  // namespace titan

@@ -1,0 +1,62 @@
+// k_step32.hip -- mixed-precision face flux: FP32 Riemann arithmetic (contracted, fast division
+// and square root), FP64 states, geometry, area scaling and CFL step.
+#define TITAN_FLUX_ONLY 1
+#include "step_kernels.h"
+#include "launch.h"
+
+namespace titan {
+
+static inline unsigned grid_of32(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+void launch_face_flux_f32(hipStream_t s, const DevMesh &M, const DevParams &P, const double *prim,
+                          const DevGeom &G, const double *X1, const double *X2, StepScalars *S, double *flux,
+                          uint8_t *branch, long long step, double dt_override) {
+    hipLaunchKernelGGL(k_face_flux<float>, dim3(grid_of32(M.n_faces, 256)), dim3(256), 0, s, M, P, prim, G, X1, X2,
+                       S, flux, branch, step, dt_override);
+}
+
+template <int K>
+__global__ void __launch_bounds__(256) k_chlld_batch32(int64_t n, const int8_t *__restrict__ n_state,
+                                                       const int8_t *__restrict__ n_disc,
+                                                       const uint8_t *__restrict__ flags,
+                                                       const double *__restrict__ normal,
+                                                       const double *__restrict__ w, const double *__restrict__ qL,
+                                                       const double *__restrict__ qR,
+                                                       const double *__restrict__ kL,
+                                                       const double *__restrict__ kR, double *__restrict__ q,
+                                                       double *__restrict__ k, double *__restrict__ ds,
+                                                       uint8_t *__restrict__ branch, int *err) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float a[8], b[8], ka[K > 0 ? K : 1], kb[K > 0 ? K : 1];
+#pragma unroll
+    for (int j = 0; j < 8; j++) { a[j] = (float)qL[8 * i + j]; b[j] = (float)qR[8 * i + j]; }
+#pragma unroll
+    for (int j = 0; j < K; j++) { ka[j] = (float)kL[K * i + j]; kb[j] = (float)kR[K * i + j]; }
+    RiemannOut<float, K> o;
+    chlld<float, K>(n_state[i], n_disc ? n_disc[i] : 0, flags ? flags[i] : 0, (float)normal[3 * i],
+                    (float)normal[3 * i + 1], (float)normal[3 * i + 2], w ? (float)w[i] : 0.0f, a, b, ka, kb, o);
+#pragma unroll
+    for (int j = 0; j < 8; j++) q[8 * i + j] = (double)o.q[j];
+#pragma unroll
+    for (int j = 0; j < K; j++) k[K * i + j] = o.ok ? (double)o.k[j] : 0.0;
+    if (ds) { ds[3 * i] = (double)o.dsl; ds[3 * i + 1] = (double)o.dsc; ds[3 * i + 2] = (double)o.dsr; }
+    if (branch) branch[i] = (uint8_t)o.branch;
+    if (!o.ok) atomicExch(err, 1);
+}
+
+void launch_chlld_batch_f32(hipStream_t s, int64_t n, int K, const int8_t *n_state, const int8_t *n_disc,
+                            const uint8_t *flags, const double *normal, const double *w, const double *qL,
+                            const double *qR, const double *kL, const double *kR, double *q, double *k, double *ds,
+                            uint8_t *branch, int *err) {
+    const dim3 g(grid_of32(n, 256)), b(256);
+    switch (K) {
+    case 0: hipLaunchKernelGGL((k_chlld_batch32<0>), g, b, 0, s, n, n_state, n_disc, flags, normal, w, qL, qR, kL, kR, q, k, ds, branch, err); break;
+    case 1: hipLaunchKernelGGL((k_chlld_batch32<1>), g, b, 0, s, n, n_state, n_disc, flags, normal, w, qL, qR, kL, kR, q, k, ds, branch, err); break;
+    case 2: hipLaunchKernelGGL((k_chlld_batch32<2>), g, b, 0, s, n, n_state, n_disc, flags, normal, w, qL, qR, kL, kR, q, k, ds, branch, err); break;
+    case 3: hipLaunchKernelGGL((k_chlld_batch32<3>), g, b, 0, s, n, n_state, n_disc, flags, normal, w, qL, qR, kL, kR, q, k, ds, branch, err); break;
+    default: hipLaunchKernelGGL((k_chlld_batch32<4>), g, b, 0, s, n, n_state, n_disc, flags, normal, w, qL, qR, kL, kR, q, k, ds, branch, err); break;
+    }
+}
+
+}  // namespace titan

@@ -1,0 +1,39 @@
+// launch.h -- host-callable launchers of the HIP kernels (defined in the .hip translation units).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "device_mesh.h"
+
+namespace titan {
+
+void launch_begin_step(hipStream_t s, StepScalars *S, long long n, const DevParams &P);
+void launch_motion(hipStream_t s, int64_t n3, const double *x_old, const double *disp, const StepScalars *S,
+                   double *x_new);
+void launch_measure(hipStream_t s, const DevMesh &M, const double *X, const DevGeom &G, int gcl);
+void launch_face_flux(hipStream_t s, int precision, const DevMesh &M, const DevParams &P, const double *prim,
+                      const DevGeom &G, const double *X1, const double *X2, StepScalars *S, double *flux,
+                      uint8_t *branch, long long step, double dt_override);
+void launch_face_flux_f32(hipStream_t s, const DevMesh &M, const DevParams &P, const double *prim,
+                          const DevGeom &G, const double *X1, const double *X2, StepScalars *S, double *flux,
+                          uint8_t *branch, long long step, double dt_override);
+void launch_cell_update(hipStream_t s, const DevMesh &M, const double *prim_old, const double *vol_old,
+                        const double *vol_new, const double *flux, const double *src, StepScalars *S,
+                        double *prim_new, double dt_override);
+
+// chlld over n independent states; err (device int) set to 1 on a fatal branch
+void launch_chlld_batch(hipStream_t s, int precision, int64_t n, int K, const int8_t *n_state,
+                        const int8_t *n_disc, const uint8_t *flags, const double *normal, const double *w,
+                        const double *qL, const double *qR, const double *kL, const double *kR, double *q,
+                        double *k, double *ds, uint8_t *branch, int *err);
+void launch_chlld_batch_f32(hipStream_t s, int64_t n, int K, const int8_t *n_state, const int8_t *n_disc,
+                            const uint8_t *flags, const double *normal, const double *w, const double *qL,
+                            const double *qR, const double *kL, const double *kR, double *q, double *k,
+                            double *ds, uint8_t *branch, int *err);
+
+// exact charge-exchange source: in10 [n*10] (n_p, c_p^2, V_p, n_H, c_H^2, V_H) -> out5 [n*5]
+void launch_cx_exact(hipStream_t s, int64_t n, const double *in10, int nodes, double *out5);
+// per-cell exact source from the plasma state and the neutral moments [Nc*5]
+void launch_cx_cells(hipStream_t s, int64_t nc, const double *prim, const double *neutrals, int nodes,
+                     double *src);
+
+}  // namespace titan

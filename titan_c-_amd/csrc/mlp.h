@@ -1,0 +1,49 @@
+// mlp.h -- charge-exchange surrogate network 8-64-64-64-5 (SiLU): host-side state and launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace titan {
+
+constexpr int kMlpIn = 8, kMlpH = 64, kMlpOut = 5;
+constexpr int64_t kMlpParams = (int64_t)kMlpH * kMlpIn + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH * kMlpH + kMlpH +
+                               kMlpOut * kMlpH + kMlpOut;  // 9221
+// packed BF16 MFMA operands: 1408 lane-vectors of 8 bf16, and 224 permuted FP32 biases
+constexpr int kMlpPackVec = 1408;
+constexpr int kMlpPackBias = 224;
+// normalisation block: in_mean[8], in_std[8], out_scale[5], box_lo[8], box_hi[8]
+constexpr int kMlpNorm = 37;
+
+struct MlpState {
+    bool ready = false;
+    float *d_params = nullptr, *d_grad = nullptr, *d_m = nullptr, *d_v = nullptr;
+    uint16_t *d_wpack = nullptr;
+    float *d_bpack = nullptr;
+    float *d_norm = nullptr;
+    float h_norm[kMlpNorm] = {};
+    int32_t *d_list = nullptr;
+    int64_t list_cap = 0;
+    unsigned long long *d_count = nullptr;
+    float *d_ws = nullptr;
+    size_t ws_floats = 0;
+    long long adam_t = 0;
+};
+
+void mlp_free(MlpState &s);
+int mlp_set(MlpState &s, hipStream_t st, const float *params, const float *in_norm16, const float *out_scale5,
+            const float *box16);
+int mlp_get(MlpState &s, hipStream_t st, float *params);
+int mlp_forward_host(MlpState &s, hipStream_t st, int64_t n, const float *x, float *y);
+int mlp_loss_grad_host(MlpState &s, hipStream_t st, int64_t n, const float *x, const float *y, float *loss,
+                       float *grad);
+int mlp_train_host(MlpState &s, hipStream_t st, int64_t n, const float *x, const float *y, int epochs, int batch,
+                   float lr, uint64_t seed, float *loss_hist);
+// Surrogate source of every cell (device arrays in device numbering); out-of-range cells get the
+// exact source.  fallback_total (device) accumulates the number of such cells.
+int mlp_cell_sources(MlpState &s, hipStream_t st, int64_t nc, const double *prim, const double *neutrals, int nodes,
+                     double *src, unsigned long long *fallback_total);
+
+void launch_cx_list(hipStream_t s, const int32_t *list, const unsigned long long *count, int64_t cap,
+                    const double *prim, const double *neutrals, int nodes, double *src);
+
+}  // namespace titan

@@ -1,0 +1,426 @@
+// riemann.h -- device Riemann solvers (Lax / HLL / HLLC / HLLD) for a moving face.
+//
+// Restates Phys_param::chlld (TITAN/source/Phys_param.cpp:882-1878) and get_bazis
+// (TITAN/source/Help_funk.cpp:219-232) for one face per lane.  Instantiated for T = double
+// (compiled with -ffp-contract=off: every expression is evaluated in the reference's order,
+// giving bit-identical fluxes and branch choices) and T = float (the mixed-precision step).
+//
+// Differences of form, not of result: the reference fills the intermediate states of every
+// region and then picks one; here the region is chosen first with the same comparisons in the
+// same order (later matching tests overwrite earlier ones, as the reference's chain of separate
+// `if`s does) and only that region's formula is evaluated.  This keeps the HLLD working set in
+// registers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace titan {
+
+template <typename T> struct RConst;
+template <> struct RConst<double> {
+    static __device__ __forceinline__ double spi4() { return 3.5449077018110318; }  // sqrt(4*pi)
+};
+template <> struct RConst<float> {
+    static __device__ __forceinline__ float spi4() { return 3.5449077018110318f; }
+};
+
+// std::max / std::min semantics: the first argument wins on ties and NaN
+template <typename T> __device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
+template <typename T> __device__ __forceinline__ T smin(T a, T b) { return (b < a) ? b : a; }
+
+template <typename T>
+__device__ __forceinline__ T spf(T ax, T ay, T az, T bx, T by, T bz) {
+    return ax * bx + ay * by + az * bz;
+}
+
+// get_bazis: t = normalize(a x n), a = x (or y when |n.x| > 0.9); m = n x t.
+template <typename T>
+__device__ __forceinline__ void get_bazis(const T n[3], T t[3], T m[3]) {
+    T a0 = T(1), a1 = T(0), a2 = T(0);
+    T nd = (n[0] * a0 + n[1] * a1) + n[2] * a2;
+    if (fabs(nd) > T(0.9)) { a0 = T(0); a1 = T(1); a2 = T(0); }
+    T c0 = a1 * n[2] - a2 * n[1];
+    T c1 = a2 * n[0] - a0 * n[2];
+    T c2 = a0 * n[1] - a1 * n[0];
+    T z = (c0 * c0 + c1 * c1) + c2 * c2;
+    if (z > T(0)) {
+        T s = sqrt(z);
+        c0 = c0 / s; c1 = c1 / s; c2 = c2 / s;
+    }
+    t[0] = c0; t[1] = c1; t[2] = c2;
+    m[0] = n[1] * t[2] - n[2] * t[1];
+    m[1] = n[2] * t[0] - n[0] * t[2];
+    m[2] = n[0] * t[1] - n[1] * t[0];
+}
+
+// Result of one face solve.  branch = n_state<<4 | ibn<<3 | region; ok = 0 on the reference's
+// fatal branches (NaN wave speeds or an impossible HLLD ordering).
+template <typename T, int K>
+struct RiemannOut {
+    T q[8];
+    T k[K > 0 ? K : 1];
+    T dsl, dsc, dsr;
+    int branch;
+    int ok;
+};
+
+template <typename T, int K>
+__device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, T be, T ge, T w,
+                                      const T qL[8], const T qR[8], const T *kL, const T *kR,
+                                      RiemannOut<T, K> &out) {
+    const T spi4 = RConst<T>::spi4();
+    const T gamma = T(5.0 / 3.0);
+    const T g1 = gamma - T(1);
+    const T epsb = T(1e-4);
+    const bool left_ydar = flags & 1, contact = (flags >> 1) & 1;
+    T wv = w;
+
+    const T r1 = qL[0], u1 = qL[1], v1 = qL[2], w1 = qL[3], p1 = qL[4];
+    const T bx1 = qL[5] / spi4, by1 = qL[6] / spi4, bz1 = qL[7] / spi4;
+    const T r2 = qR[0], u2 = qR[1], v2 = qR[2], w2 = qR[3], p2 = qR[4];
+    const T bx2 = qR[5] / spi4, by2 = qR[6] / spi4, bz2 = qR[7] / spi4;
+    const T ro = (r2 + r1) / T(2);
+    const T ap = (p2 + p1) / T(2);
+
+    T n[3] = {al, be, ge}, t[3], m[3];
+    get_bazis(n, t, m);
+
+    T vL[3], vR[3], bL[3], bR[3];
+    vL[0] = spf(n[0], n[1], n[2], u1, v1, w1);
+    vL[1] = spf(t[0], t[1], t[2], u1, v1, w1);
+    vL[2] = spf(m[0], m[1], m[2], u1, v1, w1);
+    vR[0] = spf(n[0], n[1], n[2], u2, v2, w2);
+    vR[1] = spf(t[0], t[1], t[2], u2, v2, w2);
+    vR[2] = spf(m[0], m[1], m[2], u2, v2, w2);
+    bL[0] = spf(n[0], n[1], n[2], bx1, by1, bz1);
+    bL[1] = spf(t[0], t[1], t[2], bx1, by1, bz1);
+    bL[2] = spf(m[0], m[1], m[2], bx1, by1, bz1);
+    bR[0] = spf(n[0], n[1], n[2], bx2, by2, bz2);
+    bR[1] = spf(t[0], t[1], t[2], bx2, by2, bz2);
+    bR[2] = spf(m[0], m[1], m[2], bx2, by2, bz2);
+
+    const T aaL = bL[0] / sqrt(r1);
+    const T b2L = (bL[0] * bL[0] + bL[1] * bL[1]) + bL[2] * bL[2];
+    const T b21 = b2L / r1;
+    const T cL = sqrt(gamma * p1 / r1);
+    T qp = sqrt(b21 + cL * (cL + T(2) * aaL));
+    T qm = sqrt(b21 + cL * (cL - T(2) * aaL));
+    const T cfL = (qp + qm) / T(2);
+    const T ptL = p1 + b2L / T(2);
+
+    const T aaR = bR[0] / sqrt(r2);
+    const T b2R = (bR[0] * bR[0] + bR[1] * bR[1]) + bR[2] * bR[2];
+    const T b22 = b2R / r2;
+    const T cR = sqrt(gamma * p2 / r2);
+    qp = sqrt(b22 + cR * (cR + T(2) * aaR));
+    qm = sqrt(b22 + cR * (cR - T(2) * aaR));
+    const T cfR = (qp + qm) / T(2);
+    const T ptR = p2 + b2R / T(2);
+
+    const T aC = (aaL + aaR) / T(2);
+    const T b2o = (b22 + b21) / T(2);
+    const T cC = sqrt(gamma * ap / ro);
+    qp = sqrt(b2o + cC * (cC + T(2) * aC));
+    qm = sqrt(b2o + cC * (cC - T(2) * aC));
+    const T cfC = (qp + qm) / T(2);
+    const T vC1 = (vL[0] + vR[0]) / T(2);
+
+    T SL = T(0), SR = T(0);
+    if (left_ydar) n_disc = 2;
+    if (n_disc == 0) {
+        SL = smin((vL[0] - cfL), (vR[0] - cfR));
+        SR = smax((vL[0] + cfL), (vR[0] + cfR));
+    } else if (n_disc == 1) {
+        SL = smin((vL[0] - cfL), (vC1 - cfC));
+        SR = smax((vR[0] + cfR), (vC1 + cfC));
+    } else if (n_disc == 2) {
+        const T SL_1 = smin((vL[0] - cfL), (vC1 - cfC));
+        const T SR_1 = smax((vR[0] + cfR), (vC1 + cfC));
+        const T SL_2 = smin((vL[0] - cfL), (vR[0] - cfR));
+        const T SR_2 = smax((vL[0] + cfL), (vR[0] + cfR));
+        const T oo = T(0.6);
+        const T oo1 = T(1) - oo;
+        SL = oo * SL_1 + oo1 * SL_2;
+        SR = oo * SR_1 + oo1 * SR_2;
+    } else if (n_disc == 3) {
+        SL = smin(vL[0], vR[0]) - smax(cfL, cfR);
+        SR = smax(vL[0], vR[0]) + smax(cfL, cfR);
+    }
+    out.dsr = SR;
+    out.dsl = SL;
+
+    const T suR = SR - vR[0];
+    const T suL = SL - vL[0];
+    const T SM = (suR * r2 * vR[0] - ptR + ptL - suL * r1 * vL[0]) / (suR * r2 - suL * r1);
+    out.dsc = SM;
+    if (contact) wv = SM;
+
+    if (n_state == 0) {
+        const T TR0 = fabs(vL[0] + vR[0]) / T(2) + cfC;
+        SR = TR0;
+        SL = -TR0;
+    }
+
+    const T upt1 = (u1 * u1 + v1 * v1 + w1 * w1) / T(2);
+    const T sbv1 = u1 * bx1 + v1 * by1 + w1 * bz1;
+    const T upt2 = (u2 * u2 + v2 * v2 + w2 * w2) / T(2);
+    const T sbv2 = u2 * bx2 + v2 * by2 + w2 * bz2;
+    const T e1 = p1 / g1 + r1 * upt1 + b2L / T(2);
+    const T e2 = p2 / g1 + r2 * upt2 + b2R / T(2);
+
+    T FL[8], FR[8], UL[8], UR[8];
+    FL[0] = r1 * vL[0];
+    FL[1] = r1 * vL[0] * vL[0] + ptL - bL[0] * bL[0];
+    FL[2] = r1 * vL[0] * vL[1] - bL[0] * bL[1];
+    FL[3] = r1 * vL[0] * vL[2] - bL[0] * bL[2];
+    FL[4] = (e1 + ptL) * vL[0] - bL[0] * sbv1;
+    FL[5] = T(0);
+    FL[6] = vL[0] * bL[1] - vL[1] * bL[0];
+    FL[7] = vL[0] * bL[2] - vL[2] * bL[0];
+    FR[0] = r2 * vR[0];
+    FR[1] = r2 * vR[0] * vR[0] + ptR - bR[0] * bR[0];
+    FR[2] = r2 * vR[0] * vR[1] - bR[0] * bR[1];
+    FR[3] = r2 * vR[0] * vR[2] - bR[0] * bR[2];
+    FR[4] = (e2 + ptR) * vR[0] - bR[0] * sbv2;
+    FR[5] = T(0);
+    FR[6] = vR[0] * bR[1] - vR[1] * bR[0];
+    FR[7] = vR[0] * bR[2] - vR[2] * bR[0];
+    UL[0] = r1; UL[4] = e1; UR[0] = r2; UR[4] = e2;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        UL[i + 1] = r1 * vL[i];
+        UL[i + 5] = bL[i];
+        UR[i + 1] = r2 * vR[i];
+        UR[i + 5] = bR[i];
+    }
+    T FLk[K > 0 ? K : 1], FRk[K > 0 ? K : 1];
+#pragma unroll
+    for (int i = 0; i < K; i++) { FLk[i] = kL[i] * vL[0]; FRk[i] = kR[i] * vR[0]; }
+
+    T qq[8];
+    T SN, wbn;
+    int region = 0, ibn = 0;
+    out.ok = 1;
+
+    if (n_state <= 1) {  // Lax / HLL, Phys_param.cpp:1133-1257
+        if (left_ydar) wv = SL;
+        T TL = SL, TR = SR;
+        // UZ is needed only in the star region
+        if (SL > wv || left_ydar) { TL = T(0); region = 1; }
+        else if (SL <= wv && wv <= SR) { region = 2; }
+        else if (SR < wv) { TR = T(0); region = 3; }
+        else { region = 0; out.ok = 0; }
+        const T a = TR * TL;
+        const T b = TR - TL;
+#pragma unroll
+        for (int ik = 0; ik < 8; ik++) {
+            T FW;
+            if (region == 1) FW = wv * UL[ik];
+            else if (region == 3) FW = wv * UR[ik];
+            else FW = wv * ((SR * UR[ik] - SL * UL[ik] + FL[ik] - FR[ik]) / (SR - SL));
+            const T dq = UR[ik] - UL[ik];
+            qq[ik] = (TR * FL[ik] - TL * FR[ik] + a * dq) / b - FW;
+        }
+#pragma unroll
+        for (int ik = 0; ik < K; ik++) {
+            T FW;
+            if (region == 1) FW = wv * kL[ik];
+            else if (region == 3) FW = wv * kR[ik];
+            else FW = wv * ((SR * kR[ik] - SL * kL[ik] + FLk[ik] - FRk[ik]) / (SR - SL));
+            const T dq = kR[ik] - kL[ik];
+            out.k[ik] = (TR * FLk[ik] - TL * FRk[ik] + a * dq) / b - FW;
+        }
+        SN = smax(fabs(SL), fabs(SR));
+        if (wv >= SR) wbn = wv * bR[0];
+        else if (wv <= SL) wbn = wv * bL[0];
+        else wbn = wv * (bL[0] + bR[0]) / T(2);
+    } else {
+        // UZ (HLL average) components used by HLLC and HLLD: 0..3 and 5..7
+        T UZ[8];
+#pragma unroll
+        for (int ik = 0; ik < 8; ik++) UZ[ik] = (SR * UR[ik] - SL * UL[ik] + FL[ik] - FR[ik]) / (SR - SL);
+        const T suRm = suR / (SR - SM);
+        const T suLm = suL / (SL - SM);
+        const T rzR = r2 * suRm;
+        const T rzL = r1 * suLm;
+        const T ptz = (suR * r2 * ptL - suL * r1 * ptR + r1 * r2 * suR * suL * (vR[0] - vL[0])) /
+                      (suR * r2 - suL * r1);
+        if (n_state == 2) {  // HLLC, Phys_param.cpp:1258-1475
+            T vzR1 = vR[1] + UZ[5] * (bR[1] - UZ[6]) / suR / r2;
+            T vzR2 = vR[2] + UZ[5] * (bR[2] - UZ[7]) / suR / r2;
+            T vzL1 = vL[1] + UZ[5] * (bL[1] - UZ[6]) / suL / r1;
+            T vzL2 = vL[2] + UZ[5] * (bL[2] - UZ[7]) / suL / r1;
+            T bzR1 = UZ[6], bzR2 = UZ[7], bzL1 = UZ[6], bzL2 = UZ[7];
+            const T sbvz = (UZ[5] * UZ[1] + UZ[6] * UZ[2] + UZ[7] * UZ[3]) / UZ[0];
+            const T ezR = e2 * suRm + (ptz * SM - ptR * vR[0] + UZ[5] * (sbv2 - sbvz)) / (SR - SM);
+            const T ezL = e1 * suLm + (ptz * SM - ptL * vL[0] + UZ[5] * (sbv1 - sbvz)) / (SL - SM);
+            if (fabs(UZ[5]) <= epsb) {
+                vzR1 = vR[1]; vzR2 = vR[2]; vzL1 = vL[1]; vzL2 = vL[2];
+                bzR1 = bR[1] * suRm; bzR2 = bR[2] * suRm;
+                bzL1 = bL[1] * suLm; bzL2 = bL[2] * suLm;
+            }
+            if (left_ydar) wv = SL;
+            if (SL >= wv || left_ydar) region = 1;
+            else if (SL <= wv && SM >= wv) region = 2;
+            else if (SM <= wv && SR > wv) region = 3;
+            else if (SR <= wv) region = 4;
+            else { region = 0; out.ok = 0; }
+            if (region == 2) {
+                const T UZL[8] = {rzL, SM * rzL, vzL1 * rzL, vzL2 * rzL, ezL, UZ[5], bzL1, bzL2};
+#pragma unroll
+                for (int ik = 0; ik < 8; ik++) qq[ik] = FL[ik] + SL * (UZL[ik] - UL[ik]) - wv * UZL[ik];
+#pragma unroll
+                for (int ik = 0; ik < K; ik++) {
+                    const T z = kL[ik] * suLm;
+                    out.k[ik] = FLk[ik] + SL * (z - kL[ik]) - wv * z;
+                }
+            } else if (region == 3) {
+                const T UZR[8] = {rzR, SM * rzR, vzR1 * rzR, vzR2 * rzR, ezR, UZ[5], bzR1, bzR2};
+#pragma unroll
+                for (int ik = 0; ik < 8; ik++) qq[ik] = FR[ik] + SR * (UZR[ik] - UR[ik]) - wv * UZR[ik];
+#pragma unroll
+                for (int ik = 0; ik < K; ik++) {
+                    const T z = kR[ik] * suRm;
+                    out.k[ik] = FRk[ik] + SR * (z - kR[ik]) - wv * z;
+                }
+            } else if (region == 4) {
+#pragma unroll
+                for (int ik = 0; ik < 8; ik++) qq[ik] = FR[ik] - wv * UR[ik];
+#pragma unroll
+                for (int ik = 0; ik < K; ik++) out.k[ik] = FRk[ik] - wv * kR[ik];
+            } else {
+#pragma unroll
+                for (int ik = 0; ik < 8; ik++) qq[ik] = FL[ik] - wv * UL[ik];
+#pragma unroll
+                for (int ik = 0; ik < K; ik++) out.k[ik] = FLk[ik] - wv * kL[ik];
+            }
+        } else {  // HLLD, Phys_param.cpp:1476-1870
+            const T bn = UZ[5];
+            const T bn2 = bn * bn;
+            T tvR, tbR, tvL, tbL;
+            const T ttR = r2 * suR * (SR - SM) - bn2;
+            if (fabs(ttR) < T(0.000000001)) { tvR = T(0); tbR = T(0); }
+            else { tvR = (SM - vR[0]) / ttR; tbR = (r2 * suR * suR - bn2) / ttR; }
+            const T ttL = r1 * suL * (SL - SM) - bn2;
+            if (fabs(ttL) < T(0.000000001)) { tvL = T(0); tbL = T(0); }
+            else { tvL = (SM - vL[0]) / ttL; tbL = (r1 * suL * suL - bn2) / ttL; }
+            const T vzL1 = vL[1] - bn * bL[1] * tvL;
+            const T vzL2 = vL[2] - bn * bL[2] * tvL;
+            const T vzR1 = vR[1] - bn * bR[1] * tvR;
+            const T vzR2 = vR[2] - bn * bR[2] * tvR;
+            const T bzL1 = bL[1] * tbL, bzL2 = bL[2] * tbL;
+            const T bzR1 = bR[1] * tbR, bzR2 = bR[2] * tbR;
+            const T sbvL = bn * SM + bzL1 * vzL1 + bzL2 * vzL2;
+            const T sbvR = bn * SM + bzR1 * vzR1 + bzR2 * vzR2;
+            const T ezR = e2 * suRm + (ptz * SM - ptR * vR[0] + bn * (sbv2 - sbvR)) / (SR - SM);
+            const T ezL = e1 * suLm + (ptz * SM - ptL * vL[0] + bn * (sbv1 - sbvL)) / (SL - SM);
+            const T rzRs = sqrt(rzR);
+            const T rzLs = sqrt(rzL);
+            const T rzss = rzRs + rzLs;
+            const T rzps = rzRs * rzLs;
+            T SZL = SM - fabs(bn) / rzLs;
+            T SZR = SM + fabs(bn) / rzRs;
+            T sbn;
+            if (fabs(bn) > epsb) { sbn = fabs(bn) / bn; ibn = 1; }
+            else { sbn = T(0); ibn = 0; SZL = SM; SZR = SM; }
+            int j = -1;
+            if (left_ydar) wv = SL;
+            if (SL >= wv || left_ydar) j = 1;
+            else if (SL <= wv && SZL >= wv) j = 2;
+            if (ibn == 1) {
+                if (SZL <= wv && SM >= wv) j = 3;
+                else if (SM <= wv && SZR >= wv) j = 4;
+            }
+            if (SZR <= wv && SR >= wv) j = 5;
+            else if (SR <= wv) j = 6;
+            if (j == -1) { out.ok = 0; j = 1; region = 0; }
+            else region = j;
+            if (j == 1) {
+#pragma unroll
+                for (int ik = 0; ik < 8; ik++) qq[ik] = FL[ik] - wv * UL[ik];
+#pragma unroll
+                for (int ik = 0; ik < K; ik++) out.k[ik] = FLk[ik] - wv * kL[ik];
+            } else if (j == 6) {
+#pragma unroll
+                for (int ik = 0; ik < 8; ik++) qq[ik] = FR[ik] - wv * UR[ik];
+#pragma unroll
+                for (int ik = 0; ik < K; ik++) out.k[ik] = FRk[ik] - wv * kR[ik];
+            } else if (j == 2 || j == 3) {
+                const T UZL[8] = {rzL, SM * rzL, vzL1 * rzL, vzL2 * rzL, ezL, bn, bzL1, bzL2};
+                if (j == 2) {
+#pragma unroll
+                    for (int ik = 0; ik < 8; ik++) qq[ik] = FL[ik] + SL * (UZL[ik] - UL[ik]) - wv * UZL[ik];
+#pragma unroll
+                    for (int ik = 0; ik < K; ik++) {
+                        const T z = kL[ik] * suLm;
+                        out.k[ik] = FLk[ik] + SL * (z - kL[ik]) - wv * z;
+                    }
+                } else {
+                    const T vzz1 = (rzLs * vzL1 + rzRs * vzR1 + sbn * (bzR1 - bzL1)) / rzss;
+                    const T vzz2 = (rzLs * vzL2 + rzRs * vzR2 + sbn * (bzR2 - bzL2)) / rzss;
+                    const T bzz1 = (rzLs * bzR1 + rzRs * bzL1 + sbn * rzps * (vzR1 - vzL1)) / rzss;
+                    const T bzz2 = (rzLs * bzR2 + rzRs * bzL2 + sbn * rzps * (vzR2 - vzL2)) / rzss;
+                    const T sbzz = bn * SM + bzz1 * vzz1 + bzz2 * vzz2;
+                    const T ezzL = ezL - rzLs * sbn * (sbvL - sbzz);
+                    const T UZZL[8] = {rzL, SM * rzL, vzz1 * rzL, vzz2 * rzL, ezzL, bn, bzz1, bzz2};
+#pragma unroll
+                    for (int ik = 0; ik < 8; ik++)
+                        qq[ik] = FL[ik] + SZL * (UZZL[ik] - UZL[ik]) + SL * (UZL[ik] - UL[ik]) - wv * UZZL[ik];
+#pragma unroll
+                    for (int ik = 0; ik < K; ik++) {
+                        const T z = kL[ik] * suLm;
+                        out.k[ik] = FLk[ik] + SZL * (z - z) + SL * (z - kL[ik]) - wv * z;
+                    }
+                }
+            } else {  // j == 4 or 5
+                const T UZR[8] = {rzR, SM * rzR, vzR1 * rzR, vzR2 * rzR, ezR, bn, bzR1, bzR2};
+                if (j == 5) {
+#pragma unroll
+                    for (int ik = 0; ik < 8; ik++) qq[ik] = FR[ik] + SR * (UZR[ik] - UR[ik]) - wv * UZR[ik];
+#pragma unroll
+                    for (int ik = 0; ik < K; ik++) {
+                        const T z = kR[ik] * suRm;
+                        out.k[ik] = FRk[ik] + SR * (z - kR[ik]) - wv * z;
+                    }
+                } else {
+                    const T vzz1 = (rzLs * vzL1 + rzRs * vzR1 + sbn * (bzR1 - bzL1)) / rzss;
+                    const T vzz2 = (rzLs * vzL2 + rzRs * vzR2 + sbn * (bzR2 - bzL2)) / rzss;
+                    const T bzz1 = (rzLs * bzR1 + rzRs * bzL1 + sbn * rzps * (vzR1 - vzL1)) / rzss;
+                    const T bzz2 = (rzLs * bzR2 + rzRs * bzL2 + sbn * rzps * (vzR2 - vzL2)) / rzss;
+                    const T sbzz = bn * SM + bzz1 * vzz1 + bzz2 * vzz2;
+                    const T ezzR = ezR + rzRs * sbn * (sbvR - sbzz);
+                    const T UZZR[8] = {rzR, SM * rzR, vzz1 * rzR, vzz2 * rzR, ezzR, bn, bzz1, bzz2};
+#pragma unroll
+                    for (int ik = 0; ik < 8; ik++)
+                        qq[ik] = FR[ik] + SZR * (UZZR[ik] - UZR[ik]) + SR * (UZR[ik] - UR[ik]) - wv * UZZR[ik];
+#pragma unroll
+                    for (int ik = 0; ik < K; ik++) {
+                        const T z = kR[ik] * suRm;
+                        out.k[ik] = FRk[ik] + SZR * (z - z) + SR * (z - kR[ik]) - wv * z;
+                    }
+                }
+            }
+        }
+        SN = smax(fabs(SL), fabs(SR));
+        if (wv > SR) wbn = wv * bR[0];
+        else if (wv < SL) wbn = wv * bL[0];
+        else wbn = wv * (bL[0] + bR[0]) / T(2);
+    }
+
+    const T qv0 = qq[1], qv1 = qq[2], qv2 = qq[3];
+    const T qb0 = -SN * (bR[0] - bL[0]) - wbn, qb1 = qq[6], qb2 = qq[7];
+    out.q[0] = qq[0];
+    out.q[4] = qq[4];
+#pragma unroll
+    for (int ik = 0; ik < 3; ik++) {
+        out.q[ik + 1] = n[ik] * qv0 + t[ik] * qv1 + m[ik] * qv2;
+        const T b = n[ik] * qb0 + t[ik] * qb1 + m[ik] * qb2;
+        out.q[ik + 5] = spi4 * b;
+    }
+    out.branch = (n_state << 4) | (ibn << 3) | region;
+    if (!out.ok) {
+#pragma unroll
+        for (int ik = 0; ik < 8; ik++) out.q[ik] = T(0);
+    }
+}
+
+}  // namespace titan

@@ -1,0 +1,634 @@
+// titan_host.cpp -- C ABI of the MI355X step: context, device buffers, numbering, step driver.
+//
+// Host counterpart of Setka::Go (TITAN/source/Setka_physics.cpp:1484-2278).  Everything that is
+// per face or per cell runs in HIP kernels on one stream; the host only sequences launches.  The
+// lagged time step lives on the device (StepScalars), so a titan_go call issues all its steps
+// without a host synchronisation and synchronises once at the end.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/titan_gpu.h"
+#include "device_mesh.h"
+#include "launch.h"
+#include "mlp.h"
+#include "titan_mesh.h"
+
+using namespace titan;
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            std::fprintf(stderr, "titan_gpu: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(e_), \
+                         __FILE__, __LINE__);                                                 \
+            return TITAN_E_HIP;                                                               \
+        }                                                                                     \
+    } while (0)
+
+namespace {
+
+struct Timer {
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, int>> pending;  // (family, event pair index)
+    size_t used = 0;
+    double ms[4] = {0, 0, 0, 0};
+    long cnt[4] = {0, 0, 0, 0};
+};
+
+}  // namespace
+
+struct titan_ctx {
+    hipStream_t stream = nullptr;
+    DeviceOrder ord;
+    int64_t nn = 0, nc = 0, nf = 0, nb = 0;
+    titan_params params{};
+    DevParams dp{};
+    // device topology
+    int32_t *d_cell_nodes = nullptr, *d_cell_faces = nullptr, *d_face_nodes = nullptr, *d_face_cells = nullptr;
+    int32_t *d_face_bidx = nullptr, *d_face_ref = nullptr, *d_cell_ref = nullptr;
+    uint32_t *d_cell_code = nullptr, *d_face_code = nullptr;
+    double *d_bstate = nullptr, *d_disp = nullptr;
+    double *d_X[2] = {nullptr, nullptr};
+    double *d_prim[2] = {nullptr, nullptr};
+    double *d_geom[2] = {nullptr, nullptr};  // one allocation per layer, carved into DevGeom
+    DevGeom geom[2];
+    double *d_flux = nullptr, *d_src = nullptr, *d_neutrals = nullptr;
+    StepScalars *d_S = nullptr, *d_S2 = nullptr;
+    bool started = false;
+    long long step = 0;
+    int now1 = 1, now2 = 0;
+    int last_err = 0;
+    int64_t last_err_index = -1;
+    bool timing = false;
+    Timer tm;
+    MlpState mlp;
+    DevMesh mesh() const {
+        DevMesh M;
+        M.n_nodes = nn; M.n_cells = nc; M.n_faces = nf;
+        M.cell_nodes = d_cell_nodes; M.cell_faces = d_cell_faces; M.cell_code = d_cell_code;
+        M.face_nodes = d_face_nodes; M.face_cells = d_face_cells; M.face_code = d_face_code;
+        M.face_bidx = d_face_bidx; M.bstate = d_bstate; M.face_ref = d_face_ref; M.cell_ref = d_cell_ref;
+        return M;
+    }
+};
+
+namespace {
+
+template <typename T>
+int dev_upload(T **dst, const std::vector<T> &src, hipStream_t s) {
+    HIPCHK(hipMalloc((void **)dst, std::max<size_t>(1, src.size()) * sizeof(T)));
+    if (!src.empty()) HIPCHK(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return TITAN_OK;
+}
+
+int set_err(titan_ctx *c, int code, int64_t idx) {
+    c->last_err = code;
+    c->last_err_index = idx;
+    return code;
+}
+
+void timer_mark(titan_ctx *c, int fam, bool begin) {
+    if (!c->timing) return;
+    Timer &t = c->tm;
+    if (t.used >= t.pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        t.pool.push_back(e);
+    }
+    hipEventRecord(t.pool[t.used], c->stream);
+    if (!begin) t.pending.push_back({fam, (int)t.used - 1});
+    t.used++;
+}
+
+void timer_collect(titan_ctx *c) {
+    Timer &t = c->tm;
+    for (auto &p : t.pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.pool[p.second], t.pool[p.second + 1]) == hipSuccess) {
+            t.ms[p.first] += ms;
+            t.cnt[p.first] += 1;
+        }
+    }
+    t.pending.clear();
+    t.used = 0;
+}
+
+int check_scalars(titan_ctx *c, StepScalars *dS, StepScalars *host) {
+    HIPCHK(hipMemcpyAsync(host, dS, sizeof(StepScalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (host->err_code != 0) {
+        int64_t idx = host->err_index == LLONG_MAX ? -1 : host->err_index;
+        int code = host->err_code;
+        // clear for later calls
+        int zero = 0;
+        long long big = LLONG_MAX;
+        HIPCHK(hipMemcpy(&dS->err_code, &zero, sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(&dS->err_index, &big, sizeof(long long), hipMemcpyHostToDevice));
+        return set_err(c, code, idx);
+    }
+    return TITAN_OK;
+}
+
+int init_scalars(StepScalars *d) {
+    StepScalars h;
+    std::memset(&h, 0, sizeof(h));
+    h.err_index = LLONG_MAX;
+    h.dtmin_bits[0] = h.dtmin_bits[1] = 0x7ff0000000000000ull;
+    HIPCHK(hipMemcpy(d, &h, sizeof(h), hipMemcpyHostToDevice));
+    return TITAN_OK;
+}
+
+}  // namespace
+
+extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_params *p) {
+    if (!out || !m || !p) return TITAN_E_ARG;
+    *out = nullptr;
+    if (m->n_nodes <= 0 || m->n_cells <= 0 || m->n_faces <= 0 || !m->cell_nodes || !m->face_nodes ||
+        !m->face_cells || !m->cell_faces || !m->face_bc || !m->face_method || !m->cell_zone || !m->cell_is_tvd ||
+        !m->node_xyz)
+        return TITAN_E_ARG;
+    if (m->n_cells > INT32_MAX || m->n_faces > INT32_MAX || m->n_nodes > INT32_MAX) return TITAN_E_ARG;
+    if (p->move_mesh && !m->node_disp) return TITAN_E_ARG;
+    const int64_t nn = m->n_nodes, nc = m->n_cells, nf = m->n_faces;
+    // ---- topology checks (Setka::Test_geometr, Setka.cpp:475-617) ----
+    for (int64_t f = 0; f < nf; f++) {
+        const int32_t a = m->face_cells[2 * f], b = m->face_cells[2 * f + 1];
+        if (a < 0 || a >= nc || b >= nc || a == b) return TITAN_E_TOPOLOGY;
+        if ((b < 0) != (m->face_bc[f] != TITAN_BC_INTERIOR)) return TITAN_E_TOPOLOGY;
+        if (m->face_bc[f] < 0 || m->face_bc[f] > 4 || m->face_method[f] < 0 || m->face_method[f] > 3)
+            return TITAN_E_ARG;
+        if ((m->face_bc[f] == TITAN_BC_INNER_HARD || m->face_bc[f] == TITAN_BC_OUTER_HARD) && !m->face_bstate)
+            return TITAN_E_ARG;
+        for (int i = 0; i < 4; i++)
+            if (m->face_nodes[4 * f + i] < 0 || m->face_nodes[4 * f + i] >= nn) return TITAN_E_TOPOLOGY;
+    }
+    for (int64_t c = 0; c < nc; c++) {
+        for (int j = 0; j < 6; j++) {
+            const int32_t f = m->cell_faces[6 * c + j];
+            if (f < 0 || f >= nf) return TITAN_E_TOPOLOGY;
+            if (j > 0 && f <= m->cell_faces[6 * c + j - 1]) return TITAN_E_TOPOLOGY;  // ascending, distinct
+            if (m->face_cells[2 * f] != c && m->face_cells[2 * f + 1] != c) return TITAN_E_TOPOLOGY;
+        }
+        for (int i = 0; i < 8; i++)
+            if (m->cell_nodes[8 * c + i] < 0 || m->cell_nodes[8 * c + i] >= nn) return TITAN_E_TOPOLOGY;
+    }
+
+    titan_ctx *c = new titan_ctx();
+    c->nn = nn; c->nc = nc; c->nf = nf;
+    c->params = *p;
+    c->dp.gamma = p->gamma; c->dp.kfl = p->kfl; c->dp.v_inf = p->v_inf;
+    c->dp.move_mesh = p->move_mesh; c->dp.n_disc = p->n_disc; c->dp.geom_mode = p->geom_mode;
+    c->dp.motion_omega = p->motion_omega; c->dp.motion_phase = p->motion_phase;
+    if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return TITAN_E_HIP; }
+
+    MeshView mv{nn, nc, nf, m->cell_nodes, m->face_cells, m->node_xyz};
+    build_order(mv, c->ord);
+    const DeviceOrder &o = c->ord;
+
+    std::vector<int32_t> cell_nodes(8 * nc), cell_faces(6 * nc), face_nodes(4 * nf), face_cells(2 * nf);
+    std::vector<int32_t> face_bidx(nf, -1), face_ref(nf), cell_ref(nc);
+    std::vector<uint32_t> cell_code(nc), face_code(nf);
+    std::vector<double> bstate;
+    for (int64_t d = 0; d < nc; d++) {
+        const int64_t r = o.cell_d2r[d];
+        cell_ref[d] = (int32_t)r;
+        for (int i = 0; i < 8; i++) cell_nodes[8 * d + i] = o.node_r2d[m->cell_nodes[8 * r + i]];
+        uint32_t fsign = 0;
+        for (int j = 0; j < 6; j++) {
+            const int32_t fr = m->cell_faces[6 * r + j];
+            cell_faces[6 * d + j] = o.face_r2d[fr];
+            if (m->face_cells[2 * fr] == r) fsign |= 1u << j;
+        }
+        cell_code[d] = cell_code_make(fsign, m->cell_is_tvd[r] ? 1u : 0u, (uint32_t)m->cell_zone[r]);
+    }
+    for (int64_t d = 0; d < nf; d++) {
+        const int64_t r = o.face_d2r[d];
+        face_ref[d] = (int32_t)r;
+        for (int i = 0; i < 4; i++) face_nodes[4 * d + i] = o.node_r2d[m->face_nodes[4 * r + i]];
+        face_cells[2 * d] = o.cell_r2d[m->face_cells[2 * r]];
+        const int32_t b = m->face_cells[2 * r + 1];
+        face_cells[2 * d + 1] = b >= 0 ? o.cell_r2d[b] : -1;
+        const int bc = m->face_bc[r];
+        face_code[d] = face_code_make((uint32_t)bc, (uint32_t)m->face_method[r], m->face_flags ? m->face_flags[r] : 0u);
+        if (bc == TITAN_BC_INNER_HARD || bc == TITAN_BC_OUTER_HARD) {
+            face_bidx[d] = (int32_t)(bstate.size() / 8);
+            for (int i = 0; i < 8; i++) bstate.push_back(m->face_bstate[8 * r + i]);
+        }
+    }
+    c->nb = (int64_t)bstate.size() / 8;
+    std::vector<double> X(3 * nn), disp;
+    for (int64_t d = 0; d < nn; d++)
+        for (int k = 0; k < 3; k++) X[3 * d + k] = m->node_xyz[3 * (int64_t)o.node_d2r[d] + k];
+    if (m->node_disp) {
+        disp.resize(3 * nn);
+        for (int64_t d = 0; d < nn; d++)
+            for (int k = 0; k < 3; k++) disp[3 * d + k] = m->node_disp[3 * (int64_t)o.node_d2r[d] + k];
+    }
+    int rc = TITAN_OK;
+    auto up = [&](int r) { if (rc == TITAN_OK) rc = r; };
+    up(dev_upload(&c->d_cell_nodes, cell_nodes, c->stream));
+    up(dev_upload(&c->d_cell_faces, cell_faces, c->stream));
+    up(dev_upload(&c->d_face_nodes, face_nodes, c->stream));
+    up(dev_upload(&c->d_face_cells, face_cells, c->stream));
+    up(dev_upload(&c->d_face_bidx, face_bidx, c->stream));
+    up(dev_upload(&c->d_face_ref, face_ref, c->stream));
+    up(dev_upload(&c->d_cell_ref, cell_ref, c->stream));
+    up(dev_upload(&c->d_cell_code, cell_code, c->stream));
+    up(dev_upload(&c->d_face_code, face_code, c->stream));
+    up(dev_upload(&c->d_bstate, bstate, c->stream));
+    up(dev_upload(&c->d_X[0], X, c->stream));
+    up(dev_upload(&c->d_X[1], X, c->stream));
+    if (!disp.empty()) up(dev_upload(&c->d_disp, disp, c->stream));
+    if (rc != TITAN_OK) { titan_destroy(c); return rc; }
+    const size_t geom_n = (size_t)(4 * nc + 7 * nf);
+    for (int l = 0; l < 2; l++) {
+        if (hipMalloc((void **)&c->d_prim[l], sizeof(double) * 8 * nc) != hipSuccess ||
+            hipMalloc((void **)&c->d_geom[l], sizeof(double) * geom_n) != hipSuccess) {
+            titan_destroy(c);
+            return TITAN_E_HIP;
+        }
+        double *g = c->d_geom[l];
+        c->geom[l].cell_center = g;
+        c->geom[l].cell_volume = g + 3 * nc;
+        c->geom[l].face_center = g + 4 * nc;
+        c->geom[l].face_normal = g + 4 * nc + 3 * nf;
+        c->geom[l].face_area = g + 4 * nc + 6 * nf;
+        hipMemsetAsync(c->d_prim[l], 0, sizeof(double) * 8 * nc, c->stream);
+    }
+    if (hipMalloc((void **)&c->d_flux, sizeof(double) * 9 * nf) != hipSuccess ||
+        hipMalloc((void **)&c->d_S, sizeof(StepScalars)) != hipSuccess ||
+        hipMalloc((void **)&c->d_S2, sizeof(StepScalars)) != hipSuccess) {
+        titan_destroy(c);
+        return TITAN_E_HIP;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess || init_scalars(c->d_S) != TITAN_OK ||
+        init_scalars(c->d_S2) != TITAN_OK) {
+        titan_destroy(c);
+        return TITAN_E_HIP;
+    }
+    // geometry of both layers from the initial nodes
+    launch_measure(c->stream, c->mesh(), c->d_X[0], c->geom[0], p->geom_mode);
+    launch_measure(c->stream, c->mesh(), c->d_X[1], c->geom[1], p->geom_mode);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { titan_destroy(c); return TITAN_E_HIP; }
+    *out = c;
+    return TITAN_OK;
+}
+
+extern "C" int titan_destroy(titan_ctx *c) {
+    if (!c) return TITAN_E_ARG;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void *ptrs[] = {c->d_cell_nodes, c->d_cell_faces, c->d_face_nodes, c->d_face_cells, c->d_face_bidx,
+                    c->d_face_ref, c->d_cell_ref, c->d_cell_code, c->d_face_code, c->d_bstate, c->d_disp,
+                    c->d_X[0], c->d_X[1], c->d_prim[0], c->d_prim[1], c->d_geom[0], c->d_geom[1], c->d_flux,
+                    c->d_src, c->d_neutrals, c->d_S, c->d_S2};
+    for (void *q : ptrs)
+        if (q) hipFree(q);
+    mlp_free(c->mlp);
+    for (auto e : c->tm.pool) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return TITAN_OK;
+}
+
+extern "C" int titan_last_error(titan_ctx *c, int64_t *index) {
+    if (!c) return TITAN_E_ARG;
+    if (index) *index = c->last_err_index;
+    return c->last_err;
+}
+
+extern "C" int titan_set_state(titan_ctx *c, const double *prim, int layer) {
+    if (!c || !prim || layer < 0 || layer > 1) return TITAN_E_ARG;
+    std::vector<double> h(8 * c->nc);
+    for (int64_t d = 0; d < c->nc; d++) std::memcpy(&h[8 * d], prim + 8 * (int64_t)c->ord.cell_d2r[d], 64);
+    HIPCHK(hipMemcpyAsync(c->d_prim[layer], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TITAN_OK;
+}
+
+extern "C" int titan_get_state(titan_ctx *c, double *prim, int layer) {
+    if (!c || !prim || layer < 0 || layer > 1) return TITAN_E_ARG;
+    std::vector<double> h(8 * c->nc);
+    HIPCHK(hipMemcpyAsync(h.data(), c->d_prim[layer], h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t d = 0; d < c->nc; d++) std::memcpy(prim + 8 * (int64_t)c->ord.cell_d2r[d], &h[8 * d], 64);
+    return TITAN_OK;
+}
+
+extern "C" int titan_set_nodes(titan_ctx *c, const double *xyz, int layer) {
+    if (!c || !xyz || layer < 0 || layer > 1) return TITAN_E_ARG;
+    std::vector<double> h(3 * c->nn);
+    for (int64_t d = 0; d < c->nn; d++)
+        for (int k = 0; k < 3; k++) h[3 * d + k] = xyz[3 * (int64_t)c->ord.node_d2r[d] + k];
+    HIPCHK(hipMemcpyAsync(c->d_X[layer], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TITAN_OK;
+}
+
+extern "C" int titan_get_nodes(titan_ctx *c, double *xyz, int layer) {
+    if (!c || !xyz || layer < 0 || layer > 1) return TITAN_E_ARG;
+    std::vector<double> h(3 * c->nn);
+    HIPCHK(hipMemcpyAsync(h.data(), c->d_X[layer], h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t d = 0; d < c->nn; d++)
+        for (int k = 0; k < 3; k++) xyz[3 * (int64_t)c->ord.node_d2r[d] + k] = h[3 * d + k];
+    return TITAN_OK;
+}
+
+extern "C" int titan_set_neutrals(titan_ctx *c, const double *nh5) {
+    if (!c || !nh5) return TITAN_E_ARG;
+    if (!c->d_neutrals) HIPCHK(hipMalloc((void **)&c->d_neutrals, sizeof(double) * 5 * c->nc));
+    std::vector<double> h(5 * c->nc);
+    for (int64_t d = 0; d < c->nc; d++)
+        for (int k = 0; k < 5; k++) h[5 * d + k] = nh5[5 * (int64_t)c->ord.cell_d2r[d] + k];
+    HIPCHK(hipMemcpyAsync(c->d_neutrals, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TITAN_OK;
+}
+
+extern "C" int titan_calculating_measure(titan_ctx *c, int layer) {
+    if (!c || layer < 0 || layer > 1) return TITAN_E_ARG;
+    launch_measure(c->stream, c->mesh(), c->d_X[layer], c->geom[layer], c->params.geom_mode);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TITAN_OK;
+}
+
+extern "C" int titan_get_geometry(titan_ctx *c, int layer, double *cell_center, double *cell_volume,
+                                  double *face_center, double *face_normal, double *face_area) {
+    if (!c || layer < 0 || layer > 1) return TITAN_E_ARG;
+    const int64_t nc = c->nc, nf = c->nf;
+    std::vector<double> h((size_t)(4 * nc + 7 * nf));
+    HIPCHK(hipMemcpyAsync(h.data(), c->d_geom[layer], h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double *cc = h.data(), *cv = cc + 3 * nc, *fc = cc + 4 * nc, *fn = fc + 3 * nf, *fa = fc + 6 * nf;
+    for (int64_t d = 0; d < nc; d++) {
+        const int64_t r = c->ord.cell_d2r[d];
+        if (cell_center) for (int k = 0; k < 3; k++) cell_center[3 * r + k] = cc[3 * d + k];
+        if (cell_volume) cell_volume[r] = cv[d];
+    }
+    for (int64_t d = 0; d < nf; d++) {
+        const int64_t r = c->ord.face_d2r[d];
+        if (face_center) for (int k = 0; k < 3; k++) face_center[3 * r + k] = fc[3 * d + k];
+        if (face_normal) for (int k = 0; k < 3; k++) face_normal[3 * r + k] = fn[3 * d + k];
+        if (face_area) face_area[r] = fa[d];
+    }
+    return TITAN_OK;
+}
+
+extern "C" int titan_gran_potok(titan_ctx *c, int now, double dt, double *face_flux, double *dt_min,
+                                uint8_t *branch) {
+    if (!c || now < 0 || now > 1 || !(dt > 0.0)) return TITAN_E_ARG;
+    if (init_scalars(c->d_S2) != TITAN_OK) return TITAN_E_HIP;
+    // slot (step & 1) = 1 is reset to the reference's initial 1e8
+    unsigned long long init = 0x4197D78400000000ull;  // bits of 1.0e8
+    HIPCHK(hipMemcpy(&c->d_S2->dtmin_bits[1], &init, sizeof(init), hipMemcpyHostToDevice));
+    uint8_t *d_branch = nullptr;
+    if (branch) HIPCHK(hipMalloc((void **)&d_branch, c->nf));
+    launch_face_flux(c->stream, c->params.precision, c->mesh(), c->dp, c->d_prim[now], c->geom[now], c->d_X[now],
+                     c->d_X[1 - now], c->d_S2, c->d_flux, d_branch, 1, dt);
+    HIPCHK(hipGetLastError());
+    StepScalars hs;
+    int rc = check_scalars(c, c->d_S2, &hs);
+    if (dt_min) *dt_min = __builtin_bit_cast(double, hs.dtmin_bits[1]);
+    if (face_flux) {
+        std::vector<double> h(9 * c->nf);
+        HIPCHK(hipMemcpy(h.data(), c->d_flux, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t d = 0; d < c->nf; d++) std::memcpy(face_flux + 9 * (int64_t)c->ord.face_d2r[d], &h[9 * d], 72);
+    }
+    if (branch) {
+        std::vector<uint8_t> h(c->nf);
+        HIPCHK(hipMemcpy(h.data(), d_branch, c->nf, hipMemcpyDeviceToHost));
+        for (int64_t d = 0; d < c->nf; d++) branch[c->ord.face_d2r[d]] = h[d];
+        hipFree(d_branch);
+    }
+    return rc;
+}
+
+static int compute_sources(titan_ctx *c, int layer, int mode) {
+    if (mode == TITAN_SRC_NONE) return TITAN_OK;
+    if (!c->d_neutrals) return TITAN_E_ARG;
+    if (!c->d_src) HIPCHK(hipMalloc((void **)&c->d_src, sizeof(double) * 5 * c->nc));
+    const int nodes = c->params.cx_nodes > 0 ? c->params.cx_nodes : 100;
+    if (mode == TITAN_SRC_EXACT) {
+        launch_cx_cells(c->stream, c->nc, c->d_prim[layer], c->d_neutrals, nodes, c->d_src);
+    } else if (mode == TITAN_SRC_MLP) {
+        if (!c->mlp.ready) return TITAN_E_NOT_TRAINED;
+        int rc = mlp_cell_sources(c->mlp, c->stream, c->nc, c->d_prim[layer], c->d_neutrals, nodes, c->d_src,
+                                  &c->d_S->n_fallback);
+        if (rc != TITAN_OK) return rc;
+    } else {
+        return TITAN_E_ARG;
+    }
+    HIPCHK(hipGetLastError());
+    return TITAN_OK;
+}
+
+extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
+    if (!c || steps < 0) return TITAN_E_ARG;
+    const DevMesh M = c->mesh();
+    const int64_t nn3 = 3 * c->nn;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, c->stream));
+    if (!c->started) {
+        // Setka_physics.cpp:1486-1502: copy node and cell layers 0 -> 1, measure both
+        HIPCHK(hipMemcpyAsync(c->d_X[1], c->d_X[0], sizeof(double) * nn3, hipMemcpyDeviceToDevice, c->stream));
+        launch_measure(c->stream, M, c->d_X[0], c->geom[0], c->params.geom_mode);
+        launch_measure(c->stream, M, c->d_X[1], c->geom[1], c->params.geom_mode);
+        HIPCHK(hipMemcpyAsync(c->d_prim[1], c->d_prim[0], sizeof(double) * 8 * c->nc, hipMemcpyDeviceToDevice,
+                              c->stream));
+        c->now1 = 1;
+        c->now2 = 0;
+        c->started = true;
+    }
+    for (int64_t s = 0; s < steps; s++) {
+        c->now2 = c->now1;
+        c->now1 = (c->now1 + 1) % 2;
+        c->step++;
+        const int n1 = c->now1, n2 = c->now2;
+        launch_begin_step(c->stream, c->d_S, c->step, c->dp);
+        if (c->params.move_mesh) {
+            timer_mark(c, 2, true);
+            launch_motion(c->stream, nn3, c->d_X[n1], c->d_disp, c->d_S, c->d_X[n2]);
+            timer_mark(c, 2, false);
+            timer_mark(c, 1, true);
+            launch_measure(c->stream, M, c->d_X[n2], c->geom[n2], c->params.geom_mode);
+            timer_mark(c, 1, false);
+        }
+        const double *src = nullptr;
+        if (c->params.source_mode != TITAN_SRC_NONE) {
+            timer_mark(c, 3, true);
+            int rc = compute_sources(c, n1, c->params.source_mode);
+            timer_mark(c, 3, false);
+            if (rc != TITAN_OK) return rc;
+            src = c->d_src;
+        }
+        timer_mark(c, 0, true);
+        launch_face_flux(c->stream, c->params.precision, M, c->dp, c->d_prim[n1], c->geom[n1], c->d_X[n1],
+                         c->d_X[n2], c->d_S, c->d_flux, nullptr, c->step, 0.0);
+        launch_cell_update(c->stream, M, c->d_prim[n1], c->geom[n1].cell_volume, c->geom[n2].cell_volume,
+                           c->d_flux, src, c->d_S, c->d_prim[n2], 0.0);
+        timer_mark(c, 0, false);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, c->stream));
+    StepScalars hs;
+    int rc = check_scalars(c, c->d_S, &hs);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (c->timing) timer_collect(c);
+    if (stats) {
+        stats->steps_done = c->step;
+        stats->dt_last = hs.dt;
+        stats->dt_next = __builtin_bit_cast(double, hs.dtmin_bits[c->step & 1]);
+        stats->t_sim = hs.t + (c->step > 0 ? hs.dt : 0.0);
+        stats->n_rho_floor = (int64_t)hs.n_rho_floor;
+        stats->n_p_floor = (int64_t)hs.n_p_floor;
+        stats->n_fallback = (int64_t)hs.n_fallback;
+        stats->ms_total = ms;
+    }
+    return rc;
+}
+
+extern "C" int titan_current_layer(titan_ctx *c) {
+    if (!c) return TITAN_E_ARG;
+    return c->started ? c->now2 : 0;
+}
+
+extern "C" int titan_set_timing(titan_ctx *c, int enable) {
+    if (!c) return TITAN_E_ARG;
+    c->timing = enable != 0;
+    for (int i = 0; i < 4; i++) { c->tm.ms[i] = 0; c->tm.cnt[i] = 0; }
+    return TITAN_OK;
+}
+
+extern "C" double titan_kernel_ms(titan_ctx *c, int which) {
+    if (!c || which < 0 || which > 3 || c->tm.cnt[which] == 0) return -1.0;
+    return c->tm.ms[which] / (double)c->tm.cnt[which];
+}
+
+// ---- standalone batches ------------------------------------------------------------------------
+namespace {
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) hipFree(p); }
+};
+int dup(DevBuf &b, const void *src, size_t bytes) {
+    if (!src || bytes == 0) return TITAN_OK;
+    HIPCHK(hipMalloc(&b.p, bytes));
+    HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    return TITAN_OK;
+}
+int alloc(DevBuf &b, size_t bytes) {
+    if (bytes == 0) return TITAN_OK;
+    HIPCHK(hipMalloc(&b.p, bytes));
+    return TITAN_OK;
+}
+}  // namespace
+
+extern "C" int titan_chlld_batch(int64_t n, const int8_t *n_state, const int8_t *n_disc, const uint8_t *flags,
+                                 const double *normal, const double *w, const double *qL, const double *qR, int K,
+                                 const double *kL, const double *kR, double *q, double *k, double *ds,
+                                 uint8_t *branch, int precision) {
+    if (n < 0 || K < 0 || K > 4 || !n_state || !normal || !qL || !qR || !q) return TITAN_E_ARG;
+    if (K > 0 && (!kL || !kR || !k)) return TITAN_E_ARG;
+    if (n == 0) return TITAN_OK;
+    for (int64_t i = 0; i < n; i++)
+        if (n_state[i] < 0 || n_state[i] > 3 || (n_disc && (n_disc[i] < 0 || n_disc[i] > 3))) return TITAN_E_ARG;
+    DevBuf dns, dnd, dfl, dn, dw, dqL, dqR, dkL, dkR, dq, dk, dds, dbr, derr;
+    int rc = TITAN_OK;
+    auto ck = [&](int r) { if (rc == TITAN_OK) rc = r; };
+    ck(dup(dns, n_state, n));
+    ck(dup(dnd, n_disc, n_disc ? n : 0));
+    ck(dup(dfl, flags, flags ? n : 0));
+    ck(dup(dn, normal, 24 * n));
+    ck(dup(dw, w, w ? 8 * n : 0));
+    ck(dup(dqL, qL, 64 * n));
+    ck(dup(dqR, qR, 64 * n));
+    ck(dup(dkL, kL, 8 * K * n));
+    ck(dup(dkR, kR, 8 * K * n));
+    ck(alloc(dq, 64 * n));
+    ck(alloc(dk, 8 * K * n));
+    ck(alloc(dds, 24 * n));
+    ck(alloc(dbr, n));
+    ck(alloc(derr, sizeof(int)));
+    if (rc != TITAN_OK) return rc;
+    HIPCHK(hipMemset(derr.p, 0, sizeof(int)));
+    launch_chlld_batch(nullptr, precision, n, K, (int8_t *)dns.p, (int8_t *)dnd.p, (uint8_t *)dfl.p, (double *)dn.p,
+                       (double *)dw.p, (double *)dqL.p, (double *)dqR.p, (double *)dkL.p, (double *)dkR.p,
+                       (double *)dq.p, (double *)dk.p, (double *)dds.p, (uint8_t *)dbr.p, (int *)derr.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(q, dq.p, 64 * n, hipMemcpyDeviceToHost));
+    if (K > 0) HIPCHK(hipMemcpy(k, dk.p, 8 * K * n, hipMemcpyDeviceToHost));
+    if (ds) HIPCHK(hipMemcpy(ds, dds.p, 24 * n, hipMemcpyDeviceToHost));
+    if (branch) HIPCHK(hipMemcpy(branch, dbr.p, n, hipMemcpyDeviceToHost));
+    int herr = 0;
+    HIPCHK(hipMemcpy(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost));
+    return herr ? TITAN_E_RIEMANN : TITAN_OK;
+}
+
+extern "C" int titan_cx_exact_batch(int64_t n, const double *in10, int nodes, double *out5) {
+    if (n < 0 || !in10 || !out5 || nodes < 1 || nodes > 4096) return TITAN_E_ARG;
+    if (n == 0) return TITAN_OK;
+    DevBuf din, dout;
+    int rc = dup(din, in10, 80 * n);
+    if (rc == TITAN_OK) rc = alloc(dout, 40 * n);
+    if (rc != TITAN_OK) return rc;
+    launch_cx_exact(nullptr, n, (double *)din.p, nodes, (double *)dout.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(out5, dout.p, 40 * n, hipMemcpyDeviceToHost));
+    return TITAN_OK;
+}
+
+extern "C" int titan_sources(titan_ctx *c, int layer, int mode, double *out) {
+    if (!c || layer < 0 || layer > 1 || !out) return TITAN_E_ARG;
+    int rc = compute_sources(c, layer, mode);
+    if (rc != TITAN_OK) return rc;
+    std::vector<double> h(5 * c->nc);
+    HIPCHK(hipMemcpyAsync(h.data(), c->d_src, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t d = 0; d < c->nc; d++)
+        for (int k = 0; k < 5; k++) out[5 * (int64_t)c->ord.cell_d2r[d] + k] = h[5 * d + k];
+    return TITAN_OK;
+}
+
+// ---- surrogate network entry points (kernels in k_mlp.hip) ----------------------------------
+extern "C" int64_t titan_mlp_n_params(void) { return kMlpParams; }
+
+extern "C" int titan_mlp_set(titan_ctx *c, const float *params, const float *in_norm16, const float *out_scale5,
+                             const float *box16) {
+    if (!c || !params || !in_norm16 || !out_scale5 || !box16) return TITAN_E_ARG;
+    return mlp_set(c->mlp, c->stream, params, in_norm16, out_scale5, box16);
+}
+
+extern "C" int titan_mlp_get(titan_ctx *c, float *params) {
+    if (!c || !params) return TITAN_E_ARG;
+    return mlp_get(c->mlp, c->stream, params);
+}
+
+extern "C" int titan_mlp_forward(titan_ctx *c, int64_t n, const float *x, float *y) {
+    if (!c || n < 0 || !x || !y) return TITAN_E_ARG;
+    return mlp_forward_host(c->mlp, c->stream, n, x, y);
+}
+
+extern "C" int titan_mlp_loss_grad(titan_ctx *c, int64_t n, const float *x, const float *y, float *loss,
+                                   float *grad) {
+    if (!c || n <= 0 || !x || !y || !loss || !grad) return TITAN_E_ARG;
+    return mlp_loss_grad_host(c->mlp, c->stream, n, x, y, loss, grad);
+}
+
+extern "C" int titan_mlp_train(titan_ctx *c, int64_t n, const float *x, const float *y, int epochs, int batch,
+                               float lr, uint64_t seed, float *loss_hist) {
+    if (!c || n <= 0 || !x || !y || epochs < 0 || batch <= 0) return TITAN_E_ARG;
+    return mlp_train_host(c->mlp, c->stream, n, x, y, epochs, batch, lr, seed, loss_hist);
+}

@@ -1,0 +1,28 @@
+// titan_mesh.h -- host-side mesh indexing helpers.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace titan {
+
+struct MeshView {
+    int64_t n_nodes, n_cells, n_faces;
+    const int32_t *cell_nodes;
+    const int32_t *face_cells;
+    const double *xyz;
+};
+
+// Permutations between reference numbering (r) and device numbering (d).
+struct DeviceOrder {
+    std::vector<int32_t> cell_d2r, cell_r2d, face_d2r, face_r2d, node_d2r, node_r2d;
+};
+
+int64_t connect_faces(int64_t n_cells, const int32_t *cell_nodes, int64_t n_nodes, int32_t *face_nodes,
+                      int32_t *face_cells, int32_t *cell_faces);
+int mesh_cube(int32_t n, double jitter, uint64_t seed, double *xyz, int32_t *cell_nodes);
+int mesh_shell(int32_t n_edge, int32_t n_layers, const double *radii, double *xyz, int32_t *cell_nodes);
+double uniform01(uint64_t seed, uint64_t index);
+// Cells along a Morton curve of their centres, faces by their lower device cell, nodes by first use.
+void build_order(const MeshView &m, DeviceOrder &o);
+
+}  // namespace titan
