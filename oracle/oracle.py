@@ -55,6 +55,7 @@ def lib():
             "or_measure": (None, [_P, _P, _P]),
             "or_measure_gcl": (None, [_P, _P, _P]),
             "or_det_cos": (_D, [_D]),
+            "or_face_velocities": (None, [_P, _P, _P, _P, _P, _D, _I, _P]),
             "or_motion": (None, [_I64, _P, _P, _D, _D, _D, _D, _P]),
             "or_face_flux": (_I, [_P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
             "or_update": (_I, [_P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
@@ -213,3 +214,39 @@ def mlp_loss_grad(params, x, y):
     g = np.zeros(P.size)
     loss = lib().or_mlp_loss_grad(_p(P), x.shape[0], _p(x), _p(y), _p(g))
     return loss, g
+
+
+def face_velocities(om, x1, x2, geom1, dt, gcl=False):
+    w = np.empty(om.m.n_faces)
+    x1 = np.ascontiguousarray(x1, np.float64)
+    x2 = np.ascontiguousarray(x2, np.float64)
+    lib().or_face_velocities(ctypes.byref(om.c), _p(x1), _p(x2), _p(geom1["face_normal"]), _p(geom1["face_area"]),
+                             float(dt), 1 if gcl else 0, _p(w))
+    return w
+
+
+def update(om, params, prim_old, vol_old, vol_new, flux, dt, src=None):
+    m = om.m
+    out = np.empty((m.n_cells, 8))
+    nr = ctypes.c_int64(0)
+    npf = ctypes.c_int64(0)
+    po = np.ascontiguousarray(prim_old, np.float64)
+    vo = np.ascontiguousarray(vol_old, np.float64)
+    vn = np.ascontiguousarray(vol_new, np.float64)
+    fl = np.ascontiguousarray(flux, np.float64)
+    sr = None if src is None else np.ascontiguousarray(src, np.float64)
+    rc = lib().or_update(ctypes.byref(om.c), ctypes.byref(params_c(params)), _p(po), _p(vo), _p(vn), _p(fl), _p(sr),
+                         float(dt), _p(out), ctypes.byref(nr), ctypes.byref(npf))
+    return rc, out, nr.value, npf.value
+
+
+def cx_rate_ref(U, c, a2, n1=100, n2=100):
+    return lib().or_cx_rate_ref(float(U), float(c), float(a2), int(n1), int(n2))
+
+
+def sigma_ls(g):
+    return lib().or_sigma_ls(float(g))
+
+
+def det_cos(x):
+    return lib().or_det_cos(float(x))

@@ -695,6 +695,15 @@ static double or_face_velocity_gcl(const or_mesh *m, int64_t f, const double *x1
     return (sg * V) / (dt * area1[f]);
 }
 
+/* All face velocities of a mesh between two node layers (reference or exactly conservative). */
+void or_face_velocities(const or_mesh *m, const double *x1, const double *x2, const double *normal1,
+                        const double *area1, double dt, int gcl, double *w)
+{
+    for (int64_t f = 0; f < m->n_faces; f++)
+        w[f] = gcl ? or_face_velocity_gcl(m, f, x1, x2, normal1, area1, dt)
+                   : or_face_velocity(m, f, x1, x2, normal1, dt);
+}
+
 /* Deterministic cosine (Cody-Waite reduction by pi/2 in three parts, Taylor polynomials on
  * [-pi/4, pi/4]).  Used only by the prescribed node motion so the CPU and the GPU move nodes
  * identically; accurate to a few ulp for |x| < 1e5. */

@@ -89,6 +89,8 @@ void or_measure(const or_mesh *m, const double *coords, or_geom *g);
 void or_measure_gcl(const or_mesh *m, const double *coords, or_geom *g);
 double or_face_velocity(const or_mesh *m, int64_t f, const double *x_now1, const double *x_now2,
                         const double *normal_now1, double dt);
+void or_face_velocities(const or_mesh *m, const double *x1, const double *x2, const double *normal1,
+                        const double *area1, double dt, int gcl, double *w);
 double or_det_cos(double x);
 void or_motion(int64_t n_nodes, const double *x_old, const double *disp, double dt, double t,
                double omega, double phase, double *x_new);

@@ -1,0 +1,112 @@
+"""Pins the oracle's cell update and step loop (cell loop of Setka::Go, Setka_physics.cpp:1671-2027;
+lagged dt, :1582,1606) by conservation, exact floors, the B-freeze of non-TVD cells, pressure
+recovery on a quiescent state and the reference's dt schedule."""
+import numpy as np
+
+SPI4 = np.sqrt(4 * np.pi)
+
+
+def setup(C, O, n=8, jitter=0.15):
+    mesh, prim, params = C.cube_brio_wu(n=n, jitter=jitter)
+    om = O.OracleMesh(mesh)
+    g = O.measure(om, mesh.node_xyz)
+    return mesh, prim, params, om, g
+
+
+def test_mass_conservation(C, O):
+    mesh, prim, params, om, g = setup(C, O)
+    rng = np.random.default_rng(1)
+    prim = prim.copy()
+    prim[:, 1:4] = rng.normal(0, 0.3, (mesh.n_cells, 3))
+    rc, flux, dtf, _ = O.face_flux(om, params, prim, g)
+    assert rc == 0
+    dt = 0.5 * dtf.min()
+    rc, new, nr, npf = O.update(om, params, prim, g["cell_volume"], g["cell_volume"], flux, dt)
+    assert rc == 0 and nr == 0 and npf == 0
+    V = g["cell_volume"]
+    bdry = mesh.face_cells[:, 1] < 0
+    dm = (new[:, 0] * V).sum() - (prim[:, 0] * V).sum()
+    exp = -dt * flux[bdry, 0].sum()
+    assert abs(dm - exp) <= 1e-13 * (prim[:, 0] * V).sum()
+
+
+def test_quiescent_state_preserved(C, O):
+    """Uniform state at rest with zero fluxes: the FP64 pressure recovery returns p to round-off
+    and leaves rho, V, B exactly (a wrong 8 pi or gamma-1 constant fails)."""
+    mesh, prim, params, om, g = setup(C, O)
+    u = np.array([1.3, 0.0, 0.0, 0.0, 0.7, 0.4 * SPI4, -0.2 * SPI4, 0.9 * SPI4])
+    prim = np.tile(u, (mesh.n_cells, 1))
+    flux = np.zeros((mesh.n_faces, 9))
+    rc, new, _, _ = O.update(om, params, prim, g["cell_volume"], g["cell_volume"], flux, 1e-3)
+    # (x*V)/V is x to within one rounding; p goes through E = p/(g-1) + ... and back
+    np.testing.assert_allclose(new[:, [0, 5, 6, 7]], prim[:, [0, 5, 6, 7]], rtol=2.3e-16, atol=0)
+    np.testing.assert_array_equal(new[:, 1:4], prim[:, 1:4])
+    assert np.abs(new[:, 4] / u[4] - 1).max() < 4e-15
+
+
+def test_moving_volume_ratio_and_sources(C, O):
+    """Zero fluxes, volume change V -> 2V: rho and B halve, momentum per volume halves; a source
+    S adds dt*S to the momentum density (so u changes by dt*S/rho3)."""
+    mesh, prim, params, om, g = setup(C, O)
+    V = g["cell_volume"]
+    flux = np.zeros((mesh.n_faces, 9))
+    src = np.zeros((mesh.n_cells, 5))
+    src[:, 1] = 0.25
+    dt = 1e-3
+    rc, new, _, _ = O.update(om, params, prim, V, 2 * V, flux, dt, src=src)
+    np.testing.assert_allclose(new[:, 0], prim[:, 0] / 2, rtol=1e-15)
+    np.testing.assert_allclose(new[:, 5:8], prim[:, 5:8] / 2, rtol=1e-15, atol=1e-15)
+    np.testing.assert_allclose(new[:, 1], dt * 0.25 / new[:, 0], rtol=1e-13)
+
+
+def test_floors_exact(C, O):
+    mesh, prim, params, om, g = setup(C, O, n=4, jitter=0.0)
+    V = g["cell_volume"]
+    flux = np.zeros((mesh.n_faces, 9))
+    # drain the mass of cell 0 through its first face, the pressure of cell 1 through its energy
+    c0, c1 = 0, 1
+    f0 = mesh.cell_faces[c0, 0]
+    s0 = 1.0 if mesh.face_cells[f0, 0] == c0 else -1.0
+    flux[f0, 0] = s0 * 10.0 * prim[c0, 0] * V[c0]
+    f1 = [f for f in mesh.cell_faces[c1] if f not in mesh.cell_faces[c0]][0]
+    s1 = 1.0 if mesh.face_cells[f1, 0] == c1 else -1.0
+    flux[f1, 4] = s1 * 100.0 * V[c1]
+    rc, new, nr, npf = O.update(om, params, prim, V, V, flux, 1.0)
+    assert new[c0, 0] == 0.1 and new[c0, 4] == 0.05          # rho < 1e-7 -> 0.1, p = rho/2
+    np.testing.assert_array_equal(new[c0, 1:4], prim[c0, 1:4])
+    assert new[c1, 4] == 0.000001                              # p < 1e-8 -> 1e-6
+    assert nr >= 1 and npf >= 1
+
+
+def test_b_frozen_when_not_tvd(C, O):
+    mesh, prim, params, om, g = setup(C, O)
+    mesh.cell_is_tvd[:] = 0
+    om = O.OracleMesh(mesh)
+    rc, flux, dtf, _ = O.face_flux(om, params, prim, g)
+    rc, new, _, _ = O.update(om, params, prim, g["cell_volume"], g["cell_volume"], flux, 0.5 * dtf.min())
+    np.testing.assert_array_equal(new[:, 5:8], prim[:, 5:8])
+    assert np.abs(new[:, 0] - prim[:, 0]).max() > 0
+
+
+def test_lagged_dt_schedule(C, O):
+    mesh, prim, params, om, g = setup(C, O)
+    rc, _, _, dth, _ = O.go(om, params, mesh.node_xyz, prim, 3)
+    assert rc == 0
+    assert dth[0] == 1e-5
+    rc, flux, dtf, _ = O.face_flux(om, params, prim, g)
+    assert dth[1] == min(1e8, dtf.min())
+
+
+def test_brio_wu_profile(C, O):
+    """Qualitative Brio-Wu check after 100 steps of the cube config: densities stay within the
+    initial bounds (to the 1% noise) and the discontinuity has spread into a wave fan."""
+    mesh, prim, params = C.cube_brio_wu()
+    om = O.OracleMesh(mesh)
+    rc, out, _, dth, _ = O.go(om, params, mesh.node_xyz, prim, 100, nthreads=8)
+    assert rc == 0
+    # the right-moving fast rarefaction takes the density somewhat below the right state
+    assert out[:, 0].min() > 0.05 and out[:, 0].max() < 1.02
+    assert np.all(out[:, 4] > 0)
+    cc = mesh.node_xyz[mesh.cell_nodes].mean(axis=1)
+    mid = (cc[:, 0] > 0.3) & (cc[:, 0] < 0.7)
+    assert ((out[mid, 0] > 0.2) & (out[mid, 0] < 0.9)).mean() > 0.2
