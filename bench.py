@@ -51,7 +51,7 @@ def run_variant(T, mesh, prim, params, steps, warmup, neutrals=None, mlp=None):
         st = s.Go(steps)
         t1 = time.perf_counter()
         kms = {k: s.kernel_ms(i) for i, k in enumerate(("step", "geometry", "motion", "source"))}
-        return dict(seconds=t1 - t0, stats=st, kernel_ms=kms, state=s.get_state())
+        return dict(seconds=t1 - t0, stats=st, kernel_ms=kms, state=s.get_state(), tiles=s.tile_info())
     finally:
         s.close()
 
@@ -83,7 +83,8 @@ def main():
     res = {}
     p64 = dict(params, precision=T.PREC_FP64)
     res["fp64"] = run_variant(T, mesh, prim, p64, a.steps, a.warmup)
-    print("[bench] fp64 done: %.3f s for %d steps" % (res["fp64"]["seconds"], a.steps), file=sys.stderr, flush=True)
+    print("[bench] fp64 done: %.3f s for %d steps; tiles %s" % (res["fp64"]["seconds"], a.steps, res["fp64"]["tiles"]),
+          file=sys.stderr, flush=True)
     pmx = dict(params, precision=T.PREC_MIXED)
     res["mixed"] = run_variant(T, mesh, prim, pmx, a.steps, a.warmup)
     print("[bench] mixed done: %.3f s" % res["mixed"]["seconds"], file=sys.stderr, flush=True)

@@ -190,6 +190,9 @@ int titan_mesh_shell(int32_t n_edge, int32_t n_layers, const double *radii, doub
  * 1 geometry, 2 motion, 3 source.  Returns -1 if not measured. */
 double titan_kernel_ms(titan_ctx *ctx, int which);
 int titan_set_timing(titan_ctx *ctx, int enable);
+/* Layout of the fused step: info8 = {cells per tile, max nodes per tile, max faces per tile,
+ * LDS bytes per workgroup, device LDS limit, tiles, face records (border faces counted twice), 0}. */
+int titan_tile_info(titan_ctx *ctx, int64_t *info8);
 
 #ifdef __cplusplus
 }

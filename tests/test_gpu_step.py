@@ -9,6 +9,11 @@ from helpers import rel_err
 pytestmark = pytest.mark.gpu
 
 
+def var_err(a, b):
+    """Error per conserved variable, relative to that variable's largest magnitude on the mesh."""
+    return (np.abs(a - b).max(axis=0) / np.maximum(np.abs(b).max(axis=0), 1e-300)).max()
+
+
 def small_shell(C, moving=False, n_edge=6, n_layers=10, **kw):
     return C.shell(n_edge=n_edge, n_layers=n_layers, moving=moving, r_zone1_au=80.0, **kw)
 
@@ -82,7 +87,8 @@ def test_go_shell_fixed(T, C, O):
     got, _ = _run_gpu(T, mesh, prim, params, 30)
     rc, ref, _, _, _ = O.go(O.OracleMesh(mesh), params, mesh.node_xyz, prim, 30, nthreads=8)
     assert rc == 0
-    assert rel_err(got, ref, 1e-30).max() < 1e-11
+    # Zone_1 faces use acos/atan/sin/cos/pow (GPU libm vs glibc: last-ulp differences)
+    assert var_err(got, ref) < 1e-12
 
 
 def test_go_shell_moving(T, C, O):
@@ -92,8 +98,9 @@ def test_go_shell_moving(T, C, O):
                                disp=mesh.node_disp)
     assert rc == 0
     assert np.abs(xyz - mesh.node_xyz).max() > 0.0
-    assert rel_err(xyz, rxyz, 1e-30).max() < 1e-12
-    assert rel_err(got, ref, 1e-30).max() < 1e-11
+    # the lagged dt inherits last-ulp differences of the Zone_1 faces, and the nodes move by dt
+    assert var_err(xyz, rxyz) < 1e-14
+    assert var_err(got, ref) < 1e-12
 
 
 def test_go_cube_moving_bit_exact(T, C, O):

@@ -1,0 +1,129 @@
+"""The learned charge-exchange source: exact FP64 quadrature kernel vs the oracle, BF16-MFMA
+forward pass and FP32 gradients vs the oracle's FP64 network, the out-of-range fallback, device
+training, and what the trained network gains over the step without a source."""
+import importlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_in10(rng, n):
+    a = np.zeros((n, 10))
+    a[:, 0] = np.exp(rng.uniform(-3, 3, n))
+    a[:, 1] = np.exp(rng.uniform(-3, 2, n))
+    a[:, 2:5] = rng.normal(0, 5, (n, 3))
+    a[:, 5] = np.exp(rng.uniform(-1, 1.5, n))
+    a[:, 6] = np.exp(rng.uniform(-1, 1.5, n))
+    a[:, 7:10] = rng.normal(0, 2, (n, 3))
+    return a
+
+
+def test_cx_exact_kernel_vs_oracle(T, O):
+    rng = np.random.default_rng(0)
+    a = rand_in10(rng, 64)
+    got = T.cx_exact_batch(a, nodes=100)
+    for i in range(a.shape[0]):
+        ref = O.cx_source(a[i, 0], a[i, 1], a[i, 2:5], a[i, 5], a[i, 6], a[i, 7:10], nodes=100)
+        scale = np.abs(ref).max()
+        assert np.abs(got[i] - ref).max() <= 1e-12 * scale, (i, got[i], ref)
+
+
+def _setka(T, C):
+    mesh, prim, params = C.cube_brio_wu(n=8)
+    s = T.Setka(mesh, params)
+    s.set_state(prim, 0)
+    return s, mesh, prim
+
+
+def _bf16(a):
+    a = np.ascontiguousarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def test_mlp_forward_mfma_vs_oracle(T, C, O):
+    sur = importlib.import_module("titan_c-_amd.surrogate")
+    rng = np.random.default_rng(1)
+    P = sur.init_params(3)
+    P[-5:] = rng.normal(0, 0.1, 5)
+    s, _, _ = _setka(T, C)
+    with s:
+        s.mlp_set(P, np.concatenate([np.zeros(8), np.ones(8)]), np.ones(5), np.concatenate([-1e9 * np.ones(8), 1e9 * np.ones(8)]))
+        for n in (1, 37, 64, 1000, 4097):
+            x = rng.normal(0, 1, (n, 8)).astype(np.float32)
+            y = s.mlp_forward(x)
+            # oracle on BF16-rounded weights and inputs (the MFMA operands); activations are also
+            # rounded to BF16 between layers on the device
+            ref = O.mlp_forward(_bf16(P), _bf16(x))
+            err = np.abs(y - ref).max(axis=1) / (np.abs(ref).max(axis=1) + 1e-3)
+            assert err.max() < 6e-2, (n, err.max())
+            assert np.median(err) < 1e-2
+
+
+def test_mlp_gradients_vs_oracle(T, C, O):
+    sur = importlib.import_module("titan_c-_amd.surrogate")
+    rng = np.random.default_rng(2)
+    P = sur.init_params(4)
+    x = rng.normal(0, 1, (3000, 8)).astype(np.float32)
+    yt = rng.normal(0, 1, (3000, 5)).astype(np.float32)
+    s, _, _ = _setka(T, C)
+    with s:
+        s.mlp_set(P, np.concatenate([np.zeros(8), np.ones(8)]), np.ones(5), np.concatenate([-np.ones(8), np.ones(8)]))
+        loss, g = s.mlp_loss_grad(x, yt)
+    rl, rg = O.mlp_loss_grad(P.astype(np.float64), x.astype(np.float64), yt.astype(np.float64))
+    assert abs(loss - rl) < 1e-5 * rl
+    assert np.abs(g - rg).max() < 1e-4 * np.abs(rg).max()
+
+
+def test_fallback_is_exact(T, C):
+    """With an empty training box every cell takes the exact FP64 source."""
+    sur = importlib.import_module("titan_c-_amd.surrogate")
+    mesh, prim, params = C.shell(n_edge=4, n_layers=6)
+    nh = C.neutrals(mesh.n_cells)
+    with T.Setka(mesh, params) as s:
+        s.set_state(prim, 0)
+        s.set_neutrals(nh)
+        s.mlp_set(sur.init_params(0), np.concatenate([np.zeros(8), np.ones(8)]), np.ones(5),
+                  np.concatenate([np.ones(8), -np.ones(8)]))
+        exact = s.sources(0, T.SRC_EXACT)
+        mlp = s.sources(0, T.SRC_MLP)
+    np.testing.assert_array_equal(mlp, exact)
+
+
+def test_train_and_gain(T, C):
+    """Trains the surrogate on the device (labels from the exact kernel) and checks: the loss
+    falls, the surrogate's source is close to the exact one, and after 20 steps the state with
+    the surrogate is much closer to the exact-source run than the run without a source."""
+    sur = importlib.import_module("titan_c-_amd.surrogate")
+    mesh, prim, params = C.shell(n_edge=6, n_layers=12)
+    nh = C.neutrals(mesh.n_cells)
+    in10 = sur.sample_states(prim, nh, 120_000, seed=7)
+    x, y, _ = sur.make_dataset(in10, nodes=64)
+    norm = sur.Normaliser(x, y)
+    runs = {}
+    with T.Setka(mesh, dict(params, cx_nodes=64)) as s:
+        hist = sur.train(s, x, y, norm, epochs=(14, 8, 4), lrs=(2e-3, 5e-4, 1e-4), batch=1024)
+        assert hist[-1] < 0.1 * hist[0], hist
+        s.set_state(prim, 0)
+        s.set_neutrals(nh)
+        ex = s.sources(0, T.SRC_EXACT)
+        ml = s.sources(0, T.SRC_MLP)
+        P = s.mlp_get()
+    rel = np.abs(ml - ex).max(axis=0) / np.abs(ex).max(axis=0)
+    assert rel.max() < 0.1, rel
+    in_norm, scale, box = norm.blocks()
+    for mode in (T.SRC_NONE, T.SRC_EXACT, T.SRC_MLP):
+        with T.Setka(mesh, dict(params, source_mode=mode, cx_nodes=64)) as s:
+            s.set_state(prim, 0)
+            s.set_neutrals(nh)
+            s.mlp_set(P, in_norm, scale, box)
+            s.Go(20)
+            runs[mode] = s.get_state()
+    ref = runs[T.SRC_EXACT]
+    e_none = np.abs(runs[T.SRC_NONE] - ref)[:, 1:4].max()
+    e_mlp = np.abs(runs[T.SRC_MLP] - ref)[:, 1:4].max()
+    assert e_none > 0
+    assert e_mlp < 0.2 * e_none, (e_mlp, e_none)

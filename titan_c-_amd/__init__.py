@@ -96,6 +96,7 @@ _SIGS = {
     "titan_mesh_shell": (_I, [ctypes.c_int32, ctypes.c_int32, _P, _P, _P]),
     "titan_kernel_ms": (_D, [_P, _I]),
     "titan_set_timing": (_I, [_P, _I]),
+    "titan_tile_info": (_I, [_P, _P]),
 }
 EXPORTS = tuple(_SIGS)
 
@@ -314,6 +315,12 @@ class Setka:
 
     def kernel_ms(self, which):
         return lib().titan_kernel_ms(self.ctx, which)
+
+    def tile_info(self):
+        a = np.zeros(8, np.int64)
+        _check(lib().titan_tile_info(self.ctx, _ptr(a)), "titan_tile_info", self.ctx)
+        keys = ("tile_cells", "max_nodes", "max_faces", "lds_bytes", "lds_limit", "tiles", "face_records")
+        return {k: int(v) for k, v in zip(keys, a)}
 
     # ---- surrogate ----
     def mlp_set(self, params, in_norm16, out_scale5, box16):

@@ -8,6 +8,7 @@
 //   bstate [Nb*8] f64, node xyz [2][Nn*3] f64, prim [2][Nc*8] f64 (cell-major, 64 B per cell),
 //   geometry per layer: cell centre [Nc*3], volume [Nc], face centre/normal [Nf*3], area [Nf].
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace titan {
@@ -62,5 +63,32 @@ struct StepScalars {
     int pad1;
     long long err_index;             // smallest reference id that failed, or LLONG_MAX
 };
+
+}  // namespace titan
+
+namespace titan {
+
+// Tiles of the fused step kernel: TILE consecutive device cells, the nodes they use (local
+// indices < 4096), and every face touching them (faces on a tile border appear in both tiles).
+// A face record is 16 bytes: x = 4 local node ids (12 bits each) | face code << 48; y = cells[0]
+// (device id, low 32 bits) | cells[1] << 32 (device id, or -(boundary-state index + 2)).
+struct TileMesh {
+    int64_t n_tiles;
+    int32_t tile_cells, max_nodes, max_faces, pad;
+    const int32_t *node_off;    // [n_tiles+1] into nodes
+    const int32_t *nodes;       // device node ids of each tile, ascending
+    const int32_t *own;         // [n_tiles*2] device node range whose X2 this tile writes
+    const int32_t *face_off;    // [n_tiles+1] into faces
+    const ulonglong2 *faces;    // face records (see above)
+    const int32_t *face_ref;    // reference face id of each record (error reports only)
+    const uint16_t *cell_lnodes;  // [Nc*8] tile-local node index of each cell node
+    const uint16_t *cell_lfaces;  // [Nc*6] tile-local face slot of each cell face (reference order)
+};
+
+__host__ __device__ inline unsigned long long tile_face_pack(const int ln[4], uint32_t code) {
+    return (unsigned long long)(ln[0] & 0xfff) | ((unsigned long long)(ln[1] & 0xfff) << 12) |
+           ((unsigned long long)(ln[2] & 0xfff) << 24) | ((unsigned long long)(ln[3] & 0xfff) << 36) |
+           ((unsigned long long)(code & 0xffff) << 48);
+}
 
 }  // namespace titan

@@ -2,6 +2,7 @@
 // and square root), FP64 states, geometry, area scaling and CFL step.
 #define TITAN_FLUX_ONLY 1
 #include "step_kernels.h"
+#include "tile_kernel.h"
 #include "launch.h"
 
 namespace titan {
@@ -13,6 +14,22 @@ void launch_face_flux_f32(hipStream_t s, const DevMesh &M, const DevParams &P, c
                           uint8_t *branch, long long step, double dt_override) {
     hipLaunchKernelGGL(k_face_flux<float>, dim3(grid_of32(M.n_faces, 256)), dim3(256), 0, s, M, P, prim, G, X1, X2,
                        S, flux, branch, step, dt_override);
+}
+
+void launch_step_tile_f32(hipStream_t s, const TileMesh &TM, const DevMesh &M, const DevParams &P,
+                          const double *prim1, double *prim2, const double *X1, double *X2, const double *disp,
+                          const double *cc1, double *cc2, const double *src, StepScalars *S, long long step) {
+    const size_t lds = tile_lds_bytes(TM.max_nodes, TM.tile_cells, TM.max_faces, P.move_mesh != 0);
+    static size_t attr_bytes = 0;  // dynamic LDS size the kernel has been opened up to
+    if (lds > attr_bytes) {
+        if (hipFuncSetAttribute((const void *)k_step_tile<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) ==
+            hipSuccess)
+            attr_bytes = lds;
+        else
+            (void)hipGetLastError();  // the launch below then reports the failure
+    }
+    hipLaunchKernelGGL(k_step_tile<float>, dim3((unsigned)TM.n_tiles), dim3(kTileThreads), lds, s, TM, M, P, prim1,
+                       prim2, X1, X2, disp, cc1, cc2, src, S, step);
 }
 
 template <int K>

@@ -20,6 +20,15 @@ void launch_cell_update(hipStream_t s, const DevMesh &M, const double *prim_old,
                         const double *vol_new, const double *flux, const double *src, StepScalars *S,
                         double *prim_new, double dt_override);
 
+// fused step over tiles (tile_kernel.h); precision = TITAN_PREC_*
+void launch_step_tile(hipStream_t s, int precision, const TileMesh &TM, const DevMesh &M, const DevParams &P,
+                      const double *prim1, double *prim2, const double *X1, double *X2, const double *disp,
+                      const double *cc1, double *cc2, const double *src, StepScalars *S, long long step);
+void launch_step_tile_f32(hipStream_t s, const TileMesh &TM, const DevMesh &M, const DevParams &P,
+                          const double *prim1, double *prim2, const double *X1, double *X2, const double *disp,
+                          const double *cc1, double *cc2, const double *src, StepScalars *S, long long step);
+size_t step_tile_lds_bytes(const TileMesh &TM, bool moving);
+
 // chlld over n independent states; err (device int) set to 1 on a fatal branch
 void launch_chlld_batch(hipStream_t s, int precision, int64_t n, int K, const int8_t *n_state,
                         const int8_t *n_disc, const uint8_t *flags, const double *normal, const double *w,

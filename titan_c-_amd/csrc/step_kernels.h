@@ -37,6 +37,28 @@ __device__ __forceinline__ double det_cos(double x) {
     }
 }
 
+__device__ __forceinline__ double tri_area(const double *a, const double *b, const double *c) {
+    // triangleArea3D, Help_funk.cpp:193-217
+    double v1x = b[0] - a[0], v1y = b[1] - a[1], v1z = b[2] - a[2];
+    double v2x = c[0] - a[0], v2y = c[1] - a[1], v2z = c[2] - a[2];
+    double cx = v1y * v2z - v1z * v2y;
+    double cy = v1z * v2x - v1x * v2z;
+    double cz = v1x * v2y - v1y * v2x;
+    return 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+}
+
+__device__ __forceinline__ double tet_volume(const double *A, const double *B, const double *C, const double *D) {
+    // tetrahedronVolume, Help_funk.cpp:253-272
+    double abx = B[0] - A[0], aby = B[1] - A[1], abz = B[2] - A[2];
+    double acx = C[0] - A[0], acy = C[1] - A[1], acz = C[2] - A[2];
+    double adx = D[0] - A[0], ady = D[1] - A[1], adz = D[2] - A[2];
+    double cx = aby * acz - abz * acy;
+    double cy = abz * acx - abx * acz;
+    double cz = abx * acy - aby * acx;
+    double d = cx * adx + cy * ady + cz * adz;
+    return fabs(d) / 6.0;
+}
+
 #ifndef TITAN_FLUX_ONLY
 // ---- step bookkeeping (one thread) --------------------------------------------------------
 // step n (1-based, counted over the context's life): dt = 1e-5 for n == 1, else the min over
@@ -81,15 +103,6 @@ __global__ void k_cell_center(DevMesh M, const double *__restrict__ X, DevGeom G
     G.cell_center[3 * c + 2] = zc / 8;
 }
 
-__device__ __forceinline__ double tri_area(const double *a, const double *b, const double *c) {
-    // triangleArea3D, Help_funk.cpp:193-217
-    double v1x = b[0] - a[0], v1y = b[1] - a[1], v1z = b[2] - a[2];
-    double v2x = c[0] - a[0], v2y = c[1] - a[1], v2z = c[2] - a[2];
-    double cx = v1y * v2z - v1z * v2y;
-    double cy = v1z * v2x - v1x * v2z;
-    double cz = v1x * v2y - v1y * v2x;
-    return 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
-}
 
 // Gran::Culc_measure (Gran.cpp:112-218), or the exactly conservative variant (gcl != 0).
 __global__ void k_face_geom(DevMesh M, const double *__restrict__ X, DevGeom G, int gcl) {
@@ -141,17 +154,6 @@ __global__ void k_face_geom(DevMesh M, const double *__restrict__ X, DevGeom G, 
     G.face_area[f] = A;
 }
 
-__device__ __forceinline__ double tet_volume(const double *A, const double *B, const double *C, const double *D) {
-    // tetrahedronVolume, Help_funk.cpp:253-272
-    double abx = B[0] - A[0], aby = B[1] - A[1], abz = B[2] - A[2];
-    double acx = C[0] - A[0], acy = C[1] - A[1], acz = C[2] - A[2];
-    double adx = D[0] - A[0], ady = D[1] - A[1], adz = D[2] - A[2];
-    double cx = aby * acz - abz * acy;
-    double cy = abz * acx - abx * acz;
-    double cz = abx * acy - aby * acx;
-    double d = cx * adx + cy * ady + cz * adz;
-    return fabs(d) / 6.0;
-}
 
 // Cell::Culc_volume method 1 (Cell.cpp:958-1000), or 1/3 sum s_f xbar.n A (gcl).
 __global__ void k_cell_volume(DevMesh M, const double *__restrict__ X, DevGeom G, int gcl) {
@@ -443,36 +445,15 @@ __global__ void __launch_bounds__(256) k_face_flux(DevMesh M, DevParams P, const
     block_min_to(cand, &S->dtmin_bits[step & 1]);
 }
 
-#ifndef TITAN_FLUX_ONLY
 // ---- cell update: cell loop of Setka::Go (Setka_physics.cpp:1695-2027), FP64 -------------------
-// src (nullable) [Nc*5]: rate, S_m (3), S_E per unit volume and time.
-__global__ void __launch_bounds__(256) k_cell_update(DevMesh M, const double *__restrict__ prim_old,
-                                                     const double *__restrict__ vol_old,
-                                                     const double *__restrict__ vol_new,
-                                                     const double *__restrict__ flux,
-                                                     const double *__restrict__ src, StepScalars *S,
-                                                     double *__restrict__ prim_new, double dt_override) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= M.n_cells) return;
+// Pf = signed flux sum of the cell (8 fluxes x area, PdivB); src (nullable) [Nc*5]: rate, S_m (3),
+// S_E per unit volume and time.
+__device__ __forceinline__ void cell_update_one(const DevMesh &M, uint32_t code, int64_t c,
+                                                const double *__restrict__ prim_old, double Volume, double Volume2,
+                                                const double Pf[9], const double *__restrict__ src, double time,
+                                                StepScalars *S, double *__restrict__ prim_new) {
     const double cpi4 = 4.0 * kPi;
     const double g1 = 5.0 / 3.0 - 1.0;
-    const double time = (dt_override > 0.0) ? dt_override : S->dt;
-    const uint32_t code = M.cell_code[c];
-    double Pf[9];
-#pragma unroll
-    for (int i = 0; i < 9; i++) Pf[i] = 0.0;
-#pragma unroll
-    for (int j = 0; j < 6; j++) {
-        const int64_t f = M.cell_faces[6 * c + j];
-        const bool pos = (code >> j) & 1u;
-        const double *fl = flux + 9 * f;
-#pragma unroll
-        for (int i = 0; i < 9; i++) {
-            const double v = fl[i];
-            Pf[i] += pos ? v : -v;
-        }
-    }
-    const double Volume = vol_old[c], Volume2 = vol_new[c];
     double s[8];
     load_state(prim_old, c, s);
     const double rho = s[0], vx = s[1], vy = s[2], vz = s[3], pp = s[4], bx = s[5], by = s[6], bz = s[7];
@@ -516,6 +497,33 @@ __global__ void __launch_bounds__(256) k_cell_update(DevMesh M, const double *__
     o[1] = make_double4(p3, bx3, by3, bz3);
 }
 
+#ifndef TITAN_FLUX_ONLY
+__global__ void __launch_bounds__(256) k_cell_update(DevMesh M, const double *__restrict__ prim_old,
+                                                     const double *__restrict__ vol_old,
+                                                     const double *__restrict__ vol_new,
+                                                     const double *__restrict__ flux,
+                                                     const double *__restrict__ src, StepScalars *S,
+                                                     double *__restrict__ prim_new, double dt_override) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= M.n_cells) return;
+    const double time = (dt_override > 0.0) ? dt_override : S->dt;
+    const uint32_t code = M.cell_code[c];
+    double Pf[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) Pf[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        const int64_t f = M.cell_faces[6 * c + j];
+        const bool pos = (code >> j) & 1u;
+        const double *fl = flux + 9 * f;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            const double v = fl[i];
+            Pf[i] += pos ? v : -v;
+        }
+    }
+    cell_update_one(M, code, c, prim_old, vol_old[c], vol_new[c], Pf, src, time, S, prim_new);
+}
 #endif  // TITAN_FLUX_ONLY
 
 }  // namespace titan

@@ -1,0 +1,468 @@
+// tile_kernel.h -- the fused explicit step: one workgroup per tile of TILE cells does node motion,
+// face geometry and velocity, face states, Riemann fluxes (kept in LDS), and the cell update with
+// the flux sum in the reference's order.  Included by k_step64.hip (RT = double, parity) and
+// k_step32.hip (RT = float: Riemann arithmetic and face-state rotations in FP32).
+//
+// Work per step (Setka::Go, Setka_physics.cpp:1589-2173):
+//   phase 1  tile nodes -> LDS (layer now1), new positions x + dt*omega*cos(omega t + phase)*D
+//            -> LDS (layer now2) and HBM for the nodes the tile owns;
+//   phase 2  own cells: centres at both layers (Cell::Culc_center), Zone_1 spherical frames;
+//   phase 3  tile faces: Gran::Culc_measure at now1, Gran::culc_velosity, Snos_on_Gran,
+//            Phys_param::chlld, dt candidate (Setka_physics.cpp:2428-2438) -> LDS flux slots;
+//   phase 4  own cells: volumes at both layers (Cell::Culc_volume), signed flux sum in ascending
+//            reference face order, update (Setka_physics.cpp:1695-2027) -> prim at now2;
+//   then a wavefront-shuffle min of the dt candidates and one atomic per workgroup.
+// Faces on a tile border are solved by both neighbouring tiles (same inputs, same result); no
+// flux ever leaves the workgroup and no float atomics are used, so runs repeat bit for bit.
+#pragma once
+#include "step_kernels.h"
+
+namespace titan {
+
+constexpr int kTileThreads = 256;
+
+template <typename Z> struct Trig { Z st, ct, sp, cp; };
+
+template <typename Z>
+__device__ __forceinline__ Z polar_angle_z(Z x, Z y) {
+    const Z pi = Z(kPi);
+    if (fabs(x) + fabs(y) < Z(0.000001)) return Z(0);
+    if (x < 0) return atan(y / x) + Z(1.0) * pi;
+    else if (x > 0 && y >= 0) return atan(y / x);
+    else if (x > 0 && y < 0) return atan(y / x) + Z(2.0) * pi;
+    else if (y > 0 && x >= 0 && x <= 0) return pi / Z(2.0);
+    else if (y < 0 && x >= 0 && x <= 0) return Z(3.0) * pi / Z(2.0);
+    return Z(0);
+}
+
+// (theta, phi) trig of spherical_skorost / dekard_skorost (Help_funk.cpp:108-135) for the axis
+// roles (z, x, y); shared between the V and B rotations (same values the reference recomputes).
+template <typename Z>
+__device__ __forceinline__ Trig<Z> frame_trig(Z z, Z x, Z y) {
+    const Z r = sqrt(x * x + y * y + z * z);
+    const Z the = acos(z / r);
+    const Z phi = polar_angle_z<Z>(x, y);
+    return Trig<Z>{sin(the), cos(the), sin(phi), cos(phi)};
+}
+
+// Zone_1 cell data: r1, r1^(2 gamma), trig of the equatorial (z,x,y) = (c2,c0,c1) and polar
+// (z,x,y) = (c0,c1,c2) frames.
+template <typename Z> struct Zone1Cell { Z r1, pr1; Trig<Z> e, p; };
+
+template <typename Z>
+__device__ __forceinline__ Zone1Cell<Z> zone1_cell(const double c[3], double gamma) {
+    const Z c0 = (Z)c[0], c1 = (Z)c[1], c2 = (Z)c[2];
+    Zone1Cell<Z> z;
+    z.r1 = sqrt((c0 * c0 + c1 * c1) + c2 * c2);
+    z.pr1 = pow(z.r1, Z(2.0) * (Z)gamma);
+    z.e = frame_trig<Z>(c2, c0, c1);
+    z.p = frame_trig<Z>(c0, c1, c2);
+    return z;
+}
+
+// Face-centre data of a face touching a Zone_1 cell: |G|, |G|^(2 gamma), frame choice and trig.
+template <typename Z> struct Zone1Face { Z rr, prr; bool equatorial; Trig<Z> t; };
+
+template <typename Z>
+__device__ __forceinline__ Zone1Face<Z> zone1_face(const double G[3], double gamma) {
+    const Z g0 = (Z)G[0], g1 = (Z)G[1], g2 = (Z)G[2];
+    Zone1Face<Z> f;
+    f.rr = sqrt((g0 * g0 + g1 * g1) + g2 * g2);
+    f.prr = pow(f.rr, Z(2.0) * (Z)gamma);
+    const Z the1 = acos(g2 / f.rr);
+    const Z pi = Z(kPi);
+    f.equatorial = (the1 > pi / Z(9) && the1 < Z(8) * pi / Z(9));
+    if (f.equatorial) {
+        // dekard_skorost(G2, G0, G1): r_2 = |G| in the same summation order, the_2 = the1
+        const Z phi = polar_angle_z<Z>(g0, g1);
+        f.t = Trig<Z>{sin(the1), cos(the1), sin(phi), cos(phi)};
+    } else {
+        f.t = frame_trig<Z>(g0, g1, g2);
+    }
+    return f;
+}
+
+// Setka_gran_snos.cpp:744-873 with precomputed trig: rotate V and B of a cell state to the face
+// centre (B radial x (r1/rr)^2, tangential x r1/rr), rho x r1^2/rr^2, p x (r1/rr)^(2 gamma).
+template <typename Z>
+__device__ __forceinline__ void zone1_apply(const Z s[8], const Zone1Cell<Z> &zc, const Zone1Face<Z> &zf, Z out[8]) {
+    const Trig<Z> a = zf.equatorial ? zc.e : zc.p;
+    const Trig<Z> b = zf.t;
+#pragma unroll
+    for (int ik = 0; ik < 2; ik++) {
+        const Z *V = s + (ik == 0 ? 1 : 5);
+        Z *O = out + (ik == 0 ? 1 : 5);
+        // axis roles (z, x, y)
+        const Z Vz = zf.equatorial ? V[2] : V[0];
+        const Z Vx = zf.equatorial ? V[0] : V[1];
+        const Z Vy = zf.equatorial ? V[1] : V[2];
+        Z Vr = Vx * a.st * a.cp + Vy * a.st * a.sp + Vz * a.ct;
+        Z Vtheta = Vx * a.ct * a.cp + Vy * a.ct * a.sp - Vz * a.st;
+        Z Vphi = -Vx * a.sp + Vy * a.cp;
+        if (ik == 1) {
+            Vr = Vr * ((zc.r1 / zf.rr) * (zc.r1 / zf.rr));
+            Vphi = Vphi * ((zc.r1) / (zf.rr));
+            Vtheta = Vtheta * ((zc.r1) / (zf.rr));
+        }
+        const Z ox = Vr * b.st * b.cp + Vtheta * b.ct * b.cp - Vphi * b.sp;
+        const Z oy = Vr * b.st * b.sp + Vtheta * b.ct * b.sp + Vphi * b.cp;
+        const Z oz = Vr * b.ct - Vtheta * b.st;
+        if (zf.equatorial) { O[2] = oz; O[0] = ox; O[1] = oy; }
+        else { O[0] = oz; O[1] = ox; O[2] = oy; }
+    }
+    out[0] = s[0] * (zc.r1 * zc.r1) / (zf.rr * zf.rr);
+    out[4] = s[4] * zc.pr1 / zf.prr;
+}
+
+// Face geometry at one node layer (Gran::Culc_measure, Gran.cpp:112-218; or the exactly
+// conservative variant); orientation from the centre of cells[0].
+__device__ __forceinline__ void face_geometry(const double x[4][3], const double ca[3], int gcl, double n[3],
+                                              double &A, double G[3]) {
+    double xc = 0.0, yc = 0.0, zc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) { xc += x[i][0]; yc += x[i][1]; zc += x[i][2]; }
+    xc /= 4; yc /= 4; zc /= 4;
+    double n1, n2, n3;
+    if (!gcl) {
+        double S = 0.0;
+        S += tri_area(x[0], x[1], x[2]);
+        S += tri_area(x[0], x[3], x[2]);
+        A = S;
+        const double x1 = (x[0][0] + x[1][0]) / 2.0, y1 = (x[0][1] + x[1][1]) / 2.0, z1 = (x[0][2] + x[1][2]) / 2.0;
+        const double x2 = (x[1][0] + x[2][0]) / 2.0, y2 = (x[1][1] + x[2][1]) / 2.0, z2 = (x[1][2] + x[2][2]) / 2.0;
+        const double x3 = (x[2][0] + x[3][0]) / 2.0, y3 = (x[2][1] + x[3][1]) / 2.0, z3 = (x[2][2] + x[3][2]) / 2.0;
+        const double ax = x2 - x1, ay = y2 - y1, az = z2 - z1;
+        const double bx = x3 - x1, by = y3 - y1, bz = z3 - z1;
+        n1 = ay * bz - az * by;
+        n2 = az * bx - ax * bz;
+        n3 = ax * by - ay * bx;
+        const double nn = sqrt(n1 * n1 + n2 * n2 + n3 * n3);
+        n1 /= nn; n2 /= nn; n3 /= nn;
+    } else {
+        const double ax = x[2][0] - x[0][0], ay = x[2][1] - x[0][1], az = x[2][2] - x[0][2];
+        const double bx = x[3][0] - x[1][0], by = x[3][1] - x[1][1], bz = x[3][2] - x[1][2];
+        const double s1 = 0.5 * (ay * bz - az * by);
+        const double s2 = 0.5 * (az * bx - ax * bz);
+        const double s3 = 0.5 * (ax * by - ay * bx);
+        A = sqrt(s1 * s1 + s2 * s2 + s3 * s3);
+        n1 = s1 / A; n2 = s2 / A; n3 = s3 / A;
+    }
+    if (spf(n1, n2, n3, ca[0] - xc, ca[1] - yc, ca[2] - zc) > 0.0) { n1 = -n1; n2 = -n2; n3 = -n3; }
+    n[0] = n1; n[1] = n2; n[2] = n3;
+    G[0] = xc; G[1] = yc; G[2] = zc;
+}
+
+// Swept volume of a moving face / (dt A): the exactly conservative face velocity.
+__device__ __forceinline__ double swept_velocity(const double x1[4][3], const double x2[4][3], const double n[3],
+                                                 double area, double dt) {
+    double P[8][3];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) { P[i][d] = x1[i][d]; P[4 + i][d] = x2[i][d]; }
+    const int lf[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4}, {2, 3, 7, 6}, {1, 2, 6, 5}, {0, 4, 7, 3}};
+    double V = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        const double *a0 = P[lf[j][0]], *a1 = P[lf[j][1]], *a2 = P[lf[j][2]], *a3 = P[lf[j][3]];
+        const double ax = a2[0] - a0[0], ay = a2[1] - a0[1], az = a2[2] - a0[2];
+        const double bx = a3[0] - a1[0], by = a3[1] - a1[1], bz = a3[2] - a1[2];
+        const double s1 = 0.5 * (ay * bz - az * by);
+        const double s2 = 0.5 * (az * bx - ax * bz);
+        const double s3 = 0.5 * (ax * by - ay * bx);
+        const double xc = (((a0[0] + a1[0]) + a2[0]) + a3[0]) / 4;
+        const double yc = (((a0[1] + a1[1]) + a2[1]) + a3[1]) / 4;
+        const double zc = (((a0[2] + a1[2]) + a2[2]) + a3[2]) / 4;
+        V += spf(xc, yc, zc, s1, s2, s3);
+    }
+    V = V / 3.0;
+    const double ax = P[2][0] - P[0][0], ay = P[2][1] - P[0][1], az = P[2][2] - P[0][2];
+    const double bx = P[3][0] - P[1][0], by = P[3][1] - P[1][1], bz = P[3][2] - P[1][2];
+    const double s1 = ay * bz - az * by, s2 = az * bx - ax * bz, s3 = ax * by - ay * bx;
+    const double sg = (spf(s1, s2, s3, n[0], n[1], n[2]) >= 0.0) ? 1.0 : -1.0;
+    return (sg * V) / (dt * area);
+}
+
+struct TileLds {
+    double *x1, *x2;   // [max_nodes*3]
+    double *c1, *c2;   // [T*3]
+    double *z1;        // [T*10] Zone_1 cell data (r1, r1^2g, trig e, trig p)
+    double *flux;      // [max_faces*9]
+    uint16_t *fnodes;  // [max_faces*4] local nodes of each face slot
+};
+
+// LDS carve-up; the second node layer and second centres exist only on a moving mesh.
+__device__ __forceinline__ TileLds tile_lds(double *base, const TileMesh &TM, bool moving) {
+    TileLds L;
+    double *p = base;
+    L.x1 = p; p += 3 * TM.max_nodes;
+    L.c1 = p; p += 3 * TM.tile_cells;
+    L.z1 = p; p += 10 * TM.tile_cells;
+    L.flux = p; p += 9 * TM.max_faces;
+    if (moving) {
+        L.x2 = p; p += 3 * TM.max_nodes;
+        L.c2 = p; p += 3 * TM.tile_cells;
+    } else {
+        L.x2 = L.x1;
+        L.c2 = L.c1;
+    }
+    L.fnodes = reinterpret_cast<uint16_t *>(p);
+    return L;
+}
+
+inline size_t tile_lds_bytes(int max_nodes, int tile_cells, int max_faces, bool moving) {
+    const size_t nd = 3 * (size_t)max_nodes + 13 * (size_t)tile_cells + 9 * (size_t)max_faces +
+                      (moving ? 3 * (size_t)max_nodes + 3 * (size_t)tile_cells : 0);
+    return sizeof(double) * nd + sizeof(uint16_t) * 4 * (size_t)max_faces;
+}
+
+template <typename Z>
+__device__ __forceinline__ Zone1Cell<Z> load_zone1(const double *z) {
+    Zone1Cell<Z> c;
+    c.r1 = (Z)z[0]; c.pr1 = (Z)z[1];
+    c.e = Trig<Z>{(Z)z[2], (Z)z[3], (Z)z[4], (Z)z[5]};
+    c.p = Trig<Z>{(Z)z[6], (Z)z[7], (Z)z[8], (Z)z[9]};
+    return c;
+}
+
+// Volume of an own cell at one layer from LDS nodes (Cell::Culc_volume method 1 or GCL).
+__device__ __forceinline__ double tile_volume(const double *X, const double cc[3], const uint16_t *fnodes,
+                                              const uint16_t *lf, uint32_t code, int gcl) {
+    double V = 0.0;
+    for (int j = 0; j < 6; j++) {
+        const uint16_t *fn = fnodes + 4 * lf[j];
+        const double *x0 = X + 3 * fn[0], *x1 = X + 3 * fn[1], *x2 = X + 3 * fn[2], *x3 = X + 3 * fn[3];
+        if (!gcl) {
+            V += tet_volume(cc, x0, x1, x2);
+            V += tet_volume(cc, x0, x3, x2);
+        } else {
+            const double ax = x2[0] - x0[0], ay = x2[1] - x0[1], az = x2[2] - x0[2];
+            const double bx = x3[0] - x1[0], by = x3[1] - x1[1], bz = x3[2] - x1[2];
+            const double s1 = 0.5 * (ay * bz - az * by);
+            const double s2 = 0.5 * (az * bx - ax * bz);
+            const double s3 = 0.5 * (ax * by - ay * bx);
+            const double A = sqrt(s1 * s1 + s2 * s2 + s3 * s3);
+            double n1 = s1 / A, n2 = s2 / A, n3 = s3 / A;
+            double xc = 0.0, yc = 0.0, zc = 0.0;
+            xc += x0[0]; yc += x0[1]; zc += x0[2];
+            xc += x1[0]; yc += x1[1]; zc += x1[2];
+            xc += x2[0]; yc += x2[1]; zc += x2[2];
+            xc += x3[0]; yc += x3[1]; zc += x3[2];
+            xc /= 4; yc /= 4; zc /= 4;
+            // the orientation bit of the face for this cell: n points out of cells[0]; here the
+            // face normal must point out of THIS cell, i.e. flip if n . (c - G) > 0
+            if (spf(n1, n2, n3, cc[0] - xc, cc[1] - yc, cc[2] - zc) > 0.0) { n1 = -n1; n2 = -n2; n3 = -n3; }
+            V += spf(xc, yc, zc, n1, n2, n3) * A;
+        }
+    }
+    (void)code;
+    return gcl ? V / 3.0 : V;
+}
+
+template <typename RT>
+__global__ void __launch_bounds__(kTileThreads, 2)
+    k_step_tile(TileMesh TM, DevMesh M, DevParams P, const double *__restrict__ prim1, double *__restrict__ prim2,
+                const double *__restrict__ X1g, double *__restrict__ X2g, const double *__restrict__ disp,
+                const double *__restrict__ cc1g, double *__restrict__ cc2g, const double *__restrict__ src,
+                StepScalars *S, long long step) {
+    extern __shared__ double smem[];
+    const TileLds L = tile_lds(smem, TM, P.move_mesh != 0);
+    const int t = blockIdx.x;
+    const int T = TM.tile_cells;
+    const int64_t c0 = (int64_t)t * T;
+    const int ncl = (int)min((int64_t)T, M.n_cells - c0);
+    const int nb = TM.node_off[t], nn = TM.node_off[t + 1] - nb;
+    const int fb = TM.face_off[t], nf = TM.face_off[t + 1] - fb;
+    const int own_lo = TM.own[2 * t], own_hi = TM.own[2 * t + 1];
+    const bool moving = P.move_mesh != 0;
+    const int gcl = P.geom_mode;
+    const double dt = S->dt;
+    const double fac = S->motion_fac;
+
+    // phase 1: nodes of the tile, both layers
+    for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+        const int64_t g = TM.nodes[nb + i];
+        const double a0 = X1g[3 * g], a1 = X1g[3 * g + 1], a2 = X1g[3 * g + 2];
+        L.x1[3 * i] = a0; L.x1[3 * i + 1] = a1; L.x1[3 * i + 2] = a2;
+        if (moving) {
+            const double b0 = a0 + fac * disp[3 * g], b1 = a1 + fac * disp[3 * g + 1], b2 = a2 + fac * disp[3 * g + 2];
+            L.x2[3 * i] = b0; L.x2[3 * i + 1] = b1; L.x2[3 * i + 2] = b2;
+            if (g >= own_lo && g < own_hi) { X2g[3 * g] = b0; X2g[3 * g + 1] = b1; X2g[3 * g + 2] = b2; }
+        }
+    }
+    __syncthreads();
+    const double *LX2 = moving ? L.x2 : L.x1;
+
+    // phase 2: own cells -- centres at both layers, Zone_1 frames at now1
+    for (int c = threadIdx.x; c < ncl; c += blockDim.x) {
+        const uint16_t *ln = TM.cell_lnodes + 8 * (c0 + c);
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const double *x = L.x1 + 3 * ln[i];
+            a0 += x[0]; a1 += x[1]; a2 += x[2];
+            const double *y = LX2 + 3 * ln[i];
+            b0 += y[0]; b1 += y[1]; b2 += y[2];
+        }
+        const double cA[3] = {a0 / 8, a1 / 8, a2 / 8};
+        L.c1[3 * c] = cA[0]; L.c1[3 * c + 1] = cA[1]; L.c1[3 * c + 2] = cA[2];
+        if (moving) {
+            L.c2[3 * c] = b0 / 8; L.c2[3 * c + 1] = b1 / 8; L.c2[3 * c + 2] = b2 / 8;
+            cc2g[3 * (c0 + c)] = b0 / 8; cc2g[3 * (c0 + c) + 1] = b1 / 8; cc2g[3 * (c0 + c) + 2] = b2 / 8;
+        }
+        if (((M.cell_code[c0 + c] >> 8) & 15u) == 1u) {
+            const Zone1Cell<RT> z = zone1_cell<RT>(cA, P.gamma);
+            double *o = L.z1 + 10 * c;
+            o[0] = z.r1; o[1] = z.pr1;
+            o[2] = z.e.st; o[3] = z.e.ct; o[4] = z.e.sp; o[5] = z.e.cp;
+            o[6] = z.p.st; o[7] = z.p.ct; o[8] = z.p.sp; o[9] = z.p.cp;
+        }
+    }
+    __syncthreads();
+
+    // phase 3: faces of the tile
+    double cand = 10000000.0;
+    for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+        const ulonglong2 rec = TM.faces[fb + f];
+        const int ln[4] = {(int)(rec.x & 0xfff), (int)((rec.x >> 12) & 0xfff), (int)((rec.x >> 24) & 0xfff),
+                           (int)((rec.x >> 36) & 0xfff)};
+        const uint32_t code = (uint32_t)(rec.x >> 48);
+        const uint32_t bc = code & 15u, method = (code >> 4) & 15u, flags = (code >> 8) & 255u;
+        const int64_t a = (int32_t)(rec.y & 0xffffffffu);
+        const int64_t b = (int32_t)(rec.y >> 32);
+#pragma unroll
+        for (int i = 0; i < 4; i++) L.fnodes[4 * f + i] = (uint16_t)ln[i];
+        double x1[4][3];
+#pragma unroll
+        for (int i = 0; i < 4; i++) { x1[i][0] = L.x1[3 * ln[i]]; x1[i][1] = L.x1[3 * ln[i] + 1]; x1[i][2] = L.x1[3 * ln[i] + 2]; }
+        const bool aloc = (a >= c0 && a < c0 + ncl);
+        double ca[3];
+        if (aloc) { ca[0] = L.c1[3 * (a - c0)]; ca[1] = L.c1[3 * (a - c0) + 1]; ca[2] = L.c1[3 * (a - c0) + 2]; }
+        else { ca[0] = cc1g[3 * a]; ca[1] = cc1g[3 * a + 1]; ca[2] = cc1g[3 * a + 2]; }
+        double n[3], area, G[3];
+        face_geometry(x1, ca, gcl, n, area, G);
+        double w = 0.0;
+        if (moving) {
+            double x2[4][3];
+#pragma unroll
+            for (int i = 0; i < 4; i++) { x2[i][0] = L.x2[3 * ln[i]]; x2[i][1] = L.x2[3 * ln[i] + 1]; x2[i][2] = L.x2[3 * ln[i] + 2]; }
+            if (gcl) {
+                w = swept_velocity(x1, x2, n, area, dt);
+            } else {
+                double ddot = 0.0;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const double d0 = x2[i][0] - x1[i][0], d1 = x2[i][1] - x1[i][1], d2 = x2[i][2] - x1[i][2];
+                    ddot = ddot + ((d0 * n[0] + d1 * n[1]) + d2 * n[2]);
+                }
+                w = ddot / 4 / dt;
+            }
+        }
+        // face states (Snos_on_Gran)
+        double sA[8];
+        load_state(prim1, a, sA);
+        RT qL[8], qR[8];
+        bool okbc = true;
+        double cb[3] = {0.0, 0.0, 0.0};
+        if (bc != 0) {
+            double l[8], r[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) l[i] = sA[i];
+            if (bc == 3) {
+                if (n[0] < -0.95) {
+                    if (l[1] > P.v_inf / 7.0) l[1] = P.v_inf / 5.0;
+                } else if (l[1] * n[0] + l[2] * n[1] + l[3] * n[2] < 0.0) {
+                    l[1] = 0.1 * n[0]; l[2] = 0.1 * n[1]; l[3] = 0.1 * n[2];
+                }
+#pragma unroll
+                for (int i = 0; i < 8; i++) r[i] = l[i];
+            } else if (bc == 1 || bc == 2) {
+                const double *bs = M.bstate + 8 * (int64_t)(-b - 2);
+#pragma unroll
+                for (int i = 0; i < 8; i++) r[i] = bs[i];
+            } else {
+                okbc = (bc == 4);
+#pragma unroll
+                for (int i = 0; i < 8; i++) r[i] = l[i];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++) { qL[i] = (RT)l[i]; qR[i] = (RT)r[i]; }
+        } else {
+            double sB[8];
+            load_state(prim1, b, sB);
+            const bool bloc = (b >= c0 && b < c0 + ncl);
+            if (bloc) { cb[0] = L.c1[3 * (b - c0)]; cb[1] = L.c1[3 * (b - c0) + 1]; cb[2] = L.c1[3 * (b - c0) + 2]; }
+            else { cb[0] = cc1g[3 * b]; cb[1] = cc1g[3 * b + 1]; cb[2] = cc1g[3 * b + 2]; }
+            const bool zA = ((M.cell_code[a] >> 8) & 15u) == 1u, zB = ((M.cell_code[b] >> 8) & 15u) == 1u;
+            RT tA[8], tB[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) { tA[i] = (RT)sA[i]; tB[i] = (RT)sB[i]; }
+            if (zA || zB) {
+                const Zone1Face<RT> zf = zone1_face<RT>(G, P.gamma);
+                if (zA) {
+                    const Zone1Cell<RT> zc = aloc ? load_zone1<RT>(L.z1 + 10 * (a - c0)) : zone1_cell<RT>(ca, P.gamma);
+                    zone1_apply<RT>(tA, zc, zf, qL);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 8; i++) qL[i] = tA[i];
+                }
+                if (zB) {
+                    const Zone1Cell<RT> zc = bloc ? load_zone1<RT>(L.z1 + 10 * (b - c0)) : zone1_cell<RT>(cb, P.gamma);
+                    zone1_apply<RT>(tB, zc, zf, qR);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 8; i++) qR[i] = tB[i];
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; i++) { qL[i] = tA[i]; qR[i] = tB[i]; }
+            }
+        }
+        RiemannOut<RT, 0> o;
+        chlld<RT, 0>((int)method, P.n_disc, (int)flags, (RT)n[0], (RT)n[1], (RT)n[2], (RT)w, qL, qR, nullptr, nullptr, o);
+        if (!o.ok || !okbc) report_error(S, -3, TM.face_ref[fb + f]);
+        if (bc == 0 && o.ok) {
+            const double d0 = ca[0] - cb[0], d1 = ca[1] - cb[1], d2 = ca[2] - cb[2];
+            const double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2) / 2.0;
+            const double dnt = P.kfl * dist / (smax(fabs((double)o.dsl), fabs((double)o.dsr)) + fabs(w));
+            if (dnt < cand) cand = dnt;
+        }
+        double *fl = L.flux + 9 * f;
+#pragma unroll
+        for (int i = 0; i < 8; i++) fl[i] = (double)o.q[i] * area;
+        fl[8] = 0.5 * spf(n[0], n[1], n[2], (double)qL[5] + (double)qR[5], (double)qL[6] + (double)qR[6],
+                          (double)qL[7] + (double)qR[7]) * area;
+    }
+    __syncthreads();
+
+    // phase 4: own cells -- volumes, flux sum in reference order, update
+    for (int c = threadIdx.x; c < ncl; c += blockDim.x) {
+        const int64_t cg = c0 + c;
+        const uint32_t code = M.cell_code[cg];
+        const uint16_t *lf = TM.cell_lfaces + 6 * cg;
+        const double cA[3] = {L.c1[3 * c], L.c1[3 * c + 1], L.c1[3 * c + 2]};
+        const double Volume = tile_volume(L.x1, cA, L.fnodes, lf, code, gcl);
+        double Volume2 = Volume;
+        if (moving) {
+            const double cB[3] = {L.c2[3 * c], L.c2[3 * c + 1], L.c2[3 * c + 2]};
+            Volume2 = tile_volume(L.x2, cB, L.fnodes, lf, code, gcl);
+        }
+        double Pf[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) Pf[i] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            const bool pos = (code >> j) & 1u;
+            const double *fl = L.flux + 9 * lf[j];
+#pragma unroll
+            for (int i = 0; i < 9; i++) {
+                const double v = fl[i];
+                Pf[i] += pos ? v : -v;
+            }
+        }
+        cell_update_one(M, code, cg, prim1, Volume, Volume2, Pf, src, dt, S, prim2);
+    }
+    block_min_to(cand, &S->dtmin_bits[step & 1]);
+}
+
+}  // namespace titan

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -59,6 +60,13 @@ struct titan_ctx {
     DevGeom geom[2];
     double *d_flux = nullptr, *d_src = nullptr, *d_neutrals = nullptr;
     StepScalars *d_S = nullptr, *d_S2 = nullptr;
+    // tiles of the fused step kernel
+    TileMesh tiles{};
+    int64_t tile_info[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int32_t *d_t_node_off = nullptr, *d_t_nodes = nullptr, *d_t_own = nullptr, *d_t_face_off = nullptr,
+            *d_t_face_ref = nullptr;
+    uint64_t *d_t_faces = nullptr;
+    uint16_t *d_t_cell_lnodes = nullptr, *d_t_cell_lfaces = nullptr;
     bool started = false;
     long long step = 0;
     int now1 = 1, now2 = 0;
@@ -245,6 +253,56 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
     up(dev_upload(&c->d_X[1], X, c->stream));
     if (!disp.empty()) up(dev_upload(&c->d_disp, disp, c->stream));
     if (rc != TITAN_OK) { titan_destroy(c); return rc; }
+    {
+        // tile size: the largest of 128, 112, ... cells whose LDS footprint fits the device's
+        // per-workgroup limit (TITAN_TILE_CELLS overrides the starting size)
+        int lds_limit = 65536;
+        {
+            int dev = 0, v = 0;
+            if (hipGetDevice(&dev) == hipSuccess &&
+                hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+                lds_limit = v;
+        }
+        int tile_cells = 128;
+        if (const char *e = std::getenv("TITAN_TILE_CELLS")) tile_cells = std::max(16, std::min(1024, std::atoi(e)));
+        TileData td;
+        bool fits = false;
+        for (; tile_cells >= 16; tile_cells -= 16) {
+            if (!build_tiles(nc, nf, nn, tile_cells, cell_nodes.data(), cell_faces.data(), face_nodes.data(),
+                             face_cells.data(), face_code.data(), face_bidx.data(), face_ref.data(), td))
+                continue;
+            TileMesh probe{};
+            probe.tile_cells = td.tile_cells;
+            probe.max_nodes = td.max_nodes;
+            probe.max_faces = td.max_faces;
+            if (step_tile_lds_bytes(probe, p->move_mesh != 0) <= (size_t)lds_limit) { fits = true; break; }
+        }
+        if (!fits) { titan_destroy(c); return TITAN_E_TOPOLOGY; }
+        up(dev_upload(&c->d_t_node_off, td.node_off, c->stream));
+        up(dev_upload(&c->d_t_nodes, td.nodes, c->stream));
+        up(dev_upload(&c->d_t_own, td.own, c->stream));
+        up(dev_upload(&c->d_t_face_off, td.face_off, c->stream));
+        up(dev_upload(&c->d_t_face_ref, td.face_ref, c->stream));
+        up(dev_upload(&c->d_t_faces, td.faces, c->stream));
+        up(dev_upload(&c->d_t_cell_lnodes, td.cell_lnodes, c->stream));
+        up(dev_upload(&c->d_t_cell_lfaces, td.cell_lfaces, c->stream));
+        if (rc != TITAN_OK) { titan_destroy(c); return rc; }
+        TileMesh &T = c->tiles;
+        T.n_tiles = (int64_t)td.node_off.size() - 1;
+        T.tile_cells = td.tile_cells;
+        T.max_nodes = td.max_nodes;
+        T.max_faces = td.max_faces;
+        T.node_off = c->d_t_node_off; T.nodes = c->d_t_nodes; T.own = c->d_t_own; T.face_off = c->d_t_face_off;
+        T.faces = reinterpret_cast<const ulonglong2 *>(c->d_t_faces); T.face_ref = c->d_t_face_ref;
+        T.cell_lnodes = c->d_t_cell_lnodes; T.cell_lfaces = c->d_t_cell_lfaces;
+        c->tile_info[0] = T.tile_cells;
+        c->tile_info[1] = T.max_nodes;
+        c->tile_info[2] = T.max_faces;
+        c->tile_info[3] = (int64_t)step_tile_lds_bytes(T, p->move_mesh != 0);
+        c->tile_info[4] = lds_limit;
+        c->tile_info[5] = T.n_tiles;
+        c->tile_info[6] = (int64_t)td.face_ref.size();
+    }
     const size_t geom_n = (size_t)(4 * nc + 7 * nf);
     for (int l = 0; l < 2; l++) {
         if (hipMalloc((void **)&c->d_prim[l], sizeof(double) * 8 * nc) != hipSuccess ||
@@ -285,7 +343,8 @@ extern "C" int titan_destroy(titan_ctx *c) {
     void *ptrs[] = {c->d_cell_nodes, c->d_cell_faces, c->d_face_nodes, c->d_face_cells, c->d_face_bidx,
                     c->d_face_ref, c->d_cell_ref, c->d_cell_code, c->d_face_code, c->d_bstate, c->d_disp,
                     c->d_X[0], c->d_X[1], c->d_prim[0], c->d_prim[1], c->d_geom[0], c->d_geom[1], c->d_flux,
-                    c->d_src, c->d_neutrals, c->d_S, c->d_S2};
+                    c->d_src, c->d_neutrals, c->d_S, c->d_S2, c->d_t_node_off, c->d_t_nodes, c->d_t_own,
+                    c->d_t_face_off, c->d_t_face_ref, c->d_t_faces, c->d_t_cell_lnodes, c->d_t_cell_lfaces};
     for (void *q : ptrs)
         if (q) hipFree(q);
     mlp_free(c->mlp);
@@ -362,6 +421,9 @@ extern "C" int titan_get_geometry(titan_ctx *c, int layer, double *cell_center, 
                                   double *face_center, double *face_normal, double *face_area) {
     if (!c || layer < 0 || layer > 1) return TITAN_E_ARG;
     const int64_t nc = c->nc, nf = c->nf;
+    // the fused step keeps only cell centres current: rebuild the layer's geometry from its nodes
+    launch_measure(c->stream, c->mesh(), c->d_X[layer], c->geom[layer], c->params.geom_mode);
+    HIPCHK(hipGetLastError());
     std::vector<double> h((size_t)(4 * nc + 7 * nf));
     HIPCHK(hipMemcpyAsync(h.data(), c->d_geom[layer], h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -389,6 +451,7 @@ extern "C" int titan_gran_potok(titan_ctx *c, int now, double dt, double *face_f
     HIPCHK(hipMemcpy(&c->d_S2->dtmin_bits[1], &init, sizeof(init), hipMemcpyHostToDevice));
     uint8_t *d_branch = nullptr;
     if (branch) HIPCHK(hipMalloc((void **)&d_branch, c->nf));
+    launch_measure(c->stream, c->mesh(), c->d_X[now], c->geom[now], c->params.geom_mode);
     launch_face_flux(c->stream, c->params.precision, c->mesh(), c->dp, c->d_prim[now], c->geom[now], c->d_X[now],
                      c->d_X[1 - now], c->d_S2, c->d_flux, d_branch, 1, dt);
     HIPCHK(hipGetLastError());
@@ -453,14 +516,6 @@ extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
         c->step++;
         const int n1 = c->now1, n2 = c->now2;
         launch_begin_step(c->stream, c->d_S, c->step, c->dp);
-        if (c->params.move_mesh) {
-            timer_mark(c, 2, true);
-            launch_motion(c->stream, nn3, c->d_X[n1], c->d_disp, c->d_S, c->d_X[n2]);
-            timer_mark(c, 2, false);
-            timer_mark(c, 1, true);
-            launch_measure(c->stream, M, c->d_X[n2], c->geom[n2], c->params.geom_mode);
-            timer_mark(c, 1, false);
-        }
         const double *src = nullptr;
         if (c->params.source_mode != TITAN_SRC_NONE) {
             timer_mark(c, 3, true);
@@ -470,10 +525,9 @@ extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
             src = c->d_src;
         }
         timer_mark(c, 0, true);
-        launch_face_flux(c->stream, c->params.precision, M, c->dp, c->d_prim[n1], c->geom[n1], c->d_X[n1],
-                         c->d_X[n2], c->d_S, c->d_flux, nullptr, c->step, 0.0);
-        launch_cell_update(c->stream, M, c->d_prim[n1], c->geom[n1].cell_volume, c->geom[n2].cell_volume,
-                           c->d_flux, src, c->d_S, c->d_prim[n2], 0.0);
+        launch_step_tile(c->stream, c->params.precision, c->tiles, M, c->dp, c->d_prim[n1], c->d_prim[n2], c->d_X[n1],
+                         c->d_X[n2], c->d_disp, c->geom[n1].cell_center, c->geom[n2].cell_center, src, c->d_S,
+                         c->step);
         timer_mark(c, 0, false);
     }
     HIPCHK(hipGetLastError());
@@ -631,4 +685,10 @@ extern "C" int titan_mlp_train(titan_ctx *c, int64_t n, const float *x, const fl
                                float lr, uint64_t seed, float *loss_hist) {
     if (!c || n <= 0 || !x || !y || epochs < 0 || batch <= 0) return TITAN_E_ARG;
     return mlp_train_host(c->mlp, c->stream, n, x, y, epochs, batch, lr, seed, loss_hist);
+}
+
+extern "C" int titan_tile_info(titan_ctx *c, int64_t *info8) {
+    if (!c || !info8) return TITAN_E_ARG;
+    for (int i = 0; i < 8; i++) info8[i] = c->tile_info[i];
+    return TITAN_OK;
 }

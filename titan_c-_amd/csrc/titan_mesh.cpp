@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/titan_gpu.h"
+#include "device_mesh.h"
 #include "titan_mesh.h"
 
 namespace titan {
@@ -254,6 +255,72 @@ void build_order(const MeshView &m, DeviceOrder &o) {
     }
     for (int64_t v = 0; v < nn; v++)
         if (o.node_r2d[v] < 0) { o.node_r2d[v] = (int32_t)next; o.node_d2r[next] = (int32_t)v; next++; }
+}
+
+bool build_tiles(int64_t nc, int64_t nf, int64_t nn, int tile_cells, const int32_t *cell_nodes,
+                 const int32_t *cell_faces, const int32_t *face_nodes, const int32_t *face_cells,
+                 const uint32_t *face_code, const int32_t *face_bidx, const int32_t *face_ref, TileData &t) {
+    const int64_t T = tile_cells;
+    const int64_t nt = (nc + T - 1) / T;
+    t = TileData();
+    t.tile_cells = tile_cells;
+    t.node_off.assign(nt + 1, 0);
+    t.face_off.assign(nt + 1, 0);
+    t.own.assign(2 * nt, 0);
+    t.cell_lnodes.assign(8 * nc, 0);
+    t.cell_lfaces.assign(6 * nc, 0);
+    std::vector<int32_t> nstamp(nn, -1), nlocal(nn, 0), fstamp(nf, -1), flocal(nf, 0);
+    std::vector<int32_t> lst;
+    int64_t seen_max = -1;
+    for (int64_t ti = 0; ti < nt; ti++) {
+        const int64_t c0 = ti * T, c1 = std::min(nc, c0 + T);
+        t.own[2 * ti] = (int32_t)(seen_max + 1);
+        // nodes
+        lst.clear();
+        for (int64_t c = c0; c < c1; c++)
+            for (int i = 0; i < 8; i++) {
+                const int32_t v = cell_nodes[8 * c + i];
+                if (nstamp[v] != ti) { nstamp[v] = (int32_t)ti; lst.push_back(v); }
+                seen_max = std::max<int64_t>(seen_max, v);
+            }
+        std::sort(lst.begin(), lst.end());
+        if (lst.size() > 4095) return false;
+        for (size_t i = 0; i < lst.size(); i++) nlocal[lst[i]] = (int32_t)i;
+        t.node_off[ti + 1] = t.node_off[ti] + (int32_t)lst.size();
+        t.nodes.insert(t.nodes.end(), lst.begin(), lst.end());
+        t.max_nodes = std::max<int32_t>(t.max_nodes, (int32_t)lst.size());
+        t.own[2 * ti + 1] = (int32_t)(seen_max + 1);
+        for (int64_t c = c0; c < c1; c++)
+            for (int i = 0; i < 8; i++) t.cell_lnodes[8 * c + i] = (uint16_t)nlocal[cell_nodes[8 * c + i]];
+        // faces
+        lst.clear();
+        for (int64_t c = c0; c < c1; c++)
+            for (int j = 0; j < 6; j++) {
+                const int32_t f = cell_faces[6 * c + j];
+                if (fstamp[f] != ti) { fstamp[f] = (int32_t)ti; lst.push_back(f); }
+            }
+        std::sort(lst.begin(), lst.end());
+        for (size_t i = 0; i < lst.size(); i++) flocal[lst[i]] = (int32_t)i;
+        t.face_off[ti + 1] = t.face_off[ti] + (int32_t)lst.size();
+        t.max_faces = std::max<int32_t>(t.max_faces, (int32_t)lst.size());
+        for (int32_t f : lst) {
+            int ln[4];
+            for (int i = 0; i < 4; i++) {
+                const int32_t v = face_nodes[4 * (int64_t)f + i];
+                if (nstamp[v] != ti) return false;  // a face node outside the tile's cells
+                ln[i] = nlocal[v];
+            }
+            const int32_t a = face_cells[2 * (int64_t)f];
+            int32_t b = face_cells[2 * (int64_t)f + 1];
+            if (b < 0) b = face_bidx[f] >= 0 ? -(face_bidx[f] + 2) : -1;
+            t.faces.push_back(tile_face_pack(ln, face_code[f]));
+            t.faces.push_back((uint64_t)(uint32_t)a | ((uint64_t)(uint32_t)b << 32));
+            t.face_ref.push_back(face_ref[f]);
+        }
+        for (int64_t c = c0; c < c1; c++)
+            for (int j = 0; j < 6; j++) t.cell_lfaces[6 * c + j] = (uint16_t)flocal[cell_faces[6 * c + j]];
+    }
+    return true;
 }
 
 }  // namespace titan
