@@ -15,6 +15,7 @@ def var_err(a, b):
 
 
 def small_shell(C, moving=False, n_edge=6, n_layers=10, **kw):
+    kw.setdefault("noise", 0.01)
     return C.shell(n_edge=n_edge, n_layers=n_layers, moving=moving, r_zone1_au=80.0, **kw)
 
 

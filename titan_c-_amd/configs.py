@@ -151,8 +151,14 @@ def shell_radii(n_layers, r_out_au=1500.0):
 
 
 def shell(n_edge=37, n_layers=120, moving=False, seed_motion=3, r_zone1_au=80.0, r_blend_au=100.0,
-          amp=0.05, omega=100.0):
-    """Heliosphere-like shell; returns (mesh, prim0, params)."""
+          amp=0.05, omega=100.0, noise=0.0, seed_state=4):
+    """Heliosphere-like shell; returns (mesh, prim0, params).
+
+    `noise` > 0 perturbs density and velocity cell by cell (relative amplitude `noise`, seeded).
+    The unperturbed wind is exactly radial, so every lateral face has v.n = 0 up to rounding and
+    its HLLC/HLLD contact speed SM sits on the branch point of the flux; the perturbation moves
+    parity workloads off that set.
+    """
     radii = shell_radii(n_layers)
     xyz, cn = mesh_shell(n_edge, radii)
     rin, rout = radii[0], radii[-1]
@@ -205,6 +211,13 @@ def shell(n_edge=37, n_layers=120, moving=False, seed_motion=3, r_zone1_au=80.0,
     wind = solar_wind_state(ccen)
     w = 0.5 * (1.0 + np.tanh((rc - r_blend_au * R0_AU) / (0.2 * r_blend_au * R0_AU)))
     prim = (1.0 - w)[:, None] * wind + w[:, None] * lism[None, :]
+    if noise > 0.0:
+        nc = cn.shape[0]
+        u = [2.0 * splitmix_uniform(seed_state, np.arange(nc) * 4 + k) - 1.0 for k in range(4)]
+        vmag = np.linalg.norm(prim[:, 1:4], axis=1)
+        prim[:, 0] *= 1.0 + noise * u[0]
+        for k in range(3):
+            prim[:, 1 + k] += noise * vmag * u[1 + k]
     params = default_params()
     if moving:
         params.update(move_mesh=1, motion_omega=omega, motion_phase=float(2.0 * math.pi * splitmix_uniform(seed_motion, 3)))

@@ -167,31 +167,48 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
     const T e1 = p1 / g1 + r1 * upt1 + b2L / T(2);
     const T e2 = p2 / g1 + r2 * upt2 + b2R / T(2);
 
-    T FL[8], FR[8], UL[8], UR[8];
-    FL[0] = r1 * vL[0];
-    FL[1] = r1 * vL[0] * vL[0] + ptL - bL[0] * bL[0];
-    FL[2] = r1 * vL[0] * vL[1] - bL[0] * bL[1];
-    FL[3] = r1 * vL[0] * vL[2] - bL[0] * bL[2];
-    FL[4] = (e1 + ptL) * vL[0] - bL[0] * sbv1;
-    FL[5] = T(0);
-    FL[6] = vL[0] * bL[1] - vL[1] * bL[0];
-    FL[7] = vL[0] * bL[2] - vL[2] * bL[0];
-    FR[0] = r2 * vR[0];
-    FR[1] = r2 * vR[0] * vR[0] + ptR - bR[0] * bR[0];
-    FR[2] = r2 * vR[0] * vR[1] - bR[0] * bR[1];
-    FR[3] = r2 * vR[0] * vR[2] - bR[0] * bR[2];
-    FR[4] = (e2 + ptR) * vR[0] - bR[0] * sbv2;
-    FR[5] = T(0);
-    FR[6] = vR[0] * bR[1] - vR[1] * bR[0];
-    FR[7] = vR[0] * bR[2] - vR[2] * bR[0];
-    UL[0] = r1; UL[4] = e1; UR[0] = r2; UR[4] = e2;
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        UL[i + 1] = r1 * vL[i];
-        UL[i + 5] = bL[i];
-        UR[i + 1] = r2 * vR[i];
-        UR[i + 5] = bR[i];
-    }
+    // Left/right physical fluxes and conserved states (Phys_param.cpp:1069-1117), formed where
+    // they are used so that only the selected side stays live in registers.
+    auto fl = [&](int k) -> T {
+        switch (k) {
+        case 0: return r1 * vL[0];
+        case 1: return r1 * vL[0] * vL[0] + ptL - bL[0] * bL[0];
+        case 2: return r1 * vL[0] * vL[1] - bL[0] * bL[1];
+        case 3: return r1 * vL[0] * vL[2] - bL[0] * bL[2];
+        case 4: return (e1 + ptL) * vL[0] - bL[0] * sbv1;
+        case 5: return T(0);
+        case 6: return vL[0] * bL[1] - vL[1] * bL[0];
+        default: return vL[0] * bL[2] - vL[2] * bL[0];
+        }
+    };
+    auto fr = [&](int k) -> T {
+        switch (k) {
+        case 0: return r2 * vR[0];
+        case 1: return r2 * vR[0] * vR[0] + ptR - bR[0] * bR[0];
+        case 2: return r2 * vR[0] * vR[1] - bR[0] * bR[1];
+        case 3: return r2 * vR[0] * vR[2] - bR[0] * bR[2];
+        case 4: return (e2 + ptR) * vR[0] - bR[0] * sbv2;
+        case 5: return T(0);
+        case 6: return vR[0] * bR[1] - vR[1] * bR[0];
+        default: return vR[0] * bR[2] - vR[2] * bR[0];
+        }
+    };
+    auto ul = [&](int k) -> T {
+        switch (k) {
+        case 0: return r1;
+        case 4: return e1;
+        case 1: case 2: case 3: return r1 * vL[k - 1];
+        default: return bL[k - 5];
+        }
+    };
+    auto ur = [&](int k) -> T {
+        switch (k) {
+        case 0: return r2;
+        case 4: return e2;
+        case 1: case 2: case 3: return r2 * vR[k - 1];
+        default: return bR[k - 5];
+        }
+    };
     T FLk[K > 0 ? K : 1], FRk[K > 0 ? K : 1];
 #pragma unroll
     for (int i = 0; i < K; i++) { FLk[i] = kL[i] * vL[0]; FRk[i] = kR[i] * vR[0]; }
@@ -214,11 +231,11 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
 #pragma unroll
         for (int ik = 0; ik < 8; ik++) {
             T FW;
-            if (region == 1) FW = wv * UL[ik];
-            else if (region == 3) FW = wv * UR[ik];
-            else FW = wv * ((SR * UR[ik] - SL * UL[ik] + FL[ik] - FR[ik]) / (SR - SL));
-            const T dq = UR[ik] - UL[ik];
-            qq[ik] = (TR * FL[ik] - TL * FR[ik] + a * dq) / b - FW;
+            if (region == 1) FW = wv * ul(ik);
+            else if (region == 3) FW = wv * ur(ik);
+            else FW = wv * ((SR * ur(ik) - SL * ul(ik) + fl(ik) - fr(ik)) / (SR - SL));
+            const T dq = ur(ik) - ul(ik);
+            qq[ik] = (TR * fl(ik) - TL * fr(ik) + a * dq) / b - FW;
         }
 #pragma unroll
         for (int ik = 0; ik < K; ik++) {
@@ -235,9 +252,11 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
         else wbn = wv * (bL[0] + bR[0]) / T(2);
     } else {
         // UZ (HLL average) components used by HLLC and HLLD: 0..3 and 5..7
+        // HLL average state UZ (Phys_param.cpp:1119-1122); HLLD needs only its normal field UZ[5]
         T UZ[8];
 #pragma unroll
-        for (int ik = 0; ik < 8; ik++) UZ[ik] = (SR * UR[ik] - SL * UL[ik] + FL[ik] - FR[ik]) / (SR - SL);
+        for (int ik = 0; ik < 8; ik++)
+            UZ[ik] = (n_state == 2 || ik == 5) ? (SR * ur(ik) - SL * ul(ik) + fl(ik) - fr(ik)) / (SR - SL) : T(0);
         const T suRm = suR / (SR - SM);
         const T suLm = suL / (SL - SM);
         const T rzR = r2 * suRm;
@@ -267,7 +286,7 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
             if (region == 2) {
                 const T UZL[8] = {rzL, SM * rzL, vzL1 * rzL, vzL2 * rzL, ezL, UZ[5], bzL1, bzL2};
 #pragma unroll
-                for (int ik = 0; ik < 8; ik++) qq[ik] = FL[ik] + SL * (UZL[ik] - UL[ik]) - wv * UZL[ik];
+                for (int ik = 0; ik < 8; ik++) qq[ik] = fl(ik) + SL * (UZL[ik] - ul(ik)) - wv * UZL[ik];
 #pragma unroll
                 for (int ik = 0; ik < K; ik++) {
                     const T z = kL[ik] * suLm;
@@ -276,7 +295,7 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
             } else if (region == 3) {
                 const T UZR[8] = {rzR, SM * rzR, vzR1 * rzR, vzR2 * rzR, ezR, UZ[5], bzR1, bzR2};
 #pragma unroll
-                for (int ik = 0; ik < 8; ik++) qq[ik] = FR[ik] + SR * (UZR[ik] - UR[ik]) - wv * UZR[ik];
+                for (int ik = 0; ik < 8; ik++) qq[ik] = fr(ik) + SR * (UZR[ik] - ur(ik)) - wv * UZR[ik];
 #pragma unroll
                 for (int ik = 0; ik < K; ik++) {
                     const T z = kR[ik] * suRm;
@@ -284,12 +303,12 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
                 }
             } else if (region == 4) {
 #pragma unroll
-                for (int ik = 0; ik < 8; ik++) qq[ik] = FR[ik] - wv * UR[ik];
+                for (int ik = 0; ik < 8; ik++) qq[ik] = fr(ik) - wv * ur(ik);
 #pragma unroll
                 for (int ik = 0; ik < K; ik++) out.k[ik] = FRk[ik] - wv * kR[ik];
             } else {
 #pragma unroll
-                for (int ik = 0; ik < 8; ik++) qq[ik] = FL[ik] - wv * UL[ik];
+                for (int ik = 0; ik < 8; ik++) qq[ik] = fl(ik) - wv * ul(ik);
 #pragma unroll
                 for (int ik = 0; ik < K; ik++) out.k[ik] = FLk[ik] - wv * kL[ik];
             }
@@ -336,19 +355,19 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
             else region = j;
             if (j == 1) {
 #pragma unroll
-                for (int ik = 0; ik < 8; ik++) qq[ik] = FL[ik] - wv * UL[ik];
+                for (int ik = 0; ik < 8; ik++) qq[ik] = fl(ik) - wv * ul(ik);
 #pragma unroll
                 for (int ik = 0; ik < K; ik++) out.k[ik] = FLk[ik] - wv * kL[ik];
             } else if (j == 6) {
 #pragma unroll
-                for (int ik = 0; ik < 8; ik++) qq[ik] = FR[ik] - wv * UR[ik];
+                for (int ik = 0; ik < 8; ik++) qq[ik] = fr(ik) - wv * ur(ik);
 #pragma unroll
                 for (int ik = 0; ik < K; ik++) out.k[ik] = FRk[ik] - wv * kR[ik];
             } else if (j == 2 || j == 3) {
                 const T UZL[8] = {rzL, SM * rzL, vzL1 * rzL, vzL2 * rzL, ezL, bn, bzL1, bzL2};
                 if (j == 2) {
 #pragma unroll
-                    for (int ik = 0; ik < 8; ik++) qq[ik] = FL[ik] + SL * (UZL[ik] - UL[ik]) - wv * UZL[ik];
+                    for (int ik = 0; ik < 8; ik++) qq[ik] = fl(ik) + SL * (UZL[ik] - ul(ik)) - wv * UZL[ik];
 #pragma unroll
                     for (int ik = 0; ik < K; ik++) {
                         const T z = kL[ik] * suLm;
@@ -364,7 +383,7 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
                     const T UZZL[8] = {rzL, SM * rzL, vzz1 * rzL, vzz2 * rzL, ezzL, bn, bzz1, bzz2};
 #pragma unroll
                     for (int ik = 0; ik < 8; ik++)
-                        qq[ik] = FL[ik] + SZL * (UZZL[ik] - UZL[ik]) + SL * (UZL[ik] - UL[ik]) - wv * UZZL[ik];
+                        qq[ik] = fl(ik) + SZL * (UZZL[ik] - UZL[ik]) + SL * (UZL[ik] - ul(ik)) - wv * UZZL[ik];
 #pragma unroll
                     for (int ik = 0; ik < K; ik++) {
                         const T z = kL[ik] * suLm;
@@ -375,7 +394,7 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
                 const T UZR[8] = {rzR, SM * rzR, vzR1 * rzR, vzR2 * rzR, ezR, bn, bzR1, bzR2};
                 if (j == 5) {
 #pragma unroll
-                    for (int ik = 0; ik < 8; ik++) qq[ik] = FR[ik] + SR * (UZR[ik] - UR[ik]) - wv * UZR[ik];
+                    for (int ik = 0; ik < 8; ik++) qq[ik] = fr(ik) + SR * (UZR[ik] - ur(ik)) - wv * UZR[ik];
 #pragma unroll
                     for (int ik = 0; ik < K; ik++) {
                         const T z = kR[ik] * suRm;
@@ -391,7 +410,7 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
                     const T UZZR[8] = {rzR, SM * rzR, vzz1 * rzR, vzz2 * rzR, ezzR, bn, bzz1, bzz2};
 #pragma unroll
                     for (int ik = 0; ik < 8; ik++)
-                        qq[ik] = FR[ik] + SZR * (UZZR[ik] - UZR[ik]) + SR * (UZR[ik] - UR[ik]) - wv * UZZR[ik];
+                        qq[ik] = fr(ik) + SZR * (UZZR[ik] - UZR[ik]) + SR * (UZR[ik] - ur(ik)) - wv * UZZR[ik];
 #pragma unroll
                     for (int ik = 0; ik < K; ik++) {
                         const T z = kR[ik] * suRm;

@@ -23,26 +23,23 @@ constexpr int kTileThreads = 256;
 
 template <typename Z> struct Trig { Z st, ct, sp, cp; };
 
-template <typename Z>
-__device__ __forceinline__ Z polar_angle_z(Z x, Z y) {
-    const Z pi = Z(kPi);
-    if (fabs(x) + fabs(y) < Z(0.000001)) return Z(0);
-    if (x < 0) return atan(y / x) + Z(1.0) * pi;
-    else if (x > 0 && y >= 0) return atan(y / x);
-    else if (x > 0 && y < 0) return atan(y / x) + Z(2.0) * pi;
-    else if (y > 0 && x >= 0 && x <= 0) return pi / Z(2.0);
-    else if (y < 0 && x >= 0 && x <= 0) return Z(3.0) * pi / Z(2.0);
-    return Z(0);
-}
-
 // (theta, phi) trig of spherical_skorost / dekard_skorost (Help_funk.cpp:108-135) for the axis
-// roles (z, x, y); shared between the V and B rotations (same values the reference recomputes).
+// roles (z, x, y): theta = acos(z/r), phi = polar_angle(x, y) (Help_funk.cpp:78-106).  The
+// reference evaluates sin/cos of those angles; they equal rho/r, z/r, y/rho, x/rho with
+// rho = sqrt(x^2 + y^2) in every quadrant branch of polar_angle (phi = 0 when |x| + |y| < 1e-6),
+// so they are formed directly from the coordinates (one square root and four divisions instead of
+// acos, atan and four sin/cos; the values agree to a few ulp, the accuracy at which GPU and host
+// libm already differ).  The V and B rotations share them.
 template <typename Z>
 __device__ __forceinline__ Trig<Z> frame_trig(Z z, Z x, Z y) {
     const Z r = sqrt(x * x + y * y + z * z);
-    const Z the = acos(z / r);
-    const Z phi = polar_angle_z<Z>(x, y);
-    return Trig<Z>{sin(the), cos(the), sin(phi), cos(phi)};
+    const Z rho = sqrt(x * x + y * y);
+    Trig<Z> t;
+    t.st = rho / r;
+    t.ct = z / r;
+    if (fabs(x) + fabs(y) < Z(0.000001)) { t.sp = Z(0); t.cp = Z(1); }
+    else { t.sp = y / rho; t.cp = x / rho; }
+    return t;
 }
 
 // Zone_1 cell data: r1, r1^(2 gamma), trig of the equatorial (z,x,y) = (c2,c0,c1) and polar
@@ -69,16 +66,11 @@ __device__ __forceinline__ Zone1Face<Z> zone1_face(const double G[3], double gam
     Zone1Face<Z> f;
     f.rr = sqrt((g0 * g0 + g1 * g1) + g2 * g2);
     f.prr = pow(f.rr, Z(2.0) * (Z)gamma);
+    // the frame choice keeps the reference's own test on theta (Setka_gran_snos.cpp:771-773)
     const Z the1 = acos(g2 / f.rr);
     const Z pi = Z(kPi);
     f.equatorial = (the1 > pi / Z(9) && the1 < Z(8) * pi / Z(9));
-    if (f.equatorial) {
-        // dekard_skorost(G2, G0, G1): r_2 = |G| in the same summation order, the_2 = the1
-        const Z phi = polar_angle_z<Z>(g0, g1);
-        f.t = Trig<Z>{sin(the1), cos(the1), sin(phi), cos(phi)};
-    } else {
-        f.t = frame_trig<Z>(g0, g1, g2);
-    }
+    f.t = f.equatorial ? frame_trig<Z>(g2, g0, g1) : frame_trig<Z>(g0, g1, g2);
     return f;
 }
 
