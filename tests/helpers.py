@@ -30,3 +30,8 @@ def rel_err(a, b, floor=1e-300):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), floor)
+
+
+def var_err(a, b):
+    """Error per conserved variable, relative to that variable's largest magnitude on the mesh."""
+    return (np.abs(a - b).max(axis=0) / np.maximum(np.abs(b).max(axis=0), 1e-300)).max()

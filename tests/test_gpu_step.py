@@ -4,14 +4,9 @@ import hashlib
 import numpy as np
 import pytest
 
-from helpers import rel_err
+from helpers import rel_err, var_err
 
 pytestmark = pytest.mark.gpu
-
-
-def var_err(a, b):
-    """Error per conserved variable, relative to that variable's largest magnitude on the mesh."""
-    return (np.abs(a - b).max(axis=0) / np.maximum(np.abs(b).max(axis=0), 1e-300)).max()
 
 
 def small_shell(C, moving=False, n_edge=6, n_layers=10, **kw):

@@ -6,6 +6,8 @@ import importlib
 import numpy as np
 import pytest
 
+from helpers import var_err
+
 pytestmark = pytest.mark.gpu
 
 
@@ -127,3 +129,68 @@ def test_train_and_gain(T, C):
     e_mlp = np.abs(runs[T.SRC_MLP] - ref)[:, 1:4].max()
     assert e_none > 0
     assert e_mlp < 0.2 * e_none, (e_mlp, e_none)
+
+
+def _cell_features(sur, prim, nh):
+    return sur.features(prim[:, 0], prim[:, 4] / prim[:, 0], prim[:, 1:4], nh[:, 0], nh[:, 1], nh[:, 2:5])
+
+
+def _fused_setup(T, C, sur):
+    mesh, prim, params = C.shell(n_edge=6, n_layers=10, r_zone1_au=80.0, noise=0.01)
+    nh = C.neutrals(mesh.n_cells)
+    x = _cell_features(sur, prim, nh)
+    norm = sur.Normaliser(x, np.ones((len(x), 5)))
+    in_norm, _, box = norm.blocks()
+    scale = np.full(5, 0.1, np.float32)
+    return mesh, prim, params, nh, x, in_norm, scale, box
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_surrogate_step(T, C, O, precision):
+    """One step with the network fused into the step kernel equals the oracle's update fed the
+    standalone surrogate kernel's sources: network for cells inside the training box, exact
+    quadrature (fixup kernel) for the cells outside it -- here every cell whose ln n_p is above
+    the median.  FP64: 1e-12; FP32 Riemann arithmetic: 1e-6."""
+    sur = importlib.import_module("titan_c-_amd.surrogate")
+    mesh, prim, params, nh, x, in_norm, scale, box = _fused_setup(T, C, sur)
+    box = box.copy()
+    box[8] = np.float32(np.median(x[:, 0]))
+    x32 = x.astype(np.float32)
+    n_out = int((~np.all((x32 >= box[:8]) & (x32 <= box[8:]), axis=1)).sum())
+    assert 0 < n_out < mesh.n_cells
+    P = sur.init_params(3)
+    with T.Setka(mesh, dict(params, source_mode=T.SRC_MLP, cx_nodes=32, precision=precision)) as s:
+        s.set_state(prim, 0)
+        s.set_neutrals(nh)
+        s.mlp_set(P, in_norm, scale, box)
+        src = s.sources(0, T.SRC_MLP)
+        st = s.Go(1)
+        got = s.get_state()
+    assert abs(st["n_fallback"] - n_out) <= 2, (st["n_fallback"], n_out)
+    om = O.OracleMesh(mesh)
+    g = O.measure(om, mesh.node_xyz)
+    rc, flux, _, _ = O.face_flux(om, params, prim, g)
+    assert rc == 0
+    rc, ref, _, _ = O.update(om, params, prim, g["cell_volume"], g["cell_volume"], flux, 1e-5, src)
+    assert rc == 0
+    rc, ref0, _, _ = O.update(om, params, prim, g["cell_volume"], g["cell_volume"], flux, 1e-5, None)
+    assert var_err(ref0, ref) > 1e-9  # the source is visible at this step
+    assert var_err(got, ref) < (1e-12 if precision == 0 else 1e-6)
+
+
+def test_fused_surrogate_all_fallback(T, C):
+    """With an empty training box every cell goes through the fixup kernel: the step equals the
+    step with the exact source (same quadrature, workgroup-parallel summation order)."""
+    sur = importlib.import_module("titan_c-_amd.surrogate")
+    mesh, prim, params, nh, x, in_norm, scale, _ = _fused_setup(T, C, sur)
+    empty = np.concatenate([np.ones(8), -np.ones(8)]).astype(np.float32)
+    runs = {}
+    for mode in (T.SRC_MLP, T.SRC_EXACT):
+        with T.Setka(mesh, dict(params, source_mode=mode, cx_nodes=32)) as s:
+            s.set_state(prim, 0)
+            s.set_neutrals(nh)
+            s.mlp_set(sur.init_params(3), in_norm, scale, empty)
+            st = s.Go(2)
+            runs[mode] = (s.get_state(), st)
+    assert runs[T.SRC_MLP][1]["n_fallback"] == mesh.n_cells
+    assert var_err(runs[T.SRC_MLP][0], runs[T.SRC_EXACT][0]) < 1e-12

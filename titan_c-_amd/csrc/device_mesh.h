@@ -72,6 +72,8 @@ namespace titan {
 // indices < 4096), and every face touching them (faces on a tile border appear in both tiles).
 // A face record is 16 bytes: x = 4 local node ids (12 bits each) | face code << 48; y = cells[0]
 // (device id, low 32 bits) | cells[1] << 32 (device id, or -(boundary-state index + 2)).
+constexpr int kTileThreads = 256;  // workgroup size of the fused step kernel
+
 struct TileMesh {
     int64_t n_tiles;
     int32_t tile_cells, max_nodes, max_faces, pad;

@@ -1,6 +1,8 @@
 // k_cx.hip -- exact FP64 charge-exchange source kernels (FP64, no contraction).
+#define TITAN_FLUX_ONLY 1
 #include "cx_device.h"
 #include "launch.h"
+#include "step_kernels.h"
 
 namespace titan {
 
@@ -58,6 +60,45 @@ __global__ void __launch_bounds__(256) k_cx_list(const int32_t *__restrict__ lis
 #pragma unroll
         for (int k = 0; k < 5; k++) src[5 * c + k] = o[k];
     }
+}
+
+// Cells that the surrogate step kernel left outside the training box (list and stash written by
+// k_step_tile<*, true>, count in S->n_fallback): exact FP64 source, one workgroup per listed cell
+// (quadrature spread over the workgroup, fixed-order reduction), then the cell update redone from
+// the stashed flux sum and volumes.  prim1 is the step's unchanged input layer.
+__global__ void __launch_bounds__(256) k_cx_fixup(DevMesh M, const double *__restrict__ prim1,
+                                                  const double *__restrict__ neutrals, int nodes,
+                                                  const int32_t *__restrict__ list, const double *__restrict__ stash,
+                                                  int64_t cap, StepScalars *S, double *__restrict__ prim2) {
+    extern __shared__ double sm[];
+    double *sc = sm, *red = sm + 2 * nodes;
+    const int64_t n = min((int64_t)S->n_fallback, cap);
+    if ((int64_t)blockIdx.x >= n) return;  // whole workgroup
+    cx_fill_angles(sc, nodes);
+    __syncthreads();
+    const double dt = S->dt;
+    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const int64_t c = list[i];
+        const double *s = prim1 + 8 * c;
+        const double *h = neutrals + 5 * c;
+        const double vp[3] = {s[1], s[2], s[3]}, vh[3] = {h[2], h[3], h[4]};
+        double o[5];
+        cx_source_block(s[0], s[4] / s[0], vp, h[0], h[1], vh, nodes, nodes, sc, red, o);
+        if (threadIdx.x == 0) {
+            const double *w = stash + 11 * i;
+            double Pf[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) Pf[k] = w[k];
+            cell_update_one(M, M.cell_code[c], c, prim1, w[9], w[10], Pf, o, dt, S, prim2);
+        }
+        __syncthreads();
+    }
+}
+
+void launch_cx_fixup(hipStream_t s, const DevMesh &M, const double *prim1, const double *neutrals, int nodes,
+                     const int32_t *list, const double *stash, int64_t cap, StepScalars *S, double *prim2) {
+    const size_t lds = sizeof(double) * (2 * (size_t)nodes + 3 * 256);
+    hipLaunchKernelGGL(k_cx_fixup, dim3(256), dim3(256), lds, s, M, prim1, neutrals, nodes, list, stash, cap, S, prim2);
 }
 
 void launch_cx_exact(hipStream_t s, int64_t n, const double *in10, int nodes, double *out5) {

@@ -1,12 +1,5 @@
 // k_mlp.hip -- charge-exchange surrogate: BF16 MFMA inference (FP32 accumulation) and FP32
-// training kernels (forward, backward, Adam).
-//
-// Inference maps one wavefront onto v_mfma_f32_32x32x16_bf16 with the network transposed:
-// D[32 features x 32 cells] = W[32 x 16] . H^T[16 x 32 cells].  Lane l supplies W row l%32,
-// k = 8*(l/32)..+7, and cell l%32's features k = 8*(l/32)..+7; it receives features
-// 8*(i/4) + 4*(l/32) + i%4 (i = 0..15) of cell l%32.  The rows of every hidden layer are stored
-// permuted so that accumulator registers 8*(kb%2)..+7 of tile kb/2 ARE the next layer's B operand
-// for k-block kb: activations never leave the registers between layers.
+// training kernels (forward, backward, Adam).  The wavefront forward pass lives in mlp_device.h.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,11 +11,9 @@
 #include "../../include/titan_gpu.h"
 #include "cx_device.h"
 #include "mlp.h"
+#include "mlp_device.h"
 
 namespace titan {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 #define HIPCHK_M(expr)                                                                        \
     do {                                                                                      \
@@ -32,72 +23,6 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
             return TITAN_E_HIP;                                                               \
         }                                                                                     \
     } while (0)
-
-// parameter offsets (row-major W[out][in], then b[out])
-constexpr int oW1 = 0, oB1 = oW1 + 64 * 8, oW2 = oB1 + 64, oB2 = oW2 + 64 * 64, oW3 = oB2 + 64, oB3 = oW3 + 64 * 64,
-              oW4 = oB3 + 64, oB4 = oW4 + 5 * 64;
-// packed operand bases (in lane-vectors) and bias bases
-constexpr int vL1 = 0, vL2 = 128, vL3 = 640, vL4 = 1152;
-constexpr int bL1 = 0, bL2 = 64, bL3 = 128, bL4 = 192;
-
-// logical hidden feature held by (tile t, accumulator register i, lane half h)
-__host__ __device__ inline int hidden_logical(int t, int i, int h) { return 16 * (2 * t + i / 8) + 8 * h + (i % 8); }
-// physical row of (register i, lane half h) inside a 32-row tile
-__host__ __device__ inline int tile_row(int i, int h) { return (i / 4) * 8 + h * 4 + (i % 4); }
-
-__device__ __forceinline__ float silu_f(float x) { return x / (1.0f + __expf(-x)); }
-
-__device__ __forceinline__ void act_to_b(const f32x16 &a, const f32x16 &b, bf16x8 B[4]) {
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-        B[0][e] = (__bf16)silu_f(a[e]);
-        B[1][e] = (__bf16)silu_f(a[8 + e]);
-        B[2][e] = (__bf16)silu_f(b[e]);
-        B[3][e] = (__bf16)silu_f(b[8 + e]);
-    }
-}
-
-__device__ __forceinline__ f32x16 bias_init(const float *bias, int base, int t, int h) {
-    f32x16 acc;
-#pragma unroll
-    for (int i = 0; i < 16; i++) acc[i] = bias[base + (t * 2 + h) * 16 + i];
-    return acc;
-}
-
-// One 32-cell pass of the network.  W: packed lane-vectors (LDS), bias: packed biases (LDS),
-// in: this lane's B operand of layer 1.  Returns the final accumulator (5 outputs: registers
-// 0..3 of half 0 and register 0 of half 1).
-__device__ __forceinline__ f32x16 mlp_pass(const bf16x8 *W, const float *bias, bf16x8 in, int lane) {
-    const int h = lane >> 5;
-    f32x16 a0 = bias_init(bias, bL1, 0, h), a1 = bias_init(bias, bL1, 1, h);
-    a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[vL1 + 0 * 64 + lane], in, a0, 0, 0, 0);
-    a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[vL1 + 1 * 64 + lane], in, a1, 0, 0, 0);
-    bf16x8 B[4];
-    act_to_b(a0, a1, B);
-#pragma unroll
-    for (int layer = 0; layer < 2; layer++) {
-        const int vb = layer == 0 ? vL2 : vL3, bb = layer == 0 ? bL2 : bL3;
-        f32x16 c0 = bias_init(bias, bb, 0, h), c1 = bias_init(bias, bb, 1, h);
-#pragma unroll
-        for (int kb = 0; kb < 4; kb++) {
-            c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[vb + (0 * 4 + kb) * 64 + lane], B[kb], c0, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[vb + (1 * 4 + kb) * 64 + lane], B[kb], c1, 0, 0, 0);
-        }
-        act_to_b(c0, c1, B);
-    }
-    f32x16 o = bias_init(bias, bL4, 0, h);
-#pragma unroll
-    for (int kb = 0; kb < 4; kb++) o = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[vL4 + kb * 64 + lane], B[kb], o, 0, 0, 0);
-    return o;
-}
-
-__device__ __forceinline__ void load_packed(const uint16_t *__restrict__ wpack, const float *__restrict__ bpack,
-                                            bf16x8 *sW, float *sB) {
-    const uint4 *src = reinterpret_cast<const uint4 *>(wpack);
-    uint4 *dst = reinterpret_cast<uint4 *>(sW);
-    for (int i = threadIdx.x; i < kMlpPackVec; i += blockDim.x) dst[i] = src[i];
-    for (int i = threadIdx.x; i < kMlpPackBias; i += blockDim.x) sB[i] = bpack[i];
-}
 
 // Gathers a pass's outputs: after the pass for cells 32p..32p+31, lane l < 32 holds outputs 0..3
 // of cell 32p+l (registers 0..3) and lane l >= 32 holds output 4 of cell 32p+l-32 (register 0).
@@ -172,16 +97,8 @@ __global__ void __launch_bounds__(256) k_mlp_cells(int64_t nc, const double *__r
         np = s[0];
         nh = hn[0];
         vp[0] = s[1]; vp[1] = s[2]; vp[2] = s[3];
-        const double d0 = s[1] - hn[2], d1 = s[2] - hn[3], d2 = s[3] - hn[4];
-        xr[0] = (float)log(np);
-        xr[1] = (float)log(s[4] / s[0]);
-        xr[2] = (float)log(nh);
-        xr[3] = (float)log(hn[1]);
-        xr[4] = (float)d0; xr[5] = (float)d1; xr[6] = (float)d2;
-        xr[7] = (float)sqrt((d0 * d0 + d1 * d1) + d2 * d2);
-        inrange = true;
-#pragma unroll
-        for (int e = 0; e < 8; e++) inrange = inrange && (xr[e] >= sN[21 + e]) && (xr[e] <= sN[29 + e]);
+        mlp_features(s, hn, xr);
+        inrange = mlp_in_box(xr, sN);
     } else {
 #pragma unroll
         for (int e = 0; e < 8; e++) xr[e] = 0.0f;
@@ -194,8 +111,7 @@ __global__ void __launch_bounds__(256) k_mlp_cells(int64_t nc, const double *__r
 #pragma unroll
         for (int e = 0; e < 8; e++) {
             const float v = __shfl(xr[e], src_lane, 64);
-            const float xn = (v - sN[e]) / sN[8 + e];
-            in[e] = (lane < 32) ? (__bf16)xn : (__bf16)0.0f;
+            in[e] = (lane < 32) ? mlp_normalise(v, sN, e) : (__bf16)0.0f;
         }
         const f32x16 o = mlp_pass(sW, sB, in, lane);
         stash_outputs(o, lane, pass, sOut[wave]);
@@ -203,17 +119,13 @@ __global__ void __launch_bounds__(256) k_mlp_cells(int64_t nc, const double *__r
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     if (mycell >= nc) return;
-    const double nn = np * nh;
-    double y[5];
+    float y[5];
 #pragma unroll
-    for (int k = 0; k < 5; k++) y[k] = (double)sOut[wave][lane][k] * (double)sN[16 + k];
-    double* o = src + 5 * mycell;
-    o[0] = nn * y[0];
-    o[1] = nn * y[1];
-    o[2] = nn * y[2];
-    o[3] = nn * y[3];
-    // the network predicts the energy source in the plasma frame; S_E(lab) = S_E' + V_p . S_m
-    o[4] = nn * y[4] + (vp[0] * o[1] + vp[1] * o[2] + vp[2] * o[3]);
+    for (int k = 0; k < 5; k++) y[k] = sOut[wave][lane][k];
+    double o[5];
+    mlp_source(y, sN, np, nh, vp, o);
+#pragma unroll
+    for (int k = 0; k < 5; k++) src[5 * mycell + k] = o[k];
     if (!inrange) {
         const unsigned long long k = atomicAdd(count, 1ull);
         atomicAdd(fallback_total, 1ull);
@@ -382,7 +294,8 @@ static void pack_params(const float *P, std::vector<uint16_t> &wv, std::vector<f
 }
 
 void mlp_free(MlpState &s) {
-    void *ptrs[] = {s.d_params, s.d_grad, s.d_m, s.d_v, s.d_wpack, s.d_bpack, s.d_norm, s.d_list, s.d_count, s.d_ws};
+    void *ptrs[] = {s.d_params, s.d_grad, s.d_m, s.d_v, s.d_wpack, s.d_bpack, s.d_norm, s.d_list, s.d_count, s.d_ws,
+                    s.d_stash};
     for (void *p : ptrs)
         if (p) hipFree(p);
     s = MlpState();
@@ -603,15 +516,39 @@ int mlp_train_host(MlpState &s, hipStream_t st, int64_t n, const float *x, const
     return rc;
 }
 
+static int ensure_list(MlpState &s, int64_t nc) {
+    if (s.list_cap < nc) {
+        if (s.d_list) hipFree(s.d_list);
+        if (s.d_stash) hipFree(s.d_stash);
+        s.d_list = nullptr;
+        s.d_stash = nullptr;
+        s.list_cap = 0;
+        HIPCHK_M(hipMalloc((void **)&s.d_list, sizeof(int32_t) * nc));
+        HIPCHK_M(hipMalloc((void **)&s.d_stash, sizeof(double) * 11 * nc));
+        s.list_cap = nc;
+    }
+    return TITAN_OK;
+}
+
+int mlp_fused_args(MlpState &s, int64_t nc, const double *neutrals, SurArgs *a) {
+    if (!s.ready) return TITAN_E_NOT_TRAINED;
+    const int rc = ensure_list(s, nc);
+    if (rc != TITAN_OK) return rc;
+    a->wpack = s.d_wpack;
+    a->bpack = s.d_bpack;
+    a->norm = s.d_norm;
+    a->neutrals = neutrals;
+    a->list = s.d_list;
+    a->stash = s.d_stash;
+    a->cap = s.list_cap;
+    return TITAN_OK;
+}
+
 int mlp_cell_sources(MlpState &s, hipStream_t st, int64_t nc, const double *prim, const double *neutrals, int nodes,
                      double *src, unsigned long long *fallback_total) {
     if (!s.ready) return TITAN_E_NOT_TRAINED;
-    if (s.list_cap < nc) {
-        if (s.d_list) hipFree(s.d_list);
-        s.d_list = nullptr;
-        HIPCHK_M(hipMalloc((void **)&s.d_list, sizeof(int32_t) * nc));
-        s.list_cap = nc;
-    }
+    const int rc = ensure_list(s, nc);
+    if (rc != TITAN_OK) return rc;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, s.d_count);
     const unsigned grid = (unsigned)((nc + 255) / 256);
     hipLaunchKernelGGL(k_mlp_cells, dim3(grid), dim3(256), 0, st, nc, prim, neutrals, s.d_wpack, s.d_bpack, s.d_norm, src,

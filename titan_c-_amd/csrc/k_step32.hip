@@ -18,18 +18,13 @@ void launch_face_flux_f32(hipStream_t s, const DevMesh &M, const DevParams &P, c
 
 void launch_step_tile_f32(hipStream_t s, const TileMesh &TM, const DevMesh &M, const DevParams &P,
                           const double *prim1, double *prim2, const double *X1, double *X2, const double *disp,
-                          const double *cc1, double *cc2, const double *src, StepScalars *S, long long step) {
-    const size_t lds = tile_lds_bytes(TM.max_nodes, TM.tile_cells, TM.max_faces, P.move_mesh != 0);
-    static size_t attr_bytes = 0;  // dynamic LDS size the kernel has been opened up to
-    if (lds > attr_bytes) {
-        if (hipFuncSetAttribute((const void *)k_step_tile<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) ==
-            hipSuccess)
-            attr_bytes = lds;
-        else
-            (void)hipGetLastError();  // the launch below then reports the failure
-    }
-    hipLaunchKernelGGL(k_step_tile<float>, dim3((unsigned)TM.n_tiles), dim3(kTileThreads), lds, s, TM, M, P, prim1,
-                       prim2, X1, X2, disp, cc1, cc2, src, S, step);
+                          const double *cc1, double *cc2, const double *src, const SurArgs *sur, StepScalars *S,
+                          long long step) {
+    if (sur)
+        launch_tile_variant<float, true>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, nullptr, *sur, S, step);
+    else
+        launch_tile_variant<float, false>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, src, SurArgs{}, S,
+                                          step);
 }
 
 template <int K>

@@ -45,22 +45,17 @@ void launch_cell_update(hipStream_t s, const DevMesh &M, const double *prim_old,
 
 void launch_step_tile(hipStream_t s, int precision, const TileMesh &TM, const DevMesh &M, const DevParams &P,
                       const double *prim1, double *prim2, const double *X1, double *X2, const double *disp,
-                      const double *cc1, double *cc2, const double *src, StepScalars *S, long long step) {
+                      const double *cc1, double *cc2, const double *src, const SurArgs *sur, StepScalars *S,
+                      long long step) {
     if (precision == 1) {
-        launch_step_tile_f32(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, src, S, step);
+        launch_step_tile_f32(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, src, sur, S, step);
         return;
     }
-    const size_t lds = tile_lds_bytes(TM.max_nodes, TM.tile_cells, TM.max_faces, P.move_mesh != 0);
-    static size_t attr_bytes = 0;  // dynamic LDS size the kernel has been opened up to
-    if (lds > attr_bytes) {
-        if (hipFuncSetAttribute((const void *)k_step_tile<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) ==
-            hipSuccess)
-            attr_bytes = lds;
-        else
-            (void)hipGetLastError();  // the launch below then reports the failure
-    }
-    hipLaunchKernelGGL(k_step_tile<double>, dim3((unsigned)TM.n_tiles), dim3(kTileThreads), lds, s, TM, M, P, prim1,
-                       prim2, X1, X2, disp, cc1, cc2, src, S, step);
+    if (sur)
+        launch_tile_variant<double, true>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, nullptr, *sur, S, step);
+    else
+        launch_tile_variant<double, false>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, src, SurArgs{}, S,
+                                           step);
 }
 
 size_t step_tile_lds_bytes(const TileMesh &TM, bool moving) {

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include "device_mesh.h"
+#include "mlp.h"
 
 namespace titan {
 
@@ -20,13 +21,16 @@ void launch_cell_update(hipStream_t s, const DevMesh &M, const double *prim_old,
                         const double *vol_new, const double *flux, const double *src, StepScalars *S,
                         double *prim_new, double dt_override);
 
-// fused step over tiles (tile_kernel.h); precision = TITAN_PREC_*
+// fused step over tiles (tile_kernel.h); precision = TITAN_PREC_*.  sur (nullable): surrogate
+// source fused into the update (then src is ignored and launch_cx_fixup must follow).
 void launch_step_tile(hipStream_t s, int precision, const TileMesh &TM, const DevMesh &M, const DevParams &P,
                       const double *prim1, double *prim2, const double *X1, double *X2, const double *disp,
-                      const double *cc1, double *cc2, const double *src, StepScalars *S, long long step);
+                      const double *cc1, double *cc2, const double *src, const SurArgs *sur, StepScalars *S,
+                      long long step);
 void launch_step_tile_f32(hipStream_t s, const TileMesh &TM, const DevMesh &M, const DevParams &P,
                           const double *prim1, double *prim2, const double *X1, double *X2, const double *disp,
-                          const double *cc1, double *cc2, const double *src, StepScalars *S, long long step);
+                          const double *cc1, double *cc2, const double *src, const SurArgs *sur, StepScalars *S,
+                          long long step);
 size_t step_tile_lds_bytes(const TileMesh &TM, bool moving);
 
 // chlld over n independent states; err (device int) set to 1 on a fatal branch
@@ -44,5 +48,8 @@ void launch_cx_exact(hipStream_t s, int64_t n, const double *in10, int nodes, do
 // per-cell exact source from the plasma state and the neutral moments [Nc*5]
 void launch_cx_cells(hipStream_t s, int64_t nc, const double *prim, const double *neutrals, int nodes,
                      double *src);
+// exact source + update of the cells the surrogate step listed as out of its training box
+void launch_cx_fixup(hipStream_t s, const DevMesh &M, const double *prim1, const double *neutrals, int nodes,
+                     const int32_t *list, const double *stash, int64_t cap, StepScalars *S, double *prim2);
 
 }  // namespace titan

@@ -14,6 +14,17 @@ constexpr int kMlpPackBias = 224;
 // normalisation block: in_mean[8], in_std[8], out_scale[5], box_lo[8], box_hi[8]
 constexpr int kMlpNorm = 37;
 
+// Inputs of the surrogate fused into the step kernel (device pointers, device cell order).
+struct SurArgs {
+    const uint16_t *wpack;   // packed BF16 operands
+    const float *bpack;      // packed biases
+    const float *norm;       // normalisation block
+    const double *neutrals;  // [nc*5]: n_H, c_H^2, V_H
+    int32_t *list;           // cells outside the training box this step (slot from S->n_fallback)
+    double *stash;           // [cap*11]: flux sum Pf[9], V(now1), V(now2) of each listed cell
+    int64_t cap;
+};
+
 struct MlpState {
     bool ready = false;
     float *d_params = nullptr, *d_grad = nullptr, *d_m = nullptr, *d_v = nullptr;
@@ -23,6 +34,7 @@ struct MlpState {
     float h_norm[kMlpNorm] = {};
     int32_t *d_list = nullptr;
     int64_t list_cap = 0;
+    double *d_stash = nullptr;  // fused-step fallback stash, list_cap * 11 doubles
     unsigned long long *d_count = nullptr;
     float *d_ws = nullptr;
     size_t ws_floats = 0;
@@ -42,6 +54,10 @@ int mlp_train_host(MlpState &s, hipStream_t st, int64_t n, const float *x, const
 // exact source.  fallback_total (device) accumulates the number of such cells.
 int mlp_cell_sources(MlpState &s, hipStream_t st, int64_t nc, const double *prim, const double *neutrals, int nodes,
                      double *src, unsigned long long *fallback_total);
+
+// Device pointers of the surrogate fused into the step kernel; allocates the fallback list and
+// stash (capacity nc) on first use.
+int mlp_fused_args(MlpState &s, int64_t nc, const double *neutrals, SurArgs *a);
 
 void launch_cx_list(hipStream_t s, const int32_t *list, const unsigned long long *count, int64_t cap,
                     const double *prim, const double *neutrals, int nodes, double *src);

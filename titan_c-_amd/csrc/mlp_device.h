@@ -1,0 +1,135 @@
+// mlp_device.h -- device side of the charge-exchange surrogate 8-64-64-64-5 (SiLU): the BF16 MFMA
+// forward pass of one wavefront (32 cells per pass, FP32 accumulation), the input features of a
+// cell, the training-box test and the source assembly.  Shared by the standalone surrogate
+// kernels and the fused step kernel, and written without contractible a*b+c patterns and with
+// explicit reciprocals, so every translation unit (whatever its -ffp-contract and division
+// flags) produces the same bits.
+//
+// Inference maps one wavefront onto v_mfma_f32_32x32x16_bf16 with the network transposed:
+// D[32 features x 32 cells] = W[32 x 16] . H^T[16 x 32 cells].  Lane l supplies W row l%32,
+// k = 8*(l/32)..+7, and cell l%32's features k = 8*(l/32)..+7; it receives features
+// 8*(i/4) + 4*(l/32) + i%4 (i = 0..15) of cell l%32.  The rows of every hidden layer are stored
+// permuted so that accumulator registers 8*(kb%2)..+7 of tile kb/2 ARE the next layer's B operand
+// for k-block kb: activations never leave the registers between layers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mlp.h"
+
+namespace titan {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// parameter offsets (row-major W[out][in], then b[out])
+constexpr int oW1 = 0, oB1 = oW1 + 64 * 8, oW2 = oB1 + 64, oB2 = oW2 + 64 * 64, oW3 = oB2 + 64, oB3 = oW3 + 64 * 64,
+              oW4 = oB3 + 64, oB4 = oW4 + 5 * 64;
+// packed operand bases (in lane-vectors) and bias bases
+constexpr int vL1 = 0, vL2 = 128, vL3 = 640, vL4 = 1152;
+constexpr int bL1 = 0, bL2 = 64, bL3 = 128, bL4 = 192;
+// bytes of the packed weights + biases + normalisation block when staged in LDS
+constexpr int kMlpLdsBytes = kMlpPackVec * 16 + kMlpPackBias * 4 + kMlpNorm * 4;
+// normalisation block offsets
+constexpr int kNormMean = 0, kNormStd = 8, kNormScale = 16, kNormLo = 21, kNormHi = 29;
+
+// logical hidden feature held by (tile t, accumulator register i, lane half h)
+__host__ __device__ inline int hidden_logical(int t, int i, int h) { return 16 * (2 * t + i / 8) + 8 * h + (i % 8); }
+// physical row of (register i, lane half h) inside a 32-row tile
+__host__ __device__ inline int tile_row(int i, int h) { return (i / 4) * 8 + h * 4 + (i % 4); }
+
+__device__ __forceinline__ float silu_f(float x) { return x * __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
+
+__device__ __forceinline__ void act_to_b(const f32x16 &a, const f32x16 &b, bf16x8 B[4]) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        B[0][e] = (__bf16)silu_f(a[e]);
+        B[1][e] = (__bf16)silu_f(a[8 + e]);
+        B[2][e] = (__bf16)silu_f(b[e]);
+        B[3][e] = (__bf16)silu_f(b[8 + e]);
+    }
+}
+
+__device__ __forceinline__ f32x16 bias_init(const float *bias, int base, int t, int h) {
+    f32x16 acc;
+#pragma unroll
+    for (int i = 0; i < 16; i++) acc[i] = bias[base + (t * 2 + h) * 16 + i];
+    return acc;
+}
+
+// One 32-cell pass of the network (all 64 lanes active).  W: packed lane-vectors, bias: packed
+// biases, in: this lane's B operand of layer 1.  Returns the final accumulator (5 outputs:
+// registers 0..3 of half 0 and register 0 of half 1).
+__device__ __forceinline__ f32x16 mlp_pass(const bf16x8 *W, const float *bias, bf16x8 in, int lane) {
+    const int h = lane >> 5;
+    f32x16 a0 = bias_init(bias, bL1, 0, h), a1 = bias_init(bias, bL1, 1, h);
+    a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[vL1 + 0 * 64 + lane], in, a0, 0, 0, 0);
+    a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[vL1 + 1 * 64 + lane], in, a1, 0, 0, 0);
+    bf16x8 B[4];
+    act_to_b(a0, a1, B);
+#pragma unroll
+    for (int layer = 0; layer < 2; layer++) {
+        const int vb = layer == 0 ? vL2 : vL3, bb = layer == 0 ? bL2 : bL3;
+        f32x16 c0 = bias_init(bias, bb, 0, h), c1 = bias_init(bias, bb, 1, h);
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++) {
+            c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[vb + (0 * 4 + kb) * 64 + lane], B[kb], c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[vb + (1 * 4 + kb) * 64 + lane], B[kb], c1, 0, 0, 0);
+        }
+        act_to_b(c0, c1, B);
+    }
+    f32x16 o = bias_init(bias, bL4, 0, h);
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) o = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[vL4 + kb * 64 + lane], B[kb], o, 0, 0, 0);
+    return o;
+}
+
+// Block-cooperative copy of the packed operands (and optionally the normalisation block) into LDS.
+__device__ __forceinline__ void load_packed(const uint16_t *__restrict__ wpack, const float *__restrict__ bpack,
+                                            bf16x8 *sW, float *sB) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(wpack);
+    uint4 *dst = reinterpret_cast<uint4 *>(sW);
+    for (int i = threadIdx.x; i < kMlpPackVec; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < kMlpPackBias; i += blockDim.x) sB[i] = bpack[i];
+}
+
+// Raw input features of a cell: ln n_p, ln c_p^2 (= p/rho), ln n_H, ln c_H^2, V_p - V_H, |V_p - V_H|.
+// s = (rho, Vx, Vy, Vz, p, ...), hn = (n_H, c_H^2, V_H).
+__device__ __forceinline__ void mlp_features(const double *s, const double *hn, float xr[8]) {
+    const double d0 = s[1] - hn[2], d1 = s[2] - hn[3], d2 = s[3] - hn[4];
+    xr[0] = (float)log(s[0]);
+    xr[1] = (float)log(s[4] / s[0]);
+    xr[2] = (float)log(hn[0]);
+    xr[3] = (float)log(hn[1]);
+    xr[4] = (float)d0;
+    xr[5] = (float)d1;
+    xr[6] = (float)d2;
+    xr[7] = (float)sqrt(__builtin_fma(d0, d0, __builtin_fma(d1, d1, d2 * d2)));
+}
+
+__device__ __forceinline__ bool mlp_in_box(const float xr[8], const float *norm) {
+    bool in = true;
+#pragma unroll
+    for (int e = 0; e < 8; e++) in = in && (xr[e] >= norm[kNormLo + e]) && (xr[e] <= norm[kNormHi + e]);
+    return in;
+}
+
+__device__ __forceinline__ __bf16 mlp_normalise(float v, const float *norm, int e) {
+    return (__bf16)((v - norm[kNormMean + e]) * __builtin_amdgcn_rcpf(norm[kNormStd + e]));
+}
+
+// Source of one cell from the network's 5 outputs y (per n_p n_H, normalised): rate, S_m, and
+// the plasma-frame energy source; S_E(lab) = S_E' + V_p . S_m.
+__device__ __forceinline__ void mlp_source(const float y[5], const float *norm, double n_p, double n_h,
+                                           const double vp[3], double o[5]) {
+    const double nn = n_p * n_h;
+    double yy[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) yy[k] = (double)y[k] * (double)norm[kNormScale + k];
+    o[0] = nn * yy[0];
+    o[1] = nn * yy[1];
+    o[2] = nn * yy[2];
+    o[3] = nn * yy[3];
+    o[4] = __builtin_fma(vp[2], o[3], __builtin_fma(vp[1], o[2], __builtin_fma(vp[0], o[1], nn * yy[4])));
+}
+
+}  // namespace titan

@@ -446,11 +446,11 @@ __global__ void __launch_bounds__(256) k_face_flux(DevMesh M, DevParams P, const
 }
 
 // ---- cell update: cell loop of Setka::Go (Setka_physics.cpp:1695-2027), FP64 -------------------
-// Pf = signed flux sum of the cell (8 fluxes x area, PdivB); src (nullable) [Nc*5]: rate, S_m (3),
-// S_E per unit volume and time.
+// Pf = signed flux sum of the cell (8 fluxes x area, PdivB); src (nullable): this cell's 5 source
+// values -- rate, S_m (3), S_E per unit volume and time.
 __device__ __forceinline__ void cell_update_one(const DevMesh &M, uint32_t code, int64_t c,
                                                 const double *__restrict__ prim_old, double Volume, double Volume2,
-                                                const double Pf[9], const double *__restrict__ src, double time,
+                                                const double Pf[9], const double *src, double time,
                                                 StepScalars *S, double *__restrict__ prim_new) {
     const double cpi4 = 4.0 * kPi;
     const double g1 = 5.0 / 3.0 - 1.0;
@@ -458,7 +458,7 @@ __device__ __forceinline__ void cell_update_one(const DevMesh &M, uint32_t code,
     load_state(prim_old, c, s);
     const double rho = s[0], vx = s[1], vy = s[2], vz = s[3], pp = s[4], bx = s[5], by = s[6], bz = s[7];
     double Smx = 0.0, Smy = 0.0, Smz = 0.0, SE = 0.0;
-    if (src) { Smx = src[5 * c + 1]; Smy = src[5 * c + 2]; Smz = src[5 * c + 3]; SE = src[5 * c + 4]; }
+    if (src) { Smx = src[1]; Smy = src[2]; Smz = src[3]; SE = src[4]; }
     const double dsk = spf(vx, vy, vz, bx, by, bz);
     double rho3 = rho * Volume / Volume2 - time * Pf[0] / Volume2;
     bool rho_null = false;
@@ -522,7 +522,7 @@ __global__ void __launch_bounds__(256) k_cell_update(DevMesh M, const double *__
             Pf[i] += pos ? v : -v;
         }
     }
-    cell_update_one(M, code, c, prim_old, vol_old[c], vol_new[c], Pf, src, time, S, prim_new);
+    cell_update_one(M, code, c, prim_old, vol_old[c], vol_new[c], Pf, src ? src + 5 * c : nullptr, time, S, prim_new);
 }
 #endif  // TITAN_FLUX_ONLY
 

@@ -16,10 +16,10 @@
 // flux ever leaves the workgroup and no float atomics are used, so runs repeat bit for bit.
 #pragma once
 #include "step_kernels.h"
+#include "mlp_device.h"
 
 namespace titan {
 
-constexpr int kTileThreads = 256;
 
 template <typename Z> struct Trig { Z st, ct, sp, cp; };
 
@@ -179,9 +179,15 @@ struct TileLds {
     double *x1, *x2;   // [max_nodes*3]
     double *c1, *c2;   // [T*3]
     double *z1;        // [T*10] Zone_1 cell data (r1, r1^2g, trig e, trig p)
-    double *flux;      // [max_faces*9]
+    double *flux;      // [tile_flux_doubles(max_faces)]; in the surrogate step, after the flux
+                       // sums are taken, the same slots hold the network (kMlpLdsBytes)
     uint16_t *fnodes;  // [max_faces*4] local nodes of each face slot
 };
+
+__host__ __device__ inline size_t tile_flux_doubles(int max_faces) {
+    const size_t a = 9 * (size_t)max_faces, b = (kMlpLdsBytes + 7) / 8;
+    return a > b ? a : b;
+}
 
 // LDS carve-up; the second node layer and second centres exist only on a moving mesh.
 __device__ __forceinline__ TileLds tile_lds(double *base, const TileMesh &TM, bool moving) {
@@ -190,7 +196,8 @@ __device__ __forceinline__ TileLds tile_lds(double *base, const TileMesh &TM, bo
     L.x1 = p; p += 3 * TM.max_nodes;
     L.c1 = p; p += 3 * TM.tile_cells;
     L.z1 = p; p += 10 * TM.tile_cells;
-    L.flux = p; p += 9 * TM.max_faces;
+    p += (p - base) & 1;  // 16-byte alignment of the flux slots (they also hold 16-byte vectors)
+    L.flux = p; p += tile_flux_doubles(TM.max_faces);
     if (moving) {
         L.x2 = p; p += 3 * TM.max_nodes;
         L.c2 = p; p += 3 * TM.tile_cells;
@@ -203,7 +210,7 @@ __device__ __forceinline__ TileLds tile_lds(double *base, const TileMesh &TM, bo
 }
 
 inline size_t tile_lds_bytes(int max_nodes, int tile_cells, int max_faces, bool moving) {
-    const size_t nd = 3 * (size_t)max_nodes + 13 * (size_t)tile_cells + 9 * (size_t)max_faces +
+    const size_t nd = 3 * (size_t)max_nodes + 13 * (size_t)tile_cells + 1 + tile_flux_doubles(max_faces) +
                       (moving ? 3 * (size_t)max_nodes + 3 * (size_t)tile_cells : 0);
     return sizeof(double) * nd + sizeof(uint16_t) * 4 * (size_t)max_faces;
 }
@@ -251,13 +258,103 @@ __device__ __forceinline__ double tile_volume(const double *X, const double cc[3
     return gcl ? V / 3.0 : V;
 }
 
+// Phase 4 of the surrogate step.  Lane l < 32 of wavefront w owns cell 32w + l of the tile (tile
+// cells <= 32 * waves); lanes l and l + 32 run the network pass of that cell together.
 template <typename RT>
+__device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMesh &M, const DevParams &P,
+                                                 const TileLds &L, int64_t c0, int ncl,
+                                                 const double *__restrict__ prim1, double *__restrict__ prim2,
+                                                 const SurArgs &sur, double dt, int gcl, bool moving,
+                                                 StepScalars *S) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cl = 32 * wave + (lane & 31);
+    const bool mine = lane < 32 && cl < ncl;
+    const int64_t cg = c0 + cl;
+    uint32_t code = 0;
+    double Pf[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) Pf[i] = 0.0;
+    if (mine) {
+        code = M.cell_code[cg];
+        const uint16_t *lf = TM.cell_lfaces + 6 * cg;
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            const bool pos = (code >> j) & 1u;
+            const double *fl = L.flux + 9 * lf[j];
+#pragma unroll
+            for (int i = 0; i < 9; i++) {
+                const double v = fl[i];
+                Pf[i] += pos ? v : -v;
+            }
+        }
+    }
+    __syncthreads();  // every flux slot has been read: the slots now take the network
+    bf16x8 *sW = reinterpret_cast<bf16x8 *>(L.flux);
+    float *sB = reinterpret_cast<float *>(sW + kMlpPackVec);
+    float *sN = sB + kMlpPackBias;
+    load_packed(sur.wpack, sur.bpack, sW, sB);
+    for (int i = threadIdx.x; i < kMlpNorm; i += blockDim.x) sN[i] = sur.norm[i];
+    __syncthreads();
+    if (32 * wave >= ncl) return;  // wavefront-uniform: no cell of this wavefront
+    double st[8] = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0}, hn[5] = {1.0, 1.0, 0.0, 0.0, 0.0};
+    float xr[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) xr[e] = 0.0f;
+    bool inbox = false;
+    if (mine) {
+        load_state(prim1, cg, st);
+#pragma unroll
+        for (int k = 0; k < 5; k++) hn[k] = sur.neutrals[5 * cg + k];
+        mlp_features(st, hn, xr);
+        inbox = mlp_in_box(xr, sN);
+    }
+    bf16x8 in;
+#pragma unroll
+    for (int e = 0; e < 8; e++) in[e] = (lane < 32) ? mlp_normalise(xr[e], sN, e) : (__bf16)0.0f;
+    const f32x16 o = mlp_pass(sW, sB, in, lane);
+    const float y4 = __shfl(o[0], (lane & 31) + 32, 64);  // output 4 of this cell sits in lane l + 32
+    if (!mine) return;
+    const uint16_t *lf = TM.cell_lfaces + 6 * cg;
+    const double cA[3] = {L.c1[3 * cl], L.c1[3 * cl + 1], L.c1[3 * cl + 2]};
+    const double Volume = tile_volume(L.x1, cA, L.fnodes, lf, code, gcl);
+    double Volume2 = Volume;
+    if (moving) {
+        const double cB[3] = {L.c2[3 * cl], L.c2[3 * cl + 1], L.c2[3 * cl + 2]};
+        Volume2 = tile_volume(L.x2, cB, L.fnodes, lf, code, gcl);
+    }
+    if (inbox) {
+        const float y[5] = {o[0], o[1], o[2], o[3], y4};
+        const double vp[3] = {st[1], st[2], st[3]};
+        double s5[5];
+        mlp_source(y, sN, st[0], hn[0], vp, s5);
+        cell_update_one(M, code, cg, prim1, Volume, Volume2, Pf, s5, dt, S, prim2);
+    } else {
+        const unsigned long long k = atomicAdd(&S->n_fallback, 1ull);
+        if ((int64_t)k < sur.cap) {
+            sur.list[k] = (int32_t)cg;
+            double *w = sur.stash + 11 * (int64_t)k;
+#pragma unroll
+            for (int i = 0; i < 9; i++) w[i] = Pf[i];
+            w[9] = Volume;
+            w[10] = Volume2;
+        } else {
+            report_error(S, -1, M.cell_ref[cg]);
+        }
+    }
+}
+
+// SUR = true: the charge-exchange source comes from the surrogate network, evaluated in phase 4
+// on the matrix cores (one 32-cell MFMA pass per wavefront, weights staged in the flux slots once
+// every flux sum is taken).  Cells outside the training box are not updated here: their flux
+// sums and volumes go to sur.stash at a slot of S->n_fallback, and k_cx_fixup redoes their update
+// with the exact source.  SUR = false: src (nullable) holds a precomputed source per cell.
+template <typename RT, bool SUR>
 __global__ void __launch_bounds__(kTileThreads, 2)
     k_step_tile(TileMesh TM, DevMesh M, DevParams P, const double *__restrict__ prim1, double *__restrict__ prim2,
                 const double *__restrict__ X1g, double *__restrict__ X2g, const double *__restrict__ disp,
                 const double *__restrict__ cc1g, double *__restrict__ cc2g, const double *__restrict__ src,
-                StepScalars *S, long long step) {
-    extern __shared__ double smem[];
+                SurArgs sur, StepScalars *S, long long step) {
+    extern __shared__ __align__(16) double smem[];
     const TileLds L = tile_lds(smem, TM, P.move_mesh != 0);
     const int t = blockIdx.x;
     const int T = TM.tile_cells;
@@ -427,6 +524,12 @@ __global__ void __launch_bounds__(kTileThreads, 2)
     }
     __syncthreads();
 
+    if constexpr (SUR) {
+        surrogate_update<RT>(TM, M, P, L, c0, ncl, prim1, prim2, sur, dt, gcl, moving, S);
+        block_min_to(cand, &S->dtmin_bits[step & 1]);
+        return;
+    }
+
     // phase 4: own cells -- volumes, flux sum in reference order, update
     for (int c = threadIdx.x; c < ncl; c += blockDim.x) {
         const int64_t cg = c0 + c;
@@ -452,9 +555,29 @@ __global__ void __launch_bounds__(kTileThreads, 2)
                 Pf[i] += pos ? v : -v;
             }
         }
-        cell_update_one(M, code, cg, prim1, Volume, Volume2, Pf, src, dt, S, prim2);
+        cell_update_one(M, code, cg, prim1, Volume, Volume2, Pf, src ? src + 5 * cg : nullptr, dt, S, prim2);
     }
     block_min_to(cand, &S->dtmin_bits[step & 1]);
+}
+
+// Launch of one k_step_tile instantiation; opens the kernel up to the tile's dynamic LDS size
+// once (hipFuncSetAttribute is a no-op below the default limit but required above it).
+template <typename RT, bool SUR>
+inline void launch_tile_variant(hipStream_t s, const TileMesh &TM, const DevMesh &M, const DevParams &P,
+                                const double *prim1, double *prim2, const double *X1, double *X2,
+                                const double *disp, const double *cc1, double *cc2, const double *src,
+                                const SurArgs &sur, StepScalars *S, long long step) {
+    const size_t lds = tile_lds_bytes(TM.max_nodes, TM.tile_cells, TM.max_faces, P.move_mesh != 0);
+    static size_t attr_bytes = 0;  // dynamic LDS size this instantiation has been opened up to
+    if (lds > attr_bytes) {
+        if (hipFuncSetAttribute((const void *)k_step_tile<RT, SUR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds) == hipSuccess)
+            attr_bytes = lds;
+        else
+            (void)hipGetLastError();  // the launch below then reports the failure
+    }
+    hipLaunchKernelGGL((k_step_tile<RT, SUR>), dim3((unsigned)TM.n_tiles), dim3(kTileThreads), lds, s, TM, M, P,
+                       prim1, prim2, X1, X2, disp, cc1, cc2, src, sur, S, step);
 }
 
 }  // namespace titan

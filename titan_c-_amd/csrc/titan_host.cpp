@@ -510,6 +510,15 @@ extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
         c->now2 = 0;
         c->started = true;
     }
+    // surrogate source: fused into the step kernel, out-of-box cells redone by the fixup kernel
+    const bool fused = c->params.source_mode == TITAN_SRC_MLP;
+    SurArgs sur{};
+    if (fused) {
+        if (!c->d_neutrals || c->tiles.tile_cells > kTileThreads / 2) return TITAN_E_ARG;
+        const int rc = mlp_fused_args(c->mlp, c->nc, c->d_neutrals, &sur);
+        if (rc != TITAN_OK) return rc;
+    }
+    const int nodes = c->params.cx_nodes > 0 ? c->params.cx_nodes : 100;
     for (int64_t s = 0; s < steps; s++) {
         c->now2 = c->now1;
         c->now1 = (c->now1 + 1) % 2;
@@ -517,7 +526,7 @@ extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
         const int n1 = c->now1, n2 = c->now2;
         launch_begin_step(c->stream, c->d_S, c->step, c->dp);
         const double *src = nullptr;
-        if (c->params.source_mode != TITAN_SRC_NONE) {
+        if (c->params.source_mode != TITAN_SRC_NONE && !fused) {
             timer_mark(c, 3, true);
             int rc = compute_sources(c, n1, c->params.source_mode);
             timer_mark(c, 3, false);
@@ -526,9 +535,15 @@ extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
         }
         timer_mark(c, 0, true);
         launch_step_tile(c->stream, c->params.precision, c->tiles, M, c->dp, c->d_prim[n1], c->d_prim[n2], c->d_X[n1],
-                         c->d_X[n2], c->d_disp, c->geom[n1].cell_center, c->geom[n2].cell_center, src, c->d_S,
-                         c->step);
+                         c->d_X[n2], c->d_disp, c->geom[n1].cell_center, c->geom[n2].cell_center, src,
+                         fused ? &sur : nullptr, c->d_S, c->step);
         timer_mark(c, 0, false);
+        if (fused) {
+            timer_mark(c, 3, true);
+            launch_cx_fixup(c->stream, M, c->d_prim[n1], c->d_neutrals, nodes, sur.list, sur.stash, sur.cap, c->d_S,
+                            c->d_prim[n2]);
+            timer_mark(c, 3, false);
+        }
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, c->stream));
