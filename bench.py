@@ -1,16 +1,28 @@
 """Benchmark of the explicit ideal-MHD step on one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload shell4m_moving] [--no-cpu]
+                [--precision-steps 1000] [--train-states 2000000] [--exact-steps 3]
 
-Prints ONE JSON line.  `value` is cell-updates/s (N_c x steps / wall time of exactly K timed
-steps, inputs resident in HBM) of the headline variant; the other variants, the roofline of the
-step kernel (HIP-event timing on the context's stream) and the CPU baseline (the oracle's
-OpenMP FP64 step on this host's cores, bounded sample) are reported beside it.
+Prints ONE JSON line.  `value` is cell-updates/s (N_c x K / wall time of exactly K timed steps,
+inputs resident in HBM) of the plain FP64 step, the variant whose kernel the HBM roofline is
+quoted on.  Beside it:
+  variants    cell-updates/s of (i) plain FP64, FP32-mixed, (ii) FP64 + exact charge-exchange
+              source (FP64 Maxwellian quadrature every step), (iii) FP32-mixed + surrogate source
+              (MLP fused into the step kernel, exact fallback for out-of-box cells), and the
+              ratios (iii)/(ii) and (iii)/(i);
+  roofline    the step kernel of (i): algorithmic bytes per launch / its mean duration, timed with
+              HIP events around every launch on the context's stream;
+  surrogate   training-set size, epochs, loss, and its source error against the exact kernel;
+  precision   FP32-mixed vs FP64 after --precision-steps steps from the same seeded input:
+              relative L2/Linf of rho, V, p, B and the div B norms;
+  cpu_baseline  the oracle's FP64 step (OpenMP over faces and cells) on this host's cores, on a
+              bounded sample of the same workload.
 """
 import argparse
 import importlib
 import json
 import os
+import re
 import sys
 import time
 
@@ -20,6 +32,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_step_tile_fp64_%s.md")
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 def algorithmic_bytes(n_cells, n_faces, n_nodes, moving, with_neutrals=False):
@@ -36,7 +53,7 @@ def algorithmic_bytes(n_cells, n_faces, n_nodes, moving, with_neutrals=False):
     return float(b)
 
 
-def run_variant(T, mesh, prim, params, steps, warmup, neutrals=None, mlp=None):
+def run_variant(T, mesh, prim, params, steps, warmup, neutrals=None, mlp=None, keep_state=False):
     s = T.Setka(mesh, params)
     try:
         s.set_state(prim, 0)
@@ -48,12 +65,99 @@ def run_variant(T, mesh, prim, params, steps, warmup, neutrals=None, mlp=None):
             s.Go(warmup)
         s.set_timing(True)
         t0 = time.perf_counter()
-        st = s.Go(steps)
+        st = s.Go(steps)  # returns after the stream has drained
         t1 = time.perf_counter()
-        kms = {k: s.kernel_ms(i) for i, k in enumerate(("step", "geometry", "motion", "source"))}
-        return dict(seconds=t1 - t0, stats=st, kernel_ms=kms, state=s.get_state(), tiles=s.tile_info())
+        out = dict(seconds=t1 - t0, steps=steps, stats=st, step_kernel_ms=s.kernel_ms(0),
+                   source_kernel_ms=s.kernel_ms(3), tiles=s.tile_info())
+        if keep_state:
+            out["state"] = s.get_state()
+        return out
     finally:
         s.close()
+
+
+def divb(T, mesh, params, prim, xyz):
+    """div B per cell, (1/V) sum_f s_f PdivB_f with PdivB = 1/2 (B_L + B_R).n A (the face loop's
+    ninth flux component), on the given state and node positions."""
+    with T.Setka(mesh, dict(params, move_mesh=0)) as s:
+        s.set_nodes(xyz, 0)
+        s.set_state(prim, 0)
+        flux, _, _ = s.Culc_Gran_Potok(0, 1e-5, want_branch=False)
+        vol = s.geometry(0)["cell_volume"]
+    cf = mesh.cell_faces
+    sign = np.where(mesh.face_cells[cf, 0] == np.arange(mesh.n_cells)[:, None], 1.0, -1.0)
+    return (sign * flux[cf, 8]).sum(axis=1) / vol, vol
+
+
+def precision_report(T, mesh, prim, params, steps):
+    res = {}
+    for name, prec in (("fp64", T.PREC_FP64), ("mixed", T.PREC_MIXED)):
+        with T.Setka(mesh, dict(params, precision=prec)) as s:
+            s.set_state(prim, 0)
+            t0 = time.perf_counter()
+            st = s.Go(steps)
+            res[name] = dict(state=s.get_state(), xyz=s.get_nodes(s.current_layer()), stats=st,
+                             seconds=time.perf_counter() - t0)
+    a, b = res["mixed"]["state"], res["fp64"]["state"]
+    groups = {"rho": [0], "V": [1, 2, 3], "p": [4], "B": [5, 6, 7]}
+    out = {"steps": steps, "reference": "FP64 GPU step (bit-exact with the oracle at the parity sizes)"}
+    for g, cols in groups.items():
+        d, r = a[:, cols] - b[:, cols], b[:, cols]
+        out[g] = {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(r)),
+                  "rel_linf": float(np.abs(d).max() / np.abs(r).max())}
+    bmag = np.linalg.norm(b[:, 5:8], axis=1)
+    for name in ("fp64", "mixed"):
+        dv, vol = divb(T, mesh, params, res[name]["state"], res[name]["xyz"])
+        h = np.cbrt(vol)
+        out["divB_" + name] = {"l2_h_over_B": float(np.sqrt(np.mean((dv * h / np.maximum(bmag, 1e-300)) ** 2))),
+                               "max_abs": float(np.abs(dv).max())}
+    out["rho_p_within_1e-4"] = bool(max(out["rho"]["rel_linf"], out["p"]["rel_linf"]) <= 1e-4)
+    out["t_sim"] = res["fp64"]["stats"]["t_sim"]
+    return out
+
+
+def train_surrogate(T, C, sur, prim, nh, n_states, seed=11):
+    """Training set: n_states plasma/neutral states drawn from the workload's cells (perturbed),
+    labelled by the exact FP64 quadrature kernel; trained by the device's own forward/backward
+    and Adam kernels."""
+    t0 = time.perf_counter()
+    in10 = sur.sample_states(prim, nh, n_states, seed=seed)
+    x, y, _ = sur.make_dataset(in10, nodes=100)
+    t_label = time.perf_counter() - t0
+    norm = sur.Normaliser(x, y)
+    epochs, lrs = (6, 4, 2), (2e-3, 5e-4, 1e-4)
+    mesh_t, _, par_t = C.cube_brio_wu(n=4)
+    with T.Setka(mesh_t, par_t) as s:
+        t1 = time.perf_counter()
+        hist = sur.train(s, x, y, norm, epochs=epochs, lrs=lrs, batch=4096)
+        t_train = time.perf_counter() - t1
+        P = s.mlp_get()
+    in_norm, scale, box = norm.blocks()
+    info = {"states": int(n_states), "label_s": round(t_label, 2), "train_s": round(t_train, 2),
+            "epochs": int(sum(epochs)), "loss_first": float(hist[0]), "loss_last": float(hist[-1])}
+    return (P, in_norm, scale, box), info
+
+
+def traffic_from_profile(workload):
+    """HBM bytes per launch of the FP64 step kernel measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
+    counter runs (calibrated on a copy kernel), as recorded in the profile summary, or None."""
+    path = TRAFFIC_FILE % workload
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def copy_calibration():
+    """A plain streaming kernel of known size (y = -x: 256 MiB read + 256 MiB written), for the
+    counter runs that scale FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    log("copy calibration: 3 x (%d bytes read + written)" % (x.numel() * 8))
 
 
 def main():
@@ -63,43 +167,98 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="shell4m_moving")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--exact-steps", type=int, default=3)
+    ap.add_argument("--train-states", type=int, default=2_000_000)
+    ap.add_argument("--precision-steps", type=int, default=1000)
+    ap.add_argument("--variants", default="fp64,mixed,fp64_exact,mixed_surrogate")
+    ap.add_argument("--copy-calibration", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build targets one GPU; multi-GPU runs are not part of it (use --gpus 1)",
+        print("bench.py: this build runs on one GPU; runs on %d GPUs are not part of it (use --gpus 1)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
     T = importlib.import_module("titan_c-_amd")
     C = importlib.import_module("titan_c-_amd.configs")
+    sur = importlib.import_module("titan_c-_amd.surrogate")
     T.lib()
+    if a.copy_calibration:
+        copy_calibration()
+    variants = a.variants.split(",")
     t0 = time.perf_counter()
     mesh, prim, params = C.build(a.workload)
-    t_build = time.perf_counter() - t0
+    nh = C.neutrals(mesh.n_cells)
     moving = bool(params["move_mesh"])
     nc, nf, nn = mesh.n_cells, mesh.n_faces, mesh.n_nodes
-    print("[bench] workload %s: %d cells, %d faces, %d nodes (built in %.1f s)" % (a.workload, nc, nf, nn, t_build),
-          file=sys.stderr, flush=True)
+    log("workload %s: %d cells, %d faces, %d nodes (built in %.1f s)" % (a.workload, nc, nf, nn,
+                                                                         time.perf_counter() - t0))
 
     res = {}
-    p64 = dict(params, precision=T.PREC_FP64)
-    res["fp64"] = run_variant(T, mesh, prim, p64, a.steps, a.warmup)
-    print("[bench] fp64 done: %.3f s for %d steps; tiles %s" % (res["fp64"]["seconds"], a.steps, res["fp64"]["tiles"]),
-          file=sys.stderr, flush=True)
-    pmx = dict(params, precision=T.PREC_MIXED)
-    res["mixed"] = run_variant(T, mesh, prim, pmx, a.steps, a.warmup)
-    print("[bench] mixed done: %.3f s" % res["mixed"]["seconds"], file=sys.stderr, flush=True)
+    if "fp64" in variants:
+        res["fp64"] = run_variant(T, mesh, prim, dict(params, precision=T.PREC_FP64), a.steps, a.warmup)
+        log("fp64: %.3f ms/step, step kernel %.3f ms; tiles %s" % (
+            res["fp64"]["seconds"] * 1e3 / a.steps, res["fp64"]["step_kernel_ms"], res["fp64"]["tiles"]))
+    if "mixed" in variants:
+        res["mixed"] = run_variant(T, mesh, prim, dict(params, precision=T.PREC_MIXED), a.steps, a.warmup)
+        log("mixed: %.3f ms/step" % (res["mixed"]["seconds"] * 1e3 / a.steps))
+    if "fp64_exact" in variants:
+        p = dict(params, precision=T.PREC_FP64, source_mode=T.SRC_EXACT, cx_nodes=100)
+        res["fp64_exact"] = run_variant(T, mesh, prim, p, a.exact_steps, 1, neutrals=nh)
+        log("fp64 + exact source: %.1f ms/step (source kernel %.1f ms)" % (
+            res["fp64_exact"]["seconds"] * 1e3 / a.exact_steps, res["fp64_exact"]["source_kernel_ms"]))
+    sur_info = None
+    if "mixed_surrogate" in variants:
+        mlp, sur_info = train_surrogate(T, C, sur, prim, nh, a.train_states)
+        log("surrogate trained: %s" % sur_info)
+        with T.Setka(mesh, dict(params, cx_nodes=100)) as s:
+            s.set_state(prim, 0)
+            s.set_neutrals(nh)
+            s.mlp_set(*mlp)
+            ex = s.sources(0, T.SRC_EXACT)
+            ml = s.sources(0, T.SRC_MLP)
+        cols = ("rate", "Sm_x", "Sm_y", "Sm_z", "S_E")
+        err = np.abs(ml - ex)
+        sur_info["source_err_rel_max"] = {k: float(err[:, i].max() / np.abs(ex[:, i]).max())
+                                          for i, k in enumerate(cols)}
+        sur_info["source_err_rel_median"] = {
+            k: float(np.median(err[:, i] / np.maximum(np.abs(ex[:, i]), 1e-300))) for i, k in enumerate(cols)}
+        p = dict(params, precision=T.PREC_MIXED, source_mode=T.SRC_MLP, cx_nodes=100)
+        res["mixed_surrogate"] = run_variant(T, mesh, prim, p, a.steps, a.warmup, neutrals=nh, mlp=mlp)
+        sur_info["fallback_cells_last_step"] = int(res["mixed_surrogate"]["stats"]["n_fallback"])
+        log("mixed + surrogate: %.3f ms/step" % (res["mixed_surrogate"]["seconds"] * 1e3 / a.steps))
 
-    def cups(r):
-        return nc * a.steps / r["seconds"]
+    def cups(name):
+        r = res.get(name)
+        return None if r is None else nc * r["steps"] / r["seconds"]
 
-    step_ms = res["fp64"]["seconds"] * 1e3 / a.steps
-    bytes_step = algorithmic_bytes(nc, nf, nn, moving)
-    # dominant kernel family: the face flux + cell update kernels of a step
-    k_ms = res["fp64"]["kernel_ms"]["step"]
-    ach = bytes_step / (step_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": "full step (all kernels of one step)", "achieved": round(ach, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_step": bytes_step, "step_kernels_ms": k_ms}
+    vals = {k: cups(k) for k in ("fp64", "mixed", "fp64_exact", "mixed_surrogate")}
+    ratios = {}
+    if vals["mixed_surrogate"] and vals["fp64_exact"]:
+        ratios["surrogate_vs_exact_source"] = vals["mixed_surrogate"] / vals["fp64_exact"]
+    if vals["mixed_surrogate"] and vals["fp64"]:
+        ratios["surrogate_vs_plain_fp64"] = vals["mixed_surrogate"] / vals["fp64"]
+    if vals["mixed"] and vals["fp64"]:
+        ratios["mixed_vs_plain_fp64"] = vals["mixed"] / vals["fp64"]
+
+    roof = None
+    head = "fp64" if "fp64" in res else None
+    if head:
+        bytes_step = algorithmic_bytes(nc, nf, nn, moving)
+        k_ms = res[head]["step_kernel_ms"]
+        ach = bytes_step / (k_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": "k_step_tile<double,false> (one launch = one step)",
+                "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profile(a.workload), "traffic_unit": "bytes/launch",
+                "algorithmic_bytes": bytes_step, "kernel_ms": round(k_ms, 4)}
+
+    prec = None
+    if a.precision_steps > 0:
+        try:
+            prec = precision_report(T, mesh, prim, params, a.precision_steps)
+            log("precision after %d steps: rho %s p %s" % (a.precision_steps, prec["rho"], prec["p"]))
+        except T.TitanError as e:  # a step that stops with a status code is reported, not hidden
+            prec = {"steps": a.precision_steps, "error": str(e)}
+            log("precision run stopped: %s" % e)
 
     cpu = None
     if not a.no_cpu:
@@ -108,22 +267,28 @@ def main():
         ncpu = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         om = O.OracleMesh(mesh)
         t0 = time.perf_counter()
-        rc, _, _, _, _ = O.go(om, p64, mesh.node_xyz, prim, a.cpu_steps, nthreads=ncpu,
+        rc, _, _, _, _ = O.go(om, dict(params, precision=0), mesh.node_xyz, prim, a.cpu_steps, nthreads=ncpu,
                               disp=mesh.node_disp)
         t1 = time.perf_counter()
         cpu = {"value": nc * a.cpu_steps / (t1 - t0), "unit": "cell-updates/s", "cores": ncpu, "kind": "port",
-               "sample": "%d FP64 steps of the same %s workload (oracle, OpenMP face+cell loops, geometry "
-                         "re-measured each step)" % (a.cpu_steps, a.workload)}
+               "sample": "%d FP64 steps of the same %s workload, first-order HLLC/HLLD, moving mesh re-measured "
+                         "every step (C restatement of Setka::Go, OpenMP face and cell loops)" % (
+                             a.cpu_steps, a.workload)}
+        log("cpu baseline: %.3g cell-updates/s on %d threads (rc %d)" % (cpu["value"], ncpu, rc))
 
+    r0 = res.get(head) if head else None
     out = {
         "metric": "cell-updates/s @4M cells: FP64 plain vs FP32-mixed+surrogate; % HBM peak",
-        "value": cups(res["fp64"]), "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": step_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "value": vals.get("fp64"), "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": (r0["seconds"] * 1e3 / r0["steps"]) if r0 else None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": a.workload, "n_cells": nc, "n_faces": nf, "n_nodes": nn, "moving": moving,
                    "variant": "plain FP64 step"},
-        "variants": {"fp64": cups(res["fp64"]), "mixed_fp32_riemann": cups(res["mixed"])},
+        "variants": vals,
+        "ratios": ratios,
         "roofline": roof,
+        "surrogate": sur_info,
+        "precision": prec,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

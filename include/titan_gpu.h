@@ -187,7 +187,8 @@ int titan_mesh_shell(int32_t n_edge, int32_t n_layers, const double *radii, doub
 /* ---- timing --------------------------------------------------------------------------------- */
 /* Mean device time (ms) per launch of the named kernel family over the last titan_go call,
  * measured with HIP events on the context stream: which = 0 step kernel (flux+update),
- * 1 geometry, 2 motion, 3 source.  Returns -1 if not measured. */
+ * 1 geometry, 2 motion, 3 source (the exact-source kernel, or in TITAN_SRC_MLP mode the
+ * out-of-box fallback kernel that follows each step).  Returns -1 if not measured. */
 double titan_kernel_ms(titan_ctx *ctx, int which);
 int titan_set_timing(titan_ctx *ctx, int enable);
 /* Layout of the fused step: info8 = {cells per tile, max nodes per tile, max faces per tile,

@@ -58,6 +58,7 @@ def lib():
             "or_face_velocities": (None, [_P, _P, _P, _P, _P, _D, _I, _P]),
             "or_motion": (None, [_I64, _P, _P, _D, _D, _D, _D, _P]),
             "or_face_flux": (_I, [_P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
+            "or_face_states": (_I, [_P, _P, _I64, _P, _P, _P, _P]),
             "or_update": (_I, [_P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
             "or_go": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _P, _P, _P]),
             "or_new_connect": (_I64, [_I64, _P, _I64, _P, _P, _P]),
@@ -250,3 +251,21 @@ def sigma_ls(g):
 
 def det_cos(x):
     return lib().or_det_cos(float(x))
+
+
+def face_states(om, params, prim, geom, f):
+    """Left/right states of face f (Setka::Snos_on_Gran, first order + boundary)."""
+    g = OrGeom(*[_p(geom[k]) for k, _ in OrGeom._fields_])
+    prim = np.ascontiguousarray(prim, np.float64)
+    qL, qR = np.empty(8), np.empty(8)
+    rc = lib().or_face_states(ctypes.byref(om.c), ctypes.byref(params_c(params)), int(f), _p(prim),
+                              ctypes.byref(g), _p(qL), _p(qR))
+    return rc, qL, qR
+
+
+def motion(x_old, disp, dt, t, omega, phase):
+    x_old = np.ascontiguousarray(x_old, np.float64)
+    disp = np.ascontiguousarray(disp, np.float64)
+    out = np.empty_like(x_old)
+    lib().or_motion(x_old.shape[0], _p(x_old), _p(disp), float(dt), float(t), float(omega), float(phase), _p(out))
+    return out

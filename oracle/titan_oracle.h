@@ -20,7 +20,34 @@
  *   or_update           cell loop of Setka::Go       TITAN/source/Setka_physics.cpp:1671-2027
  *   or_go               Setka::Go                    TITAN/source/Setka_physics.cpp:1484-2278
  *   or_new_connect      Setka::New_connect           TITAN/source/Setka.cpp:2066-2249
- *   or_cx_rate_table    Culc_nu_maxwel               MK_source/MK_source/MK_source/MK_source.cpp:38-103
+ *   or_cx_rate_ref      Culc_nu_maxwel               MK_source/MK_source/MK_source/MK_source.cpp:38-103
+ *   or_cx_source, or_sigma_ls, or_mlp_*  the learned-source path (no reference counterpart:
+ *                       Maxwellian CX moments with the Lindsay-Stebbings cross-section; MLP)
+ *
+ * What pins each function (tests/test_oracle_*.py), so that a dropped term, a wrong sign or
+ * index or a transposed operand fails a check that is not a retyping of the formula:
+ *   or_chlld            consistency with the reference's physical flux (Get_Potok,
+ *                       Phys_param.cpp:826-874) for equal states, rotational invariance, L/R
+ *                       mirror symmetry, HLLD -> HLLC at Bn = 0, monotone regions and flux
+ *                       continuity across the fan, the fatal branch, raspad_testing inputs.
+ *                       Last-bit parity with the compiled reference is UNPINNED: TITAN/source
+ *                       needs Eigen3/Boost/CGAL (absent), so Eigen's summation order inside
+ *                       get_bazis/normalized() is assumed to be left-to-right.
+ *   or_get_bazis        orthonormality and handedness.
+ *   or_measure(_gcl)    exact values on axis-aligned boxes, Test_geometr invariants, translation
+ *                       invariance, GCL closure and swept-volume identity.
+ *   or_face_velocities  rigid translation gives w = d.n/dt.
+ *   or_motion, or_det_cos  rigid translation; det_cos against libm cos.
+ *   or_face_states      a spherically symmetric wind is carried onto every face exactly (both
+ *                       spherical frames), boundary rules on known answers.
+ *   or_update, or_go    mass conservation, quiescent state, floors at their thresholds, frozen B,
+ *                       volume ratio and sources, lagged dt schedule, Brio-Wu profile.
+ *   or_new_connect      hand-worked two-cell mesh and face counts.
+ *   or_cx_rate_ref      the reference's own MK_source program (compiled from its source into
+ *                       oracle/_ref) at sampled table points: tests/golden/mk_source_int1.json.
+ *   or_cx_source        sigma = const closed form, Cartesian quadrature, 6-D Monte Carlo,
+ *                       bilinearity and Galilean invariance; or_sigma_ls: published values.
+ *   or_mlp_*            closed forms and finite differences.
  */
 #ifndef TITAN_ORACLE_H
 #define TITAN_ORACLE_H

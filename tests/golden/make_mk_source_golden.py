@@ -21,7 +21,7 @@ EXE = os.path.join(ROOT, "oracle", "_ref", "mk_source")
 def main():
     if not os.path.exists(EXE):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-f", "Makefile.ref"], check=True)
-    with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "_work")) as d:
+    with tempfile.TemporaryDirectory() as d:
         subprocess.run([EXE], cwd=d, check=True, stdout=subprocess.DEVNULL)
         tab = np.fromfile(os.path.join(d, "Int_1.bin"), dtype=np.float64).reshape(1000, 100)
     pts = []
