@@ -112,6 +112,7 @@ def precision_report(T, mesh, prim, params, steps):
         out["divB_" + name] = {"l2_h_over_B": float(np.sqrt(np.mean((dv * h / np.maximum(bmag, 1e-300)) ** 2))),
                                "max_abs": float(np.abs(dv).max())}
     out["rho_p_within_1e-4"] = bool(max(out["rho"]["rel_linf"], out["p"]["rel_linf"]) <= 1e-4)
+    out["mixed_fp64_face_solves_last_step"] = int(res["mixed"]["stats"]["n_fp64_faces"])
     out["t_sim"] = res["fp64"]["stats"]["t_sim"]
     return out
 

@@ -91,6 +91,8 @@ typedef struct {
     int64_t n_rho_floor, n_p_floor; /* floor activations (Setka_physics.cpp:1784,1981), last step */
     int64_t n_fallback;          /* cells that used the exact source in MLP mode, last step    */
     double ms_total;             /* wall time of the titan_go call (HIP events)                */
+    int64_t n_fp64_faces;        /* TITAN_PREC_MIXED: face solves handed to the FP64 solver (ill- 
+                                    conditioned HLLD Alfven denominators), last step            */
 } titan_step_stats;
 
 /* ---- context ------------------------------------------------------------------------------- */

@@ -72,3 +72,36 @@ def test_chlld_fp32_close(T, O):
         # FP32 Riemann arithmetic: errors relative to the face's largest flux component
         assert np.quantile(err, 0.999) < 1e-4, np.quantile(err, 0.999)
         assert (got[3] == ref[3]).mean() > 0.999
+
+
+@pytest.mark.parametrize("method", [0, 1, 2, 3])
+def test_chlld_mixed_error_scales_with_the_jump(T, O, method):
+    """The mixed solver keeps the upwind flux in FP64 and forms every wave correction from the
+    jumps: for states 1e-3 apart its error is ~1e-10 of the flux (a plain FP32 solve is ~1e-7),
+    and with both states supersonic through the face it equals the FP64 flux to rounding."""
+    rng = np.random.default_rng(200 + method)
+    n = 60_000
+    qL = random_states(rng, n)
+    qR = qL * (1.0 + 1e-3 * rng.normal(size=(n, 8)))
+    nrm = random_normals(rng, n)
+    w = rng.normal(0.0, 0.5, n)
+    ref = O.chlld_batch(method, nrm, qL, qR, w=w)
+    got = T.chlld(method, nrm, qL, qR, w=w, precision=T.PREC_MIXED)
+    scale = np.abs(ref[0]).max(axis=1, keepdims=True) + 1e-30
+    err = (np.abs(got[0] - ref[0]) / scale).max(axis=1)
+    assert np.quantile(err, 0.999) < 1e-8, np.quantile(err, 0.999)
+    if method == 0:
+        return  # Lax's speeds are symmetric about zero: it has no upwind region
+    # supersonic: push both states through the face at 20x their speed scale along the normal
+    vfast = 20.0 * (np.abs(qL[:, 1:4]).max(axis=1) + np.sqrt(qL[:, 4] / qL[:, 0]) + np.abs(qL[:, 5:8]).max(axis=1))
+    qLs, qRs = qL.copy(), qR.copy()
+    qLs[:, 1:4] += vfast[:, None] * nrm
+    qRs[:, 1:4] += vfast[:, None] * nrm
+    ref = O.chlld_batch(method, nrm, qLs, qRs, w=np.zeros(n))
+    got = T.chlld(method, nrm, qLs, qRs, w=np.zeros(n), precision=T.PREC_MIXED)
+    up = (ref[3] & 7) == 1  # the region left of the fan: the reference returns F_L
+    assert up.mean() > 0.99
+    scale = np.abs(ref[0][up]).max(axis=1, keepdims=True)
+    # FP64 except the normal-field dissipation -S_N dBn, whose fast speed is FP32
+    assert (np.abs(got[0][up][:, :5] - ref[0][up][:, :5]) / scale).max() < 1e-14
+    assert (np.abs(got[0][up] - ref[0][up]) / scale).max() < 1e-12

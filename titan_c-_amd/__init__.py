@@ -59,7 +59,8 @@ class ParamsC(ctypes.Structure):
 class StatsC(ctypes.Structure):
     _fields_ = [("steps_done", ctypes.c_int64), ("dt_last", ctypes.c_double), ("dt_next", ctypes.c_double),
                 ("t_sim", ctypes.c_double), ("n_rho_floor", ctypes.c_int64), ("n_p_floor", ctypes.c_int64),
-                ("n_fallback", ctypes.c_int64), ("ms_total", ctypes.c_double)]
+                ("n_fallback", ctypes.c_int64), ("ms_total", ctypes.c_double),
+                ("n_fp64_faces", ctypes.c_int64)]
 
 
 _lib = None

@@ -59,6 +59,7 @@ struct StepScalars {
     double pad0;
     unsigned long long dtmin_bits[2];  // running min of the CFL step, slot = step & 1
     unsigned long long n_rho_floor, n_p_floor, n_fallback;
+    unsigned long long n_fp64_faces;  // faces the mixed solver handed to the FP64 solver
     int err_code;
     int pad1;
     long long err_index;             // smallest reference id that failed, or LLONG_MAX

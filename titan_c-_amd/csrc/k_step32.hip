@@ -40,19 +40,22 @@ __global__ void __launch_bounds__(256) k_chlld_batch32(int64_t n, const int8_t *
                                                        uint8_t *__restrict__ branch, int *err) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float a[8], b[8], ka[K > 0 ? K : 1], kb[K > 0 ? K : 1];
+    double a[8], b[8], ka[K > 0 ? K : 1], kb[K > 0 ? K : 1];
 #pragma unroll
-    for (int j = 0; j < 8; j++) { a[j] = (float)qL[8 * i + j]; b[j] = (float)qR[8 * i + j]; }
+    for (int j = 0; j < 8; j++) { a[j] = qL[8 * i + j]; b[j] = qR[8 * i + j]; }
 #pragma unroll
-    for (int j = 0; j < K; j++) { ka[j] = (float)kL[K * i + j]; kb[j] = (float)kR[K * i + j]; }
-    RiemannOut<float, K> o;
-    chlld<float, K>(n_state[i], n_disc ? n_disc[i] : 0, flags ? flags[i] : 0, (float)normal[3 * i],
-                    (float)normal[3 * i + 1], (float)normal[3 * i + 2], w ? (float)w[i] : 0.0f, a, b, ka, kb, o);
+    for (int j = 0; j < K; j++) { ka[j] = kL[K * i + j]; kb[j] = kR[K * i + j]; }
+    RiemannOut<double, K> o;
+    chlld_mixed<K>(n_state[i], n_disc ? n_disc[i] : 0, flags ? flags[i] : 0, normal[3 * i], normal[3 * i + 1],
+                   normal[3 * i + 2], w ? w[i] : 0.0, a, b, ka, kb, o);
+    if (o.ok < 0)  // ill-conditioned in FP32: the FP64 solver
+        chlld_fp64_face<K>(n_state[i], n_disc ? n_disc[i] : 0, flags ? flags[i] : 0, normal[3 * i], normal[3 * i + 1],
+                           normal[3 * i + 2], w ? w[i] : 0.0, a, b, ka, kb, &o);
 #pragma unroll
-    for (int j = 0; j < 8; j++) q[8 * i + j] = (double)o.q[j];
+    for (int j = 0; j < 8; j++) q[8 * i + j] = o.q[j];
 #pragma unroll
-    for (int j = 0; j < K; j++) k[K * i + j] = o.ok ? (double)o.k[j] : 0.0;
-    if (ds) { ds[3 * i] = (double)o.dsl; ds[3 * i + 1] = (double)o.dsc; ds[3 * i + 2] = (double)o.dsr; }
+    for (int j = 0; j < K; j++) k[K * i + j] = o.ok ? o.k[j] : 0.0;
+    if (ds) { ds[3 * i] = o.dsl; ds[3 * i + 1] = o.dsc; ds[3 * i + 2] = o.dsr; }
     if (branch) branch[i] = (uint8_t)o.branch;
     if (!o.ok) atomicExch(err, 1);
 }

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include "device_mesh.h"
 #include "riemann.h"
+#include "riemann_mixed.h"
 
 namespace titan {
 
@@ -78,6 +79,7 @@ __global__ void k_begin_step(StepScalars *S, long long n, DevParams P) {
     S->n_rho_floor = 0;
     S->n_p_floor = 0;
     S->n_fallback = 0;
+    S->n_fp64_faces = 0;
 }
 
 // ---- geometry -------------------------------------------------------------------------------
@@ -385,7 +387,8 @@ __device__ __forceinline__ void block_min_to(double v, unsigned long long *slot)
 
 // ---- face flux: Setka::Culc_Gran_Potok for every face -------------------------------------------
 // flux[f*9 + 0..7] = chlld flux * area, flux[f*9 + 8] = PdivB.  T = double: parity arithmetic;
-// T = float: the Riemann arithmetic in FP32, states, geometry, area scaling in FP64.
+// T = float: chlld_mixed (FP32 wave structure, FP64 upwind flux), states, geometry, area scaling
+// in FP64.
 template <typename T>
 __global__ void __launch_bounds__(256) k_face_flux(DevMesh M, DevParams P, const double *__restrict__ prim,
                                                    DevGeom G, const double *__restrict__ X1,
@@ -416,15 +419,15 @@ __global__ void __launch_bounds__(256) k_face_flux(DevMesh M, DevParams P, const
             for (int i = 0; i < 8; i++) q[i] = o.q[i];
             dsl = o.dsl; dsr = o.dsr; br = o.branch; ok = o.ok;
         } else {
-            T qLt[8], qRt[8];
+            RiemannOut<double, 0> o;
+            chlld_mixed<0>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, nullptr, nullptr, o);
+            if (o.ok < 0) {
+                chlld_fp64_face<0>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, nullptr, nullptr, &o);
+                atomicAdd(&S->n_fp64_faces, 1ull);
+            }
 #pragma unroll
-            for (int i = 0; i < 8; i++) { qLt[i] = (T)qL[i]; qRt[i] = (T)qR[i]; }
-            RiemannOut<T, 0> o;
-            chlld<T, 0>((int)method, P.n_disc, (int)flags, (T)n[0], (T)n[1], (T)n[2], (T)w, qLt, qRt, nullptr,
-                        nullptr, o);
-#pragma unroll
-            for (int i = 0; i < 8; i++) q[i] = (double)o.q[i];
-            dsl = (double)o.dsl; dsr = (double)o.dsr; br = o.branch; ok = o.ok;
+            for (int i = 0; i < 8; i++) q[i] = o.q[i];
+            dsl = o.dsl; dsr = o.dsr; br = o.branch; ok = o.ok;
         }
         if (!ok || !okbc) report_error(S, -3, M.face_ref[f]);
         if (bc == 0 && ok) {

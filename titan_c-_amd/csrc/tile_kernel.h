@@ -1,7 +1,8 @@
 // tile_kernel.h -- the fused explicit step: one workgroup per tile of TILE cells does node motion,
 // face geometry and velocity, face states, Riemann fluxes (kept in LDS), and the cell update with
-// the flux sum in the reference's order.  Included by k_step64.hip (RT = double, parity) and
-// k_step32.hip (RT = float: Riemann arithmetic and face-state rotations in FP32).
+// the flux sum in the reference's order.  Included by k_step64.hip (RT = double: chlld in FP64,
+// parity) and k_step32.hip (RT = float: chlld_mixed, FP32 wave structure around an FP64 upwind
+// flux; face states, geometry and the update stay FP64).
 //
 // Work per step (Setka::Go, Setka_physics.cpp:1589-2173):
 //   phase 1  tile nodes -> LDS (layer now1), new positions x + dt*omega*cos(omega t + phase)*D
@@ -258,6 +259,134 @@ __device__ __forceinline__ double tile_volume(const double *X, const double cc[3
     return gcl ? V / 3.0 : V;
 }
 
+// Phase 3 for one face slot f of the tile: geometry at now1, face velocity, states (Snos_on_Gran),
+// Riemann flux x area -> LDS slot, CFL candidate.  RT = double: chlld (parity); RT = float:
+// chlld_mixed -- a face it reports ill-conditioned gets NaN in its PdivB slot and false is
+// returned, for the FP64 pass over marked slots.
+template <typename RT>
+__device__ __forceinline__ bool face_slot(int f, const TileMesh &TM, const DevMesh &M, const DevParams &P,
+                                          const TileLds &L, int fb, int64_t c0, int ncl,
+                                          const double *__restrict__ prim1, const double *__restrict__ cc1g,
+                                          bool moving, int gcl, double dt, StepScalars *S, double &cand) {
+    const ulonglong2 rec = TM.faces[fb + f];
+    const int ln[4] = {(int)(rec.x & 0xfff), (int)((rec.x >> 12) & 0xfff), (int)((rec.x >> 24) & 0xfff),
+                       (int)((rec.x >> 36) & 0xfff)};
+    const uint32_t code = (uint32_t)(rec.x >> 48);
+    const uint32_t bc = code & 15u, method = (code >> 4) & 15u, flags = (code >> 8) & 255u;
+    const int64_t a = (int32_t)(rec.y & 0xffffffffu);
+    const int64_t b = (int32_t)(rec.y >> 32);
+#pragma unroll
+    for (int i = 0; i < 4; i++) L.fnodes[4 * f + i] = (uint16_t)ln[i];
+    double x1[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; i++) { x1[i][0] = L.x1[3 * ln[i]]; x1[i][1] = L.x1[3 * ln[i] + 1]; x1[i][2] = L.x1[3 * ln[i] + 2]; }
+    const bool aloc = (a >= c0 && a < c0 + ncl);
+    double ca[3];
+    if (aloc) { ca[0] = L.c1[3 * (a - c0)]; ca[1] = L.c1[3 * (a - c0) + 1]; ca[2] = L.c1[3 * (a - c0) + 2]; }
+    else { ca[0] = cc1g[3 * a]; ca[1] = cc1g[3 * a + 1]; ca[2] = cc1g[3 * a + 2]; }
+    double n[3], area, G[3];
+    face_geometry(x1, ca, gcl, n, area, G);
+    double w = 0.0;
+    if (moving) {
+        double x2[4][3];
+#pragma unroll
+        for (int i = 0; i < 4; i++) { x2[i][0] = L.x2[3 * ln[i]]; x2[i][1] = L.x2[3 * ln[i] + 1]; x2[i][2] = L.x2[3 * ln[i] + 2]; }
+        if (gcl) {
+            w = swept_velocity(x1, x2, n, area, dt);
+        } else {
+            double ddot = 0.0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const double d0 = x2[i][0] - x1[i][0], d1 = x2[i][1] - x1[i][1], d2 = x2[i][2] - x1[i][2];
+                ddot = ddot + ((d0 * n[0] + d1 * n[1]) + d2 * n[2]);
+            }
+            w = ddot / 4 / dt;
+        }
+    }
+    // face states (Snos_on_Gran)
+    double sA[8];
+    load_state(prim1, a, sA);
+    double qL[8], qR[8];
+    bool okbc = true;
+    double cb[3] = {0.0, 0.0, 0.0};
+    if (bc != 0) {
+        double l[8], r[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) l[i] = sA[i];
+        if (bc == 3) {
+            if (n[0] < -0.95) {
+                if (l[1] > P.v_inf / 7.0) l[1] = P.v_inf / 5.0;
+            } else if (l[1] * n[0] + l[2] * n[1] + l[3] * n[2] < 0.0) {
+                l[1] = 0.1 * n[0]; l[2] = 0.1 * n[1]; l[3] = 0.1 * n[2];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++) r[i] = l[i];
+        } else if (bc == 1 || bc == 2) {
+            const double *bs = M.bstate + 8 * (int64_t)(-b - 2);
+#pragma unroll
+            for (int i = 0; i < 8; i++) r[i] = bs[i];
+        } else {
+            okbc = (bc == 4);
+#pragma unroll
+            for (int i = 0; i < 8; i++) r[i] = l[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) { qL[i] = l[i]; qR[i] = r[i]; }
+    } else {
+        double sB[8];
+        load_state(prim1, b, sB);
+        const bool bloc = (b >= c0 && b < c0 + ncl);
+        if (bloc) { cb[0] = L.c1[3 * (b - c0)]; cb[1] = L.c1[3 * (b - c0) + 1]; cb[2] = L.c1[3 * (b - c0) + 2]; }
+        else { cb[0] = cc1g[3 * b]; cb[1] = cc1g[3 * b + 1]; cb[2] = cc1g[3 * b + 2]; }
+        const bool zA = ((M.cell_code[a] >> 8) & 15u) == 1u, zB = ((M.cell_code[b] >> 8) & 15u) == 1u;
+        double tA[8], tB[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) { tA[i] = sA[i]; tB[i] = sB[i]; }
+        if (zA || zB) {
+            const Zone1Face<double> zf = zone1_face<double>(G, P.gamma);
+            if (zA) {
+                const Zone1Cell<double> zc = aloc ? load_zone1<double>(L.z1 + 10 * (a - c0)) : zone1_cell<double>(ca, P.gamma);
+                zone1_apply<double>(tA, zc, zf, qL);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; i++) qL[i] = tA[i];
+            }
+            if (zB) {
+                const Zone1Cell<double> zc = bloc ? load_zone1<double>(L.z1 + 10 * (b - c0)) : zone1_cell<double>(cb, P.gamma);
+                zone1_apply<double>(tB, zc, zf, qR);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; i++) qR[i] = tB[i];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; i++) { qL[i] = tA[i]; qR[i] = tB[i]; }
+        }
+    }
+    RiemannOut<double, 0> o;
+    if constexpr (sizeof(RT) == sizeof(double)) {
+        chlld<double, 0>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, nullptr, nullptr, o);
+    } else {
+        chlld_mixed<0>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, nullptr, nullptr, o);
+        if (o.ok < 0) {  // ill-conditioned in FP32: marked for the FP64 pass
+            L.flux[9 * f + 8] = __builtin_nan("");
+            return false;
+        }
+    }
+    if (!o.ok || !okbc) report_error(S, -3, TM.face_ref[fb + f]);
+    if (bc == 0 && o.ok) {
+        const double d0 = ca[0] - cb[0], d1 = ca[1] - cb[1], d2 = ca[2] - cb[2];
+        const double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2) / 2.0;
+        const double dnt = P.kfl * dist / (smax(fabs((double)o.dsl), fabs((double)o.dsr)) + fabs(w));
+        if (dnt < cand) cand = dnt;
+    }
+    double *fl = L.flux + 9 * f;
+#pragma unroll
+    for (int i = 0; i < 8; i++) fl[i] = o.q[i] * area;
+    fl[8] = 0.5 * spf(n[0], n[1], n[2], qL[5] + qR[5], qL[6] + qR[6], qL[7] + qR[7]) * area;
+    return true;
+}
+
 // Phase 4 of the surrogate step.  Lane l < 32 of wavefront w owns cell 32w + l of the tile (tile
 // cells <= 32 * waves); lanes l and l + 32 run the network pass of that cell together.
 template <typename RT>
@@ -400,7 +529,7 @@ __global__ void __launch_bounds__(kTileThreads, 2)
             cc2g[3 * (c0 + c)] = b0 / 8; cc2g[3 * (c0 + c) + 1] = b1 / 8; cc2g[3 * (c0 + c) + 2] = b2 / 8;
         }
         if (((M.cell_code[c0 + c] >> 8) & 15u) == 1u) {
-            const Zone1Cell<RT> z = zone1_cell<RT>(cA, P.gamma);
+            const Zone1Cell<double> z = zone1_cell<double>(cA, P.gamma);
             double *o = L.z1 + 10 * c;
             o[0] = z.r1; o[1] = z.pr1;
             o[2] = z.e.st; o[3] = z.e.ct; o[4] = z.e.sp; o[5] = z.e.cp;
@@ -411,118 +540,21 @@ __global__ void __launch_bounds__(kTileThreads, 2)
 
     // phase 3: faces of the tile
     double cand = 10000000.0;
-    for (int f = threadIdx.x; f < nf; f += blockDim.x) {
-        const ulonglong2 rec = TM.faces[fb + f];
-        const int ln[4] = {(int)(rec.x & 0xfff), (int)((rec.x >> 12) & 0xfff), (int)((rec.x >> 24) & 0xfff),
-                           (int)((rec.x >> 36) & 0xfff)};
-        const uint32_t code = (uint32_t)(rec.x >> 48);
-        const uint32_t bc = code & 15u, method = (code >> 4) & 15u, flags = (code >> 8) & 255u;
-        const int64_t a = (int32_t)(rec.y & 0xffffffffu);
-        const int64_t b = (int32_t)(rec.y >> 32);
-#pragma unroll
-        for (int i = 0; i < 4; i++) L.fnodes[4 * f + i] = (uint16_t)ln[i];
-        double x1[4][3];
-#pragma unroll
-        for (int i = 0; i < 4; i++) { x1[i][0] = L.x1[3 * ln[i]]; x1[i][1] = L.x1[3 * ln[i] + 1]; x1[i][2] = L.x1[3 * ln[i] + 2]; }
-        const bool aloc = (a >= c0 && a < c0 + ncl);
-        double ca[3];
-        if (aloc) { ca[0] = L.c1[3 * (a - c0)]; ca[1] = L.c1[3 * (a - c0) + 1]; ca[2] = L.c1[3 * (a - c0) + 2]; }
-        else { ca[0] = cc1g[3 * a]; ca[1] = cc1g[3 * a + 1]; ca[2] = cc1g[3 * a + 2]; }
-        double n[3], area, G[3];
-        face_geometry(x1, ca, gcl, n, area, G);
-        double w = 0.0;
-        if (moving) {
-            double x2[4][3];
-#pragma unroll
-            for (int i = 0; i < 4; i++) { x2[i][0] = L.x2[3 * ln[i]]; x2[i][1] = L.x2[3 * ln[i] + 1]; x2[i][2] = L.x2[3 * ln[i] + 2]; }
-            if (gcl) {
-                w = swept_velocity(x1, x2, n, area, dt);
-            } else {
-                double ddot = 0.0;
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const double d0 = x2[i][0] - x1[i][0], d1 = x2[i][1] - x1[i][1], d2 = x2[i][2] - x1[i][2];
-                    ddot = ddot + ((d0 * n[0] + d1 * n[1]) + d2 * n[2]);
-                }
-                w = ddot / 4 / dt;
+    bool pending = false;
+    for (int f = threadIdx.x; f < nf; f += blockDim.x)
+        pending |= !face_slot<RT>(f, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
+    if constexpr (sizeof(RT) == sizeof(double)) {
+        __syncthreads();
+    } else if (__syncthreads_or(pending)) {
+        // phase 3b: the slots the mixed solver marked ill-conditioned, solved in FP64
+        for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+            if (__builtin_isnan(L.flux[9 * f + 8])) {
+                face_slot<double>(f, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
+                atomicAdd(&S->n_fp64_faces, 1ull);
             }
         }
-        // face states (Snos_on_Gran)
-        double sA[8];
-        load_state(prim1, a, sA);
-        RT qL[8], qR[8];
-        bool okbc = true;
-        double cb[3] = {0.0, 0.0, 0.0};
-        if (bc != 0) {
-            double l[8], r[8];
-#pragma unroll
-            for (int i = 0; i < 8; i++) l[i] = sA[i];
-            if (bc == 3) {
-                if (n[0] < -0.95) {
-                    if (l[1] > P.v_inf / 7.0) l[1] = P.v_inf / 5.0;
-                } else if (l[1] * n[0] + l[2] * n[1] + l[3] * n[2] < 0.0) {
-                    l[1] = 0.1 * n[0]; l[2] = 0.1 * n[1]; l[3] = 0.1 * n[2];
-                }
-#pragma unroll
-                for (int i = 0; i < 8; i++) r[i] = l[i];
-            } else if (bc == 1 || bc == 2) {
-                const double *bs = M.bstate + 8 * (int64_t)(-b - 2);
-#pragma unroll
-                for (int i = 0; i < 8; i++) r[i] = bs[i];
-            } else {
-                okbc = (bc == 4);
-#pragma unroll
-                for (int i = 0; i < 8; i++) r[i] = l[i];
-            }
-#pragma unroll
-            for (int i = 0; i < 8; i++) { qL[i] = (RT)l[i]; qR[i] = (RT)r[i]; }
-        } else {
-            double sB[8];
-            load_state(prim1, b, sB);
-            const bool bloc = (b >= c0 && b < c0 + ncl);
-            if (bloc) { cb[0] = L.c1[3 * (b - c0)]; cb[1] = L.c1[3 * (b - c0) + 1]; cb[2] = L.c1[3 * (b - c0) + 2]; }
-            else { cb[0] = cc1g[3 * b]; cb[1] = cc1g[3 * b + 1]; cb[2] = cc1g[3 * b + 2]; }
-            const bool zA = ((M.cell_code[a] >> 8) & 15u) == 1u, zB = ((M.cell_code[b] >> 8) & 15u) == 1u;
-            RT tA[8], tB[8];
-#pragma unroll
-            for (int i = 0; i < 8; i++) { tA[i] = (RT)sA[i]; tB[i] = (RT)sB[i]; }
-            if (zA || zB) {
-                const Zone1Face<RT> zf = zone1_face<RT>(G, P.gamma);
-                if (zA) {
-                    const Zone1Cell<RT> zc = aloc ? load_zone1<RT>(L.z1 + 10 * (a - c0)) : zone1_cell<RT>(ca, P.gamma);
-                    zone1_apply<RT>(tA, zc, zf, qL);
-                } else {
-#pragma unroll
-                    for (int i = 0; i < 8; i++) qL[i] = tA[i];
-                }
-                if (zB) {
-                    const Zone1Cell<RT> zc = bloc ? load_zone1<RT>(L.z1 + 10 * (b - c0)) : zone1_cell<RT>(cb, P.gamma);
-                    zone1_apply<RT>(tB, zc, zf, qR);
-                } else {
-#pragma unroll
-                    for (int i = 0; i < 8; i++) qR[i] = tB[i];
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < 8; i++) { qL[i] = tA[i]; qR[i] = tB[i]; }
-            }
-        }
-        RiemannOut<RT, 0> o;
-        chlld<RT, 0>((int)method, P.n_disc, (int)flags, (RT)n[0], (RT)n[1], (RT)n[2], (RT)w, qL, qR, nullptr, nullptr, o);
-        if (!o.ok || !okbc) report_error(S, -3, TM.face_ref[fb + f]);
-        if (bc == 0 && o.ok) {
-            const double d0 = ca[0] - cb[0], d1 = ca[1] - cb[1], d2 = ca[2] - cb[2];
-            const double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2) / 2.0;
-            const double dnt = P.kfl * dist / (smax(fabs((double)o.dsl), fabs((double)o.dsr)) + fabs(w));
-            if (dnt < cand) cand = dnt;
-        }
-        double *fl = L.flux + 9 * f;
-#pragma unroll
-        for (int i = 0; i < 8; i++) fl[i] = (double)o.q[i] * area;
-        fl[8] = 0.5 * spf(n[0], n[1], n[2], (double)qL[5] + (double)qR[5], (double)qL[6] + (double)qR[6],
-                          (double)qL[7] + (double)qR[7]) * area;
+        __syncthreads();
     }
-    __syncthreads();
 
     if constexpr (SUR) {
         surrogate_update<RT>(TM, M, P, L, c0, ncl, prim1, prim2, sur, dt, gcl, moving, S);

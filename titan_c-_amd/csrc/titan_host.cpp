@@ -562,6 +562,7 @@ extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
         stats->n_rho_floor = (int64_t)hs.n_rho_floor;
         stats->n_p_floor = (int64_t)hs.n_p_floor;
         stats->n_fallback = (int64_t)hs.n_fallback;
+        stats->n_fp64_faces = (int64_t)hs.n_fp64_faces;
         stats->ms_total = ms;
     }
     return rc;
