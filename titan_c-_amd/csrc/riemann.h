@@ -263,6 +263,10 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
         const T rzL = r1 * suLm;
         const T ptz = (suR * r2 * ptL - suL * r1 * ptR + r1 * r2 * suR * suL * (vR[0] - vL[0])) /
                       (suR * r2 - suL * r1);
+        // A region left of the contact uses the left side's values and one right of it the right
+        // side's: selecting the operands first (a negation or a select is exact) gives one code
+        // path per kind of region, so a wavefront whose faces fall on both sides of the contact
+        // runs it once.
         if (n_state == 2) {  // HLLC, Phys_param.cpp:1258-1475
             T vzR1 = vR[1] + UZ[5] * (bR[1] - UZ[6]) / suR / r2;
             T vzR2 = vR[2] + UZ[5] * (bR[2] - UZ[7]) / suR / r2;
@@ -270,8 +274,6 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
             T vzL2 = vL[2] + UZ[5] * (bL[2] - UZ[7]) / suL / r1;
             T bzR1 = UZ[6], bzR2 = UZ[7], bzL1 = UZ[6], bzL2 = UZ[7];
             const T sbvz = (UZ[5] * UZ[1] + UZ[6] * UZ[2] + UZ[7] * UZ[3]) / UZ[0];
-            const T ezR = e2 * suRm + (ptz * SM - ptR * vR[0] + UZ[5] * (sbv2 - sbvz)) / (SR - SM);
-            const T ezL = e1 * suLm + (ptz * SM - ptL * vL[0] + UZ[5] * (sbv1 - sbvz)) / (SL - SM);
             if (fabs(UZ[5]) <= epsb) {
                 vzR1 = vR[1]; vzR2 = vR[2]; vzL1 = vL[1]; vzL2 = vL[2];
                 bzR1 = bR[1] * suRm; bzR2 = bR[2] * suRm;
@@ -283,34 +285,29 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
             else if (SM <= wv && SR > wv) region = 3;
             else if (SR <= wv) region = 4;
             else { region = 0; out.ok = 0; }
-            if (region == 2) {
-                const T UZL[8] = {rzL, SM * rzL, vzL1 * rzL, vzL2 * rzL, ezL, UZ[5], bzL1, bzL2};
+            const bool R = region >= 3;
+            if (region == 2 || region == 3) {
+                const T e = R ? e2 : e1, sm = R ? suRm : suLm, pt = R ? ptR : ptL, v0 = R ? vR[0] : vL[0];
+                const T sbv = R ? sbv2 : sbv1, S = R ? SR : SL, rz = R ? rzR : rzL;
+                const T ez = e * sm + (ptz * SM - pt * v0 + UZ[5] * (sbv - sbvz)) / (S - SM);
+                const T UZs[8] = {rz, SM * rz, (R ? vzR1 : vzL1) * rz, (R ? vzR2 : vzL2) * rz, ez, UZ[5],
+                                  R ? bzR1 : bzL1, R ? bzR2 : bzL2};
 #pragma unroll
-                for (int ik = 0; ik < 8; ik++) qq[ik] = fl(ik) + SL * (UZL[ik] - ul(ik)) - wv * UZL[ik];
+                for (int ik = 0; ik < 8; ik++) {
+                    const T f = R ? fr(ik) : fl(ik), u = R ? ur(ik) : ul(ik);
+                    qq[ik] = f + S * (UZs[ik] - u) - wv * UZs[ik];
+                }
 #pragma unroll
                 for (int ik = 0; ik < K; ik++) {
-                    const T z = kL[ik] * suLm;
-                    out.k[ik] = FLk[ik] + SL * (z - kL[ik]) - wv * z;
+                    const T ks = R ? kR[ik] : kL[ik];
+                    const T z = ks * sm;
+                    out.k[ik] = (R ? FRk[ik] : FLk[ik]) + S * (z - ks) - wv * z;
                 }
-            } else if (region == 3) {
-                const T UZR[8] = {rzR, SM * rzR, vzR1 * rzR, vzR2 * rzR, ezR, UZ[5], bzR1, bzR2};
-#pragma unroll
-                for (int ik = 0; ik < 8; ik++) qq[ik] = fr(ik) + SR * (UZR[ik] - ur(ik)) - wv * UZR[ik];
-#pragma unroll
-                for (int ik = 0; ik < K; ik++) {
-                    const T z = kR[ik] * suRm;
-                    out.k[ik] = FRk[ik] + SR * (z - kR[ik]) - wv * z;
-                }
-            } else if (region == 4) {
-#pragma unroll
-                for (int ik = 0; ik < 8; ik++) qq[ik] = fr(ik) - wv * ur(ik);
-#pragma unroll
-                for (int ik = 0; ik < K; ik++) out.k[ik] = FRk[ik] - wv * kR[ik];
             } else {
 #pragma unroll
-                for (int ik = 0; ik < 8; ik++) qq[ik] = fl(ik) - wv * ul(ik);
+                for (int ik = 0; ik < 8; ik++) qq[ik] = (R ? fr(ik) : fl(ik)) - wv * (R ? ur(ik) : ul(ik));
 #pragma unroll
-                for (int ik = 0; ik < K; ik++) out.k[ik] = FLk[ik] - wv * kL[ik];
+                for (int ik = 0; ik < K; ik++) out.k[ik] = (R ? FRk[ik] : FLk[ik]) - wv * (R ? kR[ik] : kL[ik]);
             }
         } else {  // HLLD, Phys_param.cpp:1476-1870
             const T bn = UZ[5];
@@ -330,8 +327,6 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
             const T bzR1 = bR[1] * tbR, bzR2 = bR[2] * tbR;
             const T sbvL = bn * SM + bzL1 * vzL1 + bzL2 * vzL2;
             const T sbvR = bn * SM + bzR1 * vzR1 + bzR2 * vzR2;
-            const T ezR = e2 * suRm + (ptz * SM - ptR * vR[0] + bn * (sbv2 - sbvR)) / (SR - SM);
-            const T ezL = e1 * suLm + (ptz * SM - ptL * vL[0] + bn * (sbv1 - sbvL)) / (SL - SM);
             const T rzRs = sqrt(rzR);
             const T rzLs = sqrt(rzL);
             const T rzss = rzRs + rzLs;
@@ -353,25 +348,23 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
             else if (SR <= wv) j = 6;
             if (j == -1) { out.ok = 0; j = 1; region = 0; }
             else region = j;
-            if (j == 1) {
+            const bool R = j >= 4;
+            if (j == 1 || j == 6) {
 #pragma unroll
-                for (int ik = 0; ik < 8; ik++) qq[ik] = fl(ik) - wv * ul(ik);
+                for (int ik = 0; ik < 8; ik++) qq[ik] = (R ? fr(ik) : fl(ik)) - wv * (R ? ur(ik) : ul(ik));
 #pragma unroll
-                for (int ik = 0; ik < K; ik++) out.k[ik] = FLk[ik] - wv * kL[ik];
-            } else if (j == 6) {
+                for (int ik = 0; ik < K; ik++) out.k[ik] = (R ? FRk[ik] : FLk[ik]) - wv * (R ? kR[ik] : kL[ik]);
+            } else {
+                const T e = R ? e2 : e1, sm = R ? suRm : suLm, pt = R ? ptR : ptL, v0 = R ? vR[0] : vL[0];
+                const T sbv0 = R ? sbv2 : sbv1, sbvs = R ? sbvR : sbvL, S = R ? SR : SL, rz = R ? rzR : rzL;
+                const T ez = e * sm + (ptz * SM - pt * v0 + bn * (sbv0 - sbvs)) / (S - SM);
+                const T vz1 = R ? vzR1 : vzL1, vz2 = R ? vzR2 : vzL2, bz1 = R ? bzR1 : bzL1, bz2 = R ? bzR2 : bzL2;
+                const T UZs[8] = {rz, SM * rz, vz1 * rz, vz2 * rz, ez, bn, bz1, bz2};
+                if (j == 2 || j == 5) {
 #pragma unroll
-                for (int ik = 0; ik < 8; ik++) qq[ik] = fr(ik) - wv * ur(ik);
-#pragma unroll
-                for (int ik = 0; ik < K; ik++) out.k[ik] = FRk[ik] - wv * kR[ik];
-            } else if (j == 2 || j == 3) {
-                const T UZL[8] = {rzL, SM * rzL, vzL1 * rzL, vzL2 * rzL, ezL, bn, bzL1, bzL2};
-                if (j == 2) {
-#pragma unroll
-                    for (int ik = 0; ik < 8; ik++) qq[ik] = fl(ik) + SL * (UZL[ik] - ul(ik)) - wv * UZL[ik];
-#pragma unroll
-                    for (int ik = 0; ik < K; ik++) {
-                        const T z = kL[ik] * suLm;
-                        out.k[ik] = FLk[ik] + SL * (z - kL[ik]) - wv * z;
+                    for (int ik = 0; ik < 8; ik++) {
+                        const T f = R ? fr(ik) : fl(ik), u = R ? ur(ik) : ul(ik);
+                        qq[ik] = f + S * (UZs[ik] - u) - wv * UZs[ik];
                     }
                 } else {
                     const T vzz1 = (rzLs * vzL1 + rzRs * vzR1 + sbn * (bzR1 - bzL1)) / rzss;
@@ -379,43 +372,24 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
                     const T bzz1 = (rzLs * bzR1 + rzRs * bzL1 + sbn * rzps * (vzR1 - vzL1)) / rzss;
                     const T bzz2 = (rzLs * bzR2 + rzRs * bzL2 + sbn * rzps * (vzR2 - vzL2)) / rzss;
                     const T sbzz = bn * SM + bzz1 * vzz1 + bzz2 * vzz2;
-                    const T ezzL = ezL - rzLs * sbn * (sbvL - sbzz);
-                    const T UZZL[8] = {rzL, SM * rzL, vzz1 * rzL, vzz2 * rzL, ezzL, bn, bzz1, bzz2};
+                    // ezzL = ezL - rzLs sbn (sbvL - sbzz), ezzR = ezR + rzRs sbn (sbvR - sbzz)
+                    const T dz = (R ? rzRs : rzLs) * sbn * (sbvs - sbzz);
+                    const T ezz = ez + (R ? dz : -dz);
+                    const T UZZ[8] = {rz, SM * rz, vzz1 * rz, vzz2 * rz, ezz, bn, bzz1, bzz2};
+                    const T SZ = R ? SZR : SZL;
 #pragma unroll
-                    for (int ik = 0; ik < 8; ik++)
-                        qq[ik] = fl(ik) + SZL * (UZZL[ik] - UZL[ik]) + SL * (UZL[ik] - ul(ik)) - wv * UZZL[ik];
-#pragma unroll
-                    for (int ik = 0; ik < K; ik++) {
-                        const T z = kL[ik] * suLm;
-                        out.k[ik] = FLk[ik] + SZL * (z - z) + SL * (z - kL[ik]) - wv * z;
+                    for (int ik = 0; ik < 8; ik++) {
+                        const T f = R ? fr(ik) : fl(ik), u = R ? ur(ik) : ul(ik);
+                        qq[ik] = f + SZ * (UZZ[ik] - UZs[ik]) + S * (UZs[ik] - u) - wv * UZZ[ik];
                     }
                 }
-            } else {  // j == 4 or 5
-                const T UZR[8] = {rzR, SM * rzR, vzR1 * rzR, vzR2 * rzR, ezR, bn, bzR1, bzR2};
-                if (j == 5) {
 #pragma unroll
-                    for (int ik = 0; ik < 8; ik++) qq[ik] = fr(ik) + SR * (UZR[ik] - ur(ik)) - wv * UZR[ik];
-#pragma unroll
-                    for (int ik = 0; ik < K; ik++) {
-                        const T z = kR[ik] * suRm;
-                        out.k[ik] = FRk[ik] + SR * (z - kR[ik]) - wv * z;
-                    }
-                } else {
-                    const T vzz1 = (rzLs * vzL1 + rzRs * vzR1 + sbn * (bzR1 - bzL1)) / rzss;
-                    const T vzz2 = (rzLs * vzL2 + rzRs * vzR2 + sbn * (bzR2 - bzL2)) / rzss;
-                    const T bzz1 = (rzLs * bzR1 + rzRs * bzL1 + sbn * rzps * (vzR1 - vzL1)) / rzss;
-                    const T bzz2 = (rzLs * bzR2 + rzRs * bzL2 + sbn * rzps * (vzR2 - vzL2)) / rzss;
-                    const T sbzz = bn * SM + bzz1 * vzz1 + bzz2 * vzz2;
-                    const T ezzR = ezR + rzRs * sbn * (sbvR - sbzz);
-                    const T UZZR[8] = {rzR, SM * rzR, vzz1 * rzR, vzz2 * rzR, ezzR, bn, bzz1, bzz2};
-#pragma unroll
-                    for (int ik = 0; ik < 8; ik++)
-                        qq[ik] = fr(ik) + SZR * (UZZR[ik] - UZR[ik]) + SR * (UZR[ik] - ur(ik)) - wv * UZZR[ik];
-#pragma unroll
-                    for (int ik = 0; ik < K; ik++) {
-                        const T z = kR[ik] * suRm;
-                        out.k[ik] = FRk[ik] + SZR * (z - z) + SR * (z - kR[ik]) - wv * z;
-                    }
+                for (int ik = 0; ik < K; ik++) {
+                    const T ks = R ? kR[ik] : kL[ik];
+                    const T z = ks * sm;
+                    const T SZk = (j == 3 || j == 4) ? (R ? SZR : SZL) : T(0);
+                    out.k[ik] = (j == 3 || j == 4) ? ((R ? FRk[ik] : FLk[ik]) + SZk * (z - z) + S * (z - ks) - wv * z)
+                                                   : ((R ? FRk[ik] : FLk[ik]) + S * (z - ks) - wv * z);
                 }
             }
         }

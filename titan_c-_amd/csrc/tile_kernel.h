@@ -43,6 +43,14 @@ __device__ __forceinline__ Trig<Z> frame_trig(Z z, Z x, Z y) {
     return t;
 }
 
+// r^(2 gamma): for gamma = 5/3, r^(10/3) = r^3 cbrt(r) (a cube root instead of a logarithm and an
+// exponential; within a few ulp of pow)
+template <typename Z>
+__device__ __forceinline__ Z pow_2gamma(Z r, double gamma) {
+    if (gamma == 5.0 / 3.0) return (r * r * r) * cbrt(r);
+    return pow(r, Z(2.0) * (Z)gamma);
+}
+
 // Zone_1 cell data: r1, r1^(2 gamma), trig of the equatorial (z,x,y) = (c2,c0,c1) and polar
 // (z,x,y) = (c0,c1,c2) frames.
 template <typename Z> struct Zone1Cell { Z r1, pr1; Trig<Z> e, p; };
@@ -52,7 +60,7 @@ __device__ __forceinline__ Zone1Cell<Z> zone1_cell(const double c[3], double gam
     const Z c0 = (Z)c[0], c1 = (Z)c[1], c2 = (Z)c[2];
     Zone1Cell<Z> z;
     z.r1 = sqrt((c0 * c0 + c1 * c1) + c2 * c2);
-    z.pr1 = pow(z.r1, Z(2.0) * (Z)gamma);
+    z.pr1 = pow_2gamma<Z>(z.r1, gamma);
     z.e = frame_trig<Z>(c2, c0, c1);
     z.p = frame_trig<Z>(c0, c1, c2);
     return z;
@@ -66,11 +74,12 @@ __device__ __forceinline__ Zone1Face<Z> zone1_face(const double G[3], double gam
     const Z g0 = (Z)G[0], g1 = (Z)G[1], g2 = (Z)G[2];
     Zone1Face<Z> f;
     f.rr = sqrt((g0 * g0 + g1 * g1) + g2 * g2);
-    f.prr = pow(f.rr, Z(2.0) * (Z)gamma);
-    // the frame choice keeps the reference's own test on theta (Setka_gran_snos.cpp:771-773)
-    const Z the1 = acos(g2 / f.rr);
-    const Z pi = Z(kPi);
-    f.equatorial = (the1 > pi / Z(9) && the1 < Z(8) * pi / Z(9));
+    f.prr = pow_2gamma<Z>(f.rr, gamma);
+    // the reference's frame test pi/9 < acos(G2/|G|) < 8 pi/9 (Setka_gran_snos.cpp:771-773), as
+    // |G2/|G|| < cos(pi/9); either frame gives the same rescaled state up to rounding
+    const Z ct = g2 / f.rr;
+    const Z c9 = Z(0.93969262078590838405);
+    f.equatorial = (ct < c9 && ct > -c9);
     f.t = f.equatorial ? frame_trig<Z>(g2, g0, g1) : frame_trig<Z>(g0, g1, g2);
     return f;
 }
