@@ -451,14 +451,11 @@ __global__ void __launch_bounds__(256) k_face_flux(DevMesh M, DevParams P, const
 // ---- cell update: cell loop of Setka::Go (Setka_physics.cpp:1695-2027), FP64 -------------------
 // Pf = signed flux sum of the cell (8 fluxes x area, PdivB); src (nullable): this cell's 5 source
 // values -- rate, S_m (3), S_E per unit volume and time.
-__device__ __forceinline__ void cell_update_one(const DevMesh &M, uint32_t code, int64_t c,
-                                                const double *__restrict__ prim_old, double Volume, double Volume2,
-                                                const double Pf[9], const double *src, double time,
-                                                StepScalars *S, double *__restrict__ prim_new) {
+__device__ __forceinline__ void cell_update_state(const DevMesh &M, uint32_t code, int64_t c, const double s[8],
+                                                  double Volume, double Volume2, const double Pf[9], const double *src,
+                                                  double time, StepScalars *S, double *__restrict__ prim_new) {
     const double cpi4 = 4.0 * kPi;
     const double g1 = 5.0 / 3.0 - 1.0;
-    double s[8];
-    load_state(prim_old, c, s);
     const double rho = s[0], vx = s[1], vy = s[2], vz = s[3], pp = s[4], bx = s[5], by = s[6], bz = s[7];
     double Smx = 0.0, Smy = 0.0, Smz = 0.0, SE = 0.0;
     if (src) { Smx = src[1]; Smy = src[2]; Smz = src[3]; SE = src[4]; }
@@ -498,6 +495,15 @@ __device__ __forceinline__ void cell_update_one(const DevMesh &M, uint32_t code,
     double4 *o = reinterpret_cast<double4 *>(prim_new + 8 * c);
     o[0] = make_double4(rho3, u3, v3, w3);
     o[1] = make_double4(p3, bx3, by3, bz3);
+}
+
+__device__ __forceinline__ void cell_update_one(const DevMesh &M, uint32_t code, int64_t c,
+                                                const double *__restrict__ prim_old, double Volume, double Volume2,
+                                                const double Pf[9], const double *src, double time,
+                                                StepScalars *S, double *__restrict__ prim_new) {
+    double s[8];
+    load_state(prim_old, c, s);
+    cell_update_state(M, code, c, s, Volume, Volume2, Pf, src, time, S, prim_new);
 }
 
 #ifndef TITAN_FLUX_ONLY

@@ -238,6 +238,7 @@ __device__ __forceinline__ Zone1Cell<Z> load_zone1(const double *z) {
 __device__ __forceinline__ double tile_volume(const double *X, const double cc[3], const uint16_t *fnodes,
                                               const uint16_t *lf, uint32_t code, int gcl) {
     double V = 0.0;
+#pragma unroll
     for (int j = 0; j < 6; j++) {
         const uint16_t *fn = fnodes + 4 * lf[j];
         const double *x0 = X + 3 * fn[0], *x1 = X + 3 * fn[1], *x2 = X + 3 * fn[2], *x3 = X + 3 * fn[3];
@@ -272,12 +273,20 @@ __device__ __forceinline__ double tile_volume(const double *X, const double cc[3
 // Riemann flux x area -> LDS slot, CFL candidate.  RT = double: chlld (parity); RT = float:
 // chlld_mixed -- a face it reports ill-conditioned gets NaN in its PdivB slot and false is
 // returned, for the FP64 pass over marked slots.
+// The face's record and the states of its cells (sB only for an interior face) come from the
+// caller, which issues those loads early (face_fetch).
+__device__ __forceinline__ void face_fetch(const ulonglong2 &rec, const double *__restrict__ prim1, double sA[8],
+                                           double sB[8]) {
+    load_state(prim1, (int32_t)(rec.y & 0xffffffffu), sA);
+    if (((rec.x >> 48) & 15u) == 0u) load_state(prim1, (int32_t)(rec.y >> 32), sB);
+}
+
 template <typename RT>
-__device__ __forceinline__ bool face_slot(int f, const TileMesh &TM, const DevMesh &M, const DevParams &P,
+__device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const double sA[8], const double sB[8],
+                                          const TileMesh &TM, const DevMesh &M, const DevParams &P,
                                           const TileLds &L, int fb, int64_t c0, int ncl,
                                           const double *__restrict__ prim1, const double *__restrict__ cc1g,
                                           bool moving, int gcl, double dt, StepScalars *S, double &cand) {
-    const ulonglong2 rec = TM.faces[fb + f];
     const int ln[4] = {(int)(rec.x & 0xfff), (int)((rec.x >> 12) & 0xfff), (int)((rec.x >> 24) & 0xfff),
                        (int)((rec.x >> 36) & 0xfff)};
     const uint32_t code = (uint32_t)(rec.x >> 48);
@@ -313,8 +322,6 @@ __device__ __forceinline__ bool face_slot(int f, const TileMesh &TM, const DevMe
         }
     }
     // face states (Snos_on_Gran)
-    double sA[8];
-    load_state(prim1, a, sA);
     double qL[8], qR[8];
     bool okbc = true;
     double cb[3] = {0.0, 0.0, 0.0};
@@ -342,8 +349,6 @@ __device__ __forceinline__ bool face_slot(int f, const TileMesh &TM, const DevMe
 #pragma unroll
         for (int i = 0; i < 8; i++) { qL[i] = l[i]; qR[i] = r[i]; }
     } else {
-        double sB[8];
-        load_state(prim1, b, sB);
         const bool bloc = (b >= c0 && b < c0 + ncl);
         if (bloc) { cb[0] = L.c1[3 * (b - c0)]; cb[1] = L.c1[3 * (b - c0) + 1]; cb[2] = L.c1[3 * (b - c0) + 2]; }
         else { cb[0] = cc1g[3 * b]; cb[1] = cc1g[3 * b + 1]; cb[2] = cc1g[3 * b + 2]; }
@@ -505,10 +510,24 @@ __global__ void __launch_bounds__(kTileThreads, 2)
     const int gcl = P.geom_mode;
     const double dt = S->dt;
     const double fac = S->motion_fac;
+    const int tid = threadIdx.x;
+    // Loads that do not depend on earlier phases are issued here, so their latency overlaps the
+    // phases before their use: this thread's first face record, its first two tile nodes, and
+    // the node list and code of its own cell.
+    ulonglong2 rec0 = {0ull, 0ull};
+    if (tid < nf) rec0 = TM.faces[fb + tid];
+    const int g_pre0 = tid < nn ? TM.nodes[nb + tid] : 0;
+    const int g_pre1 = tid + kTileThreads < nn ? TM.nodes[nb + tid + kTileThreads] : 0;
+    uint4 ln_pre = {0u, 0u, 0u, 0u};
+    uint32_t code_pre = 0u;
+    if (tid < ncl) {
+        ln_pre = *reinterpret_cast<const uint4 *>(TM.cell_lnodes + 8 * (c0 + tid));
+        code_pre = M.cell_code[c0 + tid];
+    }
 
     // phase 1: nodes of the tile, both layers
-    for (int i = threadIdx.x; i < nn; i += blockDim.x) {
-        const int64_t g = TM.nodes[nb + i];
+    for (int i = tid; i < nn; i += blockDim.x) {
+        const int64_t g = i == tid ? g_pre0 : (i == tid + kTileThreads ? g_pre1 : TM.nodes[nb + i]);
         const double a0 = X1g[3 * g], a1 = X1g[3 * g + 1], a2 = X1g[3 * g + 2];
         L.x1[3 * i] = a0; L.x1[3 * i + 1] = a1; L.x1[3 * i + 2] = a2;
         if (moving) {
@@ -521,8 +540,19 @@ __global__ void __launch_bounds__(kTileThreads, 2)
     const double *LX2 = moving ? L.x2 : L.x1;
 
     // phase 2: own cells -- centres at both layers, Zone_1 frames at now1
-    for (int c = threadIdx.x; c < ncl; c += blockDim.x) {
-        const uint16_t *ln = TM.cell_lnodes + 8 * (c0 + c);
+    for (int c = tid; c < ncl; c += blockDim.x) {
+        uint16_t ln[8];
+        uint32_t ccode;
+        if (c == tid) {
+            const uint32_t w4[4] = {ln_pre.x, ln_pre.y, ln_pre.z, ln_pre.w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) { ln[2 * i] = (uint16_t)(w4[i] & 0xffffu); ln[2 * i + 1] = (uint16_t)(w4[i] >> 16); }
+            ccode = code_pre;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; i++) ln[i] = TM.cell_lnodes[8 * (c0 + c) + i];
+            ccode = M.cell_code[c0 + c];
+        }
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
@@ -537,7 +567,7 @@ __global__ void __launch_bounds__(kTileThreads, 2)
             L.c2[3 * c] = b0 / 8; L.c2[3 * c + 1] = b1 / 8; L.c2[3 * c + 2] = b2 / 8;
             cc2g[3 * (c0 + c)] = b0 / 8; cc2g[3 * (c0 + c) + 1] = b1 / 8; cc2g[3 * (c0 + c) + 2] = b2 / 8;
         }
-        if (((M.cell_code[c0 + c] >> 8) & 15u) == 1u) {
+        if (((ccode >> 8) & 15u) == 1u) {
             const Zone1Cell<double> z = zone1_cell<double>(cA, P.gamma);
             double *o = L.z1 + 10 * c;
             o[0] = z.r1; o[1] = z.pr1;
@@ -545,20 +575,42 @@ __global__ void __launch_bounds__(kTileThreads, 2)
             o[6] = z.p.st; o[7] = z.p.ct; o[8] = z.p.sp; o[9] = z.p.cp;
         }
     }
+    // the states of this thread's first face, in flight across the barrier
+    double sA0[8], sB0[8];
+    if (tid < nf) face_fetch(rec0, prim1, sA0, sB0);
     __syncthreads();
 
     // phase 3: faces of the tile
     double cand = 10000000.0;
     bool pending = false;
-    for (int f = threadIdx.x; f < nf; f += blockDim.x)
-        pending |= !face_slot<RT>(f, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
+    if (tid < nf)  // the first face, with its prefetched record and states
+        pending = !face_slot<RT>(tid, rec0, sA0, sB0, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
+    for (int f = tid + kTileThreads; f < nf; f += kTileThreads) {
+        const ulonglong2 rec = TM.faces[fb + f];
+        double sA[8], sB[8];
+        face_fetch(rec, prim1, sA, sB);
+        pending |= !face_slot<RT>(f, rec, sA, sB, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
+    }
+    // the update phase's loads of this thread's cell, in flight across the barrier
+    uint32_t code4 = 0u;
+    uint32_t lf_pre[3] = {0u, 0u, 0u};
+    double s4[8];
+    if (!SUR && tid < ncl) {
+        code4 = M.cell_code[c0 + tid];
+        const uint32_t *lfw = reinterpret_cast<const uint32_t *>(TM.cell_lfaces + 6 * (c0 + tid));
+        lf_pre[0] = lfw[0]; lf_pre[1] = lfw[1]; lf_pre[2] = lfw[2];
+        load_state(prim1, c0 + tid, s4);
+    }
     if constexpr (sizeof(RT) == sizeof(double)) {
         __syncthreads();
     } else if (__syncthreads_or(pending)) {
         // phase 3b: the slots the mixed solver marked ill-conditioned, solved in FP64
-        for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+        for (int f = tid; f < nf; f += blockDim.x) {
             if (__builtin_isnan(L.flux[9 * f + 8])) {
-                face_slot<double>(f, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
+                const ulonglong2 rec = TM.faces[fb + f];
+                double sA[8], sB[8];
+                face_fetch(rec, prim1, sA, sB);
+                face_slot<double>(f, rec, sA, sB, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
                 atomicAdd(&S->n_fp64_faces, 1ull);
             }
         }
@@ -572,10 +624,23 @@ __global__ void __launch_bounds__(kTileThreads, 2)
     }
 
     // phase 4: own cells -- volumes, flux sum in reference order, update
-    for (int c = threadIdx.x; c < ncl; c += blockDim.x) {
+    for (int c = tid; c < ncl; c += blockDim.x) {
         const int64_t cg = c0 + c;
-        const uint32_t code = M.cell_code[cg];
-        const uint16_t *lf = TM.cell_lfaces + 6 * cg;
+        uint16_t lf[6];
+        uint32_t code;
+        double s[8];
+        if (c == tid) {
+            code = code4;
+#pragma unroll
+            for (int i = 0; i < 3; i++) { lf[2 * i] = (uint16_t)(lf_pre[i] & 0xffffu); lf[2 * i + 1] = (uint16_t)(lf_pre[i] >> 16); }
+#pragma unroll
+            for (int i = 0; i < 8; i++) s[i] = s4[i];
+        } else {
+            code = M.cell_code[cg];
+#pragma unroll
+            for (int i = 0; i < 6; i++) lf[i] = TM.cell_lfaces[6 * cg + i];
+            load_state(prim1, cg, s);
+        }
         const double cA[3] = {L.c1[3 * c], L.c1[3 * c + 1], L.c1[3 * c + 2]};
         const double Volume = tile_volume(L.x1, cA, L.fnodes, lf, code, gcl);
         double Volume2 = Volume;
@@ -596,7 +661,7 @@ __global__ void __launch_bounds__(kTileThreads, 2)
                 Pf[i] += pos ? v : -v;
             }
         }
-        cell_update_one(M, code, cg, prim1, Volume, Volume2, Pf, src ? src + 5 * cg : nullptr, dt, S, prim2);
+        cell_update_state(M, code, cg, s, Volume, Volume2, Pf, src ? src + 5 * cg : nullptr, dt, S, prim2);
     }
     block_min_to(cand, &S->dtmin_bits[step & 1]);
 }
