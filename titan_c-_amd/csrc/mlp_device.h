@@ -27,8 +27,6 @@ constexpr int oW1 = 0, oB1 = oW1 + 64 * 8, oW2 = oB1 + 64, oB2 = oW2 + 64 * 64, 
 // packed operand bases (in lane-vectors) and bias bases
 constexpr int vL1 = 0, vL2 = 128, vL3 = 640, vL4 = 1152;
 constexpr int bL1 = 0, bL2 = 64, bL3 = 128, bL4 = 192;
-// bytes of the packed weights + biases + normalisation block when staged in LDS
-constexpr int kMlpLdsBytes = kMlpPackVec * 16 + kMlpPackBias * 4 + kMlpNorm * 4;
 // normalisation block offsets
 constexpr int kNormMean = 0, kNormStd = 8, kNormScale = 16, kNormLo = 21, kNormHi = 29;
 
@@ -93,13 +91,17 @@ __device__ __forceinline__ void load_packed(const uint16_t *__restrict__ wpack, 
 }
 
 // Raw input features of a cell: ln n_p, ln c_p^2 (= p/rho), ln n_H, ln c_H^2, V_p - V_H, |V_p - V_H|.
-// s = (rho, Vx, Vy, Vz, p, ...), hn = (n_H, c_H^2, V_H).
+// s = (rho, Vx, Vy, Vz, p, ...), hn = (n_H, c_H^2, V_H).  The logarithms are FP32 (the features
+// are rounded to BF16 for the network); the velocity differences and their norm are FP64.
 __device__ __forceinline__ void mlp_features(const double *s, const double *hn, float xr[8]) {
     const double d0 = s[1] - hn[2], d1 = s[2] - hn[3], d2 = s[3] - hn[4];
-    xr[0] = (float)log(s[0]);
-    xr[1] = (float)log(s[4] / s[0]);
-    xr[2] = (float)log(hn[0]);
-    xr[3] = (float)log(hn[1]);
+    // hardware log2, scaled after the difference (no multiply-add for a contracting unit to fuse)
+    const float ln2 = 0.69314718055994531f;
+    const float l2r = __builtin_amdgcn_logf((float)s[0]);
+    xr[0] = l2r * ln2;
+    xr[1] = (__builtin_amdgcn_logf((float)s[4]) - l2r) * ln2;
+    xr[2] = __builtin_amdgcn_logf((float)hn[0]) * ln2;
+    xr[3] = __builtin_amdgcn_logf((float)hn[1]) * ln2;
     xr[4] = (float)d0;
     xr[5] = (float)d1;
     xr[6] = (float)d2;

@@ -189,15 +189,11 @@ struct TileLds {
     double *x1, *x2;   // [max_nodes*3]
     double *c1, *c2;   // [T*3]
     double *z1;        // [T*10] Zone_1 cell data (r1, r1^2g, trig e, trig p)
-    double *flux;      // [tile_flux_doubles(max_faces)]; in the surrogate step, after the flux
-                       // sums are taken, the same slots hold the network (kMlpLdsBytes)
+    double *flux;      // [max_faces*9]
     uint16_t *fnodes;  // [max_faces*4] local nodes of each face slot
 };
 
-__host__ __device__ inline size_t tile_flux_doubles(int max_faces) {
-    const size_t a = 9 * (size_t)max_faces, b = (kMlpLdsBytes + 7) / 8;
-    return a > b ? a : b;
-}
+__host__ __device__ inline size_t tile_flux_doubles(int max_faces) { return 9 * (size_t)max_faces; }
 
 // LDS carve-up; the second node layer and second centres exist only on a moving mesh.
 __device__ __forceinline__ TileLds tile_lds(double *base, const TileMesh &TM, bool moving) {
@@ -402,7 +398,9 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
 }
 
 // Phase 4 of the surrogate step.  Lane l < 32 of wavefront w owns cell 32w + l of the tile (tile
-// cells <= 32 * waves); lanes l and l + 32 run the network pass of that cell together.
+// cells <= 32 * waves); lanes l and l + 32 run the network pass of that cell together.  The
+// network's operands are read from global memory (23 KB, cache-resident across the tiles), so
+// the phase needs no barrier and each wavefront proceeds on its own.
 template <typename RT>
 __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMesh &M, const DevParams &P,
                                                  const TileLds &L, int64_t c0, int ncl,
@@ -410,36 +408,13 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
                                                  const SurArgs &sur, double dt, int gcl, bool moving,
                                                  StepScalars *S) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (32 * wave >= ncl) return;  // wavefront-uniform: no cell of this wavefront
     const int cl = 32 * wave + (lane & 31);
     const bool mine = lane < 32 && cl < ncl;
     const int64_t cg = c0 + cl;
-    uint32_t code = 0;
-    double Pf[9];
-#pragma unroll
-    for (int i = 0; i < 9; i++) Pf[i] = 0.0;
-    if (mine) {
-        code = M.cell_code[cg];
-        const uint16_t *lf = TM.cell_lfaces + 6 * cg;
-#pragma unroll
-        for (int j = 0; j < 6; j++) {
-            const bool pos = (code >> j) & 1u;
-            const double *fl = L.flux + 9 * lf[j];
-#pragma unroll
-            for (int i = 0; i < 9; i++) {
-                const double v = fl[i];
-                Pf[i] += pos ? v : -v;
-            }
-        }
-    }
-    __syncthreads();  // every flux slot has been read: the slots now take the network
-    bf16x8 *sW = reinterpret_cast<bf16x8 *>(L.flux);
-    float *sB = reinterpret_cast<float *>(sW + kMlpPackVec);
-    float *sN = sB + kMlpPackBias;
-    load_packed(sur.wpack, sur.bpack, sW, sB);
-    for (int i = threadIdx.x; i < kMlpNorm; i += blockDim.x) sN[i] = sur.norm[i];
-    __syncthreads();
-    if (32 * wave >= ncl) return;  // wavefront-uniform: no cell of this wavefront
     double st[8] = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0}, hn[5] = {1.0, 1.0, 0.0, 0.0, 0.0};
+    uint32_t code = 0;
+    uint16_t lf[6] = {0, 0, 0, 0, 0, 0};
     float xr[8];
 #pragma unroll
     for (int e = 0; e < 8; e++) xr[e] = 0.0f;
@@ -448,16 +423,31 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
         load_state(prim1, cg, st);
 #pragma unroll
         for (int k = 0; k < 5; k++) hn[k] = sur.neutrals[5 * cg + k];
+        code = M.cell_code[cg];
+#pragma unroll
+        for (int j = 0; j < 6; j++) lf[j] = TM.cell_lfaces[6 * cg + j];
         mlp_features(st, hn, xr);
-        inbox = mlp_in_box(xr, sN);
+        inbox = mlp_in_box(xr, sur.norm);
     }
     bf16x8 in;
 #pragma unroll
-    for (int e = 0; e < 8; e++) in[e] = (lane < 32) ? mlp_normalise(xr[e], sN, e) : (__bf16)0.0f;
-    const f32x16 o = mlp_pass(sW, sB, in, lane);
+    for (int e = 0; e < 8; e++) in[e] = (lane < 32) ? mlp_normalise(xr[e], sur.norm, e) : (__bf16)0.0f;
+    const f32x16 o = mlp_pass(reinterpret_cast<const bf16x8 *>(sur.wpack), sur.bpack, in, lane);
     const float y4 = __shfl(o[0], (lane & 31) + 32, 64);  // output 4 of this cell sits in lane l + 32
     if (!mine) return;
-    const uint16_t *lf = TM.cell_lfaces + 6 * cg;
+    double Pf[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) Pf[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        const bool pos = (code >> j) & 1u;
+        const double *fl = L.flux + 9 * lf[j];
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            const double v = fl[i];
+            Pf[i] += pos ? v : -v;
+        }
+    }
     const double cA[3] = {L.c1[3 * cl], L.c1[3 * cl + 1], L.c1[3 * cl + 2]};
     const double Volume = tile_volume(L.x1, cA, L.fnodes, lf, code, gcl);
     double Volume2 = Volume;
@@ -469,8 +459,8 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
         const float y[5] = {o[0], o[1], o[2], o[3], y4};
         const double vp[3] = {st[1], st[2], st[3]};
         double s5[5];
-        mlp_source(y, sN, st[0], hn[0], vp, s5);
-        cell_update_one(M, code, cg, prim1, Volume, Volume2, Pf, s5, dt, S, prim2);
+        mlp_source(y, sur.norm, st[0], hn[0], vp, s5);
+        cell_update_state(M, code, cg, st, Volume, Volume2, Pf, s5, dt, S, prim2);
     } else {
         const unsigned long long k = atomicAdd(&S->n_fallback, 1ull);
         if ((int64_t)k < sur.cap) {
