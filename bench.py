@@ -126,7 +126,7 @@ def train_surrogate(T, C, sur, prim, nh, n_states, seed=11):
     x, y, _ = sur.make_dataset(in10, nodes=100)
     t_label = time.perf_counter() - t0
     norm = sur.Normaliser(x, y)
-    epochs, lrs = (6, 4, 2), (2e-3, 5e-4, 1e-4)
+    epochs, lrs = (16, 10, 6), (2e-3, 5e-4, 1e-4)
     mesh_t, _, par_t = C.cube_brio_wu(n=4)
     with T.Setka(mesh_t, par_t) as s:
         t1 = time.perf_counter()
