@@ -77,7 +77,8 @@ constexpr int kTileThreads = 256;  // workgroup size of the fused step kernel
 
 struct TileMesh {
     int64_t n_tiles;
-    int32_t tile_cells, max_nodes, max_faces, pad;
+    int32_t tile_cells, max_nodes, max_faces;
+    int32_t vol_cached;         // vol[] holds each cell's volume at the step's old node layer
     const int32_t *node_off;    // [n_tiles+1] into nodes
     const int32_t *nodes;       // device node ids of each tile, ascending
     const int32_t *own;         // [n_tiles*2] device node range whose X2 this tile writes
@@ -86,6 +87,8 @@ struct TileMesh {
     const int32_t *face_ref;    // reference face id of each record (error reports only)
     const uint16_t *cell_lnodes;  // [Nc*8] tile-local node index of each cell node
     const uint16_t *cell_lfaces;  // [Nc*6] tile-local face slot of each cell face (reference order)
+    double *vol;                // [Nc] cell volumes: read at the old layer (if vol_cached), written at
+                                // the new one -- the next step's old layer (same nodes, same formula)
 };
 
 __host__ __device__ inline unsigned long long tile_face_pack(const int ln[4], uint32_t code) {

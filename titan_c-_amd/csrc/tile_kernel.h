@@ -448,13 +448,19 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
             Pf[i] += pos ? v : -v;
         }
     }
-    const double cA[3] = {L.c1[3 * cl], L.c1[3 * cl + 1], L.c1[3 * cl + 2]};
-    const double Volume = tile_volume(L.x1, cA, L.fnodes, lf, code, gcl);
+    double Volume;
+    if (TM.vol_cached) {
+        Volume = TM.vol[cg];
+    } else {
+        const double cA[3] = {L.c1[3 * cl], L.c1[3 * cl + 1], L.c1[3 * cl + 2]};
+        Volume = tile_volume(L.x1, cA, L.fnodes, lf, code, gcl);
+    }
     double Volume2 = Volume;
     if (moving) {
         const double cB[3] = {L.c2[3 * cl], L.c2[3 * cl + 1], L.c2[3 * cl + 2]};
         Volume2 = tile_volume(L.x2, cB, L.fnodes, lf, code, gcl);
     }
+    if (moving || !TM.vol_cached) TM.vol[cg] = Volume2;
     if (inbox) {
         const float y[5] = {o[0], o[1], o[2], o[3], y4};
         const double vp[3] = {st[1], st[2], st[3]};
@@ -585,8 +591,10 @@ __global__ void __launch_bounds__(kTileThreads, 2)
     uint32_t code4 = 0u;
     uint32_t lf_pre[3] = {0u, 0u, 0u};
     double s4[8];
+    double v4 = 0.0;
     if (!SUR && tid < ncl) {
         code4 = M.cell_code[c0 + tid];
+        if (TM.vol_cached) v4 = TM.vol[c0 + tid];
         const uint32_t *lfw = reinterpret_cast<const uint32_t *>(TM.cell_lfaces + 6 * (c0 + tid));
         lf_pre[0] = lfw[0]; lf_pre[1] = lfw[1]; lf_pre[2] = lfw[2];
         load_state(prim1, c0 + tid, s4);
@@ -619,8 +627,10 @@ __global__ void __launch_bounds__(kTileThreads, 2)
         uint16_t lf[6];
         uint32_t code;
         double s[8];
+        double vcache = 0.0;
         if (c == tid) {
             code = code4;
+            vcache = v4;
 #pragma unroll
             for (int i = 0; i < 3; i++) { lf[2 * i] = (uint16_t)(lf_pre[i] & 0xffffu); lf[2 * i + 1] = (uint16_t)(lf_pre[i] >> 16); }
 #pragma unroll
@@ -630,14 +640,19 @@ __global__ void __launch_bounds__(kTileThreads, 2)
 #pragma unroll
             for (int i = 0; i < 6; i++) lf[i] = TM.cell_lfaces[6 * cg + i];
             load_state(prim1, cg, s);
+            if (TM.vol_cached) vcache = TM.vol[cg];
         }
-        const double cA[3] = {L.c1[3 * c], L.c1[3 * c + 1], L.c1[3 * c + 2]};
-        const double Volume = tile_volume(L.x1, cA, L.fnodes, lf, code, gcl);
+        double Volume = vcache;
+        if (!TM.vol_cached) {
+            const double cA[3] = {L.c1[3 * c], L.c1[3 * c + 1], L.c1[3 * c + 2]};
+            Volume = tile_volume(L.x1, cA, L.fnodes, lf, code, gcl);
+        }
         double Volume2 = Volume;
         if (moving) {
             const double cB[3] = {L.c2[3 * c], L.c2[3 * c + 1], L.c2[3 * c + 2]};
             Volume2 = tile_volume(L.x2, cB, L.fnodes, lf, code, gcl);
         }
+        if (moving || !TM.vol_cached) TM.vol[cg] = Volume2;
         double Pf[9];
 #pragma unroll
         for (int i = 0; i < 9; i++) Pf[i] = 0.0;

@@ -67,6 +67,7 @@ struct titan_ctx {
             *d_t_face_ref = nullptr;
     uint64_t *d_t_faces = nullptr;
     uint16_t *d_t_cell_lnodes = nullptr, *d_t_cell_lfaces = nullptr;
+    double *d_vol = nullptr;  // cell volumes at the newest node layer (step kernel's cache)
     bool started = false;
     long long step = 0;
     int now1 = 1, now2 = 0;
@@ -295,6 +296,9 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
         T.node_off = c->d_t_node_off; T.nodes = c->d_t_nodes; T.own = c->d_t_own; T.face_off = c->d_t_face_off;
         T.faces = reinterpret_cast<const ulonglong2 *>(c->d_t_faces); T.face_ref = c->d_t_face_ref;
         T.cell_lnodes = c->d_t_cell_lnodes; T.cell_lfaces = c->d_t_cell_lfaces;
+        if (hipMalloc((void **)&c->d_vol, sizeof(double) * nc) != hipSuccess) { titan_destroy(c); return TITAN_E_HIP; }
+        T.vol = c->d_vol;
+        T.vol_cached = 0;
         c->tile_info[0] = T.tile_cells;
         c->tile_info[1] = T.max_nodes;
         c->tile_info[2] = T.max_faces;
@@ -344,7 +348,8 @@ extern "C" int titan_destroy(titan_ctx *c) {
                     c->d_face_ref, c->d_cell_ref, c->d_cell_code, c->d_face_code, c->d_bstate, c->d_disp,
                     c->d_X[0], c->d_X[1], c->d_prim[0], c->d_prim[1], c->d_geom[0], c->d_geom[1], c->d_flux,
                     c->d_src, c->d_neutrals, c->d_S, c->d_S2, c->d_t_node_off, c->d_t_nodes, c->d_t_own,
-                    c->d_t_face_off, c->d_t_face_ref, c->d_t_faces, c->d_t_cell_lnodes, c->d_t_cell_lfaces};
+                    c->d_t_face_off, c->d_t_face_ref, c->d_t_faces, c->d_t_cell_lnodes, c->d_t_cell_lfaces,
+                    c->d_vol};
     for (void *q : ptrs)
         if (q) hipFree(q);
     mlp_free(c->mlp);
@@ -385,6 +390,7 @@ extern "C" int titan_set_nodes(titan_ctx *c, const double *xyz, int layer) {
         for (int k = 0; k < 3; k++) h[3 * d + k] = xyz[3 * (int64_t)c->ord.node_d2r[d] + k];
     HIPCHK(hipMemcpyAsync(c->d_X[layer], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->tiles.vol_cached = 0;  // the cached volumes belong to the old coordinates
     return TITAN_OK;
 }
 
@@ -500,6 +506,7 @@ extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, c->stream));
     if (!c->started) {
+        c->tiles.vol_cached = 0;
         // Setka_physics.cpp:1486-1502: copy node and cell layers 0 -> 1, measure both
         HIPCHK(hipMemcpyAsync(c->d_X[1], c->d_X[0], sizeof(double) * nn3, hipMemcpyDeviceToDevice, c->stream));
         launch_measure(c->stream, M, c->d_X[0], c->geom[0], c->params.geom_mode);
@@ -537,6 +544,7 @@ extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
         launch_step_tile(c->stream, c->params.precision, c->tiles, M, c->dp, c->d_prim[n1], c->d_prim[n2], c->d_X[n1],
                          c->d_X[n2], c->d_disp, c->geom[n1].cell_center, c->geom[n2].cell_center, src,
                          fused ? &sur : nullptr, c->d_S, c->step);
+        c->tiles.vol_cached = 1;
         timer_mark(c, 0, false);
         if (fused) {
             timer_mark(c, 3, true);
