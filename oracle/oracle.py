@@ -21,12 +21,12 @@ _I64 = ctypes.c_int64
 class OrMesh(ctypes.Structure):
     _fields_ = [("n_nodes", _I64), ("n_cells", _I64), ("n_faces", _I64), ("cell_nodes", _P), ("cell_faces", _P),
                 ("face_nodes", _P), ("face_cells", _P), ("face_bc", _P), ("face_method", _P), ("face_flags", _P),
-                ("face_bstate", _P), ("cell_zone", _P), ("cell_is_tvd", _P)]
+                ("face_bstate", _P), ("cell_zone", _P), ("cell_is_tvd", _P), ("face_tvd", _P)]
 
 
 class OrParams(ctypes.Structure):
     _fields_ = [("gamma", _D), ("kfl", _D), ("v_inf", _D), ("move_mesh", ctypes.c_int32), ("n_disc", ctypes.c_int32),
-                ("motion_omega", _D), ("motion_phase", _D)]
+                ("motion_omega", _D), ("motion_phase", _D), ("tvd", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class OrGeom(ctypes.Structure):
@@ -59,6 +59,9 @@ def lib():
             "or_motion": (None, [_I64, _P, _P, _D, _D, _D, _D, _P]),
             "or_face_flux": (_I, [_P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
             "or_face_states": (_I, [_P, _P, _I64, _P, _P, _P, _P]),
+            "or_init_tvd": (None, [_P, _P, _P]),
+            "or_minmod": (_D, [_D, _D]),
+            "or_linear": (_D, [_D] * 7),
             "or_update": (_I, [_P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
             "or_go": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _P, _P, _P]),
             "or_new_connect": (_I64, [_I64, _P, _I64, _P, _P, _P]),
@@ -82,9 +85,10 @@ def _p(a):
 
 
 class OracleMesh:
-    """Holds contiguous copies of a configs.Mesh and its ctypes view."""
+    """Holds contiguous copies of a configs.Mesh and its ctypes view.  tvd=True also computes the
+    TVD neighbours of every face from the initial geometry (Setka::Init_TVD)."""
 
-    def __init__(self, mesh):
+    def __init__(self, mesh, tvd=False):
         self.m = mesh
         self.arrs = dict(
             cell_nodes=np.ascontiguousarray(mesh.cell_nodes, np.int32), cell_faces=np.ascontiguousarray(mesh.cell_faces, np.int32),
@@ -92,18 +96,27 @@ class OracleMesh:
             face_bc=np.ascontiguousarray(mesh.face_bc, np.int8), face_method=np.ascontiguousarray(mesh.face_method, np.int8),
             face_flags=None if mesh.face_flags is None else np.ascontiguousarray(mesh.face_flags, np.uint8),
             face_bstate=None if mesh.face_bstate is None else np.ascontiguousarray(mesh.face_bstate, np.float64),
-            cell_zone=np.ascontiguousarray(mesh.cell_zone, np.int8), cell_is_tvd=np.ascontiguousarray(mesh.cell_is_tvd, np.uint8))
+            cell_zone=np.ascontiguousarray(mesh.cell_zone, np.int8), cell_is_tvd=np.ascontiguousarray(mesh.cell_is_tvd, np.uint8),
+            face_tvd=None)
         c = OrMesh()
         c.n_nodes, c.n_cells, c.n_faces = mesh.n_nodes, mesh.n_cells, mesh.n_faces
         for k, v in self.arrs.items():
             setattr(c, k, _p(v))
         self.c = c
+        if tvd:
+            self.set_tvd(init_tvd(self, measure(self, mesh.node_xyz)))
+
+    def set_tvd(self, tvd):
+        self.arrs["face_tvd"] = np.ascontiguousarray(tvd, np.int32).reshape(-1, 2)
+        self.c.face_tvd = _p(self.arrs["face_tvd"])
 
 
 def params_c(params):
     p = OrParams()
     for k, _ in OrParams._fields_:
-        setattr(p, k, params[k])
+        if k == "pad":
+            continue
+        setattr(p, k, params.get(k, 0) if k == "tvd" else params[k])
     return p
 
 
@@ -268,4 +281,12 @@ def motion(x_old, disp, dt, t, omega, phase):
     disp = np.ascontiguousarray(disp, np.float64)
     out = np.empty_like(x_old)
     lib().or_motion(x_old.shape[0], _p(x_old), _p(disp), float(dt), float(t), float(omega), float(phase), _p(out))
+    return out
+
+
+def init_tvd(om, geom0):
+    """TVD neighbours [n_faces, 2] (AA, BB; -1 = none) from the layer-0 geometry."""
+    g = OrGeom(*[_p(geom0[k]) for k, _ in OrGeom._fields_])
+    out = np.empty((om.m.n_faces, 2), np.int32)
+    lib().or_init_tvd(ctypes.byref(om.c), ctypes.byref(g), _p(out))
     return out

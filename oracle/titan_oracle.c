@@ -807,6 +807,247 @@ static void zone1_state(const double *s, const double *Ac, const double *G, doub
     out[4] = s[4] * pow(r1, 2.0 * gamma) / pow(rr, 2.0 * gamma);
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* TVD: neighbour choice (Setka::Init_TVD, Setka_physics.cpp:648-744) and the limited
+ * reconstruction of Setka::Snos_on_Gran (Setka_gran_snos.cpp:186-736), limiter Help_funk.cpp:26-70. */
+
+static int or_signum(double x)
+{
+    if (x > 0.00000001) return 1;
+    else if (x < -0.00000001) return -1;
+    return 0;
+}
+
+double or_minmod(double x, double y)
+{
+    if (or_signum(x) + or_signum(y) == 0) return 0.0;
+    double ax = fabs(x), ay = fabs(y);
+    return ((or_signum(x) + or_signum(y)) / 2.0) * ((ay < ax) ? ay : ax);  /* std::min */
+}
+
+double or_linear(double x1, double t1, double x2, double t2, double x3, double t3, double y)
+{
+    double d = or_minmod((t1 - t2) / (x1 - x2), (t2 - t3) / (x2 - x3));
+    return (d * (y - x2) + t2);
+}
+
+static double or_linear2(double x1, double t1, double x2, double t2, double y)
+{
+    double d = (t1 - t2) / (x1 - x2);
+    return (d * (y - x2) + t2);
+}
+
+/* unit vector (Eigen normalized(): unchanged when the norm is 0) */
+static void or_normalize(double *v)
+{
+    double n = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    if (n > 0.0) { v[0] /= n; v[1] /= n; v[2] /= n; }
+}
+
+/* The neighbour of cell C across its face at slot k, or -1 (Help_funk.cpp:275-288). */
+static int64_t or_sosed(const or_mesh *m, int64_t C, int64_t gr)
+{
+    int64_t a = m->face_cells[2 * gr], b = m->face_cells[2 * gr + 1];
+    if (b < 0) return -1;
+    return a == C ? b : a;
+}
+
+/* Among C's other neighbours, the one whose centre is seen from the face centre most opposite to
+ * vec (smallest cosine; first on ties). */
+static int64_t or_tvd_pick(const or_mesh *m, const or_geom *g0, int64_t f, int64_t C, const double *vec)
+{
+    double best = 2.0;
+    int64_t pick = -1;
+    const double *G = g0->face_center + 3 * f;
+    for (int k = 0; k < 6; k++) {
+        int64_t gr = m->cell_faces[6 * C + k];
+        if (gr == f) continue;
+        int64_t cel = or_sosed(m, C, gr);
+        if (cel < 0) continue;
+        double v[3] = {g0->cell_center[3 * cel] - G[0], g0->cell_center[3 * cel + 1] - G[1],
+                       g0->cell_center[3 * cel + 2] - G[2]};
+        or_normalize(v);
+        double cosine = (vec[0] * v[0] + vec[1] * v[1]) + vec[2] * v[2];
+        if (cosine < best) { best = cosine; pick = cel; }
+    }
+    return pick;
+}
+
+void or_init_tvd(const or_mesh *m, const or_geom *g0, int32_t *tvd)
+{
+    for (int64_t f = 0; f < m->n_faces; f++) {
+        const double *n = g0->face_normal + 3 * f;
+        const double *G = g0->face_center + 3 * f;
+        int64_t AA = or_tvd_pick(m, g0, f, m->face_cells[2 * f], n);
+        tvd[2 * f] = (int32_t)AA;
+        tvd[2 * f + 1] = -1;
+        if (m->face_cells[2 * f + 1] < 0) continue;  /* boundary face: no second pick, no check */
+        double mn[3] = {-n[0], -n[1], -n[2]};
+        int64_t BB = or_tvd_pick(m, g0, f, m->face_cells[2 * f + 1], mn);
+        tvd[2 * f + 1] = (int32_t)BB;
+        /* drop a neighbour that is not behind its cell (Setka_physics.cpp:724-738) */
+        if (AA >= 0) {
+            double k3[3] = {g0->cell_center[3 * AA] - G[0], g0->cell_center[3 * AA + 1] - G[1],
+                            g0->cell_center[3 * AA + 2] - G[2]};
+            or_normalize(k3);
+            if (-((n[0] * k3[0] + n[1] * k3[1]) + n[2] * k3[2]) < 0.1) tvd[2 * f] = -1;
+        }
+        if (BB >= 0) {
+            double k3[3] = {g0->cell_center[3 * BB] - G[0], g0->cell_center[3 * BB + 1] - G[1],
+                            g0->cell_center[3 * BB + 2] - G[2]};
+            or_normalize(k3);
+            if (((n[0] * k3[0] + n[1] * k3[1]) + n[2] * k3[2]) < 0.1) tvd[2 * f + 1] = -1;
+        }
+    }
+}
+
+static double or_norm3(const double *v) { return sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); }
+
+/* Spherical components (Vr, Vphi, Vtheta) of the V (ik = 0) or B (ik = 1) of state s at centre c in
+ * the face's frame: equatorial roles (z,x,y) = (2,0,1), polar (0,1,2). */
+static void or_sph(const double *s, const double *c, int ik, int equatorial, double *o)
+{
+    const double *V = s + (ik == 0 ? 1 : 5);
+    if (equatorial) spherical_skorost(c[2], c[0], c[1], V[2], V[0], V[1], &o[0], &o[1], &o[2]);
+    else spherical_skorost(c[0], c[1], c[2], V[0], V[1], V[2], &o[0], &o[1], &o[2]);
+}
+
+static void or_cart(const double *G, int equatorial, const double *Vs, int ik, double *out)
+{
+    double *O = out + (ik == 0 ? 1 : 5);
+    if (equatorial) dekard_skorost(G[2], G[0], G[1], Vs[0], Vs[1], Vs[2], &O[2], &O[0], &O[1]);
+    else dekard_skorost(G[0], G[1], G[2], Vs[0], Vs[1], Vs[2], &O[0], &O[1], &O[2]);
+}
+
+/* Second-order states of an interior face whose four cells (AA, A | B, BB) are known; returns 0 if
+ * the face keeps the first-order states. */
+static int or_tvd_states(const or_mesh *m, const or_params *p, int64_t f, const double *prim,
+                         const or_geom *g, double *qL, double *qR)
+{
+    int64_t A = m->face_cells[2 * f], B = m->face_cells[2 * f + 1];
+    if (!p->tvd || !m->face_tvd || !m->cell_is_tvd[A] || !m->cell_is_tvd[B]) return 0;
+    int64_t AA = m->face_tvd[2 * f], BB = m->face_tvd[2 * f + 1];
+    if (AA < 0 || BB < 0) return 0;
+    const double *sAA = prim + 8 * AA, *sA = prim + 8 * A, *sB = prim + 8 * B, *sBB = prim + 8 * BB;
+    const double *G = g->face_center + 3 * f;
+    const double *Ac = g->cell_center + 3 * A, *Bc = g->cell_center + 3 * B;
+    const double *AAc = g->cell_center + 3 * AA, *BBc = g->cell_center + 3 * BB;
+    double v[3];
+    v[0] = Ac[0] - G[0]; v[1] = Ac[1] - G[1]; v[2] = Ac[2] - G[2];
+    double d1 = or_norm3(v);
+    v[0] = Bc[0] - G[0]; v[1] = Bc[1] - G[1]; v[2] = Bc[2] - G[2];
+    double d2 = or_norm3(v);
+    v[0] = Ac[0] - AAc[0]; v[1] = Ac[1] - AAc[1]; v[2] = Ac[2] - AAc[2];
+    double dd1 = or_norm3(v);
+    v[0] = Bc[0] - BBc[0]; v[1] = Bc[1] - BBc[1]; v[2] = Bc[2] - BBc[2];
+    double dd2 = or_norm3(v);
+    int zAA = m->cell_zone[AA], zA = m->cell_zone[A], zB = m->cell_zone[B], zBB = m->cell_zone[BB];
+    double gam2 = 2.0 * p->gamma;
+    if (zAA == 1 && zA == 1 && zB == 1 && zBB == 1) {
+        /* Setka_gran_snos.cpp:216-400: spherical components; B radial x r^2, tangential x r;
+         * rho x r^2, p x r^(2 gamma) */
+        double r3 = or_norm3(AAc), r1 = or_norm3(Ac), rr = or_norm3(G), r2 = or_norm3(Bc), r4 = or_norm3(BBc);
+        double the1 = acos(G[2] / rr);
+        int eq = (the1 > OR_PI / 9 && the1 < 8 * OR_PI / 9);
+        for (int ik = 0; ik < 2; ik++) {
+            double VAA[3], VA[3], VB[3], VBB[3], Vl[3], Vr[3];
+            or_sph(sAA, AAc, ik, eq, VAA); or_sph(sA, Ac, ik, eq, VA);
+            or_sph(sB, Bc, ik, eq, VB); or_sph(sBB, BBc, ik, eq, VBB);
+            for (int i = 0; i < 3; i++) {
+                if (ik == 1) {
+                    if (i == 0) {
+                        Vl[i] = or_linear(-dd1 - d1, VAA[i] * KV(r3), -d1, VA[i] * KV(r1), d2, VB[i] * KV(r2), 0.0) / KV(rr);
+                        Vr[i] = or_linear(-d1, VA[i] * KV(r1), d2, VB[i] * KV(r2), d2 + dd2, VBB[i] * KV(r4), 0.0) / KV(rr);
+                    } else {
+                        Vl[i] = or_linear(-dd1 - d1, VAA[i] * (r3), -d1, VA[i] * (r1), d2, VB[i] * (r2), 0.0) / (rr);
+                        Vr[i] = or_linear(-d1, VA[i] * (r1), d2, VB[i] * (r2), d2 + dd2, VBB[i] * (r4), 0.0) / (rr);
+                    }
+                } else {
+                    Vl[i] = or_linear(-dd1 - d1, VAA[i], -d1, VA[i], d2, VB[i], 0.0);
+                    Vr[i] = or_linear(-d1, VA[i], d2, VB[i], d2 + dd2, VBB[i], 0.0);
+                }
+            }
+            or_cart(G, eq, Vl, ik, qL);
+            or_cart(G, eq, Vr, ik, qR);
+        }
+        qL[0] = or_linear(-dd1 - d1, sAA[0] * KV(r3), -d1, sA[0] * KV(r1), d2, sB[0] * KV(r2), 0.0) / KV(rr);
+        qR[0] = or_linear(-d1, sA[0] * KV(r1), d2, sB[0] * KV(r2), d2 + dd2, sBB[0] * KV(r4), 0.0) / KV(rr);
+        qL[4] = or_linear(-dd1 - d1, sAA[4] * pow(r3, gam2), -d1, sA[4] * pow(r1, gam2), d2, sB[4] * pow(r2, gam2), 0.0) / pow(rr, gam2);
+        qR[4] = or_linear(-d1, sA[4] * pow(r1, gam2), d2, sB[4] * pow(r2, gam2), d2 + dd2, sBB[4] * pow(r4, gam2), 0.0) / pow(rr, gam2);
+    } else if (zAA == 1 && zA == 1 && zB == 1 && zBB == 2) {
+        /* :401-536: spherical V and B (unscaled), left limited over AA, A, B, right the A-B line;
+         * rho plain (the reference's r^2 assignment is overwritten by its final else), p x r^(2 gamma) */
+        double r3 = or_norm3(AAc), r1 = or_norm3(Ac), rr = or_norm3(G), r2 = or_norm3(Bc);
+        double the1 = acos(G[2] / rr);
+        int eq = (the1 > OR_PI / 9 && the1 < 8 * OR_PI / 9);
+        for (int ik = 0; ik < 2; ik++) {
+            double VAA[3], VA[3], VB[3], Vl[3], Vr[3];
+            or_sph(sAA, AAc, ik, eq, VAA); or_sph(sA, Ac, ik, eq, VA); or_sph(sB, Bc, ik, eq, VB);
+            for (int i = 0; i < 3; i++) {
+                Vl[i] = or_linear(-dd1 - d1, VAA[i], -d1, VA[i], d2, VB[i], 0.0);
+                Vr[i] = or_linear2(-d1, VA[i], d2, VB[i], 0.0);
+            }
+            or_cart(G, eq, Vl, ik, qL);
+            or_cart(G, eq, Vr, ik, qR);
+        }
+        qL[0] = or_linear(-dd1 - d1, sAA[0], -d1, sA[0], d2, sB[0], 0.0);
+        qR[0] = or_linear2(-d1, sA[0], d2, sB[0], 0.0);
+        qL[4] = or_linear(-dd1 - d1, sAA[4] * pow(r3, gam2), -d1, sA[4] * pow(r1, gam2), d2, sB[4] * pow(r2, gam2), 0.0) / pow(rr, gam2);
+        qR[4] = or_linear2(-d1, sA[4] * pow(r1, gam2), d2, sB[4] * pow(r2, gam2), 0.0) / pow(rr, gam2);
+    } else if (zAA == 1 && zA == 1 && zB == 2 && zBB == 2) {
+        /* :537-648: one-sided lines AA-A (left, spherical V and B) and B-BB (right, Cartesian);
+         * rho plain, p left x r^(2 gamma) */
+        double r3 = or_norm3(AAc), r1 = or_norm3(Ac), rr = or_norm3(G);
+        double the1 = acos(G[2] / rr);
+        int eq = (the1 > OR_PI / 9 && the1 < 8 * OR_PI / 9);
+        for (int ik = 0; ik < 2; ik++) {
+            double VAA[3], VA[3], Vl[3];
+            or_sph(sAA, AAc, ik, eq, VAA); or_sph(sA, Ac, ik, eq, VA);
+            for (int i = 0; i < 3; i++) Vl[i] = or_linear2(-dd1 - d1, VAA[i], -d1, VA[i], 0.0);
+            or_cart(G, eq, Vl, ik, qL);
+        }
+        for (int k = 1; k < 8; k++) {
+            if (k == 4) continue;
+            qR[k] = or_linear2(d2, sB[k], d2 + dd2, sBB[k], 0.0);
+        }
+        qL[0] = or_linear2(-dd1 - d1, sAA[0], -d1, sA[0], 0.0);
+        qR[0] = or_linear2(d2, sB[0], d2 + dd2, sBB[0], 0.0);
+        qL[4] = or_linear2(-dd1 - d1, sAA[4] * pow(r3, gam2), -d1, sA[4] * pow(r1, gam2), 0.0) / pow(rr, gam2);
+        qR[4] = or_linear2(d2, sB[4], d2 + dd2, sBB[4], 0.0);
+    } else if (zAA == 1 && zA == 2 && zB == 2 && zBB == 2) {  /* :649-660 */
+        for (int k = 0; k < 8; k++) {
+            qL[k] = or_linear2(-d1, sA[k], d2, sB[k], 0.0);
+            qR[k] = or_linear(-d1, sA[k], d2, sB[k], d2 + dd2, sBB[k], 0.0);
+        }
+    } else if ((zAA == 2 && zA == 2 && zB == 3 && zBB == 3) || (zAA == 3 && zA == 3 && zB == 4 && zBB == 4)) {  /* :661-673 */
+        for (int k = 0; k < 8; k++) {
+            qL[k] = or_linear2(-dd1 - d1, sAA[k], -d1, sA[k], 0.0);
+            qR[k] = or_linear2(d2, sB[k], d2 + dd2, sBB[k], 0.0);
+        }
+    } else if ((zAA == 2 && zA == 2 && zB == 2 && zBB == 3) || (zAA == 3 && zA == 3 && zB == 3 && zBB == 4)) {  /* :674-687 */
+        for (int k = 0; k < 8; k++) {
+            qL[k] = or_linear(-dd1 - d1, sAA[k], -d1, sA[k], d2, sB[k], 0.0);
+            qR[k] = or_linear2(-d1, sA[k], d2, sB[k], 0.0);
+        }
+    } else if ((zAA == 2 && zA == 3 && zB == 3 && zBB == 3) || (zAA == 3 && zA == 4 && zB == 4 && zBB == 4)) {  /* :688-701 */
+        for (int k = 0; k < 8; k++) {
+            qL[k] = or_linear2(-d1, sA[k], d2, sB[k], 0.0);
+            qR[k] = or_linear(-d1, sA[k], d2, sB[k], d2 + dd2, sBB[k], 0.0);
+        }
+    } else {  /* :702-713 */
+        for (int k = 0; k < 8; k++) {
+            qL[k] = or_linear(-dd1 - d1, sAA[k], -d1, sA[k], d2, sB[k], 0.0);
+            qR[k] = or_linear(-d1, sA[k], d2, sB[k], d2 + dd2, sBB[k], 0.0);
+        }
+    }
+    /* positivity (Setka_gran_snos.cpp:715-725) */
+    if (qL[0] < 0.0000001) qL[0] = sA[0];
+    if (qL[4] < 0.0000001) qL[4] = sA[4];
+    if (qR[0] < 0.0000001) qR[0] = sB[0];
+    if (qR[4] < 0.0000001) qR[4] = sB[4];
+    return 1;
+}
+
 int or_face_states(const or_mesh *m, const or_params *p, int64_t f, const double *prim,
                    const or_geom *g, double *qL, double *qR)
 {
@@ -839,6 +1080,7 @@ int or_face_states(const or_mesh *m, const or_params *p, int64_t f, const double
     int64_t B = m->face_cells[2 * f + 1];
     const double *sB = prim + 8 * B;
     const double *G = g->face_center + 3 * f;
+    if (or_tvd_states(m, p, f, prim, g, qL, qR)) return 0;
     if (m->cell_zone[A] == 1) zone1_state(sA, g->cell_center + 3 * A, G, p->gamma, qL);
     else for (int i = 0; i < 8; i++) qL[i] = sA[i];
     if (m->cell_zone[B] == 1) zone1_state(sB, g->cell_center + 3 * B, G, p->gamma, qR);

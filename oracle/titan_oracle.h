@@ -14,8 +14,10 @@
  *                         Gran::Culc_measure         TITAN/source/Gran.cpp:112-218
  *                         Cell::Culc_volume (m=1)    TITAN/source/Cell.cpp:958-1000
  *   or_face_velocity    Gran::culc_velosity          TITAN/source/Gran.cpp:571-589
- *   or_face_states      Setka::Snos_on_Gran (first order + boundary)
- *                                                    TITAN/source/Setka_gran_snos.cpp:3-180,737-1071
+ *   or_face_states      Setka::Snos_on_Gran (first order, TVD, boundary)
+ *                                                    TITAN/source/Setka_gran_snos.cpp:3-1071
+ *   or_init_tvd         Setka::Init_TVD              TITAN/source/Setka_physics.cpp:648-744
+ *   or_minmod, or_linear  minmod / linear           TITAN/source/Help_funk.cpp:26-70
  *   or_face_flux        Setka::Culc_Gran_Potok       TITAN/source/Setka_physics.cpp:2280-2571
  *   or_update           cell loop of Setka::Go       TITAN/source/Setka_physics.cpp:1671-2027
  *   or_go               Setka::Go                    TITAN/source/Setka_physics.cpp:1484-2278
@@ -39,7 +41,16 @@
  *   or_face_velocities  rigid translation gives w = d.n/dt.
  *   or_motion, or_det_cos  rigid translation; det_cos against libm cos.
  *   or_face_states      a spherically symmetric wind is carried onto every face exactly (both
- *                       spherical frames), boundary rules on known answers.
+ *                       spherical frames), boundary rules on known answers.  TVD: a linear field
+ *                       is reconstructed exactly (plain case and the one-sided lines at Zone_2/3
+ *                       and Zone_3/4 transitions), an extremum keeps the cell value, the
+ *                       positivity fallback, and the wind again through the all-Zone_1 case.
+ *                       The Zone_1/Zone_2 transition cases (Setka_gran_snos.cpp:401-660) are
+ *                       restated but their parity is UNPINNED: no invariant reproduces them
+ *                       exactly and no reference output exists for them.
+ *   or_init_tvd         straight-line neighbours on a regular cube; the side neighbour (cosine
+ *                       -1/sqrt 5) next to a boundary.
+ *   or_minmod, or_linear  known answers, including the 1e-8 sign threshold.
  *   or_update, or_go    mass conservation, quiescent state, floors at their thresholds, frozen B,
  *                       volume ratio and sources, lagged dt schedule, Brio-Wu profile.
  *   or_new_connect      hand-worked two-cell mesh and face counts.
@@ -74,7 +85,9 @@ typedef struct {
     const uint8_t *face_flags;   /* [n_faces] bit0 left_ydar (TS_hard), bit1 contact  */
     const double  *face_bstate;  /* [n_faces*8] fixed boundary state (rho,V,p,B)      */
     const int8_t  *cell_zone;    /* [n_cells] 1..4 (Type_cell::Zone_1..Zone_4)        */
-    const uint8_t *cell_is_tvd;  /* [n_cells] 0 => B frozen                            */
+    const uint8_t *cell_is_tvd;  /* [n_cells] 0 => B frozen and no TVD reconstruction  */
+    const int32_t *face_tvd;     /* [n_faces*2] TVD neighbours AA (behind cells[0]) and BB
+                                    (behind cells[1]), -1 = none (or_init_tvd); nullable */
 } or_mesh;
 
 typedef struct {
@@ -85,6 +98,8 @@ typedef struct {
     int32_t n_disc;      /* wave-speed estimate passed to chlld   */
     /* prescribed node motion: x += dt * (omega*cos(omega*t+phase)) * D_i */
     double motion_omega, motion_phase;
+    int32_t tvd;         /* TVD: second-order limited face states (Setka_gran_snos.cpp:186-736) */
+    int32_t pad;
 } or_params;
 
 typedef struct {
@@ -121,6 +136,12 @@ void or_face_velocities(const or_mesh *m, const double *x1, const double *x2, co
 double or_det_cos(double x);
 void or_motion(int64_t n_nodes, const double *x_old, const double *disp, double dt, double t,
                double omega, double phase, double *x_new);
+
+/* ---- TVD neighbours (Setka::Init_TVD, Setka_physics.cpp:648-744) from the layer-0 geometry:
+ * tvd[2f] = AA, tvd[2f+1] = BB (-1 = none).  Help_funk.cpp:26-70 limiter pieces exported for tests. */
+void or_init_tvd(const or_mesh *m, const or_geom *g0, int32_t *tvd);
+double or_minmod(double x, double y);
+double or_linear(double x1, double t1, double x2, double t2, double x3, double t3, double y);
 
 /* ---- face / cell passes ---- */
 int or_face_states(const or_mesh *m, const or_params *p, int64_t f, const double *prim,
