@@ -61,7 +61,8 @@ typedef struct {
     const int32_t *cell_faces;   /* [n_cells*6] Cell::grans, ascending face id                 */
     const int8_t *face_bc;       /* [n_faces] TITAN_BC_*                                       */
     const int8_t *face_method;   /* [n_faces] 0 Lax, 1 HLL, 2 HLLC, 3 HLLD (Gran::Get_method)  */
-    const uint8_t *face_flags;   /* [n_faces] bit0 left_ydar (TS_hard), bit1 contact; nullable */
+    const uint8_t *face_flags;   /* [n_faces] bit0 left_ydar (TS_hard), bit1 contact, bit2
+                                    heliopause face (Gran::type2 == HP); nullable              */
     const double *face_bstate;   /* [n_faces*8] fixed state of hard boundary faces; nullable   */
     const int8_t *cell_zone;     /* [n_cells] 1..4 = Type_cell::Zone_1..Zone_4                 */
     const uint8_t *cell_is_tvd;  /* [n_cells] Cell::is_TVD (0 => B frozen)                     */
@@ -81,6 +82,12 @@ typedef struct {
     int32_t source_mode;         /* TITAN_SRC_*                                                */
     int32_t cx_nodes;            /* quadrature nodes per direction of the exact source (e.g. 100) */
     double motion_omega, motion_phase;
+    int32_t tvd;                 /* [TVD]: second-order limited face states (Setka::Snos_on_Gran, 
+                                    Setka_gran_snos.cpp:186-736) on faces whose cells both have
+                                    is_TVD; neighbours chosen once from the initial nodes
+                                    (Setka::Init_TVD, Setka_physics.cpp:648-744)                */
+    int32_t snos_on_hp;          /* [Snos_on_HP]: TVD also on faces flagged heliopause (face_flags
+                                    bit2; Setka_gran_snos.cpp:188-189)                          */
 } titan_params;
 
 typedef struct {
@@ -135,6 +142,12 @@ int titan_gran_potok(titan_ctx *ctx, int now, double dt, double *face_flux, doub
 int titan_go(titan_ctx *ctx, int64_t steps, titan_step_stats *stats);
 /* Which layer holds the newest state after titan_go. */
 int titan_current_layer(titan_ctx *ctx);
+/* The second neighbours of every face chosen by Setka::Init_TVD (Setka_physics.cpp:648-744) from
+ * the initial nodes, in reference numbering: tvd [n_faces*2] = (AA behind cells[0], BB behind
+ * cells[1]), -1 where none qualifies (Gran::Sosed_1 / Sosed_2 = nullptr).  Independent of the
+ * params' tvd switch; the step uses a face's pair only if the switch is on and both of the
+ * face's cells have is_TVD. */
+int titan_tvd_cells(titan_ctx *ctx, int32_t *tvd);
 
 /* ---- Riemann solver ---------------------------------------------------------------------- */
 /* Phys_param::chlld (Phys_param.cpp:882-1878; Phys_param.h:240-251) over n independent face
@@ -194,7 +207,8 @@ int titan_mesh_shell(int32_t n_edge, int32_t n_layers, const double *radii, doub
 double titan_kernel_ms(titan_ctx *ctx, int which);
 int titan_set_timing(titan_ctx *ctx, int enable);
 /* Layout of the fused step: info8 = {cells per tile, max nodes per tile, max faces per tile,
- * LDS bytes per workgroup, device LDS limit, tiles, face records (border faces counted twice), 0}. */
+ * LDS bytes per workgroup, device LDS limit, tiles, face records (border faces counted twice),
+ * faces that reconstruct to second order}. */
 int titan_tile_info(titan_ctx *ctx, int64_t *info8);
 
 #ifdef __cplusplus

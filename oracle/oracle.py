@@ -26,7 +26,7 @@ class OrMesh(ctypes.Structure):
 
 class OrParams(ctypes.Structure):
     _fields_ = [("gamma", _D), ("kfl", _D), ("v_inf", _D), ("move_mesh", ctypes.c_int32), ("n_disc", ctypes.c_int32),
-                ("motion_omega", _D), ("motion_phase", _D), ("tvd", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("motion_omega", _D), ("motion_phase", _D), ("tvd", ctypes.c_int32), ("snos_on_hp", ctypes.c_int32)]
 
 
 class OrGeom(ctypes.Structure):
@@ -114,9 +114,7 @@ class OracleMesh:
 def params_c(params):
     p = OrParams()
     for k, _ in OrParams._fields_:
-        if k == "pad":
-            continue
-        setattr(p, k, params.get(k, 0) if k == "tvd" else params[k])
+        setattr(p, k, params.get(k, 0) if k in ("tvd", "snos_on_hp") else params[k])
     return p
 
 

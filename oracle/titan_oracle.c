@@ -926,6 +926,8 @@ static int or_tvd_states(const or_mesh *m, const or_params *p, int64_t f, const 
 {
     int64_t A = m->face_cells[2 * f], B = m->face_cells[2 * f + 1];
     if (!p->tvd || !m->face_tvd || !m->cell_is_tvd[A] || !m->cell_is_tvd[B]) return 0;
+    /* heliopause faces stay first order unless Snos_on_HP (Setka_gran_snos.cpp:188-189) */
+    if (m->face_flags && (m->face_flags[f] & 4) && !p->snos_on_hp) return 0;
     int64_t AA = m->face_tvd[2 * f], BB = m->face_tvd[2 * f + 1];
     if (AA < 0 || BB < 0) return 0;
     const double *sAA = prim + 8 * AA, *sA = prim + 8 * A, *sB = prim + 8 * B, *sBB = prim + 8 * BB;

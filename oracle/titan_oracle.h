@@ -82,7 +82,8 @@ typedef struct {
     const int32_t *face_cells;   /* [n_faces*2] cells[0], cells[1] (-1 on boundary)   */
     const int8_t  *face_bc;      /* [n_faces] OR_BC_*                                 */
     const int8_t  *face_method;  /* [n_faces] 0 Lax, 1 HLL, 2 HLLC, 3 HLLD            */
-    const uint8_t *face_flags;   /* [n_faces] bit0 left_ydar (TS_hard), bit1 contact  */
+    const uint8_t *face_flags;   /* [n_faces] bit0 left_ydar (TS_hard), bit1 contact,
+                                    bit2 heliopause face (Gran::type2 == HP)            */
     const double  *face_bstate;  /* [n_faces*8] fixed boundary state (rho,V,p,B)      */
     const int8_t  *cell_zone;    /* [n_cells] 1..4 (Type_cell::Zone_1..Zone_4)        */
     const uint8_t *cell_is_tvd;  /* [n_cells] 0 => B frozen and no TVD reconstruction  */
@@ -99,7 +100,7 @@ typedef struct {
     /* prescribed node motion: x += dt * (omega*cos(omega*t+phase)) * D_i */
     double motion_omega, motion_phase;
     int32_t tvd;         /* TVD: second-order limited face states (Setka_gran_snos.cpp:186-736) */
-    int32_t pad;
+    int32_t snos_on_hp;  /* Snos_on_HP: TVD also on heliopause faces (Phys_param.h:130)        */
 } or_params;
 
 typedef struct {

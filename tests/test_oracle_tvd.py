@@ -195,3 +195,39 @@ def test_zone_transition_cases_exact_on_linear_field(C, O, zones):
     # the three transition patterns and the plain case on both sides all occurred
     for z in ((a, a, b, b), (a, a, a, b), (a, b, b, b), (a, a, a, a), (b, b, b, b)):
         assert z in seen, (z, seen)
+
+
+def test_heliopause_flag_keeps_first_order(C, O):
+    """A face flagged heliopause (face_flags bit2, Gran::type2 == HP) reconstructs to second order
+    only with Snos_on_HP (Setka_gran_snos.cpp:188-189); the flag leaves the Riemann solver alone."""
+    mesh, _, params = C.cube_brio_wu(n=5, jitter=0.1)
+    g0 = O.measure(O.OracleMesh(mesh), mesh.node_xyz)
+    prim = _state_fn(g0["cell_center"])
+    prim[:, 0] += 0.3 * np.sin(7.0 * g0["cell_center"][:, 1])  # curvature: second order differs
+    flags = np.zeros(mesh.n_faces, np.uint8)
+    interior = np.nonzero(mesh.face_cells[:, 1] >= 0)[0]
+    flags[interior[::3]] = 4
+    mesh.face_flags = flags
+    om = O.OracleMesh(mesh, tvd=True)
+    g = O.measure(om, mesh.node_xyz)
+    tvd = om.arrs["face_tvd"]
+    n_diff = 0
+    for f in interior[:60]:
+        if tvd[f, 0] < 0 or tvd[f, 1] < 0:
+            continue
+        first = O.face_states(om, dict(params, tvd=0), prim, g, f)
+        hp_off = O.face_states(om, dict(params, tvd=1, snos_on_hp=0), prim, g, f)
+        hp_on = O.face_states(om, dict(params, tvd=1, snos_on_hp=1), prim, g, f)
+        if flags[f]:
+            np.testing.assert_array_equal(hp_off[1], first[1])
+            np.testing.assert_array_equal(hp_off[2], first[2])
+            n_diff += int(np.any(hp_on[1] != first[1]))
+        else:
+            np.testing.assert_array_equal(hp_off[1], hp_on[1])
+            np.testing.assert_array_equal(hp_off[2], hp_on[2])
+    assert n_diff > 3
+    # the Riemann solver ignores bit2: same fluxes as unflagged faces with TVD off
+    rc, fl4, _, _ = O.face_flux(om, dict(params, tvd=0), prim, g)
+    mesh.face_flags = None
+    rc, fl0, _, _ = O.face_flux(O.OracleMesh(mesh, tvd=True), dict(params, tvd=0), prim, g)
+    np.testing.assert_array_equal(fl4, fl0)

@@ -53,7 +53,10 @@ class ParamsC(ctypes.Structure):
     _fields_ = [("gamma", ctypes.c_double), ("kfl", ctypes.c_double), ("v_inf", ctypes.c_double),
                 ("move_mesh", ctypes.c_int32), ("n_disc", ctypes.c_int32), ("geom_mode", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("source_mode", ctypes.c_int32), ("cx_nodes", ctypes.c_int32),
-                ("motion_omega", ctypes.c_double), ("motion_phase", ctypes.c_double)]
+                ("motion_omega", ctypes.c_double), ("motion_phase", ctypes.c_double),
+                ("tvd", ctypes.c_int32), ("snos_on_hp", ctypes.c_int32)]
+
+_PARAM_DEFAULTS = {"tvd": 0, "snos_on_hp": 0}
 
 
 class StatsC(ctypes.Structure):
@@ -83,6 +86,7 @@ _SIGS = {
     "titan_gran_potok": (_I, [_P, _I, _D, _P, _P, _P]),
     "titan_go": (_I, [_P, _I64, ctypes.POINTER(StatsC)]),
     "titan_current_layer": (_I, [_P]),
+    "titan_tvd_cells": (_I, [_P, _P]),
     "titan_chlld_batch": (_I, [_I64, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _I]),
     "titan_sources": (_I, [_P, _I, _I, _P]),
     "titan_cx_exact_batch": (_I, [_I64, _P, _I, _P]),
@@ -229,7 +233,7 @@ class Setka:
         m.node_disp = keep(mesh.node_disp, np.float64)
         p = ParamsC()
         for k, _ in ParamsC._fields_:
-            setattr(p, k, params[k])
+            setattr(p, k, params.get(k, _PARAM_DEFAULTS[k]) if k in _PARAM_DEFAULTS else params[k])
         self.n_cells, self.n_faces, self.n_nodes = mesh.n_cells, mesh.n_faces, mesh.n_nodes
         self.params = dict(params)
         ctx = ctypes.c_void_p()
@@ -306,6 +310,12 @@ class Setka:
     def current_layer(self):
         return lib().titan_current_layer(self.ctx)
 
+    def tvd_cells(self):
+        """Setka::Init_TVD neighbours [n_faces, 2] (AA behind cells[0], BB behind cells[1], -1 = none)."""
+        out = np.empty((self.n_faces, 2), np.int32)
+        _check(lib().titan_tvd_cells(self.ctx, _ptr(out)), "titan_tvd_cells", self.ctx)
+        return out
+
     def sources(self, layer, mode):
         out = np.empty((self.n_cells, 5))
         _check(lib().titan_sources(self.ctx, layer, mode, _ptr(out)), "titan_sources", self.ctx)
@@ -320,7 +330,8 @@ class Setka:
     def tile_info(self):
         a = np.zeros(8, np.int64)
         _check(lib().titan_tile_info(self.ctx, _ptr(a)), "titan_tile_info", self.ctx)
-        keys = ("tile_cells", "max_nodes", "max_faces", "lds_bytes", "lds_limit", "tiles", "face_records")
+        keys = ("tile_cells", "max_nodes", "max_faces", "lds_bytes", "lds_limit", "tiles", "face_records",
+                "tvd_faces")
         return {k: int(v) for k, v in zip(keys, a)}
 
     # ---- surrogate ----

@@ -65,7 +65,7 @@ def splitmix_uniform(seed, index):
 
 def default_params(**kw):
     p = dict(gamma=GAMMA, kfl=0.4, v_inf=V_INF, move_mesh=0, n_disc=0, geom_mode=GEOM_REF, precision=PREC_FP64,
-             source_mode=SRC_NONE, cx_nodes=100, motion_omega=0.0, motion_phase=0.0)
+             source_mode=SRC_NONE, cx_nodes=100, motion_omega=0.0, motion_phase=0.0, tvd=0, snos_on_hp=0)
     p.update(kw)
     return p
 

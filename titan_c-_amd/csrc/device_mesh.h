@@ -35,6 +35,8 @@ struct DevMesh {
     const double *bstate;
     const int32_t *face_ref;   // device face -> reference face id (error reports)
     const int32_t *cell_ref;   // device cell -> reference cell id
+    const int32_t *face_tvd;   // [Nf*2] second neighbours (AA, BB) of the faces that reconstruct to
+                               // second order, (-1, -1) on every other face
 };
 
 struct DevGeom {
@@ -47,7 +49,7 @@ struct DevGeom {
 
 struct DevParams {
     double gamma, kfl, v_inf;
-    int32_t move_mesh, n_disc, geom_mode, pad;
+    int32_t move_mesh, n_disc, geom_mode, tvd;  // tvd: some face reconstructs to second order
     double motion_omega, motion_phase;
 };
 
@@ -87,6 +89,7 @@ struct TileMesh {
     const int32_t *face_ref;    // reference face id of each record (error reports only)
     const uint16_t *cell_lnodes;  // [Nc*8] tile-local node index of each cell node
     const uint16_t *cell_lfaces;  // [Nc*6] tile-local face slot of each cell face (reference order)
+    const int2 *face_tvd;       // per record: DevMesh::face_tvd of its face
     double *vol;                // [Nc] cell volumes: read at the old layer (if vol_cached), written at
                                 // the new one -- the next step's old layer (same nodes, same formula)
 };

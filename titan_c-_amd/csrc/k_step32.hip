@@ -20,11 +20,17 @@ void launch_step_tile_f32(hipStream_t s, const TileMesh &TM, const DevMesh &M, c
                           const double *prim1, double *prim2, const double *X1, double *X2, const double *disp,
                           const double *cc1, double *cc2, const double *src, const SurArgs *sur, StepScalars *S,
                           long long step) {
-    if (sur)
-        launch_tile_variant<float, true>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, nullptr, *sur, S, step);
+    const SurArgs none{};
+    const SurArgs &sa = sur ? *sur : none;
+    const double *sp = sur ? nullptr : src;
+    if (sur && P.tvd)
+        launch_tile_variant<float, true, true>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
+    else if (sur)
+        launch_tile_variant<float, true, false>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
+    else if (P.tvd)
+        launch_tile_variant<float, false, true>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
     else
-        launch_tile_variant<float, false>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, src, SurArgs{}, S,
-                                          step);
+        launch_tile_variant<float, false, false>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
 }
 
 template <int K>

@@ -51,11 +51,17 @@ void launch_step_tile(hipStream_t s, int precision, const TileMesh &TM, const De
         launch_step_tile_f32(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, src, sur, S, step);
         return;
     }
-    if (sur)
-        launch_tile_variant<double, true>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, nullptr, *sur, S, step);
+    const SurArgs none{};
+    const SurArgs &sa = sur ? *sur : none;
+    const double *sp = sur ? nullptr : src;
+    if (sur && P.tvd)
+        launch_tile_variant<double, true, true>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
+    else if (sur)
+        launch_tile_variant<double, true, false>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
+    else if (P.tvd)
+        launch_tile_variant<double, false, true>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
     else
-        launch_tile_variant<double, false>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, src, SurArgs{}, S,
-                                           step);
+        launch_tile_variant<double, false, false>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
 }
 
 size_t step_tile_lds_bytes(const TileMesh &TM, bool moving) {

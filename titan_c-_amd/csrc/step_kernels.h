@@ -10,6 +10,7 @@
 #include "device_mesh.h"
 #include "riemann.h"
 #include "riemann_mixed.h"
+#include "tvd.h"
 
 namespace titan {
 
@@ -337,6 +338,23 @@ __device__ __forceinline__ bool face_states(const DevMesh &M, const DevParams &P
     double sB[8];
     load_state(prim, B, sB);
     const uint32_t zA = (M.cell_code[A] >> 8) & 15u, zB = (M.cell_code[B] >> 8) & 15u;
+    if (P.tvd) {
+        const int64_t AA = M.face_tvd[2 * f], BB = M.face_tvd[2 * f + 1];
+        if (AA >= 0) {  // second-order face (both neighbours present)
+            double sAA[8], sBB[8];
+            load_state(prim, AA, sAA);
+            load_state(prim, BB, sBB);
+            const double *cc = G.cell_center;
+            const double cA[3] = {cc[3 * A], cc[3 * A + 1], cc[3 * A + 2]};
+            const double cB[3] = {cc[3 * B], cc[3 * B + 1], cc[3 * B + 2]};
+            const double cAA[3] = {cc[3 * AA], cc[3 * AA + 1], cc[3 * AA + 2]};
+            const double cBB[3] = {cc[3 * BB], cc[3 * BB + 1], cc[3 * BB + 2]};
+            const double Gc[3] = {G.face_center[3 * f], G.face_center[3 * f + 1], G.face_center[3 * f + 2]};
+            tvd_states((M.cell_code[AA] >> 8) & 15u, zA, zB, (M.cell_code[BB] >> 8) & 15u, sAA, sA, sB, sBB, cAA,
+                       cA, cB, cBB, Gc, P.gamma, qL, qR);
+            return true;
+        }
+    }
     if (zA == 1u || zB == 1u) {
         const double Gc[3] = {G.face_center[3 * f], G.face_center[3 * f + 1], G.face_center[3 * f + 2]};
         if (zA == 1u) {

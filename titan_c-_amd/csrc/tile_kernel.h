@@ -18,103 +18,10 @@
 #pragma once
 #include "step_kernels.h"
 #include "mlp_device.h"
+#include "sphere.h"
 
 namespace titan {
 
-
-template <typename Z> struct Trig { Z st, ct, sp, cp; };
-
-// (theta, phi) trig of spherical_skorost / dekard_skorost (Help_funk.cpp:108-135) for the axis
-// roles (z, x, y): theta = acos(z/r), phi = polar_angle(x, y) (Help_funk.cpp:78-106).  The
-// reference evaluates sin/cos of those angles; they equal rho/r, z/r, y/rho, x/rho with
-// rho = sqrt(x^2 + y^2) in every quadrant branch of polar_angle (phi = 0 when |x| + |y| < 1e-6),
-// so they are formed directly from the coordinates (one square root and four divisions instead of
-// acos, atan and four sin/cos; the values agree to a few ulp, the accuracy at which GPU and host
-// libm already differ).  The V and B rotations share them.
-template <typename Z>
-__device__ __forceinline__ Trig<Z> frame_trig(Z z, Z x, Z y) {
-    const Z r = sqrt(x * x + y * y + z * z);
-    const Z rho = sqrt(x * x + y * y);
-    Trig<Z> t;
-    t.st = rho / r;
-    t.ct = z / r;
-    if (fabs(x) + fabs(y) < Z(0.000001)) { t.sp = Z(0); t.cp = Z(1); }
-    else { t.sp = y / rho; t.cp = x / rho; }
-    return t;
-}
-
-// r^(2 gamma): for gamma = 5/3, r^(10/3) = r^3 cbrt(r) (a cube root instead of a logarithm and an
-// exponential; within a few ulp of pow)
-template <typename Z>
-__device__ __forceinline__ Z pow_2gamma(Z r, double gamma) {
-    if (gamma == 5.0 / 3.0) return (r * r * r) * cbrt(r);
-    return pow(r, Z(2.0) * (Z)gamma);
-}
-
-// Zone_1 cell data: r1, r1^(2 gamma), trig of the equatorial (z,x,y) = (c2,c0,c1) and polar
-// (z,x,y) = (c0,c1,c2) frames.
-template <typename Z> struct Zone1Cell { Z r1, pr1; Trig<Z> e, p; };
-
-template <typename Z>
-__device__ __forceinline__ Zone1Cell<Z> zone1_cell(const double c[3], double gamma) {
-    const Z c0 = (Z)c[0], c1 = (Z)c[1], c2 = (Z)c[2];
-    Zone1Cell<Z> z;
-    z.r1 = sqrt((c0 * c0 + c1 * c1) + c2 * c2);
-    z.pr1 = pow_2gamma<Z>(z.r1, gamma);
-    z.e = frame_trig<Z>(c2, c0, c1);
-    z.p = frame_trig<Z>(c0, c1, c2);
-    return z;
-}
-
-// Face-centre data of a face touching a Zone_1 cell: |G|, |G|^(2 gamma), frame choice and trig.
-template <typename Z> struct Zone1Face { Z rr, prr; bool equatorial; Trig<Z> t; };
-
-template <typename Z>
-__device__ __forceinline__ Zone1Face<Z> zone1_face(const double G[3], double gamma) {
-    const Z g0 = (Z)G[0], g1 = (Z)G[1], g2 = (Z)G[2];
-    Zone1Face<Z> f;
-    f.rr = sqrt((g0 * g0 + g1 * g1) + g2 * g2);
-    f.prr = pow_2gamma<Z>(f.rr, gamma);
-    // the reference's frame test pi/9 < acos(G2/|G|) < 8 pi/9 (Setka_gran_snos.cpp:771-773), as
-    // |G2/|G|| < cos(pi/9); either frame gives the same rescaled state up to rounding
-    const Z ct = g2 / f.rr;
-    const Z c9 = Z(0.93969262078590838405);
-    f.equatorial = (ct < c9 && ct > -c9);
-    f.t = f.equatorial ? frame_trig<Z>(g2, g0, g1) : frame_trig<Z>(g0, g1, g2);
-    return f;
-}
-
-// Setka_gran_snos.cpp:744-873 with precomputed trig: rotate V and B of a cell state to the face
-// centre (B radial x (r1/rr)^2, tangential x r1/rr), rho x r1^2/rr^2, p x (r1/rr)^(2 gamma).
-template <typename Z>
-__device__ __forceinline__ void zone1_apply(const Z s[8], const Zone1Cell<Z> &zc, const Zone1Face<Z> &zf, Z out[8]) {
-    const Trig<Z> a = zf.equatorial ? zc.e : zc.p;
-    const Trig<Z> b = zf.t;
-#pragma unroll
-    for (int ik = 0; ik < 2; ik++) {
-        const Z *V = s + (ik == 0 ? 1 : 5);
-        Z *O = out + (ik == 0 ? 1 : 5);
-        // axis roles (z, x, y)
-        const Z Vz = zf.equatorial ? V[2] : V[0];
-        const Z Vx = zf.equatorial ? V[0] : V[1];
-        const Z Vy = zf.equatorial ? V[1] : V[2];
-        Z Vr = Vx * a.st * a.cp + Vy * a.st * a.sp + Vz * a.ct;
-        Z Vtheta = Vx * a.ct * a.cp + Vy * a.ct * a.sp - Vz * a.st;
-        Z Vphi = -Vx * a.sp + Vy * a.cp;
-        if (ik == 1) {
-            Vr = Vr * ((zc.r1 / zf.rr) * (zc.r1 / zf.rr));
-            Vphi = Vphi * ((zc.r1) / (zf.rr));
-            Vtheta = Vtheta * ((zc.r1) / (zf.rr));
-        }
-        const Z ox = Vr * b.st * b.cp + Vtheta * b.ct * b.cp - Vphi * b.sp;
-        const Z oy = Vr * b.st * b.sp + Vtheta * b.ct * b.sp + Vphi * b.cp;
-        const Z oz = Vr * b.ct - Vtheta * b.st;
-        if (zf.equatorial) { O[2] = oz; O[0] = ox; O[1] = oy; }
-        else { O[0] = oz; O[1] = ox; O[2] = oy; }
-    }
-    out[0] = s[0] * (zc.r1 * zc.r1) / (zf.rr * zf.rr);
-    out[4] = s[4] * zc.pr1 / zf.prr;
-}
 
 // Face geometry at one node layer (Gran::Culc_measure, Gran.cpp:112-218; or the exactly
 // conservative variant); orientation from the centre of cells[0].
@@ -277,7 +184,7 @@ __device__ __forceinline__ void face_fetch(const ulonglong2 &rec, const double *
     if (((rec.x >> 48) & 15u) == 0u) load_state(prim1, (int32_t)(rec.y >> 32), sB);
 }
 
-template <typename RT>
+template <typename RT, bool TVD>
 __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const double sA[8], const double sB[8],
                                           const TileMesh &TM, const DevMesh &M, const DevParams &P,
                                           const TileLds &L, int fb, int64_t c0, int ncl,
@@ -348,11 +255,22 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
         const bool bloc = (b >= c0 && b < c0 + ncl);
         if (bloc) { cb[0] = L.c1[3 * (b - c0)]; cb[1] = L.c1[3 * (b - c0) + 1]; cb[2] = L.c1[3 * (b - c0) + 2]; }
         else { cb[0] = cc1g[3 * b]; cb[1] = cc1g[3 * b + 1]; cb[2] = cc1g[3 * b + 2]; }
-        const bool zA = ((M.cell_code[a] >> 8) & 15u) == 1u, zB = ((M.cell_code[b] >> 8) & 15u) == 1u;
+        const uint32_t zoneA = (M.cell_code[a] >> 8) & 15u, zoneB = (M.cell_code[b] >> 8) & 15u;
+        const bool zA = zoneA == 1u, zB = zoneB == 1u;
         double tA[8], tB[8];
 #pragma unroll
         for (int i = 0; i < 8; i++) { tA[i] = sA[i]; tB[i] = sB[i]; }
-        if (zA || zB) {
+        int2 tv = {-1, -1};
+        if constexpr (TVD) tv = TM.face_tvd[fb + f];
+        if (TVD && tv.x >= 0) {  // second order: the line AA - A -|- B - BB (both neighbours present)
+            double sAA[8], sBB[8];
+            load_state(prim1, tv.x, sAA);
+            load_state(prim1, tv.y, sBB);
+            const double cAA[3] = {cc1g[3 * (int64_t)tv.x], cc1g[3 * (int64_t)tv.x + 1], cc1g[3 * (int64_t)tv.x + 2]};
+            const double cBB[3] = {cc1g[3 * (int64_t)tv.y], cc1g[3 * (int64_t)tv.y + 1], cc1g[3 * (int64_t)tv.y + 2]};
+            tvd_states((M.cell_code[tv.x] >> 8) & 15u, zoneA, zoneB, (M.cell_code[tv.y] >> 8) & 15u, sAA, tA, tB, sBB,
+                       cAA, ca, cb, cBB, G, P.gamma, qL, qR);
+        } else if (zA || zB) {
             const Zone1Face<double> zf = zone1_face<double>(G, P.gamma);
             if (zA) {
                 const Zone1Cell<double> zc = aloc ? load_zone1<double>(L.z1 + 10 * (a - c0)) : zone1_cell<double>(ca, P.gamma);
@@ -487,7 +405,7 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
 // every flux sum is taken).  Cells outside the training box are not updated here: their flux
 // sums and volumes go to sur.stash at a slot of S->n_fallback, and k_cx_fixup redoes their update
 // with the exact source.  SUR = false: src (nullable) holds a precomputed source per cell.
-template <typename RT, bool SUR>
+template <typename RT, bool SUR, bool TVD>
 __global__ void __launch_bounds__(kTileThreads, 2)
     k_step_tile(TileMesh TM, DevMesh M, DevParams P, const double *__restrict__ prim1, double *__restrict__ prim2,
                 const double *__restrict__ X1g, double *__restrict__ X2g, const double *__restrict__ disp,
@@ -580,12 +498,12 @@ __global__ void __launch_bounds__(kTileThreads, 2)
     double cand = 10000000.0;
     bool pending = false;
     if (tid < nf)  // the first face, with its prefetched record and states
-        pending = !face_slot<RT>(tid, rec0, sA0, sB0, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
+        pending = !face_slot<RT, TVD>(tid, rec0, sA0, sB0, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
     for (int f = tid + kTileThreads; f < nf; f += kTileThreads) {
         const ulonglong2 rec = TM.faces[fb + f];
         double sA[8], sB[8];
         face_fetch(rec, prim1, sA, sB);
-        pending |= !face_slot<RT>(f, rec, sA, sB, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
+        pending |= !face_slot<RT, TVD>(f, rec, sA, sB, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
     }
     // the update phase's loads of this thread's cell, in flight across the barrier
     uint32_t code4 = 0u;
@@ -608,7 +526,7 @@ __global__ void __launch_bounds__(kTileThreads, 2)
                 const ulonglong2 rec = TM.faces[fb + f];
                 double sA[8], sB[8];
                 face_fetch(rec, prim1, sA, sB);
-                face_slot<double>(f, rec, sA, sB, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
+                face_slot<double, TVD>(f, rec, sA, sB, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
                 atomicAdd(&S->n_fp64_faces, 1ull);
             }
         }
@@ -673,7 +591,7 @@ __global__ void __launch_bounds__(kTileThreads, 2)
 
 // Launch of one k_step_tile instantiation; opens the kernel up to the tile's dynamic LDS size
 // once (hipFuncSetAttribute is a no-op below the default limit but required above it).
-template <typename RT, bool SUR>
+template <typename RT, bool SUR, bool TVD>
 inline void launch_tile_variant(hipStream_t s, const TileMesh &TM, const DevMesh &M, const DevParams &P,
                                 const double *prim1, double *prim2, const double *X1, double *X2,
                                 const double *disp, const double *cc1, double *cc2, const double *src,
@@ -681,13 +599,13 @@ inline void launch_tile_variant(hipStream_t s, const TileMesh &TM, const DevMesh
     const size_t lds = tile_lds_bytes(TM.max_nodes, TM.tile_cells, TM.max_faces, P.move_mesh != 0);
     static size_t attr_bytes = 0;  // dynamic LDS size this instantiation has been opened up to
     if (lds > attr_bytes) {
-        if (hipFuncSetAttribute((const void *)k_step_tile<RT, SUR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        if (hipFuncSetAttribute((const void *)k_step_tile<RT, SUR, TVD>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds) == hipSuccess)
             attr_bytes = lds;
         else
             (void)hipGetLastError();  // the launch below then reports the failure
     }
-    hipLaunchKernelGGL((k_step_tile<RT, SUR>), dim3((unsigned)TM.n_tiles), dim3(kTileThreads), lds, s, TM, M, P,
+    hipLaunchKernelGGL((k_step_tile<RT, SUR, TVD>), dim3((unsigned)TM.n_tiles), dim3(kTileThreads), lds, s, TM, M, P,
                        prim1, prim2, X1, X2, disp, cc1, cc2, src, sur, S, step);
 }
 

@@ -68,6 +68,10 @@ struct titan_ctx {
     uint64_t *d_t_faces = nullptr;
     uint16_t *d_t_cell_lnodes = nullptr, *d_t_cell_lfaces = nullptr;
     double *d_vol = nullptr;  // cell volumes at the newest node layer (step kernel's cache)
+    // second-order reconstruction: Init_TVD picks (device numbering) and the per-face / per-record
+    // pairs the kernels use
+    std::vector<int32_t> tvd_raw;
+    int32_t *d_face_tvd = nullptr, *d_t_face_tvd = nullptr;
     bool started = false;
     long long step = 0;
     int now1 = 1, now2 = 0;
@@ -82,6 +86,7 @@ struct titan_ctx {
         M.cell_nodes = d_cell_nodes; M.cell_faces = d_cell_faces; M.cell_code = d_cell_code;
         M.face_nodes = d_face_nodes; M.face_cells = d_face_cells; M.face_code = d_face_code;
         M.face_bidx = d_face_bidx; M.bstate = d_bstate; M.face_ref = d_face_ref; M.cell_ref = d_cell_ref;
+        M.face_tvd = d_face_tvd;
         return M;
     }
 };
@@ -192,6 +197,7 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
     c->params = *p;
     c->dp.gamma = p->gamma; c->dp.kfl = p->kfl; c->dp.v_inf = p->v_inf;
     c->dp.move_mesh = p->move_mesh; c->dp.n_disc = p->n_disc; c->dp.geom_mode = p->geom_mode;
+    c->dp.tvd = 0;
     c->dp.motion_omega = p->motion_omega; c->dp.motion_phase = p->motion_phase;
     if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return TITAN_E_HIP; }
 
@@ -240,6 +246,7 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
     }
     int rc = TITAN_OK;
     auto up = [&](int r) { if (rc == TITAN_OK) rc = r; };
+    std::vector<int32_t> rec_face;  // device face of each tile face record
     up(dev_upload(&c->d_cell_nodes, cell_nodes, c->stream));
     up(dev_upload(&c->d_cell_faces, cell_faces, c->stream));
     up(dev_upload(&c->d_face_nodes, face_nodes, c->stream));
@@ -306,6 +313,7 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
         c->tile_info[4] = lds_limit;
         c->tile_info[5] = T.n_tiles;
         c->tile_info[6] = (int64_t)td.face_ref.size();
+        rec_face.swap(td.face_dev);
     }
     const size_t geom_n = (size_t)(4 * nc + 7 * nf);
     for (int l = 0; l < 2; l++) {
@@ -337,6 +345,49 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
     launch_measure(c->stream, c->mesh(), c->d_X[0], c->geom[0], p->geom_mode);
     launch_measure(c->stream, c->mesh(), c->d_X[1], c->geom[1], p->geom_mode);
     if (hipStreamSynchronize(c->stream) != hipSuccess) { titan_destroy(c); return TITAN_E_HIP; }
+    // Setka::Init_TVD from the geometry of the initial nodes, then the pairs of the faces that
+    // reconstruct to second order (Setka_gran_snos.cpp:188-194): TVD on, both cells is_TVD, both
+    // neighbours present, not a heliopause face unless Snos_on_HP
+    {
+        std::vector<double> cc(3 * nc), fc(3 * nf), fn(3 * nf);
+        if (hipMemcpy(cc.data(), c->geom[0].cell_center, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(fc.data(), c->geom[0].face_center, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(fn.data(), c->geom[0].face_normal, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost) != hipSuccess) {
+            titan_destroy(c);
+            return TITAN_E_HIP;
+        }
+        c->tvd_raw.assign(2 * nf, -1);
+        init_tvd(nf, cell_faces.data(), face_cells.data(), cc.data(), fc.data(), fn.data(), c->tvd_raw.data());
+        std::vector<int32_t> pair(2 * nf, -1);
+        int64_t n_active = 0;
+        if (p->tvd) {
+            for (int64_t d = 0; d < nf; d++) {
+                const int32_t a = face_cells[2 * d], b = face_cells[2 * d + 1];
+                const int32_t AA = c->tvd_raw[2 * d], BB = c->tvd_raw[2 * d + 1];
+                if (b < 0 || AA < 0 || BB < 0) continue;
+                if (!((cell_code[a] >> 6) & 1u) || !((cell_code[b] >> 6) & 1u)) continue;
+                const uint32_t fflags = (face_code[d] >> 8) & 255u;
+                if ((fflags & 4u) && !p->snos_on_hp) continue;
+                pair[2 * d] = AA;
+                pair[2 * d + 1] = BB;
+                n_active++;
+            }
+        }
+        std::vector<int32_t> rec_pair(2 * rec_face.size());
+        for (size_t r = 0; r < rec_face.size(); r++) {
+            rec_pair[2 * r] = pair[2 * (int64_t)rec_face[r]];
+            rec_pair[2 * r + 1] = pair[2 * (int64_t)rec_face[r] + 1];
+        }
+        up(dev_upload(&c->d_face_tvd, pair, c->stream));
+        up(dev_upload(&c->d_t_face_tvd, rec_pair, c->stream));
+        if (rc != TITAN_OK || hipStreamSynchronize(c->stream) != hipSuccess) {
+            titan_destroy(c);
+            return rc != TITAN_OK ? rc : TITAN_E_HIP;
+        }
+        c->tiles.face_tvd = reinterpret_cast<const int2 *>(c->d_t_face_tvd);
+        c->dp.tvd = n_active > 0 ? 1 : 0;
+        c->tile_info[7] = n_active;
+    }
     *out = c;
     return TITAN_OK;
 }
@@ -349,7 +400,7 @@ extern "C" int titan_destroy(titan_ctx *c) {
                     c->d_X[0], c->d_X[1], c->d_prim[0], c->d_prim[1], c->d_geom[0], c->d_geom[1], c->d_flux,
                     c->d_src, c->d_neutrals, c->d_S, c->d_S2, c->d_t_node_off, c->d_t_nodes, c->d_t_own,
                     c->d_t_face_off, c->d_t_face_ref, c->d_t_faces, c->d_t_cell_lnodes, c->d_t_cell_lfaces,
-                    c->d_vol};
+                    c->d_vol, c->d_face_tvd, c->d_t_face_tvd};
     for (void *q : ptrs)
         if (q) hipFree(q);
     mlp_free(c->mlp);
@@ -579,6 +630,18 @@ extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
 extern "C" int titan_current_layer(titan_ctx *c) {
     if (!c) return TITAN_E_ARG;
     return c->started ? c->now2 : 0;
+}
+
+extern "C" int titan_tvd_cells(titan_ctx *c, int32_t *tvd) {
+    if (!c || !tvd) return TITAN_E_ARG;
+    for (int64_t d = 0; d < c->nf; d++) {
+        const int64_t r = c->ord.face_d2r[d];
+        for (int k = 0; k < 2; k++) {
+            const int32_t v = c->tvd_raw[2 * d + k];
+            tvd[2 * r + k] = v < 0 ? -1 : c->ord.cell_d2r[v];
+        }
+    }
+    return TITAN_OK;
 }
 
 extern "C" int titan_set_timing(titan_ctx *c, int enable) {

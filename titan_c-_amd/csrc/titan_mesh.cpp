@@ -316,11 +316,68 @@ bool build_tiles(int64_t nc, int64_t nf, int64_t nn, int tile_cells, const int32
             t.faces.push_back(tile_face_pack(ln, face_code[f]));
             t.faces.push_back((uint64_t)(uint32_t)a | ((uint64_t)(uint32_t)b << 32));
             t.face_ref.push_back(face_ref[f]);
+            t.face_dev.push_back(f);
         }
         for (int64_t c = c0; c < c1; c++)
             for (int j = 0; j < 6; j++) t.cell_lfaces[6 * c + j] = (uint16_t)flocal[cell_faces[6 * c + j]];
     }
     return true;
+}
+
+namespace {
+
+// Eigen's Vector3d::normalized(): divide by the norm unless it is zero; dot and squared norm summed
+// as (x0 y0 + x1 y1) + x2 y2
+void tvd_unit(double v[3]) {
+    const double n = std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    if (n > 0.0) { v[0] /= n; v[1] /= n; v[2] /= n; }
+}
+
+// Among cell C's other neighbours, the one whose centre, seen from the centre of face f, is most
+// opposite to vec (smallest cosine, the first one on ties); -1 if C has no other neighbour.
+int32_t tvd_pick(const int32_t *cell_faces, const int32_t *face_cells, const double *cc, const double *G,
+                 int64_t f, int32_t C, const double vec[3]) {
+    double best = 2.0;
+    int32_t pick = -1;
+    for (int k = 0; k < 6; k++) {
+        const int32_t gr = cell_faces[6 * (int64_t)C + k];
+        if (gr == f) continue;
+        const int32_t a = face_cells[2 * (int64_t)gr], b = face_cells[2 * (int64_t)gr + 1];
+        if (b < 0) continue;  // Get_Sosed across a boundary face: none
+        const int32_t cel = a == C ? b : a;
+        double v[3] = {cc[3 * (int64_t)cel] - G[0], cc[3 * (int64_t)cel + 1] - G[1], cc[3 * (int64_t)cel + 2] - G[2]};
+        tvd_unit(v);
+        const double cosine = (vec[0] * v[0] + vec[1] * v[1]) + vec[2] * v[2];
+        if (cosine < best) { best = cosine; pick = cel; }
+    }
+    return pick;
+}
+
+}  // namespace
+
+void init_tvd(int64_t nf, const int32_t *cell_faces, const int32_t *face_cells, const double *cc,
+              const double *fc, const double *fn, int32_t *tvd) {
+    for (int64_t f = 0; f < nf; f++) {
+        const double *n = fn + 3 * f, *G = fc + 3 * f;
+        const int32_t AA = tvd_pick(cell_faces, face_cells, cc, G, f, face_cells[2 * f], n);
+        tvd[2 * f] = AA;
+        tvd[2 * f + 1] = -1;
+        if (face_cells[2 * f + 1] < 0) continue;  // boundary face: one pick, no check
+        const double mn[3] = {-n[0], -n[1], -n[2]};
+        const int32_t BB = tvd_pick(cell_faces, face_cells, cc, G, f, face_cells[2 * f + 1], mn);
+        tvd[2 * f + 1] = BB;
+        // a second neighbour that does not lie behind its cell is dropped (Setka_physics.cpp:728-738)
+        if (AA >= 0) {
+            double k3[3] = {cc[3 * (int64_t)AA] - G[0], cc[3 * (int64_t)AA + 1] - G[1], cc[3 * (int64_t)AA + 2] - G[2]};
+            tvd_unit(k3);
+            if (-((n[0] * k3[0] + n[1] * k3[1]) + n[2] * k3[2]) < 0.1) tvd[2 * f] = -1;
+        }
+        if (BB >= 0) {
+            double k3[3] = {cc[3 * (int64_t)BB] - G[0], cc[3 * (int64_t)BB + 1] - G[1], cc[3 * (int64_t)BB + 2] - G[2]};
+            tvd_unit(k3);
+            if (((n[0] * k3[0] + n[1] * k3[1]) + n[2] * k3[2]) < 0.1) tvd[2 * f + 1] = -1;
+        }
+    }
 }
 
 }  // namespace titan

@@ -28,7 +28,7 @@ void build_order(const MeshView &m, DeviceOrder &o);
 // Host image of the tiles of the fused step kernel (see TileMesh in device_mesh.h).
 struct TileData {
     int32_t tile_cells = 0, max_nodes = 0, max_faces = 0;
-    std::vector<int32_t> node_off, nodes, own, face_off, face_ref;
+    std::vector<int32_t> node_off, nodes, own, face_off, face_ref, face_dev;  // face_dev: device face id
     std::vector<uint64_t> faces;  // 2 words per record
     std::vector<uint16_t> cell_lnodes, cell_lfaces;
 };
@@ -37,5 +37,12 @@ struct TileData {
 bool build_tiles(int64_t nc, int64_t nf, int64_t nn, int tile_cells, const int32_t *cell_nodes,
                  const int32_t *cell_faces, const int32_t *face_nodes, const int32_t *face_cells,
                  const uint32_t *face_code, const int32_t *face_bidx, const int32_t *face_ref, TileData &t);
+
+// Second neighbours of every face (Setka::Init_TVD, Setka_physics.cpp:648-744) from the geometry of
+// the initial nodes, all in device numbering: tvd[2f] = AA behind cells[0], tvd[2f+1] = BB behind
+// cells[1], -1 = none.  cell_faces lists each cell's faces in ascending reference face id, the
+// order of Cell::grans, so ties go to the same face as in the reference.
+void init_tvd(int64_t nf, const int32_t *cell_faces, const int32_t *face_cells, const double *cell_center,
+              const double *face_center, const double *face_normal, int32_t *tvd);
 
 }  // namespace titan
