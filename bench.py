@@ -1,7 +1,7 @@
 """Benchmark of the explicit ideal-MHD step on one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload shell4m_moving] [--no-cpu]
-                [--precision-steps 1000] [--train-states 2000000] [--exact-steps 3]
+                [--precision-steps 1000] [--train-states 2000000] [--exact-steps 3] [--variants ...]
 
 Prints ONE JSON line.  `value` is cell-updates/s (N_c x K / wall time of exactly K timed steps,
 inputs resident in HBM) of the plain FP64 step, the variant whose kernel the HBM roofline is
@@ -9,7 +9,8 @@ quoted on.  Beside it:
   variants    cell-updates/s of (i) plain FP64, FP32-mixed, (ii) FP64 + exact charge-exchange
               source (FP64 Maxwellian quadrature every step), (iii) FP32-mixed + surrogate source
               (MLP fused into the step kernel, exact fallback for out-of-box cells), and the
-              ratios (iii)/(ii) and (iii)/(i);
+              ratios (iii)/(ii) and (iii)/(i); and FP64 / FP32-mixed with second-order (TVD)
+              face states (Setka_gran_snos.cpp:186-736);
   roofline    the step kernel of (i): algorithmic bytes per launch / its mean duration, timed with
               HIP events around every launch on the context's stream;
   surrogate   training-set size, epochs, loss, and its source error against the exact kernel;
@@ -151,7 +152,9 @@ def traffic_from_profile(workload):
 
 def copy_calibration():
     """A plain streaming kernel of known size (y = -x: 256 MiB read + 256 MiB written), for the
-    counter runs that scale FETCH_SIZE / WRITE_SIZE."""
+    counter runs that scale FETCH_SIZE / WRITE_SIZE.  Runs in a process of its own: torch ships its
+    own HIP runtime, and a process holding both it and the one libtitan_gpu.so links is not one the
+    counter collection handles."""
     import torch
     x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
     y = torch.empty_like(x)
@@ -172,19 +175,21 @@ def main():
     ap.add_argument("--exact-steps", type=int, default=3)
     ap.add_argument("--train-states", type=int, default=2_000_000)
     ap.add_argument("--precision-steps", type=int, default=1000)
-    ap.add_argument("--variants", default="fp64,mixed,fp64_exact,mixed_surrogate")
-    ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--variants", default="fp64,mixed,fp64_exact,mixed_surrogate,fp64_tvd,mixed_tvd")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the streaming copy the HBM counter runs are scaled by, then exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; runs on %d GPUs are not part of it (use --gpus 1)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration()
+        return
     T = importlib.import_module("titan_c-_amd")
     C = importlib.import_module("titan_c-_amd.configs")
     sur = importlib.import_module("titan_c-_amd.surrogate")
     T.lib()
-    if a.copy_calibration:
-        copy_calibration()
     variants = a.variants.split(",")
     t0 = time.perf_counter()
     mesh, prim, params = C.build(a.workload)
@@ -228,11 +233,18 @@ def main():
         sur_info["fallback_cells_last_step"] = int(res["mixed_surrogate"]["stats"]["n_fallback"])
         log("mixed + surrogate: %.3f ms/step" % (res["mixed_surrogate"]["seconds"] * 1e3 / a.steps))
 
+    for name, prec in (("fp64_tvd", T.PREC_FP64), ("mixed_tvd", T.PREC_MIXED)):
+        if name in variants:  # second-order (TVD) face states, first-order otherwise identical
+            res[name] = run_variant(T, mesh, prim, dict(params, precision=prec, tvd=1), a.steps, a.warmup)
+            log("%s: %.3f ms/step, step kernel %.3f ms, %d second-order faces" % (
+                name, res[name]["seconds"] * 1e3 / a.steps, res[name]["step_kernel_ms"],
+                res[name]["tiles"]["tvd_faces"]))
+
     def cups(name):
         r = res.get(name)
         return None if r is None else nc * r["steps"] / r["seconds"]
 
-    vals = {k: cups(k) for k in ("fp64", "mixed", "fp64_exact", "mixed_surrogate")}
+    vals = {k: cups(k) for k in ("fp64", "mixed", "fp64_exact", "mixed_surrogate", "fp64_tvd", "mixed_tvd")}
     ratios = {}
     if vals["mixed_surrogate"] and vals["fp64_exact"]:
         ratios["surrogate_vs_exact_source"] = vals["mixed_surrogate"] / vals["fp64_exact"]
@@ -240,6 +252,8 @@ def main():
         ratios["surrogate_vs_plain_fp64"] = vals["mixed_surrogate"] / vals["fp64"]
     if vals["mixed"] and vals["fp64"]:
         ratios["mixed_vs_plain_fp64"] = vals["mixed"] / vals["fp64"]
+    if vals["fp64_tvd"] and vals["fp64"]:
+        ratios["tvd_vs_first_order_fp64"] = vals["fp64_tvd"] / vals["fp64"]
 
     roof = None
     head = "fp64" if "fp64" in res else None
@@ -247,7 +261,7 @@ def main():
         bytes_step = algorithmic_bytes(nc, nf, nn, moving)
         k_ms = res[head]["step_kernel_ms"]
         ach = bytes_step / (k_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": "k_step_tile<double,false> (one launch = one step)",
+        roof = {"bound": "hbm", "kernel": "k_step_tile<double,false,false> (one launch = one step)",
                 "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profile(a.workload), "traffic_unit": "bytes/launch",
                 "algorithmic_bytes": bytes_step, "kernel_ms": round(k_ms, 4)}

@@ -45,9 +45,11 @@
  *                       is reconstructed exactly (plain case and the one-sided lines at Zone_2/3
  *                       and Zone_3/4 transitions), an extremum keeps the cell value, the
  *                       positivity fallback, and the wind again through the all-Zone_1 case.
- *                       The Zone_1/Zone_2 transition cases (Setka_gran_snos.cpp:401-660) are
- *                       restated but their parity is UNPINNED: no invariant reproduces them
- *                       exactly and no reference output exists for them.
+ *                       Zone_1/Zone_2 transitions (Setka_gran_snos.cpp:401-660): along radial
+ *                       lines of a zone-banded shell the wind's V and r^(2 gamma) p come through
+ *                       exactly where the case interpolates them, and zig-zag densities against
+ *                       closed-form interpolation / extrapolation / extremum values fix which
+ *                       line each side uses and that density is interpolated unscaled.
  *   or_init_tvd         straight-line neighbours on a regular cube; the side neighbour (cosine
  *                       -1/sqrt 5) next to a boundary.
  *   or_minmod, or_linear  known answers, including the 1e-8 sign threshold.

@@ -153,3 +153,26 @@ def test_heliopause_faces_stay_first_order(T, C, O):
     hpf = flags == 4
     assert np.abs(f0[hpf] - f1[hpf]).max() > 0.0
     np.testing.assert_array_equal(f0[~hpf], f1[~hpf])
+
+
+def test_go_tvd_shell_zone_bands(T, C, O):
+    """Radial bands Zone_1 | Zone_2 | Zone_3 | Zone_4: every zone pattern of the reconstruction table
+    (the three Zone_1/Zone_2 transitions with spherical variables, the six plain transitions) occurs
+    on the faces; 20 steps against the oracle at 1e-12 (Zone_1 trig)."""
+    mesh, prim, params = small_shell(C)
+    r_in = np.linalg.norm(mesh.node_xyz, axis=1)[mesh.cell_nodes].min(axis=1)
+    layer = np.searchsorted(np.unique(np.round(r_in, 9)), np.round(r_in, 9))
+    mesh.cell_zone = (1 + np.minimum(layer // 3, 3)).astype(np.int8)
+    om = O.OracleMesh(mesh, tvd=True)
+    tvd = om.arrs["face_tvd"]
+    ok = (mesh.face_cells[:, 1] >= 0) & (tvd[:, 0] >= 0) & (tvd[:, 1] >= 0)
+    z = mesh.cell_zone
+    pats = {(int(z[tvd[f, 0]]), int(z[mesh.face_cells[f, 0]]), int(z[mesh.face_cells[f, 1]]), int(z[tvd[f, 1]]))
+            for f in np.nonzero(ok)[0]}
+    for p in ((1, 1, 1, 2), (1, 1, 2, 2), (1, 2, 2, 2), (2, 2, 2, 3), (2, 2, 3, 3), (2, 3, 3, 3),
+              (3, 3, 3, 4), (3, 3, 4, 4), (3, 4, 4, 4), (1, 1, 1, 1)):
+        assert p in pats, p
+    got, _, _ = _go(T, mesh, prim, params, 20)
+    rc, ref, _, _, _ = O.go(om, params, mesh.node_xyz, prim, 20, nthreads=8)
+    assert rc == 0
+    assert var_err(got, ref) < 1e-12

@@ -231,3 +231,103 @@ def test_heliopause_flag_keeps_first_order(C, O):
     mesh.face_flags = None
     rc, fl0, _, _ = O.face_flux(O.OracleMesh(mesh, tvd=True), dict(params, tvd=0), prim, g)
     np.testing.assert_array_equal(fl4, fl0)
+
+
+def _banded_shell(C, O, n_edge=4, n_layers=12):
+    """Shell whose radial layers are Zone_1 | Zone_2 | Zone_3 | Zone_4 in bands of three layers, so
+    that a radial line AA - A -|- B - BB crosses every zone transition of the reconstruction."""
+    mesh, _, params = C.shell(n_edge=n_edge, n_layers=n_layers, r_zone1_au=1e9)
+    r_in = np.linalg.norm(mesh.node_xyz, axis=1)[mesh.cell_nodes].min(axis=1)  # inner sphere of the cell
+    layer = np.searchsorted(np.unique(np.round(r_in, 9)), np.round(r_in, 9))
+    mesh.cell_zone = (1 + np.minimum(layer // 3, 3)).astype(np.int8)
+    om = O.OracleMesh(mesh, tvd=True)
+    g = O.measure(om, mesh.node_xyz)
+    return mesh, params, om, g
+
+
+def _radial_faces(mesh, om, g):
+    """Interior faces whose second neighbours lie on the face's radial line, with their zones."""
+    tvd = om.arrs["face_tvd"]
+    cc, fc = g["cell_center"], g["face_center"]
+    out = []
+    for f in range(mesh.n_faces):
+        A, B = mesh.face_cells[f]
+        if B < 0 or tvd[f, 0] < 0 or tvd[f, 1] < 0:
+            continue
+        cells = (tvd[f, 0], A, B, tvd[f, 1])
+        u = fc[f] / np.linalg.norm(fc[f])
+        pts = np.array([cc[k] for k in cells])
+        if np.abs(np.cross(pts, u)).max() > 1e-9 * np.linalg.norm(pts, axis=1).max():
+            continue
+        out.append((f, cells, tuple(int(mesh.cell_zone[k]) for k in cells)))
+    return out
+
+
+TRANSITIONS = [(1, 1, 1, 2), (1, 1, 2, 2), (1, 2, 2, 2)]
+
+
+def test_zone1_zone2_transitions_carry_the_wind(C, O):
+    """Along a radial line a symmetric wind has constant V (radial 5 + azimuthal 0.4) in both
+    spherical and Cartesian components, and r^(2 gamma) p is constant: wherever a transition case
+    interpolates those (Setka_gran_snos.cpp:401-660) the face gets the wind's value exactly."""
+    mesh, params, om, g = _banded_shell(C, O)
+    from test_oracle_face_states import _wind
+    gamma = params["gamma"]
+    prim = _wind(g["cell_center"], gamma, v_tan=0.4, b_tan=0.0, axis=2)
+    seen = set()
+    for f, cells, z in _radial_faces(mesh, om, g):
+        if z not in TRANSITIONS:
+            continue
+        # keep away from the polar axis, where the azimuthal direction is undefined
+        if np.linalg.norm(g["face_center"][f][:2]) < 1e-3 * np.linalg.norm(g["face_center"][f]):
+            continue
+        rc, qL, qR = O.face_states(om, dict(params, tvd=1), prim, g, f)
+        exp = _wind(g["face_center"][f], gamma, v_tan=0.4, b_tan=0.0, axis=2)
+        tol = 1e-12 * np.abs(prim).max(axis=0)
+        # V: spherical lines on the Zone_1 side, Cartesian lines on the plain side -- both exact
+        assert np.all(np.abs(qL[1:4] - exp[1:4]) <= tol[1:4]), (z, qL, exp)
+        assert np.all(np.abs(qR[1:4] - exp[1:4]) <= tol[1:4]), (z, qR, exp)
+        # p: scaled by r^(2 gamma) on the spherical sides (left of 1112 and 1122, right of 1112)
+        if z != (1, 2, 2, 2):
+            assert abs(qL[4] - exp[4]) <= tol[4], (z, qL[4], exp[4])
+        if z == (1, 1, 1, 2):
+            assert abs(qR[4] - exp[4]) <= tol[4], (z, qR[4], exp[4])
+        seen.add(z)
+    assert seen == set(TRANSITIONS), seen
+
+
+def test_zone1_zone2_transitions_line_choice(C, O):
+    """Zig-zag densities 1, 3, 2, 4 on AA, A, B, BB (an extremum in A and in B) tell the lines
+    apart: a minmod-limited line through three cells returns the middle cell's value, the line
+    through two cells interpolates or extrapolates linearly in the distance.  Density is
+    interpolated unscaled in all three transition cases."""
+    mesh, params, om, g = _banded_shell(C, O)
+    from test_oracle_face_states import _wind
+    base = _wind(g["cell_center"], params["gamma"], v_tan=0.0, b_tan=0.0, axis=2)
+    cc, fc = g["cell_center"], g["face_center"]
+    rho = (1.0, 3.0, 2.0, 4.0)
+    seen = set()
+    for f, cells, z in _radial_faces(mesh, om, g):
+        if z not in TRANSITIONS:
+            continue
+        prim = base.copy()
+        for k, v in zip(cells, rho):
+            prim[k, 0] = v
+        AA, A, B, BB = cells
+        d1 = np.linalg.norm(cc[A] - fc[f])
+        d2 = np.linalg.norm(cc[B] - fc[f])
+        dd1 = np.linalg.norm(cc[A] - cc[AA])
+        dd2 = np.linalg.norm(cc[B] - cc[BB])
+        limited_left = rho[1]                                          # extremum at A
+        limited_right = rho[2]                                         # extremum at B
+        interp = rho[1] + (rho[2] - rho[1]) * d1 / (d1 + d2)           # A-B line at the face
+        extrap_left = rho[1] + (rho[1] - rho[0]) * d1 / dd1            # AA-A line
+        extrap_right = rho[2] + (rho[2] - rho[3]) * d2 / dd2           # B-BB line
+        expect = {(1, 1, 1, 2): (limited_left, interp),
+                  (1, 1, 2, 2): (extrap_left, extrap_right),
+                  (1, 2, 2, 2): (interp, limited_right)}[z]
+        rc, qL, qR = O.face_states(om, dict(params, tvd=1), prim, g, f)
+        assert qL[0] == pytest.approx(expect[0], rel=1e-13), (z, qL[0], expect)
+        assert qR[0] == pytest.approx(expect[1], rel=1e-13), (z, qR[0], expect)
+        seen.add(z)
+    assert seen == set(TRANSITIONS), seen

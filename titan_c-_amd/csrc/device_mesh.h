@@ -76,6 +76,7 @@ namespace titan {
 // A face record is 16 bytes: x = 4 local node ids (12 bits each) | face code << 48; y = cells[0]
 // (device id, low 32 bits) | cells[1] << 32 (device id, or -(boundary-state index + 2)).
 constexpr int kTileThreads = 256;  // workgroup size of the fused step kernel
+constexpr int kXcds = 8;           // accelerator dies of an MI355X (workgroups dealt round robin)
 
 struct TileMesh {
     int64_t n_tiles;

@@ -413,7 +413,12 @@ __global__ void __launch_bounds__(kTileThreads, 2)
                 SurArgs sur, StepScalars *S, long long step) {
     extern __shared__ __align__(16) double smem[];
     const TileLds L = tile_lds(smem, TM, P.move_mesh != 0);
-    const int t = blockIdx.x;
+    // Workgroups go to the 8 XCDs round robin (block b on XCD b % 8), each XCD with its own L2.
+    // Tile t(b) = (b % 8) * per + b / 8 gives every XCD a contiguous run of tiles, so neighbouring
+    // tiles -- which share nodes and border-cell states -- are read through the same L2.
+    const int per = (int)((TM.n_tiles + kXcds - 1) / kXcds);
+    const int t = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+    if (t >= TM.n_tiles) return;  // padding blocks of the last run (whole workgroup, before any barrier)
     const int T = TM.tile_cells;
     const int64_t c0 = (int64_t)t * T;
     const int ncl = (int)min((int64_t)T, M.n_cells - c0);
@@ -605,7 +610,8 @@ inline void launch_tile_variant(hipStream_t s, const TileMesh &TM, const DevMesh
         else
             (void)hipGetLastError();  // the launch below then reports the failure
     }
-    hipLaunchKernelGGL((k_step_tile<RT, SUR, TVD>), dim3((unsigned)TM.n_tiles), dim3(kTileThreads), lds, s, TM, M, P,
+    const unsigned grid = (unsigned)(kXcds * ((TM.n_tiles + kXcds - 1) / kXcds));
+    hipLaunchKernelGGL((k_step_tile<RT, SUR, TVD>), dim3(grid), dim3(kTileThreads), lds, s, TM, M, P,
                        prim1, prim2, X1, X2, disp, cc1, cc2, src, sur, S, step);
 }
 
