@@ -49,6 +49,8 @@ extern "C" {
 #define TITAN_SRC_NONE 0
 #define TITAN_SRC_EXACT 1           /* FP64 Maxwellian quadrature (Lindsay-Stebbings sigma) */
 #define TITAN_SRC_MLP 2             /* BF16-MFMA surrogate, exact fallback out of range     */
+#define TITAN_SRC_BERA 3            /* the reference's closed-form source (Calc_sourse_MF_Bera) with
+                                       the hydrogen fluids of titan_set_h_fluids                */
 
 typedef struct titan_ctx titan_ctx;
 
@@ -88,6 +90,9 @@ typedef struct {
                                     (Setka::Init_TVD, Setka_physics.cpp:648-744)                */
     int32_t snos_on_hp;          /* [Snos_on_HP]: TVD also on faces flagged heliopause (face_flags
                                     bit2; Setka_gran_snos.cpp:188-189)                          */
+    double par_a_2;              /* [par_a_2] sigma = (1 - a_2 ln U)^2 of TITAN_SRC_BERA (0.102046) */
+    double par_n_H_LISM;         /* [par_n_H_LISM] (3.0); TITAN_SRC_BERA scales by n_H_LISM/Kn   */
+    double par_Kn;               /* [par_Kn] (40.0906)                                          */
 } titan_params;
 
 typedef struct {
@@ -118,6 +123,10 @@ int titan_set_nodes(titan_ctx *ctx, const double *xyz, int layer);
 int titan_get_nodes(titan_ctx *ctx, double *xyz, int layer);
 /* Neutral-H moments per cell for the charge-exchange source: [n_cells*5] = n_H, c_H^2, V_H. */
 int titan_set_neutrals(titan_ctx *ctx, const double *nh5);
+/* Hydrogen fluids per cell for TITAN_SRC_BERA (the keys rho_Hk, p_Hk, Vx_Hk, Vy_Hk, Vz_Hk of
+ * Cell::parameters, k = 1..n_h, read by Setka_physics.cpp:820-826): h [n_cells][n_h][5] =
+ * (rho_Hk, p_Hk, Vx_Hk, Vy_Hk, Vz_Hk), 1 <= n_h <= 9. */
+int titan_set_h_fluids(titan_ctx *ctx, int n_h, const double *h);
 
 /* ---- geometry ------------------------------------------------------------------------------- */
 /* Setka::Calculating_measure(layer) (Setka.cpp:2465-2502) on the device, in the context's
@@ -161,9 +170,11 @@ int titan_chlld_batch(int64_t n, const int8_t *n_state, const int8_t *n_disc, co
                       uint8_t *branch, int precision);
 
 /* ---- charge-exchange source ---------------------------------------------------------------- */
-/* Stands in for Setka::Calc_sourse_MF_Bera (Setka_physics.cpp:766-1258) as the plasma source:
- * out [n_cells*5] = exchange rate, momentum (3), energy (lab frame), per unit volume and time,
- * for the states of `layer` and the neutrals of titan_set_neutrals.  mode = TITAN_SRC_*. */
+/* The plasma source of Setka::Calc_sourse_MF_Bera (Setka_physics.cpp:766-1258) for the states of
+ * `layer`: out [n_cells*5] = mass, momentum (3), energy (lab frame) per unit volume and time.
+ * mode TITAN_SRC_BERA: the reference's closed form with the fluids of titan_set_h_fluids (mass
+ * source 0; states without PUI, Q, rho_He); TITAN_SRC_EXACT / TITAN_SRC_MLP: the Maxwellian
+ * charge-exchange integral with the neutrals of titan_set_neutrals (out[0] = exchange rate). */
 int titan_sources(titan_ctx *ctx, int layer, int mode, double *out);
 /* Standalone exact source for n states: in [n*10] = n_p, c_p^2, V_p(3), n_H, c_H^2, V_H(3). */
 int titan_cx_exact_batch(int64_t n, const double *in10, int nodes, double *out5);

@@ -26,7 +26,11 @@ class OrMesh(ctypes.Structure):
 
 class OrParams(ctypes.Structure):
     _fields_ = [("gamma", _D), ("kfl", _D), ("v_inf", _D), ("move_mesh", ctypes.c_int32), ("n_disc", ctypes.c_int32),
-                ("motion_omega", _D), ("motion_phase", _D), ("tvd", ctypes.c_int32), ("snos_on_hp", ctypes.c_int32)]
+                ("motion_omega", _D), ("motion_phase", _D), ("tvd", ctypes.c_int32), ("snos_on_hp", ctypes.c_int32),
+                ("par_a_2", _D), ("par_n_H_LISM", _D), ("par_Kn", _D)]
+
+# parameters an older params dict may lack (Phys_param.h:99-101 defaults for the last three)
+PARAM_DEFAULTS = {"tvd": 0, "snos_on_hp": 0, "par_a_2": 0.102046, "par_n_H_LISM": 3.0, "par_Kn": 40.0906}
 
 
 class OrGeom(ctypes.Structure):
@@ -63,7 +67,8 @@ def lib():
             "or_minmod": (_D, [_D, _D]),
             "or_linear": (_D, [_D] * 7),
             "or_update": (_I, [_P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
-            "or_go": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _P, _P, _P]),
+            "or_go": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _P, _P, _P, _I, _P]),
+            "or_bera_source": (None, [_P, _I, _P, _D, _D, _D, _P]),
             "or_new_connect": (_I64, [_I64, _P, _I64, _P, _P, _P]),
             "or_cx_rate_ref": (_D, [_D, _D, _D, _I, _I]),
             "or_sigma_ls": (_D, [_D]),
@@ -114,7 +119,7 @@ class OracleMesh:
 def params_c(params):
     p = OrParams()
     for k, _ in OrParams._fields_:
-        setattr(p, k, params.get(k, 0) if k in ("tvd", "snos_on_hp") else params[k])
+        setattr(p, k, params.get(k, PARAM_DEFAULTS[k]) if k in PARAM_DEFAULTS else params[k])
     return p
 
 
@@ -142,8 +147,9 @@ def face_flux(om, params, prim, geom, x1=None, x2=None, dt=1e-5):
     return rc, flux, dtf, br
 
 
-def go(om, params, xyz, prim, steps, gcl=False, nthreads=1, disp=None, t_sim=0.0):
-    """Runs or_go; returns (rc, prim_final, xyz_final, dt_hist, t_sim)."""
+def go(om, params, xyz, prim, steps, gcl=False, nthreads=1, disp=None, t_sim=0.0, h_fluids=None):
+    """Runs or_go; returns (rc, prim_final, xyz_final, dt_hist, t_sim).  h_fluids [n_cells, n_h, 5]
+    (rho_H, p_H, V_H per hydrogen fluid) adds the Calc_sourse_MF_Bera sources to every step."""
     m = om.m
     coords = np.zeros((2, m.n_nodes, 3))
     coords[0] = xyz
@@ -153,8 +159,10 @@ def go(om, params, xyz, prim, steps, gcl=False, nthreads=1, disp=None, t_sim=0.0
     layer = ctypes.c_int(0)
     ts = ctypes.c_double(t_sim)
     d = None if disp is None else np.ascontiguousarray(disp, np.float64)
+    hf = None if h_fluids is None else np.ascontiguousarray(h_fluids, np.float64).reshape(m.n_cells, -1, 5)
     rc = lib().or_go(ctypes.byref(om.c), ctypes.byref(params_c(params)), _p(coords), _p(state), _p(d), int(steps),
-                     1 if gcl else 0, int(nthreads), ctypes.byref(layer), _p(dth), ctypes.byref(ts))
+                     1 if gcl else 0, int(nthreads), ctypes.byref(layer), _p(dth), ctypes.byref(ts),
+                     0 if hf is None else hf.shape[1], _p(hf))
     return rc, state[layer.value].copy(), coords[layer.value].copy(), dth[:steps], ts.value
 
 
@@ -287,4 +295,17 @@ def init_tvd(om, geom0):
     g = OrGeom(*[_p(geom0[k]) for k, _ in OrGeom._fields_])
     out = np.empty((om.m.n_faces, 2), np.int32)
     lib().or_init_tvd(ctypes.byref(om.c), ctypes.byref(g), _p(out))
+    return out
+
+
+def bera_source(prim, h_fluids, a2=0.102046, n_h_lism=3.0, kn=40.0906):
+    """Setka::Calc_sourse_MF_Bera plasma sources [n, 5] = (rho, m_x, m_y, m_z, E) of states prim [n, 8]
+    with hydrogen fluids h_fluids [n, n_h, 5]."""
+    prim = np.ascontiguousarray(prim, np.float64).reshape(-1, 8)
+    n = prim.shape[0]
+    hf = np.ascontiguousarray(h_fluids, np.float64).reshape(n, -1, 5)
+    out = np.empty((n, 5))
+    L = lib()
+    for c in range(n):
+        L.or_bera_source(_p(prim[c]), hf.shape[1], _p(hf[c]), a2, n_h_lism, kn, _p(out[c]))
     return out

@@ -1242,10 +1242,97 @@ static void geom_free(or_geom *g)
 
 /* Setka::Go, Setka_physics.cpp:1484-2278: lagged dt (first step 1e-5), layer swap, optional
  * motion and re-measure of the new layer, face loop, cell loop. */
+/* ------------------------------------------------------------------------------------------ */
+/* Setka::Calc_sourse_MF_Bera (Setka_physics.cpp:766-1258), plasma part: momentum and energy
+ * sources of charge exchange between the protons and n_h hydrogen fluids, for a state without
+ * pickup ions, Q or rho_He -- the keys Calc_sourse_MF_Bera then reads as 0, so the proton split of
+ * Plasma_components_1 (Phys_param.cpp:628-665) has rho_He = 0 and al = 2 (zone <= 2). */
+static const double OR_ME_MP = 1.0 / 1836.15; /* MF_meDmp, Header.h:99 */
+
+/* |V1 - V2| as Eigen's norm(): square root of (d0^2 + d1^2) + d2^2 */
+static double or_dnorm(const double *V1, const double *V2)
+{
+    double d0 = V1[0] - V2[0], d1 = V1[1] - V2[1], d2 = V1[2] - V2[2];
+    return sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+}
+
+/* U_bera, Um_bera, UE_bera (Setka_physics.cpp:746-762) */
+static double or_u_bera(double T1, double T2, const double *V1, const double *V2)
+{
+    double uu = or_dnorm(V1, V2);
+    return sqrt((4.0 / OR_PI) * (T1 + T2) + KV(uu));
+}
+static double or_um_bera(double T1, double T2, const double *V1, const double *V2)
+{
+    double uu = or_dnorm(V1, V2);
+    return sqrt((16.0 / OR_PI) * T1 + (9.0 * OR_PI / 4.0) * T2 + 4.0 * KV(uu));
+}
+static double or_ue_bera(double T1, double T2, const double *V1, const double *V2)
+{
+    double uu = or_dnorm(V1, V2);
+    return sqrt((4.0 / OR_PI) * T1 + (64.0 / 9.0 / OR_PI) * T2 + KV(uu));
+}
+
+static double or_dot3(const double *a, const double *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+void or_bera_source(const double *s, int n_h, const double *h, double a2, double n_h_lism, double kn,
+                    double *out)
+{
+    /* proton density and temperature (Plasma_components_1, Phys_param.cpp:641-660) */
+    const double rho = s[0], pr = s[4], rho_He = 0.0;
+    const int al = 2;
+    double rho_p = -(OR_ME_MP * al * rho_He + 4.0 * (-rho + rho_He)) / (4.0 * (1.0 + OR_ME_MP));
+    double T_p = 8.0 * (1.0 + OR_ME_MP) * pr / (8.0 * rho - (7.0 - al + (-1.0 + al) * OR_ME_MP) * rho_He);
+    const double *Vp = s + 1;
+    /* Setka_physics.cpp:802-808 */
+    if (rho_p <= 1e-8) rho_p = 1e-8;
+    if (T_p <= 1e-8) T_p = 1e-8;
+    if (T_p > 1e5) T_p = 1e5;
+    double mx = 0.0, my = 0.0, mz = 0.0, E = 0.0;
+    for (int k = 0; k < n_h; k++) {
+        /* Setka_physics.cpp:820-826 */
+        const double rho_H = h[5 * k];
+        const double T_H = 2.0 * h[5 * k + 1] / ((rho_H < 1e-9) ? 1e-9 : rho_H);  /* std::max */
+        const double *VH = h + 5 * k + 2;
+        /* :842-859: both orders of the pair (p, H) */
+        const double U_pH = or_u_bera(T_p, T_H, Vp, VH), U_Hp = or_u_bera(T_H, T_p, VH, Vp);
+        const double Um_pH = or_um_bera(T_p, T_H, Vp, VH), Um_Hp = or_um_bera(T_H, T_p, VH, Vp);
+        const double UE_pH = or_ue_bera(T_p, T_H, Vp, VH), UE_Hp = or_ue_bera(T_H, T_p, VH, Vp);
+        const double sig_pH = KV(1.0 - a2 * log(U_pH)), sig_Hp = KV(1.0 - a2 * log(U_Hp));
+        /* :913-927 */
+        const double cHp = sig_Hp * rho_p * rho_H, cpH = sig_pH * rho_p * rho_H;
+        const double dV[3] = {Vp[0] - VH[0], Vp[1] - VH[1], Vp[2] - VH[2]};
+        const double tHp = T_p / Um_Hp, tpH = T_H / Um_pH;
+        double Hm_Hp[3], Hm_pH[3];
+        for (int i = 0; i < 3; i++) {
+            Hm_Hp[i] = cHp * (U_Hp * Vp[i] + tHp * dV[i]);
+            Hm_pH[i] = cpH * (U_pH * VH[i] - tpH * dV[i]);
+        }
+        const double HE_Hp = cHp * ((0.5 * U_Hp * or_dot3(Vp, Vp) + tHp * or_dot3(Vp, dV)) + 3.0 / 4.0 * T_p * UE_Hp);
+        const double HE_pH = cpH * ((0.5 * U_pH * or_dot3(VH, VH) - tpH * or_dot3(VH, dV)) + 3.0 / 4.0 * T_H * UE_pH);
+        /* :975-983 */
+        mx += (-Hm_Hp[0] + Hm_pH[0]);
+        my += (-Hm_Hp[1] + Hm_pH[1]);
+        mz += (-Hm_Hp[2] + Hm_pH[2]);
+        E += (-HE_Hp + HE_pH);
+    }
+    /* :1236-1240 */
+    const double ddp = n_h_lism / kn;
+    out[0] = 0.0 * ddp;
+    out[1] = mx * ddp;
+    out[2] = my * ddp;
+    out[3] = mz * ddp;
+    out[4] = E * ddp;
+}
+
 int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, const double *disp,
-          int64_t steps, int gcl, int nthreads, int *final_layer, double *dt_hist, double *t_sim)
+          int64_t steps, int gcl, int nthreads, int *final_layer, double *dt_hist, double *t_sim,
+          int n_h, const double *h_fluids)
 {
     int64_t nn3 = 3 * m->n_nodes, nc8 = 8 * m->n_cells;
+    /* charge-exchange sources of the step's old layer (Setka_physics.cpp:1741), when hydrogen
+     * fluids are given */
+    double *src = h_fluids ? (double *)malloc(sizeof(double) * 5 * m->n_cells) : NULL;
     double *X[2] = {coords, coords + nn3};
     double *S[2] = {prim, prim + nc8};
     or_geom G[2];
@@ -1276,7 +1363,13 @@ int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, co
         if (rc && !status) status = rc;
         for (int64_t f = 0; f < m->n_faces; f++)
             if (dtf[f] < loc_time) loc_time = SMIN(loc_time, dtf[f]);
-        rc = update_impl(m, p, S[now1], G[now1].cell_volume, G[now2].cell_volume, flux, NULL, time, S[now2],
+        if (src) {
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+            for (int64_t c = 0; c < m->n_cells; c++)
+                or_bera_source(S[now1] + 8 * c, n_h, h_fluids + (int64_t)5 * n_h * c, p->par_a_2, p->par_n_H_LISM,
+                               p->par_Kn, src + 5 * c);
+        }
+        rc = update_impl(m, p, S[now1], G[now1].cell_volume, G[now2].cell_volume, flux, src, time, S[now2],
                          NULL, NULL, nthreads);
         if (rc && !status) status = rc;
         tsim += time;
@@ -1288,6 +1381,7 @@ int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, co
     geom_free(&G[1]);
     free(flux);
     free(dtf);
+    free(src);
     return status;
 }
 

@@ -23,6 +23,8 @@
  *   or_go               Setka::Go                    TITAN/source/Setka_physics.cpp:1484-2278
  *   or_new_connect      Setka::New_connect           TITAN/source/Setka.cpp:2066-2249
  *   or_cx_rate_ref      Culc_nu_maxwel               MK_source/MK_source/MK_source/MK_source.cpp:38-103
+ *   or_bera_source      Setka::Calc_sourse_MF_Bera   TITAN/source/Setka_physics.cpp:746-1258
+ *                       (plasma part; Plasma_components_1, Phys_param.cpp:628-665)
  *   or_cx_source, or_sigma_ls, or_mlp_*  the learned-source path (no reference counterpart:
  *                       Maxwellian CX moments with the Lindsay-Stebbings cross-section; MLP)
  *
@@ -58,6 +60,12 @@
  *   or_new_connect      hand-worked two-cell mesh and face counts.
  *   or_cx_rate_ref      the reference's own MK_source program (compiled from its source into
  *                       oracle/_ref) at sampled table points: tests/golden/mk_source_int1.json.
+ *   or_bera_source      exact zero without atoms and at thermal + mechanical equilibrium (which
+ *                       also fixes the proton temperature of Plasma_components_1), exact
+ *                       additivity over fluids, the cold-beam transfer rates, rotation and
+ *                       Galilean covariance, and -- with sigma = 1 -- 6-D Monte Carlo Maxwellian
+ *                       averages of the momentum and energy exchange within 1.5-3 % (a sign flip
+ *                       of the thermal drag or swapped temperatures in U_m / U_E leave that band).
  *   or_cx_source        sigma = const closed form, Cartesian quadrature, 6-D Monte Carlo,
  *                       bilinearity and Galilean invariance; or_sigma_ls: published values.
  *   or_mlp_*            closed forms and finite differences.
@@ -103,6 +111,9 @@ typedef struct {
     double motion_omega, motion_phase;
     int32_t tvd;         /* TVD: second-order limited face states (Setka_gran_snos.cpp:186-736) */
     int32_t snos_on_hp;  /* Snos_on_HP: TVD also on heliopause faces (Phys_param.h:130)        */
+    double par_a_2;      /* charge-exchange cross-section slope a_2 (Phys_param.h:99)           */
+    double par_n_H_LISM; /* n_H of the LISM (Phys_param.h:100)                                  */
+    double par_Kn;       /* Knudsen number (Phys_param.h:101)                                   */
 } or_params;
 
 typedef struct {
@@ -164,7 +175,11 @@ int or_update(const or_mesh *m, const or_params *p, const double *prim_old, cons
  * layer holds the newest state; dt_hist (optional, [steps]) receives the dt used at each step. */
 int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, const double *disp,
           int64_t steps, int gcl, int nthreads, int *final_layer, double *dt_hist,
-          double *t_sim);
+          double *t_sim, int n_h, const double *h_fluids);
+/* ---- Setka::Calc_sourse_MF_Bera (Setka_physics.cpp:766-1258), plasma part: state s[8], n_h
+ * hydrogen fluids h[5k..5k+4] = (rho_Hk, p_Hk, V_Hk); out[5] = (rho, m_x, m_y, m_z, E) sources. */
+void or_bera_source(const double *s, int n_h, const double *h, double a2, double n_h_lism, double kn,
+                    double *out);
 
 /* ---- mesh indexing (New_connect) ---- */
 /* Builds faces of n_cells hexahedra; arrays sized 6*n_cells by the caller; returns n_faces. */

@@ -1,0 +1,63 @@
+"""The reference's closed-form charge-exchange source (Setka::Calc_sourse_MF_Bera,
+Setka_physics.cpp:746-1258, plasma part) on the device against the oracle: per cell through
+titan_sources, and inside whole steps of Setka::Go.  The device's log differs from glibc's in the
+last ulp, so the sources and the states are compared at 1e-12 relative to each variable's scale."""
+import numpy as np
+import pytest
+
+from helpers import var_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _shell(C):
+    mesh, prim, params = C.shell(n_edge=6, n_layers=10, r_zone1_au=80.0, noise=0.01)
+    return mesh, prim, params
+
+
+def test_bera_sources_vs_oracle(T, C, O):
+    mesh, prim, params = _shell(C)
+    h = C.h_fluids(mesh.n_cells)
+    with T.Setka(mesh, dict(params, source_mode=T.SRC_BERA)) as s:
+        s.set_state(prim, 0)
+        s.set_h_fluids(h)
+        got = s.sources(0, T.SRC_BERA)
+    ref = O.bera_source(prim, h)
+    assert np.all(got[:, 0] == 0.0)
+    scale = np.abs(ref).max(axis=0)
+    assert scale[1:].min() > 0.0
+    err = (np.abs(got - ref).max(axis=0) / np.maximum(scale, 1e-300))
+    assert err.max() < 1e-12, err
+
+
+def test_go_bera_vs_oracle(T, C, O):
+    mesh, prim, params = _shell(C)
+    h = C.h_fluids(mesh.n_cells)
+    p = dict(params, source_mode=T.SRC_BERA)
+    with T.Setka(mesh, p) as s:
+        s.set_state(prim, 0)
+        s.set_h_fluids(h)
+        s.Go(30)
+        got = s.get_state()
+    rc, ref, _, _, _ = O.go(O.OracleMesh(mesh), p, mesh.node_xyz, prim, 30, nthreads=8, h_fluids=h)
+    assert rc == 0
+    assert var_err(got, ref) < 1e-12
+    rc, ref0, _, _, _ = O.go(O.OracleMesh(mesh), params, mesh.node_xyz, prim, 30, nthreads=8)
+    assert var_err(ref0, ref) > 1e-9   # the source is visible after 30 steps
+
+
+def test_go_bera_without_atoms_is_the_plain_step(T, C):
+    mesh, prim, params = C.cube_brio_wu(n=10)
+    h = np.zeros((mesh.n_cells, 4, 5))
+    h[:, :, 1] = 0.3          # pressure but no density: every exchange term carries rho_H = 0
+    h[:, :, 2] = -1.0
+    with T.Setka(mesh, dict(params, source_mode=T.SRC_BERA)) as s:
+        s.set_state(prim, 0)
+        s.set_h_fluids(h)
+        s.Go(20)
+        a = s.get_state()
+    with T.Setka(mesh, params) as s:
+        s.set_state(prim, 0)
+        s.Go(20)
+        b = s.get_state()
+    np.testing.assert_array_equal(a, b)

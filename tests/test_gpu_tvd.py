@@ -26,8 +26,8 @@ def banded_cube(C, n=12, jitter=0.15):
     return mesh, prim, dict(params, tvd=1)
 
 
-def small_shell(C, moving=False):
-    mesh, prim, params = C.shell(n_edge=6, n_layers=10, moving=moving, r_zone1_au=80.0, noise=0.01)
+def small_shell(C, moving=False, n_layers=10):
+    mesh, prim, params = C.shell(n_edge=6, n_layers=n_layers, moving=moving, r_zone1_au=80.0, noise=0.01)
     return mesh, prim, dict(params, tvd=1)
 
 
@@ -159,7 +159,7 @@ def test_go_tvd_shell_zone_bands(T, C, O):
     """Radial bands Zone_1 | Zone_2 | Zone_3 | Zone_4: every zone pattern of the reconstruction table
     (the three Zone_1/Zone_2 transitions with spherical variables, the six plain transitions) occurs
     on the faces; 20 steps against the oracle at 1e-12 (Zone_1 trig)."""
-    mesh, prim, params = small_shell(C)
+    mesh, prim, params = small_shell(C, n_layers=12)
     r_in = np.linalg.norm(mesh.node_xyz, axis=1)[mesh.cell_nodes].min(axis=1)
     layer = np.searchsorted(np.unique(np.round(r_in, 9)), np.round(r_in, 9))
     mesh.cell_zone = (1 + np.minimum(layer // 3, 3)).astype(np.int8)

@@ -25,7 +25,7 @@ TITAN_E_NOT_TRAINED = -6
 BC_INTERIOR, BC_INNER_HARD, BC_OUTER_HARD, BC_OUTER_SOFT, BC_ZERO_GRADIENT = 0, 1, 2, 3, 4
 GEOM_REF, GEOM_GCL = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-SRC_NONE, SRC_EXACT, SRC_MLP = 0, 1, 2
+SRC_NONE, SRC_EXACT, SRC_MLP, SRC_BERA = 0, 1, 2, 3
 
 _ERRNAMES = {
     TITAN_E_ARG: "TITAN_E_ARG", TITAN_E_HIP: "TITAN_E_HIP", TITAN_E_RIEMANN: "TITAN_E_RIEMANN",
@@ -54,9 +54,11 @@ class ParamsC(ctypes.Structure):
                 ("move_mesh", ctypes.c_int32), ("n_disc", ctypes.c_int32), ("geom_mode", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("source_mode", ctypes.c_int32), ("cx_nodes", ctypes.c_int32),
                 ("motion_omega", ctypes.c_double), ("motion_phase", ctypes.c_double),
-                ("tvd", ctypes.c_int32), ("snos_on_hp", ctypes.c_int32)]
+                ("tvd", ctypes.c_int32), ("snos_on_hp", ctypes.c_int32),
+                ("par_a_2", ctypes.c_double), ("par_n_H_LISM", ctypes.c_double), ("par_Kn", ctypes.c_double)]
 
-_PARAM_DEFAULTS = {"tvd": 0, "snos_on_hp": 0}
+# parameters an older params dict may lack (Phys_param.h:99-101 defaults for the last three)
+_PARAM_DEFAULTS = {"tvd": 0, "snos_on_hp": 0, "par_a_2": 0.102046, "par_n_H_LISM": 3.0, "par_Kn": 40.0906}
 
 
 class StatsC(ctypes.Structure):
@@ -81,6 +83,7 @@ _SIGS = {
     "titan_set_nodes": (_I, [_P, _P, _I]),
     "titan_get_nodes": (_I, [_P, _P, _I]),
     "titan_set_neutrals": (_I, [_P, _P]),
+    "titan_set_h_fluids": (_I, [_P, _I, _P]),
     "titan_calculating_measure": (_I, [_P, _I]),
     "titan_get_geometry": (_I, [_P, _I, _P, _P, _P, _P, _P]),
     "titan_gran_potok": (_I, [_P, _I, _D, _P, _P, _P]),
@@ -281,6 +284,11 @@ class Setka:
     def set_neutrals(self, nh5):
         a = _c(nh5, np.float64).reshape(self.n_cells, 5)
         _check(lib().titan_set_neutrals(self.ctx, _ptr(a)), "titan_set_neutrals", self.ctx)
+
+    def set_h_fluids(self, h):
+        """Hydrogen fluids [n_cells, n_h, 5] = (rho_Hk, p_Hk, V_Hk) of the closed-form source."""
+        a = _c(h, np.float64).reshape(self.n_cells, -1, 5)
+        _check(lib().titan_set_h_fluids(self.ctx, a.shape[1], _ptr(a)), "titan_set_h_fluids", self.ctx)
 
     def Calculating_measure(self, layer):
         _check(lib().titan_calculating_measure(self.ctx, layer), "titan_calculating_measure", self.ctx)

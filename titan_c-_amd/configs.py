@@ -241,6 +241,29 @@ def neutrals(n_cells, seed=5):
     return out
 
 
+def h_fluids(n_cells, n_h=4, seed=9):
+    """Per-cell hydrogen fluids [n_cells, n_h, 5] = (rho_Hk, p_Hk, V_Hk) in code units, after the four
+    populations of TITAN's multi-fluid hydrogen (pristine LISM atoms, and atoms born in the outer
+    heliosheath, the inner heliosheath and the supersonic wind): density, temperature and speed
+    along -x per population, with +-30 % seeded per-cell perturbations.  p_Hk = rho_Hk T_Hk / 2,
+    the inverse of T_H = 2 p_H / rho_H (Setka_physics.cpp:822)."""
+    pops = [(0.10, 6.5e3, 26.0), (0.05, 1.6e4, 18.0), (0.01, 3.0e5, 100.0), (0.004, 1.0e5, 400.0)]
+    out = np.zeros((n_cells, n_h, 5))
+    idx = np.arange(n_cells, dtype=np.uint64) * np.uint64(5 * n_h)
+    for k in range(n_h):
+        n0, T0, v0 = pops[k % len(pops)]
+        u = [splitmix_uniform(seed, idx + np.uint64(5 * k + j)) for j in range(5)]
+        rho = n0 / CHAR_RHO * (0.7 + 0.6 * u[0])
+        T = 2.0 * K_B * (T0 * (0.7 + 0.6 * u[1])) / M_P / (CHAR_V * 1e5) ** 2
+        v = v0 / CHAR_V
+        out[:, k, 0] = rho
+        out[:, k, 1] = 0.5 * rho * T
+        out[:, k, 2] = -v * (0.7 + 0.6 * u[2])
+        out[:, k, 3] = 0.3 * v * (2.0 * u[3] - 1.0)
+        out[:, k, 4] = 0.3 * v * (2.0 * u[4] - 1.0)
+    return out
+
+
 WORKLOADS = {
     "cube32": dict(fn="cube", n=32),
     "shell1m": dict(fn="shell", n_edge=37, n_layers=120),

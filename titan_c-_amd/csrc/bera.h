@@ -1,0 +1,95 @@
+// bera.h -- plasma part of Setka::Calc_sourse_MF_Bera (Setka_physics.cpp:746-1258): the momentum and
+// energy the protons exchange with n_h hydrogen fluids by charge exchange, from the closed-form
+// Maxwellian-averaged relative speeds U, U_m, U_E and sigma = (1 - a_2 ln U)^2.
+//
+// The state carries no pickup ions, Q or rho_He, the keys the reference then reads as 0: the
+// proton split of Plasma_components_1 (Phys_param.cpp:628-665) has rho_He = 0, and the zone
+// determ_zone returns (Q/rho = 0 < 50: Zone 1 or 2) gives al = 2.  The plasma mass source is 0
+// (charge exchange swaps a proton for a proton); the hydrogen-fluid and pickup-ion sources of the
+// same routine are not formed.  Evaluated in the reference's operation order (FP64, no contraction
+// in the translation units that include it); only log and sqrt differ from glibc in the last ulp.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace titan {
+
+constexpr double kMeMp = 1.0 / 1836.15;  // MF_meDmp, Header.h:99
+constexpr double kBeraPi = 3.14159265358979323846;
+
+// |V1 - V2| as Eigen's norm(): sqrt((d0^2 + d1^2) + d2^2)
+__device__ __forceinline__ double bera_dnorm(const double *V1, const double *V2) {
+    const double d0 = V1[0] - V2[0], d1 = V1[1] - V2[1], d2 = V1[2] - V2[2];
+    return sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+}
+
+// U_bera, Um_bera, UE_bera (Setka_physics.cpp:746-762)
+__device__ __forceinline__ double bera_u(double T1, double T2, const double *V1, const double *V2) {
+    const double uu = bera_dnorm(V1, V2);
+    return sqrt((4.0 / kBeraPi) * (T1 + T2) + uu * uu);
+}
+__device__ __forceinline__ double bera_um(double T1, double T2, const double *V1, const double *V2) {
+    const double uu = bera_dnorm(V1, V2);
+    return sqrt((16.0 / kBeraPi) * T1 + (9.0 * kBeraPi / 4.0) * T2 + 4.0 * (uu * uu));
+}
+__device__ __forceinline__ double bera_ue(double T1, double T2, const double *V1, const double *V2) {
+    const double uu = bera_dnorm(V1, V2);
+    return sqrt((4.0 / kBeraPi) * T1 + (64.0 / 9.0 / kBeraPi) * T2 + uu * uu);
+}
+
+__device__ __forceinline__ double bera_dot(const double *a, const double *b) {
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+// s: plasma state (rho, V, p, B); h[5k..5k+4] = (rho_Hk, p_Hk, V_Hk); a2 = par_a_2;
+// ddp = par_n_H_LISM / par_Kn.  out = (rho, m_x, m_y, m_z, E) sources per unit volume and time.
+__device__ __forceinline__ void bera_source(const double s[8], int n_h, const double *__restrict__ h, double a2,
+                                            double ddp, double out[5]) {
+    // Plasma_components_1 (Phys_param.cpp:641-660) with rho_He = 0, al = 2
+    const double rho = s[0], pr = s[4], rho_He = 0.0;
+    const int al = 2;
+    double rho_p = -(kMeMp * al * rho_He + 4.0 * (-rho + rho_He)) / (4.0 * (1.0 + kMeMp));
+    double T_p = 8.0 * (1.0 + kMeMp) * pr / (8.0 * rho - (7.0 - al + (-1.0 + al) * kMeMp) * rho_He);
+    const double Vp[3] = {s[1], s[2], s[3]};
+    // Setka_physics.cpp:802-808
+    if (rho_p <= 1e-8) rho_p = 1e-8;
+    if (T_p <= 1e-8) T_p = 1e-8;
+    if (T_p > 1e5) T_p = 1e5;
+    double mx = 0.0, my = 0.0, mz = 0.0, E = 0.0;
+    for (int k = 0; k < n_h; k++) {
+        // :820-826
+        const double rho_H = h[5 * k];
+        const double T_H = 2.0 * h[5 * k + 1] / ((rho_H < 1e-9) ? 1e-9 : rho_H);
+        const double VH[3] = {h[5 * k + 2], h[5 * k + 3], h[5 * k + 4]};
+        // :842-859, both orders of the pair
+        const double U_pH = bera_u(T_p, T_H, Vp, VH), U_Hp = bera_u(T_H, T_p, VH, Vp);
+        const double Um_pH = bera_um(T_p, T_H, Vp, VH), Um_Hp = bera_um(T_H, T_p, VH, Vp);
+        const double UE_pH = bera_ue(T_p, T_H, Vp, VH), UE_Hp = bera_ue(T_H, T_p, VH, Vp);
+        const double l_pH = 1.0 - a2 * log(U_pH), l_Hp = 1.0 - a2 * log(U_Hp);
+        const double sig_pH = l_pH * l_pH, sig_Hp = l_Hp * l_Hp;
+        // :913-927
+        const double cHp = sig_Hp * rho_p * rho_H, cpH = sig_pH * rho_p * rho_H;
+        const double dV[3] = {Vp[0] - VH[0], Vp[1] - VH[1], Vp[2] - VH[2]};
+        const double tHp = T_p / Um_Hp, tpH = T_H / Um_pH;
+        double Hm_Hp[3], Hm_pH[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            Hm_Hp[i] = cHp * (U_Hp * Vp[i] + tHp * dV[i]);
+            Hm_pH[i] = cpH * (U_pH * VH[i] - tpH * dV[i]);
+        }
+        const double HE_Hp = cHp * ((0.5 * U_Hp * bera_dot(Vp, Vp) + tHp * bera_dot(Vp, dV)) + 3.0 / 4.0 * T_p * UE_Hp);
+        const double HE_pH = cpH * ((0.5 * U_pH * bera_dot(VH, VH) - tpH * bera_dot(VH, dV)) + 3.0 / 4.0 * T_H * UE_pH);
+        // :975-983
+        mx += (-Hm_Hp[0] + Hm_pH[0]);
+        my += (-Hm_Hp[1] + Hm_pH[1]);
+        mz += (-Hm_Hp[2] + Hm_pH[2]);
+        E += (-HE_Hp + HE_pH);
+    }
+    // :1236-1240
+    out[0] = 0.0 * ddp;
+    out[1] = mx * ddp;
+    out[2] = my * ddp;
+    out[3] = mz * ddp;
+    out[4] = E * ddp;
+}
+
+}  // namespace titan

@@ -1,5 +1,7 @@
-// k_cx.hip -- exact FP64 charge-exchange source kernels (FP64, no contraction).
+// k_cx.hip -- exact FP64 charge-exchange source kernels and the reference's closed-form
+// (Calc_sourse_MF_Bera) source (FP64, no contraction).
 #define TITAN_FLUX_ONLY 1
+#include "bera.h"
 #include "cx_device.h"
 #include "launch.h"
 #include "step_kernels.h"
@@ -108,6 +110,26 @@ void launch_cx_exact(hipStream_t s, int64_t n, const double *in10, int nodes, do
 void launch_cx_cells(hipStream_t s, int64_t nc, const double *prim, const double *neutrals, int nodes, double *src) {
     hipLaunchKernelGGL(k_cx_cells, dim3((unsigned)((nc + 255) / 256)), dim3(256), 16 * nodes, s, nc, prim, neutrals,
                        nodes, src);
+}
+
+// Calc_sourse_MF_Bera per cell: hfluid [Nc * n_h * 5] (rho_H, p_H, V_H per fluid)
+__global__ void __launch_bounds__(256) k_bera_cells(int64_t nc, const double *__restrict__ prim,
+                                                    const double *__restrict__ hfluid, int n_h, double a2,
+                                                    double ddp, double *__restrict__ src) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double s[8];
+    load_state(prim, c, s);
+    double o[5];
+    bera_source(s, n_h, hfluid + (int64_t)5 * n_h * c, a2, ddp, o);
+#pragma unroll
+    for (int k = 0; k < 5; k++) src[5 * c + k] = o[k];
+}
+
+void launch_bera_cells(hipStream_t s, int64_t nc, const double *prim, const double *hfluid, int n_h, double a2,
+                       double ddp, double *src) {
+    hipLaunchKernelGGL(k_bera_cells, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, nc, prim, hfluid, n_h, a2,
+                       ddp, src);
 }
 
 void launch_cx_list(hipStream_t s, const int32_t *list, const unsigned long long *count, int64_t cap,

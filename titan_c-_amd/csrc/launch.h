@@ -48,6 +48,9 @@ void launch_cx_exact(hipStream_t s, int64_t n, const double *in10, int nodes, do
 // per-cell exact source from the plasma state and the neutral moments [Nc*5]
 void launch_cx_cells(hipStream_t s, int64_t nc, const double *prim, const double *neutrals, int nodes,
                      double *src);
+// Calc_sourse_MF_Bera plasma source per cell from the state and hfluid [Nc * n_h * 5]
+void launch_bera_cells(hipStream_t s, int64_t nc, const double *prim, const double *hfluid, int n_h, double a2,
+                       double ddp, double *src);
 // exact source + update of the cells the surrogate step listed as out of its training box
 void launch_cx_fixup(hipStream_t s, const DevMesh &M, const double *prim1, const double *neutrals, int nodes,
                      const int32_t *list, const double *stash, int64_t cap, StepScalars *S, double *prim2);

@@ -59,6 +59,8 @@ struct titan_ctx {
     double *d_geom[2] = {nullptr, nullptr};  // one allocation per layer, carved into DevGeom
     DevGeom geom[2];
     double *d_flux = nullptr, *d_src = nullptr, *d_neutrals = nullptr;
+    double *d_hfluid = nullptr;  // hydrogen fluids of the closed-form source [Nc][n_h][5]
+    int n_hfluid = 0;
     StepScalars *d_S = nullptr, *d_S2 = nullptr;
     // tiles of the fused step kernel
     TileMesh tiles{};
@@ -168,6 +170,8 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
         return TITAN_E_ARG;
     if (m->n_cells > INT32_MAX || m->n_faces > INT32_MAX || m->n_nodes > INT32_MAX) return TITAN_E_ARG;
     if (p->move_mesh && !m->node_disp) return TITAN_E_ARG;
+    if (p->source_mode < TITAN_SRC_NONE || p->source_mode > TITAN_SRC_BERA) return TITAN_E_ARG;
+    if (p->source_mode == TITAN_SRC_BERA && !(p->par_Kn > 0.0)) return TITAN_E_ARG;
     const int64_t nn = m->n_nodes, nc = m->n_cells, nf = m->n_faces;
     // ---- topology checks (Setka::Test_geometr, Setka.cpp:475-617) ----
     for (int64_t f = 0; f < nf; f++) {
@@ -398,7 +402,7 @@ extern "C" int titan_destroy(titan_ctx *c) {
     void *ptrs[] = {c->d_cell_nodes, c->d_cell_faces, c->d_face_nodes, c->d_face_cells, c->d_face_bidx,
                     c->d_face_ref, c->d_cell_ref, c->d_cell_code, c->d_face_code, c->d_bstate, c->d_disp,
                     c->d_X[0], c->d_X[1], c->d_prim[0], c->d_prim[1], c->d_geom[0], c->d_geom[1], c->d_flux,
-                    c->d_src, c->d_neutrals, c->d_S, c->d_S2, c->d_t_node_off, c->d_t_nodes, c->d_t_own,
+                    c->d_src, c->d_neutrals, c->d_hfluid, c->d_S, c->d_S2, c->d_t_node_off, c->d_t_nodes, c->d_t_own,
                     c->d_t_face_off, c->d_t_face_ref, c->d_t_faces, c->d_t_cell_lnodes, c->d_t_cell_lfaces,
                     c->d_vol, c->d_face_tvd, c->d_t_face_tvd};
     for (void *q : ptrs)
@@ -462,6 +466,24 @@ extern "C" int titan_set_neutrals(titan_ctx *c, const double *nh5) {
     for (int64_t d = 0; d < c->nc; d++)
         for (int k = 0; k < 5; k++) h[5 * d + k] = nh5[5 * (int64_t)c->ord.cell_d2r[d] + k];
     HIPCHK(hipMemcpyAsync(c->d_neutrals, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TITAN_OK;
+}
+
+extern "C" int titan_set_h_fluids(titan_ctx *c, int n_h, const double *h) {
+    if (!c || !h || n_h < 1 || n_h > 9) return TITAN_E_ARG;
+    if (c->d_hfluid && n_h != c->n_hfluid) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(c->d_hfluid));
+        c->d_hfluid = nullptr;
+    }
+    const int64_t w = 5 * (int64_t)n_h;
+    if (!c->d_hfluid) HIPCHK(hipMalloc((void **)&c->d_hfluid, sizeof(double) * w * c->nc));
+    c->n_hfluid = n_h;
+    std::vector<double> v(w * c->nc);
+    for (int64_t d = 0; d < c->nc; d++)
+        std::memcpy(&v[w * d], h + w * (int64_t)c->ord.cell_d2r[d], sizeof(double) * w);
+    HIPCHK(hipMemcpyAsync(c->d_hfluid, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return TITAN_OK;
 }
@@ -531,8 +553,15 @@ extern "C" int titan_gran_potok(titan_ctx *c, int now, double dt, double *face_f
 
 static int compute_sources(titan_ctx *c, int layer, int mode) {
     if (mode == TITAN_SRC_NONE) return TITAN_OK;
-    if (!c->d_neutrals) return TITAN_E_ARG;
+    if (mode == TITAN_SRC_BERA ? !c->d_hfluid : !c->d_neutrals) return TITAN_E_ARG;
     if (!c->d_src) HIPCHK(hipMalloc((void **)&c->d_src, sizeof(double) * 5 * c->nc));
+    if (mode == TITAN_SRC_BERA) {
+        const titan_params &p = c->params;
+        launch_bera_cells(c->stream, c->nc, c->d_prim[layer], c->d_hfluid, c->n_hfluid, p.par_a_2,
+                          p.par_n_H_LISM / p.par_Kn, c->d_src);
+        HIPCHK(hipGetLastError());
+        return TITAN_OK;
+    }
     const int nodes = c->params.cx_nodes > 0 ? c->params.cx_nodes : 100;
     if (mode == TITAN_SRC_EXACT) {
         launch_cx_cells(c->stream, c->nc, c->d_prim[layer], c->d_neutrals, nodes, c->d_src);
