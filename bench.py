@@ -9,8 +9,10 @@ quoted on.  Beside it:
   variants    cell-updates/s of (i) plain FP64, FP32-mixed, (ii) FP64 + exact charge-exchange
               source (FP64 Maxwellian quadrature every step), (iii) FP32-mixed + surrogate source
               (MLP fused into the step kernel, exact fallback for out-of-box cells), and the
-              ratios (iii)/(ii) and (iii)/(i); and FP64 / FP32-mixed with second-order (TVD)
-              face states (Setka_gran_snos.cpp:186-736);
+              ratios (iii)/(ii) and (iii)/(i); FP64 / FP32-mixed with second-order (TVD)
+              face states (Setka_gran_snos.cpp:186-736); FP64 with the reference's closed-form
+              source (Calc_sourse_MF_Bera) and four hydrogen fluids;
+  checkpoint  save and load of the state and nodes in the reference's cell-parameter file;
   roofline    the step kernel of (i): algorithmic bytes per launch / its mean duration, timed with
               HIP events around every launch on the context's stream;
   surrogate   training-set size, epochs, loss, and its source error against the exact kernel;
@@ -54,12 +56,14 @@ def algorithmic_bytes(n_cells, n_faces, n_nodes, moving, with_neutrals=False):
     return float(b)
 
 
-def run_variant(T, mesh, prim, params, steps, warmup, neutrals=None, mlp=None, keep_state=False):
+def run_variant(T, mesh, prim, params, steps, warmup, neutrals=None, mlp=None, keep_state=False, h_fluids=None):
     s = T.Setka(mesh, params)
     try:
         s.set_state(prim, 0)
         if neutrals is not None:
             s.set_neutrals(neutrals)
+        if h_fluids is not None:
+            s.set_h_fluids(h_fluids)
         if mlp is not None:
             s.mlp_set(*mlp)
         if warmup:
@@ -140,6 +144,26 @@ def train_surrogate(T, C, sur, prim, nh, n_states, seed=11):
     return (P, in_norm, scale, box), info
 
 
+def checkpoint_timing(T, mesh, prim, params):
+    """Save_cell_parameters / Download_cell_parameters of the whole state and node layer through a
+    file in a temporary directory (device <-> host copies included)."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d, T.Setka(mesh, params) as s:
+        path = os.path.join(d, "parameters.bin")
+        s.set_state(prim, 0)
+        t0 = time.perf_counter()
+        s.Save_cell_parameters(path)
+        t1 = time.perf_counter()
+        nbytes = os.path.getsize(path)
+        s.Download_cell_parameters(path)
+        t2 = time.perf_counter()
+    out = {"bytes": nbytes, "save_s": round(t1 - t0, 3), "load_s": round(t2 - t1, 3),
+           "save_GBps": round(nbytes / (t1 - t0) / 1e9, 3), "load_GBps": round(nbytes / (t2 - t1) / 1e9, 3),
+           "keys": 8, "nodes": True}
+    log("checkpoint: %s" % out)
+    return out
+
+
 def traffic_from_profile(workload):
     """HBM bytes per launch of the FP64 step kernel measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
     counter runs (calibrated on a copy kernel), as recorded in the profile summary, or None."""
@@ -175,7 +199,7 @@ def main():
     ap.add_argument("--exact-steps", type=int, default=3)
     ap.add_argument("--train-states", type=int, default=2_000_000)
     ap.add_argument("--precision-steps", type=int, default=1000)
-    ap.add_argument("--variants", default="fp64,mixed,fp64_exact,mixed_surrogate,fp64_tvd,mixed_tvd")
+    ap.add_argument("--variants", default="fp64,mixed,fp64_exact,mixed_surrogate,fp64_tvd,mixed_tvd,fp64_bera,checkpoint")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the streaming copy the HBM counter runs are scaled by, then exit")
     a = ap.parse_args()
@@ -240,11 +264,21 @@ def main():
                 name, res[name]["seconds"] * 1e3 / a.steps, res[name]["step_kernel_ms"],
                 res[name]["tiles"]["tvd_faces"]))
 
+    if "fp64_bera" in variants:  # the reference's closed-form charge-exchange source, four H fluids
+        hf = C.h_fluids(nc)
+        p = dict(params, precision=T.PREC_FP64, source_mode=T.SRC_BERA)
+        res["fp64_bera"] = run_variant(T, mesh, prim, p, a.steps, a.warmup, h_fluids=hf)
+        log("fp64 + closed-form source: %.3f ms/step (source kernel %.3f ms)" % (
+            res["fp64_bera"]["seconds"] * 1e3 / a.steps, res["fp64_bera"]["source_kernel_ms"]))
+        del hf
+    ckpt = checkpoint_timing(T, mesh, prim, params) if "checkpoint" in variants else None
+
     def cups(name):
         r = res.get(name)
         return None if r is None else nc * r["steps"] / r["seconds"]
 
-    vals = {k: cups(k) for k in ("fp64", "mixed", "fp64_exact", "mixed_surrogate", "fp64_tvd", "mixed_tvd")}
+    vals = {k: cups(k) for k in ("fp64", "mixed", "fp64_exact", "mixed_surrogate", "fp64_tvd", "mixed_tvd",
+                                 "fp64_bera")}
     ratios = {}
     if vals["mixed_surrogate"] and vals["fp64_exact"]:
         ratios["surrogate_vs_exact_source"] = vals["mixed_surrogate"] / vals["fp64_exact"]
@@ -304,6 +338,7 @@ def main():
         "roofline": roof,
         "surrogate": sur_info,
         "precision": prec,
+        "checkpoint": ckpt,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -29,6 +29,8 @@ extern "C" {
 #define TITAN_E_NONFINITE_STATE (-4)/* NaN density (Setka_physics.cpp:1962-1978)               */
 #define TITAN_E_TOPOLOGY (-5)       /* face/cell topology violates Setka::Test_geometr          */
 #define TITAN_E_NOT_TRAINED (-6)    /* surrogate requested before parameters were set          */
+#define TITAN_E_IO (-7)             /* file cannot be opened, written, or is truncated / damaged
+                                       (Setka_physics.cpp:4110-4113,4213-4216 print and exit) */
 
 /* face boundary codes (Type_Gran, Setka_physics.cpp:212-236; ZERO_GRADIENT is an addition) */
 #define TITAN_BC_INTERIOR 0
@@ -197,6 +199,34 @@ int titan_mlp_loss_grad(titan_ctx *ctx, int64_t n, const float *x, const float *
  * mini-batches of `batch`, learning rate lr; loss_hist [epochs] (nullable). */
 int titan_mlp_train(titan_ctx *ctx, int64_t n, const float *x, const float *y, int epochs, int batch,
                     float lr, uint64_t seed, float *loss_hist);
+
+/* ---- checkpoint files (host) ------------------------------------------------------------------ */
+/* The file of Setka::Save_cell_parameters / Setka::Download_cell_parameters
+ * (Setka_physics.cpp:4107-4165, 4210-4285), little-endian: per cell u64 count and count x
+ * (u64 key length, key bytes, f64 value); u8 1, u64 n_nodes, n_nodes x 3 f64 (coord[0]); 1000 zero
+ * bytes.  titan_checkpoint_write writes n_keys keys for every cell, values [n_cells][n_keys], and
+ * the node block when n_nodes >= 0 (flag 0 otherwise).  titan_checkpoint_read fills values
+ * [n_cells][n_keys] with the named keys of each cell (NaN where a cell lacks a key; key order in
+ * the file is free, other keys are skipped) and, if the file has a node block, xyz [n_nodes*3]
+ * (n_nodes < 0: only report the count; xyz must then be NULL); *nodes_read = node count, 0 when
+ * the file has no node block.  A node count different from n_nodes >= 0 is TITAN_E_ARG. */
+int titan_checkpoint_write(const char *path, int64_t n_cells, int n_keys, const char *const *keys,
+                           const double *values, int64_t n_nodes, const double *xyz);
+int titan_checkpoint_read(const char *path, int64_t n_cells, int n_keys, const char *const *keys,
+                          double *values, int64_t n_nodes, double *xyz, int64_t *nodes_read);
+/* Setka::Save_cell_parameters on a context: the 8 plasma keys (rho, Vx, Vy, Vz, p, Bx, By, Bz) of
+ * the newest layer, then n_extra caller keys with extra_values [n_cells][n_extra] (e.g. the
+ * hydrogen fluids, Q, rho_He kept on the host), and the newest node layer. */
+int titan_save_cell_parameters(titan_ctx *ctx, const char *path, int n_extra, const char *const *extra_keys,
+                               const double *extra_values);
+/* Setka::Download_cell_parameters on a context: each of the 8 plasma keys a cell has in the file
+ * replaces that value in both state layers (keys a cell lacks keep their values); a node block
+ * replaces both node layers, and the next titan_go re-measures the geometry, as Setka::Go does
+ * on entry (Setka_physics.cpp:1495-1510).  The TVD neighbours stay as chosen; titan_init_tvd
+ * chooses them again from the new nodes. */
+int titan_download_cell_parameters(titan_ctx *ctx, const char *path);
+/* Setka::Init_TVD (Setka_physics.cpp:648-744) again, from the newest node layer. */
+int titan_init_tvd(titan_ctx *ctx);
 
 /* ---- mesh construction helpers (host) -------------------------------------------------------- */
 /* Setka::New_connect (Setka.cpp:2066-2249): faces of n_cells hexahedra; face_nodes [6*n_cells*4],

@@ -1,0 +1,103 @@
+"""The checkpoint file of Setka::Save_cell_parameters / Setka::Download_cell_parameters
+(Setka_physics.cpp:4107-4285) through the library's host entry points, against the oracle's
+record-by-record restatement (oracle/checkpoint.py) and a byte string assembled by hand from the
+reference's write calls.  No GPU: the raw file functions are host code."""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import checkpoint as CK
+
+KEYS = ["rho", "Vx", "Vy", "Vz", "p", "Bx", "By", "Bz"]
+
+
+def test_known_bytes(T, tmp_path):
+    """One cell {rho: 1.5, p: 0.25} and one node (1, 2, 3): the bytes the reference's out.write
+    calls produce, in this key order."""
+    path = str(tmp_path / "one.bin")
+    T.checkpoint_write(path, ["rho", "p"], [[1.5, 0.25]], xyz=[[1.0, 2.0, 3.0]])
+    want = (struct.pack("<Q", 2) + struct.pack("<Q", 3) + b"rho" + struct.pack("<d", 1.5)
+            + struct.pack("<Q", 1) + b"p" + struct.pack("<d", 0.25)
+            + b"\x01" + struct.pack("<Q", 1) + struct.pack("<ddd", 1.0, 2.0, 3.0) + b"\x00" * 1000)
+    assert open(path, "rb").read() == want
+    vals, xyz = T.checkpoint_read(path, 1, ["p", "rho", "Q"])
+    assert vals[0, 0] == 0.25 and vals[0, 1] == 1.5 and np.isnan(vals[0, 2])
+    np.testing.assert_array_equal(xyz, [[1.0, 2.0, 3.0]])
+
+
+def test_write_matches_oracle_reader(T, tmp_path):
+    rng = np.random.default_rng(2)
+    nc, nn = 257, 300
+    vals = rng.normal(size=(nc, 10))
+    xyz = rng.normal(size=(nn, 3))
+    keys = KEYS + ["Q", "rho_H1"]
+    path = str(tmp_path / "w.bin")
+    T.checkpoint_write(path, keys, vals, xyz=xyz)
+    cells, x = CK.read(path, nc)
+    assert len(cells) == nc
+    for c in range(nc):
+        assert list(cells[c]) == keys
+        np.testing.assert_array_equal([cells[c][k] for k in keys], vals[c])
+    np.testing.assert_array_equal(np.array(x), xyz)
+
+
+def test_read_any_key_order_and_extra_keys(T, tmp_path):
+    """The reference writes an unordered_map: key order differs between cells, cells may hold keys
+    the reader does not want, and a cell may lack a key."""
+    rng = np.random.default_rng(4)
+    nc = 120
+    cells = []
+    truth = np.full((nc, 8), np.nan)
+    extra = ["Q", "rho_He", "MK_n_H", "MK_n_H1", "rho_H1", "p_H1", "Vx_H1"]
+    for c in range(nc):
+        names = KEYS + extra
+        if c % 17 == 3:
+            names = [k for k in names if k != "Bz"]
+        order = list(rng.permutation(len(names))) if c % 5 else list(range(len(names)))
+        kv = {}
+        for i in order:
+            kv[names[i]] = float(rng.normal())
+        cells.append(kv)
+        for j, k in enumerate(KEYS):
+            if k in kv:
+                truth[c, j] = kv[k]
+    path = str(tmp_path / "r.bin")
+    CK.write(path, cells, xyz=None)
+    vals, xyz = T.checkpoint_read(path, nc, KEYS)
+    assert xyz is None
+    np.testing.assert_array_equal(np.isnan(vals), np.isnan(truth))
+    np.testing.assert_array_equal(vals[~np.isnan(truth)], truth[~np.isnan(truth)])
+
+
+def test_node_count_and_damage(T, tmp_path):
+    path = str(tmp_path / "n.bin")
+    T.checkpoint_write(path, ["rho"], np.ones((4, 1)), xyz=np.zeros((5, 3)))
+    with pytest.raises(T.TitanError) as e:           # another mesh
+        T.checkpoint_read(path, 4, ["rho"], n_nodes=6)
+    assert e.value.code == T.TITAN_E_ARG
+    data = open(path, "rb").read()
+    for cut in (3, 20, len(data) - 1000 - 10, len(data) - 1000 - 24 * 5):
+        bad = str(tmp_path / ("cut%d.bin" % cut))
+        open(bad, "wb").write(data[:cut])
+        with pytest.raises(T.TitanError) as e:
+            T.checkpoint_read(bad, 4, ["rho"], n_nodes=5)
+        assert e.value.code == T.TITAN_E_IO
+    # a key length far beyond any the reference writes
+    bad = str(tmp_path / "key.bin")
+    open(bad, "wb").write(struct.pack("<QQ", 1, 1 << 40) + b"\x00" * 64)
+    with pytest.raises(T.TitanError) as e:
+        T.checkpoint_read(bad, 1, ["rho"])
+    assert e.value.code == T.TITAN_E_IO
+    with pytest.raises(T.TitanError) as e:
+        T.checkpoint_read(str(tmp_path / "missing.bin"), 1, ["rho"])
+    assert e.value.code == T.TITAN_E_IO
+
+
+def test_oracle_reader_resets_mk_keys(tmp_path):
+    path = str(tmp_path / "mk.bin")
+    CK.write(path, [{"rho": 2.0, "MK_n_H1": 5.0}], xyz=[(0.0, 0.0, 1.0)])
+    cells, xyz = CK.read(path, 1, mk_keys=("MK_n_H1",))
+    assert cells[0] == {"rho": 2.0, "MK_n_H1": 0.0}
+    assert xyz == [(0.0, 0.0, 1.0)]

@@ -22,6 +22,7 @@ TITAN_E_RIEMANN = -3
 TITAN_E_NONFINITE_STATE = -4
 TITAN_E_TOPOLOGY = -5
 TITAN_E_NOT_TRAINED = -6
+TITAN_E_IO = -7
 BC_INTERIOR, BC_INNER_HARD, BC_OUTER_HARD, BC_OUTER_SOFT, BC_ZERO_GRADIENT = 0, 1, 2, 3, 4
 GEOM_REF, GEOM_GCL = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
@@ -30,7 +31,7 @@ SRC_NONE, SRC_EXACT, SRC_MLP, SRC_BERA = 0, 1, 2, 3
 _ERRNAMES = {
     TITAN_E_ARG: "TITAN_E_ARG", TITAN_E_HIP: "TITAN_E_HIP", TITAN_E_RIEMANN: "TITAN_E_RIEMANN",
     TITAN_E_NONFINITE_STATE: "TITAN_E_NONFINITE_STATE", TITAN_E_TOPOLOGY: "TITAN_E_TOPOLOGY",
-    TITAN_E_NOT_TRAINED: "TITAN_E_NOT_TRAINED",
+    TITAN_E_NOT_TRAINED: "TITAN_E_NOT_TRAINED", TITAN_E_IO: "TITAN_E_IO",
 }
 
 
@@ -99,6 +100,11 @@ _SIGS = {
     "titan_mlp_forward": (_I, [_P, _I64, _P, _P]),
     "titan_mlp_loss_grad": (_I, [_P, _I64, _P, _P, _P, _P]),
     "titan_mlp_train": (_I, [_P, _I64, _P, _P, _I, _I, ctypes.c_float, ctypes.c_uint64, _P]),
+    "titan_checkpoint_write": (_I, [ctypes.c_char_p, _I64, _I, _P, _P, _I64, _P]),
+    "titan_checkpoint_read": (_I, [ctypes.c_char_p, _I64, _I, _P, _P, _I64, _P, ctypes.POINTER(_I64)]),
+    "titan_save_cell_parameters": (_I, [_P, ctypes.c_char_p, _I, _P, _P]),
+    "titan_download_cell_parameters": (_I, [_P, ctypes.c_char_p]),
+    "titan_init_tvd": (_I, [_P]),
     "titan_mesh_connect": (_I64, [_I64, _P, _I64, _P, _P, _P]),
     "titan_mesh_cube": (_I, [ctypes.c_int32, _D, ctypes.c_uint64, _P, _P]),
     "titan_mesh_shell": (_I, [ctypes.c_int32, ctypes.c_int32, _P, _P, _P]),
@@ -140,6 +146,39 @@ def _check(rc, what, ctx=None):
             lib().titan_last_error(ctx, ctypes.byref(i))
             idx = i.value
         raise TitanError(rc, what, idx)
+
+
+# ---- checkpoint files ---------------------------------------------------------------------------
+def _keys(keys):
+    arr = (ctypes.c_char_p * len(keys))(*[k.encode() for k in keys])
+    return arr, ctypes.cast(arr, ctypes.c_void_p)
+
+
+def checkpoint_write(path, keys, values, xyz=None):
+    """Setka::Save_cell_parameters file: values [n_cells, len(keys)]; xyz [n_nodes, 3] or None."""
+    vals = _c(values, np.float64).reshape(-1, len(keys))
+    arr, kp = _keys(keys)
+    x = None if xyz is None else _c(xyz, np.float64).reshape(-1, 3)
+    _check(lib().titan_checkpoint_write(os.fsencode(path), vals.shape[0], len(keys), kp, _ptr(vals),
+                                        -1 if x is None else x.shape[0], _ptr(x)), "titan_checkpoint_write")
+
+
+def checkpoint_read(path, n_cells, keys, n_nodes=-1):
+    """Setka::Download_cell_parameters file: returns (values [n_cells, len(keys)] with NaN where a
+    cell lacks a key, xyz [n_nodes, 3] or None if the file has no node block).  n_nodes < 0 reads
+    the node count first."""
+    arr, kp = _keys(keys)
+    vals = np.empty((n_cells, len(keys)))
+    got = _I64(-1)
+    if n_nodes < 0:
+        _check(lib().titan_checkpoint_read(os.fsencode(path), n_cells, len(keys), kp, _ptr(vals), -1, None,
+                                           ctypes.byref(got)), "titan_checkpoint_read")
+        n_nodes = got.value
+    xyz = np.empty((max(n_nodes, 0), 3))
+    _check(lib().titan_checkpoint_read(os.fsencode(path), n_cells, len(keys), kp, _ptr(vals), n_nodes,
+                                       _ptr(xyz) if n_nodes > 0 else None, ctypes.byref(got)),
+           "titan_checkpoint_read")
+    return vals, (xyz if got.value > 0 else None)
 
 
 # ---- mesh helpers ---------------------------------------------------------------------------
@@ -317,6 +356,25 @@ class Setka:
 
     def current_layer(self):
         return lib().titan_current_layer(self.ctx)
+
+    def Save_cell_parameters(self, path, extra=None):
+        """Setka::Save_cell_parameters: the 8 plasma keys of the newest layer, extra keys
+        {name: [n_cells]} from the host, and the newest nodes."""
+        extra = extra or {}
+        names = list(extra)
+        vals = np.ascontiguousarray(np.stack([np.asarray(extra[k], np.float64) for k in names], axis=1)
+                                    if names else np.zeros((self.n_cells, 0)))
+        arr, kp = _keys(names) if names else (None, None)
+        _check(lib().titan_save_cell_parameters(self.ctx, os.fsencode(path), len(names), kp,
+                                                _ptr(vals) if names else None),
+               "titan_save_cell_parameters", self.ctx)
+
+    def Download_cell_parameters(self, path):
+        _check(lib().titan_download_cell_parameters(self.ctx, os.fsencode(path)), "titan_download_cell_parameters",
+               self.ctx)
+
+    def Init_TVD(self):
+        _check(lib().titan_init_tvd(self.ctx), "titan_init_tvd", self.ctx)
 
     def tvd_cells(self):
         """Setka::Init_TVD neighbours [n_faces, 2] (AA behind cells[0], BB behind cells[1], -1 = none)."""

@@ -36,7 +36,7 @@ UNITS = {
     "k_step32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "k_mlp.hip": ["-ffp-contract=fast"],
 }
-HOST_UNITS = ["titan_host.cpp", "titan_mesh.cpp"]
+HOST_UNITS = ["titan_host.cpp", "titan_mesh.cpp", "titan_io.cpp"]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
 
 

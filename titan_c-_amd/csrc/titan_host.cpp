@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -75,6 +76,7 @@ struct titan_ctx {
     std::vector<int32_t> tvd_raw;
     int32_t *d_face_tvd = nullptr, *d_t_face_tvd = nullptr;
     bool started = false;
+    bool geom_dirty = false;  // nodes replaced since the geometry was last measured
     long long step = 0;
     int now1 = 1, now2 = 0;
     int last_err = 0;
@@ -160,6 +162,53 @@ int init_scalars(StepScalars *d) {
 }
 
 }  // namespace
+
+// Setka::Init_TVD from the geometry of node layer `layer`, then the pairs of the faces that reconstruct to second
+// order (Setka_gran_snos.cpp:188-194): TVD on, both cells is_TVD, both neighbours present, not a
+// heliopause face unless Snos_on_HP.  Arrays in device numbering; rec_face = device face of each
+// tile face record.  Replaces the context's pairs (per face and per record).
+static int setup_tvd(titan_ctx *c, const std::vector<int32_t> &cell_faces, const std::vector<int32_t> &face_cells,
+                     const std::vector<uint32_t> &cell_code, const std::vector<uint32_t> &face_code,
+                     const std::vector<int32_t> &rec_face, int layer) {
+    const int64_t nc = c->nc, nf = c->nf;
+    const titan_params *p = &c->params;
+    std::vector<double> cc(3 * nc), fc(3 * nf), fn(3 * nf);
+    HIPCHK(hipMemcpy(cc.data(), c->geom[layer].cell_center, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fc.data(), c->geom[layer].face_center, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fn.data(), c->geom[layer].face_normal, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost));
+    c->tvd_raw.assign(2 * nf, -1);
+    init_tvd(nf, cell_faces.data(), face_cells.data(), cc.data(), fc.data(), fn.data(), c->tvd_raw.data());
+    std::vector<int32_t> pair(2 * nf, -1);
+    int64_t n_active = 0;
+    if (p->tvd) {
+        for (int64_t d = 0; d < nf; d++) {
+            const int32_t a = face_cells[2 * d], b = face_cells[2 * d + 1];
+            const int32_t AA = c->tvd_raw[2 * d], BB = c->tvd_raw[2 * d + 1];
+            if (b < 0 || AA < 0 || BB < 0) continue;
+            if (!((cell_code[a] >> 6) & 1u) || !((cell_code[b] >> 6) & 1u)) continue;
+            const uint32_t fflags = (face_code[d] >> 8) & 255u;
+            if ((fflags & 4u) && !p->snos_on_hp) continue;
+            pair[2 * d] = AA;
+            pair[2 * d + 1] = BB;
+            n_active++;
+        }
+    }
+    std::vector<int32_t> rec_pair(2 * rec_face.size());
+    for (size_t r = 0; r < rec_face.size(); r++) {
+        rec_pair[2 * r] = pair[2 * (int64_t)rec_face[r]];
+        rec_pair[2 * r + 1] = pair[2 * (int64_t)rec_face[r] + 1];
+    }
+    if (!c->d_face_tvd) HIPCHK(hipMalloc((void **)&c->d_face_tvd, sizeof(int32_t) * std::max<size_t>(1, pair.size())));
+    if (!c->d_t_face_tvd)
+        HIPCHK(hipMalloc((void **)&c->d_t_face_tvd, sizeof(int32_t) * std::max<size_t>(1, rec_pair.size())));
+    HIPCHK(hipMemcpy(c->d_face_tvd, pair.data(), sizeof(int32_t) * pair.size(), hipMemcpyHostToDevice));
+    if (!rec_pair.empty())
+        HIPCHK(hipMemcpy(c->d_t_face_tvd, rec_pair.data(), sizeof(int32_t) * rec_pair.size(), hipMemcpyHostToDevice));
+    c->tiles.face_tvd = reinterpret_cast<const int2 *>(c->d_t_face_tvd);
+    c->dp.tvd = n_active > 0 ? 1 : 0;
+    c->tile_info[7] = n_active;
+    return TITAN_OK;
+}
 
 extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_params *p) {
     if (!out || !m || !p) return TITAN_E_ARG;
@@ -349,48 +398,9 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
     launch_measure(c->stream, c->mesh(), c->d_X[0], c->geom[0], p->geom_mode);
     launch_measure(c->stream, c->mesh(), c->d_X[1], c->geom[1], p->geom_mode);
     if (hipStreamSynchronize(c->stream) != hipSuccess) { titan_destroy(c); return TITAN_E_HIP; }
-    // Setka::Init_TVD from the geometry of the initial nodes, then the pairs of the faces that
-    // reconstruct to second order (Setka_gran_snos.cpp:188-194): TVD on, both cells is_TVD, both
-    // neighbours present, not a heliopause face unless Snos_on_HP
     {
-        std::vector<double> cc(3 * nc), fc(3 * nf), fn(3 * nf);
-        if (hipMemcpy(cc.data(), c->geom[0].cell_center, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(fc.data(), c->geom[0].face_center, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(fn.data(), c->geom[0].face_normal, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost) != hipSuccess) {
-            titan_destroy(c);
-            return TITAN_E_HIP;
-        }
-        c->tvd_raw.assign(2 * nf, -1);
-        init_tvd(nf, cell_faces.data(), face_cells.data(), cc.data(), fc.data(), fn.data(), c->tvd_raw.data());
-        std::vector<int32_t> pair(2 * nf, -1);
-        int64_t n_active = 0;
-        if (p->tvd) {
-            for (int64_t d = 0; d < nf; d++) {
-                const int32_t a = face_cells[2 * d], b = face_cells[2 * d + 1];
-                const int32_t AA = c->tvd_raw[2 * d], BB = c->tvd_raw[2 * d + 1];
-                if (b < 0 || AA < 0 || BB < 0) continue;
-                if (!((cell_code[a] >> 6) & 1u) || !((cell_code[b] >> 6) & 1u)) continue;
-                const uint32_t fflags = (face_code[d] >> 8) & 255u;
-                if ((fflags & 4u) && !p->snos_on_hp) continue;
-                pair[2 * d] = AA;
-                pair[2 * d + 1] = BB;
-                n_active++;
-            }
-        }
-        std::vector<int32_t> rec_pair(2 * rec_face.size());
-        for (size_t r = 0; r < rec_face.size(); r++) {
-            rec_pair[2 * r] = pair[2 * (int64_t)rec_face[r]];
-            rec_pair[2 * r + 1] = pair[2 * (int64_t)rec_face[r] + 1];
-        }
-        up(dev_upload(&c->d_face_tvd, pair, c->stream));
-        up(dev_upload(&c->d_t_face_tvd, rec_pair, c->stream));
-        if (rc != TITAN_OK || hipStreamSynchronize(c->stream) != hipSuccess) {
-            titan_destroy(c);
-            return rc != TITAN_OK ? rc : TITAN_E_HIP;
-        }
-        c->tiles.face_tvd = reinterpret_cast<const int2 *>(c->d_t_face_tvd);
-        c->dp.tvd = n_active > 0 ? 1 : 0;
-        c->tile_info[7] = n_active;
+        const int trc = setup_tvd(c, cell_faces, face_cells, cell_code, face_code, rec_face, 0);
+        if (trc != TITAN_OK) { titan_destroy(c); return trc; }
     }
     *out = c;
     return TITAN_OK;
@@ -585,6 +595,14 @@ extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, c->stream));
+    if (c->started && c->geom_dirty) {
+        // nodes were replaced (titan_download_cell_parameters): measure both layers again, as
+        // Setka::Go does on every entry (Setka_physics.cpp:1495-1510)
+        c->tiles.vol_cached = 0;
+        launch_measure(c->stream, M, c->d_X[0], c->geom[0], c->params.geom_mode);
+        launch_measure(c->stream, M, c->d_X[1], c->geom[1], c->params.geom_mode);
+    }
+    c->geom_dirty = false;
     if (!c->started) {
         c->tiles.vol_cached = 0;
         // Setka_physics.cpp:1486-1502: copy node and cell layers 0 -> 1, measure both
@@ -659,6 +677,73 @@ extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
 extern "C" int titan_current_layer(titan_ctx *c) {
     if (!c) return TITAN_E_ARG;
     return c->started ? c->now2 : 0;
+}
+
+extern "C" int titan_init_tvd(titan_ctx *c) {
+    if (!c) return TITAN_E_ARG;
+    const int64_t nc = c->nc, nf = c->nf;
+    const int layer = c->started ? c->now2 : 0;  // the newest nodes
+    launch_measure(c->stream, c->mesh(), c->d_X[layer], c->geom[layer], c->params.geom_mode);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<int32_t> cell_faces(6 * nc), face_cells(2 * nf), rec_ref((size_t)c->tile_info[6]), rec_face;
+    std::vector<uint32_t> cell_code(nc), face_code(nf);
+    HIPCHK(hipMemcpy(cell_faces.data(), c->d_cell_faces, sizeof(int32_t) * cell_faces.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(face_cells.data(), c->d_face_cells, sizeof(int32_t) * face_cells.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cell_code.data(), c->d_cell_code, sizeof(uint32_t) * nc, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(face_code.data(), c->d_face_code, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rec_ref.data(), c->d_t_face_ref, sizeof(int32_t) * rec_ref.size(), hipMemcpyDeviceToHost));
+    rec_face.resize(rec_ref.size());
+    for (size_t r = 0; r < rec_ref.size(); r++) rec_face[r] = c->ord.face_r2d[rec_ref[r]];
+    return setup_tvd(c, cell_faces, face_cells, cell_code, face_code, rec_face, layer);
+}
+
+static const char *const kPlasmaKeys[8] = {"rho", "Vx", "Vy", "Vz", "p", "Bx", "By", "Bz"};
+
+extern "C" int titan_save_cell_parameters(titan_ctx *c, const char *path, int n_extra,
+                                          const char *const *extra_keys, const double *extra_values) {
+    if (!c || !path || n_extra < 0 || (n_extra > 0 && (!extra_keys || !extra_values))) return TITAN_E_ARG;
+    const int layer = c->started ? c->now2 : 0;
+    const int64_t nc = c->nc;
+    std::vector<double> prim(8 * nc), xyz(3 * c->nn);
+    int rc = titan_get_state(c, prim.data(), layer);
+    if (rc == TITAN_OK) rc = titan_get_nodes(c, xyz.data(), layer);
+    if (rc != TITAN_OK) return rc;
+    const int nk = 8 + n_extra;
+    std::vector<const char *> keys(nk);
+    for (int k = 0; k < 8; k++) keys[k] = kPlasmaKeys[k];
+    for (int k = 0; k < n_extra; k++) keys[8 + k] = extra_keys[k];
+    std::vector<double> vals((size_t)nk * nc);
+    for (int64_t i = 0; i < nc; i++) {
+        std::memcpy(&vals[(size_t)nk * i], &prim[8 * i], 64);
+        if (n_extra) std::memcpy(&vals[(size_t)nk * i + 8], extra_values + (size_t)n_extra * i, 8 * (size_t)n_extra);
+    }
+    return titan_checkpoint_write(path, nc, nk, keys.data(), vals.data(), c->nn, xyz.data());
+}
+
+extern "C" int titan_download_cell_parameters(titan_ctx *c, const char *path) {
+    if (!c || !path) return TITAN_E_ARG;
+    const int64_t nc = c->nc;
+    std::vector<double> vals(8 * nc), xyz(3 * c->nn);
+    int64_t nodes_read = 0;
+    int rc = titan_checkpoint_read(path, nc, 8, kPlasmaKeys, vals.data(), c->nn, xyz.data(), &nodes_read);
+    if (rc != TITAN_OK) return rc;
+    // a key the file gives a cell replaces it in both layers (Setka_physics.cpp:4238-4247)
+    std::vector<double> prim(8 * nc);
+    for (int layer = 0; layer < 2; layer++) {
+        rc = titan_get_state(c, prim.data(), layer);
+        if (rc != TITAN_OK) return rc;
+        for (size_t i = 0; i < prim.size(); i++)
+            if (!std::isnan(vals[i])) prim[i] = vals[i];
+        rc = titan_set_state(c, prim.data(), layer);
+        if (rc != TITAN_OK) return rc;
+    }
+    if (nodes_read > 0) {  // Setka_physics.cpp:4257-4282: both node layers
+        rc = titan_set_nodes(c, xyz.data(), 0);
+        if (rc == TITAN_OK) rc = titan_set_nodes(c, xyz.data(), 1);
+        if (rc != TITAN_OK) return rc;
+        c->geom_dirty = true;
+    }
+    return TITAN_OK;
 }
 
 extern "C" int titan_tvd_cells(titan_ctx *c, int32_t *tvd) {
