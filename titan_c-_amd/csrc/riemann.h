@@ -12,15 +12,18 @@
 // registers.
 #pragma once
 #include <hip/hip_runtime.h>
+#include "fdiv.h"
 
 namespace titan {
 
 template <typename T> struct RConst;
 template <> struct RConst<double> {
     static __device__ __forceinline__ double spi4() { return 3.5449077018110318; }  // sqrt(4*pi)
+    static __device__ __forceinline__ double inv_spi4() { return 1.0 / 3.5449077018110318; }
 };
 template <> struct RConst<float> {
     static __device__ __forceinline__ float spi4() { return 3.5449077018110318f; }
+    static __device__ __forceinline__ float inv_spi4() { return 1.0f / 3.5449077018110318f; }
 };
 
 // std::max / std::min semantics: the first argument wins on ties and NaN
@@ -67,17 +70,22 @@ template <typename T, int K>
 __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, T be, T ge, T w,
                                       const T qL[8], const T qR[8], const T *kL, const T *kR,
                                       RiemannOut<T, K> &out) {
-    const T spi4 = RConst<T>::spi4();
+    // Quotients by a constant or by a denominator several of them share go through its correctly
+    // rounded reciprocal (div_by_t, fdiv.h): the same correctly rounded results in fewer
+    // instructions.  Every such denominator is a positive constant or positive by construction.
+    const T spi4 = RConst<T>::spi4(), yspi4 = RConst<T>::inv_spi4();
     const T gamma = T(5.0 / 3.0);
-    const T g1 = gamma - T(1);
+    const T g1 = gamma - T(1), yg1 = T(1) / g1;
     const T epsb = T(1e-4);
     const bool left_ydar = flags & 1, contact = (flags >> 1) & 1;
     T wv = w;
 
     const T r1 = qL[0], u1 = qL[1], v1 = qL[2], w1 = qL[3], p1 = qL[4];
-    const T bx1 = qL[5] / spi4, by1 = qL[6] / spi4, bz1 = qL[7] / spi4;
+    const T bx1 = div_by_t<T>(qL[5], spi4, yspi4), by1 = div_by_t<T>(qL[6], spi4, yspi4),
+            bz1 = div_by_t<T>(qL[7], spi4, yspi4);
     const T r2 = qR[0], u2 = qR[1], v2 = qR[2], w2 = qR[3], p2 = qR[4];
-    const T bx2 = qR[5] / spi4, by2 = qR[6] / spi4, bz2 = qR[7] / spi4;
+    const T bx2 = div_by_t<T>(qR[5], spi4, yspi4), by2 = div_by_t<T>(qR[6], spi4, yspi4),
+            bz2 = div_by_t<T>(qR[7], spi4, yspi4);
     const T ro = (r2 + r1) / T(2);
     const T ap = (p2 + p1) / T(2);
 
@@ -150,7 +158,9 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
 
     const T suR = SR - vR[0];
     const T suL = SL - vL[0];
-    const T SM = (suR * r2 * vR[0] - ptR + ptL - suL * r1 * vL[0]) / (suR * r2 - suL * r1);
+    // suR > 0 > suL, so this denominator (also that of ptz) is positive
+    const T dsm = suR * r2 - suL * r1, ydsm = T(1) / dsm;
+    const T SM = div_by_t<T>(suR * r2 * vR[0] - ptR + ptL - suL * r1 * vL[0], dsm, ydsm);
     out.dsc = SM;
     if (contact) wv = SM;
 
@@ -164,8 +174,8 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
     const T sbv1 = u1 * bx1 + v1 * by1 + w1 * bz1;
     const T upt2 = (u2 * u2 + v2 * v2 + w2 * w2) / T(2);
     const T sbv2 = u2 * bx2 + v2 * by2 + w2 * bz2;
-    const T e1 = p1 / g1 + r1 * upt1 + b2L / T(2);
-    const T e2 = p2 / g1 + r2 * upt2 + b2R / T(2);
+    const T e1 = div_by_t<T>(p1, g1, yg1) + r1 * upt1 + b2L / T(2);
+    const T e2 = div_by_t<T>(p2, g1, yg1) + r2 * upt2 + b2R / T(2);
 
     // Left/right physical fluxes and conserved states (Phys_param.cpp:1069-1117), formed where
     // they are used so that only the selected side stays live in registers.
@@ -227,15 +237,16 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
         else if (SR < wv) { TR = T(0); region = 3; }
         else { region = 0; out.ok = 0; }
         const T a = TR * TL;
-        const T b = TR - TL;
+        const T b = TR - TL, yb = T(1) / b;
+        const T dRL = SR - SL, ydRL = T(1) / dRL;
 #pragma unroll
         for (int ik = 0; ik < 8; ik++) {
             T FW;
             if (region == 1) FW = wv * ul(ik);
             else if (region == 3) FW = wv * ur(ik);
-            else FW = wv * ((SR * ur(ik) - SL * ul(ik) + fl(ik) - fr(ik)) / (SR - SL));
+            else FW = wv * div_by_t<T>(SR * ur(ik) - SL * ul(ik) + fl(ik) - fr(ik), dRL, ydRL);
             const T dq = ur(ik) - ul(ik);
-            qq[ik] = (TR * fl(ik) - TL * fr(ik) + a * dq) / b - FW;
+            qq[ik] = div_by_t<T>(TR * fl(ik) - TL * fr(ik) + a * dq, b, yb) - FW;
         }
 #pragma unroll
         for (int ik = 0; ik < K; ik++) {
@@ -254,15 +265,15 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
         // UZ (HLL average) components used by HLLC and HLLD: 0..3 and 5..7
         // HLL average state UZ (Phys_param.cpp:1119-1122); HLLD needs only its normal field UZ[5]
         T UZ[8];
+        const T dRL = SR - SL, ydRL = T(1) / dRL;  // SR > SL
 #pragma unroll
         for (int ik = 0; ik < 8; ik++)
-            UZ[ik] = (n_state == 2 || ik == 5) ? (SR * ur(ik) - SL * ul(ik) + fl(ik) - fr(ik)) / (SR - SL) : T(0);
+            UZ[ik] = (n_state == 2 || ik == 5) ? div_by_t<T>(SR * ur(ik) - SL * ul(ik) + fl(ik) - fr(ik), dRL, ydRL) : T(0);
         const T suRm = suR / (SR - SM);
         const T suLm = suL / (SL - SM);
         const T rzR = r2 * suRm;
         const T rzL = r1 * suLm;
-        const T ptz = (suR * r2 * ptL - suL * r1 * ptR + r1 * r2 * suR * suL * (vR[0] - vL[0])) /
-                      (suR * r2 - suL * r1);
+        const T ptz = div_by_t<T>(suR * r2 * ptL - suL * r1 * ptR + r1 * r2 * suR * suL * (vR[0] - vL[0]), dsm, ydsm);
         // A region left of the contact uses the left side's values and one right of it the right
         // side's: selecting the operands first (a negation or a select is exact) gives one code
         // path per kind of region, so a wavefront whose faces fall on both sides of the contact
@@ -315,10 +326,18 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
             T tvR, tbR, tvL, tbL;
             const T ttR = r2 * suR * (SR - SM) - bn2;
             if (fabs(ttR) < T(0.000000001)) { tvR = T(0); tbR = T(0); }
-            else { tvR = (SM - vR[0]) / ttR; tbR = (r2 * suR * suR - bn2) / ttR; }
+            else {
+                const T y = T(1) / ttR;
+                tvR = div_by_t<T>(SM - vR[0], ttR, y);
+                tbR = div_by_t<T>(r2 * suR * suR - bn2, ttR, y);
+            }
             const T ttL = r1 * suL * (SL - SM) - bn2;
             if (fabs(ttL) < T(0.000000001)) { tvL = T(0); tbL = T(0); }
-            else { tvL = (SM - vL[0]) / ttL; tbL = (r1 * suL * suL - bn2) / ttL; }
+            else {
+                const T y = T(1) / ttL;
+                tvL = div_by_t<T>(SM - vL[0], ttL, y);
+                tbL = div_by_t<T>(r1 * suL * suL - bn2, ttL, y);
+            }
             const T vzL1 = vL[1] - bn * bL[1] * tvL;
             const T vzL2 = vL[2] - bn * bL[2] * tvL;
             const T vzR1 = vR[1] - bn * bR[1] * tvR;
@@ -334,7 +353,7 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
             T SZL = SM - fabs(bn) / rzLs;
             T SZR = SM + fabs(bn) / rzRs;
             T sbn;
-            if (fabs(bn) > epsb) { sbn = fabs(bn) / bn; ibn = 1; }
+            if (fabs(bn) > epsb) { sbn = bn > T(0) ? T(1) : T(-1); ibn = 1; }  // = |bn| / bn exactly
             else { sbn = T(0); ibn = 0; SZL = SM; SZR = SM; }
             int j = -1;
             if (left_ydar) wv = SL;
@@ -367,10 +386,11 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
                         qq[ik] = f + S * (UZs[ik] - u) - wv * UZs[ik];
                     }
                 } else {
-                    const T vzz1 = (rzLs * vzL1 + rzRs * vzR1 + sbn * (bzR1 - bzL1)) / rzss;
-                    const T vzz2 = (rzLs * vzL2 + rzRs * vzR2 + sbn * (bzR2 - bzL2)) / rzss;
-                    const T bzz1 = (rzLs * bzR1 + rzRs * bzL1 + sbn * rzps * (vzR1 - vzL1)) / rzss;
-                    const T bzz2 = (rzLs * bzR2 + rzRs * bzL2 + sbn * rzps * (vzR2 - vzL2)) / rzss;
+                    const T yz = T(1) / rzss;  // a sum of two positive square roots
+                    const T vzz1 = div_by_t<T>(rzLs * vzL1 + rzRs * vzR1 + sbn * (bzR1 - bzL1), rzss, yz);
+                    const T vzz2 = div_by_t<T>(rzLs * vzL2 + rzRs * vzR2 + sbn * (bzR2 - bzL2), rzss, yz);
+                    const T bzz1 = div_by_t<T>(rzLs * bzR1 + rzRs * bzL1 + sbn * rzps * (vzR1 - vzL1), rzss, yz);
+                    const T bzz2 = div_by_t<T>(rzLs * bzR2 + rzRs * bzL2 + sbn * rzps * (vzR2 - vzL2), rzss, yz);
                     const T sbzz = bn * SM + bzz1 * vzz1 + bzz2 * vzz2;
                     // ezzL = ezL - rzLs sbn (sbvL - sbzz), ezzR = ezR + rzRs sbn (sbvR - sbzz)
                     const T dz = (R ? rzRs : rzLs) * sbn * (sbvs - sbzz);

@@ -3,6 +3,7 @@
 // (Setka_gran_snos.cpp:744-873); shared by the step kernels and the TVD reconstruction.
 #pragma once
 #include <hip/hip_runtime.h>
+#include "fdiv.h"
 
 namespace titan {
 
@@ -20,10 +21,15 @@ __device__ __forceinline__ Trig<Z> frame_trig(Z z, Z x, Z y) {
     const Z r = sqrt(x * x + y * y + z * z);
     const Z rho = sqrt(x * x + y * y);
     Trig<Z> t;
-    t.st = rho / r;
-    t.ct = z / r;
+    const Z yr = Z(1) / r;
+    t.st = div_by_t<Z>(rho, r, yr);
+    t.ct = div_by_t<Z>(z, r, yr);
     if (fabs(x) + fabs(y) < Z(0.000001)) { t.sp = Z(0); t.cp = Z(1); }
-    else { t.sp = y / rho; t.cp = x / rho; }
+    else {
+        const Z yrho = Z(1) / rho;
+        t.sp = div_by_t<Z>(y, rho, yrho);
+        t.cp = div_by_t<Z>(x, rho, yrho);
+    }
     return t;
 }
 
@@ -52,7 +58,7 @@ __device__ __forceinline__ Zone1Cell<Z> zone1_cell(const double c[3], double gam
 }
 
 // Face-centre data of a face touching a Zone_1 cell: |G|, |G|^(2 gamma), frame choice and trig.
-template <typename Z> struct Zone1Face { Z rr, prr; bool equatorial; Trig<Z> t; };
+template <typename Z> struct Zone1Face { Z rr, prr, rr2, yrr, yrr2, yprr; bool equatorial; Trig<Z> t; };
 
 template <typename Z>
 __device__ __forceinline__ Zone1Face<Z> zone1_face(const double G[3], double gamma) {
@@ -60,9 +66,14 @@ __device__ __forceinline__ Zone1Face<Z> zone1_face(const double G[3], double gam
     Zone1Face<Z> f;
     f.rr = sqrt((g0 * g0 + g1 * g1) + g2 * g2);
     f.prr = pow_2gamma<Z>(f.rr, gamma);
+    // reciprocals of the face's radial scales, shared by the two cells of the face
+    f.rr2 = f.rr * f.rr;
+    f.yrr = Z(1) / f.rr;
+    f.yrr2 = Z(1) / f.rr2;
+    f.yprr = Z(1) / f.prr;
     // the reference's frame test pi/9 < acos(G2/|G|) < 8 pi/9 (Setka_gran_snos.cpp:771-773), as
     // |G2/|G|| < cos(pi/9); either frame gives the same rescaled state up to rounding
-    const Z ct = g2 / f.rr;
+    const Z ct = div_by_t<Z>(g2, f.rr, f.yrr);
     const Z c9 = Z(0.93969262078590838405);
     f.equatorial = (ct < c9 && ct > -c9);
     f.t = f.equatorial ? frame_trig<Z>(g2, g0, g1) : frame_trig<Z>(g0, g1, g2);
@@ -87,9 +98,10 @@ __device__ __forceinline__ void zone1_apply(const Z s[8], const Zone1Cell<Z> &zc
         Z Vtheta = Vx * a.ct * a.cp + Vy * a.ct * a.sp - Vz * a.st;
         Z Vphi = -Vx * a.sp + Vy * a.cp;
         if (ik == 1) {
-            Vr = Vr * ((zc.r1 / zf.rr) * (zc.r1 / zf.rr));
-            Vphi = Vphi * ((zc.r1) / (zf.rr));
-            Vtheta = Vtheta * ((zc.r1) / (zf.rr));
+            const Z q = div_by_t<Z>(zc.r1, zf.rr, zf.yrr);
+            Vr = Vr * (q * q);
+            Vphi = Vphi * q;
+            Vtheta = Vtheta * q;
         }
         const Z ox = Vr * b.st * b.cp + Vtheta * b.ct * b.cp - Vphi * b.sp;
         const Z oy = Vr * b.st * b.sp + Vtheta * b.ct * b.sp + Vphi * b.cp;
@@ -97,8 +109,8 @@ __device__ __forceinline__ void zone1_apply(const Z s[8], const Zone1Cell<Z> &zc
         if (zf.equatorial) { O[2] = oz; O[0] = ox; O[1] = oy; }
         else { O[0] = oz; O[1] = ox; O[2] = oy; }
     }
-    out[0] = s[0] * (zc.r1 * zc.r1) / (zf.rr * zf.rr);
-    out[4] = s[4] * zc.pr1 / zf.prr;
+    out[0] = div_by_t<Z>(s[0] * (zc.r1 * zc.r1), zf.rr2, zf.yrr2);
+    out[4] = div_by_t<Z>(s[4] * zc.pr1, zf.prr, zf.yprr);
 }
 
 }  // namespace titan

@@ -11,6 +11,7 @@
 #include "riemann.h"
 #include "riemann_mixed.h"
 #include "tvd.h"
+#include "fdiv.h"
 
 namespace titan {
 
@@ -58,7 +59,7 @@ __device__ __forceinline__ double tet_volume(const double *A, const double *B, c
     double cy = abz * acx - abx * acz;
     double cz = abx * acy - aby * acx;
     double d = cx * adx + cy * ady + cz * adz;
-    return fabs(d) / 6.0;
+    return div_by(fabs(d), 6.0, 1.0 / 6.0);
 }
 
 #ifndef TITAN_FLUX_ONLY
@@ -472,27 +473,33 @@ __global__ void __launch_bounds__(256) k_face_flux(DevMesh M, DevParams P, const
 __device__ __forceinline__ void cell_update_state(const DevMesh &M, uint32_t code, int64_t c, const double s[8],
                                                   double Volume, double Volume2, const double Pf[9], const double *src,
                                                   double time, StepScalars *S, double *__restrict__ prim_new) {
-    const double cpi4 = 4.0 * kPi;
-    const double g1 = 5.0 / 3.0 - 1.0;
+    // every quotient below is the reference's, correctly rounded; those by V^{n+1}, rho', 4 pi,
+    // gamma - 1 and 8 pi go through the correctly rounded reciprocal (fdiv.h)
+    const double cpi4 = 4.0 * kPi, ycpi4 = 1.0 / (4.0 * kPi);
+    const double g1 = 5.0 / 3.0 - 1.0, yg1 = 1.0 / (5.0 / 3.0 - 1.0);
+    const double c8pi = 25.13274122871834590768, y8pi = 1.0 / 25.13274122871834590768;
     const double rho = s[0], vx = s[1], vy = s[2], vz = s[3], pp = s[4], bx = s[5], by = s[6], bz = s[7];
     double Smx = 0.0, Smy = 0.0, Smz = 0.0, SE = 0.0;
     if (src) { Smx = src[1]; Smy = src[2]; Smz = src[3]; SE = src[4]; }
     const double dsk = spf(vx, vy, vz, bx, by, bz);
-    double rho3 = rho * Volume / Volume2 - time * Pf[0] / Volume2;
+    const double yV = 1.0 / Volume2;
+    auto perV = [&](double a) { return div_by(a, Volume2, yV); };
+    double rho3 = perV(rho * Volume) - perV(time * Pf[0]);
     bool rho_null = false;
     if (rho3 < 1e-7) { rho_null = true; rho3 = 0.1; atomicAdd(&S->n_rho_floor, 1ull); }
     double u3, v3, w3, bx3, by3, bz3, p3;
     if (!rho_null) {
-        u3 = (rho * vx * Volume / Volume2 - time * (Pf[1] + (bx / cpi4) * Pf[8]) / Volume2 + time * Smx) / rho3;
-        v3 = (rho * vy * Volume / Volume2 - time * (Pf[2] + (by / cpi4) * Pf[8]) / Volume2 + time * Smy) / rho3;
-        w3 = (rho * vz * Volume / Volume2 - time * (Pf[3] + (bz / cpi4) * Pf[8]) / Volume2 + time * Smz) / rho3;
-        bx3 = bx * Volume / Volume2 - time * (Pf[5] + vx * Pf[8]) / Volume2;
-        by3 = by * Volume / Volume2 - time * (Pf[6] + vy * Pf[8]) / Volume2;
-        bz3 = bz * Volume / Volume2 - time * (Pf[7] + vz * Pf[8]) / Volume2;
-        p3 = (((pp / g1 + 0.5 * rho * (vx * vx + vy * vy + vz * vz) + (bx * bx + by * by + bz * bz) / 25.13274122871834590768) *
-                   Volume / Volume2 -
-               time * (Pf[4] + (dsk / cpi4) * Pf[8]) / Volume2 + time * SE) -
-              0.5 * rho3 * (u3 * u3 + v3 * v3 + w3 * w3) - (bx3 * bx3 + by3 * by3 + bz3 * bz3) / 25.13274122871834590768) *
+        const double yr = 1.0 / rho3;
+        u3 = div_by(perV(rho * vx * Volume) - perV(time * (Pf[1] + div_by(bx, cpi4, ycpi4) * Pf[8])) + time * Smx, rho3, yr);
+        v3 = div_by(perV(rho * vy * Volume) - perV(time * (Pf[2] + div_by(by, cpi4, ycpi4) * Pf[8])) + time * Smy, rho3, yr);
+        w3 = div_by(perV(rho * vz * Volume) - perV(time * (Pf[3] + div_by(bz, cpi4, ycpi4) * Pf[8])) + time * Smz, rho3, yr);
+        bx3 = perV(bx * Volume) - perV(time * (Pf[5] + vx * Pf[8]));
+        by3 = perV(by * Volume) - perV(time * (Pf[6] + vy * Pf[8]));
+        bz3 = perV(bz * Volume) - perV(time * (Pf[7] + vz * Pf[8]));
+        p3 = ((perV((div_by(pp, g1, yg1) + 0.5 * rho * (vx * vx + vy * vy + vz * vz) +
+                     div_by(bx * bx + by * by + bz * bz, c8pi, y8pi)) * Volume) -
+               perV(time * (Pf[4] + div_by(dsk, cpi4, ycpi4) * Pf[8])) + time * SE) -
+              0.5 * rho3 * (u3 * u3 + v3 * v3 + w3 * w3) - div_by(bx3 * bx3 + by3 * by3 + bz3 * bz3, c8pi, y8pi)) *
              g1;
         if (((code >> 8) & 15u) == 2u) {  // Zone_2 magnetic decay, tau = 1
             const double tau = 1.0;

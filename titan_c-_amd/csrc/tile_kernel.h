@@ -45,8 +45,8 @@ __device__ __forceinline__ void face_geometry(const double x[4][3], const double
         n1 = ay * bz - az * by;
         n2 = az * bx - ax * bz;
         n3 = ax * by - ay * bx;
-        const double nn = sqrt(n1 * n1 + n2 * n2 + n3 * n3);
-        n1 /= nn; n2 /= nn; n3 /= nn;
+        const double nn = sqrt(n1 * n1 + n2 * n2 + n3 * n3), yn = 1.0 / nn;
+        n1 = div_by(n1, nn, yn); n2 = div_by(n2, nn, yn); n3 = div_by(n3, nn, yn);
     } else {
         const double ax = x[2][0] - x[0][0], ay = x[2][1] - x[0][1], az = x[2][2] - x[0][2];
         const double bx = x[3][0] - x[1][0], by = x[3][1] - x[1][1], bz = x[3][2] - x[1][2];
@@ -54,7 +54,8 @@ __device__ __forceinline__ void face_geometry(const double x[4][3], const double
         const double s2 = 0.5 * (az * bx - ax * bz);
         const double s3 = 0.5 * (ax * by - ay * bx);
         A = sqrt(s1 * s1 + s2 * s2 + s3 * s3);
-        n1 = s1 / A; n2 = s2 / A; n3 = s3 / A;
+        const double yA = 1.0 / A;
+        n1 = div_by(s1, A, yA); n2 = div_by(s2, A, yA); n3 = div_by(s3, A, yA);
     }
     if (spf(n1, n2, n3, ca[0] - xc, ca[1] - yc, ca[2] - zc) > 0.0) { n1 = -n1; n2 = -n2; n3 = -n3; }
     n[0] = n1; n[1] = n2; n[2] = n3;
@@ -221,7 +222,7 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
                 const double d0 = x2[i][0] - x1[i][0], d1 = x2[i][1] - x1[i][1], d2 = x2[i][2] - x1[i][2];
                 ddot = ddot + ((d0 * n[0] + d1 * n[1]) + d2 * n[2]);
             }
-            w = ddot / 4 / dt;
+            w = div_by(ddot / 4, dt, 1.0 / dt);  // 1/dt is loop-invariant: formed once per thread
         }
     }
     // face states (Snos_on_Gran)

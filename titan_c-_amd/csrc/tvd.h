@@ -44,15 +44,25 @@ __device__ __forceinline__ double tvd_minmod(double x, double y) {
     return (s / 2.0) * ((ay < ax) ? ay : ax);
 }
 
-// Help_funk.cpp:55-62, at y = 0
-__device__ __forceinline__ double tvd_linear(double x1, double t1, double x2, double t2, double x3, double t3) {
-    const double d = tvd_minmod((t1 - t2) / (x1 - x2), (t2 - t3) / (x2 - x3));
+// The abscissae of a face's lines are the same for all eight components: the differences
+// x1 - x2 and x2 - x3 and their correctly rounded reciprocals are formed once per face, and the
+// slopes (t1 - t2) / (x1 - x2) come out correctly rounded through div_by (fdiv.h).
+struct TvdGap { double d, y; };  // a difference of abscissae and its reciprocal
+
+__device__ __forceinline__ TvdGap tvd_gap(double x1, double x2) {
+    const double d = x1 - x2;
+    return {d, 1.0 / d};
+}
+
+// Help_funk.cpp:55-62, at y = 0: the minmod-limited line through (x1,t1), (x2,t2), (x3,t3)
+__device__ __forceinline__ double tvd_linear(TvdGap g12, TvdGap g23, double x2, double t1, double t2, double t3) {
+    const double d = tvd_minmod(div_by(t1 - t2, g12.d, g12.y), div_by(t2 - t3, g23.d, g23.y));
     return d * (0.0 - x2) + t2;
 }
 
-// Help_funk.cpp:64-69, at y = 0
-__device__ __forceinline__ double tvd_linear2(double x1, double t1, double x2, double t2) {
-    const double d = (t1 - t2) / (x1 - x2);
+// Help_funk.cpp:64-69, at y = 0: the line through (x1,t1), (x2,t2)
+__device__ __forceinline__ double tvd_linear2(TvdGap g12, double x2, double t1, double t2) {
+    const double d = div_by(t1 - t2, g12.d, g12.y);
     return d * (0.0 - x2) + t2;
 }
 
@@ -104,9 +114,9 @@ __device__ __forceinline__ void tvd_back(const double t[8], int mode, const Zone
         double *O = q + (ik == 0 ? 1 : 5);
         double Vr = T[0], Vphi = T[1], Vtheta = T[2];
         if (ik == 1 && mode == 1) {
-            Vr = Vr / (F.rr * F.rr);
-            Vphi = Vphi / F.rr;
-            Vtheta = Vtheta / F.rr;
+            Vr = div_by(Vr, F.rr2, F.yrr2);
+            Vphi = div_by(Vphi, F.rr, F.yrr);
+            Vtheta = div_by(Vtheta, F.rr, F.yrr);
         }
         const double ox = Vr * b.st * b.cp + Vtheta * b.ct * b.cp - Vphi * b.sp;
         const double oy = Vr * b.st * b.sp + Vtheta * b.ct * b.sp + Vphi * b.cp;
@@ -114,8 +124,8 @@ __device__ __forceinline__ void tvd_back(const double t[8], int mode, const Zone
         if (F.equatorial) { O[2] = oz; O[0] = ox; O[1] = oy; }
         else { O[0] = oz; O[1] = ox; O[2] = oy; }
     }
-    q[0] = mode == 1 ? t[0] / (F.rr * F.rr) : t[0];
-    q[4] = t[4] / F.prr;
+    q[0] = mode == 1 ? div_by(t[0], F.rr2, F.yrr2) : t[0];
+    q[4] = div_by(t[4], F.prr, F.yprr);
 }
 
 // One row of the table above: variables of the A side and the B side; left line 0 = AA,A,B
@@ -149,7 +159,7 @@ __device__ __forceinline__ void tvd_states(uint32_t zAA, uint32_t zA, uint32_t z
     const double dd1 = tvd_norm3(cA[0] - cAA[0], cA[1] - cAA[1], cA[2] - cAA[2]);
     const double dd2 = tvd_norm3(cB[0] - cBB[0], cB[1] - cBB[1], cB[2] - cBB[2]);
     const TvdCase k = tvd_case(zAA, zA, zB, zBB);
-    Zone1Face<double> F = {1.0, 1.0, true, {1.0, 0.0, 0.0, 1.0}};
+    Zone1Face<double> F = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0, true, {1.0, 0.0, 0.0, 1.0}};
     if (k.ma != 0) F = zone1_face<double>(G, gamma);
     double tAA[8], tA[8], tB[8], tBB[8];
     tvd_forward(sAA, cAA, k.ma, F, gamma, tAA);
@@ -157,13 +167,14 @@ __device__ __forceinline__ void tvd_states(uint32_t zAA, uint32_t zA, uint32_t z
     tvd_forward(sB, cB, k.mb, F, gamma, tB);
     tvd_forward(sBB, cBB, k.mb, F, gamma, tBB);
     const double xAA = -dd1 - d1, xA = -d1, xB = d2, xBB = d2 + dd2;
+    const TvdGap gL = tvd_gap(xAA, xA), gC = tvd_gap(xA, xB), gR = tvd_gap(xB, xBB);
     double l[8], r[8];
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-        l[i] = k.lm == 0 ? tvd_linear(xAA, tAA[i], xA, tA[i], xB, tB[i])
-                         : (k.lm == 1 ? tvd_linear2(xAA, tAA[i], xA, tA[i]) : tvd_linear2(xA, tA[i], xB, tB[i]));
-        r[i] = k.rm == 0 ? tvd_linear(xA, tA[i], xB, tB[i], xBB, tBB[i])
-                         : (k.rm == 1 ? tvd_linear2(xA, tA[i], xB, tB[i]) : tvd_linear2(xB, tB[i], xBB, tBB[i]));
+        l[i] = k.lm == 0 ? tvd_linear(gL, gC, xA, tAA[i], tA[i], tB[i])
+                         : (k.lm == 1 ? tvd_linear2(gL, xA, tAA[i], tA[i]) : tvd_linear2(gC, xB, tA[i], tB[i]));
+        r[i] = k.rm == 0 ? tvd_linear(gC, gR, xB, tA[i], tB[i], tBB[i])
+                         : (k.rm == 1 ? tvd_linear2(gC, xB, tA[i], tB[i]) : tvd_linear2(gR, xBB, tB[i], tBB[i]));
     }
     tvd_back(l, k.ma, F, qL);
     tvd_back(r, k.mb, F, qR);
