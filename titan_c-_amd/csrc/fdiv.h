@@ -29,4 +29,5 @@ __host__ __device__ __forceinline__ T div_by_t(T a, T b, T y) {
     else { (void)y; return a / b; }
 }
 
+
 }  // namespace titan

@@ -13,6 +13,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "fdiv.h"
+#include "fsqrt.h"
 
 namespace titan {
 
@@ -45,9 +46,9 @@ __device__ __forceinline__ void get_bazis(const T n[3], T t[3], T m[3]) {
     T c1 = a2 * n[0] - a0 * n[2];
     T c2 = a0 * n[1] - a1 * n[0];
     T z = (c0 * c0 + c1 * c1) + c2 * c2;
-    if (z > T(0)) {
-        T s = sqrt(z);
-        c0 = c0 / s; c1 = c1 / s; c2 = c2 / s;
+    if (z > T(0)) {  // z = n_y^2 + n_z^2 >= 0.19 or n_x^2 + n_z^2 >= 0.19 for a unit normal
+        const T s = sqrt_t<T>(z), ys = T(1) / s;
+        c0 = div_by_t<T>(c0, s, ys); c1 = div_by_t<T>(c1, s, ys); c2 = div_by_t<T>(c2, s, ys);
     }
     t[0] = c0; t[1] = c1; t[2] = c2;
     m[0] = n[1] * t[2] - n[2] * t[1];
@@ -106,29 +107,29 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
     bR[1] = spf(t[0], t[1], t[2], bx2, by2, bz2);
     bR[2] = spf(m[0], m[1], m[2], bx2, by2, bz2);
 
-    const T aaL = bL[0] / sqrt(r1);
+    const T aaL = bL[0] / sqrt_t<T>(r1);
     const T b2L = (bL[0] * bL[0] + bL[1] * bL[1]) + bL[2] * bL[2];
     const T b21 = b2L / r1;
-    const T cL = sqrt(gamma * p1 / r1);
-    T qp = sqrt(b21 + cL * (cL + T(2) * aaL));
-    T qm = sqrt(b21 + cL * (cL - T(2) * aaL));
+    const T cL = sqrt_t<T>(gamma * p1 / r1);
+    T qp = sqrt_t<T>(b21 + cL * (cL + T(2) * aaL));
+    T qm = sqrt_t<T>(b21 + cL * (cL - T(2) * aaL));
     const T cfL = (qp + qm) / T(2);
     const T ptL = p1 + b2L / T(2);
 
-    const T aaR = bR[0] / sqrt(r2);
+    const T aaR = bR[0] / sqrt_t<T>(r2);
     const T b2R = (bR[0] * bR[0] + bR[1] * bR[1]) + bR[2] * bR[2];
     const T b22 = b2R / r2;
-    const T cR = sqrt(gamma * p2 / r2);
-    qp = sqrt(b22 + cR * (cR + T(2) * aaR));
-    qm = sqrt(b22 + cR * (cR - T(2) * aaR));
+    const T cR = sqrt_t<T>(gamma * p2 / r2);
+    qp = sqrt_t<T>(b22 + cR * (cR + T(2) * aaR));
+    qm = sqrt_t<T>(b22 + cR * (cR - T(2) * aaR));
     const T cfR = (qp + qm) / T(2);
     const T ptR = p2 + b2R / T(2);
 
     const T aC = (aaL + aaR) / T(2);
     const T b2o = (b22 + b21) / T(2);
-    const T cC = sqrt(gamma * ap / ro);
-    qp = sqrt(b2o + cC * (cC + T(2) * aC));
-    qm = sqrt(b2o + cC * (cC - T(2) * aC));
+    const T cC = sqrt_t<T>(gamma * ap / ro);
+    qp = sqrt_t<T>(b2o + cC * (cC + T(2) * aC));
+    qm = sqrt_t<T>(b2o + cC * (cC - T(2) * aC));
     const T cfC = (qp + qm) / T(2);
     const T vC1 = (vL[0] + vR[0]) / T(2);
 
@@ -346,8 +347,8 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
             const T bzR1 = bR[1] * tbR, bzR2 = bR[2] * tbR;
             const T sbvL = bn * SM + bzL1 * vzL1 + bzL2 * vzL2;
             const T sbvR = bn * SM + bzR1 * vzR1 + bzR2 * vzR2;
-            const T rzRs = sqrt(rzR);
-            const T rzLs = sqrt(rzL);
+            const T rzRs = sqrt_t<T>(rzR);
+            const T rzLs = sqrt_t<T>(rzL);
             const T rzss = rzRs + rzLs;
             const T rzps = rzRs * rzLs;
             T SZL = SM - fabs(bn) / rzLs;

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "fdiv.h"
+#include "fsqrt.h"
 
 namespace titan {
 
@@ -18,8 +19,8 @@ template <typename Z> struct Trig { Z st, ct, sp, cp; };
 // libm already differ).  The V and B rotations share them.
 template <typename Z>
 __device__ __forceinline__ Trig<Z> frame_trig(Z z, Z x, Z y) {
-    const Z r = sqrt(x * x + y * y + z * z);
-    const Z rho = sqrt(x * x + y * y);
+    const Z r = sqrt_t<Z>(x * x + y * y + z * z);
+    const Z rho = sqrt_t<Z>(x * x + y * y);
     Trig<Z> t;
     const Z yr = Z(1) / r;
     t.st = div_by_t<Z>(rho, r, yr);
@@ -50,7 +51,7 @@ template <typename Z>
 __device__ __forceinline__ Zone1Cell<Z> zone1_cell(const double c[3], double gamma) {
     const Z c0 = (Z)c[0], c1 = (Z)c[1], c2 = (Z)c[2];
     Zone1Cell<Z> z;
-    z.r1 = sqrt((c0 * c0 + c1 * c1) + c2 * c2);
+    z.r1 = sqrt_t<Z>((c0 * c0 + c1 * c1) + c2 * c2);
     z.pr1 = pow_2gamma<Z>(z.r1, gamma);
     z.e = frame_trig<Z>(c2, c0, c1);
     z.p = frame_trig<Z>(c0, c1, c2);
@@ -64,7 +65,7 @@ template <typename Z>
 __device__ __forceinline__ Zone1Face<Z> zone1_face(const double G[3], double gamma) {
     const Z g0 = (Z)G[0], g1 = (Z)G[1], g2 = (Z)G[2];
     Zone1Face<Z> f;
-    f.rr = sqrt((g0 * g0 + g1 * g1) + g2 * g2);
+    f.rr = sqrt_t<Z>((g0 * g0 + g1 * g1) + g2 * g2);
     f.prr = pow_2gamma<Z>(f.rr, gamma);
     // reciprocals of the face's radial scales, shared by the two cells of the face
     f.rr2 = f.rr * f.rr;

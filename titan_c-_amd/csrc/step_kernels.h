@@ -12,6 +12,7 @@
 #include "riemann_mixed.h"
 #include "tvd.h"
 #include "fdiv.h"
+#include "fsqrt.h"
 
 namespace titan {
 
@@ -47,7 +48,7 @@ __device__ __forceinline__ double tri_area(const double *a, const double *b, con
     double cx = v1y * v2z - v1z * v2y;
     double cy = v1z * v2x - v1x * v2z;
     double cz = v1x * v2y - v1y * v2x;
-    return 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+    return 0.5 * sqrt_n(cx * cx + cy * cy + cz * cz);
 }
 
 __device__ __forceinline__ double tet_volume(const double *A, const double *B, const double *C, const double *D) {

@@ -45,7 +45,7 @@ __device__ __forceinline__ void face_geometry(const double x[4][3], const double
         n1 = ay * bz - az * by;
         n2 = az * bx - ax * bz;
         n3 = ax * by - ay * bx;
-        const double nn = sqrt(n1 * n1 + n2 * n2 + n3 * n3), yn = 1.0 / nn;
+        const double nn = sqrt_n(n1 * n1 + n2 * n2 + n3 * n3), yn = 1.0 / nn;
         n1 = div_by(n1, nn, yn); n2 = div_by(n2, nn, yn); n3 = div_by(n3, nn, yn);
     } else {
         const double ax = x[2][0] - x[0][0], ay = x[2][1] - x[0][1], az = x[2][2] - x[0][2];
@@ -53,7 +53,7 @@ __device__ __forceinline__ void face_geometry(const double x[4][3], const double
         const double s1 = 0.5 * (ay * bz - az * by);
         const double s2 = 0.5 * (az * bx - ax * bz);
         const double s3 = 0.5 * (ax * by - ay * bx);
-        A = sqrt(s1 * s1 + s2 * s2 + s3 * s3);
+        A = sqrt_n(s1 * s1 + s2 * s2 + s3 * s3);
         const double yA = 1.0 / A;
         n1 = div_by(s1, A, yA); n2 = div_by(s2, A, yA); n3 = div_by(s3, A, yA);
     }
@@ -155,7 +155,7 @@ __device__ __forceinline__ double tile_volume(const double *X, const double cc[3
             const double s1 = 0.5 * (ay * bz - az * by);
             const double s2 = 0.5 * (az * bx - ax * bz);
             const double s3 = 0.5 * (ax * by - ay * bx);
-            const double A = sqrt(s1 * s1 + s2 * s2 + s3 * s3);
+            const double A = sqrt_n(s1 * s1 + s2 * s2 + s3 * s3);
             double n1 = s1 / A, n2 = s2 / A, n3 = s3 / A;
             double xc = 0.0, yc = 0.0, zc = 0.0;
             xc += x0[0]; yc += x0[1]; zc += x0[2];
@@ -305,7 +305,7 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
     if (!o.ok || !okbc) report_error(S, -3, TM.face_ref[fb + f]);
     if (bc == 0 && o.ok) {
         const double d0 = ca[0] - cb[0], d1 = ca[1] - cb[1], d2 = ca[2] - cb[2];
-        const double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2) / 2.0;
+        const double dist = sqrt_n(d0 * d0 + d1 * d1 + d2 * d2) / 2.0;
         const double dnt = P.kfl * dist / (smax(fabs((double)o.dsl), fabs((double)o.dsr)) + fabs(w));
         if (dnt < cand) cand = dnt;
     }

@@ -66,7 +66,7 @@ __device__ __forceinline__ double tvd_linear2(TvdGap g12, double x2, double t1, 
     return d * (0.0 - x2) + t2;
 }
 
-__device__ __forceinline__ double tvd_norm3(double a, double b, double c) { return sqrt((a * a + b * b) + c * c); }
+__device__ __forceinline__ double tvd_norm3(double a, double b, double c) { return sqrt_n((a * a + b * b) + c * c); }
 
 // Variables of one reconstruction: 0 = primitives, 1 = spherical and radially scaled, 2 = spherical
 // with p scaled.  Spherical slots hold (r, phi, theta) at 1..3 (V) and 5..7 (B).
