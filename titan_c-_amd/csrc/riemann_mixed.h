@@ -125,7 +125,10 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
         db[i] = (float)(bR[i] - bL[i]);
     }
     const float dpt = (float)(ptRd - ptLd);
-    const float fe1 = (float)e1d, fe2 = (float)e2d;
+    // the side energies less their kinetic part (p / (gamma - 1) + b^2 / 2): what the star jumps
+    // add to the kinetic energy the mass correction carries, which is formed in FP64 (kin0 below)
+    const float fei1 = (float)(e1d - r1 * (0.5 * ((vL[0] * vL[0] + vL[1] * vL[1]) + vL[2] * vL[2]))),
+                fei2 = (float)(e2d - r2 * (0.5 * ((vR[0] * vR[0] + vR[1] * vR[1]) + vR[2] * vR[2])));
     const float b2L = (float)b2Ld, b2R = (float)b2Rd;
     const float ptL = (float)ptLd, ptR = (float)ptRd;
     const float wv0 = (float)w;
@@ -174,15 +177,15 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
     // the same speeds with FP64 velocities: the face velocity of the upwind part when the
     // reference moves the face with S_L (left_ydar), and the normal-field dissipation speed
     double SLd = vL[0] + (double)suL, SRd = vR[0] + (double)suR;
-    out.dsr = SR;
-    out.dsl = SL;
+    out.dsr = (double)SR;
+    out.dsl = (double)SL;
     const float D = suR * fr2 - suL * fr1;
     // S_M relative to each side's normal velocity, from the jumps
     const float dSML = (suR * fr2 * dv[0] - dpt) / D;  // S_M - v_L
     const float dSMR = (suL * fr1 * dv[0] - dpt) / D;  // S_M - v_R
     const float SM = fvL[0] + dSML;
     const float gL = suL - dSML, gR = suR - dSMR;  // S_L - S_M, S_R - S_M
-    out.dsc = SM;
+    out.dsc = (double)SM;
     // wv: the face velocity of the region tests (FP32); wd: the same in FP64 for the upwind part
     float wv = wv0;
     double wd = w;
@@ -210,6 +213,12 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
 #pragma unroll
     for (int i = 0; i < (K > 0 ? K : 1); i++) kcorr[i] = 0.0f;
     bool right = false;  // upwind side of the selected region
+    // Star regions: corr[0] is the mass correction (S - w)(rho* - rho); the momentum and energy it
+    // carries at the side's velocity, v corr[0] and |v|^2/2 corr[0], are added in FP64 and corr[1..4]
+    // hold only the rest.  Formed in FP32 the full jumps would each round the tangential kinetic
+    // energy of a hypersonic face separately, and the FP64 pressure recovery
+    // p = (E - rho v^2/2 - B^2/8 pi)(gamma - 1) would see that rounding times the Mach number squared.
+    bool split = false;
     int region = 0, ibn = 0;
     out.ok = 1;
     int wbn_side;  // -1 left normal field, +1 right, 0 average
@@ -221,7 +230,9 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
         else if (SR < wv) { region = 3; right = true; }
         else { region = 0; out.ok = 0; }
         if (region == 2) {
-            const float c = (SL - wv) / (SR - SL);
+            // S - w from the FP64 speeds: a face moving at nearly a wave speed (moving mesh) would
+            // otherwise lose the difference to the FP32 rounding of S and w
+            const float c = (float)((SLd - wd) / (SRd - SLd));
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 float dU, dF;
@@ -238,7 +249,7 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
         wbn_side = (wv >= SR) ? 1 : ((wv <= SL) ? -1 : 0);
     } else {
         // normal field of the HLL state (UZ[5]) relative to each side: S_R db/(S_R-S_L), S_L db/(S_R-S_L)
-        const float dbnL = SR * db[0] / (SR - SL);
+        const float dbnL = (float)SRd * db[0] / (SR - SL);
         const float bn = fbL[0] + dbnL;
         const float dbnR = dbnL - db[0];
         const float suRm = suR / gR;
@@ -263,7 +274,7 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
                 const float dSM = R ? dSMR : dSML, dr = R ? drR : drL, rz = R ? rzR : rzL;
                 const float dptz = R ? dptzR : dptzL, g = R ? gR : gL;
                 const float *fv = R ? fvR : fvL, *fb = R ? fbR : fbL;
-                const float e = R ? fe2 : fe1, pt = R ? ptR : ptL;
+                const float ei = R ? fei2 : fei1, pt = R ? ptR : ptL;
                 // HLL state relative to this side: dUZ[k] = U_hll - U_side
                 float dUZ[8];
                 const float cS = R ? SL : SR;
@@ -294,17 +305,20 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
                 float dsbv = 0.0f;  // sbv_side - sbvz
 #pragma unroll
                 for (int i = 0; i < 3; i++) dsbv -= fb[i] * dvh[i] + dBh[i] * (fv[i] + dvh[i]);
-                const float dE = (e * dSM + pt * dSM + dptz * SM + bn * dsbv) / g;
+                // e*_side - e_side less |v|^2/2 (rho* - rho)
+                const float dEi = (ei * dSM + pt * dSM + dptz * SM + bn * dsbv) / g;
+                (void)rs;
                 float dUs[8];
                 dUs[0] = dr;
-                dUs[1] = rs * dSM + SM * dr;
-                dUs[2] = fv[1] * dr + dvt1 * rz;
-                dUs[3] = fv[2] * dr + dvt2 * rz;
-                dUs[4] = dE;
+                dUs[1] = rz * dSM;  // (rho v_n)* - rho v_n less v_n dr
+                dUs[2] = dvt1 * rz;
+                dUs[3] = dvt2 * rz;
+                dUs[4] = dEi;
+                split = true;
                 dUs[5] = R ? dbnR : dbnL;
                 dUs[6] = dbt1;
                 dUs[7] = dbt2;
-                const float c = S - wv;
+                const float c = (float)((R ? SRd : SLd) - wd);  // S - w in FP64 (see HLL)
 #pragma unroll
                 for (int k = 0; k < 8; k++) corr[k] = c * dUs[k];
 #pragma unroll
@@ -344,8 +358,9 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
             const float dsbR = -(fbR[0] * dSMR + dbnR * fvR[0] + dbnR * dSMR) -
                                (fbR[1] * dvzR1 + dbzR1 * fvR[1] + dbzR1 * dvzR1) -
                                (fbR[2] * dvzR2 + dbzR2 * fvR[2] + dbzR2 * dvzR2);
-            const float dEL = (fe1 * dSML + ptL * dSML + dptzL * SM + bn * dsbL) / gL;  // e*_L - e_L
-            const float dER = (fe2 * dSMR + ptR * dSMR + dptzR * SM + bn * dsbR) / gR;
+            // e*_side - e_side less |v|^2/2 (rho* - rho)
+            const float dEL = (fei1 * dSML + ptL * dSML + dptzL * SM + bn * dsbL) / gL;
+            const float dER = (fei2 * dSMR + ptR * dSMR + dptzR * SM + bn * dsbR) / gR;
             const float rzRs = sqrtf(rzR);
             const float rzLs = sqrtf(rzL);
             const float rzss = rzRs + rzLs;
@@ -375,19 +390,18 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
                 return;
             }
             if (j >= 2 && j <= 5) {
+                split = true;
                 const bool R = right;
                 const float S = R ? SR : SL, dSM = R ? dSMR : dSML, dr = R ? drR : drL, rz = R ? rzR : rzL;
-                const float rs = R ? fr2 : fr1;
-                const float *fv = R ? fvR : fvL;
                 const float dUs[8] = {dr,
-                                      rs * dSM + SM * dr,
-                                      fv[1] * dr + (R ? dvzR1 : dvzL1) * rz,
-                                      fv[2] * dr + (R ? dvzR2 : dvzL2) * rz,
+                                      rz * dSM,
+                                      (R ? dvzR1 : dvzL1) * rz,
+                                      (R ? dvzR2 : dvzL2) * rz,
                                       R ? dER : dEL,
                                       R ? dbnR : dbnL,
                                       R ? dbzR1 : dbzL1,
                                       R ? dbzR2 : dbzL2};
-                const float c = S - wv;
+                const float c = (float)((R ? SRd : SLd) - wd);  // S - w in FP64 (see HLL)
 #pragma unroll
                 for (int k = 0; k < 8; k++) corr[k] = c * dUs[k];
 #pragma unroll
@@ -416,8 +430,9 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
                     // sbv* - sbv** = -(B* . dv + dB . v* + dB . dv) over the tangential parts
                     const float dsb2 = -(bz1 * d2v1 + d2b1 * vz1 + d2b1 * d2v1) - (bz2 * d2v2 + d2b2 * vz2 + d2b2 * d2v2);
                     const float dE2 = R ? (rzRs * sbn * dsb2) : (-rzLs * sbn * dsb2);
-                    const float SZ = R ? SZR : SZL;
-                    const float c2 = SZ - wv;
+                    // S*_side - w: S_M in FP64 (v_L + (S_M - v_L)) less the FP32 Alfven offset
+                    const double SMd = vL[0] + (double)dSML;
+                    const float c2 = (float)((R ? SMd + (double)(fabsf(bn) / rzRs) : SMd - (double)(fabsf(bn) / rzLs)) - wd);
                     corr[2] += c2 * (d2v1 * rz);
                     corr[3] += c2 * (d2v2 * rz);
                     corr[4] += c2 * dE2;
@@ -432,6 +447,14 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
     // ---- FP64: upwind part, sum, normal-field flux, rotation back --------------------------
 #pragma unroll
     for (int k = 0; k < 8; k++) base[k] = (right ? (fr(k) - wd * ur(k)) : (fl(k) - wd * ul(k))) + (double)corr[k];
+    if (split) {  // momentum and kinetic energy of the mass correction at the side's velocity
+        const double *vs = right ? vR : vL;
+        const double kin0 = (double)corr[0];
+        base[1] += vs[0] * kin0;
+        base[2] += vs[1] * kin0;
+        base[3] += vs[2] * kin0;
+        base[4] += (0.5 * ((vs[0] * vs[0] + vs[1] * vs[1]) + vs[2] * vs[2])) * kin0;
+    }
     double wbn;
     if (wbn_side > 0) wbn = wd * bR[0];
     else if (wbn_side < 0) wbn = wd * bL[0];
