@@ -31,6 +31,8 @@ extern "C" {
 #define TITAN_E_NOT_TRAINED (-6)    /* surrogate requested before parameters were set          */
 #define TITAN_E_IO (-7)             /* file cannot be opened, written, or is truncated / damaged
                                        (Setka_physics.cpp:4110-4113,4213-4216 print and exit) */
+#define TITAN_E_SURFACE (-8)        /* NaN or subnormal contact speed of a heliopause face while
+                                       fitting the surfaces (Setka_move.cpp:336-350 print and exit) */
 
 /* face boundary codes (Type_Gran, Setka_physics.cpp:212-236; ZERO_GRADIENT is an addition) */
 #define TITAN_BC_INTERIOR 0
@@ -66,7 +68,9 @@ typedef struct {
     const int8_t *face_bc;       /* [n_faces] TITAN_BC_*                                       */
     const int8_t *face_method;   /* [n_faces] 0 Lax, 1 HLL, 2 HLLC, 3 HLLD (Gran::Get_method)  */
     const uint8_t *face_flags;   /* [n_faces] bit0 left_ydar (TS_hard), bit1 contact, bit2
-                                    heliopause face (Gran::type2 == HP); nullable              */
+                                    heliopause face (Gran::type2 == HP), bit3 termination-shock
+                                    face (type2 == TS), bit4 bow-shock face (type2 == BS);
+                                    nullable                                                   */
     const double *face_bstate;   /* [n_faces*8] fixed state of hard boundary faces; nullable   */
     const int8_t *cell_zone;     /* [n_cells] 1..4 = Type_cell::Zone_1..Zone_4                 */
     const uint8_t *cell_is_tvd;  /* [n_cells] Cell::is_TVD (0 => B frozen)                     */
@@ -79,7 +83,11 @@ typedef struct {
     double gamma;                /* [gamma]                                                    */
     double kfl;                  /* [KFL]                                                      */
     double v_inf;                /* [Velosity_inf]                                             */
-    int32_t move_mesh;           /* [move_setka]: nodes move by x += dt*omega*cos(omega t+phase)*D_i */
+    int32_t move_mesh;           /* [move_setka]: 0 fixed; 1 prescribed motion
+                                    x += dt*omega*cos(omega t+phase)*D_i (node_disp); 2 fitted
+                                    surfaces: Setka::Culc_Velocity_surface + Luch::dvigenie of
+                                    every ray before each step (Setka_physics.cpp:1616-1627;
+                                    titan_set_rays)                                            */
     int32_t n_disc;              /* wave-speed estimate of chlld (0 on the step path)          */
     int32_t geom_mode;           /* TITAN_GEOM_*                                               */
     int32_t precision;           /* TITAN_PREC_*                                               */
@@ -95,7 +103,34 @@ typedef struct {
     double par_a_2;              /* [par_a_2] sigma = (1 - a_2 ln U)^2 of TITAN_SRC_BERA (0.102046) */
     double par_n_H_LISM;         /* [par_n_H_LISM] (3.0); TITAN_SRC_BERA scales by n_H_LISM/Kn   */
     double par_Kn;               /* [par_Kn] (40.0906)                                          */
+    /* fitted surfaces (Phys_param.h:137-162; defaults of Phys_param.cpp:500-520 in brackets) */
+    int32_t move_TS, move_HP, move_BS;    /* [move_TS], [move_HP], [move_BS] (1, 1, 1)         */
+    int32_t sglag_TS, sglag_HP, sglag_BS; /* smoothing [sglag_TS] (1), [sglag_HP] (1), [sglag_BS]
+                                             (0).  sglag_HP applies the sphere term near the
+                                             nose (Setka_move.cpp:566-622) only                */
+    double velocity_TS, velocity_HP, velocity_BS; /* (0.1, 0.1, 0.3)                            */
+    double sglag_TS_k_sphere, sglag_HP_k_sphere, sglag_BS_k; /* (0.01, 0.01, 0.05)              */
 } titan_params;
+
+/* Rays of node motion: Luch objects of type A_Luch / A2_Luch (Luch.h:9-25), each a straight line
+ * of nodes from the inner sphere R0 to R5 at polar angle `the` from +x and azimuth `phi`
+ * (x = r cos the, y = r sin the cos phi, z = r sin the sin phi).  Node j of a ray: j < M0 on
+ * [R0, R1); j = M0 at R1 (never moved); M1 nodes up to the termination shock at index
+ * M0+M1+1 (Yzels_opor[1]); M11 evenly spaced nodes past it and M2-M11 more up to the heliopause
+ * at M0+M1+M2+2 (Yzels_opor[2]); M3 nodes up to the bow shock at M0+M1+M2+M3+3 (Yzels_opor[3]);
+ * M4 nodes up to R5 at the last index.  nodes_per_ray = M0+M1+M2+M3+M4+5. */
+typedef struct {
+    int64_t n_rays;
+    int32_t nodes_per_ray;
+    const int32_t *ray_nodes;    /* [n_rays*nodes_per_ray] reference node ids, inner to outer   */
+    const double *ray_angles;    /* [n_rays*2] (the, phi) = Luch::parameters["the"], ["phi"]     */
+    double R0, R1, R5;           /* Geo_param::R0, R1, R5                                       */
+    int32_t M0, M1, M11, M2, M3, M4; /* Geo_param::M0 ... M4                                    */
+    double da1, da2, da3, da4, da5;  /* Geo_param::da1 ... da5 (per-ray overrides not taken)     */
+    double phi_a;                /* polar angle from +x of A_Luch[0][1]->Yzels[0]: the HP sphere
+                                    smoothing and Smooth_head_HP3 act where the angle is at most
+                                    phi_a + pi/180 (Setka_move.cpp:566-580, 2048-2064)           */
+} titan_rays;
 
 typedef struct {
     int64_t steps_done;
@@ -227,6 +262,33 @@ int titan_save_cell_parameters(titan_ctx *ctx, const char *path, int n_extra, co
 int titan_download_cell_parameters(titan_ctx *ctx, const char *path);
 /* Setka::Init_TVD (Setka_physics.cpp:648-744) again, from the newest node layer. */
 int titan_init_tvd(titan_ctx *ctx);
+
+/* ---- fitted surfaces ------------------------------------------------------------------------- */
+/* The mesh's rays (the A_Luch / A2_Luch part of Setka::All_Luch): node lists, angles and the
+ * Geo_param counts and radii of Luch::dvigenie.  Each node may lie on one ray only. */
+int titan_set_rays(titan_ctx *ctx, const titan_rays *rays);
+/* Set-up of the surface smoothing from node layer 0 (Yzel::coord[0]): Setka::Smooth_head_TS3
+ * (Setka_move.cpp:2205-2363) gives every TS node with at least 3 points its neighbours (the other
+ * nodes of its TS faces) and the centre of the sphere fitted to it and them; Smooth_head_HP3
+ * (:2046-2203) the same for HP nodes within phi_a + pi/180 of +x (rays required); Setka::
+ * Find_Yzel_Sosed_for_BS (:2365-2397) the neighbours of BS nodes.  titan_go with move_mesh 2,
+ * and titan_culc_velocity_surface, return TITAN_E_ARG while a switched-on smoothing lacks it. */
+int titan_smooth_head_ts3(titan_ctx *ctx);
+int titan_smooth_head_hp3(titan_ctx *ctx);
+int titan_find_yzel_sosed_for_bs(titan_ctx *ctx);
+/* Setka::Culc_Velocity_surface(now, time) (Setka_move.cpp:111-1357, the parts the A_Luch rays
+ * carry): from the state and nodes of layer `now`, chlld wave speeds (S_L at TS, S_M at HP, S_R
+ * at BS faces, w = 0) times velocity_K times the face normal, averaged per node with the
+ * smoothing terms, move the TS/HP/BS nodes into layer 1-now.  node_velocity [n_nodes*3]
+ * (nullable) receives the averaged velocity of every surface node (0 elsewhere).  A node sums its
+ * faces in ascending face id (the reference's order varies between runs).  Not built: the HP
+ * angle and along-ray smoothing and the B/C/D/E/H/G ray families (Setka_move.cpp:697-1130,
+ * 1280-1357), which belong to the reference's own tail mesh. */
+int titan_culc_velocity_surface(titan_ctx *ctx, int now, double time, double *node_velocity);
+/* Luch::dvigenie(layer) of every ray (Luch.cpp:6-152; Setka_physics.cpp:1620-1624). */
+int titan_dvigenie(titan_ctx *ctx, int layer);
+/* The sphere fit of Smooth_head_TS3 on the host: n >= 3 points [n*3] -> centre [3]. */
+int titan_sphere_fit(int64_t n, const double *points, double *centre);
 
 /* ---- mesh construction helpers (host) -------------------------------------------------------- */
 /* Setka::New_connect (Setka.cpp:2066-2249): faces of n_cells hexahedra; face_nodes [6*n_cells*4],

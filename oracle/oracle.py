@@ -4,6 +4,7 @@ Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this
 as the checker or the timed CPU baseline; the product path never does.
 """
 import ctypes
+import math
 import os
 import subprocess
 
@@ -67,7 +68,9 @@ def lib():
             "or_minmod": (_D, [_D, _D]),
             "or_linear": (_D, [_D] * 7),
             "or_update": (_I, [_P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
-            "or_go": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _P, _P, _P, _I, _P]),
+            "or_go": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _P, _P, _P, _I, _P, _P]),
+            "or_culc_velocity_surface": (_I, [_P, _P, _P, _P, _P, _P, _D, _P, _P]),
+            "or_dvigenie": (None, [_P, _P]),
             "or_bera_source": (None, [_P, _I, _P, _D, _D, _D, _P]),
             "or_new_connect": (_I64, [_I64, _P, _I64, _P, _P, _P]),
             "or_cx_rate_ref": (_D, [_D, _D, _D, _I, _I]),
@@ -147,9 +150,10 @@ def face_flux(om, params, prim, geom, x1=None, x2=None, dt=1e-5):
     return rc, flux, dtf, br
 
 
-def go(om, params, xyz, prim, steps, gcl=False, nthreads=1, disp=None, t_sim=0.0, h_fluids=None):
+def go(om, params, xyz, prim, steps, gcl=False, nthreads=1, disp=None, t_sim=0.0, h_fluids=None, fit=None):
     """Runs or_go; returns (rc, prim_final, xyz_final, dt_hist, t_sim).  h_fluids [n_cells, n_h, 5]
-    (rho_H, p_H, V_H per hydrogen fluid) adds the Calc_sourse_MF_Bera sources to every step."""
+    (rho_H, p_H, V_H per hydrogen fluid) adds the Calc_sourse_MF_Bera sources to every step; fit
+    (OracleFit) with move_mesh 2 moves the nodes by the fitted surfaces and their rays."""
     m = om.m
     coords = np.zeros((2, m.n_nodes, 3))
     coords[0] = xyz
@@ -162,7 +166,7 @@ def go(om, params, xyz, prim, steps, gcl=False, nthreads=1, disp=None, t_sim=0.0
     hf = None if h_fluids is None else np.ascontiguousarray(h_fluids, np.float64).reshape(m.n_cells, -1, 5)
     rc = lib().or_go(ctypes.byref(om.c), ctypes.byref(params_c(params)), _p(coords), _p(state), _p(d), int(steps),
                      1 if gcl else 0, int(nthreads), ctypes.byref(layer), _p(dth), ctypes.byref(ts),
-                     0 if hf is None else hf.shape[1], _p(hf))
+                     0 if hf is None else hf.shape[1], _p(hf), None if fit is None else ctypes.byref(fit.c(params)))
     return rc, state[layer.value].copy(), coords[layer.value].copy(), dth[:steps], ts.value
 
 
@@ -309,3 +313,190 @@ def bera_source(prim, h_fluids, a2=0.102046, n_h_lism=3.0, kn=40.0906):
     for c in range(n):
         L.or_bera_source(_p(prim[c]), hf.shape[1], _p(hf[c]), a2, n_h_lism, kn, _p(out[c]))
     return out
+
+
+# ---- fitted surfaces ---------------------------------------------------------------------------
+class OrFit(ctypes.Structure):
+    _fields_ = [("move", ctypes.c_int32 * 4), ("sglag", ctypes.c_int32 * 4), ("velocity", _D * 4),
+                ("k_smooth", _D * 4), ("phi_lim", _D), ("n_sn", _I64), ("sn_node", _P), ("sn_kind", _P),
+                ("sn_face_off", _P), ("sn_face", _P), ("sn_nb_off", _P), ("sn_nb", _P), ("sn_centre", _P),
+                ("n_rays", _I64), ("npr", ctypes.c_int32), ("ray_nodes", _P), ("ray_angles", _P),
+                ("R0", _D), ("R1", _D), ("R5", _D), ("da1", _D), ("da2", _D), ("da3", _D), ("da4", _D), ("da5", _D),
+                ("M0", ctypes.c_int32), ("M1", ctypes.c_int32), ("M11", ctypes.c_int32), ("M2", ctypes.c_int32),
+                ("M3", ctypes.c_int32), ("M4", ctypes.c_int32)]
+
+
+# Phys_param.cpp:500-520
+FIT_DEFAULTS = dict(move_TS=1, move_HP=1, move_BS=1, sglag_TS=1, sglag_HP=1, sglag_BS=0, velocity_TS=0.1,
+                    velocity_HP=0.1, velocity_BS=0.3, sglag_TS_k_sphere=0.01, sglag_HP_k_sphere=0.01, sglag_BS_k=0.05)
+
+
+def polar_angle(x, y):
+    """Help_funk.cpp:78-107."""
+    if abs(x) + abs(y) < 0.000001:
+        return 0.0
+    if x < 0:
+        return math.atan(y / x) + math.pi
+    if x > 0 and y >= 0:
+        return math.atan(y / x)
+    if x > 0 and y < 0:
+        return math.atan(y / x) + 2.0 * math.pi
+    if y > 0:
+        return math.pi / 2.0
+    if y < 0:
+        return 3.0 * math.pi / 2.0
+    return 0.0
+
+
+def sphere_fit(points):
+    """Smooth_head_TS3's centre (Setka_move.cpp:2257-2355): least squares of 2 p.c = |p|^2 (minimum
+    norm, singular values below 3 eps of the largest dropped, as Eigen's SVD solve does), then up
+    to 8 passes reweighted by Tukey biweights of |dist - mean dist| with cutoff 2.5 standard
+    deviations, stopping when the rms residual changes by less than 1e-6."""
+    P = np.asarray(points, np.float64).reshape(-1, 3)
+    n = P.shape[0]
+    A = 2.0 * P
+    b = P[:, 0] * P[:, 0] + P[:, 1] * P[:, 1] + P[:, 2] * P[:, 2]
+    rc = 3.0 * np.finfo(np.float64).eps
+    c = np.linalg.lstsq(A, b, rcond=rc)[0]
+    prev = np.finfo(np.float64).max
+    for _ in range(8):
+        d = np.linalg.norm(P - c, axis=1)
+        radius = d.mean()
+        res = np.abs(d - radius)
+        rms = math.sqrt((res * res).sum() / n)
+        if abs(prev - rms) < 1e-6:
+            break
+        prev = rms
+        mean = res.mean()
+        cutoff = 2.5 * math.sqrt(((res - mean) ** 2).sum() / n)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t = res / cutoff
+            w = np.where(res < cutoff, (1.0 - t * t) * (1.0 - t * t), 0.0)
+        c = np.linalg.lstsq(w[:, None] * A, w * b, rcond=rc)[0]
+    return c
+
+
+class OracleFit:
+    """The fitted-surface set-up of a mesh in reference numbering, derived here from the face flags
+    (bit3 TS, bit2 HP, bit4 BS): surface faces; HP faces with fewer than 4 surface neighbours
+    (macros4, Setka.cpp:58-73: the same-surface faces of the cells beside cells[0]) move nothing;
+    each surface node's faces in ascending id; its smoothing neighbours and sphere centre from
+    node layer 0 (smooth(kind, x0): Smooth_head_TS3 / Smooth_head_HP3 / Find_Yzel_Sosed_for_BS);
+    the rays (Luch::dvigenie data)."""
+
+    def __init__(self, mesh, rays):
+        fl = np.zeros(mesh.n_faces, np.uint8) if mesh.face_flags is None else np.asarray(mesh.face_flags, np.uint8)
+        kind = np.where((fl >> 3) & 1, 1, np.where((fl >> 2) & 1, 2, np.where((fl >> 4) & 1, 3, 0))).astype(np.uint8)
+        fn = np.asarray(mesh.face_nodes)
+        fc = np.asarray(mesh.face_cells)
+        kind[fc[:, 1] < 0] = 0  # only faces between two cells are fitted surfaces
+        cf = np.asarray(mesh.cell_faces)
+        sf = np.nonzero(kind)[0]
+        moves = np.ones(mesh.n_faces, bool)
+        for f in sf[kind[sf] == 2]:
+            A = fc[f, 0]
+            cnt = 0
+            for g in cf[A]:
+                if g == f:
+                    continue
+                B = fc[g, 1] if fc[g, 0] == A else fc[g, 0]
+                if B < 0:
+                    continue
+                cnt += int(sum(1 for h in cf[B] if kind[h] == 2 and h != f))
+            moves[f] = cnt >= 4
+        nkind = np.zeros(mesh.n_nodes, np.uint8)
+        faces_of = {}
+        cand = {}
+        for f in sf:  # ascending face id
+            for v in fn[f]:
+                nkind[v] = kind[f]
+                if moves[f]:
+                    faces_of.setdefault(int(v), []).append(int(f))
+                cand.setdefault(int(v), set()).update(int(u) for u in fn[f])
+        self.kind = kind
+        self.sn = np.nonzero(nkind)[0].astype(np.int32)
+        self.sn_kind = nkind[self.sn].astype(np.uint8)
+        off = [0]
+        lst = []
+        for v in self.sn:
+            lst += faces_of.get(int(v), [])
+            off.append(len(lst))
+        self.sn_face_off = np.asarray(off, np.int32)
+        self.sn_face = np.asarray(lst, np.int32)
+        self.cand = [sorted(cand[int(v)] | {int(v)}) for v in self.sn]
+        self.nb = [[] for _ in self.sn]
+        self.centre = np.zeros((len(self.sn), 3))
+        self.rays = dict(rays)
+        self.rays["ray_nodes"] = np.ascontiguousarray(rays["ray_nodes"], np.int32)
+        self.rays["ray_angles"] = np.ascontiguousarray(rays["ray_angles"], np.float64)
+        self.phi_lim = float(rays["phi_a"]) + math.pi / 180.0
+        self._keep = None
+
+    def smooth(self, kind, x0):
+        x0 = np.asarray(x0, np.float64)
+        for i, v in enumerate(self.sn):
+            if self.sn_kind[i] != kind:
+                continue
+            if kind == 2:
+                a = x0[v]
+                if polar_angle(a[0], math.sqrt(0.0 * 0.0 + a[1] * a[1] + a[2] * a[2])) > self.phi_lim:
+                    continue
+            pts = self.cand[i]
+            if len(pts) < 3:
+                continue
+            self.nb[i] = [u for u in pts if u != v]
+            if kind != 3:
+                self.centre[i] = sphere_fit(x0[pts])
+
+    def c(self, params):
+        q = dict(FIT_DEFAULTS)
+        q.update({k: params[k] for k in FIT_DEFAULTS if k in params})
+        nb_off = np.zeros(len(self.sn) + 1, np.int32)
+        nb_off[1:] = np.cumsum([len(x) for x in self.nb])
+        nb = np.asarray([u for x in self.nb for u in x], np.int32)
+        centre = np.ascontiguousarray(self.centre, np.float64)
+        r = self.rays
+        self._keep = (nb_off, nb, centre)
+        f = OrFit()
+        for k, (a, b, c) in enumerate([("move_TS", "sglag_TS", "TS"), ("move_HP", "sglag_HP", "HP"),
+                                       ("move_BS", "sglag_BS", "BS")], start=1):
+            f.move[k] = int(q[a])
+            f.sglag[k] = int(q[b])
+            f.velocity[k] = float(q["velocity_" + c])
+        f.k_smooth[1] = q["sglag_TS_k_sphere"]
+        f.k_smooth[2] = q["sglag_HP_k_sphere"]
+        f.k_smooth[3] = q["sglag_BS_k"]
+        f.phi_lim = self.phi_lim
+        f.n_sn = len(self.sn)
+        f.sn_node, f.sn_kind = _p(self.sn), _p(self.sn_kind)
+        f.sn_face_off, f.sn_face = _p(self.sn_face_off), _p(self.sn_face)
+        f.sn_nb_off, f.sn_nb, f.sn_centre = _p(nb_off), _p(nb) if nb.size else None, _p(centre)
+        f.n_rays = r["ray_nodes"].shape[0]
+        f.npr = r["ray_nodes"].shape[1]
+        f.ray_nodes, f.ray_angles = _p(r["ray_nodes"]), _p(r["ray_angles"])
+        for k in ("R0", "R1", "R5", "da1", "da2", "da3", "da4", "da5", "M0", "M1", "M11", "M2", "M3", "M4"):
+            setattr(f, k, r[k])
+        return f
+
+
+def culc_velocity_surface(om, params, fit, x, prim, time, gcl=False):
+    """or_culc_velocity_surface on layer coordinates x; returns (rc, x_next, node_velocity [n_sn, 3])
+    with x_next a copy of x whose surface nodes have moved."""
+    x = np.ascontiguousarray(x, np.float64)
+    g = measure(om, x, gcl)
+    G = OrGeom(*[_p(g[k]) for k, _ in OrGeom._fields_])
+    xn = x.copy()
+    vel = np.zeros((len(fit.sn), 3))
+    fc = fit.c(params)
+    rc = lib().or_culc_velocity_surface(ctypes.byref(om.c), ctypes.byref(params_c(params)), ctypes.byref(fc), _p(x),
+                                        _p(np.ascontiguousarray(prim, np.float64)), ctypes.byref(G), float(time),
+                                        _p(xn), _p(vel))
+    return rc, xn, vel
+
+
+def dvigenie(fit, x, params=None):
+    """or_dvigenie on a copy of x."""
+    xn = np.ascontiguousarray(x, np.float64).copy()
+    lib().or_dvigenie(ctypes.byref(fit.c(params or {})), _p(xn))
+    return xn

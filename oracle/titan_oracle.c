@@ -1325,9 +1325,179 @@ void or_bera_source(const double *s, int n_h, const double *h, double a2, double
     out[4] = E * ddp;
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* Setka::Culc_Velocity_surface (Setka_move.cpp:111-1357).  Face loops (:126-182 TS, :186-360 HP,
+ * :364-433 BS): chlld with w = 0 on the face states of layer `now`; each of the face's nodes gets
+ * velocity_K * ds * n and one count.  Smoothing (:436-506 TS sphere, :566-622 HP sphere near the
+ * nose, :1136-1210 BS neighbour mean) adds one more vector and count.  Average, then the move
+ * along the radius (:1228-1278; HP with x < 0 along the radius in the yz plane). */
+static int fit_kind(uint8_t fl) { return ((fl >> 3) & 1) ? 1 : (((fl >> 2) & 1) ? 2 : (((fl >> 4) & 1) ? 3 : 0)); }
+
+int or_culc_velocity_surface(const or_mesh *m, const or_params *p, const or_fit *fit, const double *x,
+                             const double *prim, const or_geom *g, double time, double *x_next,
+                             double *node_velocity)
+{
+    int status = 0;
+    double *sp = (double *)calloc((size_t)(3 * m->n_faces), sizeof(double));
+    for (int64_t f = 0; f < m->n_faces; f++) {
+        int kind = m->face_flags && m->face_cells[2 * f + 1] >= 0 ? fit_kind(m->face_flags[f]) : 0;
+        if (!kind || !fit->move[kind]) continue;
+        double qL[8], qR[8], q[8], dsl, dsc, dsr;
+        int br;
+        if (or_face_states(m, p, f, prim, g, qL, qR)) status = -1;
+        const double *n = g->face_normal + 3 * f;
+        if (or_chlld(m->face_method[f], 0, 0, n[0], n[1], n[2], 0.0, qL, qR, 0, NULL, NULL, q, NULL, &dsl, &dsc,
+                     &dsr, &br))
+            status = -1;
+        double ds = kind == 1 ? dsl : (kind == 2 ? dsc : dsr);
+        if (kind == 2 && (isnan(ds) || fpclassify(ds) == FP_SUBNORMAL) && !status) status = -8;
+        for (int k = 0; k < 3; k++) sp[3 * f + k] = fit->velocity[kind] * ds * n[k];
+    }
+    for (int64_t i = 0; i < fit->n_sn; i++) {
+        int kind = fit->sn_kind[i];
+        int64_t v = fit->sn_node[i];
+        const double *A = x + 3 * v;
+        double vel[3] = {0.0, 0.0, 0.0};
+        int num = 0;
+        if (fit->move[kind]) {
+            for (int32_t j = fit->sn_face_off[i]; j < fit->sn_face_off[i + 1]; j++) {
+                const double *s = sp + 3 * (int64_t)fit->sn_face[j];
+                vel[0] += s[0];
+                vel[1] += s[1];
+                vel[2] += s[2];
+                num++;
+            }
+        }
+        int32_t nb0 = fit->sn_nb_off[i], nb1 = fit->sn_nb_off[i + 1];
+        int smooth = fit->sglag[kind] != 0;
+        if (smooth && kind == 2) {
+            double phi = polar_angle(A[0], sqrt(KVV(0.0, A[1], A[2])));
+            if (phi > fit->phi_lim) smooth = 0;
+        }
+        if (smooth && kind == 3 && nb1 - nb0 <= 5) smooth = 0;
+        if (smooth && kind != 3 && nb1 == nb0) smooth = 0; /* no neighbours from the set-up */
+        if (smooth) {
+            double Bv[3], V[3];
+            double c = fit->velocity[kind] * fit->k_smooth[kind];
+            if (kind == 3) {
+                for (int k = 0; k < 3; k++) Bv[k] = A[k];
+                for (int32_t j = nb0; j < nb1; j++)
+                    for (int k = 0; k < 3; k++) Bv[k] += x[3 * (int64_t)fit->sn_nb[j] + k];
+                for (int k = 0; k < 3; k++) Bv[k] /= (nb1 - nb0 + 1.0);
+            } else {
+                const double *C = fit->sn_centre + 3 * i;
+                double d[3] = {A[0] - C[0], A[1] - C[1], A[2] - C[2]};
+                double rr = or_norm3(d);
+                double r = 0.0;
+                for (int32_t j = nb0; j < nb1; j++) {
+                    const double *P = x + 3 * (int64_t)fit->sn_nb[j];
+                    double e[3] = {P[0] - C[0], P[1] - C[1], P[2] - C[2]};
+                    r += or_norm3(e);
+                }
+                r /= (double)(nb1 - nb0);
+                double kk = r / rr;
+                for (int k = 0; k < 3; k++) Bv[k] = kk * (A[k] - C[k]) + C[k];
+            }
+            for (int k = 0; k < 3; k++) V[k] = c * (Bv[k] - A[k]) / time;
+            for (int k = 0; k < 3; k++) vel[k] += V[k];
+            num++;
+        }
+        if (num > 0)
+            for (int k = 0; k < 3; k++) vel[k] /= num;
+        if (node_velocity)
+            for (int k = 0; k < 3; k++) node_velocity[3 * i + k] = vel[k];
+        if (fit->move[kind]) {
+            double Bd[3] = {A[0], A[1], A[2]};
+            if (kind == 2 && !(A[0] >= 0.0)) Bd[0] = 0.0;
+            double z = (Bd[0] * Bd[0] + Bd[1] * Bd[1]) + Bd[2] * Bd[2]; /* squaredNorm */
+            if (z > 0.0) {
+                double sz = sqrt(z);
+                for (int k = 0; k < 3; k++) Bd[k] /= sz;
+            }
+            double sdot = ((vel[0] * Bd[0] + vel[1] * Bd[1]) + vel[2] * Bd[2]) * time;
+            for (int k = 0; k < 3; k++) x_next[3 * v + k] = A[k] + sdot * Bd[k];
+        }
+    }
+    free(sp);
+    return status;
+}
+
+/* Luch::dvigenie for A_Luch / A2_Luch (Luch.cpp:8-151): the radii of a ray's nodes from R0, R1,
+ * its TS, HP and BS nodes' distances from the origin in x (func_R, Yzel.cpp:27-30) and R5. */
+void or_dvigenie(const or_fit *fit, double *x)
+{
+    const int32_t M0 = fit->M0, M1 = fit->M1, M11 = fit->M11, M2 = fit->M2, M3 = fit->M3, M4 = fit->M4;
+    for (int64_t k = 0; k < fit->n_rays; k++) {
+        const int32_t *Y = fit->ray_nodes + k * fit->npr;
+        double the = fit->ray_angles[2 * k], phi = fit->ray_angles[2 * k + 1];
+        double R0 = fit->R0, R1 = fit->R1, R5 = fit->R5;
+        const double *o1 = x + 3 * (int64_t)Y[M0 + M1 + 1];
+        const double *o2 = x + 3 * (int64_t)Y[M0 + M1 + M2 + 2];
+        const double *o3 = x + 3 * (int64_t)Y[M0 + M1 + M2 + M3 + 3];
+        double R2 = sqrt(KVV(o1[0], o1[1], o1[2]));
+        double R3 = sqrt(KVV(o2[0], o2[1], o2[2]));
+        double R4 = sqrt(KVV(o3[0], o3[1], o3[2]));
+        double r, a, b;
+        int num = 0;
+#define OR_PUT(idx, rad)                                               \
+    do {                                                               \
+        double *X_ = x + 3 * (int64_t)Y[idx];                          \
+        X_[0] = (rad) * cos(the);                                      \
+        X_[1] = (rad) * sin(the) * cos(phi);                           \
+        X_[2] = (rad) * sin(the) * sin(phi);                           \
+    } while (0)
+        for (int j = 0; j < M0; j++) {
+            r = R0 + j * (R1 - R0) / M0;
+            OR_PUT(j, r);
+        }
+        num += M0 + 1;
+        for (int j = 0; j < M1; j++) {
+            r = R1 + (j + 1) * (R2 - R1) / (M1 + 1);
+            OR_PUT(num + j, r);
+        }
+        num += M1 + 1;
+        double dr = (R2 - R1) / (M1 + 1);
+        for (int j = 0; j < M11; j++) {
+            r = R2 + dr * (j + 1);
+            OR_PUT(num + j, r);
+        }
+        num += M11;
+        a = fit->da1;
+        b = fit->da2;
+        for (int j = 0; j < M2 - M11; j++) {
+            double s = (j + 1.0) / (M2 - M11 + 1.0);
+            double ss = s * (a * ((1.0 - 3.0 * s + 2.0 * (s * s)) * (1.0 - 3.0 * s + 2.0 * (s * s))) +
+                             s * (7.0 + b * ((1.0 - 2.0 * s) * (1.0 - 2.0 * s)) * (-1.0 + s) -
+                                  2.0 * s * (9.0 + 2.0 * s * (-5.0 + 2.0 * s))));
+            r = R2 + dr * M11 + (R3 - R2 - dr * M11) * ss;
+            OR_PUT(num + j, r);
+        }
+        num += (M2 - M11) + 1;
+        a = fit->da3;
+        b = fit->da4;
+        for (int j = 0; j < M3; j++) {
+            double s = (j + 1.0) / (M3 + 1.0);
+            double ss = s * (a - (2.0 * a + b - 3.0) * s + (a + b - 2.0) * s * s);
+            r = R3 + (R4 - R3) * ss;
+            OR_PUT(num + j, r);
+        }
+        num += M3 + 1;
+        a = fit->da5;
+        b = 2.0;
+        for (int j = 0; j < M4; j++) {
+            double s = (j + 1.0) / (M4 + 1.0);
+            double ss = a * s - (3.0 * a + b - 4.0) * s * s * s + (2.0 * a + b - 3.0) * s * s * s * s;
+            r = R4 + (R5 - R4) * ss;
+            OR_PUT(num + j, r);
+        }
+        OR_PUT(num + M4, R5);
+#undef OR_PUT
+    }
+}
+
 int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, const double *disp,
           int64_t steps, int gcl, int nthreads, int *final_layer, double *dt_hist, double *t_sim,
-          int n_h, const double *h_fluids)
+          int n_h, const double *h_fluids, const or_fit *fit)
 {
     int64_t nn3 = 3 * m->n_nodes, nc8 = 8 * m->n_cells;
     /* charge-exchange sources of the step's old layer (Setka_physics.cpp:1741), when hydrogen
@@ -1354,7 +1524,14 @@ int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, co
         time = loc_time;
         loc_time = 100000000.0;
         if (dt_hist) dt_hist[step - 1] = time;
-        if (p->move_mesh) {
+        if (p->move_mesh == 2 && fit) {
+            /* Setka_physics.cpp:1616-1627: surfaces, then every ray, then the new geometry */
+            int rc0 = or_culc_velocity_surface(m, p, fit, X[now1], S[now1], &G[now1], time, X[now2], NULL);
+            if (rc0 && !status) status = rc0;
+            or_dvigenie(fit, X[now2]);
+            if (gcl) or_measure_gcl(m, X[now2], &G[now2]);
+            else or_measure(m, X[now2], &G[now2]);
+        } else if (p->move_mesh) {
             or_motion(m->n_nodes, X[now1], disp, time, tsim, p->motion_omega, p->motion_phase, X[now2]);
             if (gcl) or_measure_gcl(m, X[now2], &G[now2]);
             else or_measure(m, X[now2], &G[now2]);

@@ -169,13 +169,47 @@ int or_update(const or_mesh *m, const or_params *p, const double *prim_old, cons
               const double *vol_new, const double *flux, const double *src, double dt,
               double *prim_new, int64_t *n_rho_floor, int64_t *n_p_floor);
 
+/* ---- fitted surfaces: Setka::Culc_Velocity_surface (Setka_move.cpp:111-1357) and Luch::dvigenie
+ * of the A_Luch / A2_Luch rays (Luch.cpp:6-152).  Surface faces: face_flags bit3 TS, bit2 HP,
+ * bit4 BS; kinds 1 TS, 2 HP, 3 BS (index 0 unused).  The per-node lists are set-up data (the
+ * surface neighbours and sphere centres of Smooth_head_TS3 / Smooth_head_HP3 /
+ * Find_Yzel_Sosed_for_BS), in reference ids. */
+typedef struct {
+    int32_t move[4], sglag[4];
+    double velocity[4], k_smooth[4];
+    double phi_lim;              /* phi_a + pi/180 */
+    int64_t n_sn;                /* surface nodes, ascending id */
+    const int32_t *sn_node;
+    const uint8_t *sn_kind;
+    const int32_t *sn_face_off;  /* [n_sn+1] faces that move the node, ascending face id */
+    const int32_t *sn_face;
+    const int32_t *sn_nb_off;    /* [n_sn+1] smoothing neighbours, ascending node id */
+    const int32_t *sn_nb;
+    const double *sn_centre;     /* [n_sn*3] */
+    int64_t n_rays;
+    int32_t npr;
+    const int32_t *ray_nodes;    /* [n_rays*npr] */
+    const double *ray_angles;    /* [n_rays*2] the, phi */
+    double R0, R1, R5, da1, da2, da3, da4, da5;
+    int32_t M0, M1, M11, M2, M3, M4;
+} or_fit;
+/* Surface velocities from layer `now` (x, prim, geometry g of that layer) moved over `time` into
+ * x_next (surface nodes only); node_velocity [n_sn*3] (nullable). Returns -1 on a chlld failure,
+ * -8 on a NaN/subnormal HP contact speed. */
+int or_culc_velocity_surface(const or_mesh *m, const or_params *p, const or_fit *fit, const double *x,
+                             const double *prim, const or_geom *g, double time, double *x_next,
+                             double *node_velocity);
+/* Every ray's nodes other than index M0 and its TS/HP/BS nodes, in x. */
+void or_dvigenie(const or_fit *fit, double *x);
+
 /* Full explicit loop of Setka::Go.  coords[2][n_nodes*3], prim[2][n_cells*8] hold both time
  * layers and are updated in place; disp [n_nodes*3] is the motion field (may be NULL when not
- * moving).  gcl selects the exactly-conservative geometry.  On return *final_layer says which
+ * moving); with fit (nullable) and move_mesh 2 the nodes move by or_culc_velocity_surface and
+ * or_dvigenie instead.  gcl selects the exactly-conservative geometry.  On return *final_layer says which
  * layer holds the newest state; dt_hist (optional, [steps]) receives the dt used at each step. */
 int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, const double *disp,
           int64_t steps, int gcl, int nthreads, int *final_layer, double *dt_hist,
-          double *t_sim, int n_h, const double *h_fluids);
+          double *t_sim, int n_h, const double *h_fluids, const or_fit *fit);
 /* ---- Setka::Calc_sourse_MF_Bera (Setka_physics.cpp:766-1258), plasma part: state s[8], n_h
  * hydrogen fluids h[5k..5k+4] = (rho_Hk, p_Hk, V_Hk); out[5] = (rho, m_x, m_y, m_z, E) sources. */
 void or_bera_source(const double *s, int n_h, const double *h, double a2, double n_h_lism, double kn,

@@ -23,6 +23,7 @@ TITAN_E_NONFINITE_STATE = -4
 TITAN_E_TOPOLOGY = -5
 TITAN_E_NOT_TRAINED = -6
 TITAN_E_IO = -7
+TITAN_E_SURFACE = -8
 BC_INTERIOR, BC_INNER_HARD, BC_OUTER_HARD, BC_OUTER_SOFT, BC_ZERO_GRADIENT = 0, 1, 2, 3, 4
 GEOM_REF, GEOM_GCL = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
@@ -31,7 +32,7 @@ SRC_NONE, SRC_EXACT, SRC_MLP, SRC_BERA = 0, 1, 2, 3
 _ERRNAMES = {
     TITAN_E_ARG: "TITAN_E_ARG", TITAN_E_HIP: "TITAN_E_HIP", TITAN_E_RIEMANN: "TITAN_E_RIEMANN",
     TITAN_E_NONFINITE_STATE: "TITAN_E_NONFINITE_STATE", TITAN_E_TOPOLOGY: "TITAN_E_TOPOLOGY",
-    TITAN_E_NOT_TRAINED: "TITAN_E_NOT_TRAINED", TITAN_E_IO: "TITAN_E_IO",
+    TITAN_E_NOT_TRAINED: "TITAN_E_NOT_TRAINED", TITAN_E_IO: "TITAN_E_IO", TITAN_E_SURFACE: "TITAN_E_SURFACE",
 }
 
 
@@ -56,10 +57,28 @@ class ParamsC(ctypes.Structure):
                 ("precision", ctypes.c_int32), ("source_mode", ctypes.c_int32), ("cx_nodes", ctypes.c_int32),
                 ("motion_omega", ctypes.c_double), ("motion_phase", ctypes.c_double),
                 ("tvd", ctypes.c_int32), ("snos_on_hp", ctypes.c_int32),
-                ("par_a_2", ctypes.c_double), ("par_n_H_LISM", ctypes.c_double), ("par_Kn", ctypes.c_double)]
+                ("par_a_2", ctypes.c_double), ("par_n_H_LISM", ctypes.c_double), ("par_Kn", ctypes.c_double),
+                ("move_TS", ctypes.c_int32), ("move_HP", ctypes.c_int32), ("move_BS", ctypes.c_int32),
+                ("sglag_TS", ctypes.c_int32), ("sglag_HP", ctypes.c_int32), ("sglag_BS", ctypes.c_int32),
+                ("velocity_TS", ctypes.c_double), ("velocity_HP", ctypes.c_double), ("velocity_BS", ctypes.c_double),
+                ("sglag_TS_k_sphere", ctypes.c_double), ("sglag_HP_k_sphere", ctypes.c_double),
+                ("sglag_BS_k", ctypes.c_double)]
 
-# parameters an older params dict may lack (Phys_param.h:99-101 defaults for the last three)
-_PARAM_DEFAULTS = {"tvd": 0, "snos_on_hp": 0, "par_a_2": 0.102046, "par_n_H_LISM": 3.0, "par_Kn": 40.0906}
+# parameters a params dict may lack: Phys_param.h:99-101 for par_*, Phys_param.cpp:500-520 for the
+# fitted surfaces
+_PARAM_DEFAULTS = {"tvd": 0, "snos_on_hp": 0, "par_a_2": 0.102046, "par_n_H_LISM": 3.0, "par_Kn": 40.0906,
+                   "move_TS": 1, "move_HP": 1, "move_BS": 1, "sglag_TS": 1, "sglag_HP": 1, "sglag_BS": 0,
+                   "velocity_TS": 0.1, "velocity_HP": 0.1, "velocity_BS": 0.3, "sglag_TS_k_sphere": 0.01,
+                   "sglag_HP_k_sphere": 0.01, "sglag_BS_k": 0.05}
+
+
+class RaysC(ctypes.Structure):
+    _fields_ = [("n_rays", ctypes.c_int64), ("nodes_per_ray", ctypes.c_int32), ("ray_nodes", ctypes.c_void_p),
+                ("ray_angles", ctypes.c_void_p), ("R0", ctypes.c_double), ("R1", ctypes.c_double),
+                ("R5", ctypes.c_double), ("M0", ctypes.c_int32), ("M1", ctypes.c_int32), ("M11", ctypes.c_int32),
+                ("M2", ctypes.c_int32), ("M3", ctypes.c_int32), ("M4", ctypes.c_int32), ("da1", ctypes.c_double),
+                ("da2", ctypes.c_double), ("da3", ctypes.c_double), ("da4", ctypes.c_double), ("da5", ctypes.c_double),
+                ("phi_a", ctypes.c_double)]
 
 
 class StatsC(ctypes.Structure):
@@ -105,6 +124,13 @@ _SIGS = {
     "titan_save_cell_parameters": (_I, [_P, ctypes.c_char_p, _I, _P, _P]),
     "titan_download_cell_parameters": (_I, [_P, ctypes.c_char_p]),
     "titan_init_tvd": (_I, [_P]),
+    "titan_set_rays": (_I, [_P, ctypes.POINTER(RaysC)]),
+    "titan_smooth_head_ts3": (_I, [_P]),
+    "titan_smooth_head_hp3": (_I, [_P]),
+    "titan_find_yzel_sosed_for_bs": (_I, [_P]),
+    "titan_culc_velocity_surface": (_I, [_P, _I, _D, _P]),
+    "titan_dvigenie": (_I, [_P, _I]),
+    "titan_sphere_fit": (_I, [_I64, _P, _P]),
     "titan_mesh_connect": (_I64, [_I64, _P, _I64, _P, _P, _P]),
     "titan_mesh_cube": (_I, [ctypes.c_int32, _D, ctypes.c_uint64, _P, _P]),
     "titan_mesh_shell": (_I, [ctypes.c_int32, ctypes.c_int32, _P, _P, _P]),
@@ -246,6 +272,14 @@ def cx_exact_batch(in10, nodes=100):
 
 
 # ---- context ---------------------------------------------------------------------------------
+def sphere_fit(points):
+    """Smooth_head_TS3's sphere centre of n >= 3 points [n, 3] (host routine of the library)."""
+    P = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    c = np.zeros(3)
+    _check(lib().titan_sphere_fit(P.shape[0], _ptr(P), _ptr(c)), "titan_sphere_fit")
+    return c
+
+
 class Setka:
     """One device context: mesh, both time layers of state and geometry, step scalars."""
 
@@ -375,6 +409,40 @@ class Setka:
 
     def Init_TVD(self):
         _check(lib().titan_init_tvd(self.ctx), "titan_init_tvd", self.ctx)
+
+    # ---- fitted surfaces (Setka_move.cpp, Luch.cpp) ----
+    def set_rays(self, rays):
+        """rays: dict with ray_nodes [n_rays, nodes_per_ray] (reference node ids, inner to outer),
+        ray_angles [n_rays, 2] (the, phi), R0, R1, R5, M0, M1, M11, M2, M3, M4, da1..da5, phi_a."""
+        nodes = np.ascontiguousarray(rays["ray_nodes"], np.int32)
+        ang = np.ascontiguousarray(rays["ray_angles"], np.float64)
+        r = RaysC()
+        r.n_rays, r.nodes_per_ray = nodes.shape[0], nodes.shape[1]
+        r.ray_nodes, r.ray_angles = _ptr(nodes), _ptr(ang)
+        for k in ("R0", "R1", "R5", "M0", "M1", "M11", "M2", "M3", "M4", "da1", "da2", "da3", "da4", "da5", "phi_a"):
+            setattr(r, k, rays[k])
+        _check(lib().titan_set_rays(self.ctx, ctypes.byref(r)), "titan_set_rays", self.ctx)
+
+    def Smooth_head_TS3(self):
+        _check(lib().titan_smooth_head_ts3(self.ctx), "titan_smooth_head_ts3", self.ctx)
+
+    def Smooth_head_HP3(self):
+        _check(lib().titan_smooth_head_hp3(self.ctx), "titan_smooth_head_hp3", self.ctx)
+
+    def Find_Yzel_Sosed_for_BS(self):
+        _check(lib().titan_find_yzel_sosed_for_bs(self.ctx), "titan_find_yzel_sosed_for_bs", self.ctx)
+
+    def Culc_Velocity_surface(self, now, time):
+        """Moves the TS/HP/BS nodes of layer `now` into layer 1-now; returns the averaged velocity
+        of every node [n_nodes, 3] (0 off the surfaces)."""
+        v = np.zeros((self.n_nodes, 3))
+        _check(lib().titan_culc_velocity_surface(self.ctx, int(now), float(time), _ptr(v)),
+               "titan_culc_velocity_surface", self.ctx)
+        return v
+
+    def dvigenie(self, layer):
+        """Luch::dvigenie(layer) of every ray."""
+        _check(lib().titan_dvigenie(self.ctx, int(layer)), "titan_dvigenie", self.ctx)
 
     def tvd_cells(self):
         """Setka::Init_TVD neighbours [n_faces, 2] (AA behind cells[0], BB behind cells[1], -1 = none)."""

@@ -3,7 +3,7 @@
 Usage: python titan_c-_amd/build.py [--jobs N]
 
 Flags per translation unit:
-  * FP64 parity kernels (k_step64, k_cx): -ffp-contract=off, IEEE division / square root, so the
+  * FP64 parity kernels (k_step64, k_cx, k_fit): -ffp-contract=off, IEEE division / square root, so the
     arithmetic is evaluated in the reference's operation order;
   * mixed-precision flux (k_step32): contracted FP32 with fast division / square root;
   * every HIP unit: packed-FP32 instructions disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
@@ -33,10 +33,11 @@ HIP_COMMON = [
 UNITS = {
     "k_step64.hip": ["-ffp-contract=off"],
     "k_cx.hip": ["-ffp-contract=off"],
+    "k_fit.hip": ["-ffp-contract=off"],
     "k_step32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "k_mlp.hip": ["-ffp-contract=fast"],
 }
-HOST_UNITS = ["titan_host.cpp", "titan_mesh.cpp", "titan_io.cpp"]
+HOST_UNITS = ["titan_host.cpp", "titan_mesh.cpp", "titan_io.cpp", "titan_fit.cpp"]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
 
 

@@ -224,6 +224,134 @@ def shell(n_edge=37, n_layers=120, moving=False, seed_motion=3, r_zone1_au=80.0,
     return mesh, prim, params
 
 
+# ---------------------------------------------------------------------------------------------
+def a_luch_radii(R0, R1, R2, R3, R4, R5, M0, M1, M11, M2, M3, M4, da1, da2, da3, da4, da5):
+    """Radii [n_rays, npr] of the nodes of rays whose termination shock, heliopause and bow shock
+    sit at R2, R3, R4 [n_rays]: the distribution of Luch::dvigenie for A_Luch rays (Luch.cpp:8-151),
+    used here to lay out the initial mesh."""
+    R2, R3, R4 = (np.asarray(a, np.float64).reshape(-1, 1) for a in (R2, R3, R4))
+    cols = []
+    for j in range(M0):
+        cols.append(np.full_like(R2, R0 + j * (R1 - R0) / M0))
+    cols.append(np.full_like(R2, R1))
+    for j in range(M1):
+        cols.append(R1 + (j + 1) * (R2 - R1) / (M1 + 1))
+    cols.append(R2)
+    dr = (R2 - R1) / (M1 + 1)
+    for j in range(M11):
+        cols.append(R2 + dr * (j + 1))
+    a, b = da1, da2
+    for j in range(M2 - M11):
+        t = (j + 1.0) / (M2 - M11 + 1.0)
+        u = 1.0 - 3.0 * t + 2.0 * t * t
+        w = 1.0 - 2.0 * t
+        ss = t * (a * u * u + t * (7.0 + b * w * w * (-1.0 + t) - 2.0 * t * (9.0 + 2.0 * t * (-5.0 + 2.0 * t))))
+        cols.append(R2 + dr * M11 + (R3 - R2 - dr * M11) * ss)
+    cols.append(R3)
+    a, b = da3, da4
+    for j in range(M3):
+        t = (j + 1.0) / (M3 + 1.0)
+        cols.append(R3 + (R4 - R3) * (t * (a - (2.0 * a + b - 3.0) * t + (a + b - 2.0) * t * t)))
+    cols.append(R4)
+    a, b = da5, 2.0
+    for j in range(M4):
+        t = (j + 1.0) / (M4 + 1.0)
+        cols.append(R4 + (R5 - R4) * (a * t - (3.0 * a + b - 4.0) * t ** 3 + (2.0 * a + b - 3.0) * t ** 4))
+    cols.append(np.full_like(R2, R5))
+    return np.concatenate(cols, axis=1)
+
+
+def shell_fitted(n_edge=6, M=(3, 6, 2, 8, 5, 5), R0_au=1.0, R1_au=10.0, R5_au=600.0, surf_au=(80.0, 120.0, 240.0),
+                 asym=(0.2, 0.25, 0.3), da=(1.0, 1.0, 1.0, 1.0, 0.5), zone1=True, noise=0.0, seed_state=4, tvd=False):
+    """Heliosphere shell whose nodes lie on A_Luch rays through the cube-sphere directions, with
+    the termination shock, heliopause and bow shock as fitted node surfaces: the TS at
+    surf_au[0] * (1 - asym[0] cos the) along a ray at polar angle `the` from the +x axis (closest
+    upwind, where the interstellar flow comes from), the HP and the BS likewise; the ray layout of
+    Luch::dvigenie with counts M = (M0, M1, M11, M2, M3, M4) and shapes da = (da1 .. da5).  Zones:
+    1 inside the TS (2 when zone1 is False), 2 to the HP, 3 to the BS, 4 outside.  TS, HP (with
+    the contact flag of contact_hard, Phys_param.cpp:523) and BS faces use HLLD (Gran::Get_method);
+    other faces HLLC where they touch Zone_1, HLLD elsewhere.  Returns (mesh, prim, params, rays)
+    with move_mesh = 2 and the surface-fitting defaults of Phys_param.cpp:500-520."""
+    M0, M1, M11, M2, M3, M4 = M
+    npr = M0 + M1 + M2 + M3 + M4 + 5
+    nl = npr - 1
+    R0, R1, R5 = R0_au * R0_AU, R1_au * R0_AU, R5_au * R0_AU
+    xyz, cn = mesh_shell(n_edge, np.linspace(R0, R5, nl + 1))
+    ns = 6 * n_edge * n_edge + 2
+    unit = xyz[:ns] / np.linalg.norm(xyz[:ns], axis=1)[:, None]
+    the = np.arccos(np.clip(unit[:, 0], -1.0, 1.0))
+    phi = np.arctan2(unit[:, 2], unit[:, 1])
+    ct = np.cos(the)
+    Rs = [surf_au[k] * R0_AU * (1.0 - asym[k] * ct) for k in range(3)]
+    rad = a_luch_radii(R0, R1, Rs[0], Rs[1], Rs[2], R5, M0, M1, M11, M2, M3, M4, *da)  # [ns, npr]
+    for k in range(npr):
+        r = rad[:, k]
+        xyz[k * ns:(k + 1) * ns, 0] = r * np.cos(the)
+        xyz[k * ns:(k + 1) * ns, 1] = r * np.sin(the) * np.cos(phi)
+        xyz[k * ns:(k + 1) * ns, 2] = r * np.sin(the) * np.sin(phi)
+    iTS, iHP, iBS = M0 + M1 + 1, M0 + M1 + M2 + 2, M0 + M1 + M2 + M3 + 3
+    layer = cn.min(axis=1) // ns
+    zone = np.where(layer < iTS, 1 if zone1 else 2, np.where(layer < iHP, 2, np.where(layer < iBS, 3, 4))).astype(np.int8)
+
+    def bc_fn(fc, cc, fn, fcl, b):
+        fl = fn.min(axis=1) // ns
+        bc = np.full(fc.shape[0], BC_INTERIOR)
+        bc[b & (fl == 0)] = BC_INNER_HARD
+        outer = b & (fl > 0)
+        bc[outer & (fc[:, 0] > 0.0)] = BC_OUTER_HARD
+        bc[outer & (fc[:, 0] <= 0.0)] = BC_OUTER_SOFT
+        return bc
+
+    def surf_kind(fn):
+        lo, hi = fn.min(axis=1) // ns, fn.max(axis=1) // ns
+        flat = lo == hi
+        return np.where(flat & (lo == iTS), 1, np.where(flat & (lo == iHP), 2, np.where(flat & (lo == iBS), 3, 0)))
+
+    def method_fn(fcl, z):
+        za = z[fcl[:, 0]]
+        zb = np.where(fcl[:, 1] >= 0, z[np.maximum(fcl[:, 1], 0)], 3)
+        return np.where((za == 1) | (zb == 1), 2, 3)
+
+    def bstate_fn(fc, bc):
+        bs = np.zeros((fc.shape[0], 8))
+        ih = bc == BC_INNER_HARD
+        bs[ih] = solar_wind_state(fc[ih])
+        bs[bc == BC_OUTER_HARD] = lism_state()
+        return bs
+
+    mesh, ccen = _finish(xyz, cn, bc_fn, method_fn, zone, bstate_fn=bstate_fn)
+    sk = surf_kind(mesh.face_nodes)
+    flags = np.zeros(mesh.n_faces, np.uint8)
+    flags[sk == 1] = 1 << 3
+    flags[sk == 2] = (1 << 2) | (1 << 1)
+    flags[sk == 3] = 1 << 4
+    mesh.face_flags = flags
+    mesh.face_method = np.where(sk > 0, 3, mesh.face_method).astype(np.int8)
+    if tvd:
+        t = np.ones(cn.shape[0], np.uint8)
+        t[mesh.face_cells[mesh.face_bc == BC_INNER_HARD, 0]] = 0
+        mesh.cell_is_tvd = t
+    rc = np.linalg.norm(ccen, axis=1)
+    wind = solar_wind_state(ccen)
+    rhp = surf_au[1] * R0_AU
+    w = 0.5 * (1.0 + np.tanh((rc - rhp) / (0.2 * rhp)))
+    prim = (1.0 - w)[:, None] * wind + w[:, None] * lism_state()[None, :]
+    if noise > 0.0:
+        nc = cn.shape[0]
+        u = [2.0 * splitmix_uniform(seed_state, np.arange(nc) * 4 + k) - 1.0 for k in range(4)]
+        vmag = np.linalg.norm(prim[:, 1:4], axis=1)
+        prim[:, 0] *= 1.0 + noise * u[0]
+        for k in range(3):
+            prim[:, 1 + k] += noise * vmag * u[1 + k]
+    params = default_params(move_mesh=2, tvd=1 if tvd else 0, move_TS=1, move_HP=1, move_BS=1, sglag_TS=1,
+                            sglag_HP=1, sglag_BS=0, velocity_TS=0.1, velocity_HP=0.1, velocity_BS=0.3,
+                            sglag_TS_k_sphere=0.01, sglag_HP_k_sphere=0.01, sglag_BS_k=0.05)
+    rays = dict(ray_nodes=(np.arange(npr)[None, :] * ns + np.arange(ns)[:, None]).astype(np.int32),
+                ray_angles=np.stack([the, phi], axis=1), R0=R0, R1=R1, R5=R5, M0=M0, M1=M1, M11=M11, M2=M2,
+                M3=M3, M4=M4, da1=da[0], da2=da[1], da3=da[2], da4=da[3], da5=da[4], phi_a=math.pi - math.pi / 180.0)
+    return mesh, prim, params, rays
+
+
 def neutrals(n_cells, seed=5):
     """Per-cell neutral-H moments [n_H, c_H^2, V_H(3)] in code units: n_H ~ U(0.08, 0.2) cm^-3,
     V_H = 26 km/s along the inflow axis (-x) with +-30 % per-component perturbation, T_H ~

@@ -53,6 +53,34 @@ struct DevParams {
     double motion_omega, motion_phase;
 };
 
+// Fitted surfaces and their rays (Setka::Culc_Velocity_surface, Setka_move.cpp:111-1357, and
+// Luch::dvigenie of the A_Luch rays, Luch.cpp:6-152).  Surface kinds: 1 TS, 2 HP, 3 BS; arrays
+// indexed by kind have slot 0 unused.
+struct FitDev {
+    int32_t n_sf, n_sn;
+    const int32_t *sf_face;      // [n_sf] device face of each surface face
+    const uint8_t *sf_kind;      // [n_sf]
+    double *sf_vel;              // [n_sf*3] velocity_K * ds * n of each surface face
+    const int32_t *sn_node;      // [n_sn] device node of each surface node
+    const uint8_t *sn_kind;      // [n_sn]
+    const int32_t *sn_face_off;  // [n_sn+1] its surface faces (indices into sf_*), ascending reference id
+    const int32_t *sn_face;
+    const int32_t *sn_nb_off;    // [n_sn+1] its smoothing neighbours (device nodes), ascending reference id
+    const int32_t *sn_nb;
+    const double *sn_centre;     // [n_sn*3] centre of the sphere smoothing (xc, yc, zc)
+    double *sn_vel;              // [n_sn*3] averaged node velocity of the last call
+    int64_t n_rays;
+    int32_t npr;                 // nodes per ray, inner to outer sphere
+    const int32_t *ray_nodes;    // [n_rays*npr] device nodes
+    const double *ray_trig;      // [n_rays*4] cos(the), sin(the), cos(phi), sin(phi)
+    double R0, R1, R5, da1, da2, da3, da4, da5;
+    int32_t M0, M1, M11, M2, M3, M4;
+    int32_t move[4], sglag[4];
+    double velocity[4];          // velocity_TS, velocity_HP, velocity_BS
+    double k_smooth[4];          // sglag_TS_k_sphere, sglag_HP_k_sphere, sglag_BS_k
+    double phi_lim;              // HP sphere smoothing where the polar angle is <= phi_a + pi/180
+};
+
 // Device-side scalars of the step loop (one instance per context).
 struct StepScalars {
     double dt;                       // dt of the current step (lagged, Setka_physics.cpp:1606)

@@ -1,0 +1,205 @@
+// k_fit.hip -- fitted-surface node motion: Setka::Culc_Velocity_surface (Setka_move.cpp:111-1357)
+// and Luch::dvigenie for the A_Luch / A2_Luch rays (Luch.cpp:6-152), the production caller of
+// the moving-mesh step (Setka_physics.cpp:1616-1627).
+//
+// Three kernels per step, all FP64 and in the reference's operation order:
+//   k_fit_faces   one thread per TS/HP/BS face: the face's states at the current layer, chlld with
+//                 w = 0, and velocity_K * (S_L | S_M | S_R) * n;
+//   k_fit_nodes   one thread per surface node: the average of its faces' vectors and of the sphere
+//                 (TS, HP) or neighbour-mean (BS) smoothing term, then the move along the radius
+//                 (HP behind the Sun: along the cylinder radius);
+//   k_fit_rays    one thread per ray node: the A_Luch distribution between the moved surfaces.
+// The reference adds the face vectors into each node under a mutex from an OpenMP loop, so its
+// summation order varies from run to run; here a node sums them in ascending reference face id,
+// which makes the motion repeat bit for bit.
+#define TITAN_FLUX_ONLY 1
+#include "step_kernels.h"
+#include "launch.h"
+
+namespace titan {
+
+namespace {
+inline unsigned grid_of(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+}  // namespace
+
+__global__ void __launch_bounds__(256) k_fit_faces(DevMesh M, DevParams P, FitDev F,
+                                                   const double *__restrict__ prim, const double *__restrict__ X,
+                                                   const double *__restrict__ cell_center, StepScalars *S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= F.n_sf) return;
+    const int64_t f = F.sf_face[i];
+    const int kind = F.sf_kind[i];
+    double Gc[3], n[3], area;
+    face_geom_one(M, X, cell_center, f, P.geom_mode, Gc, n, area);
+    const uint32_t code = M.face_code[f];
+    double qL[8], qR[8];
+    const bool okbc = face_states_at(M, P, f, code & 15u, prim, cell_center, Gc, n, qL, qR);
+    // chlld(Get_method(), normal, w = 0, ..., null_bn = false, n_disc = 0), no face flags
+    // (Setka_move.cpp:171-175, 328-333, 422-426)
+    RiemannOut<double, 0> o;
+    chlld<double, 0>((int)((code >> 4) & 15u), 0, 0, n[0], n[1], n[2], 0.0, qL, qR, nullptr, nullptr, o);
+    if (!o.ok || !okbc) report_error(S, -3, M.face_ref[f]);
+    const double ds = kind == 1 ? o.dsl : (kind == 2 ? o.dsc : o.dsr);
+    // Setka_move.cpp:336-350: a NaN or subnormal contact speed stops the reference
+    if (kind == 2 && (isnan(ds) || (ds != 0.0 && fabs(ds) < 2.2250738585072014e-308))) report_error(S, -8, M.face_ref[f]);
+    const double s = F.velocity[kind] * ds;
+    F.sf_vel[3 * i] = s * n[0];
+    F.sf_vel[3 * i + 1] = s * n[1];
+    F.sf_vel[3 * i + 2] = s * n[2];
+}
+
+// time > 0: the step length of an explicit call; otherwise the context's current dt.
+__global__ void __launch_bounds__(256) k_fit_nodes(FitDev F, const double *__restrict__ X1, double *__restrict__ X2,
+                                                   const StepScalars *__restrict__ S, double time) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= F.n_sn) return;
+    const double dt = time > 0.0 ? time : S->dt;
+    const int kind = F.sn_kind[i];
+    const int64_t g = F.sn_node[i];
+    const double A0 = X1[3 * g], A1 = X1[3 * g + 1], A2 = X1[3 * g + 2];
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    int num = 0;
+    if (F.move[kind]) {
+        for (int j = F.sn_face_off[i]; j < F.sn_face_off[i + 1]; j++) {
+            const double *s = F.sf_vel + 3 * (int64_t)F.sn_face[j];
+            v0 += s[0]; v1 += s[1]; v2 += s[2];
+            num++;
+        }
+    }
+    const int nb0 = F.sn_nb_off[i], nb1 = F.sn_nb_off[i + 1];
+    bool smooth = F.sglag[kind] != 0;
+    if (smooth && kind == 2) {
+        // Setka_move.cpp:577-580: only near the nose, polar angle from +x at the current layer
+        const double phi = polar_angle(A0, sqrt(0.0 * 0.0 + A1 * A1 + A2 * A2));
+        smooth = !(phi > F.phi_lim);
+    }
+    if (smooth && kind == 3) smooth = (nb1 - nb0) > 5;  // Setka_move.cpp:1190
+    // a TS/HP node the set-up gave no neighbours (fewer than 3 points) gets no sphere term; the
+    // reference would divide by the empty neighbour count there
+    if (smooth && kind != 3 && nb1 == nb0) smooth = false;
+    if (smooth) {
+        const double c = F.velocity[kind] * F.k_smooth[kind];
+        double V0, V1, V2;
+        if (kind == 3) {
+            // mean of the node and its BS neighbours (Setka_move.cpp:1180-1209)
+            double B0 = A0, B1 = A1, B2 = A2;
+            for (int j = nb0; j < nb1; j++) {
+                const double *p = X1 + 3 * (int64_t)F.sn_nb[j];
+                B0 += p[0]; B1 += p[1]; B2 += p[2];
+            }
+            const double cnt = (double)(nb1 - nb0) + 1.0;
+            B0 /= cnt; B1 /= cnt; B2 /= cnt;
+            V0 = c * (B0 - A0) / dt; V1 = c * (B1 - A1) / dt; V2 = c * (B2 - A2) / dt;
+        } else {
+            // onto the sphere about (xc, yc, zc) through the neighbours' mean distance
+            // (Setka_move.cpp:466-506 TS, 583-622 HP)
+            const double C0 = F.sn_centre[3 * i], C1 = F.sn_centre[3 * i + 1], C2 = F.sn_centre[3 * i + 2];
+            const double e0 = A0 - C0, e1 = A1 - C1, e2 = A2 - C2;
+            const double rr = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+            double r = 0.0;
+            for (int j = nb0; j < nb1; j++) {
+                const double *p = X1 + 3 * (int64_t)F.sn_nb[j];
+                const double d0 = p[0] - C0, d1 = p[1] - C1, d2 = p[2] - C2;
+                r += sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            }
+            r /= (double)(nb1 - nb0);
+            const double k = r / rr;
+            const double P0 = k * e0 + C0, P1 = k * e1 + C1, P2 = k * e2 + C2;
+            V0 = c * (P0 - A0) / dt; V1 = c * (P1 - A1) / dt; V2 = c * (P2 - A2) / dt;
+        }
+        v0 += V0; v1 += V1; v2 += V2;
+        num++;
+    }
+    if (num > 0) {
+        v0 /= num; v1 /= num; v2 /= num;
+    }
+    F.sn_vel[3 * i] = v0;
+    F.sn_vel[3 * i + 1] = v1;
+    F.sn_vel[3 * i + 2] = v2;
+    if (F.move[kind]) {
+        // Setka_move.cpp:1228-1278: along the unit radius (HP with x < 0: the radius in the yz plane)
+        double B0 = A0, B1 = A1, B2 = A2;
+        if (kind == 2 && !(A0 >= 0.0)) B0 = 0.0;
+        const double z = B0 * B0 + B1 * B1 + B2 * B2;
+        if (z > 0.0) {
+            const double sz = sqrt(z);
+            B0 /= sz; B1 /= sz; B2 /= sz;
+        }
+        const double s = (v0 * B0 + v1 * B1 + v2 * B2) * dt;
+        X2[3 * g] = A0 + s * B0;
+        X2[3 * g + 1] = A1 + s * B1;
+        X2[3 * g + 2] = A2 + s * B2;
+    }
+}
+
+// Luch::dvigenie, A_Luch / A2_Luch (Luch.cpp:8-151).  Node j of a ray: the inner segment up to
+// R1 (j < M0), then to the TS, a uniform run of M11 past it, the run to the HP, to the BS and to
+// R5.  The ray's nodes M0 (at R1) and its TS, HP and BS nodes are not written.
+__global__ void __launch_bounds__(256) k_fit_rays(FitDev F, double *__restrict__ X) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= F.n_rays * (int64_t)F.npr) return;
+    const int64_t ray = t % F.n_rays;
+    const int j = (int)(t / F.n_rays);
+    const int iTS = F.M0 + F.M1 + 1, iHP = iTS + F.M2 + 1, iBS = iHP + F.M3 + 1;
+    if (j == F.M0 || j == iTS || j == iHP || j == iBS) return;
+    const int32_t *rn = F.ray_nodes + ray * F.npr;
+    auto radius = [&](int k) {
+        const double *p = X + 3 * (int64_t)rn[k];
+        return sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    };
+    double r;
+    if (j < F.M0) {
+        r = F.R0 + j * (F.R1 - F.R0) / F.M0;
+    } else if (j < iTS) {
+        const double R2 = radius(iTS);
+        r = F.R1 + (j - F.M0) * (R2 - F.R1) / (F.M1 + 1);
+    } else if (j <= iTS + F.M11) {
+        const double R2 = radius(iTS);
+        const double dr = (R2 - F.R1) / (F.M1 + 1);
+        r = R2 + dr * (j - iTS);
+    } else if (j < iHP) {
+        const double R2 = radius(iTS), R3 = radius(iHP);
+        const double dr = (R2 - F.R1) / (F.M1 + 1);
+        const double a = F.da1, b = F.da2;
+        const double s = (j - iTS - F.M11 + 0.0) / (F.M2 - F.M11 + 1.0);
+        const double u = 1.0 - 3.0 * s + 2.0 * (s * s);
+        const double w = 1.0 - 2.0 * s;
+        const double ss = s * (a * (u * u) +
+                               s * (7.0 + b * (w * w) * (-1.0 + s) - 2.0 * s * (9.0 + 2.0 * s * (-5.0 + 2.0 * s))));
+        r = R2 + dr * F.M11 + (R3 - R2 - dr * F.M11) * ss;
+    } else if (j < iBS) {
+        const double R3 = radius(iHP), R4 = radius(iBS);
+        const double a = F.da3, b = F.da4;
+        const double s = (j - iHP + 0.0) / (F.M3 + 1.0);
+        const double ss = s * (a - (2.0 * a + b - 3.0) * s + (a + b - 2.0) * s * s);
+        r = R3 + (R4 - R3) * ss;
+    } else if (j < iBS + 1 + F.M4) {
+        const double R4 = radius(iBS);
+        const double a = F.da5, b = 2.0;
+        const double s = (j - iBS + 0.0) / (F.M4 + 1.0);
+        const double ss = a * s - (3.0 * a + b - 4.0) * s * s * s + (2.0 * a + b - 3.0) * s * s * s * s;
+        r = R4 + (F.R5 - R4) * ss;
+    } else {
+        r = F.R5;
+    }
+    const double *tr = F.ray_trig + 4 * ray;
+    const int64_t g = rn[j];
+    X[3 * g] = r * tr[0];
+    X[3 * g + 1] = r * tr[1] * tr[2];
+    X[3 * g + 2] = r * tr[1] * tr[3];
+}
+
+void launch_fit_surface(hipStream_t s, const DevMesh &M, const DevParams &P, const FitDev &F, const double *prim,
+                        const double *X1, double *X2, const double *cell_center, StepScalars *S, double time) {
+    if (F.n_sf > 0)
+        hipLaunchKernelGGL(k_fit_faces, dim3(grid_of(F.n_sf, 256)), dim3(256), 0, s, M, P, F, prim, X1, cell_center,
+                           S);
+    if (F.n_sn > 0) hipLaunchKernelGGL(k_fit_nodes, dim3(grid_of(F.n_sn, 256)), dim3(256), 0, s, F, X1, X2, S, time);
+}
+
+void launch_fit_rays(hipStream_t s, const FitDev &F, double *X) {
+    const int64_t n = F.n_rays * (int64_t)F.npr;
+    if (n > 0) hipLaunchKernelGGL(k_fit_rays, dim3(grid_of(n, 256)), dim3(256), 0, s, F, X);
+}
+
+}  // namespace titan

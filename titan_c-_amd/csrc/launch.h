@@ -33,6 +33,12 @@ void launch_step_tile_f32(hipStream_t s, const TileMesh &TM, const DevMesh &M, c
                           long long step);
 size_t step_tile_lds_bytes(const TileMesh &TM, bool moving);
 
+// fitted surfaces (k_fit.hip): surface faces and nodes of layer X1 -> surface nodes of X2
+// (time > 0: explicit step length, else S->dt); then every ray's other nodes of X
+void launch_fit_surface(hipStream_t s, const DevMesh &M, const DevParams &P, const FitDev &F, const double *prim,
+                        const double *X1, double *X2, const double *cell_center, StepScalars *S, double time);
+void launch_fit_rays(hipStream_t s, const FitDev &F, double *X);
+
 // chlld over n independent states; err (device int) set to 1 on a fatal branch
 void launch_chlld_batch(hipStream_t s, int precision, int64_t n, int K, const int8_t *n_state,
                         const int8_t *n_disc, const uint8_t *flags, const double *normal, const double *w,

@@ -270,7 +270,7 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
             else { region = 0; out.ok = 0; }
             if (region == 2 || region == 3) {
                 const bool R = region == 3;
-                const float rs = R ? fr2 : fr1, su = R ? suR : suL, S = R ? SR : SL;
+                const float rs = R ? fr2 : fr1, su = R ? suR : suL;
                 const float dSM = R ? dSMR : dSML, dr = R ? drR : drL, rz = R ? rzR : rzL;
                 const float dptz = R ? dptzR : dptzL, g = R ? gR : gL;
                 const float *fv = R ? fvR : fvL, *fb = R ? fbR : fbL;
@@ -307,7 +307,6 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
                 for (int i = 0; i < 3; i++) dsbv -= fb[i] * dvh[i] + dBh[i] * (fv[i] + dvh[i]);
                 // e*_side - e_side less |v|^2/2 (rho* - rho)
                 const float dEi = (ei * dSM + pt * dSM + dptz * SM + bn * dsbv) / g;
-                (void)rs;
                 float dUs[8];
                 dUs[0] = dr;
                 dUs[1] = rz * dSM;  // (rho v_n)* - rho v_n less v_n dr
@@ -392,7 +391,7 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
             if (j >= 2 && j <= 5) {
                 split = true;
                 const bool R = right;
-                const float S = R ? SR : SL, dSM = R ? dSMR : dSML, dr = R ? drR : drL, rz = R ? rzR : rzL;
+                const float dSM = R ? dSMR : dSML, dr = R ? drR : drL, rz = R ? rzR : rzL;
                 const float dUs[8] = {dr,
                                       rz * dSM,
                                       (R ? dvzR1 : dvzL1) * rz,

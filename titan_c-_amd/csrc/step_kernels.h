@@ -63,6 +63,52 @@ __device__ __forceinline__ double tet_volume(const double *A, const double *B, c
     return div_by(fabs(d), 6.0, 1.0 / 6.0);
 }
 
+// Centre, unit normal (away from cells[0], whose centre cc holds) and area of face f from the
+// nodes X: Gran::Culc_measure (Gran.cpp:112-218), or the exactly conservative variant (gcl != 0).
+__device__ __forceinline__ void face_geom_one(const DevMesh &M, const double *__restrict__ X,
+                                              const double *__restrict__ cell_center, int64_t f, int gcl,
+                                              double c[3], double n[3], double &A) {
+    double x[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const double *p = X + 3 * (int64_t)M.face_nodes[4 * f + i];
+        x[i][0] = p[0]; x[i][1] = p[1]; x[i][2] = p[2];
+    }
+    double xc = 0.0, yc = 0.0, zc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) { xc += x[i][0]; yc += x[i][1]; zc += x[i][2]; }
+    xc /= 4; yc /= 4; zc /= 4;
+    double n1, n2, n3;
+    if (!gcl) {
+        double S = 0.0;
+        S += tri_area(x[0], x[1], x[2]);
+        S += tri_area(x[0], x[3], x[2]);
+        A = S;
+        double x1 = (x[0][0] + x[1][0]) / 2.0, y1 = (x[0][1] + x[1][1]) / 2.0, z1 = (x[0][2] + x[1][2]) / 2.0;
+        double x2 = (x[1][0] + x[2][0]) / 2.0, y2 = (x[1][1] + x[2][1]) / 2.0, z2 = (x[1][2] + x[2][2]) / 2.0;
+        double x3 = (x[2][0] + x[3][0]) / 2.0, y3 = (x[2][1] + x[3][1]) / 2.0, z3 = (x[2][2] + x[3][2]) / 2.0;
+        double ax = x2 - x1, ay = y2 - y1, az = z2 - z1;
+        double bx = x3 - x1, by = y3 - y1, bz = z3 - z1;
+        n1 = ay * bz - az * by;
+        n2 = az * bx - ax * bz;
+        n3 = ax * by - ay * bx;
+        double nn = sqrt(n1 * n1 + n2 * n2 + n3 * n3);
+        n1 /= nn; n2 /= nn; n3 /= nn;
+    } else {
+        double ax = x[2][0] - x[0][0], ay = x[2][1] - x[0][1], az = x[2][2] - x[0][2];
+        double bx = x[3][0] - x[1][0], by = x[3][1] - x[1][1], bz = x[3][2] - x[1][2];
+        double s1 = 0.5 * (ay * bz - az * by);
+        double s2 = 0.5 * (az * bx - ax * bz);
+        double s3 = 0.5 * (ax * by - ay * bx);
+        A = sqrt(s1 * s1 + s2 * s2 + s3 * s3);
+        n1 = s1 / A; n2 = s2 / A; n3 = s3 / A;
+    }
+    const double *cc = cell_center + 3 * (int64_t)M.face_cells[2 * f];
+    if (spf(n1, n2, n3, cc[0] - xc, cc[1] - yc, cc[2] - zc) > 0.0) { n1 = -n1; n2 = -n2; n3 = -n3; }
+    c[0] = xc; c[1] = yc; c[2] = zc;
+    n[0] = n1; n[1] = n2; n[2] = n3;
+}
+
 #ifndef TITAN_FLUX_ONLY
 // ---- step bookkeeping (one thread) --------------------------------------------------------
 // step n (1-based, counted over the context's life): dt = 1e-5 for n == 1, else the min over
@@ -113,49 +159,14 @@ __global__ void k_cell_center(DevMesh M, const double *__restrict__ X, DevGeom G
 __global__ void k_face_geom(DevMesh M, const double *__restrict__ X, DevGeom G, int gcl) {
     int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= M.n_faces) return;
-    double x[4][3];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const double *p = X + 3 * (int64_t)M.face_nodes[4 * f + i];
-        x[i][0] = p[0]; x[i][1] = p[1]; x[i][2] = p[2];
-    }
-    double xc = 0.0, yc = 0.0, zc = 0.0;
-#pragma unroll
-    for (int i = 0; i < 4; i++) { xc += x[i][0]; yc += x[i][1]; zc += x[i][2]; }
-    xc /= 4; yc /= 4; zc /= 4;
-    double n1, n2, n3, A;
-    if (!gcl) {
-        double S = 0.0;
-        S += tri_area(x[0], x[1], x[2]);
-        S += tri_area(x[0], x[3], x[2]);
-        A = S;
-        double x1 = (x[0][0] + x[1][0]) / 2.0, y1 = (x[0][1] + x[1][1]) / 2.0, z1 = (x[0][2] + x[1][2]) / 2.0;
-        double x2 = (x[1][0] + x[2][0]) / 2.0, y2 = (x[1][1] + x[2][1]) / 2.0, z2 = (x[1][2] + x[2][2]) / 2.0;
-        double x3 = (x[2][0] + x[3][0]) / 2.0, y3 = (x[2][1] + x[3][1]) / 2.0, z3 = (x[2][2] + x[3][2]) / 2.0;
-        double ax = x2 - x1, ay = y2 - y1, az = z2 - z1;
-        double bx = x3 - x1, by = y3 - y1, bz = z3 - z1;
-        n1 = ay * bz - az * by;
-        n2 = az * bx - ax * bz;
-        n3 = ax * by - ay * bx;
-        double nn = sqrt(n1 * n1 + n2 * n2 + n3 * n3);
-        n1 /= nn; n2 /= nn; n3 /= nn;
-    } else {
-        double ax = x[2][0] - x[0][0], ay = x[2][1] - x[0][1], az = x[2][2] - x[0][2];
-        double bx = x[3][0] - x[1][0], by = x[3][1] - x[1][1], bz = x[3][2] - x[1][2];
-        double s1 = 0.5 * (ay * bz - az * by);
-        double s2 = 0.5 * (az * bx - ax * bz);
-        double s3 = 0.5 * (ax * by - ay * bx);
-        A = sqrt(s1 * s1 + s2 * s2 + s3 * s3);
-        n1 = s1 / A; n2 = s2 / A; n3 = s3 / A;
-    }
-    const double *cc = G.cell_center + 3 * (int64_t)M.face_cells[2 * f];
-    if (spf(n1, n2, n3, cc[0] - xc, cc[1] - yc, cc[2] - zc) > 0.0) { n1 = -n1; n2 = -n2; n3 = -n3; }
-    G.face_center[3 * f] = xc;
-    G.face_center[3 * f + 1] = yc;
-    G.face_center[3 * f + 2] = zc;
-    G.face_normal[3 * f] = n1;
-    G.face_normal[3 * f + 1] = n2;
-    G.face_normal[3 * f + 2] = n3;
+    double c[3], n[3], A;
+    face_geom_one(M, X, G.cell_center, f, gcl, c, n, A);
+    G.face_center[3 * f] = c[0];
+    G.face_center[3 * f + 1] = c[1];
+    G.face_center[3 * f + 2] = c[2];
+    G.face_normal[3 * f] = n[0];
+    G.face_normal[3 * f + 1] = n[1];
+    G.face_normal[3 * f + 2] = n[2];
     G.face_area[f] = A;
 }
 
@@ -302,10 +313,12 @@ __device__ __forceinline__ void load_state(const double *__restrict__ prim, int6
     s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
 }
 
-// Returns false on an unknown boundary code.
-__device__ __forceinline__ bool face_states(const DevMesh &M, const DevParams &P, int64_t f, uint32_t bc,
-                                            const double *__restrict__ prim, const DevGeom &G, const double n[3],
-                                            double qL[8], double qR[8]) {
+// Returns false on an unknown boundary code.  cell_center: centres of the layer of prim; Gc: the
+// face centre at that layer.
+__device__ __forceinline__ bool face_states_at(const DevMesh &M, const DevParams &P, int64_t f, uint32_t bc,
+                                               const double *__restrict__ prim,
+                                               const double *__restrict__ cell_center, const double Gc[3],
+                                               const double n[3], double qL[8], double qR[8]) {
     const int64_t A = M.face_cells[2 * f];
     double sA[8];
     load_state(prim, A, sA);
@@ -346,28 +359,26 @@ __device__ __forceinline__ bool face_states(const DevMesh &M, const DevParams &P
             double sAA[8], sBB[8];
             load_state(prim, AA, sAA);
             load_state(prim, BB, sBB);
-            const double *cc = G.cell_center;
+            const double *cc = cell_center;
             const double cA[3] = {cc[3 * A], cc[3 * A + 1], cc[3 * A + 2]};
             const double cB[3] = {cc[3 * B], cc[3 * B + 1], cc[3 * B + 2]};
             const double cAA[3] = {cc[3 * AA], cc[3 * AA + 1], cc[3 * AA + 2]};
             const double cBB[3] = {cc[3 * BB], cc[3 * BB + 1], cc[3 * BB + 2]};
-            const double Gc[3] = {G.face_center[3 * f], G.face_center[3 * f + 1], G.face_center[3 * f + 2]};
             tvd_states((M.cell_code[AA] >> 8) & 15u, zA, zB, (M.cell_code[BB] >> 8) & 15u, sAA, sA, sB, sBB, cAA,
                        cA, cB, cBB, Gc, P.gamma, qL, qR);
             return true;
         }
     }
     if (zA == 1u || zB == 1u) {
-        const double Gc[3] = {G.face_center[3 * f], G.face_center[3 * f + 1], G.face_center[3 * f + 2]};
         if (zA == 1u) {
-            const double Ac[3] = {G.cell_center[3 * A], G.cell_center[3 * A + 1], G.cell_center[3 * A + 2]};
+            const double Ac[3] = {cell_center[3 * A], cell_center[3 * A + 1], cell_center[3 * A + 2]};
             zone1_state(sA, Ac, Gc, P.gamma, qL);
         } else {
 #pragma unroll
             for (int i = 0; i < 8; i++) qL[i] = sA[i];
         }
         if (zB == 1u) {
-            const double Bc[3] = {G.cell_center[3 * B], G.cell_center[3 * B + 1], G.cell_center[3 * B + 2]};
+            const double Bc[3] = {cell_center[3 * B], cell_center[3 * B + 1], cell_center[3 * B + 2]};
             zone1_state(sB, Bc, Gc, P.gamma, qR);
         } else {
 #pragma unroll
@@ -378,6 +389,13 @@ __device__ __forceinline__ bool face_states(const DevMesh &M, const DevParams &P
         for (int i = 0; i < 8; i++) { qL[i] = sA[i]; qR[i] = sB[i]; }
     }
     return true;
+}
+
+__device__ __forceinline__ bool face_states(const DevMesh &M, const DevParams &P, int64_t f, uint32_t bc,
+                                            const double *__restrict__ prim, const DevGeom &G, const double n[3],
+                                            double qL[8], double qR[8]) {
+    const double Gc[3] = {G.face_center[3 * f], G.face_center[3 * f + 1], G.face_center[3 * f + 2]};
+    return face_states_at(M, P, f, bc, prim, G.cell_center, Gc, n, qL, qR);
 }
 
 __device__ __forceinline__ void report_error(StepScalars *S, int code, long long index) {

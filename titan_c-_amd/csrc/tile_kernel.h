@@ -427,6 +427,7 @@ __global__ void __launch_bounds__(kTileThreads, 2)
     const int fb = TM.face_off[t], nf = TM.face_off[t + 1] - fb;
     const int own_lo = TM.own[2 * t], own_hi = TM.own[2 * t + 1];
     const bool moving = P.move_mesh != 0;
+    const bool given = P.move_mesh == 2;
     const int gcl = P.geom_mode;
     const double dt = S->dt;
     const double fac = S->motion_fac;
@@ -450,7 +451,9 @@ __global__ void __launch_bounds__(kTileThreads, 2)
         const int64_t g = i == tid ? g_pre0 : (i == tid + kTileThreads ? g_pre1 : TM.nodes[nb + i]);
         const double a0 = X1g[3 * g], a1 = X1g[3 * g + 1], a2 = X1g[3 * g + 2];
         L.x1[3 * i] = a0; L.x1[3 * i + 1] = a1; L.x1[3 * i + 2] = a2;
-        if (moving) {
+        if (given) {  // new layer written before the step (fitted surfaces and their rays)
+            L.x2[3 * i] = X2g[3 * g]; L.x2[3 * i + 1] = X2g[3 * g + 1]; L.x2[3 * i + 2] = X2g[3 * g + 2];
+        } else if (moving) {
             const double b0 = a0 + fac * disp[3 * g], b1 = a1 + fac * disp[3 * g + 1], b2 = a2 + fac * disp[3 * g + 2];
             L.x2[3 * i] = b0; L.x2[3 * i + 1] = b1; L.x2[3 * i + 2] = b2;
             if (g >= own_lo && g < own_hi) { X2g[3 * g] = b0; X2g[3 * g + 1] = b1; X2g[3 * g + 2] = b2; }

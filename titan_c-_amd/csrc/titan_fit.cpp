@@ -1,0 +1,423 @@
+// titan_fit.cpp -- host set-up of the fitted surfaces (TS, HP, BS) and of the A_Luch rays that
+// carry the other nodes: the surface lists of Setka::New_append_surfaces, the surface neighbours
+// of each face (macros4, Setka.cpp:58-73), the node neighbours and sphere centres of
+// Smooth_head_TS3 / Smooth_head_HP3 / Find_Yzel_Sosed_for_BS (Setka_move.cpp:2046-2397), and the
+// rays' angles and indices for Luch::dvigenie (Luch.cpp:6-152).  The per-step arithmetic runs in
+// k_fit.hip.
+#include "titan_fit.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace titan {
+
+namespace {
+
+constexpr double kConstPi = 3.14159265358979323846;  // const_pi, Header.h:91
+
+// polar_angle, Help_funk.cpp:78-107
+double polar_angle_h(double x, double y) {
+    if (std::fabs(x) + std::fabs(y) < 0.000001) return 0.0;
+    if (x < 0) return std::atan(y / x) + 1.0 * kConstPi;
+    if (x > 0 && y >= 0) return std::atan(y / x);
+    if (x > 0 && y < 0) return std::atan(y / x) + 2.0 * kConstPi;
+    if (y > 0 && x >= 0 && x <= 0) return kConstPi / 2.0;
+    if (y < 0 && x >= 0 && x <= 0) return 3.0 * kConstPi / 2.0;
+    return 0.0;
+}
+
+template <typename T>
+int upload(T **dst, const std::vector<T> &src, hipStream_t s) {
+    if (*dst) { hipFree(*dst); *dst = nullptr; }
+    if (hipMalloc((void **)dst, std::max<size_t>(1, src.size()) * sizeof(T)) != hipSuccess) return TITAN_E_HIP;
+    if (!src.empty() &&
+        hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess)
+        return TITAN_E_HIP;
+    return TITAN_OK;
+}
+
+// Minimum-norm least squares min |A x - b| for A [n x 3] (row-major) through a one-sided Jacobi
+// SVD; singular values below 3 eps times the largest count as zero (the rank rule of Eigen's
+// SVD solve, which Smooth_head_TS3 calls through bdcSvd).
+void lsq3(int64_t n, const std::vector<double> &A, const std::vector<double> &b, double x[3]) {
+    std::vector<double> U(A);  // columns rotated in place
+    double V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    const double eps = std::numeric_limits<double>::epsilon();
+    for (int sweep = 0; sweep < 60; sweep++) {
+        bool rotated = false;
+        for (int p = 0; p < 2; p++) {
+            for (int q = p + 1; q < 3; q++) {
+                double al = 0.0, be = 0.0, ga = 0.0;
+                for (int64_t i = 0; i < n; i++) {
+                    const double up = U[3 * i + p], uq = U[3 * i + q];
+                    al += up * up; be += uq * uq; ga += up * uq;
+                }
+                if (ga == 0.0 || std::fabs(ga) <= eps * std::sqrt(al * be)) continue;
+                rotated = true;
+                const double zeta = (be - al) / (2.0 * ga);
+                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = c * t;
+                for (int64_t i = 0; i < n; i++) {
+                    const double up = U[3 * i + p], uq = U[3 * i + q];
+                    U[3 * i + p] = c * up - s * uq;
+                    U[3 * i + q] = s * up + c * uq;
+                }
+                for (int k = 0; k < 3; k++) {
+                    const double vp = V[k][p], vq = V[k][q];
+                    V[k][p] = c * vp - s * vq;
+                    V[k][q] = s * vp + c * vq;
+                }
+            }
+        }
+        if (!rotated) break;
+    }
+    double sig[3], smax = 0.0;
+    for (int k = 0; k < 3; k++) {
+        double s2 = 0.0;
+        for (int64_t i = 0; i < n; i++) s2 += U[3 * i + k] * U[3 * i + k];
+        sig[k] = std::sqrt(s2);
+        smax = std::max(smax, sig[k]);
+    }
+    const double thr = std::max(smax * (3.0 * eps), std::numeric_limits<double>::min());
+    x[0] = x[1] = x[2] = 0.0;
+    for (int k = 0; k < 3; k++) {
+        if (!(sig[k] >= thr) || sig[k] == 0.0) continue;
+        double ub = 0.0;
+        for (int64_t i = 0; i < n; i++) ub += U[3 * i + k] * b[i];
+        const double coef = ub / (sig[k] * sig[k]);  // (u_k . b) / sigma_k with u_k = U_k / sigma_k
+        for (int j = 0; j < 3; j++) x[j] += coef * V[j][k];
+    }
+}
+
+}  // namespace
+
+// Setka_move.cpp:2257-2355: least squares for 2 p . c = |p|^2 over the points, then up to 8
+// reweighted passes (Tukey biweights of |dist - mean dist| with a cutoff of 2.5 standard
+// deviations) until the rms residual changes by less than 1e-6.
+int sphere_fit(int64_t n, const double *pts, double centre[3]) {
+    if (n < 3 || !pts || !centre) return TITAN_E_ARG;
+    std::vector<double> A(3 * n), b(n), WA(3 * n), Wb(n), w(n, 1.0), res(n);
+    for (int64_t i = 0; i < n; i++) {
+        const double *p = pts + 3 * i;
+        for (int k = 0; k < 3; k++) A[3 * i + k] = 2.0 * p[k];
+        b[i] = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+    }
+    double c[3];
+    lsq3(n, A, b, c);
+    double prev = std::numeric_limits<double>::max();
+    for (int iter = 0; iter < 8; iter++) {
+        double radius = 0.0;
+        for (int64_t i = 0; i < n; i++) {
+            const double *p = pts + 3 * i;
+            const double d0 = p[0] - c[0], d1 = p[1] - c[1], d2 = p[2] - c[2];
+            radius += std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        }
+        radius /= (double)n;
+        double total = 0.0;
+        for (int64_t i = 0; i < n; i++) {
+            const double *p = pts + 3 * i;
+            const double d0 = p[0] - c[0], d1 = p[1] - c[1], d2 = p[2] - c[2];
+            res[i] = std::fabs(std::sqrt(d0 * d0 + d1 * d1 + d2 * d2) - radius);
+            total += res[i] * res[i];
+        }
+        const double rms = std::sqrt(total / (double)n);
+        if (std::fabs(prev - rms) < 1e-6) break;
+        prev = rms;
+        double mean = 0.0;
+        for (int64_t i = 0; i < n; i++) mean += res[i];
+        mean /= (double)n;
+        double var = 0.0;
+        for (int64_t i = 0; i < n; i++) var += (res[i] - mean) * (res[i] - mean);
+        const double cutoff = 2.5 * std::sqrt(var / (double)n);
+        for (int64_t i = 0; i < n; i++) {
+            if (res[i] < cutoff) {
+                const double t = res[i] / cutoff;
+                w[i] = (1.0 - t * t) * (1.0 - t * t);
+            } else {
+                w[i] = 0.0;
+            }
+            for (int k = 0; k < 3; k++) WA[3 * i + k] = w[i] * A[3 * i + k];
+            Wb[i] = w[i] * b[i];
+        }
+        lsq3(n, WA, Wb, c);
+    }
+    centre[0] = c[0]; centre[1] = c[1]; centre[2] = c[2];
+    return TITAN_OK;
+}
+
+struct FitState {
+    int64_t nn = 0, nf = 0;
+    // surface faces in ascending reference id: kind 1 TS, 2 HP, 3 BS
+    std::vector<int32_t> sf_ref, sf_dev;
+    std::vector<uint8_t> sf_kind;
+    // surface nodes in ascending reference id
+    std::vector<int32_t> sn_ref, sn_dev, node_sn;
+    std::vector<uint8_t> sn_kind;
+    std::vector<int32_t> sn_face_off, sn_face;  // faces that move the node (indices into sf_*)
+    std::vector<int32_t> cand_off, cand;        // the node and the nodes of its faces of its kind
+    std::vector<std::vector<int32_t>> nb;       // smoothing neighbours (reference ids)
+    std::vector<double> centre;
+    int n_kind[4] = {0, 0, 0, 0};
+    bool smoothed[4] = {false, false, false, false};
+    bool bad = false;  // a face with two surface flags, or a node on two kinds of surface
+    // rays
+    bool have_rays = false;
+    int64_t n_rays = 0;
+    int32_t npr = 0;
+    double R0 = 0, R1 = 0, R5 = 0, da[5] = {0, 0, 0, 0, 0};
+    int32_t M[6] = {0, 0, 0, 0, 0, 0};
+    double phi_lim = 0.0;
+    std::vector<int32_t> ray_dev;
+    std::vector<double> ray_trig;
+    // device copies
+    bool up_static = false, up_nb = false, up_rays = false;
+    int32_t *d_sf_face = nullptr, *d_sn_node = nullptr, *d_sn_face_off = nullptr, *d_sn_face = nullptr;
+    int32_t *d_nb_off = nullptr, *d_nb = nullptr, *d_ray_nodes = nullptr;
+    uint8_t *d_sf_kind = nullptr, *d_sn_kind = nullptr;
+    double *d_sf_vel = nullptr, *d_sn_vel = nullptr, *d_centre = nullptr, *d_ray_trig = nullptr;
+    const DeviceOrder *ord = nullptr;
+};
+
+int fit_create(FitState **out, int64_t nn, int64_t nf, const int32_t *face_nodes, const int32_t *face_cells,
+               const int32_t *cell_faces, const uint8_t *face_flags, const DeviceOrder &o) {
+    FitState *F = new FitState();
+    F->nn = nn;
+    F->nf = nf;
+    F->ord = &o;
+    std::vector<uint8_t> kind(nf, 0);
+    if (face_flags) {
+        for (int64_t f = 0; f < nf; f++) {
+            const uint8_t fl = face_flags[f];
+            const int ts = (fl >> 3) & 1, hp = (fl >> 2) & 1, bs = (fl >> 4) & 1;
+            if (ts + hp + bs > 1) { F->bad = true; continue; }
+            // a fitted surface lies between two cells: boundary faces with a surface flag are not
+            // moved (the flag still selects the TVD rule of Snos_on_HP)
+            if (face_cells[2 * f + 1] >= 0) kind[f] = ts ? 1 : (hp ? 2 : (bs ? 3 : 0));
+        }
+    }
+    std::vector<int32_t> face_sf(nf, -1);
+    for (int64_t f = 0; f < nf; f++) {
+        if (!kind[f]) continue;
+        face_sf[f] = (int32_t)F->sf_ref.size();
+        F->sf_ref.push_back((int32_t)f);
+        F->sf_dev.push_back(o.face_r2d[f]);
+        F->sf_kind.push_back(kind[f]);
+    }
+    // HP faces with fewer than 4 surface neighbours add nothing (Setka_move.cpp:197); the
+    // neighbours are the same-surface faces of the cells beside cells[0] (macros4)
+    std::vector<uint8_t> moves(F->sf_ref.size(), 1);
+    for (size_t i = 0; i < F->sf_ref.size(); i++) {
+        if (F->sf_kind[i] != 2) continue;
+        const int64_t f = F->sf_ref[i];
+        const int64_t A = face_cells[2 * f];
+        int cnt = 0;
+        for (int j = 0; j < 6; j++) {
+            const int64_t g = cell_faces[6 * A + j];
+            if (g == f) continue;
+            const int64_t B = face_cells[2 * g] == A ? face_cells[2 * g + 1] : face_cells[2 * g];
+            if (B < 0) continue;
+            for (int k = 0; k < 6; k++) {
+                const int64_t h = cell_faces[6 * B + k];
+                if (kind[h] == 2 && h != f) cnt++;
+            }
+        }
+        if (cnt < 4) moves[i] = 0;
+    }
+    std::vector<uint8_t> nkind(nn, 0);
+    for (size_t i = 0; i < F->sf_ref.size(); i++) {
+        const int64_t f = F->sf_ref[i];
+        for (int k = 0; k < 4; k++) {
+            const int32_t v = face_nodes[4 * f + k];
+            if (nkind[v] && nkind[v] != F->sf_kind[i]) F->bad = true;
+            nkind[v] = F->sf_kind[i];
+        }
+    }
+    F->node_sn.assign(nn, -1);
+    for (int64_t v = 0; v < nn; v++) {
+        if (!nkind[v]) continue;
+        F->node_sn[v] = (int32_t)F->sn_ref.size();
+        F->sn_ref.push_back((int32_t)v);
+        F->sn_dev.push_back(o.node_r2d[v]);
+        F->sn_kind.push_back(nkind[v]);
+        F->n_kind[nkind[v]]++;
+    }
+    const size_t ns = F->sn_ref.size();
+    std::vector<std::vector<int32_t>> nf_list(ns), cand(ns);
+    for (size_t i = 0; i < F->sf_ref.size(); i++) {  // ascending reference face id
+        const int64_t f = F->sf_ref[i];
+        for (int k = 0; k < 4; k++) {
+            const int32_t s = F->node_sn[face_nodes[4 * f + k]];
+            if (moves[i]) nf_list[s].push_back((int32_t)i);
+            for (int q = 0; q < 4; q++) cand[s].push_back(face_nodes[4 * f + q]);
+        }
+    }
+    F->sn_face_off.assign(1, 0);
+    F->cand_off.assign(1, 0);
+    for (size_t s = 0; s < ns; s++) {
+        for (int32_t i : nf_list[s]) F->sn_face.push_back(i);
+        F->sn_face_off.push_back((int32_t)F->sn_face.size());
+        std::vector<int32_t> &c = cand[s];
+        c.push_back(F->sn_ref[s]);
+        std::sort(c.begin(), c.end());
+        c.erase(std::unique(c.begin(), c.end()), c.end());
+        for (int32_t v : c) F->cand.push_back(v);
+        F->cand_off.push_back((int32_t)F->cand.size());
+    }
+    F->nb.assign(ns, {});
+    F->centre.assign(3 * ns, 0.0);
+    *out = F;
+    return TITAN_OK;
+}
+
+void fit_destroy(FitState *F) {
+    if (!F) return;
+    void *p[] = {F->d_sf_face, F->d_sn_node, F->d_sn_face_off, F->d_sn_face, F->d_nb_off, F->d_nb, F->d_ray_nodes,
+                 F->d_sf_kind, F->d_sn_kind, F->d_sf_vel, F->d_sn_vel, F->d_centre, F->d_ray_trig};
+    for (void *q : p)
+        if (q) hipFree(q);
+    delete F;
+}
+
+bool fit_has_rays(const FitState *F) { return F && F->have_rays; }
+
+int fit_set_rays(FitState *F, const titan_rays *r, const DeviceOrder &o) {
+    if (!F || !r || r->n_rays < 1 || !r->ray_nodes || !r->ray_angles) return TITAN_E_ARG;
+    const int32_t M0 = r->M0, M1 = r->M1, M11 = r->M11, M2 = r->M2, M3 = r->M3, M4 = r->M4;
+    if (M0 < 1 || M1 < 0 || M11 < 0 || M2 < M11 || M3 < 0 || M4 < 0) return TITAN_E_ARG;
+    if (r->nodes_per_ray != M0 + M1 + M2 + M3 + M4 + 5) return TITAN_E_ARG;
+    const int64_t tot = r->n_rays * (int64_t)r->nodes_per_ray;
+    std::vector<uint8_t> seen(F->nn, 0);
+    std::vector<int32_t> dev(tot);
+    for (int64_t i = 0; i < tot; i++) {
+        const int32_t v = r->ray_nodes[i];
+        if (v < 0 || v >= F->nn || seen[v]) return TITAN_E_ARG;  // each node on one ray
+        seen[v] = 1;
+        dev[i] = o.node_r2d[v];
+    }
+    // ray-major on the device: node j of ray k at [k*npr + j]
+    F->ray_dev.swap(dev);
+    F->ray_trig.resize(4 * r->n_rays);
+    for (int64_t k = 0; k < r->n_rays; k++) {
+        const double the = r->ray_angles[2 * k], phi = r->ray_angles[2 * k + 1];
+        F->ray_trig[4 * k] = std::cos(the);
+        F->ray_trig[4 * k + 1] = std::sin(the);
+        F->ray_trig[4 * k + 2] = std::cos(phi);
+        F->ray_trig[4 * k + 3] = std::sin(phi);
+    }
+    F->n_rays = r->n_rays;
+    F->npr = r->nodes_per_ray;
+    F->R0 = r->R0; F->R1 = r->R1; F->R5 = r->R5;
+    F->da[0] = r->da1; F->da[1] = r->da2; F->da[2] = r->da3; F->da[3] = r->da4; F->da[4] = r->da5;
+    F->M[0] = M0; F->M[1] = M1; F->M[2] = M11; F->M[3] = M2; F->M[4] = M3; F->M[5] = M4;
+    F->phi_lim = r->phi_a + kConstPi / 180.0;
+    F->have_rays = true;
+    F->up_rays = false;
+    return TITAN_OK;
+}
+
+int fit_smooth(FitState *F, int kind, const double *x0) {
+    if (!F || kind < 1 || kind > 3 || !x0) return TITAN_E_ARG;
+    if (kind == 2 && !F->have_rays) return TITAN_E_ARG;  // phi_a comes with the rays
+    std::vector<double> pts;
+    for (size_t s = 0; s < F->sn_ref.size(); s++) {
+        if (F->sn_kind[s] != kind) continue;
+        const double *a = x0 + 3 * (int64_t)F->sn_ref[s];
+        if (kind == 2) {
+            const double phi = polar_angle_h(a[0], std::sqrt(0.0 * 0.0 + a[1] * a[1] + a[2] * a[2]));
+            if (phi > F->phi_lim) continue;
+        }
+        const int32_t c0 = F->cand_off[s], c1 = F->cand_off[s + 1];
+        if (c1 - c0 < 3) continue;
+        F->nb[s].clear();
+        pts.clear();
+        for (int32_t j = c0; j < c1; j++) {
+            const int32_t v = F->cand[j];
+            if (v != F->sn_ref[s]) F->nb[s].push_back(v);
+            for (int k = 0; k < 3; k++) pts.push_back(x0[3 * (int64_t)v + k]);
+        }
+        if (kind != 3) {
+            const int rc = sphere_fit(c1 - c0, pts.data(), &F->centre[3 * s]);
+            if (rc != TITAN_OK) return rc;
+        }
+    }
+    F->smoothed[kind] = true;
+    F->up_nb = false;
+    return TITAN_OK;
+}
+
+int fit_prepare(FitState *F, const titan_params &p, hipStream_t s, FitDev *dev) {
+    if (!F || !dev || F->bad) return TITAN_E_ARG;
+    const int32_t sg[4] = {0, p.sglag_TS, p.sglag_HP, p.sglag_BS};
+    for (int k = 1; k <= 3; k++)
+        if (sg[k] && F->n_kind[k] > 0 && !F->smoothed[k]) return TITAN_E_ARG;
+    int rc = TITAN_OK;
+    auto up = [&](int r) { if (rc == TITAN_OK) rc = r; };
+    if (!F->up_static) {
+        up(upload(&F->d_sf_face, F->sf_dev, s));
+        up(upload(&F->d_sf_kind, F->sf_kind, s));
+        up(upload(&F->d_sn_node, F->sn_dev, s));
+        up(upload(&F->d_sn_kind, F->sn_kind, s));
+        up(upload(&F->d_sn_face_off, F->sn_face_off, s));
+        up(upload(&F->d_sn_face, F->sn_face, s));
+        up(upload(&F->d_sf_vel, std::vector<double>(3 * F->sf_ref.size(), 0.0), s));
+        up(upload(&F->d_sn_vel, std::vector<double>(3 * F->sn_ref.size(), 0.0), s));
+        if (rc != TITAN_OK) return rc;
+        F->up_static = true;
+    }
+    if (!F->up_nb) {
+        std::vector<int32_t> off(1, 0), nbd;
+        for (size_t i = 0; i < F->nb.size(); i++) {
+            for (int32_t v : F->nb[i]) nbd.push_back(F->ord->node_r2d[v]);
+            off.push_back((int32_t)nbd.size());
+        }
+        up(upload(&F->d_nb_off, off, s));
+        up(upload(&F->d_nb, nbd, s));
+        up(upload(&F->d_centre, F->centre, s));
+        if (rc != TITAN_OK) return rc;
+        F->up_nb = true;
+    }
+    if (F->have_rays && !F->up_rays) {
+        up(upload(&F->d_ray_nodes, F->ray_dev, s));
+        up(upload(&F->d_ray_trig, F->ray_trig, s));
+        if (rc != TITAN_OK) return rc;
+        F->up_rays = true;
+    }
+    FitDev &D = *dev;
+    D = FitDev{};
+    D.n_sf = (int32_t)F->sf_ref.size();
+    D.n_sn = (int32_t)F->sn_ref.size();
+    D.sf_face = F->d_sf_face; D.sf_kind = F->d_sf_kind; D.sf_vel = F->d_sf_vel;
+    D.sn_node = F->d_sn_node; D.sn_kind = F->d_sn_kind;
+    D.sn_face_off = F->d_sn_face_off; D.sn_face = F->d_sn_face;
+    D.sn_nb_off = F->d_nb_off; D.sn_nb = F->d_nb; D.sn_centre = F->d_centre; D.sn_vel = F->d_sn_vel;
+    D.n_rays = F->have_rays ? F->n_rays : 0;
+    D.npr = F->npr;
+    D.ray_nodes = F->d_ray_nodes; D.ray_trig = F->d_ray_trig;
+    D.R0 = F->R0; D.R1 = F->R1; D.R5 = F->R5;
+    D.da1 = F->da[0]; D.da2 = F->da[1]; D.da3 = F->da[2]; D.da4 = F->da[3]; D.da5 = F->da[4];
+    D.M0 = F->M[0]; D.M1 = F->M[1]; D.M11 = F->M[2]; D.M2 = F->M[3]; D.M3 = F->M[4]; D.M4 = F->M[5];
+    D.move[1] = p.move_TS; D.move[2] = p.move_HP; D.move[3] = p.move_BS;
+    D.sglag[1] = p.sglag_TS; D.sglag[2] = p.sglag_HP; D.sglag[3] = p.sglag_BS;
+    D.velocity[1] = p.velocity_TS; D.velocity[2] = p.velocity_HP; D.velocity[3] = p.velocity_BS;
+    D.k_smooth[1] = p.sglag_TS_k_sphere; D.k_smooth[2] = p.sglag_HP_k_sphere; D.k_smooth[3] = p.sglag_BS_k;
+    D.phi_lim = F->phi_lim;
+    return TITAN_OK;
+}
+
+int fit_node_velocity(FitState *F, hipStream_t s, int64_t nn, double *v) {
+    if (!F || !v || nn != F->nn) return TITAN_E_ARG;
+    std::vector<double> h(3 * F->sn_ref.size());
+    if (!h.empty() && (!F->d_sn_vel || hipMemcpyAsync(h.data(), F->d_sn_vel, h.size() * sizeof(double),
+                                                      hipMemcpyDeviceToHost, s) != hipSuccess))
+        return TITAN_E_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return TITAN_E_HIP;
+    std::memset(v, 0, sizeof(double) * 3 * nn);
+    for (size_t i = 0; i < F->sn_ref.size(); i++)
+        for (int k = 0; k < 3; k++) v[3 * (int64_t)F->sn_ref[i] + k] = h[3 * i + k];
+    return TITAN_OK;
+}
+
+}  // namespace titan

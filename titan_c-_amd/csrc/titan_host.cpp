@@ -19,6 +19,7 @@
 #include "launch.h"
 #include "mlp.h"
 #include "titan_mesh.h"
+#include "titan_fit.h"
 
 using namespace titan;
 
@@ -84,6 +85,7 @@ struct titan_ctx {
     bool timing = false;
     Timer tm;
     MlpState mlp;
+    FitState *fit = nullptr;  // fitted surfaces and rays (move_mesh 2)
     DevMesh mesh() const {
         DevMesh M;
         M.n_nodes = nn; M.n_cells = nc; M.n_faces = nf;
@@ -218,7 +220,7 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
         !m->node_xyz)
         return TITAN_E_ARG;
     if (m->n_cells > INT32_MAX || m->n_faces > INT32_MAX || m->n_nodes > INT32_MAX) return TITAN_E_ARG;
-    if (p->move_mesh && !m->node_disp) return TITAN_E_ARG;
+    if (p->move_mesh < 0 || p->move_mesh > 2 || (p->move_mesh == 1 && !m->node_disp)) return TITAN_E_ARG;
     if (p->source_mode < TITAN_SRC_NONE || p->source_mode > TITAN_SRC_BERA) return TITAN_E_ARG;
     if (p->source_mode == TITAN_SRC_BERA && !(p->par_Kn > 0.0)) return TITAN_E_ARG;
     const int64_t nn = m->n_nodes, nc = m->n_cells, nf = m->n_faces;
@@ -257,6 +259,10 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
     MeshView mv{nn, nc, nf, m->cell_nodes, m->face_cells, m->node_xyz};
     build_order(mv, c->ord);
     const DeviceOrder &o = c->ord;
+    {
+        const int frc = fit_create(&c->fit, nn, nf, m->face_nodes, m->face_cells, m->cell_faces, m->face_flags, o);
+        if (frc != TITAN_OK) { titan_destroy(c); return frc; }
+    }
 
     std::vector<int32_t> cell_nodes(8 * nc), cell_faces(6 * nc), face_nodes(4 * nf), face_cells(2 * nf);
     std::vector<int32_t> face_bidx(nf, -1), face_ref(nf), cell_ref(nc);
@@ -418,6 +424,7 @@ extern "C" int titan_destroy(titan_ctx *c) {
     for (void *q : ptrs)
         if (q) hipFree(q);
     mlp_free(c->mlp);
+    fit_destroy(c->fit);
     for (auto e : c->tm.pool) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -624,12 +631,27 @@ extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
         if (rc != TITAN_OK) return rc;
     }
     const int nodes = c->params.cx_nodes > 0 ? c->params.cx_nodes : 100;
+    // fitted surfaces: Culc_Velocity_surface and dvigenie of every ray write the new node layer
+    // before each step (Setka_physics.cpp:1616-1627)
+    const bool fitting = c->params.move_mesh == 2;
+    FitDev fd{};
+    if (fitting) {
+        const int rc = fit_prepare(c->fit, c->params, c->stream, &fd);
+        if (rc != TITAN_OK) return rc;
+    }
     for (int64_t s = 0; s < steps; s++) {
         c->now2 = c->now1;
         c->now1 = (c->now1 + 1) % 2;
         c->step++;
         const int n1 = c->now1, n2 = c->now2;
         launch_begin_step(c->stream, c->d_S, c->step, c->dp);
+        if (fitting) {
+            timer_mark(c, 2, true);
+            launch_fit_surface(c->stream, M, c->dp, fd, c->d_prim[n1], c->d_X[n1], c->d_X[n2],
+                               c->geom[n1].cell_center, c->d_S, 0.0);
+            launch_fit_rays(c->stream, fd, c->d_X[n2]);
+            timer_mark(c, 2, false);
+        }
         const double *src = nullptr;
         if (c->params.source_mode != TITAN_SRC_NONE && !fused) {
             timer_mark(c, 3, true);
@@ -695,6 +717,66 @@ extern "C" int titan_init_tvd(titan_ctx *c) {
     rec_face.resize(rec_ref.size());
     for (size_t r = 0; r < rec_ref.size(); r++) rec_face[r] = c->ord.face_r2d[rec_ref[r]];
     return setup_tvd(c, cell_faces, face_cells, cell_code, face_code, rec_face, layer);
+}
+
+extern "C" int titan_set_rays(titan_ctx *c, const titan_rays *rays) {
+    if (!c) return TITAN_E_ARG;
+    return fit_set_rays(c->fit, rays, c->ord);
+}
+
+// node layer 0 in reference numbering (Yzel::coord[0], which the set-up functions read)
+static int nodes_layer0(titan_ctx *c, std::vector<double> &x) {
+    x.resize(3 * c->nn);
+    return titan_get_nodes(c, x.data(), 0);
+}
+
+static int fit_smooth_ctx(titan_ctx *c, int kind) {
+    if (!c) return TITAN_E_ARG;
+    std::vector<double> x;
+    const int rc = nodes_layer0(c, x);
+    if (rc != TITAN_OK) return rc;
+    return fit_smooth(c->fit, kind, x.data());
+}
+
+extern "C" int titan_smooth_head_ts3(titan_ctx *c) { return fit_smooth_ctx(c, 1); }
+extern "C" int titan_smooth_head_hp3(titan_ctx *c) { return fit_smooth_ctx(c, 2); }
+extern "C" int titan_find_yzel_sosed_for_bs(titan_ctx *c) { return fit_smooth_ctx(c, 3); }
+
+extern "C" int titan_culc_velocity_surface(titan_ctx *c, int now, double time, double *node_velocity) {
+    if (!c || now < 0 || now > 1 || !(time > 0.0)) return TITAN_E_ARG;
+    FitDev fd{};
+    int rc = fit_prepare(c->fit, c->params, c->stream, &fd);
+    if (rc != TITAN_OK) return rc;
+    if (init_scalars(c->d_S2) != TITAN_OK) return TITAN_E_HIP;
+    // the face and cell geometry of layer `now` (Setka::Go measures it after each move)
+    launch_measure(c->stream, c->mesh(), c->d_X[now], c->geom[now], c->params.geom_mode);
+    launch_fit_surface(c->stream, c->mesh(), c->dp, fd, c->d_prim[now], c->d_X[now], c->d_X[1 - now],
+                       c->geom[now].cell_center, c->d_S2, time);
+    HIPCHK(hipGetLastError());
+    StepScalars hs;
+    rc = check_scalars(c, c->d_S2, &hs);
+    c->tiles.vol_cached = 0;
+    if (rc != TITAN_OK) return rc;
+    if (node_velocity) return fit_node_velocity(c->fit, c->stream, c->nn, node_velocity);
+    return TITAN_OK;
+}
+
+extern "C" int titan_dvigenie(titan_ctx *c, int layer) {
+    if (!c || layer < 0 || layer > 1 || !fit_has_rays(c->fit)) return TITAN_E_ARG;
+    FitDev fd{};
+    titan_params p = c->params;
+    p.sglag_TS = p.sglag_HP = p.sglag_BS = 0;  // the rays need no smoothing data
+    const int rc = fit_prepare(c->fit, p, c->stream, &fd);
+    if (rc != TITAN_OK) return rc;
+    launch_fit_rays(c->stream, fd, c->d_X[layer]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->tiles.vol_cached = 0;
+    return TITAN_OK;
+}
+
+extern "C" int titan_sphere_fit(int64_t n, const double *points, double *centre) {
+    return sphere_fit(n, points, centre);
 }
 
 static const char *const kPlasmaKeys[8] = {"rho", "Vx", "Vy", "Vz", "p", "Bx", "By", "Bz"};
