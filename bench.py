@@ -56,10 +56,16 @@ def algorithmic_bytes(n_cells, n_faces, n_nodes, moving, with_neutrals=False):
     return float(b)
 
 
-def run_variant(T, mesh, prim, params, steps, warmup, neutrals=None, mlp=None, keep_state=False, h_fluids=None):
+def run_variant(T, mesh, prim, params, steps, warmup, neutrals=None, mlp=None, keep_state=False, h_fluids=None,
+                rays=None):
     s = T.Setka(mesh, params)
     try:
         s.set_state(prim, 0)
+        if rays is not None:  # fitted surfaces: rays and the smoothing set-up of TITAN.cpp:108-112
+            s.set_rays(rays)
+            s.Smooth_head_TS3()
+            s.Smooth_head_HP3()
+            s.Find_Yzel_Sosed_for_BS()
         if neutrals is not None:
             s.set_neutrals(neutrals)
         if h_fluids is not None:
@@ -73,7 +79,7 @@ def run_variant(T, mesh, prim, params, steps, warmup, neutrals=None, mlp=None, k
         st = s.Go(steps)  # returns after the stream has drained
         t1 = time.perf_counter()
         out = dict(seconds=t1 - t0, steps=steps, stats=st, step_kernel_ms=s.kernel_ms(0),
-                   source_kernel_ms=s.kernel_ms(3), tiles=s.tile_info())
+                   source_kernel_ms=s.kernel_ms(3), motion_kernel_ms=s.kernel_ms(2), tiles=s.tile_info())
         if keep_state:
             out["state"] = s.get_state()
         return out
@@ -199,7 +205,8 @@ def main():
     ap.add_argument("--exact-steps", type=int, default=3)
     ap.add_argument("--train-states", type=int, default=2_000_000)
     ap.add_argument("--precision-steps", type=int, default=1000)
-    ap.add_argument("--variants", default="fp64,mixed,fp64_exact,mixed_surrogate,fp64_tvd,mixed_tvd,fp64_bera,checkpoint")
+    ap.add_argument("--variants",
+                    default="fp64,mixed,fp64_exact,mixed_surrogate,fp64_tvd,mixed_tvd,fp64_bera,fp64_fit,checkpoint")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the streaming copy the HBM counter runs are scaled by, then exit")
     a = ap.parse_args()
@@ -271,6 +278,21 @@ def main():
         log("fp64 + closed-form source: %.3f ms/step (source kernel %.3f ms)" % (
             res["fp64_bera"]["seconds"] * 1e3 / a.steps, res["fp64_bera"]["source_kernel_ms"]))
         del hf
+    if "fp64_fit" in variants:  # the same cell count with the TS/HP/BS fitted and the rays moved
+        fmesh, fprim, fparams, frays = C.build("shell4m_fitted")
+        res["fp64_fit"] = run_variant(T, fmesh, fprim, dict(fparams, precision=T.PREC_FP64), a.steps, a.warmup,
+                                      rays=frays)
+        log("fp64 + fitted surfaces (%d cells): %.3f ms/step, step kernel %.3f ms, surface+ray kernels %.3f ms" % (
+            fmesh.n_cells, res["fp64_fit"]["seconds"] * 1e3 / a.steps, res["fp64_fit"]["step_kernel_ms"],
+            res["fp64_fit"]["motion_kernel_ms"]))
+        fit_info = {"workload": "shell4m_fitted", "n_cells": fmesh.n_cells,
+                    "ms_per_step": res["fp64_fit"]["seconds"] * 1e3 / a.steps,
+                    "step_kernel_ms": res["fp64_fit"]["step_kernel_ms"],
+                    "surface_and_ray_kernels_ms": res["fp64_fit"]["motion_kernel_ms"],
+                    "tiles": res["fp64_fit"]["tiles"]}
+        del fmesh, fprim
+    else:
+        fit_info = None
     ckpt = checkpoint_timing(T, mesh, prim, params) if "checkpoint" in variants else None
 
     def cups(name):
@@ -278,7 +300,7 @@ def main():
         return None if r is None else nc * r["steps"] / r["seconds"]
 
     vals = {k: cups(k) for k in ("fp64", "mixed", "fp64_exact", "mixed_surrogate", "fp64_tvd", "mixed_tvd",
-                                 "fp64_bera")}
+                                 "fp64_bera", "fp64_fit")}
     ratios = {}
     if vals["mixed_surrogate"] and vals["fp64_exact"]:
         ratios["surrogate_vs_exact_source"] = vals["mixed_surrogate"] / vals["fp64_exact"]
@@ -339,6 +361,7 @@ def main():
         "surrogate": sur_info,
         "precision": prec,
         "checkpoint": ckpt,
+        "fitted_surfaces": fit_info,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -397,6 +397,8 @@ WORKLOADS = {
     "shell1m": dict(fn="shell", n_edge=37, n_layers=120),
     "shell1m_moving": dict(fn="shell", n_edge=37, n_layers=120, moving=True),
     "shell4m_moving": dict(fn="shell", n_edge=53, n_layers=240, moving=True),
+    # 240 cell layers on 6*53^2 rays: TS, HP and BS fitted (move_mesh 2)
+    "shell4m_fitted": dict(fn="shell_fitted", n_edge=53, M=(10, 70, 6, 70, 50, 36), R5_au=1500.0),
 }
 
 
@@ -406,4 +408,6 @@ def build(name, **over):
     fn = spec.pop("fn")
     if fn == "cube":
         return cube_brio_wu(**spec)
+    if fn == "shell_fitted":
+        return shell_fitted(**spec)  # (mesh, prim, params, rays)
     return shell(**spec)

@@ -73,6 +73,9 @@ struct FitDev {
     int32_t npr;                 // nodes per ray, inner to outer sphere
     const int32_t *ray_nodes;    // [n_rays*npr] device nodes
     const double *ray_trig;      // [n_rays*4] cos(the), sin(the), cos(phi), sin(phi)
+    int64_t n_nodes;
+    const int2 *node_ray;        // [n_nodes] (ray, index on the ray) of each device node; ray -1: none
+    double *ray_R;               // [n_rays*3] TS, HP, BS distances from the origin (func_R)
     double R0, R1, R5, da1, da2, da3, da4, da5;
     int32_t M0, M1, M11, M2, M3, M4;
     int32_t move[4], sglag[4];

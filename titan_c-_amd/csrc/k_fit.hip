@@ -8,7 +8,8 @@
 //   k_fit_nodes   one thread per surface node: the average of its faces' vectors and of the sphere
 //                 (TS, HP) or neighbour-mean (BS) smoothing term, then the move along the radius
 //                 (HP behind the Sun: along the cylinder radius);
-//   k_fit_rays    one thread per ray node: the A_Luch distribution between the moved surfaces.
+//   k_fit_ray_radii, k_fit_rays  each ray's moved TS/HP/BS radii, then one thread per device node:
+//                 the A_Luch distribution between them.
 // The reference adds the face vectors into each node under a mutex from an OpenMP loop, so its
 // summation order varies from run to run; here a node sums them in ascending reference face id,
 // which makes the motion repeat bit for bit.
@@ -132,33 +133,43 @@ __global__ void __launch_bounds__(256) k_fit_nodes(FitDev F, const double *__res
     }
 }
 
-// Luch::dvigenie, A_Luch / A2_Luch (Luch.cpp:8-151).  Node j of a ray: the inner segment up to
-// R1 (j < M0), then to the TS, a uniform run of M11 past it, the run to the HP, to the BS and to
-// R5.  The ray's nodes M0 (at R1) and its TS, HP and BS nodes are not written.
+// func_R (Yzel.cpp:27-30) of each ray's TS, HP and BS node: the radii Luch::dvigenie reads.
+__global__ void __launch_bounds__(256) k_fit_ray_radii(FitDev F, const double *__restrict__ X) {
+    const int64_t ray = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ray >= F.n_rays) return;
+    const int iTS = F.M0 + F.M1 + 1, iHP = iTS + F.M2 + 1, iBS = iHP + F.M3 + 1;
+    const int32_t *rn = F.ray_nodes + ray * F.npr;
+    const int idx[3] = {iTS, iHP, iBS};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double *p = X + 3 * (int64_t)rn[idx[k]];
+        F.ray_R[3 * ray + k] = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    }
+}
+
+// Luch::dvigenie, A_Luch / A2_Luch (Luch.cpp:8-151), one thread per device node (coalesced
+// writes).  Node j of a ray: the inner segment up to R1 (j < M0), then to the TS, a uniform run of
+// M11 past it, the run to the HP, to the BS and to R5.  The ray's node M0 (at R1) and its TS, HP
+// and BS nodes are not written, nor are nodes on no ray.
 __global__ void __launch_bounds__(256) k_fit_rays(FitDev F, double *__restrict__ X) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= F.n_rays * (int64_t)F.npr) return;
-    const int64_t ray = t % F.n_rays;
-    const int j = (int)(t / F.n_rays);
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= F.n_nodes) return;
+    const int2 nr = F.node_ray[g];
+    if (nr.x < 0) return;
+    const int64_t ray = nr.x;
+    const int j = nr.y;
     const int iTS = F.M0 + F.M1 + 1, iHP = iTS + F.M2 + 1, iBS = iHP + F.M3 + 1;
     if (j == F.M0 || j == iTS || j == iHP || j == iBS) return;
-    const int32_t *rn = F.ray_nodes + ray * F.npr;
-    auto radius = [&](int k) {
-        const double *p = X + 3 * (int64_t)rn[k];
-        return sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
-    };
+    const double R2 = F.ray_R[3 * ray], R3 = F.ray_R[3 * ray + 1], R4 = F.ray_R[3 * ray + 2];
     double r;
     if (j < F.M0) {
         r = F.R0 + j * (F.R1 - F.R0) / F.M0;
     } else if (j < iTS) {
-        const double R2 = radius(iTS);
         r = F.R1 + (j - F.M0) * (R2 - F.R1) / (F.M1 + 1);
     } else if (j <= iTS + F.M11) {
-        const double R2 = radius(iTS);
         const double dr = (R2 - F.R1) / (F.M1 + 1);
         r = R2 + dr * (j - iTS);
     } else if (j < iHP) {
-        const double R2 = radius(iTS), R3 = radius(iHP);
         const double dr = (R2 - F.R1) / (F.M1 + 1);
         const double a = F.da1, b = F.da2;
         const double s = (j - iTS - F.M11 + 0.0) / (F.M2 - F.M11 + 1.0);
@@ -168,13 +179,11 @@ __global__ void __launch_bounds__(256) k_fit_rays(FitDev F, double *__restrict__
                                s * (7.0 + b * (w * w) * (-1.0 + s) - 2.0 * s * (9.0 + 2.0 * s * (-5.0 + 2.0 * s))));
         r = R2 + dr * F.M11 + (R3 - R2 - dr * F.M11) * ss;
     } else if (j < iBS) {
-        const double R3 = radius(iHP), R4 = radius(iBS);
         const double a = F.da3, b = F.da4;
         const double s = (j - iHP + 0.0) / (F.M3 + 1.0);
         const double ss = s * (a - (2.0 * a + b - 3.0) * s + (a + b - 2.0) * s * s);
         r = R3 + (R4 - R3) * ss;
     } else if (j < iBS + 1 + F.M4) {
-        const double R4 = radius(iBS);
         const double a = F.da5, b = 2.0;
         const double s = (j - iBS + 0.0) / (F.M4 + 1.0);
         const double ss = a * s - (3.0 * a + b - 4.0) * s * s * s + (2.0 * a + b - 3.0) * s * s * s * s;
@@ -183,7 +192,6 @@ __global__ void __launch_bounds__(256) k_fit_rays(FitDev F, double *__restrict__
         r = F.R5;
     }
     const double *tr = F.ray_trig + 4 * ray;
-    const int64_t g = rn[j];
     X[3 * g] = r * tr[0];
     X[3 * g + 1] = r * tr[1] * tr[2];
     X[3 * g + 2] = r * tr[1] * tr[3];
@@ -198,8 +206,9 @@ void launch_fit_surface(hipStream_t s, const DevMesh &M, const DevParams &P, con
 }
 
 void launch_fit_rays(hipStream_t s, const FitDev &F, double *X) {
-    const int64_t n = F.n_rays * (int64_t)F.npr;
-    if (n > 0) hipLaunchKernelGGL(k_fit_rays, dim3(grid_of(n, 256)), dim3(256), 0, s, F, X);
+    if (F.n_rays <= 0) return;
+    hipLaunchKernelGGL(k_fit_ray_radii, dim3(grid_of(F.n_rays, 256)), dim3(256), 0, s, F, X);
+    hipLaunchKernelGGL(k_fit_rays, dim3(grid_of(F.n_nodes, 256)), dim3(256), 0, s, F, X);
 }
 
 }  // namespace titan

@@ -172,10 +172,13 @@ struct FitState {
     double phi_lim = 0.0;
     std::vector<int32_t> ray_dev;
     std::vector<double> ray_trig;
+    std::vector<int2> node_ray;  // device node -> (ray, index), ray -1 off the rays
     // device copies
     bool up_static = false, up_nb = false, up_rays = false;
     int32_t *d_sf_face = nullptr, *d_sn_node = nullptr, *d_sn_face_off = nullptr, *d_sn_face = nullptr;
     int32_t *d_nb_off = nullptr, *d_nb = nullptr, *d_ray_nodes = nullptr;
+    int2 *d_node_ray = nullptr;
+    double *d_ray_R = nullptr;
     uint8_t *d_sf_kind = nullptr, *d_sn_kind = nullptr;
     double *d_sf_vel = nullptr, *d_sn_vel = nullptr, *d_centre = nullptr, *d_ray_trig = nullptr;
     const DeviceOrder *ord = nullptr;
@@ -275,7 +278,8 @@ int fit_create(FitState **out, int64_t nn, int64_t nf, const int32_t *face_nodes
 void fit_destroy(FitState *F) {
     if (!F) return;
     void *p[] = {F->d_sf_face, F->d_sn_node, F->d_sn_face_off, F->d_sn_face, F->d_nb_off, F->d_nb, F->d_ray_nodes,
-                 F->d_sf_kind, F->d_sn_kind, F->d_sf_vel, F->d_sn_vel, F->d_centre, F->d_ray_trig};
+                 F->d_sf_kind, F->d_sn_kind, F->d_sf_vel, F->d_sn_vel, F->d_centre, F->d_ray_trig, F->d_node_ray,
+                 F->d_ray_R};
     for (void *q : p)
         if (q) hipFree(q);
     delete F;
@@ -297,7 +301,10 @@ int fit_set_rays(FitState *F, const titan_rays *r, const DeviceOrder &o) {
         seen[v] = 1;
         dev[i] = o.node_r2d[v];
     }
-    // ray-major on the device: node j of ray k at [k*npr + j]
+    // ray-major on the device: node j of ray k at [k*npr + j]; and each device node's (ray, j)
+    F->node_ray.assign(F->nn, make_int2(-1, -1));
+    for (int64_t i = 0; i < tot; i++)
+        F->node_ray[dev[i]] = make_int2((int)(i / r->nodes_per_ray), (int)(i % r->nodes_per_ray));
     F->ray_dev.swap(dev);
     F->ray_trig.resize(4 * r->n_rays);
     for (int64_t k = 0; k < r->n_rays; k++) {
@@ -382,6 +389,8 @@ int fit_prepare(FitState *F, const titan_params &p, hipStream_t s, FitDev *dev) 
     if (F->have_rays && !F->up_rays) {
         up(upload(&F->d_ray_nodes, F->ray_dev, s));
         up(upload(&F->d_ray_trig, F->ray_trig, s));
+        up(upload(&F->d_node_ray, F->node_ray, s));
+        up(upload(&F->d_ray_R, std::vector<double>(3 * F->n_rays, 0.0), s));
         if (rc != TITAN_OK) return rc;
         F->up_rays = true;
     }
@@ -396,6 +405,7 @@ int fit_prepare(FitState *F, const titan_params &p, hipStream_t s, FitDev *dev) 
     D.n_rays = F->have_rays ? F->n_rays : 0;
     D.npr = F->npr;
     D.ray_nodes = F->d_ray_nodes; D.ray_trig = F->d_ray_trig;
+    D.n_nodes = F->nn; D.node_ray = F->d_node_ray; D.ray_R = F->d_ray_R;
     D.R0 = F->R0; D.R1 = F->R1; D.R5 = F->R5;
     D.da1 = F->da[0]; D.da2 = F->da[1]; D.da3 = F->da[2]; D.da4 = F->da[3]; D.da5 = F->da[4];
     D.M0 = F->M[0]; D.M1 = F->M[1]; D.M11 = F->M[2]; D.M2 = F->M[3]; D.M3 = F->M[4]; D.M4 = F->M[5];

@@ -321,8 +321,10 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
     if (!disp.empty()) up(dev_upload(&c->d_disp, disp, c->stream));
     if (rc != TITAN_OK) { titan_destroy(c); return rc; }
     {
-        // tile size: the largest of 128, 112, ... cells whose LDS footprint fits the device's
-        // per-workgroup limit (TITAN_TILE_CELLS overrides the starting size)
+        // tile size: the largest of 128, 112, ... cells whose LDS footprint lets two workgroups
+        // share a CU (the step kernel is built for 2 waves per SIMD, and a CU's LDS equals the
+        // per-workgroup limit); failing that, the largest that fits one workgroup
+        // (TITAN_TILE_CELLS overrides the starting size)
         int lds_limit = 65536;
         {
             int dev = 0, v = 0;
@@ -330,19 +332,23 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
                 hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
                 lds_limit = v;
         }
-        int tile_cells = 128;
-        if (const char *e = std::getenv("TITAN_TILE_CELLS")) tile_cells = std::max(16, std::min(1024, std::atoi(e)));
+        int tile_start = 128;
+        if (const char *e = std::getenv("TITAN_TILE_CELLS")) tile_start = std::max(16, std::min(1024, std::atoi(e)));
         TileData td;
         bool fits = false;
-        for (; tile_cells >= 16; tile_cells -= 16) {
-            if (!build_tiles(nc, nf, nn, tile_cells, cell_nodes.data(), cell_faces.data(), face_nodes.data(),
-                             face_cells.data(), face_code.data(), face_bidx.data(), face_ref.data(), td))
-                continue;
-            TileMesh probe{};
-            probe.tile_cells = td.tile_cells;
-            probe.max_nodes = td.max_nodes;
-            probe.max_faces = td.max_faces;
-            if (step_tile_lds_bytes(probe, p->move_mesh != 0) <= (size_t)lds_limit) { fits = true; break; }
+        for (int pass = 0; pass < 2 && !fits; pass++) {
+            // 256 bytes stay free for the kernel's static LDS (the dt reduction's wavefront minima)
+            const size_t budget = pass == 0 ? (size_t)lds_limit / 2 - 256 : (size_t)lds_limit - 256;
+            for (int tile_cells = tile_start; tile_cells >= 16; tile_cells -= 16) {
+                if (!build_tiles(nc, nf, nn, tile_cells, cell_nodes.data(), cell_faces.data(), face_nodes.data(),
+                                 face_cells.data(), face_code.data(), face_bidx.data(), face_ref.data(), td))
+                    continue;
+                TileMesh probe{};
+                probe.tile_cells = td.tile_cells;
+                probe.max_nodes = td.max_nodes;
+                probe.max_faces = td.max_faces;
+                if (step_tile_lds_bytes(probe, p->move_mesh != 0) <= budget) { fits = true; break; }
+            }
         }
         if (!fits) { titan_destroy(c); return TITAN_E_TOPOLOGY; }
         up(dev_upload(&c->d_t_node_off, td.node_off, c->stream));
