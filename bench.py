@@ -201,7 +201,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="shell4m_moving")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--exact-steps", type=int, default=3)
     ap.add_argument("--train-states", type=int, default=2_000_000)
     ap.add_argument("--precision-steps", type=int, default=1000)

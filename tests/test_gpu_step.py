@@ -138,3 +138,33 @@ def test_gcl_free_stream(T, C):
     got, xyz = _run_gpu(T, mesh, prim, params, 20)
     assert np.abs(xyz - mesh.node_xyz).max() > 1e-4
     assert rel_err(got, np.broadcast_to(u, got.shape), 1e-30).max() < 1e-11
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_nan_state_stops_where_the_reference_stops(T, C, O, precision):
+    """A NaN density reaches chlld's fatal branch on the faces of its cell before any new density is
+    formed; the reference's face loop runs before its cell loop (Setka_physics.cpp:1634-1671) and
+    exits at the first failing face, so the step stops with TITAN_E_RIEMANN and the smallest failing
+    face id -- whichever workgroup runs first (errors are ranked, face loop before cell loop)."""
+    mesh, prim, params = C.cube_brio_wu(n=8)
+    prim = prim.copy()
+    bad_cell = 37
+    prim[bad_cell, 0] = np.nan
+    om = O.OracleMesh(mesh)
+    g = O.measure(om, mesh.node_xyz)
+    failing = []
+    for f in np.nonzero((mesh.face_cells == bad_cell).any(axis=1))[0]:
+        a, b = mesh.face_cells[f]
+        qL = prim[a]
+        qR = prim[b] if b >= 0 else prim[a]
+        *_, rc = O.chlld_batch(int(mesh.face_method[f]), g["face_normal"][f][None, :], qL[None, :], qR[None, :])
+        if rc != 0:
+            failing.append(int(f))
+    assert failing, "a NaN density reaches chlld's fatal branch"
+    for _ in range(2):
+        with T.Setka(mesh, dict(params, precision=precision)) as s:
+            s.set_state(prim, 0)
+            with pytest.raises(T.TitanError) as e:
+                s.Go(1)
+        assert e.value.code == T.TITAN_E_RIEMANN
+        assert e.value.index == min(failing)

@@ -84,6 +84,18 @@ struct FitDev {
     double phi_lim;              // HP sphere smoothing where the polar angle is <= phi_a + pi/180
 };
 
+// Failures of a step, ranked in the order the reference meets them -- the surface fit before the
+// face loop (Setka_physics.cpp:1616-1627), the face loop before the cell loop -- so that the
+// reported one does not depend on which workgroup ran first; within a rank the smallest reference
+// id wins (one 64-bit atomicMin of rank << 48 | id).
+enum ErrRank : unsigned {
+    kErrFitContact = 0,     // NaN / subnormal heliopause contact speed (TITAN_E_SURFACE)
+    kErrFitRiemann = 1,     // chlld's fatal branch in the surface fit (TITAN_E_RIEMANN)
+    kErrFaceRiemann = 2,    // chlld's fatal branch or an unknown boundary code in the face loop
+    kErrCellNonfinite = 3,  // NaN density after the update (TITAN_E_NONFINITE_STATE)
+    kErrCellFallback = 4,   // surrogate fallback list full (TITAN_E_ARG)
+};
+
 // Device-side scalars of the step loop (one instance per context).
 struct StepScalars {
     double dt;                       // dt of the current step (lagged, Setka_physics.cpp:1606)
@@ -93,9 +105,7 @@ struct StepScalars {
     unsigned long long dtmin_bits[2];  // running min of the CFL step, slot = step & 1
     unsigned long long n_rho_floor, n_p_floor, n_fallback;
     unsigned long long n_fp64_faces;  // faces the mixed solver handed to the FP64 solver
-    int err_code;
-    int pad1;
-    long long err_index;             // smallest reference id that failed, or LLONG_MAX
+    unsigned long long err_key;      // first failure: rank << 48 | reference id (ErrRank), ~0 = none
 };
 
 }  // namespace titan

@@ -39,10 +39,11 @@ __global__ void __launch_bounds__(256) k_fit_faces(DevMesh M, DevParams P, FitDe
     // (Setka_move.cpp:171-175, 328-333, 422-426)
     RiemannOut<double, 0> o;
     chlld<double, 0>((int)((code >> 4) & 15u), 0, 0, n[0], n[1], n[2], 0.0, qL, qR, nullptr, nullptr, o);
-    if (!o.ok || !okbc) report_error(S, -3, M.face_ref[f]);
+    if (!o.ok || !okbc) report_error(S, kErrFitRiemann, M.face_ref[f]);
     const double ds = kind == 1 ? o.dsl : (kind == 2 ? o.dsc : o.dsr);
     // Setka_move.cpp:336-350: a NaN or subnormal contact speed stops the reference
-    if (kind == 2 && (isnan(ds) || (ds != 0.0 && fabs(ds) < 2.2250738585072014e-308))) report_error(S, -8, M.face_ref[f]);
+    if (kind == 2 && (isnan(ds) || (ds != 0.0 && fabs(ds) < 2.2250738585072014e-308)))
+        report_error(S, kErrFitContact, M.face_ref[f]);
     const double s = F.velocity[kind] * ds;
     F.sf_vel[3 * i] = s * n[0];
     F.sf_vel[3 * i + 1] = s * n[1];

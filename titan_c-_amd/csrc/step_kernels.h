@@ -398,9 +398,8 @@ __device__ __forceinline__ bool face_states(const DevMesh &M, const DevParams &P
     return face_states_at(M, P, f, bc, prim, G.cell_center, Gc, n, qL, qR);
 }
 
-__device__ __forceinline__ void report_error(StepScalars *S, int code, long long index) {
-    atomicCAS(&S->err_code, 0, code);
-    atomicMin(&S->err_index, index);
+__device__ __forceinline__ void report_error(StepScalars *S, ErrRank rank, long long index) {
+    atomicMin(&S->err_key, ((unsigned long long)rank << 48) | (unsigned long long)index);
 }
 
 // Min over a block of positive doubles: 64-lane wavefront shuffle, then LDS across waves, then
@@ -467,7 +466,7 @@ __global__ void __launch_bounds__(256) k_face_flux(DevMesh M, DevParams P, const
             for (int i = 0; i < 8; i++) q[i] = o.q[i];
             dsl = o.dsl; dsr = o.dsr; br = o.branch; ok = o.ok;
         }
-        if (!ok || !okbc) report_error(S, -3, M.face_ref[f]);
+        if (!ok || !okbc) report_error(S, kErrFaceRiemann, M.face_ref[f]);
         if (bc == 0 && ok) {
             const int64_t A = M.face_cells[2 * f], B = M.face_cells[2 * f + 1];
             const double d0 = G.cell_center[3 * A] - G.cell_center[3 * B];
@@ -533,7 +532,7 @@ __device__ __forceinline__ void cell_update_state(const DevMesh &M, uint32_t cod
         u3 = vx; v3 = vy; w3 = vz; bx3 = bx; by3 = by; bz3 = bz;
         p3 = rho3 / 2.0;
     }
-    if (isnan(rho3)) report_error(S, -4, M.cell_ref[c]);
+    if (isnan(rho3)) report_error(S, kErrCellNonfinite, M.cell_ref[c]);
     if (p3 < 0.00000001) { p3 = 0.000001; atomicAdd(&S->n_p_floor, 1ull); }
     if (!((code >> 6) & 1u)) { bx3 = bx; by3 = by; bz3 = bz; }
     double4 *o = reinterpret_cast<double4 *>(prim_new + 8 * c);

@@ -302,7 +302,7 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
             return false;
         }
     }
-    if (!o.ok || !okbc) report_error(S, -3, TM.face_ref[fb + f]);
+    if (!o.ok || !okbc) report_error(S, kErrFaceRiemann, TM.face_ref[fb + f]);
     if (bc == 0 && o.ok) {
         const double d0 = ca[0] - cb[0], d1 = ca[1] - cb[1], d2 = ca[2] - cb[2];
         const double dist = sqrt_n(d0 * d0 + d1 * d1 + d2 * d2) / 2.0;
@@ -396,7 +396,7 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
             w[9] = Volume;
             w[10] = Volume2;
         } else {
-            report_error(S, -1, M.cell_ref[cg]);
+            report_error(S, kErrCellFallback, M.cell_ref[cg]);
         }
     }
 }

@@ -141,14 +141,15 @@ void timer_collect(titan_ctx *c) {
 int check_scalars(titan_ctx *c, StepScalars *dS, StepScalars *host) {
     HIPCHK(hipMemcpyAsync(host, dS, sizeof(StepScalars), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (host->err_code != 0) {
-        int64_t idx = host->err_index == LLONG_MAX ? -1 : host->err_index;
-        int code = host->err_code;
+    if (host->err_key != ~0ull) {
+        static const int kCode[5] = {TITAN_E_SURFACE, TITAN_E_RIEMANN, TITAN_E_RIEMANN, TITAN_E_NONFINITE_STATE,
+                                     TITAN_E_ARG};
+        const unsigned rank = (unsigned)(host->err_key >> 48);
+        const int64_t idx = (int64_t)(host->err_key & ((1ull << 48) - 1));
+        const int code = rank < 5 ? kCode[rank] : TITAN_E_ARG;
         // clear for later calls
-        int zero = 0;
-        long long big = LLONG_MAX;
-        HIPCHK(hipMemcpy(&dS->err_code, &zero, sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(&dS->err_index, &big, sizeof(long long), hipMemcpyHostToDevice));
+        const unsigned long long none = ~0ull;
+        HIPCHK(hipMemcpy(&dS->err_key, &none, sizeof(none), hipMemcpyHostToDevice));
         return set_err(c, code, idx);
     }
     return TITAN_OK;
@@ -157,7 +158,7 @@ int check_scalars(titan_ctx *c, StepScalars *dS, StepScalars *host) {
 int init_scalars(StepScalars *d) {
     StepScalars h;
     std::memset(&h, 0, sizeof(h));
-    h.err_index = LLONG_MAX;
+    h.err_key = ~0ull;
     h.dtmin_bits[0] = h.dtmin_bits[1] = 0x7ff0000000000000ull;
     HIPCHK(hipMemcpy(d, &h, sizeof(h), hipMemcpyHostToDevice));
     return TITAN_OK;
