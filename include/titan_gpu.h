@@ -186,6 +186,18 @@ int titan_gran_potok(titan_ctx *ctx, int now, double dt, double *face_flux, doub
 /* Setka::Go(is_inner_area, steps) (Setka_physics.cpp:1484-2278) over the whole mesh: the first
  * call copies layer 0 to layer 1 and starts the lagged dt at 1e-5; later calls continue. */
 int titan_go(titan_ctx *ctx, int64_t steps, titan_step_stats *stats);
+/* Setka::Go(is_inner_area, steps) (Setka_physics.cpp:1484-2278) as the reference runs it: the
+ * steps update only the cells of one area -- the inner area (cells whose nodes are all inner,
+ * Setka::Init_boundary_grans, Setka_physics.cpp:113-212) or the outer area (every other cell) --
+ * from the fluxes of the area's faces (faces with a cell in the area), whose interior faces alone
+ * give the next dt; cells outside the area keep their state.  Every call starts the dt schedule
+ * again at 1e-5 (Setka_physics.cpp:1582).  The mesh moves as a whole (move_mesh).  Requires
+ * titan_set_inner_area.  titan_go above is the same loop over every cell with the dt schedule
+ * continuing across calls. */
+int titan_go_area(titan_ctx *ctx, int is_inner_area, int64_t steps, titan_step_stats *stats);
+/* Cell::is_inner per cell [n_cells] (nonzero = inner), the split of Setka::Init_boundary_grans:
+ * a cell is inner when all its nodes are among the first M0 + 8 nodes of an A/A2/B/C/C2 ray. */
+int titan_set_inner_area(titan_ctx *ctx, const uint8_t *cell_is_inner);
 /* Which layer holds the newest state after titan_go. */
 int titan_current_layer(titan_ctx *ctx);
 /* The second neighbours of every face chosen by Setka::Init_TVD (Setka_physics.cpp:648-744) from

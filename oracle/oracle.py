@@ -22,16 +22,17 @@ _I64 = ctypes.c_int64
 class OrMesh(ctypes.Structure):
     _fields_ = [("n_nodes", _I64), ("n_cells", _I64), ("n_faces", _I64), ("cell_nodes", _P), ("cell_faces", _P),
                 ("face_nodes", _P), ("face_cells", _P), ("face_bc", _P), ("face_method", _P), ("face_flags", _P),
-                ("face_bstate", _P), ("cell_zone", _P), ("cell_is_tvd", _P), ("face_tvd", _P)]
+                ("face_bstate", _P), ("cell_zone", _P), ("cell_is_tvd", _P), ("face_tvd", _P), ("cell_inner", _P)]
 
 
 class OrParams(ctypes.Structure):
     _fields_ = [("gamma", _D), ("kfl", _D), ("v_inf", _D), ("move_mesh", ctypes.c_int32), ("n_disc", ctypes.c_int32),
                 ("motion_omega", _D), ("motion_phase", _D), ("tvd", ctypes.c_int32), ("snos_on_hp", ctypes.c_int32),
-                ("par_a_2", _D), ("par_n_H_LISM", _D), ("par_Kn", _D)]
+                ("par_a_2", _D), ("par_n_H_LISM", _D), ("par_Kn", _D), ("area", ctypes.c_int32)]
 
-# parameters an older params dict may lack (Phys_param.h:99-101 defaults for the last three)
-PARAM_DEFAULTS = {"tvd": 0, "snos_on_hp": 0, "par_a_2": 0.102046, "par_n_H_LISM": 3.0, "par_Kn": 40.0906}
+# parameters a params dict may lack (Phys_param.h:99-101 defaults for par_*; area 0: every cell)
+PARAM_DEFAULTS = {"tvd": 0, "snos_on_hp": 0, "par_a_2": 0.102046, "par_n_H_LISM": 3.0, "par_Kn": 40.0906,
+                  "area": 0}
 
 
 class OrGeom(ctypes.Structure):
@@ -105,7 +106,7 @@ class OracleMesh:
             face_flags=None if mesh.face_flags is None else np.ascontiguousarray(mesh.face_flags, np.uint8),
             face_bstate=None if mesh.face_bstate is None else np.ascontiguousarray(mesh.face_bstate, np.float64),
             cell_zone=np.ascontiguousarray(mesh.cell_zone, np.int8), cell_is_tvd=np.ascontiguousarray(mesh.cell_is_tvd, np.uint8),
-            face_tvd=None)
+            face_tvd=None, cell_inner=None)
         c = OrMesh()
         c.n_nodes, c.n_cells, c.n_faces = mesh.n_nodes, mesh.n_cells, mesh.n_faces
         for k, v in self.arrs.items():
@@ -113,6 +114,11 @@ class OracleMesh:
         self.c = c
         if tvd:
             self.set_tvd(init_tvd(self, measure(self, mesh.node_xyz)))
+
+    def set_inner(self, cell_inner):
+        """Cell::is_inner per cell, for params area 1 / 2."""
+        self.arrs["cell_inner"] = np.ascontiguousarray(cell_inner, np.uint8).reshape(-1)
+        self.c.cell_inner = _p(self.arrs["cell_inner"])
 
     def set_tvd(self, tvd):
         self.arrs["face_tvd"] = np.ascontiguousarray(tvd, np.int32).reshape(-1, 2)

@@ -1092,6 +1092,22 @@ int or_face_states(const or_mesh *m, const or_params *p, int64_t f, const double
 
 /* Setka::Culc_Gran_Potok (Setka_physics.cpp:2280-2571), all faces.  x_now1/x_now2 are used only
  * when the mesh moves (face velocity).  gcl = 0 reference geometry. */
+/* Setka::Go's areas (Setka_physics.cpp:113-212, 1526-1537): the cells of area 1 are the inner
+ * ones, of area 2 the others; a face is on an area's list when one of its cells is in the area. */
+static int or_cell_in_area(const or_mesh *m, int64_t c, int area)
+{
+    if (area == 0) return 1;
+    int in = m->cell_inner && m->cell_inner[c];
+    return area == 1 ? in : !in;
+}
+
+static int or_face_in_area(const or_mesh *m, int64_t f, int area)
+{
+    if (area == 0) return 1;
+    int64_t a = m->face_cells[2 * f], b = m->face_cells[2 * f + 1];
+    return or_cell_in_area(m, a, area) || (b >= 0 && or_cell_in_area(m, b, area));
+}
+
 static int face_flux_impl(const or_mesh *m, const or_params *p, const double *prim, const or_geom *g,
                           const double *x1, const double *x2, double dt, int gcl, double *flux,
                           double *dt_face, uint8_t *branch, int nthreads)
@@ -1102,6 +1118,12 @@ static int face_flux_impl(const or_mesh *m, const or_params *p, const double *pr
     for (int64_t f = 0; f < m->n_faces; f++) {
         double qL[8], qR[8], q[8], dsl, dsc, dsr;
         int br = 0;
+        if (!or_face_in_area(m, f, p->area)) { /* not on the area's face list: not evaluated */
+            for (int i = 0; i < 9; i++) flux[9 * f + i] = 0.0;
+            dt_face[f] = 10000000.0;
+            if (branch) branch[f] = 0;
+            continue;
+        }
         double area = g->face_area[f];
         const double *n = g->face_normal + 3 * f;
         double w = 0.0;
@@ -1164,6 +1186,10 @@ static int update_impl(const or_mesh *m, const or_params *p, const double *prim_
     (void)p;
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : nrf, npf) num_threads(nthreads > 0 ? nthreads : 1)
     for (int64_t c = 0; c < m->n_cells; c++) {
+        if (!or_cell_in_area(m, c, p->area)) { /* outside the area: the state carries over */
+            for (int i = 0; i < 8; i++) prim_new[8 * c + i] = prim_old[8 * c + i];
+            continue;
+        }
         double Volume = vol_old[c];
         double Volume2 = vol_new[c];
         double P[9];

@@ -99,6 +99,7 @@ typedef struct {
     const uint8_t *cell_is_tvd;  /* [n_cells] 0 => B frozen and no TVD reconstruction  */
     const int32_t *face_tvd;     /* [n_faces*2] TVD neighbours AA (behind cells[0]) and BB
                                     (behind cells[1]), -1 = none (or_init_tvd); nullable */
+    const uint8_t *cell_inner;   /* [n_cells] Cell::is_inner (Setka::Init_boundary_grans); nullable */
 } or_mesh;
 
 typedef struct {
@@ -114,6 +115,9 @@ typedef struct {
     double par_a_2;      /* charge-exchange cross-section slope a_2 (Phys_param.h:99)           */
     double par_n_H_LISM; /* n_H of the LISM (Phys_param.h:100)                                  */
     double par_Kn;       /* Knudsen number (Phys_param.h:101)                                   */
+    int32_t area;        /* 0 every cell; 1 / 2 the inner / outer area of Setka::Go
+                            (Setka_physics.cpp:1526-1537): only the area's faces are evaluated
+                            (and give dt), only its cells updated, the others keep their state */
 } or_params;
 
 typedef struct {

@@ -108,6 +108,8 @@ _SIGS = {
     "titan_get_geometry": (_I, [_P, _I, _P, _P, _P, _P, _P]),
     "titan_gran_potok": (_I, [_P, _I, _D, _P, _P, _P]),
     "titan_go": (_I, [_P, _I64, ctypes.POINTER(StatsC)]),
+    "titan_go_area": (_I, [_P, _I, _I64, ctypes.POINTER(StatsC)]),
+    "titan_set_inner_area": (_I, [_P, _P]),
     "titan_current_layer": (_I, [_P]),
     "titan_tvd_cells": (_I, [_P, _P]),
     "titan_chlld_batch": (_I, [_I64, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _I]),
@@ -383,10 +385,22 @@ class Setka:
                "titan_gran_potok", self.ctx)
         return flux, dtm.value, br
 
-    def Go(self, steps):
+    def Go(self, steps, is_inner_area=None):
+        """titan_go: `steps` steps of every cell, the dt schedule continuing across calls; with
+        is_inner_area (True / False), titan_go_area: Setka::Go(is_inner_area, steps) on one area,
+        the dt schedule restarting as in the reference."""
         st = StatsC()
-        _check(lib().titan_go(self.ctx, int(steps), ctypes.byref(st)), "titan_go", self.ctx)
+        if is_inner_area is None:
+            _check(lib().titan_go(self.ctx, int(steps), ctypes.byref(st)), "titan_go", self.ctx)
+        else:
+            _check(lib().titan_go_area(self.ctx, 1 if is_inner_area else 0, int(steps), ctypes.byref(st)),
+                   "titan_go_area", self.ctx)
         return {k: getattr(st, k) for k, _ in StatsC._fields_}
+
+    def set_inner_area(self, cell_is_inner):
+        """Cell::is_inner per cell (Setka::Init_boundary_grans)."""
+        a = np.ascontiguousarray(cell_is_inner, np.uint8).reshape(-1)
+        _check(lib().titan_set_inner_area(self.ctx, _ptr(a)), "titan_set_inner_area", self.ctx)
 
     def current_layer(self):
         return lib().titan_current_layer(self.ctx)

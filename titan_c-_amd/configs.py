@@ -352,6 +352,14 @@ def shell_fitted(n_edge=6, M=(3, 6, 2, 8, 5, 5), R0_au=1.0, R1_au=10.0, R5_au=60
     return mesh, prim, params, rays
 
 
+def inner_cells_from_rays(mesh, rays):
+    """Cell::is_inner of Setka::Init_boundary_grans (Setka_physics.cpp:113-156): the first M0 + 8
+    nodes of every ray are inner, and a cell is inner when all eight of its nodes are."""
+    inner = np.zeros(mesh.n_nodes, bool)
+    inner[np.asarray(rays["ray_nodes"])[:, :rays["M0"] + 1 + 7].reshape(-1)] = True
+    return inner[mesh.cell_nodes].all(axis=1).astype(np.uint8)
+
+
 def neutrals(n_cells, seed=5):
     """Per-cell neutral-H moments [n_H, c_H^2, V_H(3)] in code units: n_H ~ U(0.08, 0.2) cm^-3,
     V_H = 26 km/s along the inflow axis (-x) with +-30 % per-component perturbation, T_H ~

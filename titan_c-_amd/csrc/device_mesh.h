@@ -14,13 +14,25 @@
 namespace titan {
 
 // cell_code bits: 0..5 = sign of face j (1: the cell is cells[0] of its j-th face),
-// 6 = is_TVD, 8..11 = zone (Type_cell::Zone_1..4)
+// 6 = is_TVD, 7 = in the inner area (Cell::is_inner), 8..11 = zone (Type_cell::Zone_1..4)
 __host__ __device__ inline uint32_t cell_code_make(uint32_t fsign, uint32_t tvd, uint32_t zone) {
     return (fsign & 63u) | ((tvd & 1u) << 6) | ((zone & 15u) << 8);
 }
 // face_code bits: 0..3 = boundary code, 4..7 = Riemann method, 8..15 = chlld flags
 __host__ __device__ inline uint32_t face_code_make(uint32_t bc, uint32_t method, uint32_t flags) {
     return (bc & 15u) | ((method & 15u) << 4) | ((flags & 255u) << 8);
+}
+
+// The area a step of Setka::Go covers (Setka_physics.cpp:1526-1537): 0 every cell (this build's
+// continuing steps), 1 the inner area (is_inner_area = true), 2 the outer area.
+__host__ __device__ inline bool cell_in_area(uint32_t code, int area) {
+    return area == 0 || (((code >> 7) & 1u) != 0u) == (area == 1);
+}
+// The area's face list (Setka_physics.cpp:168-194): a face belongs to it when one of its cells
+// does; a boundary face goes with its one cell.
+__host__ __device__ inline bool face_in_area(uint32_t codeA, uint32_t codeB, bool boundary, int area) {
+    if (area == 0) return true;
+    return cell_in_area(codeA, area) || (!boundary && cell_in_area(codeB, area));
 }
 
 struct DevMesh {
@@ -51,6 +63,8 @@ struct DevParams {
     double gamma, kfl, v_inf;
     int32_t move_mesh, n_disc, geom_mode, tvd;  // tvd: some face reconstructs to second order
     double motion_omega, motion_phase;
+    int32_t area;                    // cell_in_area: the cells this step updates
+    int32_t pad_area;
 };
 
 // Fitted surfaces and their rays (Setka::Culc_Velocity_surface, Setka_move.cpp:111-1357, and

@@ -8,8 +8,8 @@ namespace titan {
 
 static inline unsigned grid_of(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-void launch_begin_step(hipStream_t s, StepScalars *S, long long n, const DevParams &P) {
-    hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(64), 0, s, S, n, P);
+void launch_begin_step(hipStream_t s, StepScalars *S, long long n, const DevParams &P, int restart) {
+    hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(64), 0, s, S, n, P, restart);
 }
 
 void launch_motion(hipStream_t s, int64_t n3, const double *x_old, const double *disp, const StepScalars *S,

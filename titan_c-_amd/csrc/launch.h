@@ -7,7 +7,7 @@
 
 namespace titan {
 
-void launch_begin_step(hipStream_t s, StepScalars *S, long long n, const DevParams &P);
+void launch_begin_step(hipStream_t s, StepScalars *S, long long n, const DevParams &P, int restart);
 void launch_motion(hipStream_t s, int64_t n3, const double *x_old, const double *disp, const StepScalars *S,
                    double *x_new);
 void launch_measure(hipStream_t s, const DevMesh &M, const double *X, const DevGeom &G, int gcl);

@@ -111,9 +111,10 @@ __device__ __forceinline__ void face_geom_one(const DevMesh &M, const double *__
 
 #ifndef TITAN_FLUX_ONLY
 // ---- step bookkeeping (one thread) --------------------------------------------------------
-// step n (1-based, counted over the context's life): dt = 1e-5 for n == 1, else the min over
+// step n (1-based, counted over the context's life): dt = 1e-5 for n == 1 and for the first step
+// of a call that restarts the schedule as each Setka::Go call does (restart), else the min over
 // interior faces of step n-1 (initial value 1e8, Setka_physics.cpp:1582,1605-1607).
-__global__ void k_begin_step(StepScalars *S, long long n, DevParams P) {
+__global__ void k_begin_step(StepScalars *S, long long n, DevParams P, int restart) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double dt_prev = S->dt;
     if (n == 1) {
@@ -121,7 +122,7 @@ __global__ void k_begin_step(StepScalars *S, long long n, DevParams P) {
         S->t = S->t;  // caller initialises t
     } else {
         S->t = S->t + dt_prev;
-        S->dt = __longlong_as_double((long long)S->dtmin_bits[(n - 1) & 1]);
+        S->dt = restart ? 0.00001 : __longlong_as_double((long long)S->dtmin_bits[(n - 1) & 1]);
     }
     S->dtmin_bits[n & 1] = (unsigned long long)__double_as_longlong(100000000.0);
     S->motion_fac = S->dt * (P.motion_omega * det_cos(P.motion_omega * S->t + P.motion_phase));

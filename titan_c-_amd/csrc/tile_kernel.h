@@ -199,6 +199,12 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
     const int64_t b = (int32_t)(rec.y >> 32);
 #pragma unroll
     for (int i = 0; i < 4; i++) L.fnodes[4 * f + i] = (uint16_t)ln[i];
+    // a face off the stepped area's face list is not evaluated (its cells are both frozen; its
+    // node slots above still serve their volumes)
+    if (P.area != 0 && !face_in_area(M.cell_code[a], b >= 0 ? M.cell_code[b] : 0u, bc != 0, P.area)) {
+        L.flux[9 * f + 8] = 0.0;  // not a marked slot for the mixed solver's FP64 pass
+        return true;
+    }
     double x1[4][3];
 #pragma unroll
     for (int i = 0; i < 4; i++) { x1[i][0] = L.x1[3 * ln[i]]; x1[i][1] = L.x1[3 * ln[i] + 1]; x1[i][2] = L.x1[3 * ln[i] + 2]; }
@@ -380,7 +386,11 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
         Volume2 = tile_volume(L.x2, cB, L.fnodes, lf, code, gcl);
     }
     if (moving || !TM.vol_cached) TM.vol[cg] = Volume2;
-    if (inbox) {
+    if (!cell_in_area(code, P.area)) {  // frozen: the state carries over to the new layer
+        double4 *o = reinterpret_cast<double4 *>(prim2 + 8 * cg);
+        o[0] = make_double4(st[0], st[1], st[2], st[3]);
+        o[1] = make_double4(st[4], st[5], st[6], st[7]);
+    } else if (inbox) {
         const float y[5] = {o[0], o[1], o[2], o[3], y4};
         const double vp[3] = {st[1], st[2], st[3]};
         double s5[5];
@@ -580,6 +590,12 @@ __global__ void __launch_bounds__(kTileThreads, 2)
             Volume2 = tile_volume(L.x2, cB, L.fnodes, lf, code, gcl);
         }
         if (moving || !TM.vol_cached) TM.vol[cg] = Volume2;
+        if (!cell_in_area(code, P.area)) {  // frozen: the state carries over to the new layer
+            double4 *o = reinterpret_cast<double4 *>(prim2 + 8 * cg);
+            o[0] = make_double4(s[0], s[1], s[2], s[3]);
+            o[1] = make_double4(s[4], s[5], s[6], s[7]);
+            continue;
+        }
         double Pf[9];
 #pragma unroll
         for (int i = 0; i < 9; i++) Pf[i] = 0.0;

@@ -86,6 +86,7 @@ struct titan_ctx {
     Timer tm;
     MlpState mlp;
     FitState *fit = nullptr;  // fitted surfaces and rays (move_mesh 2)
+    bool has_area = false;    // titan_set_inner_area was called
     DevMesh mesh() const {
         DevMesh M;
         M.n_nodes = nn; M.n_cells = nc; M.n_faces = nf;
@@ -601,8 +602,11 @@ static int compute_sources(titan_ctx *c, int layer, int mode) {
     return TITAN_OK;
 }
 
-extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
+// The step loop: `area` selects the cells updated (cell_in_area), `restart` starts the dt schedule
+// again at 1e-5 as each Setka::Go call does.
+static int go_impl(titan_ctx *c, int64_t steps, titan_step_stats *stats, int area, bool restart) {
     if (!c || steps < 0) return TITAN_E_ARG;
+    c->dp.area = area;
     const DevMesh M = c->mesh();
     const int64_t nn3 = 3 * c->nn;
     hipEvent_t e0, e1;
@@ -651,7 +655,7 @@ extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
         c->now1 = (c->now1 + 1) % 2;
         c->step++;
         const int n1 = c->now1, n2 = c->now2;
-        launch_begin_step(c->stream, c->d_S, c->step, c->dp);
+        launch_begin_step(c->stream, c->d_S, c->step, c->dp, restart && s == 0 ? 1 : 0);
         if (fitting) {
             timer_mark(c, 2, true);
             launch_fit_surface(c->stream, M, c->dp, fd, c->d_prim[n1], c->d_X[n1], c->d_X[n2],
@@ -701,6 +705,32 @@ extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
         stats->ms_total = ms;
     }
     return rc;
+}
+
+extern "C" int titan_go(titan_ctx *c, int64_t steps, titan_step_stats *stats) {
+    const int rc = go_impl(c, steps, stats, 0, false);
+    if (c) c->dp.area = 0;
+    return rc;
+}
+
+extern "C" int titan_go_area(titan_ctx *c, int is_inner_area, int64_t steps, titan_step_stats *stats) {
+    if (!c || !c->has_area) return TITAN_E_ARG;
+    const int rc = go_impl(c, steps, stats, is_inner_area ? 1 : 2, true);
+    c->dp.area = 0;
+    return rc;
+}
+
+extern "C" int titan_set_inner_area(titan_ctx *c, const uint8_t *cell_is_inner) {
+    if (!c || !cell_is_inner) return TITAN_E_ARG;
+    std::vector<uint32_t> code(c->nc);
+    HIPCHK(hipMemcpy(code.data(), c->d_cell_code, sizeof(uint32_t) * c->nc, hipMemcpyDeviceToHost));
+    for (int64_t d = 0; d < c->nc; d++) {
+        const bool in = cell_is_inner[c->ord.cell_d2r[d]] != 0;
+        code[d] = (code[d] & ~(1u << 7)) | (in ? (1u << 7) : 0u);
+    }
+    HIPCHK(hipMemcpy(c->d_cell_code, code.data(), sizeof(uint32_t) * c->nc, hipMemcpyHostToDevice));
+    c->has_area = true;
+    return TITAN_OK;
 }
 
 extern "C" int titan_current_layer(titan_ctx *c) {
