@@ -228,6 +228,25 @@ int titan_sources(titan_ctx *ctx, int layer, int mode, double *out);
 /* Standalone exact source for n states: in [n*10] = n_p, c_p^2, V_p(3), n_H, c_H^2, V_H(3). */
 int titan_cx_exact_batch(int64_t n, const double *in10, int nodes, double *out5);
 
+/* ---- Monte-Carlo neutral densities -------------------------------------------------------- */
+/* Setka::MK_go with MK_fly_immit (Setka_MK.cpp:1225-2316) for the interstellar hydrogen (sort 4)
+ * that enters through the outer boundary faces (bc 2, 3), on the nodes and states of `layer`:
+ * particles per face and weights by the rule of Setka_MK.cpp:1233,1380-1390 (N_per_gran =
+ * n_per_face, inflow Get_Spotok_inf), births by Gran::Get_Random_pozition and
+ * Setka::Velosity_initial, straight flights cell to cell (Setka::Time_to_vilet), charge exchange
+ * by optical depth with nu = rho_Th Velosity_1 sigma / par_Kn and the proton velocity of
+ * Setka::M_K_Change_Velosity (a new particle takes the sort of the cell's zone).  density
+ * [n_cells*5] = MK_n_H, MK_n_H1..MK_n_H4 of Cell::MK_Add_particle (Cell.cpp:1042-1083) per unit
+ * volume.  stats [6] (nullable): particles, charge exchanges, left through the outer boundary,
+ * absorbed at the inner boundary (the reference carries them across the Sun's sphere), lost (no
+ * exit found after 11 nudges to the cell centre, or max_steps cells crossed), rejection samplers
+ * that gave up after 10000 tries (a birth that does: the particle is dropped).  Random numbers:
+ * a counter-based SplitMix64 stream per particle from `seed` (the reference reads rnd_my.txt,
+ * which it does not ship); particles are numbered face by face in ascending reference face id.
+ * The tallies are FP64 atomic sums, so repeated calls agree to rounding, not bit for bit. */
+int titan_mk_go(titan_ctx *ctx, int layer, int32_t n_per_face, uint64_t seed, int32_t max_steps,
+                double *density, int64_t *stats);
+
 /* ---- surrogate network (8-64-64-64-5 SiLU) ------------------------------------------------- */
 /* Parameter layout (FP32): W1[64x8] b1[64] W2[64x64] b2[64] W3[64x64] b3[64] W4[5x64] b4[5],
  * rows = outputs; plus input normalisation mean/std [8+8], output scale [5], input box

@@ -72,6 +72,8 @@ def lib():
             "or_go": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _P, _P, _P, _I, _P, _P]),
             "or_culc_velocity_surface": (_I, [_P, _P, _P, _P, _P, _P, _D, _P, _P]),
             "or_dvigenie": (None, [_P, _P]),
+            "or_mk_setup": (_I64, [_P, _P, _D, ctypes.c_int32, _P, _P, _P]),
+            "or_mk_go": (_I, [_P, _P, _P, _P, _P, _P, _P]),
             "or_bera_source": (None, [_P, _I, _P, _D, _D, _D, _P]),
             "or_new_connect": (_I64, [_I64, _P, _I64, _P, _P, _P]),
             "or_cx_rate_ref": (_D, [_D, _D, _D, _I, _I]),
@@ -506,3 +508,38 @@ def dvigenie(fit, x, params=None):
     xn = np.ascontiguousarray(x, np.float64).copy()
     lib().or_dvigenie(ctypes.byref(fit.c(params or {})), _p(xn))
     return xn
+
+
+# ---- Monte-Carlo neutral densities --------------------------------------------------------------
+class OrMk(ctypes.Structure):
+    _fields_ = [("n_in", _I64), ("face", _P), ("count", _P), ("mu", _P), ("seed", ctypes.c_uint64),
+                ("v_inf", _D), ("a2", _D), ("kn", _D), ("max_steps", ctypes.c_int32)]
+
+
+def mk_setup(om, geom, v_inf, n_per_face):
+    """Injection faces, particles per face and weights (or_mk_setup)."""
+    m = om.m
+    face = np.zeros(m.n_faces, np.int32)
+    count = np.zeros(m.n_faces, np.int32)
+    mu = np.zeros(m.n_faces)
+    g = OrGeom(*[_p(geom[k]) for k, _ in OrGeom._fields_])
+    k = lib().or_mk_setup(ctypes.byref(om.c), ctypes.byref(g), float(v_inf), int(n_per_face), _p(face), _p(count),
+                          _p(mu))
+    return face[:k].copy(), count[:k].copy(), mu[:k].copy()
+
+
+def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000):
+    """or_mk_go on the nodes xyz and state prim; returns (densities [n_cells, 5] = sum over sorts,
+    sorts 1..4, per unit volume; stats dict)."""
+    m = om.m
+    x = np.ascontiguousarray(xyz, np.float64)
+    geom = measure(om, x)
+    face, count, mu = mk_setup(om, geom, v_inf, n_per_face)
+    g = OrGeom(*[_p(geom[k]) for k, _ in OrGeom._fields_])
+    mk = OrMk(len(face), _p(face), _p(count), _p(mu), int(seed), float(v_inf), float(a2), float(kn), int(max_steps))
+    tally = np.zeros((m.n_cells, 5))
+    stats = np.zeros(6, np.int64)
+    lib().or_mk_go(ctypes.byref(om.c), ctypes.byref(mk), _p(x), _p(np.ascontiguousarray(prim, np.float64)),
+                   ctypes.byref(g), _p(tally), _p(stats))
+    keys = ("particles", "exchanges", "left", "absorbed", "lost", "capped")
+    return tally / geom["cell_volume"][:, None], dict(zip(keys, (int(v) for v in stats)))

@@ -1521,6 +1521,306 @@ void or_dvigenie(const or_fit *fit, double *x)
     }
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* Monte-Carlo neutral densities (Setka::MK_go / MK_fly_immit, Setka_MK.cpp:1225-2316).            */
+static const double OR_SQRTPI_MK = 1.77245385; /* sqrtpi_, Header.h:90 (the reference's literal) */
+
+static uint64_t mk_mix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+/* per-particle counter stream, values in (0, 1) */
+static double mk_rand(uint64_t *s)
+{
+    *s += 0x9E3779B97F4A7C15ull;
+    return ((double)(mk_mix(*s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+/* Get_Spotok_inf (Setka_MK.cpp:2316-2324): inflow of the unit-temperature LISM Maxwellian through
+ * a face of outward normal n, per unit area */
+static double mk_spotok_inf(double v_inf, const double *n)
+{
+    double Ux = v_inf * n[0] + 0.0 * n[1] + 0.0 * n[2];
+    return -(-exp(-(Ux * Ux)) + OR_SQRTPI_MK * Ux * erfc(Ux)) / (2.0 * OR_SQRTPI_MK);
+}
+
+/* Velosity_1 (Setka_MK.cpp:40-50): mean relative speed of a neutral at relative speed u in a
+ * Maxwellian of thermal speed cp */
+static double mk_velosity_1(double u, double cp)
+{
+    if (u < 0.00001)
+        return 2.0 * cp / OR_SQRTPI_MK + 2.0 * u * u / (3.0 * cp * OR_SQRTPI_MK) -
+               u * u * u * u / (15.0 * cp * cp * cp * OR_SQRTPI_MK);
+    return exp(-u * u / (cp * cp)) * cp / OR_SQRTPI_MK + (u + (cp * cp) / (2.0 * u)) * erf(u / cp);
+}
+
+static double mk_sigma(double a2, double x) { double t = 1.0 - a2 * log(x); return t * t; }
+
+/* Gran::rayTriangleIntersect (Gran.cpp:522-561), Moller-Trumbore with the reference's 1e-6 */
+static int mk_ray_tri(const double *o, const double *d, const double *v0, const double *v1, const double *v2,
+                      double *t)
+{
+    double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+    double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+    double pv[3] = {d[1] * e2[2] - d[2] * e2[1], d[2] * e2[0] - d[0] * e2[2], d[0] * e2[1] - d[1] * e2[0]};
+    double det = (e1[0] * pv[0] + e1[1] * pv[1]) + e1[2] * pv[2];
+    if (fabs(det) < 1e-6) return 0;
+    double inv = 1.0 / det;
+    double tv[3] = {o[0] - v0[0], o[1] - v0[1], o[2] - v0[2]};
+    double u = ((tv[0] * pv[0] + tv[1] * pv[1]) + tv[2] * pv[2]) * inv;
+    if (u < 0.0 || u > 1.0) return 0;
+    double qv[3] = {tv[1] * e1[2] - tv[2] * e1[1], tv[2] * e1[0] - tv[0] * e1[2], tv[0] * e1[1] - tv[1] * e1[0]};
+    double v = ((d[0] * qv[0] + d[1] * qv[1]) + d[2] * qv[2]) * inv;
+    if (v < 0.0 || u + v > 1.0) return 0;
+    *t = ((e2[0] * qv[0] + e2[1] * qv[1]) + e2[2] * qv[2]) * inv;
+    return *t > 1e-6;
+}
+
+/* Setka::Time_to_vilet with Gran::Luch_crossing (Setka_MK.cpp:2725-2785, Gran.cpp:451-510): the
+ * earliest crossing of the cell's faces, each split into the triangles (0,1,2) and (0,2,3) */
+static int mk_exit(const or_mesh *m, const double *x, int64_t c, const double *P, const double *V, double *time,
+                   int64_t *face)
+{
+    double nv = sqrt((V[0] * V[0] + V[1] * V[1]) + V[2] * V[2]);
+    double d[3] = {V[0] / nv, V[1] / nv, V[2] / nv};
+    double tmin = 1e10;
+    int found = 0;
+    for (int j = 0; j < 6; j++) {
+        int64_t f = m->cell_faces[6 * c + j];
+        const int32_t *fn = m->face_nodes + 4 * f;
+        double t1 = 0.0, t2 = 0.0;
+        int b1 = mk_ray_tri(P, d, x + 3 * fn[0], x + 3 * fn[1], x + 3 * fn[2], &t1);
+        t1 /= nv;
+        int b2 = mk_ray_tri(P, d, x + 3 * fn[0], x + 3 * fn[2], x + 3 * fn[3], &t2);
+        t2 /= nv;
+        if (!b1 && !b2) continue;
+        double tf = (b1 && b2) ? SMIN(t1, t2) : (b1 ? t1 : t2);
+        if (tmin > tf) { tmin = tf; *face = f; found = 1; }
+    }
+    *time = tmin;
+    return found;
+}
+
+/* spherical components in the frame whose polar axis is x (spherical_skorost with (z, x, y) =
+ * (x0, x1, x2), Help_funk.cpp:108-121) and back (dekard_skorost, :123-135) */
+static void mk_to_sph(const double *X, const double *Vc, double *r, double *th, double *ph)
+{
+    double rr = sqrt(X[1] * X[1] + X[2] * X[2] + X[0] * X[0]);
+    double the = acos(X[0] / rr);
+    double phi = polar_angle(X[1], X[2]);
+    double Vz = Vc[0], Vx = Vc[1], Vy = Vc[2];
+    *r = Vx * sin(the) * cos(phi) + Vy * sin(the) * sin(phi) + Vz * cos(the);
+    *th = Vx * cos(the) * cos(phi) + Vy * cos(the) * sin(phi) - Vz * sin(the);
+    *ph = -Vx * sin(phi) + Vy * cos(phi);
+}
+static void mk_from_sph(const double *X, double r, double th, double ph, double *Vc)
+{
+    double rr = sqrt(X[1] * X[1] + X[2] * X[2] + X[0] * X[0]);
+    double the = acos(X[0] / rr);
+    double phi = polar_angle(X[1], X[2]);
+    Vc[1] = r * sin(the) * cos(phi) + th * cos(the) * cos(phi) - ph * sin(phi);
+    Vc[2] = r * sin(the) * sin(phi) + th * cos(the) * sin(phi) + ph * cos(phi);
+    Vc[0] = r * cos(the) - th * sin(the);
+}
+
+/* Setka::M_K_Change_Velosity (Setka_MK.cpp:2788-2850): the velocity of the proton a neutral of
+ * velocity V exchanged with, in units of cp, from the Maxwellian of mean U weighted by g sigma(g);
+ * the rejection loop is capped at 10000 tries (*capped set) */
+static void mk_change_velosity(uint64_t *rs, double a2, const double *U, const double *V, double *W, double cp,
+                               int *capped)
+{
+    double X = sqrt((V[0] - U[0]) * (V[0] - U[0]) + (V[1] - U[1]) * (V[1] - U[1]) + (V[2] - U[2]) * (V[2] - U[2]));
+    double p4 = 0.5 * OR_SQRTPI_MK * X / (1.0 + 0.5 * OR_SQRTPI_MK * X);
+    double v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    for (int tries = 0; tries < 10000; tries++) {
+        double k1 = mk_rand(rs), k2 = mk_rand(rs), k3 = mk_rand(rs), k4 = mk_rand(rs), k5 = mk_rand(rs),
+               k6 = mk_rand(rs);
+        double y1, y2, y3;
+        if (p4 < k1) {
+            double om1 = 1.0 - 2.0 * k4;
+            double om2 = sqrt(1.0 - om1 * om1) * cos(2.0 * OR_PI * k5);
+            double om3 = sqrt(1.0 - om1 * om1) * sin(2.0 * OR_PI * k5);
+            double lo = sqrt(-log(k2 * k3));
+            y1 = lo * om1; y2 = lo * om2; y3 = lo * om3;
+        } else {
+            y1 = sqrt(-log(k2)) * cos(OR_PI * k3);
+            y2 = sqrt(-log(k4)) * cos(2.0 * OR_PI * k5);
+            y3 = sqrt(-log(k4)) * sin(2.0 * OR_PI * k5);
+        }
+        v1 = y1 + U[0]; v2 = y2 + U[1]; v3 = y3 + U[2];
+        double u1 = V[0] - v1, u2 = V[1] - v2, u3 = V[2] - v3;
+        double uuu = sqrt(u1 * u1 + u2 * u2 + u3 * u3);
+        double yy = sqrt(y1 * y1 + y2 * y2 + y3 * y3);
+        double h = (uuu * mk_sigma(a2, uuu * cp)) / (mk_sigma(a2, X * cp) * (X + yy));
+        if (!(h < k6)) { W[0] = v1; W[1] = v2; W[2] = v3; return; }
+    }
+    *capped = 1;
+    W[0] = v1; W[1] = v2; W[2] = v3;
+}
+
+/* Setka::Velosity_initial (Setka_MK.cpp:2671-2723): a velocity entering through a face of outward
+ * normal n from the unit-temperature Maxwellian drifting at (v_inf, 0, 0) */
+static void mk_velosity_initial(uint64_t *rs, double v_inf, const double *n, double *V, int *capped)
+{
+    double t[3], mm[3];
+    or_get_bazis(n, t, mm);
+    double Vi[3] = {v_inf * n[0] + 0.0 * n[1] + 0.0 * n[2], v_inf * t[0] + 0.0 * t[1] + 0.0 * t[2],
+                    v_inf * mm[0] + 0.0 * mm[1] + 0.0 * mm[2]};
+    double p1 = fabs(Vi[0]) * OR_SQRTPI_MK / (1.0 + fabs(Vi[0]) * OR_SQRTPI_MK);
+    double z = 0.0;
+    int ok = 0;
+    for (int tries = 0; tries < 10000; tries++) {
+        double k3 = mk_rand(rs), k4 = mk_rand(rs), k5 = mk_rand(rs), k6 = mk_rand(rs);
+        if (p1 > k3) z = cos(OR_PI * k5) * sqrt(-log(k4));
+        else if (k4 <= 0.5) z = -sqrt(-log(2.0 * k4));
+        else z = sqrt(-log(2.0 * (1.0 - k4)));
+        if (!(fabs(z + Vi[0]) / (fabs(Vi[0]) + fabs(z)) < k6 || z > -Vi[0])) { ok = 1; break; }
+    }
+    if (!ok) *capped = 1;
+    double V1 = z + Vi[0];
+    double k1 = mk_rand(rs), k2 = mk_rand(rs);
+    double a = sqrt(-log(k2));
+    double V2 = Vi[1] + a * cos(2.0 * OR_PI * k1);
+    double V3 = Vi[2] + a * sin(2.0 * OR_PI * k1);
+    for (int i = 0; i < 3; i++) V[i] = V1 * n[i] + V2 * t[i] + V3 * mm[i];
+}
+
+int64_t or_mk_setup(const or_mesh *m, const or_geom *g, double v_inf, int32_t n_per_face, int32_t *face,
+                    int32_t *count, double *mu)
+{
+    double total = 0.0;
+    int64_t nb = 0;
+    for (int64_t f = 0; f < m->n_faces; f++) {
+        if (m->face_bc[f] != OR_BC_OUTER_HARD && m->face_bc[f] != OR_BC_OUTER_SOFT) continue;
+        total += mk_spotok_inf(v_inf, g->face_normal + 3 * f) * g->face_area[f];
+        nb++;
+    }
+    if (nb == 0) return 0;
+    double mu_expect = total / (1.0 * n_per_face * nb);
+    int64_t k = 0;
+    for (int64_t f = 0; f < m->n_faces; f++) {
+        if (m->face_bc[f] != OR_BC_OUTER_HARD && m->face_bc[f] != OR_BC_OUTER_SOFT) continue;
+        double sv = mk_spotok_inf(v_inf, g->face_normal + 3 * f) * g->face_area[f];
+        if (sv < 0.0000001 * total / nb) continue;
+        int32_t n = (int32_t)(sv / mu_expect) + 1;
+        int32_t lo = n_per_face < 100 ? n_per_face : 100;
+        if (n < lo) n = lo;
+        face[k] = (int32_t)f;
+        count[k] = n;
+        mu[k] = sv / n;
+        k++;
+    }
+    return k;
+}
+
+int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *prim, const or_geom *g,
+             double *tally, int64_t *stats)
+{
+    const double me = 1.0 / 1836.15;
+    int64_t pid = 0;
+    for (int i = 0; i < 6; i++) stats[i] = 0;
+    for (int64_t q = 0; q < mk->n_in; q++) {
+        int64_t f = mk->face[q];
+        const double *n = g->face_normal + 3 * f;
+        const int32_t *fn = m->face_nodes + 4 * f;
+        for (int32_t k = 0; k < mk->count[q]; k++, pid++) {
+            uint64_t rs = mk_mix(mk->seed ^ mk_mix((uint64_t)pid + 0x9E3779B97F4A7C15ull));
+            int capped = 0;
+            /* Gran::Get_Random_pozition (Gran.cpp:220-285) */
+            double s0 = 0.0, s1 = 0.0;
+            const double *A = x + 3 * fn[0], *B1 = x + 3 * fn[1], *Cc = x + 3 * fn[2], *B2 = x + 3 * fn[3];
+            {
+                double v1x = B1[0] - A[0], v1y = B1[1] - A[1], v1z = B1[2] - A[2];
+                double v2x = Cc[0] - A[0], v2y = Cc[1] - A[1], v2z = Cc[2] - A[2];
+                double cx = v1y * v2z - v1z * v2y, cy = v1z * v2x - v1x * v2z, cz = v1x * v2y - v1y * v2x;
+                s0 = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+                v1x = B2[0] - A[0]; v1y = B2[1] - A[1]; v1z = B2[2] - A[2];
+                cx = v1y * v2z - v1z * v2y; cy = v1z * v2x - v1x * v2z; cz = v1x * v2y - v1y * v2x;
+                s1 = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+            }
+            double S = s0 + s1;
+            double ksi = mk_rand(&rs);
+            const double *T1 = (ksi <= s0 / S) ? B1 : B2;
+            double r1 = mk_rand(&rs), r2 = mk_rand(&rs);
+            double P[3], V[3];
+            for (int i = 0; i < 3; i++)
+                P[i] = (1.0 - sqrt(r1)) * A[i] + (sqrt(r1) * (1.0 - r2)) * T1[i] + (r2 * sqrt(r1)) * Cc[i];
+            mk_velosity_initial(&rs, mk->v_inf, n, V, &capped);
+            stats[0]++;
+            if (capped) { stats[5]++; continue; } /* no admissible velocity within the cap: dropped */
+            int64_t c = m->face_cells[2 * f];
+            const double *cc = g->cell_center + 3 * c;
+            for (int i = 0; i < 3; i++) P[i] = P[i] + (cc[i] - P[i]) / 1000.0; /* into the cell */
+            double KSI = -log(1.0 - mk_rand(&rs)), I_do = 0.0, mu = mk->mu[q];
+            int sort = 4;
+            int done = 0;
+            for (int it = 0; it < mk->max_steps && !done; it++) {
+                double time = 0.0;
+                int64_t ex = -1;
+                int found = 0;
+                for (int k1 = 0; k1 < 12 && !found; k1++) {
+                    found = mk_exit(m, x, c, P, V, &time, &ex);
+                    if (!found) {
+                        const double *cn = g->cell_center + 3 * c;
+                        for (int i = 0; i < 3; i++) P[i] = (k1 < 10) ? P[i] + (cn[i] - P[i]) / 800.0 : cn[i];
+                    }
+                }
+                if (!found) { stats[4]++; done = 1; break; }
+                const double *s = prim + 8 * c;
+                double rho_Th = s[0] / (1.0 + me), p_Th = s[4] / 2.0;
+                if (rho_Th <= 1e-8) rho_Th = 1e-8;
+                if (p_Th <= 1e-8 / 2.0) p_Th = 1e-8 / 2.0;
+                double ro = rho_Th, cp = sqrt(2.0 * p_Th / rho_Th);
+                double l = sqrt((time * V[0]) * (time * V[0]) + (time * V[1]) * (time * V[1]) + (time * V[2]) * (time * V[2]));
+                double I = I_do;
+                double u = sqrt((V[0] - s[1]) * (V[0] - s[1]) + (V[1] - s[2]) * (V[1] - s[2]) + (V[2] - s[3]) * (V[2] - s[3]));
+                double vn = sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
+                double uz = mk_velosity_1(u, cp);
+                double nu_ex = ro * uz * mk_sigma(mk->a2, uz) / mk->kn;
+                double sig = vn / nu_ex;
+                I += l / sig;
+                if (I < KSI) {
+                    I_do = I;
+                    tally[5 * c] += time * mu;
+                    if (sort >= 1 && sort <= 4) tally[5 * c + sort] += time * mu;
+                    for (int i = 0; i < 3; i++) P[i] += 1.000001 * time * V[i];
+                    int64_t nc = m->face_cells[2 * ex] == c ? m->face_cells[2 * ex + 1] : m->face_cells[2 * ex];
+                    if (nc < 0) {
+                        if (m->face_bc[ex] == OR_BC_INNER_HARD) stats[3]++;
+                        else stats[2]++;
+                        done = 1;
+                    } else {
+                        c = nc;
+                    }
+                } else {
+                    double kk = (KSI - I_do) * sig;
+                    double t_ex = kk / vn;
+                    I_do = 0.0;
+                    for (int i = 0; i < 3; i++) P[i] += t_ex * V[i];
+                    tally[5 * c] += t_ex * mu;
+                    if (sort >= 1 && sort <= 4) tally[5 * c + sort] += t_ex * mu; /* other sorts: total only */
+                    double U[3], Vs[3], W[3];
+                    double Vp[3] = {s[1], s[2], s[3]};
+                    mk_to_sph(P, Vp, &U[0], &U[1], &U[2]);      /* (r, theta, phi) */
+                    mk_to_sph(P, V, &Vs[0], &Vs[1], &Vs[2]);
+                    for (int i = 0; i < 3; i++) { U[i] /= cp; Vs[i] /= cp; }
+                    mk_change_velosity(&rs, mk->a2, U, Vs, W, cp, &capped);
+                    for (int i = 0; i < 3; i++) W[i] *= cp;
+                    mk_from_sph(P, W[0], W[1], W[2], V);
+                    sort = m->cell_zone[c];
+                    KSI = -log(1.0 - mk_rand(&rs));
+                    stats[1]++;
+                }
+                if (it == mk->max_steps - 1 && !done) { stats[4]++; done = 1; }
+            }
+            if (capped) stats[5]++;
+        }
+    }
+    return 0;
+}
+
 int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, const double *disp,
           int64_t steps, int gcl, int nthreads, int *final_layer, double *dt_hist, double *t_sim,
           int n_h, const double *h_fluids, const or_fit *fit)

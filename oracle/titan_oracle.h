@@ -206,6 +206,33 @@ int or_culc_velocity_surface(const or_mesh *m, const or_params *p, const or_fit 
 /* Every ray's nodes other than index M0 and its TS/HP/BS nodes, in x. */
 void or_dvigenie(const or_fit *fit, double *x);
 
+/* ---- Monte-Carlo neutral densities: the flight loop of Setka::MK_go / MK_fly_immit
+ * (Setka_MK.cpp:1225-2316) for the LISM hydrogen that enters through the outer boundary (sort 4).
+ * Injection faces: face ids, particles per face and weight per particle (or_mk_setup restates
+ * the counts of Setka_MK.cpp:1233,1380-1390).  tally [n_cells*5] += mu * time per cell: the sum
+ * over sorts, then sorts 1..4 (Cell::MK_Add_particle, Cell.cpp:1042-1083).  stats [6]: particles,
+ * charge exchanges, left through the outer boundary, absorbed at the inner boundary, lost (no exit
+ * found or the step cap), rejection caps hit.  The random numbers come from a counter-based
+ * SplitMix64 stream per particle (the reference's Sensor objects read rnd_my.txt, which it does
+ * not ship). */
+typedef struct {
+    int64_t n_in;
+    const int32_t *face;
+    const int32_t *count;
+    const double *mu;
+    uint64_t seed;
+    double v_inf, a2, kn;
+    int32_t max_steps;
+} or_mk;
+/* Fluxes of the LISM Maxwellian into the domain through the outer boundary faces (bc 2, 3):
+ * flux[f] = Get_Spotok_inf(n) * area (Setka_MK.cpp:2316-2324); count and mu by the rule of
+ * Setka_MK.cpp:1233,1380-1390 with N_per_gran = n_per_face.  Returns the number of faces used;
+ * face/count/mu sized n_faces by the caller. */
+int64_t or_mk_setup(const or_mesh *m, const or_geom *g, double v_inf, int32_t n_per_face, int32_t *face,
+                    int32_t *count, double *mu);
+int or_mk_go(const or_mesh *m, const or_mk *mk, const double *coords, const double *prim, const or_geom *g,
+             double *tally, int64_t *stats);
+
 /* Full explicit loop of Setka::Go.  coords[2][n_nodes*3], prim[2][n_cells*8] hold both time
  * layers and are updated in place; disp [n_nodes*3] is the motion field (may be NULL when not
  * moving); with fit (nullable) and move_mesh 2 the nodes move by or_culc_velocity_surface and

@@ -1,0 +1,104 @@
+"""titan_mk_go (Setka::MK_go / MK_fly_immit, Setka_MK.cpp:1225-2316) on the device against the
+oracle's or_mk_go and against what the kinetic theory fixes.  The particle counts and weights are
+integer / exact work and match the oracle exactly; the flights use the same random stream per
+particle, so most particles follow the oracle's path, but the device's transcendental functions
+differ from the C library's in the last bits and a rejection or exit test can then go the other
+way: the densities are compared statistically (MC noise ~ 1/sqrt(particles per cell))."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ME = 1.0 / 1836.15
+A2 = 0.102046
+
+
+def _cube(C, n=6):
+    mesh, prim, params = C.cube_brio_wu(n=n, jitter=0.1)
+    mesh.face_bc = np.where(mesh.face_cells[:, 1] < 0, C.BC_OUTER_SOFT, C.BC_INTERIOR).astype(np.int8)
+    return mesh, prim, params
+
+
+def _lism(C, mesh):
+    prim = np.zeros((mesh.n_cells, 8))
+    prim[:, 0] = 1.0
+    prim[:, 1] = C.V_INF
+    prim[:, 4] = 1.0 / (1.0 + ME)
+    return prim
+
+
+def _run(T, mesh, prim, params, n_per_face, seed, kn):
+    with T.Setka(mesh, dict(params, par_a_2=A2, par_Kn=kn)) as s:
+        s.set_state(prim, 0)
+        return s.MK_go(n_per_face, seed=seed, layer=0)
+
+
+def test_free_flight_matches_the_oracle_and_the_closed_form(T, C, O):
+    mesh, prim, params = _cube(C)
+    d, st = _run(T, mesh, prim, params, 200, 7, 1e30)
+    od, ost = O.mk_go(O.OracleMesh(mesh), mesh.node_xyz, prim, C.V_INF, A2, 1e30, 200, seed=7)
+    assert st["particles"] == ost["particles"] and st["exchanges"] == 0
+    assert st["left"] + st["capped"] == st["particles"]
+    assert abs(st["capped"] - ost["capped"]) <= max(3, ost["capped"] // 100)
+    assert abs(d[:, 0].mean() - 1.0) < 0.012 and np.abs(d[:, 0] - 1.0).max() < 0.15
+    # same random stream per particle: the cell densities agree far inside the MC noise
+    assert np.abs(d[:, 0] - od[:, 0]).max() < 0.03
+    assert abs(d[:, 0].mean() - od[:, 0].mean()) < 2e-3
+    np.testing.assert_allclose(d[:, 4], d[:, 0], rtol=1e-12)
+
+
+@pytest.mark.parametrize("kn", [1.0, 0.3])
+def test_detailed_balance_and_oracle(T, C, O, kn):
+    mesh, _, params = _cube(C)
+    prim = _lism(C, mesh)
+    d, st = _run(T, mesh, prim, params, 200, 11, kn)
+    od, ost = O.mk_go(O.OracleMesh(mesh), mesh.node_xyz, prim, C.V_INF, A2, kn, 200, seed=11)
+    assert st["particles"] == ost["particles"] and st["lost"] == 0
+    assert abs(st["exchanges"] - ost["exchanges"]) < 0.02 * ost["exchanges"]
+    assert abs(d[:, 0].mean() - 1.0) < 0.012 and np.abs(d[:, 0] - 1.0).max() < 0.15
+    np.testing.assert_allclose(d[:, 1:].sum(axis=1), d[:, 0], rtol=1e-12)
+    assert np.all(d[:, 1] == 0.0) and np.all(d[:, 2] == 0.0)
+    # sort shares: the device's exchanged fraction agrees with the oracle's within MC noise
+    assert abs(d[:, 3].sum() / d[:, 0].sum() - od[:, 3].sum() / od[:, 0].sum()) < 0.01
+    assert np.abs(d[:, 0] - od[:, 0]).max() < 0.12
+
+
+def test_repeat_is_reproducible_to_rounding(T, C):
+    mesh, _, params = _cube(C, n=4)
+    prim = _lism(C, mesh)
+    d1, s1 = _run(T, mesh, prim, params, 100, 5, 1.0)
+    d2, s2 = _run(T, mesh, prim, params, 100, 5, 1.0)
+    assert s1 == s2
+    np.testing.assert_allclose(d1, d2, rtol=1e-12, atol=1e-15)
+    d3, _ = _run(T, mesh, prim, params, 100, 6, 1.0)
+    assert np.abs(d3 - d1).max() > 0.0            # another seed, another sample
+
+
+def test_fitted_shell_against_the_oracle(T, C, O):
+    """The fitted-shell test mesh with its solar-wind state: the inflow enters through the outer
+    sphere (bc 2, 3) and crosses the heliosheath and the wind.  With the same random stream per
+    particle the device's volume-weighted densities lie closer to the oracle's than two oracle
+    samples of different seeds lie to each other (the MC noise of the estimate)."""
+    mesh, prim, params, rays = C.shell_fitted(zone1=False)
+    d, st = _run(T, mesh, prim, params, 100, 13, 40.0906)
+    od, ost = O.mk_go(O.OracleMesh(mesh), mesh.node_xyz, prim, C.V_INF, A2, 40.0906, 100, seed=13)
+    assert st["particles"] == ost["particles"] > 0
+    assert st["lost"] == 0 and ost["lost"] == 0
+    assert st["left"] + st["absorbed"] + st["capped"] == st["particles"]
+    assert abs(st["exchanges"] - ost["exchanges"]) < 0.03 * ost["exchanges"]
+    o2, _ = O.mk_go(O.OracleMesh(mesh), mesh.node_xyz, prim, C.V_INF, A2, 40.0906, 100, seed=14)
+    vol = O.measure(O.OracleMesh(mesh), mesh.node_xyz)["cell_volume"]
+    for k in (0, 3, 4):
+        same = (np.abs(d[:, k] - od[:, k]) * vol).sum()
+        noise = (np.abs(o2[:, k] - od[:, k]) * vol).sum()
+        assert same < 0.5 * noise, (k, same, noise)
+
+
+def test_bad_arguments(T, C):
+    mesh, prim, params = _cube(C, n=4)
+    with T.Setka(mesh, params) as s:
+        s.set_state(prim, 0)
+        for kw in (dict(n_per_face=0), dict(n_per_face=10, max_steps=0), dict(n_per_face=10, layer=2)):
+            with pytest.raises(T.TitanError) as e:
+                s.MK_go(**kw)
+            assert e.value.code == T.TITAN_E_ARG

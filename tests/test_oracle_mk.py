@@ -1,0 +1,104 @@
+"""The oracle's Monte-Carlo neutrals (or_mk_go: Setka::MK_go / MK_fly_immit, Setka_MK.cpp:1225-2316)
+pinned by what the kinetic theory fixes, on the cube with every boundary face an outer boundary
+(bc 3) so the LISM hydrogen enters through all of them:
+
+* free flight (no charge exchange): a convex domain filled from every face by the inflow of a
+  drifting Maxwellian holds that Maxwellian, so the density is the LISM density 1 in every cell.
+  A wrong inflow (Get_Spotok_inf), a birth velocity not weighted by |V.n| (Velosity_initial) or a
+  birth point not uniform over the face (Get_Random_pozition) moves the cell densities off 1.
+* detailed balance: with protons in the same Maxwellian (rho_Th = 1, V = (v_inf, 0, 0),
+  cp = sqrt(2 p_Th / rho_Th) = 1) charge exchange leaves the distribution unchanged, so the density
+  stays 1 while the exchanged atoms move to the cells' zone (sort 3).  A new velocity drawn without
+  the g sigma(g) weighting of M_K_Change_Velosity, or in a wrongly rotated frame, breaks it.
+* bookkeeping: the sorts sum to the total; every particle ends once (left, absorbed, lost or
+  dropped at birth); the particle count follows the rule of Setka_MK.cpp:1233,1380-1390."""
+import math
+
+import numpy as np
+import pytest
+
+ME = 1.0 / 1836.15
+
+
+def _cube(C, n=6):
+    mesh, prim, params = C.cube_brio_wu(n=n, jitter=0.1)
+    mesh.face_bc = np.where(mesh.face_cells[:, 1] < 0, C.BC_OUTER_SOFT, C.BC_INTERIOR).astype(np.int8)
+    return mesh, prim, params
+
+
+def _lism(C, mesh):
+    prim = np.zeros((mesh.n_cells, 8))
+    prim[:, 0] = 1.0
+    prim[:, 1] = C.V_INF
+    prim[:, 4] = 1.0 / (1.0 + ME)
+    return prim
+
+
+def _ends(st):
+    return st["left"] + st["absorbed"] + st["lost"]
+
+
+def test_free_flight_holds_the_lism_density(C, O):
+    mesh, prim, _ = _cube(C)
+    om = O.OracleMesh(mesh)
+    d, st = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, 1e30, 200, seed=7)
+    assert st["exchanges"] == 0 and st["lost"] == 0 and st["absorbed"] == 0
+    dropped = st["capped"]
+    assert _ends(st) + dropped == st["particles"]
+    # MC noise of one cell ~ 0.03 at 200 particles per face; the mean over 216 cells ~ 0.002
+    assert abs(d[:, 0].mean() - 1.0) < 0.012
+    assert np.abs(d[:, 0] - 1.0).max() < 0.15
+    np.testing.assert_allclose(d[:, 4], d[:, 0], rtol=1e-12)
+    assert np.all(d[:, 1:4] == 0.0)
+
+
+@pytest.mark.parametrize("kn", [1.0, 0.3])
+def test_detailed_balance(C, O, kn):
+    mesh, _, _ = _cube(C)
+    prim = _lism(C, mesh)
+    om = O.OracleMesh(mesh)
+    d, st = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, kn, 200, seed=11)
+    assert st["exchanges"] > 10000
+    assert abs(d[:, 0].mean() - 1.0) < 0.012
+    assert np.abs(d[:, 0] - 1.0).max() < 0.15
+    np.testing.assert_allclose(d[:, 1:].sum(axis=1), d[:, 0], rtol=1e-12)
+    assert np.all(d[:, 1] == 0.0) and np.all(d[:, 2] == 0.0)
+    assert d[:, 3].mean() > 0.15                    # exchanged atoms carry the zone's sort
+
+
+def test_exchange_rate_moves_with_kn(C, O):
+    """Shorter mean free path, more exchanges and a larger share of sort 3 (monotone in 1/Kn)."""
+    mesh, _, _ = _cube(C, n=4)
+    prim = _lism(C, mesh)
+    om = O.OracleMesh(mesh)
+    shares, counts = [], []
+    for kn in (3.0, 1.0, 0.3):
+        d, st = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, kn, 100, seed=3)
+        shares.append(d[:, 3].sum() / d[:, 0].sum())
+        counts.append(st["exchanges"])
+    assert counts[0] < counts[1] < counts[2]
+    assert shares[0] < shares[1] < shares[2]
+
+
+def test_particle_counts(C, O):
+    """The count rule: mu_expect = total inflow / (N * faces); each face gets int(inflow/mu) + 1
+    but at least min(N, 100) particles; faces whose inflow is below 1e-7 of the mean are skipped."""
+    mesh, prim, _ = _cube(C, n=4)
+    om = O.OracleMesh(mesh)
+    geom = O.measure(om, mesh.node_xyz)
+    face, count, mu = O.mk_setup(om, geom, C.V_INF, 50)
+    b = np.flatnonzero(mesh.face_bc == C.BC_OUTER_SOFT)
+    n = geom["face_normal"][b]
+    ux = C.V_INF * n[:, 0]
+    sp = 1.77245385
+    flux = -(-np.exp(-ux * ux) + sp * ux * np.array([math.erfc(u) for u in ux])) / (2 * sp)
+    flux *= geom["face_area"][b]
+    keep = flux >= 1e-7 * flux.sum() / len(b)
+    np.testing.assert_array_equal(face, b[keep])
+    want = np.maximum((flux[keep] / (flux.sum() / (50 * len(b)))).astype(np.int64) + 1, 50)
+    np.testing.assert_array_equal(count, want)
+    np.testing.assert_allclose(mu * count, flux[keep], rtol=1e-13)
+    # a drifting Maxwellian's outflow through a closed surface equals its inflow (the net flux
+    # V . sum(n A) vanishes): the one-sided inflow formula against the outflow formula
+    out = (np.exp(-ux * ux) / (2 * sp) + ux * np.array([math.erfc(-u) for u in ux]) / 2)
+    np.testing.assert_allclose(flux.sum(), (out * geom["face_area"][b]).sum(), rtol=1e-9)

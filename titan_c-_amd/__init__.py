@@ -114,6 +114,7 @@ _SIGS = {
     "titan_tvd_cells": (_I, [_P, _P]),
     "titan_chlld_batch": (_I, [_I64, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _I]),
     "titan_sources": (_I, [_P, _I, _I, _P]),
+    "titan_mk_go": (_I, [_P, _I, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, _P, _P]),
     "titan_cx_exact_batch": (_I, [_I64, _P, _I, _P]),
     "titan_mlp_n_params": (_I64, []),
     "titan_mlp_set": (_I, [_P, _P, _P, _P, _P]),
@@ -468,6 +469,17 @@ class Setka:
         out = np.empty((self.n_cells, 5))
         _check(lib().titan_sources(self.ctx, layer, mode, _ptr(out)), "titan_sources", self.ctx)
         return out
+
+    def MK_go(self, n_per_face, seed=1, max_steps=100000, layer=None):
+        """Setka::MK_go (Setka_MK.cpp:1225-2316) for the interstellar hydrogen: returns (density
+        [n_cells, 5] = MK_n_H, MK_n_H1..MK_n_H4 per unit volume, stats dict)."""
+        layer = self.current_layer() if layer is None else layer
+        out = np.empty((self.n_cells, 5))
+        st = np.zeros(6, np.int64)
+        _check(lib().titan_mk_go(self.ctx, int(layer), int(n_per_face), int(seed) & (2**64 - 1), int(max_steps),
+                                 _ptr(out), _ptr(st)), "titan_mk_go", self.ctx)
+        keys = ("particles", "exchanges", "left", "absorbed", "lost", "capped")
+        return out, dict(zip(keys, (int(v) for v in st)))
 
     def set_timing(self, on=True):
         _check(lib().titan_set_timing(self.ctx, 1 if on else 0), "titan_set_timing", self.ctx)

@@ -3,7 +3,7 @@
 Usage: python titan_c-_amd/build.py [--jobs N]
 
 Flags per translation unit:
-  * FP64 parity kernels (k_step64, k_cx, k_fit): -ffp-contract=off, IEEE division / square root, so the
+  * FP64 parity kernels (k_step64, k_cx, k_fit, k_mk): -ffp-contract=off, IEEE division / square root, so the
     arithmetic is evaluated in the reference's operation order;
   * mixed-precision flux (k_step32): contracted FP32 with fast division / square root;
   * every HIP unit: packed-FP32 instructions disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
@@ -34,6 +34,7 @@ UNITS = {
     "k_step64.hip": ["-ffp-contract=off"],
     "k_cx.hip": ["-ffp-contract=off"],
     "k_fit.hip": ["-ffp-contract=off"],
+    "k_mk.hip": ["-ffp-contract=off"],
     "k_step32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "k_mlp.hip": ["-ffp-contract=fast"],
 }

@@ -98,6 +98,21 @@ struct FitDev {
     double phi_lim;              // HP sphere smoothing where the polar angle is <= phi_a + pi/180
 };
 
+// Monte-Carlo neutrals (Setka::MK_go, Setka_MK.cpp:1225-2316): particles are numbered face by face
+// over the injection faces (ascending reference face id); first[q] is the id of face q's first
+// particle, first[n_in] the particle count.
+struct MkDev {
+    int64_t n_in, n_particles;
+    const int32_t *face;        // [n_in] device face of each injection face
+    const int64_t *first;       // [n_in+1]
+    const double *mu;           // [n_in] weight of each of the face's particles
+    uint64_t seed;
+    double v_inf, a2, kn;
+    int32_t max_steps, pad;
+    double *tally;              // [Nc*5] sum over sorts, then sorts 1..4 of mu * time
+    unsigned long long *stats;  // [6] particles, charge exchanges, left, absorbed, lost, capped
+};
+
 // Failures of a step, ranked in the order the reference meets them -- the surface fit before the
 // face loop (Setka_physics.cpp:1616-1627), the face loop before the cell loop -- so that the
 // reported one does not depend on which workgroup ran first; within a rank the smallest reference

@@ -1,0 +1,343 @@
+// k_mk.hip -- Monte-Carlo neutral densities: the flight loop of Setka::MK_go / MK_fly_immit
+// (Setka_MK.cpp:1225-2316) for the interstellar hydrogen that enters through the outer boundary.
+//
+// One thread per particle.  A particle is born on an outer boundary face (Gran::Get_Random_pozition,
+// Setka::Velosity_initial), flies straight through the cells -- the exit face from the two
+// triangles of each face (Setka::Time_to_vilet, Gran::Luch_crossing, Gran::rayTriangleIntersect) --
+// while its optical depth grows by l / (|V| / nu), nu = rho_Th Velosity_1(u, cp) sigma / Kn, and
+// charge-exchanges where the depth passes -ln(1 - xi): its velocity becomes the proton's
+// (Setka::M_K_Change_Velosity, in the spherical frame of its position) and its sort the zone of the
+// cell.  Every cell it crosses gains mu * time in the sum over sorts and in its sort
+// (Cell::MK_Add_particle).  The particle ends at a boundary face.  Differences from the
+// reference, which keep every loop bounded: a particle that leaves through the inner sphere is
+// absorbed (the reference carries it across the Sun's sphere, Setka_MK.cpp:2140-2160); the birth
+// sampler gives up after 10000 tries and drops the particle (only on faces where the flow leaves
+// the domain, whose inflow is ~exp(-V_inf^2) of the upwind faces'); a particle that finds no exit
+// after 11 nudges towards the cell centre, or exceeds the step cap, is counted lost.  The tallies
+// are FP64 atomic sums (the order of additions varies; the estimate is statistical).
+#define TITAN_FLUX_ONLY 1
+#include "step_kernels.h"
+#include "launch.h"
+
+namespace titan {
+
+namespace {
+inline unsigned grid_of(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+}  // namespace
+
+constexpr double kMkSqrtPi = 1.77245385;  // sqrtpi_, Header.h:90 (the reference's literal)
+
+__device__ __forceinline__ uint64_t mk_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// per-particle counter stream, values in (0, 1)
+__device__ __forceinline__ double mk_rand(uint64_t &s) {
+    s += 0x9E3779B97F4A7C15ull;
+    return ((double)(mk_mix(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// Velosity_1 (Setka_MK.cpp:40-50)
+__device__ __forceinline__ double mk_velosity_1(double u, double cp) {
+    if (u < 0.00001)
+        return 2.0 * cp / kMkSqrtPi + 2.0 * u * u / (3.0 * cp * kMkSqrtPi) -
+               u * u * u * u / (15.0 * cp * cp * cp * kMkSqrtPi);
+    return exp(-u * u / (cp * cp)) * cp / kMkSqrtPi + (u + (cp * cp) / (2.0 * u)) * erf(u / cp);
+}
+
+__device__ __forceinline__ double mk_sigma(double a2, double x) {
+    const double t = 1.0 - a2 * log(x);
+    return t * t;
+}
+
+// Gran::rayTriangleIntersect (Gran.cpp:522-561)
+__device__ __forceinline__ bool mk_ray_tri(const double o[3], const double d[3], const double *v0, const double *v1,
+                                           const double *v2, double &t) {
+    const double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+    const double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+    const double pv[3] = {d[1] * e2[2] - d[2] * e2[1], d[2] * e2[0] - d[0] * e2[2], d[0] * e2[1] - d[1] * e2[0]};
+    const double det = (e1[0] * pv[0] + e1[1] * pv[1]) + e1[2] * pv[2];
+    if (fabs(det) < 1e-6) return false;
+    const double inv = 1.0 / det;
+    const double tv[3] = {o[0] - v0[0], o[1] - v0[1], o[2] - v0[2]};
+    const double u = ((tv[0] * pv[0] + tv[1] * pv[1]) + tv[2] * pv[2]) * inv;
+    if (u < 0.0 || u > 1.0) return false;
+    const double qv[3] = {tv[1] * e1[2] - tv[2] * e1[1], tv[2] * e1[0] - tv[0] * e1[2], tv[0] * e1[1] - tv[1] * e1[0]};
+    const double v = ((d[0] * qv[0] + d[1] * qv[1]) + d[2] * qv[2]) * inv;
+    if (v < 0.0 || u + v > 1.0) return false;
+    t = ((e2[0] * qv[0] + e2[1] * qv[1]) + e2[2] * qv[2]) * inv;
+    return t > 1e-6;
+}
+
+// Setka::Time_to_vilet / Gran::Luch_crossing: the earliest crossing of the cell's faces
+__device__ __forceinline__ bool mk_exit(const DevMesh &M, const double *__restrict__ X, int64_t c, const double P[3],
+                                        const double V[3], double &time, int64_t &face) {
+    const double nv = sqrt((V[0] * V[0] + V[1] * V[1]) + V[2] * V[2]);
+    const double d[3] = {V[0] / nv, V[1] / nv, V[2] / nv};
+    double tmin = 1e10;
+    bool found = false;
+#pragma unroll 1
+    for (int j = 0; j < 6; j++) {
+        const int64_t f = M.cell_faces[6 * c + j];
+        const int32_t *fn = M.face_nodes + 4 * f;
+        const double *x0 = X + 3 * (int64_t)fn[0], *x1 = X + 3 * (int64_t)fn[1], *x2 = X + 3 * (int64_t)fn[2],
+                     *x3 = X + 3 * (int64_t)fn[3];
+        double t1 = 0.0, t2 = 0.0;
+        const bool b1 = mk_ray_tri(P, d, x0, x1, x2, t1);
+        t1 /= nv;
+        const bool b2 = mk_ray_tri(P, d, x0, x2, x3, t2);
+        t2 /= nv;
+        if (!b1 && !b2) continue;
+        const double tf = (b1 && b2) ? smin(t1, t2) : (b1 ? t1 : t2);
+        if (tmin > tf) { tmin = tf; face = f; found = true; }
+    }
+    time = tmin;
+    return found;
+}
+
+// spherical components in the frame with polar axis x at the position X (spherical_skorost with
+// (z, x, y) = (x0, x1, x2), Help_funk.cpp:108-121) and back (dekard_skorost, :123-135); the frame's
+// sines and cosines are taken once per exchange (sincos gives the values of sin and cos)
+struct MkFrame {
+    double st, ct, sp, cp;
+};
+__device__ __forceinline__ MkFrame mk_frame(const double X[3]) {
+    const double rr = sqrt(X[1] * X[1] + X[2] * X[2] + X[0] * X[0]);
+    const double the = acos(X[0] / rr);
+    const double phi = polar_angle(X[1], X[2]);
+    MkFrame F;
+    sincos(the, &F.st, &F.ct);
+    sincos(phi, &F.sp, &F.cp);
+    return F;
+}
+__device__ __forceinline__ void mk_to_sph(const MkFrame &F, const double Vc[3], double &r, double &th, double &ph) {
+    const double Vz = Vc[0], Vx = Vc[1], Vy = Vc[2];
+    r = Vx * F.st * F.cp + Vy * F.st * F.sp + Vz * F.ct;
+    th = Vx * F.ct * F.cp + Vy * F.ct * F.sp - Vz * F.st;
+    ph = -Vx * F.sp + Vy * F.cp;
+}
+__device__ __forceinline__ void mk_from_sph(const MkFrame &F, double r, double th, double ph, double Vc[3]) {
+    Vc[1] = r * F.st * F.cp + th * F.ct * F.cp - ph * F.sp;
+    Vc[2] = r * F.st * F.sp + th * F.ct * F.sp + ph * F.cp;
+    Vc[0] = r * F.ct - th * F.st;
+}
+
+// Setka::M_K_Change_Velosity (Setka_MK.cpp:2788-2850), in units of cp; capped at 10000 tries
+__device__ __forceinline__ void mk_change_velosity(uint64_t &rs, double a2, const double U[3], const double V[3],
+                                                   double W[3], double cp, bool &capped) {
+    const double X = sqrt((V[0] - U[0]) * (V[0] - U[0]) + (V[1] - U[1]) * (V[1] - U[1]) + (V[2] - U[2]) * (V[2] - U[2]));
+    const double p4 = 0.5 * kMkSqrtPi * X / (1.0 + 0.5 * kMkSqrtPi * X);
+    const double sX = mk_sigma(a2, X * cp);
+    double v1 = 0.0, v2 = 0.0, v3 = 0.0;
+#pragma unroll 1
+    for (int tries = 0; tries < 10000; tries++) {
+        const double k1 = mk_rand(rs), k2 = mk_rand(rs), k3 = mk_rand(rs), k4 = mk_rand(rs), k5 = mk_rand(rs),
+                     k6 = mk_rand(rs);
+        double y1, y2, y3;
+        double s5, c5;
+        sincos(2.0 * kPi * k5, &s5, &c5);
+        if (p4 < k1) {
+            const double om1 = 1.0 - 2.0 * k4;
+            const double om2 = sqrt(1.0 - om1 * om1) * c5;
+            const double om3 = sqrt(1.0 - om1 * om1) * s5;
+            const double lo = sqrt(-log(k2 * k3));
+            y1 = lo * om1; y2 = lo * om2; y3 = lo * om3;
+        } else {
+            y1 = sqrt(-log(k2)) * cos(kPi * k3);
+            y2 = sqrt(-log(k4)) * c5;
+            y3 = sqrt(-log(k4)) * s5;
+        }
+        v1 = y1 + U[0]; v2 = y2 + U[1]; v3 = y3 + U[2];
+        const double u1 = V[0] - v1, u2 = V[1] - v2, u3 = V[2] - v3;
+        const double uuu = sqrt(u1 * u1 + u2 * u2 + u3 * u3);
+        const double yy = sqrt(y1 * y1 + y2 * y2 + y3 * y3);
+        const double h = (uuu * mk_sigma(a2, uuu * cp)) / (sX * (X + yy));
+        if (!(h < k6)) { W[0] = v1; W[1] = v2; W[2] = v3; return; }
+    }
+    capped = true;
+    W[0] = v1; W[1] = v2; W[2] = v3;
+}
+
+// Setka::Velosity_initial (Setka_MK.cpp:2671-2723); false when the sampler gives up
+__device__ __forceinline__ bool mk_velosity_initial(uint64_t &rs, double v_inf, const double n[3], double V[3]) {
+    double t[3], m[3];
+    get_bazis<double>(n, t, m);
+    const double Vi[3] = {v_inf * n[0] + 0.0 * n[1] + 0.0 * n[2], v_inf * t[0] + 0.0 * t[1] + 0.0 * t[2],
+                          v_inf * m[0] + 0.0 * m[1] + 0.0 * m[2]};
+    const double p1 = fabs(Vi[0]) * kMkSqrtPi / (1.0 + fabs(Vi[0]) * kMkSqrtPi);
+    double z = 0.0;
+    bool ok = false;
+#pragma unroll 1
+    for (int tries = 0; tries < 10000; tries++) {
+        const double k3 = mk_rand(rs), k4 = mk_rand(rs), k5 = mk_rand(rs), k6 = mk_rand(rs);
+        if (p1 > k3) z = cos(kPi * k5) * sqrt(-log(k4));
+        else if (k4 <= 0.5) z = -sqrt(-log(2.0 * k4));
+        else z = sqrt(-log(2.0 * (1.0 - k4)));
+        if (!(fabs(z + Vi[0]) / (fabs(Vi[0]) + fabs(z)) < k6 || z > -Vi[0])) { ok = true; break; }
+    }
+    if (!ok) return false;
+    const double V1 = z + Vi[0];
+    const double k1 = mk_rand(rs), k2 = mk_rand(rs);
+    const double a = sqrt(-log(k2));
+    double s1, c1;
+    sincos(2.0 * kPi * k1, &s1, &c1);
+    const double V2 = Vi[1] + a * c1;
+    const double V3 = Vi[2] + a * s1;
+    for (int i = 0; i < 3; i++) V[i] = V1 * n[i] + V2 * t[i] + V3 * m[i];
+    return true;
+}
+
+// normal and area of each candidate injection face, for the host's particle counts
+__global__ void __launch_bounds__(256) k_mk_gather(int64_t n, const int32_t *__restrict__ faces, DevGeom G,
+                                                   double *__restrict__ na) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t f = faces[i];
+    na[4 * i] = G.face_normal[3 * f];
+    na[4 * i + 1] = G.face_normal[3 * f + 1];
+    na[4 * i + 2] = G.face_normal[3 * f + 2];
+    na[4 * i + 3] = G.face_area[f];
+}
+
+// tallies per unit volume
+__global__ void __launch_bounds__(256) k_mk_finish(int64_t n_cells, const double *__restrict__ vol,
+                                                   double *__restrict__ tally) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 5 * n_cells) return;
+    tally[i] = tally[i] / vol[i / 5];
+}
+
+__global__ void __launch_bounds__(256) k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X,
+                                                      const double *__restrict__ prim, DevGeom G) {
+    const int64_t pid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= K.n_particles) return;
+    // injection face: the last q with first[q] <= pid
+    int lo = 0, hi = (int)K.n_in - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (K.first[mid] <= pid) lo = mid;
+        else hi = mid - 1;
+    }
+    const int q = lo;
+    const int64_t f = K.face[q];
+    const double n[3] = {G.face_normal[3 * f], G.face_normal[3 * f + 1], G.face_normal[3 * f + 2]};
+    const int32_t *fn = M.face_nodes + 4 * f;
+    uint64_t rs = mk_mix(K.seed ^ mk_mix((uint64_t)pid + 0x9E3779B97F4A7C15ull));
+    // Gran::Get_Random_pozition (Gran.cpp:220-285)
+    const double *A = X + 3 * (int64_t)fn[0], *B1 = X + 3 * (int64_t)fn[1], *Cc = X + 3 * (int64_t)fn[2],
+                 *B2 = X + 3 * (int64_t)fn[3];
+    double s0, s1;
+    {
+        double v1x = B1[0] - A[0], v1y = B1[1] - A[1], v1z = B1[2] - A[2];
+        const double v2x = Cc[0] - A[0], v2y = Cc[1] - A[1], v2z = Cc[2] - A[2];
+        double cx = v1y * v2z - v1z * v2y, cy = v1z * v2x - v1x * v2z, cz = v1x * v2y - v1y * v2x;
+        s0 = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+        v1x = B2[0] - A[0]; v1y = B2[1] - A[1]; v1z = B2[2] - A[2];
+        cx = v1y * v2z - v1z * v2y; cy = v1z * v2x - v1x * v2z; cz = v1x * v2y - v1y * v2x;
+        s1 = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+    }
+    const double S = s0 + s1;
+    const double ksi = mk_rand(rs);
+    const double *T1 = (ksi <= s0 / S) ? B1 : B2;
+    const double r1 = mk_rand(rs), r2 = mk_rand(rs);
+    double P[3], V[3];
+    for (int i = 0; i < 3; i++) P[i] = (1.0 - sqrt(r1)) * A[i] + (sqrt(r1) * (1.0 - r2)) * T1[i] + (r2 * sqrt(r1)) * Cc[i];
+    unsigned long long *st = K.stats;
+    if (!mk_velosity_initial(rs, K.v_inf, n, V)) {  // no admissible velocity within the cap: dropped
+        atomicAdd(&st[0], 1ull);
+        atomicAdd(&st[5], 1ull);
+        return;
+    }
+    int64_t c = M.face_cells[2 * f];
+    {
+        const double *cc = G.cell_center + 3 * c;
+        for (int i = 0; i < 3; i++) P[i] = P[i] + (cc[i] - P[i]) / 1000.0;  // into the cell
+    }
+    double KSI = -log(1.0 - mk_rand(rs)), I_do = 0.0;
+    const double mu = K.mu[q];
+    int sort = 4;
+    bool capped = false;
+    unsigned long long n_ex = 0;
+    int outcome = 4;  // 2 left, 3 absorbed, 4 lost
+    const double me = 1.0 / 1836.15;
+#pragma unroll 1
+    for (int it = 0; it < K.max_steps; it++) {
+        double time = 0.0;
+        int64_t ex = -1;
+        bool found = false;
+#pragma unroll 1
+        for (int k1 = 0; k1 < 12 && !found; k1++) {
+            found = mk_exit(M, X, c, P, V, time, ex);
+            if (!found) {
+                const double *cn = G.cell_center + 3 * c;
+                for (int i = 0; i < 3; i++) P[i] = (k1 < 10) ? P[i] + (cn[i] - P[i]) / 800.0 : cn[i];
+            }
+        }
+        if (!found) break;
+        const double *s = prim + 8 * c;
+        double rho_Th = s[0] / (1.0 + me), p_Th = s[4] / 2.0;
+        if (rho_Th <= 1e-8) rho_Th = 1e-8;
+        if (p_Th <= 1e-8 / 2.0) p_Th = 1e-8 / 2.0;
+        const double ro = rho_Th, cp = sqrt(2.0 * p_Th / rho_Th);
+        const double l = sqrt((time * V[0]) * (time * V[0]) + (time * V[1]) * (time * V[1]) + (time * V[2]) * (time * V[2]));
+        double I = I_do;
+        const double u = sqrt((V[0] - s[1]) * (V[0] - s[1]) + (V[1] - s[2]) * (V[1] - s[2]) + (V[2] - s[3]) * (V[2] - s[3]));
+        const double vn = sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
+        const double uz = mk_velosity_1(u, cp);
+        const double nu_ex = ro * uz * mk_sigma(K.a2, uz) / K.kn;
+        const double sig = vn / nu_ex;
+        I += l / sig;
+        if (I < KSI) {
+            I_do = I;
+            atomicAdd(&K.tally[5 * c], time * mu);
+            if (sort >= 1 && sort <= 4) atomicAdd(&K.tally[5 * c + sort], time * mu);
+            for (int i = 0; i < 3; i++) P[i] += 1.000001 * time * V[i];
+            const int64_t a = M.face_cells[2 * ex], b = M.face_cells[2 * ex + 1];
+            const int64_t nc = a == c ? b : a;
+            if (nc < 0) {
+                outcome = (M.face_code[ex] & 15u) == 1u ? 3 : 2;
+                break;
+            }
+            c = nc;
+        } else {
+            const double kk = (KSI - I_do) * sig;
+            const double t_ex = kk / vn;
+            I_do = 0.0;
+            for (int i = 0; i < 3; i++) P[i] += t_ex * V[i];
+            atomicAdd(&K.tally[5 * c], t_ex * mu);
+            if (sort >= 1 && sort <= 4) atomicAdd(&K.tally[5 * c + sort], t_ex * mu);
+            double U[3], Vs[3], W[3];
+            const double Vp[3] = {s[1], s[2], s[3]};
+            const MkFrame F = mk_frame(P);
+            mk_to_sph(F, Vp, U[0], U[1], U[2]);
+            mk_to_sph(F, V, Vs[0], Vs[1], Vs[2]);
+            for (int i = 0; i < 3; i++) { U[i] /= cp; Vs[i] /= cp; }
+            mk_change_velosity(rs, K.a2, U, Vs, W, cp, capped);
+            for (int i = 0; i < 3; i++) W[i] *= cp;
+            mk_from_sph(F, W[0], W[1], W[2], V);
+            sort = (int)((M.cell_code[c] >> 8) & 15u);
+            KSI = -log(1.0 - mk_rand(rs));
+            n_ex++;
+        }
+    }
+    atomicAdd(&st[0], 1ull);
+    if (n_ex) atomicAdd(&st[1], n_ex);
+    atomicAdd(&st[outcome], 1ull);
+    if (capped) atomicAdd(&st[5], 1ull);
+}
+
+void launch_mk_gather(hipStream_t s, int64_t n, const int32_t *faces, const DevGeom &G, double *na) {
+    if (n > 0) hipLaunchKernelGGL(k_mk_gather, dim3(grid_of(n, 256)), dim3(256), 0, s, n, faces, G, na);
+}
+
+void launch_mk_particles(hipStream_t s, const DevMesh &M, const MkDev &K, const double *X, const double *prim,
+                         const DevGeom &G) {
+    if (K.n_particles > 0)
+        hipLaunchKernelGGL(k_mk_particles, dim3(grid_of(K.n_particles, 256)), dim3(256), 0, s, M, K, X, prim, G);
+    if (M.n_cells > 0)
+        hipLaunchKernelGGL(k_mk_finish, dim3(grid_of(5 * M.n_cells, 256)), dim3(256), 0, s, M.n_cells,
+                           (const double *)G.cell_volume, K.tally);
+}
+
+}  // namespace titan

@@ -38,6 +38,10 @@ size_t step_tile_lds_bytes(const TileMesh &TM, bool moving);
 void launch_fit_surface(hipStream_t s, const DevMesh &M, const DevParams &P, const FitDev &F, const double *prim,
                         const double *X1, double *X2, const double *cell_center, StepScalars *S, double time);
 void launch_fit_rays(hipStream_t s, const FitDev &F, double *X);
+// k_mk.hip: Monte-Carlo neutral densities
+void launch_mk_gather(hipStream_t s, int64_t n, const int32_t *faces, const DevGeom &G, double *na);
+void launch_mk_particles(hipStream_t s, const DevMesh &M, const MkDev &K, const double *X, const double *prim,
+                         const DevGeom &G);
 
 // chlld over n independent states; err (device int) set to 1 on a fatal branch
 void launch_chlld_batch(hipStream_t s, int precision, int64_t n, int K, const int8_t *n_state,

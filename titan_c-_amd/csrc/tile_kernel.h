@@ -412,8 +412,8 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
 }
 
 // SUR = true: the charge-exchange source comes from the surrogate network, evaluated in phase 4
-// on the matrix cores (one 32-cell MFMA pass per wavefront, weights staged in the flux slots once
-// every flux sum is taken).  Cells outside the training box are not updated here: their flux
+// on the matrix cores (one 32-cell MFMA pass per wavefront, its operands read from global memory
+// and cache-resident across the tiles).  Cells outside the training box are not updated here: their flux
 // sums and volumes go to sur.stash at a slot of S->n_fallback, and k_cx_fixup redoes their update
 // with the exact source.  SUR = false: src (nullable) holds a precomputed source per cell.
 template <typename RT, bool SUR, bool TVD>

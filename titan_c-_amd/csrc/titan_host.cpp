@@ -976,6 +976,105 @@ extern "C" int titan_sources(titan_ctx *c, int layer, int mode, double *out) {
     return TITAN_OK;
 }
 
+// ---- Monte-Carlo neutral densities (kernels in k_mk.hip) --------------------------------------
+// Get_Spotok_inf (Setka_MK.cpp:2316-2324): inflow of the unit-temperature LISM Maxwellian drifting
+// at (v_inf, 0, 0) through a face of outward normal n, per unit area
+static double mk_spotok_inf(double v_inf, const double *n) {
+    const double sqrtpi = 1.77245385;  // sqrtpi_, Header.h:90
+    const double Ux = v_inf * n[0] + 0.0 * n[1] + 0.0 * n[2];
+    return -(-std::exp(-(Ux * Ux)) + sqrtpi * Ux * std::erfc(Ux)) / (2.0 * sqrtpi);
+}
+
+extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t seed, int32_t max_steps,
+                           double *density, int64_t *stats) {
+    if (!c || layer < 0 || layer > 1 || n_per_face < 1 || max_steps < 1 || !density) return TITAN_E_ARG;
+    if (!(c->params.par_Kn > 0.0)) return TITAN_E_ARG;
+    const int64_t nc = c->nc, nf = c->nf;
+    // candidate injection faces: the outer boundary (bc 2, 3), ascending reference face id
+    std::vector<uint32_t> code(nf);
+    HIPCHK(hipMemcpyAsync(code.data(), c->d_face_code, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<int32_t> cand;
+    for (int64_t r = 0; r < nf; r++) {
+        const int32_t d = c->ord.face_r2d[r];
+        const uint32_t bc = code[d] & 15u;
+        if (bc == TITAN_BC_OUTER_HARD || bc == TITAN_BC_OUTER_SOFT) cand.push_back(d);
+    }
+    int64_t st[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<double> h(5 * nc, 0.0);
+    if (!cand.empty()) {
+        const int64_t nb = (int64_t)cand.size();
+        // geometry of the layer (the fused step keeps only the cell centres current)
+        launch_measure(c->stream, c->mesh(), c->d_X[layer], c->geom[layer], c->params.geom_mode);
+        DevBuf dcand, dna;
+        int rc = dup(dcand, cand.data(), sizeof(int32_t) * nb);
+        if (rc == TITAN_OK) rc = alloc(dna, sizeof(double) * 4 * nb);
+        if (rc != TITAN_OK) return rc;
+        launch_mk_gather(c->stream, nb, (const int32_t *)dcand.p, c->geom[layer], (double *)dna.p);
+        HIPCHK(hipGetLastError());
+        std::vector<double> na(4 * nb);
+        HIPCHK(hipMemcpyAsync(na.data(), dna.p, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        // particles per face and weights (Setka_MK.cpp:1233,1380-1390 with N_per_gran = n_per_face)
+        const double v_inf = c->params.v_inf;
+        double total = 0.0;
+        for (int64_t i = 0; i < nb; i++) total += mk_spotok_inf(v_inf, &na[4 * i]) * na[4 * i + 3];
+        const double mu_expect = total / (1.0 * n_per_face * nb);
+        std::vector<int32_t> face;
+        std::vector<int64_t> first(1, 0);
+        std::vector<double> mu;
+        for (int64_t i = 0; i < nb; i++) {
+            const double sv = mk_spotok_inf(v_inf, &na[4 * i]) * na[4 * i + 3];
+            if (sv < 0.0000001 * total / nb) continue;
+            const double q = sv / mu_expect;
+            if (!(q < 2.0e9)) return TITAN_E_ARG;
+            int32_t n = (int32_t)q + 1;
+            const int32_t lo = n_per_face < 100 ? n_per_face : 100;
+            if (n < lo) n = lo;
+            face.push_back(cand[i]);
+            first.push_back(first.back() + n);
+            mu.push_back(sv / n);
+        }
+        MkDev K{};
+        K.n_in = (int64_t)face.size();
+        K.n_particles = first.back();
+        if (K.n_particles > (int64_t)1 << 38) return TITAN_E_ARG;  // grid of at most 2^30 workgroups
+        K.seed = seed;
+        K.v_inf = v_inf;
+        K.a2 = c->params.par_a_2;
+        K.kn = c->params.par_Kn;
+        K.max_steps = max_steps;
+        DevBuf dface, dfirst, dmu, dtally, dstats;
+        if (K.n_in > 0) {
+            rc = dup(dface, face.data(), sizeof(int32_t) * face.size());
+            if (rc == TITAN_OK) rc = dup(dfirst, first.data(), sizeof(int64_t) * first.size());
+            if (rc == TITAN_OK) rc = dup(dmu, mu.data(), sizeof(double) * mu.size());
+        }
+        if (rc == TITAN_OK) rc = alloc(dtally, sizeof(double) * 5 * nc);
+        if (rc == TITAN_OK) rc = alloc(dstats, sizeof(unsigned long long) * 6);
+        if (rc != TITAN_OK) return rc;
+        HIPCHK(hipMemsetAsync(dtally.p, 0, sizeof(double) * 5 * nc, c->stream));
+        HIPCHK(hipMemsetAsync(dstats.p, 0, sizeof(unsigned long long) * 6, c->stream));
+        K.face = (const int32_t *)dface.p;
+        K.first = (const int64_t *)dfirst.p;
+        K.mu = (const double *)dmu.p;
+        K.tally = (double *)dtally.p;
+        K.stats = (unsigned long long *)dstats.p;
+        launch_mk_particles(c->stream, c->mesh(), K, c->d_X[layer], c->d_prim[layer], c->geom[layer]);
+        HIPCHK(hipGetLastError());
+        std::vector<double> t(5 * nc);
+        HIPCHK(hipMemcpyAsync(t.data(), dtally.p, sizeof(double) * 5 * nc, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(st, dstats.p, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        h.swap(t);
+    }
+    for (int64_t d = 0; d < nc; d++)
+        for (int k = 0; k < 5; k++) density[5 * (int64_t)c->ord.cell_d2r[d] + k] = h[5 * d + k];
+    if (stats)
+        for (int i = 0; i < 6; i++) stats[i] = st[i];
+    return TITAN_OK;
+}
+
 // ---- surrogate network entry points (kernels in k_mlp.hip) ----------------------------------
 extern "C" int64_t titan_mlp_n_params(void) { return kMlpParams; }
 
