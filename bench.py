@@ -206,7 +206,10 @@ def main():
     ap.add_argument("--train-states", type=int, default=2_000_000)
     ap.add_argument("--precision-steps", type=int, default=1000)
     ap.add_argument("--variants",
-                    default="fp64,mixed,fp64_exact,mixed_surrogate,fp64_tvd,mixed_tvd,fp64_bera,fp64_fit,checkpoint")
+                    default="fp64,mixed,fp64_exact,mixed_surrogate,fp64_tvd,mixed_tvd,fp64_bera,fp64_fit,mk,checkpoint")
+    ap.add_argument("--mk-per-face", type=int, default=100,
+                    help="Monte-Carlo neutrals: particles per outer-boundary face (N_per_gran)")
+    ap.add_argument("--mk-cpu-per-face", type=int, default=1)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the streaming copy the HBM counter runs are scaled by, then exit")
     a = ap.parse_args()
@@ -278,8 +281,38 @@ def main():
         log("fp64 + closed-form source: %.3f ms/step (source kernel %.3f ms)" % (
             res["fp64_bera"]["seconds"] * 1e3 / a.steps, res["fp64_bera"]["source_kernel_ms"]))
         del hf
+    fitted = C.build("shell4m_fitted") if ("fp64_fit" in variants or "mk" in variants) else None
+    mk_info = None
+    if "mk" in variants:  # Monte-Carlo neutral densities on the fitted shell's state
+        fmesh, fprim, fparams, _ = fitted
+        with T.Setka(fmesh, fparams) as s:
+            s.set_state(fprim, 0)
+            s.MK_go(1, seed=99)  # first launch: code load
+            t0 = time.perf_counter()
+            _, st = s.MK_go(a.mk_per_face, seed=1)
+            sec = time.perf_counter() - t0
+        mk_info = {"workload": "shell4m_fitted", "n_cells": fmesh.n_cells, "n_per_face": a.mk_per_face,
+                   "stats": st, "seconds": sec, "particles_per_s": st["particles"] / sec,
+                   "segments_per_s": st["segments"] / sec}
+        log("MC neutrals: %d particles, %d segments, %d exchanges in %.3f s (%.3g segments/s)" % (
+            st["particles"], st["segments"], st["exchanges"], sec, st["segments"] / sec))
+        if not a.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as O
+            om = O.OracleMesh(fmesh)
+            geom = O.measure(om, fmesh.node_xyz)
+            t0 = time.perf_counter()
+            _, ost = O.mk_go(om, fmesh.node_xyz, fprim, fparams["v_inf"], fparams.get("par_a_2", 0.102046),
+                             fparams.get("par_Kn", 40.0906), a.mk_cpu_per_face, seed=1, geom=geom)
+            csec = time.perf_counter() - t0
+            del om, geom
+            mk_info["cpu_baseline"] = {
+                "value": ost["segments"] / csec, "unit": "segments/s", "cores": 1, "kind": "port",
+                "sample": "%d particles (%d per outer face) of the same shell4m_fitted flight, C restatement of "
+                          "MK_fly_immit, one thread" % (ost["particles"], a.mk_cpu_per_face)}
+            log("MC cpu baseline: %.3g segments/s" % (ost["segments"] / csec))
     if "fp64_fit" in variants:  # the same cell count with the TS/HP/BS fitted and the rays moved
-        fmesh, fprim, fparams, frays = C.build("shell4m_fitted")
+        fmesh, fprim, fparams, frays = fitted
         res["fp64_fit"] = run_variant(T, fmesh, fprim, dict(fparams, precision=T.PREC_FP64), a.steps, a.warmup,
                                       rays=frays)
         log("fp64 + fitted surfaces (%d cells): %.3f ms/step, step kernel %.3f ms, surface+ray kernels %.3f ms" % (
@@ -293,6 +326,7 @@ def main():
         del fmesh, fprim
     else:
         fit_info = None
+    del fitted
     ckpt = checkpoint_timing(T, mesh, prim, params) if "checkpoint" in variants else None
 
     def cups(name):
@@ -362,6 +396,7 @@ def main():
         "precision": prec,
         "checkpoint": ckpt,
         "fitted_surfaces": fit_info,
+        "mc_neutrals": mk_info,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

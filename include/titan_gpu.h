@@ -237,10 +237,11 @@ int titan_cx_exact_batch(int64_t n, const double *in10, int nodes, double *out5)
  * by optical depth with nu = rho_Th Velosity_1 sigma / par_Kn and the proton velocity of
  * Setka::M_K_Change_Velosity (a new particle takes the sort of the cell's zone).  density
  * [n_cells*5] = MK_n_H, MK_n_H1..MK_n_H4 of Cell::MK_Add_particle (Cell.cpp:1042-1083) per unit
- * volume.  stats [6] (nullable): particles, charge exchanges, left through the outer boundary,
+ * volume.  stats [7] (nullable): particles, charge exchanges, left through the outer boundary,
  * absorbed at the inner boundary (the reference carries them across the Sun's sphere), lost (no
  * exit found after 11 nudges to the cell centre, or max_steps cells crossed), rejection samplers
- * that gave up after 10000 tries (a birth that does: the particle is dropped).  Random numbers:
+ * that gave up after 10000 tries (a birth that does: the particle is dropped), flight segments
+ * (one per tally: a cell crossed, or a flight to an exchange).  Random numbers:
  * a counter-based SplitMix64 stream per particle from `seed` (the reference reads rnd_my.txt,
  * which it does not ship); particles are numbered face by face in ascending reference face id.
  * The tallies are FP64 atomic sums, so repeated calls agree to rounding, not bit for bit. */

@@ -528,18 +528,18 @@ def mk_setup(om, geom, v_inf, n_per_face):
     return face[:k].copy(), count[:k].copy(), mu[:k].copy()
 
 
-def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000):
-    """or_mk_go on the nodes xyz and state prim; returns (densities [n_cells, 5] = sum over sorts,
-    sorts 1..4, per unit volume; stats dict)."""
+def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000, geom=None):
+    """or_mk_go on the nodes xyz and state prim (geom: measure(om, xyz), computed when None);
+    returns (densities [n_cells, 5] = sum over sorts, sorts 1..4, per unit volume; stats dict)."""
     m = om.m
     x = np.ascontiguousarray(xyz, np.float64)
-    geom = measure(om, x)
+    geom = measure(om, x) if geom is None else geom
     face, count, mu = mk_setup(om, geom, v_inf, n_per_face)
     g = OrGeom(*[_p(geom[k]) for k, _ in OrGeom._fields_])
     mk = OrMk(len(face), _p(face), _p(count), _p(mu), int(seed), float(v_inf), float(a2), float(kn), int(max_steps))
     tally = np.zeros((m.n_cells, 5))
-    stats = np.zeros(6, np.int64)
+    stats = np.zeros(7, np.int64)
     lib().or_mk_go(ctypes.byref(om.c), ctypes.byref(mk), _p(x), _p(np.ascontiguousarray(prim, np.float64)),
                    ctypes.byref(g), _p(tally), _p(stats))
-    keys = ("particles", "exchanges", "left", "absorbed", "lost", "capped")
+    keys = ("particles", "exchanges", "left", "absorbed", "lost", "capped", "segments")
     return tally / geom["cell_volume"][:, None], dict(zip(keys, (int(v) for v in stats)))

@@ -1720,7 +1720,7 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
 {
     const double me = 1.0 / 1836.15;
     int64_t pid = 0;
-    for (int i = 0; i < 6; i++) stats[i] = 0;
+    for (int i = 0; i < 7; i++) stats[i] = 0;
     for (int64_t q = 0; q < mk->n_in; q++) {
         int64_t f = mk->face[q];
         const double *n = g->face_normal + 3 * f;
@@ -1768,6 +1768,7 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
                     }
                 }
                 if (!found) { stats[4]++; done = 1; break; }
+                stats[6]++;
                 const double *s = prim + 8 * c;
                 double rho_Th = s[0] / (1.0 + me), p_Th = s[4] / 2.0;
                 if (rho_Th <= 1e-8) rho_Th = 1e-8;

@@ -210,9 +210,10 @@ void or_dvigenie(const or_fit *fit, double *x);
  * (Setka_MK.cpp:1225-2316) for the LISM hydrogen that enters through the outer boundary (sort 4).
  * Injection faces: face ids, particles per face and weight per particle (or_mk_setup restates
  * the counts of Setka_MK.cpp:1233,1380-1390).  tally [n_cells*5] += mu * time per cell: the sum
- * over sorts, then sorts 1..4 (Cell::MK_Add_particle, Cell.cpp:1042-1083).  stats [6]: particles,
+ * over sorts, then sorts 1..4 (Cell::MK_Add_particle, Cell.cpp:1042-1083).  stats [7]: particles,
  * charge exchanges, left through the outer boundary, absorbed at the inner boundary, lost (no exit
- * found or the step cap), rejection caps hit.  The random numbers come from a counter-based
+ * found or the step cap), rejection caps hit.  stats[6]: flight segments (one per tally: a cell
+ * crossed or a flight to an exchange).  The random numbers come from a counter-based
  * SplitMix64 stream per particle (the reference's Sensor objects read rnd_my.txt, which it does
  * not ship). */
 typedef struct {

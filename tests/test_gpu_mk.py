@@ -40,6 +40,7 @@ def test_free_flight_matches_the_oracle_and_the_closed_form(T, C, O):
     assert st["particles"] == ost["particles"] and st["exchanges"] == 0
     assert st["left"] + st["capped"] == st["particles"]
     assert abs(st["capped"] - ost["capped"]) <= max(3, ost["capped"] // 100)
+    assert abs(st["segments"] - ost["segments"]) < 0.01 * ost["segments"]
     assert abs(d[:, 0].mean() - 1.0) < 0.012 and np.abs(d[:, 0] - 1.0).max() < 0.15
     # same random stream per particle: the cell densities agree far inside the MC noise
     assert np.abs(d[:, 0] - od[:, 0]).max() < 0.03

@@ -475,10 +475,10 @@ class Setka:
         [n_cells, 5] = MK_n_H, MK_n_H1..MK_n_H4 per unit volume, stats dict)."""
         layer = self.current_layer() if layer is None else layer
         out = np.empty((self.n_cells, 5))
-        st = np.zeros(6, np.int64)
+        st = np.zeros(7, np.int64)
         _check(lib().titan_mk_go(self.ctx, int(layer), int(n_per_face), int(seed) & (2**64 - 1), int(max_steps),
                                  _ptr(out), _ptr(st)), "titan_mk_go", self.ctx)
-        keys = ("particles", "exchanges", "left", "absorbed", "lost", "capped")
+        keys = ("particles", "exchanges", "left", "absorbed", "lost", "capped", "segments")
         return out, dict(zip(keys, (int(v) for v in st)))
 
     def set_timing(self, on=True):

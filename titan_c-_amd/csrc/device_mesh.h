@@ -110,7 +110,7 @@ struct MkDev {
     double v_inf, a2, kn;
     int32_t max_steps, pad;
     double *tally;              // [Nc*5] sum over sorts, then sorts 1..4 of mu * time
-    unsigned long long *stats;  // [6] particles, charge exchanges, left, absorbed, lost, capped
+    unsigned long long *stats;  // [7] particles, charge exchanges, left, absorbed, lost, capped, segments
 };
 
 // Failures of a step, ranked in the order the reference meets them -- the surface fit before the

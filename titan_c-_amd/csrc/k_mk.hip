@@ -14,7 +14,9 @@
 // sampler gives up after 10000 tries and drops the particle (only on faces where the flow leaves
 // the domain, whose inflow is ~exp(-V_inf^2) of the upwind faces'); a particle that finds no exit
 // after 11 nudges towards the cell centre, or exceeds the step cap, is counted lost.  The tallies
-// are FP64 atomic sums (the order of additions varies; the estimate is statistical).
+// are FP64 atomic sums (the order of additions varies; the estimate is statistical).  Persistent
+// lanes that take a new particle when theirs ends were measured slower (2.5x): a wavefront then
+// runs births, crossings and exchanges side by side.
 #define TITAN_FLUX_ONLY 1
 #include "step_kernels.h"
 #include "launch.h"
@@ -258,7 +260,7 @@ __global__ void __launch_bounds__(256) k_mk_particles(DevMesh M, MkDev K, const 
     const double mu = K.mu[q];
     int sort = 4;
     bool capped = false;
-    unsigned long long n_ex = 0;
+    unsigned long long n_ex = 0, n_seg = 0;
     int outcome = 4;  // 2 left, 3 absorbed, 4 lost
     const double me = 1.0 / 1836.15;
 #pragma unroll 1
@@ -275,6 +277,7 @@ __global__ void __launch_bounds__(256) k_mk_particles(DevMesh M, MkDev K, const 
             }
         }
         if (!found) break;
+        n_seg++;
         const double *s = prim + 8 * c;
         double rho_Th = s[0] / (1.0 + me), p_Th = s[4] / 2.0;
         if (rho_Th <= 1e-8) rho_Th = 1e-8;
@@ -323,6 +326,7 @@ __global__ void __launch_bounds__(256) k_mk_particles(DevMesh M, MkDev K, const 
     }
     atomicAdd(&st[0], 1ull);
     if (n_ex) atomicAdd(&st[1], n_ex);
+    atomicAdd(&st[6], n_seg);
     atomicAdd(&st[outcome], 1ull);
     if (capped) atomicAdd(&st[5], 1ull);
 }

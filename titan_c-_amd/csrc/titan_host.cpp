@@ -1000,7 +1000,7 @@ extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t
         const uint32_t bc = code[d] & 15u;
         if (bc == TITAN_BC_OUTER_HARD || bc == TITAN_BC_OUTER_SOFT) cand.push_back(d);
     }
-    int64_t st[6] = {0, 0, 0, 0, 0, 0};
+    int64_t st[7] = {0, 0, 0, 0, 0, 0, 0};
     std::vector<double> h(5 * nc, 0.0);
     if (!cand.empty()) {
         const int64_t nb = (int64_t)cand.size();
@@ -1051,10 +1051,10 @@ extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t
             if (rc == TITAN_OK) rc = dup(dmu, mu.data(), sizeof(double) * mu.size());
         }
         if (rc == TITAN_OK) rc = alloc(dtally, sizeof(double) * 5 * nc);
-        if (rc == TITAN_OK) rc = alloc(dstats, sizeof(unsigned long long) * 6);
+        if (rc == TITAN_OK) rc = alloc(dstats, sizeof(unsigned long long) * 7);
         if (rc != TITAN_OK) return rc;
         HIPCHK(hipMemsetAsync(dtally.p, 0, sizeof(double) * 5 * nc, c->stream));
-        HIPCHK(hipMemsetAsync(dstats.p, 0, sizeof(unsigned long long) * 6, c->stream));
+        HIPCHK(hipMemsetAsync(dstats.p, 0, sizeof(unsigned long long) * 7, c->stream));
         K.face = (const int32_t *)dface.p;
         K.first = (const int64_t *)dfirst.p;
         K.mu = (const double *)dmu.p;
@@ -1071,7 +1071,7 @@ extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t
     for (int64_t d = 0; d < nc; d++)
         for (int k = 0; k < 5; k++) density[5 * (int64_t)c->ord.cell_d2r[d] + k] = h[5 * d + k];
     if (stats)
-        for (int i = 0; i < 6; i++) stats[i] = st[i];
+        for (int i = 0; i < 7; i++) stats[i] = st[i];
     return TITAN_OK;
 }
 
