@@ -101,6 +101,14 @@ struct FitDev {
 // Monte-Carlo neutrals (Setka::MK_go, Setka_MK.cpp:1225-2316): particles are numbered face by face
 // over the injection faces (ascending reference face id); first[q] is the id of face q's first
 // particle, first[n_in] the particle count.
+struct alignas(16) MkCell {  // 240 bytes
+    double x[24];      // the cell's 8 node positions (cell_nodes order)
+    uint32_t loc[4];   // face j (device cell_faces order): its 4 nodes as 4-bit indices into x, in
+                       // bits 16 (j & 1) of loc[j >> 1], the face's own node order
+    int32_t nbr[6];    // the cell across face j, or -1 - boundary code
+    int32_t pad[2];
+};
+
 struct MkDev {
     int64_t n_in, n_particles;
     const int32_t *face;        // [n_in] device face of each injection face
@@ -109,6 +117,7 @@ struct MkDev {
     uint64_t seed;
     double v_inf, a2, kn;
     int32_t max_steps, pad;
+    const MkCell *pack;         // [Nc] flight records of the cells (k_mk_pack)
     double *tally;              // [Nc*5] sum over sorts, then sorts 1..4 of mu * time
     unsigned long long *stats;  // [7] particles, charge exchanges, left, absorbed, lost, capped, segments
 };

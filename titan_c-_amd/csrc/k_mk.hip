@@ -14,9 +14,14 @@
 // sampler gives up after 10000 tries and drops the particle (only on faces where the flow leaves
 // the domain, whose inflow is ~exp(-V_inf^2) of the upwind faces'); a particle that finds no exit
 // after 11 nudges towards the cell centre, or exceeds the step cap, is counted lost.  The tallies
-// are FP64 atomic sums (the order of additions varies; the estimate is statistical).  Persistent
-// lanes that take a new particle when theirs ends were measured slower (2.5x): a wavefront then
-// runs births, crossings and exchanges side by side.
+// are FP64 atomic sums (the order of additions varies; the estimate is statistical).
+//
+// The flight is bound by memory latency at low occupancy (245+ VGPRs).  Measured choices: each
+// cell's nodes, face-node indices and neighbours packed into one 240-byte record (one round of
+// independent loads per segment instead of 18 dependent ones); exchanges held until a quarter of
+// the wavefront's lanes have one due; a register cap of two waves per SIMD (a small spill beats
+// one wave).  Persistent lanes that take a new particle when theirs ends were slower (2.5x): a
+// wavefront then runs births, crossings and exchanges side by side.
 #define TITAN_FLUX_ONLY 1
 #include "step_kernels.h"
 #include "launch.h"
@@ -72,29 +77,52 @@ __device__ __forceinline__ bool mk_ray_tri(const double o[3], const double d[3],
     return t > 1e-6;
 }
 
-// Setka::Time_to_vilet / Gran::Luch_crossing: the earliest crossing of the cell's faces
-__device__ __forceinline__ bool mk_exit(const DevMesh &M, const double *__restrict__ X, int64_t c, const double P[3],
-                                        const double V[3], double &time, int64_t &face) {
+// Setka::Time_to_vilet / Gran::Luch_crossing: the earliest crossing of the cell's faces, each
+// split into the triangles (0,1,2) and (0,2,3) of its node order.  The cell's packed record
+// (MkCell: its 8 node positions, each face's 4 nodes as indices into them) comes in with one
+// round of independent loads; the positions go to this lane's LDS column (stride kMkThreads
+// doubles), from which each face reads its nodes by index.  jx = the exit face's slot.
+constexpr int kMkThreads = 256;
+__device__ __forceinline__ bool mk_exit(const MkCell *__restrict__ pack, int64_t c, double *__restrict__ col,
+                                        const double P[3], const double V[3], double &time, int &jx) {
+    const double2 *src = reinterpret_cast<const double2 *>(pack[c].x);
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        const double2 v = src[k];
+        col[(2 * k) * kMkThreads] = v.x;
+        col[(2 * k + 1) * kMkThreads] = v.y;
+    }
+    const uint4 lw = *reinterpret_cast<const uint4 *>(pack[c].loc);
+    const uint32_t loc[3] = {lw.x, lw.y, lw.z};
     const double nv = sqrt((V[0] * V[0] + V[1] * V[1]) + V[2] * V[2]);
     const double d[3] = {V[0] / nv, V[1] / nv, V[2] / nv};
-    double tmin = 1e10;
+    double tmin = 1e300;
     bool found = false;
 #pragma unroll 1
     for (int j = 0; j < 6; j++) {
-        const int64_t f = M.cell_faces[6 * c + j];
-        const int32_t *fn = M.face_nodes + 4 * f;
-        const double *x0 = X + 3 * (int64_t)fn[0], *x1 = X + 3 * (int64_t)fn[1], *x2 = X + 3 * (int64_t)fn[2],
-                     *x3 = X + 3 * (int64_t)fn[3];
+        const uint32_t q = (loc[j >> 1] >> (16 * (j & 1))) & 0xffffu;  // 4 nodes, 4 bits each
+        double p0[3], p1[3], p2[3], p3[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            p0[i] = col[(3 * (q & 7u) + i) * kMkThreads];
+            p1[i] = col[(3 * ((q >> 4) & 7u) + i) * kMkThreads];
+            p2[i] = col[(3 * ((q >> 8) & 7u) + i) * kMkThreads];
+            p3[i] = col[(3 * ((q >> 12) & 7u) + i) * kMkThreads];
+        }
         double t1 = 0.0, t2 = 0.0;
-        const bool b1 = mk_ray_tri(P, d, x0, x1, x2, t1);
-        t1 /= nv;
-        const bool b2 = mk_ray_tri(P, d, x0, x2, x3, t2);
-        t2 /= nv;
+        const bool b1 = mk_ray_tri(P, d, p0, p1, p2, t1);
+        const bool b2 = mk_ray_tri(P, d, p0, p2, p3, t2);
         if (!b1 && !b2) continue;
+        // distances along the unit direction: dividing each by |V| (as the reference does) keeps
+        // their order, so the division is taken once, for the nearest
         const double tf = (b1 && b2) ? smin(t1, t2) : (b1 ? t1 : t2);
-        if (tmin > tf) { tmin = tf; face = f; found = true; }
+        if (tmin > tf) { tmin = tf; jx = j; found = true; }
     }
-    time = tmin;
+    time = found ? tmin / nv : 1e10;
+    if (!(time < 1e10)) {  // the reference accepts only crossings sooner than 1e10
+        time = 1e10;
+        found = false;
+    }
     return found;
 }
 
@@ -190,6 +218,39 @@ __device__ __forceinline__ bool mk_velosity_initial(uint64_t &rs, double v_inf, 
     return true;
 }
 
+// One record per cell for the flights: the 8 node positions of the cell, each face's 4 nodes (in
+// the face's own order) as indices into them, and the cell across each face (-1 - boundary code
+// on a boundary face), faces in the device order of cell_faces.
+__global__ void __launch_bounds__(256) k_mk_pack(DevMesh M, const double *__restrict__ X, MkCell *__restrict__ pack) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= M.n_cells) return;
+    MkCell &R = pack[c];
+    int32_t cn[8];
+    for (int k = 0; k < 8; k++) {
+        cn[k] = M.cell_nodes[8 * c + k];
+        for (int i = 0; i < 3; i++) R.x[3 * k + i] = X[3 * (int64_t)cn[k] + i];
+    }
+    uint32_t loc[3] = {0u, 0u, 0u};
+    for (int j = 0; j < 6; j++) {
+        const int64_t f = M.cell_faces[6 * c + j];
+        uint32_t q = 0u;
+        for (int m = 0; m < 4; m++) {
+            const int32_t node = M.face_nodes[4 * f + m];
+            uint32_t k8 = 0u;
+            for (int k = 0; k < 8; k++)
+                if (cn[k] == node) k8 = (uint32_t)k;
+            q |= k8 << (4 * m);
+        }
+        loc[j >> 1] |= q << (16 * (j & 1));
+        const int32_t a = M.face_cells[2 * f], b = M.face_cells[2 * f + 1];
+        R.nbr[j] = b < 0 ? -1 - (int32_t)(M.face_code[f] & 15u) : (a == c ? b : a);
+    }
+    R.loc[0] = loc[0];
+    R.loc[1] = loc[1];
+    R.loc[2] = loc[2];
+    R.loc[3] = 0u;
+}
+
 // normal and area of each candidate injection face, for the host's particle counts
 __global__ void __launch_bounds__(256) k_mk_gather(int64_t n, const int32_t *__restrict__ faces, DevGeom G,
                                                    double *__restrict__ na) {
@@ -210,8 +271,10 @@ __global__ void __launch_bounds__(256) k_mk_finish(int64_t n_cells, const double
     tally[i] = tally[i] / vol[i / 5];
 }
 
-__global__ void __launch_bounds__(256) k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X,
-                                                      const double *__restrict__ prim, DevGeom G) {
+__global__ void __launch_bounds__(kMkThreads) __attribute__((amdgpu_waves_per_eu(2)))
+k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *__restrict__ prim, DevGeom G) {
+    __shared__ double lds[24 * kMkThreads];
+    double *col = lds + threadIdx.x;
     const int64_t pid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= K.n_particles) return;
     // injection face: the last q with first[q] <= pid
@@ -263,53 +326,72 @@ __global__ void __launch_bounds__(256) k_mk_particles(DevMesh M, MkDev K, const 
     unsigned long long n_ex = 0, n_seg = 0;
     int outcome = 4;  // 2 left, 3 absorbed, 4 lost
     const double me = 1.0 / 1836.15;
+    // An exchange that falls due is held (pend) until a quarter of the wavefront's live lanes hold
+    // one, or all of them do; the lanes then draw their new velocities together, so the costly
+    // exchange path runs once for many lanes instead of on nearly every segment.  Each lane keeps
+    // its own random stream, so the particle's history does not change.
+    bool pend = false;
+    double cp_ex = 1.0;
+    int it = 0;
 #pragma unroll 1
-    for (int it = 0; it < K.max_steps; it++) {
-        double time = 0.0;
-        int64_t ex = -1;
-        bool found = false;
+    while (true) {
+        if (!pend) {
+            if (it >= K.max_steps) break;
+            it++;
+            double time = 0.0;
+            int jx = 0;
+            bool found = false;
 #pragma unroll 1
-        for (int k1 = 0; k1 < 12 && !found; k1++) {
-            found = mk_exit(M, X, c, P, V, time, ex);
-            if (!found) {
-                const double *cn = G.cell_center + 3 * c;
-                for (int i = 0; i < 3; i++) P[i] = (k1 < 10) ? P[i] + (cn[i] - P[i]) / 800.0 : cn[i];
+            for (int k1 = 0; k1 < 12 && !found; k1++) {
+                found = mk_exit(K.pack, c, col, P, V, time, jx);
+                if (!found) {
+                    const double *cn = G.cell_center + 3 * c;
+                    for (int i = 0; i < 3; i++) P[i] = (k1 < 10) ? P[i] + (cn[i] - P[i]) / 800.0 : cn[i];
+                }
+            }
+            if (!found) break;
+            n_seg++;
+            const double *s = prim + 8 * c;
+            double rho_Th = s[0] / (1.0 + me), p_Th = s[4] / 2.0;
+            if (rho_Th <= 1e-8) rho_Th = 1e-8;
+            if (p_Th <= 1e-8 / 2.0) p_Th = 1e-8 / 2.0;
+            const double ro = rho_Th, cp = sqrt(2.0 * p_Th / rho_Th);
+            const double l =
+                sqrt((time * V[0]) * (time * V[0]) + (time * V[1]) * (time * V[1]) + (time * V[2]) * (time * V[2]));
+            double I = I_do;
+            const double u =
+                sqrt((V[0] - s[1]) * (V[0] - s[1]) + (V[1] - s[2]) * (V[1] - s[2]) + (V[2] - s[3]) * (V[2] - s[3]));
+            const double vn = sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
+            const double uz = mk_velosity_1(u, cp);
+            const double nu_ex = ro * uz * mk_sigma(K.a2, uz) / K.kn;
+            const double sig = vn / nu_ex;
+            I += l / sig;
+            if (I < KSI) {
+                I_do = I;
+                atomicAdd(&K.tally[5 * c], time * mu);
+                if (sort >= 1 && sort <= 4) atomicAdd(&K.tally[5 * c + sort], time * mu);
+                for (int i = 0; i < 3; i++) P[i] += 1.000001 * time * V[i];
+                const int32_t nb = K.pack[c].nbr[jx];
+                if (nb < 0) {
+                    outcome = (-1 - nb) == 1 ? 3 : 2;
+                    break;
+                }
+                c = nb;
+            } else {
+                const double kk = (KSI - I_do) * sig;
+                const double t_ex = kk / vn;
+                I_do = 0.0;
+                for (int i = 0; i < 3; i++) P[i] += t_ex * V[i];
+                atomicAdd(&K.tally[5 * c], t_ex * mu);
+                if (sort >= 1 && sort <= 4) atomicAdd(&K.tally[5 * c + sort], t_ex * mu);
+                cp_ex = cp;
+                pend = true;
             }
         }
-        if (!found) break;
-        n_seg++;
-        const double *s = prim + 8 * c;
-        double rho_Th = s[0] / (1.0 + me), p_Th = s[4] / 2.0;
-        if (rho_Th <= 1e-8) rho_Th = 1e-8;
-        if (p_Th <= 1e-8 / 2.0) p_Th = 1e-8 / 2.0;
-        const double ro = rho_Th, cp = sqrt(2.0 * p_Th / rho_Th);
-        const double l = sqrt((time * V[0]) * (time * V[0]) + (time * V[1]) * (time * V[1]) + (time * V[2]) * (time * V[2]));
-        double I = I_do;
-        const double u = sqrt((V[0] - s[1]) * (V[0] - s[1]) + (V[1] - s[2]) * (V[1] - s[2]) + (V[2] - s[3]) * (V[2] - s[3]));
-        const double vn = sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
-        const double uz = mk_velosity_1(u, cp);
-        const double nu_ex = ro * uz * mk_sigma(K.a2, uz) / K.kn;
-        const double sig = vn / nu_ex;
-        I += l / sig;
-        if (I < KSI) {
-            I_do = I;
-            atomicAdd(&K.tally[5 * c], time * mu);
-            if (sort >= 1 && sort <= 4) atomicAdd(&K.tally[5 * c + sort], time * mu);
-            for (int i = 0; i < 3; i++) P[i] += 1.000001 * time * V[i];
-            const int64_t a = M.face_cells[2 * ex], b = M.face_cells[2 * ex + 1];
-            const int64_t nc = a == c ? b : a;
-            if (nc < 0) {
-                outcome = (M.face_code[ex] & 15u) == 1u ? 3 : 2;
-                break;
-            }
-            c = nc;
-        } else {
-            const double kk = (KSI - I_do) * sig;
-            const double t_ex = kk / vn;
-            I_do = 0.0;
-            for (int i = 0; i < 3; i++) P[i] += t_ex * V[i];
-            atomicAdd(&K.tally[5 * c], t_ex * mu);
-            if (sort >= 1 && sort <= 4) atomicAdd(&K.tally[5 * c + sort], t_ex * mu);
+        const unsigned long long held = __ballot(pend), live = __ballot(true);
+        if (pend && (4 * __popcll(held) >= __popcll(live) || held == live)) {
+            const double *s = prim + 8 * c;
+            const double cp = cp_ex;
             double U[3], Vs[3], W[3];
             const double Vp[3] = {s[1], s[2], s[3]};
             const MkFrame F = mk_frame(P);
@@ -322,6 +404,7 @@ __global__ void __launch_bounds__(256) k_mk_particles(DevMesh M, MkDev K, const 
             sort = (int)((M.cell_code[c] >> 8) & 15u);
             KSI = -log(1.0 - mk_rand(rs));
             n_ex++;
+            pend = false;
         }
     }
     atomicAdd(&st[0], 1ull);
@@ -331,6 +414,10 @@ __global__ void __launch_bounds__(256) k_mk_particles(DevMesh M, MkDev K, const 
     if (capped) atomicAdd(&st[5], 1ull);
 }
 
+void launch_mk_pack(hipStream_t s, const DevMesh &M, const double *X, MkCell *pack) {
+    if (M.n_cells > 0) hipLaunchKernelGGL(k_mk_pack, dim3(grid_of(M.n_cells, 256)), dim3(256), 0, s, M, X, pack);
+}
+
 void launch_mk_gather(hipStream_t s, int64_t n, const int32_t *faces, const DevGeom &G, double *na) {
     if (n > 0) hipLaunchKernelGGL(k_mk_gather, dim3(grid_of(n, 256)), dim3(256), 0, s, n, faces, G, na);
 }
@@ -338,7 +425,8 @@ void launch_mk_gather(hipStream_t s, int64_t n, const int32_t *faces, const DevG
 void launch_mk_particles(hipStream_t s, const DevMesh &M, const MkDev &K, const double *X, const double *prim,
                          const DevGeom &G) {
     if (K.n_particles > 0)
-        hipLaunchKernelGGL(k_mk_particles, dim3(grid_of(K.n_particles, 256)), dim3(256), 0, s, M, K, X, prim, G);
+        hipLaunchKernelGGL(k_mk_particles, dim3(grid_of(K.n_particles, kMkThreads)), dim3(kMkThreads), 0, s, M, K, X,
+                           prim, G);
     if (M.n_cells > 0)
         hipLaunchKernelGGL(k_mk_finish, dim3(grid_of(5 * M.n_cells, 256)), dim3(256), 0, s, M.n_cells,
                            (const double *)G.cell_volume, K.tally);

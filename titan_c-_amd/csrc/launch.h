@@ -39,6 +39,7 @@ void launch_fit_surface(hipStream_t s, const DevMesh &M, const DevParams &P, con
                         const double *X1, double *X2, const double *cell_center, StepScalars *S, double time);
 void launch_fit_rays(hipStream_t s, const FitDev &F, double *X);
 // k_mk.hip: Monte-Carlo neutral densities
+void launch_mk_pack(hipStream_t s, const DevMesh &M, const double *X, MkCell *pack);
 void launch_mk_gather(hipStream_t s, int64_t n, const int32_t *faces, const DevGeom &G, double *na);
 void launch_mk_particles(hipStream_t s, const DevMesh &M, const MkDev &K, const double *X, const double *prim,
                          const DevGeom &G);

@@ -1044,13 +1044,14 @@ extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t
         K.a2 = c->params.par_a_2;
         K.kn = c->params.par_Kn;
         K.max_steps = max_steps;
-        DevBuf dface, dfirst, dmu, dtally, dstats;
+        DevBuf dface, dfirst, dmu, dtally, dstats, dpack;
         if (K.n_in > 0) {
             rc = dup(dface, face.data(), sizeof(int32_t) * face.size());
             if (rc == TITAN_OK) rc = dup(dfirst, first.data(), sizeof(int64_t) * first.size());
             if (rc == TITAN_OK) rc = dup(dmu, mu.data(), sizeof(double) * mu.size());
         }
         if (rc == TITAN_OK) rc = alloc(dtally, sizeof(double) * 5 * nc);
+        if (rc == TITAN_OK) rc = alloc(dpack, sizeof(MkCell) * nc);
         if (rc == TITAN_OK) rc = alloc(dstats, sizeof(unsigned long long) * 7);
         if (rc != TITAN_OK) return rc;
         HIPCHK(hipMemsetAsync(dtally.p, 0, sizeof(double) * 5 * nc, c->stream));
@@ -1060,6 +1061,8 @@ extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t
         K.mu = (const double *)dmu.p;
         K.tally = (double *)dtally.p;
         K.stats = (unsigned long long *)dstats.p;
+        K.pack = (const MkCell *)dpack.p;
+        launch_mk_pack(c->stream, c->mesh(), c->d_X[layer], (MkCell *)dpack.p);
         launch_mk_particles(c->stream, c->mesh(), K, c->d_X[layer], c->d_prim[layer], c->geom[layer]);
         HIPCHK(hipGetLastError());
         std::vector<double> t(5 * nc);
