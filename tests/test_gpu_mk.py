@@ -103,3 +103,25 @@ def test_bad_arguments(T, C):
             with pytest.raises(T.TitanError) as e:
                 s.MK_go(**kw)
             assert e.value.code == T.TITAN_E_ARG
+
+
+def test_no_outer_boundary_gives_nothing(T, C):
+    """A mesh without outer-boundary faces injects no particle: zero densities and counts."""
+    mesh, prim, params = C.cube_brio_wu(n=4, jitter=0.1)   # every boundary face zero-gradient (bc 4)
+    with T.Setka(mesh, params) as s:
+        s.set_state(prim, 0)
+        d, st = s.MK_go(50)
+    assert np.all(d == 0.0) and all(v == 0 for v in st.values())
+
+
+def test_second_call_same_context(T, C):
+    """The injection list is found once per context: a second call with the same seed repeats the
+    first (counts exactly, densities to rounding)."""
+    mesh, _, params = _cube(C, n=4)
+    prim = _lism(C, mesh)
+    with T.Setka(mesh, dict(params, par_Kn=1.0)) as s:
+        s.set_state(prim, 0)
+        d1, s1 = s.MK_go(60, seed=4, layer=0)
+        d2, s2 = s.MK_go(60, seed=4, layer=0)
+    assert s1 == s2
+    np.testing.assert_allclose(d1, d2, rtol=1e-12, atol=1e-15)

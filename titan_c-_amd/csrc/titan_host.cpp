@@ -87,6 +87,8 @@ struct titan_ctx {
     MlpState mlp;
     FitState *fit = nullptr;  // fitted surfaces and rays (move_mesh 2)
     bool has_area = false;    // titan_set_inner_area was called
+    std::vector<int32_t> mk_cand;  // outer boundary faces (device ids, ascending reference id) for titan_mk_go
+    bool mk_cand_ready = false;
     DevMesh mesh() const {
         DevMesh M;
         M.n_nodes = nn; M.n_cells = nc; M.n_faces = nf;
@@ -990,16 +992,21 @@ extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t
     if (!c || layer < 0 || layer > 1 || n_per_face < 1 || max_steps < 1 || !density) return TITAN_E_ARG;
     if (!(c->params.par_Kn > 0.0)) return TITAN_E_ARG;
     const int64_t nc = c->nc, nf = c->nf;
-    // candidate injection faces: the outer boundary (bc 2, 3), ascending reference face id
-    std::vector<uint32_t> code(nf);
-    HIPCHK(hipMemcpyAsync(code.data(), c->d_face_code, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    std::vector<int32_t> cand;
-    for (int64_t r = 0; r < nf; r++) {
-        const int32_t d = c->ord.face_r2d[r];
-        const uint32_t bc = code[d] & 15u;
-        if (bc == TITAN_BC_OUTER_HARD || bc == TITAN_BC_OUTER_SOFT) cand.push_back(d);
+    // candidate injection faces: the outer boundary (bc 2, 3), ascending reference face id (the
+    // boundary codes are fixed at titan_create, so the list is found once per context)
+    if (!c->mk_cand_ready) {
+        std::vector<uint32_t> code(nf);
+        HIPCHK(hipMemcpyAsync(code.data(), c->d_face_code, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->mk_cand.clear();
+        for (int64_t r = 0; r < nf; r++) {
+            const int32_t d = c->ord.face_r2d[r];
+            const uint32_t bc = code[d] & 15u;
+            if (bc == TITAN_BC_OUTER_HARD || bc == TITAN_BC_OUTER_SOFT) c->mk_cand.push_back(d);
+        }
+        c->mk_cand_ready = true;
     }
+    const std::vector<int32_t> &cand = c->mk_cand;
     int64_t st[7] = {0, 0, 0, 0, 0, 0, 0};
     std::vector<double> h(5 * nc, 0.0);
     if (!cand.empty()) {
