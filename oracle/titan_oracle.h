@@ -27,6 +27,11 @@
  *                       (plasma part; Plasma_components_1, Phys_param.cpp:628-665)
  *   or_cx_source, or_sigma_ls, or_mlp_*  the learned-source path (no reference counterpart:
  *                       Maxwellian CX moments with the Lindsay-Stebbings cross-section; MLP)
+ *   or_culc_velocity_surface  Setka::Culc_Velocity_surface  TITAN/source/Setka_move.cpp:111-1357
+ *   or_dvigenie         Luch::dvigenie (A_Luch)      TITAN/source/Luch.cpp:6-152
+ *   or_mk_setup, or_mk_go  Setka::MK_go / MK_fly_immit  TITAN/source/Setka_MK.cpp:1225-2850
+ *                       (Gran::Get_Random_pozition, Luch_crossing, rayTriangleIntersect,
+ *                       Gran.cpp:220-285,451-561; Cell::MK_Add_particle, Cell.cpp:1042-1083)
  *
  * What pins each function (tests/test_oracle_*.py), so that a dropped term, a wrong sign or
  * index or a transposed operand fails a check that is not a retyping of the formula:
@@ -69,6 +74,16 @@
  *   or_cx_source        sigma = const closed form, Cartesian quadrature, 6-D Monte Carlo,
  *                       bilinearity and Galilean invariance; or_sigma_ls: published values.
  *   or_mlp_*            closed forms and finite differences.
+ *   or_culc_velocity_surface, or_dvigenie  closed-form wave speeds of uniform states on the
+ *                       TS/HP/BS faces, node means, the sphere and neighbour smoothing terms,
+ *                       straight rays and the end slopes of the ray polynomials, the surface
+ *                       neighbour counts of a partial surface, a fitted free stream kept exactly.
+ *   or_mk_setup, or_mk_go  free flight holds the LISM density in every cell of a convex domain
+ *                       fed from all faces; detailed balance with protons in the same Maxwellian
+ *                       (Kn 1 and 0.3); exchange counts monotone in Kn; the particle-count rule;
+ *                       inflow against outflow through a closed surface.  Per particle the flight
+ *                       is UNPINNED against reference output: its random numbers come from files
+ *                       it does not ship.
  */
 #ifndef TITAN_ORACLE_H
 #define TITAN_ORACLE_H
