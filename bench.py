@@ -209,7 +209,8 @@ def main():
                     default="fp64,mixed,fp64_exact,mixed_surrogate,fp64_tvd,mixed_tvd,fp64_bera,fp64_fit,mk,checkpoint")
     ap.add_argument("--mk-per-face", type=int, default=100,
                     help="Monte-Carlo neutrals: particles per outer-boundary face (N_per_gran)")
-    ap.add_argument("--mk-cpu-per-face", type=int, default=1)
+    ap.add_argument("--mk-cpu-per-face", type=int, default=8,
+                    help="Monte-Carlo CPU baseline: particles per outer face of its sample (about 10 s)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the streaming copy the HBM counter runs are scaled by, then exit")
     a = ap.parse_args()
