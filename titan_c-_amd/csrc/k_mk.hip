@@ -263,12 +263,15 @@ __global__ void __launch_bounds__(256) k_mk_gather(int64_t n, const int32_t *__r
     na[4 * i + 3] = G.face_area[f];
 }
 
-// tallies per unit volume
+// tallies per unit volume (Cell::MK_normir_Moments, Cell.cpp:1175-1215), written in reference
+// cell order for the caller
 __global__ void __launch_bounds__(256) k_mk_finish(int64_t n_cells, const double *__restrict__ vol,
-                                                   double *__restrict__ tally) {
+                                                   const int32_t *__restrict__ cell_ref,
+                                                   const double *__restrict__ tally, double *__restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 5 * n_cells) return;
-    tally[i] = tally[i] / vol[i / 5];
+    const int64_t d = i / 5;
+    out[5 * (int64_t)cell_ref[d] + (i - 5 * d)] = tally[i] / vol[d];
 }
 
 __global__ void __launch_bounds__(kMkThreads) __attribute__((amdgpu_waves_per_eu(2)))
@@ -429,7 +432,7 @@ void launch_mk_particles(hipStream_t s, const DevMesh &M, const MkDev &K, const 
                            prim, G);
     if (M.n_cells > 0)
         hipLaunchKernelGGL(k_mk_finish, dim3(grid_of(5 * M.n_cells, 256)), dim3(256), 0, s, M.n_cells,
-                           (const double *)G.cell_volume, K.tally);
+                           (const double *)G.cell_volume, M.cell_ref, (const double *)K.tally, K.out);
 }
 
 }  // namespace titan

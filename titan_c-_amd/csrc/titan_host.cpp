@@ -88,6 +88,7 @@ struct titan_ctx {
     FitState *fit = nullptr;  // fitted surfaces and rays (move_mesh 2)
     bool has_area = false;    // titan_set_inner_area was called
     std::vector<int32_t> mk_cand;  // outer boundary faces (device ids, ascending reference id) for titan_mk_go
+    void *d_mk = nullptr;          // titan_mk_go: flight records, tallies, densities, counters
     bool mk_cand_ready = false;
     DevMesh mesh() const {
         DevMesh M;
@@ -430,7 +431,7 @@ extern "C" int titan_destroy(titan_ctx *c) {
                     c->d_X[0], c->d_X[1], c->d_prim[0], c->d_prim[1], c->d_geom[0], c->d_geom[1], c->d_flux,
                     c->d_src, c->d_neutrals, c->d_hfluid, c->d_S, c->d_S2, c->d_t_node_off, c->d_t_nodes, c->d_t_own,
                     c->d_t_face_off, c->d_t_face_ref, c->d_t_faces, c->d_t_cell_lnodes, c->d_t_cell_lfaces,
-                    c->d_vol, c->d_face_tvd, c->d_t_face_tvd};
+                    c->d_vol, c->d_face_tvd, c->d_t_face_tvd, c->d_mk};
     for (void *q : ptrs)
         if (q) hipFree(q);
     mlp_free(c->mlp);
@@ -1008,7 +1009,6 @@ extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t
     }
     const std::vector<int32_t> &cand = c->mk_cand;
     int64_t st[7] = {0, 0, 0, 0, 0, 0, 0};
-    std::vector<double> h(5 * nc, 0.0);
     if (!cand.empty()) {
         const int64_t nb = (int64_t)cand.size();
         // geometry of the layer (the fused step keeps only the cell centres current)
@@ -1051,35 +1051,39 @@ extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t
         K.a2 = c->params.par_a_2;
         K.kn = c->params.par_Kn;
         K.max_steps = max_steps;
-        DevBuf dface, dfirst, dmu, dtally, dstats, dpack;
+        DevBuf dface, dfirst, dmu;
         if (K.n_in > 0) {
             rc = dup(dface, face.data(), sizeof(int32_t) * face.size());
             if (rc == TITAN_OK) rc = dup(dfirst, first.data(), sizeof(int64_t) * first.size());
             if (rc == TITAN_OK) rc = dup(dmu, mu.data(), sizeof(double) * mu.size());
         }
-        if (rc == TITAN_OK) rc = alloc(dtally, sizeof(double) * 5 * nc);
-        if (rc == TITAN_OK) rc = alloc(dpack, sizeof(MkCell) * nc);
-        if (rc == TITAN_OK) rc = alloc(dstats, sizeof(unsigned long long) * 7);
         if (rc != TITAN_OK) return rc;
-        HIPCHK(hipMemsetAsync(dtally.p, 0, sizeof(double) * 5 * nc, c->stream));
-        HIPCHK(hipMemsetAsync(dstats.p, 0, sizeof(unsigned long long) * 7, c->stream));
+        if (!c->d_mk) {  // flight records, tallies, densities and counters: kept for later calls
+            const size_t bytes = sizeof(MkCell) * nc + sizeof(double) * 10 * nc + sizeof(unsigned long long) * 8;
+            HIPCHK(hipMalloc((void **)&c->d_mk, bytes));
+        }
+        MkCell *pack = (MkCell *)c->d_mk;
+        double *tally = (double *)(pack + nc);
+        double *out = tally + 5 * nc;
+        unsigned long long *dst = (unsigned long long *)(out + 5 * nc);
+        HIPCHK(hipMemsetAsync(tally, 0, sizeof(double) * 5 * nc, c->stream));
+        HIPCHK(hipMemsetAsync(dst, 0, sizeof(unsigned long long) * 7, c->stream));
         K.face = (const int32_t *)dface.p;
         K.first = (const int64_t *)dfirst.p;
         K.mu = (const double *)dmu.p;
-        K.tally = (double *)dtally.p;
-        K.stats = (unsigned long long *)dstats.p;
-        K.pack = (const MkCell *)dpack.p;
-        launch_mk_pack(c->stream, c->mesh(), c->d_X[layer], (MkCell *)dpack.p);
+        K.tally = tally;
+        K.out = out;
+        K.stats = dst;
+        K.pack = pack;
+        launch_mk_pack(c->stream, c->mesh(), c->d_X[layer], pack);
         launch_mk_particles(c->stream, c->mesh(), K, c->d_X[layer], c->d_prim[layer], c->geom[layer]);
         HIPCHK(hipGetLastError());
-        std::vector<double> t(5 * nc);
-        HIPCHK(hipMemcpyAsync(t.data(), dtally.p, sizeof(double) * 5 * nc, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(st, dstats.p, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(density, out, sizeof(double) * 5 * nc, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(st, dst, sizeof(st), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        h.swap(t);
+    } else {
+        std::fill(density, density + 5 * nc, 0.0);
     }
-    for (int64_t d = 0; d < nc; d++)
-        for (int k = 0; k < 5; k++) density[5 * (int64_t)c->ord.cell_d2r[d] + k] = h[5 * d + k];
     if (stats)
         for (int i = 0; i < 7; i++) stats[i] = st[i];
     return TITAN_OK;
