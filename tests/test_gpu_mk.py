@@ -125,3 +125,34 @@ def test_second_call_same_context(T, C):
         d2, s2 = s.MK_go(60, seed=4, layer=0)
     assert s1 == s2
     np.testing.assert_allclose(d1, d2, rtol=1e-12, atol=1e-15)
+
+
+def test_moved_layer(T, C, O):
+    """After moving steps, MK_go of the current layer flies through that layer's nodes: the device
+    agrees with the oracle run on the moved nodes and the stepped state (same seed closer than
+    another seed), and the particle counts match exactly."""
+    mesh, _, params = _cube(C, n=5)
+    prim = _lism(C, mesh)
+    rng = np.random.default_rng(8)
+    xyz = mesh.node_xyz
+    inside = np.all((xyz > 1e-9) & (xyz < 1 - 1e-9), axis=1)
+    disp = np.zeros_like(xyz)
+    disp[inside] = 0.02 * rng.normal(size=(inside.sum(), 3))
+    mesh.node_disp = disp
+    p = dict(params, move_mesh=1, motion_omega=3.0, motion_phase=0.4, par_Kn=1.0)
+    with T.Setka(mesh, p) as s:
+        s.set_state(prim, 0)
+        s.Go(3)
+        lay = s.current_layer()
+        x_now = s.get_nodes(lay)
+        p_now = s.get_state(lay)
+        d, st = s.MK_go(100, seed=21)
+    assert np.abs(x_now - mesh.node_xyz).max() > 1e-6
+    om = O.OracleMesh(mesh)
+    od, ost = O.mk_go(om, x_now, p_now, C.V_INF, A2, 1.0, 100, seed=21)
+    o2, _ = O.mk_go(om, x_now, p_now, C.V_INF, A2, 1.0, 100, seed=22)
+    assert st["particles"] == ost["particles"]
+    vol = O.measure(om, x_now)["cell_volume"]
+    same = (np.abs(d[:, 0] - od[:, 0]) * vol).sum()
+    noise = (np.abs(o2[:, 0] - od[:, 0]) * vol).sum()
+    assert same < 0.5 * noise, (same, noise)
