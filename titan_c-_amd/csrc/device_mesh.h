@@ -118,7 +118,7 @@ struct MkDev {
     double v_inf, a2, kn;
     int32_t max_steps, pad;
     const MkCell *pack;         // [Nc] flight records of the cells (k_mk_pack)
-    double *tally;              // [Nc*5] sum over sorts, then sorts 1..4 of mu * time
+    double *tally;              // [Nc*5] mu * time of sorts outside 1..4, then of sorts 1..4
     double *out;                // [Nc*5] the tallies per unit volume, reference cell order
     unsigned long long *stats;  // [7] particles, charge exchanges, left, absorbed, lost, capped, segments
 };

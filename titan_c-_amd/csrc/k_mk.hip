@@ -7,11 +7,11 @@
 // while its optical depth grows by l / (|V| / nu), nu = rho_Th Velosity_1(u, cp) sigma / Kn, and
 // charge-exchanges where the depth passes -ln(1 - xi): its velocity becomes the proton's
 // (Setka::M_K_Change_Velosity, in the spherical frame of its position) and its sort the zone of the
-// cell.  Every cell it crosses gains mu * time in the sum over sorts and in its sort
-// (Cell::MK_Add_particle).  The particle ends at a boundary face.  Differences from the
-// reference, which keep every loop bounded: a particle that leaves through the inner sphere is
-// absorbed (the reference carries it across the Sun's sphere, Setka_MK.cpp:2140-2160); the birth
-// sampler gives up after 10000 tries and drops the particle (only on faces where the flow leaves
+// cell.  Every cell it crosses gains mu * time in its sort's slot, one atomic per segment; the
+// sum over sorts (MK_n_H of Cell::MK_Add_particle) is formed from the slots at the end.  The
+// particle ends at a boundary face.  Differences from the reference, which keep every loop
+// bounded: a particle that leaves through the inner sphere is absorbed (the reference carries it
+// across the Sun's sphere, Setka_MK.cpp:2140-2160); the birth sampler gives up after 10000 tries and drops the particle (only on faces where the flow leaves
 // the domain, whose inflow is ~exp(-V_inf^2) of the upwind faces'); a particle that finds no exit
 // after 11 nudges towards the cell centre, or exceeds the step cap, is counted lost.  The tallies
 // are FP64 atomic sums (the order of additions varies; the estimate is statistical).
@@ -270,8 +270,11 @@ __global__ void __launch_bounds__(256) k_mk_finish(int64_t n_cells, const double
                                                    const double *__restrict__ tally, double *__restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 5 * n_cells) return;
-    const int64_t d = i / 5;
-    out[5 * (int64_t)cell_ref[d] + (i - 5 * d)] = tally[i] / vol[d];
+    const int64_t d = i / 5, k = i - 5 * d;
+    const double *t = tally + 5 * d;
+    // slot 0 holds the sorts outside 1..4; the total adds every slot
+    const double v = k == 0 ? (((t[0] + t[1]) + t[2]) + t[3]) + t[4] : t[k];
+    out[5 * (int64_t)cell_ref[d] + k] = v / vol[d];
 }
 
 __global__ void __launch_bounds__(kMkThreads) __attribute__((amdgpu_waves_per_eu(2)))
@@ -325,6 +328,7 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
     double KSI = -log(1.0 - mk_rand(rs)), I_do = 0.0;
     const double mu = K.mu[q];
     int sort = 4;
+    int slot = 4;  // tally slot: the sort (1..4), or 0 for a sort outside 1..4 (counted in the total only)
     bool capped = false;
     unsigned long long n_ex = 0, n_seg = 0;
     int outcome = 4;  // 2 left, 3 absorbed, 4 lost
@@ -371,8 +375,7 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
             I += l / sig;
             if (I < KSI) {
                 I_do = I;
-                atomicAdd(&K.tally[5 * c], time * mu);
-                if (sort >= 1 && sort <= 4) atomicAdd(&K.tally[5 * c + sort], time * mu);
+                atomicAdd(&K.tally[5 * c + slot], time * mu);
                 for (int i = 0; i < 3; i++) P[i] += 1.000001 * time * V[i];
                 const int32_t nb = K.pack[c].nbr[jx];
                 if (nb < 0) {
@@ -385,8 +388,7 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
                 const double t_ex = kk / vn;
                 I_do = 0.0;
                 for (int i = 0; i < 3; i++) P[i] += t_ex * V[i];
-                atomicAdd(&K.tally[5 * c], t_ex * mu);
-                if (sort >= 1 && sort <= 4) atomicAdd(&K.tally[5 * c + sort], t_ex * mu);
+                atomicAdd(&K.tally[5 * c + slot], t_ex * mu);
                 cp_ex = cp;
                 pend = true;
             }
@@ -405,6 +407,7 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
             for (int i = 0; i < 3; i++) W[i] *= cp;
             mk_from_sph(F, W[0], W[1], W[2], V);
             sort = (int)((M.cell_code[c] >> 8) & 15u);
+            slot = (sort >= 1 && sort <= 4) ? sort : 0;
             KSI = -log(1.0 - mk_rand(rs));
             n_ex++;
             pend = false;
