@@ -74,6 +74,7 @@ def lib():
             "or_dvigenie": (None, [_P, _P]),
             "or_mk_setup": (_I64, [_P, _P, _D, ctypes.c_int32, _P, _P, _P]),
             "or_mk_go": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+            "or_mk_change_velosity_batch": (None, [_I64, ctypes.c_uint64, _D, _P, _P, _D, _I, _P, _P]),
             "or_bera_source": (None, [_P, _I, _P, _D, _D, _D, _P]),
             "or_new_connect": (_I64, [_I64, _P, _I64, _P, _P, _P]),
             "or_cx_rate_ref": (_D, [_D, _D, _D, _I, _I]),
@@ -513,7 +514,7 @@ def dvigenie(fit, x, params=None):
 # ---- Monte-Carlo neutral densities --------------------------------------------------------------
 class OrMk(ctypes.Structure):
     _fields_ = [("n_in", _I64), ("face", _P), ("count", _P), ("mu", _P), ("seed", ctypes.c_uint64),
-                ("v_inf", _D), ("a2", _D), ("kn", _D), ("max_steps", ctypes.c_int32)]
+                ("v_inf", _D), ("a2", _D), ("kn", _D), ("max_steps", ctypes.c_int32), ("mistake", ctypes.c_int32)]
 
 
 def mk_setup(om, geom, v_inf, n_per_face):
@@ -528,18 +529,31 @@ def mk_setup(om, geom, v_inf, n_per_face):
     return face[:k].copy(), count[:k].copy(), mu[:k].copy()
 
 
-def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000, geom=None):
-    """or_mk_go on the nodes xyz and state prim (geom: measure(om, xyz), computed when None);
+def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000, geom=None, mistake=0):
+    """or_mk_go on the nodes xyz and state prim (geom: measure(om, xyz), computed when None;
+    mistake: or_mk.mistake, nonzero only in tests that show the pins catch an error);
     returns (densities [n_cells, 5] = sum over sorts, sorts 1..4, per unit volume; stats dict)."""
     m = om.m
     x = np.ascontiguousarray(xyz, np.float64)
     geom = measure(om, x) if geom is None else geom
     face, count, mu = mk_setup(om, geom, v_inf, n_per_face)
     g = OrGeom(*[_p(geom[k]) for k, _ in OrGeom._fields_])
-    mk = OrMk(len(face), _p(face), _p(count), _p(mu), int(seed), float(v_inf), float(a2), float(kn), int(max_steps))
+    mk = OrMk(len(face), _p(face), _p(count), _p(mu), int(seed), float(v_inf), float(a2), float(kn), int(max_steps),
+              int(mistake))
     tally = np.zeros((m.n_cells, 5))
     stats = np.zeros(7, np.int64)
     lib().or_mk_go(ctypes.byref(om.c), ctypes.byref(mk), _p(x), _p(np.ascontiguousarray(prim, np.float64)),
                    ctypes.byref(g), _p(tally), _p(stats))
     keys = ("particles", "exchanges", "left", "absorbed", "lost", "capped", "segments")
     return tally / geom["cell_volume"][:, None], dict(zip(keys, (int(v) for v in stats)))
+
+
+def mk_change_velosity(n, U, V, cp, a2, seed=1, mistake=0):
+    """n draws of the exchange sampler (or_mk_change_velosity_batch), units of cp: (W [n, 3], capped)."""
+    W = np.zeros((n, 3))
+    cap = ctypes.c_int64(0)
+    u = np.ascontiguousarray(U, np.float64)
+    v = np.ascontiguousarray(V, np.float64)
+    lib().or_mk_change_velosity_batch(int(n), int(seed), float(a2), _p(u), _p(v), float(cp), int(mistake), _p(W),
+                                      ctypes.byref(cap))
+    return W, cap.value

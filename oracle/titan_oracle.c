@@ -1629,7 +1629,7 @@ static void mk_from_sph(const double *X, double r, double th, double ph, double 
  * velocity V exchanged with, in units of cp, from the Maxwellian of mean U weighted by g sigma(g);
  * the rejection loop is capped at 10000 tries (*capped set) */
 static void mk_change_velosity(uint64_t *rs, double a2, const double *U, const double *V, double *W, double cp,
-                               int *capped)
+                               int *capped, int mistake)
 {
     double X = sqrt((V[0] - U[0]) * (V[0] - U[0]) + (V[1] - U[1]) * (V[1] - U[1]) + (V[2] - U[2]) * (V[2] - U[2]));
     double p4 = 0.5 * OR_SQRTPI_MK * X / (1.0 + 0.5 * OR_SQRTPI_MK * X);
@@ -1654,7 +1654,7 @@ static void mk_change_velosity(uint64_t *rs, double a2, const double *U, const d
         double uuu = sqrt(u1 * u1 + u2 * u2 + u3 * u3);
         double yy = sqrt(y1 * y1 + y2 * y2 + y3 * y3);
         double h = (uuu * mk_sigma(a2, uuu * cp)) / (mk_sigma(a2, X * cp) * (X + yy));
-        if (!(h < k6)) { W[0] = v1; W[1] = v2; W[2] = v3; return; }
+        if (!(h < k6) || (mistake & 2)) { W[0] = v1; W[1] = v2; W[2] = v3; return; }
     }
     *capped = 1;
     W[0] = v1; W[1] = v2; W[2] = v3;
@@ -1662,7 +1662,7 @@ static void mk_change_velosity(uint64_t *rs, double a2, const double *U, const d
 
 /* Setka::Velosity_initial (Setka_MK.cpp:2671-2723): a velocity entering through a face of outward
  * normal n from the unit-temperature Maxwellian drifting at (v_inf, 0, 0) */
-static void mk_velosity_initial(uint64_t *rs, double v_inf, const double *n, double *V, int *capped)
+static void mk_velosity_initial(uint64_t *rs, double v_inf, const double *n, double *V, int *capped, int mistake)
 {
     double t[3], mm[3];
     or_get_bazis(n, t, mm);
@@ -1676,7 +1676,9 @@ static void mk_velosity_initial(uint64_t *rs, double v_inf, const double *n, dou
         if (p1 > k3) z = cos(OR_PI * k5) * sqrt(-log(k4));
         else if (k4 <= 0.5) z = -sqrt(-log(2.0 * k4));
         else z = sqrt(-log(2.0 * (1.0 - k4)));
-        if (!(fabs(z + Vi[0]) / (fabs(Vi[0]) + fabs(z)) < k6 || z > -Vi[0])) { ok = 1; break; }
+        int weight_rejects = fabs(z + Vi[0]) / (fabs(Vi[0]) + fabs(z)) < k6;
+        if (mistake & 1) weight_rejects = 0;
+        if (!(weight_rejects || z > -Vi[0])) { ok = 1; break; }
     }
     if (!ok) *capped = 1;
     double V1 = z + Vi[0];
@@ -1685,6 +1687,18 @@ static void mk_velosity_initial(uint64_t *rs, double v_inf, const double *n, dou
     double V2 = Vi[1] + a * cos(2.0 * OR_PI * k1);
     double V3 = Vi[2] + a * sin(2.0 * OR_PI * k1);
     for (int i = 0; i < 3; i++) V[i] = V1 * n[i] + V2 * t[i] + V3 * mm[i];
+}
+
+void or_mk_change_velosity_batch(int64_t n, uint64_t seed, double a2, const double *U, const double *V, double cp,
+                                 int mistake, double *W, int64_t *capped)
+{
+    *capped = 0;
+    for (int64_t i = 0; i < n; i++) {
+        uint64_t rs = mk_mix(seed ^ mk_mix((uint64_t)i + 0x9E3779B97F4A7C15ull));
+        int cap = 0;
+        mk_change_velosity(&rs, a2, U, V, W + 3 * i, cp, &cap, mistake);
+        *capped += cap;
+    }
 }
 
 int64_t or_mk_setup(const or_mesh *m, const or_geom *g, double v_inf, int32_t n_per_face, int32_t *face,
@@ -1747,7 +1761,7 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
             double P[3], V[3];
             for (int i = 0; i < 3; i++)
                 P[i] = (1.0 - sqrt(r1)) * A[i] + (sqrt(r1) * (1.0 - r2)) * T1[i] + (r2 * sqrt(r1)) * Cc[i];
-            mk_velosity_initial(&rs, mk->v_inf, n, V, &capped);
+            mk_velosity_initial(&rs, mk->v_inf, n, V, &capped, mk->mistake);
             stats[0]++;
             if (capped) { stats[5]++; continue; } /* no admissible velocity within the cap: dropped */
             int64_t c = m->face_cells[2 * f];
@@ -1807,7 +1821,7 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
                     mk_to_sph(P, Vp, &U[0], &U[1], &U[2]);      /* (r, theta, phi) */
                     mk_to_sph(P, V, &Vs[0], &Vs[1], &Vs[2]);
                     for (int i = 0; i < 3; i++) { U[i] /= cp; Vs[i] /= cp; }
-                    mk_change_velosity(&rs, mk->a2, U, Vs, W, cp, &capped);
+                    mk_change_velosity(&rs, mk->a2, U, Vs, W, cp, &capped, mk->mistake);
                     for (int i = 0; i < 3; i++) W[i] *= cp;
                     mk_from_sph(P, W[0], W[1], W[2], V);
                     sort = m->cell_zone[c];

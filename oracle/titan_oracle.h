@@ -81,7 +81,9 @@
  *   or_mk_setup, or_mk_go  free flight holds the LISM density in every cell of a convex domain
  *                       fed from all faces; detailed balance with protons in the same Maxwellian
  *                       (Kn 1 and 0.3); exchange counts monotone in Kn; the particle-count rule;
- *                       inflow against outflow through a closed surface.  Per particle the flight
+ *                       inflow against outflow through a closed surface; the exchange sampler's
+ *                       mean and mean square against quadrature of its weighted Maxwellian;
+ *                       planted mistakes (or_mk.mistake) fail these checks.  Per particle the flight
  *                       is UNPINNED against reference output: its random numbers come from files
  *                       it does not ship.
  */
@@ -239,6 +241,9 @@ typedef struct {
     uint64_t seed;
     double v_inf, a2, kn;
     int32_t max_steps;
+    int32_t mistake;  /* 0; for tests that show the pins catch a plausible error: bit 0 drops the
+                         |V.n| weight of the birth velocity, bit 1 the g sigma(g) weight of the
+                         exchanged velocity */
 } or_mk;
 /* Fluxes of the LISM Maxwellian into the domain through the outer boundary faces (bc 2, 3):
  * flux[f] = Get_Spotok_inf(n) * area (Setka_MK.cpp:2316-2324); count and mu by the rule of
@@ -246,6 +251,10 @@ typedef struct {
  * face/count/mu sized n_faces by the caller. */
 int64_t or_mk_setup(const or_mesh *m, const or_geom *g, double v_inf, int32_t n_per_face, int32_t *face,
                     int32_t *count, double *mu);
+/* n draws of the exchange sampler alone (M_K_Change_Velosity, units of cp) for fixed U, V, one
+ * SplitMix64 stream per draw from seed; capped = draws that hit the 10000-try cap. */
+void or_mk_change_velosity_batch(int64_t n, uint64_t seed, double a2, const double *U, const double *V, double cp,
+                                 int mistake, double *W, int64_t *capped);
 int or_mk_go(const or_mesh *m, const or_mk *mk, const double *coords, const double *prim, const or_geom *g,
              double *tally, int64_t *stats);
 

@@ -8,8 +8,10 @@ pinned by what the kinetic theory fixes, on the cube with every boundary face an
   birth point not uniform over the face (Get_Random_pozition) moves the cell densities off 1.
 * detailed balance: with protons in the same Maxwellian (rho_Th = 1, V = (v_inf, 0, 0),
   cp = sqrt(2 p_Th / rho_Th) = 1) charge exchange leaves the distribution unchanged, so the density
-  stays 1 while the exchanged atoms move to the cells' zone (sort 3).  A new velocity drawn without
-  the g sigma(g) weighting of M_K_Change_Velosity, or in a wrongly rotated frame, breaks it.
+  stays 1 while the exchanged atoms move to the cells' zone (sort 3).
+* the exchange sampler (M_K_Change_Velosity) against the integral that defines it: mean and mean
+  square of the weighted Maxwellian by quadrature; dropping its g sigma(g) weight fails it (the
+  density checks above do not see that error: measured, the densities stay within noise).
 * bookkeeping: the sorts sum to the total; every particle ends once (left, absorbed, lost or
   dropped at birth); the particle count follows the rule of Setka_MK.cpp:1233,1380-1390."""
 import math
@@ -102,3 +104,41 @@ def test_particle_counts(C, O):
     # V . sum(n A) vanishes): the one-sided inflow formula against the outflow formula
     out = (np.exp(-ux * ux) / (2 * sp) + ux * np.array([math.erfc(-u) for u in ux]) / 2)
     np.testing.assert_allclose(flux.sum(), (out * geom["face_area"][b]).sum(), rtol=1e-9)
+
+
+def _weighted_maxwellian_moments(U, V, cp, a2, npts=40):
+    """Mean and mean square of the proton velocity a neutral at V meets in a charge exchange: the
+    Maxwellian exp(-|w - U|^2) weighted by g sigma(g cp), g = |V - w| (units of cp), by
+    Gauss-Hermite quadrature."""
+    x, w = np.polynomial.hermite.hermgauss(npts)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    wt = w[:, None, None] * w[None, :, None] * w[None, None, :]
+    vx, vy, vz = X + U[0], Y + U[1], Z + U[2]
+    g = np.sqrt((V[0] - vx) ** 2 + (V[1] - vy) ** 2 + (V[2] - vz) ** 2)
+    wt = wt * g * (1.0 - a2 * np.log(g * cp)) ** 2
+    z = wt.sum()
+    return np.array([(wt * vx).sum(), (wt * vy).sum(), (wt * vz).sum()]) / z, (wt * (vx ** 2 + vy ** 2 + vz ** 2)).sum() / z
+
+
+@pytest.mark.parametrize("U,V,cp", [((0.0, 0.0, 0.0), (1.5, 0.3, 0.0), 1.0),
+                                    ((0.4, -0.2, 0.1), (-2.0, 0.5, 0.7), 0.7)])
+def test_exchange_sampler_against_its_integral(O, U, V, cp):
+    """M_K_Change_Velosity draws from exp(-|w - U|^2) |V - w| sigma(|V - w| cp): 400k draws match
+    the quadrature mean (MC noise ~1e-3) and mean square; the same sampler without the rejection
+    step (or_mk.mistake bit 1, the g sigma(g) weight dropped) misses the mean by > 0.1."""
+    m, e = _weighted_maxwellian_moments(np.array(U), np.array(V), cp, 0.102046)
+    W, cap = O.mk_change_velosity(400000, U, V, cp, 0.102046, seed=3)
+    assert cap == 0
+    assert np.abs(W.mean(axis=0) - m).max() < 0.006
+    assert abs((W ** 2).sum(axis=1).mean() - e) < 0.015
+    Wb, _ = O.mk_change_velosity(400000, U, V, cp, 0.102046, seed=3, mistake=2)
+    assert np.abs(Wb.mean(axis=0) - m).max() > 0.1
+
+
+def test_free_flight_catches_an_unweighted_birth(C, O):
+    """The birth velocity without its |V.n| weight (or_mk.mistake bit 0) leaves the free-flight
+    density visibly off 1, so the free-flight pin catches that error."""
+    mesh, prim, _ = _cube(C)
+    om = O.OracleMesh(mesh)
+    d, _ = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, 1e30, 200, seed=7, mistake=1)
+    assert abs(d[:, 0].mean() - 1.0) > 0.04
