@@ -209,8 +209,8 @@ def main():
                     default="fp64,mixed,fp64_exact,mixed_surrogate,fp64_tvd,mixed_tvd,fp64_bera,fp64_fit,mk,checkpoint")
     ap.add_argument("--mk-per-face", type=int, default=100,
                     help="Monte-Carlo neutrals: particles per outer-boundary face (N_per_gran)")
-    ap.add_argument("--mk-cpu-per-face", type=int, default=8,
-                    help="Monte-Carlo CPU baseline: particles per outer face of its sample (about 10 s)")
+    ap.add_argument("--mk-cpu-per-face", type=int, default=100,
+                    help="Monte-Carlo CPU baseline: particles per outer face (100: the same call as the GPU's)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the streaming copy the HBM counter runs are scaled by, then exit")
     a = ap.parse_args()
@@ -303,14 +303,15 @@ def main():
             om = O.OracleMesh(fmesh)
             geom = O.measure(om, fmesh.node_xyz)
             t0 = time.perf_counter()
+            ncpu = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
             _, ost = O.mk_go(om, fmesh.node_xyz, fprim, fparams["v_inf"], fparams.get("par_a_2", 0.102046),
-                             fparams.get("par_Kn", 40.0906), a.mk_cpu_per_face, seed=1, geom=geom)
+                             fparams.get("par_Kn", 40.0906), a.mk_cpu_per_face, seed=1, geom=geom, nthreads=ncpu)
             csec = time.perf_counter() - t0
             del om, geom
             mk_info["cpu_baseline"] = {
-                "value": ost["segments"] / csec, "unit": "segments/s", "cores": 1, "kind": "port",
+                "value": ost["segments"] / csec, "unit": "segments/s", "cores": ncpu, "kind": "port",
                 "sample": "%d particles (%d per outer face) of the same shell4m_fitted flight, C restatement of "
-                          "MK_fly_immit, one thread" % (ost["particles"], a.mk_cpu_per_face)}
+                          "MK_fly_immit, OpenMP over particles" % (ost["particles"], a.mk_cpu_per_face)}
             log("MC cpu baseline: %.3g segments/s" % (ost["segments"] / csec))
     if "fp64_fit" in variants:  # the same cell count with the TS/HP/BS fitted and the rays moved
         fmesh, fprim, fparams, frays = fitted

@@ -514,7 +514,8 @@ def dvigenie(fit, x, params=None):
 # ---- Monte-Carlo neutral densities --------------------------------------------------------------
 class OrMk(ctypes.Structure):
     _fields_ = [("n_in", _I64), ("face", _P), ("count", _P), ("mu", _P), ("seed", ctypes.c_uint64),
-                ("v_inf", _D), ("a2", _D), ("kn", _D), ("max_steps", ctypes.c_int32), ("mistake", ctypes.c_int32)]
+                ("v_inf", _D), ("a2", _D), ("kn", _D), ("max_steps", ctypes.c_int32), ("mistake", ctypes.c_int32),
+                ("nthreads", ctypes.c_int32)]
 
 
 def mk_setup(om, geom, v_inf, n_per_face):
@@ -529,7 +530,7 @@ def mk_setup(om, geom, v_inf, n_per_face):
     return face[:k].copy(), count[:k].copy(), mu[:k].copy()
 
 
-def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000, geom=None, mistake=0):
+def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000, geom=None, mistake=0, nthreads=1):
     """or_mk_go on the nodes xyz and state prim (geom: measure(om, xyz), computed when None;
     mistake: or_mk.mistake, nonzero only in tests that show the pins catch an error);
     returns (densities [n_cells, 5] = sum over sorts, sorts 1..4, per unit volume; stats dict)."""
@@ -539,7 +540,7 @@ def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000, ge
     face, count, mu = mk_setup(om, geom, v_inf, n_per_face)
     g = OrGeom(*[_p(geom[k]) for k, _ in OrGeom._fields_])
     mk = OrMk(len(face), _p(face), _p(count), _p(mu), int(seed), float(v_inf), float(a2), float(kn), int(max_steps),
-              int(mistake))
+              int(mistake), int(nthreads))
     tally = np.zeros((m.n_cells, 5))
     stats = np.zeros(7, np.int64)
     lib().or_mk_go(ctypes.byref(om.c), ctypes.byref(mk), _p(x), _p(np.ascontiguousarray(prim, np.float64)),

@@ -1733,13 +1733,32 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
              double *tally, int64_t *stats)
 {
     const double me = 1.0 / 1836.15;
-    int64_t pid = 0;
     for (int i = 0; i < 7; i++) stats[i] = 0;
-    for (int64_t q = 0; q < mk->n_in; q++) {
+    /* particles are numbered face by face; first[q] = id of face q's first particle */
+    int64_t *first = (int64_t *)malloc(sizeof(int64_t) * (size_t)(mk->n_in + 1));
+    first[0] = 0;
+    for (int64_t q = 0; q < mk->n_in; q++) first[q + 1] = first[q] + mk->count[q];
+    const int64_t n_part = first[mk->n_in];
+    int nt = mk->nthreads > 0 ? mk->nthreads : 1;
+    int64_t s0_ = 0, s1_ = 0, s2_ = 0, s3_ = 0, s4_ = 0, s5_ = 0, s6_ = 0;
+    /* one particle per iteration; its history depends only on its id (its random stream); with
+     * several threads the tallies are added atomically, so their rounding varies with the order */
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt) \
+    reduction(+ : s0_, s1_, s2_, s3_, s4_, s5_, s6_)
+    for (int64_t pid = 0; pid < n_part; pid++) {
+        int64_t st_[7] = {0, 0, 0, 0, 0, 0, 0};
+        int64_t *stats = st_;
+        int64_t lo = 0, hi = mk->n_in - 1;
+        while (lo < hi) {
+            int64_t mid = (lo + hi + 1) >> 1;
+            if (first[mid] <= pid) lo = mid;
+            else hi = mid - 1;
+        }
+        const int64_t q = lo;
         int64_t f = mk->face[q];
         const double *n = g->face_normal + 3 * f;
         const int32_t *fn = m->face_nodes + 4 * f;
-        for (int32_t k = 0; k < mk->count[q]; k++, pid++) {
+        do {
             uint64_t rs = mk_mix(mk->seed ^ mk_mix((uint64_t)pid + 0x9E3779B97F4A7C15ull));
             int capped = 0;
             /* Gran::Get_Random_pozition (Gran.cpp:220-285) */
@@ -1763,7 +1782,7 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
                 P[i] = (1.0 - sqrt(r1)) * A[i] + (sqrt(r1) * (1.0 - r2)) * T1[i] + (r2 * sqrt(r1)) * Cc[i];
             mk_velosity_initial(&rs, mk->v_inf, n, V, &capped, mk->mistake);
             stats[0]++;
-            if (capped) { stats[5]++; continue; } /* no admissible velocity within the cap: dropped */
+            if (capped) { stats[5]++; break; } /* no admissible velocity within the cap: dropped */
             int64_t c = m->face_cells[2 * f];
             const double *cc = g->cell_center + 3 * c;
             for (int i = 0; i < 3; i++) P[i] = P[i] + (cc[i] - P[i]) / 1000.0; /* into the cell */
@@ -1798,8 +1817,14 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
                 I += l / sig;
                 if (I < KSI) {
                     I_do = I;
+                    {
+#pragma omp atomic
                     tally[5 * c] += time * mu;
-                    if (sort >= 1 && sort <= 4) tally[5 * c + sort] += time * mu;
+                    }
+                    if (sort >= 1 && sort <= 4) {
+#pragma omp atomic
+                        tally[5 * c + sort] += time * mu;
+                    }
                     for (int i = 0; i < 3; i++) P[i] += 1.000001 * time * V[i];
                     int64_t nc = m->face_cells[2 * ex] == c ? m->face_cells[2 * ex + 1] : m->face_cells[2 * ex];
                     if (nc < 0) {
@@ -1814,8 +1839,14 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
                     double t_ex = kk / vn;
                     I_do = 0.0;
                     for (int i = 0; i < 3; i++) P[i] += t_ex * V[i];
+                    {
+#pragma omp atomic
                     tally[5 * c] += t_ex * mu;
-                    if (sort >= 1 && sort <= 4) tally[5 * c + sort] += t_ex * mu; /* other sorts: total only */
+                    }
+                    if (sort >= 1 && sort <= 4) { /* other sorts: total only */
+#pragma omp atomic
+                        tally[5 * c + sort] += t_ex * mu;
+                    }
                     double U[3], Vs[3], W[3];
                     double Vp[3] = {s[1], s[2], s[3]};
                     mk_to_sph(P, Vp, &U[0], &U[1], &U[2]);      /* (r, theta, phi) */
@@ -1831,8 +1862,12 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
                 if (it == mk->max_steps - 1 && !done) { stats[4]++; done = 1; }
             }
             if (capped) stats[5]++;
-        }
+        } while (0);
+        s0_ += st_[0]; s1_ += st_[1]; s2_ += st_[2]; s3_ += st_[3]; s4_ += st_[4]; s5_ += st_[5]; s6_ += st_[6];
     }
+    stats[0] = s0_; stats[1] = s1_; stats[2] = s2_; stats[3] = s3_; stats[4] = s4_; stats[5] = s5_;
+    stats[6] = s6_;
+    free(first);
     return 0;
 }
 

@@ -244,6 +244,7 @@ typedef struct {
     int32_t mistake;  /* 0; for tests that show the pins catch a plausible error: bit 0 drops the
                          |V.n| weight of the birth velocity, bit 1 the g sigma(g) weight of the
                          exchanged velocity */
+    int32_t nthreads; /* OpenMP threads (<= 1: one) */
 } or_mk;
 /* Fluxes of the LISM Maxwellian into the domain through the outer boundary faces (bc 2, 3):
  * flux[f] = Get_Spotok_inf(n) * area (Setka_MK.cpp:2316-2324); count and mu by the rule of
