@@ -142,3 +142,15 @@ def test_free_flight_catches_an_unweighted_birth(C, O):
     om = O.OracleMesh(mesh)
     d, _ = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, 1e30, 200, seed=7, mistake=1)
     assert abs(d[:, 0].mean() - 1.0) > 0.04
+
+
+def test_threads_keep_the_histories(C, O):
+    """With several OpenMP threads each particle keeps its own random stream: the counts are the
+    single-thread counts and the densities agree to rounding (the tallies' order varies)."""
+    mesh, _, _ = _cube(C, n=4)
+    prim = _lism(C, mesh)
+    om = O.OracleMesh(mesh)
+    d1, s1 = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, 1.0, 100, seed=9)
+    d4, s4 = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, 1.0, 100, seed=9, nthreads=4)
+    assert s1 == s4
+    np.testing.assert_allclose(d4, d1, rtol=1e-12, atol=1e-15)
