@@ -244,7 +244,9 @@ int titan_cx_exact_batch(int64_t n, const double *in10, int nodes, double *out5)
  * (one per tally: a cell crossed, or a flight to an exchange).  Random numbers:
  * a counter-based SplitMix64 stream per particle from `seed` (the reference reads rnd_my.txt,
  * which it does not ship); particles are numbered face by face in ascending reference face id.
- * The tallies are FP64 atomic sums, so repeated calls agree to rounding, not bit for bit. */
+ * The tallies are FP64 atomic sums, so repeated calls agree to rounding, not bit for bit.  The
+ * first call allocates the flight records and tallies (320 bytes per cell), which the context
+ * keeps until titan_destroy; calls on one context are serialised on its stream. */
 int titan_mk_go(titan_ctx *ctx, int layer, int32_t n_per_face, uint64_t seed, int32_t max_steps,
                 double *density, int64_t *stats);
 
