@@ -82,7 +82,9 @@ def lib():
             "or_cx_source": (None, [_D, _D, _P, _D, _D, _P, _I, _I, _P]),
             "or_mlp_n_params": (_I64, []),
             "or_mlp_forward": (None, [_P, _I64, _P, _P]),
+            "or_mlp_forward_bf16": (None, [_P, _I64, _P, _P]),
             "or_mlp_loss_grad": (_D, [_P, _I64, _P, _P, _P]),
+            "or_set_mistake": (None, [_I]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -90,6 +92,33 @@ def lib():
             f.argtypes = a
         _lib = L
     return _lib
+
+
+# planted mistakes of the restatement (OR_MISTAKE_* of titan_oracle.h), for tests that show a pin
+# catches a plausible error
+MISTAKE_POWELL_MOM_8PI = 1 << 0
+MISTAKE_POWELL_IND_SIGN = 1 << 1
+MISTAKE_POWELL_ENERGY = 1 << 2
+MISTAKE_PDIVB_HALF = 1 << 3
+MISTAKE_ZONE2_8PI = 1 << 4
+MISTAKE_HLLD_VZZ_SIGN = 1 << 5
+MISTAKE_HLLD_BZZ_WEIGHTS = 1 << 6
+MISTAKE_HLLD_TB_BN2 = 1 << 7
+
+
+class planted_mistake:
+    """Context manager: the oracle runs with the given OR_MISTAKE_* bits, then with none."""
+
+    def __init__(self, bits):
+        self.bits = int(bits)
+
+    def __enter__(self):
+        lib().or_set_mistake(self.bits)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_mistake(0)
+        return False
 
 
 def _p(a):
@@ -237,6 +266,16 @@ def mlp_forward(params, x):
     x = np.ascontiguousarray(x, np.float64).reshape(-1, 8)
     y = np.zeros((x.shape[0], 5))
     lib().or_mlp_forward(_p(P), x.shape[0], _p(x), _p(y))
+    return y
+
+
+def mlp_forward_bf16(params, x):
+    """The network with the BF16 matrix-core inference's roundings (BF16 weights, features and
+    activations; FP32 biases)."""
+    P = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 8)
+    y = np.zeros((x.shape[0], 5))
+    lib().or_mlp_forward_bf16(_p(P), x.shape[0], _p(x), _p(y))
     return y
 
 

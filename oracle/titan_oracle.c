@@ -29,6 +29,12 @@
 static const double OR_PI = 3.14159265358979323846;
 static const double OR_EPSB = 1E-4; /* Header.h:96 */
 
+/* Planted mistakes for the tests that show a pin catches a plausible error (0 = the restatement;
+ * the OR_MISTAKE_* bits of titan_oracle.h).  Never set outside those tests. */
+static int or_mistake_bits = 0;
+void or_set_mistake(int bits) { or_mistake_bits = bits; }
+#define MISTAKE(b) ((or_mistake_bits & (b)) != 0)
+
 static double or_spi4(void) { return sqrt(4.0 * OR_PI); } /* Header.h:94 */
 
 /* get_bazis, Help_funk.cpp:219-232: t = normalize(a x n), m = n x t, a = x or y. */
@@ -361,7 +367,7 @@ int or_chlld(int n_state, int n_disc, int flags, double al, double be, double ge
         double tvL, tbL;
         double ttL = r1 * suL * (SL - SM) - bn2;
         if (fabs(ttL) < 0.000000001) { tvL = 0.0; tbL = 0.0; }
-        else { tvL = (SM - vL[0]) / ttL; tbL = (r1 * suL * suL - bn2) / ttL; }
+        else { tvL = (SM - vL[0]) / ttL; tbL = (r1 * suL * suL - (MISTAKE(OR_MISTAKE_HLLD_TB_BN2) ? 0.0 : bn2)) / ttL; }
         vzL[1] = vL[1] - bn * bL[1] * tvL;
         vzL[2] = vL[2] - bn * bL[2] * tvL;
         vzR[1] = vR[1] - bn * bR[1] * tvR;
@@ -385,12 +391,18 @@ int or_chlld(int n_state, int n_disc, int flags, double al, double be, double ge
         double sbn;
         if (fabs(bn) > OR_EPSB) { sbn = fabs(bn) / bn; ibn = 1; }
         else { sbn = 0.0; ibn = 0; SZL = SM; SZR = SM; }
-        vzzL[1] = (rzLs * vzL[1] + rzRs * vzR[1] + sbn * (bzR[1] - bzL[1])) / rzss;
-        vzzL[2] = (rzLs * vzL[2] + rzRs * vzR[2] + sbn * (bzR[2] - bzL[2])) / rzss;
+        const double sbv = MISTAKE(OR_MISTAKE_HLLD_VZZ_SIGN) ? -sbn : sbn;
+        vzzL[1] = (rzLs * vzL[1] + rzRs * vzR[1] + sbv * (bzR[1] - bzL[1])) / rzss;
+        vzzL[2] = (rzLs * vzL[2] + rzRs * vzR[2] + sbv * (bzR[2] - bzL[2])) / rzss;
         vzzR[1] = vzzL[1];
         vzzR[2] = vzzL[2];
-        bzzL[1] = (rzLs * bzR[1] + rzRs * bzL[1] + sbn * rzps * (vzR[1] - vzL[1])) / rzss;
-        bzzL[2] = (rzLs * bzR[2] + rzRs * bzL[2] + sbn * rzps * (vzR[2] - vzL[2])) / rzss;
+        if (MISTAKE(OR_MISTAKE_HLLD_BZZ_WEIGHTS)) {
+            bzzL[1] = (rzLs * bzL[1] + rzRs * bzR[1] + sbn * rzps * (vzR[1] - vzL[1])) / rzss;
+            bzzL[2] = (rzLs * bzL[2] + rzRs * bzR[2] + sbn * rzps * (vzR[2] - vzL[2])) / rzss;
+        } else {
+            bzzL[1] = (rzLs * bzR[1] + rzRs * bzL[1] + sbn * rzps * (vzR[1] - vzL[1])) / rzss;
+            bzzL[2] = (rzLs * bzR[2] + rzRs * bzL[2] + sbn * rzps * (vzR[2] - vzL[2])) / rzss;
+        }
         bzzR[1] = bzzL[1];
         bzzR[2] = bzzL[2];
         double sbzz = bzzL[0] * vzzL[0] + bzzL[1] * vzzL[1] + bzzL[2] * vzzL[2];
@@ -1157,7 +1169,8 @@ static int face_flux_impl(const or_mesh *m, const or_params *p, const double *pr
             if (dnt < loc_time) loc_time = SMIN(loc_time, dnt);
         }
         for (int i = 0; i < 8; i++) flux[9 * f + i] = q[i] * area;
-        flux[9 * f + 8] = 0.5 * spf(n[0], n[1], n[2], qL[5] + qR[5], qL[6] + qR[6], qL[7] + qR[7]) * area;
+        flux[9 * f + 8] = (MISTAKE(OR_MISTAKE_PDIVB_HALF) ? 1.0 : 0.5) *
+                          spf(n[0], n[1], n[2], qL[5] + qR[5], qL[6] + qR[6], qL[7] + qR[7]) * area;
         dt_face[f] = loc_time;
         if (branch) branch[f] = (uint8_t)br;
     }
@@ -1210,21 +1223,27 @@ static int update_impl(const or_mesh *m, const or_params *p, const double *prim_
         if (rho3 < 1e-7) { rho_null = 1; rho3 = 0.1; nrf++; }
         double u3, v3, w3, bx3, by3, bz3, p3;
         if (!rho_null) {
-            u3 = (rho * vx * Volume / Volume2 - time * (P[1] + (bx / cpi4) * P[8]) / Volume2 + time * Smx) / rho3;
-            v3 = (rho * vy * Volume / Volume2 - time * (P[2] + (by / cpi4) * P[8]) / Volume2 + time * Smy) / rho3;
-            w3 = (rho * vz * Volume / Volume2 - time * (P[3] + (bz / cpi4) * P[8]) / Volume2 + time * Smz) / rho3;
-            bx3 = bx * Volume / Volume2 - time * (P[5] + vx * P[8]) / Volume2;
-            by3 = by * Volume / Volume2 - time * (P[6] + vy * P[8]) / Volume2;
-            bz3 = bz * Volume / Volume2 - time * (P[7] + vz * P[8]) / Volume2;
+            /* planted mistakes: a Powell momentum term over 8 pi, an induction term of the wrong
+               sign, an energy term dropped, the Zone_2 compensation over 8 pi */
+            const double cpm = MISTAKE(OR_MISTAKE_POWELL_MOM_8PI) ? 2.0 * cpi4 : cpi4;
+            const double sind = MISTAKE(OR_MISTAKE_POWELL_IND_SIGN) ? -1.0 : 1.0;
+            const double sen = MISTAKE(OR_MISTAKE_POWELL_ENERGY) ? 0.0 : 1.0;
+            const double c2 = MISTAKE(OR_MISTAKE_ZONE2_8PI) ? 8.0 * OR_PI : 4.0 * OR_PI;
+            u3 = (rho * vx * Volume / Volume2 - time * (P[1] + (bx / cpm) * P[8]) / Volume2 + time * Smx) / rho3;
+            v3 = (rho * vy * Volume / Volume2 - time * (P[2] + (by / cpm) * P[8]) / Volume2 + time * Smy) / rho3;
+            w3 = (rho * vz * Volume / Volume2 - time * (P[3] + (bz / cpm) * P[8]) / Volume2 + time * Smz) / rho3;
+            bx3 = bx * Volume / Volume2 - time * (P[5] + sind * vx * P[8]) / Volume2;
+            by3 = by * Volume / Volume2 - time * (P[6] + sind * vy * P[8]) / Volume2;
+            bz3 = bz * Volume / Volume2 - time * (P[7] + sind * vz * P[8]) / Volume2;
             p3 = (((pp / g1 + 0.5 * rho * KVV(vx, vy, vz) + KVV(bx, by, bz) / 25.13274122871834590768) * Volume / Volume2
-                   - time * (P[4] + (dsk / cpi4) * P[8]) / Volume2 + time * SE)
+                   - time * (P[4] + sen * (dsk / cpi4) * P[8]) / Volume2 + time * SE)
                   - 0.5 * rho3 * KVV(u3, v3, w3) - KVV(bx3, by3, bz3) / 25.13274122871834590768) * g1;
             if (m->cell_zone[c] == 2) {
                 double tau = 1.0;
                 double bx4 = bx3 - time * bx3 / tau;
                 double by4 = by3 - time * by3 / tau;
                 double bz4 = bz3 - time * bz3 / tau;
-                double p4 = p3 + (time / tau - KV(time / tau)) * (g1) * KVV(bx3, by3, bz3) / (4.0 * OR_PI);
+                double p4 = p3 + (time / tau - KV(time / tau)) * (g1) * KVV(bx3, by3, bz3) / c2;
                 bx3 = bx4; by3 = by4; bz3 = bz4; p3 = p4;
             }
         } else {
@@ -2156,6 +2175,39 @@ void or_mlp_forward(const double *params, int64_t n, const double *x, double *y)
                 z[o] = acc;
             }
             for (int o = 0; o < v.nout[l]; o++) a[o] = (l < 3) ? silu(z[o]) : z[o];
+        }
+        for (int o = 0; o < MLP_OUT; o++) y[MLP_OUT * s + o] = a[o];
+    }
+}
+
+/* Round to the nearest BF16 (ties to even) of the FP32 rounding of x: the conversion the device
+ * applies to MFMA operands. */
+static double bf16_round(double x)
+{
+    float f = (float)x;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+    memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+/* The network as the BF16 matrix-core inference evaluates it: weights and every layer's input
+ * (the normalised features, then each SiLU activation) rounded to BF16, biases rounded to FP32,
+ * products and sums exact to FP64 (the device accumulates in FP32). */
+void or_mlp_forward_bf16(const double *params, int64_t n, const double *x, double *y)
+{
+    mlp_view v = mlp_layout(params);
+    for (int64_t s = 0; s < n; s++) {
+        double a[MLP_H], z[MLP_H];
+        for (int i = 0; i < MLP_IN; i++) a[i] = bf16_round(x[MLP_IN * s + i]);
+        for (int l = 0; l < 4; l++) {
+            for (int o = 0; o < v.nout[l]; o++) {
+                double acc = (double)(float)v.b[l][o];
+                for (int i = 0; i < v.nin[l]; i++) acc += bf16_round(v.W[l][o * v.nin[l] + i]) * a[i];
+                z[o] = acc;
+            }
+            for (int o = 0; o < v.nout[l]; o++) a[o] = (l < 3) ? bf16_round(silu((double)(float)z[o])) : z[o];
         }
         for (int o = 0; o < MLP_OUT; o++) y[MLP_OUT * s + o] = a[o];
     }

@@ -73,7 +73,9 @@
  *                       of the thermal drag or swapped temperatures in U_m / U_E leave that band).
  *   or_cx_source        sigma = const closed form, Cartesian quadrature, 6-D Monte Carlo,
  *                       bilinearity and Galilean invariance; or_sigma_ls: published values.
- *   or_mlp_*            closed forms and finite differences.
+ *   or_mlp_*            closed forms and finite differences; or_mlp_forward_bf16 against
+ *                       or_mlp_forward on BF16-exact weights and inputs with the activations'
+ *                       rounding bounded by 2^-9 relative.
  *   or_culc_velocity_surface, or_dvigenie  closed-form wave speeds of uniform states on the
  *                       TS/HP/BS faces, node means, the sphere and neighbour smoothing terms,
  *                       straight rays and the end slopes of the ray polynomials, the surface
@@ -95,6 +97,20 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* Planted mistakes (test-only switch, or_set_mistake): each bit makes one plausible restatement
+ * error, so a test can show that its pin catches it.  0 restores the restatement. */
+enum {
+    OR_MISTAKE_POWELL_MOM_8PI = 1 << 0,    /* Powell momentum term B/(8 pi) instead of B/(4 pi)   */
+    OR_MISTAKE_POWELL_IND_SIGN = 1 << 1,   /* Powell induction term +v div B                      */
+    OR_MISTAKE_POWELL_ENERGY = 1 << 2,     /* Powell energy term dropped                          */
+    OR_MISTAKE_PDIVB_HALF = 1 << 3,        /* PdivB without the factor 1/2 of the face mean       */
+    OR_MISTAKE_ZONE2_8PI = 1 << 4,         /* Zone_2 decay: pressure compensation over 8 pi       */
+    OR_MISTAKE_HLLD_VZZ_SIGN = 1 << 5,     /* HLLD double-star v_t: sign(Bn) term negated          */
+    OR_MISTAKE_HLLD_BZZ_WEIGHTS = 1 << 6,  /* HLLD double-star b_t: sqrt(rho*) weights on the wrong side */
+    OR_MISTAKE_HLLD_TB_BN2 = 1 << 7        /* HLLD star b_t: Bn^2 dropped from the left numerator  */
+};
+void or_set_mistake(int bits);
 
 /* boundary codes of a face (reference Type_Gran, Setka_physics.cpp:212-236) */
 enum { OR_BC_INTERIOR = 0, OR_BC_INNER_HARD = 1, OR_BC_OUTER_HARD = 2, OR_BC_OUTER_SOFT = 3,
@@ -291,6 +307,9 @@ void or_cx_source(double n_p, double cp2, const double *vp, double n_h, double c
 /* params layout: W1[64*8] b1[64] W2[64*64] b2[64] W3[64*64] b3[64] W4[5*64] b4[5] (row = out) */
 int64_t or_mlp_n_params(void);
 void or_mlp_forward(const double *params, int64_t n, const double *x, double *y);
+/* the same network with the matrix-core inference's roundings: BF16 weights and layer inputs
+ * (features and activations), FP32 biases */
+void or_mlp_forward_bf16(const double *params, int64_t n, const double *x, double *y);
 /* mean-squared-error loss over n samples and its gradient (same layout as params) */
 double or_mlp_loss_grad(const double *params, int64_t n, const double *x, const double *y_target,
                         double *grad);

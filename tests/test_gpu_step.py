@@ -4,7 +4,7 @@ import hashlib
 import numpy as np
 import pytest
 
-from helpers import rel_err, var_err
+from helpers import cell_err, cell_errors, node_err, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -83,8 +83,10 @@ def test_go_shell_fixed(T, C, O):
     got, _ = _run_gpu(T, mesh, prim, params, 30)
     rc, ref, _, _, _ = O.go(O.OracleMesh(mesh), params, mesh.node_xyz, prim, 30, nthreads=8)
     assert rc == 0
-    # Zone_1 faces use acos/atan/sin/cos/pow (GPU libm vs glibc: last-ulp differences)
-    assert var_err(got, ref) < 1e-12
+    # Zone_1 faces use acos/atan/sin/cos/pow (GPU libm vs glibc: last-ulp differences); per cell,
+    # each conserved variable relative to the cell's own value
+    print("shell fixed, 30 steps: per-cell errors", cell_errors(got, ref))
+    assert cell_err(got, ref) < 1e-12
 
 
 def test_go_shell_moving(T, C, O):
@@ -95,8 +97,9 @@ def test_go_shell_moving(T, C, O):
     assert rc == 0
     assert np.abs(xyz - mesh.node_xyz).max() > 0.0
     # the lagged dt inherits last-ulp differences of the Zone_1 faces, and the nodes move by dt
-    assert var_err(xyz, rxyz) < 1e-14
-    assert var_err(got, ref) < 1e-12
+    print("shell moving, 30 steps: per-cell errors", cell_errors(got, ref), "nodes", node_err(xyz, rxyz))
+    assert node_err(xyz, rxyz) < 1e-14
+    assert cell_err(got, ref) < 1e-12
 
 
 def test_go_cube_moving_bit_exact(T, C, O):

@@ -6,7 +6,7 @@ import importlib
 import numpy as np
 import pytest
 
-from helpers import var_err
+from helpers import cell_err, cell_errors, var_err
 
 pytestmark = pytest.mark.gpu
 
@@ -39,13 +39,6 @@ def _setka(T, C):
     return s, mesh, prim
 
 
-def _bf16(a):
-    a = np.ascontiguousarray(a, np.float32)
-    u = a.view(np.uint32).astype(np.uint64)
-    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
-    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
-
-
 def test_mlp_forward_mfma_vs_oracle(T, C, O):
     sur = importlib.import_module("titan_c-_amd.surrogate")
     rng = np.random.default_rng(1)
@@ -57,12 +50,16 @@ def test_mlp_forward_mfma_vs_oracle(T, C, O):
         for n in (1, 37, 64, 1000, 4097):
             x = rng.normal(0, 1, (n, 8)).astype(np.float32)
             y = s.mlp_forward(x)
-            # oracle on BF16-rounded weights and inputs (the MFMA operands); activations are also
-            # rounded to BF16 between layers on the device
-            ref = O.mlp_forward(_bf16(P), _bf16(x))
+            # the oracle with the device's roundings: BF16 weights, inputs and activations, FP32
+            # biases.  What remains is FP32 accumulation and SiLU, and the rare activation whose
+            # FP32 value sits on a BF16 rounding boundary and rounds the other way (one BF16 ulp,
+            # 2^-8 of that activation)
+            ref = O.mlp_forward_bf16(P, x)
             err = np.abs(y - ref).max(axis=1) / (np.abs(ref).max(axis=1) + 1e-3)
-            assert err.max() < 6e-2, (n, err.max())
-            assert np.median(err) < 1e-2
+            assert err.max() < 2e-2, (n, err.max())
+            assert np.median(err) < 1e-5, (n, np.median(err))
+            if n >= 1000:
+                assert np.percentile(err, 99) < 5e-3, (n, np.percentile(err, 99))
 
 
 def test_mlp_gradients_vs_oracle(T, C, O):
@@ -175,7 +172,8 @@ def test_fused_surrogate_step(T, C, O, precision):
     assert rc == 0
     rc, ref0, _, _ = O.update(om, params, prim, g["cell_volume"], g["cell_volume"], flux, 1e-5, None)
     assert var_err(ref0, ref) > 1e-9  # the source is visible at this step
-    assert var_err(got, ref) < (1e-12 if precision == 0 else 1e-6)
+    print("fused surrogate step, precision %d: per-cell errors %s" % (precision, cell_errors(got, ref)))
+    assert cell_err(got, ref) < (1e-12 if precision == 0 else 1e-6)
 
 
 def test_fused_surrogate_all_fallback(T, C):
@@ -193,4 +191,4 @@ def test_fused_surrogate_all_fallback(T, C):
             st = s.Go(2)
             runs[mode] = (s.get_state(), st)
     assert runs[T.SRC_MLP][1]["n_fallback"] == mesh.n_cells
-    assert var_err(runs[T.SRC_MLP][0], runs[T.SRC_EXACT][0]) < 1e-12
+    assert cell_err(runs[T.SRC_MLP][0], runs[T.SRC_EXACT][0]) < 1e-12

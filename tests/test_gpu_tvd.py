@@ -8,7 +8,7 @@ the reference, their algebraic equivalents on the device), compared to 1e-12."""
 import numpy as np
 import pytest
 
-from helpers import var_err
+from helpers import cell_err, cell_errors, var_err
 
 pytestmark = pytest.mark.gpu
 
@@ -96,7 +96,8 @@ def test_go_tvd_cube(T, C, O, precision):
         assert st["dt_last"] == dth[-1]
         np.testing.assert_array_equal(got, ref)
     else:
-        assert var_err(got, ref) < 1e-5
+        print("TVD cube, mixed precision, 50 steps: per-cell errors", cell_errors(got, ref))
+        assert cell_err(got, ref) < 1e-5
 
 
 def test_go_tvd_cube_moving_bit_exact(T, C, O):
@@ -124,7 +125,8 @@ def test_go_tvd_shell(T, C, O, moving):
     kw = dict(disp=mesh.node_disp) if moving else {}
     rc, ref, rxyz, _, _ = O.go(O.OracleMesh(mesh, tvd=True), params, mesh.node_xyz, prim, 30, nthreads=8, **kw)
     assert rc == 0
-    assert var_err(got, ref) < 1e-12
+    print("TVD shell, moving %s: per-cell errors %s" % (moving, cell_errors(got, ref)))
+    assert cell_err(got, ref) < 1e-12
     # the second-order step differs from the first-order one
     rc, ref1, _, _, _ = O.go(O.OracleMesh(mesh), dict(params, tvd=0), mesh.node_xyz, prim, 30, nthreads=8, **kw)
     assert var_err(ref1, ref) > 1e-8
@@ -175,4 +177,5 @@ def test_go_tvd_shell_zone_bands(T, C, O):
     got, _, _ = _go(T, mesh, prim, params, 20)
     rc, ref, _, _, _ = O.go(om, params, mesh.node_xyz, prim, 20, nthreads=8)
     assert rc == 0
-    assert var_err(got, ref) < 1e-12
+    print("TVD zone-banded shell: per-cell errors", cell_errors(got, ref))
+    assert cell_err(got, ref) < 1e-12

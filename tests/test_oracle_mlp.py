@@ -52,3 +52,42 @@ def test_backward_finite_differences(O):
         Pm[i] -= h
         fd = (O.mlp_loss_grad(Pp, x, y)[0] - O.mlp_loss_grad(Pm, x, y)[0]) / (2 * h)
         assert abs(fd - g[i]) < 1e-7 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def bf16(a):
+    """Nearest BF16 (ties to even) of the FP32 rounding of a."""
+    a = np.ascontiguousarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def test_forward_bf16_roundings(O):
+    """The BF16 inference restatement: on a single unit-weight path each activation is rounded to
+    BF16 (through FP32, ties to even) before the next layer; with BF16-exact weights and inputs a
+    generic network differs from the FP64 one only by those roundings, at most 2^-9 relative per
+    activation."""
+    P = np.zeros(N_PARAMS)
+    P[0] = 1.0
+    o2 = 64 * 8 + 64
+    P[o2] = 1.0
+    o3 = o2 + 64 * 64 + 64
+    P[o3] = 1.0
+    o4 = o3 + 64 * 64 + 64
+    P[o4 + 2 * 64] = 1.0
+    silu = lambda v: v / (1 + np.exp(-v))
+    x = np.zeros((5, 8))
+    x[:, 0] = [-2.3, 0.31, 1.7, 1.0 + 2.0 ** -8, 5.123]
+    y = O.mlp_forward_bf16(P, x)
+    h = bf16(x[:, 0])
+    for _ in range(3):
+        h = bf16(silu(h.astype(np.float32).astype(np.float64)))
+    np.testing.assert_array_equal(y[:, 2], h)
+    assert bf16(np.array([1.0 + 2.0 ** -8]))[0] == 1.0            # a tie goes to the even neighbour
+    assert bf16(np.array([1.0 + 3 * 2.0 ** -8]))[0] == 1.0 + 2.0 ** -6
+    rng = np.random.default_rng(3)
+    P = bf16(rng.normal(0, 0.3, N_PARAMS))
+    x = bf16(rng.normal(0, 1, (200, 8)))
+    y16, y64 = O.mlp_forward_bf16(P, x), O.mlp_forward(P, x)
+    err = np.abs(y16 - y64).max(axis=1) / np.abs(y64).max(axis=1)
+    assert 0 < err.max() < 0.05 and np.median(err) < 1e-2

@@ -132,14 +132,25 @@ def test_regions_monotone_and_flux_continuous(O, method):
         reg = br & 7
         assert np.all(np.diff(reg.astype(int)) >= 0), reg
         assert reg[0] == 1 and reg[-1] == (3 if method == 1 else (4 if method == 2 else 6))
-        # continuity of mass, momentum and energy fluxes across each region change
-        jumps = np.nonzero(np.diff(reg.astype(int)))[0]
-        h = ws[1] - ws[0]
-        scale = np.abs(f[:, :5]).max()
-        for j in jumps:
-            d = np.abs(f[j + 1, :5] - f[j, :5]).max()
-            slope = max(np.abs(f[j, :5] - f[j - 1, :5]).max(), np.abs(f[j + 2, :5] - f[j + 1, :5]).max())
-            assert d <= 4 * slope + 1e-9 * scale, (method, trial, j, reg[j], reg[j + 1], d, slope)
+        # continuity of every flux component across each region change: the ALE flux F - wU of
+        # each side's HLL-type state is continuous at its outer waves by construction, and at the
+        # contact only when the star and double-star states satisfy the jump conditions there, so
+        # the tangential-B components (6, 7) pin the double-star field
+        assert fan_discontinuity(f, reg) == 0, (method, trial)
+
+
+def fan_discontinuity(f, reg):
+    """Number of region changes across which some flux component jumps by more than the local
+    slope allows."""
+    jumps = np.nonzero(np.diff(reg.astype(int)))[0]
+    bad = 0
+    for j in jumps:
+        for comp in (slice(0, 5), slice(5, 8)):
+            scale = np.abs(f[:, comp]).max()
+            d = np.abs(f[j + 1, comp] - f[j, comp]).max()
+            slope = max(np.abs(f[j, comp] - f[j - 1, comp]).max(), np.abs(f[j + 2, comp] - f[j + 1, comp]).max())
+            bad += int(d > 4 * slope + 1e-9 * scale)
+    return bad
 
 
 def test_fatal_branch_on_nan(O):
@@ -185,3 +196,131 @@ def test_hlld_overwrite_outside_fan(O):
     _, _, ds, br, _ = O.chlld_batch(3, -nrm[i:i + 1], qR[i:i + 1], qL[i:i + 1], w=-w[i:i + 1])
     assert -w[i] < ds[0, 0]          # left of the fan ...
     assert br[0] & 7 == 3            # ... yet the double-star region was selected
+
+
+# ---- HLLD on isolated discontinuities (Miyoshi & Kusano, J. Comput. Phys. 208 (2005) 315) --------
+# HLLD resolves an isolated contact and an isolated rotational discontinuity exactly: the ALE flux
+# is F(U_L) - w U_L on the left of the discontinuity's speed and F(U_R) - w U_R on its right.  F is
+# the textbook ideal-MHD flux in Gaussian units, written here independently of the oracle.
+
+def mhd_flux(q, n):
+    rho, v, p, B = q[0], q[1:4], q[4], q[5:8]
+    vn, Bn = v @ n, B @ n
+    pt = p + B @ B / (8 * np.pi)
+    E = p / (GAMMA - 1) + 0.5 * rho * v @ v + B @ B / (8 * np.pi)
+    mom = rho * v * vn + pt * n - Bn * B / (4 * np.pi)
+    en = (E + pt) * vn - Bn * (v @ B) / (4 * np.pi)
+    ind = vn * B - Bn * v
+    return np.array([rho * vn, *mom, en, *ind])
+
+
+def frame(n):
+    a = np.array([1.0, 0.0, 0.0]) if abs(n[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    t = np.cross(a, n)
+    t /= np.linalg.norm(t)
+    return t, np.cross(n, t)
+
+
+def isolated_error(O, qL, qR, nrm, speed, ws, method=3):
+    """Largest error of chlld's flux against the exact flux of a single discontinuity moving at
+    `speed`, over face speeds ws, relative to the flux scale."""
+    m = ws.size
+    f, _, _, br, rc = O.chlld_batch(method, np.tile(nrm, (m, 1)), np.tile(qL, (m, 1)), np.tile(qR, (m, 1)), w=ws)
+    assert rc == 0
+    err = 0.0
+    for i, w in enumerate(ws):
+        q = qL if w < speed else qR
+        exact = mhd_flux(q, nrm) - w * conserved(q)
+        scale = np.abs(mhd_flux(q, nrm)).max() + abs(w) * np.abs(conserved(q)).max()
+        err = max(err, np.abs(f[i] - exact).max() / scale)
+    return err, br
+
+
+def contact_case(rng):
+    nrm = random_normals(rng, 1)[0]
+    t, m = frame(nrm)
+    qL = random_states(rng, 1)[0]
+    qL[5:8] += (0.5 + abs(qL[5:8] @ nrm)) * SPI4 * nrm       # |B_n| well above epsb
+    qR = qL.copy()
+    qR[0] = qL[0] * np.exp(rng.uniform(-2, 2))                 # only the density jumps
+    return qL, qR, nrm, qL[1:4] @ nrm
+
+
+def rotational_case(rng, right_going):
+    nrm = random_normals(rng, 1)[0]
+    t, m = frame(nrm)
+    q = random_states(rng, 1)[0]
+    rho = q[0]
+    Bn = (0.5 + rng.uniform(0, 2)) * SPI4 * rng.choice([-1.0, 1.0])
+    Bt = rng.normal(0, 2.0, 2) * SPI4
+    phi = rng.uniform(0.3, 2.8)
+    c, s = np.cos(phi), np.sin(phi)
+    BtR = np.array([c * Bt[0] - s * Bt[1], s * Bt[0] + c * Bt[1]])   # rotated, same magnitude
+    vt = rng.normal(0, 1.0, 2)
+    sg = np.sign(Bn) * (1.0 if right_going else -1.0)
+    vtR = vt - sg * (BtR - Bt) / np.sqrt(4 * np.pi * rho)             # Rankine-Hugoniot across it
+    vn = rng.normal(0, 1.0)
+    qL, qR = q.copy(), q.copy()
+    qL[1:4] = vn * nrm + vt[0] * t + vt[1] * m
+    qR[1:4] = vn * nrm + vtR[0] * t + vtR[1] * m
+    qL[5:8] = Bn * nrm + Bt[0] * t + Bt[1] * m
+    qR[5:8] = Bn * nrm + BtR[0] * t + BtR[1] * m
+    speed = vn + (1.0 if right_going else -1.0) * abs(Bn) / np.sqrt(4 * np.pi * rho)
+    return qL, qR, nrm, speed
+
+
+def hlld_isolated_errors(O, seed=60, trials=30):
+    rng = np.random.default_rng(seed)
+    e_c = e_r = 0.0
+    for _ in range(trials):
+        qL, qR, nrm, u = contact_case(rng)
+        ws = u + np.concatenate([-np.geomspace(5, 1e-3, 6), np.geomspace(1e-3, 5, 6)])
+        e, br = isolated_error(O, qL, qR, nrm, u, ws)
+        assert np.all((br >> 3) & 1 == 1)      # ibn = 1: the Alfven branch of HLLD is taken
+        e_c = max(e_c, e)
+        for rg in (True, False):
+            qL, qR, nrm, lam = rotational_case(rng, rg)
+            ws = lam + np.concatenate([-np.geomspace(5, 1e-3, 6), np.geomspace(1e-3, 5, 6)])
+            e, br = isolated_error(O, qL, qR, nrm, lam, ws)
+            e_r = max(e_r, e)
+    return e_c, e_r
+
+
+def test_hlld_isolated_contact_and_rotational_discontinuity(O):
+    e_c, e_r = hlld_isolated_errors(O)
+    assert e_c < 1e-12 and e_r < 1e-12, (e_c, e_r)
+
+
+def test_hll_does_not_resolve_the_contact(O):
+    """The contact check discriminates: HLL smears the contact (mass flux at the contact's speed
+    of order S_L S_R (rho_R - rho_L)/(S_R - S_L)), HLLD gives the exact zero."""
+    rng = np.random.default_rng(61)
+    qL, qR, nrm, u = contact_case(rng)
+    e_hll, _ = isolated_error(O, qL, qR, nrm, u, np.array([u + 1e-3]), method=1)
+    e_hlld, _ = isolated_error(O, qL, qR, nrm, u, np.array([u + 1e-3]), method=3)
+    assert e_hll > 1e-3 and e_hlld < 1e-12, (e_hll, e_hlld)
+
+
+def test_hlld_pins_catch_planted_mistakes(O):
+    """Planted errors of the star and double-star tangential states (oracle's test-only switch):
+    the sign(Bn) term of v_t**, the sqrt(rho*) weights of b_t** and the Bn^2 of b_t* each fail the
+    isolated-discontinuity check or the tangential-B flux continuity at the contact."""
+    with O.planted_mistake(O.MISTAKE_HLLD_VZZ_SIGN):
+        _, e_r = hlld_isolated_errors(O, trials=5)
+    assert e_r > 1e-3, e_r
+    with O.planted_mistake(O.MISTAKE_HLLD_TB_BN2):
+        e_c, e_r = hlld_isolated_errors(O, trials=5)
+    assert max(e_c, e_r) > 1e-3, (e_c, e_r)
+    rng = np.random.default_rng(43)
+    bad = 0
+    with O.planted_mistake(O.MISTAKE_HLLD_BZZ_WEIGHTS):
+        for trial in range(10):
+            qL, qR = random_states(rng, 1), random_states(rng, 1)
+            nrm = random_normals(rng, 1)
+            qR[0, 5:8] += (qL[0, 5:8] @ nrm[0] - qR[0, 5:8] @ nrm[0]) * nrm[0]
+            _, _, ds, _, _ = O.chlld_batch(3, nrm, qL, qR)
+            ws = np.linspace(ds[0, 0] - 1.0, ds[0, 2] + 1.0, 4001)
+            m = ws.size
+            f, _, _, br, _ = O.chlld_batch(3, np.repeat(nrm, m, 0), np.repeat(qL, m, 0), np.repeat(qR, m, 0), w=ws)
+            bad += fan_discontinuity(f, br & 7) > 0
+    assert bad >= 8, bad

@@ -110,3 +110,102 @@ def test_brio_wu_profile(C, O):
     cc = mesh.node_xyz[mesh.cell_nodes].mean(axis=1)
     mid = (cc[:, 0] > 0.3) & (cc[:, 0] < 0.7)
     assert ((out[mid, 0] > 0.2) & (out[mid, 0] < 0.9)).mean() > 0.2
+
+
+# ---- Powell div B terms, PdivB and the Zone_2 decay against closed forms -------------------------
+# Setka_physics.cpp:1916-1928 (Powell source in momentum, induction and energy), :2448-2450 (PdivB =
+# 1/2 (B_L + B_R).n A), :1932-1947 (Zone_2 decay, tau = 1).  With B = G x + B0 linear on an
+# unjittered cube, the face mean of the two cell values is B at the face centre, so a cell's
+# signed PdivB sum is the midpoint rule of the surface integral of B.n, which is exact for a
+# linear field: sum_f s_f PdivB_f = tr(G) V.  Fed alone into the update (every other flux 0, fixed
+# mesh), the Powell terms then move the cell by the closed forms
+#   d(rho v) = -dt (B/4pi) tr G,   dB = -dt v tr G,   dE = -dt (v.B/4pi) tr G.
+
+G_LIN = np.array([[0.7, -0.3, 0.2], [0.4, 1.1, -0.5], [-0.6, 0.25, -0.45]])
+B0_LIN = np.array([0.3, -0.8, 1.2])
+U_LIN = np.array([1.3, 0.2, -0.3, 0.5, 0.9])   # rho, v, p
+
+
+def energy(q):
+    return (q[:, 4] / (5.0 / 3.0 - 1.0) + 0.5 * q[:, 0] * (q[:, 1:4] ** 2).sum(1)
+            + (q[:, 5:8] ** 2).sum(1) / (8 * np.pi))
+
+
+def powell_errors(C, O, n=6, dt=1e-2):
+    """Relative errors of the four closed forms above on the interior cells of an n^3 cube."""
+    mesh, _, params, om, g = setup(C, O, n=n, jitter=0.0)
+    cc = g["cell_center"]
+    prim = np.empty((mesh.n_cells, 8))
+    prim[:, :4] = U_LIN[:4]
+    prim[:, 4] = U_LIN[4]
+    prim[:, 5:8] = (cc @ G_LIN.T + B0_LIN) * SPI4
+    rc, flux, _, _ = O.face_flux(om, params, prim, g)
+    assert rc == 0
+    V = g["cell_volume"]
+    sign = np.where(mesh.face_cells[mesh.cell_faces, 0] == np.arange(mesh.n_cells)[:, None], 1.0, -1.0)
+    bdry_face = mesh.face_cells[:, 1] < 0
+    interior = ~bdry_face[mesh.cell_faces].any(axis=1)
+    trG = np.trace(G_LIN) * SPI4
+    div = (sign * flux[mesh.cell_faces, 8]).sum(1) / V
+    e_div = np.abs(div[interior] - trG).max() / abs(trG)
+    only_pdivb = np.zeros_like(flux)
+    only_pdivb[:, 8] = flux[:, 8]
+    rc, new, nr, npf = O.update(om, params, prim, V, V, only_pdivb, dt)
+    assert rc == 0 and nr == 0 and npf == 0
+    q, s = new[interior], prim[interior]
+    B, v = s[:, 5:8], s[:, 1:4]
+    d_mom = q[:, 0:1] * q[:, 1:4] - s[:, 0:1] * v
+    exp_mom = -dt * B / (4 * np.pi) * trG
+    e_mom = np.abs(d_mom - exp_mom).max() / np.abs(exp_mom).max()
+    d_b = q[:, 5:8] - B
+    exp_b = -dt * v * trG
+    e_ind = np.abs(d_b - exp_b).max() / np.abs(exp_b).max()
+    d_e = energy(q) - energy(s)
+    exp_e = -dt * (v * B).sum(1) / (4 * np.pi) * trG
+    e_en = np.abs(d_e - exp_e).max() / np.abs(exp_e).max()
+    return dict(div=e_div, mom=e_mom, ind=e_ind, energy=e_en)
+
+
+def test_powell_terms_closed_form(C, O):
+    e = powell_errors(C, O)
+    assert e["div"] < 1e-13, e
+    assert e["mom"] < 1e-11 and e["ind"] < 1e-11 and e["energy"] < 1e-11, e
+
+
+def test_powell_pins_catch_planted_mistakes(C, O):
+    """Each planted error (oracle's test-only switch) leaves the closed forms by far more than the
+    rounding bound above; a shared sign or 4 pi error would otherwise pass the GPU comparison."""
+    cases = {O.MISTAKE_POWELL_MOM_8PI: "mom", O.MISTAKE_POWELL_IND_SIGN: "ind",
+             O.MISTAKE_POWELL_ENERGY: "energy", O.MISTAKE_PDIVB_HALF: "div"}
+    for bits, key in cases.items():
+        with O.planted_mistake(bits):
+            e = powell_errors(C, O)
+        assert e[key] > 0.1, (bits, e)
+
+
+def zone2_errors(C, O, dt=1e-2):
+    """Zone_2 decay with zero fluxes on a fixed mesh: B -> B (1 - dt/tau) and the total energy
+    changes by exactly -(dt/tau)^2 |B|^2/8pi (the compensation puts the (dt - dt^2) part of the
+    lost magnetic energy into p), tau = 1."""
+    mesh, prim, params, om, g = setup(C, O, n=4, jitter=0.15)
+    rng = np.random.default_rng(7)
+    prim = prim.copy()
+    prim[:, 1:4] = rng.normal(0, 0.5, (mesh.n_cells, 3))
+    prim[:, 5:8] = rng.normal(0, 2.0, (mesh.n_cells, 3)) * SPI4
+    mesh.cell_zone[:] = 2
+    om = O.OracleMesh(mesh)
+    V = g["cell_volume"]
+    rc, new, _, _ = O.update(om, params, prim, V, V, np.zeros((mesh.n_faces, 9)), dt)
+    assert rc == 0
+    e_b = np.abs(new[:, 5:8] - prim[:, 5:8] * (1 - dt)).max() / np.abs(prim[:, 5:8]).max()
+    exp_e = -dt * dt * (prim[:, 5:8] ** 2).sum(1) / (8 * np.pi)
+    e_e = np.abs(energy(new) - energy(prim) - exp_e).max() / np.abs(exp_e).max()
+    return e_b, e_e
+
+
+def test_zone2_decay_closed_form(C, O):
+    e_b, e_e = zone2_errors(C, O)
+    assert e_b < 1e-15 and e_e < 1e-9, (e_b, e_e)
+    with O.planted_mistake(O.MISTAKE_ZONE2_8PI):
+        e_b, e_e = zone2_errors(C, O)
+    assert e_e > 10, e_e   # the energy then changes by -dt |B|^2/8pi: 1/dt times the exact change

@@ -579,10 +579,20 @@ extern "C" int titan_gran_potok(titan_ctx *c, int now, double dt, double *face_f
     return rc;
 }
 
+// The inputs a source mode reads are present, and its output buffer exists.
+static int prepare_sources(titan_ctx *c, int mode) {
+    if (mode == TITAN_SRC_NONE) return TITAN_OK;
+    if (mode != TITAN_SRC_BERA && mode != TITAN_SRC_EXACT && mode != TITAN_SRC_MLP) return TITAN_E_ARG;
+    if (mode == TITAN_SRC_BERA ? !c->d_hfluid : !c->d_neutrals) return TITAN_E_ARG;
+    if (mode == TITAN_SRC_MLP && !c->mlp.ready) return TITAN_E_NOT_TRAINED;
+    if (!c->d_src) HIPCHK(hipMalloc((void **)&c->d_src, sizeof(double) * 5 * c->nc));
+    return TITAN_OK;
+}
+
 static int compute_sources(titan_ctx *c, int layer, int mode) {
     if (mode == TITAN_SRC_NONE) return TITAN_OK;
-    if (mode == TITAN_SRC_BERA ? !c->d_hfluid : !c->d_neutrals) return TITAN_E_ARG;
-    if (!c->d_src) HIPCHK(hipMalloc((void **)&c->d_src, sizeof(double) * 5 * c->nc));
+    const int prc = prepare_sources(c, mode);
+    if (prc != TITAN_OK) return prc;
     if (mode == TITAN_SRC_BERA) {
         const titan_params &p = c->params;
         launch_bera_cells(c->stream, c->nc, c->d_prim[layer], c->d_hfluid, c->n_hfluid, p.par_a_2,
@@ -605,17 +615,48 @@ static int compute_sources(titan_ctx *c, int layer, int mode) {
     return TITAN_OK;
 }
 
+// The two events that time a titan_go call, destroyed on every return path.
+struct CallEvents {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~CallEvents() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
 // The step loop: `area` selects the cells updated (cell_in_area), `restart` starts the dt schedule
-// again at 1e-5 as each Setka::Go call does.
+// again at 1e-5 as each Setka::Go call does.  Every argument and input check comes before the first
+// change to the context (layers, step count, dt slots), so a call that fails them leaves the
+// context as it was.
 static int go_impl(titan_ctx *c, int64_t steps, titan_step_stats *stats, int area, bool restart) {
     if (!c || steps < 0) return TITAN_E_ARG;
+    // surrogate source: fused into the step kernel, out-of-box cells redone by the fixup kernel
+    const bool fused = c->params.source_mode == TITAN_SRC_MLP;
+    SurArgs sur{};
+    if (fused) {
+        if (!c->d_neutrals || c->tiles.tile_cells > kTileThreads / 2) return TITAN_E_ARG;
+        const int rc = mlp_fused_args(c->mlp, c->nc, c->d_neutrals, &sur);
+        if (rc != TITAN_OK) return rc;
+    } else {
+        const int rc = prepare_sources(c, c->params.source_mode);
+        if (rc != TITAN_OK) return rc;
+    }
+    const int nodes = c->params.cx_nodes > 0 ? c->params.cx_nodes : 100;
+    // fitted surfaces: Culc_Velocity_surface and dvigenie of every ray write the new node layer
+    // before each step (Setka_physics.cpp:1616-1627)
+    const bool fitting = c->params.move_mesh == 2;
+    FitDev fd{};
+    if (fitting) {
+        const int rc = fit_prepare(c->fit, c->params, c->stream, &fd);
+        if (rc != TITAN_OK) return rc;
+    }
     c->dp.area = area;
     const DevMesh M = c->mesh();
     const int64_t nn3 = 3 * c->nn;
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, c->stream));
+    CallEvents ev;
+    HIPCHK(hipEventCreate(&ev.e0));
+    HIPCHK(hipEventCreate(&ev.e1));
+    HIPCHK(hipEventRecord(ev.e0, c->stream));
     if (c->started && c->geom_dirty) {
         // nodes were replaced (titan_download_cell_parameters): measure both layers again, as
         // Setka::Go does on every entry (Setka_physics.cpp:1495-1510)
@@ -635,23 +676,6 @@ static int go_impl(titan_ctx *c, int64_t steps, titan_step_stats *stats, int are
         c->now1 = 1;
         c->now2 = 0;
         c->started = true;
-    }
-    // surrogate source: fused into the step kernel, out-of-box cells redone by the fixup kernel
-    const bool fused = c->params.source_mode == TITAN_SRC_MLP;
-    SurArgs sur{};
-    if (fused) {
-        if (!c->d_neutrals || c->tiles.tile_cells > kTileThreads / 2) return TITAN_E_ARG;
-        const int rc = mlp_fused_args(c->mlp, c->nc, c->d_neutrals, &sur);
-        if (rc != TITAN_OK) return rc;
-    }
-    const int nodes = c->params.cx_nodes > 0 ? c->params.cx_nodes : 100;
-    // fitted surfaces: Culc_Velocity_surface and dvigenie of every ray write the new node layer
-    // before each step (Setka_physics.cpp:1616-1627)
-    const bool fitting = c->params.move_mesh == 2;
-    FitDev fd{};
-    if (fitting) {
-        const int rc = fit_prepare(c->fit, c->params, c->stream, &fd);
-        if (rc != TITAN_OK) return rc;
     }
     for (int64_t s = 0; s < steps; s++) {
         c->now2 = c->now1;
@@ -688,13 +712,11 @@ static int go_impl(titan_ctx *c, int64_t steps, titan_step_stats *stats, int are
         }
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventRecord(ev.e1, c->stream));
     StepScalars hs;
     int rc = check_scalars(c, c->d_S, &hs);
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventElapsedTime(&ms, ev.e0, ev.e1);
     if (c->timing) timer_collect(c);
     if (stats) {
         stats->steps_done = c->step;
