@@ -218,6 +218,22 @@ int titan_chlld_batch(int64_t n, const int8_t *n_state, const int8_t *n_disc, co
                       int K, const double *kL, const double *kR, double *q, double *k, double *ds,
                       uint8_t *branch, int precision);
 
+/* The same batch solve on DEVICE arrays the caller owns (every array argument is a device
+ * pointer; nullable ones as above), enqueued on `stream` (a hipStream_t; NULL = the null stream)
+ * and waited for before returning.  It allocates nothing: its only working memory is one 256-byte
+ * array per GPU, resident until the process ends, holding the fatal-branch and bad-code flags,
+ * cleared at the start of every call.  Calls that overlap on one GPU (other threads or unordered
+ * streams) race on those flags.  Returns TITAN_E_ARG for n > TITAN_NOALLOC_MAX_STATES or a null
+ * required array without launching anything, TITAN_E_ARG after the solve when some n_state or
+ * n_disc lies outside 0..3 (its outputs are then meaningless), TITAN_E_RIEMANN as titan_chlld_batch
+ * does.  No counterpart in the reference: an addition for callers that cannot allocate device
+ * memory (code under HIP graph capture, per-step loops, harnesses that forbid hipMalloc). */
+#define TITAN_NOALLOC_MAX_STATES ((int64_t)1 << 31)
+int titan_chlld_batch_noalloc(int64_t n, const int8_t *n_state, const int8_t *n_disc, const uint8_t *flags,
+                              const double *normal, const double *w, const double *qL, const double *qR,
+                              int K, const double *kL, const double *kR, double *q, double *k, double *ds,
+                              uint8_t *branch, int precision, void *stream);
+
 /* ---- charge-exchange source ---------------------------------------------------------------- */
 /* The plasma source of Setka::Calc_sourse_MF_Bera (Setka_physics.cpp:766-1258) for the states of
  * `layer`: out [n_cells*5] = mass, momentum (3), energy (lab frame) per unit volume and time.

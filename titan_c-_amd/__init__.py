@@ -113,6 +113,7 @@ _SIGS = {
     "titan_current_layer": (_I, [_P]),
     "titan_tvd_cells": (_I, [_P, _P]),
     "titan_chlld_batch": (_I, [_I64, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _I]),
+    "titan_chlld_batch_noalloc": (_I, [_I64, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P]),
     "titan_sources": (_I, [_P, _I, _I, _P]),
     "titan_mk_go": (_I, [_P, _I, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, _P, _P]),
     "titan_cx_exact_batch": (_I, [_I64, _P, _I, _P]),

@@ -37,6 +37,7 @@ UNITS = {
     "k_mk.hip": ["-ffp-contract=off"],
     "k_step32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "k_mlp.hip": ["-ffp-contract=fast"],
+    "k_noalloc.hip": ["-ffp-contract=off"],
 }
 HOST_UNITS = ["titan_host.cpp", "titan_mesh.cpp", "titan_io.cpp", "titan_fit.cpp"]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
