@@ -16,8 +16,11 @@ quoted on.  Beside it:
   roofline    the step kernel of (i): algorithmic bytes per launch / its mean duration, timed with
               HIP events around every launch on the context's stream;
   surrogate   training-set size, epochs, loss, and its source error against the exact kernel;
-  precision   FP32-mixed vs FP64 after --precision-steps steps from the same seeded input:
-              relative L2/Linf of rho, V, p, B and the div B norms;
+  precision   FP32-mixed vs FP64 after --precision-steps steps from the same seeded input: per-cell
+              relative errors (largest, where, percentiles), field L2/Linf and the div B norms;
+  accuracy_gate  (iii) FP32-mixed + surrogate vs (ii) FP64 + exact source after --gate-steps steps
+              on --gate-workload (default: the 1M-cell moving shell), per cell;
+  shell1m_sizes  plain FP64 on the 1M-cell fixed and moving shells with their own roofline;
   cpu_baseline  the oracle's FP64 step (OpenMP over faces and cells) on this host's cores, on a
               bounded sample of the same workload.
 """
@@ -100,6 +103,39 @@ def divb(T, mesh, params, prim, xyz):
     return (sign * flux[cf, 8]).sum(axis=1) / vol, vol
 
 
+GAMMA = 5.0 / 3.0
+AU = 0.237455   # 1 AU in code units (Phys_param.h:117)
+
+
+def per_cell_errors(got, ref, r_au):
+    """Per-cell relative errors of `got` against `ref`, each cell against its own value: rho and p
+    by |d|/|ref|, the total energy E likewise, momentum rho v and B by the norm of the vector
+    difference over the cell's |rho v| or |B|.  For each: the largest value, the radius (AU) of the
+    cell that has it, and the 99th / 99.9th percentiles and the median over the cells."""
+    def cons(q):
+        E = q[:, 4] / (GAMMA - 1) + 0.5 * q[:, 0] * (q[:, 1:4] ** 2).sum(1) + (q[:, 5:8] ** 2).sum(1) / (8 * np.pi)
+        return q[:, 0:1] * q[:, 1:4], E
+    mg, Eg = cons(got)
+    mr, Er = cons(ref)
+    tiny = 1e-300
+    e = {"rho": np.abs(got[:, 0] - ref[:, 0]) / np.maximum(np.abs(ref[:, 0]), tiny),
+         "p": np.abs(got[:, 4] - ref[:, 4]) / np.maximum(np.abs(ref[:, 4]), tiny),
+         "E": np.abs(Eg - Er) / np.maximum(np.abs(Er), tiny),
+         "rho_v": np.linalg.norm(mg - mr, axis=1) / np.maximum(np.linalg.norm(mr, axis=1), tiny),
+         "B": np.linalg.norm(got[:, 5:8] - ref[:, 5:8], axis=1) / np.maximum(np.linalg.norm(ref[:, 5:8], axis=1), tiny)}
+    out = {}
+    for k, v in e.items():
+        i = int(np.argmax(v))
+        out[k] = {"max": float(v[i]), "r_au_at_max": round(float(r_au[i]), 2),
+                  "p99": float(np.percentile(v, 99)), "p999": float(np.percentile(v, 99.9)),
+                  "median": float(np.median(v))}
+    return out
+
+
+def cell_radius_au(mesh):
+    return np.linalg.norm(mesh.node_xyz[mesh.cell_nodes].mean(axis=1), axis=1) / AU
+
+
 def precision_report(T, mesh, prim, params, steps):
     res = {}
     for name, prec in (("fp64", T.PREC_FP64), ("mixed", T.PREC_MIXED)):
@@ -111,20 +147,54 @@ def precision_report(T, mesh, prim, params, steps):
                              seconds=time.perf_counter() - t0)
     a, b = res["mixed"]["state"], res["fp64"]["state"]
     groups = {"rho": [0], "V": [1, 2, 3], "p": [4], "B": [5, 6, 7]}
-    out = {"steps": steps, "reference": "FP64 GPU step (bit-exact with the oracle at the parity sizes)"}
+    out = {"steps": steps, "reference": "FP64 GPU step (bit-exact with the oracle at the parity sizes)",
+           "norm": "per_cell: |d|/|ref| per cell (vectors: norm of the difference over the cell's own norm); "
+                   "field: rel_l2 over the mesh and rel_linf = max|d| / max|ref| of the field"}
+    out["per_cell"] = per_cell_errors(a, b, cell_radius_au(mesh))
+    out["field"] = {}
     for g, cols in groups.items():
         d, r = a[:, cols] - b[:, cols], b[:, cols]
-        out[g] = {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(r)),
-                  "rel_linf": float(np.abs(d).max() / np.abs(r).max())}
+        out["field"][g] = {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(r)),
+                           "rel_linf": float(np.abs(d).max() / np.abs(r).max())}
     bmag = np.linalg.norm(b[:, 5:8], axis=1)
     for name in ("fp64", "mixed"):
         dv, vol = divb(T, mesh, params, res[name]["state"], res[name]["xyz"])
         h = np.cbrt(vol)
         out["divB_" + name] = {"l2_h_over_B": float(np.sqrt(np.mean((dv * h / np.maximum(bmag, 1e-300)) ** 2))),
                                "max_abs": float(np.abs(dv).max())}
-    out["rho_p_within_1e-4"] = bool(max(out["rho"]["rel_linf"], out["p"]["rel_linf"]) <= 1e-4)
+    out["rho_p_per_cell_within_1e-4"] = bool(max(out["per_cell"]["rho"]["max"], out["per_cell"]["p"]["max"]) <= 1e-4)
     out["mixed_fp64_face_solves_last_step"] = int(res["mixed"]["stats"]["n_fp64_faces"])
     out["t_sim"] = res["fp64"]["stats"]["t_sim"]
+    return out
+
+
+def gate_report(T, C, mlp, workload, steps):
+    """The accuracy gate on the configuration it is about: (iii) FP32-mixed + surrogate source
+    against (ii) FP64 + exact source (FP64 quadrature every step), `steps` steps from the same
+    seeded input on `workload`, per cell.  Beside it, (i) FP64 without a source against (ii): how
+    far the source itself moves the state, the scale the surrogate's error is to be read against."""
+    mesh, prim, params = C.build(workload)
+    nh = C.neutrals(mesh.n_cells)
+    runs = {}
+    for name, prec, mode in (("exact", T.PREC_FP64, T.SRC_EXACT), ("surrogate", T.PREC_MIXED, T.SRC_MLP),
+                             ("none", T.PREC_FP64, T.SRC_NONE)):
+        with T.Setka(mesh, dict(params, precision=prec, source_mode=mode, cx_nodes=100)) as s:
+            s.set_state(prim, 0)
+            s.set_neutrals(nh)
+            s.mlp_set(*mlp)
+            t0 = time.perf_counter()
+            st = s.Go(steps)
+            runs[name] = dict(state=s.get_state(), seconds=time.perf_counter() - t0, stats=st)
+        log("gate %s: %d steps in %.1f s" % (name, steps, runs[name]["seconds"]))
+    r_au = cell_radius_au(mesh)
+    out = {"workload": workload, "n_cells": mesh.n_cells, "steps": steps,
+           "compared": "(iii) FP32-mixed + surrogate vs (ii) FP64 + exact source, per cell",
+           "per_cell": per_cell_errors(runs["surrogate"]["state"], runs["exact"]["state"], r_au),
+           "source_effect_per_cell": per_cell_errors(runs["none"]["state"], runs["exact"]["state"], r_au),
+           "fallback_cells_last_step": int(runs["surrogate"]["stats"]["n_fallback"]),
+           "seconds": {k: round(v["seconds"], 2) for k, v in runs.items()}}
+    pc = out["per_cell"]
+    out["rho_p_per_cell_within_1e-4"] = bool(max(pc["rho"]["max"], pc["p"]["max"]) <= 1e-4)
     return out
 
 
@@ -205,8 +275,12 @@ def main():
     ap.add_argument("--exact-steps", type=int, default=3)
     ap.add_argument("--train-states", type=int, default=2_000_000)
     ap.add_argument("--precision-steps", type=int, default=1000)
+    ap.add_argument("--gate-steps", type=int, default=1000,
+                    help="steps of the (iii) mixed + surrogate vs (ii) FP64 + exact source comparison (0: skip)")
+    ap.add_argument("--gate-workload", default="shell1m_moving")
     ap.add_argument("--variants",
-                    default="fp64,mixed,fp64_exact,mixed_surrogate,fp64_tvd,mixed_tvd,fp64_bera,fp64_fit,mk,checkpoint")
+                    default="fp64,mixed,fp64_exact,mixed_surrogate,fp64_tvd,mixed_tvd,fp64_bera,fp64_fit,mk,checkpoint,"
+                            "shell1m,shell1m_moving")
     ap.add_argument("--mk-per-face", type=int, default=100,
                     help="Monte-Carlo neutrals: particles per outer-boundary face (N_per_gran)")
     ap.add_argument("--mk-cpu-per-face", type=int, default=100,
@@ -330,6 +404,24 @@ def main():
         fit_info = None
     del fitted
     ckpt = checkpoint_timing(T, mesh, prim, params) if "checkpoint" in variants else None
+    sizes = {}
+    for wl in ("shell1m", "shell1m_moving"):  # the 1M-cell fixed and moving shells, plain FP64
+        if wl in variants:
+            m1, p1, par1 = C.build(wl)
+            r = run_variant(T, m1, p1, dict(par1, precision=T.PREC_FP64), a.steps, a.warmup)
+            mv = bool(par1["move_mesh"])
+            bts = algorithmic_bytes(m1.n_cells, m1.n_faces, m1.n_nodes, mv)
+            sizes[wl] = {"n_cells": m1.n_cells, "n_faces": m1.n_faces, "moving": mv,
+                         "cell_updates_per_s": m1.n_cells * r["steps"] / r["seconds"],
+                         "ms_per_step": r["seconds"] * 1e3 / r["steps"], "step_kernel_ms": r["step_kernel_ms"],
+                         "algorithmic_bytes": bts,
+                         "hbm_frac": round(bts / (r["step_kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            log("%s: %s" % (wl, sizes[wl]))
+            del m1, p1
+    gate = None
+    if a.gate_steps > 0 and "mixed_surrogate" in variants and sur_info is not None:
+        gate = gate_report(T, C, mlp, a.gate_workload, a.gate_steps)
+        log("gate: per-cell rho %s p %s" % (gate["per_cell"]["rho"], gate["per_cell"]["p"]))
 
     def cups(name):
         r = res.get(name)
@@ -362,7 +454,8 @@ def main():
     if a.precision_steps > 0:
         try:
             prec = precision_report(T, mesh, prim, params, a.precision_steps)
-            log("precision after %d steps: rho %s p %s" % (a.precision_steps, prec["rho"], prec["p"]))
+            log("precision after %d steps: per-cell rho %s p %s" % (a.precision_steps, prec["per_cell"]["rho"],
+                                                                     prec["per_cell"]["p"]))
         except T.TitanError as e:  # a step that stops with a status code is reported, not hidden
             prec = {"steps": a.precision_steps, "error": str(e)}
             log("precision run stopped: %s" % e)
@@ -396,6 +489,8 @@ def main():
         "roofline": roof,
         "surrogate": sur_info,
         "precision": prec,
+        "accuracy_gate": gate,
+        "shell1m_sizes": sizes,
         "checkpoint": ckpt,
         "fitted_surfaces": fit_info,
         "mc_neutrals": mk_info,

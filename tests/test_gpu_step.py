@@ -171,3 +171,22 @@ def test_nan_state_stops_where_the_reference_stops(T, C, O, precision):
                 s.Go(1)
         assert e.value.code == T.TITAN_E_RIEMANN
         assert e.value.index == min(failing)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_tile_layout_does_not_change_the_result(T, C, O, precision, monkeypatch):
+    """Faces on a tile border are solved by both tiles from the same inputs with the same
+    arithmetic, so the step's bits do not depend on where the tiles are cut (tile size set through
+    TITAN_TILE_CELLS at titan_create): 20 moving-shell steps, FP64 and FP32-mixed."""
+    mesh, prim, params = small_shell(C, moving=True)
+    params = dict(params, precision=precision)
+    runs = []
+    for cells in ("128", "48", "16"):
+        monkeypatch.setenv("TITAN_TILE_CELLS", cells)
+        with T.Setka(mesh, params) as s:
+            s.set_state(prim, 0)
+            s.Go(20)
+            runs.append((s.get_state(), s.tile_info()["tiles"]))
+    assert runs[0][1] < runs[1][1] < runs[2][1]
+    for got, _ in runs[1:]:
+        np.testing.assert_array_equal(got, runs[0][0])

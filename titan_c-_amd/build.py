@@ -5,7 +5,10 @@ Usage: python titan_c-_amd/build.py [--jobs N]
 Flags per translation unit:
   * FP64 parity kernels (k_step64, k_cx, k_fit, k_mk): -ffp-contract=off, IEEE division / square root, so the
     arithmetic is evaluated in the reference's operation order;
-  * mixed-precision flux (k_step32): contracted FP32 with fast division / square root;
+  * mixed-precision flux (k_step32): FP32 with fast division / square root, multiply-adds contracted
+    within an expression only (-ffp-contract=on), so a function inlined at two sites (a cell's
+    Zone_1 frame in the cell phase and in a neighbouring tile's face phase) gives the same bits at
+    both, and a face on a tile border gets the same flux from both tiles;
   * every HIP unit: packed-FP32 instructions disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
 """
 import argparse
@@ -35,7 +38,7 @@ UNITS = {
     "k_cx.hip": ["-ffp-contract=off"],
     "k_fit.hip": ["-ffp-contract=off"],
     "k_mk.hip": ["-ffp-contract=off"],
-    "k_step32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "k_step32.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "k_mlp.hip": ["-ffp-contract=fast"],
     "k_noalloc.hip": ["-ffp-contract=off"],
 }

@@ -162,6 +162,7 @@ struct TileMesh {
     int64_t n_tiles;
     int32_t tile_cells, max_nodes, max_faces;
     int32_t vol_cached;         // vol[] holds each cell's volume at the step's old node layer
+    const int32_t *cell_off;    // [n_tiles+1] first device cell of each tile (tiles are runs of cells)
     const int32_t *node_off;    // [n_tiles+1] into nodes
     const int32_t *nodes;       // device node ids of each tile, ascending
     const int32_t *own;         // [n_tiles*2] device node range whose X2 this tile writes

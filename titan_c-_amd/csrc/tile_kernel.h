@@ -298,6 +298,14 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
             for (int i = 0; i < 8; i++) { qL[i] = tA[i]; qR[i] = tB[i]; }
         }
     }
+    // what the face needs after the solve (the PdivB flux and the CFL distance) is formed before
+    // it, so that the states' field components and the two centres are not live across the solve
+    const double pdivb = 0.5 * spf(n[0], n[1], n[2], qL[5] + qR[5], qL[6] + qR[6], qL[7] + qR[7]) * area;
+    double dist = 0.0;
+    if (bc == 0) {
+        const double d0 = ca[0] - cb[0], d1 = ca[1] - cb[1], d2 = ca[2] - cb[2];
+        dist = sqrt_n(d0 * d0 + d1 * d1 + d2 * d2) / 2.0;
+    }
     RiemannOut<double, 0> o;
     if constexpr (sizeof(RT) == sizeof(double)) {
         chlld<double, 0>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, nullptr, nullptr, o);
@@ -310,15 +318,13 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
     }
     if (!o.ok || !okbc) report_error(S, kErrFaceRiemann, TM.face_ref[fb + f]);
     if (bc == 0 && o.ok) {
-        const double d0 = ca[0] - cb[0], d1 = ca[1] - cb[1], d2 = ca[2] - cb[2];
-        const double dist = sqrt_n(d0 * d0 + d1 * d1 + d2 * d2) / 2.0;
         const double dnt = P.kfl * dist / (smax(fabs((double)o.dsl), fabs((double)o.dsr)) + fabs(w));
         if (dnt < cand) cand = dnt;
     }
     double *fl = L.flux + 9 * f;
 #pragma unroll
     for (int i = 0; i < 8; i++) fl[i] = o.q[i] * area;
-    fl[8] = 0.5 * spf(n[0], n[1], n[2], qL[5] + qR[5], qL[6] + qR[6], qL[7] + qR[7]) * area;
+    fl[8] = pdivb;
     return true;
 }
 
@@ -430,9 +436,8 @@ __global__ void __launch_bounds__(kTileThreads, 2)
     const int per = (int)((TM.n_tiles + kXcds - 1) / kXcds);
     const int t = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
     if (t >= TM.n_tiles) return;  // padding blocks of the last run (whole workgroup, before any barrier)
-    const int T = TM.tile_cells;
-    const int64_t c0 = (int64_t)t * T;
-    const int ncl = (int)min((int64_t)T, M.n_cells - c0);
+    const int64_t c0 = TM.cell_off[t];
+    const int ncl = TM.cell_off[t + 1] - (int)c0;
     const int nb = TM.node_off[t], nn = TM.node_off[t + 1] - nb;
     const int fb = TM.face_off[t], nf = TM.face_off[t + 1] - fb;
     const int own_lo = TM.own[2 * t], own_hi = TM.own[2 * t + 1];
@@ -513,7 +518,7 @@ __global__ void __launch_bounds__(kTileThreads, 2)
     if (tid < nf) face_fetch(rec0, prim1, sA0, sB0);
     __syncthreads();
 
-    // phase 3: faces of the tile
+    // phase 3: faces of the tile (at most two per thread: tiles are cut at 2 x kTileThreads faces)
     double cand = 10000000.0;
     bool pending = false;
     if (tid < nf)  // the first face, with its prefetched record and states

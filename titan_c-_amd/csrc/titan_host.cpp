@@ -67,7 +67,7 @@ struct titan_ctx {
     // tiles of the fused step kernel
     TileMesh tiles{};
     int64_t tile_info[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int32_t *d_t_node_off = nullptr, *d_t_nodes = nullptr, *d_t_own = nullptr, *d_t_face_off = nullptr,
+    int32_t *d_t_cell_off = nullptr, *d_t_node_off = nullptr, *d_t_nodes = nullptr, *d_t_own = nullptr, *d_t_face_off = nullptr,
             *d_t_face_ref = nullptr;
     uint64_t *d_t_faces = nullptr;
     uint16_t *d_t_cell_lnodes = nullptr, *d_t_cell_lfaces = nullptr;
@@ -339,13 +339,16 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
         }
         int tile_start = 128;
         if (const char *e = std::getenv("TITAN_TILE_CELLS")) tile_start = std::max(16, std::min(1024, std::atoi(e)));
+        // at most two face slots per thread: a tile with more faces would hold its workgroup for a
+        // third pass of the face phase that only a few lanes use
+        const int face_cap = 2 * kTileThreads;
         TileData td;
         bool fits = false;
         for (int pass = 0; pass < 2 && !fits; pass++) {
             // 256 bytes stay free for the kernel's static LDS (the dt reduction's wavefront minima)
             const size_t budget = pass == 0 ? (size_t)lds_limit / 2 - 256 : (size_t)lds_limit - 256;
             for (int tile_cells = tile_start; tile_cells >= 16; tile_cells -= 16) {
-                if (!build_tiles(nc, nf, nn, tile_cells, cell_nodes.data(), cell_faces.data(), face_nodes.data(),
+                if (!build_tiles(nc, nf, nn, tile_cells, face_cap, cell_nodes.data(), cell_faces.data(), face_nodes.data(),
                                  face_cells.data(), face_code.data(), face_bidx.data(), face_ref.data(), td))
                     continue;
                 TileMesh probe{};
@@ -356,6 +359,7 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
             }
         }
         if (!fits) { titan_destroy(c); return TITAN_E_TOPOLOGY; }
+        up(dev_upload(&c->d_t_cell_off, td.cell_off, c->stream));
         up(dev_upload(&c->d_t_node_off, td.node_off, c->stream));
         up(dev_upload(&c->d_t_nodes, td.nodes, c->stream));
         up(dev_upload(&c->d_t_own, td.own, c->stream));
@@ -370,7 +374,7 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
         T.tile_cells = td.tile_cells;
         T.max_nodes = td.max_nodes;
         T.max_faces = td.max_faces;
-        T.node_off = c->d_t_node_off; T.nodes = c->d_t_nodes; T.own = c->d_t_own; T.face_off = c->d_t_face_off;
+        T.cell_off = c->d_t_cell_off; T.node_off = c->d_t_node_off; T.nodes = c->d_t_nodes; T.own = c->d_t_own; T.face_off = c->d_t_face_off;
         T.faces = reinterpret_cast<const ulonglong2 *>(c->d_t_faces); T.face_ref = c->d_t_face_ref;
         T.cell_lnodes = c->d_t_cell_lnodes; T.cell_lfaces = c->d_t_cell_lfaces;
         if (hipMalloc((void **)&c->d_vol, sizeof(double) * nc) != hipSuccess) { titan_destroy(c); return TITAN_E_HIP; }
@@ -429,7 +433,7 @@ extern "C" int titan_destroy(titan_ctx *c) {
     void *ptrs[] = {c->d_cell_nodes, c->d_cell_faces, c->d_face_nodes, c->d_face_cells, c->d_face_bidx,
                     c->d_face_ref, c->d_cell_ref, c->d_cell_code, c->d_face_code, c->d_bstate, c->d_disp,
                     c->d_X[0], c->d_X[1], c->d_prim[0], c->d_prim[1], c->d_geom[0], c->d_geom[1], c->d_flux,
-                    c->d_src, c->d_neutrals, c->d_hfluid, c->d_S, c->d_S2, c->d_t_node_off, c->d_t_nodes, c->d_t_own,
+                    c->d_src, c->d_neutrals, c->d_hfluid, c->d_S, c->d_S2, c->d_t_cell_off, c->d_t_node_off, c->d_t_nodes, c->d_t_own,
                     c->d_t_face_off, c->d_t_face_ref, c->d_t_faces, c->d_t_cell_lnodes, c->d_t_cell_lfaces,
                     c->d_vol, c->d_face_tvd, c->d_t_face_tvd, c->d_mk};
     for (void *q : ptrs)

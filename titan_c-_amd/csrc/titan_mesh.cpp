@@ -257,23 +257,63 @@ void build_order(const MeshView &m, DeviceOrder &o) {
         if (o.node_r2d[v] < 0) { o.node_r2d[v] = (int32_t)next; o.node_d2r[next] = (int32_t)v; next++; }
 }
 
-bool build_tiles(int64_t nc, int64_t nf, int64_t nn, int tile_cells, const int32_t *cell_nodes,
+bool build_tiles(int64_t nc, int64_t nf, int64_t nn, int tile_cells, int face_cap, const int32_t *cell_nodes,
                  const int32_t *cell_faces, const int32_t *face_nodes, const int32_t *face_cells,
                  const uint32_t *face_code, const int32_t *face_bidx, const int32_t *face_ref, TileData &t) {
-    const int64_t T = tile_cells;
-    const int64_t nt = (nc + T - 1) / T;
     t = TileData();
     t.tile_cells = tile_cells;
+    std::vector<int32_t> nstamp(nn, -1), nlocal(nn, 0), fstamp(nf, -1), flocal(nf, 0);
+    // tile boundaries: runs of consecutive cells, closed before a cell that would take the tile past
+    // tile_cells cells, face_cap faces or 4095 nodes
+    t.cell_off.push_back(0);
+    {
+        int64_t ti = 0, ncell = 0, nface = 0, nnode = 0;
+        for (int64_t c = 0; c < nc; c++) {
+            int64_t add_f = 0, add_n = 0;
+            for (int j = 0; j < 6; j++) {
+                const int32_t f = cell_faces[6 * c + j];
+                if (fstamp[f] != ti) add_f++;
+            }
+            for (int i = 0; i < 8; i++) {
+                const int32_t v = cell_nodes[8 * c + i];
+                if (nstamp[v] != ti) {
+                    bool again = false;  // a node repeated within this cell counts once
+                    for (int k = 0; k < i; k++) again |= cell_nodes[8 * c + k] == v;
+                    add_n += again ? 0 : 1;
+                }
+            }
+            if (ncell > 0 && (ncell + 1 > tile_cells || nface + add_f > face_cap || nnode + add_n > 4095)) {
+                t.cell_off.push_back((int32_t)c);
+                ti++;
+                ncell = nface = nnode = 0;
+                add_f = 6;
+                add_n = 0;
+                for (int i = 0; i < 8; i++) {
+                    bool again = false;
+                    for (int k = 0; k < i; k++) again |= cell_nodes[8 * c + k] == cell_nodes[8 * c + i];
+                    add_n += again ? 0 : 1;
+                }
+            }
+            for (int j = 0; j < 6; j++) fstamp[cell_faces[6 * c + j]] = (int32_t)ti;
+            for (int i = 0; i < 8; i++) nstamp[cell_nodes[8 * c + i]] = (int32_t)ti;
+            ncell++;
+            nface += add_f;
+            nnode += add_n;
+        }
+        t.cell_off.push_back((int32_t)nc);
+        std::fill(nstamp.begin(), nstamp.end(), -1);
+        std::fill(fstamp.begin(), fstamp.end(), -1);
+    }
+    const int64_t nt = (int64_t)t.cell_off.size() - 1;
     t.node_off.assign(nt + 1, 0);
     t.face_off.assign(nt + 1, 0);
     t.own.assign(2 * nt, 0);
     t.cell_lnodes.assign(8 * nc, 0);
     t.cell_lfaces.assign(6 * nc, 0);
-    std::vector<int32_t> nstamp(nn, -1), nlocal(nn, 0), fstamp(nf, -1), flocal(nf, 0);
     std::vector<int32_t> lst;
     int64_t seen_max = -1;
     for (int64_t ti = 0; ti < nt; ti++) {
-        const int64_t c0 = ti * T, c1 = std::min(nc, c0 + T);
+        const int64_t c0 = t.cell_off[ti], c1 = t.cell_off[ti + 1];
         t.own[2 * ti] = (int32_t)(seen_max + 1);
         // nodes
         lst.clear();

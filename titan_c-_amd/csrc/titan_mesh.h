@@ -28,13 +28,14 @@ void build_order(const MeshView &m, DeviceOrder &o);
 // Host image of the tiles of the fused step kernel (see TileMesh in device_mesh.h).
 struct TileData {
     int32_t tile_cells = 0, max_nodes = 0, max_faces = 0;
-    std::vector<int32_t> node_off, nodes, own, face_off, face_ref, face_dev;  // face_dev: device face id
+    std::vector<int32_t> cell_off, node_off, nodes, own, face_off, face_ref, face_dev;  // face_dev: device face id
     std::vector<uint64_t> faces;  // 2 words per record
     std::vector<uint16_t> cell_lnodes, cell_lfaces;
 };
-// All arrays in device numbering; face_code/face_bidx as uploaded; returns false if a tile needs
-// more than 4095 local nodes.
-bool build_tiles(int64_t nc, int64_t nf, int64_t nn, int tile_cells, const int32_t *cell_nodes,
+// Tiles are runs of consecutive device cells of at most tile_cells cells and face_cap faces (and
+// 4095 nodes).  All arrays in device numbering; face_code/face_bidx as uploaded; returns false if a
+// tile's face has a node outside the tile's cells.
+bool build_tiles(int64_t nc, int64_t nf, int64_t nn, int tile_cells, int face_cap, const int32_t *cell_nodes,
                  const int32_t *cell_faces, const int32_t *face_nodes, const int32_t *face_cells,
                  const uint32_t *face_code, const int32_t *face_bidx, const int32_t *face_ref, TileData &t);
 
