@@ -125,16 +125,21 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
     const T cfR = (qp + qm) / T(2);
     const T ptR = p2 + b2R / T(2);
 
-    const T aC = (aaL + aaR) / T(2);
-    const T b2o = (b22 + b21) / T(2);
-    const T cC = sqrt_t<T>(gamma * ap / ro);
-    qp = sqrt_t<T>(b2o + cC * (cC + T(2) * aC));
-    qm = sqrt_t<T>(b2o + cC * (cC - T(2) * aC));
-    const T cfC = (qp + qm) / T(2);
     const T vC1 = (vL[0] + vR[0]) / T(2);
 
     T SL = T(0), SR = T(0);
     if (left_ydar) n_disc = 2;
+    // the fast speed of the mean state enters only the n_disc = 1, 2 speed estimates and Lax's
+    // (Phys_param.cpp:1008-1037,1053-1059); the step path (n_disc = 0, HLLC/HLLD) skips it
+    T cfC = T(0);
+    if (n_disc == 1 || n_disc == 2 || n_state == 0) {
+        const T aC = (aaL + aaR) / T(2);
+        const T b2o = (b22 + b21) / T(2);
+        const T cC = sqrt_t<T>(gamma * ap / ro);
+        qp = sqrt_t<T>(b2o + cC * (cC + T(2) * aC));
+        qm = sqrt_t<T>(b2o + cC * (cC - T(2) * aC));
+        cfC = (qp + qm) / T(2);
+    }
     if (n_disc == 0) {
         SL = smin((vL[0] - cfL), (vR[0] - cfR));
         SR = smax((vL[0] + cfL), (vR[0] + cfR));
@@ -274,23 +279,17 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
         const T suLm = suL / (SL - SM);
         const T rzR = r2 * suRm;
         const T rzL = r1 * suLm;
-        const T ptz = div_by_t<T>(suR * r2 * ptL - suL * r1 * ptR + r1 * r2 * suR * suL * (vR[0] - vL[0]), dsm, ydsm);
+        // ptz, the star states and the energy of the star regions are formed only after the region
+        // is chosen, and only for the side(s) it uses: a face whose velocity lies outside the fan
+        // (every supersonic face) needs none of them.
+        auto star_pt = [&]() -> T {
+            return div_by_t<T>(suR * r2 * ptL - suL * r1 * ptR + r1 * r2 * suR * suL * (vR[0] - vL[0]), dsm, ydsm);
+        };
         // A region left of the contact uses the left side's values and one right of it the right
         // side's: selecting the operands first (a negation or a select is exact) gives one code
         // path per kind of region, so a wavefront whose faces fall on both sides of the contact
         // runs it once.
         if (n_state == 2) {  // HLLC, Phys_param.cpp:1258-1475
-            T vzR1 = vR[1] + UZ[5] * (bR[1] - UZ[6]) / suR / r2;
-            T vzR2 = vR[2] + UZ[5] * (bR[2] - UZ[7]) / suR / r2;
-            T vzL1 = vL[1] + UZ[5] * (bL[1] - UZ[6]) / suL / r1;
-            T vzL2 = vL[2] + UZ[5] * (bL[2] - UZ[7]) / suL / r1;
-            T bzR1 = UZ[6], bzR2 = UZ[7], bzL1 = UZ[6], bzL2 = UZ[7];
-            const T sbvz = (UZ[5] * UZ[1] + UZ[6] * UZ[2] + UZ[7] * UZ[3]) / UZ[0];
-            if (fabs(UZ[5]) <= epsb) {
-                vzR1 = vR[1]; vzR2 = vR[2]; vzL1 = vL[1]; vzL2 = vL[2];
-                bzR1 = bR[1] * suRm; bzR2 = bR[2] * suRm;
-                bzL1 = bL[1] * suLm; bzL2 = bL[2] * suLm;
-            }
             if (left_ydar) wv = SL;
             if (SL >= wv || left_ydar) region = 1;
             else if (SL <= wv && SM >= wv) region = 2;
@@ -301,9 +300,25 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
             if (region == 2 || region == 3) {
                 const T e = R ? e2 : e1, sm = R ? suRm : suLm, pt = R ? ptR : ptL, v0 = R ? vR[0] : vL[0];
                 const T sbv = R ? sbv2 : sbv1, S = R ? SR : SL, rz = R ? rzR : rzL;
+                // the star velocity and field of the chosen side; (x / su) / r with the two
+                // correctly rounded reciprocals shared by both components
+                const T su = R ? suR : suL, rr = R ? r2 : r1;
+                const T bt1 = R ? bR[1] : bL[1], bt2 = R ? bR[2] : bL[2];
+                const T vt1 = R ? vR[1] : vL[1], vt2 = R ? vR[2] : vL[2];
+                T vz1, vz2, bz1, bz2;
+                if (fabs(UZ[5]) <= epsb) {
+                    vz1 = vt1; vz2 = vt2;
+                    bz1 = bt1 * sm; bz2 = bt2 * sm;
+                } else {
+                    const T ysu = T(1) / su, yr = T(1) / rr;
+                    vz1 = vt1 + div_by_t<T>(div_by_t<T>(UZ[5] * (bt1 - UZ[6]), su, ysu), rr, yr);
+                    vz2 = vt2 + div_by_t<T>(div_by_t<T>(UZ[5] * (bt2 - UZ[7]), su, ysu), rr, yr);
+                    bz1 = UZ[6]; bz2 = UZ[7];
+                }
+                const T sbvz = (UZ[5] * UZ[1] + UZ[6] * UZ[2] + UZ[7] * UZ[3]) / UZ[0];
+                const T ptz = star_pt();
                 const T ez = e * sm + (ptz * SM - pt * v0 + UZ[5] * (sbv - sbvz)) / (S - SM);
-                const T UZs[8] = {rz, SM * rz, (R ? vzR1 : vzL1) * rz, (R ? vzR2 : vzL2) * rz, ez, UZ[5],
-                                  R ? bzR1 : bzL1, R ? bzR2 : bzL2};
+                const T UZs[8] = {rz, SM * rz, vz1 * rz, vz2 * rz, ez, UZ[5], bz1, bz2};
 #pragma unroll
                 for (int ik = 0; ik < 8; ik++) {
                     const T f = R ? fr(ik) : fl(ik), u = R ? ur(ik) : ul(ik);
@@ -324,29 +339,6 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
         } else {  // HLLD, Phys_param.cpp:1476-1870
             const T bn = UZ[5];
             const T bn2 = bn * bn;
-            T tvR, tbR, tvL, tbL;
-            const T ttR = r2 * suR * (SR - SM) - bn2;
-            if (fabs(ttR) < T(0.000000001)) { tvR = T(0); tbR = T(0); }
-            else {
-                const T y = T(1) / ttR;
-                tvR = div_by_t<T>(SM - vR[0], ttR, y);
-                tbR = div_by_t<T>(r2 * suR * suR - bn2, ttR, y);
-            }
-            const T ttL = r1 * suL * (SL - SM) - bn2;
-            if (fabs(ttL) < T(0.000000001)) { tvL = T(0); tbL = T(0); }
-            else {
-                const T y = T(1) / ttL;
-                tvL = div_by_t<T>(SM - vL[0], ttL, y);
-                tbL = div_by_t<T>(r1 * suL * suL - bn2, ttL, y);
-            }
-            const T vzL1 = vL[1] - bn * bL[1] * tvL;
-            const T vzL2 = vL[2] - bn * bL[2] * tvL;
-            const T vzR1 = vR[1] - bn * bR[1] * tvR;
-            const T vzR2 = vR[2] - bn * bR[2] * tvR;
-            const T bzL1 = bL[1] * tbL, bzL2 = bL[2] * tbL;
-            const T bzR1 = bR[1] * tbR, bzR2 = bR[2] * tbR;
-            const T sbvL = bn * SM + bzL1 * vzL1 + bzL2 * vzL2;
-            const T sbvR = bn * SM + bzR1 * vzR1 + bzR2 * vzR2;
             const T rzRs = sqrt_t<T>(rzR);
             const T rzLs = sqrt_t<T>(rzL);
             const T rzss = rzRs + rzLs;
@@ -369,6 +361,38 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
             if (j == -1) { out.ok = 0; j = 1; region = 0; }
             else region = j;
             const bool R = j >= 4;
+            // star states of the sides the region uses: the left one for j = 2, 3, 4, the right
+            // one for j = 3, 4, 5 (the double-star states combine both)
+            T vzL1 = T(0), vzL2 = T(0), bzL1 = T(0), bzL2 = T(0), sbvL = T(0);
+            T vzR1 = T(0), vzR2 = T(0), bzR1 = T(0), bzR2 = T(0), sbvR = T(0);
+            if (j >= 2 && j <= 4) {
+                T tvL, tbL;
+                const T ttL = r1 * suL * (SL - SM) - bn2;
+                if (fabs(ttL) < T(0.000000001)) { tvL = T(0); tbL = T(0); }
+                else {
+                    const T y = T(1) / ttL;
+                    tvL = div_by_t<T>(SM - vL[0], ttL, y);
+                    tbL = div_by_t<T>(r1 * suL * suL - bn2, ttL, y);
+                }
+                vzL1 = vL[1] - bn * bL[1] * tvL;
+                vzL2 = vL[2] - bn * bL[2] * tvL;
+                bzL1 = bL[1] * tbL; bzL2 = bL[2] * tbL;
+                sbvL = bn * SM + bzL1 * vzL1 + bzL2 * vzL2;
+            }
+            if (j >= 3 && j <= 5) {
+                T tvR, tbR;
+                const T ttR = r2 * suR * (SR - SM) - bn2;
+                if (fabs(ttR) < T(0.000000001)) { tvR = T(0); tbR = T(0); }
+                else {
+                    const T y = T(1) / ttR;
+                    tvR = div_by_t<T>(SM - vR[0], ttR, y);
+                    tbR = div_by_t<T>(r2 * suR * suR - bn2, ttR, y);
+                }
+                vzR1 = vR[1] - bn * bR[1] * tvR;
+                vzR2 = vR[2] - bn * bR[2] * tvR;
+                bzR1 = bR[1] * tbR; bzR2 = bR[2] * tbR;
+                sbvR = bn * SM + bzR1 * vzR1 + bzR2 * vzR2;
+            }
             if (j == 1 || j == 6) {
 #pragma unroll
                 for (int ik = 0; ik < 8; ik++) qq[ik] = (R ? fr(ik) : fl(ik)) - wv * (R ? ur(ik) : ul(ik));
@@ -377,6 +401,7 @@ __device__ __forceinline__ void chlld(int n_state, int n_disc, int flags, T al, 
             } else {
                 const T e = R ? e2 : e1, sm = R ? suRm : suLm, pt = R ? ptR : ptL, v0 = R ? vR[0] : vL[0];
                 const T sbv0 = R ? sbv2 : sbv1, sbvs = R ? sbvR : sbvL, S = R ? SR : SL, rz = R ? rzR : rzL;
+                const T ptz = star_pt();
                 const T ez = e * sm + (ptz * SM - pt * v0 + bn * (sbv0 - sbvs)) / (S - SM);
                 const T vz1 = R ? vzR1 : vzL1, vz2 = R ? vzR2 : vzL2, bz1 = R ? bzR1 : bzL1, bz2 = R ? bzR2 : bzL2;
                 const T UZs[8] = {rz, SM * rz, vz1 * rz, vz2 * rz, ez, bn, bz1, bz2};

@@ -138,6 +138,36 @@ __device__ __forceinline__ Zone1Cell<Z> load_zone1(const double *z) {
     return c;
 }
 
+// The volume terms of one face of a cell (Cell::Culc_volume method 1: the two tetrahedra of the
+// face's triangles with the cell centre; GCL: the face's xbar.n A, n out of this cell).  t[0..1]
+// (REF) or t[0] (GCL) receive them; the cell's volume is their sum in face order (then / 3 for GCL).
+__device__ __forceinline__ void volume_terms(const double *X, const double cc[3], const uint16_t *fnodes,
+                                             uint16_t lfj, int gcl, double t[2]) {
+    const uint16_t *fn = fnodes + 4 * lfj;
+    const double *x0 = X + 3 * fn[0], *x1 = X + 3 * fn[1], *x2 = X + 3 * fn[2], *x3 = X + 3 * fn[3];
+    if (!gcl) {
+        t[0] = tet_volume(cc, x0, x1, x2);
+        t[1] = tet_volume(cc, x0, x3, x2);
+    } else {
+        const double ax = x2[0] - x0[0], ay = x2[1] - x0[1], az = x2[2] - x0[2];
+        const double bx = x3[0] - x1[0], by = x3[1] - x1[1], bz = x3[2] - x1[2];
+        const double s1 = 0.5 * (ay * bz - az * by);
+        const double s2 = 0.5 * (az * bx - ax * bz);
+        const double s3 = 0.5 * (ax * by - ay * bx);
+        const double A = sqrt_n(s1 * s1 + s2 * s2 + s3 * s3);
+        double n1 = s1 / A, n2 = s2 / A, n3 = s3 / A;
+        double xc = 0.0, yc = 0.0, zc = 0.0;
+        xc += x0[0]; yc += x0[1]; zc += x0[2];
+        xc += x1[0]; yc += x1[1]; zc += x1[2];
+        xc += x2[0]; yc += x2[1]; zc += x2[2];
+        xc += x3[0]; yc += x3[1]; zc += x3[2];
+        xc /= 4; yc /= 4; zc /= 4;
+        if (spf(n1, n2, n3, cc[0] - xc, cc[1] - yc, cc[2] - zc) > 0.0) { n1 = -n1; n2 = -n2; n3 = -n3; }
+        t[0] = spf(xc, yc, zc, n1, n2, n3) * A;
+        t[1] = 0.0;
+    }
+}
+
 // Volume of an own cell at one layer from LDS nodes (Cell::Culc_volume method 1 or GCL).
 __device__ __forceinline__ double tile_volume(const double *X, const double cc[3], const uint16_t *fnodes,
                                               const uint16_t *lf, uint32_t code, int gcl) {
@@ -541,6 +571,11 @@ __global__ void __launch_bounds__(kTileThreads, 2)
         lf_pre[0] = lfw[0]; lf_pre[1] = lfw[1]; lf_pre[2] = lfw[2];
         load_state(prim1, c0 + tid, s4);
     }
+    uint32_t hw1 = 0u, hw2 = 0u;  // the volume helpers' face slots (faces 3..5 of cell tid - half)
+    if (!SUR && moving && tid >= kTileThreads / 2 && tid - kTileThreads / 2 < ncl) {
+        const uint32_t *lfw = reinterpret_cast<const uint32_t *>(TM.cell_lfaces + 6 * (c0 + tid - kTileThreads / 2));
+        hw1 = lfw[1]; hw2 = lfw[2];
+    }
     if constexpr (sizeof(RT) == sizeof(double)) {
         __syncthreads();
     } else if (__syncthreads_or(pending)) {
@@ -561,6 +596,45 @@ __global__ void __launch_bounds__(kTileThreads, 2)
         surrogate_update<RT>(TM, M, P, L, c0, ncl, prim1, prim2, sur, dt, gcl, moving, S);
         block_min_to(cand, &S->dtmin_bits[step & 1]);
         return;
+    }
+
+    // phase 4a (moving mesh): the new-layer volume of each own cell is summed over its faces in
+    // order; the terms of faces 3..5 come from the upper half of the workgroup, otherwise idle in
+    // this phase, through LDS (the Zone_1 frames' space, dead after the face phase), while the
+    // cell's own thread forms those of faces 0..2.  The sum is taken in the same order, so the
+    // volume is the same bits as one thread's.
+    const int half = kTileThreads / 2;
+    const bool split = moving && ncl <= half;
+    double v2part = 0.0;
+    if (split) {
+        double *vt = L.z1;  // [cell][6]
+        if (tid >= half && tid - half < ncl) {
+            const int c = tid - half;
+            const uint16_t lfh[3] = {(uint16_t)(hw1 >> 16), (uint16_t)(hw2 & 0xffffu), (uint16_t)(hw2 >> 16)};
+            const double cB[3] = {L.c2[3 * c], L.c2[3 * c + 1], L.c2[3 * c + 2]};
+#pragma unroll
+            for (int j = 0; j < 3; j++) volume_terms(L.x2, cB, L.fnodes, lfh[j], gcl, vt + 6 * c + 2 * j);
+        }
+        if (tid < ncl) {
+            const double cB[3] = {L.c2[3 * tid], L.c2[3 * tid + 1], L.c2[3 * tid + 2]};
+            uint16_t lf3[3] = {(uint16_t)(lf_pre[0] & 0xffffu), (uint16_t)(lf_pre[0] >> 16), (uint16_t)(lf_pre[1] & 0xffffu)};
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                double t[2];
+                volume_terms(L.x2, cB, L.fnodes, lf3[j], gcl, t);
+                v2part += t[0];
+                if (!gcl) v2part += t[1];
+            }
+        }
+        __syncthreads();
+        if (tid < ncl) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                v2part += vt[6 * tid + 2 * j];
+                if (!gcl) v2part += vt[6 * tid + 2 * j + 1];
+            }
+            if (gcl) v2part = v2part / 3.0;
+        }
     }
 
     // phase 4: own cells -- volumes, flux sum in reference order, update
@@ -590,7 +664,9 @@ __global__ void __launch_bounds__(kTileThreads, 2)
             Volume = tile_volume(L.x1, cA, L.fnodes, lf, code, gcl);
         }
         double Volume2 = Volume;
-        if (moving) {
+        if (split) {
+            Volume2 = v2part;
+        } else if (moving) {
             const double cB[3] = {L.c2[3 * c], L.c2[3 * c + 1], L.c2[3 * c + 2]};
             Volume2 = tile_volume(L.x2, cB, L.fnodes, lf, code, gcl);
         }
