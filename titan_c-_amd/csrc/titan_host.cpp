@@ -348,8 +348,9 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
             // 256 bytes stay free for the kernel's static LDS (the dt reduction's wavefront minima)
             const size_t budget = pass == 0 ? (size_t)lds_limit / 2 - 256 : (size_t)lds_limit - 256;
             for (int tile_cells = tile_start; tile_cells >= 16; tile_cells -= 16) {
-                if (!build_tiles(nc, nf, nn, tile_cells, face_cap, cell_nodes.data(), cell_faces.data(), face_nodes.data(),
-                                 face_cells.data(), face_code.data(), face_bidx.data(), face_ref.data(), td))
+                if (!build_tiles(nc, nf, nn, tile_cells, face_cap, cell_nodes.data(), cell_faces.data(),
+                                 face_nodes.data(), face_cells.data(), face_code.data(), cell_code.data(),
+                                 face_bidx.data(), face_ref.data(), td))
                     continue;
                 TileMesh probe{};
                 probe.tile_cells = td.tile_cells;
@@ -433,7 +434,8 @@ extern "C" int titan_destroy(titan_ctx *c) {
     void *ptrs[] = {c->d_cell_nodes, c->d_cell_faces, c->d_face_nodes, c->d_face_cells, c->d_face_bidx,
                     c->d_face_ref, c->d_cell_ref, c->d_cell_code, c->d_face_code, c->d_bstate, c->d_disp,
                     c->d_X[0], c->d_X[1], c->d_prim[0], c->d_prim[1], c->d_geom[0], c->d_geom[1], c->d_flux,
-                    c->d_src, c->d_neutrals, c->d_hfluid, c->d_S, c->d_S2, c->d_t_cell_off, c->d_t_node_off, c->d_t_nodes, c->d_t_own,
+                    c->d_src, c->d_neutrals, c->d_hfluid, c->d_S, c->d_S2, c->d_t_cell_off, c->d_t_node_off,
+                    c->d_t_nodes, c->d_t_own,
                     c->d_t_face_off, c->d_t_face_ref, c->d_t_faces, c->d_t_cell_lnodes, c->d_t_cell_lfaces,
                     c->d_vol, c->d_face_tvd, c->d_t_face_tvd, c->d_mk};
     for (void *q : ptrs)

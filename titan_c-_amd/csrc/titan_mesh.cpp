@@ -257,9 +257,45 @@ void build_order(const MeshView &m, DeviceOrder &o) {
         if (o.node_r2d[v] < 0) { o.node_r2d[v] = (int32_t)next; o.node_d2r[next] = (int32_t)v; next++; }
 }
 
+namespace {
+
+// Which of the hexahedron's three face directions face f is, seen from its cells[0] a: 0 for the
+// local faces (0,3,2,1)/(4,5,6,7), 1 for (0,1,5,4)/(2,3,7,6), 2 for (1,2,6,5)/(0,4,7,3) (the
+// local face table of Setka::New_connect); 3 if its nodes match none.
+int face_direction(const int32_t *cell_nodes, const int32_t *face_nodes, int64_t f, int64_t a) {
+    static const int lf[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4}, {2, 3, 7, 6}, {1, 2, 6, 5}, {0, 4, 7, 3}};
+    int32_t fv[4];
+    for (int i = 0; i < 4; i++) fv[i] = face_nodes[4 * f + i];
+    std::sort(fv, fv + 4);
+    for (int j = 0; j < 6; j++) {
+        int32_t cv[4];
+        for (int i = 0; i < 4; i++) cv[i] = cell_nodes[8 * a + lf[j][i]];
+        std::sort(cv, cv + 4);
+        if (std::equal(fv, fv + 4, cv)) return j / 2;
+    }
+    return 3;
+}
+
+}  // namespace
+
 bool build_tiles(int64_t nc, int64_t nf, int64_t nn, int tile_cells, int face_cap, const int32_t *cell_nodes,
                  const int32_t *cell_faces, const int32_t *face_nodes, const int32_t *face_cells,
-                 const uint32_t *face_code, const int32_t *face_bidx, const int32_t *face_ref, TileData &t) {
+                 const uint32_t *face_code, const uint32_t *cell_code, const int32_t *face_bidx,
+                 const int32_t *face_ref, TileData &t) {
+    // Face slots of a tile are ordered by the branches their solve takes -- boundary or interior,
+    // Riemann method, Zone_1 rescaling or not, face direction -- so that a wavefront's faces
+    // mostly take the same path (the direction separates, e.g., the radial faces of a supersonic
+    // wind, all upwind, from the lateral ones, which fall inside the wave fan).  The flux sum of
+    // a cell reads its faces through cell_lfaces in reference order, so the order of the slots
+    // changes no result.
+    std::vector<uint32_t> fkey(nf);
+    for (int64_t f = 0; f < nf; f++) {
+        const int32_t a = face_cells[2 * f], b = face_cells[2 * f + 1];
+        const uint32_t bc = face_code[f] & 15u, method = (face_code[f] >> 4) & 15u;
+        const bool z1 = ((cell_code[a] >> 8) & 15u) == 1u || (b >= 0 && ((cell_code[b] >> 8) & 15u) == 1u);
+        fkey[f] = ((bc != 0u ? 1u : 0u) << 8) | (method << 4) | ((z1 ? 1u : 0u) << 2) |
+                  (uint32_t)face_direction(cell_nodes, face_nodes, f, a);
+    }
     t = TileData();
     t.tile_cells = tile_cells;
     std::vector<int32_t> nstamp(nn, -1), nlocal(nn, 0), fstamp(nf, -1), flocal(nf, 0);
@@ -339,7 +375,9 @@ bool build_tiles(int64_t nc, int64_t nf, int64_t nn, int tile_cells, int face_ca
                 const int32_t f = cell_faces[6 * c + j];
                 if (fstamp[f] != ti) { fstamp[f] = (int32_t)ti; lst.push_back(f); }
             }
-        std::sort(lst.begin(), lst.end());
+        std::sort(lst.begin(), lst.end(), [&](int32_t x, int32_t y) {
+            return fkey[x] != fkey[y] ? fkey[x] < fkey[y] : x < y;
+        });
         for (size_t i = 0; i < lst.size(); i++) flocal[lst[i]] = (int32_t)i;
         t.face_off[ti + 1] = t.face_off[ti] + (int32_t)lst.size();
         t.max_faces = std::max<int32_t>(t.max_faces, (int32_t)lst.size());

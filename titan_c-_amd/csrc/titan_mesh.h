@@ -37,7 +37,8 @@ struct TileData {
 // tile's face has a node outside the tile's cells.
 bool build_tiles(int64_t nc, int64_t nf, int64_t nn, int tile_cells, int face_cap, const int32_t *cell_nodes,
                  const int32_t *cell_faces, const int32_t *face_nodes, const int32_t *face_cells,
-                 const uint32_t *face_code, const int32_t *face_bidx, const int32_t *face_ref, TileData &t);
+                 const uint32_t *face_code, const uint32_t *cell_code, const int32_t *face_bidx,
+                 const int32_t *face_ref, TileData &t);
 
 // Second neighbours of every face (Setka::Init_TVD, Setka_physics.cpp:648-744) from the geometry of
 // the initial nodes, all in device numbering: tvd[2f] = AA behind cells[0], tvd[2f+1] = BB behind
