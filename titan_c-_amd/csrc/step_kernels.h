@@ -423,6 +423,31 @@ __device__ __forceinline__ void block_min_to(double v, unsigned long long *slot)
     }
 }
 
+// block_min_to, and the sum over the block of a per-thread count added to *count_slot with one
+// atomic per block (none when the sum is 0).
+__device__ __forceinline__ void block_min_count_to(double v, unsigned long long cnt, unsigned long long *slot,
+                                                   unsigned long long *count_slot) {
+    __shared__ double wmin[16];
+    __shared__ unsigned long long wcnt[16];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = (o < v) ? o : v;
+        cnt += __shfl_xor(cnt, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) { wmin[wid] = v; wcnt[wid] = cnt; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wmin[0];
+        unsigned long long c = wcnt[0];
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 1; i < nw; i++) { m = (wmin[i] < m) ? wmin[i] : m; c += wcnt[i]; }
+        atomicMin(slot, (unsigned long long)__double_as_longlong(m));
+        if (c) atomicAdd(count_slot, c);
+    }
+}
+
 // ---- face flux: Setka::Culc_Gran_Potok for every face -------------------------------------------
 // flux[f*9 + 0..7] = chlld flux * area, flux[f*9 + 8] = PdivB.  T = double: parity arithmetic;
 // T = float: chlld_mixed (FP32 wave structure, FP64 upwind flux), states, geometry, area scaling

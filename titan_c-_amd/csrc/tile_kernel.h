@@ -359,9 +359,11 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
 }
 
 // Phase 4 of the surrogate step.  Lane l < 32 of wavefront w owns cell 32w + l of the tile (tile
-// cells <= 32 * waves); lanes l and l + 32 run the network pass of that cell together.  The
-// network's operands are read from global memory (23 KB, cache-resident across the tiles), so
-// the phase needs no barrier and each wavefront proceeds on its own.
+// cells <= 32 * waves); lanes l and l + 32 form that cell's new-layer volume terms together (faces
+// 0..2 and 3..5, summed by lane l in face order) and run its network pass together.  The network's
+// operands are read from global memory (23 KB, cache-resident across the tiles): staging them in
+// LDS once per workgroup measured slower, its copy and two barriers costing more than the cache
+// reads they save.  The phase needs no barrier, and each wavefront proceeds on its own.
 template <typename RT>
 __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMesh &M, const DevParams &P,
                                                  const TileLds &L, int64_t c0, int ncl,
@@ -369,10 +371,10 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
                                                  const SurArgs &sur, double dt, int gcl, bool moving,
                                                  StepScalars *S) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (32 * wave >= ncl) return;  // wavefront-uniform: no cell of this wavefront
     const int cl = 32 * wave + (lane & 31);
-    const bool mine = lane < 32 && cl < ncl;
-    const int64_t cg = c0 + cl;
+    const bool cell = cl < ncl;  // this lane pair's cell exists
+    const bool mine = lane < 32 && cell;
+    const int64_t cg = c0 + (cell ? cl : 0);
     double st[8] = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0}, hn[5] = {1.0, 1.0, 0.0, 0.0, 0.0};
     uint32_t code = 0;
     uint16_t lf[6] = {0, 0, 0, 0, 0, 0};
@@ -380,47 +382,69 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
 #pragma unroll
     for (int e = 0; e < 8; e++) xr[e] = 0.0f;
     bool inbox = false;
+    if (cell) {
+#pragma unroll
+        for (int j = 0; j < 6; j++) lf[j] = TM.cell_lfaces[6 * cg + j];
+    }
     if (mine) {
         load_state(prim1, cg, st);
 #pragma unroll
         for (int k = 0; k < 5; k++) hn[k] = sur.neutrals[5 * cg + k];
         code = M.cell_code[cg];
-#pragma unroll
-        for (int j = 0; j < 6; j++) lf[j] = TM.cell_lfaces[6 * cg + j];
         mlp_features(st, hn, xr);
         inbox = mlp_in_box(xr, sur.norm);
     }
+    // flux sum and volumes, while the flux slots still hold the fluxes
+    double Pf[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) Pf[i] = 0.0;
+    double Volume = 0.0, Volume2 = 0.0;
+    if (mine) {
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            const bool pos = (code >> j) & 1u;
+            const double *fl = L.flux + 9 * lf[j];
+#pragma unroll
+            for (int i = 0; i < 9; i++) {
+                const double v = fl[i];
+                Pf[i] += pos ? v : -v;
+            }
+        }
+        if (TM.vol_cached) {
+            Volume = TM.vol[cg];
+        } else {
+            const double cA[3] = {L.c1[3 * cl], L.c1[3 * cl + 1], L.c1[3 * cl + 2]};
+            Volume = tile_volume(L.x1, cA, L.fnodes, lf, code, gcl);
+        }
+        Volume2 = Volume;
+    }
+    if (moving) {
+        double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (cell) {
+            const double cB[3] = {L.c2[3 * cl], L.c2[3 * cl + 1], L.c2[3 * cl + 2]};
+            const int j0 = lane < 32 ? 0 : 3;
+#pragma unroll
+            for (int j = 0; j < 3; j++) volume_terms(L.x2, cB, L.fnodes, lf[j0 + j], gcl, t + 2 * j);
+        }
+        double u[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) u[k] = __shfl(t[k], (lane & 31) + 32, 64);
+        if (mine) {
+            double V = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) { V += t[2 * j]; if (!gcl) V += t[2 * j + 1]; }
+#pragma unroll
+            for (int j = 0; j < 3; j++) { V += u[2 * j]; if (!gcl) V += u[2 * j + 1]; }
+            Volume2 = gcl ? V / 3.0 : V;
+        }
+    }
+    if (32 * wave >= ncl) return;  // wavefront-uniform: no cell of this wavefront
     bf16x8 in;
 #pragma unroll
     for (int e = 0; e < 8; e++) in[e] = (lane < 32) ? mlp_normalise(xr[e], sur.norm, e) : (__bf16)0.0f;
     const f32x16 o = mlp_pass(reinterpret_cast<const bf16x8 *>(sur.wpack), sur.bpack, in, lane);
     const float y4 = __shfl(o[0], (lane & 31) + 32, 64);  // output 4 of this cell sits in lane l + 32
     if (!mine) return;
-    double Pf[9];
-#pragma unroll
-    for (int i = 0; i < 9; i++) Pf[i] = 0.0;
-#pragma unroll
-    for (int j = 0; j < 6; j++) {
-        const bool pos = (code >> j) & 1u;
-        const double *fl = L.flux + 9 * lf[j];
-#pragma unroll
-        for (int i = 0; i < 9; i++) {
-            const double v = fl[i];
-            Pf[i] += pos ? v : -v;
-        }
-    }
-    double Volume;
-    if (TM.vol_cached) {
-        Volume = TM.vol[cg];
-    } else {
-        const double cA[3] = {L.c1[3 * cl], L.c1[3 * cl + 1], L.c1[3 * cl + 2]};
-        Volume = tile_volume(L.x1, cA, L.fnodes, lf, code, gcl);
-    }
-    double Volume2 = Volume;
-    if (moving) {
-        const double cB[3] = {L.c2[3 * cl], L.c2[3 * cl + 1], L.c2[3 * cl + 2]};
-        Volume2 = tile_volume(L.x2, cB, L.fnodes, lf, code, gcl);
-    }
     if (moving || !TM.vol_cached) TM.vol[cg] = Volume2;
     if (!cell_in_area(code, P.area)) {  // frozen: the state carries over to the new layer
         double4 *o = reinterpret_cast<double4 *>(prim2 + 8 * cg);
@@ -576,6 +600,7 @@ __global__ void __launch_bounds__(kTileThreads, 2)
         const uint32_t *lfw = reinterpret_cast<const uint32_t *>(TM.cell_lfaces + 6 * (c0 + tid - kTileThreads / 2));
         hw1 = lfw[1]; hw2 = lfw[2];
     }
+    unsigned long long n_resolved = 0;  // this thread's faces re-solved in FP64 (counted per block)
     if constexpr (sizeof(RT) == sizeof(double)) {
         __syncthreads();
     } else if (__syncthreads_or(pending)) {
@@ -586,7 +611,7 @@ __global__ void __launch_bounds__(kTileThreads, 2)
                 double sA[8], sB[8];
                 face_fetch(rec, prim1, sA, sB);
                 face_slot<double, TVD>(f, rec, sA, sB, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
-                atomicAdd(&S->n_fp64_faces, 1ull);
+                n_resolved++;
             }
         }
         __syncthreads();
@@ -594,7 +619,7 @@ __global__ void __launch_bounds__(kTileThreads, 2)
 
     if constexpr (SUR) {
         surrogate_update<RT>(TM, M, P, L, c0, ncl, prim1, prim2, sur, dt, gcl, moving, S);
-        block_min_to(cand, &S->dtmin_bits[step & 1]);
+        block_min_count_to(cand, n_resolved, &S->dtmin_bits[step & 1], &S->n_fp64_faces);
         return;
     }
 
@@ -692,7 +717,7 @@ __global__ void __launch_bounds__(kTileThreads, 2)
         }
         cell_update_state(M, code, cg, s, Volume, Volume2, Pf, src ? src + 5 * cg : nullptr, dt, S, prim2);
     }
-    block_min_to(cand, &S->dtmin_bits[step & 1]);
+    block_min_count_to(cand, n_resolved, &S->dtmin_bits[step & 1], &S->n_fp64_faces);
 }
 
 // Launch of one k_step_tile instantiation; opens the kernel up to the tile's dynamic LDS size
