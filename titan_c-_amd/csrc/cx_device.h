@@ -14,7 +14,10 @@ namespace titan {
 
 // sigma_LS(E) = (4.15 - 0.531 ln E)^2 (1 - exp(-67.3/E))^4.5 1e-16 cm^2, E = m_p g^2/2 in keV,
 // times n_char L_char (0.06 cm^-3 x 4.2113 AU) so that rates are per unit code time.
+// Without contraction in every unit that includes it (the surrogate's closed-form estimate calls
+// it from units built with contraction on).
 __device__ __forceinline__ double sigma_ls(double g) {
+#pragma clang fp contract(off)
     const double e_per_g2 = 0.5 * 1.67262192369e-24 * (1.03804e6 * 1.03804e6) / 1.602176634e-9;
     const double nl = 0.06 * 4.2113 * 1.495978707e13;
     double E = e_per_g2 * g * g;

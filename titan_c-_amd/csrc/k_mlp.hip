@@ -88,16 +88,15 @@ __global__ void __launch_bounds__(256) k_mlp_cells(int64_t nc, const double *__r
     if (base >= nc) return;
     // this lane's own cell (pass lane/32) -- its raw inputs are kept for the output stage
     const int64_t mycell = base + lane;
-    double np = 1.0, nh = 0.0, vp[3] = {0, 0, 0};
+    double st[8] = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0}, hn[5] = {0.0, 1.0, 0.0, 0.0, 0.0};
     bool inrange = false;
     float xr[8];
     if (mycell < nc) {
-        const double *s = prim + 8 * mycell;
-        const double *hn = neutrals + 5 * mycell;
-        np = s[0];
-        nh = hn[0];
-        vp[0] = s[1]; vp[1] = s[2]; vp[2] = s[3];
-        mlp_features(s, hn, xr);
+#pragma unroll
+        for (int k = 0; k < 8; k++) st[k] = prim[8 * mycell + k];
+#pragma unroll
+        for (int k = 0; k < 5; k++) hn[k] = neutrals[5 * mycell + k];
+        mlp_features(st, hn, xr);
         inrange = mlp_in_box(xr, sN);
     } else {
 #pragma unroll
@@ -123,7 +122,7 @@ __global__ void __launch_bounds__(256) k_mlp_cells(int64_t nc, const double *__r
 #pragma unroll
     for (int k = 0; k < 5; k++) y[k] = sOut[wave][lane][k];
     double o[5];
-    mlp_source(y, sN, np, nh, vp, o);
+    mlp_source(y, sN, st, hn, o);
 #pragma unroll
     for (int k = 0; k < 5; k++) src[5 * mycell + k] = o[k];
     if (!inrange) {

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include "mlp.h"
+#include "cx_device.h"
 
 namespace titan {
 
@@ -119,19 +120,63 @@ __device__ __forceinline__ __bf16 mlp_normalise(float v, const float *norm, int 
     return (__bf16)((v - norm[kNormMean + e]) * __builtin_amdgcn_rcpf(norm[kNormStd + e]));
 }
 
-// Source of one cell from the network's 5 outputs y (per n_p n_H, normalised): rate, S_m, and
-// the plasma-frame energy source; S_E(lab) = S_E' + V_p . S_m.
-__device__ __forceinline__ void mlp_source(const float y[5], const float *norm, double n_p, double n_h,
-                                           const double vp[3], double o[5]) {
-    const double nn = n_p * n_h;
+// Closed-form estimates, per n_p n_H, of the five sources the network corrects (the scaled residual
+// form): over the relative-velocity Maxwellian of drift U and width c (c^2 = c_p^2 + c_H^2), the
+// sigma = const moments <g> and <g^3> and the momentum moment U u_m (u_m^2 = U^2 + 64 c^2/(9 pi)),
+// each times sigma at its speed (<g>, u_m, <g^3>^(1/3)), estimate J0, J1, J2; they give the rate
+// B0, the momentum source -B1 e and the plasma-frame energy source (U B1 (1 + kappa) - kappa B2)/2
+// (kappa = (c_p^2 - c_H^2)/c^2).  b[k] = the estimate of output k, m[k] > 0 its scale: the
+// network's output k is (source_k / (n_p n_H) - b[k]) / m[k].  The same formulas make the training
+// labels (surrogate.py).  FP64 without contraction, so the standalone and the fused kernels (built
+// with different contraction and FP32 division flags) form the same bits.
+__device__ __forceinline__ void mlp_baseline(double U, double c, double kappa, const double e[3], double b[5],
+                                             double m[5]) {
+#pragma clang fp contract(off)
+    const double rpi = 0.56418958354775628695;  // 1/sqrt(pi)
+    const double w = U / c;
+    double g1, g3;
+    if (w < 1e-3) {  // the series of the closed forms below at small drift
+        g1 = c * (2.0 * rpi) * (1.0 + w * w / 3.0);
+        g3 = c * c * c * (4.0 * rpi) * (1.0 + w * w);
+    } else {
+        const double ew = exp(-w * w) * rpi, erw = erf(w);
+        g1 = c * (ew + (w + 0.5 / w) * erw);
+        g3 = c * c * c * ((w * w + 2.5) * ew + (w * w * w + 3.0 * w + 0.75 / w) * erw);
+    }
+    const double um = sqrt(U * U + (64.0 / (9.0 * 3.14159265358979323846)) * c * c);
+    const double B0 = sigma_ls(g1) * g1;
+    const double B1 = sigma_ls(um) * U * um;
+    const double B2 = sigma_ls(cbrt(g3)) * g3;
+    b[0] = B0; m[0] = B0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) { b[1 + k] = -B1 * e[k]; m[1 + k] = B1; }
+    b[4] = 0.5 * (U * B1 * (1.0 + kappa) - kappa * B2);
+    m[4] = 0.5 * (U * B1 * (1.0 + kappa) + (fabs(kappa) + 0.01) * B2);
+#pragma unroll
+    for (int k = 0; k < 5; k++) m[k] = m[k] > 1e-30 ? m[k] : 1e-30;
+}
+
+// Source of one cell from the network's 5 outputs y (normalised): rate, S_m, and the plasma-frame
+// energy source per n_p n_H, each the closed-form estimate plus the output times its scale;
+// S_E(lab) = S_E' + V_p . S_m.  s = (rho, V, p, ...), hn = (n_H, c_H^2, V_H).
+__device__ __forceinline__ void mlp_source(const float y[5], const float *norm, const double *s, const double *hn,
+                                           double o[5]) {
+#pragma clang fp contract(off)
+    const double nn = s[0] * hn[0];
+    const double d0 = s[1] - hn[2], d1 = s[2] - hn[3], d2 = s[3] - hn[4];
+    const double U = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+    const double cp2 = s[4] / s[0], c2 = cp2 + hn[1];
+    const double e[3] = {U > 0.0 ? d0 / U : 1.0, U > 0.0 ? d1 / U : 0.0, U > 0.0 ? d2 / U : 0.0};
+    double b[5], m[5];
+    mlp_baseline(U, sqrt(c2), (cp2 - hn[1]) / c2, e, b, m);
     double yy[5];
 #pragma unroll
-    for (int k = 0; k < 5; k++) yy[k] = (double)y[k] * (double)norm[kNormScale + k];
+    for (int k = 0; k < 5; k++) yy[k] = __builtin_fma((double)y[k] * (double)norm[kNormScale + k], m[k], b[k]);
     o[0] = nn * yy[0];
     o[1] = nn * yy[1];
     o[2] = nn * yy[2];
     o[3] = nn * yy[3];
-    o[4] = __builtin_fma(vp[2], o[3], __builtin_fma(vp[1], o[2], __builtin_fma(vp[0], o[1], nn * yy[4])));
+    o[4] = __builtin_fma(s[3], o[3], __builtin_fma(s[2], o[2], __builtin_fma(s[1], o[1], nn * yy[4])));
 }
 
 }  // namespace titan

@@ -452,9 +452,8 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
         o[1] = make_double4(st[4], st[5], st[6], st[7]);
     } else if (inbox) {
         const float y[5] = {o[0], o[1], o[2], o[3], y4};
-        const double vp[3] = {st[1], st[2], st[3]};
         double s5[5];
-        mlp_source(y, sur.norm, st[0], hn[0], vp, s5);
+        mlp_source(y, sur.norm, st, hn, s5);
         cell_update_state(M, code, cg, st, Volume, Volume2, Pf, s5, dt, S, prim2);
     } else {
         const unsigned long long k = atomicAdd(&S->n_fallback, 1ull);
