@@ -136,11 +136,21 @@ def cell_radius_au(mesh):
     return np.linalg.norm(mesh.node_xyz[mesh.cell_nodes].mean(axis=1), axis=1) / AU
 
 
+def ulp_perturbed(prim, seed=7):
+    """The state with every value moved by one unit in the last place, up or down (seeded)."""
+    rng = np.random.default_rng(seed)
+    return prim * (1.0 + rng.choice([-1.0, 1.0], prim.shape) * 2.0 ** -52)
+
+
 def precision_report(T, mesh, prim, params, steps):
+    """FP32-mixed vs FP64 from the same input; beside it, FP64 from the input moved by one ulp vs
+    FP64: how far the dynamics carry a last-bit difference in the same number of steps, the floor
+    below which no arithmetic other than the FP64 step's own can agree per cell."""
     res = {}
-    for name, prec in (("fp64", T.PREC_FP64), ("mixed", T.PREC_MIXED)):
+    for name, prec, p0 in (("fp64", T.PREC_FP64, prim), ("mixed", T.PREC_MIXED, prim),
+                           ("fp64_ulp", T.PREC_FP64, ulp_perturbed(prim))):
         with T.Setka(mesh, dict(params, precision=prec)) as s:
-            s.set_state(prim, 0)
+            s.set_state(p0, 0)
             t0 = time.perf_counter()
             st = s.Go(steps)
             res[name] = dict(state=s.get_state(), xyz=s.get_nodes(s.current_layer()), stats=st,
@@ -151,6 +161,7 @@ def precision_report(T, mesh, prim, params, steps):
            "norm": "per_cell: |d|/|ref| per cell (vectors: norm of the difference over the cell's own norm); "
                    "field: rel_l2 over the mesh and rel_linf = max|d| / max|ref| of the field"}
     out["per_cell"] = per_cell_errors(a, b, cell_radius_au(mesh))
+    out["ulp_sensitivity_per_cell"] = per_cell_errors(res["fp64_ulp"]["state"], b, cell_radius_au(mesh))
     out["field"] = {}
     for g, cols in groups.items():
         d, r = a[:, cols] - b[:, cols], b[:, cols]
@@ -176,10 +187,11 @@ def gate_report(T, C, mlp, workload, steps):
     mesh, prim, params = C.build(workload)
     nh = C.neutrals(mesh.n_cells)
     runs = {}
-    for name, prec, mode in (("exact", T.PREC_FP64, T.SRC_EXACT), ("surrogate", T.PREC_MIXED, T.SRC_MLP),
-                             ("none", T.PREC_FP64, T.SRC_NONE)):
+    for name, prec, mode, p0 in (("exact", T.PREC_FP64, T.SRC_EXACT, prim), ("surrogate", T.PREC_MIXED, T.SRC_MLP, prim),
+                                 ("none", T.PREC_FP64, T.SRC_NONE, prim),
+                                 ("none_ulp", T.PREC_FP64, T.SRC_NONE, ulp_perturbed(prim))):
         with T.Setka(mesh, dict(params, precision=prec, source_mode=mode, cx_nodes=100)) as s:
-            s.set_state(prim, 0)
+            s.set_state(p0, 0)
             s.set_neutrals(nh)
             s.mlp_set(*mlp)
             t0 = time.perf_counter()
@@ -191,6 +203,7 @@ def gate_report(T, C, mlp, workload, steps):
            "compared": "(iii) FP32-mixed + surrogate vs (ii) FP64 + exact source, per cell",
            "per_cell": per_cell_errors(runs["surrogate"]["state"], runs["exact"]["state"], r_au),
            "source_effect_per_cell": per_cell_errors(runs["none"]["state"], runs["exact"]["state"], r_au),
+           "ulp_sensitivity_per_cell": per_cell_errors(runs["none_ulp"]["state"], runs["none"]["state"], r_au),
            "fallback_cells_last_step": int(runs["surrogate"]["stats"]["n_fallback"]),
            "seconds": {k: round(v["seconds"], 2) for k, v in runs.items()}}
     pc = out["per_cell"]
