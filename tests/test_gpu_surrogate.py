@@ -62,17 +62,21 @@ def test_mlp_forward_mfma_vs_oracle(T, C, O):
                 assert np.percentile(err, 99) < 5e-3, (n, np.percentile(err, 99))
 
 
-def test_mlp_gradients_vs_oracle(T, C, O):
+@pytest.mark.parametrize("n", [3000, 140_000])
+def test_mlp_gradients_vs_oracle(T, C, O, n):
+    """Loss and gradient of the FP32 MFMA training kernel against the FP64 oracle; 140000 samples
+    cross the kernel's 131072-sample gradient pass, so two passes add into one gradient."""
     sur = importlib.import_module("titan_c-_amd.surrogate")
     rng = np.random.default_rng(2)
     P = sur.init_params(4)
-    x = rng.normal(0, 1, (3000, 8)).astype(np.float32)
-    yt = rng.normal(0, 1, (3000, 5)).astype(np.float32)
+    x = rng.normal(0, 1, (n, 8)).astype(np.float32)
+    yt = rng.normal(0, 1, (n, 5)).astype(np.float32)
     s, _, _ = _setka(T, C)
     with s:
         s.mlp_set(P, np.concatenate([np.zeros(8), np.ones(8)]), np.ones(5), np.concatenate([-np.ones(8), np.ones(8)]))
         loss, g = s.mlp_loss_grad(x, yt)
     rl, rg = O.mlp_loss_grad(P.astype(np.float64), x.astype(np.float64), yt.astype(np.float64))
+    print("n %d: loss rel err %.2e, grad max err / max %.2e" % (n, abs(loss - rl) / rl, np.abs(g - rg).max() / np.abs(rg).max()))
     assert abs(loss - rl) < 1e-5 * rl
     assert np.abs(g - rg).max() < 1e-4 * np.abs(rg).max()
 

@@ -40,6 +40,7 @@ UNITS = {
     "k_mk.hip": ["-ffp-contract=off"],
     "k_step32.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "k_mlp.hip": ["-ffp-contract=fast"],
+    "k_train.hip": ["-ffp-contract=fast"],
     "k_noalloc.hip": ["-ffp-contract=off"],
 }
 HOST_UNITS = ["titan_host.cpp", "titan_mesh.cpp", "titan_io.cpp", "titan_fit.cpp"]

@@ -1,5 +1,5 @@
-// k_mlp.hip -- charge-exchange surrogate: BF16 MFMA inference (FP32 accumulation) and FP32
-// training kernels (forward, backward, Adam).  The wavefront forward pass lives in mlp_device.h.
+// k_mlp.hip -- charge-exchange surrogate: BF16 MFMA inference (FP32 accumulation), parameter
+// state and packing.  The wavefront forward pass lives in mlp_device.h, training in k_train.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -134,121 +134,6 @@ __global__ void __launch_bounds__(256) k_mlp_cells(int64_t nc, const double *__r
 
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
-// ---- training kernels (FP32) ---------------------------------------------------------------
-// layer forward: thread per (sample, output); act: 1 = SiLU
-__global__ void k_fwd_layer(int64_t n, int nin, int nout, const float *__restrict__ W, const float *__restrict__ b,
-                            const float *__restrict__ ain, float *__restrict__ z, float *__restrict__ aout, int act) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n * nout) return;
-    const int64_t s = i / nout;
-    const int o = (int)(i % nout);
-    float acc = b[o];
-    for (int k = 0; k < nin; k++) acc += W[o * nin + k] * ain[s * nin + k];
-    z[i] = acc;
-    aout[i] = act ? acc / (1.0f + expf(-acc)) : acc;
-}
-
-// dY = 2 (Y - T) / (n*5); per-block partial loss
-__global__ void k_out_grad(int64_t n, const float *__restrict__ yp, const float *__restrict__ yt,
-                           float *__restrict__ dy, float *__restrict__ loss_part) {
-    __shared__ float red[256];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    float l = 0.0f;
-    if (i < n * 5) {
-        const float e = yp[i] - yt[i];
-        dy[i] = 2.0f * e / (float)(n * 5);
-        l = e * e;
-    }
-    red[threadIdx.x] = l;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
-}
-
-// backward through a layer's weights and the previous activation:
-// dz_prev[s][k] = (sum_o dz[s][o] W[o][k]) * silu'(z_prev[s][k])
-__global__ void k_bwd_data(int64_t n, int nin, int nout, const float *__restrict__ W, const float *__restrict__ dz,
-                           const float *__restrict__ zprev, float *__restrict__ dzprev) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n * nin) return;
-    const int64_t s = i / nin;
-    const int k = (int)(i % nin);
-    float acc = 0.0f;
-    for (int o = 0; o < nout; o++) acc += dz[s * nout + o] * W[o * nin + k];
-    const float zz = zprev[i];
-    const float sg = 1.0f / (1.0f + expf(-zz));
-    dzprev[i] = acc * (sg * (1.0f + zz * (1.0f - sg)));
-}
-
-// partial weight gradient over a chunk of 256 samples: part[chunk][o*nin + k] and bias part
-__global__ void k_wgrad_part(int64_t n, int nin, int nout, const float *__restrict__ dz, const float *__restrict__ ain,
-                             float *__restrict__ part) {
-    __shared__ float sdz[64][64];
-    __shared__ float sa[64][65];
-    const int64_t s0 = (int64_t)blockIdx.x * 256;
-    const int np = nout * nin + nout;
-    float acc[17];
-#pragma unroll
-    for (int j = 0; j < 17; j++) acc[j] = 0.0f;
-    for (int sub = 0; sub < 4; sub++) {
-        const int64_t sb = s0 + 64 * sub;
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < 64 * 64; idx += blockDim.x) {
-            const int r = idx / 64, cidx = idx % 64;
-            const int64_t s = sb + r;
-            sdz[r][cidx] = (s < n && cidx < nout) ? dz[s * nout + cidx] : 0.0f;
-            sa[r][cidx] = (s < n && cidx < nin) ? ain[s * nin + cidx] : 0.0f;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 17; j++) {
-            const int p = threadIdx.x + 256 * j;
-            if (p >= np) break;
-            float a = 0.0f;
-            if (p < nout * nin) {
-                const int o = p / nin, k = p % nin;
-                for (int r = 0; r < 64; r++) a += sdz[r][o] * sa[r][k];
-            } else {
-                const int o = p - nout * nin;
-                for (int r = 0; r < 64; r++) a += sdz[r][o];
-            }
-            acc[j] += a;
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < 17; j++) {
-        const int p = threadIdx.x + 256 * j;
-        if (p < np) part[(int64_t)blockIdx.x * np + p] = acc[j];
-    }
-}
-
-// grad[off + p] (weights) / grad[boff + o] (bias) = sum over chunks, fixed order
-__global__ void k_wgrad_reduce(int64_t chunks, int nin, int nout, const float *__restrict__ part, float *__restrict__ grad,
-                               int woff, int boff) {
-    const int np = nout * nin + nout;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    float a = 0.0f;
-    for (int64_t c = 0; c < chunks; c++) a += part[c * np + p];
-    if (p < nout * nin) grad[woff + p] = a;
-    else grad[boff + (p - nout * nin)] = a;
-}
-
-__global__ void k_adam(int np, float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
-                       float *__restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    const float gi = g[i];
-    const float mi = b1 * m[i] + (1.0f - b1) * gi;
-    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-}
-
 // ---- host side -----------------------------------------------------------------------------
 static uint16_t f2bf(float f) {
     uint32_t u;
@@ -293,8 +178,8 @@ static void pack_params(const float *P, std::vector<uint16_t> &wv, std::vector<f
 }
 
 void mlp_free(MlpState &s) {
-    void *ptrs[] = {s.d_params, s.d_grad, s.d_m, s.d_v, s.d_wpack, s.d_bpack, s.d_norm, s.d_list, s.d_count, s.d_ws,
-                    s.d_stash};
+    void *ptrs[] = {s.d_params, s.d_grad, s.d_m, s.d_v,    s.d_wpack, s.d_bpack, s.d_norm,  s.d_list,
+                    s.d_count,  s.d_stash, s.d_fw, s.d_bw, s.d_part,  s.d_lacc};
     for (void *p : ptrs)
         if (p) hipFree(p);
     s = MlpState();
@@ -316,7 +201,7 @@ static int ensure_alloc(MlpState &s) {
     return TITAN_OK;
 }
 
-static int repack(MlpState &s, hipStream_t st) {
+int mlp_repack(MlpState &s, hipStream_t st) {
     std::vector<float> P(kMlpParams);
     HIPCHK_M(hipMemcpyAsync(P.data(), s.d_params, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, st));
     HIPCHK_M(hipStreamSynchronize(st));
@@ -326,6 +211,7 @@ static int repack(MlpState &s, hipStream_t st) {
     HIPCHK_M(hipMemcpyAsync(s.d_wpack, wv.data(), wv.size() * 2, hipMemcpyHostToDevice, st));
     HIPCHK_M(hipMemcpyAsync(s.d_bpack, bv.data(), bv.size() * 4, hipMemcpyHostToDevice, st));
     HIPCHK_M(hipStreamSynchronize(st));
+    s.ready = true;
     return TITAN_OK;
 }
 
@@ -343,10 +229,7 @@ int mlp_set(MlpState &s, hipStream_t st, const float *params, const float *in_no
     HIPCHK_M(hipMemsetAsync(s.d_m, 0, sizeof(float) * kMlpParams, st));
     HIPCHK_M(hipMemsetAsync(s.d_v, 0, sizeof(float) * kMlpParams, st));
     s.adam_t = 0;
-    rc = repack(s, st);
-    if (rc != TITAN_OK) return rc;
-    s.ready = true;
-    return TITAN_OK;
+    return mlp_repack(s, st);
 }
 
 int mlp_get(MlpState &s, hipStream_t st, float *params) {
@@ -371,148 +254,6 @@ int mlp_forward_host(MlpState &s, hipStream_t st, int64_t n, const float *x, flo
     hipFree(dx);
     hipFree(dy);
     return TITAN_OK;
-}
-
-// workspace: activations of a batch (z and a for 4 layers), dz buffers, partials, loss parts
-static int ensure_ws(MlpState &s, int64_t batch) {
-    const int64_t chunks = (batch + 255) / 256;
-    const size_t need = (size_t)batch * (8 + 2 * (64 * 3 + 5) + 64 * 3 + 5 + 5) + (size_t)chunks * (64 * 64 + 64) +
-                        (size_t)((batch * 5 + 255) / 256) + 16;
-    if (s.ws_floats < need) {
-        if (s.d_ws) hipFree(s.d_ws);
-        s.d_ws = nullptr;
-        HIPCHK_M(hipMalloc((void **)&s.d_ws, need * sizeof(float)));
-        s.ws_floats = need;
-    }
-    return TITAN_OK;
-}
-
-// Forward + backward of one batch already on the device; fills s.d_grad; returns mean loss.
-static int batch_grad(MlpState &s, hipStream_t st, int64_t n, const float *dx, const float *dyt, float *loss_out) {
-    const int64_t chunks = (n + 255) / 256;
-    float *w = s.d_ws;
-    float *z[4], *a[4], *dz[4];
-    const int dims[5] = {8, 64, 64, 64, 5};
-    for (int l = 0; l < 4; l++) { z[l] = w; w += n * dims[l + 1]; a[l] = w; w += n * dims[l + 1]; }
-    for (int l = 0; l < 4; l++) { dz[l] = w; w += n * dims[l + 1]; }
-    float *part = w; w += chunks * (64 * 64 + 64);
-    float *lpart = w;
-    const int woff[4] = {oW1, oW2, oW3, oW4}, boff[4] = {oB1, oB2, oB3, oB4};
-    const float *ain = dx;
-    for (int l = 0; l < 4; l++) {
-        const int64_t tot = n * dims[l + 1];
-        hipLaunchKernelGGL(k_fwd_layer, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, n, dims[l], dims[l + 1],
-                           s.d_params + woff[l], s.d_params + boff[l], ain, z[l], a[l], l < 3 ? 1 : 0);
-        ain = a[l];
-    }
-    const unsigned lblocks = (unsigned)((n * 5 + 255) / 256);
-    hipLaunchKernelGGL(k_out_grad, dim3(lblocks), dim3(256), 0, st, n, a[3], dyt, dz[3], lpart);
-    for (int l = 3; l >= 0; l--) {
-        const float *ain_l = l == 0 ? dx : a[l - 1];
-        hipLaunchKernelGGL(k_wgrad_part, dim3((unsigned)chunks), dim3(256), 0, st, n, dims[l], dims[l + 1], dz[l], ain_l,
-                           part);
-        const int np = dims[l] * dims[l + 1] + dims[l + 1];
-        hipLaunchKernelGGL(k_wgrad_reduce, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, chunks, dims[l],
-                           dims[l + 1], part, s.d_grad, woff[l], boff[l]);
-        if (l > 0) {
-            const int64_t tot = n * dims[l];
-            hipLaunchKernelGGL(k_bwd_data, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, n, dims[l],
-                               dims[l + 1], s.d_params + woff[l], dz[l], z[l - 1], dz[l - 1]);
-        }
-    }
-    HIPCHK_M(hipGetLastError());
-    if (loss_out) {
-        std::vector<float> lp(lblocks);
-        HIPCHK_M(hipMemcpyAsync(lp.data(), lpart, sizeof(float) * lblocks, hipMemcpyDeviceToHost, st));
-        HIPCHK_M(hipStreamSynchronize(st));
-        double tot = 0.0;
-        for (float v : lp) tot += v;
-        *loss_out = (float)(tot / (double)(n * 5));
-    }
-    return TITAN_OK;
-}
-
-int mlp_loss_grad_host(MlpState &s, hipStream_t st, int64_t n, const float *x, const float *y, float *loss,
-                       float *grad) {
-    if (!s.d_params) return TITAN_E_NOT_TRAINED;
-    int rc = ensure_ws(s, n);
-    if (rc != TITAN_OK) return rc;
-    float *dx = nullptr, *dy = nullptr;
-    HIPCHK_M(hipMalloc((void **)&dx, sizeof(float) * 8 * n));
-    HIPCHK_M(hipMalloc((void **)&dy, sizeof(float) * 5 * n));
-    HIPCHK_M(hipMemcpyAsync(dx, x, sizeof(float) * 8 * n, hipMemcpyHostToDevice, st));
-    HIPCHK_M(hipMemcpyAsync(dy, y, sizeof(float) * 5 * n, hipMemcpyHostToDevice, st));
-    rc = batch_grad(s, st, n, dx, dy, loss);
-    if (rc == TITAN_OK) {
-        HIPCHK_M(hipMemcpyAsync(grad, s.d_grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, st));
-        HIPCHK_M(hipStreamSynchronize(st));
-    }
-    hipFree(dx);
-    hipFree(dy);
-    return rc;
-}
-
-// gathers a shuffled batch: xb[i] = x[perm[i]]
-__global__ void k_gather_batch(int64_t nb, const int32_t *__restrict__ perm, const float *__restrict__ x,
-                               const float *__restrict__ y, float *__restrict__ xb, float *__restrict__ yb) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nb) return;
-    const int64_t j = perm[i];
-#pragma unroll
-    for (int k = 0; k < 8; k++) xb[8 * i + k] = x[8 * j + k];
-#pragma unroll
-    for (int k = 0; k < 5; k++) yb[5 * i + k] = y[5 * j + k];
-}
-
-int mlp_train_host(MlpState &s, hipStream_t st, int64_t n, const float *x, const float *y, int epochs, int batch,
-                   float lr, uint64_t seed, float *loss_hist) {
-    if (!s.d_params) return TITAN_E_NOT_TRAINED;
-    if (batch > n) batch = (int)n;
-    int rc = ensure_ws(s, batch);
-    if (rc != TITAN_OK) return rc;
-    float *dx = nullptr, *dy = nullptr, *xb = nullptr, *yb = nullptr;
-    int32_t *dperm = nullptr;
-    HIPCHK_M(hipMalloc((void **)&dx, sizeof(float) * 8 * n));
-    HIPCHK_M(hipMalloc((void **)&dy, sizeof(float) * 5 * n));
-    HIPCHK_M(hipMalloc((void **)&xb, sizeof(float) * 8 * batch));
-    HIPCHK_M(hipMalloc((void **)&yb, sizeof(float) * 5 * batch));
-    HIPCHK_M(hipMalloc((void **)&dperm, sizeof(int32_t) * n));
-    HIPCHK_M(hipMemcpyAsync(dx, x, sizeof(float) * 8 * n, hipMemcpyHostToDevice, st));
-    HIPCHK_M(hipMemcpyAsync(dy, y, sizeof(float) * 5 * n, hipMemcpyHostToDevice, st));
-    std::vector<int32_t> perm(n);
-    uint64_t state = seed * 0x9E3779B97F4A7C15ull + 1;
-    auto rnd = [&]() {
-        state ^= state << 13; state ^= state >> 7; state ^= state << 17;
-        return state;
-    };
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    for (int ep = 0; ep < epochs && rc == TITAN_OK; ep++) {
-        for (int64_t i = 0; i < n; i++) perm[i] = (int32_t)i;
-        for (int64_t i = n - 1; i > 0; i--) std::swap(perm[i], perm[rnd() % (uint64_t)(i + 1)]);
-        HIPCHK_M(hipMemcpyAsync(dperm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
-        const int64_t nbatches = n / batch;
-        double lsum = 0.0;
-        for (int64_t bi = 0; bi < nbatches; bi++) {
-            hipLaunchKernelGGL(k_gather_batch, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, st, (int64_t)batch,
-                               dperm + bi * batch, dx, dy, xb, yb);
-            float l = 0.0f;
-            const bool want = loss_hist && (bi % 16 == 0);
-            rc = batch_grad(s, st, batch, xb, yb, want ? &l : nullptr);
-            if (rc != TITAN_OK) break;
-            if (want) lsum += l;
-            s.adam_t++;
-            const float c1 = 1.0f - std::pow(b1, (float)s.adam_t), c2 = 1.0f - std::pow(b2, (float)s.adam_t);
-            hipLaunchKernelGGL(k_adam, dim3((unsigned)((kMlpParams + 255) / 256)), dim3(256), 0, st, (int)kMlpParams,
-                               s.d_params, s.d_grad, s.d_m, s.d_v, lr, b1, b2, eps, c1, c2);
-        }
-        if (loss_hist) loss_hist[ep] = (float)(lsum / (double)std::max<int64_t>(1, (nbatches + 15) / 16));
-    }
-    HIPCHK_M(hipStreamSynchronize(st));
-    hipFree(dx); hipFree(dy); hipFree(xb); hipFree(yb); hipFree(dperm);
-    if (rc != TITAN_OK) return rc;
-    rc = repack(s, st);
-    if (rc == TITAN_OK) s.ready = true;
-    return rc;
 }
 
 static int ensure_list(MlpState &s, int64_t nc) {

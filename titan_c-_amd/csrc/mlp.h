@@ -36,8 +36,11 @@ struct MlpState {
     int64_t list_cap = 0;
     double *d_stash = nullptr;  // fused-step fallback stash, list_cap * 11 doubles
     unsigned long long *d_count = nullptr;
-    float *d_ws = nullptr;
-    size_t ws_floats = 0;
+    // training: FP32 MFMA operand packs of W (forward) and W^T (backward), per-wavefront gradient
+    // partials (with their squared-error sums), the loss accumulator
+    float *d_fw = nullptr, *d_bw = nullptr, *d_part = nullptr;
+    double *d_lacc = nullptr;
+    int64_t part_waves = 0;
     long long adam_t = 0;
 };
 
@@ -45,6 +48,8 @@ void mlp_free(MlpState &s);
 int mlp_set(MlpState &s, hipStream_t st, const float *params, const float *in_norm16, const float *out_scale5,
             const float *box16);
 int mlp_get(MlpState &s, hipStream_t st, float *params);
+// Rebuilds the BF16 inference packs from the FP32 parameters and marks the network ready.
+int mlp_repack(MlpState &s, hipStream_t st);
 int mlp_forward_host(MlpState &s, hipStream_t st, int64_t n, const float *x, float *y);
 int mlp_loss_grad_host(MlpState &s, hipStream_t st, int64_t n, const float *x, const float *y, float *loss,
                        float *grad);

@@ -10,8 +10,9 @@ mlp_device.h, restated here in FP64) and m_k > 0 its magnitude.  The estimates a
 per cent, so the network learns small corrections and its BF16 arithmetic costs correspondingly
 little accuracy.  Inputs are normalised by per-feature mean/std, outputs by a per-output scale.
 Labels are the exact FP64 Maxwellian charge-exchange integrals computed on the GPU
-(titan_cx_exact_batch); training runs on the GPU through the library's FP32 forward/backward/Adam
-kernels; inference runs in the BF16 MFMA kernel with the exact source as the fallback for cells
+(titan_cx_exact_batch); training runs on the GPU in the library's FP32 matrix-core kernels (forward,
+backward and weight gradient fused per mini-batch, then one reduce + Adam kernel, mini-batches drawn
+by a keyed permutation on the device); inference runs in the BF16 MFMA kernel with the exact source as the fallback for cells
 outside the training box.
 """
 import math
