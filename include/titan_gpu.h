@@ -10,7 +10,8 @@
  *     caller's order, faces in New_connect order); no pointer is retained after a call;
  *   - one context per host thread; every call is synchronous on the context's HIP stream;
  *   - the state of a cell is 8 FP64 primitives (rho, Vx, Vy, Vz, p, Bx, By, Bz), B in the
- *     reference's Gaussian code units (magnetic pressure B^2/8pi).
+ *     reference's Gaussian code units (magnetic pressure B^2/8pi), and, when the context carries
+ *     them (titan_params.passive_scalars), the passive scalars Q and rho_He (titan_set_scalars).
  */
 #ifndef TITAN_GPU_H
 #define TITAN_GPU_H
@@ -76,6 +77,9 @@ typedef struct {
     const uint8_t *cell_is_tvd;  /* [n_cells] Cell::is_TVD (0 => B frozen)                     */
     const double *node_xyz;      /* [n_nodes*3] initial coordinates (Yzel::coord[0])           */
     const double *node_disp;     /* [n_nodes*3] motion amplitude vectors D_i; nullable         */
+    const double *face_bscalar;  /* [n_faces*2] Q, rho_He of the fixed state of hard boundary faces
+                                    (Gran::parameters, read by Setka_gran_snos.cpp:51-57); required
+                                    when passive_scalars is on and a hard face exists, else nullable */
 } titan_mesh;
 
 /* Parameters the hot path reads (Phys_param.h:95-122, set_parameters.txt names in brackets). */
@@ -110,6 +114,16 @@ typedef struct {
                                              nose (Setka_move.cpp:566-622) only                */
     double velocity_TS, velocity_HP, velocity_BS; /* (0.1, 0.1, 0.3)                            */
     double sglag_TS_k_sphere, sglag_HP_k_sphere, sglag_BS_k; /* (0.01, 0.01, 0.05)              */
+    int32_t passive_scalars;     /* 1: the step carries the plasma's passive scalars Q (heliopause
+                                    marker) and rho_He (Phys_param.cpp:232-233, always present in the
+                                    reference): reconstructed as rho is (Setka_gran_snos.cpp:364-395,
+                                    715-725, 848-852), konvect fluxes of chlld (Phys_param.cpp:
+                                    1079-1105,1216-1218,1356-1418,1701-1820) on every face
+                                    (Setka_physics.cpp:2349-2361,2485-2493), advected and clipped in
+                                    the cell loop (:1781-1817,2004-2016); Q also routes the closed-form
+                                    source's zone (determ_zone, :1427-1482) and rho_He its proton split
+                                    (Plasma_components_1, Phys_param.cpp:628-665).  0: neither is
+                                    carried (read as 0 by TITAN_SRC_BERA).                          */
 } titan_params;
 
 /* Rays of node motion: Luch objects of type A_Luch / A2_Luch (Luch.h:9-25), each a straight line
@@ -155,6 +169,10 @@ int titan_last_error(titan_ctx *ctx, int64_t *index);
 /* Cell::parameters[layer] for the 8 plasma keys, [n_cells*8] cell-major. */
 int titan_set_state(titan_ctx *ctx, const double *prim, int layer);
 int titan_get_state(titan_ctx *ctx, double *prim, int layer);
+/* Cell::parameters[layer]["Q"], ["rho_He"]: qhe [n_cells*2] = (Q, rho_He) per cell.  TITAN_E_ARG when
+ * the context does not carry the passive scalars.  Both are 0 until set. */
+int titan_set_scalars(titan_ctx *ctx, const double *qhe, int layer);
+int titan_get_scalars(titan_ctx *ctx, double *qhe, int layer);
 /* Yzel::coord[layer], [n_nodes*3]. */
 int titan_set_nodes(titan_ctx *ctx, const double *xyz, int layer);
 int titan_get_nodes(titan_ctx *ctx, double *xyz, int layer);
@@ -177,10 +195,11 @@ int titan_get_geometry(titan_ctx *ctx, int layer, double *cell_center, double *c
  * Setka::Culc_Gran_Potok (Setka_physics.cpp:2280-2571) for every face, with states of layer
  * `now`, geometry of layer `now` and, when the mesh moves, face velocities from layers now and
  * 1-now with time step dt.  Outputs (nullable): face_flux [n_faces*9] = 8 fluxes * area
- * (Prho, PVx..PVz, Pp(energy), PBx..PBz) and PdivB; dt_min = min over interior faces of the
- * CFL time step; branch [n_faces] = n_state<<4 | ibn<<3 | region. */
-int titan_gran_potok(titan_ctx *ctx, int now, double dt, double *face_flux, double *dt_min,
-                     uint8_t *branch);
+ * (Prho, PVx..PVz, Pp(energy), PBx..PBz) and PdivB; face_kflux [n_faces*2] = PQ, Prho_He (only
+ * when the context carries the passive scalars, else TITAN_E_ARG); dt_min = min over interior
+ * faces of the CFL time step; branch [n_faces] = n_state<<4 | ibn<<3 | region. */
+int titan_gran_potok(titan_ctx *ctx, int now, double dt, double *face_flux, double *face_kflux,
+                     double *dt_min, uint8_t *branch);
 
 /* ---- step --------------------------------------------------------------------------------- */
 /* Setka::Go(is_inner_area, steps) (Setka_physics.cpp:1484-2278) over the whole mesh: the first
@@ -238,7 +257,8 @@ int titan_chlld_batch_noalloc(int64_t n, const int8_t *n_state, const int8_t *n_
 /* The plasma source of Setka::Calc_sourse_MF_Bera (Setka_physics.cpp:766-1258) for the states of
  * `layer`: out [n_cells*5] = mass, momentum (3), energy (lab frame) per unit volume and time.
  * mode TITAN_SRC_BERA: the reference's closed form with the fluids of titan_set_h_fluids (mass
- * source 0; states without PUI, Q, rho_He); TITAN_SRC_EXACT / TITAN_SRC_MLP: the Maxwellian
+ * source 0; states without PUI; Q and rho_He of the layer when carried, else 0); TITAN_SRC_EXACT /
+ * TITAN_SRC_MLP: the Maxwellian
  * charge-exchange integral with the neutrals of titan_set_neutrals (out[0] = exchange rate). */
 int titan_sources(titan_ctx *ctx, int layer, int mode, double *out);
 /* Standalone exact source for n states: in [n*10] = n_p, c_p^2, V_p(3), n_H, c_H^2, V_H(3). */
@@ -300,12 +320,14 @@ int titan_checkpoint_write(const char *path, int64_t n_cells, int n_keys, const 
 int titan_checkpoint_read(const char *path, int64_t n_cells, int n_keys, const char *const *keys,
                           double *values, int64_t n_nodes, double *xyz, int64_t *nodes_read);
 /* Setka::Save_cell_parameters on a context: the 8 plasma keys (rho, Vx, Vy, Vz, p, Bx, By, Bz) of
- * the newest layer, then n_extra caller keys with extra_values [n_cells][n_extra] (e.g. the
- * hydrogen fluids, Q, rho_He kept on the host), and the newest node layer. */
+ * the newest layer, and Q, rho_He when the context carries them, then n_extra caller keys with
+ * extra_values [n_cells][n_extra] (e.g. the hydrogen fluids kept on the host), and the newest node
+ * layer. */
 int titan_save_cell_parameters(titan_ctx *ctx, const char *path, int n_extra, const char *const *extra_keys,
                                const double *extra_values);
-/* Setka::Download_cell_parameters on a context: each of the 8 plasma keys a cell has in the file
- * replaces that value in both state layers (keys a cell lacks keep their values); a node block
+/* Setka::Download_cell_parameters on a context: each of the 8 plasma keys (and Q, rho_He when
+ * carried) a cell has in the file replaces that value in both state layers (keys a cell lacks keep
+ * their values); a node block
  * replaces both node layers, and the next titan_go re-measures the geometry, as Setka::Go does
  * on entry (Setka_physics.cpp:1495-1510).  The TVD neighbours stay as chosen; titan_init_tvd
  * chooses them again from the new nodes. */

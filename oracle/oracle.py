@@ -22,7 +22,8 @@ _I64 = ctypes.c_int64
 class OrMesh(ctypes.Structure):
     _fields_ = [("n_nodes", _I64), ("n_cells", _I64), ("n_faces", _I64), ("cell_nodes", _P), ("cell_faces", _P),
                 ("face_nodes", _P), ("face_cells", _P), ("face_bc", _P), ("face_method", _P), ("face_flags", _P),
-                ("face_bstate", _P), ("cell_zone", _P), ("cell_is_tvd", _P), ("face_tvd", _P), ("cell_inner", _P)]
+                ("face_bstate", _P), ("cell_zone", _P), ("cell_is_tvd", _P), ("face_tvd", _P), ("cell_inner", _P),
+                ("face_bscal", _P)]
 
 
 class OrParams(ctypes.Structure):
@@ -85,6 +86,12 @@ def lib():
             "or_mlp_forward_bf16": (None, [_P, _I64, _P, _P]),
             "or_mlp_loss_grad": (_D, [_P, _I64, _P, _P, _P]),
             "or_set_mistake": (None, [_I]),
+            "or_face_states_k": (_I, [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _P]),
+            "or_face_flux_k": (_I, [_P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P, _P]),
+            "or_update_k": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P, _P]),
+            "or_go_k": (_I, [_P, _P, _P, _P, _P, _P, _I64, _I, _I, _P, _P, _P, _I, _P, _P]),
+            "or_determ_zone": (_I, [_D, _D, _I]),
+            "or_bera_source_k": (None, [_P, _D, _D, _I, _I, _P, _D, _D, _D, _P]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -104,6 +111,9 @@ MISTAKE_ZONE2_8PI = 1 << 4
 MISTAKE_HLLD_VZZ_SIGN = 1 << 5
 MISTAKE_HLLD_BZZ_WEIGHTS = 1 << 6
 MISTAKE_HLLD_TB_BN2 = 1 << 7
+MISTAKE_SCAL_ZONE1_PLAIN = 1 << 8
+MISTAKE_SCAL_TVD_R2_KEPT = 1 << 9
+MISTAKE_HE_CLIP_RHO = 1 << 10
 
 
 class planted_mistake:
@@ -138,7 +148,9 @@ class OracleMesh:
             face_flags=None if mesh.face_flags is None else np.ascontiguousarray(mesh.face_flags, np.uint8),
             face_bstate=None if mesh.face_bstate is None else np.ascontiguousarray(mesh.face_bstate, np.float64),
             cell_zone=np.ascontiguousarray(mesh.cell_zone, np.int8), cell_is_tvd=np.ascontiguousarray(mesh.cell_is_tvd, np.uint8),
-            face_tvd=None, cell_inner=None)
+            face_tvd=None, cell_inner=None,
+            face_bscal=None if getattr(mesh, "face_bscalar", None) is None
+            else np.ascontiguousarray(mesh.face_bscalar, np.float64))
         c = OrMesh()
         c.n_nodes, c.n_cells, c.n_faces = mesh.n_nodes, mesh.n_cells, mesh.n_faces
         for k, v in self.arrs.items():
@@ -174,7 +186,9 @@ def measure(om, xyz, gcl=False):
     return out
 
 
-def face_flux(om, params, prim, geom, x1=None, x2=None, dt=1e-5):
+def face_flux(om, params, prim, geom, x1=None, x2=None, dt=1e-5, scal=None):
+    """(rc, flux [nf,9], dt_face, branch), and with the passive scalars scal [nc,2] their fluxes
+    kflux [nf,2] as a fifth item."""
     m = om.m
     g = OrGeom(*[_p(geom[k]) for k, _ in OrGeom._fields_])
     prim = np.ascontiguousarray(prim, np.float64)
@@ -183,13 +197,21 @@ def face_flux(om, params, prim, geom, x1=None, x2=None, dt=1e-5):
     br = np.empty(m.n_faces, np.uint8)
     x1 = None if x1 is None else np.ascontiguousarray(x1, np.float64)
     x2 = None if x2 is None else np.ascontiguousarray(x2, np.float64)
-    rc = lib().or_face_flux(ctypes.byref(om.c), ctypes.byref(params_c(params)), _p(prim), ctypes.byref(g), _p(x1),
-                            _p(x2), float(dt), _p(flux), _p(dtf), _p(br))
-    return rc, flux, dtf, br
+    if scal is None:
+        rc = lib().or_face_flux(ctypes.byref(om.c), ctypes.byref(params_c(params)), _p(prim), ctypes.byref(g), _p(x1),
+                                _p(x2), float(dt), _p(flux), _p(dtf), _p(br))
+        return rc, flux, dtf, br
+    sc = np.ascontiguousarray(scal, np.float64).reshape(m.n_cells, 2)
+    kflux = np.empty((m.n_faces, 2))
+    rc = lib().or_face_flux_k(ctypes.byref(om.c), ctypes.byref(params_c(params)), _p(prim), _p(sc), ctypes.byref(g),
+                              _p(x1), _p(x2), float(dt), _p(flux), _p(kflux), _p(dtf), _p(br))
+    return rc, flux, dtf, br, kflux
 
 
-def go(om, params, xyz, prim, steps, gcl=False, nthreads=1, disp=None, t_sim=0.0, h_fluids=None, fit=None):
-    """Runs or_go; returns (rc, prim_final, xyz_final, dt_hist, t_sim).  h_fluids [n_cells, n_h, 5]
+def go(om, params, xyz, prim, steps, gcl=False, nthreads=1, disp=None, t_sim=0.0, h_fluids=None, fit=None,
+       scal=None):
+    """Runs or_go; returns (rc, prim_final, xyz_final, dt_hist, t_sim), and with the passive scalars
+    scal [n_cells, 2] (Q, rho_He) their final values as a sixth item.  h_fluids [n_cells, n_h, 5]
     (rho_H, p_H, V_H per hydrogen fluid) adds the Calc_sourse_MF_Bera sources to every step; fit
     (OracleFit) with move_mesh 2 moves the nodes by the fitted surfaces and their rays."""
     m = om.m
@@ -197,15 +219,20 @@ def go(om, params, xyz, prim, steps, gcl=False, nthreads=1, disp=None, t_sim=0.0
     coords[0] = xyz
     state = np.zeros((2, m.n_cells, 8))
     state[0] = prim
+    ks = None
+    if scal is not None:
+        ks = np.zeros((2, m.n_cells, 2))
+        ks[0] = scal
     dth = np.zeros(max(1, steps))
     layer = ctypes.c_int(0)
     ts = ctypes.c_double(t_sim)
     d = None if disp is None else np.ascontiguousarray(disp, np.float64)
     hf = None if h_fluids is None else np.ascontiguousarray(h_fluids, np.float64).reshape(m.n_cells, -1, 5)
-    rc = lib().or_go(ctypes.byref(om.c), ctypes.byref(params_c(params)), _p(coords), _p(state), _p(d), int(steps),
-                     1 if gcl else 0, int(nthreads), ctypes.byref(layer), _p(dth), ctypes.byref(ts),
-                     0 if hf is None else hf.shape[1], _p(hf), None if fit is None else ctypes.byref(fit.c(params)))
-    return rc, state[layer.value].copy(), coords[layer.value].copy(), dth[:steps], ts.value
+    rc = lib().or_go_k(ctypes.byref(om.c), ctypes.byref(params_c(params)), _p(coords), _p(state), _p(ks), _p(d),
+                       int(steps), 1 if gcl else 0, int(nthreads), ctypes.byref(layer), _p(dth), ctypes.byref(ts),
+                       0 if hf is None else hf.shape[1], _p(hf), None if fit is None else ctypes.byref(fit.c(params)))
+    out = (rc, state[layer.value].copy(), coords[layer.value].copy(), dth[:steps], ts.value)
+    return out if ks is None else out + (ks[layer.value].copy(),)
 
 
 def chlld_batch(n_state, normal, qL, qR, w=None, n_disc=None, flags=None, kL=None, kR=None):
@@ -297,7 +324,9 @@ def face_velocities(om, x1, x2, geom1, dt, gcl=False):
     return w
 
 
-def update(om, params, prim_old, vol_old, vol_new, flux, dt, src=None):
+def update(om, params, prim_old, vol_old, vol_new, flux, dt, src=None, scal=None, kflux=None):
+    """(rc, prim_new, n_rho_floor, n_p_floor), and with the passive scalars scal [nc,2] and their
+    face fluxes kflux [nf,2] the new scalars as a fifth item."""
     m = om.m
     out = np.empty((m.n_cells, 8))
     nr = ctypes.c_int64(0)
@@ -307,9 +336,16 @@ def update(om, params, prim_old, vol_old, vol_new, flux, dt, src=None):
     vn = np.ascontiguousarray(vol_new, np.float64)
     fl = np.ascontiguousarray(flux, np.float64)
     sr = None if src is None else np.ascontiguousarray(src, np.float64)
-    rc = lib().or_update(ctypes.byref(om.c), ctypes.byref(params_c(params)), _p(po), _p(vo), _p(vn), _p(fl), _p(sr),
-                         float(dt), _p(out), ctypes.byref(nr), ctypes.byref(npf))
-    return rc, out, nr.value, npf.value
+    if scal is None:
+        rc = lib().or_update(ctypes.byref(om.c), ctypes.byref(params_c(params)), _p(po), _p(vo), _p(vn), _p(fl),
+                             _p(sr), float(dt), _p(out), ctypes.byref(nr), ctypes.byref(npf))
+        return rc, out, nr.value, npf.value
+    so = np.ascontiguousarray(scal, np.float64).reshape(m.n_cells, 2)
+    kf = np.ascontiguousarray(kflux, np.float64).reshape(m.n_faces, 2)
+    sn = np.empty((m.n_cells, 2))
+    rc = lib().or_update_k(ctypes.byref(om.c), ctypes.byref(params_c(params)), _p(po), _p(so), _p(vo), _p(vn), _p(fl),
+                           _p(kf), _p(sr), float(dt), _p(out), _p(sn), ctypes.byref(nr), ctypes.byref(npf))
+    return rc, out, nr.value, npf.value, sn
 
 
 def cx_rate_ref(U, c, a2, n1=100, n2=100):
@@ -324,14 +360,21 @@ def det_cos(x):
     return lib().or_det_cos(float(x))
 
 
-def face_states(om, params, prim, geom, f):
-    """Left/right states of face f (Setka::Snos_on_Gran, first order + boundary)."""
+def face_states(om, params, prim, geom, f, scal=None):
+    """Left/right states of face f (Setka::Snos_on_Gran, first order + boundary); with the passive
+    scalars scal [nc,2] also their left/right values (kL, kR)."""
     g = OrGeom(*[_p(geom[k]) for k, _ in OrGeom._fields_])
     prim = np.ascontiguousarray(prim, np.float64)
     qL, qR = np.empty(8), np.empty(8)
-    rc = lib().or_face_states(ctypes.byref(om.c), ctypes.byref(params_c(params)), int(f), _p(prim),
-                              ctypes.byref(g), _p(qL), _p(qR))
-    return rc, qL, qR
+    if scal is None:
+        rc = lib().or_face_states(ctypes.byref(om.c), ctypes.byref(params_c(params)), int(f), _p(prim),
+                                  ctypes.byref(g), _p(qL), _p(qR))
+        return rc, qL, qR
+    sc = np.ascontiguousarray(scal, np.float64)
+    kL, kR = np.empty(2), np.empty(2)
+    rc = lib().or_face_states_k(ctypes.byref(om.c), ctypes.byref(params_c(params)), int(f), _p(prim), _p(sc),
+                                ctypes.byref(g), _p(qL), _p(qR), _p(kL), _p(kR))
+    return rc, qL, qR, kL, kR
 
 
 def motion(x_old, disp, dt, t, omega, phase):
@@ -350,17 +393,26 @@ def init_tvd(om, geom0):
     return out
 
 
-def bera_source(prim, h_fluids, a2=0.102046, n_h_lism=3.0, kn=40.0906):
+def bera_source(prim, h_fluids, a2=0.102046, n_h_lism=3.0, kn=40.0906, scal=None, cell_type=None):
     """Setka::Calc_sourse_MF_Bera plasma sources [n, 5] = (rho, m_x, m_y, m_z, E) of states prim [n, 8]
-    with hydrogen fluids h_fluids [n, n_h, 5]."""
+    with hydrogen fluids h_fluids [n, n_h, 5]; with the passive scalars scal [n, 2] (Q, rho_He) and
+    the cells' types cell_type [n] (1..4), determ_zone's proton split."""
     prim = np.ascontiguousarray(prim, np.float64).reshape(-1, 8)
     n = prim.shape[0]
     hf = np.ascontiguousarray(h_fluids, np.float64).reshape(n, -1, 5)
     out = np.empty((n, 5))
     L = lib()
     for c in range(n):
-        L.or_bera_source(_p(prim[c]), hf.shape[1], _p(hf[c]), a2, n_h_lism, kn, _p(out[c]))
+        if scal is None:
+            L.or_bera_source(_p(prim[c]), hf.shape[1], _p(hf[c]), a2, n_h_lism, kn, _p(out[c]))
+        else:
+            L.or_bera_source_k(_p(prim[c]), float(scal[c][0]), float(scal[c][1]), int(cell_type[c]), hf.shape[1],
+                               _p(hf[c]), a2, n_h_lism, kn, _p(out[c]))
     return out
+
+
+def determ_zone(Q, rho, cell_type):
+    return lib().or_determ_zone(float(Q), float(rho), int(cell_type))
 
 
 # ---- fitted surfaces ---------------------------------------------------------------------------

@@ -933,8 +933,53 @@ static void or_cart(const double *G, int equatorial, const double *Vs, int ik, d
 
 /* Second-order states of an interior face whose four cells (AA, A | B, BB) are known; returns 0 if
  * the face keeps the first-order states. */
+/* The passive scalars Q, rho_He are r^2 names like rho (Phys_param.cpp:284-286), so each zone case
+ * of Setka_gran_snos.cpp:216-713 reconstructs them by rho's line: values a, b, c, d of AA, A, B,
+ * BB; radii r3, r1, rr, r2, r4 of AA, A, G, B, BB.  (In the cases 1112 and 1122 the r^2 assignment
+ * is followed by a second if-chain whose final else overwrites it with the plain line.) */
+static void or_tvd_r2(int zAA, int zA, int zB, int zBB, double a, double b, double c, double d, double d1,
+                      double d2, double dd1, double dd2, double r3, double r1, double rr, double r2, double r4,
+                      double *l, double *r)
+{
+    if (zAA == 1 && zA == 1 && zB == 1 && zBB == 1) { /* :364-374 */
+        *l = or_linear(-dd1 - d1, a * KV(r3), -d1, b * KV(r1), d2, c * KV(r2), 0.0) / KV(rr);
+        *r = or_linear(-d1, b * KV(r1), d2, c * KV(r2), d2 + dd2, d * KV(r4), 0.0) / KV(rr);
+    } else if (zAA == 1 && zA == 1 && zB == 1 && zBB == 2) { /* :502-534, final else */
+        if (MISTAKE(OR_MISTAKE_SCAL_TVD_R2_KEPT)) {
+            *l = or_linear(-dd1 - d1, a * KV(r3), -d1, b * KV(r1), d2, c * KV(r2), 0.0) / KV(rr);
+            *r = or_linear2(-d1, b * KV(r1), d2, c * KV(r2), 0.0) / KV(rr);
+            return;
+        }
+        *l = or_linear(-dd1 - d1, a, -d1, b, d2, c, 0.0);
+        *r = or_linear2(-d1, b, d2, c, 0.0);
+    } else if (zAA == 1 && zA == 1 && zB == 2 && zBB == 2) { /* :618-646, final else */
+        if (MISTAKE(OR_MISTAKE_SCAL_TVD_R2_KEPT)) {
+            *l = or_linear2(-dd1 - d1, a * KV(r3), -d1, b * KV(r1), 0.0) / KV(rr);
+            *r = or_linear2(d2, c, d2 + dd2, d, 0.0);
+            return;
+        }
+        *l = or_linear2(-dd1 - d1, a, -d1, b, 0.0);
+        *r = or_linear2(d2, c, d2 + dd2, d, 0.0);
+    } else if (zAA == 1 && zA == 2 && zB == 2 && zBB == 2) { /* :649-660 */
+        *l = or_linear2(-d1, b, d2, c, 0.0);
+        *r = or_linear(-d1, b, d2, c, d2 + dd2, d, 0.0);
+    } else if ((zAA == 2 && zA == 2 && zB == 3 && zBB == 3) || (zAA == 3 && zA == 3 && zB == 4 && zBB == 4)) {
+        *l = or_linear2(-dd1 - d1, a, -d1, b, 0.0); /* :661-673 */
+        *r = or_linear2(d2, c, d2 + dd2, d, 0.0);
+    } else if ((zAA == 2 && zA == 2 && zB == 2 && zBB == 3) || (zAA == 3 && zA == 3 && zB == 3 && zBB == 4)) {
+        *l = or_linear(-dd1 - d1, a, -d1, b, d2, c, 0.0); /* :674-687 */
+        *r = or_linear2(-d1, b, d2, c, 0.0);
+    } else if ((zAA == 2 && zA == 3 && zB == 3 && zBB == 3) || (zAA == 3 && zA == 4 && zB == 4 && zBB == 4)) {
+        *l = or_linear2(-d1, b, d2, c, 0.0); /* :688-701 */
+        *r = or_linear(-d1, b, d2, c, d2 + dd2, d, 0.0);
+    } else { /* :702-713 */
+        *l = or_linear(-dd1 - d1, a, -d1, b, d2, c, 0.0);
+        *r = or_linear(-d1, b, d2, c, d2 + dd2, d, 0.0);
+    }
+}
+
 static int or_tvd_states(const or_mesh *m, const or_params *p, int64_t f, const double *prim,
-                         const or_geom *g, double *qL, double *qR)
+                         const double *scal, const or_geom *g, double *qL, double *qR, double *kL, double *kR)
 {
     int64_t A = m->face_cells[2 * f], B = m->face_cells[2 * f + 1];
     if (!p->tvd || !m->face_tvd || !m->cell_is_tvd[A] || !m->cell_is_tvd[B]) return 0;
@@ -1059,15 +1104,32 @@ static int or_tvd_states(const or_mesh *m, const or_params *p, int64_t f, const 
     if (qL[4] < 0.0000001) qL[4] = sA[4];
     if (qR[0] < 0.0000001) qR[0] = sB[0];
     if (qR[4] < 0.0000001) qR[4] = sB[4];
+    if (scal) {
+        const double r3 = or_norm3(AAc), r1 = or_norm3(Ac), rr = or_norm3(G), r2 = or_norm3(Bc), r4 = or_norm3(BBc);
+        for (int j = 0; j < 2; j++) {
+            const double kA = scal[2 * A + j], kB = scal[2 * B + j];
+            or_tvd_r2(zAA, zA, zB, zBB, scal[2 * AA + j], kA, kB, scal[2 * BB + j], d1, d2, dd1, dd2, r3, r1, rr, r2,
+                      r4, &kL[j], &kR[j]);
+            if (kL[j] < 0.0000001) kL[j] = kA; /* :718-723 */
+            if (kR[j] < 0.0000001) kR[j] = kB;
+        }
+    }
     return 1;
 }
 
-int or_face_states(const or_mesh *m, const or_params *p, int64_t f, const double *prim,
-                   const or_geom *g, double *qL, double *qR)
+int or_face_states_k(const or_mesh *m, const or_params *p, int64_t f, const double *prim, const double *scal,
+                     const or_geom *g, double *qL, double *qR, double *kL, double *kR)
 {
     int bc = m->face_bc[f];
     int64_t A = m->face_cells[2 * f];
     const double *sA = prim + 8 * A;
+    if (scal && bc != OR_BC_INTERIOR) { /* the plasma names of the cell (and of the face's fixed
+                                            state): Setka_gran_snos.cpp:20-57 */
+        for (int j = 0; j < 2; j++) {
+            kL[j] = scal[2 * A + j];
+            kR[j] = (bc == OR_BC_INNER_HARD || bc == OR_BC_OUTER_HARD) ? m->face_bscal[2 * f + j] : kL[j];
+        }
+    }
     if (bc != OR_BC_INTERIOR) {
         const double *n = g->face_normal + 3 * f;
         if (bc == OR_BC_OUTER_SOFT) { /* plasma_condition 1: Setka_gran_snos.cpp:18-50 */
@@ -1094,12 +1156,30 @@ int or_face_states(const or_mesh *m, const or_params *p, int64_t f, const double
     int64_t B = m->face_cells[2 * f + 1];
     const double *sB = prim + 8 * B;
     const double *G = g->face_center + 3 * f;
-    if (or_tvd_states(m, p, f, prim, g, qL, qR)) return 0;
+    if (or_tvd_states(m, p, f, prim, scal, g, qL, qR, kL, kR)) return 0;
     if (m->cell_zone[A] == 1) zone1_state(sA, g->cell_center + 3 * A, G, p->gamma, qL);
     else for (int i = 0; i < 8; i++) qL[i] = sA[i];
     if (m->cell_zone[B] == 1) zone1_state(sB, g->cell_center + 3 * B, G, p->gamma, qR);
     else for (int i = 0; i < 8; i++) qR[i] = sB[i];
+    if (scal) { /* Q and rho_He like rho: x kv(r1) / kv(rr) from a Zone_1 cell (Setka_gran_snos.cpp:848-852,
+                   1005-1009), else the cell's own (:877-880, 1037-1040) */
+        const double rr = sqrt((G[0] * G[0] + G[1] * G[1]) + G[2] * G[2]);
+        const double *Ac = g->cell_center + 3 * A, *Bc = g->cell_center + 3 * B;
+        const double rA = sqrt((Ac[0] * Ac[0] + Ac[1] * Ac[1]) + Ac[2] * Ac[2]);
+        const double rB = sqrt((Bc[0] * Bc[0] + Bc[1] * Bc[1]) + Bc[2] * Bc[2]);
+        const int plain = MISTAKE(OR_MISTAKE_SCAL_ZONE1_PLAIN);
+        for (int j = 0; j < 2; j++) {
+            kL[j] = m->cell_zone[A] == 1 && !plain ? scal[2 * A + j] * KV(rA) / KV(rr) : scal[2 * A + j];
+            kR[j] = m->cell_zone[B] == 1 && !plain ? scal[2 * B + j] * KV(rB) / KV(rr) : scal[2 * B + j];
+        }
+    }
     return 0;
+}
+
+int or_face_states(const or_mesh *m, const or_params *p, int64_t f, const double *prim,
+                   const or_geom *g, double *qL, double *qR)
+{
+    return or_face_states_k(m, p, f, prim, NULL, g, qL, qR, NULL, NULL);
 }
 
 /* Setka::Culc_Gran_Potok (Setka_physics.cpp:2280-2571), all faces.  x_now1/x_now2 are used only
@@ -1120,18 +1200,21 @@ static int or_face_in_area(const or_mesh *m, int64_t f, int area)
     return or_cell_in_area(m, a, area) || (b >= 0 && or_cell_in_area(m, b, area));
 }
 
-static int face_flux_impl(const or_mesh *m, const or_params *p, const double *prim, const or_geom *g,
-                          const double *x1, const double *x2, double dt, int gcl, double *flux,
-                          double *dt_face, uint8_t *branch, int nthreads)
+/* scal (nullable): the passive scalars [n_cells*2] of the layer; then kflux [n_faces*2] = their konvect
+ * fluxes * area (Setka_physics.cpp:2349-2361, 2485-2493). */
+static int face_flux_impl(const or_mesh *m, const or_params *p, const double *prim, const double *scal,
+                          const or_geom *g, const double *x1, const double *x2, double dt, int gcl, double *flux,
+                          double *kflux, double *dt_face, uint8_t *branch, int nthreads)
 {
     int status = 0;
     (void)nthreads;
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
     for (int64_t f = 0; f < m->n_faces; f++) {
-        double qL[8], qR[8], q[8], dsl, dsc, dsr;
+        double qL[8], qR[8], q[8], kL[2], kR[2], k[2] = {0.0, 0.0}, dsl, dsc, dsr;
         int br = 0;
         if (!or_face_in_area(m, f, p->area)) { /* not on the area's face list: not evaluated */
             for (int i = 0; i < 9; i++) flux[9 * f + i] = 0.0;
+            if (scal) kflux[2 * f] = kflux[2 * f + 1] = 0.0;
             dt_face[f] = 10000000.0;
             if (branch) branch[f] = 0;
             continue;
@@ -1152,14 +1235,16 @@ static int face_flux_impl(const or_mesh *m, const or_params *p, const double *pr
             double d0 = ca[0] - cb[0], d1 = ca[1] - cb[1], d2 = ca[2] - cb[2];
             dist = sqrt(KVV(d0, d1, d2)) / 2.0;
         }
-        int rc = or_face_states(m, p, f, prim, g, qL, qR);
+        int rc = or_face_states_k(m, p, f, prim, scal, g, qL, qR, kL, kR);
         if (rc == 0)
             rc = or_chlld(m->face_method[f], p->n_disc, m->face_flags ? m->face_flags[f] : 0, n[0], n[1],
-                          n[2], w, qL, qR, 0, NULL, NULL, q, NULL, &dsl, &dsc, &dsr, &br);
+                          n[2], w, qL, qR, scal ? 2 : 0, scal ? kL : NULL, scal ? kR : NULL, q, scal ? k : NULL, &dsl,
+                          &dsc, &dsr, &br);
         if (rc != 0) {
 #pragma omp critical(or_status)
             { if (status == 0) status = rc; }
             for (int i = 0; i < 9; i++) flux[9 * f + i] = 0.0;
+            if (scal) kflux[2 * f] = kflux[2 * f + 1] = 0.0;
             dt_face[f] = loc_time;
             if (branch) branch[f] = (uint8_t)br;
             continue;
@@ -1171,6 +1256,7 @@ static int face_flux_impl(const or_mesh *m, const or_params *p, const double *pr
         for (int i = 0; i < 8; i++) flux[9 * f + i] = q[i] * area;
         flux[9 * f + 8] = (MISTAKE(OR_MISTAKE_PDIVB_HALF) ? 1.0 : 0.5) *
                           spf(n[0], n[1], n[2], qL[5] + qR[5], qL[6] + qR[6], qL[7] + qR[7]) * area;
+        if (scal) { kflux[2 * f] = k[0] * area; kflux[2 * f + 1] = k[1] * area; }
         dt_face[f] = loc_time;
         if (branch) branch[f] = (uint8_t)br;
     }
@@ -1181,15 +1267,24 @@ int or_face_flux(const or_mesh *m, const or_params *p, const double *prim, const
                  const double *x_now1, const double *x_now2, double dt, double *flux, double *dt_face,
                  uint8_t *branch)
 {
-    return face_flux_impl(m, p, prim, g, x_now1, x_now2, dt, 0, flux, dt_face, branch, 1);
+    return face_flux_impl(m, p, prim, NULL, g, x_now1, x_now2, dt, 0, flux, NULL, dt_face, branch, 1);
 }
 
-/* Cell loop of Setka::Go (Setka_physics.cpp:1695-2027) for the plasma without passive scalars
- * and PUI: flux sum in the cell's ascending face order, Powell source, FP64 pressure recovery,
- * Zone_2 magnetic decay, floors, B frozen for cells with is_TVD == false. */
-static int update_impl(const or_mesh *m, const or_params *p, const double *prim_old, const double *vol_old,
-                       const double *vol_new, const double *flux, const double *src, double time,
-                       double *prim_new, int64_t *n_rho_floor, int64_t *n_p_floor, int nthreads)
+int or_face_flux_k(const or_mesh *m, const or_params *p, const double *prim, const double *scal, const or_geom *g,
+                   const double *x_now1, const double *x_now2, double dt, double *flux, double *kflux,
+                   double *dt_face, uint8_t *branch)
+{
+    return face_flux_impl(m, p, prim, scal, g, x_now1, x_now2, dt, 0, flux, kflux, dt_face, branch, 1);
+}
+
+/* Cell loop of Setka::Go (Setka_physics.cpp:1695-2027) for the plasma without PUI: flux sum in the
+ * cell's ascending face order, Powell source, FP64 pressure recovery, Zone_2 magnetic decay, floors,
+ * B frozen for cells with is_TVD == false; with scal_old (nullable) the passive scalars Q, rho_He
+ * advected by their fluxes kflux and clipped (:1781-1817, 2004-2016) into scal_new. */
+static int update_impl(const or_mesh *m, const or_params *p, const double *prim_old, const double *scal_old,
+                       const double *vol_old, const double *vol_new, const double *flux, const double *kflux,
+                       const double *src, double time, double *prim_new, double *scal_new, int64_t *n_rho_floor,
+                       int64_t *n_p_floor, int nthreads)
 {
     const double cpi4 = 4.0 * OR_PI;
     const double g1 = 5.0 / 3.0 - 1.0;
@@ -1201,6 +1296,7 @@ static int update_impl(const or_mesh *m, const or_params *p, const double *prim_
     for (int64_t c = 0; c < m->n_cells; c++) {
         if (!or_cell_in_area(m, c, p->area)) { /* outside the area: the state carries over */
             for (int i = 0; i < 8; i++) prim_new[8 * c + i] = prim_old[8 * c + i];
+            if (scal_old) { scal_new[2 * c] = scal_old[2 * c]; scal_new[2 * c + 1] = scal_old[2 * c + 1]; }
             continue;
         }
         double Volume = vol_old[c];
@@ -1221,6 +1317,24 @@ static int update_impl(const or_mesh *m, const or_params *p, const double *prim_
         double rho3 = rho * Volume / Volume2 - time * P[0] / Volume2;
         int rho_null = 0;
         if (rho3 < 1e-7) { rho_null = 1; rho3 = 0.1; nrf++; }
+        double Q3 = 0.0, rho_He3 = 0.0;
+        if (scal_old) {
+            double PQ = 0.0, PHe = 0.0;
+            for (int j = 0; j < 6; j++) {
+                int64_t f = m->cell_faces[6 * c + j];
+                int sign_potok = 1;
+                if (m->face_cells[2 * f] != c) sign_potok = -1;
+                PQ += sign_potok * kflux[2 * f];
+                PHe += sign_potok * kflux[2 * f + 1];
+            }
+            const double Q = scal_old[2 * c], rho_He = scal_old[2 * c + 1];
+            Q3 = Q * Volume / Volume2 - time * PQ / Volume2;
+            rho_He3 = rho_He * Volume / Volume2 - time * PHe / Volume2;
+            if (rho_He3 < 0.0) rho_He3 = 0.0;
+            if (rho_He3 > rho3) rho_He3 = MISTAKE(OR_MISTAKE_HE_CLIP_RHO) ? rho3 : rho3 * 0.1;
+            scal_new[2 * c] = Q3;
+            scal_new[2 * c + 1] = rho_He3;
+        }
         double u3, v3, w3, bx3, by3, bz3, p3;
         if (!rho_null) {
             /* planted mistakes: a Powell momentum term over 8 pi, an induction term of the wrong
@@ -1269,7 +1383,17 @@ int or_update(const or_mesh *m, const or_params *p, const double *prim_old, cons
               const double *vol_new, const double *flux, const double *src, double dt, double *prim_new,
               int64_t *n_rho_floor, int64_t *n_p_floor)
 {
-    return update_impl(m, p, prim_old, vol_old, vol_new, flux, src, dt, prim_new, n_rho_floor, n_p_floor, 1);
+    return update_impl(m, p, prim_old, NULL, vol_old, vol_new, flux, NULL, src, dt, prim_new, NULL, n_rho_floor,
+                       n_p_floor, 1);
+}
+
+int or_update_k(const or_mesh *m, const or_params *p, const double *prim_old, const double *scal_old,
+                const double *vol_old, const double *vol_new, const double *flux, const double *kflux,
+                const double *src, double dt, double *prim_new, double *scal_new, int64_t *n_rho_floor,
+                int64_t *n_p_floor)
+{
+    return update_impl(m, p, prim_old, scal_old, vol_old, vol_new, flux, kflux, src, dt, prim_new, scal_new,
+                       n_rho_floor, n_p_floor, 1);
 }
 
 static void geom_alloc(or_geom *g, const or_mesh *m)
@@ -1320,12 +1444,28 @@ static double or_ue_bera(double T1, double T2, const double *V1, const double *V
 
 static double or_dot3(const double *a, const double *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
 
+/* Setka::determ_zone (Setka_physics.cpp:1427-1482) */
+int or_determ_zone(double Q, double rho, int cell_type)
+{
+    if (Q / rho < 50.0) return cell_type == 1 ? 1 : 2;
+    return cell_type == 4 ? 4 : 3;
+}
+
 void or_bera_source(const double *s, int n_h, const double *h, double a2, double n_h_lism, double kn,
                     double *out)
 {
+    or_bera_source_k(s, 0.0, 0.0, 1, n_h, h, a2, n_h_lism, kn, out);
+}
+
+/* With the passive scalars: Calc_sourse_MF_Bera(cell, ..., zone = determ_zone) passes zone - 1 to
+ * Plasma_components_1 (Setka_physics.cpp:769, 783), whose al is 2 for zone - 1 <= 2. */
+void or_bera_source_k(const double *s, double Q, double rho_He, int cell_type, int n_h, const double *h, double a2,
+                      double n_h_lism, double kn, double *out)
+{
     /* proton density and temperature (Plasma_components_1, Phys_param.cpp:641-660) */
-    const double rho = s[0], pr = s[4], rho_He = 0.0;
-    const int al = 2;
+    const double rho = s[0], pr = s[4];
+    const int zone_ = or_determ_zone(Q, rho, cell_type) - 1;
+    const int al = zone_ <= 2 ? 2 : 1;
     double rho_p = -(OR_ME_MP * al * rho_He + 4.0 * (-rho + rho_He)) / (4.0 * (1.0 + OR_ME_MP));
     double T_p = 8.0 * (1.0 + OR_ME_MP) * pr / (8.0 * rho - (7.0 - al + (-1.0 + al) * OR_ME_MP) * rho_He);
     const double *Vp = s + 1;
@@ -1894,12 +2034,22 @@ int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, co
           int64_t steps, int gcl, int nthreads, int *final_layer, double *dt_hist, double *t_sim,
           int n_h, const double *h_fluids, const or_fit *fit)
 {
-    int64_t nn3 = 3 * m->n_nodes, nc8 = 8 * m->n_cells;
+    return or_go_k(m, p, coords, prim, NULL, disp, steps, gcl, nthreads, final_layer, dt_hist, t_sim, n_h, h_fluids,
+                   fit);
+}
+
+int or_go_k(const or_mesh *m, const or_params *p, double *coords, double *prim, double *scal, const double *disp,
+            int64_t steps, int gcl, int nthreads, int *final_layer, double *dt_hist, double *t_sim,
+            int n_h, const double *h_fluids, const or_fit *fit)
+{
+    int64_t nn3 = 3 * m->n_nodes, nc8 = 8 * m->n_cells, nc2 = 2 * m->n_cells;
     /* charge-exchange sources of the step's old layer (Setka_physics.cpp:1741), when hydrogen
      * fluids are given */
     double *src = h_fluids ? (double *)malloc(sizeof(double) * 5 * m->n_cells) : NULL;
     double *X[2] = {coords, coords + nn3};
     double *S[2] = {prim, prim + nc8};
+    double *K[2] = {scal, scal ? scal + nc2 : NULL};
+    double *kflux = scal ? (double *)malloc(sizeof(double) * 2 * m->n_faces) : NULL;
     or_geom G[2];
     geom_alloc(&G[0], m);
     geom_alloc(&G[1], m);
@@ -1909,6 +2059,7 @@ int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, co
     if (gcl) { or_measure_gcl(m, X[0], &G[0]); or_measure_gcl(m, X[1], &G[1]); }
     else { or_measure(m, X[0], &G[0]); or_measure(m, X[1], &G[1]); }
     memcpy(S[1], S[0], sizeof(double) * nc8);
+    if (scal) memcpy(K[1], K[0], sizeof(double) * nc2);
     int now1 = 1, now2 = 0;
     double time = 0.00001, loc_time = 0.00001;
     double tsim = t_sim ? *t_sim : 0.0;
@@ -1931,18 +2082,20 @@ int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, co
             if (gcl) or_measure_gcl(m, X[now2], &G[now2]);
             else or_measure(m, X[now2], &G[now2]);
         }
-        int rc = face_flux_impl(m, p, S[now1], &G[now1], X[now1], X[now2], time, gcl, flux, dtf, NULL, nthreads);
+        int rc = face_flux_impl(m, p, S[now1], K[now1], &G[now1], X[now1], X[now2], time, gcl, flux, kflux, dtf, NULL,
+                                nthreads);
         if (rc && !status) status = rc;
         for (int64_t f = 0; f < m->n_faces; f++)
             if (dtf[f] < loc_time) loc_time = SMIN(loc_time, dtf[f]);
         if (src) {
 #pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
             for (int64_t c = 0; c < m->n_cells; c++)
-                or_bera_source(S[now1] + 8 * c, n_h, h_fluids + (int64_t)5 * n_h * c, p->par_a_2, p->par_n_H_LISM,
-                               p->par_Kn, src + 5 * c);
+                or_bera_source_k(S[now1] + 8 * c, scal ? K[now1][2 * c] : 0.0, scal ? K[now1][2 * c + 1] : 0.0,
+                                 m->cell_zone[c], n_h, h_fluids + (int64_t)5 * n_h * c, p->par_a_2, p->par_n_H_LISM,
+                                 p->par_Kn, src + 5 * c);
         }
-        rc = update_impl(m, p, S[now1], G[now1].cell_volume, G[now2].cell_volume, flux, src, time, S[now2],
-                         NULL, NULL, nthreads);
+        rc = update_impl(m, p, S[now1], K[now1], G[now1].cell_volume, G[now2].cell_volume, flux, kflux, src, time,
+                         S[now2], K[now2], NULL, NULL, nthreads);
         if (rc && !status) status = rc;
         tsim += time;
         if (status) break;
@@ -1952,6 +2105,7 @@ int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, co
     geom_free(&G[0]);
     geom_free(&G[1]);
     free(flux);
+    free(kflux);
     free(dtf);
     free(src);
     return status;

@@ -62,6 +62,12 @@
  *   or_minmod, or_linear  known answers, including the 1e-8 sign threshold.
  *   or_update, or_go    mass conservation, quiescent state, floors at their thresholds, frozen B,
  *                       volume ratio and sources, lagged dt schedule, Brio-Wu profile.
+ *   the _k forms        (passive scalars Q, rho_He) a scalar equal to 2 rho (or rho/4) -- a power of
+ *                       two, so every operation scales exactly -- keeps that ratio bit for bit through
+ *                       reconstruction (all zone cases, Zone_1 and TVD), every chlld region and the
+ *                       update, which fixes each scalar formula to be rho's; total Q conserved on a
+ *                       closed box; the rho_He clips at their thresholds; determ_zone's four outcomes
+ *                       and the al switch of the closed-form source against its al = 1 / al = 2 forms.
  *   or_new_connect      hand-worked two-cell mesh and face counts.
  *   or_cx_rate_ref      the reference's own MK_source program (compiled from its source into
  *                       oracle/_ref) at sampled table points: tests/golden/mk_source_int1.json.
@@ -108,7 +114,10 @@ enum {
     OR_MISTAKE_ZONE2_8PI = 1 << 4,         /* Zone_2 decay: pressure compensation over 8 pi       */
     OR_MISTAKE_HLLD_VZZ_SIGN = 1 << 5,     /* HLLD double-star v_t: sign(Bn) term negated          */
     OR_MISTAKE_HLLD_BZZ_WEIGHTS = 1 << 6,  /* HLLD double-star b_t: sqrt(rho*) weights on the wrong side */
-    OR_MISTAKE_HLLD_TB_BN2 = 1 << 7        /* HLLD star b_t: Bn^2 dropped from the left numerator  */
+    OR_MISTAKE_HLLD_TB_BN2 = 1 << 7,       /* HLLD star b_t: Bn^2 dropped from the left numerator  */
+    OR_MISTAKE_SCAL_ZONE1_PLAIN = 1 << 8,  /* passive scalars from a Zone_1 cell not rescaled by r^2   */
+    OR_MISTAKE_SCAL_TVD_R2_KEPT = 1 << 9,  /* TVD 1112 / 1122: the scalars' r^2 lines kept (not overwritten) */
+    OR_MISTAKE_HE_CLIP_RHO = 1 << 10       /* rho_He above the new density clipped to it, not to a tenth */
 };
 void or_set_mistake(int bits);
 
@@ -133,6 +142,8 @@ typedef struct {
     const int32_t *face_tvd;     /* [n_faces*2] TVD neighbours AA (behind cells[0]) and BB
                                     (behind cells[1]), -1 = none (or_init_tvd); nullable */
     const uint8_t *cell_inner;   /* [n_cells] Cell::is_inner (Setka::Init_boundary_grans); nullable */
+    const double  *face_bscal;   /* [n_faces*2] Q, rho_He of the fixed boundary state; needed by the
+                                    *_k functions on hard faces                                  */
 } or_mesh;
 
 typedef struct {
@@ -197,6 +208,19 @@ double or_linear(double x1, double t1, double x2, double t2, double x3, double t
 /* ---- face / cell passes ---- */
 int or_face_states(const or_mesh *m, const or_params *p, int64_t f, const double *prim,
                    const or_geom *g, double *qL, double *qR);
+/* The _k forms carry the passive scalars Q, rho_He (scal [n_cells*2], kflux [n_faces*2], scal_new). */
+int or_face_states_k(const or_mesh *m, const or_params *p, int64_t f, const double *prim, const double *scal,
+                     const or_geom *g, double *qL, double *qR, double *kL, double *kR);
+int or_face_flux_k(const or_mesh *m, const or_params *p, const double *prim, const double *scal, const or_geom *g,
+                   const double *x_now1, const double *x_now2, double dt, double *flux, double *kflux,
+                   double *dt_face, uint8_t *branch);
+int or_update_k(const or_mesh *m, const or_params *p, const double *prim_old, const double *scal_old,
+                const double *vol_old, const double *vol_new, const double *flux, const double *kflux,
+                const double *src, double dt, double *prim_new, double *scal_new, int64_t *n_rho_floor,
+                int64_t *n_p_floor);
+int or_determ_zone(double Q, double rho, int cell_type);
+void or_bera_source_k(const double *s, double Q, double rho_He, int cell_type, int n_h, const double *h, double a2,
+                      double n_h_lism, double kn, double *out);
 /* flux[f*9 + 0..7] = chlld flux * area, flux[f*9+8] = PdivB; dt_face[f] (1e7 on non-interior) */
 int or_face_flux(const or_mesh *m, const or_params *p, const double *prim, const or_geom *g,
                  const double *x_now1, const double *x_now2, double dt, double *flux,
@@ -283,6 +307,10 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *coords, const doub
 int or_go(const or_mesh *m, const or_params *p, double *coords, double *prim, const double *disp,
           int64_t steps, int gcl, int nthreads, int *final_layer, double *dt_hist,
           double *t_sim, int n_h, const double *h_fluids, const or_fit *fit);
+/* or_go with the passive scalars: scal [2][n_cells*2] (layer 0 in), the newest in layer *final_layer */
+int or_go_k(const or_mesh *m, const or_params *p, double *coords, double *prim, double *scal, const double *disp,
+            int64_t steps, int gcl, int nthreads, int *final_layer, double *dt_hist, double *t_sim, int n_h,
+            const double *h_fluids, const or_fit *fit);
 /* ---- Setka::Calc_sourse_MF_Bera (Setka_physics.cpp:766-1258), plasma part: state s[8], n_h
  * hydrogen fluids h[5k..5k+4] = (rho_Hk, p_Hk, V_Hk); out[5] = (rho, m_x, m_y, m_z, E) sources. */
 void or_bera_source(const double *s, int n_h, const double *h, double a2, double n_h_lism, double kn,

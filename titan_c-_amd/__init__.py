@@ -48,7 +48,8 @@ class MeshC(ctypes.Structure):
                 ("cell_nodes", ctypes.c_void_p), ("face_nodes", ctypes.c_void_p), ("face_cells", ctypes.c_void_p),
                 ("cell_faces", ctypes.c_void_p), ("face_bc", ctypes.c_void_p), ("face_method", ctypes.c_void_p),
                 ("face_flags", ctypes.c_void_p), ("face_bstate", ctypes.c_void_p), ("cell_zone", ctypes.c_void_p),
-                ("cell_is_tvd", ctypes.c_void_p), ("node_xyz", ctypes.c_void_p), ("node_disp", ctypes.c_void_p)]
+                ("cell_is_tvd", ctypes.c_void_p), ("node_xyz", ctypes.c_void_p), ("node_disp", ctypes.c_void_p),
+                ("face_bscalar", ctypes.c_void_p)]
 
 
 class ParamsC(ctypes.Structure):
@@ -62,14 +63,14 @@ class ParamsC(ctypes.Structure):
                 ("sglag_TS", ctypes.c_int32), ("sglag_HP", ctypes.c_int32), ("sglag_BS", ctypes.c_int32),
                 ("velocity_TS", ctypes.c_double), ("velocity_HP", ctypes.c_double), ("velocity_BS", ctypes.c_double),
                 ("sglag_TS_k_sphere", ctypes.c_double), ("sglag_HP_k_sphere", ctypes.c_double),
-                ("sglag_BS_k", ctypes.c_double)]
+                ("sglag_BS_k", ctypes.c_double), ("passive_scalars", ctypes.c_int32)]
 
 # parameters a params dict may lack: Phys_param.h:99-101 for par_*, Phys_param.cpp:500-520 for the
 # fitted surfaces
 _PARAM_DEFAULTS = {"tvd": 0, "snos_on_hp": 0, "par_a_2": 0.102046, "par_n_H_LISM": 3.0, "par_Kn": 40.0906,
                    "move_TS": 1, "move_HP": 1, "move_BS": 1, "sglag_TS": 1, "sglag_HP": 1, "sglag_BS": 0,
                    "velocity_TS": 0.1, "velocity_HP": 0.1, "velocity_BS": 0.3, "sglag_TS_k_sphere": 0.01,
-                   "sglag_HP_k_sphere": 0.01, "sglag_BS_k": 0.05}
+                   "sglag_HP_k_sphere": 0.01, "sglag_BS_k": 0.05, "passive_scalars": 0}
 
 
 class RaysC(ctypes.Structure):
@@ -106,7 +107,9 @@ _SIGS = {
     "titan_set_h_fluids": (_I, [_P, _I, _P]),
     "titan_calculating_measure": (_I, [_P, _I]),
     "titan_get_geometry": (_I, [_P, _I, _P, _P, _P, _P, _P]),
-    "titan_gran_potok": (_I, [_P, _I, _D, _P, _P, _P]),
+    "titan_set_scalars": (_I, [_P, _P, _I]),
+    "titan_get_scalars": (_I, [_P, _P, _I]),
+    "titan_gran_potok": (_I, [_P, _I, _D, _P, _P, _P, _P]),
     "titan_go": (_I, [_P, _I64, ctypes.POINTER(StatsC)]),
     "titan_go_area": (_I, [_P, _I, _I64, ctypes.POINTER(StatsC)]),
     "titan_set_inner_area": (_I, [_P, _P]),
@@ -311,6 +314,7 @@ class Setka:
         m.cell_is_tvd = keep(mesh.cell_is_tvd, np.uint8)
         m.node_xyz = keep(mesh.node_xyz, np.float64)
         m.node_disp = keep(mesh.node_disp, np.float64)
+        m.face_bscalar = keep(getattr(mesh, "face_bscalar", None), np.float64)
         p = ParamsC()
         for k, _ in ParamsC._fields_:
             setattr(p, k, params.get(k, _PARAM_DEFAULTS[k]) if k in _PARAM_DEFAULTS else params[k])
@@ -349,6 +353,18 @@ class Setka:
         _check(lib().titan_get_state(self.ctx, _ptr(a), layer), "titan_get_state", self.ctx)
         return a
 
+    def set_scalars(self, qhe, layer=0):
+        """Passive scalars [n_cells, 2] = (Q, rho_He) of a layer (params passive_scalars = 1)."""
+        a = _c(qhe, np.float64).reshape(self.n_cells, 2)
+        _check(lib().titan_set_scalars(self.ctx, _ptr(a), layer), "titan_set_scalars", self.ctx)
+
+    def get_scalars(self, layer=None):
+        if layer is None:
+            layer = self.current_layer()
+        a = np.empty((self.n_cells, 2))
+        _check(lib().titan_get_scalars(self.ctx, _ptr(a), layer), "titan_get_scalars", self.ctx)
+        return a
+
     def set_nodes(self, xyz, layer):
         a = _c(xyz, np.float64).reshape(self.n_nodes, 3)
         _check(lib().titan_set_nodes(self.ctx, _ptr(a), layer), "titan_set_nodes", self.ctx)
@@ -378,13 +394,17 @@ class Setka:
                "titan_get_geometry", self.ctx)
         return dict(cell_center=cc, cell_volume=cv, face_center=fc, face_normal=fn, face_area=fa)
 
-    def Culc_Gran_Potok(self, now, dt, want_branch=True):
-        """The face loop: returns (flux [nf,9], dt_min, branch [nf])."""
+    def Culc_Gran_Potok(self, now, dt, want_branch=True, want_scalars=False):
+        """The face loop: returns (flux [nf,9], dt_min, branch [nf]), and with want_scalars the
+        passive scalar fluxes [nf,2] (PQ, Prho_He) as a fourth item."""
         flux = np.empty((self.n_faces, 9))
+        kflux = np.empty((self.n_faces, 2)) if want_scalars else None
         dtm = ctypes.c_double(0.0)
         br = np.empty(self.n_faces, np.uint8) if want_branch else None
-        _check(lib().titan_gran_potok(self.ctx, now, float(dt), _ptr(flux), ctypes.byref(dtm), _ptr(br)),
+        _check(lib().titan_gran_potok(self.ctx, now, float(dt), _ptr(flux), _ptr(kflux), ctypes.byref(dtm), _ptr(br)),
                "titan_gran_potok", self.ctx)
+        if want_scalars:
+            return flux, dtm.value, br, kflux
         return flux, dtm.value, br
 
     def Go(self, steps, is_inner_area=None):

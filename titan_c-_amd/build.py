@@ -35,10 +35,12 @@ HIP_COMMON = [
 ]
 UNITS = {
     "k_step64.hip": ["-ffp-contract=off"],
+    "k_step64sc.hip": ["-ffp-contract=off"],
     "k_cx.hip": ["-ffp-contract=off"],
     "k_fit.hip": ["-ffp-contract=off"],
     "k_mk.hip": ["-ffp-contract=off"],
     "k_step32.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "k_step32sc.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "k_mlp.hip": ["-ffp-contract=fast"],
     "k_train.hip": ["-ffp-contract=fast"],
     "k_noalloc.hip": ["-ffp-contract=off"],

@@ -50,6 +50,7 @@ class Mesh:
     cell_is_tvd: np.ndarray
     node_xyz: np.ndarray
     node_disp: Optional[np.ndarray] = None
+    face_bscalar: Optional[np.ndarray] = None  # [n_faces, 2] Q, rho_He of the hard faces' fixed states
 
 
 def splitmix_uniform(seed, index):
@@ -146,6 +147,33 @@ def lism_state():
     return np.array([rho, v, 0.0, 0.0, p, -b * math.cos(a), b * math.sin(a), 0.0])
 
 
+HE_FRACTION = 0.15  # rho_He / rho of the test states
+
+
+def passive_scalars(mesh, prim, r_blend_au=100.0):
+    """Initial passive scalars [n_cells, 2] of a shell state: Q/rho = 1 in the solar wind and 100 in
+    the interstellar medium, blended over radius as the shell's state is (so Q/rho crosses the
+    zone switch 50 of determ_zone), and rho_He = HE_FRACTION * rho."""
+    ccen = mesh.node_xyz[mesh.cell_nodes].mean(axis=1)
+    rc = np.linalg.norm(ccen, axis=1)
+    w = 0.5 * (1.0 + np.tanh((rc - r_blend_au * R0_AU) / (0.2 * r_blend_au * R0_AU)))
+    out = np.empty((mesh.n_cells, 2))
+    out[:, 0] = (1.0 + 99.0 * w) * prim[:, 0]
+    out[:, 1] = HE_FRACTION * prim[:, 0]
+    return out
+
+
+def _hard_face_scalars(mesh):
+    """Passive scalars of the hard faces' fixed states [n_faces, 2]: Q/rho = 1 in the wind (inner
+    sphere), 100 in the interstellar inflow (the reference's initial states, Setka_physics.cpp:
+    272,542), rho_He = HE_FRACTION * rho."""
+    bq = np.zeros((mesh.n_faces, 2))
+    rho_b = mesh.face_bstate[:, 0]
+    bq[:, 0] = np.where(mesh.face_bc == BC_OUTER_HARD, 100.0, 1.0) * rho_b
+    bq[:, 1] = HE_FRACTION * rho_b
+    return bq
+
+
 def shell_radii(n_layers, r_out_au=1500.0):
     return R0_AU * r_out_au ** (np.arange(n_layers + 1) / n_layers)
 
@@ -207,6 +235,7 @@ def shell(n_edge=37, n_layers=120, moving=False, seed_motion=3, r_zone1_au=80.0,
         amp_n = amp * rn * np.sin(3.0 * th + phase[0]) * np.sin(2.0 * ph + phase[1]) * taper
         disp = amp_n[:, None] * xyz / rn[:, None]
     mesh, _ = _finish(xyz, cn, bc_fn, method_fn, zone, tvd_fn=tvd_fn, bstate_fn=bstate_fn, disp=disp)
+    mesh.face_bscalar = _hard_face_scalars(mesh)
     # interior: r^-2 wind blended to the interstellar state around r_blend (tanh in r)
     wind = solar_wind_state(ccen)
     w = 0.5 * (1.0 + np.tanh((rc - r_blend_au * R0_AU) / (0.2 * r_blend_au * R0_AU)))
@@ -320,6 +349,7 @@ def shell_fitted(n_edge=6, M=(3, 6, 2, 8, 5, 5), R0_au=1.0, R1_au=10.0, R5_au=60
         return bs
 
     mesh, ccen = _finish(xyz, cn, bc_fn, method_fn, zone, bstate_fn=bstate_fn)
+    mesh.face_bscalar = _hard_face_scalars(mesh)
     sk = surf_kind(mesh.face_nodes)
     flags = np.zeros(mesh.n_faces, np.uint8)
     flags[sk == 1] = 1 << 3

@@ -2,12 +2,14 @@
 // energy the protons exchange with n_h hydrogen fluids by charge exchange, from the closed-form
 // Maxwellian-averaged relative speeds U, U_m, U_E and sigma = (1 - a_2 ln U)^2.
 //
-// The state carries no pickup ions, Q or rho_He, the keys the reference then reads as 0: the
-// proton split of Plasma_components_1 (Phys_param.cpp:628-665) has rho_He = 0, and the zone
-// determ_zone returns (Q/rho = 0 < 50: Zone 1 or 2) gives al = 2.  The plasma mass source is 0
-// (charge exchange swaps a proton for a proton); the hydrogen-fluid and pickup-ion sources of the
-// same routine are not formed.  Evaluated in the reference's operation order (FP64, no contraction
-// in the translation units that include it); only log and sqrt differ from glibc in the last ulp.
+// The proton split of Plasma_components_1 (Phys_param.cpp:628-665) reads rho_He and a zone: the
+// zone of Setka::determ_zone (Setka_physics.cpp:1427-1482; 1 or 2 by cell type where Q/rho < 50,
+// else 4 for a Zone_4 cell and 3 otherwise), passed on as zone - 1 (Setka_physics.cpp:769,783), so
+// al = 2 unless zone - 1 > 2.  A state without the passive scalars reads Q = rho_He = 0 (zone 1
+// or 2, al = 2).  The plasma mass source is 0 (charge exchange swaps a proton for a proton); the
+// hydrogen-fluid and pickup-ion sources of the same routine are not formed.  Evaluated in the
+// reference's operation order (FP64, no contraction in the translation units that include it);
+// only log and sqrt differ from glibc in the last ulp.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -40,13 +42,20 @@ __device__ __forceinline__ double bera_dot(const double *a, const double *b) {
     return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
 }
 
-// s: plasma state (rho, V, p, B); h[5k..5k+4] = (rho_Hk, p_Hk, V_Hk); a2 = par_a_2;
-// ddp = par_n_H_LISM / par_Kn.  out = (rho, m_x, m_y, m_z, E) sources per unit volume and time.
-__device__ __forceinline__ void bera_source(const double s[8], int n_h, const double *__restrict__ h, double a2,
-                                            double ddp, double out[5]) {
-    // Plasma_components_1 (Phys_param.cpp:641-660) with rho_He = 0, al = 2
-    const double rho = s[0], pr = s[4], rho_He = 0.0;
-    const int al = 2;
+// Setka::determ_zone (Setka_physics.cpp:1427-1482): 1..4 from Q/rho and the cell type (1..4).
+__host__ __device__ inline int bera_zone(double Q, double rho, int cell_type) {
+    if (Q / rho < 50.0) return cell_type == 1 ? 1 : 2;
+    return cell_type == 4 ? 4 : 3;
+}
+
+// s: plasma state (rho, V, p, B); rho_He, zone (bera_zone); h[5k..5k+4] = (rho_Hk, p_Hk, V_Hk);
+// a2 = par_a_2; ddp = par_n_H_LISM / par_Kn.  out = (rho, m_x, m_y, m_z, E) sources per unit volume
+// and time.
+__device__ __forceinline__ void bera_source(const double s[8], double rho_He, int zone, int n_h,
+                                            const double *__restrict__ h, double a2, double ddp, double out[5]) {
+    // Plasma_components_1 (Phys_param.cpp:641-660) at zone - 1
+    const double rho = s[0], pr = s[4];
+    const int al = (zone - 1 <= 2) ? 2 : 1;
     double rho_p = -(kMeMp * al * rho_He + 4.0 * (-rho + rho_He)) / (4.0 * (1.0 + kMeMp));
     double T_p = 8.0 * (1.0 + kMeMp) * pr / (8.0 * rho - (7.0 - al + (-1.0 + al) * kMeMp) * rho_He);
     const double Vp[3] = {s[1], s[2], s[3]};

@@ -6,6 +6,7 @@
 //   the summation order of Setka_physics.cpp:1702-1733), cell_code [Nc] u32,
 //   face_nodes [Nf*4] i32, face_cells [Nf*2] i32, face_code [Nf] u32, face_bidx [Nf] i32,
 //   bstate [Nb*8] f64, node xyz [2][Nn*3] f64, prim [2][Nc*8] f64 (cell-major, 64 B per cell),
+//   passive scalars (when carried) scal [2][Nc*2] f64 = (Q, rho_He) per cell and bscal [Nb*2],
 //   geometry per layer: cell centre [Nc*3], volume [Nc], face centre/normal [Nf*3], area [Nf].
 #pragma once
 #include <hip/hip_runtime.h>
@@ -45,6 +46,7 @@ struct DevMesh {
     const uint32_t *face_code;
     const int32_t *face_bidx;
     const double *bstate;
+    const double *bscal;       // [Nb*2] Q, rho_He of the fixed boundary states (passive scalars)
     const int32_t *face_ref;   // device face -> reference face id (error reports)
     const int32_t *cell_ref;   // device cell -> reference cell id
     const int32_t *face_tvd;   // [Nf*2] second neighbours (AA, BB) of the faces that reconstruct to
@@ -64,8 +66,13 @@ struct DevParams {
     int32_t move_mesh, n_disc, geom_mode, tvd;  // tvd: some face reconstructs to second order
     double motion_omega, motion_phase;
     int32_t area;                    // cell_in_area: the cells this step updates
-    int32_t pad_area;
+    int32_t n_scal;                  // passive scalars carried by the step: 0, or 2 (Q, rho_He)
 };
+
+// The passive scalars of the plasma (Phys_param.cpp:232-233): Q, the heliopause marker, and the
+// helium density rho_He, advected with the flow (the konvect part of Setka::Culc_Gran_Potok,
+// Setka_physics.cpp:2349-2361,2485-2493).
+constexpr int kScal = 2;
 
 // Fitted surfaces and their rays (Setka::Culc_Velocity_surface, Setka_move.cpp:111-1357, and
 // Luch::dvigenie of the A_Luch rays, Luch.cpp:6-152).  Surface kinds: 1 TS, 2 HP, 3 BS; arrays

@@ -112,24 +112,29 @@ void launch_cx_cells(hipStream_t s, int64_t nc, const double *prim, const double
                        nodes, src);
 }
 
-// Calc_sourse_MF_Bera per cell: hfluid [Nc * n_h * 5] (rho_H, p_H, V_H per fluid)
+// Calc_sourse_MF_Bera per cell: hfluid [Nc * n_h * 5] (rho_H, p_H, V_H per fluid); scal [Nc*2]
+// (Q, rho_He; nullable: both 0) and the cell's zone code select the proton split (bera_zone).
 __global__ void __launch_bounds__(256) k_bera_cells(int64_t nc, const double *__restrict__ prim,
+                                                    const double *__restrict__ scal,
+                                                    const uint32_t *__restrict__ cell_code,
                                                     const double *__restrict__ hfluid, int n_h, double a2,
                                                     double ddp, double *__restrict__ src) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     double s[8];
     load_state(prim, c, s);
+    const double Q = scal ? scal[2 * c] : 0.0, rho_He = scal ? scal[2 * c + 1] : 0.0;
+    const int zone = bera_zone(Q, s[0], (int)((cell_code[c] >> 8) & 15u));
     double o[5];
-    bera_source(s, n_h, hfluid + (int64_t)5 * n_h * c, a2, ddp, o);
+    bera_source(s, rho_He, zone, n_h, hfluid + (int64_t)5 * n_h * c, a2, ddp, o);
 #pragma unroll
     for (int k = 0; k < 5; k++) src[5 * c + k] = o[k];
 }
 
-void launch_bera_cells(hipStream_t s, int64_t nc, const double *prim, const double *hfluid, int n_h, double a2,
-                       double ddp, double *src) {
-    hipLaunchKernelGGL(k_bera_cells, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, nc, prim, hfluid, n_h, a2,
-                       ddp, src);
+void launch_bera_cells(hipStream_t s, int64_t nc, const double *prim, const double *scal, const uint32_t *cell_code,
+                       const double *hfluid, int n_h, double a2, double ddp, double *src) {
+    hipLaunchKernelGGL(k_bera_cells, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, nc, prim, scal, cell_code,
+                       hfluid, n_h, a2, ddp, src);
 }
 
 void launch_cx_list(hipStream_t s, const int32_t *list, const unsigned long long *count, int64_t cap,

@@ -10,27 +10,18 @@ namespace titan {
 static inline unsigned grid_of32(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 void launch_face_flux_f32(hipStream_t s, const DevMesh &M, const DevParams &P, const double *prim,
-                          const DevGeom &G, const double *X1, const double *X2, StepScalars *S, double *flux,
-                          uint8_t *branch, long long step, double dt_override) {
-    hipLaunchKernelGGL(k_face_flux<float>, dim3(grid_of32(M.n_faces, 256)), dim3(256), 0, s, M, P, prim, G, X1, X2,
-                       S, flux, branch, step, dt_override);
+                          const double *scal, const DevGeom &G, const double *X1, const double *X2, StepScalars *S,
+                          double *flux, double *kflux, uint8_t *branch, long long step, double dt_override) {
+    hipLaunchKernelGGL(k_face_flux<float>, dim3(grid_of32(M.n_faces, 256)), dim3(256), 0, s, M, P, prim, scal, G, X1,
+                       X2, S, flux, kflux, branch, step, dt_override);
 }
 
 void launch_step_tile_f32(hipStream_t s, const TileMesh &TM, const DevMesh &M, const DevParams &P,
                           const double *prim1, double *prim2, const double *X1, double *X2, const double *disp,
                           const double *cc1, double *cc2, const double *src, const SurArgs *sur, StepScalars *S,
                           long long step) {
-    const SurArgs none{};
-    const SurArgs &sa = sur ? *sur : none;
-    const double *sp = sur ? nullptr : src;
-    if (sur && P.tvd)
-        launch_tile_variant<float, true, true>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
-    else if (sur)
-        launch_tile_variant<float, true, false>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
-    else if (P.tvd)
-        launch_tile_variant<float, false, true>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
-    else
-        launch_tile_variant<float, false, false>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
+    launch_tile_variants<float, false>(s, TM, M, P, prim1, prim2, nullptr, nullptr, X1, X2, disp, cc1, cc2, src, sur, S,
+                                       step);
 }
 
 template <int K>

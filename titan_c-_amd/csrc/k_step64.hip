@@ -26,46 +26,35 @@ void launch_measure(hipStream_t s, const DevMesh &M, const double *X, const DevG
 }
 
 void launch_face_flux(hipStream_t s, int precision, const DevMesh &M, const DevParams &P, const double *prim,
-                      const DevGeom &G, const double *X1, const double *X2, StepScalars *S, double *flux,
-                      uint8_t *branch, long long step, double dt_override) {
+                      const double *scal, const DevGeom &G, const double *X1, const double *X2, StepScalars *S,
+                      double *flux, double *kflux, uint8_t *branch, long long step, double dt_override) {
     if (precision == 1) {
-        launch_face_flux_f32(s, M, P, prim, G, X1, X2, S, flux, branch, step, dt_override);
+        launch_face_flux_f32(s, M, P, prim, scal, G, X1, X2, S, flux, kflux, branch, step, dt_override);
         return;
     }
-    hipLaunchKernelGGL(k_face_flux<double>, dim3(grid_of(M.n_faces, 256)), dim3(256), 0, s, M, P, prim, G, X1, X2,
-                       S, flux, branch, step, dt_override);
-}
-
-void launch_cell_update(hipStream_t s, const DevMesh &M, const double *prim_old, const double *vol_old,
-                        const double *vol_new, const double *flux, const double *src, StepScalars *S,
-                        double *prim_new, double dt_override) {
-    hipLaunchKernelGGL(k_cell_update, dim3(grid_of(M.n_cells, 256)), dim3(256), 0, s, M, prim_old, vol_old, vol_new,
-                       flux, src, S, prim_new, dt_override);
+    hipLaunchKernelGGL(k_face_flux<double>, dim3(grid_of(M.n_faces, 256)), dim3(256), 0, s, M, P, prim, scal, G, X1,
+                       X2, S, flux, kflux, branch, step, dt_override);
 }
 
 void launch_step_tile(hipStream_t s, int precision, const TileMesh &TM, const DevMesh &M, const DevParams &P,
-                      const double *prim1, double *prim2, const double *X1, double *X2, const double *disp,
-                      const double *cc1, double *cc2, const double *src, const SurArgs *sur, StepScalars *S,
-                      long long step) {
+                      const double *prim1, double *prim2, const double *sc1, double *sc2, const double *X1, double *X2,
+                      const double *disp, const double *cc1, double *cc2, const double *src, const SurArgs *sur,
+                      StepScalars *S, long long step) {
+    if (P.n_scal) {
+        if (precision == 1) launch_step_tile_sc32(s, TM, M, P, prim1, prim2, sc1, sc2, X1, X2, disp, cc1, cc2, src, sur, S, step);
+        else launch_step_tile_sc64(s, TM, M, P, prim1, prim2, sc1, sc2, X1, X2, disp, cc1, cc2, src, sur, S, step);
+        return;
+    }
     if (precision == 1) {
         launch_step_tile_f32(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, src, sur, S, step);
         return;
     }
-    const SurArgs none{};
-    const SurArgs &sa = sur ? *sur : none;
-    const double *sp = sur ? nullptr : src;
-    if (sur && P.tvd)
-        launch_tile_variant<double, true, true>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
-    else if (sur)
-        launch_tile_variant<double, true, false>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
-    else if (P.tvd)
-        launch_tile_variant<double, false, true>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
-    else
-        launch_tile_variant<double, false, false>(s, TM, M, P, prim1, prim2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
+    launch_tile_variants<double, false>(s, TM, M, P, prim1, prim2, nullptr, nullptr, X1, X2, disp, cc1, cc2, src, sur, S,
+                                        step);
 }
 
-size_t step_tile_lds_bytes(const TileMesh &TM, bool moving) {
-    return tile_lds_bytes(TM.max_nodes, TM.tile_cells, TM.max_faces, moving);
+size_t step_tile_lds_bytes(const TileMesh &TM, bool moving, bool scal) {
+    return tile_lds_bytes(TM.max_nodes, TM.tile_cells, TM.max_faces, moving, scal);
 }
 
 // ---- standalone Riemann batch ----------------------------------------------------------------

@@ -11,27 +11,34 @@ void launch_begin_step(hipStream_t s, StepScalars *S, long long n, const DevPara
 void launch_motion(hipStream_t s, int64_t n3, const double *x_old, const double *disp, const StepScalars *S,
                    double *x_new);
 void launch_measure(hipStream_t s, const DevMesh &M, const double *X, const DevGeom &G, int gcl);
+// every face: flux [Nf*9]; with P.n_scal, scal [Nc*2] (nullable otherwise) gives kflux [Nf*2]
 void launch_face_flux(hipStream_t s, int precision, const DevMesh &M, const DevParams &P, const double *prim,
-                      const DevGeom &G, const double *X1, const double *X2, StepScalars *S, double *flux,
-                      uint8_t *branch, long long step, double dt_override);
+                      const double *scal, const DevGeom &G, const double *X1, const double *X2, StepScalars *S,
+                      double *flux, double *kflux, uint8_t *branch, long long step, double dt_override);
 void launch_face_flux_f32(hipStream_t s, const DevMesh &M, const DevParams &P, const double *prim,
-                          const DevGeom &G, const double *X1, const double *X2, StepScalars *S, double *flux,
-                          uint8_t *branch, long long step, double dt_override);
-void launch_cell_update(hipStream_t s, const DevMesh &M, const double *prim_old, const double *vol_old,
-                        const double *vol_new, const double *flux, const double *src, StepScalars *S,
-                        double *prim_new, double dt_override);
+                          const double *scal, const DevGeom &G, const double *X1, const double *X2, StepScalars *S,
+                          double *flux, double *kflux, uint8_t *branch, long long step, double dt_override);
 
 // fused step over tiles (tile_kernel.h); precision = TITAN_PREC_*.  sur (nullable): surrogate
-// source fused into the update (then src is ignored and launch_cx_fixup must follow).
+// source fused into the update (then src is ignored and launch_cx_fixup must follow).  With
+// P.n_scal the passive scalars go sc1 -> sc2 (nullable otherwise).
 void launch_step_tile(hipStream_t s, int precision, const TileMesh &TM, const DevMesh &M, const DevParams &P,
-                      const double *prim1, double *prim2, const double *X1, double *X2, const double *disp,
-                      const double *cc1, double *cc2, const double *src, const SurArgs *sur, StepScalars *S,
-                      long long step);
+                      const double *prim1, double *prim2, const double *sc1, double *sc2, const double *X1, double *X2,
+                      const double *disp, const double *cc1, double *cc2, const double *src, const SurArgs *sur,
+                      StepScalars *S, long long step);
+void launch_step_tile_sc64(hipStream_t s, const TileMesh &TM, const DevMesh &M, const DevParams &P,
+                           const double *prim1, double *prim2, const double *sc1, double *sc2, const double *X1,
+                           double *X2, const double *disp, const double *cc1, double *cc2, const double *src,
+                           const SurArgs *sur, StepScalars *S, long long step);
+void launch_step_tile_sc32(hipStream_t s, const TileMesh &TM, const DevMesh &M, const DevParams &P,
+                           const double *prim1, double *prim2, const double *sc1, double *sc2, const double *X1,
+                           double *X2, const double *disp, const double *cc1, double *cc2, const double *src,
+                           const SurArgs *sur, StepScalars *S, long long step);
 void launch_step_tile_f32(hipStream_t s, const TileMesh &TM, const DevMesh &M, const DevParams &P,
                           const double *prim1, double *prim2, const double *X1, double *X2, const double *disp,
                           const double *cc1, double *cc2, const double *src, const SurArgs *sur, StepScalars *S,
                           long long step);
-size_t step_tile_lds_bytes(const TileMesh &TM, bool moving);
+size_t step_tile_lds_bytes(const TileMesh &TM, bool moving, bool scal);
 
 // fitted surfaces (k_fit.hip): surface faces and nodes of layer X1 -> surface nodes of X2
 // (time > 0: explicit step length, else S->dt); then every ray's other nodes of X
@@ -59,9 +66,10 @@ void launch_cx_exact(hipStream_t s, int64_t n, const double *in10, int nodes, do
 // per-cell exact source from the plasma state and the neutral moments [Nc*5]
 void launch_cx_cells(hipStream_t s, int64_t nc, const double *prim, const double *neutrals, int nodes,
                      double *src);
-// Calc_sourse_MF_Bera plasma source per cell from the state and hfluid [Nc * n_h * 5]
-void launch_bera_cells(hipStream_t s, int64_t nc, const double *prim, const double *hfluid, int n_h, double a2,
-                       double ddp, double *src);
+// Calc_sourse_MF_Bera plasma source per cell from the state, the passive scalars scal [Nc*2]
+// (nullable: Q = rho_He = 0) with the cell codes' zones, and hfluid [Nc * n_h * 5]
+void launch_bera_cells(hipStream_t s, int64_t nc, const double *prim, const double *scal, const uint32_t *cell_code,
+                       const double *hfluid, int n_h, double a2, double ddp, double *src);
 // exact source + update of the cells the surrogate step listed as out of its training box
 void launch_cx_fixup(hipStream_t s, const DevMesh &M, const double *prim1, const double *neutrals, int nodes,
                      const int32_t *list, const double *stash, int64_t cap, StepScalars *S, double *prim2);

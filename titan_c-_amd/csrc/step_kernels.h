@@ -4,7 +4,8 @@
 //   Setka::Calculating_measure (Setka.cpp:2465-2502)  -> k_cell_center, k_face_geom, k_cell_volume
 //   Culc_Velocity_surface + Luch::dvigenie (prescribed) -> k_motion
 //   face loop, Setka_physics.cpp:1634-1662           -> k_face_flux<T> (+ wave-shuffle dt min)
-//   cell loop, Setka_physics.cpp:1671-2027           -> k_cell_update (CSR gather, FP64)
+//   cell loop, Setka_physics.cpp:1671-2027           -> cell_update_state, scalar_update (called by
+//                                                       the fused tile kernel, tile_kernel.h)
 #pragma once
 #include <hip/hip_runtime.h>
 #include "device_mesh.h"
@@ -315,15 +316,28 @@ __device__ __forceinline__ void load_state(const double *__restrict__ prim, int6
 }
 
 // Returns false on an unknown boundary code.  cell_center: centres of the layer of prim; Gc: the
-// face centre at that layer.
+// face centre at that layer.  K = kScal: the passive scalars of the same layer (scal [Nc*2]) give
+// kL, kR -- the cell values, the boundary state's, or rho's reconstruction (Zone_1 x r1^2/rr^2,
+// Setka_gran_snos.cpp:848-852; TVD as rho).
+template <int K = 0>
 __device__ __forceinline__ bool face_states_at(const DevMesh &M, const DevParams &P, int64_t f, uint32_t bc,
                                                const double *__restrict__ prim,
                                                const double *__restrict__ cell_center, const double Gc[3],
-                                               const double n[3], double qL[8], double qR[8]) {
+                                               const double n[3], double qL[8], double qR[8],
+                                               const double *__restrict__ scal = nullptr, double *kL = nullptr,
+                                               double *kR = nullptr) {
     const int64_t A = M.face_cells[2 * f];
     double sA[8];
     load_state(prim, A, sA);
+    double kA[K > 0 ? K : 1];
+#pragma unroll
+    for (int j = 0; j < K; j++) kA[j] = scal[K * A + j];
     if (bc != 0) {
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            kL[j] = kA[j];
+            kR[j] = (bc == 1 || bc == 2) ? M.bscal[K * (int64_t)M.face_bidx[f] + j] : kA[j];
+        }
         if (bc == 3) {  // Outer_Soft: plasma_condition 1
 #pragma unroll
             for (int i = 0; i < 8; i++) qL[i] = sA[i];
@@ -353,50 +367,70 @@ __device__ __forceinline__ bool face_states_at(const DevMesh &M, const DevParams
     const int64_t B = M.face_cells[2 * f + 1];
     double sB[8];
     load_state(prim, B, sB);
+    double kB[K > 0 ? K : 1];
+#pragma unroll
+    for (int j = 0; j < K; j++) kB[j] = scal[K * B + j];
     const uint32_t zA = (M.cell_code[A] >> 8) & 15u, zB = (M.cell_code[B] >> 8) & 15u;
     if (P.tvd) {
         const int64_t AA = M.face_tvd[2 * f], BB = M.face_tvd[2 * f + 1];
         if (AA >= 0) {  // second-order face (both neighbours present)
-            double sAA[8], sBB[8];
+            double sAA[8], sBB[8], kAA[K > 0 ? K : 1], kBB[K > 0 ? K : 1];
             load_state(prim, AA, sAA);
             load_state(prim, BB, sBB);
+#pragma unroll
+            for (int j = 0; j < K; j++) { kAA[j] = scal[K * AA + j]; kBB[j] = scal[K * BB + j]; }
             const double *cc = cell_center;
             const double cA[3] = {cc[3 * A], cc[3 * A + 1], cc[3 * A + 2]};
             const double cB[3] = {cc[3 * B], cc[3 * B + 1], cc[3 * B + 2]};
             const double cAA[3] = {cc[3 * AA], cc[3 * AA + 1], cc[3 * AA + 2]};
             const double cBB[3] = {cc[3 * BB], cc[3 * BB + 1], cc[3 * BB + 2]};
-            tvd_states((M.cell_code[AA] >> 8) & 15u, zA, zB, (M.cell_code[BB] >> 8) & 15u, sAA, sA, sB, sBB, cAA,
-                       cA, cB, cBB, Gc, P.gamma, qL, qR);
+            tvd_states<K>((M.cell_code[AA] >> 8) & 15u, zA, zB, (M.cell_code[BB] >> 8) & 15u, sAA, sA, sB, sBB, cAA,
+                          cA, cB, cBB, Gc, P.gamma, qL, qR, kAA, kA, kB, kBB, kL, kR);
             return true;
         }
     }
     if (zA == 1u || zB == 1u) {
+        const double rr = sqrt((Gc[0] * Gc[0] + Gc[1] * Gc[1]) + Gc[2] * Gc[2]);
         if (zA == 1u) {
             const double Ac[3] = {cell_center[3 * A], cell_center[3 * A + 1], cell_center[3 * A + 2]};
             zone1_state(sA, Ac, Gc, P.gamma, qL);
+            const double r1 = sqrt((Ac[0] * Ac[0] + Ac[1] * Ac[1]) + Ac[2] * Ac[2]);
+#pragma unroll
+            for (int j = 0; j < K; j++) kL[j] = kA[j] * (r1 * r1) / (rr * rr);
         } else {
 #pragma unroll
             for (int i = 0; i < 8; i++) qL[i] = sA[i];
+#pragma unroll
+            for (int j = 0; j < K; j++) kL[j] = kA[j];
         }
         if (zB == 1u) {
             const double Bc[3] = {cell_center[3 * B], cell_center[3 * B + 1], cell_center[3 * B + 2]};
             zone1_state(sB, Bc, Gc, P.gamma, qR);
+            const double r1 = sqrt((Bc[0] * Bc[0] + Bc[1] * Bc[1]) + Bc[2] * Bc[2]);
+#pragma unroll
+            for (int j = 0; j < K; j++) kR[j] = kB[j] * (r1 * r1) / (rr * rr);
         } else {
 #pragma unroll
             for (int i = 0; i < 8; i++) qR[i] = sB[i];
+#pragma unroll
+            for (int j = 0; j < K; j++) kR[j] = kB[j];
         }
     } else {
 #pragma unroll
         for (int i = 0; i < 8; i++) { qL[i] = sA[i]; qR[i] = sB[i]; }
+#pragma unroll
+        for (int j = 0; j < K; j++) { kL[j] = kA[j]; kR[j] = kB[j]; }
     }
     return true;
 }
 
+template <int K = 0>
 __device__ __forceinline__ bool face_states(const DevMesh &M, const DevParams &P, int64_t f, uint32_t bc,
                                             const double *__restrict__ prim, const DevGeom &G, const double n[3],
-                                            double qL[8], double qR[8]) {
+                                            double qL[8], double qR[8], const double *__restrict__ scal = nullptr,
+                                            double *kL = nullptr, double *kR = nullptr) {
     const double Gc[3] = {G.face_center[3 * f], G.face_center[3 * f + 1], G.face_center[3 * f + 2]};
-    return face_states_at(M, P, f, bc, prim, G.cell_center, Gc, n, qL, qR);
+    return face_states_at<K>(M, P, f, bc, prim, G.cell_center, Gc, n, qL, qR, scal, kL, kR);
 }
 
 __device__ __forceinline__ void report_error(StepScalars *S, ErrRank rank, long long index) {
@@ -449,64 +483,70 @@ __device__ __forceinline__ void block_min_count_to(double v, unsigned long long 
 }
 
 // ---- face flux: Setka::Culc_Gran_Potok for every face -------------------------------------------
-// flux[f*9 + 0..7] = chlld flux * area, flux[f*9 + 8] = PdivB.  T = double: parity arithmetic;
-// T = float: chlld_mixed (FP32 wave structure, FP64 upwind flux), states, geometry, area scaling
-// in FP64.
+// flux[f*9 + 0..7] = chlld flux * area, flux[f*9 + 8] = PdivB; with K = kScal passive scalars,
+// kflux[f*2 + j] = their konvect flux * area.  T = double: parity arithmetic; T = float:
+// chlld_mixed (FP32 wave structure, FP64 upwind flux), states, geometry, area scaling in FP64.
+template <typename T, int K>
+__device__ __forceinline__ double face_flux_one(const DevMesh &M, const DevParams &P, const double *__restrict__ prim,
+                                                const double *__restrict__ scal, const DevGeom &G,
+                                                const double *__restrict__ X1, const double *__restrict__ X2,
+                                                StepScalars *S, double *__restrict__ flux, double *__restrict__ kflux,
+                                                uint8_t *__restrict__ branch, double dt, int64_t f) {
+    double cand = 10000000.0;
+    const uint32_t code = M.face_code[f];
+    const uint32_t bc = code & 15u, method = (code >> 4) & 15u, flags = (code >> 8) & 255u;
+    const double n[3] = {G.face_normal[3 * f], G.face_normal[3 * f + 1], G.face_normal[3 * f + 2]};
+    const double area = G.face_area[f];
+    double w = 0.0;
+    if (P.move_mesh) {
+        w = P.geom_mode ? face_velocity_gcl(M, f, X1, X2, n, area, dt) : face_velocity_ref(M, f, X1, X2, n, dt);
+    }
+    double qL[8], qR[8], kL[K > 0 ? K : 1], kR[K > 0 ? K : 1];
+    const bool okbc = face_states<K>(M, P, f, bc, prim, G, n, qL, qR, scal, kL, kR);
+    RiemannOut<double, K> o;
+    if (sizeof(T) == sizeof(double)) {
+        chlld<double, K>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, kL, kR, o);
+    } else {
+        chlld_mixed<K>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, kL, kR, o);
+        if (o.ok < 0) {
+            chlld_fp64_face<K>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, kL, kR, &o);
+            atomicAdd(&S->n_fp64_faces, 1ull);
+        }
+    }
+    const bool ok = o.ok;
+    if (!ok || !okbc) report_error(S, kErrFaceRiemann, M.face_ref[f]);
+    if (bc == 0 && ok) {
+        const int64_t A = M.face_cells[2 * f], B = M.face_cells[2 * f + 1];
+        const double d0 = G.cell_center[3 * A] - G.cell_center[3 * B];
+        const double d1 = G.cell_center[3 * A + 1] - G.cell_center[3 * B + 1];
+        const double d2 = G.cell_center[3 * A + 2] - G.cell_center[3 * B + 2];
+        const double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2) / 2.0;
+        const double dnt = P.kfl * dist / (smax(fabs(o.dsl), fabs(o.dsr)) + fabs(w));
+        if (dnt < cand) cand = dnt;
+    }
+    double *fo = flux + 9 * f;
+#pragma unroll
+    for (int i = 0; i < 8; i++) fo[i] = o.q[i] * area;
+    fo[8] = 0.5 * spf(n[0], n[1], n[2], qL[5] + qR[5], qL[6] + qR[6], qL[7] + qR[7]) * area;
+#pragma unroll
+    for (int j = 0; j < K; j++) kflux[K * f + j] = o.k[j] * area;
+    if (branch) branch[f] = (uint8_t)o.branch;
+    return cand;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_face_flux(DevMesh M, DevParams P, const double *__restrict__ prim,
-                                                   DevGeom G, const double *__restrict__ X1,
-                                                   const double *__restrict__ X2, StepScalars *S,
-                                                   double *__restrict__ flux, uint8_t *__restrict__ branch,
+                                                   const double *__restrict__ scal, DevGeom G,
+                                                   const double *__restrict__ X1, const double *__restrict__ X2,
+                                                   StepScalars *S, double *__restrict__ flux,
+                                                   double *__restrict__ kflux, uint8_t *__restrict__ branch,
                                                    long long step, double dt_override) {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double cand = 10000000.0;
     if (f < M.n_faces) {
-        const uint32_t code = M.face_code[f];
-        const uint32_t bc = code & 15u, method = (code >> 4) & 15u, flags = (code >> 8) & 255u;
-        const double n[3] = {G.face_normal[3 * f], G.face_normal[3 * f + 1], G.face_normal[3 * f + 2]};
-        const double area = G.face_area[f];
         const double dt = (dt_override > 0.0) ? dt_override : S->dt;
-        double w = 0.0;
-        if (P.move_mesh) {
-            w = P.geom_mode ? face_velocity_gcl(M, f, X1, X2, n, area, dt) : face_velocity_ref(M, f, X1, X2, n, dt);
-        }
-        double qL[8], qR[8];
-        const bool okbc = face_states(M, P, f, bc, prim, G, n, qL, qR);
-        double q[8], dsl, dsr;
-        int br;
-        bool ok;
-        if (sizeof(T) == sizeof(double)) {
-            RiemannOut<double, 0> o;
-            chlld<double, 0>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, nullptr, nullptr, o);
-#pragma unroll
-            for (int i = 0; i < 8; i++) q[i] = o.q[i];
-            dsl = o.dsl; dsr = o.dsr; br = o.branch; ok = o.ok;
-        } else {
-            RiemannOut<double, 0> o;
-            chlld_mixed<0>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, nullptr, nullptr, o);
-            if (o.ok < 0) {
-                chlld_fp64_face<0>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, nullptr, nullptr, &o);
-                atomicAdd(&S->n_fp64_faces, 1ull);
-            }
-#pragma unroll
-            for (int i = 0; i < 8; i++) q[i] = o.q[i];
-            dsl = o.dsl; dsr = o.dsr; br = o.branch; ok = o.ok;
-        }
-        if (!ok || !okbc) report_error(S, kErrFaceRiemann, M.face_ref[f]);
-        if (bc == 0 && ok) {
-            const int64_t A = M.face_cells[2 * f], B = M.face_cells[2 * f + 1];
-            const double d0 = G.cell_center[3 * A] - G.cell_center[3 * B];
-            const double d1 = G.cell_center[3 * A + 1] - G.cell_center[3 * B + 1];
-            const double d2 = G.cell_center[3 * A + 2] - G.cell_center[3 * B + 2];
-            const double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2) / 2.0;
-            const double dnt = P.kfl * dist / (smax(fabs(dsl), fabs(dsr)) + fabs(w));
-            if (dnt < cand) cand = dnt;
-        }
-        double *o = flux + 9 * f;
-#pragma unroll
-        for (int i = 0; i < 8; i++) o[i] = q[i] * area;
-        o[8] = 0.5 * spf(n[0], n[1], n[2], qL[5] + qR[5], qL[6] + qR[6], qL[7] + qR[7]) * area;
-        if (branch) branch[f] = (uint8_t)br;
+        if (P.n_scal) cand = face_flux_one<T, kScal>(M, P, prim, scal, G, X1, X2, S, flux, kflux, branch, dt, f);
+        else cand = face_flux_one<T, 0>(M, P, prim, scal, G, X1, X2, S, flux, kflux, branch, dt, f);
     }
     block_min_to(cand, &S->dtmin_bits[step & 1]);
 }
@@ -575,33 +615,23 @@ __device__ __forceinline__ void cell_update_one(const DevMesh &M, uint32_t code,
     cell_update_state(M, code, c, s, Volume, Volume2, Pf, src, time, S, prim_new);
 }
 
-#ifndef TITAN_FLUX_ONLY
-__global__ void __launch_bounds__(256) k_cell_update(DevMesh M, const double *__restrict__ prim_old,
-                                                     const double *__restrict__ vol_old,
-                                                     const double *__restrict__ vol_new,
-                                                     const double *__restrict__ flux,
-                                                     const double *__restrict__ src, StepScalars *S,
-                                                     double *__restrict__ prim_new, double dt_override) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= M.n_cells) return;
-    const double time = (dt_override > 0.0) ? dt_override : S->dt;
-    const uint32_t code = M.cell_code[c];
-    double Pf[9];
-#pragma unroll
-    for (int i = 0; i < 9; i++) Pf[i] = 0.0;
-#pragma unroll
-    for (int j = 0; j < 6; j++) {
-        const int64_t f = M.cell_faces[6 * c + j];
-        const bool pos = (code >> j) & 1u;
-        const double *fl = flux + 9 * f;
-#pragma unroll
-        for (int i = 0; i < 9; i++) {
-            const double v = fl[i];
-            Pf[i] += pos ? v : -v;
-        }
-    }
-    cell_update_one(M, code, c, prim_old, vol_old[c], vol_new[c], Pf, src ? src + 5 * c : nullptr, time, S, prim_new);
+// The passive scalars' part of the cell loop (Setka_physics.cpp:1781-1817, 2004-2016): Q and rho_He
+// advected like rho (Q3 = Q V/V' - dt P_Q/V'), rho_He clipped at 0 and, when above the new
+// density (after the density floor), set to a tenth of it.  k: the cell's (Q, rho_He); Pk: their
+// signed flux sums; rho, Prho: the cell's density and its flux sum, from which the new density is
+// formed as the update forms it.  Independent of the sources.
+__device__ __forceinline__ void scalar_update(const double k[kScal], double rho, double Prho, const double Pk[kScal],
+                                              double Volume, double Volume2, double time, double out[kScal]) {
+    const double yV = 1.0 / Volume2;
+    auto perV = [&](double a) { return div_by(a, Volume2, yV); };
+    double rho3 = perV(rho * Volume) - perV(time * Prho);
+    if (rho3 < 1e-7) rho3 = 0.1;
+    const double Q3 = perV(k[0] * Volume) - perV(time * Pk[0]);
+    double He3 = perV(k[1] * Volume) - perV(time * Pk[1]);
+    if (He3 < 0.0) He3 = 0.0;
+    if (He3 > rho3) He3 = rho3 * 0.1;
+    out[0] = Q3;
+    out[1] = He3;
 }
-#endif  // TITAN_FLUX_ONLY
 
 }  // namespace titan

@@ -98,13 +98,15 @@ struct TileLds {
     double *c1, *c2;   // [T*3]
     double *z1;        // [T*10] Zone_1 cell data (r1, r1^2g, trig e, trig p)
     double *flux;      // [max_faces*9]
+    double *kflux;     // [max_faces*kScal] passive scalar fluxes (only when carried)
     uint16_t *fnodes;  // [max_faces*4] local nodes of each face slot
 };
 
 __host__ __device__ inline size_t tile_flux_doubles(int max_faces) { return 9 * (size_t)max_faces; }
 
-// LDS carve-up; the second node layer and second centres exist only on a moving mesh.
-__device__ __forceinline__ TileLds tile_lds(double *base, const TileMesh &TM, bool moving) {
+// LDS carve-up; the second node layer and second centres exist only on a moving mesh, the passive
+// scalar fluxes only when the step carries them.
+__device__ __forceinline__ TileLds tile_lds(double *base, const TileMesh &TM, bool moving, bool scal) {
     TileLds L;
     double *p = base;
     L.x1 = p; p += 3 * TM.max_nodes;
@@ -112,6 +114,8 @@ __device__ __forceinline__ TileLds tile_lds(double *base, const TileMesh &TM, bo
     L.z1 = p; p += 10 * TM.tile_cells;
     p += (p - base) & 1;  // 16-byte alignment of the flux slots (they also hold 16-byte vectors)
     L.flux = p; p += tile_flux_doubles(TM.max_faces);
+    L.kflux = p;
+    if (scal) p += kScal * (size_t)TM.max_faces;
     if (moving) {
         L.x2 = p; p += 3 * TM.max_nodes;
         L.c2 = p; p += 3 * TM.tile_cells;
@@ -123,8 +127,9 @@ __device__ __forceinline__ TileLds tile_lds(double *base, const TileMesh &TM, bo
     return L;
 }
 
-inline size_t tile_lds_bytes(int max_nodes, int tile_cells, int max_faces, bool moving) {
+inline size_t tile_lds_bytes(int max_nodes, int tile_cells, int max_faces, bool moving, bool scal) {
     const size_t nd = 3 * (size_t)max_nodes + 13 * (size_t)tile_cells + 1 + tile_flux_doubles(max_faces) +
+                      (scal ? kScal * (size_t)max_faces : 0) +
                       (moving ? 3 * (size_t)max_nodes + 3 * (size_t)tile_cells : 0);
     return sizeof(double) * nd + sizeof(uint16_t) * 4 * (size_t)max_faces;
 }
@@ -209,18 +214,33 @@ __device__ __forceinline__ double tile_volume(const double *X, const double cc[3
 // returned, for the FP64 pass over marked slots.
 // The face's record and the states of its cells (sB only for an interior face) come from the
 // caller, which issues those loads early (face_fetch).
-__device__ __forceinline__ void face_fetch(const ulonglong2 &rec, const double *__restrict__ prim1, double sA[8],
-                                           double sB[8]) {
-    load_state(prim1, (int32_t)(rec.y & 0xffffffffu), sA);
-    if (((rec.x >> 48) & 15u) == 0u) load_state(prim1, (int32_t)(rec.y >> 32), sB);
+template <bool SC>
+__device__ __forceinline__ void face_fetch(const ulonglong2 &rec, const double *__restrict__ prim1,
+                                           const double *__restrict__ sc1, double sA[8], double sB[8],
+                                           double kA[kScal], double kB[kScal]) {
+    const int64_t a = (int32_t)(rec.y & 0xffffffffu), b = (int32_t)(rec.y >> 32);
+    load_state(prim1, a, sA);
+    if constexpr (SC) {
+        const double2 v = *reinterpret_cast<const double2 *>(sc1 + kScal * a);
+        kA[0] = v.x; kA[1] = v.y;
+    }
+    if (((rec.x >> 48) & 15u) == 0u) {
+        load_state(prim1, b, sB);
+        if constexpr (SC) {
+            const double2 v = *reinterpret_cast<const double2 *>(sc1 + kScal * b);
+            kB[0] = v.x; kB[1] = v.y;
+        }
+    }
 }
 
-template <typename RT, bool TVD>
+template <typename RT, bool TVD, bool SC>
 __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const double sA[8], const double sB[8],
-                                          const TileMesh &TM, const DevMesh &M, const DevParams &P,
-                                          const TileLds &L, int fb, int64_t c0, int ncl,
-                                          const double *__restrict__ prim1, const double *__restrict__ cc1g,
-                                          bool moving, int gcl, double dt, StepScalars *S, double &cand) {
+                                          const double kA[kScal], const double kB[kScal], const TileMesh &TM,
+                                          const DevMesh &M, const DevParams &P, const TileLds &L, int fb, int64_t c0,
+                                          int ncl, const double *__restrict__ prim1, const double *__restrict__ sc1,
+                                          const double *__restrict__ cc1g, bool moving, int gcl, double dt,
+                                          StepScalars *S, double &cand) {
+    constexpr int K = SC ? kScal : 0;
     const int ln[4] = {(int)(rec.x & 0xfff), (int)((rec.x >> 12) & 0xfff), (int)((rec.x >> 24) & 0xfff),
                        (int)((rec.x >> 36) & 0xfff)};
     const uint32_t code = (uint32_t)(rec.x >> 48);
@@ -235,6 +255,7 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
         L.flux[9 * f + 8] = 0.0;  // not a marked slot for the mixed solver's FP64 pass
         return true;
     }
+    double kL[K > 0 ? K : 1], kR[K > 0 ? K : 1];
     double x1[4][3];
 #pragma unroll
     for (int i = 0; i < 4; i++) { x1[i][0] = L.x1[3 * ln[i]]; x1[i][1] = L.x1[3 * ln[i] + 1]; x1[i][2] = L.x1[3 * ln[i] + 2]; }
@@ -288,6 +309,11 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
         }
 #pragma unroll
         for (int i = 0; i < 8; i++) { qL[i] = l[i]; qR[i] = r[i]; }
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            kL[j] = kA[j];
+            kR[j] = (bc == 1 || bc == 2) ? M.bscal[kScal * (int64_t)(-b - 2) + j] : kA[j];
+        }
     } else {
         const bool bloc = (b >= c0 && b < c0 + ncl);
         if (bloc) { cb[0] = L.c1[3 * (b - c0)]; cb[1] = L.c1[3 * (b - c0) + 1]; cb[2] = L.c1[3 * (b - c0) + 2]; }
@@ -300,32 +326,44 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
         int2 tv = {-1, -1};
         if constexpr (TVD) tv = TM.face_tvd[fb + f];
         if (TVD && tv.x >= 0) {  // second order: the line AA - A -|- B - BB (both neighbours present)
-            double sAA[8], sBB[8];
+            double sAA[8], sBB[8], kAA[K > 0 ? K : 1], kBB[K > 0 ? K : 1];
             load_state(prim1, tv.x, sAA);
             load_state(prim1, tv.y, sBB);
+#pragma unroll
+            for (int j = 0; j < K; j++) { kAA[j] = sc1[K * (int64_t)tv.x + j]; kBB[j] = sc1[K * (int64_t)tv.y + j]; }
             const double cAA[3] = {cc1g[3 * (int64_t)tv.x], cc1g[3 * (int64_t)tv.x + 1], cc1g[3 * (int64_t)tv.x + 2]};
             const double cBB[3] = {cc1g[3 * (int64_t)tv.y], cc1g[3 * (int64_t)tv.y + 1], cc1g[3 * (int64_t)tv.y + 2]};
-            tvd_states((M.cell_code[tv.x] >> 8) & 15u, zoneA, zoneB, (M.cell_code[tv.y] >> 8) & 15u, sAA, tA, tB, sBB,
-                       cAA, ca, cb, cBB, G, P.gamma, qL, qR);
+            tvd_states<K>((M.cell_code[tv.x] >> 8) & 15u, zoneA, zoneB, (M.cell_code[tv.y] >> 8) & 15u, sAA, tA, tB,
+                          sBB, cAA, ca, cb, cBB, G, P.gamma, qL, qR, kAA, kA, kB, kBB, kL, kR);
         } else if (zA || zB) {
             const Zone1Face<double> zf = zone1_face<double>(G, P.gamma);
             if (zA) {
                 const Zone1Cell<double> zc = aloc ? load_zone1<double>(L.z1 + 10 * (a - c0)) : zone1_cell<double>(ca, P.gamma);
                 zone1_apply<double>(tA, zc, zf, qL);
+#pragma unroll
+                for (int j = 0; j < K; j++) kL[j] = div_by(kA[j] * (zc.r1 * zc.r1), zf.rr2, zf.yrr2);
             } else {
 #pragma unroll
                 for (int i = 0; i < 8; i++) qL[i] = tA[i];
+#pragma unroll
+                for (int j = 0; j < K; j++) kL[j] = kA[j];
             }
             if (zB) {
                 const Zone1Cell<double> zc = bloc ? load_zone1<double>(L.z1 + 10 * (b - c0)) : zone1_cell<double>(cb, P.gamma);
                 zone1_apply<double>(tB, zc, zf, qR);
+#pragma unroll
+                for (int j = 0; j < K; j++) kR[j] = div_by(kB[j] * (zc.r1 * zc.r1), zf.rr2, zf.yrr2);
             } else {
 #pragma unroll
                 for (int i = 0; i < 8; i++) qR[i] = tB[i];
+#pragma unroll
+                for (int j = 0; j < K; j++) kR[j] = kB[j];
             }
         } else {
 #pragma unroll
             for (int i = 0; i < 8; i++) { qL[i] = tA[i]; qR[i] = tB[i]; }
+#pragma unroll
+            for (int j = 0; j < K; j++) { kL[j] = kA[j]; kR[j] = kB[j]; }
         }
     }
     // what the face needs after the solve (the PdivB flux and the CFL distance) is formed before
@@ -336,11 +374,11 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
         const double d0 = ca[0] - cb[0], d1 = ca[1] - cb[1], d2 = ca[2] - cb[2];
         dist = sqrt_n(d0 * d0 + d1 * d1 + d2 * d2) / 2.0;
     }
-    RiemannOut<double, 0> o;
+    RiemannOut<double, K> o;
     if constexpr (sizeof(RT) == sizeof(double)) {
-        chlld<double, 0>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, nullptr, nullptr, o);
+        chlld<double, K>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, kL, kR, o);
     } else {
-        chlld_mixed<0>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, nullptr, nullptr, o);
+        chlld_mixed<K>((int)method, P.n_disc, (int)flags, n[0], n[1], n[2], w, qL, qR, kL, kR, o);
         if (o.ok < 0) {  // ill-conditioned in FP32: marked for the FP64 pass
             L.flux[9 * f + 8] = __builtin_nan("");
             return false;
@@ -355,6 +393,8 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
 #pragma unroll
     for (int i = 0; i < 8; i++) fl[i] = o.q[i] * area;
     fl[8] = pdivb;
+#pragma unroll
+    for (int j = 0; j < K; j++) L.kflux[K * f + j] = o.k[j] * area;
     return true;
 }
 
@@ -364,10 +404,11 @@ __device__ __forceinline__ bool face_slot(int f, const ulonglong2 rec, const dou
 // operands are read from global memory (23 KB, cache-resident across the tiles): staging them in
 // LDS once per workgroup measured slower, its copy and two barriers costing more than the cache
 // reads they save.  The phase needs no barrier, and each wavefront proceeds on its own.
-template <typename RT>
+template <typename RT, bool SC>
 __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMesh &M, const DevParams &P,
                                                  const TileLds &L, int64_t c0, int ncl,
                                                  const double *__restrict__ prim1, double *__restrict__ prim2,
+                                                 const double *__restrict__ sc1, double *__restrict__ sc2,
                                                  const SurArgs &sur, double dt, int gcl, bool moving,
                                                  StepScalars *S) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -398,6 +439,7 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
     double Pf[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) Pf[i] = 0.0;
+    double Pk[kScal] = {0.0, 0.0};
     double Volume = 0.0, Volume2 = 0.0;
     if (mine) {
 #pragma unroll
@@ -408,6 +450,13 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
             for (int i = 0; i < 9; i++) {
                 const double v = fl[i];
                 Pf[i] += pos ? v : -v;
+            }
+            if constexpr (SC) {
+#pragma unroll
+                for (int i = 0; i < kScal; i++) {
+                    const double v = L.kflux[kScal * lf[j] + i];
+                    Pk[i] += pos ? v : -v;
+                }
             }
         }
         if (TM.vol_cached) {
@@ -446,6 +495,15 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
     const float y4 = __shfl(o[0], (lane & 31) + 32, 64);  // output 4 of this cell sits in lane l + 32
     if (!mine) return;
     if (moving || !TM.vol_cached) TM.vol[cg] = Volume2;
+    if constexpr (SC) {  // source-independent: every cell's scalars, the fallback cells' included
+        const double2 kv = *reinterpret_cast<const double2 *>(sc1 + kScal * cg);
+        double k3[kScal] = {kv.x, kv.y};
+        if (cell_in_area(code, P.area)) {
+            const double k[kScal] = {kv.x, kv.y};
+            scalar_update(k, st[0], Pf[0], Pk, Volume, Volume2, dt, k3);
+        }
+        *reinterpret_cast<double2 *>(sc2 + kScal * cg) = make_double2(k3[0], k3[1]);
+    }
     if (!cell_in_area(code, P.area)) {  // frozen: the state carries over to the new layer
         double4 *o = reinterpret_cast<double4 *>(prim2 + 8 * cg);
         o[0] = make_double4(st[0], st[1], st[2], st[3]);
@@ -475,14 +533,18 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
 // and cache-resident across the tiles).  Cells outside the training box are not updated here: their flux
 // sums and volumes go to sur.stash at a slot of S->n_fallback, and k_cx_fixup redoes their update
 // with the exact source.  SUR = false: src (nullable) holds a precomputed source per cell.
-template <typename RT, bool SUR, bool TVD>
+//
+// SC = true: the passive scalars Q and rho_He (sc1 -> sc2, [Nc*2]) are carried: reconstructed and
+// solved with the face states, summed and updated with the cell (scalar_update).
+template <typename RT, bool SUR, bool TVD, bool SC>
 __global__ void __launch_bounds__(kTileThreads, 2)
     k_step_tile(TileMesh TM, DevMesh M, DevParams P, const double *__restrict__ prim1, double *__restrict__ prim2,
-                const double *__restrict__ X1g, double *__restrict__ X2g, const double *__restrict__ disp,
-                const double *__restrict__ cc1g, double *__restrict__ cc2g, const double *__restrict__ src,
-                SurArgs sur, StepScalars *S, long long step) {
+                const double *__restrict__ sc1, double *__restrict__ sc2, const double *__restrict__ X1g,
+                double *__restrict__ X2g, const double *__restrict__ disp, const double *__restrict__ cc1g,
+                double *__restrict__ cc2g, const double *__restrict__ src, SurArgs sur, StepScalars *S,
+                long long step) {
     extern __shared__ __align__(16) double smem[];
-    const TileLds L = tile_lds(smem, TM, P.move_mesh != 0);
+    const TileLds L = tile_lds(smem, TM, P.move_mesh != 0, SC);
     // Workgroups go to the 8 XCDs round robin (block b on XCD b % 8), each XCD with its own L2.
     // Tile t(b) = (b % 8) * per + b / 8 gives every XCD a contiguous run of tiles, so neighbouring
     // tiles -- which share nodes and border-cell states -- are read through the same L2.
@@ -567,20 +629,22 @@ __global__ void __launch_bounds__(kTileThreads, 2)
         }
     }
     // the states of this thread's first face, in flight across the barrier
-    double sA0[8], sB0[8];
-    if (tid < nf) face_fetch(rec0, prim1, sA0, sB0);
+    double sA0[8], sB0[8], kA0[kScal], kB0[kScal];
+    if (tid < nf) face_fetch<SC>(rec0, prim1, sc1, sA0, sB0, kA0, kB0);
     __syncthreads();
 
     // phase 3: faces of the tile (at most two per thread: tiles are cut at 2 x kTileThreads faces)
     double cand = 10000000.0;
     bool pending = false;
     if (tid < nf)  // the first face, with its prefetched record and states
-        pending = !face_slot<RT, TVD>(tid, rec0, sA0, sB0, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
+        pending = !face_slot<RT, TVD, SC>(tid, rec0, sA0, sB0, kA0, kB0, TM, M, P, L, fb, c0, ncl, prim1, sc1, cc1g,
+                                          moving, gcl, dt, S, cand);
     for (int f = tid + kTileThreads; f < nf; f += kTileThreads) {
         const ulonglong2 rec = TM.faces[fb + f];
-        double sA[8], sB[8];
-        face_fetch(rec, prim1, sA, sB);
-        pending |= !face_slot<RT, TVD>(f, rec, sA, sB, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
+        double sA[8], sB[8], kA[kScal], kB[kScal];
+        face_fetch<SC>(rec, prim1, sc1, sA, sB, kA, kB);
+        pending |= !face_slot<RT, TVD, SC>(f, rec, sA, sB, kA, kB, TM, M, P, L, fb, c0, ncl, prim1, sc1, cc1g, moving,
+                                           gcl, dt, S, cand);
     }
     // the update phase's loads of this thread's cell, in flight across the barrier
     uint32_t code4 = 0u;
@@ -607,9 +671,10 @@ __global__ void __launch_bounds__(kTileThreads, 2)
         for (int f = tid; f < nf; f += blockDim.x) {
             if (__builtin_isnan(L.flux[9 * f + 8])) {
                 const ulonglong2 rec = TM.faces[fb + f];
-                double sA[8], sB[8];
-                face_fetch(rec, prim1, sA, sB);
-                face_slot<double, TVD>(f, rec, sA, sB, TM, M, P, L, fb, c0, ncl, prim1, cc1g, moving, gcl, dt, S, cand);
+                double sA[8], sB[8], kA[kScal], kB[kScal];
+                face_fetch<SC>(rec, prim1, sc1, sA, sB, kA, kB);
+                face_slot<double, TVD, SC>(f, rec, sA, sB, kA, kB, TM, M, P, L, fb, c0, ncl, prim1, sc1, cc1g, moving,
+                                           gcl, dt, S, cand);
                 n_resolved++;
             }
         }
@@ -617,7 +682,7 @@ __global__ void __launch_bounds__(kTileThreads, 2)
     }
 
     if constexpr (SUR) {
-        surrogate_update<RT>(TM, M, P, L, c0, ncl, prim1, prim2, sur, dt, gcl, moving, S);
+        surrogate_update<RT, SC>(TM, M, P, L, c0, ncl, prim1, prim2, sc1, sc2, sur, dt, gcl, moving, S);
         block_min_count_to(cand, n_resolved, &S->dtmin_bits[step & 1], &S->n_fp64_faces);
         return;
     }
@@ -699,6 +764,7 @@ __global__ void __launch_bounds__(kTileThreads, 2)
             double4 *o = reinterpret_cast<double4 *>(prim2 + 8 * cg);
             o[0] = make_double4(s[0], s[1], s[2], s[3]);
             o[1] = make_double4(s[4], s[5], s[6], s[7]);
+            if constexpr (SC) *reinterpret_cast<double2 *>(sc2 + kScal * cg) = *reinterpret_cast<const double2 *>(sc1 + kScal * cg);
             continue;
         }
         double Pf[9];
@@ -714,6 +780,23 @@ __global__ void __launch_bounds__(kTileThreads, 2)
                 Pf[i] += pos ? v : -v;
             }
         }
+        if constexpr (SC) {
+            double Pk[kScal] = {0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                const bool pos = (code >> j) & 1u;
+#pragma unroll
+                for (int i = 0; i < kScal; i++) {
+                    const double v = L.kflux[kScal * lf[j] + i];
+                    Pk[i] += pos ? v : -v;
+                }
+            }
+            const double2 kv = *reinterpret_cast<const double2 *>(sc1 + kScal * cg);
+            const double k[kScal] = {kv.x, kv.y};
+            double k3[kScal];
+            scalar_update(k, s[0], Pf[0], Pk, Volume, Volume2, dt, k3);
+            *reinterpret_cast<double2 *>(sc2 + kScal * cg) = make_double2(k3[0], k3[1]);
+        }
         cell_update_state(M, code, cg, s, Volume, Volume2, Pf, src ? src + 5 * cg : nullptr, dt, S, prim2);
     }
     block_min_count_to(cand, n_resolved, &S->dtmin_bits[step & 1], &S->n_fp64_faces);
@@ -721,23 +804,42 @@ __global__ void __launch_bounds__(kTileThreads, 2)
 
 // Launch of one k_step_tile instantiation; opens the kernel up to the tile's dynamic LDS size
 // once (hipFuncSetAttribute is a no-op below the default limit but required above it).
-template <typename RT, bool SUR, bool TVD>
+template <typename RT, bool SUR, bool TVD, bool SC>
 inline void launch_tile_variant(hipStream_t s, const TileMesh &TM, const DevMesh &M, const DevParams &P,
-                                const double *prim1, double *prim2, const double *X1, double *X2,
-                                const double *disp, const double *cc1, double *cc2, const double *src,
-                                const SurArgs &sur, StepScalars *S, long long step) {
-    const size_t lds = tile_lds_bytes(TM.max_nodes, TM.tile_cells, TM.max_faces, P.move_mesh != 0);
+                                const double *prim1, double *prim2, const double *sc1, double *sc2,
+                                const double *X1, double *X2, const double *disp, const double *cc1, double *cc2,
+                                const double *src, const SurArgs &sur, StepScalars *S, long long step) {
+    const size_t lds = tile_lds_bytes(TM.max_nodes, TM.tile_cells, TM.max_faces, P.move_mesh != 0, SC);
     static size_t attr_bytes = 0;  // dynamic LDS size this instantiation has been opened up to
     if (lds > attr_bytes) {
-        if (hipFuncSetAttribute((const void *)k_step_tile<RT, SUR, TVD>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds) == hipSuccess)
+        if (hipFuncSetAttribute((const void *)k_step_tile<RT, SUR, TVD, SC>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess)
             attr_bytes = lds;
         else
             (void)hipGetLastError();  // the launch below then reports the failure
     }
     const unsigned grid = (unsigned)(kXcds * ((TM.n_tiles + kXcds - 1) / kXcds));
-    hipLaunchKernelGGL((k_step_tile<RT, SUR, TVD>), dim3(grid), dim3(kTileThreads), lds, s, TM, M, P,
-                       prim1, prim2, X1, X2, disp, cc1, cc2, src, sur, S, step);
+    hipLaunchKernelGGL((k_step_tile<RT, SUR, TVD, SC>), dim3(grid), dim3(kTileThreads), lds, s, TM, M, P, prim1,
+                       prim2, sc1, sc2, X1, X2, disp, cc1, cc2, src, sur, S, step);
+}
+
+// The four source / reconstruction variants of one precision and scalar choice.
+template <typename RT, bool SC>
+inline void launch_tile_variants(hipStream_t s, const TileMesh &TM, const DevMesh &M, const DevParams &P,
+                                 const double *prim1, double *prim2, const double *sc1, double *sc2,
+                                 const double *X1, double *X2, const double *disp, const double *cc1, double *cc2,
+                                 const double *src, const SurArgs *sur, StepScalars *S, long long step) {
+    const SurArgs none{};
+    const SurArgs &sa = sur ? *sur : none;
+    const double *sp = sur ? nullptr : src;
+    if (sur && P.tvd)
+        launch_tile_variant<RT, true, true, SC>(s, TM, M, P, prim1, prim2, sc1, sc2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
+    else if (sur)
+        launch_tile_variant<RT, true, false, SC>(s, TM, M, P, prim1, prim2, sc1, sc2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
+    else if (P.tvd)
+        launch_tile_variant<RT, false, true, SC>(s, TM, M, P, prim1, prim2, sc1, sc2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
+    else
+        launch_tile_variant<RT, false, false, SC>(s, TM, M, P, prim1, prim2, sc1, sc2, X1, X2, disp, cc1, cc2, sp, sa, S, step);
 }
 
 }  // namespace titan

@@ -58,6 +58,8 @@ struct titan_ctx {
     double *d_bstate = nullptr, *d_disp = nullptr;
     double *d_X[2] = {nullptr, nullptr};
     double *d_prim[2] = {nullptr, nullptr};
+    double *d_scal[2] = {nullptr, nullptr};  // passive scalars (Q, rho_He) per layer, when carried
+    double *d_bscal = nullptr;               // their fixed boundary values [Nb*2]
     double *d_geom[2] = {nullptr, nullptr};  // one allocation per layer, carved into DevGeom
     DevGeom geom[2];
     double *d_flux = nullptr, *d_src = nullptr, *d_neutrals = nullptr;
@@ -95,7 +97,7 @@ struct titan_ctx {
         M.n_nodes = nn; M.n_cells = nc; M.n_faces = nf;
         M.cell_nodes = d_cell_nodes; M.cell_faces = d_cell_faces; M.cell_code = d_cell_code;
         M.face_nodes = d_face_nodes; M.face_cells = d_face_cells; M.face_code = d_face_code;
-        M.face_bidx = d_face_bidx; M.bstate = d_bstate; M.face_ref = d_face_ref; M.cell_ref = d_cell_ref;
+        M.face_bidx = d_face_bidx; M.bstate = d_bstate; M.bscal = d_bscal; M.face_ref = d_face_ref; M.cell_ref = d_cell_ref;
         M.face_tvd = d_face_tvd;
         return M;
     }
@@ -228,6 +230,8 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
     if (p->move_mesh < 0 || p->move_mesh > 2 || (p->move_mesh == 1 && !m->node_disp)) return TITAN_E_ARG;
     if (p->source_mode < TITAN_SRC_NONE || p->source_mode > TITAN_SRC_BERA) return TITAN_E_ARG;
     if (p->source_mode == TITAN_SRC_BERA && !(p->par_Kn > 0.0)) return TITAN_E_ARG;
+    if (p->passive_scalars < 0 || p->passive_scalars > 1) return TITAN_E_ARG;
+    const bool scal = p->passive_scalars != 0;
     const int64_t nn = m->n_nodes, nc = m->n_cells, nf = m->n_faces;
     // ---- topology checks (Setka::Test_geometr, Setka.cpp:475-617) ----
     for (int64_t f = 0; f < nf; f++) {
@@ -236,7 +240,8 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
         if ((b < 0) != (m->face_bc[f] != TITAN_BC_INTERIOR)) return TITAN_E_TOPOLOGY;
         if (m->face_bc[f] < 0 || m->face_bc[f] > 4 || m->face_method[f] < 0 || m->face_method[f] > 3)
             return TITAN_E_ARG;
-        if ((m->face_bc[f] == TITAN_BC_INNER_HARD || m->face_bc[f] == TITAN_BC_OUTER_HARD) && !m->face_bstate)
+        if ((m->face_bc[f] == TITAN_BC_INNER_HARD || m->face_bc[f] == TITAN_BC_OUTER_HARD) &&
+            (!m->face_bstate || (scal && !m->face_bscalar)))
             return TITAN_E_ARG;
         for (int i = 0; i < 4; i++)
             if (m->face_nodes[4 * f + i] < 0 || m->face_nodes[4 * f + i] >= nn) return TITAN_E_TOPOLOGY;
@@ -259,6 +264,7 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
     c->dp.move_mesh = p->move_mesh; c->dp.n_disc = p->n_disc; c->dp.geom_mode = p->geom_mode;
     c->dp.tvd = 0;
     c->dp.motion_omega = p->motion_omega; c->dp.motion_phase = p->motion_phase;
+    c->dp.n_scal = scal ? kScal : 0;
     if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return TITAN_E_HIP; }
 
     MeshView mv{nn, nc, nf, m->cell_nodes, m->face_cells, m->node_xyz};
@@ -272,7 +278,7 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
     std::vector<int32_t> cell_nodes(8 * nc), cell_faces(6 * nc), face_nodes(4 * nf), face_cells(2 * nf);
     std::vector<int32_t> face_bidx(nf, -1), face_ref(nf), cell_ref(nc);
     std::vector<uint32_t> cell_code(nc), face_code(nf);
-    std::vector<double> bstate;
+    std::vector<double> bstate, bscal;
     for (int64_t d = 0; d < nc; d++) {
         const int64_t r = o.cell_d2r[d];
         cell_ref[d] = (int32_t)r;
@@ -297,6 +303,8 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
         if (bc == TITAN_BC_INNER_HARD || bc == TITAN_BC_OUTER_HARD) {
             face_bidx[d] = (int32_t)(bstate.size() / 8);
             for (int i = 0; i < 8; i++) bstate.push_back(m->face_bstate[8 * r + i]);
+            if (scal)
+                for (int i = 0; i < kScal; i++) bscal.push_back(m->face_bscalar[kScal * r + i]);
         }
     }
     c->nb = (int64_t)bstate.size() / 8;
@@ -321,6 +329,7 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
     up(dev_upload(&c->d_cell_code, cell_code, c->stream));
     up(dev_upload(&c->d_face_code, face_code, c->stream));
     up(dev_upload(&c->d_bstate, bstate, c->stream));
+    if (scal) up(dev_upload(&c->d_bscal, bscal, c->stream));
     up(dev_upload(&c->d_X[0], X, c->stream));
     up(dev_upload(&c->d_X[1], X, c->stream));
     if (!disp.empty()) up(dev_upload(&c->d_disp, disp, c->stream));
@@ -356,7 +365,7 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
                 probe.tile_cells = td.tile_cells;
                 probe.max_nodes = td.max_nodes;
                 probe.max_faces = td.max_faces;
-                if (step_tile_lds_bytes(probe, p->move_mesh != 0) <= budget) { fits = true; break; }
+                if (step_tile_lds_bytes(probe, p->move_mesh != 0, scal) <= budget) { fits = true; break; }
             }
         }
         if (!fits) { titan_destroy(c); return TITAN_E_TOPOLOGY; }
@@ -384,7 +393,7 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
         c->tile_info[0] = T.tile_cells;
         c->tile_info[1] = T.max_nodes;
         c->tile_info[2] = T.max_faces;
-        c->tile_info[3] = (int64_t)step_tile_lds_bytes(T, p->move_mesh != 0);
+        c->tile_info[3] = (int64_t)step_tile_lds_bytes(T, p->move_mesh != 0, scal);
         c->tile_info[4] = lds_limit;
         c->tile_info[5] = T.n_tiles;
         c->tile_info[6] = (int64_t)td.face_ref.size();
@@ -404,8 +413,15 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
         c->geom[l].face_normal = g + 4 * nc + 3 * nf;
         c->geom[l].face_area = g + 4 * nc + 6 * nf;
         hipMemsetAsync(c->d_prim[l], 0, sizeof(double) * 8 * nc, c->stream);
+        if (scal) {
+            if (hipMalloc((void **)&c->d_scal[l], sizeof(double) * kScal * nc) != hipSuccess) {
+                titan_destroy(c);
+                return TITAN_E_HIP;
+            }
+            hipMemsetAsync(c->d_scal[l], 0, sizeof(double) * kScal * nc, c->stream);
+        }
     }
-    if (hipMalloc((void **)&c->d_flux, sizeof(double) * 9 * nf) != hipSuccess ||
+    if (hipMalloc((void **)&c->d_flux, sizeof(double) * (9 + (scal ? kScal : 0)) * nf) != hipSuccess ||
         hipMalloc((void **)&c->d_S, sizeof(StepScalars)) != hipSuccess ||
         hipMalloc((void **)&c->d_S2, sizeof(StepScalars)) != hipSuccess) {
         titan_destroy(c);
@@ -433,7 +449,8 @@ extern "C" int titan_destroy(titan_ctx *c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->d_cell_nodes, c->d_cell_faces, c->d_face_nodes, c->d_face_cells, c->d_face_bidx,
                     c->d_face_ref, c->d_cell_ref, c->d_cell_code, c->d_face_code, c->d_bstate, c->d_disp,
-                    c->d_X[0], c->d_X[1], c->d_prim[0], c->d_prim[1], c->d_geom[0], c->d_geom[1], c->d_flux,
+                    c->d_X[0], c->d_X[1], c->d_prim[0], c->d_prim[1], c->d_scal[0], c->d_scal[1], c->d_bscal,
+                    c->d_geom[0], c->d_geom[1], c->d_flux,
                     c->d_src, c->d_neutrals, c->d_hfluid, c->d_S, c->d_S2, c->d_t_cell_off, c->d_t_node_off,
                     c->d_t_nodes, c->d_t_own,
                     c->d_t_face_off, c->d_t_face_ref, c->d_t_faces, c->d_t_cell_lnodes, c->d_t_cell_lfaces,
@@ -469,6 +486,26 @@ extern "C" int titan_get_state(titan_ctx *c, double *prim, int layer) {
     HIPCHK(hipMemcpyAsync(h.data(), c->d_prim[layer], h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int64_t d = 0; d < c->nc; d++) std::memcpy(prim + 8 * (int64_t)c->ord.cell_d2r[d], &h[8 * d], 64);
+    return TITAN_OK;
+}
+
+extern "C" int titan_set_scalars(titan_ctx *c, const double *qhe, int layer) {
+    if (!c || !qhe || layer < 0 || layer > 1 || !c->d_scal[layer]) return TITAN_E_ARG;
+    std::vector<double> h(kScal * c->nc);
+    for (int64_t d = 0; d < c->nc; d++)
+        std::memcpy(&h[kScal * d], qhe + kScal * (int64_t)c->ord.cell_d2r[d], sizeof(double) * kScal);
+    HIPCHK(hipMemcpyAsync(c->d_scal[layer], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TITAN_OK;
+}
+
+extern "C" int titan_get_scalars(titan_ctx *c, double *qhe, int layer) {
+    if (!c || !qhe || layer < 0 || layer > 1 || !c->d_scal[layer]) return TITAN_E_ARG;
+    std::vector<double> h(kScal * c->nc);
+    HIPCHK(hipMemcpyAsync(h.data(), c->d_scal[layer], h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t d = 0; d < c->nc; d++)
+        std::memcpy(qhe + kScal * (int64_t)c->ord.cell_d2r[d], &h[kScal * d], sizeof(double) * kScal);
     return TITAN_OK;
 }
 
@@ -555,9 +592,10 @@ extern "C" int titan_get_geometry(titan_ctx *c, int layer, double *cell_center, 
     return TITAN_OK;
 }
 
-extern "C" int titan_gran_potok(titan_ctx *c, int now, double dt, double *face_flux, double *dt_min,
-                                uint8_t *branch) {
+extern "C" int titan_gran_potok(titan_ctx *c, int now, double dt, double *face_flux, double *face_kflux,
+                                double *dt_min, uint8_t *branch) {
     if (!c || now < 0 || now > 1 || !(dt > 0.0)) return TITAN_E_ARG;
+    if (face_kflux && !c->dp.n_scal) return TITAN_E_ARG;
     if (init_scalars(c->d_S2) != TITAN_OK) return TITAN_E_HIP;
     // slot (step & 1) = 1 is reset to the reference's initial 1e8
     unsigned long long init = 0x4197D78400000000ull;  // bits of 1.0e8
@@ -565,8 +603,9 @@ extern "C" int titan_gran_potok(titan_ctx *c, int now, double dt, double *face_f
     uint8_t *d_branch = nullptr;
     if (branch) HIPCHK(hipMalloc((void **)&d_branch, c->nf));
     launch_measure(c->stream, c->mesh(), c->d_X[now], c->geom[now], c->params.geom_mode);
-    launch_face_flux(c->stream, c->params.precision, c->mesh(), c->dp, c->d_prim[now], c->geom[now], c->d_X[now],
-                     c->d_X[1 - now], c->d_S2, c->d_flux, d_branch, 1, dt);
+    double *d_kflux = c->d_flux + 9 * c->nf;  // allocated with the flux when the scalars are carried
+    launch_face_flux(c->stream, c->params.precision, c->mesh(), c->dp, c->d_prim[now], c->d_scal[now], c->geom[now],
+                     c->d_X[now], c->d_X[1 - now], c->d_S2, c->d_flux, d_kflux, d_branch, 1, dt);
     HIPCHK(hipGetLastError());
     StepScalars hs;
     int rc = check_scalars(c, c->d_S2, &hs);
@@ -575,6 +614,12 @@ extern "C" int titan_gran_potok(titan_ctx *c, int now, double dt, double *face_f
         std::vector<double> h(9 * c->nf);
         HIPCHK(hipMemcpy(h.data(), c->d_flux, h.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (int64_t d = 0; d < c->nf; d++) std::memcpy(face_flux + 9 * (int64_t)c->ord.face_d2r[d], &h[9 * d], 72);
+    }
+    if (face_kflux) {
+        std::vector<double> h(kScal * c->nf);
+        HIPCHK(hipMemcpy(h.data(), d_kflux, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t d = 0; d < c->nf; d++)
+            std::memcpy(face_kflux + kScal * (int64_t)c->ord.face_d2r[d], &h[kScal * d], sizeof(double) * kScal);
     }
     if (branch) {
         std::vector<uint8_t> h(c->nf);
@@ -601,8 +646,8 @@ static int compute_sources(titan_ctx *c, int layer, int mode) {
     if (prc != TITAN_OK) return prc;
     if (mode == TITAN_SRC_BERA) {
         const titan_params &p = c->params;
-        launch_bera_cells(c->stream, c->nc, c->d_prim[layer], c->d_hfluid, c->n_hfluid, p.par_a_2,
-                          p.par_n_H_LISM / p.par_Kn, c->d_src);
+        launch_bera_cells(c->stream, c->nc, c->d_prim[layer], c->d_scal[layer], c->d_cell_code, c->d_hfluid,
+                          c->n_hfluid, p.par_a_2, p.par_n_H_LISM / p.par_Kn, c->d_src);
         HIPCHK(hipGetLastError());
         return TITAN_OK;
     }
@@ -679,6 +724,9 @@ static int go_impl(titan_ctx *c, int64_t steps, titan_step_stats *stats, int are
         launch_measure(c->stream, M, c->d_X[1], c->geom[1], c->params.geom_mode);
         HIPCHK(hipMemcpyAsync(c->d_prim[1], c->d_prim[0], sizeof(double) * 8 * c->nc, hipMemcpyDeviceToDevice,
                               c->stream));
+        if (c->dp.n_scal)
+            HIPCHK(hipMemcpyAsync(c->d_scal[1], c->d_scal[0], sizeof(double) * kScal * c->nc, hipMemcpyDeviceToDevice,
+                                  c->stream));
         c->now1 = 1;
         c->now2 = 0;
         c->started = true;
@@ -705,9 +753,9 @@ static int go_impl(titan_ctx *c, int64_t steps, titan_step_stats *stats, int are
             src = c->d_src;
         }
         timer_mark(c, 0, true);
-        launch_step_tile(c->stream, c->params.precision, c->tiles, M, c->dp, c->d_prim[n1], c->d_prim[n2], c->d_X[n1],
-                         c->d_X[n2], c->d_disp, c->geom[n1].cell_center, c->geom[n2].cell_center, src,
-                         fused ? &sur : nullptr, c->d_S, c->step);
+        launch_step_tile(c->stream, c->params.precision, c->tiles, M, c->dp, c->d_prim[n1], c->d_prim[n2],
+                         c->d_scal[n1], c->d_scal[n2], c->d_X[n1], c->d_X[n2], c->d_disp, c->geom[n1].cell_center,
+                         c->geom[n2].cell_center, src, fused ? &sur : nullptr, c->d_S, c->step);
         c->tiles.vol_cached = 1;
         timer_mark(c, 0, false);
         if (fused) {
@@ -847,25 +895,28 @@ extern "C" int titan_sphere_fit(int64_t n, const double *points, double *centre)
     return sphere_fit(n, points, centre);
 }
 
-static const char *const kPlasmaKeys[8] = {"rho", "Vx", "Vy", "Vz", "p", "Bx", "By", "Bz"};
+static const char *const kPlasmaKeys[10] = {"rho", "Vx", "Vy", "Vz", "p", "Bx", "By", "Bz", "Q", "rho_He"};
 
 extern "C" int titan_save_cell_parameters(titan_ctx *c, const char *path, int n_extra,
                                           const char *const *extra_keys, const double *extra_values) {
     if (!c || !path || n_extra < 0 || (n_extra > 0 && (!extra_keys || !extra_values))) return TITAN_E_ARG;
     const int layer = c->started ? c->now2 : 0;
     const int64_t nc = c->nc;
-    std::vector<double> prim(8 * nc), xyz(3 * c->nn);
+    const int np = 8 + c->dp.n_scal;  // the plasma keys, with Q and rho_He when the step carries them
+    std::vector<double> prim(8 * nc), qhe(kScal * nc), xyz(3 * c->nn);
     int rc = titan_get_state(c, prim.data(), layer);
+    if (rc == TITAN_OK && c->dp.n_scal) rc = titan_get_scalars(c, qhe.data(), layer);
     if (rc == TITAN_OK) rc = titan_get_nodes(c, xyz.data(), layer);
     if (rc != TITAN_OK) return rc;
-    const int nk = 8 + n_extra;
+    const int nk = np + n_extra;
     std::vector<const char *> keys(nk);
-    for (int k = 0; k < 8; k++) keys[k] = kPlasmaKeys[k];
-    for (int k = 0; k < n_extra; k++) keys[8 + k] = extra_keys[k];
+    for (int k = 0; k < np; k++) keys[k] = kPlasmaKeys[k];
+    for (int k = 0; k < n_extra; k++) keys[np + k] = extra_keys[k];
     std::vector<double> vals((size_t)nk * nc);
     for (int64_t i = 0; i < nc; i++) {
         std::memcpy(&vals[(size_t)nk * i], &prim[8 * i], 64);
-        if (n_extra) std::memcpy(&vals[(size_t)nk * i + 8], extra_values + (size_t)n_extra * i, 8 * (size_t)n_extra);
+        if (c->dp.n_scal) std::memcpy(&vals[(size_t)nk * i + 8], &qhe[kScal * i], sizeof(double) * kScal);
+        if (n_extra) std::memcpy(&vals[(size_t)nk * i + np], extra_values + (size_t)n_extra * i, 8 * (size_t)n_extra);
     }
     return titan_checkpoint_write(path, nc, nk, keys.data(), vals.data(), c->nn, xyz.data());
 }
@@ -873,18 +924,25 @@ extern "C" int titan_save_cell_parameters(titan_ctx *c, const char *path, int n_
 extern "C" int titan_download_cell_parameters(titan_ctx *c, const char *path) {
     if (!c || !path) return TITAN_E_ARG;
     const int64_t nc = c->nc;
-    std::vector<double> vals(8 * nc), xyz(3 * c->nn);
+    const int np = 8 + c->dp.n_scal;
+    std::vector<double> vals((size_t)np * nc), xyz(3 * c->nn);
     int64_t nodes_read = 0;
-    int rc = titan_checkpoint_read(path, nc, 8, kPlasmaKeys, vals.data(), c->nn, xyz.data(), &nodes_read);
+    int rc = titan_checkpoint_read(path, nc, np, kPlasmaKeys, vals.data(), c->nn, xyz.data(), &nodes_read);
     if (rc != TITAN_OK) return rc;
     // a key the file gives a cell replaces it in both layers (Setka_physics.cpp:4238-4247)
-    std::vector<double> prim(8 * nc);
+    std::vector<double> prim(8 * nc), qhe(kScal * nc);
     for (int layer = 0; layer < 2; layer++) {
         rc = titan_get_state(c, prim.data(), layer);
+        if (rc == TITAN_OK && c->dp.n_scal) rc = titan_get_scalars(c, qhe.data(), layer);
         if (rc != TITAN_OK) return rc;
-        for (size_t i = 0; i < prim.size(); i++)
-            if (!std::isnan(vals[i])) prim[i] = vals[i];
+        for (int64_t i = 0; i < nc; i++) {
+            for (int k = 0; k < 8; k++)
+                if (!std::isnan(vals[(size_t)np * i + k])) prim[8 * i + k] = vals[(size_t)np * i + k];
+            for (int k = 0; k < c->dp.n_scal; k++)
+                if (!std::isnan(vals[(size_t)np * i + 8 + k])) qhe[kScal * i + k] = vals[(size_t)np * i + 8 + k];
+        }
         rc = titan_set_state(c, prim.data(), layer);
+        if (rc == TITAN_OK && c->dp.n_scal) rc = titan_set_scalars(c, qhe.data(), layer);
         if (rc != TITAN_OK) return rc;
     }
     if (nodes_read > 0) {  // Setka_physics.cpp:4257-4282: both node layers

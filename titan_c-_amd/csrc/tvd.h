@@ -24,6 +24,10 @@
 // reference assigns rho x r^2 and then overwrites it with the plain interpolation of its final
 // branch; the table gives the value it keeps.)  A reconstructed density or pressure below 1e-7
 // falls back to the cell's own value (Setka_gran_snos.cpp:715-725).
+//
+// The passive scalars Q and rho_He belong to the reference's r^2 names with rho
+// (Phys_param.cpp:284-286), so every case reconstructs them exactly as it does rho (x r^2 in the
+// 1 1 1 1 row, plain elsewhere), with the same 1e-7 fallback to the cell's own value.
 #pragma once
 #include "sphere.h"
 
@@ -148,12 +152,16 @@ __device__ __forceinline__ TvdCase tvd_case(uint32_t zAA, uint32_t zA, uint32_t 
 
 // Second-order states (qL from the A side, qR from the B side) of an interior face with cells A
 // (cells[0]) and B and second neighbours AA and BB: zones z*, states s*, centres c* at the layer
-// of the face centre G.
+// of the face centre G.  K passive scalars k* (nullable when K = 0) give kL, kR.
+template <int K = 0>
 __device__ __forceinline__ void tvd_states(uint32_t zAA, uint32_t zA, uint32_t zB, uint32_t zBB,
                                            const double sAA[8], const double sA[8], const double sB[8],
                                            const double sBB[8], const double cAA[3], const double cA[3],
                                            const double cB[3], const double cBB[3], const double G[3],
-                                           double gamma, double qL[8], double qR[8]) {
+                                           double gamma, double qL[8], double qR[8],
+                                           const double *kAA = nullptr, const double *kA = nullptr,
+                                           const double *kB = nullptr, const double *kBB = nullptr,
+                                           double *kL = nullptr, double *kR = nullptr) {
     const double d1 = tvd_norm3(cA[0] - G[0], cA[1] - G[1], cA[2] - G[2]);
     const double d2 = tvd_norm3(cB[0] - G[0], cB[1] - G[1], cB[2] - G[2]);
     const double dd1 = tvd_norm3(cA[0] - cAA[0], cA[1] - cAA[1], cA[2] - cAA[2]);
@@ -168,13 +176,17 @@ __device__ __forceinline__ void tvd_states(uint32_t zAA, uint32_t zA, uint32_t z
     tvd_forward(sBB, cBB, k.mb, F, gamma, tBB);
     const double xAA = -dd1 - d1, xA = -d1, xB = d2, xBB = d2 + dd2;
     const TvdGap gL = tvd_gap(xAA, xA), gC = tvd_gap(xA, xB), gR = tvd_gap(xB, xBB);
+    auto left = [&](double a, double b, double c) {
+        return k.lm == 0 ? tvd_linear(gL, gC, xA, a, b, c) : (k.lm == 1 ? tvd_linear2(gL, xA, a, b) : tvd_linear2(gC, xB, b, c));
+    };
+    auto right = [&](double a, double b, double c) {
+        return k.rm == 0 ? tvd_linear(gC, gR, xB, a, b, c) : (k.rm == 1 ? tvd_linear2(gC, xB, a, b) : tvd_linear2(gR, xBB, b, c));
+    };
     double l[8], r[8];
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-        l[i] = k.lm == 0 ? tvd_linear(gL, gC, xA, tAA[i], tA[i], tB[i])
-                         : (k.lm == 1 ? tvd_linear2(gL, xA, tAA[i], tA[i]) : tvd_linear2(gC, xB, tA[i], tB[i]));
-        r[i] = k.rm == 0 ? tvd_linear(gC, gR, xB, tA[i], tB[i], tBB[i])
-                         : (k.rm == 1 ? tvd_linear2(gC, xB, tA[i], tB[i]) : tvd_linear2(gR, xBB, tB[i], tBB[i]));
+        l[i] = left(tAA[i], tA[i], tB[i]);
+        r[i] = right(tA[i], tB[i], tBB[i]);
     }
     tvd_back(l, k.ma, F, qL);
     tvd_back(r, k.mb, F, qR);
@@ -182,6 +194,30 @@ __device__ __forceinline__ void tvd_states(uint32_t zAA, uint32_t zA, uint32_t z
     if (qL[4] < 0.0000001) qL[4] = sA[4];
     if (qR[0] < 0.0000001) qR[0] = sB[0];
     if (qR[4] < 0.0000001) qR[4] = sB[4];
+    if constexpr (K > 0) {
+        // the scalars' own lines: rho's variables (x r^2 only in the radially scaled mode)
+        double wAA = 1.0, wA = 1.0, wB = 1.0, wBB = 1.0;
+        if (k.ma == 1) {
+            const double rAA = tvd_norm3(cAA[0], cAA[1], cAA[2]), rA = tvd_norm3(cA[0], cA[1], cA[2]);
+            wAA = rAA * rAA;
+            wA = rA * rA;
+        }
+        if (k.mb == 1) {
+            const double rB = tvd_norm3(cB[0], cB[1], cB[2]), rBB = tvd_norm3(cBB[0], cBB[1], cBB[2]);
+            wB = rB * rB;
+            wBB = rBB * rBB;
+        }
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const double a = k.ma == 1 ? kAA[j] * wAA : kAA[j], b = k.ma == 1 ? kA[j] * wA : kA[j];
+            const double c = k.mb == 1 ? kB[j] * wB : kB[j], d = k.mb == 1 ? kBB[j] * wBB : kBB[j];
+            // the lines take each cell in the variables of its own side, as tvd_forward does
+            const double lv = left(a, b, c), rv = right(b, c, d);
+            kL[j] = k.ma == 1 ? div_by(lv, F.rr2, F.yrr2) : lv;
+            kR[j] = k.mb == 1 ? div_by(rv, F.rr2, F.yrr2) : rv;
+            if (kL[j] < 0.0000001) kL[j] = kA[j];
+            if (kR[j] < 0.0000001) kR[j] = kB[j];
+        }
+    }
 }
-
 }  // namespace titan
