@@ -10,8 +10,9 @@ quoted on.  Beside it:
               source (FP64 Maxwellian quadrature every step), (iii) FP32-mixed + surrogate source
               (MLP fused into the step kernel, exact fallback for out-of-box cells), and the
               ratios (iii)/(ii) and (iii)/(i); FP64 / FP32-mixed with second-order (TVD)
-              face states (Setka_gran_snos.cpp:186-736); FP64 with the reference's closed-form
-              source (Calc_sourse_MF_Bera) and four hydrogen fluids;
+              face states (Setka_gran_snos.cpp:186-736); FP64 carrying the passive scalars Q and
+              rho_He (which the reference's step always carries); FP64 with the reference's
+              closed-form source (Calc_sourse_MF_Bera) and four hydrogen fluids;
   checkpoint  save and load of the state and nodes in the reference's cell-parameter file;
   roofline    the step kernel of (i): algorithmic bytes per launch / its mean duration, timed with
               HIP events around every launch on the context's stream;
@@ -60,10 +61,12 @@ def algorithmic_bytes(n_cells, n_faces, n_nodes, moving, with_neutrals=False):
 
 
 def run_variant(T, mesh, prim, params, steps, warmup, neutrals=None, mlp=None, keep_state=False, h_fluids=None,
-                rays=None):
+                rays=None, scal=None):
     s = T.Setka(mesh, params)
     try:
         s.set_state(prim, 0)
+        if scal is not None:  # the passive scalars Q, rho_He (params passive_scalars = 1)
+            s.set_scalars(scal, 0)
         if rays is not None:  # fitted surfaces: rays and the smoothing set-up of TITAN.cpp:108-112
             s.set_rays(rays)
             s.Smooth_head_TS3()
@@ -292,8 +295,8 @@ def main():
                     help="steps of the (iii) mixed + surrogate vs (ii) FP64 + exact source comparison (0: skip)")
     ap.add_argument("--gate-workload", default="shell1m_moving")
     ap.add_argument("--variants",
-                    default="fp64,mixed,fp64_exact,mixed_surrogate,fp64_tvd,mixed_tvd,fp64_bera,fp64_fit,mk,checkpoint,"
-                            "shell1m,shell1m_moving")
+                    default="fp64,mixed,fp64_exact,mixed_surrogate,fp64_tvd,mixed_tvd,fp64_scalars,fp64_bera,fp64_fit,mk,"
+                            "checkpoint,shell1m,shell1m_moving")
     ap.add_argument("--mk-per-face", type=int, default=100,
                     help="Monte-Carlo neutrals: particles per outer-boundary face (N_per_gran)")
     ap.add_argument("--mk-cpu-per-face", type=int, default=100,
@@ -361,6 +364,12 @@ def main():
             log("%s: %.3f ms/step, step kernel %.3f ms, %d second-order faces" % (
                 name, res[name]["seconds"] * 1e3 / a.steps, res[name]["step_kernel_ms"],
                 res[name]["tiles"]["tvd_faces"]))
+
+    if "fp64_scalars" in variants:  # the passive scalars Q and rho_He the reference always carries
+        res["fp64_scalars"] = run_variant(T, mesh, prim, dict(params, precision=T.PREC_FP64, passive_scalars=1),
+                                          a.steps, a.warmup, scal=C.passive_scalars(mesh, prim))
+        log("fp64 + passive scalars: %.3f ms/step, step kernel %.3f ms" % (
+            res["fp64_scalars"]["seconds"] * 1e3 / a.steps, res["fp64_scalars"]["step_kernel_ms"]))
 
     if "fp64_bera" in variants:  # the reference's closed-form charge-exchange source, four H fluids
         hf = C.h_fluids(nc)
@@ -441,7 +450,7 @@ def main():
         return None if r is None else nc * r["steps"] / r["seconds"]
 
     vals = {k: cups(k) for k in ("fp64", "mixed", "fp64_exact", "mixed_surrogate", "fp64_tvd", "mixed_tvd",
-                                 "fp64_bera", "fp64_fit")}
+                                 "fp64_scalars", "fp64_bera", "fp64_fit")}
     ratios = {}
     if vals["mixed_surrogate"] and vals["fp64_exact"]:
         ratios["surrogate_vs_exact_source"] = vals["mixed_surrogate"] / vals["fp64_exact"]
@@ -451,6 +460,8 @@ def main():
         ratios["mixed_vs_plain_fp64"] = vals["mixed"] / vals["fp64"]
     if vals["fp64_tvd"] and vals["fp64"]:
         ratios["tvd_vs_first_order_fp64"] = vals["fp64_tvd"] / vals["fp64"]
+    if vals["fp64_scalars"] and vals["fp64"]:
+        ratios["passive_scalars_vs_plain_fp64"] = vals["fp64_scalars"] / vals["fp64"]
 
     roof = None
     head = "fp64" if "fp64" in res else None
