@@ -273,11 +273,14 @@ int titan_cx_exact_batch(int64_t n, const double *in10, int nodes, double *out5)
  * by optical depth with nu = rho_Th Velosity_1 sigma / par_Kn and the proton velocity of
  * Setka::M_K_Change_Velosity (a new particle takes the sort of the cell's zone).  density
  * [n_cells*5] = MK_n_H, MK_n_H1..MK_n_H4 of Cell::MK_Add_particle (Cell.cpp:1042-1083) per unit
- * volume.  stats [7] (nullable): particles, charge exchanges, left through the outer boundary,
- * absorbed at the inner boundary (the reference carries them across the Sun's sphere), lost (no
- * exit found after 11 nudges to the cell centre, or max_steps cells crossed), rejection samplers
- * that gave up after 10000 tries (a birth that does: the particle is dropped), flight segments
- * (one per tally: a cell crossed, or a flight to an exchange).  Random numbers:
+ * volume.  A particle that comes within 1.01 R0 of the centre after crossing a face (R0: the
+ * radius of the inner boundary faces' nodes) is carried along its velocity to where it leaves that
+ * sphere and goes on in the cell the ray from the centre through that point enters
+ * (Setka_MK.cpp:2138-2160).  stats [8] (nullable): particles, charge exchanges, left through the
+ * outer boundary, passes through the Sun's sphere, lost (no exit found after 11 nudges to the cell
+ * centre, no cell after a pass, or max_steps cells crossed), births dropped (the birth sampler gave
+ * up after 10000 tries), flight segments (one per tally: a cell crossed, or a flight to an exchange),
+ * flights in which an exchange sampler gave up after 10000 tries.  Random numbers:
  * a counter-based SplitMix64 stream per particle from `seed` (the reference reads rnd_my.txt,
  * which it does not ship); particles are numbered face by face in ascending reference face id.
  * The tallies are FP64 atomic sums, so repeated calls agree to rounding, not bit for bit.  The

@@ -606,7 +606,20 @@ def dvigenie(fit, x, params=None):
 class OrMk(ctypes.Structure):
     _fields_ = [("n_in", _I64), ("face", _P), ("count", _P), ("mu", _P), ("seed", ctypes.c_uint64),
                 ("v_inf", _D), ("a2", _D), ("kn", _D), ("max_steps", ctypes.c_int32), ("mistake", ctypes.c_int32),
-                ("nthreads", ctypes.c_int32)]
+                ("nthreads", ctypes.c_int32), ("R0", _D)]
+
+
+MK_STATS = ("particles", "exchanges", "left", "sun_passes", "lost", "births_dropped", "segments",
+            "exchange_capped")
+
+
+def inner_radius(mesh, xyz):
+    """Radius of the inner (Sun's) sphere: the largest distance from the centre of an inner boundary
+    face's node (the shells' inner nodes lie on it); 0 without inner boundary faces."""
+    f = np.flatnonzero(np.asarray(mesh.face_bc) == 1)
+    if f.size == 0:
+        return 0.0
+    return float(np.linalg.norm(np.asarray(xyz)[np.asarray(mesh.face_nodes)[f].reshape(-1)], axis=1).max())
 
 
 def mk_setup(om, geom, v_inf, n_per_face):
@@ -631,13 +644,12 @@ def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000, ge
     face, count, mu = mk_setup(om, geom, v_inf, n_per_face)
     g = OrGeom(*[_p(geom[k]) for k, _ in OrGeom._fields_])
     mk = OrMk(len(face), _p(face), _p(count), _p(mu), int(seed), float(v_inf), float(a2), float(kn), int(max_steps),
-              int(mistake), int(nthreads))
+              int(mistake), int(nthreads), inner_radius(m, x))
     tally = np.zeros((m.n_cells, 5))
-    stats = np.zeros(7, np.int64)
+    stats = np.zeros(8, np.int64)
     lib().or_mk_go(ctypes.byref(om.c), ctypes.byref(mk), _p(x), _p(np.ascontiguousarray(prim, np.float64)),
                    ctypes.byref(g), _p(tally), _p(stats))
-    keys = ("particles", "exchanges", "left", "absorbed", "lost", "capped", "segments")
-    return tally / geom["cell_volume"][:, None], dict(zip(keys, (int(v) for v in stats)))
+    return tally / geom["cell_volume"][:, None], dict(zip(MK_STATS, (int(v) for v in stats)))
 
 
 def mk_change_velosity(n, U, V, cp, a2, seed=1, mistake=0):

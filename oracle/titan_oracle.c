@@ -1888,24 +1888,71 @@ int64_t or_mk_setup(const or_mesh *m, const or_geom *g, double v_inf, int32_t n_
     return k;
 }
 
+/* findSphereIntersectionTime (Setka_MK.cpp:4-38): the first non-negative time at which X + t V is at
+ * radius R. */
+static int mk_sphere_time(const double *X, const double *V, double R, double *time)
+{
+    const double a = V[0] * V[0] + V[1] * V[1] + V[2] * V[2];
+    const double b = 2.0 * (X[0] * V[0] + X[1] * V[1] + X[2] * V[2]);
+    const double c = (X[0] * X[0] + X[1] * X[1] + X[2] * X[2]) - R * R;
+    const double D = b * b - 4 * a * c;
+    if (D < 0) return 0;
+    const double sqrtD = sqrt(D);
+    const double t1 = (-b - sqrtD) / (2 * a);
+    const double t2 = (-b + sqrtD) / (2 * a);
+    if (t1 >= 0) { *time = t1; return 1; }
+    if (t2 >= 0) { *time = t2; return 1; }
+    return 0;
+}
+
+/* The ray from the origin through P hits triangle (a, b, c) (Moller-Trumbore, t > 0). */
+static int mk_origin_ray_tri(const double *P, const double *a, const double *b, const double *c)
+{
+    const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    const double h[3] = {P[1] * e2[2] - P[2] * e2[1], P[2] * e2[0] - P[0] * e2[2], P[0] * e2[1] - P[1] * e2[0]};
+    const double det = e1[0] * h[0] + e1[1] * h[1] + e1[2] * h[2];
+    if (fabs(det) < 1e-300) return 0;
+    const double s[3] = {-a[0], -a[1], -a[2]};
+    const double u = (s[0] * h[0] + s[1] * h[1] + s[2] * h[2]) / det;
+    if (u < 0.0 || u > 1.0) return 0;
+    const double q[3] = {s[1] * e1[2] - s[2] * e1[1], s[2] * e1[0] - s[0] * e1[2], s[0] * e1[1] - s[1] * e1[0]};
+    const double v = (P[0] * q[0] + P[1] * q[1] + P[2] * q[2]) / det;
+    if (v < 0.0 || u + v > 1.0) return 0;
+    return (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) / det > 0.0;
+}
+
+/* The cell a particle re-enters the mesh by after passing through the Sun's sphere: the cell on the
+ * inner boundary face that the ray from the centre through its position crosses (every inner face
+ * tried in turn).  Returns -1 if none. */
+static int64_t mk_locate_inner(const or_mesh *m, const double *x, const double *P)
+{
+    for (int64_t f = 0; f < m->n_faces; f++) {
+        if (m->face_bc[f] != OR_BC_INNER_HARD) continue;
+        const int32_t *fn = m->face_nodes + 4 * f;
+        const double *n0 = x + 3 * fn[0], *n1 = x + 3 * fn[1], *n2 = x + 3 * fn[2], *n3 = x + 3 * fn[3];
+        if (mk_origin_ray_tri(P, n0, n1, n2) || mk_origin_ray_tri(P, n0, n2, n3)) return m->face_cells[2 * f];
+    }
+    return -1;
+}
+
 int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *prim, const or_geom *g,
              double *tally, int64_t *stats)
 {
     const double me = 1.0 / 1836.15;
-    for (int i = 0; i < 7; i++) stats[i] = 0;
+    for (int i = 0; i < 8; i++) stats[i] = 0;
     /* particles are numbered face by face; first[q] = id of face q's first particle */
     int64_t *first = (int64_t *)malloc(sizeof(int64_t) * (size_t)(mk->n_in + 1));
     first[0] = 0;
     for (int64_t q = 0; q < mk->n_in; q++) first[q + 1] = first[q] + mk->count[q];
     const int64_t n_part = first[mk->n_in];
     int nt = mk->nthreads > 0 ? mk->nthreads : 1;
-    int64_t s0_ = 0, s1_ = 0, s2_ = 0, s3_ = 0, s4_ = 0, s5_ = 0, s6_ = 0;
+    int64_t s0_ = 0, s1_ = 0, s2_ = 0, s3_ = 0, s4_ = 0, s5_ = 0, s6_ = 0, s7_ = 0;
     /* one particle per iteration; its history depends only on its id (its random stream); with
      * several threads the tallies are added atomically, so their rounding varies with the order */
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nt) \
-    reduction(+ : s0_, s1_, s2_, s3_, s4_, s5_, s6_)
+    reduction(+ : s0_, s1_, s2_, s3_, s4_, s5_, s6_, s7_)
     for (int64_t pid = 0; pid < n_part; pid++) {
-        int64_t st_[7] = {0, 0, 0, 0, 0, 0, 0};
+        int64_t st_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         int64_t *stats = st_;
         int64_t lo = 0, hi = mk->n_in - 1;
         while (lo < hi) {
@@ -1986,9 +2033,27 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
                     }
                     for (int i = 0; i < 3; i++) P[i] += 1.000001 * time * V[i];
                     int64_t nc = m->face_cells[2 * ex] == c ? m->face_cells[2 * ex + 1] : m->face_cells[2 * ex];
-                    if (nc < 0) {
-                        if (m->face_bc[ex] == OR_BC_INNER_HARD) stats[3]++;
-                        else stats[2]++;
+                    if (nc < 0 && m->face_bc[ex] != OR_BC_INNER_HARD) { /* out through the outer boundary */
+                        stats[2]++;
+                        done = 1;
+                    } else if (nc < 0 && (mk->mistake & 4)) { /* planted: absorbed at the inner sphere */
+                        stats[2]++;
+                        done = 1;
+                    } else if (mk->R0 > 0.0 && sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2]) < 1.01 * mk->R0) {
+                        /* through the Sun's sphere to its far side (Setka_MK.cpp:2138-2160), on into
+                           the cell there */
+                        double t_;
+                        if (mk_sphere_time(P, V, 1.01 * mk->R0, &t_)) {
+                            for (int i = 0; i < 3; i++) P[i] += t_ * V[i];
+                        } else {
+                            const double nv = sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
+                            for (int i = 0; i < 3; i++) P[i] += (2.02 * mk->R0) * V[i] / nv;
+                        }
+                        stats[3]++;
+                        c = mk_locate_inner(m, x, P);
+                        if (c < 0) { stats[4]++; done = 1; }
+                    } else if (nc < 0) { /* an inner face without the sphere: nowhere to go */
+                        stats[4]++;
                         done = 1;
                     } else {
                         c = nc;
@@ -2020,12 +2085,13 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
                 }
                 if (it == mk->max_steps - 1 && !done) { stats[4]++; done = 1; }
             }
-            if (capped) stats[5]++;
+            if (capped) stats[7]++; /* some exchange sampler of the flight gave up */
         } while (0);
         s0_ += st_[0]; s1_ += st_[1]; s2_ += st_[2]; s3_ += st_[3]; s4_ += st_[4]; s5_ += st_[5]; s6_ += st_[6];
+        s7_ += st_[7];
     }
     stats[0] = s0_; stats[1] = s1_; stats[2] = s2_; stats[3] = s3_; stats[4] = s4_; stats[5] = s5_;
-    stats[6] = s6_;
+    stats[6] = s6_; stats[7] = s7_;
     free(first);
     return 0;
 }

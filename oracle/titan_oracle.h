@@ -267,10 +267,14 @@ void or_dvigenie(const or_fit *fit, double *x);
  * (Setka_MK.cpp:1225-2316) for the LISM hydrogen that enters through the outer boundary (sort 4).
  * Injection faces: face ids, particles per face and weight per particle (or_mk_setup restates
  * the counts of Setka_MK.cpp:1233,1380-1390).  tally [n_cells*5] += mu * time per cell: the sum
- * over sorts, then sorts 1..4 (Cell::MK_Add_particle, Cell.cpp:1042-1083).  stats [7]: particles,
- * charge exchanges, left through the outer boundary, absorbed at the inner boundary, lost (no exit
- * found or the step cap), rejection caps hit.  stats[6]: flight segments (one per tally: a cell
- * crossed or a flight to an exchange).  The random numbers come from a counter-based
+ * over sorts, then sorts 1..4 (Cell::MK_Add_particle, Cell.cpp:1042-1083).  A particle that comes
+ * within 1.01 R0 of the centre after crossing a face is carried along its velocity to where it
+ * leaves that sphere and goes on in the cell the ray from the centre through that point enters
+ * (Setka_MK.cpp:2138-2160).  stats [8]: particles, charge exchanges, left through the outer
+ * boundary, passes through the Sun's sphere, lost (no exit found, no cell after a pass, or the
+ * step cap), births dropped (the birth sampler's cap), flight segments (one per tally: a cell
+ * crossed or a flight to an exchange), flights in which an exchange sampler hit its cap.  The
+ * random numbers come from a counter-based
  * SplitMix64 stream per particle (the reference's Sensor objects read rnd_my.txt, which it does
  * not ship). */
 typedef struct {
@@ -283,8 +287,10 @@ typedef struct {
     int32_t max_steps;
     int32_t mistake;  /* 0; for tests that show the pins catch a plausible error: bit 0 drops the
                          |V.n| weight of the birth velocity, bit 1 the g sigma(g) weight of the
-                         exchanged velocity */
+                         exchanged velocity, bit 2 absorbs the particles at the inner sphere */
     int32_t nthreads; /* OpenMP threads (<= 1: one) */
+    double R0;        /* radius of the inner (Sun's) sphere the particles pass through (Geo_param::R0;
+                         0: no sphere) */
 } or_mk;
 /* Fluxes of the LISM Maxwellian into the domain through the outer boundary faces (bc 2, 3):
  * flux[f] = Get_Spotok_inf(n) * area (Setka_MK.cpp:2316-2324); count and mu by the rule of

@@ -38,8 +38,8 @@ def test_free_flight_matches_the_oracle_and_the_closed_form(T, C, O):
     d, st = _run(T, mesh, prim, params, 200, 7, 1e30)
     od, ost = O.mk_go(O.OracleMesh(mesh), mesh.node_xyz, prim, C.V_INF, A2, 1e30, 200, seed=7)
     assert st["particles"] == ost["particles"] and st["exchanges"] == 0
-    assert st["left"] + st["capped"] == st["particles"]
-    assert abs(st["capped"] - ost["capped"]) <= max(3, ost["capped"] // 100)
+    assert st["left"] + st["births_dropped"] == st["particles"]
+    assert abs(st["births_dropped"] - ost["births_dropped"]) <= max(3, ost["births_dropped"] // 100)
     assert abs(st["segments"] - ost["segments"]) < 0.01 * ost["segments"]
     assert abs(d[:, 0].mean() - 1.0) < 0.012 and np.abs(d[:, 0] - 1.0).max() < 0.15
     # same random stream per particle: the cell densities agree far inside the MC noise
@@ -85,7 +85,8 @@ def test_fitted_shell_against_the_oracle(T, C, O):
     od, ost = O.mk_go(O.OracleMesh(mesh), mesh.node_xyz, prim, C.V_INF, A2, 40.0906, 100, seed=13)
     assert st["particles"] == ost["particles"] > 0
     assert st["lost"] == 0 and ost["lost"] == 0
-    assert st["left"] + st["absorbed"] + st["capped"] == st["particles"]
+    assert st["left"] + st["births_dropped"] == st["particles"]
+    assert abs(st["sun_passes"] - ost["sun_passes"]) < 0.1 * ost["sun_passes"] + 10
     assert abs(st["exchanges"] - ost["exchanges"]) < 0.03 * ost["exchanges"]
     o2, _ = O.mk_go(O.OracleMesh(mesh), mesh.node_xyz, prim, C.V_INF, A2, 40.0906, 100, seed=14)
     vol = O.measure(O.OracleMesh(mesh), mesh.node_xyz)["cell_volume"]
@@ -93,6 +94,24 @@ def test_fitted_shell_against_the_oracle(T, C, O):
         same = (np.abs(d[:, k] - od[:, k]) * vol).sum()
         noise = (np.abs(o2[:, k] - od[:, k]) * vol).sum()
         assert same < 0.5 * noise, (k, same, noise)
+
+
+def test_pass_through_the_sun(T, C, O):
+    """The Sun shell (LISM free flight around the inner sphere): the device carries the particles
+    across the sphere as the oracle does -- no particle lost, the pass counts alike, the density 1
+    past the first layer and in the cone behind the Sun, the cell densities close to the oracle's
+    (same random streams)."""
+    mesh, prim, params = C.sun_shell()
+    d, st = _run(T, mesh, prim, params, 1000, 5, 1e30)
+    od, ost = O.mk_go(O.OracleMesh(mesh), mesh.node_xyz, prim, C.V_INF, A2, 1e30, 1000, seed=5, nthreads=8)
+    assert st["particles"] == ost["particles"] and st["lost"] == 0 and ost["lost"] == 0
+    assert st["left"] + st["births_dropped"] == st["particles"]
+    assert abs(st["sun_passes"] - ost["sun_passes"]) < 0.01 * ost["sun_passes"] + 10
+    from test_oracle_mk import _sun_layers
+    lay = _sun_layers(O, mesh, d)
+    assert np.abs(lay[:, 0] - 1.0).max() < 0.012, lay
+    assert np.abs(lay[:, 1] - 1.0).max() < 0.03, lay
+    assert np.abs(d[:, 0] - od[:, 0]).max() < 0.03
 
 
 def test_bad_arguments(T, C):

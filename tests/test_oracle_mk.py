@@ -12,8 +12,12 @@ pinned by what the kinetic theory fixes, on the cube with every boundary face an
 * the exchange sampler (M_K_Change_Velosity) against the integral that defines it: mean and mean
   square of the weighted Maxwellian by quadrature; dropping its g sigma(g) weight fails it (the
   density checks above do not see that error: measured, the densities stay within noise).
-* bookkeeping: the sorts sum to the total; every particle ends once (left, absorbed, lost or
-  dropped at birth); the particle count follows the rule of Setka_MK.cpp:1233,1380-1390."""
+* the pass through the Sun's sphere (Setka_MK.cpp:2138-2160): on a shell around the sphere the
+  straight-line jump across it keeps the free-flight density at 1 in every cell past the first
+  layer, the cone behind the Sun included, where absorbing the particles instead (or_mk.mistake
+  bit 2) leaves a deep shadow.
+* bookkeeping: the sorts sum to the total; every particle ends once (left, lost or dropped at
+  birth); the particle count follows the rule of Setka_MK.cpp:1233,1380-1390."""
 import math
 
 import numpy as np
@@ -37,15 +41,15 @@ def _lism(C, mesh):
 
 
 def _ends(st):
-    return st["left"] + st["absorbed"] + st["lost"]
+    return st["left"] + st["lost"]
 
 
 def test_free_flight_holds_the_lism_density(C, O):
     mesh, prim, _ = _cube(C)
     om = O.OracleMesh(mesh)
     d, st = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, 1e30, 200, seed=7)
-    assert st["exchanges"] == 0 and st["lost"] == 0 and st["absorbed"] == 0
-    dropped = st["capped"]
+    assert st["exchanges"] == 0 and st["lost"] == 0 and st["sun_passes"] == 0
+    dropped = st["births_dropped"]
     assert _ends(st) + dropped == st["particles"]
     # MC noise of one cell ~ 0.03 at 200 particles per face; the mean over 216 cells ~ 0.002
     assert abs(d[:, 0].mean() - 1.0) < 0.012
@@ -154,3 +158,33 @@ def test_threads_keep_the_histories(C, O):
     d4, s4 = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, 1.0, 100, seed=9, nthreads=4)
     assert s1 == s4
     np.testing.assert_allclose(d4, d1, rtol=1e-12, atol=1e-15)
+
+
+def _sun_layers(O, mesh, d):
+    """Mean densities of the layers of the Sun shell and of the cone behind the Sun (x < 0, within
+    45 degrees of the -x axis), layers 1..4 (the first layer, whose inner hundredth the jump skips,
+    is left out)."""
+    g = O.measure(O.OracleMesh(mesh), mesh.node_xyz)
+    cc = g["cell_center"]
+    r = np.linalg.norm(cc, axis=1)
+    behind = (cc[:, 0] < 0) & (np.linalg.norm(cc[:, 1:], axis=1) < -cc[:, 0])
+    out = []
+    for L in range(1, 5):
+        m = (r > 4.0 ** (L / 8)) & (r < 4.0 ** ((L + 1) / 8))
+        out.append((d[m, 0].mean(), d[m & behind, 0].mean()))
+    return np.array(out)
+
+
+def test_pass_through_the_sun_keeps_the_free_flight_density(C, O):
+    mesh, prim, _ = C.sun_shell()
+    om = O.OracleMesh(mesh)
+    d, st = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, 1e30, 1000, seed=5, nthreads=4)
+    assert st["sun_passes"] > 0.03 * st["particles"] and st["lost"] == 0
+    assert _ends(st) + st["births_dropped"] == st["particles"]
+    lay = _sun_layers(O, mesh, d)
+    # MC noise of a layer mean ~0.3 %, of the 32-cell cone ~1 %
+    assert np.abs(lay[:, 0] - 1.0).max() < 0.012, lay
+    assert np.abs(lay[:, 1] - 1.0).max() < 0.03, lay
+    # absorbing the particles at the sphere instead leaves a shadow behind the Sun
+    da, _ = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, 1e30, 1000, seed=5, nthreads=4, mistake=4)
+    assert _sun_layers(O, mesh, da)[:2, 1].max() < 0.5

@@ -36,6 +36,10 @@ _ERRNAMES = {
 }
 
 
+# titan_mk_go's counters
+MK_STATS = ("particles", "exchanges", "left", "sun_passes", "lost", "births_dropped", "segments", "exchange_capped")
+
+
 class TitanError(RuntimeError):
     def __init__(self, code, what, index=-1):
         self.code = code
@@ -496,11 +500,10 @@ class Setka:
         [n_cells, 5] = MK_n_H, MK_n_H1..MK_n_H4 per unit volume, stats dict)."""
         layer = self.current_layer() if layer is None else layer
         out = np.empty((self.n_cells, 5))
-        st = np.zeros(7, np.int64)
+        st = np.zeros(8, np.int64)
         _check(lib().titan_mk_go(self.ctx, int(layer), int(n_per_face), int(seed) & (2**64 - 1), int(max_steps),
                                  _ptr(out), _ptr(st)), "titan_mk_go", self.ctx)
-        keys = ("particles", "exchanges", "left", "absorbed", "lost", "capped", "segments")
-        return out, dict(zip(keys, (int(v) for v in st)))
+        return out, dict(zip(MK_STATS, (int(v) for v in st)))
 
     def set_timing(self, on=True):
         _check(lib().titan_set_timing(self.ctx, 1 if on else 0), "titan_set_timing", self.ctx)

@@ -253,6 +253,34 @@ def shell(n_edge=37, n_layers=120, moving=False, seed_motion=3, r_zone1_au=80.0,
     return mesh, prim, params
 
 
+def sun_shell(n_edge=6, n_layers=8, r_out=4.0, v_inf=V_INF):
+    """A small spherical shell around the Sun's sphere (radius 1, outer radius r_out, log-spaced
+    layers), filled with the interstellar Maxwellian of the Monte-Carlo neutrals (rho_Th = 1,
+    V = (v_inf, 0, 0), p_Th = 1/2: cp = 1): every outer face is an inflow face (bc 3), the inner
+    sphere a hard face (bc 1).  For the neutrals' pass through the Sun's sphere, where the heliosphere
+    shell's 1:1500 radii leave too few particles near it.  Returns (mesh, prim, params)."""
+    radii = r_out ** (np.arange(n_layers + 1) / n_layers)
+    xyz, cn = mesh_shell(n_edge, radii)
+    zone = np.full(cn.shape[0], 3, np.int8)
+
+    def bc_fn(fc, cc, fn, fcl, b):
+        rf = np.linalg.norm(fc, axis=1)
+        bc = np.full(fc.shape[0], BC_INTERIOR)
+        bc[b & (rf < math.sqrt(r_out))] = BC_INNER_HARD
+        bc[b & (rf >= math.sqrt(r_out))] = BC_OUTER_SOFT
+        return bc
+
+    me = 1.0 / 1836.15
+    lism = np.array([1.0 + me, v_inf, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0])
+
+    def bstate_fn(fc, bc):
+        return np.tile(lism, (fc.shape[0], 1))
+
+    mesh, _ = _finish(xyz, cn, bc_fn, lambda fcl, z: np.full(fcl.shape[0], 3), zone, bstate_fn=bstate_fn)
+    prim = np.tile(lism, (mesh.n_cells, 1))
+    return mesh, prim, default_params(v_inf=v_inf)
+
+
 # ---------------------------------------------------------------------------------------------
 def a_luch_radii(R0, R1, R2, R3, R4, R5, M0, M1, M11, M2, M3, M4, da1, da2, da3, da4, da5):
     """Radii [n_rays, npr] of the nodes of rays whose termination shock, heliopause and bow shock

@@ -127,7 +127,15 @@ struct MkDev {
     const MkCell *pack;         // [Nc] flight records of the cells (k_mk_pack)
     double *tally;              // [Nc*5] mu * time of sorts outside 1..4, then of sorts 1..4
     double *out;                // [Nc*5] the tallies per unit volume, reference cell order
-    unsigned long long *stats;  // [7] particles, charge exchanges, left, absorbed, lost, capped, segments
+    unsigned long long *stats;  // [8] particles, charge exchanges, left, passes through the Sun's
+                                // sphere, lost, births dropped, segments, flights with a capped exchange
+    // the Sun's sphere (Setka_MK.cpp:2138-2160): its radius R0 (0: none) and the inner boundary
+    // faces -- device face ids ascending, and per face the inner face across each of its 4 edges
+    // (edge k joins its nodes k and k+1 mod 4)
+    double R0;
+    int32_t n_inner, pad2;
+    const int32_t *inner_face;  // [n_inner]
+    const int32_t *inner_nb;    // [n_inner*4]
 };
 
 // Failures of a step, ranked in the order the reference meets them -- the surface fit before the

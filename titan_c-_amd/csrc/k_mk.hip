@@ -9,11 +9,14 @@
 // (Setka::M_K_Change_Velosity, in the spherical frame of its position) and its sort the zone of the
 // cell.  Every cell it crosses gains mu * time in its sort's slot, one atomic per segment; the
 // sum over sorts (MK_n_H of Cell::MK_Add_particle) is formed from the slots at the end.  The
-// particle ends at a boundary face.  Differences from the reference, which keep every loop
-// bounded: a particle that leaves through the inner sphere is absorbed (the reference carries it
-// across the Sun's sphere, Setka_MK.cpp:2140-2160); the birth sampler gives up after 10000 tries and drops the particle (only on faces where the flow leaves
-// the domain, whose inflow is ~exp(-V_inf^2) of the upwind faces'); a particle that finds no exit
-// after 11 nudges towards the cell centre, or exceeds the step cap, is counted lost.  The tallies
+// particle ends at the outer boundary.  A particle that comes within 1.01 R0 of the centre after
+// crossing a face is carried along its velocity to where it leaves that sphere and goes on in the
+// cell there (Setka_MK.cpp:2138-2160): the cell on the inner boundary face the ray from the centre
+// through its new position crosses, found by a walk over the inner faces (mk_locate_inner).
+// Differences from the reference, which keep every loop bounded: the birth sampler gives up after
+// 10000 tries and drops the particle (only on faces where the flow leaves the domain, whose inflow
+// is ~exp(-V_inf^2) of the upwind faces'); a particle that finds no exit after 11 nudges towards
+// the cell centre, or exceeds the step cap, is counted lost.  The tallies
 // are FP64 atomic sums (the order of additions varies; the estimate is statistical).
 //
 // The flight is bound by memory latency at low occupancy (245+ VGPRs).  Measured choices: each
@@ -218,6 +221,78 @@ __device__ __forceinline__ bool mk_velosity_initial(uint64_t &rs, double v_inf, 
     return true;
 }
 
+// findSphereIntersectionTime (Setka_MK.cpp:4-38): the first non-negative time at which X + t V is at
+// radius R.
+__device__ __forceinline__ bool mk_sphere_time(const double X[3], const double V[3], double R, double &time) {
+    const double a = V[0] * V[0] + V[1] * V[1] + V[2] * V[2];
+    const double b = 2.0 * (X[0] * V[0] + X[1] * V[1] + X[2] * V[2]);
+    const double c = (X[0] * X[0] + X[1] * X[1] + X[2] * X[2]) - R * R;
+    const double D = b * b - 4 * a * c;
+    if (D < 0) return false;
+    const double sqrtD = sqrt(D);
+    const double t1 = (-b - sqrtD) / (2 * a);
+    const double t2 = (-b + sqrtD) / (2 * a);
+    if (t1 >= 0) { time = t1; return true; }
+    if (t2 >= 0) { time = t2; return true; }
+    return false;
+}
+
+__device__ __forceinline__ void mk_cross(const double *a, const double *b, double o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Whether the direction d lies inside inner face i as seen from the centre; if not, the edge to
+// cross next (the first whose plane through the centre puts d and the face's centre on opposite
+// sides).  Returns -1 inside, else the edge index.
+__device__ __forceinline__ int mk_inner_side(const DevMesh &M, const MkDev &K, const double *__restrict__ X, int i,
+                                             const double d[3]) {
+    const int64_t f = K.inner_face[i];
+    const double *a[4];
+    double cen[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 4; k++) {
+        a[k] = X + 3 * (int64_t)M.face_nodes[4 * f + k];
+        for (int e = 0; e < 3; e++) cen[e] += a[k][e];
+    }
+    for (int k = 0; k < 4; k++) {
+        double ne[3];
+        mk_cross(a[k], a[(k + 1) & 3], ne);
+        const double sc = ne[0] * cen[0] + ne[1] * cen[1] + ne[2] * cen[2];
+        const double sd = ne[0] * d[0] + ne[1] * d[1] + ne[2] * d[2];
+        if ((sc > 0.0 && sd < 0.0) || (sc < 0.0 && sd > 0.0)) return k;
+    }
+    return -1;
+}
+
+// The cell on the inner boundary face the ray from the centre through P crosses: a visibility walk
+// over the inner faces from face `start` (a few steps across neighbouring faces towards P); past
+// the step cap every inner face is tried in turn.  -1 if none holds it.
+__device__ int mk_locate_inner(const DevMesh &M, const MkDev &K, const double *__restrict__ X, int start,
+                               const double P[3]) {
+    int i = start;
+    const int cap = 64 + 4 * (int)sqrt((double)K.n_inner);
+    for (int step = 0; step < cap && i >= 0; step++) {
+        const int k = mk_inner_side(M, K, X, i, P);
+        if (k < 0) return M.face_cells[2 * (int64_t)K.inner_face[i]];
+        i = K.inner_nb[4 * i + k];
+    }
+    for (int j = 0; j < K.n_inner; j++)
+        if (mk_inner_side(M, K, X, j, P) < 0) return M.face_cells[2 * (int64_t)K.inner_face[j]];
+    return -1;
+}
+
+// Index of device face f among the inner faces (ascending), or 0 if it is not one.
+__device__ __forceinline__ int mk_inner_index(const MkDev &K, int64_t f) {
+    int lo = 0, hi = K.n_inner - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (K.inner_face[mid] < f) lo = mid + 1;
+        else hi = mid;
+    }
+    return (K.n_inner > 0 && K.inner_face[lo] == f) ? lo : 0;
+}
+
 // One record per cell for the flights: the 8 node positions of the cell, each face's 4 nodes (in
 // the face's own order) as indices into them, and the cell across each face (-1 - boundary code
 // on a boundary face), faces in the device order of cell_faces.
@@ -320,6 +395,7 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
         atomicAdd(&st[5], 1ull);
         return;
     }
+    unsigned long long n_pass = 0;
     int64_t c = M.face_cells[2 * f];
     {
         const double *cc = G.cell_center + 3 * c;
@@ -331,7 +407,7 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
     int slot = 4;  // tally slot: the sort (1..4), or 0 for a sort outside 1..4 (counted in the total only)
     bool capped = false;
     unsigned long long n_ex = 0, n_seg = 0;
-    int outcome = 4;  // 2 left, 3 absorbed, 4 lost
+    int outcome = 4;  // 2 left, 4 lost
     const double me = 1.0 / 1836.15;
     // An exchange that falls due is held (pend) until a quarter of the wavefront's live lanes hold
     // one, or all of them do; the lanes then draw their new velocities together, so the costly
@@ -339,6 +415,7 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
     // its own random stream, so the particle's history does not change.
     bool pend = false;
     double cp_ex = 1.0;
+    int64_t pass_face = -2;  // -2: no pass due; else the inner face crossed (-1: an interior face)
     int it = 0;
 #pragma unroll 1
     while (true) {
@@ -378,11 +455,17 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
                 atomicAdd(&K.tally[5 * c + slot], time * mu);
                 for (int i = 0; i < 3; i++) P[i] += 1.000001 * time * V[i];
                 const int32_t nb = K.pack[c].nbr[jx];
-                if (nb < 0) {
-                    outcome = (-1 - nb) == 1 ? 3 : 2;
+                if (nb < 0 && (-1 - nb) != 1) {  // out through the outer boundary
+                    outcome = 2;
                     break;
                 }
-                c = nb;
+                if (K.R0 > 0.0 && sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2]) < 1.01 * K.R0) {
+                    pass_face = nb < 0 ? M.cell_faces[6 * c + jx] : -1;  // the pass follows below
+                } else if (nb < 0) {  // an inner face without the sphere: nowhere to go (lost)
+                    break;
+                } else {
+                    c = nb;
+                }
             } else {
                 const double kk = (KSI - I_do) * sig;
                 const double t_ex = kk / vn;
@@ -392,6 +475,22 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
                 cp_ex = cp;
                 pend = true;
             }
+        }
+        if (pass_face != -2) {
+            // through the Sun's sphere to its far side, on into the cell there (its own block, so
+            // that its temporaries do not add to the segment's)
+            double t_;
+            if (mk_sphere_time(P, V, 1.01 * K.R0, t_)) {
+                for (int i = 0; i < 3; i++) P[i] += t_ * V[i];
+            } else {
+                const double nv = sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
+                for (int i = 0; i < 3; i++) P[i] += (2.02 * K.R0) * V[i] / nv;
+            }
+            n_pass++;
+            const int cn = mk_locate_inner(M, K, X, pass_face >= 0 ? mk_inner_index(K, pass_face) : 0, P);
+            pass_face = -2;
+            if (cn < 0) break;  // lost
+            c = cn;
         }
         const unsigned long long held = __ballot(pend), live = __ballot(true);
         if (pend && (4 * __popcll(held) >= __popcll(live) || held == live)) {
@@ -415,9 +514,10 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
     }
     atomicAdd(&st[0], 1ull);
     if (n_ex) atomicAdd(&st[1], n_ex);
+    if (n_pass) atomicAdd(&st[3], n_pass);
     atomicAdd(&st[6], n_seg);
     atomicAdd(&st[outcome], 1ull);
-    if (capped) atomicAdd(&st[5], 1ull);
+    if (capped) atomicAdd(&st[7], 1ull);
 }
 
 void launch_mk_pack(hipStream_t s, const DevMesh &M, const double *X, MkCell *pack) {

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/titan_gpu.h"
@@ -90,6 +91,10 @@ struct titan_ctx {
     FitState *fit = nullptr;  // fitted surfaces and rays (move_mesh 2)
     bool has_area = false;    // titan_set_inner_area was called
     std::vector<int32_t> mk_cand;  // outer boundary faces (device ids, ascending reference id) for titan_mk_go
+    // the Sun's sphere of titan_mk_go: inner boundary faces (device ids ascending), the inner face
+    // across each of their 4 edges, and the sphere's radius (0 without inner faces)
+    std::vector<int32_t> mk_inner, mk_inner_nb;
+    double mk_R0 = 0.0;
     void *d_mk = nullptr;          // titan_mk_go: flight records, tallies, densities, counters
     bool mk_cand_ready = false;
     DevMesh mesh() const {
@@ -308,6 +313,29 @@ extern "C" int titan_create(titan_ctx **out, const titan_mesh *m, const titan_pa
         }
     }
     c->nb = (int64_t)bstate.size() / 8;
+    {
+        // inner boundary faces, their edge neighbours among them, and the radius of their nodes
+        std::unordered_map<uint64_t, int32_t> edge;  // node pair -> first (face index * 4 + edge)
+        for (int64_t d = 0; d < nf; d++)
+            if ((face_code[d] & 15u) == (uint32_t)TITAN_BC_INNER_HARD) c->mk_inner.push_back((int32_t)d);
+        c->mk_inner_nb.assign(4 * c->mk_inner.size(), -1);
+        for (size_t i = 0; i < c->mk_inner.size(); i++) {
+            const int64_t d = c->mk_inner[i];
+            for (int k = 0; k < 4; k++) {
+                const uint32_t a = (uint32_t)face_nodes[4 * d + k], b = (uint32_t)face_nodes[4 * d + ((k + 1) & 3)];
+                const uint64_t key = a < b ? ((uint64_t)a << 32 | b) : ((uint64_t)b << 32 | a);
+                const auto it = edge.find(key);
+                if (it == edge.end()) {
+                    edge.emplace(key, (int32_t)(4 * i + k));
+                } else {
+                    c->mk_inner_nb[4 * i + k] = it->second / 4;
+                    c->mk_inner_nb[it->second] = (int32_t)i;
+                }
+                const double *x = m->node_xyz + 3 * (int64_t)o.node_d2r[face_nodes[4 * d + k]];
+                c->mk_R0 = std::max(c->mk_R0, std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]));
+            }
+        }
+    }
     std::vector<double> X(3 * nn), disp;
     for (int64_t d = 0; d < nn; d++)
         for (int k = 0; k < 3; k++) X[3 * d + k] = m->node_xyz[3 * (int64_t)o.node_d2r[d] + k];
@@ -1094,7 +1122,7 @@ extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t
         c->mk_cand_ready = true;
     }
     const std::vector<int32_t> &cand = c->mk_cand;
-    int64_t st[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (!cand.empty()) {
         const int64_t nb = (int64_t)cand.size();
         // geometry of the layer (the fused step keeps only the cell centres current)
@@ -1137,7 +1165,16 @@ extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t
         K.a2 = c->params.par_a_2;
         K.kn = c->params.par_Kn;
         K.max_steps = max_steps;
-        DevBuf dface, dfirst, dmu;
+        K.R0 = c->mk_R0;
+        K.n_inner = (int32_t)c->mk_inner.size();
+        DevBuf dface, dfirst, dmu, dinner, dinner_nb;
+        if (K.n_inner > 0) {
+            rc = dup(dinner, c->mk_inner.data(), sizeof(int32_t) * c->mk_inner.size());
+            if (rc == TITAN_OK) rc = dup(dinner_nb, c->mk_inner_nb.data(), sizeof(int32_t) * c->mk_inner_nb.size());
+            if (rc != TITAN_OK) return rc;
+            K.inner_face = (const int32_t *)dinner.p;
+            K.inner_nb = (const int32_t *)dinner_nb.p;
+        }
         if (K.n_in > 0) {
             rc = dup(dface, face.data(), sizeof(int32_t) * face.size());
             if (rc == TITAN_OK) rc = dup(dfirst, first.data(), sizeof(int64_t) * first.size());
@@ -1153,7 +1190,7 @@ extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t
         double *out = tally + 5 * nc;
         unsigned long long *dst = (unsigned long long *)(out + 5 * nc);
         HIPCHK(hipMemsetAsync(tally, 0, sizeof(double) * 5 * nc, c->stream));
-        HIPCHK(hipMemsetAsync(dst, 0, sizeof(unsigned long long) * 7, c->stream));
+        HIPCHK(hipMemsetAsync(dst, 0, sizeof(unsigned long long) * 8, c->stream));
         K.face = (const int32_t *)dface.p;
         K.first = (const int64_t *)dfirst.p;
         K.mu = (const double *)dmu.p;
@@ -1171,7 +1208,7 @@ extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t
         std::fill(density, density + 5 * nc, 0.0);
     }
     if (stats)
-        for (int i = 0; i < 7; i++) stats[i] = st[i];
+        for (int i = 0; i < 8; i++) stats[i] = st[i];
     return TITAN_OK;
 }
 
