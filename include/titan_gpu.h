@@ -146,6 +146,33 @@ typedef struct {
                                     phi_a + pi/180 (Setka_move.cpp:566-580, 2048-2064)           */
 } titan_rays;
 
+/* The other ray families of Luch::dvigenie (Luch.cpp:153-739), moved after the A rays by every
+ * fitted step and by titan_dvigenie, family by family in the reference's order (Setka.cpp:1850-1880
+ * sorts the rays A, B, C, D, E; H and G follow): B_Luch, C_Luch and C2_Luch, D_Luch, E_Luch, H_Luch,
+ * G_Luch.  Each ray: its nodes (Luch::Yzels, reference ids, inner to outer), support nodes
+ * (Luch::Yzels_opor[0..3], -1 where its type has fewer) and angles (Luch::parameters "the", "phi";
+ * D, E, H and G read only phi).  R0, R1, R5 and M0..M4 are those of titan_set_rays, which must come
+ * first.  Nodes each type needs (indices on the ray): B M0+M1+M2+M3+M4+5, C M0+M1+M11+N4+N5+2,
+ * D M2-M11+M3+M4+4, E M3+M4+3, H 5, G M2-M11-MF+2; support slots read: B 1-3, C 1, D 0-2, E 0-2,
+ * H 0-3, G 0-2.  B sets the x of its support nodes 2, 3 and D of its 1, 2.  Rays of one family run
+ * in parallel unless one reads or writes a node another of its family writes; such a family runs
+ * ray after ray in the given order, as the reference does.  Per-ray overrides of the Geo_param
+ * values (Luch::parameters "dd3", "ba1", ...) are not taken. */
+typedef struct {
+    int64_t n_rays;
+    const int8_t *ray_type;      /* TITAN_RAY_B ... TITAN_RAY_G                                  */
+    const int64_t *ray_off;      /* [n_rays+1] offsets into ray_nodes                            */
+    const int32_t *ray_nodes;
+    const int32_t *ray_opor;     /* [n_rays*4]                                                   */
+    const double *ray_angles;    /* [n_rays*2] (the, phi)                                        */
+    double L6, L7;               /* Geo_param::L6, L7: x of the tail's first and last C-ray node */
+    int32_t N4, N5, MF;          /* Geo_param::N4, N5, MF                                        */
+    double dd1, dd2, dd3, dd4, dd5, dd6, dd7, dd8; /* Geo_param (Hermite slopes; the C rays take
+                                    dd5 and dd6 truncated to int, as Luch.cpp:166-167 does)      */
+    double ba1, ba2, ba3, ba4, ba5, md1, md2, md3, ea1, ea2, ea3; /* Geo_param spacing laws      */
+} titan_rays_ext;
+enum { TITAN_RAY_B = 1, TITAN_RAY_C = 2, TITAN_RAY_D = 3, TITAN_RAY_E = 4, TITAN_RAY_H = 5, TITAN_RAY_G = 6 };
+
 typedef struct {
     int64_t steps_done;
     double dt_last;              /* dt used by the last step                                   */
@@ -342,6 +369,10 @@ int titan_init_tvd(titan_ctx *ctx);
 /* The mesh's rays (the A_Luch / A2_Luch part of Setka::All_Luch): node lists, angles and the
  * Geo_param counts and radii of Luch::dvigenie.  Each node may lie on one ray only. */
 int titan_set_rays(titan_ctx *ctx, const titan_rays *rays);
+/* The B, C, D, E, H and G rays (Luch.cpp:153-739; Setka.cpp:1850-1900 builds them): replaces any
+ * earlier set.  TITAN_E_ARG on a ray too short for its type, a support slot or node out of range,
+ * N5 < 2 with C rays, or no titan_set_rays before; a later titan_set_rays drops them. */
+int titan_set_rays_ext(titan_ctx *ctx, const titan_rays_ext *rays);
 /* Set-up of the surface smoothing from node layer 0 (Yzel::coord[0]): Setka::Smooth_head_TS3
  * (Setka_move.cpp:2205-2363) gives every TS node with at least 3 points its neighbours (the other
  * nodes of its TS faces) and the centre of the sphere fitted to it and them; Smooth_head_HP3

@@ -73,6 +73,7 @@ def lib():
             "or_go": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _P, _P, _P, _I, _P, _P]),
             "or_culc_velocity_surface": (_I, [_P, _P, _P, _P, _P, _P, _D, _P, _P]),
             "or_dvigenie": (None, [_P, _P]),
+            "or_dvigenie_ext": (None, [_P, _P]),
             "or_mk_setup": (_I64, [_P, _P, _D, ctypes.c_int32, _P, _P, _P]),
             "or_mk_go": (_I, [_P, _P, _P, _P, _P, _P, _P]),
             "or_mk_change_velosity_batch": (None, [_I64, ctypes.c_uint64, _D, _P, _P, _D, _I, _P, _P]),
@@ -423,7 +424,55 @@ class OrFit(ctypes.Structure):
                 ("n_rays", _I64), ("npr", ctypes.c_int32), ("ray_nodes", _P), ("ray_angles", _P),
                 ("R0", _D), ("R1", _D), ("R5", _D), ("da1", _D), ("da2", _D), ("da3", _D), ("da4", _D), ("da5", _D),
                 ("M0", ctypes.c_int32), ("M1", ctypes.c_int32), ("M11", ctypes.c_int32), ("M2", ctypes.c_int32),
-                ("M3", ctypes.c_int32), ("M4", ctypes.c_int32)]
+                ("M3", ctypes.c_int32), ("M4", ctypes.c_int32), ("ext", _P)]
+
+
+class OrRaysExt(ctypes.Structure):
+    _fields_ = [("n", _I64), ("type", _P), ("off", _P), ("nodes", _P), ("opor", _P), ("angles", _P),
+                ("R0", _D), ("R1", _D), ("R5", _D), ("L6", _D), ("L7", _D)] + \
+               [(k, ctypes.c_int32) for k in ("M0", "M1", "M11", "M2", "M3", "M4", "MF", "N4", "N5")] + \
+               [(k, _D) for k in ("dd1", "dd2", "dd3", "dd4", "dd5", "dd6", "dd7", "dd8", "ba1", "ba2", "ba3", "ba4",
+                                  "ba5", "md1", "md2", "md3", "ea1", "ea2", "ea3")]
+
+
+# Luch::type of the other ray families (Luch.cpp:153-739)
+RAY_B, RAY_C, RAY_D, RAY_E, RAY_H, RAY_G = 1, 2, 3, 4, 5, 6
+# Geo_param.h:40-84 defaults, with the values Setka.cpp:1130-1136 sets for every run
+EXT_GEO_DEFAULTS = dict(dd1=1.0, dd2=1.0, dd3=19.0, dd4=15.0, dd5=19.0, dd6=14.0, dd7=0.7, dd8=1.0, ba1=1.0, ba2=1.0,
+                        ba3=1.0, ba4=1.0, ba5=0.3, md1=0.2, md2=0.5, md3=0.3, ea1=0.1, ea2=0.5, ea3=0.2,
+                        L6=-60.0, L7=-160.0)
+
+
+class RaysExt:
+    """The other ray families as flat arrays: type [n] int8, off [n+1], nodes, opor [n, 4] (-1 unused),
+    angles [n, 2] (the, phi); geo holds R0, R1, R5, M0..M4, MF, N4, N5 and the EXT_GEO_DEFAULTS keys."""
+
+    def __init__(self, types, node_lists, opor, angles, geo):
+        self.type = np.ascontiguousarray(types, np.int8)
+        self.off = np.zeros(len(node_lists) + 1, np.int64)
+        self.off[1:] = np.cumsum([len(x) for x in node_lists])
+        self.nodes = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for x in node_lists])
+                                          if node_lists else np.zeros(0, np.int32), np.int32)
+        self.opor = np.ascontiguousarray(opor, np.int32).reshape(-1, 4)
+        self.angles = np.ascontiguousarray(angles, np.float64).reshape(-1, 2)
+        self.geo = dict(EXT_GEO_DEFAULTS)
+        self.geo.update(geo)
+
+    def c(self):
+        r = OrRaysExt()
+        r.n = len(self.type)
+        r.type, r.off, r.nodes, r.opor, r.angles = (_p(self.type), _p(self.off), _p(self.nodes), _p(self.opor),
+                                                   _p(self.angles))
+        for k, _ in OrRaysExt._fields_[6:]:
+            setattr(r, k, self.geo[k])
+        return r
+
+
+def dvigenie_ext(ext, x):
+    """or_dvigenie_ext on a copy of x."""
+    xn = np.ascontiguousarray(x, np.float64).copy()
+    lib().or_dvigenie_ext(ctypes.byref(ext.c()), _p(xn))
+    return xn
 
 
 # Phys_param.cpp:500-520
@@ -532,6 +581,7 @@ class OracleFit:
         self.rays["ray_angles"] = np.ascontiguousarray(rays["ray_angles"], np.float64)
         self.phi_lim = float(rays["phi_a"]) + math.pi / 180.0
         self._keep = None
+        self.ext = None  # RaysExt of the other ray families, moved after the A rays
 
     def smooth(self, kind, x0):
         x0 = np.asarray(x0, np.float64)
@@ -577,6 +627,9 @@ class OracleFit:
         f.ray_nodes, f.ray_angles = _p(r["ray_nodes"]), _p(r["ray_angles"])
         for k in ("R0", "R1", "R5", "da1", "da2", "da3", "da4", "da5", "M0", "M1", "M11", "M2", "M3", "M4"):
             setattr(f, k, r[k])
+        if self.ext is not None:
+            self._ext_c = self.ext.c()
+            f.ext = ctypes.cast(ctypes.pointer(self._ext_c), ctypes.c_void_p)
         return f
 
 

@@ -12,6 +12,7 @@
  * a 3-vector; the reference binary cannot be built here (Eigen absent), so this is an assumption
  * that only affects the last bit.
  */
+#define _GNU_SOURCE 1 /* sincos */
 #include "titan_oracle.h"
 #include <math.h>
 #include <stdlib.h>
@@ -1607,6 +1608,11 @@ int or_culc_velocity_surface(const or_mesh *m, const or_params *p, const or_fit 
     return status;
 }
 
+/* cos and sin of a ray angle from one glibc sincos call.  glibc 2.35's sincos and cos differ in the
+ * last bit for some arguments (cos(1.3646395517420515)), and whether a compiler turns cos(a) beside
+ * sin(a) into sincos is its own choice; the library's host set-up takes the ray trig the same way. */
+static void or_sincos(double a, double *s, double *c) { sincos(a, s, c); }
+
 /* Luch::dvigenie for A_Luch / A2_Luch (Luch.cpp:8-151): the radii of a ray's nodes from R0, R1,
  * its TS, HP and BS nodes' distances from the origin in x (func_R, Yzel.cpp:27-30) and R5. */
 void or_dvigenie(const or_fit *fit, double *x)
@@ -1614,7 +1620,9 @@ void or_dvigenie(const or_fit *fit, double *x)
     const int32_t M0 = fit->M0, M1 = fit->M1, M11 = fit->M11, M2 = fit->M2, M3 = fit->M3, M4 = fit->M4;
     for (int64_t k = 0; k < fit->n_rays; k++) {
         const int32_t *Y = fit->ray_nodes + k * fit->npr;
-        double the = fit->ray_angles[2 * k], phi = fit->ray_angles[2 * k + 1];
+        double st, ct, sp, cp;
+        or_sincos(fit->ray_angles[2 * k], &st, &ct);
+        or_sincos(fit->ray_angles[2 * k + 1], &sp, &cp);
         double R0 = fit->R0, R1 = fit->R1, R5 = fit->R5;
         const double *o1 = x + 3 * (int64_t)Y[M0 + M1 + 1];
         const double *o2 = x + 3 * (int64_t)Y[M0 + M1 + M2 + 2];
@@ -1627,9 +1635,9 @@ void or_dvigenie(const or_fit *fit, double *x)
 #define OR_PUT(idx, rad)                                               \
     do {                                                               \
         double *X_ = x + 3 * (int64_t)Y[idx];                          \
-        X_[0] = (rad) * cos(the);                                      \
-        X_[1] = (rad) * sin(the) * cos(phi);                           \
-        X_[2] = (rad) * sin(the) * sin(phi);                           \
+        X_[0] = (rad) * ct;                                            \
+        X_[1] = (rad) * st * cp;                                       \
+        X_[2] = (rad) * st * sp;                                       \
     } while (0)
         for (int j = 0; j < M0; j++) {
             r = R0 + j * (R1 - R0) / M0;
@@ -2141,6 +2149,7 @@ int or_go_k(const or_mesh *m, const or_params *p, double *coords, double *prim, 
             int rc0 = or_culc_velocity_surface(m, p, fit, X[now1], S[now1], &G[now1], time, X[now2], NULL);
             if (rc0 && !status) status = rc0;
             or_dvigenie(fit, X[now2]);
+            if (fit->ext) or_dvigenie_ext(fit->ext, X[now2]);
             if (gcl) or_measure_gcl(m, X[now2], &G[now2]);
             else or_measure(m, X[now2], &G[now2]);
         } else if (p->move_mesh) {
@@ -2175,6 +2184,222 @@ int or_go_k(const or_mesh *m, const or_params *p, double *coords, double *prim, 
     free(dtf);
     free(src);
     return status;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Luch::dvigenie of the B, C/C2, D, E, H and G rays (Luch.cpp:153-739), each restated in its
+ * operation order.  P(k) is node k of the ray; r_yz the distance of a node from the x axis
+ * (sqrt(kv(y) + kv(z)), Yzel::func_Ryz). */
+static double or_ryz(const double *p) { return sqrt(p[1] * p[1] + p[2] * p[2]); }
+
+/* the cubic from (x0, y0) with slope (t1, tt1) to (x1, y1) with slope (t2, tt2) at parameter u */
+static void or_hermite(double x0, double y0, double x1, double y1, double t1, double tt1, double t2, double tt2,
+                       double u, double *x, double *y)
+{
+    double a1 = x0, b1 = t1, c1 = -2 * t1 - t2 - 3.0 * x0 + 3.0 * x1, d1 = t1 + t2 + 2.0 * x0 - 2.0 * x1;
+    double a2 = y0, b2 = tt1, c2 = -2 * tt1 - tt2 - 3.0 * y0 + 3.0 * y1, d2 = tt1 + tt2 + 2.0 * y0 - 2.0 * y1;
+    *x = a1 + b1 * u + c1 * u * u + d1 * u * u * u;
+    *y = a2 + b2 * u + c2 * u * u + d2 * u * u * u;
+}
+
+/* the two spacing laws of the rays: s (a - (2a + b - 3) s + (a + b - 2) s^2) (slopes a, b at the
+ * ends) and a s - (3a + b - 4) s^3 + (2a + b - 3) s^4 */
+static double or_sp1(double s, double a, double b) { return s * (a - (2.0 * a + b - 3.0) * s + (a + b - 2.0) * s * s); }
+static double or_sp2(double s, double a, double b)
+{
+    return a * s - (3.0 * a + b - 4.0) * s * s * s + (2.0 * a + b - 3.0) * s * s * s * s;
+}
+
+void or_dvigenie_ext(const or_rays_ext *R, double *x)
+{
+    static const int order[6] = {OR_RAY_B, OR_RAY_C, OR_RAY_D, OR_RAY_E, OR_RAY_H, OR_RAY_G};
+    for (int fam = 0; fam < 6; fam++) {
+        for (int64_t k = 0; k < R->n; k++) {
+            if (R->type[k] != order[fam]) continue;
+            const int32_t *Y = R->nodes + R->off[k];
+            const int32_t *O = R->opor + 4 * k;
+            double st, ct, sp, cp;
+            or_sincos(R->angles[2 * k], &st, &ct);
+            or_sincos(R->angles[2 * k + 1], &sp, &cp);
+#define P(i) (x + 3 * (int64_t)Y[i])
+#define Q(i) (x + 3 * (int64_t)O[i])
+#define PUT_R(i, rad)                                   \
+    do {                                                \
+        double *X_ = P(i);                              \
+        X_[0] = (rad) * ct;                       \
+        X_[1] = (rad) * st * cp;            \
+        X_[2] = (rad) * st * sp;            \
+    } while (0)
+#define PUT_P(i, xx, rad)                               \
+    do {                                                \
+        double *X_ = P(i);                              \
+        X_[0] = (xx);                                   \
+        X_[1] = (rad) * cp;                       \
+        X_[2] = (rad) * sp;                       \
+    } while (0)
+            const int M0 = R->M0, M1 = R->M1, M11 = R->M11, M2 = R->M2, M3 = R->M3, M4 = R->M4;
+            const double R0 = R->R0, R1 = R->R1, R5 = R->R5;
+            int num = 0;
+            double r = 0.0, xx, yy;
+            if (R->type[k] == OR_RAY_B) { /* Luch.cpp:254-421 */
+                const double *o1 = Q(1), *o2 = Q(2), *o3 = Q(3);
+                double R2 = sqrt(KVV(o1[0], o1[1], o1[2]));
+                double R3 = sqrt(KV(o2[1]) + KV(o2[2]));
+                double R4 = sqrt(KV(o3[1]) + KV(o3[2]));
+                for (int j = 0; j < M0; j++) {
+                    r = R0 + j * (R1 - R0) / M0;
+                    PUT_R(j + num, r);
+                }
+                num += M0;
+                PUT_R(num, R1);
+                num += 1;
+                for (int j = 0; j < M1; j++) {
+                    r = R1 + (j + 1) * (R2 - R1) / (M1 + 1);
+                    PUT_R(j + num, r);
+                }
+                num += M1 + 1;
+                double dr = (R2 - R1) / (M1 + 1);
+                for (int j = 0; j < M11; j++) {
+                    r = R2 + dr * (j + 1);
+                    PUT_R(num + j, r);
+                }
+                num += M11;
+                double x3 = r * ct;
+                Q(2)[0] = x3;
+                Q(3)[0] = x3;
+                double t1 = ct * R->dd3, tt1 = st * R->dd3, t2 = 0.0, tt2 = 1.0 * R->dd4;
+                double x0 = r * ct, y0 = r * st;
+                for (int j = 0; j < M2 - M11; j++) {
+                    double s = 1.0 * (j + 1) / (M2 - M11 + 1);
+                    double ss = or_sp1(s, R->ba1, R->ba2);
+                    or_hermite(x0, y0, x3, R3, t1, tt1, t2, tt2, ss, &xx, &yy);
+                    PUT_P(num + j, xx, yy);
+                }
+                num += (M2 - M11) + 1;
+                for (int j = 0; j < M3; j++) {
+                    double s = (j + 1.0) / (M3 + 1.0);
+                    r = R3 + or_sp1(s, R->ba3, R->ba4) * (R4 - R3);
+                    PUT_P(num + j, x3, r);
+                }
+                num += M3 + 1;
+                for (int j = 0; j < M4; j++) {
+                    double s = (j + 1.0) / (M4 + 1.0);
+                    r = R4 + (R5 - R4) * or_sp2(s, R->ba5, 2.0);
+                    PUT_P(num + j, x3, r);
+                }
+                num += M4;
+                PUT_P(num, x3, R5);
+            } else if (R->type[k] == OR_RAY_C) { /* Luch.cpp:153-252 (C_Luch and C2_Luch) */
+                const double *o1 = Q(1);
+                double R2 = sqrt(KVV(o1[0], o1[1], o1[2]));
+                const int dd5 = (int)R->dd5, dd6 = (int)R->dd6; /* int in the reference */
+                for (int j = 0; j < M0; j++) {
+                    r = R0 + j * (R1 - R0) / M0;
+                    PUT_R(num + j, r);
+                }
+                num += M0 + 1;
+                for (int j = 0; j < M1; j++) {
+                    r = R1 + (j + 1) * (R2 - R1) / (M1 + 1);
+                    PUT_R(num + j, r);
+                }
+                double dr = (R2 - R1) / (M1 + 1);
+                num += M1 + 1;
+                for (int j = 0; j < M11; j++) {
+                    r = R2 + dr * (j + 1);
+                    PUT_R(num + j, r);
+                }
+                num += M11;
+                double x0 = r * ct, y0 = r * st;
+                double t1 = (ct - 1.0) / 2.0 * dd5, tt1 = st * dd5, t2 = -1.0 * dd6, tt2 = 0.0;
+                double y = 0.0, z = 0.0;
+                for (int j = 0; j < R->N4; j++) {
+                    double s = 1.0 * (j + 1) / (R->N4 + 1);
+                    or_hermite(x0, y0, R->L6, y0, t1, tt1, t2, tt2, s, &xx, &yy);
+                    z = yy * sp;
+                    y = yy * cp;
+                    P(num + j)[0] = xx;
+                    P(num + j)[1] = y;
+                    P(num + j)[2] = z;
+                }
+                num += R->N4;
+                for (int j = 0; j < R->N5; j++) {
+                    P(num + j)[0] = R->L6 + (j) * (R->L7 - R->L6) / (R->N5 - 1);
+                    P(num + j)[1] = y;
+                    P(num + j)[2] = z;
+                }
+            } else if (R->type[k] == OR_RAY_D) { /* Luch.cpp:423-523 */
+                double x0 = Q(0)[0], y0 = sqrt(KV(Q(0)[1]) + KV(Q(0)[2]));
+                Q(1)[0] = x0;
+                double y1 = sqrt(KV(Q(1)[1]) + KV(Q(1)[2]));
+                Q(2)[0] = x0;
+                double y2 = sqrt(KV(Q(2)[1]) + KV(Q(2)[2]));
+                num += 1;
+                for (int j = 0; j < M2 - M11; j++) {
+                    double s = 1.0 * (j + 1) / (M2 - M11 + 1);
+                    r = y0 + (y1 - y0) * or_sp1(s, 0.1, 0.3);
+                    PUT_P(num + j, x0, r);
+                }
+                num += (M2 - M11) + 1;
+                for (int j = 0; j < M3; j++) {
+                    double s = 1.0 * (j + 1) / (M3 + 1);
+                    r = y1 + (y2 - y1) * or_sp1(s, R->md1, R->md2);
+                    PUT_P(num + j, x0, r);
+                }
+                num += M3 + 1;
+                for (int j = 0; j < M4; j++) {
+                    double s = (j + 1.0) / (M4 + 1.0);
+                    r = y2 + (R5 - y2) * or_sp2(s, R->md3, 2.0);
+                    PUT_P(num + j, x0, r);
+                }
+                num += M4;
+                PUT_P(num, x0, R5);
+            } else if (R->type[k] == OR_RAY_E) { /* Luch.cpp:525-602 */
+                double y0 = sqrt(KV(Q(1)[1]) + KV(Q(1)[2])), y1 = sqrt(KV(Q(2)[1]) + KV(Q(2)[2]));
+                double xe = Q(0)[0];
+                P(num)[0] = xe;
+                num += 1;
+                for (int j = 0; j < M3; j++) {
+                    double s = 1.0 * (j + 1) / (M3 + 1);
+                    r = y0 + (y1 - y0) * or_sp1(s, R->ea1, R->ea2);
+                    PUT_P(num + j, xe, r);
+                }
+                num += M3;
+                P(num)[0] = xe;
+                num += 1;
+                for (int j = 0; j < M4; j++) {
+                    double s = (j + 1.0) / (M4 + 1.0);
+                    r = y1 + (R5 - y1) * or_sp2(s, R->ea3, 2.0);
+                    PUT_P(num + j, xe, r);
+                }
+                num += M4;
+                PUT_P(num, xe, R5);
+            } else if (R->type[k] == OR_RAY_H) { /* Luch.cpp:604-661 */
+                const double *z3 = Q(0), *z1 = Q(1), *z4 = Q(2), *z2 = Q(3);
+                double t1 = (z1[0] - z3[0]) * R->dd7, tt1 = (or_ryz(z1) - or_ryz(z3)) * R->dd7;
+                double t2 = (z4[0] - z2[0]) * R->dd8, tt2 = (or_ryz(z4) - or_ryz(z2)) * R->dd8;
+                num += 1;
+                for (int j = 0; j < 4; j++) {
+                    double s = 1.0 * (j + 1) / (5);
+                    or_hermite(z1[0], or_ryz(z1), z2[0], or_ryz(z2), t1, tt1, t2, tt2, s, &xx, &yy);
+                    PUT_P(num + j, xx, yy);
+                }
+            } else if (R->type[k] == OR_RAY_G) { /* Luch.cpp:663-736 */
+                const double *z1 = Q(1), *z2 = Q(2), *z3 = Q(0);
+                double t1 = (z1[0] - z3[0]) * R->dd1, tt1 = (or_ryz(z1) - or_ryz(z3)) * R->dd1;
+                double t2 = 0.0, tt2 = 1.0 * R->dd2;
+                num = 1;
+                for (int j = 0; j < M2 - M11 - R->MF + 1; j++) {
+                    double s = 1.0 * (j + 1.0) / (M2 - M11 - R->MF + 1 + 1.0);
+                    or_hermite(z1[0], or_ryz(z1), z2[0], or_ryz(z2), t1, tt1, t2, tt2, s, &xx, &yy);
+                    PUT_P(num + j, xx, yy);
+                }
+            }
+#undef P
+#undef Q
+#undef PUT_R
+#undef PUT_P
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------------------------ */

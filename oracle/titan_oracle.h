@@ -253,7 +253,30 @@ typedef struct {
     const double *ray_angles;    /* [n_rays*2] the, phi */
     double R0, R1, R5, da1, da2, da3, da4, da5;
     int32_t M0, M1, M11, M2, M3, M4;
+    const struct or_rays_ext_s *ext; /* the other ray families (nullable), moved after the A rays */
 } or_fit;
+
+/* The other ray families of Luch::dvigenie (Luch.cpp:153-739): B_Luch (around the flank), C_Luch /
+ * C2_Luch (tailward, bending to the x axis), D_Luch, E_Luch (in planes x = const of the tail),
+ * H_Luch, G_Luch (cubic bridges between support nodes).  Each ray: its type, nodes (Luch::Yzels,
+ * inner to outer), support nodes (Luch::Yzels_opor[0..3], -1 where the type has fewer) and angles.
+ * or_dvigenie_ext moves the families in the reference's order (Setka.cpp:1850-1880: B, C, D, E,
+ * then H, G), one ray after another; B and D also set the x of support nodes of theirs.  Geo_param
+ * values as named; per-ray overrides of them are not taken. */
+enum { OR_RAY_B = 1, OR_RAY_C = 2, OR_RAY_D = 3, OR_RAY_E = 4, OR_RAY_H = 5, OR_RAY_G = 6 };
+typedef struct or_rays_ext_s {
+    int64_t n;
+    const int8_t *type;
+    const int64_t *off;        /* [n+1] into nodes */
+    const int32_t *nodes;
+    const int32_t *opor;       /* [n*4] */
+    const double *angles;      /* [n*2] the, phi */
+    double R0, R1, R5, L6, L7;
+    int32_t M0, M1, M11, M2, M3, M4, MF, N4, N5;
+    double dd1, dd2, dd3, dd4, dd5, dd6, dd7, dd8;
+    double ba1, ba2, ba3, ba4, ba5, md1, md2, md3, ea1, ea2, ea3;
+} or_rays_ext;
+void or_dvigenie_ext(const or_rays_ext *r, double *x);
 /* Surface velocities from layer `now` (x, prim, geometry g of that layer) moved over `time` into
  * x_next (surface nodes only); node_velocity [n_sn*3] (nullable). Returns -1 on a chlld failure,
  * -8 on a NaN/subnormal HP contact speed. */

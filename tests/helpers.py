@@ -1,4 +1,6 @@
 """Shared generators for the test suite."""
+import math
+
 import numpy as np
 
 
@@ -77,3 +79,106 @@ def cell_err(got, ref):
 def node_err(got, ref):
     """Largest per-node relative position error |dx|/|x|."""
     return float((np.linalg.norm(np.asarray(got) - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-300)).max())
+
+
+# ---- synthetic sets of the B, C, D, E, H, G rays (Luch.cpp:153-739) ------------------------------
+EXT_M = dict(M0=3, M1=4, M11=2, M2=8, M3=5, M4=4, MF=2, N4=5, N5=6)
+
+
+def ext_ray_need(t, g):
+    """Nodes a ray of type t needs (the index count Luch::dvigenie runs over)."""
+    M0, M1, M11, M2, M3, M4 = (g[k] for k in ("M0", "M1", "M11", "M2", "M3", "M4"))
+    return {1: M0 + M1 + M2 + M3 + M4 + 5, 2: M0 + M1 + M11 + g["N4"] + g["N5"] + 2, 3: M2 - M11 + M3 + M4 + 4,
+            4: M3 + M4 + 3, 5: 5, 6: M2 - M11 - g["MF"] + 2}[t]
+
+
+def ext_ray_set(rng, first_node, n_nodes, counts=(6, 6, 6, 6, 6, 6), serial=(), geo=None):
+    """Rays of each family over nodes first_node.. of a node array of n_nodes, laid out as the
+    reference builds them (Setka.cpp:1600-1900): B rays run out along (the, phi) with support nodes
+    at their own TS, HP and BS indices; C rays bend tailward from their TS node; D rays start at a
+    node of a C ray's bend, E rays take their x from a C ray's tail node; H rays bridge a B ray's HP
+    node to a D ray's, G rays a B ray's TS node to its HP node.  Families named in `serial` get rays
+    that read a node the previous ray of the family writes.  Returns (types, node_lists, opor
+    [n, 4], angles [n, 2], geo, xyz [n_nodes, 3] with plausible starting positions of the nodes
+    used)."""
+    g = dict(EXT_M, R0=1.0, R1=10.0, R5=600.0, L6=-60.0, L7=-160.0, dd1=1.0, dd2=1.0, dd3=19.0, dd4=15.0, dd5=19.0,
+             dd6=14.0, dd7=0.7, dd8=1.0, ba1=1.3, ba2=0.8, ba3=0.7, ba4=1.6, ba5=0.3, md1=0.2, md2=0.5, md3=0.3,
+             ea1=0.1, ea2=0.5, ea3=0.2)
+    g.update(geo or {})
+    M0, M1, M2, M3 = g["M0"], g["M1"], g["M2"], g["M3"]
+    xyz = rng.uniform(-300.0, 300.0, (n_nodes, 3))
+    nxt = [first_node]
+
+    def take(n):
+        a = nxt[0]
+        nxt[0] += n
+        assert nxt[0] <= n_nodes
+        return list(range(a, a + n))
+
+    def put(v, r, the, phi, jit=0.05):
+        d = np.array([math.cos(the), math.sin(the) * math.cos(phi), math.sin(the) * math.sin(phi)])
+        xyz[v] = r * d * (1.0 + jit * rng.uniform(-1, 1))
+
+    def put_cyl(v, x, r, phi):
+        xyz[v] = [x, r * math.cos(phi), r * math.sin(phi)]
+
+    types, lists, opor, angles = [], [], [], []
+    iTS, iHP, iBS = M0 + M1 + 1, M0 + M1 + M2 + 2, M0 + M1 + M2 + M3 + 3
+    B, Cr, D = [], [], []
+    for _ in range(counts[0]):
+        the, phi = rng.uniform(1.2, 1.9), rng.uniform(0, 2 * math.pi)
+        nodes = take(ext_ray_need(1, g))
+        put(nodes[iTS], rng.uniform(75, 95), the, phi)
+        put_cyl(nodes[iHP], rng.uniform(-20, 20), rng.uniform(110, 140), phi)
+        put_cyl(nodes[iBS], rng.uniform(-20, 20), rng.uniform(220, 280), phi)
+        types.append(1); lists.append(nodes); angles.append((the, phi))
+        opor.append([nodes[M0], nodes[iTS], nodes[iHP], nodes[iBS]])
+        B.append(nodes)
+    c_bend = M0 + M1 + g["M11"] + 2
+    for _ in range(counts[1]):
+        the, phi = rng.uniform(2.0, 2.8), rng.uniform(0, 2 * math.pi)
+        nodes = take(ext_ray_need(2, g))
+        put(nodes[iTS], rng.uniform(85, 110), the, phi)
+        types.append(2); lists.append(nodes); angles.append((the, phi))
+        opor.append([nodes[M0], nodes[iTS], -1, -1])
+        Cr.append(nodes)
+    nD = ext_ray_need(3, g)
+    for i in range(counts[2]):
+        c = Cr[i % len(Cr)]
+        phi = rng.uniform(0, 2 * math.pi)
+        own = take(nD - 1)
+        nodes = [c[c_bend + int(rng.integers(0, g["N4"]))]] + own
+        hp, bs = nodes[M2 - g["M11"] + 1], nodes[M2 - g["M11"] + M3 + 2]
+        put_cyl(hp, rng.uniform(-40, 0), rng.uniform(115, 145), phi)
+        put_cyl(bs, rng.uniform(-40, 0), rng.uniform(230, 290), phi)
+        types.append(3); lists.append(nodes); angles.append((0.0, phi))
+        opor.append([nodes[0], hp, bs, -1])
+        D.append(nodes)
+    for i in range(counts[3]):
+        c = Cr[i % len(Cr)]
+        phi = rng.uniform(0, 2 * math.pi)
+        nodes = take(ext_ray_need(4, g))
+        put_cyl(nodes[0], rng.uniform(-120, -80), rng.uniform(120, 150), phi)
+        put_cyl(nodes[M3 + 1], rng.uniform(-120, -80), rng.uniform(240, 300), phi)
+        types.append(4); lists.append(nodes); angles.append((0.0, phi))
+        opor.append([c[c_bend + g["N4"] + int(rng.integers(0, g["N5"]))], nodes[0], nodes[M3 + 1], -1])
+    prev = None
+    for i in range(counts[4]):
+        b, d = B[i % len(B)], D[i % len(D)]
+        own = take(4)
+        z3 = b[iHP - 1] if not (5 in serial and prev is not None) else prev[2]
+        z1, z2, z4 = b[iHP], d[M2 - g["M11"] + 1], d[M2 - g["M11"] + 2]
+        types.append(5); lists.append([z1] + own + [z2]); angles.append((0.0, rng.uniform(0, 2 * math.pi)))
+        opor.append([z3, z1, z4, z2])
+        prev = own
+    prev = None
+    nG = g["M2"] - g["M11"] - g["MF"] + 1
+    for i in range(counts[5]):
+        b = B[i % len(B)]
+        own = take(nG)
+        z3 = b[iTS - 1] if not (6 in serial and prev is not None) else prev[1]
+        types.append(6); lists.append([b[iTS]] + own + [b[iHP]]); angles.append((0.0, rng.uniform(0, 2 * math.pi)))
+        opor.append([z3, b[iTS], b[iHP], -1])
+        prev = own
+    return (np.asarray(types, np.int8), lists, np.asarray(opor, np.int32), np.asarray(angles, np.float64), g,
+            xyz)

@@ -86,6 +86,19 @@ class RaysC(ctypes.Structure):
                 ("phi_a", ctypes.c_double)]
 
 
+class RaysExtC(ctypes.Structure):
+    _fields_ = [("n_rays", ctypes.c_int64), ("ray_type", ctypes.c_void_p), ("ray_off", ctypes.c_void_p),
+                ("ray_nodes", ctypes.c_void_p), ("ray_opor", ctypes.c_void_p), ("ray_angles", ctypes.c_void_p),
+                ("L6", ctypes.c_double), ("L7", ctypes.c_double), ("N4", ctypes.c_int32), ("N5", ctypes.c_int32),
+                ("MF", ctypes.c_int32)] + \
+               [(k, ctypes.c_double) for k in ("dd1", "dd2", "dd3", "dd4", "dd5", "dd6", "dd7", "dd8", "ba1", "ba2",
+                                               "ba3", "ba4", "ba5", "md1", "md2", "md3", "ea1", "ea2", "ea3")]
+
+
+# Luch::type of the other ray families (Luch.cpp:153-739)
+RAY_B, RAY_C, RAY_D, RAY_E, RAY_H, RAY_G = 1, 2, 3, 4, 5, 6
+
+
 class StatsC(ctypes.Structure):
     _fields_ = [("steps_done", ctypes.c_int64), ("dt_last", ctypes.c_double), ("dt_next", ctypes.c_double),
                 ("t_sim", ctypes.c_double), ("n_rho_floor", ctypes.c_int64), ("n_p_floor", ctypes.c_int64),
@@ -136,6 +149,7 @@ _SIGS = {
     "titan_download_cell_parameters": (_I, [_P, ctypes.c_char_p]),
     "titan_init_tvd": (_I, [_P]),
     "titan_set_rays": (_I, [_P, ctypes.POINTER(RaysC)]),
+    "titan_set_rays_ext": (_I, [_P, ctypes.POINTER(RaysExtC)]),
     "titan_smooth_head_ts3": (_I, [_P]),
     "titan_smooth_head_hp3": (_I, [_P]),
     "titan_find_yzel_sosed_for_bs": (_I, [_P]),
@@ -462,6 +476,26 @@ class Setka:
         for k in ("R0", "R1", "R5", "M0", "M1", "M11", "M2", "M3", "M4", "da1", "da2", "da3", "da4", "da5", "phi_a"):
             setattr(r, k, rays[k])
         _check(lib().titan_set_rays(self.ctx, ctypes.byref(r)), "titan_set_rays", self.ctx)
+
+    def set_rays_ext(self, types, node_lists, opor, angles, geo):
+        """The B, C, D, E, H and G rays: types [n] (RAY_B ... RAY_G), node_lists (reference node ids
+        of each ray, inner to outer), opor [n, 4] support nodes (-1 unused), angles [n, 2] (the, phi),
+        geo: L6, L7, N4, N5, MF, dd1..dd8, ba1..ba5, md1..md3, ea1..ea3 (Geo_param)."""
+        t = np.ascontiguousarray(types, np.int8)
+        off = np.zeros(len(node_lists) + 1, np.int64)
+        off[1:] = np.cumsum([len(x) for x in node_lists])
+        nodes = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for x in node_lists])
+                                     if len(node_lists) else np.zeros(1, np.int32), np.int32)
+        op = np.ascontiguousarray(opor, np.int32).reshape(-1, 4)
+        ang = np.ascontiguousarray(angles, np.float64).reshape(-1, 2)
+        r = RaysExtC()
+        r.n_rays = len(t)
+        r.ray_type, r.ray_off, r.ray_nodes, r.ray_opor, r.ray_angles = (_ptr(t), _ptr(off), _ptr(nodes), _ptr(op),
+                                                                        _ptr(ang))
+        for k, _ in RaysExtC._fields_[6:]:
+            setattr(r, k, geo[k])
+        self._rays_ext_keep = (t, off, nodes, op, ang)
+        _check(lib().titan_set_rays_ext(self.ctx, ctypes.byref(r)), "titan_set_rays_ext", self.ctx)
 
     def Smooth_head_TS3(self):
         _check(lib().titan_smooth_head_ts3(self.ctx), "titan_smooth_head_ts3", self.ctx)

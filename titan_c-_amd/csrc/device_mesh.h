@@ -77,6 +77,23 @@ constexpr int kScal = 2;
 // Fitted surfaces and their rays (Setka::Culc_Velocity_surface, Setka_move.cpp:111-1357, and
 // Luch::dvigenie of the A_Luch rays, Luch.cpp:6-152).  Surface kinds: 1 TS, 2 HP, 3 BS; arrays
 // indexed by kind have slot 0 unused.
+// The B, C, D, E, H, G rays of Luch::dvigenie (Luch.cpp:153-739).  Rays sit in `order` family by
+// family (fam_off[f] .. fam_off[f+1] for family f = 1..6), the given order within one.
+struct RayExtDev {
+    int64_t n;
+    const int8_t *type;          // [n] 1 B, 2 C/C2, 3 D, 4 E, 5 H, 6 G
+    const int64_t *off;          // [n+1] into nodes
+    const int32_t *nodes;        // device nodes, inner to outer
+    const int32_t *opor;         // [n*4] device support nodes, -1 unused
+    const double *trig;          // [n*4] cos(the), sin(the), cos(phi), sin(phi)
+    const int32_t *order;        // rays by family
+    int64_t fam_off[8];
+    int32_t serial;              // bit f: family f has rays that touch a node another of them writes
+    int32_t N4, N5, MF;
+    double L6, L7, dd1, dd2, dd3, dd4, dd5i, dd6i, dd7, dd8;  // dd5i, dd6i truncated to integers
+    double ba1, ba2, ba3, ba4, ba5, md1, md2, md3, ea1, ea2, ea3;
+};
+
 struct FitDev {
     int32_t n_sf, n_sn;
     const int32_t *sf_face;      // [n_sf] device face of each surface face
@@ -103,6 +120,7 @@ struct FitDev {
     double velocity[4];          // velocity_TS, velocity_HP, velocity_BS
     double k_smooth[4];          // sglag_TS_k_sphere, sglag_HP_k_sphere, sglag_BS_k
     double phi_lim;              // HP sphere smoothing where the polar angle is <= phi_a + pi/180
+    RayExtDev ext;               // the other ray families (ext.n = 0: none)
 };
 
 // Monte-Carlo neutrals (Setka::MK_go, Setka_MK.cpp:1225-2316): particles are numbered face by face

@@ -1,15 +1,17 @@
 // k_fit.hip -- fitted-surface node motion: Setka::Culc_Velocity_surface (Setka_move.cpp:111-1357)
-// and Luch::dvigenie for the A_Luch / A2_Luch rays (Luch.cpp:6-152), the production caller of
+// and Luch::dvigenie for every ray family (Luch.cpp:6-739), the production caller of
 // the moving-mesh step (Setka_physics.cpp:1616-1627).
 //
-// Three kernels per step, all FP64 and in the reference's operation order:
+// The kernels of a step, all FP64 and in the reference's operation order:
 //   k_fit_faces   one thread per TS/HP/BS face: the face's states at the current layer, chlld with
 //                 w = 0, and velocity_K * (S_L | S_M | S_R) * n;
 //   k_fit_nodes   one thread per surface node: the average of its faces' vectors and of the sphere
 //                 (TS, HP) or neighbour-mean (BS) smoothing term, then the move along the radius
 //                 (HP behind the Sun: along the cylinder radius);
 //   k_fit_ray_radii, k_fit_rays  each ray's moved TS/HP/BS radii, then one thread per device node:
-//                 the A_Luch distribution between them.
+//                 the A_Luch distribution between them;
+//   k_fit_rays_ext  the B, C, D, E, H and G rays (Luch.cpp:153-739), one launch per family in that
+//                 order, one thread per ray.
 // The reference adds the face vectors into each node under a mutex from an OpenMP loop, so its
 // summation order varies from run to run; here a node sums them in ascending reference face id,
 // which makes the motion repeat bit for bit.
@@ -198,6 +200,203 @@ __global__ void __launch_bounds__(256) k_fit_rays(FitDev F, double *__restrict__
     X[3 * g + 2] = r * tr[1] * tr[3];
 }
 
+namespace {
+// the cubic from (x0, y0) with slope (t1, tt1) to (x1, y1) with slope (t2, tt2), at u
+__device__ inline void hermite(double x0, double y0, double x1, double y1, double t1, double tt1, double t2,
+                               double tt2, double u, double &x, double &y) {
+    const double a1 = x0, b1 = t1, c1 = -2 * t1 - t2 - 3.0 * x0 + 3.0 * x1, d1 = t1 + t2 + 2.0 * x0 - 2.0 * x1;
+    const double a2 = y0, b2 = tt1, c2 = -2 * tt1 - tt2 - 3.0 * y0 + 3.0 * y1, d2 = tt1 + tt2 + 2.0 * y0 - 2.0 * y1;
+    x = a1 + b1 * u + c1 * u * u + d1 * u * u * u;
+    y = a2 + b2 * u + c2 * u * u + d2 * u * u * u;
+}
+// spacing with end slopes a, b: s (a - (2a + b - 3) s + (a + b - 2) s^2)
+__device__ inline double spacing1(double s, double a, double b) {
+    return s * (a - (2.0 * a + b - 3.0) * s + (a + b - 2.0) * s * s);
+}
+// spacing with slope a at 0, b at 1 and no curvature at 0: a s - (3a + b - 4) s^3 + (2a + b - 3) s^4
+__device__ inline double spacing2(double s, double a, double b) {
+    return a * s - (3.0 * a + b - 4.0) * s * s * s + (2.0 * a + b - 3.0) * s * s * s * s;
+}
+__device__ inline double ryz(const double *p) { return sqrt(0.0 * 0.0 + p[1] * p[1] + p[2] * p[2]); }
+
+// One ray of family B..G, in the reference's operation order.  pol(): a node at polar radius r
+// (x = r cos the, y = r sin the cos phi, z = r sin the sin phi); cyl(): at x, distance r from the axis.
+__device__ void move_ext_ray(const FitDev &F, int64_t k, double *__restrict__ X) {
+    const RayExtDev &E = F.ext;
+    const int32_t *Y = E.nodes + E.off[k];
+    const int32_t *O = E.opor + 4 * k;
+    const double ct = E.trig[4 * k], st = E.trig[4 * k + 1], cp = E.trig[4 * k + 2], sp = E.trig[4 * k + 3];
+    auto P = [&](int i) { return X + 3 * (int64_t)Y[i]; };
+    auto Q = [&](int i) { return X + 3 * (int64_t)O[i]; };
+    auto pol = [&](int i, double r) {
+        double *p = P(i);
+        p[0] = r * ct;
+        p[1] = r * st * cp;
+        p[2] = r * st * sp;
+    };
+    auto cyl = [&](int i, double x, double r) {
+        double *p = P(i);
+        p[0] = x;
+        p[1] = r * cp;
+        p[2] = r * sp;
+    };
+    const int M0 = F.M0, M1 = F.M1, M11 = F.M11, M2 = F.M2, M3 = F.M3, M4 = F.M4;
+    const double R0 = F.R0, R1 = F.R1, R5 = F.R5;
+    const int t = E.type[k];
+    int num = 0;
+    double r = 0.0, xx, yy;
+    if (t == 1 || t == 2) {  // B (Luch.cpp:254-421), C / C2 (:153-252): the inner, common part
+        const double *o1 = Q(1);
+        const double R2 = sqrt(o1[0] * o1[0] + o1[1] * o1[1] + o1[2] * o1[2]);
+        double R3 = 0.0, R4 = 0.0;
+        if (t == 1) {
+            R3 = sqrt(Q(2)[1] * Q(2)[1] + Q(2)[2] * Q(2)[2]);
+            R4 = sqrt(Q(3)[1] * Q(3)[1] + Q(3)[2] * Q(3)[2]);
+        }
+        for (int j = 0; j < M0; j++) {
+            r = R0 + j * (R1 - R0) / M0;
+            pol(j, r);
+        }
+        num = M0;
+        if (t == 1) pol(num, R1);
+        num += 1;
+        for (int j = 0; j < M1; j++) {
+            r = R1 + (j + 1) * (R2 - R1) / (M1 + 1);
+            pol(num + j, r);
+        }
+        num += M1 + 1;
+        const double dr = (R2 - R1) / (M1 + 1);
+        for (int j = 0; j < M11; j++) {
+            r = R2 + dr * (j + 1);
+            pol(num + j, r);
+        }
+        num += M11;
+        const double x0 = r * ct, y0 = r * st;
+        if (t == 1) {
+            const double x3 = r * ct;
+            Q(2)[0] = x3;
+            Q(3)[0] = x3;
+            const double t1 = ct * E.dd3, tt1 = st * E.dd3, t2 = 0.0, tt2 = 1.0 * E.dd4;
+            for (int j = 0; j < M2 - M11; j++) {
+                const double s = 1.0 * (j + 1) / (M2 - M11 + 1);
+                hermite(x0, y0, x3, R3, t1, tt1, t2, tt2, spacing1(s, E.ba1, E.ba2), xx, yy);
+                cyl(num + j, xx, yy);
+            }
+            num += (M2 - M11) + 1;
+            for (int j = 0; j < M3; j++) {
+                const double s = (j + 1.0) / (M3 + 1.0);
+                r = R3 + spacing1(s, E.ba3, E.ba4) * (R4 - R3);
+                cyl(num + j, x3, r);
+            }
+            num += M3 + 1;
+            for (int j = 0; j < M4; j++) {
+                const double s = (j + 1.0) / (M4 + 1.0);
+                r = R4 + (R5 - R4) * spacing2(s, E.ba5, 2.0);
+                cyl(num + j, x3, r);
+            }
+            num += M4;
+            cyl(num, x3, R5);
+        } else {
+            const double t1 = (ct - 1.0) / 2.0 * E.dd5i, tt1 = st * E.dd5i, t2 = -1.0 * E.dd6i, tt2 = 0.0;
+            double y = 0.0, z = 0.0;
+            for (int j = 0; j < E.N4; j++) {
+                const double s = 1.0 * (j + 1) / (E.N4 + 1);
+                hermite(x0, y0, E.L6, y0, t1, tt1, t2, tt2, s, xx, yy);
+                z = yy * sp;
+                y = yy * cp;
+                double *p = P(num + j);
+                p[0] = xx;
+                p[1] = y;
+                p[2] = z;
+            }
+            num += E.N4;
+            for (int j = 0; j < E.N5; j++) {
+                double *p = P(num + j);
+                p[0] = E.L6 + (j) * (E.L7 - E.L6) / (E.N5 - 1);
+                p[1] = y;
+                p[2] = z;
+            }
+        }
+    } else if (t == 3) {  // D (Luch.cpp:423-523)
+        const double x0 = Q(0)[0], y0 = sqrt(Q(0)[1] * Q(0)[1] + Q(0)[2] * Q(0)[2]);
+        Q(1)[0] = x0;
+        const double y1 = sqrt(Q(1)[1] * Q(1)[1] + Q(1)[2] * Q(1)[2]);
+        Q(2)[0] = x0;
+        const double y2 = sqrt(Q(2)[1] * Q(2)[1] + Q(2)[2] * Q(2)[2]);
+        num = 1;
+        for (int j = 0; j < M2 - M11; j++) {
+            const double s = 1.0 * (j + 1) / (M2 - M11 + 1);
+            cyl(num + j, x0, y0 + (y1 - y0) * spacing1(s, 0.1, 0.3));
+        }
+        num += (M2 - M11) + 1;
+        for (int j = 0; j < M3; j++) {
+            const double s = 1.0 * (j + 1) / (M3 + 1);
+            cyl(num + j, x0, y1 + (y2 - y1) * spacing1(s, E.md1, E.md2));
+        }
+        num += M3 + 1;
+        for (int j = 0; j < M4; j++) {
+            const double s = (j + 1.0) / (M4 + 1.0);
+            cyl(num + j, x0, y2 + (R5 - y2) * spacing2(s, E.md3, 2.0));
+        }
+        num += M4;
+        cyl(num, x0, R5);
+    } else if (t == 4) {  // E (Luch.cpp:525-602)
+        const double y0 = sqrt(Q(1)[1] * Q(1)[1] + Q(1)[2] * Q(1)[2]);
+        const double y1 = sqrt(Q(2)[1] * Q(2)[1] + Q(2)[2] * Q(2)[2]);
+        const double xe = Q(0)[0];
+        P(0)[0] = xe;
+        num = 1;
+        for (int j = 0; j < M3; j++) {
+            const double s = 1.0 * (j + 1) / (M3 + 1);
+            cyl(num + j, xe, y0 + (y1 - y0) * spacing1(s, E.ea1, E.ea2));
+        }
+        num += M3;
+        P(num)[0] = xe;
+        num += 1;
+        for (int j = 0; j < M4; j++) {
+            const double s = (j + 1.0) / (M4 + 1.0);
+            cyl(num + j, xe, y1 + (R5 - y1) * spacing2(s, E.ea3, 2.0));
+        }
+        num += M4;
+        cyl(num, xe, R5);
+    } else if (t == 5) {  // H (Luch.cpp:604-661)
+        const double *z3 = Q(0), *z1 = Q(1), *z4 = Q(2), *z2 = Q(3);
+        const double t1 = (z1[0] - z3[0]) * E.dd7, tt1 = (ryz(z1) - ryz(z3)) * E.dd7;
+        const double t2 = (z4[0] - z2[0]) * E.dd8, tt2 = (ryz(z4) - ryz(z2)) * E.dd8;
+        const double x0 = z1[0], y0 = ryz(z1), x1 = z2[0], y1 = ryz(z2);
+        for (int j = 0; j < 4; j++) {
+            const double s = 1.0 * (j + 1) / (5);
+            hermite(x0, y0, x1, y1, t1, tt1, t2, tt2, s, xx, yy);
+            cyl(1 + j, xx, yy);
+        }
+    } else if (t == 6) {  // G (Luch.cpp:663-736)
+        const double *z1 = Q(1), *z2 = Q(2), *z3 = Q(0);
+        const double t1 = (z1[0] - z3[0]) * E.dd1, tt1 = (ryz(z1) - ryz(z3)) * E.dd1;
+        const double t2 = 0.0, tt2 = 1.0 * E.dd2;
+        const double x0 = z1[0], y0 = ryz(z1), x1 = z2[0], y1 = ryz(z2);
+        const int n = M2 - M11 - E.MF + 1;
+        for (int j = 0; j < n; j++) {
+            const double s = 1.0 * (j + 1.0) / (n + 1.0);
+            hermite(x0, y0, x1, y1, t1, tt1, t2, tt2, s, xx, yy);
+            cyl(1 + j, xx, yy);
+        }
+    }
+}
+}  // namespace
+
+// The rays of family `fam`: one thread per ray, or one thread taking them in order when rays of the
+// family touch nodes another of them writes.
+__global__ void __launch_bounds__(64) k_fit_rays_ext(FitDev F, double *__restrict__ X, int fam) {
+    const int64_t b = F.ext.fam_off[fam], e = F.ext.fam_off[fam + 1];
+    if ((F.ext.serial >> fam) & 1) {
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            for (int64_t i = b; i < e; i++) move_ext_ray(F, F.ext.order[i], X);
+        return;
+    }
+    const int64_t i = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < e) move_ext_ray(F, F.ext.order[i], X);
+}
+
 void launch_fit_surface(hipStream_t s, const DevMesh &M, const DevParams &P, const FitDev &F, const double *prim,
                         const double *X1, double *X2, const double *cell_center, StepScalars *S, double time) {
     if (F.n_sf > 0)
@@ -210,6 +409,13 @@ void launch_fit_rays(hipStream_t s, const FitDev &F, double *X) {
     if (F.n_rays <= 0) return;
     hipLaunchKernelGGL(k_fit_ray_radii, dim3(grid_of(F.n_rays, 256)), dim3(256), 0, s, F, X);
     hipLaunchKernelGGL(k_fit_rays, dim3(grid_of(F.n_nodes, 256)), dim3(256), 0, s, F, X);
+    // B, C, D, E, H, G after the A rays (Setka.cpp:1850-1880)
+    for (int fam = 1; fam <= 6; fam++) {
+        const int64_t cnt = F.ext.fam_off[fam + 1] - F.ext.fam_off[fam];
+        if (cnt <= 0) continue;
+        const unsigned g = ((F.ext.serial >> fam) & 1) ? 1u : grid_of(cnt, 64);
+        hipLaunchKernelGGL(k_fit_rays_ext, dim3(g), dim3(64), 0, s, F, X, fam);
+    }
 }
 
 }  // namespace titan

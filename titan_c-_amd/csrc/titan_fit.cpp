@@ -29,6 +29,14 @@ double polar_angle_h(double x, double y) {
     return 0.0;
 }
 
+// cos(the), sin(the), cos(phi), sin(phi) of a ray from one glibc sincos call per angle: glibc's
+// sincos and cos differ in the last bit for some arguments (cos(1.3646395517420515)), and whether a
+// compiler turns cos(a) beside sin(a) into sincos is its own choice, so the call is made explicit.
+void ray_trig(const double *the_phi, double *t) {
+    ::sincos(the_phi[0], &t[1], &t[0]);
+    ::sincos(the_phi[1], &t[3], &t[2]);
+}
+
 template <typename T>
 int upload(T **dst, const std::vector<T> &src, hipStream_t s) {
     if (*dst) { hipFree(*dst); *dst = nullptr; }
@@ -173,6 +181,17 @@ struct FitState {
     std::vector<int32_t> ray_dev;
     std::vector<double> ray_trig;
     std::vector<int2> node_ray;  // device node -> (ray, index), ray -1 off the rays
+    // the other ray families
+    RayExtDev ext{};             // host copy of the scalars; pointers filled at upload
+    std::vector<int8_t> ext_type;
+    std::vector<int64_t> ext_off;
+    std::vector<int32_t> ext_nodes, ext_opor, ext_order;
+    std::vector<double> ext_trig;
+    bool up_ext = false;
+    int8_t *d_ext_type = nullptr;
+    int64_t *d_ext_off = nullptr;
+    int32_t *d_ext_nodes = nullptr, *d_ext_opor = nullptr, *d_ext_order = nullptr;
+    double *d_ext_trig = nullptr;
     // device copies
     bool up_static = false, up_nb = false, up_rays = false;
     int32_t *d_sf_face = nullptr, *d_sn_node = nullptr, *d_sn_face_off = nullptr, *d_sn_face = nullptr;
@@ -279,7 +298,7 @@ void fit_destroy(FitState *F) {
     if (!F) return;
     void *p[] = {F->d_sf_face, F->d_sn_node, F->d_sn_face_off, F->d_sn_face, F->d_nb_off, F->d_nb, F->d_ray_nodes,
                  F->d_sf_kind, F->d_sn_kind, F->d_sf_vel, F->d_sn_vel, F->d_centre, F->d_ray_trig, F->d_node_ray,
-                 F->d_ray_R};
+                 F->d_ray_R, F->d_ext_type, F->d_ext_off, F->d_ext_nodes, F->d_ext_opor, F->d_ext_order, F->d_ext_trig};
     for (void *q : p)
         if (q) hipFree(q);
     delete F;
@@ -307,13 +326,7 @@ int fit_set_rays(FitState *F, const titan_rays *r, const DeviceOrder &o) {
         F->node_ray[dev[i]] = make_int2((int)(i / r->nodes_per_ray), (int)(i % r->nodes_per_ray));
     F->ray_dev.swap(dev);
     F->ray_trig.resize(4 * r->n_rays);
-    for (int64_t k = 0; k < r->n_rays; k++) {
-        const double the = r->ray_angles[2 * k], phi = r->ray_angles[2 * k + 1];
-        F->ray_trig[4 * k] = std::cos(the);
-        F->ray_trig[4 * k + 1] = std::sin(the);
-        F->ray_trig[4 * k + 2] = std::cos(phi);
-        F->ray_trig[4 * k + 3] = std::sin(phi);
-    }
+    for (int64_t k = 0; k < r->n_rays; k++) ray_trig(r->ray_angles + 2 * k, &F->ray_trig[4 * k]);
     F->n_rays = r->n_rays;
     F->npr = r->nodes_per_ray;
     F->R0 = r->R0; F->R1 = r->R1; F->R5 = r->R5;
@@ -322,6 +335,129 @@ int fit_set_rays(FitState *F, const titan_rays *r, const DeviceOrder &o) {
     F->phi_lim = r->phi_a + kConstPi / 180.0;
     F->have_rays = true;
     F->up_rays = false;
+    F->ext = RayExtDev{};  // the other families were checked against the old M0..M4
+    return TITAN_OK;
+}
+
+namespace {
+// Indices on a ray of type t (1 B .. 6 G) that Luch::dvigenie writes (Luch.cpp:153-739), and the
+// node count it needs.
+int ext_written(int t, const int32_t M[6], int32_t N4, int32_t N5, int32_t MF, std::vector<int> &w) {
+    const int M0 = M[0], M1 = M[1], M11 = M[2], M2 = M[3], M3 = M[4], M4 = M[5];
+    w.clear();
+    auto run = [&](int a, int n) { for (int j = 0; j < n; j++) w.push_back(a + j); };
+    switch (t) {
+    case 1:
+        run(0, M0 + 1);
+        run(M0 + 1, M1);
+        run(M0 + M1 + 2, M11);
+        run(M0 + M1 + M11 + 2, M2 - M11);
+        run(M0 + M1 + M2 + 3, M3);
+        run(M0 + M1 + M2 + M3 + 4, M4 + 1);
+        return M0 + M1 + M2 + M3 + M4 + 5;
+    case 2:
+        run(0, M0);
+        run(M0 + 1, M1);
+        run(M0 + M1 + 2, M11 + N4 + N5);
+        return M0 + M1 + M11 + N4 + N5 + 2;
+    case 3:
+        run(1, M2 - M11);
+        run(M2 - M11 + 2, M3);
+        run(M2 - M11 + M3 + 3, M4 + 1);
+        return M2 - M11 + M3 + M4 + 4;
+    case 4:
+        run(0, M3 + M4 + 3);
+        return M3 + M4 + 3;
+    case 5:
+        run(1, 4);
+        return 5;
+    case 6:
+        run(1, M2 - M11 - MF + 1);
+        return M2 - M11 - MF + 2;
+    }
+    return -1;
+}
+const int kExtSlots[7] = {0, 0xE, 0x2, 0x7, 0x7, 0xF, 0x7};  // support slots each type reads
+}  // namespace
+
+int fit_set_rays_ext(FitState *F, const titan_rays_ext *r, const DeviceOrder &o) {
+    if (!F || !r || !F->have_rays || r->n_rays < 0) return TITAN_E_ARG;
+    const int64_t n = r->n_rays;
+    if (n > 0 && (!r->ray_type || !r->ray_off || !r->ray_nodes || !r->ray_opor || !r->ray_angles)) return TITAN_E_ARG;
+    if (n > 0 && r->ray_off[0] != 0) return TITAN_E_ARG;
+    const int32_t *M = F->M;
+    if (r->N4 < 0 || r->N5 < 0 || r->MF < 0) return TITAN_E_ARG;
+    std::vector<int> w;
+    std::vector<int64_t> fam_n(8, 0);
+    for (int64_t k = 0; k < n; k++) {
+        const int t = r->ray_type[k];
+        if (t < 1 || t > 6) return TITAN_E_ARG;
+        if (t == 2 && r->N5 < 2) return TITAN_E_ARG;  // the tail spacing divides by N5 - 1
+        if (t == 6 && M[3] - M[2] - r->MF + 1 < 0) return TITAN_E_ARG;
+        const int need = ext_written(t, M, r->N4, r->N5, r->MF, w);
+        const int64_t len = r->ray_off[k + 1] - r->ray_off[k];
+        if (len < need || len < 1) return TITAN_E_ARG;
+        for (int64_t i = r->ray_off[k]; i < r->ray_off[k + 1]; i++)
+            if (r->ray_nodes[i] < 0 || r->ray_nodes[i] >= F->nn) return TITAN_E_ARG;
+        for (int q = 0; q < 4; q++)
+            if (((kExtSlots[t] >> q) & 1) && (r->ray_opor[4 * k + q] < 0 || r->ray_opor[4 * k + q] >= F->nn))
+                return TITAN_E_ARG;
+        fam_n[t]++;
+    }
+    RayExtDev &E = F->ext;
+    E = RayExtDev{};
+    E.n = n;
+    E.fam_off[0] = E.fam_off[1] = 0;
+    for (int f = 1; f <= 6; f++) E.fam_off[f + 1] = E.fam_off[f] + fam_n[f];
+    F->ext_order.assign(n, 0);
+    std::vector<int64_t> fill(E.fam_off, E.fam_off + 8);
+    for (int64_t k = 0; k < n; k++) F->ext_order[fill[r->ray_type[k]]++] = (int32_t)k;
+    // a family runs ray after ray when one of its rays reads or writes a node another writes
+    std::vector<int64_t> writer(F->nn, -1);
+    for (int f = 1; f <= 6; f++) {
+        bool serial = false;
+        std::vector<int32_t> touched;
+        for (int64_t i = E.fam_off[f]; i < E.fam_off[f + 1] && !serial; i++) {
+            const int64_t k = F->ext_order[i];
+            ext_written(f, M, r->N4, r->N5, r->MF, w);
+            std::vector<int32_t> wr;
+            for (int j : w) wr.push_back(r->ray_nodes[r->ray_off[k] + j]);
+            if (f == 1) { wr.push_back(r->ray_opor[4 * k + 2]); wr.push_back(r->ray_opor[4 * k + 3]); }
+            if (f == 3) { wr.push_back(r->ray_opor[4 * k + 1]); wr.push_back(r->ray_opor[4 * k + 2]); }
+            for (int32_t v : wr) {
+                if (writer[v] >= 0 && writer[v] != k) serial = true;
+                writer[v] = k;
+                touched.push_back(v);
+            }
+        }
+        for (int64_t i = E.fam_off[f]; i < E.fam_off[f + 1] && !serial; i++) {
+            const int64_t k = F->ext_order[i];
+            for (int q = 0; q < 4; q++) {
+                if (!((kExtSlots[f] >> q) & 1)) continue;
+                const int64_t wk = writer[r->ray_opor[4 * k + q]];
+                if (wk >= 0 && wk != k) serial = true;
+            }
+        }
+        if (serial) E.serial |= 1 << f;
+        for (int32_t v : touched) writer[v] = -1;
+    }
+    F->ext_type.assign(r->ray_type, r->ray_type + n);
+    F->ext_off.assign(r->ray_off, r->ray_off + n + 1);
+    const int64_t tot = n > 0 ? r->ray_off[n] : 0;
+    F->ext_nodes.resize(tot);
+    for (int64_t i = 0; i < tot; i++) F->ext_nodes[i] = o.node_r2d[r->ray_nodes[i]];
+    F->ext_opor.resize(4 * n);
+    for (int64_t i = 0; i < 4 * n; i++) F->ext_opor[i] = r->ray_opor[i] >= 0 ? o.node_r2d[r->ray_opor[i]] : -1;
+    F->ext_trig.resize(4 * n);
+    for (int64_t k = 0; k < n; k++) ray_trig(r->ray_angles + 2 * k, &F->ext_trig[4 * k]);
+    E.N4 = r->N4; E.N5 = r->N5; E.MF = r->MF;
+    E.L6 = r->L6; E.L7 = r->L7;
+    E.dd1 = r->dd1; E.dd2 = r->dd2; E.dd3 = r->dd3; E.dd4 = r->dd4;
+    E.dd5i = (double)(int)r->dd5; E.dd6i = (double)(int)r->dd6;  // `int dd5 = geo->dd5` (Luch.cpp:166-167)
+    E.dd7 = r->dd7; E.dd8 = r->dd8;
+    E.ba1 = r->ba1; E.ba2 = r->ba2; E.ba3 = r->ba3; E.ba4 = r->ba4; E.ba5 = r->ba5;
+    E.md1 = r->md1; E.md2 = r->md2; E.md3 = r->md3; E.ea1 = r->ea1; E.ea2 = r->ea2; E.ea3 = r->ea3;
+    F->up_ext = false;
     return TITAN_OK;
 }
 
@@ -394,6 +530,16 @@ int fit_prepare(FitState *F, const titan_params &p, hipStream_t s, FitDev *dev) 
         if (rc != TITAN_OK) return rc;
         F->up_rays = true;
     }
+    if (F->ext.n > 0 && !F->up_ext) {
+        up(upload(&F->d_ext_type, F->ext_type, s));
+        up(upload(&F->d_ext_off, F->ext_off, s));
+        up(upload(&F->d_ext_nodes, F->ext_nodes, s));
+        up(upload(&F->d_ext_opor, F->ext_opor, s));
+        up(upload(&F->d_ext_order, F->ext_order, s));
+        up(upload(&F->d_ext_trig, F->ext_trig, s));
+        if (rc != TITAN_OK) return rc;
+        F->up_ext = true;
+    }
     FitDev &D = *dev;
     D = FitDev{};
     D.n_sf = (int32_t)F->sf_ref.size();
@@ -414,6 +560,11 @@ int fit_prepare(FitState *F, const titan_params &p, hipStream_t s, FitDev *dev) 
     D.velocity[1] = p.velocity_TS; D.velocity[2] = p.velocity_HP; D.velocity[3] = p.velocity_BS;
     D.k_smooth[1] = p.sglag_TS_k_sphere; D.k_smooth[2] = p.sglag_HP_k_sphere; D.k_smooth[3] = p.sglag_BS_k;
     D.phi_lim = F->phi_lim;
+    if (F->have_rays && F->ext.n > 0) {
+        D.ext = F->ext;
+        D.ext.type = F->d_ext_type; D.ext.off = F->d_ext_off; D.ext.nodes = F->d_ext_nodes;
+        D.ext.opor = F->d_ext_opor; D.ext.order = F->d_ext_order; D.ext.trig = F->d_ext_trig;
+    }
     return TITAN_OK;
 }
 

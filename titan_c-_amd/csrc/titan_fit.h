@@ -15,6 +15,7 @@ int fit_create(FitState **out, int64_t nn, int64_t nf, const int32_t *face_nodes
                const int32_t *cell_faces, const uint8_t *face_flags, const DeviceOrder &o);
 void fit_destroy(FitState *F);
 int fit_set_rays(FitState *F, const titan_rays *r, const DeviceOrder &o);
+int fit_set_rays_ext(FitState *F, const titan_rays_ext *r, const DeviceOrder &o);
 // kind 1: Setka::Smooth_head_TS3, 2: Smooth_head_HP3, 3: Find_Yzel_Sosed_for_BS, from the node
 // coordinates x0 [nn*3] (reference numbering) of node layer 0.
 int fit_smooth(FitState *F, int kind, const double *x0);

@@ -868,6 +868,11 @@ extern "C" int titan_set_rays(titan_ctx *c, const titan_rays *rays) {
     return fit_set_rays(c->fit, rays, c->ord);
 }
 
+extern "C" int titan_set_rays_ext(titan_ctx *c, const titan_rays_ext *rays) {
+    if (!c) return TITAN_E_ARG;
+    return fit_set_rays_ext(c->fit, rays, c->ord);
+}
+
 // node layer 0 in reference numbering (Yzel::coord[0], which the set-up functions read)
 static int nodes_layer0(titan_ctx *c, std::vector<double> &x) {
     x.resize(3 * c->nn);
