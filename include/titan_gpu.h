@@ -33,7 +33,9 @@ extern "C" {
 #define TITAN_E_IO (-7)             /* file cannot be opened, written, or is truncated / damaged
                                        (Setka_physics.cpp:4110-4113,4213-4216 print and exit) */
 #define TITAN_E_SURFACE (-8)        /* NaN or subnormal contact speed of a heliopause face while
-                                       fitting the surfaces (Setka_move.cpp:336-350 print and exit) */
+                                       fitting the surfaces (Setka_move.cpp:336-350 print and exit),
+                                       or an HP grid parabola through two points of equal abscissa
+                                       (solveQuadraticEquation, Help_funk.cpp:290-300, exits)   */
 
 /* face boundary codes (Type_Gran, Setka_physics.cpp:212-236; ZERO_GRADIENT is an addition) */
 #define TITAN_BC_INTERIOR 0
@@ -369,10 +371,53 @@ int titan_init_tvd(titan_ctx *ctx);
 /* The mesh's rays (the A_Luch / A2_Luch part of Setka::All_Luch): node lists, angles and the
  * Geo_param counts and radii of Luch::dvigenie.  Each node may lie on one ray only. */
 int titan_set_rays(titan_ctx *ctx, const titan_rays *rays);
+/* HP smoothing on the reference's ray grids A_Luch[i][j], B_Luch[i][j], E_Luch[i][j], D_Luch[i][j]
+ * (azimuth i, place j along the grid), given as node lists built from them; applied by every fitted
+ * step and titan_culc_velocity_surface with sglag_HP.  Each listed HP node gets, after the sphere
+ * term, the angle term when titan_smooth_angle_hp gave it a circle centre (Setka_move.cpp:703-745:
+ * towards the circle about (x, xcc, ycc) through the mean distance of its smoothing neighbours,
+ * velocity_HP * sglag_HP_k_angle, no x part) and, for grid nodes, the four macros1 terms
+ * (Setka_move.cpp:4-99, 880-1130): parabolas in (x, r_yz) through AA3, AA1, AA11 and AA33, AA3, AA1
+ * (weight sglag_HP_along) and in (polar angle about x, r_yz) through AA2, AA4, AA44 and AA4, AA2, AA22
+ * (weight sglag_HP_angle), each times velocity_HP * sglag_HP_k.  After the move, with move_HP, the
+ * tail correction (Setka_move.cpp:1281-1323) on each azimuth's D-ray HP nodes: node N4-5 onto the
+ * line in (x, r_yz) through nodes N4-4 and N4-6, nodes N4-3 .. onto the r_yz of node N4-4; with
+ * move_BS the BS correction (:1325-1357): each azimuth's listed BS nodes onto the r_yz of its
+ * reference node.  Replaces any earlier grid; n_grid = n_tail = n_bs = 0 switches it off. */
+typedef struct {
+    int64_t n_grid;
+    const int32_t *grid_node;    /* [n_grid] HP nodes: Yzels_opor[2] of A rays (j >= 2) and B rays,
+                                    Yzels_opor[1] of E rays and D rays (j < N4 - 1)              */
+    const int32_t *grid_nb;      /* [n_grid*8] AA1, AA11 (one and two back along the grid), AA3,
+                                    AA33 (one and two ahead), AA2, AA22 (azimuth i-1, i-2), AA4,
+                                    AA44 (i+1, i+2): Setka_move.cpp:906-1128                     */
+    double sglag_HP_k_angle, sglag_HP_k, sglag_HP_angle, sglag_HP_along; /* (0.001, 0.03, 1.8, 1.0),
+                                    Phys_param.cpp:513-519                                       */
+    int32_t n_tail, tail_len, N4; /* tail correction: n_tail azimuths of tail_len D rays (N4 >= 6) */
+    const int32_t *tail;         /* [n_tail*tail_len] D_Luch[i][k]->Yzels_opor[1]                 */
+    int32_t n_bs;                /* BS correction: n_bs azimuths                                  */
+    const int32_t *bs_ref;       /* [n_bs] A_Luch[i][last]->Yzels_opor[3]                         */
+    const int32_t *bs_off;       /* [n_bs+1] into bs_nodes                                        */
+    const int32_t *bs_nodes;     /* B_Luch[i][*]->Yzels_opor[3], E_Luch[i][*] and D_Luch[i][*]->
+                                    Yzels_opor[2]                                                */
+} titan_hp_grid;
+
 /* The B, C, D, E, H and G rays (Luch.cpp:153-739; Setka.cpp:1850-1900 builds them): replaces any
  * earlier set.  TITAN_E_ARG on a ray too short for its type, a support slot or node out of range,
  * N5 < 2 with C rays, or no titan_set_rays before; a later titan_set_rays drops them. */
 int titan_set_rays_ext(titan_ctx *ctx, const titan_rays_ext *rays);
+/* Setka::Smooth_angle_HP (Setka_move.cpp:2663-2760) from node layer 0: for each of n HP nodes, the
+ * circle fit (fitCircleRobust, :2544-2620) of its (y, z) and those of its 6 azimuthal neighbours nb6
+ * (azimuth i-1, i-2, i+1, i+2, i+3, i-3) gives its centre (xcc, ycc), and the neighbours join its
+ * smoothing set (the set titan_smooth_head_hp3 later clears and refills near the nose, as the
+ * reference's Yzel_sosed_sglag2 is).  TITAN_E_ARG for a node that is no HP surface node. */
+int titan_smooth_angle_hp(titan_ctx *ctx, int64_t n, const int32_t *nodes, const int32_t *nb6);
+/* The HP grid smoothing and the corrections after the move (titan_hp_grid).  TITAN_E_ARG for a
+ * grid node that is no HP surface node or appears twice, a tail node that is no HP surface node, a
+ * BS node that is no BS surface node, N4 < 6 or tail_len < N4 - 3 with tails. */
+int titan_set_hp_grid(titan_ctx *ctx, const titan_hp_grid *grid);
+/* fitCircleRobust (Setka_move.cpp:2544-2620) of n >= 3 points [n*2]: centre_radius = (cx, cy, R). */
+int titan_circle_fit(int64_t n, const double *points, double *centre_radius);
 /* Set-up of the surface smoothing from node layer 0 (Yzel::coord[0]): Setka::Smooth_head_TS3
  * (Setka_move.cpp:2205-2363) gives every TS node with at least 3 points its neighbours (the other
  * nodes of its TS faces) and the centre of the sphere fitted to it and them; Smooth_head_HP3

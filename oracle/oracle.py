@@ -74,6 +74,8 @@ def lib():
             "or_culc_velocity_surface": (_I, [_P, _P, _P, _P, _P, _P, _D, _P, _P]),
             "or_dvigenie": (None, [_P, _P]),
             "or_dvigenie_ext": (None, [_P, _P]),
+            "or_circle_fit": (_D, [_I64, _P, _P]),
+            "or_hp_terms": (_I, [_P, _P, _D, _D, _P, ctypes.c_int32, _D, _D, _D, _D, _P]),
             "or_mk_setup": (_I64, [_P, _P, _D, ctypes.c_int32, _P, _P, _P]),
             "or_mk_go": (_I, [_P, _P, _P, _P, _P, _P, _P]),
             "or_mk_change_velosity_batch": (None, [_I64, ctypes.c_uint64, _D, _P, _P, _D, _I, _P, _P]),
@@ -424,7 +426,13 @@ class OrFit(ctypes.Structure):
                 ("n_rays", _I64), ("npr", ctypes.c_int32), ("ray_nodes", _P), ("ray_angles", _P),
                 ("R0", _D), ("R1", _D), ("R5", _D), ("da1", _D), ("da2", _D), ("da3", _D), ("da4", _D), ("da5", _D),
                 ("M0", ctypes.c_int32), ("M1", ctypes.c_int32), ("M11", ctypes.c_int32), ("M2", ctypes.c_int32),
-                ("M3", ctypes.c_int32), ("M4", ctypes.c_int32), ("ext", _P)]
+                ("M3", ctypes.c_int32), ("M4", ctypes.c_int32), ("ext", _P), ("sn_cc", _P), ("sn_grid", _P),
+                ("k_angle", _D), ("k_hp", _D), ("hp_angle", _D), ("hp_along", _D), ("n_tail", ctypes.c_int32),
+                ("tail_len", ctypes.c_int32), ("N4", ctypes.c_int32), ("tail", _P), ("n_bs", ctypes.c_int32),
+                ("bs_ref", _P), ("bs_off", _P), ("bs_nodes", _P)]
+
+# Phys_param.cpp:513-519
+HP_GRID_DEFAULTS = dict(sglag_HP_k_angle=0.001, sglag_HP_k=0.03, sglag_HP_angle=1.8, sglag_HP_along=1.0)
 
 
 class OrRaysExt(ctypes.Structure):
@@ -582,6 +590,43 @@ class OracleFit:
         self.phi_lim = float(rays["phi_a"]) + math.pi / 180.0
         self._keep = None
         self.ext = None  # RaysExt of the other ray families, moved after the A rays
+        self.cc = np.full((len(self.sn), 2), np.nan)  # Smooth_angle_HP circle centres
+        self.grid = np.full((len(self.sn), 8), -1, np.int32)
+        self.hp_k = dict(HP_GRID_DEFAULTS)
+        self.tail = np.zeros((0, 0), np.int32)
+        self.N4 = 0
+        self.bs_ref = np.zeros(0, np.int32)
+        self.bs_lists = []
+        self._sn_index = {int(v): i for i, v in enumerate(self.sn)}
+
+    def smooth_angle_hp(self, nodes, nb6, x0, centres=None):
+        """Setka::Smooth_angle_HP (Setka_move.cpp:2663-2760): the circle centre of each HP node and
+        its 6 azimuthal neighbours in the yz plane of x0, and the neighbours added to the node's
+        smoothing set.  centres [n, 2] replace the fit (the library's own, for bit-level comparison)."""
+        x0 = np.asarray(x0, np.float64)
+        nb6 = np.asarray(nb6, np.int64).reshape(-1, 6)
+        for k, v in enumerate(nodes):
+            i = self._sn_index[int(v)]
+            if centres is None:
+                pts = np.ascontiguousarray(x0[[int(v)] + list(nb6[k])][:, 1:3])
+                self.cc[i] = circle_fit(pts)[0]
+            else:
+                self.cc[i] = centres[k]
+            self.nb[i] = sorted(set(self.nb[i]) | {int(u) for u in nb6[k]})
+
+    def set_hp_grid(self, nodes, nb8, tail=None, N4=0, bs_ref=(), bs_lists=(), **coef):
+        """The macros1 neighbours of HP grid nodes (AA1, AA11, AA3, AA33, AA2, AA22, AA4, AA44), the
+        D-ray HP nodes per azimuth for the tail correction, the BS correction lists, and the
+        sglag_HP_k_angle / _k / _angle / _along coefficients."""
+        self.grid[:] = -1
+        for k, v in enumerate(nodes):
+            self.grid[self._sn_index[int(v)]] = np.asarray(nb8[k], np.int32)
+        self.tail = np.ascontiguousarray(tail if tail is not None else np.zeros((0, 0)), np.int32)
+        self.N4 = int(N4)
+        self.bs_ref = np.ascontiguousarray(bs_ref, np.int32)
+        self.bs_lists = [np.asarray(x, np.int32) for x in bs_lists]
+        self.hp_k = dict(HP_GRID_DEFAULTS)
+        self.hp_k.update(coef)
 
     def smooth(self, kind, x0):
         x0 = np.asarray(x0, np.float64)
@@ -630,6 +675,18 @@ class OracleFit:
         if self.ext is not None:
             self._ext_c = self.ext.c()
             f.ext = ctypes.cast(ctypes.pointer(self._ext_c), ctypes.c_void_p)
+        cc = np.ascontiguousarray(self.cc, np.float64)
+        grid = np.ascontiguousarray(self.grid, np.int32)
+        bs_off = np.zeros(len(self.bs_lists) + 1, np.int32)
+        bs_off[1:] = np.cumsum([len(x) for x in self.bs_lists])
+        bs_nodes = np.ascontiguousarray(np.concatenate(self.bs_lists) if self.bs_lists else np.zeros(1), np.int32)
+        self._keep2 = (cc, grid, bs_off, bs_nodes)
+        f.sn_cc, f.sn_grid = _p(cc), _p(grid)
+        f.k_angle, f.k_hp = self.hp_k["sglag_HP_k_angle"], self.hp_k["sglag_HP_k"]
+        f.hp_angle, f.hp_along = self.hp_k["sglag_HP_angle"], self.hp_k["sglag_HP_along"]
+        f.n_tail, f.tail_len, f.N4 = self.tail.shape[0], self.tail.shape[1] if self.tail.ndim == 2 else 0, self.N4
+        f.tail = _p(self.tail) if self.tail.size else None
+        f.n_bs, f.bs_ref, f.bs_off, f.bs_nodes = len(self.bs_lists), _p(self.bs_ref), _p(bs_off), _p(bs_nodes)
         return f
 
 
@@ -646,6 +703,25 @@ def culc_velocity_surface(om, params, fit, x, prim, time, gcl=False):
                                         _p(np.ascontiguousarray(prim, np.float64)), ctypes.byref(G), float(time),
                                         _p(xn), _p(vel))
     return rc, xn, vel
+
+
+def hp_terms(A, P8, centre, ring, c_angle_term, c_along, c_angle, time):
+    """or_hp_terms: (rc, V [5, 3]) -- the angle term, then the four macros1 terms of node A."""
+    A = np.ascontiguousarray(A, np.float64)
+    P8 = np.ascontiguousarray(P8, np.float64).reshape(8, 3)
+    ring = np.ascontiguousarray(ring, np.float64).reshape(-1, 3)
+    V = np.zeros((5, 3))
+    rc = lib().or_hp_terms(_p(A), _p(P8), float(centre[0]), float(centre[1]), _p(ring), ring.shape[0],
+                           float(c_angle_term), float(c_along), float(c_angle), float(time), _p(V))
+    return rc, V
+
+
+def circle_fit(points):
+    """or_circle_fit: (centre [2], radius) of points [n, 2]."""
+    P = np.ascontiguousarray(points, np.float64).reshape(-1, 2)
+    c = np.zeros(2)
+    r = lib().or_circle_fit(P.shape[0], _p(P), _p(c))
+    return c, r
 
 
 def dvigenie(fit, x, params=None):

@@ -14,6 +14,7 @@
  */
 #define _GNU_SOURCE 1 /* sincos */
 #include "titan_oracle.h"
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1517,6 +1518,160 @@ void or_bera_source_k(const double *s, double Q, double rho_He, int cell_type, i
  * velocity_K * ds * n and one count.  Smoothing (:436-506 TS sphere, :566-622 HP sphere near the
  * nose, :1136-1210 BS neighbour mean) adds one more vector and count.  Average, then the move
  * along the radius (:1228-1278; HP with x < 0 along the radius in the yz plane). */
+/* solveQuadraticEquation (Help_funk.cpp:290-305): the parabola a x^2 + b x + c through three points;
+ * returns -1 where two abscissae coincide */
+static int or_quad3(double x1, double y1, double x2, double y2, double x3, double y3, double *a, double *b, double *c)
+{
+    double denom = (x1 - x2) * (x1 - x3) * (x2 - x3);
+    if (denom == 0) return -1;
+    *a = (x3 * (y2 - y1) + x2 * (y1 - y3) + x1 * (y3 - y2)) / denom;
+    *b = (x3 * x3 * (y1 - y2) + x2 * x2 * (y3 - y1) + x1 * x1 * (y2 - y3)) / denom;
+    *c = (x2 * x3 * (x2 - x3) * y1 + x3 * x1 * (x3 - x1) * y2 + x1 * x2 * (x1 - x2) * y3) / denom;
+    return 0;
+}
+
+/* 3x3 symmetric solve by elimination with partial pivoting (the LM step of the circle fit) */
+static int or_solve3(double H[3][3], const double g[3], double d[3])
+{
+    double M[3][4];
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) M[i][j] = H[i][j];
+        M[i][3] = g[i];
+    }
+    for (int c = 0; c < 3; c++) {
+        int piv = c;
+        for (int r = c + 1; r < 3; r++)
+            if (fabs(M[r][c]) > fabs(M[piv][c])) piv = r;
+        if (M[piv][c] == 0.0) return -1;
+        for (int j = 0; j < 4; j++) { double t = M[c][j]; M[c][j] = M[piv][j]; M[piv][j] = t; }
+        for (int r = c + 1; r < 3; r++) {
+            double f = M[r][c] / M[c][c];
+            for (int j = c; j < 4; j++) M[r][j] -= f * M[c][j];
+        }
+    }
+    for (int i = 2; i >= 0; i--) {
+        double t = M[i][3];
+        for (int j = i + 1; j < 3; j++) t -= M[i][j] * d[j];
+        d[i] = t / M[i][i];
+    }
+    return 0;
+}
+
+double or_circle_fit(int64_t n, const double *pts, double *centre)
+{
+    /* fitCircleRobust (Setka_move.cpp:2544-2620), called with max_iter 8 and tol 1e-8: Levenberg-
+     * Marquardt on (cx, cy, R) from the origin and the mean distance; the damping is multiplied by 10
+     * after every step (the reference compares the error with prev_error after setting one to the
+     * other), and a step with a non-finite update only raises the damping */
+    double P[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = 0; i < n; i++) P[2] += sqrt(pts[2 * i] * pts[2 * i] + pts[2 * i + 1] * pts[2 * i + 1]);
+    P[2] /= (double)n;
+    double lambda = 0.001, prev = DBL_MAX;
+    for (int it = 0; it < 8; it++) {
+        double H[3][3] = {{0}}, g[3] = {0, 0, 0}, err = 0.0;
+        for (int64_t i = 0; i < n; i++) {
+            double dx = pts[2 * i] - P[0], dy = pts[2 * i + 1] - P[1];
+            double dist = sqrt(dx * dx + dy * dy), res, J[3];
+            if (dist < 1e-10) {
+                res = -P[2];
+                J[0] = 0.0; J[1] = 0.0; J[2] = -1.0;
+            } else {
+                res = dist - P[2];
+                J[0] = -dx / dist; J[1] = -dy / dist; J[2] = -1.0;
+            }
+            err += res * res;
+            for (int a = 0; a < 3; a++) {
+                g[a] += J[a] * -res;
+                for (int b = 0; b < 3; b++) H[a][b] += J[a] * J[b];
+            }
+        }
+        err = sqrt(err / (double)n);
+        if (fabs(prev - err) < 1e-8) break;
+        prev = err;
+        for (int a = 0; a < 3; a++) H[a][a] *= 1.0 + lambda;
+        double d[3];
+        if (or_solve3(H, g, d) || !isfinite(d[0]) || !isfinite(d[1]) || !isfinite(d[2])) {
+            lambda *= 10.0;
+            continue;
+        }
+        for (int a = 0; a < 3; a++) P[a] += d[a];
+        lambda = err < prev ? lambda / 2.0 : lambda * 10.0;
+    }
+    centre[0] = P[0];
+    centre[1] = P[1];
+    return P[2];
+}
+
+static double or_ryz0(const double *p) { return sqrt(KVV(0.0, p[1], p[2])); }
+
+/* The angle term (Setka_move.cpp:703-745) of an HP node A: towards the circle about (x, cy, cz)
+ * through the mean distance of its neighbours' (y, z) from its centre; V[0] = 0. */
+static void or_hp_angle_term(const double *A, double cy, double cz, const double *x, const int32_t *nb, int32_t nnb,
+                             double c, double time, double V[3])
+{
+    double d1 = A[1] - cy, d2 = A[2] - cz;
+    double rr = sqrt(d1 * d1 + d2 * d2);
+    double r = 0.0;
+    for (int32_t j = 0; j < nnb; j++) {
+        const double *P = x + 3 * (int64_t)nb[j];
+        double e1 = cy - P[1], e2 = cz - P[2];
+        r += sqrt(e1 * e1 + e2 * e2);
+    }
+    r /= (double)nnb;
+    double kk = r / rr;
+    double B1 = kk * (A[1] - cy) + cy, B2 = kk * (A[2] - cz) + cz;
+    V[0] = 0.0;
+    V[1] = c * (B1 - A[1]) / time;
+    V[2] = c * (B2 - A[2]) / time;
+}
+
+/* macros1 (Setka_move.cpp:4-99) of an HP node A with grid neighbours P8 = AA1, AA11, AA3, AA33, AA2,
+ * AA22, AA4, AA44: parabolas in (x, r_yz) through AA3, AA1, AA11 and AA33, AA3, AA1, and in (polar
+ * angle about x, r_yz) through AA2, AA4, AA44 and AA4, AA2, AA22, evaluated at the node; term q
+ * scales A's (y, z) to that r_yz, weighted c_along (q = 0, 1) or c_angle (q = 2, 3).  -1 when a
+ * parabola's abscissae repeat. */
+static int or_hp_grid_terms(const double *A, const double *const P8[8], double c_along, double c_angle,
+                            double time, double V[4][3])
+{
+    const double *a1 = P8[0], *a11 = P8[1], *a3 = P8[2], *a33 = P8[3], *a2 = P8[4], *a22 = P8[5], *a4 = P8[6],
+                 *a44 = P8[7];
+    int bad = 0;
+    double p0 = A[0], p1 = or_ryz0(A), qa = 0, qb = 0, qc = 0, rad[4];
+    bad |= or_quad3(a3[0], or_ryz0(a3), a1[0], or_ryz0(a1), a11[0], or_ryz0(a11), &qa, &qb, &qc);
+    rad[0] = qa * KV(p0) + qb * p0 + qc;
+    bad |= or_quad3(a33[0], or_ryz0(a33), a3[0], or_ryz0(a3), a1[0], or_ryz0(a1), &qa, &qb, &qc);
+    rad[1] = qa * KV(p0) + qb * p0 + qc;
+    p0 = polar_angle(A[1], A[2]);
+    double t2 = polar_angle(a2[1], a2[2]), t4 = polar_angle(a4[1], a4[2]);
+    double t22 = polar_angle(a22[1], a22[2]), t44 = polar_angle(a44[1], a44[2]);
+    bad |= or_quad3(t2, or_ryz0(a2), t4, or_ryz0(a4), t44, or_ryz0(a44), &qa, &qb, &qc);
+    rad[2] = qa * KV(p0) + qb * p0 + qc;
+    bad |= or_quad3(t4, or_ryz0(a4), t2, or_ryz0(a2), t22, or_ryz0(a22), &qa, &qb, &qc);
+    rad[3] = qa * KV(p0) + qb * p0 + qc;
+    for (int q = 0; q < 4; q++) {
+        double c = q < 2 ? c_along : c_angle;
+        double B1 = A[1], B2 = A[2];
+        B1 *= rad[q] / p1;
+        B2 *= rad[q] / p1;
+        V[q][0] = c * (A[0] - A[0]) / time;
+        V[q][1] = c * (B1 - A[1]) / time;
+        V[q][2] = c * (B2 - A[2]) / time;
+    }
+    return bad ? -1 : 0;
+}
+
+int or_hp_terms(const double *A, const double *P8, double cy, double cz, const double *ring, int32_t n_ring,
+                double c_angle_term, double c_along, double c_angle, double time, double *V5)
+{
+    const double *Q8[8];
+    for (int q = 0; q < 8; q++) Q8[q] = P8 + 3 * q;
+    int32_t idx[64];
+    if (n_ring > 64) return -2;
+    for (int32_t j = 0; j < n_ring; j++) idx[j] = j;
+    or_hp_angle_term(A, cy, cz, ring, idx, n_ring, c_angle_term, time, V5);
+    return or_hp_grid_terms(A, Q8, c_along, c_angle, time, (double (*)[3])(V5 + 3));
+}
+
 static int fit_kind(uint8_t fl) { return ((fl >> 3) & 1) ? 1 : (((fl >> 2) & 1) ? 2 : (((fl >> 4) & 1) ? 3 : 0)); }
 
 int or_culc_velocity_surface(const or_mesh *m, const or_params *p, const or_fit *fit, const double *x,
@@ -1588,6 +1743,25 @@ int or_culc_velocity_surface(const or_mesh *m, const or_params *p, const or_fit 
             for (int k = 0; k < 3; k++) vel[k] += V[k];
             num++;
         }
+        if (kind == 2 && fit->sglag[2] && fit->sn_cc && !isnan(fit->sn_cc[2 * i]) && nb1 > nb0) {
+            double V[3];
+            or_hp_angle_term(A, fit->sn_cc[2 * i], fit->sn_cc[2 * i + 1], x, fit->sn_nb + nb0, nb1 - nb0,
+                             fit->velocity[2] * fit->k_angle, time, V);
+            for (int k = 0; k < 3; k++) vel[k] += V[k];
+            num++;
+        }
+        if (kind == 2 && fit->sglag[2] && fit->sn_grid && fit->sn_grid[8 * i] >= 0) {
+            const double *P8[8];
+            double V[4][3];
+            for (int q = 0; q < 8; q++) P8[q] = x + 3 * (int64_t)fit->sn_grid[8 * i + q];
+            if (or_hp_grid_terms(A, P8, fit->velocity[2] * fit->hp_along * fit->k_hp,
+                                 fit->velocity[2] * fit->hp_angle * fit->k_hp, time, V))
+                status = -9;
+            for (int q = 0; q < 4; q++) {
+                for (int k = 0; k < 3; k++) vel[k] += V[q][k];
+                num++;
+            }
+        }
         if (num > 0)
             for (int k = 0; k < 3; k++) vel[k] /= num;
         if (node_velocity)
@@ -1602,6 +1776,42 @@ int or_culc_velocity_surface(const or_mesh *m, const or_params *p, const or_fit 
             }
             double sdot = ((vel[0] * Bd[0] + vel[1] * Bd[1]) + vel[2] * Bd[2]) * time;
             for (int k = 0; k < 3; k++) x_next[3 * v + k] = A[k] + sdot * Bd[k];
+        }
+    }
+    if (fit->move[2] && fit->n_tail > 0) {
+        /* Setka_move.cpp:1281-1323: along each azimuth's D rays the HP node N4-5 onto the line
+         * through nodes N4-4 and N4-6 in (x, r_yz), and nodes N4-3 .. on to the r_yz of node N4-4 */
+        const int N4 = fit->N4, NN = fit->tail_len - 1;
+        for (int32_t a = 0; a < fit->n_tail; a++) {
+            const int32_t *L = fit->tail + (int64_t)a * fit->tail_len;
+#define TQ(k) (x_next + 3 * (int64_t)L[k])
+            double h1 = or_ryz0(TQ(N4 - 4)), xx1 = TQ(N4 - 4)[0];
+            double h2 = or_ryz0(TQ(N4 - 6)), xx2 = TQ(N4 - 6)[0];
+            double *yz = TQ(N4 - 5);
+            double hh = or_ryz0(yz), xx = yz[0];
+            double hhh = or_linear2(xx1, h1, xx2, h2, xx);
+            yz[1] = yz[1] * hhh / hh;
+            yz[2] = yz[2] * hhh / hh;
+            for (int k = N4 - 3; k <= NN; k++) {
+                double *q = TQ(k);
+                double hq = or_ryz0(q);
+                q[1] = q[1] * h1 / hq;
+                q[2] = q[2] * h1 / hq;
+            }
+#undef TQ
+        }
+    }
+    if (fit->move[3] && fit->n_bs > 0) {
+        /* Setka_move.cpp:1325-1357: each azimuth's B, E and D rays' BS nodes onto the r_yz of the
+         * last A ray's BS node */
+        for (int32_t a = 0; a < fit->n_bs; a++) {
+            double H = or_ryz0(x_next + 3 * (int64_t)fit->bs_ref[a]);
+            for (int32_t j = fit->bs_off[a]; j < fit->bs_off[a + 1]; j++) {
+                double *q = x_next + 3 * (int64_t)fit->bs_nodes[j];
+                double h2 = or_ryz0(q);
+                q[1] *= H / h2;
+                q[2] *= H / h2;
+            }
         }
     }
     free(sp);

@@ -254,6 +254,17 @@ typedef struct {
     double R0, R1, R5, da1, da2, da3, da4, da5;
     int32_t M0, M1, M11, M2, M3, M4;
     const struct or_rays_ext_s *ext; /* the other ray families (nullable), moved after the A rays */
+    /* HP smoothing on the ray grids, with sglag[2] (Setka_move.cpp:700-1130): */
+    const double *sn_cc;         /* [n_sn*2] circle centre (xcc, ycc) of Smooth_angle_HP, NaN: none (nullable) */
+    const int32_t *sn_grid;      /* [n_sn*8] AA1, AA11, AA3, AA33, AA2, AA22, AA4, AA44, -1 row: none (nullable) */
+    double k_angle, k_hp, hp_angle, hp_along; /* sglag_HP_k_angle, sglag_HP_k, sglag_HP_angle, sglag_HP_along */
+    /* corrections after the move (Setka_move.cpp:1280-1357) */
+    int32_t n_tail, tail_len, N4;
+    const int32_t *tail;         /* [n_tail*tail_len] the HP nodes (Yzels_opor[1]) of each azimuth's D rays */
+    int32_t n_bs;
+    const int32_t *bs_ref;       /* [n_bs] the last A ray's BS node of each azimuth */
+    const int32_t *bs_off;       /* [n_bs+1] */
+    const int32_t *bs_nodes;     /* the BS nodes of its B, E and D rays */
 } or_fit;
 
 /* The other ray families of Luch::dvigenie (Luch.cpp:153-739): B_Luch (around the flank), C_Luch /
@@ -279,10 +290,19 @@ typedef struct or_rays_ext_s {
 void or_dvigenie_ext(const or_rays_ext *r, double *x);
 /* Surface velocities from layer `now` (x, prim, geometry g of that layer) moved over `time` into
  * x_next (surface nodes only); node_velocity [n_sn*3] (nullable). Returns -1 on a chlld failure,
- * -8 on a NaN/subnormal HP contact speed. */
+ * -8 on a NaN/subnormal HP contact speed, -9 on three grid points with a repeated abscissa
+ * (solveQuadraticEquation, Help_funk.cpp:290-305, stops the reference there). */
 int or_culc_velocity_surface(const or_mesh *m, const or_params *p, const or_fit *fit, const double *x,
                              const double *prim, const or_geom *g, double time, double *x_next,
                              double *node_velocity);
+/* The HP grid terms of one node A [3] (test access to the step's own routines): V5 [5*3] = the angle
+ * term (circle centre (cy, cz), neighbours ring [n_ring*3], n_ring <= 64, weight c_angle_term), then
+ * the four macros1 terms of its neighbours P8 [8*3] (AA1, AA11, AA3, AA33, AA2, AA22, AA4, AA44).
+ * Returns -1 on a repeated abscissa. */
+int or_hp_terms(const double *A, const double *P8, double cy, double cz, const double *ring, int32_t n_ring,
+                double c_angle_term, double c_along, double c_angle, double time, double *V5);
+/* fitCircleRobust (Setka_move.cpp:2544-2620) of n points [n*2]: centre [2], returns the radius */
+double or_circle_fit(int64_t n, const double *pts, double *centre);
 /* Every ray's nodes other than index M0 and its TS/HP/BS nodes, in x. */
 void or_dvigenie(const or_fit *fit, double *x);
 

@@ -182,3 +182,77 @@ def ext_ray_set(rng, first_node, n_nodes, counts=(6, 6, 6, 6, 6, 6), serial=(), 
         prev = own
     return (np.asarray(types, np.int8), lists, np.asarray(opor, np.int32), np.asarray(angles, np.float64), g,
             xyz)
+
+
+# ---- HP grid lists on the fitted shell (Setka_move.cpp:700-1357 reads them off the ray grids) -----
+def _wrap(a):
+    return (a + math.pi) % (2 * math.pi) - math.pi
+
+
+def hp_grid_lists(rays, xyz, band=0.12, the_lo=0.35, the_hi=2.8, n_grid=None, seed=3):
+    """Ray-grid neighbour lists built from the shell's ray directions: for an HP node at polar angle
+    the (from +x) and azimuth phi, its azimuthal neighbours are the HP nodes of the band |d the| <
+    band ordered by azimuth (i-1, i-2, i+1, i+2, i+3, i-3 for the angle term; i-1, i-2, i+1, i+2 for
+    macros1), its along-grid neighbours the HP nodes nearest in direction to (the -+ d, phi) and
+    (the -+ 2d, phi).  Returns dict(angle_nodes, nb6, grid_nodes, nb8, tail [n_az, 5], bs_ref,
+    bs_lists) in reference node ids; tails and BS lists are short latitude runs on the flank."""
+    M0, M1, M2, M3 = rays["M0"], rays["M1"], rays["M2"], rays["M3"]
+    iHP, iBS = M0 + M1 + M2 + 2, M0 + M1 + M2 + M3 + 3
+    rn = np.asarray(rays["ray_nodes"])
+    the, phi = rays["ray_angles"][:, 0], rays["ray_angles"][:, 1]
+    hp, bs = rn[:, iHP], rn[:, iBS]
+    unit = np.stack([np.cos(the), np.sin(the) * np.cos(phi), np.sin(the) * np.sin(phi)], axis=1)
+    d = float(np.median(np.sort(np.arccos(np.clip(unit @ unit.T, -1, 1)), axis=1)[:, 1]))
+    rng = np.random.default_rng(seed)
+    ok = np.where((the > the_lo) & (the < the_hi))[0]
+    angle_nodes, nb6, grid_nodes, nb8 = [], [], [], []
+    for k in ok:
+        ring = np.where(np.abs(the - the[k]) < band)[0]
+        ring = ring[ring != k]
+        if len(ring) < 8:
+            continue
+        dphi = _wrap(phi[ring] - phi[k])
+        left = ring[dphi < 0][np.argsort(-dphi[dphi < 0])]
+        right = ring[dphi > 0][np.argsort(dphi[dphi > 0])]
+        if len(left) < 3 or len(right) < 3:
+            continue
+        angle_nodes.append(hp[k])
+        nb6.append([hp[left[0]], hp[left[1]], hp[right[0]], hp[right[1]], hp[right[2]], hp[left[2]]])
+        along = []
+        for s in (-1, -2, 1, 2):
+            t = the[k] + s * d
+            tgt = np.array([math.cos(t), math.sin(t) * math.cos(phi[k]), math.sin(t) * math.sin(phi[k])])
+            j = int(np.argmax(unit @ tgt))
+            along.append(j)
+        xs = [xyz[hp[j], 0] for j in along] + [xyz[hp[k], 0]]
+        if len(set(along)) < 4 or k in along or len(set(np.round(xs, 12))) < 5:
+            continue
+        grid_nodes.append(hp[k])
+        nb8.append([hp[along[0]], hp[along[1]], hp[along[2]], hp[along[3]], hp[left[0]], hp[left[1]], hp[right[0]],
+                    hp[right[1]]])
+    if n_grid is not None and len(grid_nodes) > n_grid:
+        pick = np.sort(rng.choice(len(grid_nodes), n_grid, replace=False))
+        grid_nodes = [grid_nodes[i] for i in pick]
+        nb8 = [nb8[i] for i in pick]
+    # tails: runs of 5 HP nodes across the flank (the from 80 to 100 degrees) at one azimuth;
+    # BS lists: BS nodes of a latitude band, onto the first one's r_yz
+    tail, bs_ref, bs_lists, used = [], [], [], set()
+    for k in np.where(np.abs(the - 0.5 * math.pi) < 0.05)[0]:
+        run = []
+        for s in range(5):
+            t = the[k] + (s - 2) * d
+            tgt = np.array([math.cos(t), math.sin(t) * math.cos(phi[k]), math.sin(t) * math.sin(phi[k])])
+            run.append(int(np.argmax(unit @ tgt)))
+        if len(set(run)) == 5 and not (set(run) & used):
+            used |= set(run)
+            tail.append([hp[j] for j in run])
+    for k in np.where(np.abs(the - 0.5 * math.pi) < 0.05)[0][:6]:
+        band_k = np.where((np.abs(the - the[k]) < 0.5 * d) & (np.abs(_wrap(phi - phi[k])) < 3 * d))[0]
+        band_k = [j for j in band_k if j != k and bs[j] not in bs_ref and all(bs[j] not in b for b in bs_lists)]
+        if band_k and bs[k] not in [x for b in bs_lists for x in b]:
+            bs_ref.append(bs[k])
+            bs_lists.append([bs[j] for j in band_k[:3]])
+    return dict(angle_nodes=np.asarray(angle_nodes, np.int32), nb6=np.asarray(nb6, np.int32),
+                grid_nodes=np.asarray(grid_nodes, np.int32), nb8=np.asarray(nb8, np.int32),
+                tail=np.asarray(tail[:8], np.int32).reshape(-1, 5), bs_ref=np.asarray(bs_ref, np.int32),
+                bs_lists=bs_lists)

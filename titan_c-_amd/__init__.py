@@ -95,6 +95,19 @@ class RaysExtC(ctypes.Structure):
                                                "ba3", "ba4", "ba5", "md1", "md2", "md3", "ea1", "ea2", "ea3")]
 
 
+class HpGridC(ctypes.Structure):
+    _fields_ = [("n_grid", ctypes.c_int64), ("grid_node", ctypes.c_void_p), ("grid_nb", ctypes.c_void_p),
+                ("sglag_HP_k_angle", ctypes.c_double), ("sglag_HP_k", ctypes.c_double),
+                ("sglag_HP_angle", ctypes.c_double), ("sglag_HP_along", ctypes.c_double),
+                ("n_tail", ctypes.c_int32), ("tail_len", ctypes.c_int32), ("N4", ctypes.c_int32),
+                ("tail", ctypes.c_void_p), ("n_bs", ctypes.c_int32), ("bs_ref", ctypes.c_void_p),
+                ("bs_off", ctypes.c_void_p), ("bs_nodes", ctypes.c_void_p)]
+
+
+# Phys_param.cpp:513-519
+HP_GRID_DEFAULTS = dict(sglag_HP_k_angle=0.001, sglag_HP_k=0.03, sglag_HP_angle=1.8, sglag_HP_along=1.0)
+
+
 # Luch::type of the other ray families (Luch.cpp:153-739)
 RAY_B, RAY_C, RAY_D, RAY_E, RAY_H, RAY_G = 1, 2, 3, 4, 5, 6
 
@@ -150,6 +163,9 @@ _SIGS = {
     "titan_init_tvd": (_I, [_P]),
     "titan_set_rays": (_I, [_P, ctypes.POINTER(RaysC)]),
     "titan_set_rays_ext": (_I, [_P, ctypes.POINTER(RaysExtC)]),
+    "titan_smooth_angle_hp": (_I, [_P, _I64, _P, _P]),
+    "titan_set_hp_grid": (_I, [_P, ctypes.POINTER(HpGridC)]),
+    "titan_circle_fit": (_I, [_I64, _P, _P]),
     "titan_smooth_head_ts3": (_I, [_P]),
     "titan_smooth_head_hp3": (_I, [_P]),
     "titan_find_yzel_sosed_for_bs": (_I, [_P]),
@@ -303,6 +319,14 @@ def sphere_fit(points):
     c = np.zeros(3)
     _check(lib().titan_sphere_fit(P.shape[0], _ptr(P), _ptr(c)), "titan_sphere_fit")
     return c
+
+
+def circle_fit(points):
+    """fitCircleRobust's (centre [2], radius) of n >= 3 points [n, 2] (host routine of the library)."""
+    P = np.ascontiguousarray(points, np.float64).reshape(-1, 2)
+    c = np.zeros(3)
+    _check(lib().titan_circle_fit(P.shape[0], _ptr(P), _ptr(c)), "titan_circle_fit")
+    return c[:2], float(c[2])
 
 
 class Setka:
@@ -496,6 +520,39 @@ class Setka:
             setattr(r, k, geo[k])
         self._rays_ext_keep = (t, off, nodes, op, ang)
         _check(lib().titan_set_rays_ext(self.ctx, ctypes.byref(r)), "titan_set_rays_ext", self.ctx)
+
+    def Smooth_angle_HP(self, nodes, nb6):
+        """Setka::Smooth_angle_HP: circle centres from node layer 0 and the azimuthal neighbours nb6
+        [n, 6] (i-1, i-2, i+1, i+2, i+3, i-3) of HP nodes."""
+        nd = np.ascontiguousarray(nodes, np.int32)
+        nb = np.ascontiguousarray(nb6, np.int32).reshape(-1, 6)
+        _check(lib().titan_smooth_angle_hp(self.ctx, len(nd), _ptr(nd), _ptr(nb)), "titan_smooth_angle_hp",
+               self.ctx)
+
+    def set_hp_grid(self, nodes=(), nb8=None, tail=None, N4=0, bs_ref=(), bs_lists=(), **coef):
+        """HP grid smoothing (nodes, nb8 [n, 8]: AA1, AA11, AA3, AA33, AA2, AA22, AA4, AA44), the tail
+        correction (tail [n_az, len] D-ray HP nodes, N4) and the BS correction (bs_ref [n_az],
+        bs_lists: the BS nodes of each azimuth); coef: sglag_HP_k_angle, _k, _angle, _along."""
+        k = dict(HP_GRID_DEFAULTS)
+        k.update(coef)
+        nd = np.ascontiguousarray(nodes, np.int32)
+        nb = np.ascontiguousarray(nb8 if nb8 is not None else np.zeros((0, 8)), np.int32).reshape(-1, 8)
+        tl = np.ascontiguousarray(tail if tail is not None else np.zeros((0, 0)), np.int32)
+        br = np.ascontiguousarray(bs_ref, np.int32)
+        off = np.zeros(len(bs_lists) + 1, np.int32)
+        off[1:] = np.cumsum([len(x) for x in bs_lists])
+        bn = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for x in bs_lists])
+                                  if len(bs_lists) else np.zeros(1), np.int32)
+        g = HpGridC()
+        g.n_grid, g.grid_node, g.grid_nb = len(nd), _ptr(nd), _ptr(nb)
+        for key in HP_GRID_DEFAULTS:
+            setattr(g, key, k[key])
+        g.n_tail = tl.shape[0] if tl.ndim == 2 else 0
+        g.tail_len = tl.shape[1] if tl.ndim == 2 else 0
+        g.N4, g.tail = int(N4), _ptr(tl)
+        g.n_bs, g.bs_ref, g.bs_off, g.bs_nodes = len(br), _ptr(br), _ptr(off), _ptr(bn)
+        self._hp_keep = (nd, nb, tl, br, off, bn)
+        _check(lib().titan_set_hp_grid(self.ctx, ctypes.byref(g)), "titan_set_hp_grid", self.ctx)
 
     def Smooth_head_TS3(self):
         _check(lib().titan_smooth_head_ts3(self.ctx), "titan_smooth_head_ts3", self.ctx)

@@ -121,6 +121,16 @@ struct FitDev {
     double k_smooth[4];          // sglag_TS_k_sphere, sglag_HP_k_sphere, sglag_BS_k
     double phi_lim;              // HP sphere smoothing where the polar angle is <= phi_a + pi/180
     RayExtDev ext;               // the other ray families (ext.n = 0: none)
+    // HP smoothing on the ray grids (Setka_move.cpp:700-1130), with sglag[2]
+    const double *sn_cc;         // [n_sn*2] Smooth_angle_HP circle centre (xcc, ycc); NaN: none; null: no table
+    const int32_t *sn_grid;      // [n_sn*8] device nodes AA1, AA11, AA3, AA33, AA2, AA22, AA4, AA44; -1: none
+    double k_angle, k_hp, hp_angle, hp_along;  // sglag_HP_k_angle, sglag_HP_k, sglag_HP_angle, sglag_HP_along
+    // corrections after the move (Setka_move.cpp:1280-1357)
+    int32_t n_tail, tail_len, N4;
+    const int32_t *tail;         // [n_tail*tail_len] device HP nodes of each azimuth's D rays
+    int32_t n_bs;
+    const int32_t *bs_ref, *bs_off, *bs_nodes;  // per azimuth: the reference BS node, its CSR list
+    const int32_t *sn_ref;       // [n_sn] reference node id of each surface node (error reports)
 };
 
 // Monte-Carlo neutrals (Setka::MK_go, Setka_MK.cpp:1225-2316): particles are numbered face by face
@@ -163,9 +173,10 @@ struct MkDev {
 enum ErrRank : unsigned {
     kErrFitContact = 0,     // NaN / subnormal heliopause contact speed (TITAN_E_SURFACE)
     kErrFitRiemann = 1,     // chlld's fatal branch in the surface fit (TITAN_E_RIEMANN)
-    kErrFaceRiemann = 2,    // chlld's fatal branch or an unknown boundary code in the face loop
-    kErrCellNonfinite = 3,  // NaN density after the update (TITAN_E_NONFINITE_STATE)
-    kErrCellFallback = 4,   // surrogate fallback list full (TITAN_E_ARG)
+    kErrFitQuad = 2,        // HP grid parabola through points with a repeated abscissa (TITAN_E_SURFACE)
+    kErrFaceRiemann = 3,    // chlld's fatal branch or an unknown boundary code in the face loop
+    kErrCellNonfinite = 4,  // NaN density after the update (TITAN_E_NONFINITE_STATE)
+    kErrCellFallback = 5,   // surrogate fallback list full (TITAN_E_ARG)
 };
 
 // Device-side scalars of the step loop (one instance per context).

@@ -52,9 +52,25 @@ __global__ void __launch_bounds__(256) k_fit_faces(DevMesh M, DevParams P, FitDe
     F.sf_vel[3 * i + 2] = s * n[2];
 }
 
+namespace {
+// solveQuadraticEquation (Help_funk.cpp:290-305): a x^2 + b x + c through three points, evaluated
+// at t; false where two abscissae coincide
+__device__ inline bool quad3_at(double x1, double y1, double x2, double y2, double x3, double y3, double t,
+                                double &r) {
+    const double denom = (x1 - x2) * (x1 - x3) * (x2 - x3);
+    if (denom == 0) return false;
+    const double a = (x3 * (y2 - y1) + x2 * (y1 - y3) + x1 * (y3 - y2)) / denom;
+    const double b = (x3 * x3 * (y1 - y2) + x2 * x2 * (y3 - y1) + x1 * x1 * (y2 - y3)) / denom;
+    const double c = (x2 * x3 * (x2 - x3) * y1 + x3 * x1 * (x3 - x1) * y2 + x1 * x2 * (x1 - x2) * y3) / denom;
+    r = a * (t * t) + b * t + c;
+    return true;
+}
+__device__ inline double ryz0(const double *p) { return sqrt(0.0 * 0.0 + p[1] * p[1] + p[2] * p[2]); }
+}  // namespace
+
 // time > 0: the step length of an explicit call; otherwise the context's current dt.
 __global__ void __launch_bounds__(256) k_fit_nodes(FitDev F, const double *__restrict__ X1, double *__restrict__ X2,
-                                                   const StepScalars *__restrict__ S, double time) {
+                                                   StepScalars *__restrict__ S, double time) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= F.n_sn) return;
     const double dt = time > 0.0 ? time : S->dt;
@@ -114,6 +130,59 @@ __global__ void __launch_bounds__(256) k_fit_nodes(FitDev F, const double *__res
         v0 += V0; v1 += V1; v2 += V2;
         num++;
     }
+    if (kind == 2 && F.sglag[2]) {
+        if (F.sn_cc && !isnan(F.sn_cc[2 * i]) && nb1 > nb0) {
+            // Setka_move.cpp:703-745: towards the circle about (x, xcc, ycc) through the mean distance
+            // of the neighbours' (y, z) from its centre; no x component
+            const double cy = F.sn_cc[2 * i], cz = F.sn_cc[2 * i + 1];
+            const double d1 = A1 - cy, d2 = A2 - cz;
+            const double rr = sqrt(d1 * d1 + d2 * d2);
+            double r = 0.0;
+            for (int j = nb0; j < nb1; j++) {
+                const double *p = X1 + 3 * (int64_t)F.sn_nb[j];
+                const double e1 = cy - p[1], e2 = cz - p[2];
+                r += sqrt(e1 * e1 + e2 * e2);
+            }
+            r /= (double)(nb1 - nb0);
+            const double k = r / rr;
+            const double c = F.velocity[2] * F.k_angle;
+            const double B1 = k * (A1 - cy) + cy, B2 = k * (A2 - cz) + cz;
+            v0 += 0.0;
+            v1 += c * (B1 - A1) / dt;
+            v2 += c * (B2 - A2) / dt;
+            num++;
+        }
+        if (F.sn_grid && F.sn_grid[8 * i] >= 0) {
+            // macros1 (Setka_move.cpp:4-99): parabolas through the grid neighbours in (x, r_yz) and in
+            // (polar angle about x, r_yz), each pulling r_yz towards its value at the node
+            const int32_t *G = F.sn_grid + 8 * i;
+            const double *a1 = X1 + 3 * (int64_t)G[0], *a11 = X1 + 3 * (int64_t)G[1], *a3 = X1 + 3 * (int64_t)G[2],
+                         *a33 = X1 + 3 * (int64_t)G[3], *a2 = X1 + 3 * (int64_t)G[4], *a22 = X1 + 3 * (int64_t)G[5],
+                         *a4 = X1 + 3 * (int64_t)G[6], *a44 = X1 + 3 * (int64_t)G[7];
+            const double Ap[3] = {A0, A1, A2};
+            const double p1 = ryz0(Ap);
+            double rad[4];
+            bool ok = quad3_at(a3[0], ryz0(a3), a1[0], ryz0(a1), a11[0], ryz0(a11), A0, rad[0]);
+            ok &= quad3_at(a33[0], ryz0(a33), a3[0], ryz0(a3), a1[0], ryz0(a1), A0, rad[1]);
+            const double t0 = polar_angle(A1, A2);
+            const double t2 = polar_angle(a2[1], a2[2]), t4 = polar_angle(a4[1], a4[2]);
+            const double t22 = polar_angle(a22[1], a22[2]), t44 = polar_angle(a44[1], a44[2]);
+            ok &= quad3_at(t2, ryz0(a2), t4, ryz0(a4), t44, ryz0(a44), t0, rad[2]);
+            ok &= quad3_at(t4, ryz0(a4), t2, ryz0(a2), t22, ryz0(a22), t0, rad[3]);
+            if (!ok) report_error(S, kErrFitQuad, F.sn_ref[i]);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const double c = F.velocity[2] * (q < 2 ? F.hp_along : F.hp_angle) * F.k_hp;
+                double B1 = A1, B2 = A2;
+                B1 *= rad[q] / p1;
+                B2 *= rad[q] / p1;
+                v0 += c * (A0 - A0) / dt;
+                v1 += c * (B1 - A1) / dt;
+                v2 += c * (B2 - A2) / dt;
+                num++;
+            }
+        }
+    }
     if (num > 0) {
         v0 /= num; v1 /= num; v2 /= num;
     }
@@ -133,6 +202,44 @@ __global__ void __launch_bounds__(256) k_fit_nodes(FitDev F, const double *__res
         X2[3 * g] = A0 + s * B0;
         X2[3 * g + 1] = A1 + s * B1;
         X2[3 * g + 2] = A2 + s * B2;
+    }
+}
+
+// After the move (Setka_move.cpp:1281-1357): threads [0, n_tail) take an azimuth's D-ray HP nodes
+// -- node N4-5 onto the line in (x, r_yz) through nodes N4-4 and N4-6, nodes N4-3 .. onto the r_yz
+// of node N4-4 -- and the next n_bs threads an azimuth's BS nodes, onto the r_yz of its reference
+// node.  The two sets of nodes are disjoint (HP, BS).
+__global__ void __launch_bounds__(64) k_fit_fix(FitDev F, double *__restrict__ X) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < F.n_tail) {
+        if (!F.move[2]) return;
+        const int32_t *L = F.tail + (int64_t)t * F.tail_len;
+        const int N4 = F.N4, NN = F.tail_len - 1;
+        auto Q = [&](int k) { return X + 3 * (int64_t)L[k]; };
+        const double h1 = ryz0(Q(N4 - 4)), xx1 = Q(N4 - 4)[0];
+        const double h2 = ryz0(Q(N4 - 6)), xx2 = Q(N4 - 6)[0];
+        double *yz = Q(N4 - 5);
+        const double hh = ryz0(yz), xx = yz[0];
+        const double d = (h1 - h2) / (xx1 - xx2);  // linear2 (Help_funk.cpp:65-70)
+        const double hhh = d * (xx - xx2) + h2;
+        yz[1] = yz[1] * hhh / hh;
+        yz[2] = yz[2] * hhh / hh;
+        for (int k = N4 - 3; k <= NN; k++) {
+            double *q = Q(k);
+            const double hq = ryz0(q);
+            q[1] = q[1] * h1 / hq;
+            q[2] = q[2] * h1 / hq;
+        }
+        return;
+    }
+    const int a = t - F.n_tail;
+    if (a >= F.n_bs || !F.move[3]) return;
+    const double H = ryz0(X + 3 * (int64_t)F.bs_ref[a]);
+    for (int j = F.bs_off[a]; j < F.bs_off[a + 1]; j++) {
+        double *q = X + 3 * (int64_t)F.bs_nodes[j];
+        const double h2 = ryz0(q);
+        q[1] *= H / h2;
+        q[2] *= H / h2;
     }
 }
 
@@ -403,6 +510,8 @@ void launch_fit_surface(hipStream_t s, const DevMesh &M, const DevParams &P, con
         hipLaunchKernelGGL(k_fit_faces, dim3(grid_of(F.n_sf, 256)), dim3(256), 0, s, M, P, F, prim, X1, cell_center,
                            S);
     if (F.n_sn > 0) hipLaunchKernelGGL(k_fit_nodes, dim3(grid_of(F.n_sn, 256)), dim3(256), 0, s, F, X1, X2, S, time);
+    if (F.n_tail + F.n_bs > 0)
+        hipLaunchKernelGGL(k_fit_fix, dim3(grid_of(F.n_tail + F.n_bs, 64)), dim3(64), 0, s, F, X2);
 }
 
 void launch_fit_rays(hipStream_t s, const FitDev &F, double *X) {

@@ -188,6 +188,18 @@ struct FitState {
     std::vector<int32_t> ext_nodes, ext_opor, ext_order;
     std::vector<double> ext_trig;
     bool up_ext = false;
+    // HP grid smoothing and the corrections after the move
+    std::vector<double> cc;             // [ns*2] circle centres, NaN: none
+    bool have_cc = false, have_grid = false;
+    std::vector<int32_t> grid;          // [ns*8] reference ids, -1: none
+    double hp_k[4] = {0, 0, 0, 0};      // k_angle, k, angle, along
+    int32_t n_tail = 0, tail_len = 0, N4 = 0;
+    std::vector<int32_t> tail;          // reference ids
+    std::vector<int32_t> bs_ref, bs_off, bs_nodes;
+    bool up_grid = false;
+    double *d_cc = nullptr;
+    int32_t *d_grid = nullptr, *d_tail = nullptr, *d_bs_ref = nullptr, *d_bs_off = nullptr, *d_bs_nodes = nullptr;
+    int32_t *d_sn_ref = nullptr;
     int8_t *d_ext_type = nullptr;
     int64_t *d_ext_off = nullptr;
     int32_t *d_ext_nodes = nullptr, *d_ext_opor = nullptr, *d_ext_order = nullptr;
@@ -290,6 +302,8 @@ int fit_create(FitState **out, int64_t nn, int64_t nf, const int32_t *face_nodes
     }
     F->nb.assign(ns, {});
     F->centre.assign(3 * ns, 0.0);
+    F->cc.assign(2 * ns, std::numeric_limits<double>::quiet_NaN());
+    F->grid.assign(8 * ns, -1);
     *out = F;
     return TITAN_OK;
 }
@@ -298,7 +312,8 @@ void fit_destroy(FitState *F) {
     if (!F) return;
     void *p[] = {F->d_sf_face, F->d_sn_node, F->d_sn_face_off, F->d_sn_face, F->d_nb_off, F->d_nb, F->d_ray_nodes,
                  F->d_sf_kind, F->d_sn_kind, F->d_sf_vel, F->d_sn_vel, F->d_centre, F->d_ray_trig, F->d_node_ray,
-                 F->d_ray_R, F->d_ext_type, F->d_ext_off, F->d_ext_nodes, F->d_ext_opor, F->d_ext_order, F->d_ext_trig};
+                 F->d_ray_R, F->d_ext_type, F->d_ext_off, F->d_ext_nodes, F->d_ext_opor, F->d_ext_order, F->d_ext_trig,
+                 F->d_cc, F->d_grid, F->d_tail, F->d_bs_ref, F->d_bs_off, F->d_bs_nodes, F->d_sn_ref};
     for (void *q : p)
         if (q) hipFree(q);
     delete F;
@@ -461,6 +476,151 @@ int fit_set_rays_ext(FitState *F, const titan_rays_ext *r, const DeviceOrder &o)
     return TITAN_OK;
 }
 
+// 3x3 solve by elimination with partial pivoting; false when singular.
+static bool solve3(double H[3][3], const double g[3], double d[3]) {
+    double M[3][4];
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) M[i][j] = H[i][j];
+        M[i][3] = g[i];
+    }
+    for (int c = 0; c < 3; c++) {
+        int piv = c;
+        for (int r = c + 1; r < 3; r++)
+            if (std::fabs(M[r][c]) > std::fabs(M[piv][c])) piv = r;
+        if (M[piv][c] == 0.0) return false;
+        if (piv != c)
+            for (int j = 0; j < 4; j++) std::swap(M[c][j], M[piv][j]);
+        for (int r = c + 1; r < 3; r++) {
+            const double f = M[r][c] / M[c][c];
+            for (int j = c; j < 4; j++) M[r][j] -= f * M[c][j];
+        }
+    }
+    for (int i = 2; i >= 0; i--) {
+        double t = M[i][3];
+        for (int j = i + 1; j < 3; j++) t -= M[i][j] * d[j];
+        d[i] = t / M[i][i];
+    }
+    return true;
+}
+
+// fitCircleRobust with max_iter 8, tol 1e-8 (Setka_move.cpp:2544-2620): Levenberg-Marquardt on the
+// centre and radius, from the origin and the points' mean distance from it.  The reference sets
+// prev_error to the new error before comparing the two, so the damping grows tenfold every step; a
+// step whose update is not finite only raises the damping.
+int circle_fit(int64_t n, const double *pts, double *out) {
+    if (n < 3 || !pts || !out) return TITAN_E_ARG;
+    double P[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = 0; i < n; i++) P[2] += std::sqrt(pts[2 * i] * pts[2 * i] + pts[2 * i + 1] * pts[2 * i + 1]);
+    P[2] /= (double)n;
+    double lambda = 0.001, prev = std::numeric_limits<double>::max();
+    for (int it = 0; it < 8; it++) {
+        double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, g[3] = {0, 0, 0}, err = 0.0;
+        for (int64_t i = 0; i < n; i++) {
+            const double dx = pts[2 * i] - P[0], dy = pts[2 * i + 1] - P[1];
+            const double dist = std::sqrt(dx * dx + dy * dy);
+            double res, J[3] = {0.0, 0.0, -1.0};
+            if (dist < 1e-10) {
+                res = -P[2];
+            } else {
+                res = dist - P[2];
+                J[0] = -dx / dist;
+                J[1] = -dy / dist;
+            }
+            err += res * res;
+            for (int a = 0; a < 3; a++) {
+                g[a] -= J[a] * res;
+                for (int b = 0; b < 3; b++) H[a][b] += J[a] * J[b];
+            }
+        }
+        err = std::sqrt(err / (double)n);
+        if (std::fabs(prev - err) < 1e-8) break;
+        prev = err;
+        for (int a = 0; a < 3; a++) H[a][a] *= 1.0 + lambda;
+        double d[3];
+        if (!solve3(H, g, d) || !std::isfinite(d[0]) || !std::isfinite(d[1]) || !std::isfinite(d[2])) {
+            lambda *= 10.0;
+            continue;
+        }
+        for (int a = 0; a < 3; a++) P[a] += d[a];
+        lambda = err < prev ? lambda / 2.0 : lambda * 10.0;
+    }
+    out[0] = P[0];
+    out[1] = P[1];
+    out[2] = P[2];
+    return TITAN_OK;
+}
+
+int fit_smooth_angle(FitState *F, int64_t n, const int32_t *nodes, const int32_t *nb6, const double *x0) {
+    if (!F || n < 0 || (n > 0 && (!nodes || !nb6)) || !x0) return TITAN_E_ARG;
+    for (int64_t k = 0; k < n; k++) {
+        if (nodes[k] < 0 || nodes[k] >= F->nn) return TITAN_E_ARG;
+        const int32_t s = F->node_sn[nodes[k]];
+        if (s < 0 || F->sn_kind[s] != 2) return TITAN_E_ARG;
+        for (int q = 0; q < 6; q++)
+            if (nb6[6 * k + q] < 0 || nb6[6 * k + q] >= F->nn) return TITAN_E_ARG;
+    }
+    for (int64_t k = 0; k < n; k++) {
+        const int32_t s = F->node_sn[nodes[k]];
+        double pts[14], cr[3];
+        pts[0] = x0[3 * (int64_t)nodes[k] + 1];
+        pts[1] = x0[3 * (int64_t)nodes[k] + 2];
+        for (int q = 0; q < 6; q++) {
+            pts[2 + 2 * q] = x0[3 * (int64_t)nb6[6 * k + q] + 1];
+            pts[3 + 2 * q] = x0[3 * (int64_t)nb6[6 * k + q] + 2];
+        }
+        circle_fit(7, pts, cr);
+        F->cc[2 * s] = cr[0];
+        F->cc[2 * s + 1] = cr[1];
+        std::vector<int32_t> &nb = F->nb[s];  // a set: each neighbour once, ascending id
+        for (int q = 0; q < 6; q++) nb.push_back(nb6[6 * k + q]);
+        std::sort(nb.begin(), nb.end());
+        nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+    }
+    F->have_cc = F->have_cc || n > 0;
+    F->up_nb = false;
+    F->up_grid = false;
+    return TITAN_OK;
+}
+
+int fit_set_hp_grid(FitState *F, const titan_hp_grid *g, const DeviceOrder &o) {
+    if (!F || !g || g->n_grid < 0 || g->n_tail < 0 || g->n_bs < 0) return TITAN_E_ARG;
+    if (g->n_grid > 0 && (!g->grid_node || !g->grid_nb)) return TITAN_E_ARG;
+    if (g->n_tail > 0 && (!g->tail || g->N4 < 6 || g->tail_len < g->N4 - 3)) return TITAN_E_ARG;
+    if (g->n_bs > 0 && (!g->bs_ref || !g->bs_off || !g->bs_nodes || g->bs_off[0] != 0)) return TITAN_E_ARG;
+    auto kind_of = [&](int32_t v) { return (v < 0 || v >= F->nn || F->node_sn[v] < 0) ? 0 : F->sn_kind[F->node_sn[v]]; };
+    std::vector<int32_t> grid(8 * F->sn_ref.size(), -1);
+    for (int64_t k = 0; k < g->n_grid; k++) {
+        if (kind_of(g->grid_node[k]) != 2) return TITAN_E_ARG;
+        const int32_t s = F->node_sn[g->grid_node[k]];
+        if (grid[8 * s] >= 0) return TITAN_E_ARG;
+        for (int q = 0; q < 8; q++) {
+            const int32_t v = g->grid_nb[8 * k + q];
+            if (v < 0 || v >= F->nn) return TITAN_E_ARG;
+            grid[8 * s + q] = v;
+        }
+    }
+    const int64_t nt = (int64_t)g->n_tail * g->tail_len;
+    for (int64_t i = 0; i < nt; i++)
+        if (kind_of(g->tail[i]) != 2) return TITAN_E_ARG;
+    for (int32_t a = 0; a < g->n_bs; a++) {
+        if (kind_of(g->bs_ref[a]) != 3 || g->bs_off[a + 1] < g->bs_off[a]) return TITAN_E_ARG;
+        for (int32_t j = g->bs_off[a]; j < g->bs_off[a + 1]; j++)
+            if (kind_of(g->bs_nodes[j]) != 3) return TITAN_E_ARG;
+    }
+    F->grid.swap(grid);
+    F->have_grid = g->n_grid > 0;
+    F->hp_k[0] = g->sglag_HP_k_angle; F->hp_k[1] = g->sglag_HP_k;
+    F->hp_k[2] = g->sglag_HP_angle; F->hp_k[3] = g->sglag_HP_along;
+    F->n_tail = g->n_tail; F->tail_len = g->n_tail > 0 ? g->tail_len : 0; F->N4 = g->N4;
+    F->tail.assign(g->tail, g->tail + (g->n_tail > 0 ? nt : 0));
+    F->bs_ref.assign(g->bs_ref, g->bs_ref + g->n_bs);
+    F->bs_off.assign(g->bs_off, g->bs_off + (g->n_bs > 0 ? g->n_bs + 1 : 0));
+    F->bs_nodes.assign(g->bs_nodes, g->bs_nodes + (g->n_bs > 0 ? g->bs_off[g->n_bs] : 0));
+    (void)o;
+    F->up_grid = false;
+    return TITAN_OK;
+}
+
 int fit_smooth(FitState *F, int kind, const double *x0) {
     if (!F || kind < 1 || kind > 3 || !x0) return TITAN_E_ARG;
     if (kind == 2 && !F->have_rays) return TITAN_E_ARG;  // phi_a comes with the rays
@@ -540,6 +700,23 @@ int fit_prepare(FitState *F, const titan_params &p, hipStream_t s, FitDev *dev) 
         if (rc != TITAN_OK) return rc;
         F->up_ext = true;
     }
+    if (!F->up_grid) {
+        const DeviceOrder &o = *F->ord;
+        auto dev_of = [&](const std::vector<int32_t> &v) {
+            std::vector<int32_t> d(v.size());
+            for (size_t i = 0; i < v.size(); i++) d[i] = v[i] >= 0 ? o.node_r2d[v[i]] : -1;
+            return d;
+        };
+        up(upload(&F->d_cc, F->cc, s));
+        up(upload(&F->d_grid, dev_of(F->grid), s));
+        up(upload(&F->d_tail, dev_of(F->tail), s));
+        up(upload(&F->d_bs_ref, dev_of(F->bs_ref), s));
+        up(upload(&F->d_bs_off, F->bs_off, s));
+        up(upload(&F->d_bs_nodes, dev_of(F->bs_nodes), s));
+        up(upload(&F->d_sn_ref, F->sn_ref, s));
+        if (rc != TITAN_OK) return rc;
+        F->up_grid = true;
+    }
     FitDev &D = *dev;
     D = FitDev{};
     D.n_sf = (int32_t)F->sf_ref.size();
@@ -560,6 +737,12 @@ int fit_prepare(FitState *F, const titan_params &p, hipStream_t s, FitDev *dev) 
     D.velocity[1] = p.velocity_TS; D.velocity[2] = p.velocity_HP; D.velocity[3] = p.velocity_BS;
     D.k_smooth[1] = p.sglag_TS_k_sphere; D.k_smooth[2] = p.sglag_HP_k_sphere; D.k_smooth[3] = p.sglag_BS_k;
     D.phi_lim = F->phi_lim;
+    D.sn_cc = F->have_cc ? F->d_cc : nullptr;
+    D.sn_grid = F->have_grid ? F->d_grid : nullptr;
+    D.k_angle = F->hp_k[0]; D.k_hp = F->hp_k[1]; D.hp_angle = F->hp_k[2]; D.hp_along = F->hp_k[3];
+    D.n_tail = F->n_tail; D.tail_len = F->tail_len; D.N4 = F->N4; D.tail = F->d_tail;
+    D.n_bs = (int32_t)F->bs_ref.size(); D.bs_ref = F->d_bs_ref; D.bs_off = F->d_bs_off; D.bs_nodes = F->d_bs_nodes;
+    D.sn_ref = F->d_sn_ref;
     if (F->have_rays && F->ext.n > 0) {
         D.ext = F->ext;
         D.ext.type = F->d_ext_type; D.ext.off = F->d_ext_off; D.ext.nodes = F->d_ext_nodes;

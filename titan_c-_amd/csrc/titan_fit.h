@@ -16,6 +16,12 @@ int fit_create(FitState **out, int64_t nn, int64_t nf, const int32_t *face_nodes
 void fit_destroy(FitState *F);
 int fit_set_rays(FitState *F, const titan_rays *r, const DeviceOrder &o);
 int fit_set_rays_ext(FitState *F, const titan_rays_ext *r, const DeviceOrder &o);
+// Setka::Smooth_angle_HP from node layer 0 x0 (reference numbering): circle centres and the
+// azimuthal neighbours added to each listed HP node's smoothing set.
+int fit_smooth_angle(FitState *F, int64_t n, const int32_t *nodes, const int32_t *nb6, const double *x0);
+int fit_set_hp_grid(FitState *F, const titan_hp_grid *g, const DeviceOrder &o);
+// fitCircleRobust (Setka_move.cpp:2544-2620) of n points [n*2] -> centre_radius [3].
+int circle_fit(int64_t n, const double *pts, double *centre_radius);
 // kind 1: Setka::Smooth_head_TS3, 2: Smooth_head_HP3, 3: Find_Yzel_Sosed_for_BS, from the node
 // coordinates x0 [nn*3] (reference numbering) of node layer 0.
 int fit_smooth(FitState *F, int kind, const double *x0);

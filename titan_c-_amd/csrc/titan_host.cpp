@@ -153,11 +153,11 @@ int check_scalars(titan_ctx *c, StepScalars *dS, StepScalars *host) {
     HIPCHK(hipMemcpyAsync(host, dS, sizeof(StepScalars), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (host->err_key != ~0ull) {
-        static const int kCode[5] = {TITAN_E_SURFACE, TITAN_E_RIEMANN, TITAN_E_RIEMANN, TITAN_E_NONFINITE_STATE,
-                                     TITAN_E_ARG};
+        static const int kCode[6] = {TITAN_E_SURFACE, TITAN_E_RIEMANN, TITAN_E_SURFACE, TITAN_E_RIEMANN,
+                                     TITAN_E_NONFINITE_STATE, TITAN_E_ARG};
         const unsigned rank = (unsigned)(host->err_key >> 48);
         const int64_t idx = (int64_t)(host->err_key & ((1ull << 48) - 1));
-        const int code = rank < 5 ? kCode[rank] : TITAN_E_ARG;
+        const int code = rank < 6 ? kCode[rank] : TITAN_E_ARG;
         // clear for later calls
         const unsigned long long none = ~0ull;
         HIPCHK(hipMemcpy(&dS->err_key, &none, sizeof(none), hipMemcpyHostToDevice));
@@ -885,6 +885,23 @@ static int fit_smooth_ctx(titan_ctx *c, int kind) {
     const int rc = nodes_layer0(c, x);
     if (rc != TITAN_OK) return rc;
     return fit_smooth(c->fit, kind, x.data());
+}
+
+extern "C" int titan_smooth_angle_hp(titan_ctx *c, int64_t n, const int32_t *nodes, const int32_t *nb6) {
+    if (!c) return TITAN_E_ARG;
+    std::vector<double> x;
+    const int rc = nodes_layer0(c, x);
+    if (rc != TITAN_OK) return rc;
+    return fit_smooth_angle(c->fit, n, nodes, nb6, x.data());
+}
+
+extern "C" int titan_set_hp_grid(titan_ctx *c, const titan_hp_grid *g) {
+    if (!c) return TITAN_E_ARG;
+    return fit_set_hp_grid(c->fit, g, c->ord);
+}
+
+extern "C" int titan_circle_fit(int64_t n, const double *points, double *centre_radius) {
+    return circle_fit(n, points, centre_radius);
 }
 
 extern "C" int titan_smooth_head_ts3(titan_ctx *c) { return fit_smooth_ctx(c, 1); }
