@@ -290,6 +290,16 @@ int titan_chlld_batch_noalloc(int64_t n, const int8_t *n_state, const int8_t *n_
  * TITAN_SRC_MLP: the Maxwellian
  * charge-exchange integral with the neutrals of titan_set_neutrals (out[0] = exchange rate). */
 int titan_sources(titan_ctx *ctx, int layer, int mode, double *out);
+/* Calc_sourse_MF_Bera's hydrogen-fluid outputs (Setka_physics.cpp:1005-1094, 1241-1250) beside its
+ * plasma ones, from the fluids of titan_set_h_fluids (n_h <= TITAN_MAX_H_FLUIDS) at layer `layer`:
+ * out_h [n_cells][n_h][5] = (rho, m_x, m_y, m_z, E) source of each fluid -- what its exchanges with
+ * the protons take from it, plus the atoms those exchanges make, which join fluid
+ * arise[(zone-1)*n_h + k] (1-based, zone = determ_zone of the cell; NULL: fluid `zone`, the
+ * hydrogen_arise tables of the run without pickup ions, Phys_param.cpp:149-171) with the protons'
+ * momentum and energy; all times n_H_LISM/Kn, then over n_H_LISM.  out [n_cells*5] (nullable):
+ * the plasma sources, as titan_sources(TITAN_SRC_BERA).  Pickup-ion fluids are not carried. */
+#define TITAN_MAX_H_FLUIDS 16
+int titan_sources_h(titan_ctx *ctx, int layer, const int8_t *arise, double *out, double *out_h);
 /* Standalone exact source for n states: in [n*10] = n_p, c_p^2, V_p(3), n_H, c_H^2, V_H(3). */
 int titan_cx_exact_batch(int64_t n, const double *in10, int nodes, double *out5);
 

@@ -95,6 +95,7 @@ def lib():
             "or_go_k": (_I, [_P, _P, _P, _P, _P, _P, _I64, _I, _I, _P, _P, _P, _I, _P, _P]),
             "or_determ_zone": (_I, [_D, _D, _I]),
             "or_bera_source_k": (None, [_P, _D, _D, _I, _I, _P, _D, _D, _D, _P]),
+            "or_bera_source_h": (None, [_P, _D, _D, _I, _I, _P, _D, _D, _D, _P, _P, _P]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -412,6 +413,23 @@ def bera_source(prim, h_fluids, a2=0.102046, n_h_lism=3.0, kn=40.0906, scal=None
             L.or_bera_source_k(_p(prim[c]), float(scal[c][0]), float(scal[c][1]), int(cell_type[c]), hf.shape[1],
                                _p(hf[c]), a2, n_h_lism, kn, _p(out[c]))
     return out
+
+
+def bera_source_h(prim, h_fluids, a2=0.102046, n_h_lism=3.0, kn=40.0906, scal=None, cell_type=None, arise=None):
+    """or_bera_source_h per cell: (plasma sources [n, 5], hydrogen-fluid sources [n, n_h, 5]); arise
+    [4, n_h] (1-based target fluid per zone and source fluid) or None for fluid = zone."""
+    prim = np.ascontiguousarray(prim, np.float64).reshape(-1, 8)
+    n = prim.shape[0]
+    hf = np.ascontiguousarray(h_fluids, np.float64).reshape(n, -1, 5)
+    nh = hf.shape[1]
+    out = np.empty((n, 5))
+    hout = np.empty((n, nh, 5))
+    ar = None if arise is None else np.ascontiguousarray(arise, np.int8).reshape(4, nh)
+    L = lib()
+    for c in range(n):
+        Q, He, ct = (0.0, 0.0, 1) if scal is None else (float(scal[c][0]), float(scal[c][1]), int(cell_type[c]))
+        L.or_bera_source_h(_p(prim[c]), Q, He, ct, nh, _p(hf[c]), a2, n_h_lism, kn, _p(ar), _p(out[c]), _p(hout[c]))
+    return out, hout
 
 
 def determ_zone(Q, rho, cell_type):

@@ -22,6 +22,7 @@
 #include <omp.h>
 #endif
 
+#define OR_MAX_H 16 /* hydrogen fluids or_bera_source_h takes */
 #define KV(x) ((x) * (x))
 #define KVV(x, y, z) ((x) * (x) + (y) * (y) + (z) * (z))
 /* std::max / std::min semantics (first argument returned on ties and NaN) */
@@ -1464,9 +1465,23 @@ void or_bera_source(const double *s, int n_h, const double *h, double a2, double
 void or_bera_source_k(const double *s, double Q, double rho_He, int cell_type, int n_h, const double *h, double a2,
                       double n_h_lism, double kn, double *out)
 {
+    or_bera_source_h(s, Q, rho_He, cell_type, n_h, h, a2, n_h_lism, kn, NULL, out, NULL);
+}
+
+/* The plasma sources and, in hout [n_h*5] (nullable), the hydrogen fluids' (rho, m_x, m_y, m_z, E):
+ * each fluid loses what its exchanges with the protons take (Setka_physics.cpp:1012-1036), and the
+ * atoms those exchanges make join fluid arise[(zone - 1) * n_h + k] (1-based; NULL: fluid `zone`,
+ * the hydrogen_arise tables of the run without pickup ions, Phys_param.cpp:149-171) carrying the
+ * protons' momentum and energy (:1043-1094); then every source times n_H_LISM / Kn and the
+ * hydrogen ones divided by n_H_LISM (:1236-1250).  Pickup ions are not carried. */
+void or_bera_source_h(const double *s, double Q, double rho_He, int cell_type, int n_h, const double *h, double a2,
+                      double n_h_lism, double kn, const int8_t *arise, double *out, double *hout)
+{
     /* proton density and temperature (Plasma_components_1, Phys_param.cpp:641-660) */
     const double rho = s[0], pr = s[4];
-    const int zone_ = or_determ_zone(Q, rho, cell_type) - 1;
+    const int zone = or_determ_zone(Q, rho, cell_type);
+    const int zone_ = zone - 1;
+    double g_rho[OR_MAX_H], g_m[OR_MAX_H][3], g_E[OR_MAX_H];
     const int al = zone_ <= 2 ? 2 : 1;
     double rho_p = -(OR_ME_MP * al * rho_He + 4.0 * (-rho + rho_He)) / (4.0 * (1.0 + OR_ME_MP));
     double T_p = 8.0 * (1.0 + OR_ME_MP) * pr / (8.0 * rho - (7.0 - al + (-1.0 + al) * OR_ME_MP) * rho_He);
@@ -1502,6 +1517,16 @@ void or_bera_source_k(const double *s, double Q, double rho_He, int cell_type, i
         my += (-Hm_Hp[1] + Hm_pH[1]);
         mz += (-Hm_Hp[2] + Hm_pH[2]);
         E += (-HE_Hp + HE_pH);
+        if (hout && k < OR_MAX_H) {
+            /* :1012-1036, the fluid's losses: Hrho[p+H], Hm[p+H], HE[p+H] */
+            hout[5 * k] = 0.0 + (-(cpH * U_pH));
+            for (int i = 0; i < 3; i++) hout[5 * k + 1 + i] = 0.0 + (-Hm_pH[i]);
+            hout[5 * k + 4] = 0.0 + (-HE_pH);
+            /* the atoms made from this fluid: Hrho[H+p], Hm[H+p], HE[H+p] */
+            g_rho[k] = cHp * U_Hp;
+            for (int i = 0; i < 3; i++) g_m[k][i] = Hm_Hp[i];
+            g_E[k] = HE_Hp;
+        }
     }
     /* :1236-1240 */
     const double ddp = n_h_lism / kn;
@@ -1510,6 +1535,19 @@ void or_bera_source_k(const double *s, double Q, double rho_He, int cell_type, i
     out[2] = my * ddp;
     out[3] = mz * ddp;
     out[4] = E * ddp;
+    if (hout) {
+        /* :1043-1094, rows of the arise table in order */
+        for (int k = 0; k < n_h && k < OR_MAX_H; k++) {
+            int t = (arise ? arise[zone_ * n_h + k] : zone) - 1;
+            if (t < 0 || t >= n_h) continue;
+            hout[5 * t] += g_rho[k];
+            for (int i = 0; i < 3; i++) hout[5 * t + 1 + i] += g_m[k][i];
+            hout[5 * t + 4] += g_E[k];
+        }
+        /* :1236-1250 */
+        for (int k = 0; k < 5 * n_h; k++) hout[k] *= ddp;
+        for (int k = 0; k < 5 * n_h; k++) hout[k] /= n_h_lism;
+    }
 }
 
 /* ------------------------------------------------------------------------------------------ */

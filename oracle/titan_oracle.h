@@ -221,6 +221,11 @@ int or_update_k(const or_mesh *m, const or_params *p, const double *prim_old, co
 int or_determ_zone(double Q, double rho, int cell_type);
 void or_bera_source_k(const double *s, double Q, double rho_He, int cell_type, int n_h, const double *h, double a2,
                       double n_h_lism, double kn, double *out);
+/* or_bera_source_k with the hydrogen fluids' sources too: hout [n_h*5] (rho, m_x, m_y, m_z, E) per
+ * fluid, n_h <= 16 (Setka_physics.cpp:1005-1094, 1241-1250); arise [4*n_h] the fluid (1-based) the
+ * atoms made from fluid k join in zone z at [(z-1)*n_h + k], NULL for fluid z (no pickup ions). */
+void or_bera_source_h(const double *s, double Q, double rho_He, int cell_type, int n_h, const double *h, double a2,
+                      double n_h_lism, double kn, const int8_t *arise, double *out, double *hout);
 /* flux[f*9 + 0..7] = chlld flux * area, flux[f*9+8] = PdivB; dt_face[f] (1e7 on non-interior) */
 int or_face_flux(const or_mesh *m, const or_params *p, const double *prim, const or_geom *g,
                  const double *x_now1, const double *x_now2, double dt, double *flux,

@@ -304,3 +304,28 @@ def test_scalar_calls_need_the_switch(T, C):
         s.set_state(prim, 0)
         with pytest.raises(T.TitanError):
             s.Culc_Gran_Potok(0, 1e-5, want_scalars=True)
+
+
+@pytest.mark.parametrize("table", [False, True])
+def test_bera_fluid_sources_vs_oracle(T, C, O, table):
+    """titan_sources_h: the hydrogen fluids' sources in every zone (default routing, or a caller's
+    arise table), per cell and fluid, against the oracle at 1e-12 of each variable's per-cell scale
+    (the fluid's loss term), and the plasma part equal to titan_sources(BERA)."""
+    mesh, prim, params = _banded(C)
+    scal = _scal(prim, seed=3)
+    h = C.h_fluids(mesh.n_cells)
+    ar = np.array([[2, 1, 4, 3], [1, 1, 1, 1], [4, 4, 2, 2], [3, 4, 1, 2]], np.int8) if table else None
+    with T.Setka(mesh, dict(params, source_mode=T.SRC_BERA)) as s:
+        s.set_state(prim, 0)
+        s.set_scalars(scal, 0)
+        s.set_h_fluids(h)
+        plasma, got = s.sources_h(0, 4, ar)
+        plain = s.sources(0, T.SRC_BERA)
+    np.testing.assert_array_equal(plasma, plain)
+    ref_p, ref = O.bera_source_h(prim, h, scal=scal, cell_type=mesh.cell_zone, arise=ar)
+    np.testing.assert_array_equal(ref_p, O.bera_source(prim, h, scal=scal, cell_type=mesh.cell_zone))
+    scale = np.abs(ref).max(axis=1, keepdims=True) + 1e-300  # per cell and variable, over the fluids
+    err = (np.abs(got - ref) / scale).max()
+    print(f"fluid sources table={table}: max per-cell error {err:.3e}", flush=True)
+    assert err < 1e-12
+    assert np.abs(ref[:, :, 1:]).max() > 0.0

@@ -140,3 +140,68 @@ def test_bera_vs_maxwellian_monte_carlo(O, case):
     e_mc = n_p * rho_h * (g * 0.5 * ((wh * wh).sum(1) - (wp * wp).sum(1))).mean()
     assert np.linalg.norm(out[1:4] - m_mc) <= tol * np.linalg.norm(m_mc), (out[1:4], m_mc)
     assert abs(out[4] - e_mc) <= tol * abs(e_mc), (out[4], e_mc)
+
+
+# ---- the hydrogen fluids' outputs (Setka_physics.cpp:1005-1094, 1241-1250) --------------------
+def _fluids(rng, n_h):
+    return np.stack([np.concatenate([[rng.uniform(0.05, 0.3)], [rng.uniform(0.01, 0.1)], rng.uniform(-3, 1, 3)])
+                     for _ in range(n_h)])
+
+
+@pytest.mark.parametrize("zone_cell", [(0.0, 1), (0.0, 2), (100.0, 3), (100.0, 4)])
+def test_fluid_sources_conserve_with_the_plasma(O, zone_cell):
+    """Charge exchange swaps a proton and an atom: the atoms made equal the atoms lost (mass summed
+    over the fluids is zero), and the fluids gain the momentum and energy the plasma loses; all
+    hydrogen sources carry an extra 1/n_H_LISM."""
+    rng = np.random.default_rng(11)
+    q, ct = zone_cell
+    s = _state(rho=0.9, v=(0.5, -0.1, 0.2), p=0.4)
+    h = _fluids(rng, 4)
+    out, hout = O.bera_source_h(s[None], h[None], scal=[(q * s[0], 0.1)], cell_type=[ct])
+    out, hout = out[0], hout[0]
+    ref = O.bera_source(s[None], h[None], scal=[(q * s[0], 0.1)], cell_type=[ct])[0]
+    np.testing.assert_array_equal(out, ref)  # the plasma part is unchanged
+    scale = np.abs(hout[:, 0]).max()
+    assert abs(hout[:, 0].sum()) < 1e-14 * scale
+    for i in range(1, 5):
+        assert abs(out[i] + NH * hout[:, i].sum()) < 1e-13 * np.abs(out[i])
+    assert np.abs(hout).min() >= 0.0 and np.abs(hout[:, 1:]).max() > 0.0
+
+
+def test_fluid_sources_route_to_the_zone_fluid(O):
+    """Without pickup ions the atoms made from every fluid join fluid `zone` (the hydrogen_arise
+    tables of Phys_param.cpp:149-171): one fluid with atoms in zone 3 loses what fluid 3 gains."""
+    s = _state()
+    h = np.zeros((4, 5))
+    h[:, 1] = 0.05
+    h[1] = [0.2, 0.05, -2.0, 0.3, 0.1]
+    out, hout = O.bera_source_h(s[None], h[None], scal=[(100.0 * s[0], 0.0)], cell_type=[3])
+    hout = hout[0]
+    assert np.all(hout[[0, 3]] == 0.0)
+    assert hout[1, 0] < 0.0 and hout[2, 0] > 0.0
+    assert abs(hout[1, 0] + hout[2, 0]) < 1e-15 * abs(hout[2, 0])
+    # in zone 2 the same atoms join fluid 2, the one they came from: its mass source cancels
+    out, h2 = O.bera_source_h(s[None], h[None], scal=[(0.0, 0.0)], cell_type=[2])
+    assert abs(h2[0, 1, 0]) < 1e-15 * abs(hout[2, 0])
+    # a caller's table sends zone 3's atoms to fluid 4
+    ar = np.tile(np.arange(1, 5), (4, 1))
+    ar[2] = 4
+    out, h3 = O.bera_source_h(s[None], h[None], scal=[(100.0 * s[0], 0.0)], cell_type=[3], arise=ar)
+    np.testing.assert_array_equal(h3[0, 3], hout[2])
+    assert np.all(h3[0, 2] == 0.0)
+
+
+def test_fluid_sources_cold_beam(O):
+    """A cold beam crossing cold protons: fluid k loses sigma n_p n_H |dV| atoms per unit time with
+    momentum n_H-weighted V_H, and the atoms made carry V_p (times n_H_LISM/Kn over n_H_LISM)."""
+    rho, vp = 1.1, np.array([0.4, -0.3, 0.2])
+    s = np.array([rho, *vp, 1e-12, 0.0, 0.0, 0.0])
+    vh = np.array([-2.0, 0.5, 0.3])
+    h = np.array([[0.3, 1e-12, *vh], [0.0, 0.0, 0.0, 0.0, 0.0]])
+    out, hout = O.bera_source_h(s[None], h[None])   # zone 1: atoms made join fluid 1
+    g = np.linalg.norm(vp - vh)
+    sig = (1.0 - A2 * np.log(g)) ** 2
+    n_p = rho / (1.0 + ME_MP)
+    rate = sig * n_p * 0.3 * g * DDP / NH
+    np.testing.assert_allclose(hout[0, 0, 0], 0.0, atol=1e-8 * rate)
+    np.testing.assert_allclose(hout[0, 0, 1:4], rate * (vp - vh), rtol=1e-6)

@@ -148,6 +148,7 @@ _SIGS = {
     "titan_chlld_batch": (_I, [_I64, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _I]),
     "titan_chlld_batch_noalloc": (_I, [_I64, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P]),
     "titan_sources": (_I, [_P, _I, _I, _P]),
+    "titan_sources_h": (_I, [_P, _I, _P, _P, _P]),
     "titan_mk_go": (_I, [_P, _I, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, _P, _P]),
     "titan_cx_exact_batch": (_I, [_I64, _P, _I, _P]),
     "titan_mlp_n_params": (_I64, []),
@@ -585,6 +586,15 @@ class Setka:
         out = np.empty((self.n_cells, 5))
         _check(lib().titan_sources(self.ctx, layer, mode, _ptr(out)), "titan_sources", self.ctx)
         return out
+
+    def sources_h(self, layer, n_h, arise=None):
+        """Calc_sourse_MF_Bera with the hydrogen fluids' outputs: (plasma [n_cells, 5], fluids
+        [n_cells, n_h, 5]); arise [4, n_h] 1-based target fluid per zone, None for fluid = zone."""
+        out = np.empty((self.n_cells, 5))
+        hout = np.empty((self.n_cells, n_h, 5))
+        ar = None if arise is None else np.ascontiguousarray(arise, np.int8).reshape(4, n_h)
+        _check(lib().titan_sources_h(self.ctx, layer, _ptr(ar), _ptr(out), _ptr(hout)), "titan_sources_h", self.ctx)
+        return out, hout
 
     def MK_go(self, n_per_face, seed=1, max_steps=100000, layer=None):
         """Setka::MK_go (Setka_MK.cpp:1225-2316) for the interstellar hydrogen: returns (density

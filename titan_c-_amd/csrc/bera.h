@@ -6,8 +6,8 @@
 // zone of Setka::determ_zone (Setka_physics.cpp:1427-1482; 1 or 2 by cell type where Q/rho < 50,
 // else 4 for a Zone_4 cell and 3 otherwise), passed on as zone - 1 (Setka_physics.cpp:769,783), so
 // al = 2 unless zone - 1 > 2.  A state without the passive scalars reads Q = rho_He = 0 (zone 1
-// or 2, al = 2).  The plasma mass source is 0 (charge exchange swaps a proton for a proton); the
-// hydrogen-fluid and pickup-ion sources of the same routine are not formed.  Evaluated in the
+// or 2, al = 2).  The plasma mass source is 0 (charge exchange swaps a proton for a proton).  The
+// hydrogen fluids' sources are formed on request; pickup ions are not carried.  Evaluated in the
 // reference's operation order (FP64, no contraction in the translation units that include it);
 // only log and sqrt differ from glibc in the last ulp.
 #pragma once
@@ -48,11 +48,21 @@ __host__ __device__ inline int bera_zone(double Q, double rho, int cell_type) {
     return cell_type == 4 ? 4 : 3;
 }
 
+constexpr int kBeraMaxH = 16;  // hydrogen fluids the fluid sources take
+
 // s: plasma state (rho, V, p, B); rho_He, zone (bera_zone); h[5k..5k+4] = (rho_Hk, p_Hk, V_Hk);
 // a2 = par_a_2; ddp = par_n_H_LISM / par_Kn.  out = (rho, m_x, m_y, m_z, E) sources per unit volume
-// and time.
+// and time.  hout (nullable, n_h <= kBeraMaxH): the hydrogen fluids' (rho, m_x, m_y, m_z, E) --
+// each fluid's losses to the protons (Setka_physics.cpp:1012-1036), then the atoms made from fluid
+// k added to fluid arise[(zone-1)*n_h + k] (1-based; null: fluid `zone`, the run without pickup
+// ions, Phys_param.cpp:149-171) in fluid order (:1043-1094), all times ddp and over n_h_lism
+// (:1236-1250).
+template <bool H = false>
 __device__ __forceinline__ void bera_source(const double s[8], double rho_He, int zone, int n_h,
-                                            const double *__restrict__ h, double a2, double ddp, double out[5]) {
+                                            const double *__restrict__ h, double a2, double ddp, double out[5],
+                                            double *__restrict__ hout = nullptr,
+                                            const int8_t *__restrict__ arise = nullptr, double n_h_lism = 1.0) {
+    double gain[H ? 5 * kBeraMaxH : 1];  // the atoms made from each fluid, added after every loss
     // Plasma_components_1 (Phys_param.cpp:641-660) at zone - 1
     const double rho = s[0], pr = s[4];
     const int al = (zone - 1 <= 2) ? 2 : 1;
@@ -92,6 +102,22 @@ __device__ __forceinline__ void bera_source(const double s[8], double rho_He, in
         my += (-Hm_Hp[1] + Hm_pH[1]);
         mz += (-Hm_Hp[2] + Hm_pH[2]);
         E += (-HE_Hp + HE_pH);
+        if constexpr (H) {
+            // :1012-1036: the fluid's losses Hrho[p+H], Hm[p+H], HE[p+H]; the atoms made from it
+            // carry Hrho[H+p], Hm[H+p], HE[H+p]
+            double *o = hout + 5 * k;
+            o[0] = 0.0 + (-(cpH * U_pH));
+            o[1] = 0.0 + (-Hm_pH[0]);
+            o[2] = 0.0 + (-Hm_pH[1]);
+            o[3] = 0.0 + (-Hm_pH[2]);
+            o[4] = 0.0 + (-HE_pH);
+            double *gk = gain + 5 * k;
+            gk[0] = cHp * U_Hp;
+            gk[1] = Hm_Hp[0];
+            gk[2] = Hm_Hp[1];
+            gk[3] = Hm_Hp[2];
+            gk[4] = HE_Hp;
+        }
     }
     // :1236-1240
     out[0] = 0.0 * ddp;
@@ -99,6 +125,15 @@ __device__ __forceinline__ void bera_source(const double s[8], double rho_He, in
     out[2] = my * ddp;
     out[3] = mz * ddp;
     out[4] = E * ddp;
+    if constexpr (H) {
+        for (int k = 0; k < n_h; k++) {  // :1043-1094, rows of the arise table in order
+            const int t = (arise ? arise[(zone - 1) * n_h + k] : zone) - 1;
+            if (t < 0 || t >= n_h) continue;
+#pragma unroll
+            for (int i = 0; i < 5; i++) hout[5 * t + i] += gain[5 * k + i];
+        }
+        for (int k = 0; k < 5 * n_h; k++) hout[k] = hout[k] * ddp / n_h_lism;  // :1236-1250
+    }
 }
 
 }  // namespace titan

@@ -1,6 +1,7 @@
 // k_cx.hip -- exact FP64 charge-exchange source kernels and the reference's closed-form
 // (Calc_sourse_MF_Bera) source (FP64, no contraction).
 #define TITAN_FLUX_ONLY 1
+#include "../../include/titan_gpu.h"
 #include "bera.h"
 #include "cx_device.h"
 #include "launch.h"
@@ -129,6 +130,35 @@ __global__ void __launch_bounds__(256) k_bera_cells(int64_t nc, const double *__
     bera_source(s, rho_He, zone, n_h, hfluid + (int64_t)5 * n_h * c, a2, ddp, o);
 #pragma unroll
     for (int k = 0; k < 5; k++) src[5 * c + k] = o[k];
+}
+
+static_assert(kBeraMaxH == TITAN_MAX_H_FLUIDS, "the hydrogen fluids the sources take");
+
+// The hydrogen fluids' sources of each cell (with its plasma sources): hout [nc][n_h][5].
+__global__ void __launch_bounds__(256) k_bera_cells_h(int64_t nc, const double *__restrict__ prim,
+                                                      const double *__restrict__ scal,
+                                                      const uint32_t *__restrict__ cell_code,
+                                                      const double *__restrict__ hfluid, int n_h, double a2,
+                                                      double ddp, double n_h_lism, const int8_t *__restrict__ arise,
+                                                      double *__restrict__ src, double *__restrict__ hout) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double s[8];
+    load_state(prim, c, s);
+    const double Q = scal ? scal[2 * c] : 0.0, rho_He = scal ? scal[2 * c + 1] : 0.0;
+    const int zone = bera_zone(Q, s[0], (int)((cell_code[c] >> 8) & 15u));
+    double o[5];
+    bera_source<true>(s, rho_He, zone, n_h, hfluid + (int64_t)5 * n_h * c, a2, ddp, o, hout + (int64_t)5 * n_h * c,
+                      arise, n_h_lism);
+#pragma unroll
+    for (int k = 0; k < 5; k++) src[5 * c + k] = o[k];
+}
+
+void launch_bera_cells_h(hipStream_t s, int64_t nc, const double *prim, const double *scal, const uint32_t *cell_code,
+                         const double *hfluid, int n_h, double a2, double ddp, double n_h_lism, const int8_t *arise,
+                         double *src, double *hout) {
+    hipLaunchKernelGGL(k_bera_cells_h, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, nc, prim, scal, cell_code,
+                       hfluid, n_h, a2, ddp, n_h_lism, arise, src, hout);
 }
 
 void launch_bera_cells(hipStream_t s, int64_t nc, const double *prim, const double *scal, const uint32_t *cell_code,

@@ -70,6 +70,10 @@ void launch_cx_cells(hipStream_t s, int64_t nc, const double *prim, const double
 // (nullable: Q = rho_He = 0) with the cell codes' zones, and hfluid [Nc * n_h * 5]
 void launch_bera_cells(hipStream_t s, int64_t nc, const double *prim, const double *scal, const uint32_t *cell_code,
                        const double *hfluid, int n_h, double a2, double ddp, double *src);
+// the same with the hydrogen fluids' sources hout [nc][n_h][5] (n_h <= kBeraMaxH); arise: nullable
+void launch_bera_cells_h(hipStream_t s, int64_t nc, const double *prim, const double *scal, const uint32_t *cell_code,
+                         const double *hfluid, int n_h, double a2, double ddp, double n_h_lism, const int8_t *arise,
+                         double *src, double *hout);
 // exact source + update of the cells the surrogate step listed as out of its training box
 void launch_cx_fixup(hipStream_t s, const DevMesh &M, const double *prim1, const double *neutrals, int nodes,
                      const int32_t *list, const double *stash, int64_t cap, StepScalars *S, double *prim2);

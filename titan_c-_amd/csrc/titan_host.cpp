@@ -1115,6 +1115,38 @@ extern "C" int titan_sources(titan_ctx *c, int layer, int mode, double *out) {
     return TITAN_OK;
 }
 
+extern "C" int titan_sources_h(titan_ctx *c, int layer, const int8_t *arise, double *out, double *out_h) {
+    if (!c || layer < 0 || layer > 1 || !out_h || !c->d_hfluid) return TITAN_E_ARG;
+    const int n_h = c->n_hfluid;
+    if (n_h < 1 || n_h > TITAN_MAX_H_FLUIDS) return TITAN_E_ARG;
+    if (arise)
+        for (int i = 0; i < 4 * n_h; i++)
+            if (arise[i] < 1 || arise[i] > n_h) return TITAN_E_ARG;
+    const int prc = prepare_sources(c, TITAN_SRC_BERA);
+    if (prc != TITAN_OK) return prc;
+    const int64_t nh5 = (int64_t)5 * n_h * c->nc;
+    DevBuf d_h, d_ar;
+    int rc = alloc(d_h, sizeof(double) * nh5);
+    if (rc == TITAN_OK && arise) rc = dup(d_ar, arise, 4 * n_h);
+    if (rc != TITAN_OK) return rc;
+    const titan_params &p = c->params;
+    launch_bera_cells_h(c->stream, c->nc, c->d_prim[layer], c->d_scal[layer], c->d_cell_code, c->d_hfluid, n_h,
+                        p.par_a_2, p.par_n_H_LISM / p.par_Kn, p.par_n_H_LISM, (const int8_t *)d_ar.p, c->d_src,
+                        (double *)d_h.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(nh5), s5(5 * c->nc);
+    HIPCHK(hipMemcpyAsync(h.data(), d_h.p, nh5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(s5.data(), c->d_src, s5.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t d = 0; d < c->nc; d++) {
+        const int64_t r = c->ord.cell_d2r[d];
+        for (int k = 0; k < 5 * n_h; k++) out_h[5 * n_h * r + k] = h[5 * n_h * d + k];
+        if (out)
+            for (int k = 0; k < 5; k++) out[5 * r + k] = s5[5 * d + k];
+    }
+    return TITAN_OK;
+}
+
 // ---- Monte-Carlo neutral densities (kernels in k_mk.hip) --------------------------------------
 // Get_Spotok_inf (Setka_MK.cpp:2316-2324): inflow of the unit-temperature LISM Maxwellian drifting
 // at (v_inf, 0, 0) through a face of outward normal n, per unit area
