@@ -198,7 +198,11 @@ def gate_report(T, C, mlp, workload, steps):
             s.set_neutrals(nh)
             s.mlp_set(*mlp)
             t0 = time.perf_counter()
-            st = s.Go(steps)
+            try:
+                st = s.Go(steps)
+            except T.TitanError as e:  # a run that stops with a status code is reported, not hidden
+                log("gate %s stopped: %s" % (name, e))
+                return {"workload": workload, "n_cells": mesh.n_cells, "steps": steps, "error": "%s: %s" % (name, e)}
             runs[name] = dict(state=s.get_state(), seconds=time.perf_counter() - t0, stats=st)
         log("gate %s: %d steps in %.1f s" % (name, steps, runs[name]["seconds"]))
     r_au = cell_radius_au(mesh)
@@ -443,7 +447,8 @@ def main():
     gate = None
     if a.gate_steps > 0 and "mixed_surrogate" in variants and sur_info is not None:
         gate = gate_report(T, C, mlp, a.gate_workload, a.gate_steps)
-        log("gate: per-cell rho %s p %s" % (gate["per_cell"]["rho"], gate["per_cell"]["p"]))
+        if "per_cell" in gate:
+            log("gate: per-cell rho %s p %s" % (gate["per_cell"]["rho"], gate["per_cell"]["p"]))
 
     def cups(name):
         r = res.get(name)

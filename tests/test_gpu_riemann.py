@@ -105,3 +105,39 @@ def test_chlld_mixed_error_scales_with_the_jump(T, O, method):
     # FP64 except the normal-field dissipation -S_N dBn, whose fast speed is FP32
     assert (np.abs(got[0][up][:, :5] - ref[0][up][:, :5]) / scale).max() < 1e-14
     assert (np.abs(got[0][up] - ref[0][up]) / scale).max() < 1e-12
+
+
+@pytest.mark.parametrize("method", [0, 1, 2, 3])
+def test_chlld_mixed_field_along_the_normal_at_the_alfven_speed(T, O, method):
+    """Field along the normal and the sound speed a hair above the Alfven speed: the fast speed's
+    inner radicand b^2/rho + c (c - 2 b_n/sqrt(rho)) = (c - b_n/sqrt(rho))^2 + (b^2 - b_n^2)/rho is
+    ~1e-11 of c^2, which FP64 keeps and FP32 rounding takes below zero about half the time.  Every
+    face still solves in the mixed solver (no fatal branch) and stays close to the FP64 flux."""
+    rng = np.random.default_rng(31 + method)
+    n = 20_000
+    s4 = np.sqrt(4.0 * np.pi)
+    gamma = 5.0 / 3.0
+    rho = rng.uniform(0.01, 2.0, n)
+    bn = rng.uniform(0.2, 3.0, n) * rng.choice([-1.0, 1.0], n)  # b = B / sqrt(4 pi)
+    p = bn ** 2 / gamma * (1.0 + 1e-5 * rng.uniform(0.1, 1.0, n))  # c^2 = (b_n^2 / rho)(1 + eps)
+    nrm = random_normals(rng, n)
+    qL = np.zeros((n, 8))
+    qL[:, 0], qL[:, 4] = rho, p
+    qL[:, 5:8] = (bn * s4)[:, None] * nrm
+    qL[:, 1:4] = rng.normal(0.0, 3.0, (n, 3))
+    qR = qL.copy()
+    qR[:, 1:4] += rng.normal(0.0, 0.05, (n, 3))  # a velocity jump; the radicands stay degenerate
+    # the FP32 radicand on the host, in the solver's operation order, for the side whose sign of
+    # b_n makes it cancel: a good share of the faces have it below zero
+    f = np.float32
+    aa = (bn * np.sign(bn)).astype(f) / np.sqrt(rho.astype(f))
+    c = np.sqrt(f(gamma) * p.astype(f) / rho.astype(f))
+    rad = (bn ** 2).astype(f) / rho.astype(f) + c * (c - f(2.0) * aa)
+    assert (rad < 0).mean() > 0.1
+    ref = O.chlld_batch(method, nrm, qL, qR, w=np.zeros(n))
+    assert (ref[3] & 7).min() > 0  # FP64 solves every face
+    got = T.chlld(method, nrm, qL, qR, w=np.zeros(n), precision=T.PREC_MIXED)
+    assert got[4] == T.TITAN_OK
+    assert (got[3] & 7).min() > 0
+    scale = np.abs(ref[0]).max(axis=1, keepdims=True)
+    assert (np.abs(got[0] - ref[0]) / scale).max() < 1e-4

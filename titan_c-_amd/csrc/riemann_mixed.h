@@ -134,25 +134,29 @@ __device__ __forceinline__ void chlld_mixed(int n_state, int n_disc, int flags, 
     const float wv0 = (float)w;
 
     // ---- FP32: wave speeds (Phys_param.cpp:978-1046) -----------------------------------------
+    // b^2/rho + c (c -+ 2 b_n/sqrt(rho)) = (c -+ b_n/sqrt(rho))^2 + (b^2 - b_n^2)/rho is never negative,
+    // but with the field along the normal and the sound speed at the Alfven speed it is a remainder
+    // that FP32 rounding can take below zero: it is clamped at zero before the square root (in FP64,
+    // where the reference takes it as it comes, such a state keeps its positive remainder).
     const float ro = (fr2 + fr1) / 2.0f;
     const float ap = (fp2 + fp1) / 2.0f;
     const float aaL = fbL[0] / sqrtf(fr1);
     const float b21 = b2L / fr1;
     const float cL = sqrtf(gamma * fp1 / fr1);
-    float qp = sqrtf(b21 + cL * (cL + 2.0f * aaL));
-    float qm = sqrtf(b21 + cL * (cL - 2.0f * aaL));
+    float qp = sqrtf(fmaxf(b21 + cL * (cL + 2.0f * aaL), 0.0f));
+    float qm = sqrtf(fmaxf(b21 + cL * (cL - 2.0f * aaL), 0.0f));
     const float cfL = (qp + qm) / 2.0f;
     const float aaR = fbR[0] / sqrtf(fr2);
     const float b22 = b2R / fr2;
     const float cR = sqrtf(gamma * fp2 / fr2);
-    qp = sqrtf(b22 + cR * (cR + 2.0f * aaR));
-    qm = sqrtf(b22 + cR * (cR - 2.0f * aaR));
+    qp = sqrtf(fmaxf(b22 + cR * (cR + 2.0f * aaR), 0.0f));
+    qm = sqrtf(fmaxf(b22 + cR * (cR - 2.0f * aaR), 0.0f));
     const float cfR = (qp + qm) / 2.0f;
     const float aC = (aaL + aaR) / 2.0f;
     const float b2o = (b22 + b21) / 2.0f;
     const float cC = sqrtf(gamma * ap / ro);
-    qp = sqrtf(b2o + cC * (cC + 2.0f * aC));
-    qm = sqrtf(b2o + cC * (cC - 2.0f * aC));
+    qp = sqrtf(fmaxf(b2o + cC * (cC + 2.0f * aC), 0.0f));
+    qm = sqrtf(fmaxf(b2o + cC * (cC - 2.0f * aC), 0.0f));
     const float cfC = (qp + qm) / 2.0f;
 
     // S_L, S_R relative to each side's normal velocity (su = S - v), formed from the fast speeds
