@@ -442,9 +442,9 @@ int titan_find_yzel_sosed_for_bs(titan_ctx *ctx);
  * at BS faces, w = 0) times velocity_K times the face normal, averaged per node with the
  * smoothing terms, move the TS/HP/BS nodes into layer 1-now.  node_velocity [n_nodes*3]
  * (nullable) receives the averaged velocity of every surface node (0 elsewhere).  A node sums its
- * faces in ascending face id (the reference's order varies between runs).  Not built: the HP
- * angle and along-ray smoothing and the B/C/D/E/H/G ray families (Setka_move.cpp:697-1130,
- * 1280-1357), which belong to the reference's own tail mesh. */
+ * faces in ascending face id (the reference's order varies between runs).  With the HP grid of
+ * titan_set_hp_grid and the circle centres of titan_smooth_angle_hp, the HP angle and along-grid
+ * terms and the tail and BS corrections apply as in the fitted step. */
 int titan_culc_velocity_surface(titan_ctx *ctx, int now, double time, double *node_velocity);
 /* Luch::dvigenie(layer) of every ray (Luch.cpp:6-152; Setka_physics.cpp:1620-1624). */
 int titan_dvigenie(titan_ctx *ctx, int layer);
