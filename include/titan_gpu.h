@@ -327,6 +327,41 @@ int titan_cx_exact_batch(int64_t n, const double *in10, int nodes, double *out5)
  * keeps until titan_destroy; calls on one context are serialised on its stream. */
 int titan_mk_go(titan_ctx *ctx, int layer, int32_t n_per_face, uint64_t seed, int32_t max_steps,
                 double *density, int64_t *stats);
+/* Setka::Set_MK_Zone (Setka_MK.cpp:142-683) from the centres of node layer 0: cell_mk [n_cells]
+ * (nullable) = the MK zone 1..7 of each cell (Zone_1 -> 1; Zone_2, Zone_3, Zone_4 -> 2 / 3, 4 / 5,
+ * 6 / 7 by centre x > 0 / x <= 0), face_mask [n_faces] (nullable) bit z-1 = the face is on zone z's
+ * list (MK_Grans[z-1], by the rules of Setka_MK.cpp:356-641 on the face flags TS/HP/BS, boundary
+ * codes, cell zones and centres).  Then the velocity distributions of the listed interior faces,
+ * zeroed: per face, sort 1..4 and side, the AMR_f grid of Gran::Read_AMR -- (0, 20) x (-20, 20)^2 in
+ * the face's frame (Vn, get_bazis(Vn)), 3 x 6 x 6 bins -- refined uniformly r times in each direction
+ * (r = 1..4; the reference refines adaptively, AMR_f::Refine): 108 r^3 FP64 bins per face, sort
+ * and side, 6,912 r^3 bytes per face.  TITAN_E_TOPOLOGY (titan_last_error: the cell or face)
+ * for a cell zone outside 1..4 (Error 86743207564) or a face a zone lists twice (ERROR 6435856408).
+ * Replaces any earlier zones and distributions. */
+int titan_set_mk_zone(titan_ctx *ctx, int32_t r, int8_t *cell_mk, uint8_t *face_mask);
+/* One pass of the zone scheme: MK_prepare(zone) + MK_go(zone) + MK_delete (Setka_MK.cpp:686-1752;
+ * Setka.cpp:221-226 runs them over the zones 6,4,2,1,3,5,7,5,3,1,2,4,6,7,4,5,2,3,1).  The flux into
+ * the zone through each listed face -- an interior face: each sort's distribution on the side
+ * entering the zone (the weight it holds, Culk_SpotokV); a boundary face: Get_Spotok_inf of sort 4 --
+ * gives mu_expect = total / (n_per_face x listed faces) and, sort by sort and face by face (ascending
+ * reference id), the particle counts and weights of Setka_MK.cpp:1380-1390.  A particle from an
+ * interior face draws its velocity from the distribution (AMR_f::Get_random_velosity: a bin with
+ * probability its weight, Vn ~ Vn across it, Vt, Vm uniform); from a boundary face as titan_mk_go.
+ * Flights stay in the zone's cells and end on a listed face, whose distribution on the side the
+ * particle leaves by gains its weight in the bin of its velocity (AMR_f::Add_particle); the zone's
+ * outgoing distributions are cleared first.  density [n_cells*5]: the zone's cells receive MK_n_H,
+ * MK_n_H1..4 per unit volume; other cells are left as they are.  stats [9] (nullable): those of
+ * titan_mk_go, then the flights that ended on the zone's border.  TITAN_E_ARG without
+ * titan_set_mk_zone; TITAN_E_TOPOLOGY when a listed face's entering cell lies outside the zone
+ * (Error 9767653421).  Each pass allocates the running sums of its incoming distributions
+ * (864 r^3 bytes per listed interior face) for the call. */
+int titan_mk_go_zone(titan_ctx *ctx, int layer, int32_t zone, int32_t n_per_face, uint64_t seed, int32_t max_steps,
+                     double *density, int64_t *stats);
+/* The distribution of a listed interior face (reference face id, sort 1..4, side 0: Vn along the
+ * normal, 1: against it) to or from the caller: w [108 r^3] weight per bin, (i, j, k) with k fastest
+ * (the reference saves them as data_AMR/func_grans_AMR_* files between passes). */
+int titan_mk_dist_get(titan_ctx *ctx, int64_t face, int32_t sort, int32_t side, double *w);
+int titan_mk_dist_set(titan_ctx *ctx, int64_t face, int32_t sort, int32_t side, const double *w);
 
 /* ---- surrogate network (8-64-64-64-5 SiLU) ------------------------------------------------- */
 /* Parameter layout (FP32): W1[64x8] b1[64] W2[64x64] b2[64] W3[64x64] b3[64] W4[5x64] b4[5],

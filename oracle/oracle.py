@@ -79,6 +79,10 @@ def lib():
             "or_mk_setup": (_I64, [_P, _P, _D, ctypes.c_int32, _P, _P, _P]),
             "or_mk_go": (_I, [_P, _P, _P, _P, _P, _P, _P]),
             "or_mk_change_velosity_batch": (None, [_I64, ctypes.c_uint64, _D, _P, _P, _D, _I, _P, _P]),
+            "or_set_mk_zone": (_I, [_P, _P, _I, _P, _P]),
+            "or_mk_setup_zone": (_I64, [_P, _P, _D, ctypes.c_int32, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P, _P]),
+            "or_mk_deposit": (None, [_P, ctypes.c_int32, _I, _I, _P, _P, _D]),
+            "or_mk_sample_batch": (None, [_P, ctypes.c_int32, _I, _I, _P, _I64, ctypes.c_uint64, _P]),
             "or_bera_source": (None, [_P, _I, _P, _D, _D, _D, _P]),
             "or_new_connect": (_I64, [_I64, _P, _I64, _P, _P, _P]),
             "or_cx_rate_ref": (_D, [_D, _D, _D, _I, _I]),
@@ -753,11 +757,33 @@ def dvigenie(fit, x, params=None):
 class OrMk(ctypes.Structure):
     _fields_ = [("n_in", _I64), ("face", _P), ("count", _P), ("mu", _P), ("seed", ctypes.c_uint64),
                 ("v_inf", _D), ("a2", _D), ("kn", _D), ("max_steps", ctypes.c_int32), ("mistake", ctypes.c_int32),
-                ("nthreads", ctypes.c_int32), ("R0", _D)]
+                ("nthreads", ctypes.c_int32), ("R0", _D), ("zone", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                ("cell_mk", _P), ("face_mask", _P), ("in_sort", _P), ("in_side", _P), ("dist", _P)]
+
+
+class OrMkDist(ctypes.Structure):
+    _fields_ = [("n_slots", ctypes.c_int32), ("r", ctypes.c_int32), ("slot", _P), ("w", _P)]
+
+
+class MkDist:
+    """The velocity distributions of the zone borders (or_mk_dist): a slot per interior face some
+    zone lists, [slot, sort 1..4, side 0/1, 108 r^3 bins] of deposited weight."""
+
+    def __init__(self, mesh, face_mask, r=1):
+        fm = np.asarray(face_mask)
+        listed = (fm != 0) & (np.asarray(mesh.face_cells)[:, 1] >= 0) & (np.asarray(mesh.face_bc) == 0)
+        self.slot = np.full(mesh.n_faces, -1, np.int32)
+        self.slot[listed] = np.arange(int(listed.sum()), dtype=np.int32)
+        self.r = int(r)
+        self.w = np.zeros((int(listed.sum()), 4, 2, 108 * self.r ** 3))
+        self.c = OrMkDist(self.w.shape[0], self.r, _p(self.slot), _p(self.w))
+
+    def bins(self):
+        return 3 * self.r, 6 * self.r, 6 * self.r
 
 
 MK_STATS = ("particles", "exchanges", "left", "sun_passes", "lost", "births_dropped", "segments",
-            "exchange_capped")
+            "exchange_capped", "zone_border")
 
 
 def inner_radius(mesh, xyz):
@@ -796,7 +822,72 @@ def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000, ge
     stats = np.zeros(8, np.int64)
     lib().or_mk_go(ctypes.byref(om.c), ctypes.byref(mk), _p(x), _p(np.ascontiguousarray(prim, np.float64)),
                    ctypes.byref(g), _p(tally), _p(stats))
-    return tally / geom["cell_volume"][:, None], dict(zip(MK_STATS, (int(v) for v in stats)))
+    return tally / geom["cell_volume"][:, None], dict(zip(MK_STATS[:8], (int(v) for v in stats)))
+
+
+def set_mk_zone(om, geom, mistake=0):
+    """or_set_mk_zone: (rc, cell MK zone [n_cells], face zone mask [n_faces])."""
+    m = om.m
+    cell_mk = np.zeros(m.n_cells, np.int8)
+    mask = np.zeros(m.n_faces, np.uint8)
+    g = OrGeom(*[_p(geom[k]) for k, _ in OrGeom._fields_])
+    rc = lib().or_set_mk_zone(ctypes.byref(om.c), ctypes.byref(g), int(mistake), _p(cell_mk), _p(mask))
+    return rc, cell_mk, mask
+
+
+def mk_setup_zone(om, geom, v_inf, n_per_face, zone, cell_mk, mask, dist):
+    """or_mk_setup_zone: injections (face, sort, side, count, mu), or raises on rc < 0."""
+    m = om.m
+    n = 4 * m.n_faces
+    face, count = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    sort, side, mu = np.zeros(n, np.int8), np.zeros(n, np.int8), np.zeros(n)
+    g = OrGeom(*[_p(geom[k]) for k, _ in OrGeom._fields_])
+    k = lib().or_mk_setup_zone(ctypes.byref(om.c), ctypes.byref(g), float(v_inf), int(n_per_face), int(zone),
+                               _p(np.ascontiguousarray(cell_mk, np.int8)), _p(np.ascontiguousarray(mask, np.uint8)),
+                               ctypes.byref(dist.c), _p(face), _p(sort), _p(side), _p(count), _p(mu))
+    if k < 0:
+        raise ValueError("or_mk_setup_zone: %d" % k)
+    return face[:k].copy(), sort[:k].copy(), side[:k].copy(), count[:k].copy(), mu[:k].copy()
+
+
+def mk_go_zone(om, xyz, prim, v_inf, a2, kn, n_per_face, zone, cell_mk, mask, dist, seed=1, max_steps=100000,
+               geom=None, nthreads=1):
+    """One pass of MK_go(zone) (or_mk_setup_zone + or_mk_go): the densities of the zone's cells per
+    unit volume ([n_cells, 5], zero elsewhere), the stats dict, and the injections; deposits into
+    dist, whose outgoing distributions of the zone's faces are cleared first (MK_go's Fill_null)."""
+    m = om.m
+    x = np.ascontiguousarray(xyz, np.float64)
+    geom = measure(om, x) if geom is None else geom
+    cm = np.ascontiguousarray(cell_mk, np.int8)
+    mk_ = np.ascontiguousarray(mask, np.uint8)
+    face, sort, side, count, mu = mk_setup_zone(om, geom, v_inf, n_per_face, zone, cm, mk_, dist)
+    fc = np.asarray(m.face_cells)
+    for f in np.flatnonzero((mk_ >> (zone - 1)) & 1):
+        if dist.slot[f] >= 0:
+            dist.w[dist.slot[f], :, 0 if cm[fc[f, 0]] == zone else 1, :] = 0.0
+    g = OrGeom(*[_p(geom[k]) for k, _ in OrGeom._fields_])
+    mk = OrMk(len(face), _p(face), _p(count), _p(mu), int(seed), float(v_inf), float(a2), float(kn), int(max_steps),
+              0, int(nthreads), inner_radius(m, x), int(zone), 0, _p(cm), _p(mk_), _p(sort), _p(side),
+              ctypes.cast(ctypes.pointer(dist.c), ctypes.c_void_p))
+    tally = np.zeros((m.n_cells, 5))
+    stats = np.zeros(9, np.int64)
+    lib().or_mk_go(ctypes.byref(om.c), ctypes.byref(mk), _p(x), _p(np.ascontiguousarray(prim, np.float64)),
+                   ctypes.byref(g), _p(tally), _p(stats))
+    dens = tally / geom["cell_volume"][:, None]
+    dens[cm != zone] = 0.0
+    return dens, dict(zip(MK_STATS, (int(v) for v in stats))), (face, sort, side, count, mu)
+
+
+def mk_deposit(dist, slot, sort, side, n, V, mu):
+    lib().or_mk_deposit(ctypes.byref(dist.c), int(slot), int(sort), int(side),
+                        _p(np.ascontiguousarray(n, np.float64)), _p(np.ascontiguousarray(V, np.float64)), float(mu))
+
+
+def mk_sample(dist, slot, sort, side, n, n_draws, seed=1):
+    V = np.zeros((int(n_draws), 3))
+    lib().or_mk_sample_batch(ctypes.byref(dist.c), int(slot), int(sort), int(side),
+                             _p(np.ascontiguousarray(n, np.float64)), int(n_draws), int(seed), _p(V))
+    return V
 
 
 def mk_change_velosity(n, U, V, cp, a2, seed=1, mistake=0):

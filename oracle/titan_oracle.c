@@ -2191,6 +2191,220 @@ static int64_t mk_locate_inner(const or_mesh *m, const double *x, const double *
     return -1;
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* Monte-Carlo zones: Setka::Set_MK_Zone (Setka_MK.cpp:142-683), the face distributions of the
+ * zone borders (AMR_f.cpp:4-50,240-316; AMR_cell.cpp:273-327; Gran.cpp:16-99) and the counts of
+ * MK_prepare / MK_go(zone) (Setka_MK.cpp:686-1100,1225-1400). */
+int or_set_mk_zone(const or_mesh *m, const or_geom *g, int mistake, int8_t *cell_mk, uint8_t *face_mask)
+{
+    for (int64_t c = 0; c < m->n_cells; c++) {
+        const int z = m->cell_zone[c];
+        const double x = g->cell_center[3 * c];
+        if (z == 1) cell_mk[c] = 1;
+        else if (z == 2) cell_mk[c] = x > 0 ? 2 : 3;
+        else if (z == 3) cell_mk[c] = x > 0 ? 4 : 5;
+        else if (z == 4) cell_mk[c] = x > 0 ? 6 : 7;
+        else return -1;
+    }
+    for (int64_t f = 0; f < m->n_faces; f++) {
+        const int bc = m->face_bc[f];
+        const int fl = m->face_flags ? m->face_flags[f] : 0;
+        const int ts = (fl >> 3) & 1, hp = (fl >> 2) & 1, bs = (fl >> 4) & 1;
+        const int us2 = !ts && !hp && !bs; /* Gran::type2 == Us */
+        const int64_t a = m->face_cells[2 * f], b = m->face_cells[2 * f + 1];
+        const int two = b >= 0; /* gr->cells.size() == 2 */
+        const int za = m->cell_zone[a], zb = two ? m->cell_zone[b] : 0;
+        const double X = g->face_center[3 * f];
+        const double xa = g->cell_center[3 * a], xb = two ? g->cell_center[3 * b] : 0.0;
+        const int across = two && xa * xb < 0.0;
+        int n[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (ts) n[0]++; /* zone 1: Gran_TS */
+        if (!(X < -0.000001)) { /* zone 2 */
+            if (hp) n[1]++;
+            if (ts) n[1]++;
+            if (bc == OR_BC_OUTER_SOFT && za == 2) n[1]++;
+            if (two && za == 2 && zb == 2 && across && !(mistake & 1)) n[1]++;
+        }
+        if (!(X > 0.000001)) { /* zone 3 */
+            if (hp) n[2]++;
+            if (ts) n[2]++;
+            if (bc == OR_BC_OUTER_SOFT && za == 2) n[2]++;
+            if (two && us2) {
+                if (za == 2 && zb == 3) n[2]++;
+                if (za == 2 && zb == 2 && across) n[2]++;
+            }
+        }
+        if (hp && X > 0) n[3]++; /* zone 4 */
+        if (bs) n[3]++;
+        if (two && za == 3 && zb == 3 && across) n[3]++;
+        if (hp && X < 0.0) n[4]++; /* zone 5 */
+        if (bc == OR_BC_OUTER_SOFT && za == 3) n[4]++;
+        if (two && us2) {
+            if (za == 2 && zb == 3) n[4]++;
+            if (za == 3 && zb == 3 && across) n[4]++;
+            if (za == 3 && zb == 4 && xa < 0) n[4]++;
+        }
+        if (bs) n[5]++; /* zone 6 */
+        if (bc == OR_BC_OUTER_HARD && X > 0) n[5]++;
+        if (two && za == 4 && zb == 4 && across) n[5]++;
+        if (bc != OR_BC_INTERIOR && X < 0 && za == 4) n[6]++; /* zone 7 */
+        if (two && us2) {
+            if (za == 3 && zb == 4 && xa < 0) n[6]++;
+            if (za == 4 && zb == 4 && across) n[6]++;
+        }
+        uint8_t mask = 0;
+        for (int z = 0; z < 7; z++) {
+            if (n[z] > 1) return -2;
+            if (n[z]) mask |= (uint8_t)(1u << z);
+        }
+        face_mask[f] = mask;
+    }
+    return 0;
+}
+
+static int64_t mk_dist_bins(const or_mk_dist *d) { return 108 * (int64_t)d->r * d->r * d->r; }
+
+static double *mk_dist_at(const or_mk_dist *d, int32_t slot, int sort, int side)
+{
+    return d->w + (((int64_t)slot * 4 + (sort - 1)) * 2 + side) * mk_dist_bins(d);
+}
+
+/* the face frame of a distribution: Vn = the normal (side 0) or its negative (side 1), then
+ * get_bazis (Gran::Read_AMR, AMR_f::Set_bazis) */
+static void mk_dist_frame(const double *n, int side, double *e, double *t, double *mm)
+{
+    for (int i = 0; i < 3; i++) e[i] = side ? -n[i] : n[i];
+    or_get_bazis(e, t, mm);
+}
+
+void or_mk_deposit(or_mk_dist *d, int32_t slot, int sort, int side, const double *n, const double *V, double mu)
+{
+    double e[3], t[3], mm[3];
+    mk_dist_frame(n, side, e, t, mm);
+    const int nx = 3 * d->r, ny = 6 * d->r, nz = 6 * d->r;
+    const double xL = 0.0, xR = 20.0, yL = -20.0, yR = 20.0, zL = -20.0, zR = 20.0;
+    double x = dot3(V, e), y = dot3(V, t), z = dot3(V, mm);
+    if (x < xL) x = xL + 1e-7;
+    if (x > xR) x = xR - 1e-7;
+    if (y < yL) y = yL + 1e-7;
+    if (y > yR) y = yR - 1e-7;
+    if (z < zL) z = zL + 1e-7;
+    if (z > zR) z = zR - 1e-7;
+    const double dx = (xR - xL) / nx, dy = (yR - yL) / ny, dz = (zR - zL) / nz;
+    int i = (x >= xL && x <= xR) ? (int)((x - xL) / dx) : 0;
+    int j = (y >= yL && y <= yR) ? (int)((y - yL) / dy) : 0;
+    int k = (z >= zL && z <= zR) ? (int)((z - zL) / dz) : 0;
+    if (i >= nx) i = nx - 1;
+    if (j >= ny) j = ny - 1;
+    if (k >= nz) k = nz - 1;
+    double *w = mk_dist_at(d, slot, sort, side);
+#pragma omp atomic
+    w[((int64_t)i * ny + j) * nz + k] += mu;
+}
+
+/* AMR_f::Get_random_velosity with AMR_cell::Get_random_velosity_in_cell: a bin with probability
+ * its weight, then Vn with density ~ Vn across the bin, Vt and Vm uniform */
+static void mk_dist_sample(uint64_t *rs, const or_mk_dist *d, const double *w, const double *n, int side, double *V)
+{
+    const int nx = 3 * d->r, ny = 6 * d->r, nz = 6 * d->r;
+    const int64_t nb = mk_dist_bins(d);
+    double total = 0.0;
+    for (int64_t b = 0; b < nb; b++) total += w[b];
+    const double ksi = mk_rand(rs) * total;
+    double SS = 0.0;
+    int64_t pick = -1, last = 0;
+    for (int64_t b = 0; b < nb; b++) {
+        if (w[b] > 0.0) last = b;
+        if (SS + w[b] > ksi) { pick = b; break; }
+        SS += w[b];
+    }
+    if (pick < 0) pick = last; /* rounding at the top of the sum */
+    const int i = (int)(pick / ((int64_t)ny * nz)), j = (int)((pick / nz) % ny), k = (int)(pick % nz);
+    const double dx = 20.0 / nx, dy = 40.0 / ny, dz = 40.0 / nz;
+    const double xl = 0.0 + i * dx, xr = 0.0 + (i + 1) * dx, yl = -20.0 + j * dy, yr = -20.0 + (j + 1) * dy,
+                 zl = -20.0 + k * dz, zr = -20.0 + (k + 1) * dz;
+    const double cx = (xr + xl) / 2.0, cy = (yr + yl) / 2.0, cz = (zr + zl) / 2.0;
+    const double hx = xr - xl, hy = yr - yl, hz = zr - zl;
+    const double L = cx - hx / 2.0, R = cx + hx / 2.0;
+    const double v0 = sqrt(L * L + mk_rand(rs) * (R * R - L * L));
+    const double v1 = (cy - hy / 2.0) + mk_rand(rs) * hy;
+    const double v2 = (cz - hz / 2.0) + mk_rand(rs) * hz;
+    double e[3], t[3], mm[3];
+    mk_dist_frame(n, side, e, t, mm);
+    for (int q = 0; q < 3; q++) V[q] = (v0 * e[q] + v1 * t[q]) + v2 * mm[q];
+}
+
+void or_mk_sample_batch(const or_mk_dist *d, int32_t slot, int sort, int side, const double *n, int64_t n_draws,
+                        uint64_t seed, double *V)
+{
+    const double *w = mk_dist_at(d, slot, sort, side);
+    for (int64_t i = 0; i < n_draws; i++) {
+        uint64_t rs = mk_mix(seed ^ mk_mix((uint64_t)i + 0x9E3779B97F4A7C15ull));
+        mk_dist_sample(&rs, d, w, n, side, V + 3 * i);
+    }
+}
+
+int64_t or_mk_setup_zone(const or_mesh *m, const or_geom *g, double v_inf, int32_t n_per_face, int32_t zone,
+                         const int8_t *cell_mk, const uint8_t *face_mask, const or_mk_dist *dist, int32_t *face,
+                         int8_t *sort, int8_t *side, int32_t *count, double *mu)
+{
+    const uint8_t bit = (uint8_t)(1u << (zone - 1));
+    const int64_t nb = dist ? mk_dist_bins(dist) : 0;
+    double S = 0.0;
+    int64_t nlist = 0;
+    double *ff = (double *)calloc((size_t)m->n_faces, sizeof(double));      /* flux of the face */
+    double *fs = (double *)calloc((size_t)m->n_faces * 4, sizeof(double));  /* per sort */
+    for (int64_t f = 0; f < m->n_faces; f++) {
+        if (!(face_mask[f] & bit)) continue;
+        nlist++;
+        const int64_t a = m->face_cells[2 * f], b = m->face_cells[2 * f + 1];
+        const int ni2 = cell_mk[a] == zone ? 1 : 0; /* the side entering the zone */
+        const int64_t enter = ni2 ? a : b;
+        if (enter < 0 || cell_mk[enter] != zone) { free(ff); free(fs); return -3; }
+        if (m->face_bc[f] == OR_BC_INTERIOR) {
+            const int32_t sl = dist ? dist->slot[f] : -1;
+            for (int s = 1; s <= 4 && sl >= 0; s++) {
+                const double *w = mk_dist_at(dist, sl, s, ni2);
+                double v = 0.0;
+                for (int64_t q = 0; q < nb; q++) v += w[q];
+                fs[4 * f + s - 1] = v;
+                ff[f] += v;
+            }
+        } else {
+            /* the boundary Maxwellian of sort 4 (Get_Spotok_inf of the outward normal) */
+            const double v = mk_spotok_inf(v_inf, g->face_normal + 3 * f) * g->face_area[f];
+            fs[4 * f + 3] = v;
+            ff[f] = v;
+        }
+        S += ff[f];
+    }
+    int64_t k = 0;
+    if (nlist > 0 && S > 0.0) { /* no inflow at all (no pass has fed the zone yet): no particles */
+        const double mu_expect = S / (1.0 * n_per_face * nlist);
+        for (int s = 1; s <= 4; s++) {
+            for (int64_t f = 0; f < m->n_faces; f++) {
+                if (!(face_mask[f] & bit)) continue;
+                if (ff[f] < 0.0000001 * S / nlist) continue;
+                const double sv = fs[4 * f + s - 1];
+                if (sv < 0.0000001 * ff[f]) continue;
+                int32_t n = (int32_t)(sv / mu_expect) + 1;
+                const int32_t lo = n_per_face < 100 ? n_per_face : 100;
+                if (n < lo) n = lo;
+                face[k] = (int32_t)f;
+                sort[k] = (int8_t)s;
+                side[k] = m->face_bc[f] == OR_BC_INTERIOR ? (int8_t)(cell_mk[m->face_cells[2 * f]] == zone ? 1 : 0)
+                                                          : (int8_t)-1;
+                count[k] = n;
+                mu[k] = sv / n;
+                k++;
+            }
+        }
+    }
+    free(ff);
+    free(fs);
+    return k;
+}
+
 int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *prim, const or_geom *g,
              double *tally, int64_t *stats)
 {
@@ -2202,13 +2416,13 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
     for (int64_t q = 0; q < mk->n_in; q++) first[q + 1] = first[q] + mk->count[q];
     const int64_t n_part = first[mk->n_in];
     int nt = mk->nthreads > 0 ? mk->nthreads : 1;
-    int64_t s0_ = 0, s1_ = 0, s2_ = 0, s3_ = 0, s4_ = 0, s5_ = 0, s6_ = 0, s7_ = 0;
+    int64_t s0_ = 0, s1_ = 0, s2_ = 0, s3_ = 0, s4_ = 0, s5_ = 0, s6_ = 0, s7_ = 0, s8_ = 0;
     /* one particle per iteration; its history depends only on its id (its random stream); with
      * several threads the tallies are added atomically, so their rounding varies with the order */
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nt) \
-    reduction(+ : s0_, s1_, s2_, s3_, s4_, s5_, s6_, s7_)
+    reduction(+ : s0_, s1_, s2_, s3_, s4_, s5_, s6_, s7_, s8_)
     for (int64_t pid = 0; pid < n_part; pid++) {
-        int64_t st_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int64_t st_[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         int64_t *stats = st_;
         int64_t lo = 0, hi = mk->n_in - 1;
         while (lo < hi) {
@@ -2242,14 +2456,19 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
             double P[3], V[3];
             for (int i = 0; i < 3; i++)
                 P[i] = (1.0 - sqrt(r1)) * A[i] + (sqrt(r1) * (1.0 - r2)) * T1[i] + (r2 * sqrt(r1)) * Cc[i];
-            mk_velosity_initial(&rs, mk->v_inf, n, V, &capped, mk->mistake);
+            const int in_side = mk->in_side ? mk->in_side[q] : -1;
+            if (in_side >= 0) /* from the face's distribution (AMR_f::Get_random_velosity) */
+                mk_dist_sample(&rs, mk->dist, mk_dist_at(mk->dist, mk->dist->slot[f], mk->in_sort[q], in_side), n,
+                               in_side, V);
+            else
+                mk_velosity_initial(&rs, mk->v_inf, n, V, &capped, mk->mistake);
             stats[0]++;
             if (capped) { stats[5]++; break; } /* no admissible velocity within the cap: dropped */
-            int64_t c = m->face_cells[2 * f];
+            int64_t c = in_side == 0 ? m->face_cells[2 * f + 1] : m->face_cells[2 * f];
             const double *cc = g->cell_center + 3 * c;
             for (int i = 0; i < 3; i++) P[i] = P[i] + (cc[i] - P[i]) / 1000.0; /* into the cell */
             double KSI = -log(1.0 - mk_rand(&rs)), I_do = 0.0, mu = mk->mu[q];
-            int sort = 4;
+            int sort = mk->in_sort ? mk->in_sort[q] : 4;
             int done = 0;
             for (int it = 0; it < mk->max_steps && !done; it++) {
                 double time = 0.0;
@@ -2311,6 +2530,17 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
                     } else if (nc < 0) { /* an inner face without the sphere: nowhere to go */
                         stats[4]++;
                         done = 1;
+                    } else if (mk->zone > 0 && (mk->face_mask[ex] >> (mk->zone - 1)) & 1) {
+                        /* the zone's border (Setka_MK.cpp:2165-2184): into the face's outgoing
+                           distribution, on the side the particle leaves by */
+                        const int nn = mk->cell_mk[m->face_cells[2 * ex]] == mk->zone ? 0 : 1;
+                        if (sort >= 1 && sort <= 4 && mk->dist->slot[ex] >= 0)
+                            or_mk_deposit(mk->dist, mk->dist->slot[ex], sort, nn, g->face_normal + 3 * ex, V, mu);
+                        stats[8]++;
+                        done = 1;
+                    } else if (mk->zone > 0 && mk->cell_mk[nc] != mk->zone) {
+                        stats[4]++; /* into another zone through a face no list holds: lost */
+                        done = 1;
                     } else {
                         c = nc;
                     }
@@ -2345,9 +2575,11 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
         } while (0);
         s0_ += st_[0]; s1_ += st_[1]; s2_ += st_[2]; s3_ += st_[3]; s4_ += st_[4]; s5_ += st_[5]; s6_ += st_[6];
         s7_ += st_[7];
+        s8_ += st_[8];
     }
     stats[0] = s0_; stats[1] = s1_; stats[2] = s2_; stats[3] = s3_; stats[4] = s4_; stats[5] = s5_;
     stats[6] = s6_; stats[7] = s7_;
+    if (mk->zone > 0) stats[8] = s8_;
     free(first);
     return 0;
 }

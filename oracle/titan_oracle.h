@@ -32,6 +32,10 @@
  *   or_mk_setup, or_mk_go  Setka::MK_go / MK_fly_immit  TITAN/source/Setka_MK.cpp:1225-2850
  *                       (Gran::Get_Random_pozition, Luch_crossing, rayTriangleIntersect,
  *                       Gran.cpp:220-285,451-561; Cell::MK_Add_particle, Cell.cpp:1042-1083)
+ *   or_set_mk_zone      Setka::Set_MK_Zone           TITAN/source/Setka_MK.cpp:142-683
+ *   or_mk_setup_zone, or_mk_go (zone > 0)  MK_prepare + MK_go(zone_MK)  Setka_MK.cpp:686-1752
+ *   or_mk_deposit, or_mk_sample_batch  AMR_f::Add_particle, Get_random_velosity  AMR_f.cpp:4-316,
+ *                       AMR_cell.cpp:273-327, Gran::Read_AMR (Gran.cpp:16-99)
  *
  * What pins each function (tests/test_oracle_*.py), so that a dropped term, a wrong sign or
  * index or a transposed operand fails a check that is not a retyping of the formula:
@@ -94,6 +98,14 @@
  *                       planted mistakes (or_mk.mistake) fail these checks.  Per particle the flight
  *                       is UNPINNED against reference output: its random numbers come from files
  *                       it does not ship.
+ *   or_set_mk_zone      every interior face between two MK zones is on both zones' lists and
+ *                       every listed face touches its zone (a dropped rule, the mistake bit, fails).
+ *   or_mk_deposit, or_mk_sample_batch  draws land in each bin in proportion to its deposited
+ *                       weight (chi-square); within a bin Vn has density ~ Vn (closed-form mean)
+ *                       and Vt is uniform.
+ *   or_mk_setup_zone, or_mk_go (zone)  a zone without outer boundary deposits exactly the weight
+ *                       it injects; the 19-pass zone sequence of Setka.cpp:200-226 reproduces the
+ *                       one-zone flight's mass per zone (12% free flight, 25% with exchange, r = 4).
  */
 #ifndef TITAN_ORACLE_H
 #define TITAN_ORACLE_H
@@ -339,7 +351,52 @@ typedef struct {
     int32_t nthreads; /* OpenMP threads (<= 1: one) */
     double R0;        /* radius of the inner (Sun's) sphere the particles pass through (Geo_param::R0;
                          0: no sphere) */
+    /* Monte-Carlo zones (Setka::MK_go(zone_MK)): zone 0 is one zone holding the whole mesh; zone
+     * 1..7 flies inside the cells of that MK zone, injects at the faces its list holds and ends a
+     * flight on them, depositing into the face's outgoing distribution (interior faces) */
+    int32_t zone, pad_;
+    const int8_t *cell_mk;     /* [n_cells] MK zone of each cell (or_set_mk_zone); zone > 0 */
+    const uint8_t *face_mask;  /* [n_faces] bit z-1: the face is on zone z's list; zone > 0 */
+    const int8_t *in_sort;     /* [n_in] sort of each injection (1..4); NULL: 4 */
+    const int8_t *in_side;     /* [n_in] -1: the boundary Maxwellian (Velosity_initial); 0 / 1: the
+                                  face's distribution on that side; NULL: -1 */
+    struct or_mk_dist *dist;   /* zone > 0: the distributions read and deposited into */
 } or_mk;
+/* Velocity distributions on the zone borders (AMR_f at its unrefined grid of Gran::Read_AMR,
+ * Gran.cpp:16-99): (0, 20) x (-20, 20)^2 in the face's frame (Vn, get_bazis(Vn)), 3r x 6r x 6r
+ * bins, per listed interior face (slot), sort 1..4 and side (0: Vn = the face normal, what leaves
+ * cells[0]; 1: Vn = -normal).  w holds the weight deposited per bin (AMR_f::Add_particle,
+ * AMR_f.cpp:240-316); the reference's normalised f (Normir_velocity_volume) times the bin's volume,
+ * its centre speed and the face area -- what Culk_SpotokV and Get_random_velosity read -- is the
+ * same number. */
+typedef struct or_mk_dist {
+    int32_t n_slots, r;
+    const int32_t *slot;  /* [n_faces] slot of each listed interior face, -1 otherwise */
+    double *w;            /* [n_slots][4][2][108 r^3]: bins (i, j, k) of the 3r x 6r x 6r grid, k fastest */
+} or_mk_dist;
+/* Setka::Set_MK_Zone (Setka_MK.cpp:142-683) on the centres of g: cell_mk [n_cells] = 1 for Zone_1
+ * cells, 2 / 3, 4 / 5, 6 / 7 for Zone_2, Zone_3, Zone_4 cells with centre x > 0 / x <= 0; face_mask
+ * [n_faces] bit z-1 when the rules of zone z (Setka_MK.cpp:356-641) list the face.  mistake bit 0
+ * (tests only): zone 2 drops its rule for faces between Zone_2 cells across x = 0.  Returns 0; -1 a
+ * cell zone outside 1..4 (Error 86743207564); -2 a zone whose rules list a face twice
+ * (ERROR 6435856408). */
+int or_set_mk_zone(const or_mesh *m, const or_geom *g, int mistake, int8_t *cell_mk, uint8_t *face_mask);
+/* MK_prepare + the counts of MK_go(zone) (Setka_MK.cpp:686-1100,1225-1400): the flux into the zone
+ * through every listed face (interior faces: the weight of each sort's distribution on the side
+ * entering the zone; boundary faces: Get_Spotok_inf for sort 4), mu_expect = total / (n_per_face x
+ * listed faces), then per sort 1..4 and listed face (ascending id) the injections: face, sort,
+ * side, count and weight, sized 4 n_faces by the caller.  Returns their number, or -3 when a listed
+ * face's entering cell is not in the zone (Error 9767653421). */
+int64_t or_mk_setup_zone(const or_mesh *m, const or_geom *g, double v_inf, int32_t n_per_face, int32_t zone,
+                         const int8_t *cell_mk, const uint8_t *face_mask, const or_mk_dist *dist, int32_t *face,
+                         int8_t *sort, int8_t *side, int32_t *count, double *mu);
+/* AMR_f::Add_particle into distribution (slot, sort, side) for a face of normal n (test access):
+ * the bin of V in the frame (Vn, t, m), coordinates clipped 1e-7 inside the box. */
+void or_mk_deposit(or_mk_dist *d, int32_t slot, int sort, int side, const double *n, const double *V, double mu);
+/* n draws of AMR_f::Get_random_velosity from distribution (slot, sort, side) of a face of normal n
+ * (test access), one SplitMix64 stream per draw from seed: V [n*3]. */
+void or_mk_sample_batch(const or_mk_dist *d, int32_t slot, int sort, int side, const double *n, int64_t n_draws,
+                        uint64_t seed, double *V);
 /* Fluxes of the LISM Maxwellian into the domain through the outer boundary faces (bc 2, 3):
  * flux[f] = Get_Spotok_inf(n) * area (Setka_MK.cpp:2316-2324); count and mu by the rule of
  * Setka_MK.cpp:1233,1380-1390 with N_per_gran = n_per_face.  Returns the number of faces used;
@@ -350,6 +407,7 @@ int64_t or_mk_setup(const or_mesh *m, const or_geom *g, double v_inf, int32_t n_
  * SplitMix64 stream per draw from seed; capped = draws that hit the 10000-try cap. */
 void or_mk_change_velosity_batch(int64_t n, uint64_t seed, double a2, const double *U, const double *V, double cp,
                                  int mistake, double *W, int64_t *capped);
+/* With mk->zone > 0, stats has a ninth entry: flights that ended on the zone's border. */
 int or_mk_go(const or_mesh *m, const or_mk *mk, const double *coords, const double *prim, const or_geom *g,
              double *tally, int64_t *stats);
 

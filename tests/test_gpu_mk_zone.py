@@ -1,0 +1,110 @@
+"""The Monte-Carlo zones on the device (titan_set_mk_zone, titan_mk_go_zone, titan_mk_dist_get/set:
+Setka::Set_MK_Zone, MK_prepare / MK_go(zone_MK), Setka_MK.cpp:142-1752) against the oracle's
+or_set_mk_zone / or_mk_setup_zone / or_mk_go, on a heliosphere shell whose bow shock is fitted only
+upwind.  The zones and lists are integer work and match exactly; so do the particle counts once both
+sides hold the same incoming distributions; the flights share their random streams per particle, and
+densities and deposits are compared within MC noise."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+A2 = 0.102046
+SEQUENCE = (6, 4, 2, 1, 3, 5, 7, 5, 3, 1, 2, 4, 6, 7, 4, 5, 2, 3, 1)
+
+
+@pytest.fixture(scope="module")
+def shell(C, O):
+    mesh, prim, params, _ = C.shell_fitted(n_edge=6, bs_upwind=True)
+    om = O.OracleMesh(mesh)
+    g = O.measure(om, mesh.node_xyz)
+    return mesh, prim, dict(params, par_a_2=A2, par_Kn=40.0906), om, g
+
+
+def _mass(d, cm, vol, z):
+    return (d[cm == z, 0] * vol[cm == z]).sum()
+
+
+def test_set_mk_zone_matches_the_oracle(T, shell, O):
+    mesh, prim, params, om, g = shell
+    rc, ocm, omask = O.set_mk_zone(om, g)
+    assert rc == 0
+    with T.Setka(mesh, params) as s:
+        cm, mask = s.Set_MK_Zone(r=1)
+    np.testing.assert_array_equal(cm, ocm)
+    np.testing.assert_array_equal(mask, omask)
+
+
+def test_zone_passes_match_the_oracle(T, shell, O):
+    mesh, prim, params, om, g = shell
+    kn = params["par_Kn"]
+    rc, cm, mask = O.set_mk_zone(om, g)
+    vol = g["cell_volume"]
+    od = O.MkDist(mesh, mask, r=2)
+    with T.Setka(mesh, params) as s:
+        s.set_state(prim, 0)
+        s.Set_MK_Zone(r=2)
+        # zone 6: the boundary Maxwellian only; counts exact, densities and deposits within noise
+        d6, st6 = s.MK_go_zone(6, 200, seed=3, layer=0)
+        o6, ost6, _ = O.mk_go_zone(om, mesh.node_xyz, prim, params["v_inf"], A2, kn, 200, 6, cm, mask, od, seed=3,
+                                   geom=g, nthreads=4)
+        assert st6["particles"] == ost6["particles"] and st6["lost"] == 0
+        assert abs(st6["zone_border"] - ost6["zone_border"]) < 0.02 * ost6["zone_border"]
+        assert abs(_mass(d6, cm, vol, 6) / _mass(o6, cm, vol, 6) - 1.0) < 0.01
+        assert (d6[cm != 6] == 0).all()
+        listed4 = np.flatnonzero((mask >> 3) & 1)
+        side_in4 = np.where(cm[np.asarray(mesh.face_cells)[listed4, 0]] == 4, 1, 0)
+        gw = sum(s.mk_dist(f, 4, sd).sum() for f, sd in zip(listed4, side_in4) if od.slot[f] >= 0)
+        ow = sum(od.w[od.slot[f], 3, sd].sum() for f, sd in zip(listed4, side_in4) if od.slot[f] >= 0)
+        assert gw > 0 and abs(gw / ow - 1.0) < 0.02
+        # the same incoming distributions on both sides: zone 4's counts then match exactly
+        for f, sd in zip(listed4, side_in4):
+            if od.slot[f] >= 0:
+                for srt in range(1, 5):
+                    s.mk_dist(f, srt, sd, od.w[od.slot[f], srt - 1, sd])
+        d4, st4 = s.MK_go_zone(4, 100, seed=5, layer=0)
+        o4, ost4, inj = O.mk_go_zone(om, mesh.node_xyz, prim, params["v_inf"], A2, kn, 100, 4, cm, mask, od, seed=5,
+                                     geom=g, nthreads=4)
+        assert st4["particles"] == ost4["particles"] == inj[3].sum()
+        assert st4["left"] == 0 and st4["lost"] == 0 and st4["zone_border"] == st4["particles"]
+        assert abs(st4["exchanges"] - ost4["exchanges"]) < 0.03 * ost4["exchanges"] + 10
+        assert abs(_mass(d4, cm, vol, 4) / _mass(o4, cm, vol, 4) - 1.0) < 0.02
+        # zone 4 has no outer boundary: what it injects it deposits on its border
+        out4 = np.where(cm[np.asarray(mesh.face_cells)[listed4, 0]] == 4, 0, 1)
+        dep = sum(s.mk_dist(f, srt, sd).sum() for f, sd in zip(listed4, out4) if od.slot[f] >= 0
+                  for srt in range(1, 5))
+        injected = (inj[3] * inj[4]).sum()
+        assert abs(dep / injected - 1.0) < 1e-9
+
+
+def test_zone_sequence_converges_to_the_one_zone_flight(T, shell, O):
+    mesh, prim, params, om, g = shell
+    vol = g["cell_volume"]
+    p = dict(params, par_Kn=1e30)  # free flight
+    with T.Setka(mesh, p) as s:
+        s.set_state(prim, 0)
+        whole, _ = s.MK_go(1000, seed=1, layer=0)
+        cm, _ = s.Set_MK_Zone(r=4)
+        dz = np.zeros((mesh.n_cells, 5))
+        for i, z in enumerate(SEQUENCE):
+            _, st = s.MK_go_zone(z, 40, seed=i + 1, layer=0, density=dz)
+            assert st["lost"] == 0
+    ratio = np.array([_mass(dz, cm, vol, z) / _mass(whole, cm, vol, z) for z in range(1, 8)])
+    assert (np.abs(ratio - 1.0) < 0.12).all(), ratio
+
+
+def test_zone_arguments(T, shell):
+    mesh, prim, params, om, g = shell
+    with T.Setka(mesh, params) as s:
+        s.set_state(prim, 0)
+        with pytest.raises(T.TitanError):
+            s.MK_go_zone(1, 10, layer=0)  # no zones yet
+        with pytest.raises(T.TitanError):
+            s.Set_MK_Zone(r=5)
+        s.Set_MK_Zone(r=1)
+        with pytest.raises(T.TitanError):
+            s.MK_go_zone(8, 10, layer=0)
+        # zone 1 before any pass fed it: no inflow, no particles, its cells zero
+        d, st = s.MK_go_zone(1, 10, layer=0, density=np.ones((mesh.n_cells, 5)))
+        cm = np.asarray(s.Set_MK_Zone(r=1)[0])
+        assert st["particles"] == 0 and (d[cm == 1] == 0).all() and (d[cm != 1] == 1).all()

@@ -38,6 +38,10 @@ _ERRNAMES = {
 
 # titan_mk_go's counters
 MK_STATS = ("particles", "exchanges", "left", "sun_passes", "lost", "births_dropped", "segments", "exchange_capped")
+# titan_mk_go_zone's: the same, then the flights that ended on the zone's border
+MK_ZONE_STATS = MK_STATS + ("zone_border",)
+# the order in which Setka::Algoritm(2) runs the Monte-Carlo zones (Setka.cpp:200-226)
+MK_ZONE_SEQUENCE = (6, 4, 2, 1, 3, 5, 7, 5, 3, 1, 2, 4, 6, 7, 4, 5, 2, 3, 1)
 
 
 class TitanError(RuntimeError):
@@ -150,6 +154,10 @@ _SIGS = {
     "titan_sources": (_I, [_P, _I, _I, _P]),
     "titan_sources_h": (_I, [_P, _I, _P, _P, _P]),
     "titan_mk_go": (_I, [_P, _I, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, _P, _P]),
+    "titan_set_mk_zone": (_I, [_P, ctypes.c_int32, _P, _P]),
+    "titan_mk_go_zone": (_I, [_P, _I, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, _P, _P]),
+    "titan_mk_dist_get": (_I, [_P, _I64, ctypes.c_int32, ctypes.c_int32, _P]),
+    "titan_mk_dist_set": (_I, [_P, _I64, ctypes.c_int32, ctypes.c_int32, _P]),
     "titan_cx_exact_batch": (_I, [_I64, _P, _I, _P]),
     "titan_mlp_n_params": (_I64, []),
     "titan_mlp_set": (_I, [_P, _P, _P, _P, _P]),
@@ -605,6 +613,39 @@ class Setka:
         _check(lib().titan_mk_go(self.ctx, int(layer), int(n_per_face), int(seed) & (2**64 - 1), int(max_steps),
                                  _ptr(out), _ptr(st)), "titan_mk_go", self.ctx)
         return out, dict(zip(MK_STATS, (int(v) for v in st)))
+
+    def Set_MK_Zone(self, r=1):
+        """Setka::Set_MK_Zone (Setka_MK.cpp:142-683) and the zone borders' distributions with r-fold
+        refined bins: returns (MK zone per cell [n_cells], zone-list mask per face [n_faces])."""
+        cm = np.zeros(self.n_cells, np.int8)
+        fm = np.zeros(self.n_faces, np.uint8)
+        _check(lib().titan_set_mk_zone(self.ctx, int(r), _ptr(cm), _ptr(fm)), "titan_set_mk_zone", self.ctx)
+        self._mk_r = int(r)
+        return cm, fm
+
+    def MK_go_zone(self, zone, n_per_face, seed=1, max_steps=100000, layer=None, density=None):
+        """MK_prepare(zone) + MK_go(zone): the zone's cells of `density` ([n_cells, 5], created as
+        zeros when None) get their densities; returns (density, stats dict)."""
+        layer = self.current_layer() if layer is None else layer
+        out = np.zeros((self.n_cells, 5)) if density is None else density
+        assert out.shape == (self.n_cells, 5) and out.dtype == np.float64 and out.flags.c_contiguous
+        st = np.zeros(9, np.int64)
+        _check(lib().titan_mk_go_zone(self.ctx, int(layer), int(zone), int(n_per_face), int(seed) & (2**64 - 1),
+                                      int(max_steps), _ptr(out), _ptr(st)), "titan_mk_go_zone", self.ctx)
+        return out, dict(zip(MK_ZONE_STATS, (int(v) for v in st)))
+
+    def mk_dist(self, face, sort, side, w=None):
+        """A listed interior face's distribution (108 r^3 bins): read it, or with w write it."""
+        n = 108 * self._mk_r ** 3
+        if w is None:
+            out = np.zeros(n)
+            _check(lib().titan_mk_dist_get(self.ctx, int(face), int(sort), int(side), _ptr(out)), "titan_mk_dist_get",
+                   self.ctx)
+            return out
+        a = np.ascontiguousarray(w, np.float64).reshape(n)
+        _check(lib().titan_mk_dist_set(self.ctx, int(face), int(sort), int(side), _ptr(a)), "titan_mk_dist_set",
+               self.ctx)
+        return a
 
     def set_timing(self, on=True):
         _check(lib().titan_set_timing(self.ctx, 1 if on else 0), "titan_set_timing", self.ctx)

@@ -319,7 +319,8 @@ def a_luch_radii(R0, R1, R2, R3, R4, R5, M0, M1, M11, M2, M3, M4, da1, da2, da3,
 
 
 def shell_fitted(n_edge=6, M=(3, 6, 2, 8, 5, 5), R0_au=1.0, R1_au=10.0, R5_au=600.0, surf_au=(80.0, 120.0, 240.0),
-                 asym=(0.2, 0.25, 0.3), da=(1.0, 1.0, 1.0, 1.0, 0.5), zone1=True, noise=0.0, seed_state=4, tvd=False):
+                 asym=(0.2, 0.25, 0.3), da=(1.0, 1.0, 1.0, 1.0, 0.5), zone1=True, noise=0.0, seed_state=4, tvd=False,
+                 bs_upwind=False):
     """Heliosphere shell whose nodes lie on A_Luch rays through the cube-sphere directions, with
     the termination shock, heliopause and bow shock as fitted node surfaces: the TS at
     surf_au[0] * (1 - asym[0] cos the) along a ray at polar angle `the` from the +x axis (closest
@@ -327,8 +328,11 @@ def shell_fitted(n_edge=6, M=(3, 6, 2, 8, 5, 5), R0_au=1.0, R1_au=10.0, R5_au=60
     Luch::dvigenie with counts M = (M0, M1, M11, M2, M3, M4) and shapes da = (da1 .. da5).  Zones:
     1 inside the TS (2 when zone1 is False), 2 to the HP, 3 to the BS, 4 outside.  TS, HP (with
     the contact flag of contact_hard, Phys_param.cpp:523) and BS faces use HLLD (Gran::Get_method);
-    other faces HLLC where they touch Zone_1, HLLD elsewhere.  Returns (mesh, prim, params, rays)
-    with move_mesh = 2 and the surface-fitting defaults of Phys_param.cpp:500-520."""
+    other faces HLLC where they touch Zone_1, HLLD elsewhere.  bs_upwind: the bow shock is a fitted
+    surface only where its faces' centres have x > 0, as on the reference's own mesh, whose tail has
+    no bow shock (the Zone_3 / Zone_4 faces there are plain faces, Setka_MK.cpp:560-641).  Returns
+    (mesh, prim, params, rays) with move_mesh = 2 and the surface-fitting defaults of
+    Phys_param.cpp:500-520."""
     M0, M1, M11, M2, M3, M4 = M
     npr = M0 + M1 + M2 + M3 + M4 + 5
     nl = npr - 1
@@ -379,6 +383,8 @@ def shell_fitted(n_edge=6, M=(3, 6, 2, 8, 5, 5), R0_au=1.0, R1_au=10.0, R5_au=60
     mesh, ccen = _finish(xyz, cn, bc_fn, method_fn, zone, bstate_fn=bstate_fn)
     mesh.face_bscalar = _hard_face_scalars(mesh)
     sk = surf_kind(mesh.face_nodes)
+    if bs_upwind:
+        sk[(sk == 3) & (xyz[mesh.face_nodes].mean(axis=1)[:, 0] <= 0.0)] = 0
     flags = np.zeros(mesh.n_faces, np.uint8)
     flags[sk == 1] = 1 << 3
     flags[sk == 2] = (1 << 2) | (1 << 1)

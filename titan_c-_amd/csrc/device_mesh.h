@@ -155,8 +155,9 @@ struct MkDev {
     const MkCell *pack;         // [Nc] flight records of the cells (k_mk_pack)
     double *tally;              // [Nc*5] mu * time of sorts outside 1..4, then of sorts 1..4
     double *out;                // [Nc*5] the tallies per unit volume, reference cell order
-    unsigned long long *stats;  // [8] particles, charge exchanges, left, passes through the Sun's
-                                // sphere, lost, births dropped, segments, flights with a capped exchange
+    unsigned long long *stats;  // [9] particles, charge exchanges, left, passes through the Sun's
+                                // sphere, lost, births dropped, segments, flights with a capped
+                                // exchange, flights ended on the zone's border
     // the Sun's sphere (Setka_MK.cpp:2138-2160): its radius R0 (0: none) and the inner boundary
     // faces -- device face ids ascending, and per face the inner face across each of its 4 edges
     // (edge k joins its nodes k and k+1 mod 4)
@@ -164,6 +165,20 @@ struct MkDev {
     int32_t n_inner, pad2;
     const int32_t *inner_face;  // [n_inner]
     const int32_t *inner_nb;    // [n_inner*4]
+    // Monte-Carlo zones (Setka::MK_go(zone_MK), Setka_MK.cpp:1225-2316): zone 0 flies over the
+    // whole mesh; zone 1..7 flies inside that zone's cells, starts particles on the faces its list
+    // holds and ends them there.  An injection with side >= 0 draws its velocity from the face's
+    // distribution on that side (in_cdf: its bins' running sums); a flight that ends on an interior
+    // listed face deposits its weight into the face's distribution on the side it leaves by.
+    int32_t zone, r;            // zone; bins 3r x 6r x 6r per distribution
+    const int8_t *cell_mk;      // [Nc] MK zone of each cell
+    const uint8_t *face_mask;   // [Nf] bit z-1: on zone z's list
+    const int32_t *dslot;       // [Nf] distribution slot of a listed interior face, -1 otherwise
+    double *dist;               // [n_slots][4][2][108 r^3] weight per bin
+    const int8_t *in_sort;      // [n_in] 1..4
+    const int8_t *in_side;      // [n_in] -1: the boundary Maxwellian; 0 / 1: the distribution's side
+    const int64_t *in_cdf;      // [n_in] offset of the injection's running sums in cdf, -1: none
+    const double *cdf;          // running sums of the incoming distributions' bins
 };
 
 // Failures of a step, ranked in the order the reference meets them -- the surface fit before the

@@ -293,6 +293,95 @@ __device__ __forceinline__ int mk_inner_index(const MkDev &K, int64_t f) {
     return (K.n_inner > 0 && K.inner_face[lo] == f) ? lo : 0;
 }
 
+// ---- zone borders: the velocity distributions of AMR_f (AMR_f.cpp:4-50,240-316) at the unrefined
+// grid of Gran::Read_AMR (Gran.cpp:66-99) refined r times in each direction: (0, 20) x (-20, 20)^2 in
+// the face's frame (Vn, get_bazis(Vn)), Vn = the normal (side 0, what leaves cells[0]) or its
+// negative (side 1), bins (i, j, k) with k fastest, each holding the weight deposited in it.
+__device__ __forceinline__ void mk_dist_frame(const double n[3], int side, double e[3], double t[3], double m[3]) {
+    for (int i = 0; i < 3; i++) e[i] = side ? -n[i] : n[i];
+    get_bazis<double>(e, t, m);
+}
+
+// AMR_f::Add_particle: the bin of V in the face's frame, coordinates clipped 1e-7 inside the box
+__device__ __forceinline__ void mk_deposit(double *w, int r, const double n[3], int side, const double V[3], double mu) {
+    double e[3], t[3], m[3];
+    mk_dist_frame(n, side, e, t, m);
+    const int nx = 3 * r, ny = 6 * r, nz = 6 * r;
+    const double xL = 0.0, xR = 20.0, yL = -20.0, yR = 20.0, zL = -20.0, zR = 20.0;
+    double x = (V[0] * e[0] + V[1] * e[1]) + V[2] * e[2];
+    double y = (V[0] * t[0] + V[1] * t[1]) + V[2] * t[2];
+    double z = (V[0] * m[0] + V[1] * m[1]) + V[2] * m[2];
+    if (x < xL) x = xL + 1e-7;
+    if (x > xR) x = xR - 1e-7;
+    if (y < yL) y = yL + 1e-7;
+    if (y > yR) y = yR - 1e-7;
+    if (z < zL) z = zL + 1e-7;
+    if (z > zR) z = zR - 1e-7;
+    const double dx = (xR - xL) / nx, dy = (yR - yL) / ny, dz = (zR - zL) / nz;
+    // a coordinate that is not a number (none is expected) goes to bin 0 rather than out of the grid
+    int i = (x >= xL && x <= xR) ? (int)((x - xL) / dx) : 0;
+    int j = (y >= yL && y <= yR) ? (int)((y - yL) / dy) : 0;
+    int k = (z >= zL && z <= zR) ? (int)((z - zL) / dz) : 0;
+    if (i >= nx) i = nx - 1;
+    if (j >= ny) j = ny - 1;
+    if (k >= nz) k = nz - 1;
+    atomicAdd(&w[((int64_t)i * ny + j) * nz + k], mu);
+}
+
+// AMR_f::Get_random_velosity with AMR_cell::Get_random_velosity_in_cell: the first bin whose running
+// sum passes ksi = xi * (the total), then Vn with density ~ Vn across it, Vt and Vm uniform
+__device__ __forceinline__ void mk_dist_sample(uint64_t &rs, int r, const double *__restrict__ cdf, const double n[3],
+                                               int side, double V[3]) {
+    const int nx = 3 * r, ny = 6 * r, nz = 6 * r;
+    const int64_t nb = (int64_t)nx * ny * nz;
+    const double ksi = mk_rand(rs) * cdf[nb - 1];
+    int64_t lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (cdf[mid] > ksi) hi = mid;
+        else lo = mid + 1;
+    }
+    if (!(cdf[lo] > ksi))  // rounding at the top of the sum: the last bin with weight
+        while (lo > 0 && cdf[lo] == cdf[lo - 1]) lo--;
+    const int i = (int)(lo / ((int64_t)ny * nz)), j = (int)((lo / nz) % ny), k = (int)(lo % nz);
+    const double dx = 20.0 / nx, dy = 40.0 / ny, dz = 40.0 / nz;
+    const double xl = 0.0 + i * dx, xr = 0.0 + (i + 1) * dx, yl = -20.0 + j * dy, yr = -20.0 + (j + 1) * dy,
+                 zl = -20.0 + k * dz, zr = -20.0 + (k + 1) * dz;
+    const double cx = (xr + xl) / 2.0, cy = (yr + yl) / 2.0, cz = (zr + zl) / 2.0;
+    const double hx = xr - xl, hy = yr - yl, hz = zr - zl;
+    const double L = cx - hx / 2.0, R = cx + hx / 2.0;
+    const double v0 = sqrt(L * L + mk_rand(rs) * (R * R - L * L));
+    const double v1 = (cy - hy / 2.0) + mk_rand(rs) * hy;
+    const double v2 = (cz - hz / 2.0) + mk_rand(rs) * hz;
+    double e[3], t[3], m[3];
+    mk_dist_frame(n, side, e, t, m);
+    for (int q = 0; q < 3; q++) V[q] = (v0 * e[q] + v1 * t[q]) + v2 * m[q];
+}
+
+// running sums of each incoming distribution's bins, in bin order (the order Culk_SpotokV and
+// Get_random_velosity add them in): one thread per distribution
+__global__ void __launch_bounds__(64) k_mkz_cdf(int64_t n, const int64_t *__restrict__ off, const double *__restrict__ dist,
+                                                int64_t nb, double *__restrict__ cdf) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const double *w = dist + off[g];
+    double *o = cdf + g * nb;
+    double s = 0.0;
+    for (int64_t b = 0; b < nb; b++) {
+        s = s + w[b];
+        o[b] = s;
+    }
+}
+
+// zero n distributions (their outgoing side, before a pass: MK_go's Fill_null)
+__global__ void __launch_bounds__(256) k_mkz_clear(int64_t n, const int64_t *__restrict__ off, double *__restrict__ dist,
+                                                   int64_t nb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * nb) return;
+    const int64_t g = i / nb;
+    dist[off[g] + (i - g * nb)] = 0.0;
+}
+
 // One record per cell for the flights: the 8 node positions of the cell, each face's 4 nodes (in
 // the face's own order) as indices into them, and the cell across each face (-1 - boundary code
 // on a boundary face), faces in the device order of cell_faces.
@@ -390,24 +479,27 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
     double P[3], V[3];
     for (int i = 0; i < 3; i++) P[i] = (1.0 - sqrt(r1)) * A[i] + (sqrt(r1) * (1.0 - r2)) * T1[i] + (r2 * sqrt(r1)) * Cc[i];
     unsigned long long *st = K.stats;
-    if (!mk_velosity_initial(rs, K.v_inf, n, V)) {  // no admissible velocity within the cap: dropped
+    const int in_side = K.zone > 0 ? (int)K.in_side[q] : -1;
+    if (in_side >= 0) {  // from the face's distribution (AMR_f::Get_random_velosity)
+        mk_dist_sample(rs, K.r, K.cdf + K.in_cdf[q], n, in_side, V);
+    } else if (!mk_velosity_initial(rs, K.v_inf, n, V)) {  // no admissible velocity within the cap: dropped
         atomicAdd(&st[0], 1ull);
         atomicAdd(&st[5], 1ull);
         return;
     }
     unsigned long long n_pass = 0;
-    int64_t c = M.face_cells[2 * f];
+    int64_t c = in_side == 0 ? M.face_cells[2 * f + 1] : M.face_cells[2 * f];
     {
         const double *cc = G.cell_center + 3 * c;
         for (int i = 0; i < 3; i++) P[i] = P[i] + (cc[i] - P[i]) / 1000.0;  // into the cell
     }
     double KSI = -log(1.0 - mk_rand(rs)), I_do = 0.0;
     const double mu = K.mu[q];
-    int sort = 4;
-    int slot = 4;  // tally slot: the sort (1..4), or 0 for a sort outside 1..4 (counted in the total only)
+    int sort = K.zone > 0 ? (int)K.in_sort[q] : 4;
+    int slot = (sort >= 1 && sort <= 4) ? sort : 0;  // tally slot: the sort, or 0 for one outside 1..4
     bool capped = false;
     unsigned long long n_ex = 0, n_seg = 0;
-    int outcome = 4;  // 2 left, 4 lost
+    int outcome = 4;  // 2 left, 4 lost, 8 ended on the zone's border
     const double me = 1.0 / 1836.15;
     // An exchange that falls due is held (pend) until a quarter of the wavefront's live lanes hold
     // one, or all of them do; the lanes then draw their new velocities together, so the costly
@@ -463,6 +555,24 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
                     pass_face = nb < 0 ? M.cell_faces[6 * c + jx] : -1;  // the pass follows below
                 } else if (nb < 0) {  // an inner face without the sphere: nowhere to go (lost)
                     break;
+                } else if (K.zone > 0) {
+                    const int64_t fx = M.cell_faces[6 * c + jx];
+                    if ((K.face_mask[fx] >> (K.zone - 1)) & 1u) {
+                        // the zone's border (Setka_MK.cpp:2165-2184): into the face's outgoing
+                        // distribution, on the side the particle leaves by
+                        const int nn = K.cell_mk[M.face_cells[2 * fx]] == K.zone ? 0 : 1;
+                        const int32_t ds = K.dslot[fx];
+                        if (slot > 0 && ds >= 0) {
+                            const double nf[3] = {G.face_normal[3 * fx], G.face_normal[3 * fx + 1],
+                                                  G.face_normal[3 * fx + 2]};
+                            const int64_t nbins = 108 * (int64_t)K.r * K.r * K.r;
+                            mk_deposit(K.dist + (((int64_t)ds * 4 + (slot - 1)) * 2 + nn) * nbins, K.r, nf, nn, V, mu);
+                        }
+                        outcome = 8;
+                        break;
+                    }
+                    if (K.cell_mk[nb] != K.zone) break;  // into another zone through a face no list holds: lost
+                    c = nb;
                 } else {
                     c = nb;
                 }
@@ -536,6 +646,14 @@ void launch_mk_particles(hipStream_t s, const DevMesh &M, const MkDev &K, const 
     if (M.n_cells > 0)
         hipLaunchKernelGGL(k_mk_finish, dim3(grid_of(5 * M.n_cells, 256)), dim3(256), 0, s, M.n_cells,
                            (const double *)G.cell_volume, M.cell_ref, (const double *)K.tally, K.out);
+}
+
+void launch_mkz_cdf(hipStream_t s, int64_t n, const int64_t *off, const double *dist, int64_t nb, double *cdf) {
+    if (n > 0) hipLaunchKernelGGL(k_mkz_cdf, dim3(grid_of(n, 64)), dim3(64), 0, s, n, off, dist, nb, cdf);
+}
+
+void launch_mkz_clear(hipStream_t s, int64_t n, const int64_t *off, double *dist, int64_t nb) {
+    if (n > 0) hipLaunchKernelGGL(k_mkz_clear, dim3(grid_of(n * nb, 256)), dim3(256), 0, s, n, off, dist, nb);
 }
 
 }  // namespace titan

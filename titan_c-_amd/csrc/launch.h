@@ -50,6 +50,9 @@ void launch_mk_pack(hipStream_t s, const DevMesh &M, const double *X, MkCell *pa
 void launch_mk_gather(hipStream_t s, int64_t n, const int32_t *faces, const DevGeom &G, double *na);
 void launch_mk_particles(hipStream_t s, const DevMesh &M, const MkDev &K, const double *X, const double *prim,
                          const DevGeom &G);
+// zone borders: running sums of n distributions at offsets off (into cdf [n*nb]); zeroing n of them
+void launch_mkz_cdf(hipStream_t s, int64_t n, const int64_t *off, const double *dist, int64_t nb, double *cdf);
+void launch_mkz_clear(hipStream_t s, int64_t n, const int64_t *off, double *dist, int64_t nb);
 
 // chlld over n independent states; err (device int) set to 1 on a fatal branch
 void launch_chlld_batch(hipStream_t s, int precision, int64_t n, int K, const int8_t *n_state,

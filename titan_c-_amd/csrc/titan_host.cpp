@@ -97,6 +97,19 @@ struct titan_ctx {
     double mk_R0 = 0.0;
     void *d_mk = nullptr;          // titan_mk_go: flight records, tallies, densities, counters
     bool mk_cand_ready = false;
+    // Monte-Carlo zones (titan_set_mk_zone): MK zone per cell, zone-list mask and distribution slot
+    // per face (device numbering), and the distributions themselves
+    int mkz_r = 0;  // 0: not set
+    std::vector<int8_t> mkz_cell;
+    std::vector<uint8_t> mkz_mask;
+    std::vector<int32_t> mkz_slot;
+    std::vector<uint32_t> mkz_fcode;
+    std::vector<int32_t> mkz_fcells;
+    int64_t mkz_slots = 0;
+    int8_t *d_mkz_cell = nullptr;
+    uint8_t *d_mkz_mask = nullptr;
+    int32_t *d_mkz_slot = nullptr;
+    double *d_mkz_w = nullptr;
     DevMesh mesh() const {
         DevMesh M;
         M.n_nodes = nn; M.n_cells = nc; M.n_faces = nf;
@@ -482,7 +495,8 @@ extern "C" int titan_destroy(titan_ctx *c) {
                     c->d_src, c->d_neutrals, c->d_hfluid, c->d_S, c->d_S2, c->d_t_cell_off, c->d_t_node_off,
                     c->d_t_nodes, c->d_t_own,
                     c->d_t_face_off, c->d_t_face_ref, c->d_t_faces, c->d_t_cell_lnodes, c->d_t_cell_lfaces,
-                    c->d_vol, c->d_face_tvd, c->d_t_face_tvd, c->d_mk};
+                    c->d_vol, c->d_face_tvd, c->d_t_face_tvd, c->d_mk, c->d_mkz_cell, c->d_mkz_mask,
+                    c->d_mkz_slot, c->d_mkz_w};
     for (void *q : ptrs)
         if (q) hipFree(q);
     mlp_free(c->mlp);
@@ -1156,6 +1170,118 @@ static double mk_spotok_inf(double v_inf, const double *n) {
     return -(-std::exp(-(Ux * Ux)) + sqrtpi * Ux * std::erfc(Ux)) / (2.0 * sqrtpi);
 }
 
+// The injections of one MK_go call: faces (device ids), particles per face and weights, and in the
+// zone form each one's sort, side (-1: the boundary Maxwellian) and the offset of its distribution's
+// running sums.
+struct MkInj {
+    std::vector<int32_t> face;
+    std::vector<int64_t> first{0};
+    std::vector<double> mu;
+    std::vector<int8_t> sort, side;
+    std::vector<int64_t> cdf;
+};
+
+// Particles per face and weight (Setka_MK.cpp:1233,1380-1390 with N_per_gran = n_per_face)
+static int mk_count(double sv, double mu_expect, int32_t n_per_face, int32_t *n) {
+    const double q = sv / mu_expect;
+    if (!(q < 2.0e9)) return TITAN_E_ARG;
+    *n = (int32_t)q + 1;
+    const int32_t lo = n_per_face < 100 ? n_per_face : 100;
+    if (*n < lo) *n = lo;
+    return TITAN_OK;
+}
+
+// Normal and area of the given faces at the geometry of `layer` (measured here: the fused step
+// keeps only the cell centres current).
+static int mk_face_na(titan_ctx *c, int layer, const std::vector<int32_t> &faces, std::vector<double> &na) {
+    const int64_t nb = (int64_t)faces.size();
+    na.assign(4 * nb, 0.0);
+    if (nb == 0) return TITAN_OK;
+    DevBuf dcand, dna;
+    int rc = dup(dcand, faces.data(), sizeof(int32_t) * nb);
+    if (rc == TITAN_OK) rc = alloc(dna, sizeof(double) * 4 * nb);
+    if (rc != TITAN_OK) return rc;
+    launch_mk_gather(c->stream, nb, (const int32_t *)dcand.p, c->geom[layer], (double *)dna.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(na.data(), dna.p, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TITAN_OK;
+}
+
+// The flights of one MK_go call (k_mk_particles), densities per unit volume in reference order.
+static int mk_flights(titan_ctx *c, int layer, const MkInj &inj, int zone, const double *d_cdf, uint64_t seed,
+                      int32_t max_steps, std::vector<double> &dens, int64_t st[9]) {
+    const int64_t nc = c->nc;
+    MkDev K{};
+    K.n_in = (int64_t)inj.face.size();
+    K.n_particles = inj.first.back();
+    if (K.n_particles > (int64_t)1 << 38) return TITAN_E_ARG;  // grid of at most 2^30 workgroups
+    K.seed = seed;
+    K.v_inf = c->params.v_inf;
+    K.a2 = c->params.par_a_2;
+    K.kn = c->params.par_Kn;
+    K.max_steps = max_steps;
+    K.R0 = c->mk_R0;
+    K.n_inner = (int32_t)c->mk_inner.size();
+    DevBuf dface, dfirst, dmu, dinner, dinner_nb, dsort, dside, dcdf;
+    int rc = TITAN_OK;
+    if (K.n_inner > 0) {
+        rc = dup(dinner, c->mk_inner.data(), sizeof(int32_t) * c->mk_inner.size());
+        if (rc == TITAN_OK) rc = dup(dinner_nb, c->mk_inner_nb.data(), sizeof(int32_t) * c->mk_inner_nb.size());
+        if (rc != TITAN_OK) return rc;
+        K.inner_face = (const int32_t *)dinner.p;
+        K.inner_nb = (const int32_t *)dinner_nb.p;
+    }
+    if (K.n_in > 0) {
+        rc = dup(dface, inj.face.data(), sizeof(int32_t) * inj.face.size());
+        if (rc == TITAN_OK) rc = dup(dfirst, inj.first.data(), sizeof(int64_t) * inj.first.size());
+        if (rc == TITAN_OK) rc = dup(dmu, inj.mu.data(), sizeof(double) * inj.mu.size());
+        if (rc == TITAN_OK && zone > 0) rc = dup(dsort, inj.sort.data(), inj.sort.size());
+        if (rc == TITAN_OK && zone > 0) rc = dup(dside, inj.side.data(), inj.side.size());
+        if (rc == TITAN_OK && zone > 0) rc = dup(dcdf, inj.cdf.data(), sizeof(int64_t) * inj.cdf.size());
+    }
+    if (rc != TITAN_OK) return rc;
+    if (!c->d_mk) {  // flight records, tallies, densities and counters: kept for later calls
+        const size_t bytes = sizeof(MkCell) * nc + sizeof(double) * 10 * nc + sizeof(unsigned long long) * 9;
+        HIPCHK(hipMalloc((void **)&c->d_mk, bytes));
+    }
+    MkCell *pack = (MkCell *)c->d_mk;
+    double *tally = (double *)(pack + nc);
+    double *out = tally + 5 * nc;
+    unsigned long long *dst = (unsigned long long *)(out + 5 * nc);
+    HIPCHK(hipMemsetAsync(tally, 0, sizeof(double) * 5 * nc, c->stream));
+    HIPCHK(hipMemsetAsync(dst, 0, sizeof(unsigned long long) * 9, c->stream));
+    K.face = (const int32_t *)dface.p;
+    K.first = (const int64_t *)dfirst.p;
+    K.mu = (const double *)dmu.p;
+    K.tally = tally;
+    K.out = out;
+    K.stats = dst;
+    K.pack = pack;
+    if (zone > 0) {
+        K.zone = zone;
+        K.r = c->mkz_r;
+        K.cell_mk = c->d_mkz_cell;
+        K.face_mask = c->d_mkz_mask;
+        K.dslot = c->d_mkz_slot;
+        K.dist = c->d_mkz_w;
+        K.in_sort = (const int8_t *)dsort.p;
+        K.in_side = (const int8_t *)dside.p;
+        K.in_cdf = (const int64_t *)dcdf.p;
+        K.cdf = d_cdf;
+    }
+    launch_mk_pack(c->stream, c->mesh(), c->d_X[layer], pack);
+    launch_mk_particles(c->stream, c->mesh(), K, c->d_X[layer], c->d_prim[layer], c->geom[layer]);
+    HIPCHK(hipGetLastError());
+    dens.resize(5 * nc);
+    unsigned long long h[9];
+    HIPCHK(hipMemcpyAsync(dens.data(), out, sizeof(double) * 5 * nc, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(h, dst, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 9; i++) st[i] = (int64_t)h[i];
+    return TITAN_OK;
+}
+
 extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t seed, int32_t max_steps,
                            double *density, int64_t *stats) {
     if (!c || layer < 0 || layer > 1 || n_per_face < 1 || max_steps < 1 || !density) return TITAN_E_ARG;
@@ -1176,94 +1302,284 @@ extern "C" int titan_mk_go(titan_ctx *c, int layer, int32_t n_per_face, uint64_t
         c->mk_cand_ready = true;
     }
     const std::vector<int32_t> &cand = c->mk_cand;
-    int64_t st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t st[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (!cand.empty()) {
         const int64_t nb = (int64_t)cand.size();
-        // geometry of the layer (the fused step keeps only the cell centres current)
         launch_measure(c->stream, c->mesh(), c->d_X[layer], c->geom[layer], c->params.geom_mode);
-        DevBuf dcand, dna;
-        int rc = dup(dcand, cand.data(), sizeof(int32_t) * nb);
-        if (rc == TITAN_OK) rc = alloc(dna, sizeof(double) * 4 * nb);
+        std::vector<double> na;
+        int rc = mk_face_na(c, layer, cand, na);
         if (rc != TITAN_OK) return rc;
-        launch_mk_gather(c->stream, nb, (const int32_t *)dcand.p, c->geom[layer], (double *)dna.p);
-        HIPCHK(hipGetLastError());
-        std::vector<double> na(4 * nb);
-        HIPCHK(hipMemcpyAsync(na.data(), dna.p, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        // particles per face and weights (Setka_MK.cpp:1233,1380-1390 with N_per_gran = n_per_face)
         const double v_inf = c->params.v_inf;
         double total = 0.0;
         for (int64_t i = 0; i < nb; i++) total += mk_spotok_inf(v_inf, &na[4 * i]) * na[4 * i + 3];
         const double mu_expect = total / (1.0 * n_per_face * nb);
-        std::vector<int32_t> face;
-        std::vector<int64_t> first(1, 0);
-        std::vector<double> mu;
+        MkInj inj;
         for (int64_t i = 0; i < nb; i++) {
             const double sv = mk_spotok_inf(v_inf, &na[4 * i]) * na[4 * i + 3];
             if (sv < 0.0000001 * total / nb) continue;
-            const double q = sv / mu_expect;
-            if (!(q < 2.0e9)) return TITAN_E_ARG;
-            int32_t n = (int32_t)q + 1;
-            const int32_t lo = n_per_face < 100 ? n_per_face : 100;
-            if (n < lo) n = lo;
-            face.push_back(cand[i]);
-            first.push_back(first.back() + n);
-            mu.push_back(sv / n);
+            int32_t n = 0;
+            if (mk_count(sv, mu_expect, n_per_face, &n) != TITAN_OK) return TITAN_E_ARG;
+            inj.face.push_back(cand[i]);
+            inj.first.push_back(inj.first.back() + n);
+            inj.mu.push_back(sv / n);
         }
-        MkDev K{};
-        K.n_in = (int64_t)face.size();
-        K.n_particles = first.back();
-        if (K.n_particles > (int64_t)1 << 38) return TITAN_E_ARG;  // grid of at most 2^30 workgroups
-        K.seed = seed;
-        K.v_inf = v_inf;
-        K.a2 = c->params.par_a_2;
-        K.kn = c->params.par_Kn;
-        K.max_steps = max_steps;
-        K.R0 = c->mk_R0;
-        K.n_inner = (int32_t)c->mk_inner.size();
-        DevBuf dface, dfirst, dmu, dinner, dinner_nb;
-        if (K.n_inner > 0) {
-            rc = dup(dinner, c->mk_inner.data(), sizeof(int32_t) * c->mk_inner.size());
-            if (rc == TITAN_OK) rc = dup(dinner_nb, c->mk_inner_nb.data(), sizeof(int32_t) * c->mk_inner_nb.size());
-            if (rc != TITAN_OK) return rc;
-            K.inner_face = (const int32_t *)dinner.p;
-            K.inner_nb = (const int32_t *)dinner_nb.p;
-        }
-        if (K.n_in > 0) {
-            rc = dup(dface, face.data(), sizeof(int32_t) * face.size());
-            if (rc == TITAN_OK) rc = dup(dfirst, first.data(), sizeof(int64_t) * first.size());
-            if (rc == TITAN_OK) rc = dup(dmu, mu.data(), sizeof(double) * mu.size());
-        }
+        std::vector<double> dens;
+        rc = mk_flights(c, layer, inj, 0, nullptr, seed, max_steps, dens, st);
         if (rc != TITAN_OK) return rc;
-        if (!c->d_mk) {  // flight records, tallies, densities and counters: kept for later calls
-            const size_t bytes = sizeof(MkCell) * nc + sizeof(double) * 10 * nc + sizeof(unsigned long long) * 8;
-            HIPCHK(hipMalloc((void **)&c->d_mk, bytes));
-        }
-        MkCell *pack = (MkCell *)c->d_mk;
-        double *tally = (double *)(pack + nc);
-        double *out = tally + 5 * nc;
-        unsigned long long *dst = (unsigned long long *)(out + 5 * nc);
-        HIPCHK(hipMemsetAsync(tally, 0, sizeof(double) * 5 * nc, c->stream));
-        HIPCHK(hipMemsetAsync(dst, 0, sizeof(unsigned long long) * 8, c->stream));
-        K.face = (const int32_t *)dface.p;
-        K.first = (const int64_t *)dfirst.p;
-        K.mu = (const double *)dmu.p;
-        K.tally = tally;
-        K.out = out;
-        K.stats = dst;
-        K.pack = pack;
-        launch_mk_pack(c->stream, c->mesh(), c->d_X[layer], pack);
-        launch_mk_particles(c->stream, c->mesh(), K, c->d_X[layer], c->d_prim[layer], c->geom[layer]);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(density, out, sizeof(double) * 5 * nc, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(st, dst, sizeof(st), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        std::memcpy(density, dens.data(), sizeof(double) * 5 * nc);
     } else {
         std::fill(density, density + 5 * nc, 0.0);
     }
     if (stats)
         for (int i = 0; i < 8; i++) stats[i] = st[i];
     return TITAN_OK;
+}
+
+// ---- Monte-Carlo zones ------------------------------------------------------------------------
+// Setka::Set_MK_Zone (Setka_MK.cpp:142-683) from the centres of node layer 0: the MK zone of each
+// cell and the zones whose lists hold each face; then the distributions of the listed interior
+// faces (zeroed).
+extern "C" int titan_set_mk_zone(titan_ctx *c, int32_t r, int8_t *cell_mk, uint8_t *face_mask) {
+    if (!c || r < 1 || r > 4) return TITAN_E_ARG;
+    const int64_t nc = c->nc, nf = c->nf;
+    launch_measure(c->stream, c->mesh(), c->d_X[0], c->geom[0], c->params.geom_mode);
+    std::vector<double> cc(3 * nc), fcen(3 * nf);
+    std::vector<uint32_t> ccode(nc), fcode(nf);
+    std::vector<int32_t> fcells(2 * nf);
+    HIPCHK(hipMemcpyAsync(cc.data(), c->geom[0].cell_center, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(fcen.data(), c->geom[0].face_center, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(ccode.data(), c->d_cell_code, sizeof(uint32_t) * nc, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(fcode.data(), c->d_face_code, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(fcells.data(), c->d_face_cells, sizeof(int32_t) * 2 * nf, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<int8_t> cm(nc);
+    for (int64_t d = 0; d < nc; d++) {
+        const uint32_t z = (ccode[d] >> 8) & 15u;
+        const double x = cc[3 * d];
+        if (z == 1) cm[d] = 1;
+        else if (z == 2) cm[d] = x > 0 ? 2 : 3;
+        else if (z == 3) cm[d] = x > 0 ? 4 : 5;
+        else if (z == 4) cm[d] = x > 0 ? 6 : 7;
+        else return set_err(c, TITAN_E_TOPOLOGY, c->ord.cell_d2r[d]);  // Error 86743207564
+    }
+    std::vector<uint8_t> mask(nf, 0);
+    std::vector<int32_t> slot(nf, -1);
+    int64_t n_slots = 0;
+    for (int64_t d = 0; d < nf; d++) {
+        const uint32_t bc = fcode[d] & 15u, fl = (fcode[d] >> 8) & 255u;
+        const bool ts = (fl >> 3) & 1u, hp = (fl >> 2) & 1u, bs = (fl >> 4) & 1u;
+        const bool us2 = !ts && !hp && !bs;  // Gran::type2 == Us
+        const int64_t a = fcells[2 * d], b = fcells[2 * d + 1];
+        const bool two = b >= 0;  // gr->cells.size() == 2
+        const uint32_t za = (ccode[a] >> 8) & 15u, zb = two ? (ccode[b] >> 8) & 15u : 0u;
+        const double X = fcen[3 * d];
+        const double xa = cc[3 * a], xb = two ? cc[3 * b] : 0.0;
+        const bool across = two && xa * xb < 0.0;
+        int n[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (ts) n[0]++;                               // zone 1: Gran_TS (Setka_MK.cpp:356-361)
+        if (!(X < -0.000001)) {                      // zone 2 (:364-409)
+            if (hp) n[1]++;
+            if (ts) n[1]++;
+            if (bc == TITAN_BC_OUTER_SOFT && za == 2) n[1]++;
+            if (two && za == 2 && zb == 2 && across) n[1]++;
+        }
+        if (!(X > 0.000001)) {                       // zone 3 (:412-462)
+            if (hp) n[2]++;
+            if (ts) n[2]++;
+            if (bc == TITAN_BC_OUTER_SOFT && za == 2) n[2]++;
+            if (two && us2) {
+                if (za == 2 && zb == 3) n[2]++;
+                if (za == 2 && zb == 2 && across) n[2]++;
+            }
+        }
+        if (hp && X > 0) n[3]++;                      // zone 4 (:465-500)
+        if (bs) n[3]++;
+        if (two && za == 3 && zb == 3 && across) n[3]++;
+        if (hp && X < 0.0) n[4]++;                    // zone 5 (:503-557)
+        if (bc == TITAN_BC_OUTER_SOFT && za == 3) n[4]++;
+        if (two && us2) {
+            if (za == 2 && zb == 3) n[4]++;
+            if (za == 3 && zb == 3 && across) n[4]++;
+            if (za == 3 && zb == 4 && xa < 0) n[4]++;
+        }
+        if (bs) n[5]++;                               // zone 6 (:560-592)
+        if (bc == TITAN_BC_OUTER_HARD && X > 0) n[5]++;
+        if (two && za == 4 && zb == 4 && across) n[5]++;
+        if (bc != TITAN_BC_INTERIOR && X < 0 && za == 4) n[6]++;  // zone 7 (:595-631)
+        if (two && us2) {
+            if (za == 3 && zb == 4 && xa < 0) n[6]++;
+            if (za == 4 && zb == 4 && across) n[6]++;
+        }
+        for (int z = 0; z < 7; z++) {
+            if (n[z] > 1) return set_err(c, TITAN_E_TOPOLOGY, c->ord.face_d2r[d]);  // ERROR 6435856408
+            if (n[z]) mask[d] |= (uint8_t)(1u << z);
+        }
+        if (mask[d] && bc == TITAN_BC_INTERIOR && two) slot[d] = (int32_t)n_slots++;
+    }
+    const size_t nbin = 108 * (size_t)r * r * r;
+    const size_t wbytes = sizeof(double) * 8 * nbin * (size_t)std::max<int64_t>(n_slots, 1);
+    for (void *q : {(void *)c->d_mkz_cell, (void *)c->d_mkz_mask, (void *)c->d_mkz_slot, (void *)c->d_mkz_w})
+        if (q) hipFree(q);
+    c->d_mkz_cell = nullptr; c->d_mkz_mask = nullptr; c->d_mkz_slot = nullptr; c->d_mkz_w = nullptr;
+    c->mkz_r = 0;
+    HIPCHK(hipMalloc((void **)&c->d_mkz_w, wbytes));
+    HIPCHK(hipMemsetAsync(c->d_mkz_w, 0, wbytes, c->stream));
+    if (dev_upload(&c->d_mkz_cell, cm, c->stream) != TITAN_OK || dev_upload(&c->d_mkz_mask, mask, c->stream) != TITAN_OK ||
+        dev_upload(&c->d_mkz_slot, slot, c->stream) != TITAN_OK)
+        return TITAN_E_HIP;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->mkz_cell = std::move(cm);
+    c->mkz_mask = std::move(mask);
+    c->mkz_slot = std::move(slot);
+    c->mkz_fcode = std::move(fcode);
+    c->mkz_fcells = std::move(fcells);
+    c->mkz_slots = n_slots;
+    c->mkz_r = r;
+    if (cell_mk)
+        for (int64_t d = 0; d < nc; d++) cell_mk[c->ord.cell_d2r[d]] = c->mkz_cell[d];
+    if (face_mask)
+        for (int64_t d = 0; d < nf; d++) face_mask[c->ord.face_d2r[d]] = c->mkz_mask[d];
+    return TITAN_OK;
+}
+
+// MK_prepare + MK_go(zone) + MK_delete (Setka_MK.cpp:686-1752, Setka.cpp:221-226) on the state and
+// nodes of `layer`.
+extern "C" int titan_mk_go_zone(titan_ctx *c, int layer, int32_t zone, int32_t n_per_face, uint64_t seed,
+                                int32_t max_steps, double *density, int64_t *stats) {
+    if (!c || layer < 0 || layer > 1 || zone < 1 || zone > 7 || n_per_face < 1 || max_steps < 1 || !density)
+        return TITAN_E_ARG;
+    if (!(c->params.par_Kn > 0.0) || c->mkz_r == 0) return TITAN_E_ARG;
+    const int64_t nc = c->nc, nf = c->nf;
+    const int64_t nbin = 108 * (int64_t)c->mkz_r * c->mkz_r * c->mkz_r;
+    const uint8_t bit = (uint8_t)(1u << (zone - 1));
+    // the zone's list in ascending reference face id, each face's side entering the zone
+    std::vector<int32_t> list, bfaces;
+    std::vector<int8_t> ni2;
+    std::vector<int64_t> grp;       // running-sum groups: offsets of the incoming distributions
+    std::vector<int64_t> clr;       // the outgoing ones, cleared before the flights
+    for (int64_t r = 0; r < nf; r++) {
+        const int32_t d = c->ord.face_r2d[r];
+        if (!(c->mkz_mask[d] & bit)) continue;
+        const int32_t a = c->mkz_fcells[2 * d], b = c->mkz_fcells[2 * d + 1];
+        const int8_t side = c->mkz_cell[a] == zone ? 1 : 0;
+        const int32_t enter = side ? a : b;
+        if (enter < 0 || c->mkz_cell[enter] != zone) return set_err(c, TITAN_E_TOPOLOGY, r);  // Error 9767653421
+        list.push_back(d);
+        ni2.push_back(side);
+        if ((c->mkz_fcode[d] & 15u) != TITAN_BC_INTERIOR) bfaces.push_back(d);
+        const int32_t sl = c->mkz_slot[d];
+        if (sl >= 0)
+            for (int s = 0; s < 4; s++) {
+                grp.push_back((((int64_t)sl * 4 + s) * 2 + side) * nbin);
+                clr.push_back((((int64_t)sl * 4 + s) * 2 + (1 - side)) * nbin);
+            }
+    }
+    int64_t st[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (list.empty()) {
+        for (int64_t d = 0; d < nc; d++)
+            if (c->mkz_cell[d] == zone)
+                for (int k = 0; k < 5; k++) density[5 * (int64_t)c->ord.cell_d2r[d] + k] = 0.0;
+        if (stats)
+            for (int i = 0; i < 9; i++) stats[i] = 0;
+        return TITAN_OK;
+    }
+    launch_measure(c->stream, c->mesh(), c->d_X[layer], c->geom[layer], c->params.geom_mode);
+    // running sums of the incoming distributions; their totals are the fluxes (Culk_SpotokV)
+    const int64_t ng = (int64_t)grp.size();
+    DevBuf dgrp, dclr, dcdf;
+    int rc = dup(dgrp, grp.data(), sizeof(int64_t) * ng);
+    if (rc == TITAN_OK) rc = dup(dclr, clr.data(), sizeof(int64_t) * clr.size());
+    if (rc == TITAN_OK) rc = alloc(dcdf, sizeof(double) * (size_t)std::max<int64_t>(ng, 1) * nbin);
+    if (rc != TITAN_OK) return rc;
+    std::vector<double> gsum(ng);
+    if (ng > 0) {
+        launch_mkz_cdf(c->stream, ng, (const int64_t *)dgrp.p, c->d_mkz_w, nbin, (double *)dcdf.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy2DAsync(gsum.data(), sizeof(double), (const double *)dcdf.p + (nbin - 1), sizeof(double) * nbin,
+                                sizeof(double), ng, hipMemcpyDeviceToHost, c->stream));
+    }
+    std::vector<double> na;
+    rc = mk_face_na(c, layer, bfaces, na);  // synchronises the stream
+    if (rc != TITAN_OK) return rc;
+    // fluxes into the zone per face and sort (MK_prepare, Setka_MK.cpp:851-898)
+    const int64_t nl = (int64_t)list.size();
+    std::vector<double> ff(nl, 0.0), fs(4 * nl, 0.0);
+    std::vector<int64_t> gof(nl, -1);
+    const double v_inf = c->params.v_inf;
+    double S = 0.0;
+    for (int64_t i = 0, ib = 0, ig = 0; i < nl; i++) {
+        const int32_t d = list[i];
+        if ((c->mkz_fcode[d] & 15u) == TITAN_BC_INTERIOR) {
+            if (c->mkz_slot[d] >= 0) {
+                gof[i] = ig;
+                for (int s = 0; s < 4; s++) {
+                    fs[4 * i + s] = gsum[ig + s];
+                    ff[i] += gsum[ig + s];
+                }
+                ig += 4;
+            }
+        } else {
+            const double v = mk_spotok_inf(v_inf, &na[4 * ib]) * na[4 * ib + 3];
+            fs[4 * i + 3] = v;
+            ff[i] = v;
+            ib++;
+        }
+        S += ff[i];
+    }
+    const double mu_expect = S / (1.0 * n_per_face * nl);
+    MkInj inj;
+    for (int s = 1; s <= 4 && S > 0.0; s++)  // no inflow at all (no pass has fed the zone yet): no particles
+        for (int64_t i = 0; i < nl; i++) {
+            if (ff[i] < 0.0000001 * S / nl) continue;
+            const double sv = fs[4 * i + s - 1];
+            if (sv < 0.0000001 * ff[i]) continue;
+            int32_t n = 0;
+            if (mk_count(sv, mu_expect, n_per_face, &n) != TITAN_OK) return TITAN_E_ARG;
+            const bool interior = (c->mkz_fcode[list[i]] & 15u) == TITAN_BC_INTERIOR;
+            inj.face.push_back(list[i]);
+            inj.first.push_back(inj.first.back() + n);
+            inj.mu.push_back(sv / n);
+            inj.sort.push_back((int8_t)s);
+            inj.side.push_back(interior ? ni2[i] : (int8_t)-1);
+            inj.cdf.push_back(interior ? (gof[i] + s - 1) * nbin : (int64_t)-1);
+        }
+    // the outgoing distributions of the zone's faces start empty (MK_go's Fill_null)
+    launch_mkz_clear(c->stream, (int64_t)clr.size(), (const int64_t *)dclr.p, c->d_mkz_w, nbin);
+    HIPCHK(hipGetLastError());
+    std::vector<double> dens;
+    rc = mk_flights(c, layer, inj, zone, (const double *)dcdf.p, seed, max_steps, dens, st);
+    if (rc != TITAN_OK) return rc;
+    // the zone's cells get their densities (Cell::MK_normir_Moments over the zone, Setka_MK.cpp:1674-1694)
+    for (int64_t d = 0; d < nc; d++)
+        if (c->mkz_cell[d] == zone) {
+            const int64_t r = c->ord.cell_d2r[d];
+            for (int k = 0; k < 5; k++) density[5 * r + k] = dens[5 * r + k];
+        }
+    if (stats)
+        for (int i = 0; i < 9; i++) stats[i] = st[i];
+    return TITAN_OK;
+}
+
+// One face distribution (reference face id, sort 1..4, side 0 / 1) to or from the caller: w [108 r^3].
+static int mk_dist_copy(titan_ctx *c, int64_t face, int32_t sort, int32_t side, double *w, bool set) {
+    if (!c || !w || c->mkz_r == 0 || face < 0 || face >= c->nf || sort < 1 || sort > 4 || side < 0 || side > 1)
+        return TITAN_E_ARG;
+    const int32_t sl = c->mkz_slot[c->ord.face_r2d[face]];
+    if (sl < 0) return TITAN_E_ARG;
+    const int64_t nbin = 108 * (int64_t)c->mkz_r * c->mkz_r * c->mkz_r;
+    double *p = c->d_mkz_w + (((int64_t)sl * 4 + (sort - 1)) * 2 + side) * nbin;
+    if (set) HIPCHK(hipMemcpyAsync(p, w, sizeof(double) * nbin, hipMemcpyHostToDevice, c->stream));
+    else HIPCHK(hipMemcpyAsync(w, p, sizeof(double) * nbin, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TITAN_OK;
+}
+
+extern "C" int titan_mk_dist_get(titan_ctx *c, int64_t face, int32_t sort, int32_t side, double *w) {
+    return mk_dist_copy(c, face, sort, side, w, false);
+}
+
+extern "C" int titan_mk_dist_set(titan_ctx *c, int64_t face, int32_t sort, int32_t side, const double *w) {
+    return mk_dist_copy(c, face, sort, side, const_cast<double *>(w), true);
 }
 
 // ---- surrogate network entry points (kernels in k_mlp.hip) ----------------------------------
