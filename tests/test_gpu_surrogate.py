@@ -196,3 +196,23 @@ def test_fused_surrogate_all_fallback(T, C):
             runs[mode] = (s.get_state(), st)
     assert runs[T.SRC_MLP][1]["n_fallback"] == mesh.n_cells
     assert cell_err(runs[T.SRC_MLP][0], runs[T.SRC_EXACT][0]) < 1e-12
+
+
+def test_device_estimates_match_the_fp64_restatement(T, C):
+    """With every weight and bias zero the network's outputs are zero, and the surrogate's source is
+    n_p n_H times the closed-form estimates alone: the device forms them in FP32 from hardware
+    transcendentals (erf by Abramowitz & Stegun 7.1.26), surrogate.baseline in FP64.  They agree
+    to 2e-5 of each component's largest value on every cell of the shell."""
+    sur = importlib.import_module("titan_c-_amd.surrogate")
+    mesh, prim, params, nh, x, in_norm, scale, _ = _fused_setup(T, C, sur)
+    wide = np.concatenate([np.full(8, -1e30), np.full(8, 1e30)]).astype(np.float32)
+    P = np.zeros(T.lib().titan_mlp_n_params(), np.float32)
+    with T.Setka(mesh, dict(params, source_mode=T.SRC_MLP)) as s:
+        s.set_state(prim, 0)
+        s.set_neutrals(nh)
+        s.mlp_set(P, in_norm, scale, wide)
+        got = s.sources(0, T.SRC_MLP)
+    ref = sur.sources(np.zeros((mesh.n_cells, 5)), prim[:, 0], prim[:, 4] / prim[:, 0], prim[:, 1:4], nh[:, 0],
+                      nh[:, 1], nh[:, 2:5])
+    err = np.abs(got - ref).max(axis=0) / np.abs(ref).max(axis=0)
+    assert (err < 2e-5).all(), err

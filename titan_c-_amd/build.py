@@ -41,7 +41,7 @@ UNITS = {
     "k_mk.hip": ["-ffp-contract=off"],
     "k_step32.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "k_step32sc.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
-    "k_mlp.hip": ["-ffp-contract=fast"],
+    "k_mlp.hip": ["-ffp-contract=on"],
     "k_train.hip": ["-ffp-contract=fast"],
     "k_noalloc.hip": ["-ffp-contract=off"],
 }

@@ -120,40 +120,74 @@ __device__ __forceinline__ __bf16 mlp_normalise(float v, const float *norm, int 
     return (__bf16)((v - norm[kNormMean + e]) * __builtin_amdgcn_rcpf(norm[kNormStd + e]));
 }
 
+// FP32 pieces of the closed-form estimates, each built from hardware instructions only
+// (v_log_f32, v_exp_f32, v_sqrt_f32, v_rcp_f32) so that every translation unit, whatever its
+// contraction and FP32 division / square-root flags, forms the same bits.
+__device__ __forceinline__ float est_exp(float x) {
+#pragma clang fp contract(off)
+    return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+}
+__device__ __forceinline__ float est_ln(float x) {
+#pragma clang fp contract(off)
+    return __builtin_amdgcn_logf(x) * 0.69314718055994531f;
+}
+// erf by Abramowitz & Stegun 7.1.26 (absolute error below 1.5e-7), x >= 0
+__device__ __forceinline__ float est_erf(float x) {
+#pragma clang fp contract(off)
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * x);
+    const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f +
+                                                                                   t * 1.061405429f))));
+    return 1.0f - poly * est_exp(-(x * x));
+}
+// the Lindsay-Stebbings cross-section of sigma_ls (cx_device.h) in FP32
+__device__ __forceinline__ float est_sigma_ls(float g) {
+#pragma clang fp contract(off)
+    const float e_per_g2 = (float)(0.5 * 1.67262192369e-24 * (1.03804e6 * 1.03804e6) / 1.602176634e-9);
+    const float nl16 = (float)(1e-16 * 0.06 * 4.2113 * 1.495978707e13);
+    float E = e_per_g2 * (g * g);
+    if (E < 1e-12f) E = 1e-12f;
+    const float a = 4.15f - 0.531f * est_ln(E);
+    const float b = 1.0f - est_exp(-67.3f * __builtin_amdgcn_rcpf(E));
+    // b^4.5 (b in [0, 1]: log2(0) = -inf gives 0)
+    return ((a * a) * __builtin_amdgcn_exp2f(4.5f * __builtin_amdgcn_logf(b))) * nl16;
+}
+
 // Closed-form estimates, per n_p n_H, of the five sources the network corrects (the scaled residual
 // form): over the relative-velocity Maxwellian of drift U and width c (c^2 = c_p^2 + c_H^2), the
 // sigma = const moments <g> and <g^3> and the momentum moment U u_m (u_m^2 = U^2 + 64 c^2/(9 pi)),
 // each times sigma at its speed (<g>, u_m, <g^3>^(1/3)), estimate J0, J1, J2; they give the rate
 // B0, the momentum source -B1 e and the plasma-frame energy source (U B1 (1 + kappa) - kappa B2)/2
 // (kappa = (c_p^2 - c_H^2)/c^2).  b[k] = the estimate of output k, m[k] > 0 its scale: the
-// network's output k is (source_k / (n_p n_H) - b[k]) / m[k].  The same formulas make the training
-// labels (surrogate.py).  FP64 without contraction, so the standalone and the fused kernels (built
-// with different contraction and FP32 division flags) form the same bits.
-__device__ __forceinline__ void mlp_baseline(double U, double c, double kappa, const double e[3], double b[5],
-                                             double m[5]) {
+// network's output k is (source_k / (n_p n_H) - b[k]) / m[k].  The training labels take the same
+// formulas in FP64 (surrogate.py); these FP32 estimates stay within ~1e-6 of them, far inside the
+// network's own error, and cost a few hardware transcendentals instead of a dozen FP64 library
+// calls per cell.
+__device__ __forceinline__ void mlp_baseline(float U, float c, float kappa, const float e[3], float b[5], float m[5]) {
 #pragma clang fp contract(off)
-    const double rpi = 0.56418958354775628695;  // 1/sqrt(pi)
-    const double w = U / c;
-    double g1, g3;
-    if (w < 1e-3) {  // the series of the closed forms below at small drift
-        g1 = c * (2.0 * rpi) * (1.0 + w * w / 3.0);
-        g3 = c * c * c * (4.0 * rpi) * (1.0 + w * w);
+    const float rpi = 0.56418958354775628695f;  // 1/sqrt(pi)
+    const float yc = __builtin_amdgcn_rcpf(c);
+    const float w = U * yc;
+    float g1, g3;
+    if (w < 1e-3f) {  // the series of the closed forms below at small drift
+        g1 = c * (2.0f * rpi) * (1.0f + w * w * (1.0f / 3.0f));
+        g3 = c * c * c * (4.0f * rpi) * (1.0f + w * w);
     } else {
-        const double ew = exp(-w * w) * rpi, erw = erf(w);
-        g1 = c * (ew + (w + 0.5 / w) * erw);
-        g3 = c * c * c * ((w * w + 2.5) * ew + (w * w * w + 3.0 * w + 0.75 / w) * erw);
+        const float ew = est_exp(-(w * w)) * rpi, erw = est_erf(w), yw = __builtin_amdgcn_rcpf(w);
+        g1 = c * (ew + (w + 0.5f * yw) * erw);
+        g3 = c * c * c * ((w * w + 2.5f) * ew + (w * w * w + 3.0f * w + 0.75f * yw) * erw);
     }
-    const double um = sqrt(U * U + (64.0 / (9.0 * 3.14159265358979323846)) * c * c);
-    const double B0 = sigma_ls(g1) * g1;
-    const double B1 = sigma_ls(um) * U * um;
-    const double B2 = sigma_ls(cbrt(g3)) * g3;
+    const float um = __builtin_amdgcn_sqrtf(U * U + (float)(64.0 / (9.0 * 3.14159265358979323846)) * (c * c));
+    const float cg3 = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(g3) * (1.0f / 3.0f));  // <g^3>^(1/3)
+    const float B0 = est_sigma_ls(g1) * g1;
+    const float B1 = est_sigma_ls(um) * U * um;
+    const float B2 = est_sigma_ls(cg3) * g3;
     b[0] = B0; m[0] = B0;
 #pragma unroll
     for (int k = 0; k < 3; k++) { b[1 + k] = -B1 * e[k]; m[1 + k] = B1; }
-    b[4] = 0.5 * (U * B1 * (1.0 + kappa) - kappa * B2);
-    m[4] = 0.5 * (U * B1 * (1.0 + kappa) + (fabs(kappa) + 0.01) * B2);
+    b[4] = 0.5f * (U * B1 * (1.0f + kappa) - kappa * B2);
+    m[4] = 0.5f * (U * B1 * (1.0f + kappa) + (fabsf(kappa) + 0.01f) * B2);
 #pragma unroll
-    for (int k = 0; k < 5; k++) m[k] = m[k] > 1e-30 ? m[k] : 1e-30;
+    for (int k = 0; k < 5; k++) m[k] = m[k] > 1e-30f ? m[k] : 1e-30f;
 }
 
 // Source of one cell from the network's 5 outputs y (normalised): rate, S_m, and the plasma-frame
@@ -163,15 +197,18 @@ __device__ __forceinline__ void mlp_source(const float y[5], const float *norm, 
                                            double o[5]) {
 #pragma clang fp contract(off)
     const double nn = s[0] * hn[0];
-    const double d0 = s[1] - hn[2], d1 = s[2] - hn[3], d2 = s[3] - hn[4];
-    const double U = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
-    const double cp2 = s[4] / s[0], c2 = cp2 + hn[1];
-    const double e[3] = {U > 0.0 ? d0 / U : 1.0, U > 0.0 ? d1 / U : 0.0, U > 0.0 ? d2 / U : 0.0};
-    double b[5], m[5];
-    mlp_baseline(U, sqrt(c2), (cp2 - hn[1]) / c2, e, b, m);
+    // the relative frame in FP32 from FP64 differences (reciprocals and roots by hardware instruction)
+    const float d0 = (float)(s[1] - hn[2]), d1 = (float)(s[2] - hn[3]), d2 = (float)(s[3] - hn[4]);
+    const float U = __builtin_amdgcn_sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
+    const float cp2 = (float)s[4] * __builtin_amdgcn_rcpf((float)s[0]), c2 = cp2 + (float)hn[1];
+    const float yU = __builtin_amdgcn_rcpf(U);
+    const float e[3] = {U > 0.0f ? d0 * yU : 1.0f, U > 0.0f ? d1 * yU : 0.0f, U > 0.0f ? d2 * yU : 0.0f};
+    float b[5], m[5];
+    mlp_baseline(U, __builtin_amdgcn_sqrtf(c2), (cp2 - (float)hn[1]) * __builtin_amdgcn_rcpf(c2), e, b, m);
     double yy[5];
 #pragma unroll
-    for (int k = 0; k < 5; k++) yy[k] = __builtin_fma((double)y[k] * (double)norm[kNormScale + k], m[k], b[k]);
+    for (int k = 0; k < 5; k++)
+        yy[k] = __builtin_fma((double)y[k] * (double)norm[kNormScale + k], (double)m[k], (double)b[k]);
     o[0] = nn * yy[0];
     o[1] = nn * yy[1];
     o[2] = nn * yy[2];

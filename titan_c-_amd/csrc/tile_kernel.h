@@ -427,7 +427,8 @@ __device__ __forceinline__ void surrogate_update(const TileMesh &TM, const DevMe
 #pragma unroll
         for (int j = 0; j < 6; j++) lf[j] = TM.cell_lfaces[6 * cg + j];
     }
-    if (mine) {
+    if (mine) {  // loaded here: issued before the face phase's barrier, they would stay live across the
+                 // mixed solver's FP64 re-solve call and spill
         load_state(prim1, cg, st);
 #pragma unroll
         for (int k = 0; k < 5; k++) hn[k] = sur.neutrals[5 * cg + k];

@@ -37,8 +37,10 @@ def sigma_ls(g):
 
 def baseline(U, c, kappa, e):
     """Closed-form estimates b [n, 5] of the sources per n_p n_H and their scales m [n, 5] (the
-    formulas of mlp_baseline, mlp_device.h): <g> and <g^3> of the relative-velocity Maxwellian of
-    drift U and width c, U u_m with u_m^2 = U^2 + 64 c^2 / (9 pi), sigma at <g>, u_m, <g^3>^(1/3)."""
+    formulas of mlp_baseline, mlp_device.h, which forms them in FP32 from hardware transcendentals
+    and the erf of Abramowitz & Stegun 7.1.26; this FP64 restatement makes the labels): <g> and
+    <g^3> of the relative-velocity Maxwellian of drift U and width c, U u_m with u_m^2 = U^2 +
+    64 c^2 / (9 pi), sigma at <g>, u_m, <g^3>^(1/3)."""
     rpi = 1.0 / math.sqrt(math.pi)
     w = U / c
     small = w < 1e-3
