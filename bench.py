@@ -40,6 +40,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_step_tile_fp64_%s.md")
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_k_step_tile_fp64_%s.md")
+# MI355X VALU issue: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles per SIMD at 2.4 GHz
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 
 
 def log(msg):
@@ -270,6 +273,16 @@ def traffic_from_profile(workload):
     return float(m.group(1)) if m else None
 
 
+def valu_from_profile(workload):
+    """VALU wave-instructions per launch of the FP64 step kernel (rocprofv3 SQ_INSTS_VALU), as
+    recorded in the profile summary, or None."""
+    path = VALU_FILE % workload
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"valu_wave_instructions_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
 def copy_calibration():
     """A plain streaming kernel of known size (y = -x: 256 MiB read + 256 MiB written), for the
     counter runs that scale FETCH_SIZE / WRITE_SIZE.  Runs in a process of its own: torch ships its
@@ -474,10 +487,17 @@ def main():
         bytes_step = algorithmic_bytes(nc, nf, nn, moving)
         k_ms = res[head]["step_kernel_ms"]
         ach = bytes_step / (k_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": "k_step_tile<double,false,false> (one launch = one step)",
+        roof = {"bound": "hbm", "kernel": "k_step_tile<double,false,false,false> (one launch = one step)",
                 "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profile(a.workload), "traffic_unit": "bytes/launch",
                 "algorithmic_bytes": bytes_step, "kernel_ms": round(k_ms, 4)}
+        nv = valu_from_profile(a.workload)
+        if nv:  # the roofline that binds: VALU issue (SQ_INSTS_VALU of the same kernel and workload)
+            va = nv / (k_ms * 1e-3)
+            roof["valu"] = {"achieved": round(va / 1e9, 1), "peak": round(VALU_PEAK_WAVE_INSTR_PER_S / 1e9, 1),
+                            "unit": "G wave-instructions/s", "frac": round(va / VALU_PEAK_WAVE_INSTR_PER_S, 4),
+                            "instructions_per_launch": nv}
+            roof["binding"] = "valu"
 
     prec = None
     if a.precision_steps > 0:

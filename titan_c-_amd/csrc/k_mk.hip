@@ -441,6 +441,9 @@ __global__ void __launch_bounds__(256) k_mk_finish(int64_t n_cells, const double
     out[5 * (int64_t)cell_ref[d] + k] = v / vol[d];
 }
 
+// ZONE: the zone form (titan_mk_go_zone); the one-zone flight is its own instantiation, so the
+// zone tests and deposits add nothing to its registers
+template <bool ZONE>
 __global__ void __launch_bounds__(kMkThreads) __attribute__((amdgpu_waves_per_eu(2)))
 k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *__restrict__ prim, DevGeom G) {
     __shared__ double lds[24 * kMkThreads];
@@ -479,8 +482,8 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
     double P[3], V[3];
     for (int i = 0; i < 3; i++) P[i] = (1.0 - sqrt(r1)) * A[i] + (sqrt(r1) * (1.0 - r2)) * T1[i] + (r2 * sqrt(r1)) * Cc[i];
     unsigned long long *st = K.stats;
-    const int in_side = K.zone > 0 ? (int)K.in_side[q] : -1;
-    if (in_side >= 0) {  // from the face's distribution (AMR_f::Get_random_velosity)
+    const int in_side = ZONE ? (int)K.in_side[q] : -1;
+    if (ZONE && in_side >= 0) {  // from the face's distribution (AMR_f::Get_random_velosity)
         mk_dist_sample(rs, K.r, K.cdf + K.in_cdf[q], n, in_side, V);
     } else if (!mk_velosity_initial(rs, K.v_inf, n, V)) {  // no admissible velocity within the cap: dropped
         atomicAdd(&st[0], 1ull);
@@ -495,7 +498,7 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
     }
     double KSI = -log(1.0 - mk_rand(rs)), I_do = 0.0;
     const double mu = K.mu[q];
-    int sort = K.zone > 0 ? (int)K.in_sort[q] : 4;
+    int sort = ZONE ? (int)K.in_sort[q] : 4;
     int slot = (sort >= 1 && sort <= 4) ? sort : 0;  // tally slot: the sort, or 0 for one outside 1..4
     bool capped = false;
     unsigned long long n_ex = 0, n_seg = 0;
@@ -555,7 +558,7 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
                     pass_face = nb < 0 ? M.cell_faces[6 * c + jx] : -1;  // the pass follows below
                 } else if (nb < 0) {  // an inner face without the sphere: nowhere to go (lost)
                     break;
-                } else if (K.zone > 0) {
+                } else if (ZONE) {
                     const int64_t fx = M.cell_faces[6 * c + jx];
                     if ((K.face_mask[fx] >> (K.zone - 1)) & 1u) {
                         // the zone's border (Setka_MK.cpp:2165-2184): into the face's outgoing
@@ -640,9 +643,14 @@ void launch_mk_gather(hipStream_t s, int64_t n, const int32_t *faces, const DevG
 
 void launch_mk_particles(hipStream_t s, const DevMesh &M, const MkDev &K, const double *X, const double *prim,
                          const DevGeom &G) {
-    if (K.n_particles > 0)
-        hipLaunchKernelGGL(k_mk_particles, dim3(grid_of(K.n_particles, kMkThreads)), dim3(kMkThreads), 0, s, M, K, X,
-                           prim, G);
+    if (K.n_particles > 0) {
+        if (K.zone > 0)
+            hipLaunchKernelGGL(k_mk_particles<true>, dim3(grid_of(K.n_particles, kMkThreads)), dim3(kMkThreads), 0, s, M,
+                               K, X, prim, G);
+        else
+            hipLaunchKernelGGL(k_mk_particles<false>, dim3(grid_of(K.n_particles, kMkThreads)), dim3(kMkThreads), 0, s, M,
+                               K, X, prim, G);
+    }
     if (M.n_cells > 0)
         hipLaunchKernelGGL(k_mk_finish, dim3(grid_of(5 * M.n_cells, 256)), dim3(256), 0, s, M.n_cells,
                            (const double *)G.cell_volume, M.cell_ref, (const double *)K.tally, K.out);
