@@ -396,6 +396,20 @@ int titan_checkpoint_write(const char *path, int64_t n_cells, int n_keys, const 
                            const double *values, int64_t n_nodes, const double *xyz);
 int titan_checkpoint_read(const char *path, int64_t n_cells, int n_keys, const char *const *keys,
                           double *values, int64_t n_nodes, double *xyz, int64_t *nodes_read);
+/* The file of Setka::Save_cell_MK_parameters / Setka::Download_cell_MK_parameters
+ * (Setka_physics.cpp:4165-4209, 4287-4336), little-endian: u64 key count, each key once (u64
+ * length, bytes), then per cell (All_Cell order) one f64 per key in that order, then 1000 zero
+ * bytes.  titan_mk_parameters_write writes values [n_cells][n_keys] (e.g. MK_n_H, MK_n_H1..4 of
+ * titan_mk_go).  titan_mk_parameters_read copies the file's value of every named key it holds into
+ * values [n_cells][n_keys], except for the cells whose MK zone (cell_mk, nullable) is zone_except,
+ * which keep theirs (the zone the caller is about to compute); keys the file lacks are left as
+ * they are; *keys_found (nullable) = how many of the named keys the file holds.  TITAN_E_IO for a
+ * missing or truncated file; TITAN_E_ARG when the values and the 1000-byte trailer do not fill
+ * the file exactly (a file of another cell count). */
+int titan_mk_parameters_write(const char *path, int64_t n_cells, int n_keys, const char *const *keys,
+                              const double *values);
+int titan_mk_parameters_read(const char *path, int64_t n_cells, int n_keys, const char *const *keys, double *values,
+                             const int8_t *cell_mk, int32_t zone_except, int32_t *keys_found);
 /* Setka::Save_cell_parameters on a context: the 8 plasma keys (rho, Vx, Vy, Vz, p, Bx, By, Bz) of
  * the newest layer, and Q, rho_He when the context carries them, then n_extra caller keys with
  * extra_values [n_cells][n_extra] (e.g. the hydrogen fluids kept on the host), and the newest node

@@ -101,3 +101,64 @@ def test_oracle_reader_resets_mk_keys(tmp_path):
     cells, xyz = CK.read(path, 1, mk_keys=("MK_n_H1",))
     assert cells[0] == {"rho": 2.0, "MK_n_H1": 0.0}
     assert xyz == [(0.0, 0.0, 1.0)]
+
+
+def test_mk_parameters_known_bytes(T, tmp_path):
+    """Two cells of (MK_n_H, MK_n_H1): the bytes of Save_cell_MK_parameters' out.write calls
+    (Setka_physics.cpp:4173-4208)."""
+    path = str(tmp_path / "mk1.bin")
+    T.mk_parameters_write(path, [[0.5, 0.25], [2.0, -1.0]], keys=("MK_n_H", "MK_n_H1"))
+    want = (struct.pack("<Q", 2) + struct.pack("<Q", 6) + b"MK_n_H" + struct.pack("<Q", 7) + b"MK_n_H1"
+            + struct.pack("<dddd", 0.5, 0.25, 2.0, -1.0) + b"\x00" * 1000)
+    assert open(path, "rb").read() == want
+    got = np.full((2, 3), 9.0)
+    assert T.mk_parameters_read(path, got, keys=("MK_n_H1", "Q", "MK_n_H")) == 2
+    np.testing.assert_array_equal(got, [[0.25, 9.0, 0.5], [-1.0, 9.0, 2.0]])
+
+
+def test_mk_parameters_against_the_oracle(T, tmp_path):
+    """Writer and reader against write_mk / read_mk, with the zone being computed kept
+    (Download_cell_MK_parameters' zone_except, Setka_physics.cpp:4323)."""
+    rng = np.random.default_rng(7)
+    nc = 301
+    keys = list(T.MK_PARAMETERS)
+    vals = rng.normal(size=(nc, 5))
+    zone = rng.integers(1, 8, size=nc).astype(np.int8)
+    path = str(tmp_path / "mk2.bin")
+    T.mk_parameters_write(path, vals)
+    cells = [{k: -7.0 for k in keys} for _ in range(nc)]
+    assert CK.read_mk(path, cells, cell_zone=zone, zone_except=3) == keys
+    mine = np.full((nc, 5), -7.0)
+    assert T.mk_parameters_read(path, mine, cell_mk=zone, zone_except=3) == 5
+    np.testing.assert_array_equal(mine, [[kv[k] for k in keys] for kv in cells])
+    assert (mine[zone == 3] == -7.0).all() and (mine[zone != 3] == vals[zone != 3]).all()
+    # the oracle's writer, the library's reader
+    path2 = str(tmp_path / "mk3.bin")
+    CK.write_mk(path2, keys[::-1], vals[:, ::-1])
+    back = np.zeros((nc, 5))
+    T.mk_parameters_read(path2, back)
+    np.testing.assert_array_equal(back, vals)
+
+
+def test_mk_parameters_damage(T, tmp_path):
+    path = str(tmp_path / "mk4.bin")
+    T.mk_parameters_write(path, np.ones((6, 5)))
+    data = open(path, "rb").read()
+    for cut in (4, 30, len(data) - 1000 - 8):
+        bad = str(tmp_path / ("mkcut%d.bin" % cut))
+        open(bad, "wb").write(data[:cut])
+        with pytest.raises(T.TitanError) as e:
+            T.mk_parameters_read(bad, np.zeros((6, 5)))
+        assert e.value.code == T.TITAN_E_IO
+    for n in (5, 7):                             # a file of another cell count
+        with pytest.raises(T.TitanError) as e:
+            T.mk_parameters_read(path, np.zeros((n, 5)))
+        assert e.value.code == T.TITAN_E_ARG
+    with pytest.raises(T.TitanError) as e:       # more cells than the file holds values for
+        T.mk_parameters_read(path, np.zeros((40, 5)))
+    assert e.value.code == T.TITAN_E_IO
+    with pytest.raises(ValueError):
+        T.mk_parameters_read(path, np.zeros((6, 4)))
+    with pytest.raises(T.TitanError) as e:
+        T.mk_parameters_read(str(tmp_path / "none.bin"), np.zeros((6, 5)))
+    assert e.value.code == T.TITAN_E_IO

@@ -127,6 +127,7 @@ _lib = None
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
 _D = ctypes.c_double
 
 _SIGS = {
@@ -167,6 +168,8 @@ _SIGS = {
     "titan_mlp_train": (_I, [_P, _I64, _P, _P, _I, _I, ctypes.c_float, ctypes.c_uint64, _P]),
     "titan_checkpoint_write": (_I, [ctypes.c_char_p, _I64, _I, _P, _P, _I64, _P]),
     "titan_checkpoint_read": (_I, [ctypes.c_char_p, _I64, _I, _P, _P, _I64, _P, ctypes.POINTER(_I64)]),
+    "titan_mk_parameters_write": (_I, [ctypes.c_char_p, _I64, _I, _P, _P]),
+    "titan_mk_parameters_read": (_I, [ctypes.c_char_p, _I64, _I, _P, _P, _P, _I32, ctypes.POINTER(_I32)]),
     "titan_save_cell_parameters": (_I, [_P, ctypes.c_char_p, _I, _P, _P]),
     "titan_download_cell_parameters": (_I, [_P, ctypes.c_char_p]),
     "titan_init_tvd": (_I, [_P]),
@@ -255,6 +258,35 @@ def checkpoint_read(path, n_cells, keys, n_nodes=-1):
                                        _ptr(xyz) if n_nodes > 0 else None, ctypes.byref(got)),
            "titan_checkpoint_read")
     return vals, (xyz if got.value > 0 else None)
+
+
+MK_PARAMETERS = ("MK_n_H", "MK_n_H1", "MK_n_H2", "MK_n_H3", "MK_n_H4")
+
+
+def mk_parameters_write(path, values, keys=MK_PARAMETERS):
+    """Setka::Save_cell_MK_parameters file: values [n_cells, len(keys)] (titan_mk_go's density by
+    default: MK_n_H, MK_n_H1..4)."""
+    vals = _c(values, np.float64).reshape(-1, len(keys))
+    arr, kp = _keys(keys)
+    _check(lib().titan_mk_parameters_write(os.fsencode(path), vals.shape[0], len(keys), kp, _ptr(vals)),
+           "titan_mk_parameters_write")
+
+
+def mk_parameters_read(path, values, keys=MK_PARAMETERS, cell_mk=None, zone_except=-1):
+    """Setka::Download_cell_MK_parameters: overwrites values [n_cells, len(keys)] (float64,
+    C-contiguous, in place) with the file's, except in the cells of MK zone zone_except (cell_mk
+    from Set_MK_Zone); returns how many of the keys the file holds."""
+    if not (isinstance(values, np.ndarray) and values.dtype == np.float64 and values.flags.c_contiguous
+            and values.ndim == 2 and values.shape[1] == len(keys)):
+        raise ValueError("values must be a C-contiguous float64 array [n_cells, len(keys)]")
+    arr, kp = _keys(keys)
+    cm = None if cell_mk is None else _c(cell_mk, np.int8)
+    if cm is not None and cm.shape[0] != values.shape[0]:
+        raise ValueError("cell_mk must hold one zone per cell")
+    got = _I32(0)
+    _check(lib().titan_mk_parameters_read(os.fsencode(path), values.shape[0], len(keys), kp, _ptr(values),
+                                          _ptr(cm), zone_except, ctypes.byref(got)), "titan_mk_parameters_read")
+    return got.value
 
 
 # ---- mesh helpers ---------------------------------------------------------------------------

@@ -174,3 +174,79 @@ extern "C" int titan_checkpoint_read(const char *path, int64_t n_cells, int n_ke
     }
     return TITAN_OK;
 }
+
+// TITAN's Monte-Carlo parameter file (Setka::Save_cell_MK_parameters /
+// Setka::Download_cell_MK_parameters, Setka_physics.cpp:4165-4209, 4287-4336): u64 key count, the
+// keys (u64 length, bytes) once, then per cell in All_Cell order one f64 per key in that order,
+// then 1000 zero bytes.
+extern "C" int titan_mk_parameters_write(const char *path, int64_t n_cells, int n_keys, const char *const *keys,
+                                         const double *values) {
+    if (!path || n_cells < 0 || n_keys < 0 || (n_keys > 0 && (!keys || !values))) return TITAN_E_ARG;
+    std::vector<uint8_t> head(8);
+    const uint64_t cnt = (uint64_t)n_keys;
+    std::memcpy(head.data(), &cnt, 8);
+    for (int k = 0; k < n_keys; k++) {
+        if (!keys[k]) return TITAN_E_ARG;
+        const uint64_t ks = std::strlen(keys[k]);
+        if (ks == 0 || ks > kMaxKey) return TITAN_E_ARG;
+        const size_t at = head.size();
+        head.resize(at + 8 + ks);
+        std::memcpy(&head[at], &ks, 8);
+        std::memcpy(&head[at + 8], keys[k], ks);
+    }
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return TITAN_E_IO;
+    bool ok = std::fwrite(head.data(), 1, head.size(), f) == head.size();
+    if (ok && n_cells > 0 && n_keys > 0)
+        ok = std::fwrite(values, sizeof(double) * (size_t)n_keys, (size_t)n_cells, f) == (size_t)n_cells;
+    if (ok) {
+        const std::vector<uint8_t> zeros(kTrailer, 0);
+        ok = std::fwrite(zeros.data(), 1, zeros.size(), f) == zeros.size();
+    }
+    if (std::fclose(f) != 0) ok = false;
+    return ok ? TITAN_OK : TITAN_E_IO;
+}
+
+extern "C" int titan_mk_parameters_read(const char *path, int64_t n_cells, int n_keys, const char *const *keys,
+                                        double *values, const int8_t *cell_mk, int32_t zone_except,
+                                        int32_t *keys_found) {
+    if (!path || n_cells < 0 || n_keys < 0 || (n_keys > 0 && (!keys || !values))) return TITAN_E_ARG;
+    std::unordered_map<std::string, int> want;
+    for (int k = 0; k < n_keys; k++) {
+        if (!keys[k]) return TITAN_E_ARG;
+        want.emplace(keys[k], k);
+    }
+    std::vector<uint8_t> buf;
+    int rc = load_file(path, buf);
+    if (rc != TITAN_OK) return rc;
+    Reader r{buf.data(), buf.data() + buf.size()};
+    uint64_t cnt = 0;
+    if (!r.take(&cnt, 8) || cnt > (uint64_t)(r.end - r.p) / 8) return TITAN_E_IO;
+    std::vector<int> slot(cnt, -1);
+    int found = 0;
+    for (uint64_t j = 0; j < cnt; j++) {
+        uint64_t ks = 0;
+        if (!r.take(&ks, 8) || ks > kMaxKey) return TITAN_E_IO;
+        const char *kp = reinterpret_cast<const char *>(r.p);
+        if (!r.skip(ks)) return TITAN_E_IO;
+        auto it = want.find(std::string(kp, ks));
+        if (it != want.end()) {
+            slot[j] = it->second;
+            found++;
+        }
+    }
+    // the file does not state its cell count: the values and the trailer must fill it exactly
+    const uint64_t left = (uint64_t)(r.end - r.p);
+    if (cnt > 0 && left / (8 * cnt) < (uint64_t)n_cells) return TITAN_E_IO;
+    if (left != 8 * cnt * (uint64_t)n_cells + kTrailer) return TITAN_E_ARG;
+    for (int64_t c = 0; c < n_cells; c++) {
+        const bool keep = cell_mk && cell_mk[c] == zone_except;  // the zone being computed keeps its values
+        for (uint64_t j = 0; j < cnt; j++) {
+            double v;
+            r.take(&v, 8);
+            if (!keep && slot[j] >= 0) values[c * n_keys + slot[j]] = v;
+        }
+    }
+    if (keys_found) *keys_found = found;
+    return TITAN_OK;
+}
