@@ -310,7 +310,9 @@ int titan_cx_exact_batch(int64_t n, const double *in10, int nodes, double *out5)
  * n_per_face, inflow Get_Spotok_inf), births by Gran::Get_Random_pozition and
  * Setka::Velosity_initial, straight flights cell to cell (Setka::Time_to_vilet), charge exchange
  * by optical depth with nu = rho_Th Velosity_1 sigma / par_Kn and the proton velocity of
- * Setka::M_K_Change_Velosity (a new particle takes the sort of the cell's zone).  density
+ * Setka::M_K_Change_Velosity (a new particle takes the sort of the cell's zone); rho_Th and p_Th
+ * are Phys_param::Plasma_components_1 at the cell's type, with the layer's rho_He when the passive
+ * scalars are carried (else 0), as the flight calls it (Setka_MK.cpp:1944-1966).  density
  * [n_cells*5] = MK_n_H, MK_n_H1..MK_n_H4 of Cell::MK_Add_particle (Cell.cpp:1042-1083) per unit
  * volume.  A particle that comes within 1.01 R0 of the centre after crossing a face (R0: the
  * radius of the inner boundary faces' nodes) is carried along its velocity to where it leaves that

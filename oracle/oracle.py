@@ -80,6 +80,7 @@ def lib():
             "or_mk_go": (_I, [_P, _P, _P, _P, _P, _P, _P]),
             "or_mk_change_velosity_batch": (None, [_I64, ctypes.c_uint64, _D, _P, _P, _D, _I, _P, _P]),
             "or_set_mk_zone": (_I, [_P, _P, _I, _P, _P]),
+            "or_plasma_components_1": (None, [_D, _D, _D, _I, _P]),
             "or_mk_setup_zone": (_I64, [_P, _P, _D, ctypes.c_int32, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P, _P]),
             "or_mk_deposit": (None, [_P, ctypes.c_int32, _I, _I, _P, _P, _D]),
             "or_mk_sample_batch": (None, [_P, ctypes.c_int32, _I, _I, _P, _I64, ctypes.c_uint64, _P]),
@@ -758,7 +759,7 @@ class OrMk(ctypes.Structure):
     _fields_ = [("n_in", _I64), ("face", _P), ("count", _P), ("mu", _P), ("seed", ctypes.c_uint64),
                 ("v_inf", _D), ("a2", _D), ("kn", _D), ("max_steps", ctypes.c_int32), ("mistake", ctypes.c_int32),
                 ("nthreads", ctypes.c_int32), ("R0", _D), ("zone", ctypes.c_int32), ("pad_", ctypes.c_int32),
-                ("cell_mk", _P), ("face_mask", _P), ("in_sort", _P), ("in_side", _P), ("dist", _P)]
+                ("cell_mk", _P), ("face_mask", _P), ("in_sort", _P), ("in_side", _P), ("dist", _P), ("scal", _P)]
 
 
 class OrMkDist(ctypes.Structure):
@@ -807,9 +808,11 @@ def mk_setup(om, geom, v_inf, n_per_face):
     return face[:k].copy(), count[:k].copy(), mu[:k].copy()
 
 
-def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000, geom=None, mistake=0, nthreads=1):
+def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000, geom=None, mistake=0, nthreads=1,
+          scal=None):
     """or_mk_go on the nodes xyz and state prim (geom: measure(om, xyz), computed when None;
-    mistake: or_mk.mistake, nonzero only in tests that show the pins catch an error);
+    mistake: or_mk.mistake, nonzero only in tests that show the pins catch an error; scal: the
+    passive scalars [n_cells, 2] = Q, rho_He, or None for rho_He = 0);
     returns (densities [n_cells, 5] = sum over sorts, sorts 1..4, per unit volume; stats dict)."""
     m = om.m
     x = np.ascontiguousarray(xyz, np.float64)
@@ -818,11 +821,20 @@ def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000, ge
     g = OrGeom(*[_p(geom[k]) for k, _ in OrGeom._fields_])
     mk = OrMk(len(face), _p(face), _p(count), _p(mu), int(seed), float(v_inf), float(a2), float(kn), int(max_steps),
               int(mistake), int(nthreads), inner_radius(m, x))
+    sc = None if scal is None else np.ascontiguousarray(scal, np.float64).reshape(m.n_cells, 2)
+    mk.scal = _p(sc)
     tally = np.zeros((m.n_cells, 5))
     stats = np.zeros(8, np.int64)
     lib().or_mk_go(ctypes.byref(om.c), ctypes.byref(mk), _p(x), _p(np.ascontiguousarray(prim, np.float64)),
                    ctypes.byref(g), _p(tally), _p(stats))
     return tally / geom["cell_volume"][:, None], dict(zip(MK_STATS[:8], (int(v) for v in stats)))
+
+
+def plasma_components_1(rho, p, rho_He, zone):
+    """or_plasma_components_1: (rho_Th, p_Th, T_Th)."""
+    th = (ctypes.c_double * 3)()
+    lib().or_plasma_components_1(float(rho), float(p), float(rho_He), int(zone), th)
+    return th[0], th[1], th[2]
 
 
 def set_mk_zone(om, geom, mistake=0):
@@ -851,7 +863,7 @@ def mk_setup_zone(om, geom, v_inf, n_per_face, zone, cell_mk, mask, dist):
 
 
 def mk_go_zone(om, xyz, prim, v_inf, a2, kn, n_per_face, zone, cell_mk, mask, dist, seed=1, max_steps=100000,
-               geom=None, nthreads=1):
+               geom=None, nthreads=1, scal=None):
     """One pass of MK_go(zone) (or_mk_setup_zone + or_mk_go): the densities of the zone's cells per
     unit volume ([n_cells, 5], zero elsewhere), the stats dict, and the injections; deposits into
     dist, whose outgoing distributions of the zone's faces are cleared first (MK_go's Fill_null)."""
@@ -869,6 +881,8 @@ def mk_go_zone(om, xyz, prim, v_inf, a2, kn, n_per_face, zone, cell_mk, mask, di
     mk = OrMk(len(face), _p(face), _p(count), _p(mu), int(seed), float(v_inf), float(a2), float(kn), int(max_steps),
               0, int(nthreads), inner_radius(m, x), int(zone), 0, _p(cm), _p(mk_), _p(sort), _p(side),
               ctypes.cast(ctypes.pointer(dist.c), ctypes.c_void_p))
+    sc = None if scal is None else np.ascontiguousarray(scal, np.float64).reshape(m.n_cells, 2)
+    mk.scal = _p(sc)
     tally = np.zeros((m.n_cells, 5))
     stats = np.zeros(9, np.int64)
     lib().or_mk_go(ctypes.byref(om.c), ctypes.byref(mk), _p(x), _p(np.ascontiguousarray(prim, np.float64)),

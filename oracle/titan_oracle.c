@@ -2405,6 +2405,18 @@ int64_t or_mk_setup_zone(const or_mesh *m, const or_geom *g, double v_inf, int32
     return k;
 }
 
+/* Phys_param::Plasma_components_1 (Phys_param.cpp:628-665): the thermal protons of a plasma cell
+ * holding helium, th = (rho_Th, p_Th, T_Th); al = 2 electrons per helium ion for zone <= 2, else 1.
+ * Pinned by mass (rho = rho_Th (1 + me) + rho_He (1 + al me / 4)), by p_Th = rho_Th T_Th / 2, and by
+ * rho_He = 0 (rho_Th = rho / (1 + me), p_Th = p / 2). */
+void or_plasma_components_1(double rho, double p, double rho_He, int zone, double th[3]) {
+    const double me = OR_ME_MP;
+    const int al = zone <= 2 ? 2 : 1;
+    th[0] = -(me * al * rho_He + 4.0 * (-rho + rho_He)) / (4.0 * (1.0 + me));
+    th[1] = p * (4.0 * rho - (4.0 + al * me) * rho_He) / (8.0 * rho + (-7.0 + al + (1.0 - al) * me) * rho_He);
+    th[2] = 8.0 * (1.0 + me) * p / (8.0 * rho - (7.0 - al + (-1.0 + al) * me) * rho_He);
+}
+
 int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *prim, const or_geom *g,
              double *tally, int64_t *stats)
 {
@@ -2484,7 +2496,10 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
                 if (!found) { stats[4]++; done = 1; break; }
                 stats[6]++;
                 const double *s = prim + 8 * c;
-                double rho_Th = s[0] / (1.0 + me), p_Th = s[4] / 2.0;
+                /* the thermal protons at the cell's type, as the flight calls it (Setka_MK.cpp:1944-1966) */
+                double th[3];
+                or_plasma_components_1(s[0], s[4], mk->scal ? mk->scal[2 * c + 1] : 0.0, m->cell_zone[c], th);
+                double rho_Th = th[0], p_Th = th[1];
                 if (rho_Th <= 1e-8) rho_Th = 1e-8;
                 if (p_Th <= 1e-8 / 2.0) p_Th = 1e-8 / 2.0;
                 double ro = rho_Th, cp = sqrt(2.0 * p_Th / rho_Th);
@@ -2565,7 +2580,10 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
                     mk_change_velosity(&rs, mk->a2, U, Vs, W, cp, &capped, mk->mistake);
                     for (int i = 0; i < 3; i++) W[i] *= cp;
                     mk_from_sph(P, W[0], W[1], W[2], V);
-                    sort = m->cell_zone[c];
+                    /* P.sort = determ_zone(P.cel, 0) (Setka_MK.cpp:2088, 2121): Q routes it when
+                     * the scalars are carried; without them the cell's type stands in (what Q gives
+                     * where the marker follows the types, Q/rho >= 50 exactly past the HP) */
+                    sort = mk->scal ? or_determ_zone(mk->scal[2 * c], s[0], m->cell_zone[c]) : m->cell_zone[c];
                     KSI = -log(1.0 - mk_rand(&rs));
                     stats[1]++;
                 }

@@ -361,6 +361,9 @@ typedef struct {
     const int8_t *in_side;     /* [n_in] -1: the boundary Maxwellian (Velosity_initial); 0 / 1: the
                                   face's distribution on that side; NULL: -1 */
     struct or_mk_dist *dist;   /* zone > 0: the distributions read and deposited into */
+    const double *scal;        /* [n_cells*2] Q, rho_He per cell, for the thermal-proton split of
+                                  Plasma_components_1 at the cell's type (Setka_MK.cpp:1944-1966);
+                                  NULL: rho_He = 0 */
 } or_mk;
 /* Velocity distributions on the zone borders (AMR_f at its unrefined grid of Gran::Read_AMR,
  * Gran.cpp:16-99): (0, 20) x (-20, 20)^2 in the face's frame (Vn, get_bazis(Vn)), 3r x 6r x 6r
@@ -408,6 +411,9 @@ int64_t or_mk_setup(const or_mesh *m, const or_geom *g, double v_inf, int32_t n_
 void or_mk_change_velosity_batch(int64_t n, uint64_t seed, double a2, const double *U, const double *V, double cp,
                                  int mistake, double *W, int64_t *capped);
 /* With mk->zone > 0, stats has a ninth entry: flights that ended on the zone's border. */
+/* Phys_param::Plasma_components_1 (Phys_param.cpp:628-665): th = (rho_Th, p_Th, T_Th) of a plasma
+ * cell (rho, p, rho_He) of type zone 1..4. */
+void or_plasma_components_1(double rho, double p, double rho_He, int zone, double th[3]);
 int or_mk_go(const or_mesh *m, const or_mk *mk, const double *coords, const double *prim, const or_geom *g,
              double *tally, int64_t *stats);
 

@@ -175,3 +175,34 @@ def test_moved_layer(T, C, O):
     same = (np.abs(d[:, 0] - od[:, 0]) * vol).sum()
     noise = (np.abs(o2[:, 0] - od[:, 0]) * vol).sum()
     assert same < 0.5 * noise, (same, noise)
+
+
+def test_helium_split_matches_the_oracle(T, C, O):
+    """With the passive scalars carried, the flight's protons are the thermal part of
+    Plasma_components_1 at the cell's type (Setka_MK.cpp:1944-1966): rho_He = 0.3 rho and p set so
+    that cp stays 1, on both sides; the exchange count follows the oracle's, and falls to 0.7 of
+    the helium-free count as rho_Th does."""
+    mesh, _, params = _cube(C, n=5)
+    prim = _lism(C, mesh)
+    he = 0.3
+    al = np.where(np.asarray(mesh.cell_zone) <= 2, 2, 1)
+    r_th = -(ME * al * he + 4.0 * (-1.0 + he)) / (4.0 * (1.0 + ME))
+    ph = prim.copy()
+    ph[:, 4] = (r_th / 2.0) * (8.0 + (-7.0 + al + (1.0 - al) * ME) * he) / (4.0 - (4.0 + al * ME) * he)
+    scal = np.zeros((mesh.n_cells, 2))
+    scal[:, 1] = he
+    with T.Setka(mesh, dict(params, par_a_2=A2, par_Kn=0.3, passive_scalars=1)) as s:
+        s.set_state(ph, 0)
+        s.set_scalars(scal, 0)
+        d, st = s.MK_go(150, seed=22, layer=0)
+        s.set_scalars(np.zeros_like(scal), 0)
+        s.set_state(prim, 0)
+        d0, st0 = s.MK_go(150, seed=21, layer=0)
+    od, ost = O.mk_go(O.OracleMesh(mesh), mesh.node_xyz, ph, C.V_INF, A2, 0.3, 150, seed=22, nthreads=4, scal=scal)
+    assert st["particles"] == ost["particles"] and st["lost"] == 0
+    assert abs(st["exchanges"] - ost["exchanges"]) < 0.01 * ost["exchanges"]
+    assert abs(d[:, 0].mean() - od[:, 0].mean()) < 2e-3
+    assert abs(st["exchanges"] / st0["exchanges"] - r_th.mean() * (1.0 + ME)) < 0.02
+    # Q = 0 carried: determ_zone sends the exchanged atoms of these type-3 cells to sort 2, not 3
+    assert d[:, 3].sum() == 0 and od[:, 3].sum() == 0 and d[:, 2].sum() > 0
+    assert abs(d[:, 2].sum() / d[:, 0].sum() - od[:, 2].sum() / od[:, 0].sum()) < 0.01

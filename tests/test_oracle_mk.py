@@ -188,3 +188,65 @@ def test_pass_through_the_sun_keeps_the_free_flight_density(C, O):
     # absorbing the particles at the sphere instead leaves a shadow behind the Sun
     da, _ = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, 1e30, 1000, seed=5, nthreads=4, mistake=4)
     assert _sun_layers(O, mesh, da)[:2, 1].max() < 0.5
+
+
+@pytest.mark.parametrize("zone", [1, 2, 3, 4])
+def test_thermal_proton_split(O, zone):
+    """Plasma_components_1 (Phys_param.cpp:628-665) pinned by the mass it splits, rho = rho_Th
+    (1 + me) + rho_He (1 + al me / 4) with al = 2 electrons per helium ion in zones 1-2 and 1 in
+    3-4, by p_Th = rho_Th T_Th / 2 (T_Th is the reference's separate expression), and by the
+    helium-free plasma: rho_Th = rho / (1 + me) exactly, p_Th = p / 2 to one rounding."""
+    al = 2 if zone <= 2 else 1
+    rng = np.random.default_rng(zone)
+    for rho, p, he in zip(rng.uniform(0.1, 5.0, 50), rng.uniform(0.01, 3.0, 50), rng.uniform(0.0, 0.3, 50)):
+        he = he * rho
+        r_th, p_th, t_th = O.plasma_components_1(rho, p, he, zone)
+        assert abs(r_th * (1.0 + ME) + he * (1.0 + al * ME / 4.0) - rho) < 1e-14 * rho
+        assert abs(p_th - r_th * t_th / 2.0) < 1e-14 * p
+        r0, p0, _ = O.plasma_components_1(rho, p, 0.0, zone)
+        assert r0 == rho / (1.0 + ME)
+        assert abs(p0 - p / 2.0) <= 2.3e-16 * p
+
+
+def test_helium_takes_its_share_of_the_exchanges(C, O):
+    """The flight sees the thermal protons only: with helium rho_He = 0.3 rho and p set so that
+    cp = sqrt(2 p_Th / rho_Th) stays 1, detailed balance still holds (density 1) and the expected
+    exchange count scales with rho_Th, here to 0.7 of the helium-free count (MC noise ~0.6%)."""
+    mesh, _, _ = _cube(C, n=5)
+    om = O.OracleMesh(mesh)
+    prim = _lism(C, mesh)
+    d0, st0 = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, 0.3, 150, seed=21, nthreads=4)
+    he = 0.3
+    zones = np.asarray(mesh.cell_zone)
+    al = np.where(zones <= 2, 2, 1)
+    r_th = -(ME * al * he + 4.0 * (-1.0 + he)) / (4.0 * (1.0 + ME))
+    ph = prim.copy()
+    ph[:, 4] = (r_th / 2.0) * (8.0 + (-7.0 + al + (1.0 - al) * ME) * he) / (4.0 - (4.0 + al * ME) * he)
+    scal = np.zeros((mesh.n_cells, 2))
+    scal[:, 1] = he
+    d1, st1 = O.mk_go(om, mesh.node_xyz, ph, C.V_INF, 0.102046, 0.3, 150, seed=22, nthreads=4, scal=scal)
+    assert abs(d1[:, 0].mean() - 1.0) < 0.012
+    ratio = st1["exchanges"] / st0["exchanges"]
+    assert abs(ratio - r_th.mean() * (1.0 + ME)) < 0.02, ratio
+
+
+def test_the_marker_routes_the_exchanged_sort(C, O):
+    """P.sort = determ_zone(P.cel, 0) (Setka_MK.cpp:2088, 2121; Setka_physics.cpp:1427-1482): with
+    the scalars carried, Q/rho < 50 gives sort 1 in type-1 cells and 2 elsewhere, Q/rho >= 50 gives
+    4 in type-4 cells and 3 elsewhere; without them the cell's type is the sort."""
+    mesh, _, _ = _cube(C, n=4)
+    mesh.cell_zone = (np.arange(mesh.n_cells) % 4 + 1).astype(np.int8)  # every type
+    om = O.OracleMesh(mesh)
+    prim = _lism(C, mesh)
+    scal = np.zeros((mesh.n_cells, 2))
+    d_lo, _ = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, 0.5, 60, seed=4, scal=scal)
+    scal[:, 0] = 60.0 * prim[:, 0]
+    d_hi, _ = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, 0.5, 60, seed=4, scal=scal)
+    d_no, _ = O.mk_go(om, mesh.node_xyz, prim, C.V_INF, 0.102046, 0.5, 60, seed=4)
+    # columns: total, then sorts 1..4 (sort 4 is also the birth sort)
+    assert d_lo[:, 1].sum() > 0 and d_lo[:, 2].sum() > 0 and d_lo[:, 3].sum() == 0
+    assert d_hi[:, 1].sum() == 0 and d_hi[:, 2].sum() == 0 and d_hi[:, 3].sum() > 0
+    assert (d_no[:, 1:4].sum(axis=0) > 0).all()
+    # the same flights either way: only the labels move
+    np.testing.assert_allclose(d_lo[:, 0], d_hi[:, 0], rtol=1e-12)
+    np.testing.assert_allclose(d_lo[:, 0], d_no[:, 0], rtol=1e-12)

@@ -179,6 +179,8 @@ struct MkDev {
     const int8_t *in_side;      // [n_in] -1: the boundary Maxwellian; 0 / 1: the distribution's side
     const int64_t *in_cdf;      // [n_in] offset of the injection's running sums in cdf, -1: none
     const double *cdf;          // running sums of the incoming distributions' bins
+    const double *scal;         // [Nc*kScal] Q, rho_He of the flown layer (null: rho_He = 0), for the
+                                // thermal-proton split of Plasma_components_1
 };
 
 // Failures of a step, ranked in the order the reference meets them -- the surface fit before the

@@ -27,6 +27,7 @@
 // wavefront then runs births, crossings and exchanges side by side.
 #define TITAN_FLUX_ONLY 1
 #include "step_kernels.h"
+#include "bera.h"
 #include "launch.h"
 
 namespace titan {
@@ -531,7 +532,13 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
             if (!found) break;
             n_seg++;
             const double *s = prim + 8 * c;
-            double rho_Th = s[0] / (1.0 + me), p_Th = s[4] / 2.0;
+            // the thermal protons: Plasma_components_1 (Phys_param.cpp:641-660) at the cell's type,
+            // as the flight calls it (Setka_MK.cpp:1944-1966)
+            const double rho_He = K.scal ? K.scal[kScal * c + 1] : 0.0;
+            const int al = ((M.cell_code[c] >> 8) & 15u) <= 2u ? 2 : 1;
+            double rho_Th = -(me * al * rho_He + 4.0 * (-s[0] + rho_He)) / (4.0 * (1.0 + me));
+            double p_Th = s[4] * (4.0 * s[0] - (4.0 + al * me) * rho_He) /
+                          (8.0 * s[0] + (-7.0 + al + (1.0 - al) * me) * rho_He);
             if (rho_Th <= 1e-8) rho_Th = 1e-8;
             if (p_Th <= 1e-8 / 2.0) p_Th = 1e-8 / 2.0;
             const double ro = rho_Th, cp = sqrt(2.0 * p_Th / rho_Th);
@@ -618,7 +625,11 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
             mk_change_velosity(rs, K.a2, U, Vs, W, cp, capped);
             for (int i = 0; i < 3; i++) W[i] *= cp;
             mk_from_sph(F, W[0], W[1], W[2], V);
+            // P.sort = determ_zone(P.cel, 0) (Setka_MK.cpp:2088, 2121): Q routes it when the scalars
+            // are carried; without them the cell's type stands in (what Q gives where the marker
+            // follows the types, Q/rho >= 50 exactly past the HP)
             sort = (int)((M.cell_code[c] >> 8) & 15u);
+            if (K.scal) sort = bera_zone(K.scal[kScal * c], prim[8 * c], sort);
             slot = (sort >= 1 && sort <= 4) ? sort : 0;
             KSI = -log(1.0 - mk_rand(rs));
             n_ex++;

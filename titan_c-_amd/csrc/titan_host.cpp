@@ -1258,6 +1258,7 @@ static int mk_flights(titan_ctx *c, int layer, const MkInj &inj, int zone, const
     K.out = out;
     K.stats = dst;
     K.pack = pack;
+    K.scal = c->d_scal[layer];  // null unless the passive scalars are carried
     if (zone > 0) {
         K.zone = zone;
         K.r = c->mkz_r;
