@@ -329,6 +329,33 @@ int titan_cx_exact_batch(int64_t n, const double *in10, int nodes, double *out5)
  * keeps until titan_destroy; calls on one context are serialised on its stream. */
 int titan_mk_go(titan_ctx *ctx, int layer, int32_t n_per_face, uint64_t seed, int32_t max_steps,
                 double *density, int64_t *stats);
+
+/* The pickup-ion source tallies of MK_source_S for the following titan_mk_go / titan_mk_go_zone calls
+ * (Cell::Init_S, Cell::MK_Add_pui_source, Cell.cpp:77-87, 1087-1133, called at Setka_MK.cpp:2013,
+ * 2098): each flight segment adds mu t to S-[cell][iw] and mu t nu_ex to S+[cell][iw][k-1], where
+ * iw = int(u / w_r * n_w) clamped to 0..n_w-1 bins the relative speed u = |V - V_p| of the atom and
+ * the cell's plasma, nu_ex is the segment's exchange rate, and k = arise[(zone-1)*4 + sort-1] is
+ * the pickup-ion fluid the exchange feeds (0: none) at the zone of Setka::determ_zone (Q routes it
+ * when the passive scalars are carried, else the cell's type stands in).  arise [16] (nullable):
+ * values 0..2; NULL takes proton_arise_1..4 of the run with pickup ions, parent 0
+ * (Phys_param.cpp:93-126).  After the flights the library applies Cell::MK_normir_Moments and
+ * Cell::MK_calc_Sm (Cell.cpp:1135-1169, 1225-1241): S- becomes the loss rate at each bin's centre
+ * speed, S+ a density per unit cell volume and unit velocity-space volume.  n_w = 0 turns the
+ * tallies off; n_w <= 4096.  The arrays take 24 n_w bytes per cell on the device. */
+int titan_set_mk_source_s(titan_ctx *ctx, int32_t n_w, double w_r, const int8_t *arise);
+/* The last Monte-Carlo call's S- [n_cells*n_w] and S+ [n_cells*n_w*2] (per cell the column-major
+ * (2, n_w) matrix of Cell::pui_Sp), reference cell order.  TITAN_E_ARG before such a call. */
+int titan_mk_get_s(titan_ctx *ctx, double *sm, double *sp);
+/* Cell::MK_normir_Moments + Cell::MK_calc_Sm on n cells' raw tallies (host arrays, same layout):
+ * vol [n], par_a_2, par_Kn. */
+int titan_mk_s_finish_batch(int64_t n, int32_t n_w, double w_r, double a2, double kn, const double *vol,
+                            const double *sm, const double *sp, double *sm_out, double *sp_out);
+/* One cell's file of Cell::write_S_ToFile / Cell::read_S_FromFile (Cell.cpp:89-125, 374-420;
+ * "data_SpSm/func_cells_SpSm_<number>.bin"): int32 2, n_w, n_w, then S- [n_w] and S+ [n_w*2].
+ * The reader returns TITAN_E_ARG for other sizes (the reference exits), TITAN_E_IO for a missing
+ * or truncated file. */
+int titan_mk_s_write(const char *path, int32_t n_w, const double *sm, const double *sp);
+int titan_mk_s_read(const char *path, int32_t n_w, double *sm, double *sp);
 /* Setka::Set_MK_Zone (Setka_MK.cpp:142-683) from the centres of node layer 0: cell_mk [n_cells]
  * (nullable) = the MK zone 1..7 of each cell (Zone_1 -> 1; Zone_2, Zone_3, Zone_4 -> 2 / 3, 4 / 5,
  * 6 / 7 by centre x > 0 / x <= 0), face_mask [n_faces] (nullable) bit z-1 = the face is on zone z's

@@ -81,6 +81,7 @@ def lib():
             "or_mk_change_velosity_batch": (None, [_I64, ctypes.c_uint64, _D, _P, _P, _D, _I, _P, _P]),
             "or_set_mk_zone": (_I, [_P, _P, _I, _P, _P]),
             "or_plasma_components_1": (None, [_D, _D, _D, _I, _P]),
+            "or_mk_pui_finish": (None, [ctypes.c_int64, ctypes.c_int32, _D, _D, _D, _P, _P, _P, _P, _P]),
             "or_mk_setup_zone": (_I64, [_P, _P, _D, ctypes.c_int32, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P, _P]),
             "or_mk_deposit": (None, [_P, ctypes.c_int32, _I, _I, _P, _P, _D]),
             "or_mk_sample_batch": (None, [_P, ctypes.c_int32, _I, _I, _P, _I64, ctypes.c_uint64, _P]),
@@ -759,7 +760,38 @@ class OrMk(ctypes.Structure):
     _fields_ = [("n_in", _I64), ("face", _P), ("count", _P), ("mu", _P), ("seed", ctypes.c_uint64),
                 ("v_inf", _D), ("a2", _D), ("kn", _D), ("max_steps", ctypes.c_int32), ("mistake", ctypes.c_int32),
                 ("nthreads", ctypes.c_int32), ("R0", _D), ("zone", ctypes.c_int32), ("pad_", ctypes.c_int32),
-                ("cell_mk", _P), ("face_mask", _P), ("in_sort", _P), ("in_side", _P), ("dist", _P), ("scal", _P)]
+                ("cell_mk", _P), ("face_mask", _P), ("in_sort", _P), ("in_side", _P), ("dist", _P), ("scal", _P),
+                ("pui_nw", ctypes.c_int32), ("pad3_", ctypes.c_int32), ("pui_wr", _D), ("pui_arise", _P),
+                ("pui_sm", _P), ("pui_sp", _P)]
+
+
+# proton_arise_1..4 of the run with pickup ions (Phys_param.cpp:93-126), parent 0 (a thermal proton),
+# rows sort 1..4: the pickup-ion fluid (1, 2; 0: none) an exchange of that atom feeds, zone by zone
+PUI_ARISE = np.array([[0, 1, 1, 1], [2, 0, 2, 2], [1, 1, 0, 1], [1, 1, 1, 0]], np.int8)
+
+
+def _pui(mk, m, pui):
+    """Attaches the pickup-ion tallies of pui = dict(nw, wr, arise=PUI_ARISE) to mk; returns the
+    raw arrays (sm [n_cells, nw], sp [n_cells, nw, 2]) or None."""
+    if not pui:
+        return None
+    nw = int(pui["nw"])
+    sm = np.zeros((m.n_cells, nw))
+    sp = np.zeros((m.n_cells, nw, 2))
+    ar = np.ascontiguousarray(pui.get("arise", PUI_ARISE), np.int8).reshape(16)
+    mk.pui_nw, mk.pui_wr, mk.pui_arise, mk.pui_sm, mk.pui_sp = nw, float(pui["wr"]), _p(ar), _p(sm), _p(sp)
+    return sm, sp, ar
+
+
+def mk_pui_finish(vol, sm, sp, wr, a2, kn):
+    """or_mk_pui_finish: (sm_out [n, nw], sp_out [n, nw, 2]) from raw tallies."""
+    sm = np.ascontiguousarray(sm, np.float64)
+    sp = np.ascontiguousarray(sp, np.float64)
+    n, nw = sm.shape
+    so, po = np.zeros_like(sm), np.zeros_like(sp)
+    lib().or_mk_pui_finish(n, nw, float(wr), float(a2), float(kn), _p(np.ascontiguousarray(vol, np.float64)), _p(sm),
+                           _p(sp), _p(so), _p(po))
+    return so, po
 
 
 class OrMkDist(ctypes.Structure):
@@ -809,10 +841,11 @@ def mk_setup(om, geom, v_inf, n_per_face):
 
 
 def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000, geom=None, mistake=0, nthreads=1,
-          scal=None):
+          scal=None, pui=None):
     """or_mk_go on the nodes xyz and state prim (geom: measure(om, xyz), computed when None;
     mistake: or_mk.mistake, nonzero only in tests that show the pins catch an error; scal: the
-    passive scalars [n_cells, 2] = Q, rho_He, or None for rho_He = 0);
+    passive scalars [n_cells, 2] = Q, rho_He, or None for rho_He = 0; pui: dict(nw, wr, arise) for
+    the pickup-ion tallies, which then come back as a third item, the raw (sm, sp));
     returns (densities [n_cells, 5] = sum over sorts, sorts 1..4, per unit volume; stats dict)."""
     m = om.m
     x = np.ascontiguousarray(xyz, np.float64)
@@ -823,11 +856,13 @@ def mk_go(om, xyz, prim, v_inf, a2, kn, n_per_face, seed=1, max_steps=100000, ge
               int(mistake), int(nthreads), inner_radius(m, x))
     sc = None if scal is None else np.ascontiguousarray(scal, np.float64).reshape(m.n_cells, 2)
     mk.scal = _p(sc)
+    pt = _pui(mk, m, pui)
     tally = np.zeros((m.n_cells, 5))
     stats = np.zeros(8, np.int64)
     lib().or_mk_go(ctypes.byref(om.c), ctypes.byref(mk), _p(x), _p(np.ascontiguousarray(prim, np.float64)),
                    ctypes.byref(g), _p(tally), _p(stats))
-    return tally / geom["cell_volume"][:, None], dict(zip(MK_STATS[:8], (int(v) for v in stats)))
+    out = (tally / geom["cell_volume"][:, None], dict(zip(MK_STATS[:8], (int(v) for v in stats))))
+    return out if pt is None else out + (pt[:2],)
 
 
 def plasma_components_1(rho, p, rho_He, zone):
@@ -863,7 +898,7 @@ def mk_setup_zone(om, geom, v_inf, n_per_face, zone, cell_mk, mask, dist):
 
 
 def mk_go_zone(om, xyz, prim, v_inf, a2, kn, n_per_face, zone, cell_mk, mask, dist, seed=1, max_steps=100000,
-               geom=None, nthreads=1, scal=None):
+               geom=None, nthreads=1, scal=None, pui=None):
     """One pass of MK_go(zone) (or_mk_setup_zone + or_mk_go): the densities of the zone's cells per
     unit volume ([n_cells, 5], zero elsewhere), the stats dict, and the injections; deposits into
     dist, whose outgoing distributions of the zone's faces are cleared first (MK_go's Fill_null)."""
@@ -883,13 +918,15 @@ def mk_go_zone(om, xyz, prim, v_inf, a2, kn, n_per_face, zone, cell_mk, mask, di
               ctypes.cast(ctypes.pointer(dist.c), ctypes.c_void_p))
     sc = None if scal is None else np.ascontiguousarray(scal, np.float64).reshape(m.n_cells, 2)
     mk.scal = _p(sc)
+    pt = _pui(mk, m, pui)
     tally = np.zeros((m.n_cells, 5))
     stats = np.zeros(9, np.int64)
     lib().or_mk_go(ctypes.byref(om.c), ctypes.byref(mk), _p(x), _p(np.ascontiguousarray(prim, np.float64)),
                    ctypes.byref(g), _p(tally), _p(stats))
     dens = tally / geom["cell_volume"][:, None]
     dens[cm != zone] = 0.0
-    return dens, dict(zip(MK_STATS, (int(v) for v in stats))), (face, sort, side, count, mu)
+    out = (dens, dict(zip(MK_STATS, (int(v) for v in stats))), (face, sort, side, count, mu))
+    return out if pt is None else out + (pt[:2],)
 
 
 def mk_deposit(dist, slot, sort, side, n, V, mu):

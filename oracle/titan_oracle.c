@@ -2405,6 +2405,62 @@ int64_t or_mk_setup_zone(const or_mesh *m, const or_geom *g, double v_inf, int32
     return k;
 }
 
+/* Cell::MK_Add_pui_source (Cell.cpp:1087-1133) with parent 0 (a thermal proton), at the zone of
+ * Setka::determ_zone(P.cel, 0) (Setka_MK.cpp:2010, 2088) */
+static void mk_pui_add(const or_mk *mk, int64_t c, int zone, int sort, double u, double nu_ex, double mu, double t)
+{
+    const int nw = mk->pui_nw;
+    const double xi = u / mk->pui_wr * nw;
+    int iw = xi >= (double)nw ? nw - 1 : (xi > 0.0 ? (int)xi : 0);
+    if (iw >= nw) iw = nw - 1;
+#pragma omp atomic
+    mk->pui_sm[c * nw + iw] += mu * t;
+    if (zone < 1 || zone > 4 || sort < 1 || sort > 4) return;
+    const int k = mk->pui_arise[(zone - 1) * 4 + (sort - 1)];
+    if (k == 0) return;
+#pragma omp atomic
+    mk->pui_sp[(c * nw + iw) * 2 + (k - 1)] += mu * t * nu_ex;
+}
+
+void or_mk_pui_finish(int64_t n, int32_t nw, double wr, double a2, double kn, const double *vol, const double *sm,
+                      const double *sp, double *sm_out, double *sp_out)
+{
+    const double pi = 3.14159265358979323846;
+    const double dthe = pi / 40.0;
+    double *ff = (double *)malloc(sizeof(double) * (size_t)nw);
+    for (int64_t c = 0; c < n; c++) {
+        /* MK_normir_Moments */
+        for (int ij = 0; ij < nw; ij++) {
+            const double w1 = ij * wr / nw, w2 = (ij + 1) * wr / nw;
+            ff[ij] = sm[c * nw + ij] / vol[c];
+            for (int ki = 0; ki < 2; ki++)
+                sp_out[(c * nw + ij) * 2 + ki] =
+                    sp[(c * nw + ij) * 2 + ki] / (vol[c] * 4.0 * pi * (1.0 / 3.0) * (w2 * w2 * w2 - w1 * w1 * w1));
+        }
+        /* MK_calc_Sm */
+        for (int ij = 0; ij < nw; ij++) {
+            const double w = ((ij + 0.5) * wr / nw);
+            double acc = 0.0;
+            for (int j = 0; j < nw; j++) {
+                const double Vh = ((j + 0.5) * wr / nw);
+                const double f = ff[j];
+                if (f <= 0.0) continue;
+                for (int k = 1; k <= 40; k++) {
+                    const double the = dthe * k;
+                    const double d = sqrt(Vh * Vh + w * w - 2.0 * w * Vh * cos(the));
+                    if (d > 0.000000001) {
+                        const double sg = (1.0 - a2 * log(d)) * (1.0 - a2 * log(d));
+                        acc += f * d * sg * sin(the) * dthe * 2.0 * pi;
+                    }
+                }
+            }
+            acc = acc / (4.0 * pi);
+            sm_out[c * nw + ij] = acc / kn;
+        }
+    }
+    free(ff);
+}
+
 /* Phys_param::Plasma_components_1 (Phys_param.cpp:628-665): the thermal protons of a plasma cell
  * holding helium, th = (rho_Th, p_Th, T_Th); al = 2 electrons per helium ion for zone <= 2, else 1.
  * Pinned by mass (rho = rho_Th (1 + me) + rho_He (1 + al me / 4)), by p_Th = rho_Th T_Th / 2, and by
@@ -2420,7 +2476,6 @@ void or_plasma_components_1(double rho, double p, double rho_He, int zone, doubl
 int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *prim, const or_geom *g,
              double *tally, int64_t *stats)
 {
-    const double me = 1.0 / 1836.15;
     for (int i = 0; i < 8; i++) stats[i] = 0;
     /* particles are numbered face by face; first[q] = id of face q's first particle */
     int64_t *first = (int64_t *)malloc(sizeof(int64_t) * (size_t)(mk->n_in + 1));
@@ -2521,6 +2576,9 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
 #pragma omp atomic
                         tally[5 * c + sort] += time * mu;
                     }
+                    if (mk->pui_nw > 0)
+                        mk_pui_add(mk, c, mk->scal ? or_determ_zone(mk->scal[2 * c], s[0], m->cell_zone[c]) : m->cell_zone[c],
+                                   sort, (mk->mistake & 8) ? vn : u, nu_ex, mu, time);
                     for (int i = 0; i < 3; i++) P[i] += 1.000001 * time * V[i];
                     int64_t nc = m->face_cells[2 * ex] == c ? m->face_cells[2 * ex + 1] : m->face_cells[2 * ex];
                     if (nc < 0 && m->face_bc[ex] != OR_BC_INNER_HARD) { /* out through the outer boundary */
@@ -2572,6 +2630,9 @@ int or_mk_go(const or_mesh *m, const or_mk *mk, const double *x, const double *p
 #pragma omp atomic
                         tally[5 * c + sort] += t_ex * mu;
                     }
+                    if (mk->pui_nw > 0)
+                        mk_pui_add(mk, c, mk->scal ? or_determ_zone(mk->scal[2 * c], s[0], m->cell_zone[c]) : m->cell_zone[c],
+                                   sort, (mk->mistake & 8) ? vn : u, nu_ex, mu, t_ex);
                     double U[3], Vs[3], W[3];
                     double Vp[3] = {s[1], s[2], s[3]};
                     mk_to_sph(P, Vp, &U[0], &U[1], &U[2]);      /* (r, theta, phi) */

@@ -347,7 +347,8 @@ typedef struct {
     int32_t max_steps;
     int32_t mistake;  /* 0; for tests that show the pins catch a plausible error: bit 0 drops the
                          |V.n| weight of the birth velocity, bit 1 the g sigma(g) weight of the
-                         exchanged velocity, bit 2 absorbs the particles at the inner sphere */
+                         exchanged velocity, bit 2 absorbs the particles at the inner sphere,
+                         bit 3 bins the pickup-ion tallies by |V| instead of |V - V_p| */
     int32_t nthreads; /* OpenMP threads (<= 1: one) */
     double R0;        /* radius of the inner (Sun's) sphere the particles pass through (Geo_param::R0;
                          0: no sphere) */
@@ -364,6 +365,16 @@ typedef struct {
     const double *scal;        /* [n_cells*2] Q, rho_He per cell, for the thermal-proton split of
                                   Plasma_components_1 at the cell's type (Setka_MK.cpp:1944-1966);
                                   NULL: rho_He = 0 */
+    /* the pickup-ion source tallies of MK_source_S (Cell::MK_Add_pui_source, Cell.cpp:1087-1133),
+     * one per flight segment at the relative speed u = |V - V_p|, bin iw = int(u / pui_wr * pui_nw)
+     * clamped to the grid: pui_sm [n_cells][nw] += mu time; pui_sp [n_cells][nw][2] (row k-1 of the
+     * column-major (2, nw) matrix) += mu time nu_ex, k = pui_arise[(zone-1)*4 + sort-1] (0: none),
+     * zone = determ_zone of the cell (Q routes it when carried, else the cell's type).  pui_nw = 0:
+     * no tallies. */
+    int32_t pui_nw, pad3_;
+    double pui_wr;
+    const int8_t *pui_arise;   /* [4*4] */
+    double *pui_sm, *pui_sp;
 } or_mk;
 /* Velocity distributions on the zone borders (AMR_f at its unrefined grid of Gran::Read_AMR,
  * Gran.cpp:16-99): (0, 20) x (-20, 20)^2 in the face's frame (Vn, get_bazis(Vn)), 3r x 6r x 6r
@@ -411,6 +422,12 @@ int64_t or_mk_setup(const or_mesh *m, const or_geom *g, double v_inf, int32_t n_
 void or_mk_change_velosity_batch(int64_t n, uint64_t seed, double a2, const double *U, const double *V, double cp,
                                  int mistake, double *W, int64_t *capped);
 /* With mk->zone > 0, stats has a ninth entry: flights that ended on the zone's border. */
+/* Cell::MK_normir_Moments (Cell.cpp:1225-1241) and Cell::MK_calc_Sm (Cell.cpp:1135-1169) of the
+ * pickup-ion tallies of n cells: sm_out[c][ij] = the loss rate at w_ij = (ij + 0.5) wr / nw from the
+ * volume-normalised sm tally, sum_j sum_{k=1..40} ff_j d sigma(d) sin(th_k) dth 2 pi / (4 pi) / kn
+ * with d = |w - Vh_j| at angle th_k = k pi / 40; sp_out = sp / (V 4 pi / 3 (w2^3 - w1^3)). */
+void or_mk_pui_finish(int64_t n, int32_t nw, double wr, double a2, double kn, const double *vol, const double *sm,
+                      const double *sp, double *sm_out, double *sp_out);
 /* Phys_param::Plasma_components_1 (Phys_param.cpp:628-665): th = (rho_Th, p_Th, T_Th) of a plasma
  * cell (rho, p, rho_He) of type zone 1..4. */
 void or_plasma_components_1(double rho, double p, double rho_He, int zone, double th[3]);

@@ -162,3 +162,25 @@ def test_mk_parameters_damage(T, tmp_path):
     with pytest.raises(T.TitanError) as e:
         T.mk_parameters_read(str(tmp_path / "none.bin"), np.zeros((6, 5)))
     assert e.value.code == T.TITAN_E_IO
+
+
+def test_pickup_source_file_known_bytes(T, tmp_path):
+    """One cell's data_SpSm file (Cell::write_S_ToFile, Cell.cpp:89-125): int rows 2, cols n_w,
+    size n_w, then pui_Sm and pui_Sp as Eigen stores the (2, n_w) matrix, column by column."""
+    path = str(tmp_path / "func_cells_SpSm_7.bin")
+    sm = np.array([1.0, 2.0, 3.0])
+    sp = np.array([[10.0, 20.0], [11.0, 21.0], [12.0, 22.0]])  # [iw][k]
+    T.mk_s_write(path, sm, sp)
+    want = struct.pack("<iii", 2, 3, 3) + struct.pack("<3d", 1, 2, 3) + struct.pack("<6d", 10, 20, 11, 21, 12, 22)
+    assert open(path, "rb").read() == want
+    a, b = T.mk_s_read(path, 3)
+    np.testing.assert_array_equal(a, sm)
+    np.testing.assert_array_equal(b, sp)
+    with pytest.raises(T.TitanError) as e:
+        T.mk_s_read(path, 4)                          # other sizes
+    assert e.value.code == T.TITAN_E_ARG
+    bad = str(tmp_path / "cut.bin")
+    open(bad, "wb").write(want[:-8])
+    with pytest.raises(T.TitanError) as e:
+        T.mk_s_read(bad, 3)
+    assert e.value.code == T.TITAN_E_IO

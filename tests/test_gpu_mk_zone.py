@@ -108,3 +108,18 @@ def test_zone_arguments(T, shell):
         d, st = s.MK_go_zone(1, 10, layer=0, density=np.ones((mesh.n_cells, 5)))
         cm = np.asarray(s.Set_MK_Zone(r=1)[0])
         assert st["particles"] == 0 and (d[cm == 1] == 0).all() and (d[cm != 1] == 1).all()
+
+
+def test_zone_pass_tallies_pickup_sources_in_its_cells(T, shell):
+    """MK_source_S in the zone form: Cell::Init_S clears the zone's cells, the flights stay in them."""
+    mesh, prim, params, om, g = shell
+    with T.Setka(mesh, params) as s:
+        s.set_state(prim, 0)
+        cm, _ = s.Set_MK_Zone(r=1)
+        s.MK_source_S(32, 8.0)
+        d, st = s.MK_go_zone(6, 100, seed=3, layer=0)
+        sm, sp = s.mk_S()
+    cm = np.asarray(cm)
+    assert st["particles"] > 0
+    assert (sm[cm != 6] == 0).all() and (sp[cm != 6] == 0).all()
+    assert sm[cm == 6].sum() > 0 and sp[cm == 6].sum() > 0

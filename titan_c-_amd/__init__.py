@@ -159,6 +159,11 @@ _SIGS = {
     "titan_mk_go_zone": (_I, [_P, _I, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, _P, _P]),
     "titan_mk_dist_get": (_I, [_P, _I64, ctypes.c_int32, ctypes.c_int32, _P]),
     "titan_mk_dist_set": (_I, [_P, _I64, ctypes.c_int32, ctypes.c_int32, _P]),
+    "titan_set_mk_source_s": (_I, [_P, ctypes.c_int32, _D, _P]),
+    "titan_mk_get_s": (_I, [_P, _P, _P]),
+    "titan_mk_s_finish_batch": (_I, [_I64, ctypes.c_int32, _D, _D, _D, _P, _P, _P, _P, _P]),
+    "titan_mk_s_write": (_I, [ctypes.c_char_p, ctypes.c_int32, _P, _P]),
+    "titan_mk_s_read": (_I, [ctypes.c_char_p, ctypes.c_int32, _P, _P]),
     "titan_cx_exact_batch": (_I, [_I64, _P, _I, _P]),
     "titan_mlp_n_params": (_I64, []),
     "titan_mlp_set": (_I, [_P, _P, _P, _P, _P]),
@@ -258,6 +263,34 @@ def checkpoint_read(path, n_cells, keys, n_nodes=-1):
                                        _ptr(xyz) if n_nodes > 0 else None, ctypes.byref(got)),
            "titan_checkpoint_read")
     return vals, (xyz if got.value > 0 else None)
+
+
+def mk_s_finish_batch(vol, sm, sp, w_r, a2, kn):
+    """Cell::MK_normir_Moments + Cell::MK_calc_Sm on the device for raw tallies sm [n, n_w],
+    sp [n, n_w, 2] and cell volumes vol [n]: (sm_out, sp_out)."""
+    sm = _c(sm, np.float64)
+    sp = _c(sp, np.float64)
+    n, nw = sm.shape
+    if sp.shape != (n, nw, 2):
+        raise ValueError("sp must be [n, n_w, 2]")
+    so, po = np.zeros_like(sm), np.zeros_like(sp)
+    _check(lib().titan_mk_s_finish_batch(n, nw, float(w_r), float(a2), float(kn), _ptr(_c(vol, np.float64)), _ptr(sm),
+                                         _ptr(sp), _ptr(so), _ptr(po)), "titan_mk_s_finish_batch")
+    return so, po
+
+
+def mk_s_write(path, sm, sp):
+    """Cell::write_S_ToFile for one cell: sm [n_w], sp [n_w, 2]."""
+    sm = _c(sm, np.float64).reshape(-1)
+    sp = _c(sp, np.float64).reshape(sm.shape[0], 2)
+    _check(lib().titan_mk_s_write(os.fsencode(path), sm.shape[0], _ptr(sm), _ptr(sp)), "titan_mk_s_write")
+
+
+def mk_s_read(path, n_w):
+    """Cell::read_S_FromFile for one cell: (sm [n_w], sp [n_w, 2])."""
+    sm, sp = np.zeros(int(n_w)), np.zeros((int(n_w), 2))
+    _check(lib().titan_mk_s_read(os.fsencode(path), int(n_w), _ptr(sm), _ptr(sp)), "titan_mk_s_read")
+    return sm, sp
 
 
 MK_PARAMETERS = ("MK_n_H", "MK_n_H1", "MK_n_H2", "MK_n_H3", "MK_n_H4")
@@ -678,6 +711,24 @@ class Setka:
         _check(lib().titan_mk_dist_set(self.ctx, int(face), int(sort), int(side), _ptr(a)), "titan_mk_dist_set",
                self.ctx)
         return a
+
+    def MK_source_S(self, n_w, w_r, arise=None):
+        """Turns on (n_w > 0) the pickup-ion source tallies S-, S+ of MK_source_S for the next
+        MK_go / MK_go_zone calls: n_w speed bins on (0, w_r) (pui_nW, pui_wR); arise [4, 4] (zone,
+        sort) -> pickup-ion fluid 0..2, None for the reference's tables with pickup ions."""
+        a = None if arise is None else np.ascontiguousarray(arise, np.int8).reshape(16)
+        _check(lib().titan_set_mk_source_s(self.ctx, int(n_w), float(w_r), _ptr(a)), "titan_set_mk_source_s",
+               self.ctx)
+        self._pui_nw = int(n_w)
+
+    def mk_S(self):
+        """The last Monte-Carlo call's (S- [n_cells, n_w], S+ [n_cells, n_w, 2]) after
+        MK_normir_Moments and MK_calc_Sm."""
+        nw = getattr(self, "_pui_nw", 0)
+        sm = np.zeros((self.n_cells, max(nw, 1)))
+        sp = np.zeros((self.n_cells, max(nw, 1), 2))
+        _check(lib().titan_mk_get_s(self.ctx, _ptr(sm), _ptr(sp)), "titan_mk_get_s", self.ctx)
+        return sm, sp
 
     def set_timing(self, on=True):
         _check(lib().titan_set_timing(self.ctx, 1 if on else 0), "titan_set_timing", self.ctx)

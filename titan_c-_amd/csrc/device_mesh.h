@@ -181,6 +181,13 @@ struct MkDev {
     const double *cdf;          // running sums of the incoming distributions' bins
     const double *scal;         // [Nc*kScal] Q, rho_He of the flown layer (null: rho_He = 0), for the
                                 // thermal-proton split of Plasma_components_1
+    // pickup-ion source tallies of MK_source_S (Cell::MK_Add_pui_source): per segment, bin iw of
+    // the relative speed u = |V - V_p| on (0, pui_wr) in pui_nw bins; pui_sm[c*nw + iw] += mu t,
+    // pui_sp[(c*nw + iw)*2 + k-1] += mu t nu_ex with k = pui_arise[(zone-1)*4 + sort-1] (0: none)
+    int32_t pui_nw;
+    int8_t pui_arise[16];
+    double pui_wr;
+    double *pui_sm, *pui_sp;
 };
 
 // Failures of a step, ranked in the order the reference meets them -- the surface fit before the

@@ -442,9 +442,77 @@ __global__ void __launch_bounds__(256) k_mk_finish(int64_t n_cells, const double
     out[5 * (int64_t)cell_ref[d] + k] = v / vol[d];
 }
 
-// ZONE: the zone form (titan_mk_go_zone); the one-zone flight is its own instantiation, so the
-// zone tests and deposits add nothing to its registers
-template <bool ZONE>
+// Cell::MK_Add_pui_source (Cell.cpp:1087-1133) with parent 0 (a thermal proton), at the zone of
+// Setka::determ_zone(P.cel, 0) (Setka_MK.cpp:2010, 2088): Q routes it when the scalars are carried,
+// else the cell's type stands in
+__device__ __forceinline__ void mk_pui_add(const MkDev &K, const DevMesh &M, const double *__restrict__ prim,
+                                           int64_t c, int sort, double u, double nu_ex, double mu, double t) {
+    const int nw = K.pui_nw;
+    const double xi = u / K.pui_wr * nw;
+    const int iw = xi >= (double)nw ? nw - 1 : (xi > 0.0 ? (int)xi : 0);
+    atomicAdd(&K.pui_sm[c * nw + iw], mu * t);
+    int zone = (int)((M.cell_code[c] >> 8) & 15u);
+    if (K.scal) zone = bera_zone(K.scal[kScal * c], prim[8 * c], zone);
+    if (zone < 1 || zone > 4 || sort < 1 || sort > 4) return;
+    const int k = K.pui_arise[(zone - 1) * 4 + (sort - 1)];
+    if (k > 0) atomicAdd(&K.pui_sp[(c * nw + iw) * 2 + (k - 1)], mu * t * nu_ex);
+}
+
+// Cell::MK_normir_Moments (Cell.cpp:1225-1241) and Cell::MK_calc_Sm (Cell.cpp:1135-1169) of the
+// pickup-ion tallies, one workgroup per cell and one thread per speed bin, in place: S- becomes the
+// loss rate at w_ij from the volume-normalised tally (sum over bins j with ff_j > 0 and the 40
+// angles th_k = k pi / 40, whose sines and cosines come in trig[0..39], trig[40..79]), S+ is divided
+// by V 4 pi / 3 (w2^3 - w1^3).  vol in device cell order.
+__global__ void __launch_bounds__(256) k_mk_pui_finish(int64_t n, int32_t nw, double wr, double a2, double kn,
+                                                       const double *__restrict__ vol, const double *__restrict__ trig,
+                                                       double *__restrict__ sm, double *__restrict__ sp) {
+    extern __shared__ double ff[];  // [nw] normalised S- of the cell, then the 80 angle values
+    double *tr = ff + nw;
+    const double pi = 3.14159265358979323846;
+    const double dthe = pi / 40.0;
+    for (int64_t c = blockIdx.x; c < n; c += gridDim.x) {
+        const double V = vol[c];
+        for (int i = threadIdx.x; i < 80; i += blockDim.x) tr[i] = trig[i];
+        bool any = false;
+        for (int ij = threadIdx.x; ij < nw; ij += blockDim.x) {
+            const double w1 = ij * wr / nw, w2 = (ij + 1) * wr / nw;
+            const double f = sm[c * nw + ij] / V;
+            ff[ij] = f;
+            any |= f > 0.0;
+            for (int ki = 0; ki < 2; ki++)
+                sp[(c * nw + ij) * 2 + ki] =
+                    sp[(c * nw + ij) * 2 + ki] / (V * 4.0 * pi * (1.0 / 3.0) * (w2 * w2 * w2 - w1 * w1 * w1));
+        }
+        any = __syncthreads_or(any);
+        for (int ij = threadIdx.x; ij < nw; ij += blockDim.x) {
+            double acc = 0.0;
+            if (any) {
+                const double w = ((ij + 0.5) * wr / nw);
+                for (int j = 0; j < nw; j++) {
+                    const double Vh = ((j + 0.5) * wr / nw);
+                    const double f = ff[j];
+                    if (f <= 0.0) continue;
+                    for (int k = 1; k <= 40; k++) {
+                        const double d = sqrt(Vh * Vh + w * w - 2.0 * w * Vh * tr[40 + k - 1]);
+                        if (d > 0.000000001) {
+                            const double lg = log(d);
+                            const double sg = (1.0 - a2 * lg) * (1.0 - a2 * lg);
+                            acc += f * d * sg * tr[k - 1] * dthe * 2.0 * pi;
+                        }
+                    }
+                }
+                acc = acc / (4.0 * pi);
+            }
+            sm[c * nw + ij] = acc / kn;
+        }
+        __syncthreads();
+    }
+}
+
+// ZONE: the zone form (titan_mk_go_zone); PUI: the pickup-ion tallies.  The one-zone flight
+// without tallies is its own instantiation, so the zone tests, deposits and tallies add nothing
+// to its registers.
+template <bool ZONE, bool PUI>
 __global__ void __launch_bounds__(kMkThreads) __attribute__((amdgpu_waves_per_eu(2)))
 k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *__restrict__ prim, DevGeom G) {
     __shared__ double lds[24 * kMkThreads];
@@ -555,6 +623,7 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
             if (I < KSI) {
                 I_do = I;
                 atomicAdd(&K.tally[5 * c + slot], time * mu);
+                if (PUI) mk_pui_add(K, M, prim, c, sort, u, nu_ex, mu, time);
                 for (int i = 0; i < 3; i++) P[i] += 1.000001 * time * V[i];
                 const int32_t nb = K.pack[c].nbr[jx];
                 if (nb < 0 && (-1 - nb) != 1) {  // out through the outer boundary
@@ -592,6 +661,7 @@ k_mk_particles(DevMesh M, MkDev K, const double *__restrict__ X, const double *_
                 I_do = 0.0;
                 for (int i = 0; i < 3; i++) P[i] += t_ex * V[i];
                 atomicAdd(&K.tally[5 * c + slot], t_ex * mu);
+                if (PUI) mk_pui_add(K, M, prim, c, sort, u, nu_ex, mu, t_ex);
                 cp_ex = cp;
                 pend = true;
             }
@@ -655,16 +725,29 @@ void launch_mk_gather(hipStream_t s, int64_t n, const int32_t *faces, const DevG
 void launch_mk_particles(hipStream_t s, const DevMesh &M, const MkDev &K, const double *X, const double *prim,
                          const DevGeom &G) {
     if (K.n_particles > 0) {
-        if (K.zone > 0)
-            hipLaunchKernelGGL(k_mk_particles<true>, dim3(grid_of(K.n_particles, kMkThreads)), dim3(kMkThreads), 0, s, M,
-                               K, X, prim, G);
+        const dim3 grid(grid_of(K.n_particles, kMkThreads)), block(kMkThreads);
+        const bool pui = K.pui_nw > 0;
+        if (K.zone > 0 && pui)
+            hipLaunchKernelGGL((k_mk_particles<true, true>), grid, block, 0, s, M, K, X, prim, G);
+        else if (K.zone > 0)
+            hipLaunchKernelGGL((k_mk_particles<true, false>), grid, block, 0, s, M, K, X, prim, G);
+        else if (pui)
+            hipLaunchKernelGGL((k_mk_particles<false, true>), grid, block, 0, s, M, K, X, prim, G);
         else
-            hipLaunchKernelGGL(k_mk_particles<false>, dim3(grid_of(K.n_particles, kMkThreads)), dim3(kMkThreads), 0, s, M,
-                               K, X, prim, G);
+            hipLaunchKernelGGL((k_mk_particles<false, false>), grid, block, 0, s, M, K, X, prim, G);
     }
     if (M.n_cells > 0)
         hipLaunchKernelGGL(k_mk_finish, dim3(grid_of(5 * M.n_cells, 256)), dim3(256), 0, s, M.n_cells,
                            (const double *)G.cell_volume, M.cell_ref, (const double *)K.tally, K.out);
+}
+
+void launch_mk_pui_finish(hipStream_t s, int64_t n, int32_t nw, double wr, double a2, double kn, const double *vol,
+                          const double *trig, double *sm, double *sp) {
+    if (n > 0 && nw > 0) {
+        const int64_t blocks = n < 65536 ? n : 65536;
+        hipLaunchKernelGGL(k_mk_pui_finish, dim3((unsigned)blocks), dim3(nw < 256 ? 64 * ((nw + 63) / 64) : 256),
+                           sizeof(double) * (nw + 80), s, n, nw, wr, a2, kn, vol, trig, sm, sp);
+    }
 }
 
 void launch_mkz_cdf(hipStream_t s, int64_t n, const int64_t *off, const double *dist, int64_t nb, double *cdf) {

@@ -110,6 +110,14 @@ struct titan_ctx {
     uint8_t *d_mkz_mask = nullptr;
     int32_t *d_mkz_slot = nullptr;
     double *d_mkz_w = nullptr;
+    // pickup-ion source tallies of MK_source_S (titan_set_mk_source_s): bins, range, arise table;
+    // S- [Nc][nw] then S+ [Nc][nw][2] (device cell order), the 40 angles' sines and cosines
+    int32_t pui_nw = 0;
+    double pui_wr = 0.0;
+    int8_t pui_arise[16] = {};
+    double *d_pui = nullptr;
+    double *d_pui_trig = nullptr;
+    bool pui_valid = false;  // the arrays hold the last Monte-Carlo call's values
     DevMesh mesh() const {
         DevMesh M;
         M.n_nodes = nn; M.n_cells = nc; M.n_faces = nf;
@@ -496,7 +504,7 @@ extern "C" int titan_destroy(titan_ctx *c) {
                     c->d_t_nodes, c->d_t_own,
                     c->d_t_face_off, c->d_t_face_ref, c->d_t_faces, c->d_t_cell_lnodes, c->d_t_cell_lfaces,
                     c->d_vol, c->d_face_tvd, c->d_t_face_tvd, c->d_mk, c->d_mkz_cell, c->d_mkz_mask,
-                    c->d_mkz_slot, c->d_mkz_w};
+                    c->d_mkz_slot, c->d_mkz_w, c->d_pui, c->d_pui_trig};
     for (void *q : ptrs)
         if (q) hipFree(q);
     mlp_free(c->mlp);
@@ -1259,6 +1267,29 @@ static int mk_flights(titan_ctx *c, int layer, const MkInj &inj, int zone, const
     K.stats = dst;
     K.pack = pack;
     K.scal = c->d_scal[layer];  // null unless the passive scalars are carried
+    c->pui_valid = false;
+    if (c->pui_nw > 0) {  // MK_source_S: the tallies start from zero (Cell::Init_S)
+        const size_t cnt = (size_t)3 * c->pui_nw * nc;
+        if (!c->d_pui) HIPCHK(hipMalloc((void **)&c->d_pui, sizeof(double) * cnt));
+        if (!c->d_pui_trig) {
+            // the 40 angles of Cell::MK_calc_Sm, sin and cos as the host's libm gives them
+            double tr[80];
+            const double dthe = 3.14159265358979323846 / 40.0;
+            for (int k = 1; k <= 40; k++) {
+                const double the = dthe * k;
+                tr[k - 1] = std::sin(the);
+                tr[40 + k - 1] = std::cos(the);
+            }
+            HIPCHK(hipMalloc((void **)&c->d_pui_trig, sizeof(tr)));
+            HIPCHK(hipMemcpyAsync(c->d_pui_trig, tr, sizeof(tr), hipMemcpyHostToDevice, c->stream));
+        }
+        HIPCHK(hipMemsetAsync(c->d_pui, 0, sizeof(double) * cnt, c->stream));
+        K.pui_nw = c->pui_nw;
+        K.pui_wr = c->pui_wr;
+        for (int i = 0; i < 16; i++) K.pui_arise[i] = c->pui_arise[i];
+        K.pui_sm = c->d_pui;
+        K.pui_sp = c->d_pui + (size_t)c->pui_nw * nc;
+    }
     if (zone > 0) {
         K.zone = zone;
         K.r = c->mkz_r;
@@ -1273,6 +1304,9 @@ static int mk_flights(titan_ctx *c, int layer, const MkInj &inj, int zone, const
     }
     launch_mk_pack(c->stream, c->mesh(), c->d_X[layer], pack);
     launch_mk_particles(c->stream, c->mesh(), K, c->d_X[layer], c->d_prim[layer], c->geom[layer]);
+    if (K.pui_nw > 0)
+        launch_mk_pui_finish(c->stream, nc, K.pui_nw, K.pui_wr, c->params.par_a_2, c->params.par_Kn,
+                             c->geom[layer].cell_volume, c->d_pui_trig, K.pui_sm, K.pui_sp);
     HIPCHK(hipGetLastError());
     dens.resize(5 * nc);
     unsigned long long h[9];
@@ -1280,6 +1314,71 @@ static int mk_flights(titan_ctx *c, int layer, const MkInj &inj, int zone, const
     HIPCHK(hipMemcpyAsync(h, dst, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int i = 0; i < 9; i++) st[i] = (int64_t)h[i];
+    c->pui_valid = K.pui_nw > 0;
+    return TITAN_OK;
+}
+
+constexpr int32_t kMaxPuiBins = 4096;
+
+extern "C" int titan_set_mk_source_s(titan_ctx *c, int32_t n_w, double w_r, const int8_t *arise) {
+    if (!c || n_w < 0 || n_w > kMaxPuiBins || (n_w > 0 && !(w_r > 0.0 && std::isfinite(w_r)))) return TITAN_E_ARG;
+    // proton_arise_1..4 of the run with pickup ions (Phys_param.cpp:93-126), parent 0, sorts 1..4
+    static const int8_t kArise[16] = {0, 1, 1, 1, 2, 0, 2, 2, 1, 1, 0, 1, 1, 1, 1, 0};
+    const int8_t *a = arise ? arise : kArise;
+    for (int i = 0; i < 16; i++)
+        if (a[i] < 0 || a[i] > 2) return TITAN_E_ARG;
+    if (n_w != c->pui_nw && c->d_pui) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(c->d_pui));
+        c->d_pui = nullptr;
+    }
+    c->pui_nw = n_w;
+    c->pui_wr = w_r;
+    for (int i = 0; i < 16; i++) c->pui_arise[i] = a[i];
+    c->pui_valid = false;
+    return TITAN_OK;
+}
+
+extern "C" int titan_mk_get_s(titan_ctx *c, double *sm, double *sp) {
+    if (!c || !sm || !sp || c->pui_nw <= 0) return TITAN_E_ARG;
+    if (!c->pui_valid) return TITAN_E_ARG;  // no Monte-Carlo call with the tallies on yet
+    const int64_t nc = c->nc, nw = c->pui_nw;
+    std::vector<double> h((size_t)3 * nw * nc);
+    HIPCHK(hipMemcpyAsync(h.data(), c->d_pui, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double *hs = h.data(), *hp = h.data() + nw * nc;
+    for (int64_t d = 0; d < nc; d++) {
+        const int64_t r = c->ord.cell_d2r[d];
+        std::memcpy(sm + r * nw, hs + d * nw, sizeof(double) * nw);
+        std::memcpy(sp + r * nw * 2, hp + d * nw * 2, sizeof(double) * nw * 2);
+    }
+    return TITAN_OK;
+}
+
+extern "C" int titan_mk_s_finish_batch(int64_t n, int32_t n_w, double w_r, double a2, double kn, const double *vol,
+                                       const double *sm, const double *sp, double *sm_out, double *sp_out) {
+    if (n < 0 || n_w < 1 || n_w > kMaxPuiBins || !(w_r > 0.0) || !vol || !sm || !sp || !sm_out || !sp_out)
+        return TITAN_E_ARG;
+    if (n == 0) return TITAN_OK;
+    double tr[80];
+    const double dthe = 3.14159265358979323846 / 40.0;
+    for (int k = 1; k <= 40; k++) {
+        const double the = dthe * k;
+        tr[k - 1] = std::sin(the);
+        tr[40 + k - 1] = std::cos(the);
+    }
+    DevBuf dv, dsm, dsp, dtr;
+    int rc = dup(dv, vol, sizeof(double) * n);
+    if (rc == TITAN_OK) rc = dup(dsm, sm, sizeof(double) * n * n_w);
+    if (rc == TITAN_OK) rc = dup(dsp, sp, sizeof(double) * n * n_w * 2);
+    if (rc == TITAN_OK) rc = dup(dtr, tr, sizeof(tr));
+    if (rc != TITAN_OK) return rc;
+    launch_mk_pui_finish(nullptr, n, n_w, w_r, a2, kn, (const double *)dv.p, (const double *)dtr.p, (double *)dsm.p,
+                         (double *)dsp.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(sm_out, dsm.p, sizeof(double) * n * n_w, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sp_out, dsp.p, sizeof(double) * n * n_w * 2, hipMemcpyDeviceToHost));
     return TITAN_OK;
 }
 

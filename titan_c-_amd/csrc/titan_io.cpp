@@ -250,3 +250,31 @@ extern "C" int titan_mk_parameters_read(const char *path, int64_t n_cells, int n
     if (keys_found) *keys_found = found;
     return TITAN_OK;
 }
+
+// One cell's pickup-ion source file of Cell::write_S_ToFile / Cell::read_S_FromFile (Cell.cpp:89-125,
+// 374-420): int32 rows (2), int32 cols (n_w), int32 size (n_w), S- [n_w], then S+ as the
+// column-major (2, n_w) matrix, i.e. [n_w][2].
+extern "C" int titan_mk_s_write(const char *path, int32_t n_w, const double *sm, const double *sp) {
+    if (!path || n_w < 1 || !sm || !sp) return TITAN_E_ARG;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return TITAN_E_IO;
+    const int32_t head[3] = {2, n_w, n_w};
+    bool ok = std::fwrite(head, sizeof(int32_t), 3, f) == 3;
+    if (ok) ok = std::fwrite(sm, sizeof(double), (size_t)n_w, f) == (size_t)n_w;
+    if (ok) ok = std::fwrite(sp, sizeof(double), 2 * (size_t)n_w, f) == 2 * (size_t)n_w;
+    if (std::fclose(f) != 0) ok = false;
+    return ok ? TITAN_OK : TITAN_E_IO;
+}
+
+extern "C" int titan_mk_s_read(const char *path, int32_t n_w, double *sm, double *sp) {
+    if (!path || n_w < 1 || !sm || !sp) return TITAN_E_ARG;
+    std::vector<uint8_t> buf;
+    int rc = load_file(path, buf);
+    if (rc != TITAN_OK) return rc;
+    Reader r{buf.data(), buf.data() + buf.size()};
+    int32_t head[3];
+    if (!r.take(head, sizeof(head))) return TITAN_E_IO;
+    if (head[0] != 2 || head[1] != n_w || head[2] != n_w) return TITAN_E_ARG;  // other sizes (read_S_FromFile exits)
+    if (!r.take(sm, sizeof(double) * (size_t)n_w) || !r.take(sp, sizeof(double) * 2 * (size_t)n_w)) return TITAN_E_IO;
+    return TITAN_OK;
+}
