@@ -36,6 +36,8 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# speed bins and range of the pickup-ion source tallies timed beside the Monte-Carlo call
+PUI_BINS, PUI_RANGE = 50, 20.0
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
@@ -405,9 +407,17 @@ def main():
             t0 = time.perf_counter()
             _, st = s.MK_go(a.mk_per_face, seed=1)
             sec = time.perf_counter() - t0
+            # the same call with the pickup-ion source tallies (MK_source_S) and their finishing pass
+            s.MK_source_S(PUI_BINS, PUI_RANGE)
+            t0 = time.perf_counter()
+            s.MK_go(a.mk_per_face, seed=1)
+            psec = time.perf_counter() - t0
+            s.MK_source_S(0, 0.0)
         mk_info = {"workload": "shell4m_fitted", "n_cells": fmesh.n_cells, "n_per_face": a.mk_per_face,
                    "stats": st, "seconds": sec, "particles_per_s": st["particles"] / sec,
-                   "segments_per_s": st["segments"] / sec}
+                   "segments_per_s": st["segments"] / sec,
+                   "with_pickup_sources": {"n_w": PUI_BINS, "w_r": PUI_RANGE, "seconds": psec}}
+        log("MC neutrals with the pickup-ion tallies (%d bins): %.3f s" % (PUI_BINS, psec))
         log("MC neutrals: %d particles, %d segments, %d exchanges in %.3f s (%.3g segments/s)" % (
             st["particles"], st["segments"], st["exchanges"], sec, st["segments"] / sec))
         if not a.no_cpu:

@@ -116,7 +116,7 @@ def test_zone_pass_tallies_pickup_sources_in_its_cells(T, shell):
     with T.Setka(mesh, params) as s:
         s.set_state(prim, 0)
         cm, _ = s.Set_MK_Zone(r=1)
-        s.MK_source_S(32, 8.0)
+        s.MK_source_S(32, 8.0, np.ones((4, 4), np.int8))  # the reference's tables give zone 4's sort 4 nothing
         d, st = s.MK_go_zone(6, 100, seed=3, layer=0)
         sm, sp = s.mk_S()
     cm = np.asarray(cm)
