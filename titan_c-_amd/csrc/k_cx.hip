@@ -66,21 +66,108 @@ __global__ void __launch_bounds__(256) k_cx_list(const int32_t *__restrict__ lis
 }
 
 // Cells that the surrogate step kernel left outside the training box (list and stash written by
-// k_step_tile<*, true>, count in S->n_fallback): exact FP64 source, one workgroup per listed cell
-// (quadrature spread over the workgroup, fixed-order reduction), then the cell update redone from
-// the stashed flux sum and volumes.  prim1 is the step's unchanged input layer.
+// k_step_tile<*, true>, count in S->n_fallback) get the exact FP64 source and their update redone
+// from the stashed flux sum and volumes; prim1 is the step's unchanged input layer.  The first
+// kFixPar listed cells spread their quadrature over kFixSlices workgroups each (k_cx_fix_part:
+// one slice of the N1 x N2 points per workgroup, thread-strided, fixed-order tree reduction into
+// part), and k_cx_fix_finish sums the slices in order and updates them, one thread per cell; the
+// cells past kFixPar take one workgroup each (k_cx_fixup from `first`).
+__global__ void __launch_bounds__(256) k_cx_fix_part(const double *__restrict__ prim1,
+                                                     const double *__restrict__ neutrals, int nodes,
+                                                     const int32_t *__restrict__ list, int64_t cap,
+                                                     const StepScalars *S, double *__restrict__ part) {
+    extern __shared__ double sm[];
+    double *sc = sm, *red = sm + 2 * nodes;
+    const int64_t n = min(min((int64_t)S->n_fallback, cap), (int64_t)kFixPar);
+    const int64_t items = n * kFixSlices;
+    if ((int64_t)blockIdx.x >= items) return;  // whole workgroup
+    cx_fill_angles(sc, nodes);
+    __syncthreads();
+    const int np = nodes * nodes;
+    const int t = threadIdx.x, nt = blockDim.x;
+    for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const int64_t i = it / kFixSlices;
+        const int sl = (int)(it - i * kFixSlices);
+        const int64_t c = list[i];
+        const double *s = prim1 + 8 * c;
+        const double *h = neutrals + 5 * c;
+        const double vp[3] = {s[1], s[2], s[3]}, vh[3] = {h[2], h[3], h[4]};
+        const CxFrame f = cx_frame(s[4] / s[0], vp, h[1], vh, nodes, nodes);
+        const int p0 = (int)((int64_t)np * sl / kFixSlices), p1 = (int)((int64_t)np * (sl + 1) / kFixSlices);
+        double J0 = 0.0, J1 = 0.0, J2 = 0.0;
+        for (int p = p0 + t; p < p1; p += nt) {
+            const int ii = p / nodes, jj = p - ii * nodes;
+            double R0;
+            const double wr = cx_radial(f, ii, R0);
+            double gs, gpar, g2;
+            cx_point(f, R0, jj, sc, gs, gpar, g2);
+            const double w = wr * gs;
+            J0 += w;
+            J1 += w * gpar;
+            J2 += w * g2;
+        }
+        red[t] = J0; red[nt + t] = J1; red[2 * nt + t] = J2;
+        __syncthreads();
+        for (int hh = nt / 2; hh > 0; hh >>= 1) {
+            if (t < hh) {
+                red[t] += red[t + hh];
+                red[nt + t] += red[nt + t + hh];
+                red[2 * nt + t] += red[2 * nt + t + hh];
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            part[3 * it] = red[0];
+            part[3 * it + 1] = red[nt];
+            part[3 * it + 2] = red[2 * nt];
+        }
+        __syncthreads();  // red is reused by the next item
+    }
+}
+
+__global__ void __launch_bounds__(64) k_cx_fix_finish(DevMesh M, const double *__restrict__ prim1,
+                                                      const double *__restrict__ neutrals, int nodes,
+                                                      const int32_t *__restrict__ list,
+                                                      const double *__restrict__ stash, int64_t cap, StepScalars *S,
+                                                      const double *__restrict__ part, double *__restrict__ prim2) {
+    const int64_t n = min(min((int64_t)S->n_fallback, cap), (int64_t)kFixPar);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t c = list[i];
+    const double *s = prim1 + 8 * c;
+    const double *h = neutrals + 5 * c;
+    const double vp[3] = {s[1], s[2], s[3]}, vh[3] = {h[2], h[3], h[4]};
+    const double cp2 = s[4] / s[0];
+    const CxFrame f = cx_frame(cp2, vp, h[1], vh, nodes, nodes);
+    double J0 = 0.0, J1 = 0.0, J2 = 0.0;
+    for (int sl = 0; sl < kFixSlices; sl++) {
+        const double *q = part + 3 * (i * kFixSlices + sl);
+        J0 += q[0];
+        J1 += q[1];
+        J2 += q[2];
+    }
+    double o[5];
+    cx_finish(f, J0, J1, J2, s[0], cp2, vp, h[0], h[1], vh, o);
+    const double *w = stash + 11 * i;
+    double Pf[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) Pf[k] = w[k];
+    cell_update_one(M, M.cell_code[c], c, prim1, w[9], w[10], Pf, o, S->dt, S, prim2);
+}
+
 __global__ void __launch_bounds__(256) k_cx_fixup(DevMesh M, const double *__restrict__ prim1,
                                                   const double *__restrict__ neutrals, int nodes,
                                                   const int32_t *__restrict__ list, const double *__restrict__ stash,
-                                                  int64_t cap, StepScalars *S, double *__restrict__ prim2) {
+                                                  int64_t cap, int64_t first, StepScalars *S,
+                                                  double *__restrict__ prim2) {
     extern __shared__ double sm[];
     double *sc = sm, *red = sm + 2 * nodes;
     const int64_t n = min((int64_t)S->n_fallback, cap);
-    if ((int64_t)blockIdx.x >= n) return;  // whole workgroup
+    if (first + (int64_t)blockIdx.x >= n) return;  // whole workgroup
     cx_fill_angles(sc, nodes);
     __syncthreads();
     const double dt = S->dt;
-    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+    for (int64_t i = first + blockIdx.x; i < n; i += gridDim.x) {
         const int64_t c = list[i];
         const double *s = prim1 + 8 * c;
         const double *h = neutrals + 5 * c;
@@ -99,9 +186,15 @@ __global__ void __launch_bounds__(256) k_cx_fixup(DevMesh M, const double *__res
 }
 
 void launch_cx_fixup(hipStream_t s, const DevMesh &M, const double *prim1, const double *neutrals, int nodes,
-                     const int32_t *list, const double *stash, int64_t cap, StepScalars *S, double *prim2) {
+                     const int32_t *list, const double *stash, int64_t cap, double *part, StepScalars *S,
+                     double *prim2) {
     const size_t lds = sizeof(double) * (2 * (size_t)nodes + 3 * 256);
-    hipLaunchKernelGGL(k_cx_fixup, dim3(256), dim3(256), lds, s, M, prim1, neutrals, nodes, list, stash, cap, S, prim2);
+    hipLaunchKernelGGL(k_cx_fix_part, dim3(kFixPar * kFixSlices / 4), dim3(256), lds, s, prim1, neutrals, nodes, list,
+                       cap, S, part);
+    hipLaunchKernelGGL(k_cx_fix_finish, dim3(kFixPar / 64), dim3(64), 0, s, M, prim1, neutrals, nodes, list, stash,
+                       cap, S, (const double *)part, prim2);
+    hipLaunchKernelGGL(k_cx_fixup, dim3(256), dim3(256), lds, s, M, prim1, neutrals, nodes, list, stash, cap,
+                       (int64_t)kFixPar, S, prim2);
 }
 
 void launch_cx_exact(hipStream_t s, int64_t n, const double *in10, int nodes, double *out5) {

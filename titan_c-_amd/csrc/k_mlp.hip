@@ -179,7 +179,7 @@ static void pack_params(const float *P, std::vector<uint16_t> &wv, std::vector<f
 
 void mlp_free(MlpState &s) {
     void *ptrs[] = {s.d_params, s.d_grad, s.d_m, s.d_v,    s.d_wpack, s.d_bpack, s.d_norm,  s.d_list,
-                    s.d_count,  s.d_stash, s.d_fw, s.d_bw, s.d_part,  s.d_lacc};
+                    s.d_count,  s.d_stash, s.d_fw, s.d_bw, s.d_part,  s.d_lacc,  s.d_fixpart};
     for (void *p : ptrs)
         if (p) hipFree(p);
     s = MlpState();
@@ -274,6 +274,7 @@ int mlp_fused_args(MlpState &s, int64_t nc, const double *neutrals, SurArgs *a) 
     if (!s.ready) return TITAN_E_NOT_TRAINED;
     const int rc = ensure_list(s, nc);
     if (rc != TITAN_OK) return rc;
+    if (!s.d_fixpart) HIPCHK_M(hipMalloc((void **)&s.d_fixpart, sizeof(double) * 3 * kFixPar * kFixSlices));
     a->wpack = s.d_wpack;
     a->bpack = s.d_bpack;
     a->norm = s.d_norm;
@@ -281,6 +282,7 @@ int mlp_fused_args(MlpState &s, int64_t nc, const double *neutrals, SurArgs *a) 
     a->list = s.d_list;
     a->stash = s.d_stash;
     a->cap = s.list_cap;
+    a->fixpart = s.d_fixpart;
     return TITAN_OK;
 }
 

@@ -82,6 +82,7 @@ void launch_bera_cells_h(hipStream_t s, int64_t nc, const double *prim, const do
                          double *src, double *hout);
 // exact source + update of the cells the surrogate step listed as out of its training box
 void launch_cx_fixup(hipStream_t s, const DevMesh &M, const double *prim1, const double *neutrals, int nodes,
-                     const int32_t *list, const double *stash, int64_t cap, StepScalars *S, double *prim2);
+                     const int32_t *list, const double *stash, int64_t cap, double *part, StepScalars *S,
+                     double *prim2);
 
 }  // namespace titan

@@ -23,7 +23,11 @@ struct SurArgs {
     int32_t *list;           // cells outside the training box this step (slot from S->n_fallback)
     double *stash;           // [cap*11]: flux sum Pf[9], V(now1), V(now2) of each listed cell
     int64_t cap;
+    double *fixpart;         // [kFixPar*kFixSlices*3]: quadrature partial sums of the first listed cells
 };
+
+// The exact source of the first kFixPar listed cells is spread over kFixSlices workgroups each.
+constexpr int kFixPar = 256, kFixSlices = 16;
 
 struct MlpState {
     bool ready = false;
@@ -35,6 +39,7 @@ struct MlpState {
     int32_t *d_list = nullptr;
     int64_t list_cap = 0;
     double *d_stash = nullptr;  // fused-step fallback stash, list_cap * 11 doubles
+    double *d_fixpart = nullptr;  // kFixPar * kFixSlices * 3 quadrature partial sums
     unsigned long long *d_count = nullptr;
     // training: FP32 MFMA operand packs of W (forward) and W^T (backward), per-wavefront gradient
     // partials (with their squared-error sums), the loss accumulator

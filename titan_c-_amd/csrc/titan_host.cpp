@@ -810,8 +810,8 @@ static int go_impl(titan_ctx *c, int64_t steps, titan_step_stats *stats, int are
         timer_mark(c, 0, false);
         if (fused) {
             timer_mark(c, 3, true);
-            launch_cx_fixup(c->stream, M, c->d_prim[n1], c->d_neutrals, nodes, sur.list, sur.stash, sur.cap, c->d_S,
-                            c->d_prim[n2]);
+            launch_cx_fixup(c->stream, M, c->d_prim[n1], c->d_neutrals, nodes, sur.list, sur.stash, sur.cap,
+                            sur.fixpart, c->d_S, c->d_prim[n2]);
             timer_mark(c, 3, false);
         }
     }
