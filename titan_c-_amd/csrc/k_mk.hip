@@ -458,21 +458,43 @@ __device__ __forceinline__ void mk_pui_add(const MkDev &K, const DevMesh &M, con
     if (k > 0) atomicAdd(&K.pui_sp[(c * nw + iw) * 2 + (k - 1)], mu * t * nu_ex);
 }
 
-// Cell::MK_normir_Moments (Cell.cpp:1225-1241) and Cell::MK_calc_Sm (Cell.cpp:1135-1169) of the
-// pickup-ion tallies, one workgroup per cell and one thread per speed bin, in place: S- becomes the
-// loss rate at w_ij from the volume-normalised tally (sum over bins j with ff_j > 0 and the 40
-// angles th_k = k pi / 40, whose sines and cosines come in trig[0..39], trig[40..79]), S+ is divided
-// by V 4 pi / 3 (w2^3 - w1^3).  vol in device cell order.
-__global__ void __launch_bounds__(256) k_mk_pui_finish(int64_t n, int32_t nw, double wr, double a2, double kn,
-                                                       const double *__restrict__ vol, const double *__restrict__ trig,
-                                                       double *__restrict__ sm, double *__restrict__ sp) {
-    extern __shared__ double ff[];  // [nw] normalised S- of the cell, then the 80 angle values
-    double *tr = ff + nw;
+// The angle integral of Cell::MK_calc_Sm (Cell.cpp:1135-1169) does not depend on the cell: for the
+// speed bins ij (w) and j (Vh), tt[j*nw + ij] = sum over the 40 angles th_k = k pi / 40 of
+// d sigma(d) sin(th_k) dth 2 pi, d = |w - Vh| at th_k (terms with d <= 1e-9 left out); the
+// angles' sines and cosines come in trig[0..39], trig[40..79].  One thread per (j, ij).
+__global__ void __launch_bounds__(256) k_mk_pui_table(int32_t nw, double wr, double a2,
+                                                      const double *__restrict__ trig, double *__restrict__ tt) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nw * nw) return;
+    const int j = (int)(q / nw), ij = (int)(q - (int64_t)j * nw);
     const double pi = 3.14159265358979323846;
     const double dthe = pi / 40.0;
+    const double w = ((ij + 0.5) * wr / nw), Vh = ((j + 0.5) * wr / nw);
+    double acc = 0.0;
+    for (int k = 1; k <= 40; k++) {
+        const double d = sqrt(Vh * Vh + w * w - 2.0 * w * Vh * trig[40 + k - 1]);
+        if (d > 0.000000001) {
+            const double lg = log(d);
+            const double sg = (1.0 - a2 * lg) * (1.0 - a2 * lg);
+            acc += d * sg * trig[k - 1] * dthe * 2.0 * pi;
+        }
+    }
+    tt[q] = acc;
+}
+
+// Cell::MK_normir_Moments (Cell.cpp:1225-1241) and Cell::MK_calc_Sm of the pickup-ion tallies, one
+// workgroup per cell and one thread per speed bin, in place: S- becomes the loss rate at w_ij,
+// sum over the bins j with ff_j > 0 of ff_j tt[j][ij], over 4 pi and par_Kn, from the
+// volume-normalised tally ff (staged in LDS); S+ is divided by V 4 pi / 3 (w2^3 - w1^3).  The
+// reference sums ff_j d sigma sin dth 2 pi term by term; taking ff_j out of the angle sum changes
+// the result only by rounding.  vol in device cell order.
+__global__ void __launch_bounds__(256) k_mk_pui_finish(int64_t n, int32_t nw, double wr, double kn,
+                                                       const double *__restrict__ vol, const double *__restrict__ tt,
+                                                       double *__restrict__ sm, double *__restrict__ sp) {
+    extern __shared__ double ff[];  // [nw] normalised S- of the cell
+    const double pi = 3.14159265358979323846;
     for (int64_t c = blockIdx.x; c < n; c += gridDim.x) {
         const double V = vol[c];
-        for (int i = threadIdx.x; i < 80; i += blockDim.x) tr[i] = trig[i];
         bool any = false;
         for (int ij = threadIdx.x; ij < nw; ij += blockDim.x) {
             const double w1 = ij * wr / nw, w2 = (ij + 1) * wr / nw;
@@ -487,19 +509,10 @@ __global__ void __launch_bounds__(256) k_mk_pui_finish(int64_t n, int32_t nw, do
         for (int ij = threadIdx.x; ij < nw; ij += blockDim.x) {
             double acc = 0.0;
             if (any) {
-                const double w = ((ij + 0.5) * wr / nw);
                 for (int j = 0; j < nw; j++) {
-                    const double Vh = ((j + 0.5) * wr / nw);
                     const double f = ff[j];
                     if (f <= 0.0) continue;
-                    for (int k = 1; k <= 40; k++) {
-                        const double d = sqrt(Vh * Vh + w * w - 2.0 * w * Vh * tr[40 + k - 1]);
-                        if (d > 0.000000001) {
-                            const double lg = log(d);
-                            const double sg = (1.0 - a2 * lg) * (1.0 - a2 * lg);
-                            acc += f * d * sg * tr[k - 1] * dthe * 2.0 * pi;
-                        }
-                    }
+                    acc += f * tt[(int64_t)j * nw + ij];
                 }
                 acc = acc / (4.0 * pi);
             }
@@ -742,11 +755,13 @@ void launch_mk_particles(hipStream_t s, const DevMesh &M, const MkDev &K, const 
 }
 
 void launch_mk_pui_finish(hipStream_t s, int64_t n, int32_t nw, double wr, double a2, double kn, const double *vol,
-                          const double *trig, double *sm, double *sp) {
+                          const double *trig, double *tt, double *sm, double *sp) {
     if (n > 0 && nw > 0) {
+        const int64_t nt = (int64_t)nw * nw;
+        hipLaunchKernelGGL(k_mk_pui_table, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, nw, wr, a2, trig, tt);
         const int64_t blocks = n < 65536 ? n : 65536;
         hipLaunchKernelGGL(k_mk_pui_finish, dim3((unsigned)blocks), dim3(nw < 256 ? 64 * ((nw + 63) / 64) : 256),
-                           sizeof(double) * (nw + 80), s, n, nw, wr, a2, kn, vol, trig, sm, sp);
+                           sizeof(double) * nw, s, n, nw, wr, kn, vol, (const double *)tt, sm, sp);
     }
 }
 

@@ -51,9 +51,10 @@ void launch_mk_gather(hipStream_t s, int64_t n, const int32_t *faces, const DevG
 void launch_mk_particles(hipStream_t s, const DevMesh &M, const MkDev &K, const double *X, const double *prim,
                          const DevGeom &G);
 // zone borders: running sums of n distributions at offsets off (into cdf [n*nb]); zeroing n of them
-// Cell::MK_normir_Moments + Cell::MK_calc_Sm of n cells' pickup-ion tallies, in place (k_mk.hip)
+// Cell::MK_normir_Moments + Cell::MK_calc_Sm of n cells' pickup-ion tallies, in place (k_mk.hip);
+// tt: nw*nw doubles of scratch for the cell-independent angle integrals
 void launch_mk_pui_finish(hipStream_t s, int64_t n, int32_t nw, double wr, double a2, double kn, const double *vol,
-                          const double *trig, double *sm, double *sp);
+                          const double *trig, double *tt, double *sm, double *sp);
 void launch_mkz_cdf(hipStream_t s, int64_t n, const int64_t *off, const double *dist, int64_t nb, double *cdf);
 void launch_mkz_clear(hipStream_t s, int64_t n, const int64_t *off, double *dist, int64_t nb);
 

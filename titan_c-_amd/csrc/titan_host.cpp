@@ -1269,8 +1269,9 @@ static int mk_flights(titan_ctx *c, int layer, const MkInj &inj, int zone, const
     K.scal = c->d_scal[layer];  // null unless the passive scalars are carried
     c->pui_valid = false;
     if (c->pui_nw > 0) {  // MK_source_S: the tallies start from zero (Cell::Init_S)
-        const size_t cnt = (size_t)3 * c->pui_nw * nc;
-        if (!c->d_pui) HIPCHK(hipMalloc((void **)&c->d_pui, sizeof(double) * cnt));
+        const size_t cnt = (size_t)3 * c->pui_nw * nc;  // S-, S+; then the angle-integral table
+        if (!c->d_pui)
+            HIPCHK(hipMalloc((void **)&c->d_pui, sizeof(double) * (cnt + (size_t)c->pui_nw * c->pui_nw)));
         if (!c->d_pui_trig) {
             // the 40 angles of Cell::MK_calc_Sm, sin and cos as the host's libm gives them
             double tr[80];
@@ -1306,7 +1307,8 @@ static int mk_flights(titan_ctx *c, int layer, const MkInj &inj, int zone, const
     launch_mk_particles(c->stream, c->mesh(), K, c->d_X[layer], c->d_prim[layer], c->geom[layer]);
     if (K.pui_nw > 0)
         launch_mk_pui_finish(c->stream, nc, K.pui_nw, K.pui_wr, c->params.par_a_2, c->params.par_Kn,
-                             c->geom[layer].cell_volume, c->d_pui_trig, K.pui_sm, K.pui_sp);
+                             c->geom[layer].cell_volume, c->d_pui_trig, c->d_pui + (size_t)3 * K.pui_nw * nc, K.pui_sm,
+                             K.pui_sp);
     HIPCHK(hipGetLastError());
     dens.resize(5 * nc);
     unsigned long long h[9];
@@ -1367,14 +1369,15 @@ extern "C" int titan_mk_s_finish_batch(int64_t n, int32_t n_w, double w_r, doubl
         tr[k - 1] = std::sin(the);
         tr[40 + k - 1] = std::cos(the);
     }
-    DevBuf dv, dsm, dsp, dtr;
+    DevBuf dv, dsm, dsp, dtr, dtt;
     int rc = dup(dv, vol, sizeof(double) * n);
     if (rc == TITAN_OK) rc = dup(dsm, sm, sizeof(double) * n * n_w);
     if (rc == TITAN_OK) rc = dup(dsp, sp, sizeof(double) * n * n_w * 2);
     if (rc == TITAN_OK) rc = dup(dtr, tr, sizeof(tr));
+    if (rc == TITAN_OK) rc = alloc(dtt, sizeof(double) * n_w * n_w);
     if (rc != TITAN_OK) return rc;
-    launch_mk_pui_finish(nullptr, n, n_w, w_r, a2, kn, (const double *)dv.p, (const double *)dtr.p, (double *)dsm.p,
-                         (double *)dsp.p);
+    launch_mk_pui_finish(nullptr, n, n_w, w_r, a2, kn, (const double *)dv.p, (const double *)dtr.p, (double *)dtt.p,
+                         (double *)dsm.p, (double *)dsp.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(sm_out, dsm.p, sizeof(double) * n * n_w, hipMemcpyDeviceToHost));
